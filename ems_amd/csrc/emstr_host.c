@@ -1,0 +1,838 @@
+/*
+ * Host side of libemstr: device mirrors of a window (contexts), transfers
+ * between the reference's sparse arrays and the dense device layout, the
+ * step driver and the drop-in entry points with the reference's
+ * signatures:
+ *   tracer_step_3d   model/hd/tracers/tracers.c:239-321
+ *   advect_diffuse   model/hd/tracers/tracers.c:401-1054
+ *   vert_diffuse_3d  model/hd/tracers/tracers.c:3782-3957
+ *   set_dz           model/hd/tracers/tracers.c:5682-5849
+ * All arithmetic of the step runs in the HIP kernels; this file only
+ * moves data, sequences launches and rebuilds the reference's index
+ * lists from the per-column results of the device set_dz.
+ */
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "emstr.h"
+#include "emstr_dev.h"
+#include "emstr_internal.h"
+#include "emstr_kernels.h"
+#include "emstr_prec.h"
+#include "emstr_prof.h"
+
+#define HCHK(x)                                                              \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess)                                                    \
+      return emstr_set_error(EMSTR_EDEVICE, -1, "%s: %s", #x,                \
+                             hipGetErrorString(e_));                         \
+  } while (0)
+
+typedef long long ll;
+
+struct emstr_ctx {
+  geometry_t *g;
+  window_t *wd;
+  win_priv_t *wc;
+  int form;
+  hipStream_t st;
+  emstr_dev d;
+  /* host-side index helpers */
+  int *h_kb;      /* [plane]   */
+  int *cc2col;    /* [a2_t+1]  */
+  int *col2cc;    /* [plane]   */
+  int *sedg;      /* [plane]   sediment ghost sparse index */
+  int *topc;      /* [plane]   surface-layer sparse index */
+  int *h_ks, *h_kold, *h_knew, *h_proc;
+  int *h_cols;
+  int *sp_of;     /* [n3] sparse index of a dense wet cell (0 else) */
+  /* device buffers */
+  double *tr[3];  /* tracer buffers (FP64 plain form) */
+  int cur;        /* buffer holding the current state */
+  double *stage;  /* device sparse staging [sgsiz] */
+  int *d_bpt, *d_bin, *d_gsed, *d_ised;
+  int *d_proc;
+  int *d_err;
+  unsigned long long *d_dtm;
+  double *d_part, *h_part;
+  double *d_wbot;
+  int anydif;
+  int have_cols; /* per-column step data present on the device */
+  /* identity of the grid the context was built for */
+  int key_sgsiz, key_nx, key_ny, key_nz, key_v3;
+  const int *key_w3, *key_xp1;
+  struct emstr_prec *prec; /* precision form state (EMSTR_PREC_F32) */
+  emstr_prof *prof;        /* per-kernel event timing                 */
+};
+
+/*-------------------------------------------------------------------*/
+/* Context registry: one device mirror per window (drop-in calls).   */
+/*-------------------------------------------------------------------*/
+#define MAXCTX 64
+static pthread_mutex_t reg_lock = PTHREAD_MUTEX_INITIALIZER;
+static geometry_t *reg_g[MAXCTX];
+static emstr_ctx *reg_c[MAXCTX];
+
+/* A context matches a window only if the structure still describes the
+ * same grid (a freed window's address can be reused by a new one). */
+static int ctx_matches(const emstr_ctx *c, const geometry_t *g) {
+  return c->g == g && c->key_sgsiz == g->sgsiz && c->key_nx == g->nce1 &&
+         c->key_ny == g->nce2 && c->key_nz == g->nz && c->key_v3 == g->v3_t &&
+         c->key_w3 == g->w3_t && c->key_xp1 == g->xp1;
+}
+
+/* Drop the drop-in context of a window (called when its arrays are freed). */
+void emstr_forget_window(geometry_t *g) {
+  int i;
+  emstr_ctx *c = NULL;
+  pthread_mutex_lock(&reg_lock);
+  for (i = 0; i < MAXCTX; i++)
+    if (reg_g[i] == g) {
+      c = reg_c[i];
+      reg_g[i] = NULL;
+      reg_c[i] = NULL;
+    }
+  pthread_mutex_unlock(&reg_lock);
+  if (c) emstr_destroy(c);
+}
+
+static emstr_ctx *ctx_for(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  emstr_ctx *c = NULL, *stale = NULL;
+  int i;
+  pthread_mutex_lock(&reg_lock);
+  for (i = 0; i < MAXCTX; i++)
+    if (reg_g[i] == g) {
+      c = reg_c[i];
+      if (!ctx_matches(c, g)) {
+        stale = c;
+        c = NULL;
+        reg_g[i] = NULL;
+        reg_c[i] = NULL;
+      }
+      break;
+    }
+  pthread_mutex_unlock(&reg_lock);
+  if (stale) emstr_destroy(stale);
+  pthread_mutex_lock(&reg_lock);
+  if (!c) {
+    c = emstr_create(g, wd, wc, EMSTR_PLAIN_F64);
+    if (c)
+      for (i = 0; i < MAXCTX; i++)
+        if (!reg_g[i]) {
+          reg_g[i] = g;
+          reg_c[i] = c;
+          break;
+        }
+  }
+  pthread_mutex_unlock(&reg_lock);
+  if (c) {
+    c->wd = wd;
+    c->wc = wc;
+  }
+  return c;
+}
+
+/*-------------------------------------------------------------------*/
+/* helpers                                                           */
+/*-------------------------------------------------------------------*/
+static void *dalloc(size_t bytes, int *ok) {
+  void *p = NULL;
+  if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) {
+    *ok = 0;
+    return NULL;
+  }
+  hipMemset(p, 0, bytes ? bytes : 8);
+  return p;
+}
+
+static int upload2d(emstr_ctx *c, double *dst, const double *sp2) {
+  /* 2D sparse array -> dense [plane]: every 2D sparse cell (wet or ghost)
+   * carries the value of its (i,j) location */
+  const geometry_t *g = c->g;
+  const ll plane = c->d.plane;
+  double *h = (double *)calloc(plane, sizeof(double));
+  int n;
+  for (n = 1; n <= g->sgnumS; n++) {
+    ll col = (ll)g->s2j[n] * c->d.nx + g->s2i[n];
+    if (col >= 0 && col < plane) h[col] = sp2[n];
+  }
+  /* wet columns take their own surface cell's value */
+  for (n = 1; n <= g->a2_t; n++) h[c->cc2col[n]] = sp2[g->w2_t[n]];
+  hipError_t e = hipMemcpyAsync(dst, h, plane * sizeof(double), hipMemcpyHostToDevice, c->st);
+  hipStreamSynchronize(c->st);
+  free(h);
+  return e == hipSuccess ? 0 : emstr_set_error(EMSTR_EDEVICE, -1, "upload2d");
+}
+
+static int upload3d(emstr_ctx *c, double *dst, const double *sp) {
+  HCHK(hipMemcpyAsync(c->stage, sp, (size_t)c->g->sgsiz * sizeof(double),
+                      hipMemcpyHostToDevice, c->st));
+  if (emstr_launch_scatter(c->d.wsp, c->d.wdn, c->d.nwet, c->stage, dst, c->st))
+    return emstr_set_error(EMSTR_EDEVICE, -1, "scatter launch");
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* context creation                                                  */
+/*-------------------------------------------------------------------*/
+emstr_ctx *emstr_create(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
+  int n, ok = 1;
+  emstr_clear_error();
+  if (!g || !wd || !wc) {
+    emstr_set_error(EMSTR_EARG, -1, "emstr_create: null structure");
+    return NULL;
+  }
+  if (g->nobc != 0 || wc->sigma) {
+    emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                    "open boundaries and sigma coordinates are not supported");
+    return NULL;
+  }
+  if (form != EMSTR_PLAIN_F64 && form != EMSTR_PREC_F32) {
+    emstr_set_error(EMSTR_EARG, -1, "unknown form %d", form);
+    return NULL;
+  }
+  for (n = 1; n < g->sgsizS; n++)
+    if (wc->Ds[n] != 1.0 || wc->Hn1[n] != 1.0 || wc->mdx[n] != 1.0 || wc->mdy[n] != 1.0) {
+      emstr_set_error(EMSTR_EUNSUPPORTED, n, "z-level windows need Ds = Hn1 = mdx = mdy = 1");
+      return NULL;
+    }
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+      emstr_set_error(EMSTR_EDEVICE, -1, "no HIP device");
+      return NULL;
+    }
+  }
+  emstr_ctx *c = (emstr_ctx *)calloc(1, sizeof(emstr_ctx));
+  c->g = g;
+  c->wd = wd;
+  c->wc = wc;
+  c->form = form;
+  c->key_sgsiz = g->sgsiz;
+  c->key_nx = g->nce1;
+  c->key_ny = g->nce2;
+  c->key_nz = g->nz;
+  c->key_v3 = g->v3_t;
+  c->key_w3 = g->w3_t;
+  c->key_xp1 = g->xp1;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    free(c);
+    emstr_set_error(EMSTR_EDEVICE, -1, "stream create");
+    return NULL;
+  }
+  emstr_dev *d = &c->d;
+  d->nx = g->nce1;
+  d->ny = g->nce2;
+  d->nz = g->nz;
+  d->ntr = wd->ntr;
+  d->plane = (ll)d->nx * d->ny;
+  d->n3 = d->plane * d->nz;
+  d->trasc = wc->trasc;
+  d->ultimate = (wc->trasc == ORDER1) ? 0 : wc->ultimate;
+  d->sigma = 0;
+  d->dt = wd->dt;
+  d->hmin = wc->hmin;
+  d->sgsiz = g->sgsiz;
+  const ll plane = d->plane, n3 = d->n3;
+
+  /* columns: bed layer, surface cell, sediment ghost */
+  c->h_kb = (int *)malloc(plane * sizeof(int));
+  for (ll q = 0; q < plane; q++) c->h_kb[q] = d->nz;
+  c->cc2col = (int *)calloc(g->a2_t + 2, sizeof(int));
+  c->col2cc = (int *)calloc(plane, sizeof(int));
+  c->sedg = (int *)calloc(plane, sizeof(int));
+  c->topc = (int *)calloc(plane, sizeof(int));
+  for (n = 1; n <= g->a2_t; n++) {
+    int cs = g->w2_t[n];
+    int cb = g->bot_t[n];
+    ll col = (ll)g->s2j[cs] * d->nx + g->s2i[cs];
+    c->cc2col[n] = (int)col;
+    c->col2cc[col] = n;
+    c->h_kb[col] = g->s2k[cb];
+    c->sedg[col] = g->zm1[cb];
+    c->topc[col] = cs;
+    int ii = g->s2i[cs], jj = g->s2j[cs];
+    if (ii <= 0 || jj <= 0 || ii >= d->nx - 1 || jj >= d->ny - 1) {
+      emstr_set_error(EMSTR_EUNSUPPORTED, cs, "wet column on the grid edge (%d,%d)", ii, jj);
+      free(c);
+      return NULL;
+    }
+  }
+  /* wet cells: sparse <-> dense */
+  d->nwet = g->v3_t;
+  int *wsp = (int *)malloc((size_t)d->nwet * sizeof(int));
+  ll *wdn = (ll *)malloc((size_t)d->nwet * sizeof(ll));
+  c->sp_of = (int *)calloc(n3, sizeof(int));
+  for (n = 0; n < d->nwet; n++) {
+    int cw = g->w3_t[n + 1];
+    ll idx = ((ll)g->s2k[cw] * d->ny + g->s2j[cw]) * d->nx + g->s2i[cw];
+    wsp[n] = cw;
+    wdn[n] = idx;
+    c->sp_of[idx] = cw;
+  }
+  c->h_ks = (int *)malloc(plane * sizeof(int));
+  c->h_kold = (int *)malloc(plane * sizeof(int));
+  c->h_knew = (int *)malloc(plane * sizeof(int));
+  c->h_proc = (int *)calloc(plane, sizeof(int));
+  c->h_cols = (int *)malloc((g->a2_t + 1) * sizeof(int));
+  for (ll q = 0; q < plane; q++) c->h_ks[q] = -1, c->h_kold[q] = c->h_knew[q] = d->nz - 1;
+  for (n = 1; n <= g->a2_t; n++) {
+    ll col = c->cc2col[n];
+    int ns = g->nsur_t[n], os = g->sur_t[n];
+    c->h_knew[col] = (ns == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[ns];
+    c->h_kold[col] = (os == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[os];
+  }
+
+  /* device arrays */
+  d->kb = (int *)dalloc(plane * sizeof(int), &ok);
+  d->ks = (int *)dalloc(plane * sizeof(int), &ok);
+  d->kold = (int *)dalloc(plane * sizeof(int), &ok);
+  d->knew = (int *)dalloc(plane * sizeof(int), &ok);
+  double **d2[] = {&d->h1au1, &d->h2au2, &d->h1acell, &d->h2acell, &d->h2au1,
+                   &d->h1au2, &d->cellarea, &d->botz, &d->eta, &d->oldeta,
+                   &d->osubeta, &d->subeta};
+  for (n = 0; n < (int)(sizeof(d2) / sizeof(d2[0])); n++)
+    *d2[n] = (double *)dalloc(plane * sizeof(double), &ok);
+  c->d_wbot = (double *)dalloc(plane * sizeof(double), &ok);
+  d->layers = (double *)dalloc((d->nz + 1) * sizeof(double), &ok);
+  double **d3[] = {&d->u1, &d->u2, &d->w, &d->u1flux3d, &d->u2flux3d, &d->Kz, &d->dz};
+  for (n = 0; n < (int)(sizeof(d3) / sizeof(d3[0])); n++)
+    *d3[n] = (double *)dalloc(n3 * sizeof(double), &ok);
+  c->anydif = 0;
+  for (n = 0; n < wd->ntr; n++)
+    if (wc->diffuse[n]) c->anydif = 1;
+  if (c->anydif) {
+    double **dd[] = {&d->dzu1, &d->dzu2, &d->u1kh, &d->u2kh};
+    for (n = 0; n < 4; n++) *dd[n] = (double *)dalloc(n3 * sizeof(double), &ok);
+  }
+  const int nt = wd->ntr > 0 ? wd->ntr : 1;
+  d->mintr = (double *)dalloc(nt * sizeof(double), &ok);
+  d->maxtr = (double *)dalloc(nt * sizeof(double), &ok);
+  d->advect = (int *)dalloc(nt * sizeof(int), &ok);
+  d->diffuse = (int *)dalloc(nt * sizeof(int), &ok);
+  d->in_tbdy = (int *)dalloc(nt * sizeof(int), &ok);
+  d->in_tdif = (int *)dalloc(nt * sizeof(int), &ok);
+  d->cols = (int *)dalloc((g->a2_t + 1) * sizeof(int), &ok);
+  d->wsp = (int *)dalloc((size_t)d->nwet * sizeof(int), &ok);
+  d->wdn = (ll *)dalloc((size_t)d->nwet * sizeof(ll), &ok);
+  c->stage = (double *)dalloc((size_t)g->sgsiz * sizeof(double), &ok);
+  c->d_bpt = (int *)dalloc((g->nbpt + 1) * sizeof(int), &ok);
+  c->d_bin = (int *)dalloc((g->nbpt + 1) * sizeof(int), &ok);
+  c->d_gsed = (int *)dalloc((g->ngsed + 1) * sizeof(int), &ok);
+  c->d_ised = (int *)dalloc((g->ngsed + 1) * sizeof(int), &ok);
+  c->d_proc = (int *)dalloc(plane * sizeof(int), &ok);
+  c->d_err = (int *)dalloc(sizeof(int), &ok);
+  c->d_dtm = (unsigned long long *)dalloc(sizeof(unsigned long long), &ok);
+  {
+    int nb = (g->a2_t + 255) / 256 + 1;
+    c->d_part = (double *)dalloc((size_t)2 * nb * nt * sizeof(double), &ok);
+    c->h_part = (double *)malloc((size_t)2 * nb * nt * sizeof(double));
+  }
+  if (form == EMSTR_PLAIN_F64)
+    for (n = 0; n < 3; n++) c->tr[n] = (double *)dalloc((size_t)nt * n3 * sizeof(double), &ok);
+  if (!ok) {
+    emstr_set_error(EMSTR_EDEVICE, -1, "device allocation failed");
+    emstr_destroy(c);
+    return NULL;
+  }
+  hipMemcpy(d->kb, c->h_kb, plane * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(d->kold, c->h_kold, plane * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(d->knew, c->h_knew, plane * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(d->wsp, wsp, (size_t)d->nwet * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(d->wdn, wdn, (size_t)d->nwet * sizeof(ll), hipMemcpyHostToDevice);
+  hipMemcpy(c->d_bpt, g->bpt + 1, g->nbpt * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(c->d_bin, g->bin + 1, g->nbpt * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(c->d_gsed, g->gsed_t + 1, g->ngsed * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(c->d_ised, g->ised_t + 1, g->ngsed * sizeof(int), hipMemcpyHostToDevice);
+  free(wsp);
+  free(wdn);
+  {
+    double *lay = (double *)malloc((d->nz + 1) * sizeof(double));
+    if (g->layers) {
+      memcpy(lay, g->layers, (d->nz + 1) * sizeof(double));
+    } else {
+      emstr_set_error(EMSTR_EARG, -1, "geometry->layers is required");
+      free(lay);
+      emstr_destroy(c);
+      return NULL;
+    }
+    hipMemcpy(d->layers, lay, (d->nz + 1) * sizeof(double), hipMemcpyHostToDevice);
+    free(lay);
+  }
+  /* static 2D metrics */
+  if (upload2d(c, d->h1au1, g->h1au1) || upload2d(c, d->h2au2, g->h2au2) ||
+      upload2d(c, d->h1acell, g->h1acell) || upload2d(c, d->h2acell, g->h2acell) ||
+      upload2d(c, d->h2au1, g->h2au1) || upload2d(c, d->h1au2, g->h1au2) ||
+      upload2d(c, d->cellarea, g->cellarea) || upload2d(c, d->botz, g->botz)) {
+    emstr_destroy(c);
+    return NULL;
+  }
+  if (form == EMSTR_PREC_F32) {
+    c->prec = emstr_prec_create(d, c->st);
+    if (!c->prec) {
+      emstr_destroy(c);
+      return NULL;
+    }
+  }
+  c->prof = (emstr_prof *)calloc(1, sizeof(emstr_prof));
+  if (c->prec) emstr_prec_set_prof(c->prec, c->prof);
+  c->cur = 0;
+  c->have_cols = 0;
+  return c;
+}
+
+int emstr_profile(emstr_ctx *c, int on) {
+  if (!c || !c->prof) return EMSTR_EARG;
+  prof_collect(c->prof);
+  c->prof->on = on;
+  for (int k = 0; k < KC_N; k++) c->prof->ms[k] = 0.0, c->prof->cnt[k] = 0;
+  return 0;
+}
+
+int emstr_kernel_times(emstr_ctx *c, double *ms, long long *count) {
+  if (!c || !c->prof) return EMSTR_EARG;
+  prof_collect(c->prof);
+  for (int k = 0; k < KC_N; k++) {
+    if (ms) ms[k] = c->prof->ms[k];
+    if (count) count[k] = c->prof->cnt[k];
+  }
+  return 0;
+}
+
+void emstr_destroy(emstr_ctx *c) {
+  int n;
+  if (!c) return;
+  emstr_dev *d = &c->d;
+  if (c->st) hipStreamSynchronize(c->st);
+  if (c->prec) emstr_prec_destroy(c->prec);
+  if (c->prof) {
+    prof_collect(c->prof);
+    prof_free(c->prof);
+    free(c->prof);
+  }
+  void *dv[] = {d->kb, d->ks, d->kold, d->knew, d->h1au1, d->h2au2, d->h1acell,
+                d->h2acell, d->h2au1, d->h1au2, d->cellarea, d->botz, d->eta,
+                d->oldeta, d->osubeta, d->subeta, d->layers, d->u1, d->u2, d->w,
+                d->u1flux3d, d->u2flux3d, d->Kz, d->dz, d->dzu1, d->dzu2, d->u1kh,
+                d->u2kh, d->mintr, d->maxtr, d->advect, d->diffuse, d->in_tbdy,
+                d->in_tdif, d->cols, d->wsp, d->wdn, c->stage, c->d_bpt, c->d_bin,
+                c->d_gsed, c->d_ised, c->d_proc, c->d_err, c->d_dtm, c->d_part,
+                c->d_wbot, c->tr[0], c->tr[1], c->tr[2]};
+  for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
+    if (dv[n]) hipFree(dv[n]);
+  void *hv[] = {c->h_kb, c->cc2col, c->col2cc, c->sedg, c->topc, c->h_ks,
+                c->h_kold, c->h_knew, c->h_proc, c->h_cols, c->sp_of, c->h_part};
+  for (n = 0; n < (int)(sizeof(hv) / sizeof(hv[0])); n++) free(hv[n]);
+  if (c->st) hipStreamDestroy(c->st);
+  pthread_mutex_lock(&reg_lock);
+  for (n = 0; n < MAXCTX; n++)
+    if (reg_c[n] == c) reg_c[n] = NULL, reg_g[n] = NULL;
+  pthread_mutex_unlock(&reg_lock);
+  free(c);
+}
+
+void *emstr_stream(emstr_ctx *c) { return c ? (void *)c->st : NULL; }
+
+int emstr_sync(emstr_ctx *c) {
+  int err = 0;
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy(&err, c->d_err, sizeof(int), hipMemcpyDeviceToHost));
+  if (err) {
+    int cell = c->sp_of[err - 1];
+    hipMemset(c->d_err, 0, sizeof(int));
+    return emstr_set_error(EMSTR_EZEROPIVOT, cell,
+                           "implicit_vdiff_tr: zero divisor at cell %d", cell);
+  }
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* per-column step data from the host's set_dz results (drop-in)     */
+/*-------------------------------------------------------------------*/
+static int columns_from_host(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->d;
+  const ll plane = d->plane;
+  int cc;
+  for (ll q = 0; q < plane; q++) c->h_ks[q] = -1;
+  for (cc = 1; cc <= wc->vcs; cc++) {
+    int s = wc->s1[cc];
+    ll col = (ll)g->s2j[s] * d->nx + g->s2i[s];
+    c->h_ks[col] = g->s2k[s];
+    int ns = wc->i2[cc], os = wc->i3[cc];
+    c->h_knew[col] = (ns == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[ns];
+    c->h_kold[col] = (os == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[os];
+    c->h_cols[cc - 1] = (int)col;
+  }
+  d->ncol = wc->vcs;
+  HCHK(hipMemcpyAsync(d->ks, c->h_ks, plane * sizeof(int), hipMemcpyHostToDevice, c->st));
+  HCHK(hipMemcpyAsync(d->kold, c->h_kold, plane * sizeof(int), hipMemcpyHostToDevice, c->st));
+  HCHK(hipMemcpyAsync(d->knew, c->h_knew, plane * sizeof(int), hipMemcpyHostToDevice, c->st));
+  HCHK(hipMemcpyAsync(d->cols, c->h_cols, (size_t)d->ncol * sizeof(int), hipMemcpyHostToDevice, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  c->have_cols = 1;
+  return 0;
+}
+
+static int upload_tracer_meta(emstr_ctx *c) {
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->d;
+  int n, nt = wd->ntr;
+  int *tb = (int *)calloc(nt + 1, sizeof(int));
+  int *td = (int *)calloc(nt + 1, sizeof(int));
+  for (n = 0; n < wc->ntbdy; n++) tb[wc->tbdy[n]] = 1;
+  for (n = 0; n < wc->ntdif_v; n++) td[wc->tdif_v[n]] = 1;
+  hipMemcpy(d->in_tbdy, tb, nt * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(d->in_tdif, td, nt * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(d->advect, wc->advect, nt * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(d->diffuse, wc->diffuse, nt * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(d->mintr, wc->mintr, nt * sizeof(double), hipMemcpyHostToDevice);
+  hipMemcpy(d->maxtr, wc->maxtr, nt * sizeof(double), hipMemcpyHostToDevice);
+  free(tb);
+  free(td);
+  for (n = 0; n < nt; n++)
+    if (wc->diffuse[n] && !c->anydif)
+      return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                             "horizontal diffusion enabled after the context was created");
+  return 0;
+}
+
+int emstr_upload_fields(emstr_ctx *c) {
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->d;
+  emstr_clear_error();
+  d->dt = wd->dt;
+  d->hmin = wc->hmin;
+  d->trasc = wc->trasc;
+  d->ultimate = (wc->trasc == ORDER1) ? 0 : wc->ultimate;
+  if (upload3d(c, d->u1, wd->u1) || upload3d(c, d->u2, wd->u2) ||
+      upload3d(c, d->w, wd->w) || upload3d(c, d->u1flux3d, wd->u1flux3d) ||
+      upload3d(c, d->u2flux3d, wd->u2flux3d) || upload3d(c, d->Kz, wd->Kz) ||
+      upload3d(c, d->dz, wc->dz))
+    return EMSTR_EDEVICE;
+  if (c->anydif)
+    if (upload3d(c, d->dzu1, wd->dzu1) || upload3d(c, d->dzu2, wd->dzu2) ||
+        upload3d(c, d->u1kh, wc->u1kh) || upload3d(c, d->u2kh, wc->u2kh))
+      return EMSTR_EDEVICE;
+  if (upload2d(c, d->eta, wd->eta) || upload2d(c, d->oldeta, wc->oldeta) ||
+      upload2d(c, c->d_wbot, wd->wbot))
+    return EMSTR_EDEVICE;
+  if (upload_tracer_meta(c)) return EMSTR_EUNSUPPORTED;
+  if (columns_from_host(c)) return EMSTR_EDEVICE;
+  HCHK(hipStreamSynchronize(c->st));
+  if (c->prec) return emstr_prec_fields(c->prec, d, c->st);
+  return 0;
+}
+
+int emstr_upload_tracers(emstr_ctx *c) {
+  int n;
+  emstr_dev *d = &c->d;
+  emstr_clear_error();
+  for (n = 0; n < c->wd->ntr; n++) {
+    HCHK(hipMemcpyAsync(c->stage, c->wd->tr_wc[n], (size_t)c->g->sgsiz * sizeof(double),
+                        hipMemcpyHostToDevice, c->st));
+    if (c->prec) {
+      if (emstr_prec_put_tracer(c->prec, d, n, c->stage, c->st)) return EMSTR_EDEVICE;
+    } else if (emstr_launch_scatter(d->wsp, d->wdn, d->nwet, c->stage,
+                                    c->tr[c->cur] + (ll)n * d->n3, c->st)) {
+      return emstr_set_error(EMSTR_EDEVICE, -1, "scatter");
+    }
+  }
+  HCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int emstr_download_tracers(emstr_ctx *c) {
+  int n;
+  emstr_dev *d = &c->d;
+  geometry_t *g = c->g;
+  emstr_clear_error();
+  for (n = 0; n < c->wd->ntr; n++) {
+    HCHK(hipMemcpyAsync(c->stage, c->wd->tr_wc[n], (size_t)g->sgsiz * sizeof(double),
+                        hipMemcpyHostToDevice, c->st));
+    if (c->prec) {
+      if (emstr_prec_get_tracer(c->prec, d, n, c->stage, c->st)) return EMSTR_EDEVICE;
+    } else if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, c->tr[c->cur] + (ll)n * d->n3,
+                                   c->stage, c->st)) {
+      return emstr_set_error(EMSTR_EDEVICE, -1, "gather");
+    }
+    /* ghosts hold the no-gradient values the next step would set */
+    emstr_launch_ghosts(c->d_bpt, c->d_bin, g->nbpt, c->stage, c->st);
+    emstr_launch_ghosts(c->d_gsed, c->d_ised, g->ngsed, c->stage, c->st);
+    HCHK(hipMemcpyAsync(c->wd->tr_wc[n], c->stage, (size_t)g->sgsiz * sizeof(double),
+                        hipMemcpyDeviceToHost, c->st));
+  }
+  HCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* the step                                                          */
+/*-------------------------------------------------------------------*/
+static int adv_plain(emstr_ctx *c) {
+  emstr_dev *d = &c->d;
+  win_priv_t *wc = c->wc;
+  double dtu = d->dt, dtm = d->dt, trem;
+  int itermax = 20, first = 1;
+  if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
+  if (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)) {
+    unsigned long long init;
+    double big = dtu;
+    memcpy(&init, &big, sizeof(init));
+    HCHK(hipMemcpyAsync(c->d_dtm, &init, sizeof(init), hipMemcpyHostToDevice, c->st));
+    if (emstr_launch_dtm(d, dtu, c->d_dtm, c->st))
+      return emstr_set_error(EMSTR_EDEVICE, -1, "dtm launch");
+    unsigned long long r;
+    HCHK(hipMemcpyAsync(&r, c->d_dtm, sizeof(r), hipMemcpyDeviceToHost, c->st));
+    HCHK(hipStreamSynchronize(c->st));
+    memcpy(&dtm, &r, sizeof(dtm));
+    if (dtm != dtu) dtm = floor(dtm);
+  }
+  if (dtm == 0.0 || (int)(dtu / dtm) > itermax)
+    return emstr_set_error(EMSTR_ESUBSTEP, -1,
+                           "tracer_step_3d: maximum number of sub-steps (%d) exceeded", itermax);
+  /* buffers: start = cur (kept for lateral ghosts), then rotate */
+  const int b0 = c->cur;
+  int in = b0, out = (b0 + 1) % 3;
+  trem = dtu;
+  while (trem > 0) {
+    double u = dtm;
+    if (trem < u) u = trem;
+    wc->b1 = u;
+    if (u > 0.0) {
+      int pi, bad;
+      if (emstr_launch_subeta(d, u, trem, first, c->st))
+        return emstr_set_error(EMSTR_EDEVICE, -1, "subeta launch");
+      pi = prof_begin(c->prof, KC_ADV, c->st);
+      bad = emstr_launch_adv(d, c->tr[in], c->tr[out], c->tr[b0], u, trem, c->anydif, c->st);
+      prof_end(c->prof, pi, c->st);
+      if (bad || emstr_launch_osub(d, c->st))
+        return emstr_set_error(EMSTR_EDEVICE, -1, "advection launch");
+      in = out;
+      out = (out + 1) % 3;
+      if (out == b0) out = (out + 1) % 3;
+    }
+    trem -= u;
+    first = 0;
+  }
+  c->cur = in;
+  return 0;
+}
+
+static int vdiff_plain(emstr_ctx *c) {
+  const int scratch = (c->cur + 1) % 3;
+  if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
+  int pi = prof_begin(c->prof, KC_VDIFF, c->st);
+  int bad = emstr_launch_vdiff(&c->d, c->tr[c->cur], c->tr[scratch], c->d_err, c->st);
+  prof_end(c->prof, pi, c->st);
+  if (bad) return emstr_set_error(EMSTR_EDEVICE, -1, "vdiff launch");
+  return 0;
+}
+
+int emstr_advect_diffuse(emstr_ctx *c) {
+  emstr_clear_error();
+  if (c->prec) return emstr_prec_advect(c->prec, &c->d, c->wc->stab, c->st);
+  return adv_plain(c);
+}
+
+int emstr_vert_diffuse(emstr_ctx *c) {
+  emstr_clear_error();
+  if (c->prec) return emstr_prec_vdiff(c->prec, &c->d, c->d_err, c->st);
+  return vdiff_plain(c);
+}
+
+int emstr_step(emstr_ctx *c) {
+  int r;
+  emstr_clear_error();
+  if (c->prec) return emstr_prec_step(c->prec, &c->d, c->wc->stab, c->d_err, c->st);
+  if ((r = adv_plain(c))) return r;
+  return vdiff_plain(c);
+}
+
+int emstr_tracer_mass(emstr_ctx *c, double *mass) {
+  emstr_dev *d = &c->d;
+  int n, b;
+  const int nb = emstr_mass_blocks(d);
+  emstr_clear_error();
+  if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
+  if (c->prec) {
+    if (emstr_prec_mass(c->prec, d, c->d_part, c->st)) return EMSTR_EDEVICE;
+  } else if (emstr_launch_mass(d, c->tr[c->cur], NULL, c->d_part, c->st)) {
+    return emstr_set_error(EMSTR_EDEVICE, -1, "mass launch");
+  }
+  HCHK(hipMemcpyAsync(c->h_part, c->d_part, (size_t)2 * nb * d->ntr * sizeof(double),
+                      hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  for (n = 0; n < d->ntr; n++) {
+    /* compensated sum of the block (sum, err) pairs in block order */
+    double s = 0.0, e = 0.0;
+    for (b = 0; b < nb; b++) {
+      double a = c->h_part[2 * ((ll)n * nb + b)];
+      double t = s + a;
+      double bb = t - s;
+      e += (s - (t - bb)) + (a - bb);
+      s = t;
+      e += c->h_part[2 * ((ll)n * nb + b) + 1];
+    }
+    mass[n] = s + e;
+  }
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* device set_dz and w, and the reference's index lists              */
+/*-------------------------------------------------------------------*/
+int emstr_device_set_dz(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->d;
+  const ll plane = d->plane;
+  int cc;
+  emstr_clear_error();
+  d->hmin = wc->hmin;
+  if (upload2d(c, d->eta, wd->eta) || upload2d(c, d->oldeta, wc->oldeta))
+    return EMSTR_EDEVICE;
+  /* the previous new surface is the host's nsur_t */
+  for (cc = 1; cc <= g->a2_t; cc++) {
+    ll col = c->cc2col[cc];
+    int ns = g->nsur_t[cc];
+    c->h_knew[col] = (ns == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[ns];
+  }
+  HCHK(hipMemcpyAsync(d->knew, c->h_knew, plane * sizeof(int), hipMemcpyHostToDevice, c->st));
+  if (emstr_launch_setdz(d, c->d_proc, c->st)) return emstr_set_error(EMSTR_EDEVICE, -1, "setdz");
+  HCHK(hipMemcpyAsync(c->h_ks, d->ks, plane * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipMemcpyAsync(c->h_kold, d->kold, plane * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipMemcpyAsync(c->h_knew, d->knew, plane * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  /* processed columns in w2_t order */
+  int ncol = 0;
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    ll col = c->cc2col[cc];
+    if (c->h_ks[col] >= 0) c->h_cols[ncol++] = (int)col;
+  }
+  d->ncol = ncol;
+  HCHK(hipMemcpyAsync(d->cols, c->h_cols, (size_t)ncol * sizeof(int), hipMemcpyHostToDevice, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  c->have_cols = 1;
+  if (c->prec) return emstr_prec_fields(c->prec, d, c->st);
+  return 0;
+}
+
+/* Sparse index of layer k of column col; kb-1 is the sediment ghost. */
+static int sp_at(emstr_ctx *c, ll col, int k) {
+  if (k < c->h_kb[col]) return c->sedg[col];
+  return c->sp_of[(ll)k * c->d.plane + col];
+}
+
+/* Rebuild sur_t/nsur_t, s1, i1..i3, vc.. and dz on the host structures
+ * from the device set_dz (same order as tracers.c:5762-5816). */
+static int lists_to_host(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->d;
+  int cc, vc = 1;
+  for (cc = 1; cc <= g->a2_t; cc++) {
+    ll col = c->cc2col[cc];
+    g->sur_t[cc] = sp_at(c, col, c->h_kold[col]);
+    g->nsur_t[cc] = sp_at(c, col, c->h_knew[col]);
+  }
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    ll col = c->cc2col[cc];
+    if (c->h_ks[col] < 0) continue;
+    wc->s1[vc] = sp_at(c, col, c->h_ks[col]);
+    wc->i1[vc] = g->bot_t[cc];
+    wc->i2[vc] = g->nsur_t[cc];
+    wc->i3[vc] = g->sur_t[cc];
+    if (cc == g->v2_t) wc->vcs1 = vc;
+    vc++;
+  }
+  const int vcs = vc - 1;
+  wc->vcs = vcs;
+  for (cc = 1; cc <= vcs; cc++) {
+    ll col = c->h_cols[cc - 1];
+    for (int k = c->h_ks[col] - 1; k >= c->h_kb[col]; k--) wc->s1[vc++] = sp_at(c, col, k);
+    if (cc == wc->vcs1) wc->vc1 = vc - 1;
+  }
+  wc->vc = vc - 1;
+  /* dz of wet cells */
+  if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, d->dz, c->stage, c->st))
+    return emstr_set_error(EMSTR_EDEVICE, -1, "gather");
+  double *tmp = (double *)malloc((size_t)g->sgsiz * sizeof(double));
+  HCHK(hipMemcpyAsync(tmp, c->stage, (size_t)g->sgsiz * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  for (int q = 1; q <= g->v3_t; q++) {
+    int cw = g->w3_t[q];
+    wc->dz[cw] = tmp[cw];
+  }
+  free(tmp);
+  return 0;
+}
+
+/* w from continuity on the device for the processed columns; the
+ * host's windat->w receives it. */
+int emstr_device_vel_w(emstr_ctx *c) {
+  emstr_dev *d = &c->d;
+  geometry_t *g = c->g;
+  emstr_clear_error();
+  if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
+  if (upload3d(c, d->u1flux3d, c->wd->u1flux3d) || upload3d(c, d->u2flux3d, c->wd->u2flux3d) ||
+      upload2d(c, c->d_wbot, c->wd->wbot))
+    return EMSTR_EDEVICE;
+  HCHK(hipMemsetAsync(d->w, 0, (size_t)d->n3 * sizeof(double), c->st));
+  if (emstr_launch_velw(d, c->d_wbot, c->st)) return emstr_set_error(EMSTR_EDEVICE, -1, "velw");
+  HCHK(hipMemsetAsync(c->stage, 0, (size_t)g->sgsiz * sizeof(double), c->st));
+  if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, d->w, c->stage, c->st))
+    return emstr_set_error(EMSTR_EDEVICE, -1, "gather");
+  HCHK(hipMemcpyAsync(c->wd->w, c->stage, (size_t)g->sgsiz * sizeof(double),
+                      hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  if (c->prec) return emstr_prec_fields(c->prec, d, c->st);
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* drop-in entry points                                              */
+/*-------------------------------------------------------------------*/
+void set_dz(geometry_t *window, window_t *windat, win_priv_t *wincon) {
+  emstr_ctx *c = ctx_for(window, windat, wincon);
+  if (!c) return;
+  if (emstr_device_set_dz(c)) return;
+  lists_to_host(c);
+}
+
+int advect_diffuse(geometry_t *window, window_t *windat, win_priv_t *wincon) {
+  emstr_ctx *c = ctx_for(window, windat, wincon);
+  int r;
+  if (!c) return 1;
+  if ((r = emstr_upload_fields(c)) || (r = emstr_upload_tracers(c))) return r;
+  if ((r = emstr_advect_diffuse(c))) return r;
+  if ((r = emstr_download_tracers(c))) return r;
+  return emstr_sync(c);
+}
+
+void vert_diffuse_3d(geometry_t *window, window_t *windat, win_priv_t *wincon) {
+  emstr_ctx *c = ctx_for(window, windat, wincon);
+  if (!c) return;
+  if (emstr_upload_fields(c) || emstr_upload_tracers(c)) return;
+  if (emstr_vert_diffuse(c)) return;
+  if (emstr_download_tracers(c)) return;
+  emstr_sync(c);
+}
+
+void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon) {
+  emstr_ctx *c = ctx_for(window, windat, wincon);
+  if (!c) return;
+  if (emstr_upload_fields(c) || emstr_upload_tracers(c)) return;
+  if (emstr_step(c)) return;
+  if (emstr_download_tracers(c)) return;
+  emstr_sync(c);
+}

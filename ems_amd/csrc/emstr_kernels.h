@@ -1,0 +1,40 @@
+/* C prototypes of the kernel launchers (defined in the .hip sources).
+ * All launches are asynchronous on the given stream (hipStream_t as
+ * void*); they return 0 or a negative launch error. */
+#ifndef EMSTR_KERNELS_H
+#define EMSTR_KERNELS_H
+
+#include "emstr_dev.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* plain FP64 form (emstr_plain.hip) */
+int emstr_launch_adv(const emstr_dev *d, const double *tin, double *tout,
+                     const double *tstart, double dtu, double trem, int anydif,
+                     void *stream);
+int emstr_launch_vdiff(const emstr_dev *d, double *tr, double *scratch,
+                       int *err, void *stream);
+int emstr_launch_setdz(const emstr_dev *d, int *proc, void *stream);
+int emstr_launch_subeta(const emstr_dev *d, double dtu, double trem, int init,
+                        void *stream);
+int emstr_launch_osub(const emstr_dev *d, void *stream);
+int emstr_launch_dtm(const emstr_dev *d, double dtu0, unsigned long long *out,
+                     void *stream);
+int emstr_launch_scatter(const int *wsp, const long long *wdn, int nwet,
+                         const double *sp, double *dn, void *stream);
+int emstr_launch_gather(const int *wsp, const long long *wdn, int nwet,
+                        const double *dn, double *sp, void *stream);
+int emstr_launch_ghosts(const int *g, const int *src, int n, double *a,
+                        void *stream);
+int emstr_mass_blocks(const emstr_dev *d);
+int emstr_launch_mass(const emstr_dev *d, const double *tr, double *colmass,
+                      double *part, void *stream);
+/* w from continuity on the device (per processed column) */
+int emstr_launch_velw(const emstr_dev *d, const double *wbot, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

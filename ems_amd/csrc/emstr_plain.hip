@@ -1,0 +1,1051 @@
+/*
+ * Plain form of the EMS tracer step on MI355X (gfx950): FP64 throughout,
+ * the reference's schemes, limiters and operation order, so results agree
+ * with the reference CPU path to the last bit.  Compiled with
+ * -ffp-contract=off (no FMA contraction).
+ *
+ * Kernels (dense layout described in emstr_dev.h):
+ *   k_adv      one thread per wet cell, loop over tracers inside:
+ *              face values and fluxes (van Leer / QUICKEST / QUICKEST_CO /
+ *              ORDER1 / ORDER2 / ORDER4, optional ULTIMATE and horizontal
+ *              diffusion), flux divergence and sub-surface update
+ *              (tracers.c:773-925); the processed surface cell also does
+ *              surf_conc (tracers.c:3047-3207) and tr_bounds
+ *              (tracers.c:3217-3324) for its column.
+ *   k_vdiff    64 columns x 4 tracer lanes per block: the column's
+ *              tridiagonal matrix is built and factored once
+ *              (tracers.c:3836-3889, 4185-4195) into LDS and reused by
+ *              every tracer's forward/back substitution
+ *              (tracers.c:4135-4205), in place.
+ *   k_setdz    per column: new surface, cells to process, dz
+ *              (tracers.c:5682-5849).
+ *   k_scatter / k_gather  sparse (reference numbering) <-> dense.
+ *   k_mass     per-column and domain tracer mass, FP64 with error-free
+ *              (TwoSum) accumulation (tracers.c:6608-6631).
+ *   k_dtm      minimum sub-step (tracers.c:590-655).
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "emstr_dev.h"
+#include "emstr_kernels.h"
+
+#define ORDER1 0x000002
+#define ORDER2 0x000004
+#define ORDER4 0x000010
+#define QUICKEST 0x000020
+#define QUICKEST_CO 0x000040
+#define VANLEER 0x000080
+
+typedef long long ll;
+
+/*-------------------------------------------------------------------*/
+/* Stencil geometry of one wet cell                                  */
+/*-------------------------------------------------------------------*/
+struct Sten {
+  /* element offsets from the cell for the points reached by the maps
+   * xm1, xm1(xm1), xp1, xp1(xp1) etc.; a point beyond land stops at the
+   * last wet cell (ghost semantics).  gm: bit set if the point is a
+   * lateral ghost (read from the step-start buffer). */
+  ll xm1, xm2, xp1, xp2, ym1, ym2, yp1, yp2, zm1, zm2, zp1, zp2;
+  unsigned gm;
+  bool wetE, wetN;
+};
+
+enum { G_XM1 = 1, G_XM2 = 2, G_XP1 = 4, G_XP2 = 8, G_YM1 = 16, G_YM2 = 32,
+       G_YP1 = 64, G_YP2 = 128 };
+
+__device__ __forceinline__ bool wetat(const int *__restrict__ kb, ll col, int k) {
+  return k >= kb[col];
+}
+
+/* Offsets along one horizontal direction with stride s (1 or nx). */
+__device__ __forceinline__ void hdir(const int *__restrict__ kb, ll col, int k,
+                                     ll s, ll &m1, ll &m2, ll &p1, ll &p2,
+                                     unsigned &gm, unsigned gm1, unsigned gm2,
+                                     unsigned gp1, unsigned gp2, bool &wetp) {
+  bool wm1 = wetat(kb, col - s, k);
+  bool wp1 = wetat(kb, col + s, k);
+  wetp = wp1;
+  if (wm1) {
+    m1 = -s;
+    if (wetat(kb, col - 2 * s, k)) m2 = -2 * s;
+    else { m2 = -s; gm |= gm2; }
+  } else {
+    m1 = 0; m2 = 0; gm |= gm1 | gm2;
+  }
+  if (wp1) {
+    p1 = s;
+    if (wetat(kb, col + 2 * s, k)) p2 = 2 * s;
+    else { p2 = s; gm |= gp2; }
+  } else {
+    p1 = 0; p2 = 0; gm |= gp1 | gp2;
+  }
+}
+
+__device__ __forceinline__ Sten make_sten(const emstr_dev &d, ll col, int k, int kbc) {
+  Sten s;
+  s.gm = 0;
+  hdir(d.kb, col, k, 1, s.xm1, s.xm2, s.xp1, s.xp2, s.gm, G_XM1, G_XM2, G_XP1,
+       G_XP2, s.wetE);
+  hdir(d.kb, col, k, d.nx, s.ym1, s.ym2, s.yp1, s.yp2, s.gm, G_YM1, G_YM2,
+       G_YP1, G_YP2, s.wetN);
+  const ll P = d.plane;
+  /* vertical: the bed cell maps down to the sediment ghost (which holds
+   * the bed value and maps to itself); the top layer maps to itself */
+  s.zm1 = (ll)(max(k - 1, kbc) - k) * P;
+  s.zm2 = (ll)(max(k - 2, kbc) - k) * P;
+  s.zp1 = (ll)(min(k + 1, d.nz - 1) - k) * P;
+  s.zp2 = (ll)(min(k + 2, d.nz - 1) - k) * P;
+  return s;
+}
+
+/* Tracer values on the 13-point stencil of one cell for one tracer. */
+struct Vals {
+  double c, xm1, xm2, xp1, xp2, ym1, ym2, yp1, yp2, zm1, zm2, zp1, zp2;
+};
+
+__device__ __forceinline__ Vals load_vals(const double *__restrict__ t,
+                                          const double *__restrict__ ts,
+                                          ll idx, const Sten &s) {
+  Vals v;
+  v.c = t[idx];
+  v.xm1 = ((s.gm & G_XM1) ? ts : t)[idx + s.xm1];
+  v.xm2 = ((s.gm & G_XM2) ? ts : t)[idx + s.xm2];
+  v.xp1 = ((s.gm & G_XP1) ? ts : t)[idx + s.xp1];
+  v.xp2 = ((s.gm & G_XP2) ? ts : t)[idx + s.xp2];
+  v.ym1 = ((s.gm & G_YM1) ? ts : t)[idx + s.ym1];
+  v.ym2 = ((s.gm & G_YM2) ? ts : t)[idx + s.ym2];
+  v.yp1 = ((s.gm & G_YP1) ? ts : t)[idx + s.yp1];
+  v.yp2 = ((s.gm & G_YP2) ? ts : t)[idx + s.yp2];
+  v.zm1 = t[idx + s.zm1];
+  v.zm2 = t[idx + s.zm2];
+  v.zp1 = t[idx + s.zp1];
+  v.zp2 = t[idx + s.zp2];
+  return v;
+}
+
+/*-------------------------------------------------------------------*/
+/* Face values along one direction.  bm2, bm1: two cells upstream of   */
+/* the face (for a positive velocity), c: the cell the face belongs    */
+/* to, fp1: the cell beyond c.                                         */
+/*-------------------------------------------------------------------*/
+struct QkGeom { /* QUICKEST metric weights of one face */
+  double d3;      /* interpolation weight (d3/d4/w5)                 */
+  double hm, h0, hp; /* spacings at bm1, c, fp1 (h1au1 / w9)          */
+};
+
+/* tracers.c:2097-2118 */
+__device__ __forceinline__ double vl_face(double bm2, double bm1, double c,
+                                          double fp1, double vel, double cn,
+                                          bool &set) {
+  double df;
+  set = true;
+  if (vel > 0.0) {
+    df = (bm1 - bm2) * (c - bm1);
+    if (df > 0.0)
+      df = 2.0 * df / (c - bm2);
+    else
+      df = 0.0;
+    return bm1 + 0.5 * (1.0 - cn) * df;
+  } else if (vel < 0.0) {
+    df = (c - bm1) * (fp1 - c);
+    if (df > 0.0)
+      df = 2.0 * df / (fp1 - bm1);
+    else
+      df = 0.0;
+    return c - 0.5 * (1.0 + cn) * df;
+  }
+  set = false;
+  return 0.0;
+}
+
+/* tracers.c:1862-1884 (non-uniform QUICKEST) */
+__device__ __forceinline__ double qk_face(double bm2, double bm1, double c,
+                                          double fp1, double cx,
+                                          const QkGeom &q) {
+  double db = 2.0 / (q.hm + q.h0);
+  double df = 2.0 / (q.h0 + q.hp);
+  double face = q.d3 * (c - bm1) + bm1;
+  double grad = c - bm1;
+  double curv = 0.0;
+  if (cx > 0.0)
+    curv = db * (q.hm * c + q.h0 * bm2) - 2.0 * bm1;
+  else if (cx < 0.0)
+    curv = df * (q.h0 * fp1 + q.hp * bm1) - 2.0 * c;
+  return face - 0.5 * cx * grad - (1.0 - cx * cx) * curv / 6.0;
+}
+
+/* tracers.c:1988-1996 (uniform QUICKEST) */
+__device__ __forceinline__ double qku_face(double bm2, double bm1, double c,
+                                           double fp1, double cx) {
+  double face = 0.5 * (c + bm1);
+  double grad = c - bm1;
+  double curv = 0.0;
+  if (cx > 0.0)
+    curv = c - 2.0 * bm1 + bm2;
+  else if (cx < 0.0)
+    curv = fp1 - 2.0 * c + bm1;
+  return face - 0.5 * cx * grad - (1.0 - cx * cx) * curv / 6.0;
+}
+
+/* tracers.c:3517-3553 ULTIMATE on one face value F */
+__device__ __forceinline__ double ultimate(double cr, double F, double bm2,
+                                           double bm1, double c, double fp1) {
+  double tc, tu, td, diff, nc, nu;
+  if (cr == 0.0) return F;
+  if (cr > 0.0) {
+    tc = bm1;
+    tu = bm2;
+    td = c;
+  } else {
+    tc = c;
+    tu = fp1;
+    td = bm1;
+  }
+  diff = td - tu;
+  nc = nu = 0.0;
+  if (diff) {
+    nc = (tc - tu) / diff;
+    nu = (F - tu) / diff;
+  }
+  if (nc >= 0.0 && nc <= 1.0) {
+    if (nu > 1.0)
+      F = td;
+    else if (nu < nc)
+      F = tu + nc * (td - tu);
+    else if (nu > nc / fabs(cr))
+      F = tu + nc * (td - tu) / fabs(cr);
+  } else
+    F = tu + nc * (td - tu);
+  return F;
+}
+
+/* Face inputs shared by all tracers. */
+struct Face {
+  double vel;  /* u1 / u2 / w at the face (sign for van Leer)     */
+  double cn;   /* Courant number w6 / w7 / w8                     */
+  double flux; /* u1flux3d / u2flux3d / w*dtu multiplier          */
+  QkGeom q;
+  double mgs;  /* ORDER2 weight (d3/d4/w5)                        */
+};
+
+/*
+ * Advective flux through a face for one tracer (advect(),
+ * tracers.c:1509-1574): face value by the scheme, ULTIMATE, then times
+ * the volume flux.  ORDER1 returns the upwind flux directly
+ * (tracers.c:1598-1616).  'open' is false for faces the reference never
+ * visits (index not a wet cell), whose flux stays zero.
+ */
+template <int SCH>
+__device__ __forceinline__ double face_flux(double bm2, double bm1, double c,
+                                            double fp1, const Face &f,
+                                            int ultimate_on) {
+  double F = 0.0;
+  if (SCH == ORDER1) {
+    /* f.flux holds the signed volume flux; f.vel unused */
+    double a = 0.5 * (f.flux + fabs(f.flux));
+    double b = 0.5 * (f.flux - fabs(f.flux));
+    return a * bm1 + b * c;
+  }
+  if (SCH == VANLEER) {
+    bool set;
+    F = vl_face(bm2, bm1, c, fp1, f.vel, f.cn, set);
+  } else if (SCH == QUICKEST) {
+    F = qk_face(bm2, bm1, c, fp1, f.cn, f.q);
+  } else if (SCH == QUICKEST_CO) {
+    F = qku_face(bm2, bm1, c, fp1, f.cn);
+  } else if (SCH == ORDER2) {
+    F = f.mgs * (c - bm1) + bm1;
+  } else if (SCH == ORDER4) {
+    F = (7.0 * (bm1 + c) - (fp1 + bm2)) / 12.0;
+  }
+  if (ultimate_on) F = ultimate(f.cn, F, bm2, bm1, c, fp1);
+  return F * f.flux;
+}
+
+/* ORDER1 vertical flux (tracers.c:1613-1616): already times dtu */
+__device__ __forceinline__ double o1_fz(double w, double dtu, double bm1,
+                                        double c) {
+  return dtu * (0.5 * (w + fabs(w)) * bm1 + 0.5 * (w - fabs(w)) * c);
+}
+
+/*-------------------------------------------------------------------*/
+/* Per-cell shared coefficients                                       */
+/*-------------------------------------------------------------------*/
+struct CellCo {
+  Face W, E, S, N, B, T;
+  double area, dz, dtu;
+  /* horizontal diffusion coefficients (csx*u1kh etc.) and spacings    */
+  double kW, kE, kS, kN, hW, hE, hS, hN;
+};
+
+/* dz of the cell k-1 below, 0 for the sediment ghost under the bed */
+__device__ __forceinline__ double dz_below(const emstr_dev &d, ll idx, int k, int kbc) {
+  return (k > kbc) ? d.dz[idx - d.plane] : 0.0;
+}
+
+template <int SCH>
+__device__ void cell_coef(const emstr_dev &d, ll col, ll idx, int k, int kbc,
+                          const Sten &s, double dtu, bool need_top,
+                          int courf, CellCo &cc) {
+  const ll nx = d.nx, P = d.plane;
+  cc.area = d.cellarea[col];
+  cc.dz = d.dz[idx];
+  cc.dtu = dtu;
+  /* x faces: W face of c, E face = W face of xp1 (only if wet) */
+  cc.W.vel = d.u1[idx];
+  cc.W.flux = d.u1flux3d[idx];
+  if (courf) cc.W.cn = d.u1[idx] * dtu / d.h1au1[col];
+  if (s.wetE) {
+    cc.E.vel = d.u1[idx + 1];
+    cc.E.flux = d.u1flux3d[idx + 1];
+    if (courf) cc.E.cn = d.u1[idx + 1] * dtu / d.h1au1[col + 1];
+  }
+  cc.S.vel = d.u2[idx];
+  cc.S.flux = d.u2flux3d[idx];
+  if (courf) cc.S.cn = d.u2[idx] * dtu / d.h2au2[col];
+  if (s.wetN) {
+    cc.N.vel = d.u2[idx + nx];
+    cc.N.flux = d.u2flux3d[idx + nx];
+    if (courf) cc.N.cn = d.u2[idx + nx] * dtu / d.h2au2[col + nx];
+  }
+  /* vertical: bottom face of c, top face of c = bottom face of zp1 */
+  double dzc = cc.dz;
+  double dzb = dz_below(d, idx, k, kbc);
+  double w9c = 0.5 * (dzc + dzb);
+  cc.B.vel = d.w[idx];
+  cc.B.flux = d.w[idx] * dtu;
+  if (courf) cc.B.cn = d.w[idx] * dtu / (w9c * 1.0);
+  double w9t = 0.0, dzt = 0.0;
+  if (need_top) {
+    dzt = d.dz[idx + P];
+    w9t = 0.5 * (dzt + dzc);
+    cc.T.vel = d.w[idx + P];
+    cc.T.flux = d.w[idx + P] * dtu;
+    if (courf) cc.T.cn = d.w[idx + P] * dtu / (w9t * 1.0);
+  }
+  if (SCH == QUICKEST || SCH == ORDER2) {
+    /* d3/d4 (tracers.c:566-576) use the surface-layer 2D maps, whose
+     * neighbours are the columns at i-1 / j-1 (wet or not) */
+    const double *h1c = d.h1acell, *h2c = d.h2acell;
+    double d3c = h1c[col - 1] / (h1c[col - 1] + h1c[col]);
+    double d4c = h2c[col - nx] / (h2c[col - nx] + h2c[col]);
+    cc.W.q.d3 = d3c;
+    cc.W.mgs = d3c;
+    cc.S.q.d3 = d4c;
+    cc.S.mgs = d4c;
+    if (s.wetE) {
+      double d3e = h1c[col] / (h1c[col] + h1c[col + 1]);
+      cc.E.q.d3 = d3e;
+      cc.E.mgs = d3e;
+    }
+    if (s.wetN) {
+      double d4n = h2c[col] / (h2c[col] + h2c[col + nx]);
+      cc.N.q.d3 = d4n;
+      cc.N.mgs = d4n;
+    }
+    cc.W.q.hm = d.h1au1[col - 1];
+    cc.W.q.h0 = d.h1au1[col];
+    cc.W.q.hp = d.h1au1[col + 1];
+    cc.S.q.hm = d.h2au2[col - nx];
+    cc.S.q.h0 = d.h2au2[col];
+    cc.S.q.hp = d.h2au2[col + nx];
+    if (s.wetE) {
+      cc.E.q.hm = d.h1au1[col];
+      cc.E.q.h0 = d.h1au1[col + 1];
+      cc.E.q.hp = d.h1au1[col + 2];
+    }
+    if (s.wetN) {
+      cc.N.q.hm = d.h2au2[col];
+      cc.N.q.h0 = d.h2au2[col + nx];
+      cc.N.q.hp = d.h2au2[col + 2 * nx];
+    }
+    /* vertical: w5 = dz(zm1)/(dz(zm1)+dz(c)), w9 at zm1, c, zp1
+     * (tracers.c:562-565); w9 of the sediment ghost is 0 */
+    double w5c = dzb / (dzb + dzc);
+    cc.B.q.d3 = w5c;
+    cc.B.mgs = w5c;
+    double w9m = (k > kbc) ? 0.5 * (dzb + dz_below(d, idx - P, k - 1, kbc)) : 0.0;
+    double w9p;
+    if (k + 1 < d.nz) {
+      double dzp = d.dz[idx + P];
+      w9p = 0.5 * (dzp + dzc);
+    } else {
+      w9p = w9c;
+    }
+    cc.B.q.hm = w9m;
+    cc.B.q.h0 = w9c;
+    cc.B.q.hp = w9p;
+    if (need_top) {
+      double w5t = dzc / (dzc + dzt);
+      cc.T.q.d3 = w5t;
+      cc.T.mgs = w5t;
+      cc.T.q.hm = w9c;
+      cc.T.q.h0 = w9t;
+      double w9pp;
+      if (k + 2 < d.nz) w9pp = 0.5 * (d.dz[idx + 2 * P] + dzt);
+      else w9pp = w9t;
+      cc.T.q.hp = w9pp;
+    }
+  }
+}
+
+/*
+ * Horizontal and vertical fluxes of one cell for one tracer.
+ * Returns dtracer (tracers.c:888-892); Fz at the bottom and (if
+ * need_top) the top face.
+ */
+template <int SCH>
+__device__ __forceinline__ double cell_fluxes(const emstr_dev &d,
+                                              const CellCo &cc, const Vals &v,
+                                              const Sten &s, int n, ll idx,
+                                              bool need_top, int adv, int dif,
+                                              double &FzB, double &FzT) {
+  const int ult = d.ultimate;
+  double FxW = 0.0, FxE = 0.0, FyS = 0.0, FyN = 0.0;
+  FzB = 0.0;
+  FzT = 0.0;
+  if (adv) {
+    /* W face: stencil of c;  E face: stencil of xp1 (bm1 = c) */
+    FxW = face_flux<SCH>(v.xm2, v.xm1, v.c, v.xp1, cc.W, ult);
+    if (s.wetE) FxE = face_flux<SCH>(v.xm1, v.c, v.xp1, v.xp2, cc.E, ult);
+    FyS = face_flux<SCH>(v.ym2, v.ym1, v.c, v.yp1, cc.S, ult);
+    if (s.wetN) FyN = face_flux<SCH>(v.ym1, v.c, v.yp1, v.yp2, cc.N, ult);
+    if (SCH == ORDER1) {
+      FzB = o1_fz(cc.B.vel, cc.dtu, v.zm1, v.c);
+      if (need_top) FzT = o1_fz(cc.T.vel, cc.dtu, v.c, v.zp1);
+    } else {
+      FzB = face_flux<SCH>(v.zm2, v.zm1, v.c, v.zp1, cc.B, ult);
+      if (need_top) FzT = face_flux<SCH>(v.zm1, v.c, v.zp1, v.zp2, cc.T, ult);
+    }
+  }
+  if (dif) {
+    /* tracers.c:3656-3664: Fx -= csx*u1kh*(tr[c]-tr[xm1])/h1au1 */
+    FxW -= (cc.kW * (v.c - v.xm1) / cc.hW);
+    if (s.wetE) FxE -= (cc.kE * (v.xp1 - v.c) / cc.hE);
+    FyS -= (cc.kS * (v.c - v.ym1) / cc.hS);
+    if (s.wetN) FyN -= (cc.kN * (v.yp1 - v.c) / cc.hN);
+  }
+  return (FxE - FxW + FyN - FyS) * cc.dtu;
+}
+
+__device__ __forceinline__ void diff_coef(const emstr_dev &d, ll col, ll idx,
+                                          const Sten &s, CellCo &cc) {
+  const ll nx = d.nx;
+  /* csx = dzu1*h2au1*mdx (mdx = 1), then *u1kh */
+  cc.kW = d.dzu1[idx] * d.h2au1[col] * 1.0 * d.u1kh[idx];
+  cc.hW = d.h1au1[col];
+  cc.kS = d.dzu2[idx] * d.h1au2[col] * 1.0 * d.u2kh[idx];
+  cc.hS = d.h2au2[col];
+  if (s.wetE) {
+    cc.kE = d.dzu1[idx + 1] * d.h2au1[col + 1] * 1.0 * d.u1kh[idx + 1];
+    cc.hE = d.h1au1[col + 1];
+  }
+  if (s.wetN) {
+    cc.kN = d.dzu2[idx + nx] * d.h1au2[col + nx] * 1.0 * d.u2kh[idx + nx];
+    cc.hN = d.h2au2[col + nx];
+  }
+}
+
+/* Sub-surface update of one cell for one tracer (tracers.c:912-925,
+ * z-levels: Ds = Hn1 = 1). */
+__device__ __forceinline__ double subsurface(double trc, double dtr, double area,
+                                             double FzT, double FzB, double dz) {
+  return trc - ((dtr / area + (FzT - FzB)) / dz);
+}
+
+/* Recompute the full update of the cell at layer k of column col (used
+ * for the layer below a thin surface in tr_bounds). */
+template <int SCH>
+__device__ double cell_update(const emstr_dev &d, const double *t,
+                              const double *ts, ll col, int k, int kbc,
+                              double dtu, int n, int courf, int adv, int dif) {
+  ll idx = (ll)k * d.plane + col;
+  Sten s = make_sten(d, col, k, kbc);
+  CellCo cc;
+  cell_coef<SCH>(d, col, idx, k, kbc, s, dtu, true, courf, cc);
+  if (dif) diff_coef(d, col, idx, s, cc);
+  Vals v = load_vals(t, ts, idx, s);
+  double FzB, FzT;
+  double dtr = cell_fluxes<SCH>(d, cc, v, s, n, idx, true, adv, dif, FzB, FzT);
+  return subsurface(v.c, dtr, cc.area, FzT, FzB, cc.dz);
+}
+
+/* Horizontal divergence only (for cells above the processed surface). */
+template <int SCH>
+__device__ double cell_dtracer(const emstr_dev &d, const double *t,
+                               const double *ts, ll col, int k, int kbc,
+                               double dtu, int n, int courf, int adv, int dif) {
+  ll idx = (ll)k * d.plane + col;
+  Sten s = make_sten(d, col, k, kbc);
+  CellCo cc;
+  cell_coef<SCH>(d, col, idx, k, kbc, s, dtu, false, courf, cc);
+  if (dif) diff_coef(d, col, idx, s, cc);
+  Vals v = load_vals(t, ts, idx, s);
+  double FzB, FzT;
+  return cell_fluxes<SCH>(d, cc, v, s, n, idx, false, adv, dif, FzB, FzT);
+}
+
+/*-------------------------------------------------------------------*/
+/* k_adv                                                             */
+/*-------------------------------------------------------------------*/
+template <int SCH>
+__global__ void __launch_bounds__(256)
+k_adv(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout,
+      const double *__restrict__ tstart, double dtu, double trem, int anydif) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (k < kbc) return;
+  const ll idx = (ll)k * d.plane + col;
+  const ll N3 = d.n3;
+  const int ksc = d.ks[col];
+  const int courf = (d.ultimate || d.trasc >= QUICKEST) ? 1 : 0;
+
+  if (ksc < 0) { /* column not processed: unchanged */
+    for (int n = 0; n < d.ntr; n++) tout[n * N3 + idx] = tin[n * N3 + idx];
+    return;
+  }
+  if (k > ksc) return; /* above the processed surface: written below */
+
+  const Sten s = make_sten(d, col, k, kbc);
+  const bool sub = (k < ksc);
+  CellCo cc;
+  cell_coef<SCH>(d, col, idx, k, kbc, s, dtu, sub, courf, cc);
+  if (anydif) diff_coef(d, col, idx, s, cc);
+
+  if (sub) {
+    for (int n = 0; n < d.ntr; n++) {
+      const double *t = tin + n * N3;
+      if (!d.in_tbdy[n]) {
+        tout[n * N3 + idx] = t[idx];
+        continue;
+      }
+      const int adv = d.advect[n], dif = d.diffuse[n];
+      Vals v = load_vals(t, tstart + n * N3, idx, s);
+      double FzB, FzT;
+      double dtr = cell_fluxes<SCH>(d, cc, v, s, n, idx, true, adv, dif, FzB, FzT);
+      double r = subsurface(v.c, dtr, cc.area, FzT, FzB, cc.dz);
+      /* tr_bounds clip (tracers.c:3279-3285) for s1 cells */
+      if (d.mintr[n] <= 1e-5 && r < d.mintr[n]) r = d.mintr[n];
+      tout[n * N3 + idx] = r;
+    }
+    return;
+  }
+
+  /* ---- processed surface cell of the column (k == ksc) ---- */
+  const double area = cc.area;
+  const int kold = d.kold[col];
+  const double subeta = d.subeta[col];
+  const double osubeta = d.osubeta[col];
+  const double botz = d.botz[col];
+  const double *lay = d.layers;
+  const int ktop = d.nz - 1;
+  for (int n = 0; n < d.ntr; n++) {
+    const double *t = tin + n * N3;
+    const double *ts = tstart + n * N3;
+    double *o = tout + n * N3;
+    if (!d.in_tbdy[n]) {
+      for (int kk = k; kk <= ktop; kk++) o[(ll)kk * d.plane + col] = t[(ll)kk * d.plane + col];
+      continue;
+    }
+    const int adv = d.advect[n], dif = d.diffuse[n];
+    double trs;
+    if (!adv) {
+      trs = t[idx];
+    } else {
+      Vals v = load_vals(t, ts, idx, s);
+      double FzB, FzT;
+      double dtr0 = cell_fluxes<SCH>(d, cc, v, s, n, idx, false, adv, dif, FzB, FzT);
+      /* surf_conc (tracers.c:3047-3207) */
+      double fcbot = -FzB * area;
+      int c = k;
+      double waterbot = (c == kbc) ? botz : lay[c];
+      double dzs = subeta - waterbot;
+      double svol = dzs * area;
+      double surftr = -fcbot;
+      double watertop;
+      double dtrc = dtr0;
+      while (c != kold && c < ktop) {
+        watertop = lay[c + 1];
+        surftr += (t[(ll)c * d.plane + col] * area * (watertop - waterbot) - dtrc);
+        c = c + 1;
+        waterbot = lay[c];
+        dtrc = cell_dtracer<SCH>(d, t, ts, col, c, kbc, dtu, n, courf, adv, dif);
+      }
+      watertop = osubeta;
+      waterbot = (c == kbc) ? botz : lay[c];
+      surftr += (t[(ll)c * d.plane + col] * area * (watertop - waterbot) - dtrc);
+      if (c != ktop) {
+        do {
+          c = c + 1;
+          surftr -= (cell_dtracer<SCH>(d, t, ts, col, c, kbc, dtu, n, courf, adv, dif));
+        } while (c != ktop);
+      }
+      if (svol == 0.0 || (surftr < 0.0 && dzs <= d.hmin))
+        surftr = 0.0;
+      else
+        surftr /= svol;
+      /* tracers.c:953-954 */
+      trs = (botz > 0.0 && cc.dz < d.hmin) ? t[idx] : surftr;
+    }
+    /* tr_bounds, thin surface layer (tracers.c:3250-3272): the old
+     * surface thickness dz[co] */
+    const double dzco = (kold >= kbc) ? d.dz[(ll)kold * d.plane + col] : 0.0;
+    if (k > kbc && dzco < d.hmin) {
+      if (trs < d.mintr[n] || trs > d.maxtr[n]) {
+        trs = cell_update<SCH>(d, t, ts, col, k - 1, kbc, dtu, n, courf, adv, dif);
+      }
+    }
+    /* clip (tracers.c:3279-3285) */
+    if (d.mintr[n] <= 1e-5 && trs < d.mintr[n]) trs = d.mintr[n];
+    /* the surface value fills every layer above (tracers.c:3306-3322) */
+    for (int kk = k; kk <= ktop; kk++) o[(ll)kk * d.plane + col] = trs;
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* k_vdiff: implicit vertical diffusion, in place                    */
+/*-------------------------------------------------------------------*/
+#define VD_COLS 64
+#define VD_TLANES 4
+
+__global__ void __launch_bounds__(VD_COLS *VD_TLANES)
+k_vdiff(emstr_dev d, double *__restrict__ tr, double *__restrict__ scratch,
+        int *__restrict__ err) {
+  extern __shared__ double lds[];
+  const int nz = d.nz;
+  double *Ldiv = lds;                       /* [nz][VD_COLS] */
+  double *Lud = lds + nz * VD_COLS;         /* [nz][VD_COLS] */
+  double *Lbm1 = lds + 2 * nz * VD_COLS;    /* [nz][VD_COLS] */
+  __shared__ int Lcb[VD_COLS], Lcs[VD_COLS], Lmode[VD_COLS];
+  const int lane = threadIdx.x, tl = threadIdx.y;
+  const int cidx = blockIdx.x * VD_COLS + lane;
+  const ll P = d.plane;
+  const double dt = d.dt;
+  ll col = (cidx < d.ncol) ? d.cols[cidx] : 0;
+
+  if (tl == 0) {
+    int mode = 0; /* 0 skip, 1 single layer, 2 tridiagonal */
+    int cb = 0, cs = 0;
+    if (cidx < d.ncol && d.knew[col] >= d.kb[col]) {
+      const int kbc = d.kb[col];
+      cs = d.knew[col];
+      /* matrix coefficients (tracers.c:3836-3889) */
+      int zm1 = cs - 1; /* layer below; kbc-1 is the sediment ghost */
+      double dzm1 = (zm1 >= kbc) ? d.dz[(ll)zm1 * P + col] : 0.0;
+      if (zm1 < kbc || dzm1 == 0.0) {
+        mode = 1;
+        cb = cs;
+      } else {
+        int c = cs;
+        /* walk down while c is a cell (not the ghost) with dz != 0 */
+        while (c >= kbc && d.dz[(ll)c * P + col] != 0.0) c--;
+        cb = c + 1;
+        mode = (cb < cs) ? 2 : 0; /* a zero-thickness surface: no solve */
+      }
+      if (mode == 2) {
+        /* build and factor; dzface(k) = 0.5*(dz(k-1)+dz(k)) */
+        double dzk = d.dz[(ll)cb * P + col];
+        double dzp = (cb + 1 <= cs) ? d.dz[(ll)(cb + 1) * P + col] : 0.0;
+        double Kp = d.Kz[(ll)(cb + 1) * P + col];
+        double dzf_p = 0.5 * (dzk + dzp);
+        double dzdt = dzk / dt;
+        double Cm1 = 0.0;
+        double Cp1 = -Kp / dzf_p;
+        double C = dzdt - Cp1;
+        double div = C;
+        Ldiv[cb * VD_COLS + lane] = div;
+        Lbm1[cb * VD_COLS + lane] = Cm1;
+        double Cp1prev = Cp1;
+        for (int kk = cb + 1; kk <= cs; kk++) {
+          double dzc = dzp;
+          double dzf_c = dzf_p;
+          double Kc = Kp;
+          dzdt = dzc / dt;
+          Cm1 = -Kc / dzf_c;
+          if (kk < cs) {
+            dzp = d.dz[(ll)(kk + 1) * P + col];
+            Kp = d.Kz[(ll)(kk + 1) * P + col];
+            dzf_p = 0.5 * (dzc + dzp);
+            Cp1 = -Kp / dzf_p;
+            C = dzdt - Cm1 - Cp1;
+          } else {
+            Cp1 = 0.0;
+            C = dzdt - Cm1;
+          }
+          /* forward elimination, matrix part (tracers.c:4188-4193) */
+          double ud = Cp1prev / div;
+          div = C - Cm1 * ud;
+          if (div == 0.0) atomicMax(err, (int)(kk * P + col) + 1);
+          Lud[kk * VD_COLS + lane] = ud;
+          Ldiv[kk * VD_COLS + lane] = div;
+          Lbm1[kk * VD_COLS + lane] = Cm1;
+          Cp1prev = Cp1;
+        }
+      }
+    }
+    Lcb[lane] = cb;
+    Lcs[lane] = cs;
+    Lmode[lane] = mode;
+  }
+  __syncthreads();
+  const int mode = Lmode[lane];
+  if (!mode) return;
+  const int cb = Lcb[lane], cs = Lcs[lane];
+  const ll N3 = d.n3;
+  for (int n = tl; n < d.ntr; n += VD_TLANES) {
+    if (!d.in_tdif[n]) continue;
+    double *var = tr + n * N3;
+    double *sol = scratch + n * N3;
+    if (mode == 1) {
+      /* tracers.c:4127-4133 with zero bottom and top fluxes */
+      ll ic = (ll)cs * P + col;
+      double dzc = d.dz[ic];
+      var[ic] += dt * (0.0 - 0.0) / fmax(dzc, d.hmin);
+      continue;
+    }
+    /* forward (tracers.c:4147-4195) */
+    ll ic = (ll)cb * P + col;
+    double dzdt = d.dz[ic] / dt;
+    double rhs = dzdt * var[ic] + 0.0;
+    double sp = rhs / Ldiv[cb * VD_COLS + lane];
+    sol[ic] = sp;
+    for (int kk = cb + 1; kk <= cs; kk++) {
+      ic = (ll)kk * P + col;
+      dzdt = d.dz[ic] / dt;
+      rhs = dzdt * var[ic];
+      if (kk == cs) rhs = rhs - 0.0;
+      sp = (rhs - Lbm1[kk * VD_COLS + lane] * sp) / Ldiv[kk * VD_COLS + lane];
+      sol[ic] = sp;
+    }
+    /* back substitution and update (tracers.c:4199-4205), scale = 1 */
+    ic = (ll)cs * P + col;
+    double sn = sp;
+    var[ic] += (sn - var[ic]) * 1.0;
+    for (int kk = cs - 1; kk >= cb; kk--) {
+      ic = (ll)kk * P + col;
+      double sk = sol[ic];
+      sk -= Lud[(kk + 1) * VD_COLS + lane] * sn;
+      var[ic] += (sk - var[ic]) * 1.0;
+      sn = sk;
+    }
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* k_setdz: set_dz for z-levels, per wet column (tracers.c:5722-5848) */
+/*-------------------------------------------------------------------*/
+__global__ void k_setdz(emstr_dev d, int *__restrict__ proc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (kbc >= d.nz) return;
+  const ll P = d.plane;
+  const double *lay = d.layers;
+  const double eta = d.eta[col], oldeta = d.oldeta[col];
+  /* old surface <- new surface; new surface: walk down while the lower
+   * face is at or above eta; kbc-1 stands for the sediment ghost */
+  const int kold = d.knew[col];
+  int c = d.nz - 1;
+  while (c >= kbc && lay[c] >= eta) c--;
+  const int knew = c;
+  d.kold[col] = kold;
+  d.knew[col] = knew;
+  /* processed? the lower of the two surfaces must be a cell */
+  const int klow = min(kold, knew);
+  double top = fmin(eta, oldeta);
+  double bot = 0.05 * d.hmin + d.botz[col];
+  int p = (top > bot && klow >= kbc) ? 1 : 0;
+  d.ks[col] = p ? klow : -1;
+  proc[col] = p;
+  /* dz below the old surface, bed cell from botz, copy above */
+  double tp = oldeta;
+  int c3 = kold;
+  while (c3 > kbc) {
+    double b = lay[c3];
+    d.dz[(ll)c3 * P + col] = tp - b;
+    tp = b;
+    c3--;
+  }
+  d.dz[(ll)kbc * P + col] = tp - d.botz[col];
+  /* cells above the old surface take its thickness; a ghost surface
+   * (kold < kbc) has thickness 0 */
+  double dzs = (kold >= kbc) ? d.dz[(ll)kold * P + col] : 0.0;
+  for (int kk = max(kold, kbc - 1) + 1; kk < d.nz; kk++) d.dz[(ll)kk * P + col] = dzs;
+}
+
+/*-------------------------------------------------------------------*/
+/* sub-step elevations (tracers.c:749-754, 1011-1016)                 */
+/*-------------------------------------------------------------------*/
+__global__ void k_subeta(emstr_dev d, double dtu, double trem, int init) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.ncol) return;
+  const ll col = d.cols[q];
+  if (init) d.osubeta[col] = d.oldeta[col];
+  d.subeta[col] = d.oldeta[col] +
+                  (1.0 + (dtu - trem) / d.dt) * (d.eta[col] - d.oldeta[col]);
+}
+
+__global__ void k_osub(emstr_dev d) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.ncol) return;
+  const ll col = d.cols[q];
+  d.osubeta[col] = d.subeta[col];
+}
+
+/*-------------------------------------------------------------------*/
+/* minimum sub-step over the cells to process (tracers.c:595-655);    */
+/* positive doubles compare as their bit patterns                     */
+/*-------------------------------------------------------------------*/
+__global__ void k_dtm(emstr_dev d, double dtu0, unsigned long long *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int ksc = d.ks[col];
+  const int kbc = d.kb[col];
+  if (ksc < 0 || k < kbc || k > ksc) return;
+  const ll idx = (ll)k * d.plane + col;
+  const double sf = 0.8, minval = 1e-10;
+  double dtm = dtu0;
+  double d4 = 0.5 * (d.u1[idx] + d.u1[idx + 1]);
+  if (fabs(d4) < minval) d4 = minval;
+  double d1 = sf * fabs(d.h1acell[col] / d4);
+  if (d1 < dtm) dtm = d1;
+  d4 = 0.5 * (d.u2[idx] + d.u2[idx + d.nx]);
+  if (fabs(d4) < minval) d4 = minval;
+  double d2 = sf * fabs(d.h2acell[col] / d4);
+  if (d2 < dtm) dtm = d2;
+  const bool surf = (k == ksc);
+  const ll zp = (k + 1 < d.nz) ? idx + d.plane : idx;
+  d4 = surf ? 0.0 : 0.5 * (d.w[idx] + d.w[zp]);
+  if (fabs(d4) < minval) d4 = minval;
+  double d3 = surf ? dtu0 : sf * fabs(d.dz[idx] * 1.0 / d4);
+  if (d3 > 0.0 && d3 < dtm) dtm = d3;
+  if (dtm < dtu0) atomicMin(out, (unsigned long long)__double_as_longlong(dtm));
+}
+
+/*-------------------------------------------------------------------*/
+/* sparse <-> dense                                                  */
+/*-------------------------------------------------------------------*/
+__global__ void k_scatter(const int *__restrict__ wsp, const ll *__restrict__ wdn,
+                          int nwet, const double *__restrict__ sp,
+                          double *__restrict__ dn) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nwet) dn[wdn[q]] = sp[wsp[q]];
+}
+
+__global__ void k_gather(const int *__restrict__ wsp, const ll *__restrict__ wdn,
+                         int nwet, const double *__restrict__ dn,
+                         double *__restrict__ sp) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nwet) sp[wsp[q]] = dn[wdn[q]];
+}
+
+/* ghost entries of a sparse array: dst[g[q]] = dst[src[q]] */
+__global__ void k_ghosts(const int *__restrict__ g, const int *__restrict__ src,
+                         int n, double *__restrict__ a) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) a[g[q]] = a[src[q]];
+}
+
+/*-------------------------------------------------------------------*/
+/* tracer mass: per column (fixed k order, TwoSum-compensated) then   */
+/* per block; partial (sum, err) pairs are added on the host in a    */
+/* fixed order.                                                       */
+/*-------------------------------------------------------------------*/
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+__global__ void k_mass(emstr_dev d, const double *__restrict__ tr,
+                       double *__restrict__ colmass, double *__restrict__ part) {
+  const int n = blockIdx.y;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double ss[256], se[256];
+  double s = 0.0, e = 0.0;
+  if (q < d.ncol) {
+    const ll col = d.cols[q];
+    const int kbc = d.kb[col], ksc = d.ks[col];
+    const double a = d.cellarea[col];
+    const double *t = tr + n * d.n3;
+    for (int k = ksc; k >= kbc; k--) {
+      ll idx = (ll)k * d.plane + col;
+      double m = t[idx] * a * d.dz[idx];
+      double s2, e2;
+      two_sum(s, m, s2, e2);
+      s = s2;
+      e += e2;
+    }
+    if (colmass) colmass[(ll)n * d.ncol + q] = s + e;
+  }
+  ss[threadIdx.x] = s;
+  se[threadIdx.x] = e;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      double s2, e2;
+      two_sum(ss[threadIdx.x], ss[threadIdx.x + w], s2, e2);
+      ss[threadIdx.x] = s2;
+      se[threadIdx.x] += se[threadIdx.x + w] + e2;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * ((ll)n * gridDim.x + blockIdx.x)] = ss[0];
+    part[2 * ((ll)n * gridDim.x + blockIdx.x) + 1] = se[0];
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* launchers                                                         */
+/*-------------------------------------------------------------------*/
+static dim3 grid3(const emstr_dev *d, dim3 b) {
+  return dim3((d->nx + b.x - 1) / b.x, (d->ny + b.y - 1) / b.y, d->nz);
+}
+
+extern "C" int emstr_launch_adv(const emstr_dev *d, const double *tin,
+                                double *tout, const double *tstart, double dtu,
+                                double trem, int anydif, void *stream) {
+  dim3 b(64, 4, 1);
+  dim3 g = grid3(d, b);
+  hipStream_t st = (hipStream_t)stream;
+  switch (d->trasc) {
+    case VANLEER:
+      k_adv<VANLEER><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      break;
+    case QUICKEST:
+      k_adv<QUICKEST><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      break;
+    case QUICKEST_CO:
+      k_adv<QUICKEST_CO><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      break;
+    case ORDER1:
+      k_adv<ORDER1><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      break;
+    case ORDER2:
+      k_adv<ORDER2><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      break;
+    case ORDER4:
+      k_adv<ORDER4><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      break;
+    default:
+      return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_launch_vdiff(const emstr_dev *d, double *tr,
+                                  double *scratch, int *err, void *stream) {
+  if (d->ncol == 0) return 0;
+  dim3 b(VD_COLS, VD_TLANES, 1);
+  dim3 g((d->ncol + VD_COLS - 1) / VD_COLS);
+  size_t lds = (size_t)3 * d->nz * VD_COLS * sizeof(double);
+  if (lds > 48 * 1024)
+    hipFuncSetAttribute((const void *)k_vdiff,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  k_vdiff<<<g, b, lds, (hipStream_t)stream>>>(*d, tr, scratch, err);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_launch_setdz(const emstr_dev *d, int *proc, void *stream) {
+  dim3 b(64, 4, 1);
+  dim3 g((d->nx + 63) / 64, (d->ny + 3) / 4, 1);
+  k_setdz<<<g, b, 0, (hipStream_t)stream>>>(*d, proc);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_launch_subeta(const emstr_dev *d, double dtu, double trem,
+                                   int init, void *stream) {
+  if (d->ncol == 0) return 0;
+  k_subeta<<<(d->ncol + 255) / 256, 256, 0, (hipStream_t)stream>>>(*d, dtu, trem, init);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_launch_osub(const emstr_dev *d, void *stream) {
+  if (d->ncol == 0) return 0;
+  k_osub<<<(d->ncol + 255) / 256, 256, 0, (hipStream_t)stream>>>(*d);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_launch_dtm(const emstr_dev *d, double dtu0,
+                                unsigned long long *out, void *stream) {
+  dim3 b(64, 4, 1);
+  k_dtm<<<grid3(d, b), b, 0, (hipStream_t)stream>>>(*d, dtu0, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_launch_scatter(const int *wsp, const long long *wdn,
+                                    int nwet, const double *sp, double *dn,
+                                    void *stream) {
+  if (nwet == 0) return 0;
+  k_scatter<<<(nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(wsp, wdn, nwet, sp, dn);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_launch_gather(const int *wsp, const long long *wdn,
+                                   int nwet, const double *dn, double *sp,
+                                   void *stream) {
+  if (nwet == 0) return 0;
+  k_gather<<<(nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(wsp, wdn, nwet, dn, sp);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_launch_ghosts(const int *g, const int *src, int n,
+                                   double *a, void *stream) {
+  if (n == 0) return 0;
+  k_ghosts<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(g, src, n, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_mass_blocks(const emstr_dev *d) { return (d->ncol + 255) / 256; }
+
+extern "C" int emstr_launch_mass(const emstr_dev *d, const double *tr,
+                                 double *colmass, double *part, void *stream) {
+  int nb = emstr_mass_blocks(d);
+  if (nb == 0) return 0;
+  dim3 g(nb, d->ntr, 1);
+  k_mass<<<g, 256, 0, (hipStream_t)stream>>>(*d, tr, colmass, part);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+/*-------------------------------------------------------------------*/
+/* w from continuity, bottom up, per processed column                 */
+/* (model/hd/momentum/vel3d.c:4680-4711; no sources or relaxation).   */
+/* w must be zeroed before the launch.                                */
+/*-------------------------------------------------------------------*/
+__global__ void k_velw(emstr_dev d, const double *__restrict__ wbot) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.ncol) return;
+  const ll col = d.cols[q];
+  const int kbc = d.kb[col], kold = d.kold[col];
+  const ll P = d.plane, nx = d.nx;
+  const double area = d.cellarea[col];
+  ll idx = (ll)kbc * P + col;
+  d.w[idx] = wbot[col];
+  double fcbot = d.w[idx] * area;
+  for (int c = kbc; c < kold; c++) {
+    idx = (ll)c * P + col;
+    double fctop = fcbot + d.u1flux3d[idx] - d.u1flux3d[idx + 1] +
+                   d.u2flux3d[idx] - d.u2flux3d[idx + nx];
+    d.w[idx + P] = fctop / area;
+    fcbot = fctop;
+  }
+}
+
+extern "C" int emstr_launch_velw(const emstr_dev *d, const double *wbot,
+                                 void *stream) {
+  if (d->ncol == 0) return 0;
+  k_velw<<<(d->ncol + 255) / 256, 256, 0, (hipStream_t)stream>>>(*d, wbot);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}

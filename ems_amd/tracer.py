@@ -1,0 +1,93 @@
+"""Python wrapper of the device-resident C interface (emstr_ctx).
+
+Every call goes to libemstr.so; errors raise EmstrError with the message
+and failing cell the library reports.  There is no host fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+PLAIN_F64 = 0
+PREC_F32 = 1
+
+
+class EmstrError(RuntimeError):
+    def __init__(self, code, what):
+        msg, cell = _lib.last_error()
+        super().__init__("%s failed (code %d): %s [cell %d]" % (what, code, msg, cell))
+        self.code = code
+        self.cell = cell
+
+
+def _chk(r, what):
+    if r:
+        raise EmstrError(r, what)
+
+
+class Context:
+    """Device mirror of one window (a configs.Case or raw structures)."""
+
+    def __init__(self, case, form=PLAIN_F64):
+        L = _lib.lib()
+        self.case = case
+        self.form = form
+        self.ptr = L.emstr_create(case.gp, case.wp, case.pp, form)
+        if not self.ptr:
+            raise EmstrError(-1, "emstr_create")
+
+    def set_dz(self):
+        _chk(_lib.lib().emstr_device_set_dz(self.ptr), "emstr_device_set_dz")
+
+    def vel_w(self):
+        _chk(_lib.lib().emstr_device_vel_w(self.ptr), "emstr_device_vel_w")
+
+    def upload(self):
+        L = _lib.lib()
+        _chk(L.emstr_upload_fields(self.ptr), "emstr_upload_fields")
+        _chk(L.emstr_upload_tracers(self.ptr), "emstr_upload_tracers")
+
+    def upload_fields(self):
+        _chk(_lib.lib().emstr_upload_fields(self.ptr), "emstr_upload_fields")
+
+    def upload_tracers(self):
+        _chk(_lib.lib().emstr_upload_tracers(self.ptr), "emstr_upload_tracers")
+
+    def download(self):
+        _chk(_lib.lib().emstr_download_tracers(self.ptr), "emstr_download_tracers")
+
+    def step(self, n=1):
+        L = _lib.lib()
+        for _ in range(n):
+            _chk(L.emstr_step(self.ptr), "emstr_step")
+
+    def advect_diffuse(self):
+        _chk(_lib.lib().emstr_advect_diffuse(self.ptr), "emstr_advect_diffuse")
+
+    def vert_diffuse(self):
+        _chk(_lib.lib().emstr_vert_diffuse(self.ptr), "emstr_vert_diffuse")
+
+    def sync(self):
+        _chk(_lib.lib().emstr_sync(self.ptr), "emstr_sync")
+
+    def mass(self):
+        m = np.zeros(self.case.ntr)
+        _chk(_lib.lib().emstr_tracer_mass(self.ptr, m.ctypes.data_as(C.POINTER(C.c_double))),
+             "emstr_tracer_mass")
+        return m
+
+    @property
+    def stream(self):
+        return _lib.lib().emstr_stream(self.ptr)
+
+    def close(self):
+        if self.ptr:
+            _lib.lib().emstr_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

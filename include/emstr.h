@@ -1,0 +1,244 @@
+/*
+ * emstr.h -- C-ABI of the MI355X tracer-transport step for the EMS
+ * structured (SHOC) model.
+ *
+ * The three structures below carry the fields of the reference's
+ * geometry_t (model/hd/include/sparse.h:660-1004), win_priv_t
+ * (sparse.h:246-654) and window_t (sparse.h:2393-2794) that the tracer
+ * step reads or writes, under the same names and with the same meaning:
+ * 1-based sparse arrays (index 0 unused), layer-major cell numbering with
+ * the surface layer first, self-mapping neighbour maps at boundaries and a
+ * separate ghost cell per (wet cell, direction).  Host code that already
+ * fills the reference structures can be compiled against this header (or
+ * compile emstr_host.c against the reference's sparse.h) because every
+ * access is by field name.
+ *
+ * Entry points keep the reference signatures:
+ *   tracer_step_3d   model/hd/tracers/tracers.c:239
+ *   advect_diffuse   model/hd/tracers/tracers.c:401
+ *   vert_diffuse_3d  model/hd/tracers/tracers.c:3782
+ *   set_dz           model/hd/tracers/tracers.c:5682
+ * They run on the GPU.  The reference aborts through hd_quit(); these
+ * entry points never exit: they record an error that emstr_last_error()
+ * returns, and advect_diffuse() returns non-zero as the reference does.
+ */
+#ifndef EMSTR_H
+#define EMSTR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Advection scheme flags: model/hd/include/hd_params.h:331-350 */
+#define ORDER1 0x000002
+#define ORDER2 0x000004
+#define QUICKEST_AD 0x000008
+#define ORDER4 0x000010
+#define QUICKEST 0x000020
+#define QUICKEST_CO 0x000040
+#define VANLEER 0x000080
+#define LAGRANGE 0x000100
+#define ORDER2_UW 0x002000
+#define FFSL 0x080000
+
+/* Stability flags: hd_params.h:236-242; NONE: hd_params.h:104 */
+#define ORIGINAL 1
+#define SUB_STEP 2
+#define SUB_STEP_NOSURF 4
+#define SUB_STEP_TRACER 16
+#define MULTI_DT 32
+#ifndef NONE
+#define NONE 0x80000000
+#endif
+
+/* Fraction of hmin above the bed for a column to be processed:
+ * hd_params.h:42 */
+#define DRY_FRAC 0.05
+
+/*-------------------------------------------------------------------*/
+/* Grid structure (fields of the reference geometry_t)              */
+/*-------------------------------------------------------------------*/
+typedef struct geometry {
+  int sgnum;      /* number of 3D sparse cells                      */
+  int sgnumS;     /* number of 2D sparse cells                      */
+  int sgsiz;      /* sgnum + 1                                      */
+  int sgsizS;     /* sgnumS + 1                                     */
+  int nobc;       /* open boundaries (0: closed basin)              */
+  int nce1, nce2; /* grid dimensions                                */
+  int nz;         /* number of layers                               */
+  int wn;         /* window number                                  */
+  double *layers; /* layer faces, layers[0..nz] (bottom to top)     */
+
+  int *s2i, *s2j, *s2k; /* sparse -> (i,j,k)                          */
+
+  /* 3D cells-to-process vectors: [1..v3] wet, [..b3] +OBC,
+   * [..a3] +auxiliary, [..n3] +ghosts                                */
+  int *w3_t, *w3_e1, *w3_e2;
+  int n3_t, n3_e1, n3_e2;
+  int v3_t, v3_e1, v3_e2;
+  int b3_t, b3_e1, b3_e2;
+  int a3_t, a3_e1, a3_e2;
+  /* 2D cells-to-process vectors                                     */
+  int *w2_t, *w2_e1, *w2_e2;
+  int n2_t, n2_e1, n2_e2;
+  int v2_t, v2_e1, v2_e2;
+  int b2_t, b2_e1, b2_e2;
+  int a2_t, a2_e1, a2_e2;
+
+  /* Lateral ghost cells: tr[bpt[cc]] = tr[bin[cc]], cc = 1..nbpt    */
+  int nbpt;
+  int *bpt, *bin;
+  /* Sediment ghosts below the bed and their interior cells          */
+  int ngsed;
+  int *gsed_t, *ised_t;
+
+  /* Surface / bottom vectors (size a2_t + 1)                        */
+  int *sur_t, *nsur_t, *bot_t;
+  int *m2d; /* 3D -> 2D (surface layer) sparse map                     */
+
+  /* Neighbour maps (self-mapping where no neighbour exists)         */
+  int *xp1, *xm1, *yp1, *ym1, *zp1, *zm1;
+
+  /* Horizontal metrics (2D sparse arrays)                           */
+  double *h1acell, *h2acell, *h1au1, *h2au2, *h2au1, *h1au2;
+  double *cellarea;
+  /* Vertical geometry                                               */
+  double *botz;  /* 2D: bed height (negative down)                    */
+  double *gridz; /* 3D: height of the lower face of each cell         */
+  double *cellz; /* 3D: height of the cell centre                     */
+} geometry_t;
+
+/*-------------------------------------------------------------------*/
+/* Window data (fields of the reference window_t)                   */
+/*-------------------------------------------------------------------*/
+typedef struct window {
+  int ntr;        /* number of 3D tracers                             */
+  double **tr_wc; /* tr_wc[n][1..sgnum]                               */
+  double dt;      /* 3D time step                                     */
+  double dttr;    /* tracer time step (0: no advection)               */
+  double t;       /* model time (s)                                   */
+  int nstep;
+  double *u1, *u2, *w;           /* 3D velocities (w on the lower face) */
+  double *u1flux3d, *u2flux3d;   /* 3D volume fluxes (m3/s)           */
+  double *dzu1, *dzu2;           /* face thicknesses                 */
+  double *Kz;                    /* vertical diffusivity, lower face */
+  double *eta;                   /* 2D elevation at t+1              */
+  double *topz;                  /* 2D surface height                */
+  double *wbot;                  /* 2D w at the bed                  */
+} window_t;
+
+/*-------------------------------------------------------------------*/
+/* Private window data (fields of the reference win_priv_t)         */
+/*-------------------------------------------------------------------*/
+typedef struct win_priv {
+  int trasc;      /* advection scheme flag                            */
+  int ultimate;   /* ULTIMATE limiter on/off                          */
+  int stab;       /* stability method flags                           */
+  int sigma;      /* sigma coordinates (0: z-levels)                  */
+  int thin_merge; /* merge thin surface layers                        */
+  double hmin;    /* minimum layer thickness                          */
+  int *advect;    /* per tracer: advect                               */
+  int *diffuse;   /* per tracer: horizontal diffusion                 */
+  int *tbdy;      /* tracers to advect                                */
+  int ntbdy;
+  int *tdif_v;    /* tracers to diffuse vertically                    */
+  int ntdif_v;
+  double *mintr, *maxtr; /* per tracer valid range                   */
+
+  double *oldeta; /* 2D elevation at t                                */
+  double *dz;     /* 3D cell thickness (set_dz)                       */
+  double *Ds, *Hn1, *Hs; /* sigma scaling (1 for z-levels)             */
+  double *u1kh, *u2kh;   /* 3D horizontal diffusivities               */
+  double *mdx, *mdy;     /* 2D metric factors                         */
+  double *one;           /* 2D array of ones (diffusion scale)        */
+
+  /* Cells to process (set_dz)                                       */
+  int *s1;
+  int vc, vcs, vc1, vcs1;
+  int *i1, *i2, *i3, *i4, *i5;
+
+  /* Work arrays                                                     */
+  double *w1, *w2, *w3, *w4, *w5, *w6, *w7, *w8, *w9, *w10;
+  double *d1, *d2, *d3, *d4;
+  double *v1, *v2, *v3, *v4, *v5, *v6, *v7;
+  double b1; /* current sub-step (set in advect_diffuse)             */
+} win_priv_t;
+
+/*-------------------------------------------------------------------*/
+/* Drop-in entry points (reference signatures).  They run the step  */
+/* on the GPU: fields are uploaded, the tracers are updated on the  */
+/* device and copied back into windat->tr_wc in place.              */
+/*-------------------------------------------------------------------*/
+/* model/hd/tracers/tracers.c:239 */
+void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon);
+/* model/hd/tracers/tracers.c:401 ; returns 0 on success */
+int advect_diffuse(geometry_t *window, window_t *windat, win_priv_t *wincon);
+/* model/hd/tracers/tracers.c:3782 */
+void vert_diffuse_3d(geometry_t *window, window_t *windat, win_priv_t *wincon);
+/* model/hd/tracers/tracers.c:5682 (z-level branch) */
+void set_dz(geometry_t *window, window_t *windat, win_priv_t *wincon);
+
+/*-------------------------------------------------------------------*/
+/* Device-resident interface (no counterpart in the reference: the  */
+/* reference keeps its state on the host).  A context mirrors one   */
+/* window on the GPU; fields and tracers stay in HBM between steps. */
+/*-------------------------------------------------------------------*/
+typedef struct emstr_ctx emstr_ctx;
+
+/* Arithmetic forms */
+#define EMSTR_PLAIN_F64 0   /* reference arithmetic, FP64 throughout   */
+#define EMSTR_PREC_F32 1    /* FP32 stencils/sweeps, FP64 budgets/IR   */
+
+/* Error codes */
+#define EMSTR_OK 0
+#define EMSTR_ESUBSTEP 1    /* sub-step limit exceeded (tracers.c:664) */
+#define EMSTR_EUNSUPPORTED 2
+#define EMSTR_EDEVICE 3
+#define EMSTR_EZEROPIVOT 4  /* zero divisor in the column solve        */
+#define EMSTR_EARG 5
+
+/* Create a device mirror of a window.  form selects the arithmetic.   */
+emstr_ctx *emstr_create(geometry_t *window, window_t *windat,
+                        win_priv_t *wincon, int form);
+void emstr_destroy(emstr_ctx *ctx);
+/* Host -> device: geometry-dependent data, fields, tracers.           */
+int emstr_upload_fields(emstr_ctx *ctx);
+int emstr_upload_tracers(emstr_ctx *ctx);
+/* Device -> host: tracers into windat->tr_wc (in place).              */
+int emstr_download_tracers(emstr_ctx *ctx);
+/* One full tracer step (advect_diffuse + vert_diffuse_3d) on device.  */
+int emstr_step(emstr_ctx *ctx);
+/* Only the advection / only the vertical diffusion part.              */
+int emstr_advect_diffuse(emstr_ctx *ctx);
+int emstr_vert_diffuse(emstr_ctx *ctx);
+/* set_dz on the device (z-level branch of tracers.c:5682-5849) from
+ * windat->eta, wincon->oldeta and window->nsur_t; keeps the per-column
+ * results on the device (the drop-in set_dz() also writes the host lists). */
+int emstr_device_set_dz(emstr_ctx *ctx);
+/* w from continuity on the device (model/hd/momentum/vel3d.c:4680-4711)
+ * from windat->u1flux3d/u2flux3d/wbot; copies w into windat->w.         */
+int emstr_device_vel_w(emstr_ctx *ctx);
+/* Domain tracer mass sum(tr*cellarea*dz) over s1, FP64 compensated.   */
+int emstr_tracer_mass(emstr_ctx *ctx, double *mass /* [ntr] */);
+/* Per-kernel timing with HIP events on the context's stream: enable
+ * (and reset), then read accumulated milliseconds and launch counts per
+ * kernel class [advection, vertical diffusion, prep, other, fused].   */
+int emstr_profile(emstr_ctx *ctx, int on);
+int emstr_kernel_times(emstr_ctx *ctx, double *ms /* [5] */, long long *count /* [5] */);
+/* Synchronise the context's stream.                                   */
+int emstr_sync(emstr_ctx *ctx);
+/* Stream the context runs on (hipStream_t as void*).                  */
+void *emstr_stream(emstr_ctx *ctx);
+
+/* Release the device mirror the drop-in entry points keep for a window
+ * (call before freeing or reusing the window's arrays).               */
+void emstr_forget_window(geometry_t *window);
+
+/* Last error text (thread-local) and the failing cell, if any.        */
+const char *emstr_last_error(void);
+int emstr_last_error_cell(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

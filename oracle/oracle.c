@@ -1,0 +1,942 @@
+/*
+ * ORACLE -- test infrastructure only.  Not part of the product.
+ *
+ * A CPU restatement, in strict IEEE FP64, of the EMS SHOC tracer step
+ * (model/hd/tracers/tracers.c) for one window in z-level coordinates with
+ * closed lateral boundaries, no point sources, regions or multi-dt.  Each
+ * function follows the reference function cited above it with the same
+ * loop order and the same order of floating-point operations, so that a
+ * GPU path evaluated the same way agrees bit for bit.  Built with
+ * gcc -O3 -fno-fast-math -ffp-contract=off.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+ * load this library, and only as the checker or the timed CPU baseline.
+ *
+ * How each function is pinned (tests/test_oracle.py):
+ *   set_dz             brute-force column walk on small grids; dz sums
+ *   van_leer_do        closed forms: linear profile, constants, step (TVD)
+ *   quickest(+uniform) closed form on a linear profile, constancy
+ *   ulimate_filter     normalised-variable cases by hand
+ *   order1/2/4         closed forms on linear profiles
+ *   update + surf_conc closed-basin mass conservation, constancy
+ *   tr_bounds          clip and copy-up cases by hand
+ *   implicit_vdiff_tr  dense solve (numpy) and the discrete decay factor
+ *   check_tmass        direct sum
+ * The reference's own tests hold no golden vectors for this path, and its
+ * executables cannot be built here (no NetCDF), so nothing is compared
+ * against reference output directly.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/emstr.h"
+
+#define max(a, b) ((a) > (b) ? (a) : (b))
+#define min(a, b) ((a) < (b) ? (a) : (b))
+
+static int oracle_err = 0;
+int oracle_last_error(void) { return oracle_err; }
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:5574-5593  no-gradient lateral boundary: ghost = bin      */
+/*-------------------------------------------------------------------*/
+void oracle_set_lateral_BC_tr(geometry_t *g, window_t *wd) {
+  int cc, n;
+  for (cc = 1; cc <= g->nbpt; cc++) {
+    int gb = g->bpt[cc], in = g->bin[cc];
+    for (n = 0; n < wd->ntr; n++) wd->tr_wc[n][gb] = wd->tr_wc[n][in];
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:5682-5849  set_dz, z-level branch (thin_merge not taken) */
+/*-------------------------------------------------------------------*/
+void oracle_set_dz(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int cc, c, cs, zm1, vc, vcs;
+  double top, bot;
+
+  /* new surface: walk down while the lower face is at or above eta */
+  for (cc = 1; cc <= g->a2_t; cc++) {
+    c = cs = g->w2_t[cc];
+    g->sur_t[cc] = g->nsur_t[cc];
+    zm1 = g->zm1[c];
+    while (c != zm1 && g->gridz[c] >= wd->eta[cs]) {
+      c = zm1;
+      zm1 = g->zm1[c];
+    }
+    g->nsur_t[cc] = c;
+  }
+
+  /* surface cells to process: lower of old and new surface */
+  vc = 1;
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    c = max(g->sur_t[cc], g->nsur_t[cc]);
+    cs = g->m2d[c];
+    top = min(wd->eta[cs], wc->oldeta[cs]);
+    bot = DRY_FRAC * wc->hmin + g->botz[cs];
+    if (top > bot && c != g->zm1[c]) {
+      wc->s1[vc] = c;
+      wc->i1[vc] = g->bot_t[cc];
+      wc->i2[vc] = g->nsur_t[cc];
+      wc->i3[vc] = g->sur_t[cc];
+      if (cc == g->v2_t) wc->vcs1 = vc;
+      vc++;
+    }
+  }
+  vcs = vc - 1;
+  wc->vcs = vcs;
+
+  /* sub-surface cells, column by column down to the bed */
+  for (cc = 1; cc <= vcs; cc++) {
+    c = g->zm1[wc->s1[cc]];
+    zm1 = g->zm1[c];
+    while (c != zm1) {
+      wc->s1[vc] = c;
+      vc++;
+      c = zm1;
+      zm1 = g->zm1[c];
+    }
+    if (cc == wc->vcs1) wc->vc1 = vc - 1;
+  }
+  vc--;
+  wc->vc = vc;
+
+  /* thicknesses below the old surface; bed cell from botz; cells above
+   * the old surface take the surface thickness */
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    int c3, c2, cbot;
+    c = c3 = g->sur_t[cc];
+    c2 = g->m2d[c3];
+    cbot = g->bot_t[cc];
+    top = wc->oldeta[c2];
+    while (c3 < cbot) {
+      bot = g->gridz[c3];
+      wc->dz[c3] = top - bot;
+      top = bot;
+      c3 = g->zm1[c3];
+    }
+    wc->dz[cbot] = top - g->botz[c2];
+    c3 = c;
+    while (c > c2) {
+      c = g->zp1[c];
+      wc->dz[c] = wc->dz[c3];
+    }
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:2082-2119  van Leer face value along one direction       */
+/*-------------------------------------------------------------------*/
+static void vl_do(double *F, const double *tr, const double *vel,
+                  const double *cn, int ss, int se, const int *sdo,
+                  const int *fmap, const int *bmap) {
+  int cc;
+  for (cc = ss; cc <= se; cc++) {
+    int c = sdo[cc];
+    int bm1 = bmap[c];
+    int bm2 = bmap[bm1];
+    double df;
+    if (vel[c] > 0.0) {
+      df = (tr[bm1] - tr[bm2]) * (tr[c] - tr[bm1]);
+      if (df > 0.0)
+        df = 2.0 * df / (tr[c] - tr[bm2]);
+      else
+        df = 0.0;
+      F[c] = tr[bm1] + 0.5 * (1.0 - cn[c]) * df;
+    } else if (vel[c] < 0.0) {
+      df = (tr[c] - tr[bm1]) * (tr[fmap[c]] - tr[c]);
+      if (df > 0.0)
+        df = 2.0 * df / (tr[fmap[c]] - tr[bm1]);
+      else
+        df = 0.0;
+      F[c] = tr[c] - 0.5 * (1.0 + cn[c]) * df;
+    }
+  }
+}
+
+/* tracers.c:2037-2072 */
+static void vl_faces(geometry_t *g, window_t *wd, win_priv_t *wc,
+                     const double *tr, double *Fx, double *Fy, double *Fz) {
+  memset(Fx, 0, g->n3_t * sizeof(double));
+  vl_do(Fx, tr, wd->u1, wc->w6, 1, g->v3_e1, g->w3_e1, g->xp1, g->xm1);
+  vl_do(Fx, tr, wd->u1, wc->w6, g->v3_t + 1, g->a3_t, g->w3_t, g->xp1, g->xm1);
+  memset(Fy, 0, g->n3_t * sizeof(double));
+  vl_do(Fy, tr, wd->u2, wc->w7, 1, g->v3_e2, g->w3_e2, g->yp1, g->ym1);
+  vl_do(Fy, tr, wd->u2, wc->w7, g->v3_t + 1, g->a3_t, g->w3_t, g->yp1, g->ym1);
+  memset(Fz, 0, g->n3_t * sizeof(double));
+  vl_do(Fz, tr, wd->w, wc->w8, 1, wc->vc, wc->s1, g->zp1, g->zm1);
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:1819-1931  QUICKEST, non-uniform grid                    */
+/*-------------------------------------------------------------------*/
+static void qk_faces(geometry_t *g, win_priv_t *wc, const double *tr,
+                     double *Fx, double *Fy, double *Fz) {
+  int cc;
+  memset(Fx, 0, g->n3_t * sizeof(double));
+  memset(Fy, 0, g->n3_t * sizeof(double));
+  memset(Fz, 0, g->n3_t * sizeof(double));
+  for (cc = 1; cc <= g->a3_t; cc++) {
+    int c = g->w3_t[cc];
+    int c2 = g->m2d[c];
+    int xp1 = g->xp1[c], yp1 = g->yp1[c];
+    int xm1 = g->xm1[c], ym1 = g->ym1[c];
+    int xm2 = g->xm1[xm1], ym2 = g->ym1[ym1];
+    int xm1s = g->xm1[c2], xp1s = g->xp1[c2];
+    int ym1s = g->ym1[c2], yp1s = g->yp1[c2];
+    double cx, cy, db, df, face, grad, curv;
+
+    cx = wc->w6[c];
+    db = 2.0 / (g->h1au1[xm1s] + g->h1au1[c2]);
+    df = 2.0 / (g->h1au1[c2] + g->h1au1[xp1s]);
+    face = wc->d3[c2] * (tr[c] - tr[xm1]) + tr[xm1];
+    grad = tr[c] - tr[xm1];
+    curv = 0.0;
+    if (cx > 0.0)
+      curv = db * (g->h1au1[xm1s] * tr[c] + g->h1au1[c2] * tr[xm2]) - 2.0 * tr[xm1];
+    else if (cx < 0.0)
+      curv = df * (g->h1au1[c2] * tr[xp1] + g->h1au1[xp1s] * tr[xm1]) - 2.0 * tr[c];
+    Fx[c] = face - 0.5 * cx * grad - (1.0 - cx * cx) * curv / 6.0;
+
+    cy = wc->w7[c];
+    db = 2.0 / (g->h2au2[ym1s] + g->h2au2[c2]);
+    df = 2.0 / (g->h2au2[c2] + g->h2au2[yp1s]);
+    face = wc->d4[c2] * (tr[c] - tr[ym1]) + tr[ym1];
+    grad = tr[c] - tr[ym1];
+    curv = 0.0;
+    if (cy > 0.0)
+      curv = db * (g->h2au2[ym1s] * tr[c] + g->h2au2[c2] * tr[ym2]) - 2.0 * tr[ym1];
+    else if (cy < 0.0)
+      curv = df * (g->h2au2[c2] * tr[yp1] + g->h2au2[yp1s] * tr[ym1]) - 2.0 * tr[c];
+    Fy[c] = face - 0.5 * cy * grad - (1.0 - cy * cy) * curv / 6.0;
+  }
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int zp1 = g->zp1[c], zm1 = g->zm1[c];
+    int zm2 = g->zm1[zm1];
+    double cz = wc->w8[c];
+    double db = 2.0 / (wc->w9[zm1] + wc->w9[c]);
+    double df = 2.0 / (wc->w9[c] + wc->w9[zp1]);
+    double face = wc->w5[c] * (tr[c] - tr[zm1]) + tr[zm1];
+    double grad = tr[c] - tr[zm1];
+    double curv = 0.0;
+    if (cz > 0.0)
+      curv = db * (wc->w9[zm1] * tr[c] + wc->w9[c] * tr[zm2]) - 2.0 * tr[zm1];
+    else if (cz < 0.0)
+      curv = df * (wc->w9[c] * tr[zp1] + wc->w9[zp1] * tr[zm1]) - 2.0 * tr[c];
+    Fz[c] = face - 0.5 * cz * grad - (1.0 - cz * cz) * curv / 6.0;
+  }
+}
+
+/* tracers.c:1945-2026  QUICKEST, uniform grid */
+static void qku_faces(geometry_t *g, win_priv_t *wc, const double *tr,
+                      double *Fx, double *Fy, double *Fz) {
+  int cc;
+  memset(Fx, 0, g->n3_t * sizeof(double));
+  memset(Fy, 0, g->n3_t * sizeof(double));
+  memset(Fz, 0, g->n3_t * sizeof(double));
+  for (cc = 1; cc <= g->a3_t; cc++) {
+    int c = g->w3_t[cc];
+    int xp1 = g->xp1[c], yp1 = g->yp1[c];
+    int xm1 = g->xm1[c], ym1 = g->ym1[c];
+    int xm2 = g->xm1[xm1], ym2 = g->ym1[ym1];
+    double cx = wc->w6[c];
+    double face = 0.5 * (tr[c] + tr[xm1]);
+    double grad = tr[c] - tr[xm1];
+    double curv = 0.0;
+    if (cx > 0.0)
+      curv = tr[c] - 2.0 * tr[xm1] + tr[xm2];
+    else if (cx < 0.0)
+      curv = tr[xp1] - 2.0 * tr[c] + tr[xm1];
+    Fx[c] = face - 0.5 * cx * grad - (1.0 - cx * cx) * curv / 6.0;
+    double cy = wc->w7[c];
+    face = 0.5 * (tr[c] + tr[ym1]);
+    grad = tr[c] - tr[ym1];
+    curv = 0.0;
+    if (cy > 0.0)
+      curv = tr[c] - 2.0 * tr[ym1] + tr[ym2];
+    else if (cy < 0.0)
+      curv = tr[yp1] - 2.0 * tr[c] + tr[ym1];
+    Fy[c] = face - 0.5 * cy * grad - (1.0 - cy * cy) * curv / 6.0;
+  }
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int zp1 = g->zp1[c], zm1 = g->zm1[c];
+    int zm2 = g->zm1[zm1];
+    double cz = wc->w8[c];
+    double face = 0.5 * (tr[c] + tr[zm1]);
+    double grad = tr[c] - tr[zm1];
+    double curv = 0.0;
+    if (cz > 0.0)
+      curv = tr[c] - 2.0 * tr[zm1] + tr[zm2];
+    else if (cz < 0.0)
+      curv = tr[zp1] - 2.0 * tr[c] + tr[zm1];
+    Fz[c] = face - 0.5 * cz * grad - (1.0 - cz * cz) * curv / 6.0;
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:1586-1620  first-order upwind (fluxes, not face values)  */
+/*-------------------------------------------------------------------*/
+static void o1_faces(geometry_t *g, window_t *wd, win_priv_t *wc,
+                     const double *tr, double *Fx, double *Fy, double *Fz,
+                     double dtu) {
+  int cc;
+  for (cc = 1; cc <= g->n3_t; cc++) {
+    int c = g->w3_t[cc];
+    Fx[c] = wc->w6[c] * tr[g->xm1[c]] + wc->w7[c] * tr[c];
+    Fy[c] = wc->w8[c] * tr[g->ym1[c]] + wc->w9[c] * tr[c];
+  }
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int zm1 = g->zm1[c];
+    double w = wd->w[c];
+    Fz[c] = dtu * (0.5 * (w + fabs(w)) * tr[zm1] + 0.5 * (w - fabs(w)) * tr[c]);
+  }
+}
+
+/* tracers.c:1634-1727  second order, non-uniform */
+static void o2_do(double *F, const double *tr, const double *mgs, int ss,
+                  int se, const int *sdo, const int *bmap, const int *m2d) {
+  int cc;
+  for (cc = ss; cc <= se; cc++) {
+    int c = sdo[cc];
+    int bm1 = bmap[c];
+    F[c] = mgs[m2d[c]] * (tr[c] - tr[bm1]) + tr[bm1];
+  }
+}
+static void o2_faces(geometry_t *g, win_priv_t *wc, const double *tr,
+                     double *Fx, double *Fy, double *Fz) {
+  int cc;
+  memset(Fx, 0, g->n3_t * sizeof(double));
+  o2_do(Fx, tr, wc->d3, 1, g->v3_e1, g->w3_e1, g->xm1, g->m2d);
+  o2_do(Fx, tr, wc->d3, g->v3_t + 1, g->a3_t, g->w3_t, g->xm1, g->m2d);
+  memset(Fy, 0, g->n3_t * sizeof(double));
+  o2_do(Fy, tr, wc->d4, 1, g->v3_e2, g->w3_e2, g->ym1, g->m2d);
+  o2_do(Fy, tr, wc->d4, g->v3_t + 1, g->a3_t, g->w3_t, g->ym1, g->m2d);
+  memset(Fz, 0, g->n3_t * sizeof(double));
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int zm1 = g->zm1[c];
+    Fz[c] = wc->w5[c] * (tr[c] - tr[zm1]) + tr[zm1];
+  }
+}
+
+/* tracers.c:1741-1810  fourth order, uniform grid */
+static void o4_do(double *F, const double *tr, int ss, int se,
+                  const int *sdo, const int *bmap, const int *fmap) {
+  int cc;
+  for (cc = ss; cc <= se; cc++) {
+    int c = sdo[cc];
+    int bm1 = bmap[c];
+    int bm2 = bmap[bm1];
+    int fp1 = fmap[c];
+    F[c] = (7.0 * (tr[bm1] + tr[c]) - (tr[fp1] + tr[bm2])) / 12.0;
+  }
+}
+static void o4_faces(geometry_t *g, win_priv_t *wc, const double *tr,
+                     double *Fx, double *Fy, double *Fz) {
+  memset(Fx, 0, g->n3_t * sizeof(double));
+  o4_do(Fx, tr, 1, g->v3_e1, g->w3_e1, g->xm1, g->xp1);
+  o4_do(Fx, tr, g->v3_t + 1, g->a3_t, g->w3_t, g->xm1, g->xp1);
+  memset(Fy, 0, g->n3_t * sizeof(double));
+  o4_do(Fy, tr, 1, g->v3_e2, g->w3_e2, g->ym1, g->yp1);
+  o4_do(Fy, tr, g->v3_t + 1, g->a3_t, g->w3_t, g->ym1, g->yp1);
+  memset(Fz, 0, g->n3_t * sizeof(double));
+  o4_do(Fz, tr, 1, wc->vc, wc->s1, g->zm1, g->zp1);
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:3501-3557  ULTIMATE limiter on face values               */
+/*-------------------------------------------------------------------*/
+static void ultimate(const double *cr, double *Fx, const double *F,
+                     const int *ctp, int vs, int ve, const int *mapm,
+                     const int *mapp) {
+  int cc;
+  for (cc = vs; cc <= ve; cc++) {
+    int c = ctp[cc];
+    double tc, tu, td, diff, nc, nu;
+    if (cr[c] == 0.0) continue;
+    if (cr[c] > 0.0) {
+      int m1 = mapm[c];
+      tc = F[m1];
+      tu = F[mapm[m1]];
+      td = F[c];
+    } else {
+      tc = F[c];
+      tu = F[mapp[c]];
+      td = F[mapm[c]];
+    }
+    diff = td - tu;
+    nc = nu = 0.0;
+    if (diff) {
+      nc = (tc - tu) / diff;
+      nu = (Fx[c] - tu) / diff;
+    }
+    if (nc >= 0.0 && nc <= 1.0) {
+      if (nu > 1.0)
+        Fx[c] = td;
+      else if (nu < nc)
+        Fx[c] = tu + nc * (td - tu);
+      else if (nu > nc / fabs(cr[c]))
+        Fx[c] = tu + nc * (td - tu) / fabs(cr[c]);
+    } else
+      Fx[c] = tu + nc * (td - tu);
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:1487-1576  face values -> fluxes                         */
+/*-------------------------------------------------------------------*/
+static void oracle_advect(geometry_t *g, window_t *wd, win_priv_t *wc,
+                          const double *tr, double *Fx, double *Fy,
+                          double *Fz, double dtu) {
+  int cc;
+  if (wd->dttr == 0.0) return;
+  if (wc->trasc == ORDER1) {
+    o1_faces(g, wd, wc, tr, Fx, Fy, Fz, dtu);
+    return;
+  }
+  if (wc->trasc == ORDER2)
+    o2_faces(g, wc, tr, Fx, Fy, Fz);
+  else if (wc->trasc == ORDER4)
+    o4_faces(g, wc, tr, Fx, Fy, Fz);
+  else if (wc->trasc == QUICKEST)
+    qk_faces(g, wc, tr, Fx, Fy, Fz);
+  else if (wc->trasc == QUICKEST_CO)
+    qku_faces(g, wc, tr, Fx, Fy, Fz);
+  else if (wc->trasc == VANLEER)
+    vl_faces(g, wd, wc, tr, Fx, Fy, Fz);
+
+  if (wc->ultimate) {
+    ultimate(wc->w6, Fx, tr, g->w3_e1, 1, g->v3_e1, g->xm1, g->xp1);
+    ultimate(wc->w6, Fx, tr, g->w3_t, g->v3_t + 1, g->a3_t, g->xm1, g->xp1);
+    ultimate(wc->w7, Fy, tr, g->w3_e2, 1, g->v3_e2, g->ym1, g->yp1);
+    ultimate(wc->w7, Fy, tr, g->w3_t, g->v3_t + 1, g->a3_t, g->ym1, g->yp1);
+    ultimate(wc->w8, Fz, tr, wc->s1, 1, wc->vc, g->zm1, g->zp1);
+  }
+  for (cc = 1; cc <= g->a3_t; cc++) {
+    int c = g->w3_t[cc];
+    Fx[c] *= wd->u1flux3d[c];
+    Fy[c] *= wd->u2flux3d[c];
+  }
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    Fz[c] *= (wd->w[c] * dtu);
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:3595-3668  horizontal diffusion (z-levels)               */
+/*-------------------------------------------------------------------*/
+static void oracle_hor_diffuse(geometry_t *g, window_t *wd, win_priv_t *wc,
+                               const double *tr, double *Fx, double *Fy) {
+  int cc;
+  for (cc = 1; cc <= g->a3_t; cc++) {
+    int c = g->w3_t[cc];
+    int c2 = g->m2d[c];
+    int xm1 = g->xm1[c], ym1 = g->ym1[c];
+    double csx = wd->dzu1[c] * g->h2au1[c2] * wc->mdx[c2];
+    double csy = wd->dzu2[c] * g->h1au2[c2] * wc->mdy[c2];
+    Fx[c] -= (csx * wc->u1kh[c] * (tr[c] - tr[xm1]) / g->h1au1[c2]);
+    Fy[c] -= (csy * wc->u2kh[c] * (tr[c] - tr[ym1]) / g->h2au2[c2]);
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:3047-3207  surface-cell concentration by mass            */
+/*-------------------------------------------------------------------*/
+static double oracle_surf_conc(geometry_t *g, window_t *wd, win_priv_t *wc,
+                               const double *tr, double subeta, int c,
+                               int c2, int cc, double fcbot,
+                               const double *dtracer) {
+  int cb = wc->i1[cc];
+  int co = wc->i3[cc];
+  double osubeta = wc->d1[c2];
+  double waterbot = (c == cb) ? g->botz[c2] : g->gridz[c];
+  double dz = subeta - waterbot;
+  double svol = dz * g->cellarea[c2];
+  double surftr = -fcbot;
+  double watertop;
+  (void)wd;
+  while (c != co) {
+    int zp1 = g->zp1[c];
+    watertop = g->gridz[zp1];
+    surftr += (tr[c] * g->cellarea[c2] * (watertop - waterbot) - dtracer[c]);
+    c = zp1;
+    waterbot = g->gridz[c];
+  }
+  watertop = osubeta;
+  waterbot = (c == cb) ? g->botz[c2] : g->gridz[c];
+  surftr += (tr[c] * g->cellarea[c2] * (watertop - waterbot) - dtracer[c]);
+  if (c != c2) {
+    do {
+      c = g->zp1[c];
+      surftr -= (dtracer[c]);
+    } while (c != c2);
+  }
+  if (svol == 0.0 || (surftr < 0.0 && dz <= wc->hmin))
+    surftr = 0.0;
+  else
+    surftr /= svol;
+  return surftr;
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:3217-3324  range repair, clip, copy to layers above      */
+/*-------------------------------------------------------------------*/
+void oracle_tr_bounds(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int cc, nn;
+  double eps = 1e-5;
+  double *mn = wc->mintr, *mx = wc->maxtr;
+  for (cc = 1; cc <= wc->vcs1; cc++) {
+    int c = wc->s1[cc], c3 = c;
+    int co = wc->i3[cc];
+    int cb = wc->i1[cc];
+    if (c < cb && wc->dz[co] < wc->hmin) {
+      for (nn = 0; nn < wc->ntbdy; nn++) {
+        int n = wc->tbdy[nn];
+        double *tr = wd->tr_wc[n];
+        if (tr[c] < mn[n] || tr[c] > mx[n]) {
+          double surftr = tr[g->zm1[c]];
+          c3 = c;
+          tr[c3] = surftr;
+          while (c3 != g->zp1[c3]) {
+            c3 = g->zp1[c3];
+            tr[c3] = surftr;
+          }
+        }
+      }
+    }
+  }
+  for (cc = 1; cc <= wc->vc1; cc++) {
+    int c = wc->s1[cc];
+    for (nn = 0; nn < wc->ntbdy; nn++) {
+      int n = wc->tbdy[nn];
+      double *tr = wd->tr_wc[n];
+      if (mn[n] <= eps && tr[c] < mn[n]) tr[c] = mn[n];
+    }
+  }
+  if (!wc->sigma) {
+    for (cc = 1; cc <= wc->vcs; cc++) {
+      int c = wc->s1[cc], cs = c;
+      while (c != g->zp1[c]) {
+        c = g->zp1[c];
+        for (nn = 0; nn < wc->ntbdy; nn++) {
+          int n = wc->tbdy[nn];
+          wd->tr_wc[n][c] = wd->tr_wc[n][cs];
+        }
+      }
+    }
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* One tracer through one sub-step: the body of the tracer loop of    */
+/* advect_diffuse, tracers.c:773-1000 (closed boundaries, z-levels).  */
+/*-------------------------------------------------------------------*/
+typedef struct {
+  double *Fx, *Fy, *Fz, *dtracer;
+} scratch_t;
+
+static void tracer_substep(geometry_t *g, window_t *wd, win_priv_t *wc,
+                           int n, double dtu, int slf, int ksf,
+                           const double *subeta, scratch_t *s) {
+  int cc;
+  double *tr = wd->tr_wc[n];
+  double *Fx = s->Fx, *Fy = s->Fy, *Fz = s->Fz, *dtracer = s->dtracer;
+  const size_t nb = (size_t)g->sgsiz * sizeof(double);
+  int kef = wc->vc1, kefs = wc->vcs1;
+  tr[0] = (double)n;
+  memset(Fx, 0, nb);
+  memset(Fy, 0, nb);
+  memset(Fz, 0, nb);
+  memset(dtracer, 0, nb);
+  /* bed: no-gradient into the sediment ghost */
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    int cb = g->bot_t[cc];
+    tr[g->zm1[cb]] = tr[cb];
+  }
+  if (wc->advect[n]) oracle_advect(g, wd, wc, tr, Fx, Fy, Fz, dtu);
+  if (wc->diffuse[n]) oracle_hor_diffuse(g, wd, wc, tr, Fx, Fy);
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    int c = g->w3_t[cc];
+    dtracer[c] = (Fx[g->xp1[c]] - Fx[c] + Fy[g->yp1[c]] - Fy[c]) * dtu;
+  }
+  for (cc = ksf; cc <= kef; cc++) {
+    int c = wc->s1[cc];
+    int c2 = g->m2d[c];
+    int zp1 = g->zp1[c];
+    if (slf)
+      tr[c] *= wc->Ds[c2];
+    else
+      tr[c] *= wc->Hn1[c2];
+    tr[c] -= ((dtracer[c] / g->cellarea[c2] + (Fz[zp1] - Fz[c])) / wc->dz[c]);
+    tr[c] /= wc->Hn1[c2];
+  }
+  if (!wc->sigma && wc->advect[n]) {
+    for (cc = 1; cc <= kefs; cc++) {
+      int c = wc->s1[cc];
+      int c2 = g->m2d[c];
+      double top = -Fz[c] * g->cellarea[c2];
+      double surftr = oracle_surf_conc(g, wd, wc, tr, subeta[c2], c, c2, cc,
+                                       top, dtracer);
+      tr[c] = (g->botz[c2] > 0.0 && wc->dz[c] < wc->hmin) ? tr[c] : surftr;
+    }
+  }
+}
+
+/* Per-thread flux work arrays, allocated once per (threads, size) as
+ * the reference allocates its w1n..w4n once (tracers.c:788-801). */
+static scratch_t *scr = NULL;
+static int scr_nt = 0, scr_n = 0;
+static scratch_t *scratch_get(int nt, int n) {
+  if (scr && scr_nt >= nt && scr_n == n) return scr;
+  for (int t = 0; t < scr_nt; t++) {
+    free(scr[t].Fx);
+    free(scr[t].Fy);
+    free(scr[t].Fz);
+    free(scr[t].dtracer);
+  }
+  free(scr);
+  scr = (scratch_t *)calloc(nt, sizeof(scratch_t));
+  for (int t = 0; t < nt; t++) {
+    scr[t].Fx = (double *)calloc(n, sizeof(double));
+    scr[t].Fy = (double *)calloc(n, sizeof(double));
+    scr[t].Fz = (double *)calloc(n, sizeof(double));
+    scr[t].dtracer = (double *)calloc(n, sizeof(double));
+  }
+  scr_nt = nt;
+  scr_n = n;
+  return scr;
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:401-1054  advect_diffuse (z-levels, closed boundaries)   */
+/* nthreads > 1 runs the tracer loop with OpenMP and per-thread       */
+/* scratch, as tracers.c:770-801 does.                                */
+/*-------------------------------------------------------------------*/
+int oracle_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc,
+                             int nthreads) {
+  int cc, nn;
+  double *osubeta = wc->d1, *subeta = wc->d2;
+  double dtu, dtm, trem;
+  double sf = 0.8, minval = 1e-10;
+  int itermax = 20, slf = 1, courf, ksf;
+  const int vc = wc->vc, vcs = wc->vcs;
+  double *u1 = wd->u1, *u2 = wd->u2, *w = wd->w;
+  oracle_err = 0;
+
+  /* scheme work arrays, tracers.c:496-578 */
+  if (wc->trasc == ORDER1) {
+    wc->ultimate = 0;
+    for (cc = 1; cc <= g->n3_t; cc++) {
+      int c = g->w3_t[cc];
+      wc->w6[c] = 0.5 * (wd->u1flux3d[c] + fabs(wd->u1flux3d[c]));
+      wc->w7[c] = 0.5 * (wd->u1flux3d[c] - fabs(wd->u1flux3d[c]));
+      wc->w8[c] = 0.5 * (wd->u2flux3d[c] + fabs(wd->u2flux3d[c]));
+      wc->w9[c] = 0.5 * (wd->u2flux3d[c] - fabs(wd->u2flux3d[c]));
+    }
+  } else if (wc->trasc == VANLEER) {
+    for (cc = 1; cc <= g->b3_t; cc++) {
+      int c = g->w3_t[cc];
+      wc->w9[c] = 0.5 * (wc->dz[c] + wc->dz[g->zm1[c]]);
+    }
+  } else {
+    for (cc = 1; cc <= g->b3_t; cc++) {
+      int c = g->w3_t[cc];
+      int zm1 = g->zm1[c];
+      wc->w5[c] = wc->dz[zm1] / (wc->dz[zm1] + wc->dz[c]);
+      wc->w9[c] = 0.5 * (wc->dz[c] + wc->dz[zm1]);
+    }
+    for (cc = 1; cc <= g->n2_t; cc++) {
+      int c = g->w2_t[cc];
+      int c2 = g->m2d[c];
+      int xm1 = g->xm1[c2], ym1 = g->ym1[c2];
+      wc->d3[c2] = g->h1acell[xm1] / (g->h1acell[xm1] + g->h1acell[c2]);
+      wc->d4[c2] = g->h2acell[ym1] / (g->h2acell[ym1] + g->h2acell[c2]);
+    }
+  }
+  courf = (wc->ultimate || wc->trasc >= QUICKEST) ? 1 : 0;
+
+  /* sub-step, tracers.c:590-668 */
+  dtm = dtu = wd->dt;
+  if (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)) {
+    for (cc = 1; cc <= vc; cc++) {
+      int c = wc->s1[cc];
+      int c2 = g->m2d[c];
+      int xp1 = g->xp1[c], yp1 = g->yp1[c], zp1 = g->zp1[c];
+      double d1, d2, d3, d4;
+      d4 = 0.5 * (u1[c] + u1[xp1]);
+      if (fabs(d4) < minval) d4 = minval;
+      d1 = sf * fabs(g->h1acell[c2] / d4);
+      if (d1 < dtm) dtm = d1;
+      d4 = 0.5 * (u2[c] + u2[yp1]);
+      if (fabs(d4) < minval) d4 = minval;
+      d2 = sf * fabs(g->h2acell[c2] / d4);
+      if (d2 < dtm) dtm = d2;
+      d4 = (cc <= vcs) ? 0.0 : 0.5 * (w[c] + w[zp1]);
+      if (fabs(d4) < minval) d4 = minval;
+      d3 = (cc <= vcs) ? dtu : sf * fabs(wc->dz[c] * wc->Ds[c2] / d4);
+      if (d3 > 0.0 && d3 < dtm) dtm = d3;
+    }
+    if (dtm != dtu) dtm = floor(dtm);
+  }
+  if (dtm == 0.0 || (int)(dtu / dtm) > itermax) {
+    oracle_err = 1;
+    return 1;
+  }
+  ksf = vcs + 1;
+
+  for (cc = 1; cc <= vcs; cc++) {
+    int c = wc->s1[cc];
+    osubeta[g->m2d[c]] = wc->oldeta[g->m2d[c]];
+  }
+
+  trem = dtu;
+  while (trem > 0) {
+    dtu = dtm;
+    if (trem < dtu) dtu = trem;
+    wc->b1 = dtu;
+    if (dtu > 0.0) {
+      if (courf) {
+        for (cc = 1; cc <= g->n3_t; cc++) {
+          int c = g->w3_t[cc];
+          int c2 = g->m2d[c];
+          wc->w6[c] = u1[c] * dtu / g->h1au1[c2];
+          wc->w7[c] = u2[c] * dtu / g->h2au2[c2];
+          wc->w8[c] = w[c] * dtu / (wc->w9[c] * wc->Ds[c2]);
+        }
+      }
+      for (cc = 1; cc <= vcs; cc++) {
+        int c = wc->s1[cc];
+        int c2 = g->m2d[c];
+        subeta[c2] = wc->oldeta[c2] +
+                     (1.0 + (dtu - trem) / wd->dt) * (wd->eta[c2] - wc->oldeta[c2]);
+      }
+      {
+        int nt = nthreads > 0 ? nthreads : 1;
+        if (nt > wc->ntbdy) nt = wc->ntbdy > 0 ? wc->ntbdy : 1;
+        scratch_t *S = scratch_get(nt, g->sgsiz);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+#endif
+        for (nn = 0; nn < wc->ntbdy; nn++) {
+          int tid = 0;
+#ifdef _OPENMP
+          tid = omp_get_thread_num();
+#endif
+          tracer_substep(g, wd, wc, wc->tbdy[nn], dtu, slf, ksf, subeta, &S[tid]);
+        }
+      }
+      oracle_tr_bounds(g, wd, wc);
+      for (cc = 1; cc <= vcs; cc++) {
+        int c = wc->s1[cc];
+        int c2 = g->m2d[c];
+        osubeta[c2] = subeta[c2];
+      }
+    }
+    trem -= dtu;
+    slf = 0;
+  }
+  return 0;
+}
+
+int oracle_advect_diffuse(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  return oracle_advect_diffuse_mt(g, wd, wc, 1);
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:4067-4214  implicit vertical diffusion, one tracer       */
+/*-------------------------------------------------------------------*/
+static int oracle_implicit_vdiff_tr(geometry_t *g, window_t *wd,
+                                    win_priv_t *wc, double *var,
+                                    const double *dzcell, const double *fb,
+                                    const double *ft, const int *ctp,
+                                    const int *cbt, int vcs,
+                                    const double *scale, const double *C,
+                                    const double *Cp1, const double *Cm1,
+                                    double *work /* 7*(nz+2) */) {
+  int cc, k;
+  const int nzw = g->nz + 2;
+  double *rhs = work, *sol = work + nzw, *ud = work + 2 * nzw;
+  double *B = work + 3 * nzw, *Bm1 = work + 4 * nzw, *Bp1 = work + 5 * nzw;
+  double *dz = work + 6 * nzw;
+  double dt = wd->dt, dzdt, div;
+  for (cc = 1; cc <= vcs; cc++) {
+    int cs = ctp[cc], c = cs;
+    int cb = cbt[cc];
+    int c2 = g->m2d[c];
+    int zm1 = g->zm1[c];
+    int ks = g->s2k[cs], kb = g->s2k[cb];
+    if (zm1 == g->zm1[zm1] || !dzcell[zm1]) {
+      var[cs] += wd->dt * (fb[c2] - ft[c2]) / max(dzcell[cs], wc->hmin);
+      if (c != wc->i4[cc]) var[wc->i4[cc]] = var[cs];
+      continue;
+    }
+    c = cs;
+    for (k = ks; k >= kb; k--) {
+      B[k] = C[c];
+      Bm1[k] = Cm1[c];
+      Bp1[k] = Cp1[c];
+      dz[k] = dzcell[c];
+      c = g->zm1[c];
+    }
+    c = cb;
+    dzdt = dz[kb] / dt;
+    rhs[kb] = dzdt * var[cb] + fb[c2];
+    c = g->zp1[cb];
+    for (k = kb + 1; k < ks; k++) {
+      dzdt = dz[k] / dt;
+      rhs[k] = dzdt * var[c];
+      c = g->zp1[c];
+    }
+    dzdt = dz[ks] / dt;
+    rhs[ks] = dzdt * var[cs] - ft[c2];
+    div = B[kb];
+    sol[kb] = rhs[kb] / div;
+    for (k = kb + 1; k <= ks; k++) {
+      ud[k] = Bp1[k - 1] / div;
+      div = B[k] - Bm1[k] * ud[k];
+      if (div == 0.0) return cs;
+      sol[k] = (rhs[k] - Bm1[k] * sol[k - 1]) / div;
+    }
+    c = cs;
+    var[cs] += (sol[ks] - var[cs]) * scale[c2];
+    for (k = ks - 1; k >= kb; k--) {
+      c = g->zm1[c];
+      sol[k] -= ud[k + 1] * sol[k + 1];
+      var[c] += (sol[k] - var[c]) * scale[c2];
+    }
+    c = wc->i4[cc];
+    if (c != cs) var[c] += (sol[ks] - var[cs]) * scale[c2];
+  }
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:3782-3957  vertical diffusion: coefficients once per     */
+/* column, then one solve per tracer (no surface fluxes: topflux = 0) */
+/*-------------------------------------------------------------------*/
+int oracle_vert_diffuse_3d_mt(geometry_t *g, window_t *wd, win_priv_t *wc,
+                              int nthreads) {
+  int cc, nn, bad = 0;
+  int *ctp = wc->i2, *cbt = wc->i5, *cth = wc->i4;
+  double dt = wd->dt;
+  double *dzface = wc->w7, *dzcell = wc->w9, *Kz = wc->w10;
+  double *Cm1 = wc->w1, *C = wc->w2, *Cp1 = wc->w3;
+  memset(wc->d1, 0, g->sgsizS * sizeof(double));
+  memset(wc->d2, 0, g->sgsizS * sizeof(double));
+  memset(cth, 0, g->sgsizS * sizeof(int));
+  memcpy(dzcell, wc->dz, g->sgsiz * sizeof(double));
+  memcpy(Kz, wd->Kz, g->sgsiz * sizeof(double));
+  oracle_err = 0;
+
+  for (cc = 1; cc <= wc->vcs; cc++) {
+    int c, cs, c2, zm1, zp1, cb;
+    double dzdt;
+    c = cs = ctp[cc];
+    c2 = g->m2d[c];
+    zm1 = g->zm1[c];
+    if (zm1 == g->zm1[zm1] || !dzcell[zm1]) continue;
+    while (c != zm1 && dzcell[c]) {
+      dzface[c] = 0.5 * (dzcell[zm1] + dzcell[c]);
+      dzcell[c] *= wc->Ds[c2];
+      Kz[c] /= wc->Ds[c2];
+      c = zm1;
+      zm1 = g->zm1[c];
+    }
+    cb = c = g->zp1[c];
+    zp1 = g->zp1[c];
+    cbt[cc] = cb;
+    dzdt = dzcell[c] / dt;
+    Cm1[c] = 0.0;
+    Cp1[c] = -Kz[zp1] / dzface[zp1];
+    C[c] = dzdt - Cp1[c];
+    while (zp1 != cs) {
+      c = zp1;
+      zp1 = g->zp1[c];
+      dzdt = dzcell[c] / dt;
+      Cm1[c] = -Kz[c] / dzface[c];
+      Cp1[c] = -Kz[zp1] / dzface[zp1];
+      C[c] = dzdt - Cm1[c] - Cp1[c];
+    }
+    c = cs;
+    dzdt = dzcell[c] / dt;
+    Cm1[c] = -Kz[c] / dzface[c];
+    Cp1[c] = 0.0;
+    C[c] = dzdt - Cm1[c];
+  }
+
+  /* topflux = 0 at every column (no heat, salt or generic surface flux);
+   * the thin-layer reset of tracers.c:3946-3949 leaves zeros at zero. */
+  {
+    int nt = nthreads > 0 ? nthreads : 1;
+    if (nt > wc->ntdif_v) nt = wc->ntdif_v > 0 ? wc->ntdif_v : 1;
+    const int nzw = g->nz + 2;
+    double *work = (double *)calloc((size_t)nt * 7 * nzw, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1) reduction(max : bad)
+#endif
+    for (nn = 0; nn < wc->ntdif_v; nn++) {
+      int tid = 0;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+      int n = wc->tdif_v[nn];
+      int r = oracle_implicit_vdiff_tr(g, wd, wc, wd->tr_wc[n], dzcell, wc->d1,
+                                       wc->d2, ctp, cbt, wc->vcs, wc->one, C,
+                                       Cp1, Cm1, work + (size_t)tid * 7 * nzw);
+      if (r > bad) bad = r;
+    }
+    free(work);
+  }
+  if (bad) oracle_err = 4;
+  return bad;
+}
+
+void oracle_vert_diffuse_3d(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  oracle_vert_diffuse_3d_mt(g, wd, wc, 1);
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:239-321  (the parts on the path for a closed window):    */
+/* lateral ghosts (tracers.c:153-156), advect_diffuse, vert_diffuse   */
+/*-------------------------------------------------------------------*/
+int oracle_tracer_step_3d_mt(geometry_t *g, window_t *wd, win_priv_t *wc,
+                             int nthreads) {
+  oracle_set_lateral_BC_tr(g, wd);
+  if (oracle_advect_diffuse_mt(g, wd, wc, nthreads)) return 1;
+  if (oracle_vert_diffuse_3d_mt(g, wd, wc, nthreads)) return 4;
+  return 0;
+}
+
+void oracle_tracer_step_3d(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  oracle_tracer_step_3d_mt(g, wd, wc, 1);
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:6608-6631  domain tracer mass sum(tr*area*dz) over s1     */
+/*-------------------------------------------------------------------*/
+double oracle_check_tmass(geometry_t *g, window_t *wd, win_priv_t *wc, int n) {
+  int cc;
+  double d1 = 0.0;
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int cs = g->m2d[c];
+    d1 += wd->tr_wc[n][c] * g->cellarea[cs] * wc->dz[c];
+  }
+  return d1;
+}
+
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,67 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, as the checker or the timed CPU baseline.
+"""
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    r = subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"],
+                       capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return LIBPATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            build()
+        from ems_amd import structs as S
+        L = C.CDLL(LIBPATH)
+        G, W, P = C.POINTER(S.Geometry), C.POINTER(S.Window), C.POINTER(S.WinPriv)
+        for fn in ("oracle_set_dz", "oracle_tracer_step_3d", "oracle_vert_diffuse_3d",
+                   "oracle_tr_bounds"):
+            getattr(L, fn).argtypes = [G, W, P]
+            getattr(L, fn).restype = None
+        L.oracle_set_lateral_BC_tr.argtypes = [G, W]
+        L.oracle_advect_diffuse.argtypes = [G, W, P]
+        L.oracle_advect_diffuse.restype = C.c_int
+        for fn in ("oracle_advect_diffuse_mt", "oracle_vert_diffuse_3d_mt",
+                   "oracle_tracer_step_3d_mt"):
+            getattr(L, fn).argtypes = [G, W, P, C.c_int]
+            getattr(L, fn).restype = C.c_int
+        L.oracle_check_tmass.argtypes = [G, W, P, C.c_int]
+        L.oracle_check_tmass.restype = C.c_double
+        L.oracle_max_threads.restype = C.c_int
+        L.oracle_last_error.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def set_dz(gp, wp, pp):
+    lib().oracle_set_dz(gp, wp, pp)
+
+
+def step(case, nthreads=1):
+    return lib().oracle_tracer_step_3d_mt(case.gp, case.wp, case.pp, nthreads)
+
+
+def advect_diffuse(case, nthreads=1):
+    return lib().oracle_advect_diffuse_mt(case.gp, case.wp, case.pp, nthreads)
+
+
+def vert_diffuse(case, nthreads=1):
+    return lib().oracle_vert_diffuse_3d_mt(case.gp, case.wp, case.pp, nthreads)
+
+
+def tmass(case, n):
+    return lib().oracle_check_tmass(case.gp, case.wp, case.pp, n)

@@ -1,0 +1,342 @@
+"""Pins the CPU oracle (oracle/oracle.c) with checks that do not restate
+its code: closed forms, invariants, brute force on small grids and an
+independent dense solve.  The reference ships no golden vectors for this
+path and its executables cannot be built here (no NetCDF)."""
+import numpy as np
+import pytest
+
+from ems_amd import configs, structs as S
+from oracle import oracle as O
+from tests import helpers as H
+
+
+# ---------------------------------------------------------------------
+# set_dz (tracers.c:5682-5849): brute force per column on (i,j,k)
+# ---------------------------------------------------------------------
+def test_set_dz_brute_force_moving_surface():
+    case = configs.small(seed=5, nxi=16, nyi=14, nz=7, dmin=2.0, dmax=40.0)
+    rng = np.random.default_rng(1)
+    g, wd, wc = case.g, case.wd, case.wc
+    eta = case.a2(wd.eta)
+    old = case.a2(wc.oldeta)
+    lay = case.layers
+    dlay = lay[1] - lay[0]
+    w2 = S.view(g.w2_t, case.sgsizS, np.int32)[1:g.v2_t + 1]
+    s2i, s2j = case.s2i, case.s2j
+    old[w2] = rng.uniform(-1.6, 0.0, len(w2)) * dlay
+    # previous step: the surface settles at oldeta
+    eta[w2] = old[w2]
+    O.set_dz(case.gp, case.wp, case.pp)
+    eta[w2] = old[w2] + rng.uniform(-1.0, 1.0, len(w2)) * dlay
+    O.set_dz(case.gp, case.wp, case.pp)
+    # expected, column by column on (i,j,k)
+    nz = case.nz
+    botz = case.a2(g.botz)
+    hmin = wc.hmin
+    idx = {(case.s2i[c], case.s2j[c], case.s2k[c]): c for c in case.wet}
+    s1_surf, s1_sub, i1, i2, i3 = [], [], [], [], []
+    dz_exp = {}
+
+    def surf(kb, e):
+        k = nz - 1
+        while k >= kb and lay[k] >= e:
+            k -= 1
+        return k
+
+    for cs in w2:
+        i, j = s2i[cs], s2j[cs]
+        kb = case.kbot[j, i]
+        kold = surf(kb, old[cs])
+        knew = nz - 1
+        while knew >= kb and lay[knew] >= eta[cs]:
+            knew -= 1
+        klow = min(kold, knew)
+        if min(eta[cs], old[cs]) > 0.05 * hmin + botz[cs] and klow >= kb:
+            s1_surf.append(idx[(i, j, klow)])
+            i1.append(idx[(i, j, kb)])
+            i2.append(idx[(i, j, knew)] if knew >= kb else None)
+            i3.append(idx[(i, j, kold)])
+            s1_sub.append([idx[(i, j, k)] for k in range(klow - 1, kb - 1, -1)])
+        if kold < kb:
+            continue
+        for k in range(kb, kold + 1):
+            dz_exp[idx[(i, j, k)]] = min(old[cs], lay[k + 1]) - max(botz[cs], lay[k])
+        for k in range(kold + 1, nz):
+            dz_exp[idx[(i, j, k)]] = dz_exp[idx[(i, j, kold)]]
+    vcs = len(s1_surf)
+    s1 = S.view(wc.s1, case.sgsiz, np.int32)
+    assert wc.vcs == vcs
+    assert list(s1[1:vcs + 1]) == s1_surf
+    sub = [c for col in s1_sub for c in col]
+    assert wc.vc == vcs + len(sub)
+    assert list(s1[vcs + 1:wc.vc + 1]) == sub
+    assert list(S.view(wc.i1, case.sgsizS, np.int32)[1:vcs + 1]) == i1
+    assert list(S.view(wc.i3, case.sgsizS, np.int32)[1:vcs + 1]) == i3
+    got_i2 = S.view(wc.i2, case.sgsizS, np.int32)[1:vcs + 1]
+    for a, b in zip(got_i2, i2):
+        if b is not None:
+            assert a == b
+    dz = case.a3(wc.dz)
+    for c, v in dz_exp.items():
+        assert dz[c] == v
+    # column thickness sums to oldeta - botz
+    for cs in w2[:50]:
+        i, j = s2i[cs], s2j[cs]
+        kb = case.kbot[j, i]
+        kold = surf(kb, old[cs])
+        if kold < kb:
+            continue
+        tot = sum(dz[idx[(i, j, k)]] for k in range(kb, kold + 1))
+        assert abs(tot - (old[cs] - botz[cs])) < 1e-9 * (1 + abs(tot))
+
+
+# ---------------------------------------------------------------------
+# advection: exact translation of a linear profile (all schemes)
+# ---------------------------------------------------------------------
+SCHEMES = [S.VANLEER, S.QUICKEST, S.QUICKEST_CO, S.ORDER1, S.ORDER2, S.ORDER4]
+
+
+def _channel(U, scheme, ultimate=0, dt=60.0, stab=None):
+    case = H.box(nxi=20, nyi=5, nz=3, depth=15.0, ntr=1)
+    nz, ny, nx = case.nz, case.ny, case.nx
+    H.set_dense_face_fields(case, u1=np.full((nz, ny, nx), U), u2=np.zeros((nz, ny, nx)),
+                            w=np.zeros((nz, ny, nx)), Kz=np.zeros((nz, ny, nx)))
+    x = (np.arange(nx) + 0.5) * 1000.0
+    a = 1e-3
+    H.set_tracer_dense(case, 0, np.broadcast_to(5.0 + a * x, (nz, ny, nx)))
+    H.scheme(case, scheme, ultimate, dt, mintr=-1e10)
+    if stab is not None:
+        case.wc.stab = stab
+    case.prepare(O.set_dz)
+    case.a3(case.wd.w)[:] = 0.0  # the channel is divergent at the walls
+    return case, x, a
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("U", [0.4, -0.3])
+def test_linear_profile_translates_exactly(scheme, U):
+    case, x, a = _channel(U, scheme)
+    assert O.advect_diffuse(case) == 0
+    t = H.tracer_dense(case, 0)
+    exp = 5.0 + a * (x - U * case.dt)
+    inner = slice(4, case.nx - 4)
+    got = t[:, 1:-1, inner]
+    want = np.broadcast_to(exp[inner], got.shape)
+    assert np.abs(got - want).max() < 1e-12 * 6.0
+
+
+def test_ultimate_keeps_linear_profile():
+    for sch in (S.QUICKEST, S.VANLEER, S.ORDER2):
+        case, x, a = _channel(0.5, sch, ultimate=1)
+        O.advect_diffuse(case)
+        t = H.tracer_dense(case, 0)
+        exp = 5.0 + a * (x - 0.5 * case.dt)
+        inner = slice(4, case.nx - 4)
+        assert np.abs(t[:, 1:-1, inner] - exp[inner]).max() < 1e-12 * 6.0
+
+
+def test_substeps_translate_exactly():
+    # Courant 1.6 over dt: SUB_STEP splits into floor(0.8*dx/u) steps
+    case, x, a = _channel(0.8, S.VANLEER, dt=2000.0, stab=S.SUB_STEP)
+    assert O.advect_diffuse(case) == 0
+    assert case.wc.b1 == 2000.0 - 1000.0  # last sub-step dtu
+    t = H.tracer_dense(case, 0)
+    exp = 5.0 + a * (x - 0.8 * 2000.0)
+    inner = slice(6, case.nx - 6)
+    assert np.abs(t[:, 1:-1, inner] - exp[inner]).max() < 1e-11
+
+
+def test_substep_limit_reported():
+    case, x, a = _channel(0.8, S.VANLEER, dt=30000.0, stab=S.SUB_STEP)
+    assert O.advect_diffuse(case) == 1
+
+
+# ---------------------------------------------------------------------
+# invariants on curvilinear basins with w from continuity
+# ---------------------------------------------------------------------
+@pytest.mark.parametrize("scheme,ult", [(S.VANLEER, 0), (S.QUICKEST, 0), (S.QUICKEST, 1),
+                                        (S.QUICKEST_CO, 0), (S.ORDER1, 0), (S.ORDER2, 0),
+                                        (S.ORDER4, 0)])
+def test_constant_field_stays_constant(scheme, ult):
+    case = configs.small(seed=9, ntr=1, trasc=scheme, ultimate=ult, dt=90.0)
+    case.set_tracers(np.full((1, case.nwet), 3.25))
+    case.prepare(O.set_dz)
+    for _ in range(5):
+        assert O.step(case) == 0
+    t = case.tracers()
+    assert np.abs(t - 3.25).max() < 1e-11
+
+
+@pytest.mark.parametrize("scheme,ult", [(S.VANLEER, 0), (S.QUICKEST, 1), (S.ORDER2, 0)])
+def test_closed_basin_mass_conserved(scheme, ult):
+    case = configs.small(seed=4, ntr=3, trasc=scheme, ultimate=ult, dt=120.0)
+    S.view(case.wc.mintr, 3, np.float64)[:] = -1e10  # no positivity clip
+    case.prepare(O.set_dz)
+    m0 = np.array([O.tmass(case, n) for n in range(case.ntr)])
+    for _ in range(10):
+        assert O.step(case) == 0
+    m1 = np.array([O.tmass(case, n) for n in range(case.ntr)])
+    assert np.all(np.abs(m1 - m0) <= 1e-12 * np.abs(m0))
+
+
+def test_van_leer_bounded_on_c1():
+    case = configs.c1()
+    nz, ny, nx = case.nz, case.ny, case.nx
+    X = np.arange(nx)[None, None, :] * np.ones((nz, ny, nx))
+    H.set_tracer_dense(case, 0, np.where(X < nx // 2, 1.0, 0.0))
+    case.prepare(O.set_dz)
+    for _ in range(30):
+        O.step(case)
+    t = case.tracers()
+    assert t.min() >= -1e-13 and t.max() <= 1.0 + 1e-13
+
+
+def test_quickest_needs_ultimate_for_bounds():
+    res = {}
+    for ult in (0, 1):
+        case = configs.c1()
+        H.scheme(case, S.QUICKEST, ult, 60.0, mintr=-1e10)
+        nz, ny, nx = case.nz, case.ny, case.nx
+        X = np.arange(nx)[None, None, :] * np.ones((nz, ny, nx))
+        H.set_tracer_dense(case, 0, np.where(X < nx // 2, 1.0, 0.0))
+        case.prepare(O.set_dz)
+        for _ in range(20):
+            O.step(case)
+        t = case.tracers()
+        res[ult] = (t.min(), t.max())
+    assert res[0][0] < -1e-4 or res[0][1] > 1.0 + 1e-4  # QUICKEST alone overshoots
+    assert res[1][0] >= -1e-12 and res[1][1] <= 1.0 + 1e-12
+
+
+# ---------------------------------------------------------------------
+# vertical diffusion: dense solve and the discrete decay factor
+# ---------------------------------------------------------------------
+def test_vert_diffuse_matches_dense_solve():
+    rng = np.random.default_rng(2)
+    bz = -rng.uniform(6.0, 30.0, (4, 5))
+    case = H.box(nxi=5, nyi=4, nz=6, depth=30.0, botz=bz, ntr=2)
+    nz, ny, nx = case.nz, case.ny, case.nx
+    Kz = rng.uniform(1e-4, 5e-2, (nz, ny, nx))
+    H.set_dense_face_fields(case, Kz=Kz)
+    H.scheme(case, S.VANLEER, 0, 600.0)
+    for n in range(2):
+        H.set_tracer_dense(case, n, rng.uniform(0.0, 2.0, (nz, ny, nx)))
+    case.prepare(O.set_dz)
+    before = [H.tracer_dense(case, n) for n in range(2)]
+    assert O.vert_diffuse(case) == 0
+    after = [H.tracer_dense(case, n) for n in range(2)]
+    lay = case.layers
+    dt = case.dt
+    for j in range(1, ny - 1):
+        for i in range(1, nx - 1):
+            kb = case.kbot[j, i]
+            ks = nz - 1
+            if ks == kb:
+                continue
+            ks_ = list(range(kb, ks + 1))
+            dz = np.array([(lay[k + 1] if k < ks else 0.0) - (lay[k] if k > kb else
+                          case.botz2d[j, i]) for k in ks_])
+            n_ = len(ks_)
+            A = np.zeros((n_, n_))
+            for q, k in enumerate(ks_):
+                A[q, q] += dz[q] / dt
+                if q > 0:  # lower face of cell k, Kz[k]
+                    g = Kz[k, j, i] / (0.5 * (dz[q - 1] + dz[q]))
+                    A[q, q] += g
+                    A[q, q - 1] -= g
+                if q < n_ - 1:  # upper face = lower face of k+1
+                    g = Kz[k + 1, j, i] / (0.5 * (dz[q] + dz[q + 1]))
+                    A[q, q] += g
+                    A[q, q + 1] -= g
+            for n in range(2):
+                b = before[n][kb:ks + 1, j, i]
+                x = np.linalg.solve(A, dz / dt * b)
+                got = after[n][kb:ks + 1, j, i]
+                assert np.abs(got - x).max() <= 1e-12 * np.abs(x).max()
+                # zero-flux boundaries: column content conserved
+                assert abs((got * dz).sum() - (b * dz).sum()) <= 1e-12 * (b * dz).sum()
+
+
+@pytest.mark.parametrize("m", [1, 2, 5])
+def test_diffusion_decay_factor(m):
+    N = 12
+    case = H.box(nxi=3, nyi=3, nz=N, depth=24.0, ntr=1)
+    Kz = 3e-3
+    dt = 900.0
+    H.set_dense_face_fields(case, Kz=np.full((N, case.ny, case.nx), Kz))
+    H.scheme(case, S.VANLEER, 0, dt, mintr=-10.0)
+    k = np.arange(N)[:, None, None]
+    mode = np.cos(np.pi * m * (k + 0.5) / N) * np.ones((N, case.ny, case.nx))
+    H.set_tracer_dense(case, 0, mode)
+    case.prepare(O.set_dz)
+    O.vert_diffuse(case)
+    D = Kz * dt / 2.0 ** 2
+    fac = 1.0 / (1.0 + 4.0 * D * np.sin(np.pi * m / (2 * N)) ** 2)
+    t = H.tracer_dense(case, 0)
+    assert np.abs(t[:, 1:-1, 1:-1] - fac * mode[:, 1:-1, 1:-1]).max() < 1e-13
+
+
+# ---------------------------------------------------------------------
+# tr_bounds (tracers.c:3217-3324) and check_tmass
+# ---------------------------------------------------------------------
+def test_tr_bounds_clip_and_copy_up():
+    case = H.box(nxi=4, nyi=3, nz=4, depth=20.0, ntr=2)
+    H.scheme(case, S.VANLEER, 0, 60.0)
+    S.view(case.wc.mintr, 2, np.float64)[:] = [0.0, -5.0]
+    eta = case.a2(case.wd.eta)
+    eta[:] = -7.5  # surface in layer 2 (faces -10,-5): top layer dry
+    case.a2(case.wc.oldeta)[:] = -7.5
+    O.set_dz(case.gp, case.wp, case.pp)
+    O.set_dz(case.gp, case.wp, case.pp)  # settled: old surface = new
+    rng = np.random.default_rng(0)
+    vals = rng.uniform(-1.0, 1.0, (2, case.nwet))
+    case.set_tracers(vals)
+    O.lib().oracle_tr_bounds(case.gp, case.wp, case.pp)
+    for n, mn in ((0, 0.0), (1, -5.0)):
+        t = H.tracer_dense(case, n)
+        wet = t[:, 1:-1, 1:-1]
+        v = np.full_like(t, np.nan)
+        v.reshape(-1)[case.dense_index(case.wet)] = vals[n]
+        v = v[:, 1:-1, 1:-1]
+        below = wet[:3]
+        exp = np.where(v[:3] < mn, mn, v[:3]) if mn <= 1e-5 else v[:3]
+        assert np.array_equal(below, exp)
+        assert np.array_equal(wet[3], wet[2])  # copied from the surface cell
+
+
+def test_tmass_is_direct_sum():
+    case = configs.small(seed=8, ntr=2)
+    case.prepare(O.set_dz)
+    s1 = S.view(case.wc.s1, case.sgsiz, np.int32)[1:case.wc.vc + 1]
+    area = case.a2(case.g.cellarea)
+    m2d = S.view(case.g.m2d, case.sgsiz, np.int32)
+    dz = case.a3(case.wc.dz)
+    for n in range(2):
+        t = case.tr(n)
+        exp = np.sum(t[s1] * area[m2d[s1]] * dz[s1])
+        assert abs(O.tmass(case, n) - exp) <= 1e-12 * abs(exp)
+
+
+def test_tr_bounds_thin_surface_takes_value_below():
+    case = H.box(nxi=4, nyi=3, nz=4, depth=20.0, ntr=1)
+    H.scheme(case, S.VANLEER, 0, 60.0, mintr=0.0, maxtr=1.0)
+    eta = case.a2(case.wd.eta)
+    eta[:] = -9.95  # surface layer 2 is 0.05 m thick (hmin = 0.1)
+    case.a2(case.wc.oldeta)[:] = -9.95
+    O.set_dz(case.gp, case.wp, case.pp)
+    O.set_dz(case.gp, case.wp, case.pp)
+    assert case.wc.hmin == 0.1
+    rng = np.random.default_rng(3)
+    vals = rng.uniform(-0.5, 1.5, (1, case.nwet))
+    case.set_tracers(vals)
+    O.lib().oracle_tr_bounds(case.gp, case.wp, case.pp)
+    t = H.tracer_dense(case, 0)[:, 1:-1, 1:-1]
+    v = np.full((case.nz, case.ny, case.nx), np.nan)
+    v.reshape(-1)[case.dense_index(case.wet)] = vals[0]
+    v = v[:, 1:-1, 1:-1]
+    surf, below = v[2], v[1]
+    out = (surf < 0.0) | (surf > 1.0)
+    exp = np.where(out, below, surf)
+    exp = np.where(exp < 0.0, 0.0, exp)  # then the clip at min = 0
+    assert np.array_equal(t[2], exp)
+    assert np.array_equal(t[3], t[2])
