@@ -55,7 +55,11 @@ def step(case, nthreads=1):
     return lib().oracle_tracer_step_3d_mt(case.gp, case.wp, case.pp, nthreads)
 
 
-def advect_diffuse(case, nthreads=1):
+def advect_diffuse(case, nthreads=1, lateral=True):
+    """advect_diffuse as tracer_step_3d calls it: tracer_step fills the
+    lateral ghosts first (tracers.c:153-156)."""
+    if lateral:
+        lib().oracle_set_lateral_BC_tr(case.gp, case.wp)
     return lib().oracle_advect_diffuse_mt(case.gp, case.wp, case.pp, nthreads)
 
 
