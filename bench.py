@@ -4,12 +4,18 @@
 Workload: the 400x600x40 curvilinear basin with 40 tracers (van Leer,
 dt = 90 s, seeded; 25 % land, 2-300 m), built by ems_amd.configs.c3().
 One step = advect_diffuse + vert_diffuse_3d over all tracers with fields
-and tracers resident in HBM.  The FP32 precision form is the headline
-value; the FP64 plain form runs on the same inputs for the ratio and as
-the accuracy reference (it agrees bit for bit with the CPU oracle).
+and tracers resident in HBM, including the per-step coefficient prep.
+The batched FP32 form is the headline value; the FP64 plain form runs on
+the same inputs for the ratio and as the accuracy reference (it agrees bit
+for bit with the CPU oracle); the unfused FP32 precision form is reported
+beside them.
 
 Prints one JSON line (rank 0, N = 1).  --gpus N with N > 1 exits: this
 build runs on one GPU.
+
+  --forms plain,precision,batched   which forms to run (profiling runs)
+  --copy-probe MB                   time a plain device copy of MB MiB and
+                                    exit (calibrates the HBM byte counters)
 """
 import argparse
 import ctypes as C
@@ -78,6 +84,27 @@ def kernel_times(ctx, steps):
     return list(ms), list(cnt)
 
 
+def copy_probe(mb):
+    """A plain device-to-device copy of mb MiB, five times: its
+    FETCH_SIZE / WRITE_SIZE per dispatch against the known 2 x mb MiB
+    gives the scale factor of those counters."""
+    import torch
+    n = mb * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    y.copy_(x)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b)
+    print(json.dumps({"copy_probe_MiB": mb, "ms": ms,
+                      "GBps": 2 * mb * (1 << 20) / (ms * 1e-3) / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,11 +114,18 @@ def main():
     ap.add_argument("--ntr", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="plain,precision,batched")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--copy-probe", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the tracer step on one GPU only; "
-              "multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+              "runs on several GPUs are not part of it", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    forms = set(args.forms.split(","))
 
     from ems_amd import _lib, configs
     from ems_amd import tracer as T
@@ -107,88 +141,92 @@ def main():
     if msg:
         raise RuntimeError("set_dz: %s" % msg)
     L.emstr_forget_window(case.gp)
-    ctx64 = T.Context(case, T.PLAIN_F64)
-    ctx64.upload_fields()
-    ctx64.vel_w()
-    ctx64.upload()
-    ctx32 = T.Context(case, T.PREC_F32)
-    ctx32.upload()
-    m64_0, m32_0 = ctx64.mass(), ctx32.mass()
     ev = HipEvents()
     nwet, ntr = case.nwet, case.ntr
-    K, W = args.steps, args.warmup
-
-    ctx64.step(W)
-    ms64 = timed_steps(ctx64, ev, K)
-    ctx32.step(W)
-    ms32 = timed_steps(ctx32, ev, K)
-    print("bench: fp64 %.4f ms/step, fp32 %.4f ms/step" % (ms64 / K, ms32 / K),
-          file=sys.stderr, flush=True)
-    PK = 20
-    kt32, kc32 = kernel_times(ctx32, PK)
-    kt64, kc64 = kernel_times(ctx64, PK)
+    K, W, PK = args.steps, args.warmup, args.profile_steps
+    cu = float(nwet) * ntr
+    names = ["advect", "vdiff", "prep", "other", "fused"]
+    res = {}
+    ctx = {}
+    for form, kind in (("plain", T.PLAIN_F64), ("precision", T.PREC_F32),
+                       ("batched", T.BATCH_F32)):
+        if form not in forms:
+            continue
+        c = T.Context(case, kind)
+        if form == "plain":
+            c.upload_fields()
+            c.vel_w()
+        c.upload()
+        m0 = c.mass()
+        c.step(W)
+        ms = timed_steps(c, ev, K)
+        kt, kc = kernel_times(c, PK)
+        res[form] = {"ms": ms / K, "m0": m0, "kt": kt, "kc": kc}
+        ctx[form] = c
+        print("bench: %s %.4f ms/step" % (form, ms / K), file=sys.stderr, flush=True)
     nsteps = W + K + PK
 
-    # accuracy of the FP32 form against the FP64 plain form, same steps
-    m64_1, m32_1 = ctx64.mass(), ctx32.mass()
-    ctx64.download()
-    ref = case.tracers()
-    ctx32.download()
-    f32 = case.tracers()
-    span = np.maximum(np.abs(ref).max(axis=1), 1e-30)
-    err = (np.abs(f32 - ref).max(axis=1) / span)
-    rms = np.sqrt(((f32 - ref) ** 2).mean(axis=1)) / span
-    drift32 = np.abs(m32_1 - m32_0) / np.abs(m32_0)
-    drift64 = np.abs(m64_1 - m64_0) / np.abs(m64_0)
-
-    cu = float(nwet) * ntr
-    v32 = cu * K / (ms32 * 1e-3)
-    v64 = cu * K / (ms64 * 1e-3)
-    # dominant FP32 kernel and its algorithmic bytes per launch
-    names = ["advect", "vdiff", "prep", "other", "fused"]
-    dom = int(np.argmax(kt32))
-    alg = {0: nwet * ntr * 8 + nwet * 7 * 4,
-           1: nwet * ntr * 8 + nwet * 3 * 4,
-           2: nwet * (7 * 8 + 10 * 4),
-           4: nwet * ntr * 8 + nwet * 10 * 4}.get(dom, nwet * ntr * 8)
-    avg_ms = kt32[dom] / max(kc32[dom], 1)
-    achieved = alg / (avg_ms * 1e-3) / 1e9
-    # step-level algorithmic bytes (N3*ntr*2*s + N3*8*F, F = 7)
-    step_bytes32 = nwet * ntr * 8 + nwet * 56
-    step_bytes64 = nwet * ntr * 16 + nwet * 56
-
-    out = {
-        "metric": METRIC,
-        "value": v32,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": ms32 / K,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded c3 generator, seed %d)" % args.seed,
-        "config": {"workload": "c3: 400x600x40 curvilinear basin, 25%% land, 2-300 m, "
-                               "40 tracers, van Leer, dt=90 s",
-                   "n3_wet": nwet, "ntr": ntr, "columns": int(case.g.v2_t)},
-        "fp64_plain": {"value": v64, "ms_per_step": ms64 / K,
-                       "step_hbm_fraction": step_bytes64 / (ms64 / K * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "ratio_fp32_over_fp64": v32 / v64,
-        "step_hbm_fraction": step_bytes32 / (ms32 / K * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "accuracy": {"steps": nsteps, "max_rel_err": float(err.max()),
-                     "rms_rel_err": float(rms.max()),
-                     "mass_drift_fp32": float(drift32.max()),
-                     "mass_drift_fp64": float(drift64.max())},
-        "kernels_ms_per_step": {"fp32": {names[i]: kt32[i] / PK for i in range(5) if kc32[i]},
-                                "fp64": {names[i]: kt64[i] / PK for i in range(5) if kc64[i]}},
-        "roofline": {"bound": "hbm", "kernel": "fp32 " + names[dom],
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
-    }
-    ctx64.close()
-    ctx32.close()
+    out = {"metric": METRIC, "value": None, "unit": "cell-updates/s", "n_gpus": 1,
+           "steps": K, "warmup": W, "ms_per_step": None,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (seeded c3 generator, seed %d)" % args.seed,
+           "config": {"workload": "c3: 400x600x40 curvilinear basin, 25% land, 2-300 m, "
+                                  "40 tracers, van Leer, dt=90 s",
+                      "n3_wet": nwet, "ntr": ntr, "columns": int(case.g.v2_t),
+                      "form": "batched FP32 (fused persistent kernel, hipGraph)"}}
+    if "batched" in res:
+        rB = res["batched"]
+        out["value"] = cu / (rB["ms"] * 1e-3)
+        out["ms_per_step"] = rB["ms"]
+        ktB, kcB = rB["kt"], rB["kc"]
+        dom = int(np.argmax(ktB))
+        # algorithmic bytes per launch of the dominant kernel: the fused
+        # kernel reads and writes every tracer of every wet cell once (FP32)
+        # and reads the 16 FP32 coefficients of the cell
+        alg = {4: nwet * (ntr * 8 + 16 * 4),
+               2: nwet * (7 * 8 + 16 * 4)}.get(dom, nwet * ntr * 8)
+        avg_ms = ktB[dom] / max(kcB[dom], 1)
+        achieved = alg / (avg_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": "batched " + names[dom],
+                           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "alg_bytes_per_launch": alg, "avg_launch_ms": avg_ms}
+        out["step_hbm_fraction"] = (nwet * ntr * 8 + nwet * 56) / (rB["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+    kms = {}
+    for form, r in res.items():
+        kms[form] = {names[i]: r["kt"][i] / PK for i in range(5) if r["kc"][i]}
+    out["kernels_ms_per_step"] = kms
+    if "plain" in res:
+        r64 = res["plain"]
+        out["fp64_plain"] = {"value": cu / (r64["ms"] * 1e-3), "ms_per_step": r64["ms"],
+                             "step_hbm_fraction": (nwet * ntr * 16 + nwet * 56) /
+                             (r64["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if "batched" in res:
+            out["ratio_fp32_over_fp64"] = r64["ms"] / res["batched"]["ms"]
+    if "precision" in res:
+        out["fp32_precision_unfused"] = {"value": cu / (res["precision"]["ms"] * 1e-3),
+                                         "ms_per_step": res["precision"]["ms"]}
+    if "plain" in res and "batched" in res:
+        # accuracy of the batched FP32 form against the FP64 plain form
+        c64, cB = ctx["plain"], ctx["batched"]
+        m64_1, mB_1 = c64.mass(), cB.mass()
+        c64.download()
+        ref = case.tracers()
+        cB.download()
+        fB = case.tracers()
+        span = np.maximum(np.abs(ref).max(axis=1), 1e-30)
+        err = (np.abs(fB - ref).max(axis=1) / span)
+        rms = np.sqrt(((fB - ref) ** 2).mean(axis=1)) / span
+        m64_0, mB_0 = res["plain"]["m0"], res["batched"]["m0"]
+        driftB = np.abs(mB_1 - mB_0) / np.abs(mB_0)
+        drift64 = np.abs(m64_1 - m64_0) / np.abs(m64_0)
+        out["accuracy"] = {"steps": nsteps, "max_rel_err": float(err.max()),
+                           "rms_rel_err": float(rms.max()),
+                           "mass_drift_fp32": float(driftB.max()),
+                           "mass_drift_fp64": float(drift64.max()),
+                           "worst_tracer": int(np.argmax(err))}
+    for c in ctx.values():
+        c.close()
 
     if not args.no_cpu:
         # CPU baseline: the oracle (FP64 C restatement) with OpenMP over

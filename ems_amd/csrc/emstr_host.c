@@ -22,6 +22,7 @@
 #include "emstr_dev.h"
 #include "emstr_internal.h"
 #include "emstr_kernels.h"
+#include "emstr_batch.h"
 #include "emstr_prec.h"
 #include "emstr_prof.h"
 
@@ -67,6 +68,8 @@ struct emstr_ctx {
   int key_sgsiz, key_nx, key_ny, key_nz, key_v3;
   const int *key_w3, *key_xp1;
   struct emstr_prec *prec; /* precision form state (EMSTR_PREC_F32) */
+  struct emstr_batch *batch; /* batched form state (EMSTR_BATCH_F32) */
+  int use_graph;
   emstr_prof *prof;        /* per-kernel event timing                 */
 };
 
@@ -192,7 +195,7 @@ emstr_ctx *emstr_create(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
                     "open boundaries and sigma coordinates are not supported");
     return NULL;
   }
-  if (form != EMSTR_PLAIN_F64 && form != EMSTR_PREC_F32) {
+  if (form != EMSTR_PLAIN_F64 && form != EMSTR_PREC_F32 && form != EMSTR_BATCH_F32) {
     emstr_set_error(EMSTR_EARG, -1, "unknown form %d", form);
     return NULL;
   }
@@ -379,8 +382,17 @@ emstr_ctx *emstr_create(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
       return NULL;
     }
   }
+  if (form == EMSTR_BATCH_F32) {
+    c->batch = emstr_batch_create(d, c->h_kb, c->st);
+    if (!c->batch) {
+      emstr_destroy(c);
+      return NULL;
+    }
+    c->use_graph = 1;
+  }
   c->prof = (emstr_prof *)calloc(1, sizeof(emstr_prof));
   if (c->prec) emstr_prec_set_prof(c->prec, c->prof);
+  if (c->batch) emstr_batch_set_prof(c->batch, c->prof);
   c->cur = 0;
   c->have_cols = 0;
   return c;
@@ -410,6 +422,7 @@ void emstr_destroy(emstr_ctx *c) {
   emstr_dev *d = &c->d;
   if (c->st) hipStreamSynchronize(c->st);
   if (c->prec) emstr_prec_destroy(c->prec);
+  if (c->batch) emstr_batch_destroy(c->batch);
   if (c->prof) {
     prof_collect(c->prof);
     prof_free(c->prof);
@@ -454,6 +467,25 @@ int emstr_sync(emstr_ctx *c) {
 /*-------------------------------------------------------------------*/
 /* per-column step data from the host's set_dz results (drop-in)     */
 /*-------------------------------------------------------------------*/
+/* the FP32 forms rebuild their coefficients when fields change */
+static int form_fields(emstr_ctx *c) {
+  if (c->prec) return emstr_prec_fields(c->prec, &c->d, c->st);
+  if (c->batch) {
+    /* the fused kernel advects and diffuses every tracer */
+    win_priv_t *wc = c->wc;
+    int n;
+    if (wc->ntbdy != c->wd->ntr || wc->ntdif_v != c->wd->ntr)
+      return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                             "batched form: every tracer must be advected and diffused");
+    for (n = 0; n < c->wd->ntr; n++)
+      if (!wc->advect[n] || wc->diffuse[n])
+        return emstr_set_error(EMSTR_EUNSUPPORTED, n,
+                               "batched form: tracer %d: advect on, horizontal diffusion off", n);
+    return emstr_batch_fields(c->batch, &c->d, c->st);
+  }
+  return 0;
+}
+
 static int columns_from_host(emstr_ctx *c) {
   geometry_t *g = c->g;
   win_priv_t *wc = c->wc;
@@ -528,8 +560,7 @@ int emstr_upload_fields(emstr_ctx *c) {
   if (upload_tracer_meta(c)) return EMSTR_EUNSUPPORTED;
   if (columns_from_host(c)) return EMSTR_EDEVICE;
   HCHK(hipStreamSynchronize(c->st));
-  if (c->prec) return emstr_prec_fields(c->prec, d, c->st);
-  return 0;
+  return form_fields(c);
 }
 
 int emstr_upload_tracers(emstr_ctx *c) {
@@ -539,7 +570,9 @@ int emstr_upload_tracers(emstr_ctx *c) {
   for (n = 0; n < c->wd->ntr; n++) {
     HCHK(hipMemcpyAsync(c->stage, c->wd->tr_wc[n], (size_t)c->g->sgsiz * sizeof(double),
                         hipMemcpyHostToDevice, c->st));
-    if (c->prec) {
+    if (c->batch) {
+      if (emstr_batch_put(c->batch, d, n, c->stage, c->st)) return EMSTR_EDEVICE;
+    } else if (c->prec) {
       if (emstr_prec_put_tracer(c->prec, d, n, c->stage, c->st)) return EMSTR_EDEVICE;
     } else if (emstr_launch_scatter(d->wsp, d->wdn, d->nwet, c->stage,
                                     c->tr[c->cur] + (ll)n * d->n3, c->st)) {
@@ -558,7 +591,9 @@ int emstr_download_tracers(emstr_ctx *c) {
   for (n = 0; n < c->wd->ntr; n++) {
     HCHK(hipMemcpyAsync(c->stage, c->wd->tr_wc[n], (size_t)g->sgsiz * sizeof(double),
                         hipMemcpyHostToDevice, c->st));
-    if (c->prec) {
+    if (c->batch) {
+      if (emstr_batch_get(c->batch, d, n, c->stage, c->st)) return EMSTR_EDEVICE;
+    } else if (c->prec) {
       if (emstr_prec_get_tracer(c->prec, d, n, c->stage, c->st)) return EMSTR_EDEVICE;
     } else if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, c->tr[c->cur] + (ll)n * d->n3,
                                    c->stage, c->st)) {
@@ -639,19 +674,44 @@ static int vdiff_plain(emstr_ctx *c) {
 
 int emstr_advect_diffuse(emstr_ctx *c) {
   emstr_clear_error();
+  if (c->batch)
+    return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                           "batched form: advection and diffusion are fused");
   if (c->prec) return emstr_prec_advect(c->prec, &c->d, c->wc->stab, c->st);
   return adv_plain(c);
 }
 
 int emstr_vert_diffuse(emstr_ctx *c) {
   emstr_clear_error();
+  if (c->batch)
+    return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                           "batched form: advection and diffusion are fused");
   if (c->prec) return emstr_prec_vdiff(c->prec, &c->d, c->d_err, c->st);
   return vdiff_plain(c);
+}
+
+int emstr_steps(emstr_ctx *c, int nsteps) {
+  int r, k;
+  emstr_clear_error();
+  if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
+  if (c->batch)
+    return emstr_batch_steps(c->batch, &c->d, c->wc->stab, nsteps, c->st, c->use_graph);
+  for (k = 0; k < nsteps; k++)
+    if ((r = emstr_step(c))) return r;
+  return 0;
+}
+
+int emstr_tune(emstr_ctx *c, int tile_x, int tile_y, int grid, int use_graph) {
+  if (!c) return EMSTR_EARG;
+  if (use_graph >= 0) c->use_graph = use_graph;
+  if (c->batch) return emstr_batch_tune(c->batch, c->h_kb, tile_x, tile_y, grid);
+  return 0;
 }
 
 int emstr_step(emstr_ctx *c) {
   int r;
   emstr_clear_error();
+  if (c->batch) return emstr_steps(c, 1);
   if (c->prec) return emstr_prec_step(c->prec, &c->d, c->wc->stab, c->d_err, c->st);
   if ((r = adv_plain(c))) return r;
   return vdiff_plain(c);
@@ -663,7 +723,9 @@ int emstr_tracer_mass(emstr_ctx *c, double *mass) {
   const int nb = emstr_mass_blocks(d);
   emstr_clear_error();
   if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
-  if (c->prec) {
+  if (c->batch) {
+    if (emstr_batch_mass(c->batch, d, c->d_part, c->st)) return EMSTR_EDEVICE;
+  } else if (c->prec) {
     if (emstr_prec_mass(c->prec, d, c->d_part, c->st)) return EMSTR_EDEVICE;
   } else if (emstr_launch_mass(d, c->tr[c->cur], NULL, c->d_part, c->st)) {
     return emstr_set_error(EMSTR_EDEVICE, -1, "mass launch");
@@ -723,8 +785,7 @@ int emstr_device_set_dz(emstr_ctx *c) {
   HCHK(hipMemcpyAsync(d->cols, c->h_cols, (size_t)ncol * sizeof(int), hipMemcpyHostToDevice, c->st));
   HCHK(hipStreamSynchronize(c->st));
   c->have_cols = 1;
-  if (c->prec) return emstr_prec_fields(c->prec, d, c->st);
-  return 0;
+  return form_fields(c);
 }
 
 /* Sparse index of layer k of column col; kb-1 is the sediment ghost. */
@@ -795,8 +856,7 @@ int emstr_device_vel_w(emstr_ctx *c) {
   HCHK(hipMemcpyAsync(c->wd->w, c->stage, (size_t)g->sgsiz * sizeof(double),
                       hipMemcpyDeviceToHost, c->st));
   HCHK(hipStreamSynchronize(c->st));
-  if (c->prec) return emstr_prec_fields(c->prec, d, c->st);
-  return 0;
+  return form_fields(c);
 }
 
 /*-------------------------------------------------------------------*/

@@ -11,6 +11,7 @@ from . import _lib
 
 PLAIN_F64 = 0
 PREC_F32 = 1
+BATCH_F32 = 2
 
 
 class EmstrError(RuntimeError):
@@ -58,9 +59,10 @@ class Context:
         _chk(_lib.lib().emstr_download_tracers(self.ptr), "emstr_download_tracers")
 
     def step(self, n=1):
-        L = _lib.lib()
-        for _ in range(n):
-            _chk(L.emstr_step(self.ptr), "emstr_step")
+        _chk(_lib.lib().emstr_steps(self.ptr, n), "emstr_steps")
+
+    def tune(self, tile_x=0, tile_y=0, grid=0, use_graph=-1):
+        _chk(_lib.lib().emstr_tune(self.ptr, tile_x, tile_y, grid, use_graph), "emstr_tune")
 
     def advect_diffuse(self):
         _chk(_lib.lib().emstr_advect_diffuse(self.ptr), "emstr_advect_diffuse")
