@@ -1,7 +1,8 @@
 """Loads libemstr.so (the C-ABI library built from ems_amd/csrc) via ctypes.
 
 The library holds the HIP kernels and their C host layer.  There is no
-fallback: if the library is missing, loading raises.
+fallback: if the library is missing, loading raises.  EMSTR_LIB, if set,
+names another build of the library to load instead of the in-tree one.
 """
 import ctypes as C
 import os
@@ -9,7 +10,7 @@ import os
 from . import structs as S
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libemstr.so")
+LIBPATH = os.environ.get("EMSTR_LIB") or os.path.join(HERE, "libemstr.so")
 
 _lib = None
 

@@ -4,7 +4,8 @@
 
 HIP sources are compiled with hipcc for gfx950 only, without FMA
 contraction (the plain form reproduces the reference's FP64 operation
-order) and without packed-FP32 instructions; the C sources with gcc.
+order; the batched FP32 form alone may contract) and without packed-FP32
+instructions; the C sources with gcc.
 """
 import os
 import subprocess
@@ -23,6 +24,9 @@ C_SRC = ["emstr_host.c", "emstr_grid.c", "emstr_err.c"]
 HIPFLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
             "-packed-fp32-ops", "-Wno-unused-result"]
+# the batched FP32 form is not bound to the reference's operation order:
+# let the compiler fuse multiply-adds there
+HIP_EXTRA = {"emstr_batch.hip": ["-ffp-contract=fast"]}
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall",
           "-Wno-unused-function", "-D__HIP_PLATFORM_AMD__",
           "-I" + os.path.join(ROCM, "include")]
@@ -42,7 +46,8 @@ def build(verbose=False):
     hipcc = os.path.join(ROCM, "bin", "hipcc")
     for s in HIP_SRC:
         o = os.path.join(BUILD, s.replace(".hip", ".o"))
-        _run([hipcc] + HIPFLAGS + INC + ["-c", os.path.join(CSRC, s), "-o", o])
+        _run([hipcc] + HIPFLAGS + HIP_EXTRA.get(s, []) + INC +
+             ["-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     for s in C_SRC:
         o = os.path.join(BUILD, s.replace(".c", ".o"))

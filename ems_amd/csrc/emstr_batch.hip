@@ -88,6 +88,12 @@ __device__ __forceinline__ ll cidx(ll col, int nz, int k) { return col * nz + k;
 /*-------------------------------------------------------------------*/
 /* prep: per-cell FP32 coefficients from the FP64 fields              */
 /*-------------------------------------------------------------------*/
+/* the factor of the limited term of the van Leer face value for
+ * transport q with Courant number cn: 1 - cn if q > 0, 1 + cn if q < 0 */
+__device__ __forceinline__ float vlfac(double q, double cn) {
+  return (q > 0.0) ? (float)(1.0 - cn) : (q < 0.0) ? (float)(1.0 + cn) : 1.0f;
+}
+
 __global__ void k_prepB(emstr_dev d, emstr_batch b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -111,17 +117,25 @@ __global__ void k_prepB(emstr_dev d, emstr_batch b) {
   /* the top face's transport and Courant number are the bottom face's
    * of the cell above (identical expressions) */
   const double qT = top ? d.w[idx + P] * dt * area : 0.0;
-  float *c = b.co + cidx(col, d.nz, k) * C_N;
+  /* the van Leer face value needs 0.5 (1 - cn) for q > 0 and 0.5 (1 + cn)
+   * for q < 0 (tracers.c:2097-2118); the factor 2 of the limiter is
+   * folded in, so the slot holds 1 - cn or 1 + cn */
+  const double cW = d.u1[idx] * dt / d.h1au1[col];
+  const double cE = we ? d.u1[idx + 1] * dt / d.h1au1[col + 1] : 0.0;
+  const double cS = d.u2[idx] * dt / d.h2au2[col];
+  const double cN = wn ? d.u2[idx + nx] * dt / d.h2au2[col + nx] : 0.0;
+  const double cB = d.w[idx] * dt / (0.5 * (dz + dzb));
+  float c[C_N];
   c[C_QW] = (float)qW;
-  c[C_CW] = (float)(d.u1[idx] * dt / d.h1au1[col]);
+  c[C_CW] = vlfac(qW, cW);
   c[C_QE] = (float)qE;
-  c[C_CE] = we ? (float)(d.u1[idx + 1] * dt / d.h1au1[col + 1]) : 0.0f;
+  c[C_CE] = vlfac(qE, cE);
   c[C_QS] = (float)qS;
-  c[C_CS] = (float)(d.u2[idx] * dt / d.h2au2[col]);
+  c[C_CS] = vlfac(qS, cS);
   c[C_QN] = (float)qN;
-  c[C_CN] = wn ? (float)(d.u2[idx + nx] * dt / d.h2au2[col + nx]) : 0.0f;
+  c[C_CN] = vlfac(qN, cN);
   c[C_QB] = (float)qB;
-  c[C_CB] = (float)(d.w[idx] * dt / (0.5 * (dz + dzb)));
+  c[C_CB] = vlfac(qB, cB);
   c[C_RV] = (dz != 0.0) ? (float)(1.0 / (area * dz)) : 0.0f;
   c[C_D] = (float)(qE - qW + qN - qS + qT - qB);
   /* vertical diffusion rows of the new surface column, normalised by
@@ -135,6 +149,11 @@ __global__ void k_prepB(emstr_dev d, emstr_batch b) {
   }
   c[C_GM] = gm;
   c[C_GP] = gp;
+  c[C_UD] = 0.0f;
+  c[C_RD] = 1.0f;
+  float4 *c4 = (float4 *)(b.co + cidx(col, d.nz, k) * C_N);
+  for (int w = 0; w < C_N / 4; w++)
+    c4[w] = make_float4(c[4 * w], c[4 * w + 1], c[4 * w + 2], c[4 * w + 3]);
 }
 
 /* per-column information, one thread per processed-or-wet column */
@@ -254,88 +273,21 @@ __global__ void k_colB(emstr_dev d, emstr_batch b) {
 /* the fused step                                                    */
 /*-------------------------------------------------------------------*/
 /* van Leer face value (tracers.c:2097-2118) for transport q through the
- * face between bm1 and c, with Courant number cn.  The limiter's division
- * uses the hardware reciprocal (about 1 ulp). */
+ * face between bm1 and c; f is 1 - cn for q > 0 and 1 + cn for q < 0 (the
+ * prep folds the Courant number and the limiter's factor 2 into it).  The
+ * limiter's division uses the hardware reciprocal (about 1 ulp). */
 __device__ __forceinline__ float vlf(float bm2, float bm1, float c, float fp1,
-                                     float q, float cn) {
-  float df;
+                                     float q, float f) {
   if (q > 0.0f) {
-    df = (bm1 - bm2) * (c - bm1);
-    df = (df > 0.0f) ? __fdividef(2.0f * df, c - bm2) : 0.0f;
-    return bm1 + 0.5f * (1.0f - cn) * df;
+    const float df = (bm1 - bm2) * (c - bm1);
+    const float t = df * __builtin_amdgcn_rcpf(c - bm2);
+    return (df > 0.0f) ? bm1 + f * t : bm1;
   } else if (q < 0.0f) {
-    df = (c - bm1) * (fp1 - c);
-    df = (df > 0.0f) ? __fdividef(2.0f * df, fp1 - bm1) : 0.0f;
-    return c - 0.5f * (1.0f + cn) * df;
+    const float df = (c - bm1) * (fp1 - c);
+    const float t = df * __builtin_amdgcn_rcpf(fp1 - bm1);
+    return (df > 0.0f) ? c - f * t : c;
   }
   return c; /* no transport: F - c = 0 */
-}
-
-/* Offsets (in floats) from a cell to its twelve stencil neighbours.  A
- * neighbour that is land or below the bed is replaced by the cell the map
- * walk came from, whose ghost holds that value after the lateral boundary
- * fill (tracers.c:5574-5593); the bed maps down to its sediment ghost
- * (the bed value, tracers.c:827) and the top layer maps up to itself.
- * Everything here is uniform across the wave. */
-struct Sten {
-  ll w1, w2, e1, e2, s1, s2, n1, n2, zm1, zm2, zp1, zp2;
-  bool ew, nw; /* east / north face open */
-};
-
-__device__ __forceinline__ void sten(Sten &s, int k, int kb, int top, const ColInfo &ci,
-                                     ll zstep, ll cstep, ll rstep) {
-  const bool w1 = k >= ci.kw1, e1 = k >= ci.ke1, s1 = k >= ci.ks1, n1 = k >= ci.kn1;
-  s.w1 = w1 ? -cstep : 0;
-  s.e1 = e1 ? cstep : 0;
-  s.w2 = w1 ? ((k >= ci.kw2) ? -2 * cstep : s.w1) : 0;
-  s.e2 = e1 ? ((k >= ci.ke2) ? 2 * cstep : s.e1) : 0;
-  s.s1 = s1 ? -rstep : 0;
-  s.n1 = n1 ? rstep : 0;
-  s.s2 = s1 ? ((k >= ci.ks2) ? -2 * rstep : s.s1) : 0;
-  s.n2 = n1 ? ((k >= ci.kn2) ? 2 * rstep : s.n1) : 0;
-  s.zm1 = (ll)(max(k - 1, kb) - k) * zstep;
-  s.zm2 = (ll)(max(k - 2, kb) - k) * zstep;
-  s.zp1 = (ll)(min(k + 1, top) - k) * zstep;
-  s.zp2 = (ll)(min(k + 2, top) - k) * zstep;
-  s.ew = e1;
-  s.nw = n1;
-}
-
-/* The stencil values of one cell for one tracer lane; pc is the cell's
- * address for lane 0, the lane reads pc[off + ln].  All loads are issued
- * together. */
-struct Vals {
-  float v0, xw1, xw2, xe1, xe2, ys1, ys2, yn1, yn2, zm1, zm2, zp1, zp2;
-};
-
-__device__ __forceinline__ void load_vals(Vals &v, const float *__restrict__ pc, unsigned ln,
-                                          const Sten &s, bool vert) {
-  v.v0 = pc[ln];
-  v.xw1 = pc[s.w1 + ln];
-  v.xw2 = pc[s.w2 + ln];
-  v.xe1 = pc[s.e1 + ln];
-  v.xe2 = pc[s.e2 + ln];
-  v.ys1 = pc[s.s1 + ln];
-  v.ys2 = pc[s.s2 + ln];
-  v.yn1 = pc[s.n1 + ln];
-  v.yn2 = pc[s.n2 + ln];
-  if (vert) {
-    v.zm1 = pc[s.zm1 + ln];
-    v.zm2 = pc[s.zm2 + ln];
-    v.zp1 = pc[s.zp1 + ln];
-    v.zp2 = pc[s.zp2 + ln];
-  }
-}
-
-/* Horizontal anomaly outflow sum_f s_f (F_f - ref) q_f of a cell */
-__device__ __forceinline__ float hsum(const Vals &v, const Sten &s,
-                                      const float *__restrict__ c, float ref) {
-  const float FW = vlf(v.xw2, v.xw1, v.v0, v.xe1, c[C_QW], c[C_CW]);
-  const float FE = s.ew ? vlf(v.xw1, v.v0, v.xe1, v.xe2, c[C_QE], c[C_CE]) : ref;
-  const float FS = vlf(v.ys2, v.ys1, v.v0, v.yn1, c[C_QS], c[C_CS]);
-  const float FN = s.nw ? vlf(v.ys1, v.v0, v.yn1, v.yn2, c[C_QN], c[C_CN]) : ref;
-  return (FE - ref) * c[C_QE] - (FW - ref) * c[C_QW] + (FN - ref) * c[C_QN] -
-         (FS - ref) * c[C_QS];
 }
 
 /*-------------------------------------------------------------------*/
@@ -348,6 +300,16 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
   x ^= x >> 15;
   x *= 0x846ca68bU;
   x ^= x >> 16;
+  return x;
+}
+
+/* per-cell key: one multiply round over the cell index and the lane's
+ * (tracer, step) hash */
+__device__ __forceinline__ uint32_t srkey(uint32_t cellkey, uint32_t hl) {
+  uint32_t x = cellkey ^ hl;
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
   return x;
 }
 
@@ -365,109 +327,227 @@ __device__ __forceinline__ float sround(double x, uint32_t r) {
 }
 
 /*-------------------------------------------------------------------*/
-/* the fused step: one workgroup (CW waves) per water column          */
+/* the fused step: one wave walks one water column                    */
 /*-------------------------------------------------------------------*/
-#define CW 4 /* waves per column */
+#define WPB 4 /* waves per workgroup, each on its own columns */
 
 struct KArgs {
   int nz, ntr, nord, g; /* g: which 64 tracers this launch takes */
+  int nact;             /* active lanes: min(64, ntr - 64 g)       */
+  int pad;
   ll nx;
   double hmin, dt;
   ll plane;
 };
 
+/* lane access: p is a wave-uniform byte address, o the lane's byte offset */
+__device__ __forceinline__ float ldb(const char *p, uint32_t o) {
+  return *(const float *)(p + o);
+}
+__device__ __forceinline__ void stb(char *p, uint32_t o, float v) { *(float *)(p + o) = v; }
+__device__ __forceinline__ float rdl(int v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(v, l));
+}
+
+/* Base addresses of the columns around one column; a neighbour that is
+ * land or below the bed at layer k is replaced by the cell the map walk
+ * came from, whose ghost holds that value after the lateral boundary fill
+ * (tracers.c:5574-5593).  All wave-uniform. */
+struct Nbr {
+  const char *c, *w1, *w2, *e1, *e2, *s1, *s2, *n1, *n2;
+  int kw1, kw2, ke1, ke2, ks1, ks2, kn1, kn2;
+};
+
+struct HVals {
+  float w1, w2, e1, e2, s1, s2, n1, n2;
+};
+
+/* the eight horizontal neighbours at layer k; o = lane offset + k * row */
+__device__ __forceinline__ void load_h(HVals &h, const Nbr &nb, int k, uint32_t o) {
+  const bool w1 = k >= nb.kw1, e1 = k >= nb.ke1, s1 = k >= nb.ks1, n1 = k >= nb.kn1;
+  const char *pw1 = w1 ? nb.w1 : nb.c, *pe1 = e1 ? nb.e1 : nb.c;
+  const char *ps1 = s1 ? nb.s1 : nb.c, *pn1 = n1 ? nb.n1 : nb.c;
+  const char *pw2 = (w1 && k >= nb.kw2) ? nb.w2 : pw1;
+  const char *pe2 = (e1 && k >= nb.ke2) ? nb.e2 : pe1;
+  const char *ps2 = (s1 && k >= nb.ks2) ? nb.s2 : ps1;
+  const char *pn2 = (n1 && k >= nb.kn2) ? nb.n2 : pn1;
+  h.w1 = ldb(pw1, o);
+  h.w2 = ldb(pw2, o);
+  h.e1 = ldb(pe1, o);
+  h.e2 = ldb(pe2, o);
+  h.s1 = ldb(ps1, o);
+  h.s2 = ldb(ps2, o);
+  h.n1 = ldb(pn1, o);
+  h.n2 = ldb(pn2, o);
+}
+
+/* horizontal anomaly outflow sum_f s_f (F_f - ref) q_f of a cell whose
+ * value is v0, from the face transports q and van Leer factors f */
+__device__ __forceinline__ float hflux(const HVals &h, bool ew, bool nw, float v0, float ref,
+                                       float qW, float fW, float qE, float fE, float qS,
+                                       float fS, float qN, float fN) {
+  const float FW = vlf(h.w2, h.w1, v0, h.e1, qW, fW);
+  const float FE = ew ? vlf(h.w1, v0, h.e1, h.e2, qE, fE) : ref;
+  const float FS = vlf(h.s2, h.s1, v0, h.n1, qS, fS);
+  const float FN = nw ? vlf(h.s1, v0, h.n1, h.n2, qN, fN) : ref;
+  return (FE - ref) * qE - (FW - ref) * qW + (FN - ref) * qN - (FS - ref) * qS;
+}
+
+/*
+ * One wave per water column, lanes = tracers.  The wave walks up from the
+ * bed: the vertical part of the stencil is a sliding register window, the
+ * horizontal neighbours of the next layer are loaded while the current
+ * layer is computed, the cell coefficients arrive four layers per vector
+ * register (lane l holds word l) a chunk ahead and are read with readlane,
+ * and the flux through a cell's top face is the next cell's bottom-face
+ * flux.  The advected value b_k of each layer is written to the output (it
+ * stays in L2 until the final value replaces it) and the forward sweep of
+ * the vertical solve runs one layer behind the walk, keeping s_k in the
+ * wave's LDS slice.  The back substitution runs on the way down and writes
+ * the final values.
+ */
 template <int SCH>
-__global__ void __launch_bounds__(64 * CW)
-k_col(const KArgs a, const float *__restrict__ tin, float *__restrict__ tout,
-      const float *__restrict__ co, const ColInfo *__restrict__ cinfo,
-      const int *__restrict__ order, const float *__restrict__ mintr,
-      const float *__restrict__ maxtr, const double *__restrict__ layers,
-      const double *__restrict__ Kz, const double *__restrict__ dzf,
-      float *__restrict__ scr, const int *__restrict__ stepctr) {
-  /* LDS per column: Lb[nz][64] advected values, Ls[nz][64] forward sweep
-   * then increments, Lpre[64] unclipped update of the layer below the
-   * surface, Ltrs[64] new surface value */
+__global__ void __launch_bounds__(64 * WPB)
+k_walk(const KArgs a, const float *__restrict__ tin, float *__restrict__ tout,
+       const float *__restrict__ co, const ColInfo *__restrict__ cinfo,
+       const int *__restrict__ order, const float *__restrict__ mintr,
+       const float *__restrict__ maxtr, const double *__restrict__ layers,
+       const double *__restrict__ Kz, const double *__restrict__ dzf,
+       float *__restrict__ scr, const int *__restrict__ stepctr) {
   extern __shared__ float lds[];
-  const int nz = a.nz, NT = a.ntr, top = nz - 1;
-  float *Lb = lds;
-  float *Ls = lds + nz * 64;
-  float *Lpre = lds + 2 * nz * 64;
-  float *Ltrs = Lpre + 64;
+  const int nz = a.nz, NT = a.ntr, top = nz - 1, nact = a.nact;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int n = a.g * 64 + lane;
-  const bool act = n < NT;
-  const unsigned ln = act ? n : NT - 1; /* inactive lanes shadow the last tracer */
-  const ll zstep = NT;
-  const ll cstep = (ll)nz * NT;
-  const ll rstep = a.nx * cstep;
+  const int li = min(lane, nact - 1); /* inactive lanes shadow the last tracer */
+  /* LDS slice of this wave: s_k (then d_k) [nz][nact], ud_k [nz] */
+  float *Ls = lds + wv * nz * (nact + 1);
+  float *Lud = Ls + nz * nact;
+  const int ln = a.g * 64 + li;
+  const bool act = lane < nact;
+  const uint32_t lb = (uint32_t)ln * 4u;
+  const uint32_t rowb = (uint32_t)NT * 4u; /* bytes per layer of a column */
+  const ll colb = (ll)nz * NT * 4;         /* bytes per column */
+  const ll jb = a.nx * colb;               /* bytes per row of columns */
   const float mn = mintr[ln];
   const float mx = maxtr[ln];
   const bool clip = mn <= 1e-5f;
-  const uint32_t hl = mix32(ln * 0x85EBCA77U ^ (uint32_t)stepctr[0] * 0xC2B2AE3DU);
+  const uint32_t hl = mix32((uint32_t)ln * 0x85EBCA77U ^ (uint32_t)stepctr[0] * 0xC2B2AE3DU);
+  const int nslot = gridDim.x * WPB;
 
-  for (int q = blockIdx.x; q < a.nord; q += gridDim.x) {
+  for (int q = blockIdx.x * WPB + wv; q < a.nord; q += nslot) {
     const int colq = order[q];
     if (colq < 0) continue;
     const ll col = colq;
     const ColInfo ci = cinfo[col];
     const int kb = ci.kb, ks = ci.ks;
-    const float *tc = tin + col * cstep; /* the column, lane 0 */
-    float *oc = tout + col * cstep;
+    Nbr nb;
+    nb.c = (const char *)tin + col * colb;
+    nb.w1 = nb.c - colb;
+    nb.w2 = nb.c - 2 * colb;
+    nb.e1 = nb.c + colb;
+    nb.e2 = nb.c + 2 * colb;
+    nb.s1 = nb.c - jb;
+    nb.s2 = nb.c - 2 * jb;
+    nb.n1 = nb.c + jb;
+    nb.n2 = nb.c + 2 * jb;
+    nb.kw1 = ci.kw1;
+    nb.kw2 = ci.kw2;
+    nb.ke1 = ci.ke1;
+    nb.ke2 = ci.ke2;
+    nb.ks1 = ci.ks1;
+    nb.ks2 = ci.ks2;
+    nb.kn1 = ci.kn1;
+    nb.kn2 = ci.kn2;
+    char *oc = (char *)tout + col * colb;
+    const char *tc = nb.c;
     const float *cob = co + col * nz * C_N;
-    const uint32_t hcol = (uint32_t)(col * nz) * 0x9E3779B1U;
+    const uint32_t hcol = (uint32_t)(col * nz);
     if (!(ci.flags & F_PROC)) {
-      for (int k = kb + wv; k <= top; k += CW)
-        if (act) oc[k * zstep + ln] = tc[k * zstep + ln];
+      for (int k = kb; k <= top; k++)
+        if (act) stb(oc, lb + k * rowb, ldb(tc, lb + k * rowb));
       continue;
     }
-    /* ---- phase 1: the layers up to the processed surface, spread over
-     * the waves ---- */
-    for (int k = kb + wv; k <= ks; k += CW) {
-      const float *c = cob + k * C_N;
-      const float *pc = tc + k * zstep;
-      Sten s;
-      sten(s, k, kb, top, ci, zstep, cstep, rstep);
-      Vals v;
-      load_vals(v, pc, ln, s, true);
-      const float v0 = v.v0;
-      const float FB = vlf(v.zm2, v.zm1, v0, v.zp1, c[C_QB], c[C_CB]);
-      const float fba = (FB - v0) * c[C_QB];
-      const float h = hsum(v, s, c, v0);
-      const uint32_t key = mix32((hcol + (uint32_t)k * 0x9E3779B1U) ^ hl);
+    const bool solve = !(ci.flags & F_SINGLE);
+    const int cb = solve ? ci.cb : nz, cs = solve ? ci.cs : -1;
+
+    /* coefficient chunks: layers kc..kc+3 and kc+4..kc+7 */
+    int kc = kb;
+    int ch0 = __float_as_int(cob[kc * C_N + lane]);
+    int ch1 = __float_as_int(cob[(kc + 4) * C_N + lane]);
+    /* vertical window t[k-2 .. k+2], clamped to [kb, top] */
+    float zm2, zm1, v0, zp1, zp2;
+    v0 = ldb(tc, lb + kb * rowb);
+    zm1 = zm2 = v0;
+    zp1 = ldb(tc, lb + min(kb + 1, top) * rowb);
+    zp2 = ldb(tc, lb + min(kb + 2, top) * rowb);
+    HVals h;
+    load_h(h, nb, kb, lb + kb * rowb);
+    /* bottom-face flux of the bed cell */
+    float FB = vlf(zm2, zm1, v0, zp1, rdl(ch0, C_QB), rdl(ch0, C_CB));
+    /* forward-sweep state: b_{k-2}, b_{k-1}, s_{k-2}, coefficients of k-1 */
+    float bm2 = 0.0f, bm1 = 0.0f, sprev = 0.0f;
+    float gm1 = 0.0f, gp1 = 0.0f, rd1 = 1.0f;
+    float pre = 0.0f; /* unclipped update of the layer below the surface */
+    float trs = 0.0f;
+
+    for (int k = kb; k <= ks; k++) {
+      const int r = 16 * (k - kc); /* word of layer k in ch0 */
+      /* prefetch the next layer's horizontal neighbours and t[k+3] */
+      HVals hn;
+      float zp3 = zp2;
       if (k < ks) {
-        /* sub-surface update (tracers.c:912-925) in anomaly form; the
-         * top face is the bottom face of the cell above */
-        const float qT = c[C_N + C_QB];
-        const float FT = vlf(v.zm1, v0, v.zp1, v.zp2, qT, c[C_N + C_CB]);
-        const float ht = h + (FT - v0) * qT - fba;
-        const double an = (double)v0 - (double)((ht + v0 * c[C_D]) * c[C_RV]);
-        float af = sround(an, key);
-        if (k == ks - 1) Lpre[lane] = af;
-        if (clip && af < mn) af = mn;
-        Lb[k * 64 + lane] = af;
+        load_h(hn, nb, k + 1, lb + (k + 1) * rowb);
+        zp3 = ldb(tc, lb + min(k + 3, top) * rowb);
+      }
+      const float qW = rdl(ch0, r + C_QW), fW = rdl(ch0, r + C_CW);
+      const float qE = rdl(ch0, r + C_QE), fE = rdl(ch0, r + C_CE);
+      const float qS = rdl(ch0, r + C_QS), fS = rdl(ch0, r + C_CS);
+      const float qN = rdl(ch0, r + C_QN), fN = rdl(ch0, r + C_CN);
+      const float qB = rdl(ch0, r + C_QB);
+      const float gmk = rdl(ch0, r + C_GM), gpk = rdl(ch0, r + C_GP);
+      const float rdk = rdl(ch0, r + C_RD), udk = rdl(ch0, r + C_UD);
+      const float fba = (FB - v0) * qB;
+      const float hh = hflux(h, k >= ci.ke1, k >= ci.kn1, v0, v0, qW, fW, qE, fE, qS, fS,
+                             qN, fN);
+      const uint32_t key = srkey(hcol + (uint32_t)k, hl);
+      float bk;
+      if (k < ks) {
+        /* sub-surface update (tracers.c:912-925) in anomaly form; the top
+         * face is the bottom face of the cell above */
+        const int nsrc = (r < 48) ? ch0 : ch1;
+        const int rn = (r + 16) & 63;
+        const float qT = rdl(nsrc, rn + C_QB);
+        const float FT = vlf(zm1, v0, zp1, zp2, qT, rdl(nsrc, rn + C_CB));
+        const float ht = hh + (FT - v0) * qT - fba;
+        const double an =
+            (double)v0 - (double)((ht + v0 * rdl(ch0, r + C_D)) * rdl(ch0, r + C_RV));
+        bk = sround(an, key);
+        pre = bk;
+        if (clip && bk < mn) bk = mn;
+        FB = FT;
       } else {
         /* processed surface cell: surf_conc (tracers.c:3047-3207) as an
          * FP64 volume balance, v0 * Vc + anomaly terms, Vc in FP64 */
-        double m = (double)fba - (double)h;
+        double m = (double)fba - (double)hh;
         if (ci.flags & F_MOVE) {
           int cc = ks;
           double wb = (ks == kb) ? ci.botz : layers[ks];
           while (cc != ci.kold && cc < top) {
             const double vol = ci.area * (layers[cc + 1] - wb);
-            if (cc > ks) m += ((double)tc[cc * zstep + ln] - (double)v0) * vol;
+            if (cc > ks) m += ((double)ldb(tc, lb + cc * rowb) - (double)v0) * vol;
             cc++;
             wb = layers[cc];
           }
           if (cc > ks)
-            m += ((double)tc[cc * zstep + ln] - (double)v0) *
+            m += ((double)ldb(tc, lb + cc * rowb) - (double)v0) *
                  (ci.area * (ci.osub - ((cc == kb) ? ci.botz : layers[cc])));
           for (int ka = ks + 1; ka <= top; ka++) {
-            Sten sa;
-            sten(sa, ka, kb, top, ci, zstep, cstep, rstep);
-            Vals va;
-            load_vals(va, tc + ka * zstep, ln, sa, false);
-            m -= (double)hsum(va, sa, cob + ka * C_N, v0);
+            HVals ha;
+            load_h(ha, nb, ka, lb + ka * rowb);
+            const float *ca = cob + ka * C_N;
+            m -= (double)hflux(ha, ka >= ci.ke1, ka >= ci.kn1, ldb(tc, lb + ka * rowb), v0,
+                               ca[C_QW], ca[C_CW], ca[C_QE], ca[C_CE], ca[C_QS], ca[C_CS],
+                               ca[C_QN], ca[C_CN]);
           }
         }
         double surftr = (double)v0 * ci.vc + m;
@@ -476,94 +556,133 @@ k_col(const KArgs a, const float *__restrict__ tin, float *__restrict__ tout,
         else
           surftr /= ci.svol;
         /* tracers.c:953-954 */
-        Ltrs[lane] = (ci.flags & F_KEEPOLD) ? v0 : sround(surftr, key);
+        trs = (ci.flags & F_KEEPOLD) ? v0 : sround(surftr, key);
+        /* tr_bounds (tracers.c:3217-3324): thin old surface layer, clip */
+        if ((ci.flags & F_THIN) && (trs < mn || trs > mx)) trs = pre;
+        if (clip && trs < mn) trs = mn;
+        bk = trs;
       }
-    }
-    __syncthreads();
-    /* ---- phase 2 (one wave): tr_bounds at the surface
-     * (tracers.c:3217-3324), then implicit vertical diffusion in increment
-     * form: (A/dzdt) d = b - (A/dzdt) b, x = b + d ---- */
-    if (wv == 0) {
-      float trs = Ltrs[lane];
-      if ((ci.flags & F_THIN) && (trs < mn || trs > mx)) trs = Lpre[lane];
-      if (clip && trs < mn) trs = mn;
-      for (int k = ks; k <= top; k++) Lb[k * 64 + lane] = trs;
-      if (ci.flags & F_SINGLE) {
-        for (int k = kb; k <= top; k++)
-          if (act) oc[k * zstep + ln] = Lb[k * 64 + lane];
-      } else {
-        const int cb = ci.cb, cs = ci.cs;
-        for (int k = kb; k < cb; k++)
-          if (act) oc[k * zstep + ln] = Lb[k * 64 + lane];
-        for (int k = cs + 1; k <= top; k++)
-          if (act) oc[k * zstep + ln] = Lb[k * 64 + lane];
-        /* forward sweep with the factorisation from the prep:
-         * s_k = (r_k - gm_k s_{k-1}) / div_k */
-        float sp = 0.0f;
-        float bkm = Lb[cb * 64 + lane], bk = bkm;
-        for (int k = cb; k <= cs; k++) {
-          const float *cm = cob + k * C_N;
-          const float bkp = (k < cs) ? Lb[(k + 1) * 64 + lane] : bk;
-          const float gm = (k > cb) ? cm[C_GM] : 0.0f, gp = cm[C_GP];
-          const float r = -(gm * (bkm - bk) + gp * (bkp - bk));
-          sp = (r - gm * sp) * cm[C_RD];
-          Ls[k * 64 + lane] = sp;
-          bkm = bk;
-          bk = bkp;
-        }
-        const bool stiff = (ci.flags & F_STIFF) != 0;
-        float dn = sp;
-        for (int k = cs; k >= cb; k--) {
-          if (k < cs) {
-            dn = Ls[k * 64 + lane] - cob[(k + 1) * C_N + C_UD] * dn;
-            Ls[k * 64 + lane] = dn;
-          }
-          if (!stiff && act)
-            oc[k * zstep + ln] =
-                sround((double)Lb[k * 64 + lane] + (double)dn,
-                       mix32((hcol + (uint32_t)k * 0x9E3779B1U) ^ hl ^ 0x5bd1e995U));
-        }
-        if (stiff) {
-          /* one FP64 residual pass of iterative refinement, the matrix
-           * and right-hand side in FP64 from Kz and dz
-           * (tracers.c:3841-3889) */
-          const ll P = a.plane;
-          const double dt = a.dt;
-          float *sc = scr + col * cstep;
-          float e = 0.0f;
-          for (int k = cb; k <= cs; k++) {
-            const ll ix = (ll)k * P + col;
-            const double dzc = dzf[ix], dzdt = dzc / dt;
-            double cm1 = 0.0, cp1 = 0.0;
-            if (k > cb) cm1 = -Kz[ix] / (0.5 * (dzf[ix - P] + dzc));
-            if (k < cs) cp1 = -Kz[ix + P] / (0.5 * (dzc + dzf[ix + P]));
-            const double bv = (double)Lb[k * 64 + lane];
-            const double dk = (double)Ls[k * 64 + lane];
-            const double bvm = (k > cb) ? (double)Lb[(k - 1) * 64 + lane] : bv;
-            const double bvp = (k < cs) ? (double)Lb[(k + 1) * 64 + lane] : bv;
-            const double dkm = (k > cb) ? (double)Ls[(k - 1) * 64 + lane] : 0.0;
-            const double dkp = (k < cs) ? (double)Ls[(k + 1) * 64 + lane] : 0.0;
-            const double rhs = -(cm1 * (bvm - bv) + cp1 * (bvp - bv));
-            const double Ad = cm1 * dkm + (dzdt - cm1 - cp1) * dk + cp1 * dkp;
-            const float rho = (float)((rhs - Ad) / dzdt);
-            const float *cm = cob + k * C_N;
-            const float gm = (k > cb) ? cm[C_GM] : 0.0f;
-            e = (rho - gm * e) * cm[C_RD];
-            sc[k * zstep + ln] = e;
-          }
-          float ec = 0.0f;
-          for (int k = cs; k >= cb; k--) {
-            const float ek = sc[k * zstep + ln];
-            ec = (k == cs) ? ek : ek - cob[(k + 1) * C_N + C_UD] * ec;
-            if (act)
-              oc[k * zstep + ln] =
-                  sround((double)Lb[k * 64 + lane] + (double)Ls[k * 64 + lane] + (double)ec,
-                         mix32((hcol + (uint32_t)k * 0x9E3779B1U) ^ hl ^ 0x5bd1e995U));
-          }
+      if (act) stb(oc, lb + k * rowb, bk);
+      Lud[k] = udk;
+      /* forward sweep for layer k-1, now that b_k is known:
+       * s = (r - gm s_prev) / div, r = -(gm (b_{k-2} - b_{k-1}) +
+       * gp (b_k - b_{k-1})) */
+      {
+        const int kf = k - 1;
+        if (kf >= cb && kf <= cs) {
+          const float gm = (kf > cb) ? gm1 : 0.0f;
+          const float rr = -(gm * (bm2 - bm1) + gp1 * (bk - bm1));
+          sprev = (rr - gm * sprev) * rd1;
+          Ls[kf * nact + li] = sprev;
         }
       }
+      gm1 = gmk;
+      gp1 = gpk;
+      rd1 = rdk;
+      bm2 = bm1;
+      bm1 = bk;
+      /* slide */
+      zm2 = zm1;
+      zm1 = v0;
+      v0 = zp1;
+      zp1 = zp2;
+      zp2 = zp3;
+      if (k < ks) h = hn;
+      if (r == 48) {
+        /* next chunk */
+        kc += 4;
+        ch0 = ch1;
+        ch1 = __float_as_int(cob[(kc + 4) * C_N + lane]);
+      }
     }
-    __syncthreads();
+    /* the surface value fills every layer above (tracers.c:3306-3322) and
+     * the forward sweep runs on to the new surface */
+    for (int k = ks + 1; k <= top; k++) {
+      if (act) stb(oc, lb + k * rowb, trs);
+      const float *cf = cob + k * C_N;
+      const float gmk = cf[C_GM], gpk = cf[C_GP], rdk = cf[C_RD];
+      Lud[k] = cf[C_UD];
+      const int kf = k - 1;
+      if (kf >= cb && kf <= cs) {
+        const float gm = (kf > cb) ? gm1 : 0.0f;
+        const float rr = -(gm * (bm2 - bm1) + gp1 * (trs - bm1));
+        sprev = (rr - gm * sprev) * rd1;
+        Ls[kf * nact + li] = sprev;
+      }
+      gm1 = gmk;
+      gp1 = gpk;
+      rd1 = rdk;
+      bm2 = bm1;
+      bm1 = trs;
+    }
+    if (cs == top) {
+      /* last row: b_{k+1} does not enter (gp = 0) */
+      const float gm = (cs > cb) ? gm1 : 0.0f;
+      const float rr = -(gm * (bm2 - bm1));
+      sprev = (rr - gm * sprev) * rd1;
+      Ls[cs * nact + li] = sprev;
+    }
+    if (!solve) continue;
+    /* back substitution: d_k = s_k - ud_{k+1} d_{k+1}, x_k = b_k + d_k */
+    const bool stiff = (ci.flags & F_STIFF) != 0;
+    float dn = 0.0f;
+    float bnext = ldb(oc, lb + cs * rowb);
+    for (int k = cs; k >= cb; k--) {
+      const float bv = bnext;
+      if (k > cb) bnext = ldb(oc, lb + (k - 1) * rowb);
+      const float sk = Ls[k * nact + li];
+      dn = (k == cs) ? sk : sk - Lud[k + 1] * dn;
+      if (stiff) {
+        Ls[k * nact + li] = dn;
+      } else if (act) {
+        stb(oc, lb + k * rowb,
+            sround((double)bv + (double)dn, srkey(hcol + (uint32_t)k, hl ^ 0x5bd1e995U)));
+      }
+    }
+    if (stiff) {
+      /* one FP64 residual pass of iterative refinement, the matrix and
+       * right-hand side in FP64 from Kz and dz (tracers.c:3841-3889) */
+      const ll P = a.plane;
+      const double dt = a.dt;
+      char *sc = (char *)scr + col * colb;
+      float e = 0.0f;
+      for (int k = cb; k <= cs; k++) {
+        const ll ix = (ll)k * P + col;
+        const double dzc = dzf[ix], dzdt = dzc / dt;
+        double cm1 = 0.0, cp1 = 0.0;
+        if (k > cb) cm1 = -Kz[ix] / (0.5 * (dzf[ix - P] + dzc));
+        if (k < cs) cp1 = -Kz[ix + P] / (0.5 * (dzc + dzf[ix + P]));
+        const double bv = (double)ldb(oc, lb + k * rowb);
+        const double dk = (double)Ls[k * nact + li];
+        const double bvm = (k > cb) ? (double)ldb(oc, lb + (k - 1) * rowb) : bv;
+        const double bvp = (k < cs) ? (double)ldb(oc, lb + (k + 1) * rowb) : bv;
+        const double dkm = (k > cb) ? (double)Ls[(k - 1) * nact + li] : 0.0;
+        const double dkp = (k < cs) ? (double)Ls[(k + 1) * nact + li] : 0.0;
+        const double rhs = -(cm1 * (bvm - bv) + cp1 * (bvp - bv));
+        const double Ad = cm1 * dkm + (dzdt - cm1 - cp1) * dk + cp1 * dkp;
+        const float rho = (float)((rhs - Ad) / dzdt);
+        const float *cm = cob + k * C_N;
+        const float gm = (k > cb) ? cm[C_GM] : 0.0f;
+        e = (rho - gm * e) * cm[C_RD];
+        stb(sc, lb + k * rowb, e);
+      }
+      float ec = 0.0f;
+      for (int k = cs; k >= cb; k--) {
+        const float ek = ldb(sc, lb + k * rowb);
+        ec = (k == cs) ? ek : ek - Lud[k + 1] * ec;
+        stb(sc, lb + k * rowb, ec);
+      }
+      /* b_k is still needed below k while the residual is formed: write
+       * the results after the pass */
+      for (int k = cb; k <= cs; k++) {
+        const float bv = ldb(oc, lb + k * rowb);
+        const float ek = ldb(sc, lb + k * rowb);
+        if (act)
+          stb(oc, lb + k * rowb,
+              sround((double)bv + (double)Ls[k * nact + li] + (double)ek,
+                     srkey(hcol + (uint32_t)k, hl ^ 0x5bd1e995U)));
+      }
+    }
   }
 }
 
@@ -644,14 +763,17 @@ __global__ void k_massB(emstr_dev d, emstr_batch b, const float *__restrict__ tr
 /*-------------------------------------------------------------------*/
 /* host interface                                                    */
 /*-------------------------------------------------------------------*/
-/* dynamic LDS of the fused kernel: Lb, Ls [nz][64], Lpre, Ltrs [64] */
-static size_t col_lds(int nz) { return ((size_t)2 * nz * 64 + 128) * sizeof(float); }
+/* dynamic LDS of the fused kernel: one [nz][nact + 1] slice per wave */
+static size_t walk_lds(int nz, int nact) {
+  return (size_t)WPB * nz * (nact + 1) * sizeof(float);
+}
 
 /* Column order: tiles of tile_x x tile_y columns in row-major tile
  * order, columns row-major inside a tile; the wet columns of that order
- * are split into 8 contiguous bands, and position q of the persistent
- * loop maps to band q % 8, so the waves resident on one XCD walk one band
- * (workgroups are dealt to the 8 XCDs round robin). */
+ * are split into 8 contiguous bands.  Position q of the persistent loop is
+ * taken by wave q % WPB of workgroup q / WPB, and workgroups are dealt to
+ * the 8 XCDs round robin, so position q maps to band (q / WPB) % 8 and the
+ * waves resident on one XCD walk one band together. */
 static void build_order(emstr_batch *b, const int *kb, int nz) {
   const int nx = b->nx, ny = b->ny, tx = b->tile_x, ty = b->tile_y;
   int *lin = (int *)malloc(sizeof(int) * (size_t)b->plane);
@@ -664,12 +786,15 @@ static void build_order(emstr_batch *b, const int *kb, int nz) {
           if (kb[col] < nz) lin[n++] = (int)col;
         }
   b->ncol = n;
-  const int per = (n + 7) / 8;
+  /* positions per band, a multiple of WPB */
+  const int per = ((n + 7) / 8 + WPB - 1) / WPB * WPB;
   b->nord = 8 * per;
   int *ord = (int *)malloc(sizeof(int) * (size_t)(b->nord > 0 ? b->nord : 1));
   for (int q = 0; q < b->nord; q++) {
-    const int src = (q % 8) * per + q / 8;
-    ord[q] = (src < n) ? lin[src] : -1;
+    const int m = q / WPB;
+    const int band = m % 8, idx = (m / 8) * WPB + q % WPB;
+    const int src = band * ((n + 7) / 8) + idx;
+    ord[q] = (idx < (n + 7) / 8 && src < n) ? lin[src] : -1;
   }
   hipMemcpy(b->order, ord, sizeof(int) * (size_t)b->nord, hipMemcpyHostToDevice);
   free(ord);
@@ -697,9 +822,10 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   for (int q = 0; q < 2; q++)
     ok &= hipMalloc(&b->tr[q], ncell * d->ntr * sizeof(float)) == hipSuccess;
   ok &= hipMalloc(&b->scr, ncell * d->ntr * sizeof(float)) == hipSuccess;
-  ok &= hipMalloc(&b->co, ncell * C_N * sizeof(float)) == hipSuccess;
+  /* 8 layers of padding: the walk reads coefficients in 4-layer chunks */
+  ok &= hipMalloc(&b->co, (ncell + 8) * C_N * sizeof(float)) == hipSuccess;
   ok &= hipMalloc(&b->ci, (size_t)d->plane * sizeof(ColInfo)) == hipSuccess;
-  ok &= hipMalloc(&b->order, ((size_t)d->plane + 8) * sizeof(int)) == hipSuccess;
+  ok &= hipMalloc(&b->order, ((size_t)d->plane + 8 * WPB + 8) * sizeof(int)) == hipSuccess;
   ok &= hipMalloc(&b->mintr, (d->ntr + 1) * sizeof(float)) == hipSuccess;
   ok &= hipMalloc(&b->maxtr, (d->ntr + 1) * sizeof(float)) == hipSuccess;
   ok &= hipMalloc(&b->stepctr, sizeof(int)) == hipSuccess;
@@ -708,7 +834,7 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
     emstr_batch_destroy(b);
     return NULL;
   }
-  hipMemset(b->co, 0, ncell * C_N * sizeof(float));
+  hipMemset(b->co, 0, (ncell + 8) * C_N * sizeof(float));
   hipMemset(b->stepctr, 0, sizeof(int));
   build_order(b, h_kb, d->nz);
   int dev = 0, ncu = 256;
@@ -716,12 +842,12 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   /* persistent grid: as many workgroups as can be resident at once */
   int per_cu = 0;
-  const size_t lds = col_lds(b->nz);
+  const size_t lds = walk_lds(b->nz, b->ntr < 64 ? b->ntr : 64);
   if (lds > 48 * 1024)
-    (void)hipFuncSetAttribute((const void *)k_col<VANLEER>,
+    (void)hipFuncSetAttribute((const void *)k_walk<VANLEER>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_col<VANLEER>,
-                                                   64 * CW, lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_walk<VANLEER>,
+                                                   64 * WPB, lds) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   b->grid = ncu * per_cu;
@@ -795,9 +921,11 @@ static int batch_launch(emstr_batch *b, const emstr_dev *d, hipStream_t st) {
   ka.hmin = d->hmin;
   ka.dt = d->dt;
   ka.plane = d->plane;
+  ka.pad = 0;
   for (int g = 0; g < b->nt64; g++) {
     ka.g = g;
-    k_col<VANLEER><<<b->grid, 64 * CW, col_lds(b->nz), st>>>(
+    ka.nact = b->ntr - 64 * g < 64 ? b->ntr - 64 * g : 64;
+    k_walk<VANLEER><<<b->grid, 64 * WPB, walk_lds(b->nz, ka.nact), st>>>(
         ka, in, out, b->co, b->ci, b->order, b->mintr, b->maxtr, d->layers, d->Kz, d->dz,
         b->scr, b->stepctr);
   }
