@@ -180,10 +180,13 @@ def main():
         out["ms_per_step"] = rB["ms"]
         ktB, kcB = rB["kt"], rB["kc"]
         dom = int(np.argmax(ktB))
-        # algorithmic bytes per launch of the dominant kernel: the fused
-        # kernel reads and writes every tracer of every wet cell once (FP32)
-        # and reads the 16 FP32 coefficients of the cell
-        alg = {4: nwet * (ntr * 8 + 16 * 4),
+        # algorithmic bytes per launch of the dominant kernel (FP32 tracers):
+        # advection reads and writes every tracer of every wet cell once and
+        # reads the cell's 16 coefficient words; the vertical solve reads
+        # and writes every tracer once and reads 4 coefficient words
+        alg = {0: nwet * (ntr * 8 + 16 * 4),
+               1: nwet * (ntr * 8 + 4 * 4),
+               4: nwet * (ntr * 8 + 16 * 4),
                2: nwet * (7 * 8 + 16 * 4)}.get(dom, nwet * ntr * 8)
         avg_ms = ktB[dom] / max(kcB[dom], 1)
         achieved = alg / (avg_ms * 1e-3) / 1e9

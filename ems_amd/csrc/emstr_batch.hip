@@ -1,35 +1,36 @@
 /*
- * Batched form of the EMS tracer step on MI355X (gfx950): every tracer of
- * the step and every water column in one fused persistent kernel.
+ * Batched form of the EMS tracer step on MI355X (gfx950): all tracers of
+ * the step over all water columns, FP32 state with FP64 budgets.
  *
- * Mapping: one wavefront per water column, one lane per tracer.  Tracers
- * are stored tracer-innermost, [column][layer][tracer] in FP32, so the
- * tracers of one cell are one contiguous 4*ntr-byte load and a column
- * walk reads each layer once.  Everything that does not depend on the
- * tracer (face transports, Courant numbers, the column matrix, wetness of
- * neighbours, the surface volume balance) is per cell or per column and
- * is read with scalar loads, once, for all tracers.
+ * Layout: tracers are stored tracer-innermost, [column][layer][tracer] in
+ * FP32, so the tracers of one cell are one contiguous 4*ntr-byte row and
+ * a wave with one lane per tracer reads a cell with one coalesced load.
+ * Everything that does not depend on the tracer (face transports, van
+ * Leer factors, the normalised column matrix and its LU factors) is
+ * prepared per cell once per step (k_prepB, k_colB) and read by all lanes.
  *
- * Per column the wave walks up from the bed:
- *   - face values (van Leer, tracers.c:2097-2118) on the 13-point
- *     stencil; the vertical part of the stencil is a register window;
- *   - the flux divergence and sub-surface update (tracers.c:888-925) in
- *     anomaly form:  tr - (sum_f s_f (F_f - tr) q_f + tr*D) / V
- *     with D the FP64 net volume divergence of the cell; this is the
- *     reference's tr - (sum_f s_f F_f q_f)/V, but rounding the transports
- *     q to FP32 only perturbs the small (F - tr) terms;
- *   - surf_conc (tracers.c:3047-3207) as an FP64 volume balance with the
- *     column's consistent volume precomputed in FP64;
- *   - tr_bounds (tracers.c:3217-3324): thin-layer repair, clip, copy up;
- *   - implicit vertical diffusion (tracers.c:3782-3957, 4067-4214) in
- *     increment form: A (x - b) = dzdt*b - A b, so constants are kept
- *     exactly; the forward sweep runs inside the walk, the increments are
- *     held in LDS and the back substitution runs on the way down;
- *   - one FP64 residual pass of iterative refinement in columns whose
- *     diffusion number exceeds IR_DMAX.
- * FP64 stays in the coefficient prep (layer thicknesses from set_dz, the
- * divergence D, the surface volume balance, the refinement residual) and
- * in the mass budgets.
+ * Per step:
+ *   k_prepB / k_colB  per-cell coefficients, per-column information (the
+ *                     FP64 surface volume balance, diffusion extent,
+ *                     stiffness, LU factors of the column matrix);
+ *   k_advB            cell-parallel advection: face values (van Leer,
+ *                     tracers.c:2097-2118) on the 13-point stencil and the
+ *                     flux divergence and sub-surface update
+ *                     (tracers.c:888-925) in anomaly form
+ *                       tr - (sum_f s_f (F_f - tr) q_f + tr*D) / V,
+ *                     D the FP64 net volume divergence of the cell, so that
+ *                     rounding the transports q to FP32 only perturbs the
+ *                     small (F - tr) terms; the processed surface cell by
+ *                     surf_conc (tracers.c:3047-3207) as an FP64 volume
+ *                     balance;
+ *   k_vertB           per column: tr_bounds (tracers.c:3217-3324), then the
+ *                     implicit vertical diffusion (tracers.c:3782-3957,
+ *                     4067-4214) in increment form, A (x - b) = dzdt*b - A b,
+ *                     with one FP64 residual pass of iterative refinement
+ *                     in columns whose diffusion number exceeds IR_DMAX.
+ * Every FP32 state write is stochastically rounded from an FP64 value with
+ * a deterministic hash of (cell, tracer, step), which keeps sub-ulp
+ * increments from biasing the state and the tracer mass.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -81,6 +82,13 @@ struct emstr_batch {
   int graph_ok;
   int graph_cur; /* buffer parity the graph was captured for */
   int *stepctr;  /* device step counter (stochastic rounding keys) */
+  /* kernel A: per wave slot lists of cell descriptors and of their
+   * columns' (processed surface + 1) | flags << 8 */
+  unsigned long long *cells;
+  int *kfl;
+  int niter;  /* list length per slot (a multiple of 64)               */
+  int gridA;  /* kernel A workgroups (4 wave slots each)               */
+  ll nlist;   /* gridA * 4 * niter                                     */
 };
 
 __device__ __forceinline__ ll cidx(ll col, int nz, int k) { return col * nz + k; }
@@ -393,45 +401,280 @@ __device__ __forceinline__ float hflux(const HVals &h, bool ew, bool nw, float v
   return (FE - ref) * qE - (FW - ref) * qW + (FN - ref) * qN - (FS - ref) * qS;
 }
 
+/*-------------------------------------------------------------------*/
+/* kernel A: cell-parallel advection                                  */
+/*-------------------------------------------------------------------*/
+/* Cell descriptor: column (bits 0-31), layer k (32-39), open neighbours
+ * at layer k (40-47, M_* below; a second neighbour is open only if the
+ * first one is), bed layer kb (48-55).  ~0 marks padding. */
+enum { M_W1 = 1, M_W2 = 2, M_E1 = 4, M_E2 = 8, M_S1 = 16, M_S2 = 32, M_N1 = 64, M_N2 = 128 };
+#define NO_CELL (~0ULL)
+
+struct AArgs {
+  int nz, ntr, g, nact; /* g: which 64 tracers this launch takes */
+  int niter;            /* iterations per wave slot (a multiple of 64) */
+  int pad;
+  ll nx;
+  double hmin;
+};
+
+/* the 13-point stencil of one cell for one tracer lane */
+struct St13 {
+  float v0, zm1, zm2, zp1, zp2, w1, w2, e1, e2, s1, s2, n1, n2;
+};
+
+__device__ __forceinline__ unsigned long long rdl64(unsigned long long v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+/* Issue the loads of the stencil of cell dsc (wave-uniform) and of its
+ * coefficient words (lanes 0..15: cell k, 16..31: cell k + 1). */
+__device__ __forceinline__ void st13_load(St13 &v, int &cw, const char *tin,
+                                          const float *co, unsigned long long dsc, ll colb,
+                                          ll jb, uint32_t lb, uint32_t rowb, int nz,
+                                          int lane) {
+  const ll col = (ll)(uint32_t)dsc;
+  const int k = (int)((dsc >> 32) & 255), m = (int)((dsc >> 40) & 255);
+  const int kb = (int)((dsc >> 48) & 255), top = nz - 1;
+  const char *c = tin + col * colb;
+  const uint32_t o = lb + k * rowb;
+  const char *pw1 = (m & M_W1) ? c - colb : c, *pe1 = (m & M_E1) ? c + colb : c;
+  const char *ps1 = (m & M_S1) ? c - jb : c, *pn1 = (m & M_N1) ? c + jb : c;
+  const char *pw2 = (m & M_W2) ? c - 2 * colb : pw1, *pe2 = (m & M_E2) ? c + 2 * colb : pe1;
+  const char *ps2 = (m & M_S2) ? c - 2 * jb : ps1, *pn2 = (m & M_N2) ? c + 2 * jb : pn1;
+  v.v0 = ldb(c, o);
+  v.zm1 = ldb(c, lb + max(k - 1, kb) * rowb);
+  v.zm2 = ldb(c, lb + max(k - 2, kb) * rowb);
+  v.zp1 = ldb(c, lb + min(k + 1, top) * rowb);
+  v.zp2 = ldb(c, lb + min(k + 2, top) * rowb);
+  v.w1 = ldb(pw1, o);
+  v.w2 = ldb(pw2, o);
+  v.e1 = ldb(pe1, o);
+  v.e2 = ldb(pe2, o);
+  v.s1 = ldb(ps1, o);
+  v.s2 = ldb(ps2, o);
+  v.n1 = ldb(pn1, o);
+  v.n2 = ldb(pn2, o);
+  cw = __float_as_int(co[(col * nz + k) * C_N + (lane & 31)]);
+}
+
 /*
- * One wave per water column, lanes = tracers.  The wave walks up from the
- * bed: the vertical part of the stencil is a sliding register window, the
- * horizontal neighbours of the next layer are loaded while the current
- * layer is computed, the cell coefficients arrive four layers per vector
- * register (lane l holds word l) a chunk ahead and are read with readlane,
- * and the flux through a cell's top face is the next cell's bottom-face
- * flux.  The advected value b_k of each layer is written to the output (it
- * stays in L2 until the final value replaces it) and the forward sweep of
- * the vertical solve runs one layer behind the walk, keeping s_k in the
- * wave's LDS slice.  The back substitution runs on the way down and writes
- * the final values.
+ * Kernel A: every wet cell of every column (all tracers of one cell per
+ * wave, lanes = tracers), cells taken in layer-synchronous tile order so
+ * that the neighbours a cell reads were read by other waves moments
+ * before.  Each wave slot has its own list of cells (descriptors and the
+ * column's processed surface and flags, 64 at a time, lane-distributed),
+ * and the stencil and coefficients of its next cell are loaded while the
+ * current one is computed.  Writes the advected value b of sub-surface
+ * cells and the raw new surface value (surf_conc) of the processed surface
+ * cell to the output; tr_bounds and the vertical solve follow in kernel B.
  */
 template <int SCH>
-__global__ void __launch_bounds__(64 * WPB)
-k_walk(const KArgs a, const float *__restrict__ tin, float *__restrict__ tout,
-       const float *__restrict__ co, const ColInfo *__restrict__ cinfo,
-       const int *__restrict__ order, const float *__restrict__ mintr,
-       const float *__restrict__ maxtr, const double *__restrict__ layers,
-       const double *__restrict__ Kz, const double *__restrict__ dzf,
+__global__ void __launch_bounds__(256)
+k_advB(const AArgs a, const float *__restrict__ tin, float *__restrict__ tout,
+       const float *__restrict__ co, const unsigned long long *__restrict__ cells,
+       const int *__restrict__ kfl, const ColInfo *__restrict__ cinfo,
+       const float *__restrict__ mintr, const double *__restrict__ layers,
        float *__restrict__ scr, const int *__restrict__ stepctr) {
-  extern __shared__ float lds[];
   const int nz = a.nz, NT = a.ntr, top = nz - 1, nact = a.nact;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = min(lane, nact - 1); /* inactive lanes shadow the last tracer */
-  /* LDS slice of this wave: s_k (then d_k) [nz][nact], ud_k [nz] */
+  const int ln = a.g * 64 + li;
+  const bool act = lane < nact;
+  const uint32_t lb = (uint32_t)ln * 4u;
+  const uint32_t rowb = (uint32_t)NT * 4u;
+  const ll colb = (ll)nz * NT * 4;
+  const ll jb = a.nx * colb;
+  const float mn = mintr[ln];
+  const bool clip = mn <= 1e-5f;
+  const uint32_t hl = mix32((uint32_t)ln * 0x85EBCA77U ^ (uint32_t)stepctr[0] * 0xC2B2AE3DU);
+  const char *tinb = (const char *)tin;
+  const int slot = blockIdx.x * 4 + wv;
+  const unsigned long long *myc = cells + (ll)slot * a.niter;
+  const int *myk = kfl + (ll)slot * a.niter;
+  const int niter = a.niter;
+  if (niter == 0) return;
+
+  unsigned long long dcur = myc[lane], dnxt = NO_CELL;
+  int kcur = myk[lane], knxt = 0;
+  if (niter > 64) {
+    dnxt = myc[64 + lane];
+    knxt = myk[64 + lane];
+  }
+  St13 vn;
+  int cwn = 0;
+  {
+    const unsigned long long d0 = rdl64(dcur, 0);
+    if (d0 != NO_CELL) st13_load(vn, cwn, tinb, co, d0, colb, jb, lb, rowb, nz, lane);
+  }
+  for (int j = 0; j < niter; j++) {
+    const int jl = j & 63;
+    const unsigned long long d = rdl64(dcur, jl);
+    const int kf = __builtin_amdgcn_readlane(kcur, jl);
+    const St13 v = vn;
+    const int cw = cwn;
+    /* the next cell's loads */
+    const unsigned long long dn = (jl < 63) ? rdl64(dcur, jl + 1) : rdl64(dnxt, 0);
+    if (j + 1 < niter && dn != NO_CELL)
+      st13_load(vn, cwn, tinb, co, dn, colb, jb, lb, rowb, nz, lane);
+    if (jl == 63) {
+      dcur = dnxt;
+      kcur = knxt;
+      if (j + 65 < niter) {
+        dnxt = myc[j + 65 + lane];
+        knxt = myk[j + 65 + lane];
+      }
+    }
+    if (d == NO_CELL) continue;
+    const int k = (int)((d >> 32) & 255);
+    const int ks = (kf & 255) - 1, flags = kf >> 8;
+    if (k > ks) continue; /* above the processed surface, or not processed */
+    const ll col = (ll)(uint32_t)d;
+    const int m = (int)((d >> 40) & 255);
+    const uint32_t o = lb + k * rowb;
+    char *oc = (char *)tout + col * colb;
+    const float v0 = v.v0;
+    const float qB = rdl(cw, C_QB);
+    const float FB = vlf(v.zm2, v.zm1, v0, v.zp1, qB, rdl(cw, C_CB));
+    const float fba = (FB - v0) * qB;
+    HVals h;
+    h.w1 = v.w1;
+    h.w2 = v.w2;
+    h.e1 = v.e1;
+    h.e2 = v.e2;
+    h.s1 = v.s1;
+    h.s2 = v.s2;
+    h.n1 = v.n1;
+    h.n2 = v.n2;
+    const float hh = hflux(h, (m & M_E1) != 0, (m & M_N1) != 0, v0, v0, rdl(cw, C_QW),
+                           rdl(cw, C_CW), rdl(cw, C_QE), rdl(cw, C_CE), rdl(cw, C_QS),
+                           rdl(cw, C_CS), rdl(cw, C_QN), rdl(cw, C_CN));
+    const uint32_t key = srkey((uint32_t)(col * nz) + (uint32_t)k, hl);
+    if (k < ks) {
+      /* sub-surface update (tracers.c:912-925) in anomaly form; the top
+       * face is the bottom face of the cell above */
+      const float qT = rdl(cw, C_N + C_QB);
+      const float FT = vlf(v.zm1, v0, v.zp1, v.zp2, qT, rdl(cw, C_N + C_CB));
+      const float ht = hh + (FT - v0) * qT - fba;
+      const double an = (double)v0 - (double)((ht + v0 * rdl(cw, C_D)) * rdl(cw, C_RV));
+      float bk = sround(an, key);
+      /* the thin-layer repair of tr_bounds needs the unclipped value */
+      if (k == ks - 1 && (flags & F_THIN) && act) stb((char *)scr + col * colb, o, bk);
+      if (clip && bk < mn) bk = mn;
+      if (act) stb(oc, o, bk);
+    } else {
+      /* processed surface cell: surf_conc (tracers.c:3047-3207) as an
+       * FP64 volume balance, v0 * Vc + anomaly terms, Vc in FP64 */
+      const ColInfo ci = cinfo[col];
+      const char *tc = tinb + col * colb;
+      double m2 = (double)fba - (double)hh;
+      if (flags & F_MOVE) {
+        const int kb = ci.kb;
+        int cc = ks;
+        double wb = (ks == kb) ? ci.botz : layers[ks];
+        while (cc != ci.kold && cc < top) {
+          const double vol = ci.area * (layers[cc + 1] - wb);
+          if (cc > ks) m2 += ((double)ldb(tc, lb + cc * rowb) - (double)v0) * vol;
+          cc++;
+          wb = layers[cc];
+        }
+        if (cc > ks)
+          m2 += ((double)ldb(tc, lb + cc * rowb) - (double)v0) *
+                (ci.area * (ci.osub - ((cc == kb) ? ci.botz : layers[cc])));
+        Nbr nb;
+        nb.c = tc;
+        nb.w1 = tc - colb;
+        nb.w2 = tc - 2 * colb;
+        nb.e1 = tc + colb;
+        nb.e2 = tc + 2 * colb;
+        nb.s1 = tc - jb;
+        nb.s2 = tc - 2 * jb;
+        nb.n1 = tc + jb;
+        nb.n2 = tc + 2 * jb;
+        nb.kw1 = ci.kw1;
+        nb.kw2 = ci.kw2;
+        nb.ke1 = ci.ke1;
+        nb.ke2 = ci.ke2;
+        nb.ks1 = ci.ks1;
+        nb.ks2 = ci.ks2;
+        nb.kn1 = ci.kn1;
+        nb.kn2 = ci.kn2;
+        const float *cob = co + col * nz * C_N;
+        for (int ka = ks + 1; ka <= top; ka++) {
+          HVals ha;
+          load_h(ha, nb, ka, lb + ka * rowb);
+          const float *ca = cob + ka * C_N;
+          m2 -= (double)hflux(ha, ka >= ci.ke1, ka >= ci.kn1, ldb(tc, lb + ka * rowb), v0,
+                              ca[C_QW], ca[C_CW], ca[C_QE], ca[C_CE], ca[C_QS], ca[C_CS],
+                              ca[C_QN], ca[C_CN]);
+        }
+      }
+      double surftr = (double)v0 * ci.vc + m2;
+      if (ci.svol == 0.0 || (surftr < 0.0 && ci.dzs <= a.hmin))
+        surftr = 0.0;
+      else
+        surftr /= ci.svol;
+      /* tracers.c:953-954; tr_bounds is applied in kernel B */
+      const float trs = (flags & F_KEEPOLD) ? v0 : sround(surftr, key);
+      if (act) stb(oc, o, trs);
+    }
+  }
+}
+
+/* per list entry: the processed surface (+1) and flags of its column */
+__global__ void k_kflB(const unsigned long long *__restrict__ cells, ll n,
+                       const ColInfo *__restrict__ cinfo, int *__restrict__ kfl) {
+  const ll i = (ll)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long d = cells[i];
+  int v = 0;
+  if (d != NO_CELL) {
+    const ColInfo *c = cinfo + (uint32_t)d;
+    v = (c->ks + 1) | (c->flags << 8);
+  }
+  kfl[i] = v;
+}
+
+/*-------------------------------------------------------------------*/
+/* kernel B: tr_bounds and the vertical solve, one wave per column     */
+/*-------------------------------------------------------------------*/
+/*
+ * Lanes = tracers.  Applies tr_bounds at the processed surface (thin old
+ * surface layer, clip; tracers.c:3217-3324), fills the layers above with
+ * the surface value, then solves (A/dzdt) d = b - (A/dzdt) b over the
+ * diffusion extent (tracers.c:3782-3957, 4067-4214) with the LU factors
+ * from the prep and writes x = b + d.  The advected values b are read
+ * eight layers at a time; s_k and ud_k live in the wave's LDS slice.
+ */
+
+template <int SCH>
+__global__ void __launch_bounds__(64 * WPB)
+k_vertB(const KArgs a, const float *__restrict__ tin, float *__restrict__ tout,
+        const float *__restrict__ co, const ColInfo *__restrict__ cinfo,
+        const int *__restrict__ order, const float *__restrict__ mintr,
+        const float *__restrict__ maxtr, const double *__restrict__ Kz,
+        const double *__restrict__ dzf, float *__restrict__ scr,
+        const int *__restrict__ stepctr) {
+  extern __shared__ float lds[];
+  const int nz = a.nz, NT = a.ntr, top = nz - 1, nact = a.nact;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int li = min(lane, nact - 1);
   float *Ls = lds + wv * nz * (nact + 1);
   float *Lud = Ls + nz * nact;
   const int ln = a.g * 64 + li;
   const bool act = lane < nact;
   const uint32_t lb = (uint32_t)ln * 4u;
-  const uint32_t rowb = (uint32_t)NT * 4u; /* bytes per layer of a column */
-  const ll colb = (ll)nz * NT * 4;         /* bytes per column */
-  const ll jb = a.nx * colb;               /* bytes per row of columns */
+  const uint32_t rowb = (uint32_t)NT * 4u;
+  const ll colb = (ll)nz * NT * 4;
   const float mn = mintr[ln];
   const float mx = maxtr[ln];
   const bool clip = mn <= 1e-5f;
-  const uint32_t hl = mix32((uint32_t)ln * 0x85EBCA77U ^ (uint32_t)stepctr[0] * 0xC2B2AE3DU);
+  const uint32_t hl = mix32((uint32_t)ln * 0x85EBCA77U ^ (uint32_t)stepctr[0] * 0xC2B2AE3DU) ^
+                      0x5bd1e995U;
   const int nslot = gridDim.x * WPB;
 
   for (int q = blockIdx.x * WPB + wv; q < a.nord; q += nslot) {
@@ -440,203 +683,81 @@ k_walk(const KArgs a, const float *__restrict__ tin, float *__restrict__ tout,
     const ll col = colq;
     const ColInfo ci = cinfo[col];
     const int kb = ci.kb, ks = ci.ks;
-    Nbr nb;
-    nb.c = (const char *)tin + col * colb;
-    nb.w1 = nb.c - colb;
-    nb.w2 = nb.c - 2 * colb;
-    nb.e1 = nb.c + colb;
-    nb.e2 = nb.c + 2 * colb;
-    nb.s1 = nb.c - jb;
-    nb.s2 = nb.c - 2 * jb;
-    nb.n1 = nb.c + jb;
-    nb.n2 = nb.c + 2 * jb;
-    nb.kw1 = ci.kw1;
-    nb.kw2 = ci.kw2;
-    nb.ke1 = ci.ke1;
-    nb.ke2 = ci.ke2;
-    nb.ks1 = ci.ks1;
-    nb.ks2 = ci.ks2;
-    nb.kn1 = ci.kn1;
-    nb.kn2 = ci.kn2;
     char *oc = (char *)tout + col * colb;
-    const char *tc = nb.c;
-    const float *cob = co + col * nz * C_N;
-    const uint32_t hcol = (uint32_t)(col * nz);
+    const char *tc = (const char *)tin + col * colb;
     if (!(ci.flags & F_PROC)) {
       for (int k = kb; k <= top; k++)
         if (act) stb(oc, lb + k * rowb, ldb(tc, lb + k * rowb));
       continue;
     }
-    const bool solve = !(ci.flags & F_SINGLE);
-    const int cb = solve ? ci.cb : nz, cs = solve ? ci.cs : -1;
-
-    /* coefficient chunks: layers kc..kc+3 and kc+4..kc+7 */
-    int kc = kb;
-    int ch0 = __float_as_int(cob[kc * C_N + lane]);
-    int ch1 = __float_as_int(cob[(kc + 4) * C_N + lane]);
-    /* vertical window t[k-2 .. k+2], clamped to [kb, top] */
-    float zm2, zm1, v0, zp1, zp2;
-    v0 = ldb(tc, lb + kb * rowb);
-    zm1 = zm2 = v0;
-    zp1 = ldb(tc, lb + min(kb + 1, top) * rowb);
-    zp2 = ldb(tc, lb + min(kb + 2, top) * rowb);
-    HVals h;
-    load_h(h, nb, kb, lb + kb * rowb);
-    /* bottom-face flux of the bed cell */
-    float FB = vlf(zm2, zm1, v0, zp1, rdl(ch0, C_QB), rdl(ch0, C_CB));
-    /* forward-sweep state: b_{k-2}, b_{k-1}, s_{k-2}, coefficients of k-1 */
-    float bm2 = 0.0f, bm1 = 0.0f, sprev = 0.0f;
-    float gm1 = 0.0f, gp1 = 0.0f, rd1 = 1.0f;
-    float pre = 0.0f; /* unclipped update of the layer below the surface */
-    float trs = 0.0f;
-
-    for (int k = kb; k <= ks; k++) {
-      const int r = 16 * (k - kc); /* word of layer k in ch0 */
-      /* prefetch the next layer's horizontal neighbours and t[k+3] */
-      HVals hn;
-      float zp3 = zp2;
-      if (k < ks) {
-        load_h(hn, nb, k + 1, lb + (k + 1) * rowb);
-        zp3 = ldb(tc, lb + min(k + 3, top) * rowb);
-      }
-      const float qW = rdl(ch0, r + C_QW), fW = rdl(ch0, r + C_CW);
-      const float qE = rdl(ch0, r + C_QE), fE = rdl(ch0, r + C_CE);
-      const float qS = rdl(ch0, r + C_QS), fS = rdl(ch0, r + C_CS);
-      const float qN = rdl(ch0, r + C_QN), fN = rdl(ch0, r + C_CN);
-      const float qB = rdl(ch0, r + C_QB);
-      const float gmk = rdl(ch0, r + C_GM), gpk = rdl(ch0, r + C_GP);
-      const float rdk = rdl(ch0, r + C_RD), udk = rdl(ch0, r + C_UD);
-      const float fba = (FB - v0) * qB;
-      const float hh = hflux(h, k >= ci.ke1, k >= ci.kn1, v0, v0, qW, fW, qE, fE, qS, fS,
-                             qN, fN);
-      const uint32_t key = srkey(hcol + (uint32_t)k, hl);
-      float bk;
-      if (k < ks) {
-        /* sub-surface update (tracers.c:912-925) in anomaly form; the top
-         * face is the bottom face of the cell above */
-        const int nsrc = (r < 48) ? ch0 : ch1;
-        const int rn = (r + 16) & 63;
-        const float qT = rdl(nsrc, rn + C_QB);
-        const float FT = vlf(zm1, v0, zp1, zp2, qT, rdl(nsrc, rn + C_CB));
-        const float ht = hh + (FT - v0) * qT - fba;
-        const double an =
-            (double)v0 - (double)((ht + v0 * rdl(ch0, r + C_D)) * rdl(ch0, r + C_RV));
-        bk = sround(an, key);
-        pre = bk;
-        if (clip && bk < mn) bk = mn;
-        FB = FT;
-      } else {
-        /* processed surface cell: surf_conc (tracers.c:3047-3207) as an
-         * FP64 volume balance, v0 * Vc + anomaly terms, Vc in FP64 */
-        double m = (double)fba - (double)hh;
-        if (ci.flags & F_MOVE) {
-          int cc = ks;
-          double wb = (ks == kb) ? ci.botz : layers[ks];
-          while (cc != ci.kold && cc < top) {
-            const double vol = ci.area * (layers[cc + 1] - wb);
-            if (cc > ks) m += ((double)ldb(tc, lb + cc * rowb) - (double)v0) * vol;
-            cc++;
-            wb = layers[cc];
-          }
-          if (cc > ks)
-            m += ((double)ldb(tc, lb + cc * rowb) - (double)v0) *
-                 (ci.area * (ci.osub - ((cc == kb) ? ci.botz : layers[cc])));
-          for (int ka = ks + 1; ka <= top; ka++) {
-            HVals ha;
-            load_h(ha, nb, ka, lb + ka * rowb);
-            const float *ca = cob + ka * C_N;
-            m -= (double)hflux(ha, ka >= ci.ke1, ka >= ci.kn1, ldb(tc, lb + ka * rowb), v0,
-                               ca[C_QW], ca[C_CW], ca[C_QE], ca[C_CE], ca[C_QS], ca[C_CS],
-                               ca[C_QN], ca[C_CN]);
-          }
-        }
-        double surftr = (double)v0 * ci.vc + m;
-        if (ci.svol == 0.0 || (surftr < 0.0 && ci.dzs <= a.hmin))
-          surftr = 0.0;
-        else
-          surftr /= ci.svol;
-        /* tracers.c:953-954 */
-        trs = (ci.flags & F_KEEPOLD) ? v0 : sround(surftr, key);
-        /* tr_bounds (tracers.c:3217-3324): thin old surface layer, clip */
-        if ((ci.flags & F_THIN) && (trs < mn || trs > mx)) trs = pre;
-        if (clip && trs < mn) trs = mn;
-        bk = trs;
-      }
-      if (act) stb(oc, lb + k * rowb, bk);
-      Lud[k] = udk;
-      /* forward sweep for layer k-1, now that b_k is known:
-       * s = (r - gm s_prev) / div, r = -(gm (b_{k-2} - b_{k-1}) +
-       * gp (b_k - b_{k-1})) */
-      {
-        const int kf = k - 1;
-        if (kf >= cb && kf <= cs) {
-          const float gm = (kf > cb) ? gm1 : 0.0f;
-          const float rr = -(gm * (bm2 - bm1) + gp1 * (bk - bm1));
-          sprev = (rr - gm * sprev) * rd1;
-          Ls[kf * nact + li] = sprev;
-        }
-      }
-      gm1 = gmk;
-      gp1 = gpk;
-      rd1 = rdk;
-      bm2 = bm1;
-      bm1 = bk;
-      /* slide */
-      zm2 = zm1;
-      zm1 = v0;
-      v0 = zp1;
-      zp1 = zp2;
-      zp2 = zp3;
-      if (k < ks) h = hn;
-      if (r == 48) {
-        /* next chunk */
-        kc += 4;
-        ch0 = ch1;
-        ch1 = __float_as_int(cob[(kc + 4) * C_N + lane]);
-      }
-    }
-    /* the surface value fills every layer above (tracers.c:3306-3322) and
-     * the forward sweep runs on to the new surface */
-    for (int k = ks + 1; k <= top; k++) {
+    /* tr_bounds at the surface */
+    float trs = ldb(oc, lb + ks * rowb);
+    if ((ci.flags & F_THIN) && (trs < mn || trs > mx))
+      trs = ldb((const char *)scr + col * colb, lb + (ks - 1) * rowb);
+    if (clip && trs < mn) trs = mn;
+    for (int k = ks; k <= top; k++)
       if (act) stb(oc, lb + k * rowb, trs);
-      const float *cf = cob + k * C_N;
-      const float gmk = cf[C_GM], gpk = cf[C_GP], rdk = cf[C_RD];
-      Lud[k] = cf[C_UD];
-      const int kf = k - 1;
-      if (kf >= cb && kf <= cs) {
-        const float gm = (kf > cb) ? gm1 : 0.0f;
-        const float rr = -(gm * (bm2 - bm1) + gp1 * (trs - bm1));
-        sprev = (rr - gm * sprev) * rd1;
-        Ls[kf * nact + li] = sprev;
+    if (ci.flags & F_SINGLE) continue;
+    const int cb = ci.cb, cs = ci.cs;
+    const float *cob = co + col * nz * C_N;
+    const uint32_t hcol = (uint32_t)(col * nz);
+    /* forward sweep: s_k = (r_k - gm_k s_{k-1}) / div_k,
+     * r_k = -(gm (b_{k-1} - b_k) + gp (b_{k+1} - b_k)) */
+    float sp = 0.0f, bkm = 0.0f, bk = 0.0f;
+    for (int k0 = cb; k0 <= cs; k0 += 8) {
+      /* coefficient words of layers k0..k0+3 and k0+4..k0+7, lane l holding
+       * word l, issued with the b loads of the chunk */
+      const int cA = __float_as_int(cob[k0 * C_N + lane]);
+      const int cB = __float_as_int(cob[(k0 + 4) * C_N + lane]);
+      float bb[9];
+#pragma unroll
+      for (int j = 0; j < 9; j++) {
+        const int k = min(k0 + j, cs);
+        bb[j] = (k < ks) ? ldb(oc, lb + k * rowb) : trs;
       }
-      gm1 = gmk;
-      gp1 = gpk;
-      rd1 = rdk;
-      bm2 = bm1;
-      bm1 = trs;
+      if (k0 == cb) {
+        bk = bb[0];
+        bkm = bk;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int k = k0 + j;
+        if (k <= cs) {
+          const int cv = (j < 4) ? cA : cB;
+          const int w = 16 * (j & 3);
+          const float gm = (k > cb) ? rdl(cv, w + C_GM) : 0.0f, gp = rdl(cv, w + C_GP);
+          const float bkp = (k < cs) ? bb[j + 1] : bk;
+          const float r = -(gm * (bkm - bk) + gp * (bkp - bk));
+          sp = (r - gm * sp) * rdl(cv, w + C_RD);
+          Ls[k * nact + li] = sp;
+          Lud[k] = rdl(cv, w + C_UD);
+          bkm = bk;
+          bk = bkp;
+        }
+      }
     }
-    if (cs == top) {
-      /* last row: b_{k+1} does not enter (gp = 0) */
-      const float gm = (cs > cb) ? gm1 : 0.0f;
-      const float rr = -(gm * (bm2 - bm1));
-      sprev = (rr - gm * sprev) * rd1;
-      Ls[cs * nact + li] = sprev;
-    }
-    if (!solve) continue;
-    /* back substitution: d_k = s_k - ud_{k+1} d_{k+1}, x_k = b_k + d_k */
-    const bool stiff = (ci.flags & F_STIFF) != 0;
+    /* back substitution: d_k = s_k - ud_{k+1} d_{k+1} */
     float dn = 0.0f;
-    float bnext = ldb(oc, lb + cs * rowb);
-    for (int k = cs; k >= cb; k--) {
-      const float bv = bnext;
-      if (k > cb) bnext = ldb(oc, lb + (k - 1) * rowb);
-      const float sk = Ls[k * nact + li];
-      dn = (k == cs) ? sk : sk - Lud[k + 1] * dn;
-      if (stiff) {
-        Ls[k * nact + li] = dn;
-      } else if (act) {
-        stb(oc, lb + k * rowb,
-            sround((double)bv + (double)dn, srkey(hcol + (uint32_t)k, hl ^ 0x5bd1e995U)));
+    const bool stiff = (ci.flags & F_STIFF) != 0;
+    for (int k0 = cs; k0 >= cb; k0 -= 8) {
+      float bb[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int k = max(k0 - j, cb);
+        bb[j] = (k < ks) ? ldb(oc, lb + k * rowb) : trs;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int k = k0 - j;
+        if (k >= cb) {
+          const float sk = Ls[k * nact + li];
+          dn = (k == cs) ? sk : sk - Lud[k + 1] * dn;
+          if (stiff)
+            Ls[k * nact + li] = dn;
+          else if (act)
+            stb(oc, lb + k * rowb,
+                sround((double)bb[j] + (double)dn, srkey(hcol + (uint32_t)k, hl)));
+        }
       }
     }
     if (stiff) {
@@ -652,10 +773,10 @@ k_walk(const KArgs a, const float *__restrict__ tin, float *__restrict__ tout,
         double cm1 = 0.0, cp1 = 0.0;
         if (k > cb) cm1 = -Kz[ix] / (0.5 * (dzf[ix - P] + dzc));
         if (k < cs) cp1 = -Kz[ix + P] / (0.5 * (dzc + dzf[ix + P]));
-        const double bv = (double)ldb(oc, lb + k * rowb);
+        const double bv = (double)((k < ks) ? ldb(oc, lb + k * rowb) : trs);
         const double dk = (double)Ls[k * nact + li];
-        const double bvm = (k > cb) ? (double)ldb(oc, lb + (k - 1) * rowb) : bv;
-        const double bvp = (k < cs) ? (double)ldb(oc, lb + (k + 1) * rowb) : bv;
+        const double bvm = (k > cb) ? (double)((k - 1 < ks) ? ldb(oc, lb + (k - 1) * rowb) : trs) : bv;
+        const double bvp = (k < cs) ? (double)((k + 1 < ks) ? ldb(oc, lb + (k + 1) * rowb) : trs) : bv;
         const double dkm = (k > cb) ? (double)Ls[(k - 1) * nact + li] : 0.0;
         const double dkp = (k < cs) ? (double)Ls[(k + 1) * nact + li] : 0.0;
         const double rhs = -(cm1 * (bvm - bv) + cp1 * (bvp - bv));
@@ -672,15 +793,14 @@ k_walk(const KArgs a, const float *__restrict__ tin, float *__restrict__ tout,
         ec = (k == cs) ? ek : ek - Lud[k + 1] * ec;
         stb(sc, lb + k * rowb, ec);
       }
-      /* b_k is still needed below k while the residual is formed: write
-       * the results after the pass */
+      /* every residual is formed before any b_k is replaced */
       for (int k = cb; k <= cs; k++) {
-        const float bv = ldb(oc, lb + k * rowb);
+        const float bv = (k < ks) ? ldb(oc, lb + k * rowb) : trs;
         const float ek = ldb(sc, lb + k * rowb);
         if (act)
           stb(oc, lb + k * rowb,
               sround((double)bv + (double)Ls[k * nact + li] + (double)ek,
-                     srkey(hcol + (uint32_t)k, hl ^ 0x5bd1e995U)));
+                     srkey(hcol + (uint32_t)k, hl)));
       }
     }
   }
@@ -763,9 +883,73 @@ __global__ void k_massB(emstr_dev d, emstr_batch b, const float *__restrict__ tr
 /*-------------------------------------------------------------------*/
 /* host interface                                                    */
 /*-------------------------------------------------------------------*/
-/* dynamic LDS of the fused kernel: one [nz][nact + 1] slice per wave */
-static size_t walk_lds(int nz, int nact) {
+/* dynamic LDS of kernel B: one [nz][nact + 1] slice per wave */
+static size_t vert_lds(int nz, int nact) {
   return (size_t)WPB * nz * (nact + 1) * sizeof(float);
+}
+
+/* Cell lists of kernel A.  Cells are ordered tile by tile (tile_x x
+ * tile_y columns, row-major tiles), inside a tile layer by layer from the
+ * deepest, inside a layer row-major; the sequence is split into 8
+ * contiguous bands, one per XCD (workgroups are dealt to the XCDs round
+ * robin).  The S wave slots of XCD x take band x in turns: cell c of the
+ * band is iteration c / S of the slot of rank c % S, so at any time the
+ * slots of one XCD work on one stretch of one tile layer. */
+static int build_cells(emstr_batch *b, const int *kb, int nz) {
+  const int nx = b->nx, ny = b->ny, tx = b->tile_x, ty = b->tile_y;
+  ll n = 0;
+  for (ll c = 0; c < b->plane; c++)
+    if (kb[c] < nz) n += nz - kb[c];
+  unsigned long long *seq = (unsigned long long *)malloc(sizeof(unsigned long long) * (size_t)(n + 1));
+  ll q = 0;
+  for (int J = 0; J < ny; J += ty)
+    for (int I = 0; I < nx; I += tx)
+      for (int k = 0; k < nz; k++)
+        for (int j = J; j < J + ty && j < ny; j++)
+          for (int i = I; i < I + tx && i < nx; i++) {
+            const ll col = (ll)j * nx + i;
+            if (kb[col] > k) continue;
+            const bool w1 = kb[col - 1] <= k, e1 = kb[col + 1] <= k;
+            const bool s1 = kb[col - nx] <= k, n1 = kb[col + nx] <= k;
+            const bool w2 = w1 && kb[col - 2] <= k, e2 = e1 && kb[col + 2] <= k;
+            const bool s2 = s1 && kb[col - 2 * nx] <= k, n2 = n1 && kb[col + 2 * nx] <= k;
+            const unsigned m = (w1 ? M_W1 : 0) | (w2 ? M_W2 : 0) | (e1 ? M_E1 : 0) |
+                               (e2 ? M_E2 : 0) | (s1 ? M_S1 : 0) | (s2 ? M_S2 : 0) |
+                               (n1 ? M_N1 : 0) | (n2 ? M_N2 : 0);
+            seq[q++] = (unsigned long long)(uint32_t)col | ((unsigned long long)k << 32) |
+                       ((unsigned long long)m << 40) | ((unsigned long long)kb[col] << 48);
+          }
+  const ll nslot = (ll)b->gridA * 4, S = nslot / 8;
+  const ll P = (n + 7) / 8;
+  ll niter = (P + S - 1) / S;
+  niter = (niter + 63) / 64 * 64;
+  if (niter == 0) niter = 64;
+  const ll total = nslot * niter;
+  unsigned long long *lst = (unsigned long long *)malloc(sizeof(unsigned long long) * (size_t)total);
+  for (ll t = 0; t < total; t++) lst[t] = NO_CELL;
+  for (int x = 0; x < 8; x++)
+    for (ll c = 0; c < P; c++) {
+      const ll src = x * P + c;
+      if (src >= n) break;
+      const ll r = c % S, it = c / S;
+      const ll slot = ((r / 4) * 8 + x) * 4 + (r % 4);
+      lst[slot * niter + it] = seq[src];
+    }
+  if (b->cells) (void)hipFree(b->cells);
+  if (b->kfl) (void)hipFree(b->kfl);
+  b->cells = NULL;
+  b->kfl = NULL;
+  bool ok = hipMalloc(&b->cells, sizeof(unsigned long long) * (size_t)total) == hipSuccess;
+  ok &= hipMalloc(&b->kfl, sizeof(int) * (size_t)total) == hipSuccess;
+  if (ok)
+    ok = hipMemcpy(b->cells, lst, sizeof(unsigned long long) * (size_t)total,
+                   hipMemcpyHostToDevice) == hipSuccess;
+  free(lst);
+  free(seq);
+  if (!ok) return emstr_set_error(EMSTR_EDEVICE, -1, "batched form: cell lists");
+  b->niter = (int)niter;
+  b->nlist = total;
+  return 0;
 }
 
 /* Column order: tiles of tile_x x tile_y columns in row-major tile
@@ -815,8 +999,8 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   b->nx = d->nx;
   b->ny = d->ny;
   b->plane = d->plane;
-  b->tile_x = 64;
-  b->tile_y = 8;
+  b->tile_x = 32;
+  b->tile_y = 32;
   const size_t ncell = (size_t)d->plane * d->nz;
   bool ok = true;
   for (int q = 0; q < 2; q++)
@@ -836,21 +1020,31 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   }
   hipMemset(b->co, 0, (ncell + 8) * C_N * sizeof(float));
   hipMemset(b->stepctr, 0, sizeof(int));
-  build_order(b, h_kb, d->nz);
   int dev = 0, ncu = 256;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  /* persistent grid: as many workgroups as can be resident at once */
+  /* persistent grids: as many workgroups as can be resident at once */
   int per_cu = 0;
-  const size_t lds = walk_lds(b->nz, b->ntr < 64 ? b->ntr : 64);
+  const size_t lds = vert_lds(b->nz, b->ntr < 64 ? b->ntr : 64);
   if (lds > 48 * 1024)
-    (void)hipFuncSetAttribute((const void *)k_walk<VANLEER>,
+    (void)hipFuncSetAttribute((const void *)k_vertB<VANLEER>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_walk<VANLEER>,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_vertB<VANLEER>,
                                                    64 * WPB, lds) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   b->grid = ncu * per_cu;
+  per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_advB<VANLEER>, 256,
+                                                   0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  b->gridA = (ncu * per_cu + 7) / 8 * 8;
+  build_order(b, h_kb, d->nz);
+  if (build_cells(b, h_kb, d->nz)) {
+    emstr_batch_destroy(b);
+    return NULL;
+  }
   return b;
 }
 
@@ -858,9 +1052,9 @@ extern "C" void emstr_batch_destroy(struct emstr_batch *b) {
   if (!b) return;
   if (b->gexec) hipGraphExecDestroy(b->gexec);
   if (b->graph) hipGraphDestroy(b->graph);
-  void *v[] = {b->tr[0], b->tr[1], b->scr, b->co, b->ci, b->order, b->mintr, b->maxtr,
-               b->stepctr};
-  for (int q = 0; q < 9; q++)
+  void *v[] = {b->tr[0], b->tr[1], b->scr,     b->co,    b->ci,  b->order,
+               b->mintr, b->maxtr, b->stepctr, b->cells, b->kfl};
+  for (int q = 0; q < 11; q++)
     if (v[q]) hipFree(v[q]);
   free(b);
 }
@@ -871,8 +1065,9 @@ extern "C" int emstr_batch_tune(struct emstr_batch *b, const int *h_kb, int tile
                                 int grid) {
   if (tile_x > 0) b->tile_x = tile_x;
   if (tile_y > 0) b->tile_y = tile_y;
-  if (grid > 0) b->grid = grid;
+  if (grid > 0) b->gridA = (grid + 7) / 8 * 8;
   build_order(b, h_kb, b->nz);
+  if (build_cells(b, h_kb, b->nz)) return -1;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec), b->gexec = NULL;
   if (b->graph) (void)hipGraphDestroy(b->graph), b->graph = NULL;
   b->graph_ok = 0;
@@ -909,10 +1104,12 @@ static int batch_launch(emstr_batch *b, const emstr_dev *d, hipStream_t st) {
   int pi = prof_begin(b->prof, KC_PREP, st);
   k_prepB<<<gp, bp, 0, st>>>(*d, *b);
   k_colB<<<dim3((d->nx + 63) / 64, (d->ny + 3) / 4, 1), bp, 0, st>>>(*d, *b);
-  prof_end(b->prof, pi, st);
   const float *in = b->tr[b->cur];
   float *out = b->tr[1 - b->cur];
-  pi = prof_begin(b->prof, KC_FUSED, st);
+  if (b->nlist > 0)
+    k_kflB<<<(unsigned)((b->nlist + 255) / 256), 256, 0, st>>>(b->cells, b->nlist, b->ci,
+                                                                 b->kfl);
+  prof_end(b->prof, pi, st);
   KArgs ka;
   ka.nz = b->nz;
   ka.ntr = b->ntr;
@@ -922,14 +1119,26 @@ static int batch_launch(emstr_batch *b, const emstr_dev *d, hipStream_t st) {
   ka.dt = d->dt;
   ka.plane = d->plane;
   ka.pad = 0;
+  AArgs aa;
+  aa.nz = b->nz;
+  aa.ntr = b->ntr;
+  aa.niter = b->niter;
+  aa.pad = 0;
+  aa.nx = b->nx;
+  aa.hmin = d->hmin;
   for (int g = 0; g < b->nt64; g++) {
-    ka.g = g;
-    ka.nact = b->ntr - 64 * g < 64 ? b->ntr - 64 * g : 64;
-    k_walk<VANLEER><<<b->grid, 64 * WPB, walk_lds(b->nz, ka.nact), st>>>(
-        ka, in, out, b->co, b->ci, b->order, b->mintr, b->maxtr, d->layers, d->Kz, d->dz,
-        b->scr, b->stepctr);
+    ka.g = aa.g = g;
+    ka.nact = aa.nact = b->ntr - 64 * g < 64 ? b->ntr - 64 * g : 64;
+    pi = prof_begin(b->prof, KC_ADV, st);
+    k_advB<VANLEER><<<b->gridA, 256, 0, st>>>(aa, in, out, b->co, b->cells, b->kfl, b->ci,
+                                              b->mintr, d->layers, b->scr, b->stepctr);
+    prof_end(b->prof, pi, st);
+    pi = prof_begin(b->prof, KC_VDIFF, st);
+    k_vertB<VANLEER><<<b->grid, 64 * WPB, vert_lds(b->nz, ka.nact), st>>>(
+        ka, in, out, b->co, b->ci, b->order, b->mintr, b->maxtr, d->Kz, d->dz, b->scr,
+        b->stepctr);
+    prof_end(b->prof, pi, st);
   }
-  prof_end(b->prof, pi, st);
   k_tick<<<1, 1, 0, st>>>(b->stepctr);
   b->cur = 1 - b->cur;
   return hipGetLastError() == hipSuccess ? 0 : -1;
