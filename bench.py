@@ -84,6 +84,20 @@ def kernel_times(ctx, steps):
     return list(ms), list(cnt)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of a kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
+    calibration recorded there); None if the kernel is not listed."""
+    path = os.path.join(ROOT, "profiles", "advB_c3_hbm_traffic.md")
+    if kernel is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic" and f[1] == kernel:
+            return int(f[2])
+    return None
+
+
 def copy_probe(mb):
     """A plain device-to-device copy of mb MiB, five times: its
     FETCH_SIZE / WRITE_SIZE per dispatch against the known 2 x mb MiB
@@ -114,6 +128,7 @@ def main():
     ap.add_argument("--ntr", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--forms", default="plain,precision,batched")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0)
@@ -190,9 +205,11 @@ def main():
                2: nwet * (7 * 8 + 16 * 4)}.get(dom, nwet * ntr * 8)
         avg_ms = ktB[dom] / max(kcB[dom], 1)
         achieved = alg / (avg_ms * 1e-3) / 1e9
+        kname = {0: "k_advB", 1: "k_vertB"}.get(dom)
         out["roofline"] = {"bound": "hbm", "kernel": "batched " + names[dom],
                            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "frac": achieved / HBM_PEAK_GBS,
+                           "traffic": measured_traffic(kname),
                            "alg_bytes_per_launch": alg, "avg_launch_ms": avg_ms}
         out["step_hbm_fraction"] = (nwet * ntr * 8 + nwet * 56) / (rB["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     kms = {}
@@ -230,6 +247,23 @@ def main():
                            "worst_tracer": int(np.argmax(err))}
     for c in ctx.values():
         c.close()
+
+    if not args.no_dropin:
+        # the drop-in entry point on host buffers (fields uploaded, tracers
+        # copied in and out over PCIe every call, plain FP64 form): the
+        # PCIe-inclusive rate, reported beside the device-resident value
+        L.tracer_step_3d(case.gp, case.wp, case.pp)
+        t = time.time()
+        nd = 3
+        for _ in range(nd):
+            L.tracer_step_3d(case.gp, case.wp, case.pp)
+        msg, cell = _lib.last_error()
+        if msg:
+            raise RuntimeError("tracer_step_3d: %s" % msg)
+        dms = (time.time() - t) / nd * 1e3
+        L.emstr_forget_window(case.gp)
+        out["dropin_host_buffers"] = {"ms_per_call": dms, "value": cu / (dms * 1e-3),
+                                      "unit": "cell-updates/s", "form": "plain FP64"}
 
     if not args.no_cpu:
         # CPU baseline: the oracle (FP64 C restatement) with OpenMP over

@@ -188,8 +188,9 @@ typedef struct emstr_ctx emstr_ctx;
 /* Arithmetic forms */
 #define EMSTR_PLAIN_F64 0   /* reference arithmetic, FP64 throughout   */
 #define EMSTR_PREC_F32 1    /* FP32 stencils/sweeps, FP64 budgets/IR   */
-#define EMSTR_BATCH_F32 2   /* precision form, all tracers and columns
-                               fused in one persistent kernel, hipGraph */
+#define EMSTR_BATCH_F32 2   /* precision form, all tracers per cell in
+                               one lane-per-tracer pass: cell-parallel
+                               advection + per-column solve, hipGraph   */
 
 /* Error codes */
 #define EMSTR_OK 0
@@ -212,8 +213,10 @@ int emstr_download_tracers(emstr_ctx *ctx);
 int emstr_step(emstr_ctx *ctx);
 /* nsteps tracer steps (the batched form replays a captured hipGraph).  */
 int emstr_steps(emstr_ctx *ctx, int nsteps);
-/* Batched form launch shape: column tile (tile_x x tile_y), persistent
- * grid size, graph replay on/off; 0 (or -1 for use_graph) keeps a value. */
+/* Batched form launch shape: tile (tile_x x tile_y columns) of the cell
+ * and column orders, workgroups of the persistent advection grid (a
+ * multiple of 8), graph replay on/off; 0 (or -1 for use_graph) keeps a
+ * value. */
 int emstr_tune(emstr_ctx *ctx, int tile_x, int tile_y, int grid, int use_graph);
 /* Only the advection / only the vertical diffusion part.              */
 int emstr_advect_diffuse(emstr_ctx *ctx);
