@@ -160,3 +160,36 @@ def test_c3_batch_vs_plain_full_size():
     assert e.max() <= 1e-5, e
     assert np.all(np.abs(mb1 - mb0) <= 1e-6 * np.abs(mb0))
     assert np.all(np.abs(mp1 - mp0) <= 1e-12 * np.abs(mp0))
+
+
+def test_c3_batch_1000_steps_mass_drift():
+    """Full c3 over 1000 steps: relative mass drift of every tracer stays
+    within 1e-6 (FP64 compensated budgets of the FP32 state) and the
+    error against the FP64 plain form stays within 1e-5 at step 400."""
+    say("batch c3 1000 steps: drift and error")
+    from ems_amd import _lib
+    case = configs.c3()
+    L = _lib.lib()
+    L.set_dz(case.gp, case.wp, case.pp)
+    L.emstr_forget_window(case.gp)
+    p = T.Context(case, T.PLAIN_F64)
+    p.upload_fields()
+    p.vel_w()
+    p.upload()
+    b = T.Context(case, T.BATCH_F32)
+    b.upload()
+    mb0 = b.mass()
+    p.step(400)
+    b.step(400)
+    p.download()
+    ref = case.tracers()
+    b.download()
+    e400 = errors(case.tracers(), ref)
+    b.step(600)
+    mb1 = b.mass()
+    p.close()
+    b.close()
+    drift = np.abs(mb1 - mb0) / np.abs(mb0)
+    say("c3 error at 400 steps", e400.max(), "drift after 1000 steps", drift.max())
+    assert e400.max() <= 1e-5, e400
+    assert drift.max() <= 1e-6, drift
