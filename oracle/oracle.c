@@ -390,13 +390,348 @@ static void ultimate(const double *cr, double *Fx, const double *F,
 }
 
 /*-------------------------------------------------------------------*/
+/* Flux-form semi-Lagrangian scheme (FFSL), structured grid.          */
+/* Trajectory arrays: per sub-step, tracer independent (the reference */
+/* keeps them in win_priv_t; here they are the oracle's own).         */
+/*-------------------------------------------------------------------*/
+static int ff_n = 0;
+static double *ff_nu, *ff_nv, *ff_nw;
+static int *ff_clxf, *ff_clyf, *ff_clzf, *ff_clxc, *ff_clyc, *ff_clzc;
+static double *ff_crfxf, *ff_crfyf, *ff_crfzf, *ff_crfxc, *ff_crfyc, *ff_crfzc;
+
+static void ff_alloc(int n) {
+  if (ff_n == n) return;
+  if (ff_n) {
+    free(ff_nu); free(ff_nv); free(ff_nw);
+    free(ff_clxf); free(ff_clyf); free(ff_clzf);
+    free(ff_clxc); free(ff_clyc); free(ff_clzc);
+    free(ff_crfxf); free(ff_crfyf); free(ff_crfzf);
+    free(ff_crfxc); free(ff_crfyc); free(ff_crfzc);
+  }
+  ff_nu = calloc(n, sizeof(double)); ff_nv = calloc(n, sizeof(double));
+  ff_nw = calloc(n, sizeof(double));
+  ff_clxf = calloc(n, sizeof(int)); ff_clyf = calloc(n, sizeof(int));
+  ff_clzf = calloc(n, sizeof(int)); ff_clxc = calloc(n, sizeof(int));
+  ff_clyc = calloc(n, sizeof(int)); ff_clzc = calloc(n, sizeof(int));
+  ff_crfxf = calloc(n, sizeof(double)); ff_crfyf = calloc(n, sizeof(double));
+  ff_crfzf = calloc(n, sizeof(double)); ff_crfxc = calloc(n, sizeof(double));
+  ff_crfyc = calloc(n, sizeof(double)); ff_crfzc = calloc(n, sizeof(double));
+  ff_n = n;
+}
+
+/* the trajectory arrays of the last sub-step, for tests */
+void oracle_ffsl_maps(int **cl, double **crf) {
+  cl[0] = ff_clxf; cl[1] = ff_clyf; cl[2] = ff_clzf;
+  cl[3] = ff_clxc; cl[4] = ff_clyc; cl[5] = ff_clzc;
+  crf[0] = ff_crfxf; crf[1] = ff_crfyf; crf[2] = ff_crfzf;
+  crf[3] = ff_crfxc; crf[4] = ff_crfyc; crf[5] = ff_crfzc;
+}
+
+/* tracers.c:2771-2808  horizontal backward trajectory */
+static void ff_do_hor(const double *vel, double dt, const double *h, const int *m2d,
+                      int ss, int se, const int *sdo, const int *fmap, const int *bmap,
+                      int *cl, double *crf) {
+  int c, cc, c2, cp;
+  double ltraj, lrem, dx;
+  for (cc = ss; cc <= se; cc++) {
+    c = sdo[cc];
+    if (vel[c] == 0.0) continue;
+    ltraj = fabs(vel[c]) * dt;
+    lrem = ltraj;
+    cp = cl[c];
+    c2 = m2d[cp];
+    dx = h[c2];
+    while (dx < lrem) {
+      lrem -= dx;
+      cp = (vel[c] < 0.0) ? fmap[cp] : bmap[cp];
+      c2 = m2d[cp];
+      if (h[c2] > 0) dx = h[c2];
+    }
+    crf[c] = max(min(lrem / dx, 1.0), 0.0);
+    cl[c] = cp;
+  }
+}
+
+/* tracers.c:2820-2859  vertical backward trajectory */
+static void ff_do_vert(const double *vel, double dt, const double *h, int ss, int se,
+                       const int *sdo, const int *fmap, const int *bmap, int *cl,
+                       double *crf) {
+  int c, cc, cp, cpb;
+  double ltraj, lrem, dz;
+  for (cc = ss; cc <= se; cc++) {
+    c = sdo[cc];
+    if (vel[c] == 0.0) continue;
+    ltraj = fabs(vel[c]) * dt;
+    lrem = ltraj;
+    cp = cpb = cl[c];
+    dz = fabs((h[cp] > 0) ? h[cp] : h[c]);
+    while (dz < lrem) {
+      lrem -= dz;
+      cp = (vel[c] < 0.0) ? fmap[cp] : bmap[cp];
+      if (cpb == cp) {
+        lrem = fmod(lrem, dz);
+        break;
+      }
+      if (h[cp] > 0) dz = fabs(h[cp]);
+      cpb = cp;
+    }
+    crf[c] = max(min(lrem / dz, 1.0), 0.0);
+    cl[c] = cp;
+  }
+}
+
+/* tracers.c:2132-2251  trajectories of one sub-step */
+static void oracle_prep_ff_sl(geometry_t *g, window_t *wd, win_priv_t *wc, double dt) {
+  int c, cc;
+  const size_t nd = (size_t)g->sgsiz * sizeof(double), ni = (size_t)g->sgsiz * sizeof(int);
+  memset(ff_crfxf, 0, nd); memset(ff_crfyf, 0, nd); memset(ff_crfzf, 0, nd);
+  memset(ff_crfxc, 0, nd); memset(ff_crfyc, 0, nd); memset(ff_crfzc, 0, nd);
+  memset(ff_clxf, 0, ni); memset(ff_clyf, 0, ni); memset(ff_clzf, 0, ni);
+  memset(ff_clxc, 0, ni); memset(ff_clyc, 0, ni); memset(ff_clzc, 0, ni);
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    c = g->w3_t[cc];
+    ff_clxc[c] = (ff_nu[c] < 0.0) ? g->xp1[c] : g->w3_t[cc];
+  }
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    c = g->w3_t[cc];
+    ff_clyc[c] = (ff_nv[c] < 0.0) ? g->yp1[c] : g->w3_t[cc];
+  }
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    c = g->w3_t[cc];
+    ff_clzc[c] = (ff_nw[c] < 0.0) ? g->zp1[c] : g->w3_t[cc];
+  }
+  ff_do_hor(ff_nu, dt, g->h1au1, g->m2d, 1, g->b3_t, g->w3_t, g->xp1, g->xm1, ff_clxc, ff_crfxc);
+  ff_do_hor(ff_nv, dt, g->h2au2, g->m2d, 1, g->b3_t, g->w3_t, g->yp1, g->ym1, ff_clyc, ff_crfyc);
+  ff_do_vert(ff_nw, dt, wc->w9, 1, g->b3_t, g->w3_t, g->zp1, g->zm1, ff_clzc, ff_crfzc);
+  for (cc = 1; cc <= g->b3_e1; cc++) {
+    c = g->w3_e1[cc];
+    ff_clxf[c] = (wd->u1[c] > 0.0) ? g->xm1[c] : g->w3_e1[cc];
+  }
+  ff_do_hor(wd->u1, dt, g->h1acell, g->m2d, 1, g->b3_e1, g->w3_e1, g->xp1, g->xm1, ff_clxf,
+            ff_crfxf);
+  for (cc = 1; cc <= g->b3_e2; cc++) {
+    c = g->w3_e2[cc];
+    ff_clyf[c] = (wd->u2[c] > 0.0) ? g->ym1[c] : g->w3_e2[cc];
+  }
+  ff_do_hor(wd->u2, dt, g->h2acell, g->m2d, 1, g->b3_e2, g->w3_e2, g->yp1, g->ym1, ff_clyf,
+            ff_crfyf);
+  for (cc = 1; cc <= wc->vc; cc++) {
+    c = wc->s1[cc];
+    ff_clzf[c] = (wd->w[c] > 0.0) ? g->zm1[c] : wc->s1[cc];
+  }
+  ff_do_vert(wd->w, dt, wc->dz, 1, wc->vc, wc->s1, g->zp1, g->zm1, ff_clzf, ff_crfzf);
+}
+
+/* tracers.c:2717-2726  monotone clamp of a transverse interpolation */
+static double ff_monoco(int c, int cm, int cp, const double *tr, const double *trmod,
+                        double sc) {
+  double v, mn, mx;
+  mn = min(trmod[cm], trmod[cp]);
+  mx = max(trmod[cm], trmod[cp]);
+  v = max(mn * sc, tr[c]);
+  v = min(mx * sc, v);
+  return v;
+}
+
+/* tracers.c:2868-2910  van Leer face value on the fractional part */
+static void ff_van_leer(double *F, const double *tr, const double *vel, const double *crf,
+                        const int *cl, int ss, int se, const int *sdo, const int *fmap,
+                        const int *bmap) {
+  int c, cc, cp, bm1, bm2;
+  double df;
+  for (cc = ss; cc <= se; cc++) {
+    c = sdo[cc];
+    if (crf[c] == 0.0) continue;
+    cp = cl[c];
+    if (vel[c] > 0.0) cp = fmap[cp];
+    bm1 = bmap[cp];
+    bm2 = bmap[bm1];
+    if (vel[c] > 0.0) {
+      df = (tr[bm1] - tr[bm2]) * (tr[cp] - tr[bm1]);
+      if (df > 0.0)
+        df = 2.0 * df / (tr[cp] - tr[bm2]);
+      else
+        df = 0.0;
+      F[c] = tr[bm1] + 0.5 * (1.0 - crf[c]) * df;
+    } else if (vel[c] < 0.0) {
+      df = (tr[cp] - tr[bm1]) * (tr[fmap[cp]] - tr[cp]);
+      if (df > 0.0)
+        df = 2.0 * df / (tr[fmap[cp]] - tr[bm1]);
+      else
+        df = 0.0;
+      F[c] = tr[cp] - 0.5 * (1.0 - crf[c]) * df;
+    }
+  }
+}
+
+/* interpolation of a along the trajectory (cl, crf) of c in the
+ * direction of velocity v: the transverse term of tracers.c:2508-2565 */
+static double ff_tv(const double *a, double v, int clc, double crfc, const int *bmap,
+                    int *upo, int *cpo) {
+  int up1 = (v < 0.0) ? clc : bmap[clc];
+  int cp = (v < 0.0) ? bmap[up1] : clc;
+  *upo = up1;
+  *cpo = cp;
+  return a[cp] + (crfc * (a[up1] - a[cp]));
+}
+
+/* tracers.c:2447-2706  FFSL fluxes of one tracer (final fluxes: the
+ * volume flux and the vertical sign are applied here) */
+static void oracle_ff_sl3d(geometry_t *g, window_t *wd, win_priv_t *wc, const double *tr,
+                           double *Fx, double *Fy, double *Fz, double *tr_mod,
+                           double *tr_mod_x, double *tr_mod_y, double *tr_mod_z,
+                           double dt) {
+  const double onesixth = 1.0 / 6.0;
+  const double *u1 = wd->u1, *u2 = wd->u2, *w = wd->w;
+  const double *u1vm = wd->u1flux3d, *u2vm = wd->u2flux3d;
+  const size_t nd = (size_t)g->sgsiz * sizeof(double);
+  int c, cc, cp, up1, c2;
+  double dx, dy, dz, ltraj, lrem;
+  memset(tr_mod, 0, nd);
+  memset(Fx, 0, nd);
+  memset(Fy, 0, nd);
+  memset(Fz, 0, nd);
+  /* transverse terms (x, y, z) with the monotone clamp */
+  memcpy(tr_mod_x, tr, nd);
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    c = g->w3_t[cc];
+    up1 = (ff_nu[c] < 0.0) ? ff_clxc[c] : g->xm1[ff_clxc[c]];
+    cp = (ff_nu[c] < 0.0) ? g->xm1[up1] : ff_clxc[c];
+    tr_mod_x[c] = tr[cp];
+    if (ff_crfxc[c] > 0.) tr_mod_x[c] += (ff_crfxc[c] * (tr[up1] - tr[cp]));
+    tr_mod_x[c] = ff_monoco(c, cp, up1, tr_mod_x, tr, 1.0);
+  }
+  memcpy(tr_mod_y, tr, nd);
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    c = g->w3_t[cc];
+    up1 = (ff_nv[c] < 0.0) ? ff_clyc[c] : g->ym1[ff_clyc[c]];
+    cp = (ff_nv[c] < 0.0) ? g->ym1[up1] : ff_clyc[c];
+    tr_mod_y[c] = tr[cp];
+    if (ff_crfyc[c] > 0.) tr_mod_y[c] += (ff_crfyc[c] * (tr[up1] - tr[cp]));
+    tr_mod_y[c] = ff_monoco(c, cp, up1, tr_mod_y, tr, 1.0);
+  }
+  memcpy(tr_mod_z, tr, nd);
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    c = g->w3_t[cc];
+    up1 = (ff_nw[c] < 0.0) ? ff_clzc[c] : g->zm1[ff_clzc[c]];
+    cp = (ff_nw[c] < 0.0) ? g->zm1[up1] : ff_clzc[c];
+    tr_mod_z[c] = tr[cp];
+    if (ff_crfzc[c] > 0.) tr_mod_z[c] += (ff_crfzc[c] * (tr[up1] - tr[cp]));
+    tr_mod_z[c] = ff_monoco(c, cp, up1, tr_mod_z, tr, 1.0);
+  }
+  /* x direction */
+  memcpy(tr_mod, tr, nd);
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    int u, q;
+    c = g->w3_t[cc];
+    tr_mod[c] += (tr[c] + tr_mod_y[c] + tr_mod_z[c]);
+    tr_mod[c] += ff_tv(tr_mod_y, ff_nw[c], ff_clzc[c], ff_crfzc[c], g->zm1, &u, &q);
+    tr_mod[c] += ff_tv(tr_mod_z, ff_nv[c], ff_clyc[c], ff_crfyc[c], g->ym1, &u, &q);
+    tr_mod[c] *= onesixth;
+  }
+  ff_van_leer(Fx, tr_mod, u1, ff_crfxf, ff_clxf, 1, g->b3_e1, g->w3_e1, g->xp1, g->xm1);
+  for (cc = 1; cc <= g->b3_e1; cc++) {
+    c = g->w3_e1[cc];
+    if (u1[c] == 0.0) {
+      Fx[c] = 0.0;
+      continue;
+    }
+    cp = ff_clxf[c];
+    c2 = g->m2d[cp];
+    Fx[c] *= (ff_crfxf[c] * g->h1acell[c2]);
+    cp = (u1[c] > 0.0) ? g->xm1[c] : g->w3_e1[cc];
+    c2 = g->m2d[cp];
+    dx = g->h1acell[c2];
+    ltraj = fabs(u1[c] * dt);
+    lrem = ltraj;
+    while (dx < lrem) {
+      Fx[c] += (dx * tr_mod[cp]);
+      lrem -= dx;
+      cp = (u1[c] < 0) ? g->xp1[cp] : g->xm1[cp];
+      c2 = g->m2d[cp];
+      if (g->h1acell[c2] > 0) dx = g->h1acell[c2];
+    }
+    Fx[c] *= (u1vm[c] / ltraj);
+  }
+  /* y direction */
+  memcpy(tr_mod, tr, nd);
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    int u, q;
+    c = g->w3_t[cc];
+    tr_mod[c] += (tr[c] + tr_mod_x[c] + tr_mod_z[c]);
+    tr_mod[c] += ff_tv(tr_mod_x, ff_nw[c], ff_clzc[c], ff_crfzc[c], g->zm1, &u, &q);
+    tr_mod[c] += ff_tv(tr_mod_z, ff_nu[c], ff_clxc[c], ff_crfxc[c], g->xm1, &u, &q);
+    tr_mod[c] *= onesixth;
+  }
+  ff_van_leer(Fy, tr_mod, u2, ff_crfyf, ff_clyf, 1, g->b3_e2, g->w3_e2, g->yp1, g->ym1);
+  for (cc = 1; cc <= g->b3_e2; cc++) {
+    c = g->w3_e2[cc];
+    if (u2[c] == 0.0) {
+      Fy[c] = 0.0;
+      continue;
+    }
+    cp = ff_clyf[c];
+    c2 = g->m2d[cp];
+    Fy[c] *= (ff_crfyf[c] * g->h2acell[c2]);
+    cp = (u2[c] > 0.0) ? g->ym1[c] : g->w3_e2[cc];
+    c2 = g->m2d[cp];
+    dy = g->h2acell[c2];
+    ltraj = fabs(u2[c] * dt);
+    lrem = ltraj;
+    while (dy < lrem) {
+      Fy[c] += (dy * tr_mod[cp]);
+      lrem -= dy;
+      cp = (u2[c] < 0) ? g->yp1[cp] : g->ym1[cp];
+      c2 = g->m2d[cp];
+      if (g->h2acell[c2] > 0) dy = g->h2acell[c2];
+    }
+    Fy[c] *= (u2vm[c] / ltraj);
+  }
+  /* z direction */
+  memcpy(tr_mod, tr, nd);
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    int u, q;
+    c = g->w3_t[cc];
+    tr_mod[c] += (tr[c] + tr_mod_x[c] + tr_mod_y[c]);
+    tr_mod[c] += ff_tv(tr_mod_y, ff_nu[c], ff_clxc[c], ff_crfxc[c], g->xm1, &u, &q);
+    tr_mod[c] += ff_tv(tr_mod_x, ff_nv[c], ff_clyc[c], ff_crfyc[c], g->ym1, &u, &q);
+    tr_mod[c] *= onesixth;
+  }
+  ff_van_leer(Fz, tr_mod, w, ff_crfzf, ff_clzf, 1, wc->vc, wc->s1, g->zp1, g->zm1);
+  for (cc = 1; cc <= wc->vc; cc++) {
+    c = wc->s1[cc];
+    if (w[c] == 0.0) {
+      Fz[c] = 0.0;
+      continue;
+    }
+    cp = ff_clzf[c];
+    Fz[c] *= (ff_crfzf[c] * wc->dz[cp]);
+    cp = (w[c] > 0.0) ? g->zm1[c] : wc->s1[cc];
+    dz = fabs(wc->dz[cp]);
+    lrem = fabs(w[c] * dt);
+    while (dz < lrem) {
+      Fz[c] += (dz * tr_mod[cp]);
+      lrem -= dz;
+      cp = (w[c] < 0.0) ? g->zp1[cp] : g->zm1[cp];
+      if (wc->dz[cp] > 0) dz = wc->dz[cp];
+    }
+    if (w[c] < 0) Fz[c] *= -1;
+  }
+}
+
+/*-------------------------------------------------------------------*/
 /* tracers.c:1487-1576  face values -> fluxes                         */
 /*-------------------------------------------------------------------*/
 static void oracle_advect(geometry_t *g, window_t *wd, win_priv_t *wc,
                           const double *tr, double *Fx, double *Fy,
-                          double *Fz, double dtu) {
+                          double *Fz, double *tm, double *tmx, double *tmy,
+                          double *tmz, double dtu) {
   int cc;
   if (wd->dttr == 0.0) return;
+  if (wc->trasc == FFSL) {
+    oracle_ff_sl3d(g, wd, wc, tr, Fx, Fy, Fz, tm, tmx, tmy, tmz, dtu);
+    return;
+  }
   if (wc->trasc == ORDER1) {
     o1_faces(g, wd, wc, tr, Fx, Fy, Fz, dtu);
     return;
@@ -541,6 +876,7 @@ void oracle_tr_bounds(geometry_t *g, window_t *wd, win_priv_t *wc) {
 /*-------------------------------------------------------------------*/
 typedef struct {
   double *Fx, *Fy, *Fz, *dtracer;
+  double *tm, *tmx, *tmy, *tmz; /* FFSL modified tracers */
 } scratch_t;
 
 static void tracer_substep(geometry_t *g, window_t *wd, win_priv_t *wc,
@@ -561,12 +897,22 @@ static void tracer_substep(geometry_t *g, window_t *wd, win_priv_t *wc,
     int cb = g->bot_t[cc];
     tr[g->zm1[cb]] = tr[cb];
   }
-  if (wc->advect[n]) oracle_advect(g, wd, wc, tr, Fx, Fy, Fz, dtu);
+  /* tracers.c:836-841: FFSL also fills the sediment ghosts from above */
+  if (wc->trasc == FFSL) {
+    for (cc = 1; cc <= g->ngsed; cc++) {
+      int cb = g->gsed_t[cc];
+      tr[cb] = tr[g->zp1[cb]];
+    }
+  }
+  if (wc->advect[n])
+    oracle_advect(g, wd, wc, tr, Fx, Fy, Fz, s->tm, s->tmx, s->tmy, s->tmz, dtu);
   if (wc->diffuse[n]) oracle_hor_diffuse(g, wd, wc, tr, Fx, Fy);
   for (cc = 1; cc <= g->b3_t; cc++) {
     int c = g->w3_t[cc];
     dtracer[c] = (Fx[g->xp1[c]] - Fx[c] + Fy[g->yp1[c]] - Fy[c]) * dtu;
   }
+  /* tracers.c:902-903 */
+  if (wc->trasc == FFSL && wd->nstep == 0) memset(dtracer, 0, nb);
   for (cc = ksf; cc <= kef; cc++) {
     int c = wc->s1[cc];
     int c2 = g->m2d[c];
@@ -601,6 +947,10 @@ static scratch_t *scratch_get(int nt, int n) {
     free(scr[t].Fy);
     free(scr[t].Fz);
     free(scr[t].dtracer);
+    free(scr[t].tm);
+    free(scr[t].tmx);
+    free(scr[t].tmy);
+    free(scr[t].tmz);
   }
   free(scr);
   scr = (scratch_t *)calloc(nt, sizeof(scratch_t));
@@ -609,6 +959,10 @@ static scratch_t *scratch_get(int nt, int n) {
     scr[t].Fy = (double *)calloc(n, sizeof(double));
     scr[t].Fz = (double *)calloc(n, sizeof(double));
     scr[t].dtracer = (double *)calloc(n, sizeof(double));
+    scr[t].tm = (double *)calloc(n, sizeof(double));
+    scr[t].tmx = (double *)calloc(n, sizeof(double));
+    scr[t].tmy = (double *)calloc(n, sizeof(double));
+    scr[t].tmz = (double *)calloc(n, sizeof(double));
   }
   scr_nt = nt;
   scr_n = n;
@@ -646,6 +1000,33 @@ int oracle_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc,
       int c = g->w3_t[cc];
       wc->w9[c] = 0.5 * (wc->dz[c] + wc->dz[g->zm1[c]]);
     }
+  } else if (wc->trasc == FFSL) {
+    /* tracers.c:524-559: cell-centred velocities (Leonard et al. 1996) */
+    ff_alloc(g->sgsiz);
+    memset(ff_nu, 0, g->sgsiz * sizeof(double));
+    memset(ff_nv, 0, g->sgsiz * sizeof(double));
+    memset(ff_nw, 0, g->sgsiz * sizeof(double));
+    itermax = 200;
+    for (cc = 1; cc <= g->b3_t; cc++) {
+      int c = g->w3_t[cc];
+      int xp1, yp1, zp1;
+      wc->w9[c] = 0.5 * (wc->dz[c] + wc->dz[g->zm1[c]]);
+      xp1 = g->xp1[c];
+      if ((u1[c] >= 0.0) && (u1[xp1] > 0.0))
+        ff_nu[c] = u1[c];
+      else if ((u1[c] < 0.0) && (u1[xp1] <= 0.0))
+        ff_nu[c] = u1[xp1];
+      yp1 = g->yp1[c];
+      if ((u2[c] >= 0.0) && (u2[yp1] > 0.0))
+        ff_nv[c] = u2[c];
+      else if ((u2[c] < 0.0) && (u2[yp1] <= 0.0))
+        ff_nv[c] = u2[yp1];
+      zp1 = g->zp1[c];
+      if ((w[c] >= 0.0) && (w[zp1] > 0.0))
+        ff_nw[c] = w[c];
+      else if ((w[c] < 0.0) && (w[zp1] <= 0.0))
+        ff_nw[c] = w[zp1];
+    }
   } else {
     for (cc = 1; cc <= g->b3_t; cc++) {
       int c = g->w3_t[cc];
@@ -671,15 +1052,19 @@ int oracle_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc,
       int c2 = g->m2d[c];
       int xp1 = g->xp1[c], yp1 = g->yp1[c], zp1 = g->zp1[c];
       double d1, d2, d3, d4;
+      /* FFSL is limited by the velocity gradients (Lipschitz number) */
       d4 = 0.5 * (u1[c] + u1[xp1]);
+      if (wc->trasc == FFSL) d4 = max(fabs(u1[xp1] - u1[c]), wc->u1kh[c] / g->h1acell[c2]);
       if (fabs(d4) < minval) d4 = minval;
       d1 = sf * fabs(g->h1acell[c2] / d4);
       if (d1 < dtm) dtm = d1;
       d4 = 0.5 * (u2[c] + u2[yp1]);
+      if (wc->trasc == FFSL) d4 = max(fabs(u2[yp1] - u2[c]), wc->u2kh[c] / g->h2acell[c2]);
       if (fabs(d4) < minval) d4 = minval;
       d2 = sf * fabs(g->h2acell[c2] / d4);
       if (d2 < dtm) dtm = d2;
       d4 = (cc <= vcs) ? 0.0 : 0.5 * (w[c] + w[zp1]);
+      if (wc->trasc == FFSL) d4 = (cc <= vcs) ? 0.0 : fabs(w[zp1] - w[c]);
       if (fabs(d4) < minval) d4 = minval;
       d3 = (cc <= vcs) ? dtu : sf * fabs(wc->dz[c] * wc->Ds[c2] / d4);
       if (d3 > 0.0 && d3 < dtm) dtm = d3;
@@ -718,6 +1103,8 @@ int oracle_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc,
         subeta[c2] = wc->oldeta[c2] +
                      (1.0 + (dtu - trem) / wd->dt) * (wd->eta[c2] - wc->oldeta[c2]);
       }
+      /* tracers.c:763-765 */
+      if (wc->trasc == FFSL) oracle_prep_ff_sl(g, wd, wc, dtu);
       {
         int nt = nthreads > 0 ? nthreads : 1;
         if (nt > wc->ntbdy) nt = wc->ntbdy > 0 ? wc->ntbdy : 1;

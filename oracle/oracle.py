@@ -7,6 +7,8 @@ import ctypes as C
 import os
 import subprocess
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liboracle.so")
 _lib = None
@@ -42,6 +44,8 @@ def lib():
         L.oracle_check_tmass.argtypes = [G, W, P, C.c_int]
         L.oracle_check_tmass.restype = C.c_double
         L.oracle_max_threads.restype = C.c_int
+        L.oracle_ffsl_maps.argtypes = [C.POINTER(C.POINTER(C.c_int)),
+                                       C.POINTER(C.POINTER(C.c_double))]
         L.oracle_last_error.restype = C.c_int
         _lib = L
     return _lib
@@ -69,3 +73,15 @@ def vert_diffuse(case, nthreads=1):
 
 def tmass(case, n):
     return lib().oracle_check_tmass(case.gp, case.wp, case.pp, n)
+
+
+def ffsl_maps(case):
+    """FFSL trajectory arrays of the last sub-step: (cl, crf), each a list
+    of six numpy copies [x face, y face, z face, x centre, y centre,
+    z centre] over the sparse array size."""
+    cl = (C.POINTER(C.c_int) * 6)()
+    crf = (C.POINTER(C.c_double) * 6)()
+    lib().oracle_ffsl_maps(cl, crf)
+    n = case.sgsiz
+    return ([np.ctypeslib.as_array(cl[i], shape=(n,)).copy() for i in range(6)],
+            [np.ctypeslib.as_array(crf[i], shape=(n,)).copy() for i in range(6)])

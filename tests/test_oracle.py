@@ -340,3 +340,103 @@ def test_tr_bounds_thin_surface_takes_value_below():
     exp = np.where(exp < 0.0, 0.0, exp)  # then the clip at min = 0
     assert np.array_equal(t[2], exp)
     assert np.array_equal(t[3], t[2])
+
+
+# ---------------------------------------------------------------------
+# FFSL (structured): trajectories and fluxes
+# ---------------------------------------------------------------------
+@pytest.mark.parametrize("U,dt", [(0.4, 60.0), (-0.3, 60.0), (0.8, 2000.0), (-0.7, 3100.0)])
+def test_ffsl_linear_profile_translates_exactly(U, dt):
+    """Courant numbers 0.024 to 2.2: the flux is the integral of the
+    (linear) profile over the upstream trajectory, so the profile
+    translates exactly, including multi-cell trajectories."""
+    case, x, a = _channel(U, S.FFSL, dt=dt)
+    case.wd.nstep = 1
+    assert O.advect_diffuse(case) == 0
+    t = H.tracer_dense(case, 0)
+    exp = 5.0 + a * (x - U * case.dt)
+    inner = slice(6, case.nx - 6)
+    assert np.abs(t[:, 1:-1, inner] - exp[inner]).max() < 1e-12 * 6.0
+
+
+def test_ffsl_face_trajectories_match_closed_form():
+    """Uniform 1 km cells, u = 0.8 m/s, dt = 2000 s: the departure point of
+    every west face lies 1.6 cells upstream, in cell i-2 at 0.6 of it."""
+    case, x, a = _channel(0.8, S.FFSL, dt=2000.0)
+    case.wd.nstep = 1
+    assert O.advect_diffuse(case) == 0
+    cl, crf = O.ffsl_maps(case)
+    g = case.g
+    wet = S.view(g.w3_t, case.sgsiz, np.int32)[1:g.v3_t + 1]
+    s2i = S.view(g.s2i, case.sgsiz, np.int32)
+    s2j = S.view(g.s2j, case.sgsiz, np.int32)
+    s2k = S.view(g.s2k, case.sgsiz, np.int32)
+    idx = {(s2i[c], s2j[c], s2k[c]): c for c in wet}
+    n = 0
+    for c in wet:
+        i, j, k = s2i[c], s2j[c], s2k[c]
+        if i < 3:
+            continue
+        assert cl[0][c] == idx[(i - 2, j, k)]
+        assert abs(crf[0][c] - 0.6) < 1e-14
+        n += 1
+    assert n > 0
+
+
+def test_ffsl_vertical_trajectories_match_closed_form():
+    """Uniform 5 m layers, w = -2.2e-3 m/s (down), dt = 3000 s: the top-face
+    departure point lies 1.32 layers above, in layer k+2 at 0.32 of it."""
+    case = H.box(nxi=4, nyi=4, nz=8, depth=40.0, ntr=1)
+    nz, ny, nx = case.nz, case.ny, case.nx
+    H.set_dense_face_fields(case, u1=np.zeros((nz, ny, nx)), u2=np.zeros((nz, ny, nx)),
+                            w=np.full((nz, ny, nx), -2.2e-3), Kz=np.zeros((nz, ny, nx)))
+    H.scheme(case, S.FFSL, 0, 3000.0, mintr=-1e10)
+    case.wd.nstep = 1
+    case.prepare(O.set_dz)
+    case.a3(case.wd.w)[:] = -2.2e-3  # prepare() sets w from continuity
+    assert O.advect_diffuse(case) == 0
+    cl, crf = O.ffsl_maps(case)
+    g = case.g
+    wet = S.view(g.w3_t, case.sgsiz, np.int32)[1:g.v3_t + 1]
+    s2i = S.view(g.s2i, case.sgsiz, np.int32)
+    s2j = S.view(g.s2j, case.sgsiz, np.int32)
+    s2k = S.view(g.s2k, case.sgsiz, np.int32)
+    idx = {(s2i[c], s2j[c], s2k[c]): c for c in wet}
+    n = 0
+    for c in wet:
+        i, j, k = s2i[c], s2j[c], s2k[c]
+        if k + 2 > nz - 1:
+            continue
+        # face at the bottom of cell k, w < 0: the walk starts in cell k
+        assert cl[2][c] == idx[(i, j, k + 1)], (k, cl[2][c])
+        assert abs(crf[2][c] - 0.32) < 1e-12
+        n += 1
+    assert n > 0
+
+
+def test_ffsl_no_horizontal_flux_on_the_first_step():
+    """tracers.c:902-903: at nstep 0 the FFSL horizontal divergence is
+    dropped, so a horizontally advected profile does not move."""
+    case, x, a = _channel(0.4, S.FFSL, dt=60.0)
+    case.wd.nstep = 0
+    t0 = H.tracer_dense(case, 0).copy()
+    assert O.advect_diffuse(case) == 0
+    assert np.array_equal(H.tracer_dense(case, 0)[:, 1:-1, 1:-1], t0[:, 1:-1, 1:-1])
+
+
+def test_ffsl_constant_and_mass_on_curvilinear_basin():
+    case = configs.small(seed=9, ntr=2, trasc=S.FFSL, ultimate=0, dt=900.0)
+    case.wd.nstep = 1
+    S.view(case.wc.mintr, 2, np.float64)[:] = -1e10
+    c = case.tracers()
+    c[0] = 3.25
+    case.set_tracers(c)
+    case.prepare(O.set_dz)
+    m0 = np.array([O.tmass(case, n) for n in range(case.ntr)])
+    for _ in range(5):
+        assert O.step(case) == 0
+        case.wd.nstep += 1
+    t = case.tracers()
+    m1 = np.array([O.tmass(case, n) for n in range(case.ntr)])
+    assert np.abs(t[0] - 3.25).max() < 1e-11
+    assert np.all(np.abs(m1 - m0) <= 1e-12 * np.abs(m0))
