@@ -52,6 +52,15 @@ typedef struct emstr_dev {
   int nwet, sgsiz;
   int *wsp;
   long long *wdn;
+
+  /* FFSL (trasc == FFSL only): cell-centred velocities, w9, trajectory
+   * heads (positions, see emstr_plain.hip) and fractions of the x, y, z
+   * faces and x, y, z centres, and the one-tracer work arrays          */
+  int nstep0; /* windat->nstep == 0: no horizontal divergence (FFSL) */
+  double *fnu, *fnv, *fnw, *fw9;
+  long long *fcl; /* [6][n3] */
+  double *fcrf;   /* [6][n3] */
+  double *ftmx, *ftmy, *ftmz, *ftm, *ffx, *ffy, *ffz;
 } emstr_dev;
 
 #ifdef __cplusplus

@@ -59,10 +59,12 @@ struct emstr_ctx {
   int *d_bpt, *d_bin, *d_gsed, *d_ised;
   int *d_proc;
   int *d_err;
+  int *d_fferr;   /* FFSL trajectory walk bound exceeded */
   unsigned long long *d_dtm;
   double *d_part, *h_part;
   double *d_wbot;
   int anydif;
+  int have_kh;   /* dzu1/dzu2/u1kh/u2kh on the device */
   int have_cols; /* per-column step data present on the device */
   /* identity of the grid the context was built for */
   int key_sgsiz, key_nx, key_ny, key_nz, key_v3;
@@ -309,9 +311,10 @@ emstr_ctx *emstr_create(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
   c->anydif = 0;
   for (n = 0; n < wd->ntr; n++)
     if (wc->diffuse[n]) c->anydif = 1;
-  if (c->anydif) {
+  if (c->anydif || wc->trasc == FFSL) {
     double **dd[] = {&d->dzu1, &d->dzu2, &d->u1kh, &d->u2kh};
     for (n = 0; n < 4; n++) *dd[n] = (double *)dalloc(n3 * sizeof(double), &ok);
+    c->have_kh = 1;
   }
   const int nt = wd->ntr > 0 ? wd->ntr : 1;
   d->mintr = (double *)dalloc(nt * sizeof(double), &ok);
@@ -330,6 +333,7 @@ emstr_ctx *emstr_create(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
   c->d_ised = (int *)dalloc((g->ngsed + 1) * sizeof(int), &ok);
   c->d_proc = (int *)dalloc(plane * sizeof(int), &ok);
   c->d_err = (int *)dalloc(sizeof(int), &ok);
+  c->d_fferr = (int *)dalloc(sizeof(int), &ok);
   c->d_dtm = (unsigned long long *)dalloc(sizeof(unsigned long long), &ok);
   {
     int nb = (g->a2_t + 255) / 256 + 1;
@@ -435,7 +439,9 @@ void emstr_destroy(emstr_ctx *c) {
                 d->u2kh, d->mintr, d->maxtr, d->advect, d->diffuse, d->in_tbdy,
                 d->in_tdif, d->cols, d->wsp, d->wdn, c->stage, c->d_bpt, c->d_bin,
                 c->d_gsed, c->d_ised, c->d_proc, c->d_err, c->d_dtm, c->d_part,
-                c->d_wbot, c->tr[0], c->tr[1], c->tr[2]};
+                c->d_wbot, c->tr[0], c->tr[1], c->tr[2], c->d_fferr, d->fnu, d->fnv,
+                d->fnw, d->fw9, d->fcl, d->fcrf, d->ftmx, d->ftmy, d->ftmz, d->ftm,
+                d->ffx, d->ffy, d->ffz};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->h_kb, c->cc2col, c->col2cc, c->sedg, c->topc, c->h_ks,
@@ -550,7 +556,11 @@ int emstr_upload_fields(emstr_ctx *c) {
       upload3d(c, d->u2flux3d, wd->u2flux3d) || upload3d(c, d->Kz, wd->Kz) ||
       upload3d(c, d->dz, wc->dz))
     return EMSTR_EDEVICE;
-  if (c->anydif)
+  d->nstep0 = (wd->nstep == 0);
+  if (d->trasc == FFSL && !c->have_kh)
+    return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                           "FFSL selected after the context was created");
+  if (c->have_kh)
     if (upload3d(c, d->dzu1, wd->dzu1) || upload3d(c, d->dzu2, wd->dzu2) ||
         upload3d(c, d->u1kh, wc->u1kh) || upload3d(c, d->u2kh, wc->u2kh))
       return EMSTR_EDEVICE;
@@ -612,12 +622,35 @@ int emstr_download_tracers(emstr_ctx *c) {
 /*-------------------------------------------------------------------*/
 /* the step                                                          */
 /*-------------------------------------------------------------------*/
+/* FFSL work arrays (trajectories, transverse tracers, fluxes), made on
+ * first use: 184 bytes per dense cell */
+static int ffsl_alloc(emstr_ctx *c) {
+  emstr_dev *d = &c->d;
+  const size_t n3 = (size_t)d->n3;
+  int ok = 1, n;
+  if (d->fnu) return 0;
+  double **a[] = {&d->fnu, &d->fnv, &d->fnw, &d->fw9, &d->ftmx, &d->ftmy,
+                  &d->ftmz, &d->ftm, &d->ffx, &d->ffy, &d->ffz};
+  for (n = 0; n < (int)(sizeof(a) / sizeof(a[0])); n++)
+    *a[n] = (double *)dalloc(n3 * sizeof(double), &ok);
+  d->fcl = (ll *)dalloc(6 * n3 * sizeof(ll), &ok);
+  d->fcrf = (double *)dalloc(6 * n3 * sizeof(double), &ok);
+  if (!ok) return emstr_set_error(EMSTR_EDEVICE, -1, "FFSL work arrays: allocation failed");
+  return 0;
+}
+
 static int adv_plain(emstr_ctx *c) {
   emstr_dev *d = &c->d;
   win_priv_t *wc = c->wc;
   double dtu = d->dt, dtm = d->dt, trem;
-  int itermax = 20, first = 1;
+  int itermax = 20, first = 1, n;
+  const int ffsl = (d->trasc == FFSL);
   if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
+  if (ffsl) {
+    /* tracers.c:524-559 */
+    itermax = 200;
+    if (ffsl_alloc(c)) return EMSTR_EDEVICE;
+  }
   if (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)) {
     unsigned long long init;
     double big = dtu;
@@ -647,7 +680,20 @@ static int adv_plain(emstr_ctx *c) {
       if (emstr_launch_subeta(d, u, trem, first, c->st))
         return emstr_set_error(EMSTR_EDEVICE, -1, "subeta launch");
       pi = prof_begin(c->prof, KC_ADV, c->st);
-      bad = emstr_launch_adv(d, c->tr[in], c->tr[out], c->tr[b0], u, trem, c->anydif, c->st);
+      if (ffsl) {
+        /* tracers.c:763-765, then the fluxes and update of each tracer */
+        bad = emstr_launch_ffsl_traj(d, u, c->d_fferr, c->st);
+        for (n = 0; n < d->ntr && !bad; n++) {
+          bad = emstr_launch_ffsl_fluxes(d, c->tr[in] + (ll)n * d->n3,
+                                         c->tr[b0] + (ll)n * d->n3, u, c->d_fferr, c->st);
+          if (!bad)
+            bad = emstr_launch_adv(d, c->tr[in], c->tr[out], c->tr[b0], u, trem, c->anydif,
+                                   n, n + 1, c->st);
+        }
+      } else {
+        bad = emstr_launch_adv(d, c->tr[in], c->tr[out], c->tr[b0], u, trem, c->anydif, 0,
+                               d->ntr, c->st);
+      }
       prof_end(c->prof, pi, c->st);
       if (bad || emstr_launch_osub(d, c->st))
         return emstr_set_error(EMSTR_EDEVICE, -1, "advection launch");
@@ -659,6 +705,16 @@ static int adv_plain(emstr_ctx *c) {
     first = 0;
   }
   c->cur = in;
+  if (ffsl) {
+    int e = 0;
+    HCHK(hipMemcpyAsync(&e, c->d_fferr, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HCHK(hipStreamSynchronize(c->st));
+    if (e) {
+      HCHK(hipMemsetAsync(c->d_fferr, 0, sizeof(int), c->st));
+      return emstr_set_error(EMSTR_ESUBSTEP, -1,
+                             "ff_sl: a trajectory crosses more than the walk bound of cells");
+    }
+  }
   return 0;
 }
 

@@ -13,7 +13,13 @@ extern "C" {
 /* plain FP64 form (emstr_plain.hip) */
 int emstr_launch_adv(const emstr_dev *d, const double *tin, double *tout,
                      const double *tstart, double dtu, double trem, int anydif,
-                     void *stream);
+                     int n0, int n1, void *stream);
+/* FFSL: cell-centred velocities and trajectories of one sub-step, then
+ * the fluxes of one tracer (t: its buffer, ts: its step-start buffer);
+ * err is set when a trajectory crosses more cells than the walk bound */
+int emstr_launch_ffsl_traj(const emstr_dev *d, double dtu, int *err, void *stream);
+int emstr_launch_ffsl_fluxes(const emstr_dev *d, const double *t, const double *ts,
+                             double dtu, int *err, void *stream);
 int emstr_launch_vdiff(const emstr_dev *d, double *tr, double *scratch,
                        int *err, void *stream);
 int emstr_launch_setdz(const emstr_dev *d, int *proc, void *stream);

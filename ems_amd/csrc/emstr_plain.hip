@@ -36,8 +36,13 @@
 #define QUICKEST 0x000020
 #define QUICKEST_CO 0x000040
 #define VANLEER 0x000080
+#define FFSL 0x080000
 
 typedef long long ll;
+
+/* the reference's max()/min() macros (ternaries, not fmax/fmin) */
+__device__ __forceinline__ double fmx(double a, double b) { return (a > b) ? a : b; }
+__device__ __forceinline__ double fmn(double a, double b) { return (a < b) ? a : b; }
 
 /*-------------------------------------------------------------------*/
 /* Stencil geometry of one wet cell                                  */
@@ -406,7 +411,17 @@ __device__ __forceinline__ double cell_fluxes(const emstr_dev &d,
   double FxW = 0.0, FxE = 0.0, FyS = 0.0, FyN = 0.0;
   FzB = 0.0;
   FzT = 0.0;
-  if (adv) {
+  if constexpr (SCH == FFSL) {
+    if (adv) {
+      /* fluxes of the trajectory integrals (k_ffsl_flux) */
+      FxW = d.ffx[idx];
+      if (s.wetE) FxE = d.ffx[idx + 1];
+      FyS = d.ffy[idx];
+      if (s.wetN) FyN = d.ffy[idx + d.nx];
+      FzB = d.ffz[idx];
+      if (need_top) FzT = d.ffz[idx + d.plane];
+    }
+  } else if (adv) {
     /* W face: stencil of c;  E face: stencil of xp1 (bm1 = c) */
     FxW = face_flux<SCH>(v.xm2, v.xm1, v.c, v.xp1, cc.W, ult);
     if (s.wetE) FxE = face_flux<SCH>(v.xm1, v.c, v.xp1, v.xp2, cc.E, ult);
@@ -427,6 +442,8 @@ __device__ __forceinline__ double cell_fluxes(const emstr_dev &d,
     FyS -= (cc.kS * (v.c - v.ym1) / cc.hS);
     if (s.wetN) FyN -= (cc.kN * (v.yp1 - v.c) / cc.hN);
   }
+  /* tracers.c:902-903: FFSL drops the horizontal divergence at nstep 0 */
+  if (SCH == FFSL && d.nstep0) return 0.0;
   return (FxE - FxW + FyN - FyS) * cc.dtu;
 }
 
@@ -493,7 +510,8 @@ __device__ double cell_dtracer(const emstr_dev &d, const double *t,
 template <int SCH>
 __global__ void __launch_bounds__(256)
 k_adv(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout,
-      const double *__restrict__ tstart, double dtu, double trem, int anydif) {
+      const double *__restrict__ tstart, double dtu, double trem, int anydif, int n0,
+      int n1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
@@ -507,7 +525,7 @@ k_adv(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout,
   const int courf = (d.ultimate || d.trasc >= QUICKEST) ? 1 : 0;
 
   if (ksc < 0) { /* column not processed: unchanged */
-    for (int n = 0; n < d.ntr; n++) tout[n * N3 + idx] = tin[n * N3 + idx];
+    for (int n = n0; n < n1; n++) tout[n * N3 + idx] = tin[n * N3 + idx];
     return;
   }
   if (k > ksc) return; /* above the processed surface: written below */
@@ -519,7 +537,7 @@ k_adv(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout,
   if (anydif) diff_coef(d, col, idx, s, cc);
 
   if (sub) {
-    for (int n = 0; n < d.ntr; n++) {
+    for (int n = n0; n < n1; n++) {
       const double *t = tin + n * N3;
       if (!d.in_tbdy[n]) {
         tout[n * N3 + idx] = t[idx];
@@ -545,7 +563,7 @@ k_adv(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout,
   const double botz = d.botz[col];
   const double *lay = d.layers;
   const int ktop = d.nz - 1;
-  for (int n = 0; n < d.ntr; n++) {
+  for (int n = n0; n < n1; n++) {
     const double *t = tin + n * N3;
     const double *ts = tstart + n * N3;
     double *o = tout + n * N3;
@@ -606,6 +624,475 @@ k_adv(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout,
     /* the surface value fills every layer above (tracers.c:3306-3322) */
     for (int kk = k; kk <= ktop; kk++) o[(ll)kk * d.plane + col] = trs;
   }
+}
+
+static dim3 grid3(const emstr_dev *d, dim3 b);
+
+/*-------------------------------------------------------------------*/
+/* FFSL, structured grid (tracers.c:2132-2910)                        */
+/*-------------------------------------------------------------------*/
+/* A trajectory position is a wet cell or a ghost: p = idx * 8 + g with
+ * idx a dense cell index and g = P_WET (the wet cell idx), P_GW..P_GN
+ * (the lateral ghost west/east/south/north of wet cell idx) or P_SED
+ * (the sediment ghost under bed cell idx).  The maps follow the grid's
+ * ghost convention: a ghost maps back to its cell in the inward
+ * direction and to itself in every other direction.  A lateral ghost
+ * holds the step-start value of its cell (tracers.c:5574-5593), the
+ * sediment ghost the bed value (tracers.c:827-841); a ghost's metrics
+ * are those of the column it lies in. */
+enum { P_WET = 0, P_GW = 1, P_GE = 2, P_GS = 3, P_GN = 4, P_SED = 5 };
+#define FF_MAXWALK 4096 /* bound on the cells a trajectory crosses */
+
+__device__ __forceinline__ ll ppack(ll idx, int g) { return idx * 8 + g; }
+__device__ __forceinline__ ll pidx(ll p) { return p >> 3; }
+__device__ __forceinline__ int pgh(ll p) { return (int)(p & 7); }
+
+__device__ __forceinline__ ll p_xm1(const emstr_dev &d, ll p) {
+  const ll idx = pidx(p);
+  const int g = pgh(p);
+  if (g == P_WET)
+    return wetat(d.kb, idx % d.plane - 1, (int)(idx / d.plane)) ? ppack(idx - 1, P_WET)
+                                                               : ppack(idx, P_GW);
+  return (g == P_GE) ? ppack(idx, P_WET) : p;
+}
+__device__ __forceinline__ ll p_xp1(const emstr_dev &d, ll p) {
+  const ll idx = pidx(p);
+  const int g = pgh(p);
+  if (g == P_WET)
+    return wetat(d.kb, idx % d.plane + 1, (int)(idx / d.plane)) ? ppack(idx + 1, P_WET)
+                                                               : ppack(idx, P_GE);
+  return (g == P_GW) ? ppack(idx, P_WET) : p;
+}
+__device__ __forceinline__ ll p_ym1(const emstr_dev &d, ll p) {
+  const ll idx = pidx(p);
+  const int g = pgh(p);
+  if (g == P_WET)
+    return wetat(d.kb, idx % d.plane - d.nx, (int)(idx / d.plane)) ? ppack(idx - d.nx, P_WET)
+                                                                  : ppack(idx, P_GS);
+  return (g == P_GN) ? ppack(idx, P_WET) : p;
+}
+__device__ __forceinline__ ll p_yp1(const emstr_dev &d, ll p) {
+  const ll idx = pidx(p);
+  const int g = pgh(p);
+  if (g == P_WET)
+    return wetat(d.kb, idx % d.plane + d.nx, (int)(idx / d.plane)) ? ppack(idx + d.nx, P_WET)
+                                                                  : ppack(idx, P_GN);
+  return (g == P_GS) ? ppack(idx, P_WET) : p;
+}
+__device__ __forceinline__ ll p_zm1(const emstr_dev &d, ll p) {
+  const ll idx = pidx(p);
+  if (pgh(p) != P_WET) return p;
+  return ((int)(idx / d.plane) > d.kb[idx % d.plane]) ? ppack(idx - d.plane, P_WET)
+                                                      : ppack(idx, P_SED);
+}
+__device__ __forceinline__ ll p_zp1(const emstr_dev &d, ll p) {
+  const ll idx = pidx(p);
+  const int g = pgh(p);
+  if (g == P_SED) return ppack(idx, P_WET);
+  if (g != P_WET) return p;
+  return ((int)(idx / d.plane) < d.nz - 1) ? ppack(idx + d.plane, P_WET) : p;
+}
+/* the column of a position (its 2D metrics) */
+__device__ __forceinline__ ll p_col(const emstr_dev &d, ll p) {
+  const ll col = pidx(p) % d.plane;
+  switch (pgh(p)) {
+    case P_GW: return col - 1;
+    case P_GE: return col + 1;
+    case P_GS: return col - d.nx;
+    case P_GN: return col + d.nx;
+    default: return col;
+  }
+}
+/* a (the tracer or a modified copy) at p; a ghost holds the tracer */
+__device__ __forceinline__ double p_val(const double *a, const double *t, const double *ts,
+                                        ll p) {
+  const int g = pgh(p);
+  const ll idx = pidx(p);
+  return (g == P_WET) ? a[idx] : ((g == P_SED) ? t[idx] : ts[idx]);
+}
+/* a 3-D cell field (dz, w9) at p: 0 at ghosts (never set there) */
+__device__ __forceinline__ double p_f3(const double *f, ll p) {
+  return (pgh(p) == P_WET) ? f[pidx(p)] : 0.0;
+}
+
+/* tracers.c:2771-2808 horizontal trajectory from head p (pcl, pcrf) */
+template <int DIR>
+__device__ __forceinline__ void ff_hor(const emstr_dev &d, double vel, double dt,
+                                       const double *h, ll &cl, double &crf, int *err) {
+  if (vel == 0.0) return;
+  const double ltraj = fabs(vel) * dt;
+  double lrem = ltraj;
+  ll cp = cl;
+  double dx = h[p_col(d, cp)];
+  int it = 0;
+  while (dx < lrem) {
+    lrem -= dx;
+    if (DIR == 0) cp = (vel < 0.0) ? p_xp1(d, cp) : p_xm1(d, cp);
+    else cp = (vel < 0.0) ? p_yp1(d, cp) : p_ym1(d, cp);
+    const double hh = h[p_col(d, cp)];
+    if (hh > 0) dx = hh;
+    if (++it > FF_MAXWALK) {
+      atomicExch(err, 1);
+      break;
+    }
+  }
+  crf = fmx(fmn(lrem / dx, 1.0), 0.0);
+  cl = cp;
+}
+
+/* tracers.c:2820-2859 vertical trajectory (h: w9 or dz) */
+__device__ __forceinline__ void ff_vert(const emstr_dev &d, ll c, double vel, double dt,
+                                        const double *h, ll &cl, double &crf, int *err) {
+  if (vel == 0.0) return;
+  const double ltraj = fabs(vel) * dt;
+  double lrem = ltraj;
+  ll cp = cl, cpb = cl;
+  const double hcp = p_f3(h, cp);
+  double dz = fabs((hcp > 0) ? hcp : p_f3(h, c));
+  int it = 0;
+  while (dz < lrem) {
+    lrem -= dz;
+    cp = (vel < 0.0) ? p_zp1(d, cp) : p_zm1(d, cp);
+    if (cpb == cp) {
+      lrem = fmod(lrem, dz);
+      break;
+    }
+    const double hh = p_f3(h, cp);
+    if (hh > 0) dz = fabs(hh);
+    cpb = cp;
+    if (++it > FF_MAXWALK) {
+      atomicExch(err, 1);
+      break;
+    }
+  }
+  crf = fmx(fmn(lrem / dz, 1.0), 0.0);
+  cl = cp;
+}
+
+/* cell-centred velocities and w9 (tracers.c:524-559), per wet cell */
+__global__ void k_ffsl_vel(emstr_dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (k < kbc) return;
+  const ll idx = (ll)k * d.plane + col;
+  d.fw9[idx] = 0.5 * (d.dz[idx] + ((k > kbc) ? d.dz[idx - d.plane] : 0.0));
+  double nu = 0.0, nv = 0.0, nw = 0.0;
+  const double u1c = d.u1[idx], u1p = d.u1[idx + 1];
+  if ((u1c >= 0.0) && (u1p > 0.0))
+    nu = u1c;
+  else if ((u1c < 0.0) && (u1p <= 0.0))
+    nu = u1p;
+  const double u2c = d.u2[idx], u2p = d.u2[idx + d.nx];
+  if ((u2c >= 0.0) && (u2p > 0.0))
+    nv = u2c;
+  else if ((u2c < 0.0) && (u2p <= 0.0))
+    nv = u2p;
+  const double wc = d.w[idx], wp = (k < d.nz - 1) ? d.w[idx + d.plane] : wc;
+  if ((wc >= 0.0) && (wp > 0.0))
+    nw = wc;
+  else if ((wc < 0.0) && (wp <= 0.0))
+    nw = wp;
+  d.fnu[idx] = nu;
+  d.fnv[idx] = nv;
+  d.fnw[idx] = nw;
+}
+
+/* trajectories of one sub-step (tracers.c:2132-2251), per wet cell:
+ * slots 0..2 the x, y, z faces of the cell, 3..5 its centre */
+__global__ void k_ffsl_traj(emstr_dev d, double dt, int *err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (k < kbc) return;
+  const ll idx = (ll)k * d.plane + col;
+  const ll N3 = d.n3;
+  const ll c = ppack(idx, P_WET);
+  ll cl;
+  double crf;
+  /* centres */
+  const double nu = d.fnu[idx], nv = d.fnv[idx], nw = d.fnw[idx];
+  cl = (nu < 0.0) ? p_xp1(d, c) : c;
+  crf = 0.0;
+  ff_hor<0>(d, nu, dt, d.h1au1, cl, crf, err);
+  d.fcl[3 * N3 + idx] = cl;
+  d.fcrf[3 * N3 + idx] = crf;
+  cl = (nv < 0.0) ? p_yp1(d, c) : c;
+  crf = 0.0;
+  ff_hor<1>(d, nv, dt, d.h2au2, cl, crf, err);
+  d.fcl[4 * N3 + idx] = cl;
+  d.fcrf[4 * N3 + idx] = crf;
+  cl = (nw < 0.0) ? p_zp1(d, c) : c;
+  crf = 0.0;
+  ff_vert(d, c, nw, dt, d.fw9, cl, crf, err);
+  d.fcl[5 * N3 + idx] = cl;
+  d.fcrf[5 * N3 + idx] = crf;
+  /* faces: west and south faces of the cell, and its bottom face when it
+   * is a cell to process */
+  const double u1 = d.u1[idx], u2 = d.u2[idx];
+  cl = (u1 > 0.0) ? p_xm1(d, c) : c;
+  crf = 0.0;
+  ff_hor<0>(d, u1, dt, d.h1acell, cl, crf, err);
+  d.fcl[idx] = cl;
+  d.fcrf[idx] = crf;
+  cl = (u2 > 0.0) ? p_ym1(d, c) : c;
+  crf = 0.0;
+  ff_hor<1>(d, u2, dt, d.h2acell, cl, crf, err);
+  d.fcl[N3 + idx] = cl;
+  d.fcrf[N3 + idx] = crf;
+  crf = 0.0;
+  cl = c;
+  const int ksc = d.ks[col];
+  if (ksc >= 0 && k <= ksc) {
+    const double w = d.w[idx];
+    cl = (w > 0.0) ? p_zm1(d, c) : c;
+    ff_vert(d, c, w, dt, d.dz, cl, crf, err);
+  }
+  d.fcl[2 * N3 + idx] = cl;
+  d.fcrf[2 * N3 + idx] = crf;
+}
+
+/* interpolation of a along the trajectory (clc, crfc) of a cell in the
+ * direction of velocity v (the transverse terms of tracers.c:2508-2565) */
+template <int DIR>
+__device__ __forceinline__ double ff_tv(const emstr_dev &d, const double *a, const double *t,
+                                        const double *ts, double v, ll clc, double crfc) {
+  ll up1, cp;
+  if (DIR == 0) {
+    up1 = (v < 0.0) ? clc : p_xm1(d, clc);
+    cp = (v < 0.0) ? p_xm1(d, up1) : clc;
+  } else if (DIR == 1) {
+    up1 = (v < 0.0) ? clc : p_ym1(d, clc);
+    cp = (v < 0.0) ? p_ym1(d, up1) : clc;
+  } else {
+    up1 = (v < 0.0) ? clc : p_zm1(d, clc);
+    cp = (v < 0.0) ? p_zm1(d, up1) : clc;
+  }
+  const double acp = p_val(a, t, ts, cp);
+  return acp + (crfc * (p_val(a, t, ts, up1) - acp));
+}
+
+/* one-dimensional transverse tracer along direction DIR with the
+ * monotone clamp (tracers.c:2508-2546, monoco :2717-2726) */
+template <int DIR>
+__device__ __forceinline__ double ff_mod1(const emstr_dev &d, const double *t, const double *ts,
+                                          double v, ll clc, double crfc) {
+  ll up1, cp;
+  if (DIR == 0) {
+    up1 = (v < 0.0) ? clc : p_xm1(d, clc);
+    cp = (v < 0.0) ? p_xm1(d, up1) : clc;
+  } else if (DIR == 1) {
+    up1 = (v < 0.0) ? clc : p_ym1(d, clc);
+    cp = (v < 0.0) ? p_ym1(d, up1) : clc;
+  } else {
+    up1 = (v < 0.0) ? clc : p_zm1(d, clc);
+    cp = (v < 0.0) ? p_zm1(d, up1) : clc;
+  }
+  const double tcp = p_val(t, t, ts, cp), tup = p_val(t, t, ts, up1);
+  double r = tcp;
+  if (crfc > 0.) r += (crfc * (tup - tcp));
+  const double mn = fmn(tcp, tup), mx = fmx(tcp, tup);
+  r = fmx(mn * 1.0, r);
+  r = fmn(mx * 1.0, r);
+  return r;
+}
+
+/* the transverse tracers of one tracer, per wet cell */
+__global__ void k_ffsl_mod1(emstr_dev d, const double *__restrict__ t,
+                            const double *__restrict__ ts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  if (k < d.kb[col]) return;
+  const ll idx = (ll)k * d.plane + col;
+  const ll N3 = d.n3;
+  d.ftmx[idx] = ff_mod1<0>(d, t, ts, d.fnu[idx], d.fcl[3 * N3 + idx], d.fcrf[3 * N3 + idx]);
+  d.ftmy[idx] = ff_mod1<1>(d, t, ts, d.fnv[idx], d.fcl[4 * N3 + idx], d.fcrf[4 * N3 + idx]);
+  d.ftmz[idx] = ff_mod1<2>(d, t, ts, d.fnw[idx], d.fcl[5 * N3 + idx], d.fcrf[5 * N3 + idx]);
+}
+
+/* the modified tracer for the fluxes along DIR (tracers.c:2568-2690) */
+template <int DIR>
+__global__ void k_ffsl_mod(emstr_dev d, const double *__restrict__ t,
+                           const double *__restrict__ ts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  if (k < d.kb[col]) return;
+  const ll idx = (ll)k * d.plane + col;
+  const ll N3 = d.n3;
+  const double onesixth = 1.0 / 6.0;
+  const double tc = t[idx];
+  double tm = tc;
+  if (DIR == 0) {
+    tm += (tc + d.ftmy[idx] + d.ftmz[idx]);
+    tm += ff_tv<2>(d, d.ftmy, t, ts, d.fnw[idx], d.fcl[5 * N3 + idx], d.fcrf[5 * N3 + idx]);
+    tm += ff_tv<1>(d, d.ftmz, t, ts, d.fnv[idx], d.fcl[4 * N3 + idx], d.fcrf[4 * N3 + idx]);
+  } else if (DIR == 1) {
+    tm += (tc + d.ftmx[idx] + d.ftmz[idx]);
+    tm += ff_tv<2>(d, d.ftmx, t, ts, d.fnw[idx], d.fcl[5 * N3 + idx], d.fcrf[5 * N3 + idx]);
+    tm += ff_tv<0>(d, d.ftmz, t, ts, d.fnu[idx], d.fcl[3 * N3 + idx], d.fcrf[3 * N3 + idx]);
+  } else {
+    tm += (tc + d.ftmx[idx] + d.ftmy[idx]);
+    tm += ff_tv<0>(d, d.ftmy, t, ts, d.fnu[idx], d.fcl[3 * N3 + idx], d.fcrf[3 * N3 + idx]);
+    tm += ff_tv<1>(d, d.ftmx, t, ts, d.fnv[idx], d.fcl[4 * N3 + idx], d.fcrf[4 * N3 + idx]);
+  }
+  tm *= onesixth;
+  d.ftm[idx] = tm;
+}
+
+/* van Leer face value on the modified tracer a (tracers.c:2868-2910) */
+template <int DIR>
+__device__ __forceinline__ double ff_vl(const emstr_dev &d, const double *a, const double *t,
+                                        const double *ts, ll clc, double vel, double crf) {
+  ll cp = clc, bm1, bm2, fp = 0;
+  if (DIR == 0) {
+    if (vel > 0.0) cp = p_xp1(d, cp);
+    bm1 = p_xm1(d, cp);
+    bm2 = p_xm1(d, bm1);
+    if (vel < 0.0) fp = p_xp1(d, cp);
+  } else if (DIR == 1) {
+    if (vel > 0.0) cp = p_yp1(d, cp);
+    bm1 = p_ym1(d, cp);
+    bm2 = p_ym1(d, bm1);
+    if (vel < 0.0) fp = p_yp1(d, cp);
+  } else {
+    if (vel > 0.0) cp = p_zp1(d, cp);
+    bm1 = p_zm1(d, cp);
+    bm2 = p_zm1(d, bm1);
+    if (vel < 0.0) fp = p_zp1(d, cp);
+  }
+  double df;
+  const double acp = p_val(a, t, ts, cp), abm1 = p_val(a, t, ts, bm1);
+  if (vel > 0.0) {
+    const double abm2 = p_val(a, t, ts, bm2);
+    df = (abm1 - abm2) * (acp - abm1);
+    if (df > 0.0)
+      df = 2.0 * df / (acp - abm2);
+    else
+      df = 0.0;
+    return abm1 + 0.5 * (1.0 - crf) * df;
+  }
+  const double afp = p_val(a, t, ts, fp);
+  df = (acp - abm1) * (afp - acp);
+  if (df > 0.0)
+    df = 2.0 * df / (afp - abm1);
+  else
+    df = 0.0;
+  return acp - 0.5 * (1.0 - crf) * df;
+}
+
+/* fluxes along DIR of one tracer from the modified tracer ftm
+ * (tracers.c:2578-2705): the fractional part by van Leer, the whole
+ * cells of the trajectory by their integrals, times the volume flux
+ * (horizontal) or with the sign of w (vertical) */
+template <int DIR>
+__global__ void k_ffsl_flux(emstr_dev d, const double *__restrict__ t,
+                            const double *__restrict__ ts, double dt, int *err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (k < kbc) return;
+  const ll idx = (ll)k * d.plane + col;
+  const ll N3 = d.n3;
+  const ll c = ppack(idx, P_WET);
+  const double *a = d.ftm;
+  const ll clc = d.fcl[DIR * N3 + idx];
+  const double crf = d.fcrf[DIR * N3 + idx];
+  double F = 0.0;
+  if (DIR < 2) {
+    const double vel = (DIR == 0) ? d.u1[idx] : d.u2[idx];
+    const double *h = (DIR == 0) ? d.h1acell : d.h2acell;
+    if (vel != 0.0) {
+      if (crf != 0.0) F = ff_vl<DIR>(d, a, t, ts, clc, vel, crf);
+      F *= (crf * h[p_col(d, clc)]);
+      ll cp = (vel > 0.0) ? ((DIR == 0) ? p_xm1(d, c) : p_ym1(d, c)) : c;
+      double dx = h[p_col(d, cp)];
+      const double ltraj = fabs(vel * dt);
+      double lrem = ltraj;
+      int it = 0;
+      while (dx < lrem) {
+        F += (dx * p_val(a, t, ts, cp));
+        lrem -= dx;
+        if (DIR == 0) cp = (vel < 0) ? p_xp1(d, cp) : p_xm1(d, cp);
+        else cp = (vel < 0) ? p_yp1(d, cp) : p_ym1(d, cp);
+        const double hh = h[p_col(d, cp)];
+        if (hh > 0) dx = hh;
+        if (++it > FF_MAXWALK) {
+          atomicExch(err, 1);
+          break;
+        }
+      }
+      F *= (((DIR == 0) ? d.u1flux3d[idx] : d.u2flux3d[idx]) / ltraj);
+    }
+    if (DIR == 0) d.ffx[idx] = F;
+    else d.ffy[idx] = F;
+    return;
+  }
+  const int ksc = d.ks[col];
+  if (ksc >= 0 && k <= ksc) {
+    const double w = d.w[idx];
+    if (w != 0.0) {
+      if (crf != 0.0) F = ff_vl<2>(d, a, t, ts, clc, w, crf);
+      F *= (crf * p_f3(d.dz, clc));
+      ll cp = (w > 0.0) ? p_zm1(d, c) : c;
+      double dz = fabs(p_f3(d.dz, cp));
+      double lrem = fabs(w * dt);
+      int it = 0;
+      while (dz < lrem) {
+        F += (dz * p_val(a, t, ts, cp));
+        lrem -= dz;
+        cp = (w < 0.0) ? p_zp1(d, cp) : p_zm1(d, cp);
+        const double hh = p_f3(d.dz, cp);
+        if (hh > 0) dz = hh;
+        if (++it > FF_MAXWALK) {
+          atomicExch(err, 1);
+          break;
+        }
+      }
+      if (w < 0) F *= -1;
+    }
+  }
+  d.ffz[idx] = F;
+}
+
+extern "C" int emstr_launch_ffsl_traj(const emstr_dev *d, double dtu, int *err,
+                                      void *stream) {
+  dim3 b(64, 4, 1);
+  dim3 g = grid3(d, b);
+  hipStream_t st = (hipStream_t)stream;
+  k_ffsl_vel<<<g, b, 0, st>>>(*d);
+  k_ffsl_traj<<<g, b, 0, st>>>(*d, dtu, err);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int emstr_launch_ffsl_fluxes(const emstr_dev *d, const double *t,
+                                        const double *ts, double dtu, int *err,
+                                        void *stream) {
+  dim3 b(64, 4, 1);
+  dim3 g = grid3(d, b);
+  hipStream_t st = (hipStream_t)stream;
+  k_ffsl_mod1<<<g, b, 0, st>>>(*d, t, ts);
+  k_ffsl_mod<0><<<g, b, 0, st>>>(*d, t, ts);
+  k_ffsl_flux<0><<<g, b, 0, st>>>(*d, t, ts, dtu, err);
+  k_ffsl_mod<1><<<g, b, 0, st>>>(*d, t, ts);
+  k_ffsl_flux<1><<<g, b, 0, st>>>(*d, t, ts, dtu, err);
+  k_ffsl_mod<2><<<g, b, 0, st>>>(*d, t, ts);
+  k_ffsl_flux<2><<<g, b, 0, st>>>(*d, t, ts, dtu, err);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 /*-------------------------------------------------------------------*/
@@ -816,17 +1303,22 @@ __global__ void k_dtm(emstr_dev d, double dtu0, unsigned long long *out) {
   const ll idx = (ll)k * d.plane + col;
   const double sf = 0.8, minval = 1e-10;
   double dtm = dtu0;
+  const bool ffsl = (d.trasc == FFSL);
   double d4 = 0.5 * (d.u1[idx] + d.u1[idx + 1]);
+  /* FFSL is limited by the velocity gradients (Lipschitz number) */
+  if (ffsl) d4 = fmx(fabs(d.u1[idx + 1] - d.u1[idx]), d.u1kh[idx] / d.h1acell[col]);
   if (fabs(d4) < minval) d4 = minval;
   double d1 = sf * fabs(d.h1acell[col] / d4);
   if (d1 < dtm) dtm = d1;
   d4 = 0.5 * (d.u2[idx] + d.u2[idx + d.nx]);
+  if (ffsl) d4 = fmx(fabs(d.u2[idx + d.nx] - d.u2[idx]), d.u2kh[idx] / d.h2acell[col]);
   if (fabs(d4) < minval) d4 = minval;
   double d2 = sf * fabs(d.h2acell[col] / d4);
   if (d2 < dtm) dtm = d2;
   const bool surf = (k == ksc);
   const ll zp = (k + 1 < d.nz) ? idx + d.plane : idx;
   d4 = surf ? 0.0 : 0.5 * (d.w[idx] + d.w[zp]);
+  if (ffsl) d4 = surf ? 0.0 : fabs(d.w[zp] - d.w[idx]);
   if (fabs(d4) < minval) d4 = minval;
   double d3 = surf ? dtu0 : sf * fabs(d.dz[idx] * 1.0 / d4);
   if (d3 > 0.0 && d3 < dtm) dtm = d3;
@@ -916,28 +1408,31 @@ static dim3 grid3(const emstr_dev *d, dim3 b) {
 
 extern "C" int emstr_launch_adv(const emstr_dev *d, const double *tin,
                                 double *tout, const double *tstart, double dtu,
-                                double trem, int anydif, void *stream) {
+                                double trem, int anydif, int n0, int n1, void *stream) {
   dim3 b(64, 4, 1);
   dim3 g = grid3(d, b);
   hipStream_t st = (hipStream_t)stream;
   switch (d->trasc) {
+    case FFSL:
+      k_adv<FFSL><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
+      break;
     case VANLEER:
-      k_adv<VANLEER><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      k_adv<VANLEER><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
       break;
     case QUICKEST:
-      k_adv<QUICKEST><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      k_adv<QUICKEST><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
       break;
     case QUICKEST_CO:
-      k_adv<QUICKEST_CO><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      k_adv<QUICKEST_CO><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
       break;
     case ORDER1:
-      k_adv<ORDER1><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      k_adv<ORDER1><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
       break;
     case ORDER2:
-      k_adv<ORDER2><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      k_adv<ORDER2><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
       break;
     case ORDER4:
-      k_adv<ORDER4><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif);
+      k_adv<ORDER4><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
       break;
     default:
       return -1;

@@ -78,6 +78,70 @@ def test_step_bit_exact_small(name, kw):
     assert_exact(gpu, ora)
 
 
+FFSL_CASES = [
+    # Courant numbers below 1
+    ("ffsl", dict(dt=120.0), 0, 4),
+    # horizontal trajectories over up to 6 cells, vertical over several layers
+    ("ffsl_long_trajectories", dict(dt=2400.0, vmax=1.2), 0, 2),
+    # Lipschitz sub-step: 18 sub-steps, lateral ghosts stale after the first
+    ("ffsl_substeps", dict(dt=3600.0, vmax=1.2), S.SUB_STEP, 2),
+    # 120 sub-steps: beyond the 20 of the other schemes (itermax 200)
+    ("ffsl_many_substeps", dict(dt=14400.0, vmax=1.5), S.SUB_STEP, 1),
+]
+
+
+@pytest.mark.parametrize("name,kw,stab,nsteps", FFSL_CASES)
+def test_ffsl_bit_exact_small(name, kw, stab, nsteps):
+    say("case small", name)
+    case = configs.small(seed=21, ntr=3, trasc=S.FFSL, **kw)
+    case.wd.nstep = 1
+    case.wc.stab = stab
+    case.prepare(O.set_dz)
+    t0 = case.tracers()
+    gpu, ora = run_both(case, nsteps)
+    assert np.abs(ora - t0).max() > 1e-3  # the step moved the tracers
+    assert_exact(gpu, ora)
+
+
+def test_ffsl_first_step_and_diffusion():
+    """nstep 0 drops the horizontal divergence (tracers.c:902-903); with
+    horizontal diffusion on, the FFSL sub-step also reads u1kh/u2kh."""
+    say("case small ffsl nstep 0 + diffusion")
+    case = configs.small(seed=8, ntr=2, trasc=S.FFSL, dt=600.0)
+    case.wd.nstep = 0
+    S.view(case.wc.diffuse, 2, np.int32)[:] = [1, 0]
+    rng = np.random.default_rng(3)
+    case.a3(case.wc.u1kh)[:] = rng.uniform(0.0, 50.0, case.sgsiz)
+    case.a3(case.wc.u2kh)[:] = rng.uniform(0.0, 50.0, case.sgsiz)
+    case.wc.stab = S.SUB_STEP
+    case.prepare(O.set_dz)
+    gpu, ora = run_both(case, 1)
+    assert_exact(gpu, ora)
+    case.wd.nstep = 1
+    gpu, ora = run_both(case, 2)
+    assert_exact(gpu, ora)
+
+
+def test_ffsl_moving_surface():
+    say("case small ffsl moving surface")
+    case = configs.small(seed=13, ntr=2, nz=6, dmin=2.0, dmax=30.0, trasc=S.FFSL,
+                         dt=600.0)
+    case.wd.nstep = 1
+    rng = np.random.default_rng(4)
+    g = case.g
+    w2 = S.view(g.w2_t, case.sgsizS, np.int32)[1:g.v2_t + 1]
+    dl = case.layers[1] - case.layers[0]
+    old = case.a2(case.wc.oldeta)
+    eta = case.a2(case.wd.eta)
+    old[w2] = rng.uniform(-1.3, 0.0, len(w2)) * dl
+    eta[w2] = old[w2]
+    O.set_dz(case.gp, case.wp, case.pp)
+    eta[w2] = old[w2] + rng.uniform(-0.9, 0.9, len(w2)) * dl
+    case.prepare(O.set_dz)
+    gpu, ora = run_both(case, 1)
+    assert_exact(gpu, ora)
+
+
 def test_step_bit_exact_c1():
     say("case c1 20 steps")
     case = configs.c1()
