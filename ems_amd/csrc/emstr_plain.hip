@@ -36,6 +36,7 @@
 #define QUICKEST 0x000020
 #define QUICKEST_CO 0x000040
 #define VANLEER 0x000080
+#define ORDER2_UW 0x002000
 #define FFSL 0x080000
 
 typedef long long ll;
@@ -264,6 +265,12 @@ __device__ __forceinline__ double face_flux(double bm2, double bm1, double c,
     F = f.mgs * (c - bm1) + bm1;
   } else if (SCH == ORDER4) {
     F = (7.0 * (bm1 + c) - (fp1 + bm2)) / 12.0;
+  } else if (SCH == ORDER2_UW) {
+    /* tracers.c:3024-3030 */
+    if (f.vel > 0.0)
+      F = 0.5 * (3.0 - f.cn) * bm1 - 0.5 * (1.0 - f.cn) * bm2;
+    else if (f.vel < 0.0)
+      F = 0.5 * (3.0 - f.cn) * c - 0.5 * (1.0 - f.cn) * fp1;
   }
   if (ultimate_on) F = ultimate(f.cn, F, bm2, bm1, c, fp1);
   return F * f.flux;
@@ -1433,6 +1440,9 @@ extern "C" int emstr_launch_adv(const emstr_dev *d, const double *tin,
       break;
     case ORDER4:
       k_adv<ORDER4><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
+      break;
+    case ORDER2_UW:
+      k_adv<ORDER2_UW><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
       break;
     default:
       return -1;

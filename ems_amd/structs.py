@@ -86,6 +86,7 @@ ORDER4 = 0x000010
 QUICKEST = 0x000020
 QUICKEST_CO = 0x000040
 VANLEER = 0x000080
+ORDER2_UW = 0x002000
 FFSL = 0x080000
 SUB_STEP = 2
 NONE = 0x80000000 - (1 << 32)  # as a signed C int

@@ -350,6 +350,36 @@ static void o4_faces(geometry_t *g, win_priv_t *wc, const double *tr,
   o4_do(Fz, tr, 1, wc->vc, wc->s1, g->zm1, g->zp1);
 }
 
+/* tracers.c:3010-3036  second-order upwind face value (Leonard 1994) */
+static void o2uw_do(double *F, const double *tr, const double *vel, const double *cn,
+                    int ss, int se, const int *sdo, const int *fmap, const int *bmap) {
+  int cc;
+  for (cc = ss; cc <= se; cc++) {
+    int c = sdo[cc];
+    int bm1 = bmap[c];
+    int bm2 = bmap[bm1];
+    if (vel[c] > 0.0)
+      F[c] = 0.5 * (3.0 - cn[c]) * tr[bm1] - 0.5 * (1.0 - cn[c]) * tr[bm2];
+    else if (vel[c] < 0.0)
+      F[c] = 0.5 * (3.0 - cn[c]) * tr[c] - 0.5 * (1.0 - cn[c]) * tr[fmap[c]];
+    else
+      F[c] = 0.0;
+  }
+}
+
+/* tracers.c:2963-2997  order2_upwind: wet faces, auxiliary faces, s1 */
+static void o2uw_faces(geometry_t *g, window_t *wd, win_priv_t *wc, const double *tr,
+                       double *Fx, double *Fy, double *Fz) {
+  memset(Fx, 0, g->n3_t * sizeof(double));
+  o2uw_do(Fx, tr, wd->u1, wc->w6, 1, g->v3_e1, g->w3_e1, g->xp1, g->xm1);
+  o2uw_do(Fx, tr, wd->u1, wc->w6, g->v3_t + 1, g->a3_t, g->w3_t, g->xp1, g->xm1);
+  memset(Fy, 0, g->n3_t * sizeof(double));
+  o2uw_do(Fy, tr, wd->u2, wc->w7, 1, g->v3_e2, g->w3_e2, g->yp1, g->ym1);
+  o2uw_do(Fy, tr, wd->u2, wc->w7, g->v3_t + 1, g->a3_t, g->w3_t, g->yp1, g->ym1);
+  memset(Fz, 0, g->n3_t * sizeof(double));
+  o2uw_do(Fz, tr, wd->w, wc->w8, 1, wc->vc, wc->s1, g->zp1, g->zm1);
+}
+
 /*-------------------------------------------------------------------*/
 /* tracers.c:3501-3557  ULTIMATE limiter on face values               */
 /*-------------------------------------------------------------------*/
@@ -746,6 +776,8 @@ static void oracle_advect(geometry_t *g, window_t *wd, win_priv_t *wc,
     qku_faces(g, wc, tr, Fx, Fy, Fz);
   else if (wc->trasc == VANLEER)
     vl_faces(g, wd, wc, tr, Fx, Fy, Fz);
+  else if (wc->trasc == ORDER2_UW)
+    o2uw_faces(g, wd, wc, tr, Fx, Fy, Fz);
 
   if (wc->ultimate) {
     ultimate(wc->w6, Fx, tr, g->w3_e1, 1, g->v3_e1, g->xm1, g->xp1);

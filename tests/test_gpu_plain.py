@@ -66,6 +66,8 @@ CASES = [
     ("order1", dict(trasc=S.ORDER1)),
     ("order2", dict(trasc=S.ORDER2)),
     ("order4", dict(trasc=S.ORDER4)),
+    ("order2_upwind", dict(trasc=S.ORDER2_UW)),
+    ("order2_upwind+ultimate", dict(trasc=S.ORDER2_UW, ultimate=1)),
 ]
 
 

@@ -93,7 +93,8 @@ def test_set_dz_brute_force_moving_surface():
 # ---------------------------------------------------------------------
 # advection: exact translation of a linear profile (all schemes)
 # ---------------------------------------------------------------------
-SCHEMES = [S.VANLEER, S.QUICKEST, S.QUICKEST_CO, S.ORDER1, S.ORDER2, S.ORDER4]
+SCHEMES = [S.VANLEER, S.QUICKEST, S.QUICKEST_CO, S.ORDER1, S.ORDER2, S.ORDER4,
+           S.ORDER2_UW]
 
 
 def _channel(U, scheme, ultimate=0, dt=60.0, stab=None):
@@ -122,6 +123,32 @@ def test_linear_profile_translates_exactly(scheme, U):
     inner = slice(4, case.nx - 4)
     got = t[:, 1:-1, inner]
     want = np.broadcast_to(exp[inner], got.shape)
+    assert np.abs(got - want).max() < 1e-12 * 6.0
+
+
+@pytest.mark.parametrize("U", [0.4, -0.3])
+def test_order2_upwind_is_mean_of_two_cell_extrapolation(U):
+    """order2_upwind (tracers.c:3010-3036) on a quadratic profile: each
+    face value is the straight line through the two cells on the
+    upstream side (for u < 0 the reference pairs c with xp1 and keeps the
+    signed Courant number), evaluated half a sweep length from the face,
+    at x_f - |u| dt / 2.  Built here with numpy.polyfit."""
+    case, x, a = _channel(U, S.ORDER2_UW)
+    h, dt, nx = 1000.0, case.dt, case.nx
+    t0 = 5.0 + 2e-6 * (x - 9000.0) ** 2
+    H.set_tracer_dense(case, 0, np.broadcast_to(t0, (case.nz, case.ny, nx)))
+    assert O.advect_diffuse(case) == 0
+    t = H.tracer_dense(case, 0)
+
+    def face(i):  # west face of cell i
+        xf = x[i] - 0.5 * h
+        pts = [i - 2, i - 1] if U > 0 else [i, i + 1]
+        line = np.polyfit(x[pts], t0[pts], 1)
+        return np.polyval(line, xf - 0.5 * abs(U) * dt)
+
+    inner = range(4, nx - 4)
+    want = np.array([t0[i] - U * dt / h * (face(i + 1) - face(i)) for i in inner])
+    got = t[:, 1:-1, 4:nx - 4]
     assert np.abs(got - want).max() < 1e-12 * 6.0
 
 
