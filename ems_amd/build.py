@@ -40,6 +40,34 @@ def _run(cmd):
     return r
 
 
+# libemstr_us.so: the COMPAS path (unstructured meshes)
+US_HIP_SRC = []
+US_C_SRC = ["emstr_mesh.c"]
+
+
+def build_us(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    hipcc = os.path.join(ROCM, "bin", "hipcc")
+    for s in US_HIP_SRC:
+        o = os.path.join(BUILD, "us_" + s.replace(".hip", ".o"))
+        _run([hipcc] + HIPFLAGS + HIP_EXTRA.get(s, []) + INC +
+             ["-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    for s in US_C_SRC:
+        o = os.path.join(BUILD, "us_" + s.replace(".c", ".o"))
+        _run(["gcc"] + CFLAGS + INC + ["-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    out = os.path.join(HERE, "libemstr_us.so")
+    tmp = out + ".tmp"
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-lpthread", "-lm"])
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
@@ -65,3 +93,4 @@ def build(verbose=False):
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_us(verbose=True)

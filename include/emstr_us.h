@@ -1,0 +1,173 @@
+/*
+ * libemstr_us: the tracer step of EMS COMPAS (unstructured meshes,
+ * model/hd-us) on one MI355X, as a C-ABI shared library.
+ *
+ * The structures carry the fields of the COMPAS geometry_t, window_t
+ * and win_priv_t (model/hd-us/include/sparse.h:880-1213, 2393-2567,
+ * 246-807) that the tracer path reads, with the same meaning: 1-based
+ * arrays (index 0 unused), cell centres numbered surface layer first,
+ * edges of the surface layer first, ghosts after the wet cells.  COMPAS
+ * is a separate model from SHOC (its own executable), so this library
+ * is separate from libemstr and exports the same entry-point names;
+ * include one of emstr.h / emstr_us.h, not both.
+ */
+#ifndef EMSTR_US_H
+#define EMSTR_US_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Advection scheme flags: model/hd-us/include/hd_params.h */
+#define ORDER1 0x000002
+#define ORDER2 0x000004
+#define QUICKEST 0x000020
+#define VANLEER 0x000080
+#define LAGRANGE 0x000100
+#define FFSL 0x080000
+
+/* Stability flags */
+#define SUB_STEP 2
+#define SUB_STEP_NOSURF 4
+#define SUB_STEP_TRACER 16
+#define MULTI_DT 32
+#ifndef NONE
+#define NONE 0x80000000
+#endif
+
+/* Fraction of hmin above the bed for a column to be processed */
+#define DRY_FRAC 0.05
+
+/*-------------------------------------------------------------------*/
+/* Mesh structure (fields of the COMPAS geometry_t)                  */
+/*-------------------------------------------------------------------*/
+typedef struct geometry {
+  int szc, sze;     /* 3D cell-centre / edge array sizes              */
+  int szcS, szeS;   /* 2D sizes (surface layer)                       */
+  int nobc;         /* open boundaries (0: closed basin)              */
+  int nz;           /* number of layers                               */
+  int npem;         /* maximum edges per cell                         */
+  int wn;           /* window number                                  */
+  double *layers;   /* layer faces, layers[0..nz] (bottom to top)     */
+  int *s2i, *s2k;   /* centre -> mesh cell, layer                     */
+
+  /* Connectivity.  An edge e separates e2c[e][0] and e2c[e][1]; a
+   * positive u1[e] flows from e2c[e][1] into e2c[e][0]; e2e[e][0] is
+   * the index of e among the edges of e2c[e][0], e2e[e][1] among those
+   * of e2c[e][1].  c2c[j][c] is the centre across edge j of c (a ghost
+   * at land), c2e[j][c] that edge; eSc[j][c2] = 1 if edge j points out
+   * of the cell (c2 == e2c[e][1]), else -1.                          */
+  int *npe;         /* [szcS] edges of each cell                      */
+  int **c2c;        /* [npem+1][szc]                                  */
+  int **c2e;        /* [npem+1][szc]                                  */
+  int **e2c;        /* [sze][2]                                       */
+  int **e2e;        /* [sze][2]                                       */
+  int **eSc;        /* [npem+1][szcS]                                 */
+  int *ep, *em;     /* [sze] edge beyond e2c[e][0] / e2c[e][1] with
+                     * the index of e (pp_us.c:1272-1282, e2e())       */
+  int *m2d;         /* [szc] centre -> surface centre                 */
+  int *m2de;        /* [sze] edge -> surface edge                     */
+  int *zp1, *zm1;   /* [szc] vertical maps (self-mapping at the ends) */
+  int *zp1e, *zm1e; /* [sze]                                          */
+
+  /* Cells and edges to process: [1..v3] wet, [..b3] +OBC, [..a3]
+   * +auxiliary, [..n3] +ghosts                                       */
+  int *w3_t, *w3_e1;
+  int n3_t, v3_t, b3_t, a3_t;
+  int n3_e1, v3_e1, b3_e1, a3_e1;
+  int *w2_t, *w2_e1;
+  int n2_t, v2_t, b2_t, a2_t;
+  int n2_e1, v2_e1, b2_e1, a2_e1;
+
+  /* Ghosts: lateral tr[bpt[cc]] = tr[bin[cc]]; sediment below beds  */
+  int nbpt;
+  int *bpt, *bin;
+  int ngsed;
+  int *gsed_t, *ised_t;
+
+  int *sur_t, *nsur_t, *bot_t; /* [szcS] surface / bed centres        */
+
+  /* Metrics                                                          */
+  double *h1au1;    /* [szeS] edge length                             */
+  double *h2au1;    /* [szeS] distance between the centres of an edge */
+  double *cellarea; /* [szcS]                                         */
+  double *costhu1, *sinthu1; /* [szeS] edge normal direction          */
+  double *cellx, *celly;     /* [szcS] centre coordinates             */
+  double *botz;     /* [szcS] bed height (negative down)              */
+  double *gridz;    /* [szc] height of the lower face of each cell    */
+  double *cellz;    /* [szc] height of the cell centre                */
+} geometry_t;
+
+/*-------------------------------------------------------------------*/
+/* Window data (fields of the COMPAS window_t)                       */
+/*-------------------------------------------------------------------*/
+typedef struct window {
+  int ntr;
+  double **tr_wc;   /* tr_wc[n][1..szc-1]                             */
+  double dt, dttr, t;
+  int nstep;
+  double *u1;       /* [sze] edge-normal velocity                     */
+  double *u1flux3d; /* [sze] edge volume flux (m3/s)                  */
+  double *dzu1;     /* [sze] edge face thickness                      */
+  double *u, *v;    /* [szc] cell-centred east / north velocity       */
+  double *w;        /* [szc] vertical velocity on the lower face      */
+  double *Kz;       /* [szc] vertical diffusivity, lower face         */
+  double *eta;      /* [szcS] elevation at t+1                        */
+  double *topz;     /* [szcS]                                         */
+} window_t;
+
+/*-------------------------------------------------------------------*/
+/* Private window data (fields of the COMPAS win_priv_t)             */
+/*-------------------------------------------------------------------*/
+typedef struct win_priv {
+  int trasc, ultimate, stab, sigma, thin_merge;
+  int rkstage;      /* Runge-Kutta stages (1: forward Euler)          */
+  int means;        /* mean-velocity transport flags (0 here)         */
+  double hmin;
+  double trsf;      /* FFSL sub-step safety factor                    */
+  int *advect, *diffuse;
+  int *tbdy, ntbdy;
+  int *tdif_v, ntdif_v;
+  double *mintr, *maxtr;
+
+  double *oldeta;   /* [szcS]                                         */
+  double *dz;       /* [szc] cell thickness (set_dz)                  */
+  double *Ds, *Hn1, *Hs; /* [szcS] sigma scaling (1 for z-levels)     */
+  double *u1kh;     /* [szc] cell-centred horizontal diffusivity      */
+  double *u2kh;     /* [sze] edge diffusivity used by hor_diffuse     */
+  double *mdx;      /* [szeS] metric factor                           */
+  double *one;      /* [szcS] ones (diffusion scale)                  */
+
+  int *s1;
+  int vc, vcs, vc1, vcs1;
+  int *i1, *i2, *i3, *i4, *i5;
+
+  double *w1, *w2, *w3, *w4, *w5, *w6, *w7, *w8, *w9, *w10; /* [max(szc,sze)] */
+  double *d1, *d2, *d3, *d4;              /* [max(szcS,szeS)]         */
+  double *v1, *v2, *v3, *v4, *v5, *v6, *v7; /* [nz+1]                 */
+  double b1;        /* current sub-step                               */
+} win_priv_t;
+
+/*-------------------------------------------------------------------*/
+/* Drop-in entry points (COMPAS signatures)                          */
+/*-------------------------------------------------------------------*/
+/* model/hd-us/tracers/tracers.c:261 */
+void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon);
+/* model/hd-us/tracers/tracers.c:428; 0 on success, else the column
+ * (2D centre) where the sub-step limit was exceeded (tracers.c:688) */
+int advect_diffuse(geometry_t *window, window_t *windat, win_priv_t *wincon);
+/* model/hd-us/tracers/tracers.c:5562 */
+void vert_diffuse_3d(geometry_t *window, window_t *windat, win_priv_t *wincon);
+/* model/hd-us/tracers/tracers.c:7231 (z-level branch) */
+void set_dz(geometry_t *window, window_t *windat, win_priv_t *wincon);
+
+/* Errors of the last call on this thread ("" when none) */
+const char *emstr_us_last_error(void);
+int emstr_us_last_error_cell(void);
+/* Drop the device mirror of a window (after its arrays are freed) */
+void emstr_us_forget_window(geometry_t *window);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

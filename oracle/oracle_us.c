@@ -1,0 +1,781 @@
+/*
+ * ORACLE -- test infrastructure only.  Not part of the product.
+ *
+ * A CPU restatement, in strict IEEE FP64, of the EMS COMPAS tracer step
+ * (model/hd-us/tracers/tracers.c) for one z-level window with closed
+ * lateral boundaries, no point sources, regions or multi-dt, Runge-Kutta
+ * with one stage (the default), schemes van Leer and first order, with or
+ * without ULTIMATE.  Each function follows the reference function cited
+ * above it with the same loop order and the same order of floating-point
+ * operations.  Built with gcc -O3 -fno-fast-math -ffp-contract=off.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+ * load this library, and only as the checker or the timed CPU baseline.
+ *
+ * How each function is pinned (tests/test_oracle_us.py):
+ *   van_leer (edges)    on a quadrilateral mesh the COMPAS step equals the
+ *                       SHOC oracle's step on the same grid (the edge
+ *                       stencils cm/cp, signs eSc and the update are all
+ *                       exercised); constancy and mass on hexagons
+ *   set_dz              same column walk as SHOC; maps checked on meshes
+ *   surf_concrk +
+ *   runge_kutta (1)     closed-basin mass conservation on hexagons
+ *   tr_bounds, vert_diffuse_3d, implicit_vdiff_tr
+ *                       as SHOC (same maths on the same vertical maps);
+ *                       the quadrilateral equivalence covers them
+ * The reference's own tests hold no golden vectors for this path, and its
+ * executables cannot be built here (no NetCDF), so nothing is compared
+ * against reference output directly.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/emstr_us.h"
+
+#define max(a, b) ((a) > (b) ? (a) : (b))
+#define min(a, b) ((a) < (b) ? (a) : (b))
+
+static int ous_err = 0;
+int oracle_us_last_error(void) { return ous_err; }
+
+/* tracers.c:110-129, the column of rkweights for one stage */
+static const double rkw1[3] = {1.0, 0.0, 1.0};
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:7120-7137  no-gradient lateral boundary: ghost = bin      */
+/*-------------------------------------------------------------------*/
+void oracle_us_set_lateral_BC_tr(geometry_t *g, window_t *wd) {
+  int cc, n;
+  for (cc = 1; cc <= g->nbpt; cc++) {
+    int gb = g->bpt[cc], in = g->bin[cc];
+    for (n = 0; n < wd->ntr; n++) wd->tr_wc[n][gb] = wd->tr_wc[n][in];
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:7231-7380  set_dz, z-level branch (thin_merge not taken) */
+/* Differs from SHOC in two places: a column is processed when its   */
+/* water top is >= (not >) DRY_FRAC*hmin above the bed, and vcs1 is  */
+/* the last processed wet column (cc <= v2_t).                        */
+/*-------------------------------------------------------------------*/
+void oracle_us_set_dz(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int cc, c, cs, zm1, vc, vcs;
+  double top, bot;
+
+  for (cc = 1; cc <= g->a2_t; cc++) {
+    c = cs = g->w2_t[cc];
+    g->sur_t[cc] = g->nsur_t[cc];
+    zm1 = g->zm1[c];
+    while (c != zm1 && g->gridz[c] >= wd->eta[cs]) {
+      c = zm1;
+      zm1 = g->zm1[c];
+    }
+    g->nsur_t[cc] = c;
+  }
+
+  vc = 1;
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    c = max(g->sur_t[cc], g->nsur_t[cc]);
+    cs = g->m2d[c];
+    top = min(wd->eta[cs], wc->oldeta[cs]);
+    bot = DRY_FRAC * wc->hmin + g->botz[cs];
+    if (top >= bot && c != g->zm1[c]) {
+      wc->s1[vc] = c;
+      wc->i1[vc] = g->bot_t[cc];
+      wc->i2[vc] = g->nsur_t[cc];
+      wc->i3[vc] = g->sur_t[cc];
+      if (cc <= g->v2_t) wc->vcs1 = vc;
+      vc++;
+    }
+  }
+  vcs = vc - 1;
+  wc->vcs = vcs;
+
+  for (cc = 1; cc <= vcs; cc++) {
+    c = g->zm1[wc->s1[cc]];
+    zm1 = g->zm1[c];
+    while (c != zm1) {
+      wc->s1[vc] = c;
+      vc++;
+      c = zm1;
+      zm1 = g->zm1[c];
+    }
+    if (cc == wc->vcs1) wc->vc1 = vc - 1;
+  }
+  vc--;
+  wc->vc = vc;
+
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    int c3, c2, cbot;
+    c = c3 = g->sur_t[cc];
+    c2 = g->m2d[c3];
+    cbot = g->bot_t[cc];
+    top = wc->oldeta[c2];
+    while (c3 < cbot) {
+      bot = g->gridz[c3];
+      wc->dz[c3] = top - bot;
+      top = bot;
+      c3 = g->zm1[c3];
+    }
+    wc->dz[cbot] = top - g->botz[c2];
+    c3 = c;
+    while (c > c2) {
+      c = g->zp1[c];
+      wc->dz[c] = wc->dz[c3];
+    }
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:2512-2573  van Leer face value (by cells / by values)    */
+/*-------------------------------------------------------------------*/
+static void vl_tr(double *F, const double *vel, const double *cn, int e, double tp1,
+                  double tp2, double tm1, double tm2) {
+  double df;
+  if (vel[e] > 0.0) {
+    df = (tm1 - tm2) * (tp1 - tm1);
+    if (df > 0.0)
+      df = 2.0 * df / (tp1 - tm2);
+    else
+      df = 0.0;
+    F[e] = tm1 + 0.5 * (1.0 - cn[e]) * df;
+  } else if (vel[e] < 0.0) {
+    df = (tp1 - tm1) * (tp2 - tp1);
+    if (df > 0.0)
+      df = 2.0 * df / (tp2 - tm1);
+    else
+      df = 0.0;
+    F[e] = tp1 - 0.5 * (1.0 + cn[e]) * df;
+  }
+}
+
+/* tracers.c:2368-2421: edges (second cells beyond each side through
+ * the edges opposite), then the vertical faces of s1 */
+static void vl_faces(geometry_t *g, window_t *wd, win_priv_t *wc, const double *tr,
+                     double *Fx, double *Fz) {
+  int ee, cc;
+  memset(Fx, 0, g->sze * sizeof(double));
+  memset(Fz, 0, g->szc * sizeof(double));
+  for (ee = 1; ee <= g->n3_e1; ee++) {
+    int e = g->w3_e1[ee];
+    int c1 = g->e2c[e][0];
+    int c2 = g->e2c[e][1];
+    int cm = g->c2c[g->e2e[e][0]][c2];
+    int cp = g->c2c[g->e2e[e][1]][c1];
+    vl_tr(Fx, wd->u1, wc->w6, e, tr[c1], tr[cp], tr[c2], tr[cm]);
+  }
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int zp1 = g->zp1[c], zm1 = g->zm1[c], zm2 = g->zm1[zm1];
+    vl_tr(Fz, wd->w, wc->w8, c, tr[c], tr[zp1], tr[zm1], tr[zm2]);
+  }
+}
+
+/* tracers.c:1753-1788 */
+static void o1_faces(geometry_t *g, window_t *wd, win_priv_t *wc, const double *tr,
+                     double *Fx, double *Fz, double dtu) {
+  int ee, cc;
+  memset(Fx, 0, g->sze * sizeof(double));
+  for (ee = 1; ee <= g->n3_e1; ee++) {
+    int e = g->w3_e1[ee];
+    int c1 = g->e2c[e][0];
+    int c2 = g->e2c[e][1];
+    Fx[e] = wc->w6[e] * tr[c2] + wc->w7[e] * tr[c1];
+  }
+  memset(Fz, 0, g->szc * sizeof(double));
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int zm1 = g->zm1[c];
+    double *w = wd->w;
+    Fz[c] = dtu * (0.5 * (w[c] + fabs(w[c])) * tr[zm1] + 0.5 * (w[c] - fabs(w[c])) * tr[c]);
+  }
+}
+
+/* tracers.c:3785-3833 */
+static void ult_filter(const double *cr, double *Fx, const double *F, int e, int c,
+                       double trp, int cm1, double trm) {
+  double nc, nu, tc, tu, td, diff;
+  if (cr[e] == 0.0) return;
+  if (cr[e] > 0.0) {
+    tc = F[cm1];
+    tu = trm;
+    td = F[c];
+  } else {
+    tc = F[c];
+    tu = trp;
+    td = F[cm1];
+  }
+  diff = td - tu;
+  nc = nu = 0.0;
+  if (diff) {
+    nc = (tc - tu) / diff;
+    nu = (Fx[e] - tu) / diff;
+  }
+  if (nc >= 0.0 && nc <= 1.0) {
+    if (nu > 1.0)
+      Fx[e] = td;
+    else if (nu < nc)
+      Fx[e] = tu + nc * (td - tu);
+    else if (nu > nc / fabs(cr[e]))
+      Fx[e] = tu + nc * (td - tu) / fabs(cr[e]);
+  } else
+    Fx[e] = tu + nc * (td - tu);
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:1637-1743  advect: face values, ULTIMATE, times fluxes    */
+/*-------------------------------------------------------------------*/
+static void oracle_us_advect(geometry_t *g, window_t *wd, win_priv_t *wc, const double *tr,
+                             double *Fx, double *Fz, double dtu) {
+  int ee, cc;
+  if (wd->dttr == 0.0) return;
+  if (wc->trasc == ORDER1) {
+    o1_faces(g, wd, wc, tr, Fx, Fz, dtu);
+    return;
+  }
+  vl_faces(g, wd, wc, tr, Fx, Fz);
+  if (wc->ultimate) {
+    /* the outer cells through the edge maps ep/em (tracers.c:1697-1700) */
+    for (ee = 1; ee <= g->n3_e1; ee++) {
+      int e = g->w3_e1[ee];
+      int c = g->e2c[e][0];
+      int cm1 = g->e2c[e][1];
+      int cp = g->e2c[g->ep[e]][0];
+      int cm2 = g->e2c[g->em[e]][1];
+      ult_filter(wc->w6, Fx, tr, e, c, tr[cp], cm1, tr[cm2]);
+    }
+    for (cc = 1; cc <= wc->vc; cc++) {
+      int c = wc->s1[cc];
+      int cp = g->zp1[c], cm1 = g->zm1[c], cm2 = g->zm1[cm1];
+      ult_filter(wc->w8, Fz, tr, c, c, tr[cp], cm1, tr[cm2]);
+    }
+  }
+  for (ee = 1; ee <= g->n3_e1; ee++) {
+    int e = g->w3_e1[ee];
+    Fx[e] *= wd->u1flux3d[e];
+  }
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    Fz[c] *= (wd->w[c] * dtu);
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:5383-5449  horizontal diffusion (z-levels: no limiter)    */
+/*-------------------------------------------------------------------*/
+static void oracle_us_hor_diffuse(geometry_t *g, window_t *wd, win_priv_t *wc,
+                                  const double *tr, double *Fx) {
+  int ee;
+  for (ee = 1; ee <= g->a3_e1; ee++) {
+    int e = g->w3_e1[ee];
+    int es = g->m2de[e];
+    int c1 = g->e2c[e][0];
+    int c2 = g->e2c[e][1];
+    double csx = wd->dzu1[e] * g->h1au1[es] * wc->mdx[es];
+    Fx[e] -= (csx * wc->u2kh[e] * (tr[c1] - tr[c2]) / g->h2au1[es]);
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:1170-1214  runge_kutta, one stage                         */
+/*-------------------------------------------------------------------*/
+static double rk1(win_priv_t *wc, int c2, double ovol, double nvol, double ntr0,
+                  double grad, int slf) {
+  double sf = (slf) ? wc->Ds[c2] : wc->Hn1[c2];
+  double vol = ovol + rkw1[2] * (nvol - ovol);
+  double tr = sf * rkw1[0] * ntr0 + sf * rkw1[1] * ntr0 + rkw1[2] * grad;
+  tr /= (wc->Hn1[c2] * vol);
+  return tr;
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:3085-3260  surf_concrk: the surface-column mass at t,     */
+/* its volumes at t and t+1 and the flux sum grad                      */
+/*-------------------------------------------------------------------*/
+static double oracle_us_surf_concrk(geometry_t *g, win_priv_t *wc, const double *tr,
+                                    double osubeta, double subeta, int c, int c2, int cc,
+                                    double *gradc, double *ovol, double *nvol,
+                                    double fcbot, const double *dtracer) {
+  int cs = c;
+  int cb = wc->i1[cc];
+  int co = wc->i3[cc];
+  double waterbot = (c == cb) ? g->botz[c2] : g->gridz[c];
+  double dz = subeta - waterbot;
+  double surftr, watertop, vol, grad;
+  *nvol = dz * g->cellarea[c2];
+  surftr = *ovol = 0.0;
+  grad = -fcbot;
+  while (c != co) {
+    int zp1 = g->zp1[c];
+    watertop = g->gridz[zp1];
+    vol = (watertop - waterbot) * g->cellarea[c2];
+    *ovol += vol;
+    surftr += (tr[c] * vol);
+    grad -= dtracer[c];
+    c = zp1;
+    waterbot = g->gridz[c];
+  }
+  watertop = osubeta;
+  waterbot = (c == cb) ? g->botz[c2] : g->gridz[c];
+  vol = (watertop - waterbot) * g->cellarea[c2];
+  *ovol += vol;
+  surftr += tr[c] * vol;
+  grad -= dtracer[c];
+  if (c != c2) {
+    do {
+      c = g->zp1[c];
+      grad -= dtracer[c];
+    } while (c != c2);
+  }
+  if (*nvol == 0.0 || (surftr < 0.0 && dz <= wc->hmin)) surftr = 0.0;
+  (void)cs;
+  *gradc = grad;
+  return surftr;
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:3270-3370  tr_bounds (as SHOC)                            */
+/*-------------------------------------------------------------------*/
+void oracle_us_tr_bounds(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int cc, nn;
+  double eps = 1e-5;
+  double *mn = wc->mintr, *mx = wc->maxtr;
+  for (cc = 1; cc <= wc->vcs1; cc++) {
+    int c = wc->s1[cc], c3 = c;
+    int co = wc->i3[cc];
+    int cb = wc->i1[cc];
+    if (c < cb && wc->dz[co] < wc->hmin) {
+      for (nn = 0; nn < wc->ntbdy; nn++) {
+        int n = wc->tbdy[nn];
+        double *tr = wd->tr_wc[n];
+        if (tr[c] < mn[n] || tr[c] > mx[n]) {
+          double surftr = tr[g->zm1[c]];
+          c3 = c;
+          tr[c3] = surftr;
+          while (c3 != g->zp1[c3]) {
+            c3 = g->zp1[c3];
+            tr[c3] = surftr;
+          }
+        }
+      }
+    }
+  }
+  for (cc = 1; cc <= wc->vc1; cc++) {
+    int c = wc->s1[cc];
+    for (nn = 0; nn < wc->ntbdy; nn++) {
+      int n = wc->tbdy[nn];
+      double *tr = wd->tr_wc[n];
+      if (mn[n] <= eps && tr[c] < mn[n]) tr[c] = mn[n];
+    }
+  }
+  if (!wc->sigma) {
+    for (cc = 1; cc <= wc->vcs; cc++) {
+      int c = wc->s1[cc], cs = c;
+      while (c != g->zp1[c]) {
+        c = g->zp1[c];
+        for (nn = 0; nn < wc->ntbdy; nn++) {
+          int n = wc->tbdy[nn];
+          wd->tr_wc[n][c] = wd->tr_wc[n][cs];
+        }
+      }
+    }
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* One tracer through one sub-step: the body of the tracer loop of    */
+/* advect_diffuse, tracers.c:803-1098 (one Runge-Kutta stage).        */
+/*-------------------------------------------------------------------*/
+typedef struct {
+  double *Fx, *Fz, *dtracer, *rk, *gr;
+} uscr_t;
+
+static void us_tracer_substep(geometry_t *g, window_t *wd, win_priv_t *wc, int n, double dtu,
+                              int slf, int ksf, const double *osubeta,
+                              const double *subeta, uscr_t *s) {
+  int cc, j;
+  double *tr = wd->tr_wc[n];
+  double *Fx = s->Fx, *Fz = s->Fz, *dtracer = s->dtracer;
+  const int kef = wc->vc1, kefs = wc->vcs1;
+  tr[0] = (double)n;
+  memset(s->gr, 0, g->szc * sizeof(double));
+  memset(Fx, 0, g->sze * sizeof(double));
+  memset(Fz, 0, g->szc * sizeof(double));
+  memset(dtracer, 0, g->szc * sizeof(double));
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    int cb = g->bot_t[cc];
+    tr[g->zm1[cb]] = tr[cb];
+  }
+  if (wc->advect[n]) oracle_us_advect(g, wd, wc, tr, Fx, Fz, dtu);
+  if (wc->diffuse[n]) oracle_us_hor_diffuse(g, wd, wc, tr, Fx);
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    int c = g->w3_t[cc];
+    int c2 = g->m2d[c];
+    dtracer[c] = 0.0;
+    for (j = 1; j <= g->npe[c2]; j++) {
+      int e = g->c2e[j][c];
+      dtracer[c] += (g->eSc[j][c2] * Fx[e] * dtu);
+    }
+  }
+  for (cc = ksf; cc <= kef; cc++) {
+    int c = wc->s1[cc];
+    int c2 = g->m2d[c];
+    int zp1 = g->zp1[c];
+    s->rk[c] = tr[c];
+    if (slf)
+      tr[c] *= wc->Ds[c2];
+    else
+      tr[c] *= wc->Hn1[c2];
+    s->gr[c] = -((dtracer[c] / g->cellarea[c2] + (Fz[zp1] - Fz[c])) / wc->dz[c]);
+    tr[c] = rk1(wc, c2, 1.0, 1.0, s->rk[c], s->gr[c], slf);
+  }
+  if (!wc->sigma && wc->advect[n]) {
+    for (cc = 1; cc <= kefs; cc++) {
+      int c = wc->s1[cc];
+      int c2 = g->m2d[c];
+      double top = -Fz[c] * g->cellarea[c2];
+      double ovol, nvol, grad, surftr;
+      s->rk[c] = oracle_us_surf_concrk(g, wc, tr, osubeta[c2], subeta[c2], c, c2, cc,
+                                       &grad, &ovol, &nvol, top, dtracer);
+      s->gr[c] = grad;
+      surftr = rk1(wc, c2, ovol, nvol, s->rk[c], s->gr[c], slf);
+      tr[c] = (g->botz[c2] > 0.0 && wc->dz[c] < wc->hmin) ? tr[c] : surftr;
+    }
+  }
+}
+
+static uscr_t *uscr = NULL;
+static int uscr_nt = 0, uscr_n = 0;
+static uscr_t *uscr_get(int nt, int n) {
+  int t;
+  if (uscr && uscr_nt >= nt && uscr_n == n) return uscr;
+  for (t = 0; t < uscr_nt; t++) {
+    free(uscr[t].Fx);
+    free(uscr[t].Fz);
+    free(uscr[t].dtracer);
+    free(uscr[t].rk);
+    free(uscr[t].gr);
+  }
+  free(uscr);
+  uscr = (uscr_t *)calloc(nt, sizeof(uscr_t));
+  for (t = 0; t < nt; t++) {
+    uscr[t].Fx = (double *)calloc(n, sizeof(double));
+    uscr[t].Fz = (double *)calloc(n, sizeof(double));
+    uscr[t].dtracer = (double *)calloc(n, sizeof(double));
+    uscr[t].rk = (double *)calloc(n, sizeof(double));
+    uscr[t].gr = (double *)calloc(n, sizeof(double));
+  }
+  uscr_nt = nt;
+  uscr_n = n;
+  return uscr;
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:428-1157  advect_diffuse (z-levels, closed boundaries,   */
+/* one Runge-Kutta stage).  Returns 0, or the column (2D centre) at   */
+/* which the sub-step limit was exceeded (tracers.c:682-689).         */
+/*-------------------------------------------------------------------*/
+int oracle_us_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int nthreads) {
+  int cc, nn, ee;
+  double *osubeta = wc->d1, *subeta = wc->d2;
+  double dtu, dtm, trem;
+  double sf = 0.8, minval = 1e-10;
+  int itermax = 20, slf = 1, courf, ksf, ii = 0;
+  const int vc = wc->vc, vcs = wc->vcs;
+  double *u1 = wd->u1, *w = wd->w;
+  ous_err = 0;
+  if (wc->trasc == NONE) return 0;
+
+  courf = (wc->ultimate || wc->trasc >= QUICKEST) ? 1 : 0;
+  if (wc->trasc & ORDER1) {
+    wc->ultimate = 0;
+    for (ee = 1; ee <= g->n3_e1; ee++) {
+      int e = g->w3_e1[ee];
+      wc->w6[e] = 0.5 * (wd->u1flux3d[e] + fabs(wd->u1flux3d[e]));
+      wc->w7[e] = 0.5 * (wd->u1flux3d[e] - fabs(wd->u1flux3d[e]));
+    }
+    courf = 0;
+  }
+  if (wc->trasc & VANLEER) {
+    for (cc = 1; cc <= g->b3_t; cc++) {
+      int c = g->w3_t[cc];
+      wc->w9[c] = 0.5 * (wc->dz[c] + wc->dz[g->zm1[c]]);
+    }
+  }
+
+  /* sub-step, tracers.c:601-690 */
+  dtm = dtu = wd->dt;
+  if (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)) {
+    for (cc = 1; cc <= vc; cc++) {
+      int c = wc->s1[cc];
+      int cs = g->m2d[c];
+      int zp1 = g->zp1[c];
+      int j;
+      double d1, d3, d4;
+      for (j = 1; j <= g->npe[cs]; j++) {
+        int e = g->c2e[j][c];
+        int e2 = g->m2de[e];
+        d4 = u1[e];
+        if (fabs(d4) < minval) d4 = minval;
+        d1 = sf * fabs(g->h2au1[e2] / d4);
+        if (d1 < dtm) {
+          dtm = d1;
+          ii = cs;
+        }
+      }
+      d4 = (cc <= vcs) ? 0.0 : 0.5 * (w[c] + w[zp1]);
+      if (fabs(d4) < minval) d4 = minval;
+      /* the reference scales by Ds of the last edge's centre; Ds = 1
+       * at z-levels, so the column's own Ds is used */
+      d3 = (cc <= vcs) ? dtu : sf * fabs(wc->dz[c] * wc->Ds[cs] / d4);
+      if (d3 > 0.0 && d3 < dtm) {
+        dtm = d3;
+        ii = c;
+      }
+    }
+    if (dtm != dtu) dtm = floor(dtm);
+  }
+  if (dtm == 0.0 || (int)(dtu / dtm) > itermax) {
+    ous_err = 1;
+    return g->m2d[ii] ? g->m2d[ii] : 1;
+  }
+  ksf = vcs + 1;
+
+  for (cc = 1; cc <= vcs; cc++) {
+    int c = wc->s1[cc];
+    osubeta[g->m2d[c]] = wc->oldeta[g->m2d[c]];
+  }
+
+  trem = dtu;
+  while (trem > 0) {
+    dtu = dtm;
+    if (trem < dtu) dtu = trem;
+    wc->b1 = dtu;
+    if (dtu > 0.0) {
+      if (courf) {
+        for (ee = 1; ee <= g->n3_e1; ee++) {
+          int e = g->w3_e1[ee];
+          int e2 = g->m2de[e];
+          wc->w6[e] = fmod(u1[e] * dtu / g->h2au1[e2], 1.0);
+        }
+        for (cc = 1; cc <= g->n3_t; cc++) {
+          int c = g->w3_t[cc];
+          int c2 = g->m2d[c];
+          wc->w8[c] = fmod(w[c] * dtu / (wc->w9[c] * wc->Ds[c2]), 1.0);
+        }
+      }
+      for (cc = 1; cc <= vcs; cc++) {
+        int c = wc->s1[cc];
+        int c2 = g->m2d[c];
+        subeta[c2] = wc->oldeta[c2] + (1.0 + (dtu - trem) / wd->dt) *
+                                          (wd->eta[c2] - wc->oldeta[c2]);
+      }
+      {
+        int nt = nthreads > 0 ? nthreads : 1;
+        int nsz = g->szc > g->sze ? g->szc : g->sze;
+        if (nt > wc->ntbdy) nt = wc->ntbdy > 0 ? wc->ntbdy : 1;
+        uscr_t *S = uscr_get(nt, nsz);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+#endif
+        for (nn = 0; nn < wc->ntbdy; nn++) {
+          int tid = 0;
+#ifdef _OPENMP
+          tid = omp_get_thread_num();
+#endif
+          us_tracer_substep(g, wd, wc, wc->tbdy[nn], dtu, slf, ksf, osubeta, subeta,
+                            &S[tid]);
+        }
+      }
+      oracle_us_tr_bounds(g, wd, wc);
+      for (cc = 1; cc <= vcs; cc++) {
+        int c = wc->s1[cc];
+        int c2 = g->m2d[c];
+        osubeta[c2] = subeta[c2];
+      }
+    }
+    trem -= dtu;
+    slf = 0;
+  }
+  return 0;
+}
+
+int oracle_us_advect_diffuse(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  return oracle_us_advect_diffuse_mt(g, wd, wc, 1);
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:5753-5896  implicit vertical diffusion, one tracer       */
+/*-------------------------------------------------------------------*/
+static int oracle_us_implicit_vdiff_tr(geometry_t *g, window_t *wd, win_priv_t *wc,
+                                       double *var, const double *dzcell, const double *fb,
+                                       const double *ft, const int *ctp, const int *cbt,
+                                       int vcs, const double *scale, const double *C,
+                                       const double *Cp1, const double *Cm1, double *work) {
+  int cc, k;
+  const int nzw = g->nz + 2;
+  double *rhs = work, *sol = work + nzw, *ud = work + 2 * nzw;
+  double *B = work + 3 * nzw, *Bm1 = work + 4 * nzw, *Bp1 = work + 5 * nzw;
+  double *dz = work + 6 * nzw;
+  double dt = wd->dt, dzdt, div;
+  for (cc = 1; cc <= vcs; cc++) {
+    int cs = ctp[cc], c = cs;
+    int cb = cbt[cc];
+    int c2 = g->m2d[c];
+    int zm1 = g->zm1[c];
+    int ks = g->s2k[cs], kb = g->s2k[cb];
+    if (zm1 == g->zm1[zm1] || !dzcell[zm1]) {
+      var[cs] += wd->dt * (fb[c2] - ft[c2]) / max(dzcell[cs], wc->hmin);
+      if (c != wc->i4[cc]) var[wc->i4[cc]] = var[cs];
+      continue;
+    }
+    c = cs;
+    for (k = ks; k >= kb; k--) {
+      B[k] = C[c];
+      Bm1[k] = Cm1[c];
+      Bp1[k] = Cp1[c];
+      dz[k] = dzcell[c];
+      c = g->zm1[c];
+    }
+    dzdt = dz[kb] / dt;
+    rhs[kb] = dzdt * var[cb] + fb[c2];
+    c = g->zp1[cb];
+    for (k = kb + 1; k < ks; k++) {
+      dzdt = dz[k] / dt;
+      rhs[k] = dzdt * var[c];
+      c = g->zp1[c];
+    }
+    dzdt = dz[ks] / dt;
+    rhs[ks] = dzdt * var[cs] - ft[c2];
+    div = B[kb];
+    sol[kb] = rhs[kb] / div;
+    for (k = kb + 1; k <= ks; k++) {
+      ud[k] = Bp1[k - 1] / div;
+      div = B[k] - Bm1[k] * ud[k];
+      if (div == 0.0) return cs;
+      sol[k] = (rhs[k] - Bm1[k] * sol[k - 1]) / div;
+    }
+    c = cs;
+    var[cs] += (sol[ks] - var[cs]) * scale[c2];
+    for (k = ks - 1; k >= kb; k--) {
+      c = g->zm1[c];
+      sol[k] -= ud[k + 1] * sol[k + 1];
+      var[c] += (sol[k] - var[c]) * scale[c2];
+    }
+    c = wc->i4[cc];
+    if (c != cs) var[c] += (sol[ks] - var[cs]) * scale[c2];
+  }
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:5562-5744  vertical diffusion (no surface fluxes)         */
+/*-------------------------------------------------------------------*/
+int oracle_us_vert_diffuse_3d_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int nthreads) {
+  int cc, nn, bad = 0;
+  int *ctp = wc->i2, *cbt = wc->i5, *cth = wc->i4;
+  double dt = wd->dt;
+  double *dzface = wc->w7, *dzcell = wc->w9, *Kz = wc->w10;
+  double *Cm1 = wc->w1, *C = wc->w2, *Cp1 = wc->w3;
+  if (wc->trasc == NONE) return 0;
+  memset(wc->d1, 0, g->szcS * sizeof(double));
+  memset(wc->d2, 0, g->szcS * sizeof(double));
+  memset(cth, 0, g->szcS * sizeof(int));
+  memcpy(dzcell, wc->dz, g->szc * sizeof(double));
+  memcpy(Kz, wd->Kz, g->szc * sizeof(double));
+  ous_err = 0;
+
+  for (cc = 1; cc <= wc->vcs; cc++) {
+    int c, cs, c2, zm1, zp1, cb;
+    double dzdt;
+    c = cs = ctp[cc];
+    c2 = g->m2d[c];
+    zm1 = g->zm1[c];
+    if (zm1 == g->zm1[zm1] || !dzcell[zm1]) continue;
+    while (c != zm1 && dzcell[c]) {
+      dzface[c] = 0.5 * (dzcell[zm1] + dzcell[c]);
+      dzcell[c] *= wc->Ds[c2];
+      Kz[c] /= wc->Ds[c2];
+      c = zm1;
+      zm1 = g->zm1[c];
+    }
+    cb = c = g->zp1[c];
+    zp1 = g->zp1[c];
+    cbt[cc] = cb;
+    dzdt = dzcell[c] / dt;
+    Cm1[c] = 0.0;
+    Cp1[c] = -Kz[zp1] / dzface[zp1];
+    C[c] = dzdt - Cp1[c];
+    while (zp1 != cs) {
+      c = zp1;
+      zp1 = g->zp1[c];
+      dzdt = dzcell[c] / dt;
+      Cm1[c] = -Kz[c] / dzface[c];
+      Cp1[c] = -Kz[zp1] / dzface[zp1];
+      C[c] = dzdt - Cm1[c] - Cp1[c];
+    }
+    c = cs;
+    dzdt = dzcell[c] / dt;
+    Cm1[c] = -Kz[c] / dzface[c];
+    Cp1[c] = 0.0;
+    C[c] = dzdt - Cm1[c];
+  }
+  {
+    int nt = nthreads > 0 ? nthreads : 1;
+    if (nt > wc->ntdif_v) nt = wc->ntdif_v > 0 ? wc->ntdif_v : 1;
+    const int nzw = g->nz + 2;
+    double *work = (double *)calloc((size_t)nt * 7 * nzw, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1) reduction(max : bad)
+#endif
+    for (nn = 0; nn < wc->ntdif_v; nn++) {
+      int tid = 0;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+      int n = wc->tdif_v[nn];
+      int r = oracle_us_implicit_vdiff_tr(g, wd, wc, wd->tr_wc[n], dzcell, wc->d1, wc->d2,
+                                          ctp, cbt, wc->vcs, wc->one, C, Cp1, Cm1,
+                                          work + (size_t)tid * 7 * nzw);
+      if (r > bad) bad = r;
+    }
+    free(work);
+  }
+  if (bad) ous_err = 4;
+  return bad;
+}
+
+void oracle_us_vert_diffuse_3d(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  oracle_us_vert_diffuse_3d_mt(g, wd, wc, 1);
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:185-200 (lateral ghosts) and tracers.c:261-330            */
+/*-------------------------------------------------------------------*/
+int oracle_us_tracer_step_3d_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int nthreads) {
+  oracle_us_set_lateral_BC_tr(g, wd);
+  if (oracle_us_advect_diffuse_mt(g, wd, wc, nthreads)) return 1;
+  if (oracle_us_vert_diffuse_3d_mt(g, wd, wc, nthreads)) return 4;
+  return 0;
+}
+
+void oracle_us_tracer_step_3d(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  oracle_us_tracer_step_3d_mt(g, wd, wc, 1);
+}
+
+/* domain tracer mass sum(tr*area*dz) over s1 (tracers.c:6608-6631 of
+ * SHOC, the same sum in COMPAS) */
+double oracle_us_tmass(geometry_t *g, window_t *wd, win_priv_t *wc, int n) {
+  int cc;
+  double d1 = 0.0;
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int cs = g->m2d[c];
+    d1 += wd->tr_wc[n][c] * g->cellarea[cs] * wc->dz[c];
+  }
+  return d1;
+}

@@ -1,0 +1,70 @@
+"""ctypes wrapper of the COMPAS CPU oracle (oracle/liboracle_us.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, as the checker or the timed CPU baseline.
+"""
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboracle_us.so")
+_lib = None
+
+
+def build():
+    r = subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle_us.so"],
+                       capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return LIBPATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            build()
+        from ems_amd import structs_us as U
+        L = C.CDLL(LIBPATH)
+        G, W, P = C.POINTER(U.Geometry), C.POINTER(U.Window), C.POINTER(U.WinPriv)
+        for fn in ("oracle_us_set_dz", "oracle_us_tracer_step_3d",
+                   "oracle_us_vert_diffuse_3d", "oracle_us_tr_bounds"):
+            getattr(L, fn).argtypes = [G, W, P]
+            getattr(L, fn).restype = None
+        L.oracle_us_set_lateral_BC_tr.argtypes = [G, W]
+        L.oracle_us_advect_diffuse.argtypes = [G, W, P]
+        L.oracle_us_advect_diffuse.restype = C.c_int
+        for fn in ("oracle_us_advect_diffuse_mt", "oracle_us_vert_diffuse_3d_mt",
+                   "oracle_us_tracer_step_3d_mt"):
+            getattr(L, fn).argtypes = [G, W, P, C.c_int]
+            getattr(L, fn).restype = C.c_int
+        L.oracle_us_tmass.argtypes = [G, W, P, C.c_int]
+        L.oracle_us_tmass.restype = C.c_double
+        L.oracle_us_last_error.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def set_dz(gp, wp, pp):
+    lib().oracle_us_set_dz(gp, wp, pp)
+
+
+def step(case, nthreads=1):
+    return lib().oracle_us_tracer_step_3d_mt(case.gp, case.wp, case.pp, nthreads)
+
+
+def advect_diffuse(case, nthreads=1, lateral=True):
+    """advect_diffuse as tracer_step_3d calls it, after tracer_step has
+    filled the lateral ghosts (tracers.c:185-200)."""
+    if lateral:
+        lib().oracle_us_set_lateral_BC_tr(case.gp, case.wp)
+    return lib().oracle_us_advect_diffuse_mt(case.gp, case.wp, case.pp, nthreads)
+
+
+def vert_diffuse(case, nthreads=1):
+    return lib().oracle_us_vert_diffuse_3d_mt(case.gp, case.wp, case.pp, nthreads)
+
+
+def tmass(case, n):
+    return lib().oracle_us_tmass(case.gp, case.wp, case.pp, n)
