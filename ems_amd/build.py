@@ -41,8 +41,10 @@ def _run(cmd):
 
 
 # libemstr_us.so: the COMPAS path (unstructured meshes)
-US_HIP_SRC = []
-US_C_SRC = ["emstr_mesh.c"]
+US_HIP_SRC = ["emstr_us.hip"]
+US_C_SRC = ["emstr_mesh.c", "emstr_us_host.c", "emstr_err.c"]
+# the column kernels of the plain form (set_dz, vertical diffusion, ...)
+US_SHARED_HIP = ["emstr_plain.hip"]
 
 
 def build_us(verbose=False):
@@ -53,6 +55,12 @@ def build_us(verbose=False):
         o = os.path.join(BUILD, "us_" + s.replace(".hip", ".o"))
         _run([hipcc] + HIPFLAGS + HIP_EXTRA.get(s, []) + INC +
              ["-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    for s in US_SHARED_HIP:
+        o = os.path.join(BUILD, s.replace(".hip", ".o"))
+        if not os.path.exists(o):
+            _run([hipcc] + HIPFLAGS + HIP_EXTRA.get(s, []) + INC +
+                 ["-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     for s in US_C_SRC:
         o = os.path.join(BUILD, "us_" + s.replace(".c", ".o"))

@@ -22,6 +22,8 @@ typedef struct emstr_dev {
   long long plane; /* nx*ny    */
   long long n3;    /* nz*plane */
   int trasc, ultimate, sigma;
+  int us;       /* COMPAS window: set_dz processes a column whose water top
+                   is >= (not >) DRY_FRAC*hmin above the bed            */
   double dt, hmin;
 
   /* per column [ny][nx] */

@@ -1256,7 +1256,8 @@ __global__ void k_setdz(emstr_dev d, int *__restrict__ proc) {
   const int klow = min(kold, knew);
   double top = fmin(eta, oldeta);
   double bot = 0.05 * d.hmin + d.botz[col];
-  int p = (top > bot && klow >= kbc) ? 1 : 0;
+  /* tracers.c:5773 (SHOC) tests top > bot, hd-us tracers.c:7325 top >= bot */
+  int p = ((d.us ? (top >= bot) : (top > bot)) && klow >= kbc) ? 1 : 0;
   d.ks[col] = p ? klow : -1;
   proc[col] = p;
   /* dz below the old surface, bed cell from botz, copy above */

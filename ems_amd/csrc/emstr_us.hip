@@ -1,0 +1,376 @@
+/*
+ * Plain form of the EMS COMPAS tracer step on MI355X (gfx950): FP64, the
+ * reference's edge-based schemes, limiter and operation order, so results
+ * agree with the reference CPU path to the last bit.  Compiled with
+ * -ffp-contract=off (no FMA contraction).
+ *
+ * Kernels (layout in emstr_usdev.h):
+ *   k_usadv<SCH>  one thread per wet centre, loop over tracers inside:
+ *                 the fluxes through the cell's edges (van Leer on edges
+ *                 with ULTIMATE, or first order, tracers.c:1637-1743,
+ *                 2368-2573, 3785-3833; horizontal diffusion,
+ *                 tracers.c:5383-5449), the vertical fluxes, the edge-sign
+ *                 divergence (tracers.c:949-959) and the one-stage
+ *                 Runge-Kutta update (tracers.c:1000-1016, 1170-1214);
+ *                 the processed surface centre also does surf_concrk
+ *                 (tracers.c:3085-3260) and tr_bounds (tracers.c:3270-3370)
+ *                 for its column.
+ *   k_usdtm       minimum sub-step over the cells to process
+ *                 (tracers.c:601-683).
+ * The column kernels (set_dz, sub-step elevations, vertical diffusion,
+ * mass, scatter/gather) are those of the plain SHOC form.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "emstr_us_kernels.h"
+
+#define ORDER1 0x000002
+#define VANLEER 0x000080
+
+typedef long long ll;
+
+__device__ __forceinline__ bool wetk(const int *__restrict__ kb, int col, int k) {
+  return col >= 0 && k >= kb[col];
+}
+
+/* van Leer face value from tracer values (tracers.c:2544-2573); *set is
+ * false when the velocity is zero (the face keeps its zero) */
+__device__ __forceinline__ double us_vl(double vel, double cn, double tp1, double tp2,
+                                        double tm1, double tm2) {
+  double df;
+  if (vel > 0.0) {
+    df = (tm1 - tm2) * (tp1 - tm1);
+    if (df > 0.0)
+      df = 2.0 * df / (tp1 - tm2);
+    else
+      df = 0.0;
+    return tm1 + 0.5 * (1.0 - cn) * df;
+  } else if (vel < 0.0) {
+    df = (tp1 - tm1) * (tp2 - tp1);
+    if (df > 0.0)
+      df = 2.0 * df / (tp2 - tm1);
+    else
+      df = 0.0;
+    return tp1 - 0.5 * (1.0 + cn) * df;
+  }
+  return 0.0;
+}
+
+/* ultimate_filter (tracers.c:3785-3833) on face value F; trc: tracer at
+ * the face's cell c, trcm1 at its cell cm1 */
+__device__ __forceinline__ double us_ult(double cr, double F, double trc, double trcm1,
+                                         double trp, double trm) {
+  double nc, nu, tc, tu, td, diff;
+  if (cr == 0.0) return F;
+  if (cr > 0.0) {
+    tc = trcm1;
+    tu = trm;
+    td = trc;
+  } else {
+    tc = trc;
+    tu = trp;
+    td = trcm1;
+  }
+  diff = td - tu;
+  nc = nu = 0.0;
+  if (diff) {
+    nc = (tc - tu) / diff;
+    nu = (F - tu) / diff;
+  }
+  if (nc >= 0.0 && nc <= 1.0) {
+    if (nu > 1.0)
+      F = td;
+    else if (nu < nc)
+      F = tu + nc * (td - tu);
+    else if (nu > nc / fabs(cr))
+      F = tu + nc * (td - tu) / fabs(cr);
+  } else
+    F = tu + nc * (td - tu);
+  return F;
+}
+
+/*
+ * Flux through wet surface edge e at layer k for one tracer: the face
+ * value by the scheme, ULTIMATE, times u1flux3d (advect(),
+ * tracers.c:1659-1728), then horizontal diffusion (tracers.c:5438-5446).
+ * A far cell that is not wet at this layer is a ghost of the near cell:
+ * its step-start value.
+ */
+template <int SCH>
+__device__ __forceinline__ double us_edge_flux(const emstr_usdev &D, int e, int k,
+                                               const double *__restrict__ t,
+                                               const double *__restrict__ ts, double dtu,
+                                               int ult, int adv, int dif) {
+  const emstr_dev &d = D.d;
+  const ll P = d.plane;
+  const int c1 = D.ec1[e], c2 = D.ec2[e];
+  const ll i1 = (ll)k * P + c1, i2 = (ll)k * P + c2;
+  const ll ie = (ll)k * D.ne2 + e;
+  double F = 0.0;
+  if (adv) {
+    if (SCH == ORDER1) {
+      /* tracers.c:545-549, 1772-1775 */
+      const double f = D.flux[ie];
+      const double w6 = 0.5 * (f + fabs(f));
+      const double w7 = 0.5 * (f - fabs(f));
+      F = w6 * t[i2] + w7 * t[i1];
+    } else {
+      const double vel = D.u1[ie];
+      const double cn = fmod(vel * dtu / D.h2e[e], 1.0);
+      const int cp = D.ecp[e], cm = D.ecm[e];
+      const bool wp = wetk(d.kb, cp, k), wm = wetk(d.kb, cm, k);
+      const double tp1 = t[i1], tm1 = t[i2];
+      const double tp2 = wp ? t[(ll)k * P + cp] : ts[i1];
+      const double tm2 = wm ? t[(ll)k * P + cm] : ts[i2];
+      F = us_vl(vel, cn, tp1, tp2, tm1, tm2);
+      if (ult) {
+        const unsigned sel = D.esel[e];
+        const double trp = (wp && (sel & 1)) ? tp2 : tp1;
+        const double trm = wm ? ((sel & 2) ? tm2 : tm1) : ts[i2];
+        F = us_ult(cn, F, tp1, tm1, trp, trm);
+      }
+      F *= D.flux[ie];
+    }
+  }
+  if (dif) {
+    const double csx = D.dzu[ie] * D.h1e[e] * 1.0;
+    F -= (csx * D.kh[ie] * (t[i1] - t[i2]) / D.h2e[e]);
+  }
+  return F;
+}
+
+/* dtracer of the cell at layer k of column col (tracers.c:949-959) */
+template <int SCH>
+__device__ double us_dtracer(const emstr_usdev &D, int col, int k, const double *t,
+                             const double *ts, double dtu, int ult, int adv, int dif) {
+  const emstr_dev &d = D.d;
+  const int ncol = (int)d.plane;
+  double dtr = 0.0;
+  for (int j = 0; j < D.npe[col]; j++) {
+    const int e = D.ce[j * ncol + col];
+    const int other = (D.ec1[e] == col) ? D.ec2[e] : D.ec1[e];
+    double F = 0.0;
+    if (wetk(d.kb, other, k)) F = us_edge_flux<SCH>(D, e, k, t, ts, dtu, ult, adv, dif);
+    dtr += ((double)D.esc[j * ncol + col] * F * dtu);
+  }
+  return dtr;
+}
+
+/* vertical flux through the lower face of the cell at layer k
+ * (tracers.c:2405-2412 van Leer, 1777-1785 first order, 1712-1720
+ * ULTIMATE, 1735-1738 times w*dtu); tracer values at the layers below
+ * the bed are the bed value (sediment ghost) */
+template <int SCH>
+__device__ __forceinline__ double us_fz(const emstr_dev &d, int col, int k, int kbc,
+                                        const double *__restrict__ t, double dtu, int ult) {
+  const ll P = d.plane;
+  const ll idx = (ll)k * P + col;
+  const double w = d.w[idx];
+  const double tc = t[idx];
+  const double tzm1 = t[(ll)max(k - 1, kbc) * P + col];
+  if (SCH == ORDER1)
+    return dtu * (0.5 * (w + fabs(w)) * tzm1 + 0.5 * (w - fabs(w)) * tc);
+  const double tzm2 = t[(ll)max(k - 2, kbc) * P + col];
+  const double tzp1 = t[(ll)min(k + 1, d.nz - 1) * P + col];
+  const double dzb = (k > kbc) ? d.dz[idx - P] : 0.0;
+  const double w9 = 0.5 * (d.dz[idx] + dzb);
+  const double cn = fmod(w * dtu / (w9 * 1.0), 1.0);
+  double F = us_vl(w, cn, tc, tzp1, tzm1, tzm2);
+  if (ult) F = us_ult(cn, F, tc, tzm1, tzp1, tzm2);
+  return F * (w * dtu);
+}
+
+/* one-stage Runge-Kutta (tracers.c:1170-1214, rkweights column 0):
+ * Ds = Hn1 = 1 at z-levels */
+__device__ __forceinline__ double us_rk1(double ovol, double nvol, double ntr0, double grad) {
+  const double sf = 1.0;
+  const double vol = ovol + 1.0 * (nvol - ovol);
+  double tr = sf * 1.0 * ntr0 + sf * 0.0 * ntr0 + 1.0 * grad;
+  tr /= (1.0 * vol);
+  return tr;
+}
+
+/* new value of the sub-surface cell at layer k (tracers.c:1000-1016) */
+template <int SCH>
+__device__ double us_subsurface(const emstr_usdev &D, int col, int k, int kbc,
+                                const double *t, const double *ts, double dtu, int ult,
+                                int adv, int dif) {
+  const emstr_dev &d = D.d;
+  const ll idx = (ll)k * d.plane + col;
+  const double dtr = us_dtracer<SCH>(D, col, k, t, ts, dtu, ult, adv, dif);
+  double FzB = 0.0, FzT = 0.0;
+  if (adv) {
+    FzB = us_fz<SCH>(d, col, k, kbc, t, dtu, ult);
+    FzT = us_fz<SCH>(d, col, k + 1, kbc, t, dtu, ult);
+  }
+  const double grad = -((dtr / d.cellarea[col] + (FzT - FzB)) / d.dz[idx]);
+  return us_rk1(1.0, 1.0, t[idx], grad);
+}
+
+template <int SCH>
+__global__ void __launch_bounds__(256)
+k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout,
+        const double *__restrict__ tstart, double dtu, int anydif, int ult) {
+  const emstr_dev &d = D.d;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.z;
+  if (col >= d.nx) return;
+  const int kbc = d.kb[col];
+  if (k < kbc) return;
+  const ll P = d.plane;
+  const ll idx = (ll)k * P + col;
+  const ll N3 = d.n3;
+  const int ksc = d.ks[col];
+  if (ksc < 0) { /* column not processed: unchanged */
+    for (int n = 0; n < d.ntr; n++) tout[n * N3 + idx] = tin[n * N3 + idx];
+    return;
+  }
+  if (k > ksc) return; /* above the processed surface: written below */
+
+  if (k < ksc) {
+    for (int n = 0; n < d.ntr; n++) {
+      const double *t = tin + n * N3;
+      if (!d.in_tbdy[n]) {
+        tout[n * N3 + idx] = t[idx];
+        continue;
+      }
+      const int adv = d.advect[n], dif = anydif ? d.diffuse[n] : 0;
+      double r = us_subsurface<SCH>(D, col, k, kbc, t, tstart + n * N3, dtu, ult, adv, dif);
+      /* tr_bounds clip (tracers.c:3328-3331) */
+      if (d.mintr[n] <= 1e-5 && r < d.mintr[n]) r = d.mintr[n];
+      tout[n * N3 + idx] = r;
+    }
+    return;
+  }
+
+  /* ---- processed surface centre of the column (k == ksc) ---- */
+  const double area = d.cellarea[col];
+  const int kold = d.kold[col];
+  const double subeta = d.subeta[col];
+  const double osubeta = d.osubeta[col];
+  const double botz = d.botz[col];
+  const double *lay = d.layers;
+  const int ktop = d.nz - 1;
+  for (int n = 0; n < d.ntr; n++) {
+    const double *t = tin + n * N3;
+    const double *ts = tstart + n * N3;
+    double *o = tout + n * N3;
+    if (!d.in_tbdy[n]) {
+      for (int kk = k; kk <= ktop; kk++) o[(ll)kk * P + col] = t[(ll)kk * P + col];
+      continue;
+    }
+    const int adv = d.advect[n], dif = anydif ? d.diffuse[n] : 0;
+    double trs;
+    if (!adv) {
+      trs = t[idx];
+    } else {
+      /* surf_concrk (tracers.c:3085-3260) and runge_kutta */
+      const double FzB = us_fz<SCH>(d, col, k, kbc, t, dtu, ult);
+      const double top = -FzB * area;
+      int c = k;
+      double waterbot = (c == kbc) ? botz : lay[c];
+      const double dzs = subeta - waterbot;
+      const double nvol = dzs * area;
+      double surftr = 0.0, ovol = 0.0, vol, watertop;
+      double grad = -top;
+      while (c != kold && c < ktop) {
+        watertop = lay[c + 1];
+        vol = (watertop - waterbot) * area;
+        ovol += vol;
+        surftr += (t[(ll)c * P + col] * vol);
+        grad -= us_dtracer<SCH>(D, col, c, t, ts, dtu, ult, adv, dif);
+        c = c + 1;
+        waterbot = lay[c];
+      }
+      watertop = osubeta;
+      waterbot = (c == kbc) ? botz : lay[c];
+      vol = (watertop - waterbot) * area;
+      ovol += vol;
+      surftr += t[(ll)c * P + col] * vol;
+      grad -= us_dtracer<SCH>(D, col, c, t, ts, dtu, ult, adv, dif);
+      if (c != ktop) {
+        do {
+          c = c + 1;
+          grad -= us_dtracer<SCH>(D, col, c, t, ts, dtu, ult, adv, dif);
+        } while (c != ktop);
+      }
+      if (nvol == 0.0 || (surftr < 0.0 && dzs <= d.hmin)) surftr = 0.0;
+      const double r = us_rk1(ovol, nvol, surftr, grad);
+      /* tracers.c:1056-1057 */
+      trs = (botz > 0.0 && d.dz[idx] < d.hmin) ? t[idx] : r;
+    }
+    /* tr_bounds, thin surface layer (tracers.c:3303-3321): the old
+     * surface thickness dz[co] */
+    const double dzco = (kold >= kbc) ? d.dz[(ll)kold * P + col] : 0.0;
+    if (k > kbc && dzco < d.hmin) {
+      if (trs < d.mintr[n] || trs > d.maxtr[n])
+        trs = us_subsurface<SCH>(D, col, k - 1, kbc, t, ts, dtu, ult, adv, dif);
+    }
+    /* clip (tracers.c:3328-3331) */
+    if (d.mintr[n] <= 1e-5 && trs < d.mintr[n]) trs = d.mintr[n];
+    /* the surface value fills every layer above (tracers.c:3355-3366) */
+    for (int kk = k; kk <= ktop; kk++) o[(ll)kk * P + col] = trs;
+  }
+}
+
+/* minimum sub-step (tracers.c:601-683): edge velocities over the centre
+ * distance, w over dz below the surface; positive doubles compare as
+ * their bit patterns */
+__global__ void k_usdtm(emstr_usdev D, double dtu0, unsigned long long *out) {
+  const emstr_dev &d = D.d;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.z;
+  if (col >= d.nx) return;
+  const int ksc = d.ks[col];
+  const int kbc = d.kb[col];
+  if (ksc < 0 || k < kbc || k > ksc) return;
+  const ll P = d.plane;
+  const ll idx = (ll)k * P + col;
+  const int ncol = (int)P;
+  const double sf = 0.8, minval = 1e-10;
+  double dtm = dtu0, d4;
+  for (int j = 0; j < D.npe[col]; j++) {
+    const int e = D.ce[j * ncol + col];
+    d4 = D.u1[(ll)k * D.ne2 + e];
+    if (fabs(d4) < minval) d4 = minval;
+    const double d1 = sf * fabs(D.h2e[e] / d4);
+    if (d1 < dtm) dtm = d1;
+  }
+  const bool surf = (k == ksc);
+  const ll zp = (k + 1 < d.nz) ? idx + P : idx;
+  d4 = surf ? 0.0 : 0.5 * (d.w[idx] + d.w[zp]);
+  if (fabs(d4) < minval) d4 = minval;
+  const double d3 = surf ? dtu0 : sf * fabs(d.dz[idx] * 1.0 / d4);
+  if (d3 > 0.0 && d3 < dtm) dtm = d3;
+  if (dtm < dtu0) atomicMin(out, (unsigned long long)__double_as_longlong(dtm));
+}
+
+extern "C" int emstr_us_launch_adv(const emstr_usdev *D, const double *tin, double *tout,
+                                   const double *tstart, double dtu, int anydif, int ult,
+                                   void *stream) {
+  const emstr_dev *d = &D->d;
+  dim3 b(256, 1, 1);
+  dim3 g((d->nx + 255) / 256, 1, d->nz);
+  hipStream_t st = (hipStream_t)stream;
+  switch (d->trasc) {
+    case VANLEER:
+      k_usadv<VANLEER><<<g, b, 0, st>>>(*D, tin, tout, tstart, dtu, anydif, ult);
+      break;
+    case ORDER1:
+      k_usadv<ORDER1><<<g, b, 0, st>>>(*D, tin, tout, tstart, dtu, anydif, 0);
+      break;
+    default:
+      return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned long long *out,
+                                   void *stream) {
+  const emstr_dev *d = &D->d;
+  dim3 b(256, 1, 1);
+  dim3 g((d->nx + 255) / 256, 1, d->nz);
+  k_usdtm<<<g, b, 0, (hipStream_t)stream>>>(*D, dtu0, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}

@@ -1,0 +1,815 @@
+/*
+ * Host layer of libemstr_us: the COMPAS tracer step (model/hd-us) on the
+ * GPU behind the reference's entry points.
+ *
+ * A context mirrors one window on the device in the layout of
+ * emstr_usdev.h.  It is built from the window's own maps: wet mesh cells
+ * become columns in surface order, the surface-layer edges become the
+ * dense edges of every layer, and the edge stencils the reference reaches
+ * through c2c/e2e and ep/em are reduced to per-edge columns and selectors,
+ * checked against the 3D maps of every layer when the context is made.
+ *
+ * Scope (rejected with an error otherwise): z-levels, closed boundaries,
+ * one Runge-Kutta stage, schemes van Leer (with or without ULTIMATE) and
+ * first order, no mean-velocity transport, no thin-layer merging, and no
+ * flux or velocity through land edges.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "emstr_us.h"
+#include "emstr_us_kernels.h"
+
+typedef long long ll;
+
+/* error record (emstr_err.c) */
+int emstr_set_error(int code, int cell, const char *fmt, ...);
+void emstr_clear_error(void);
+const char *emstr_last_error(void);
+int emstr_last_error_cell(void);
+
+#define E_UNSUP 2
+#define E_DEVICE 3
+#define E_ARG 5
+#define E_SUBSTEP 1
+#define E_PIVOT 4
+
+#define HCHK(x)                                                             \
+  do {                                                                      \
+    hipError_t _e = (x);                                                    \
+    if (_e != hipSuccess)                                                   \
+      return emstr_set_error(E_DEVICE, -1, "%s: %s", #x, hipGetErrorString(_e)); \
+  } while (0)
+
+typedef struct usctx {
+  geometry_t *g;
+  window_t *wd;
+  win_priv_t *wc;
+  hipStream_t st;
+  emstr_usdev D;
+  int ncol, ne2, nesp;
+  int *cs2col;   /* [szcS] surface centre -> column (-1)   */
+  int *col2cs;   /* [ncol]                                  */
+  int *h_kb, *h_ks, *h_kold, *h_knew, *h_cols;
+  int *sedg;     /* [ncol] sediment ghost centre            */
+  int *sp_of;    /* [nz*ncol] dense cell -> sparse centre   */
+  /* device */
+  double *tr[3];
+  int cur;
+  double *stage; /* [max(szc, sze)] */
+  int *d_esp;    /* wet edges: sparse index and dense index */
+  ll *d_edn;
+  int *d_bpt, *d_bin, *d_gsed, *d_ised;
+  int *d_proc, *d_err;
+  unsigned long long *d_dtm;
+  int anydif, have_cols;
+  int key_szc, key_sze;
+} usctx;
+
+static void *dalloc(size_t bytes, int *ok) {
+  void *p = NULL;
+  if (bytes == 0) bytes = 8;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    *ok = 0;
+    return NULL;
+  }
+  hipMemset(p, 0, bytes);
+  return p;
+}
+
+static int up(usctx *c, void *dst, const void *src, size_t bytes) {
+  HCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->st));
+  return 0;
+}
+
+static void us_destroy(usctx *c) {
+  int n;
+  if (!c) return;
+  emstr_usdev *D = &c->D;
+  emstr_dev *d = &D->d;
+  if (c->st) hipStreamSynchronize(c->st);
+  void *dv[] = {d->kb, d->ks, d->kold, d->knew, d->cellarea, d->botz, d->eta, d->oldeta,
+                d->osubeta, d->subeta, d->layers, d->w, d->Kz, d->dz, d->mintr, d->maxtr,
+                d->advect, d->diffuse, d->in_tbdy, d->in_tdif, d->cols, d->wsp, d->wdn,
+                D->npe, D->ce, D->esc, D->ec1, D->ec2, D->ecp, D->ecm, D->esel, D->h1e,
+                D->h2e, D->u1, D->flux, D->dzu, D->kh, c->tr[0], c->tr[1], c->tr[2],
+                c->stage, c->d_esp, c->d_edn, c->d_bpt, c->d_bin, c->d_gsed, c->d_ised,
+                c->d_proc, c->d_err, c->d_dtm};
+  for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
+    if (dv[n]) hipFree(dv[n]);
+  void *hv[] = {c->cs2col, c->col2cs, c->h_kb, c->h_ks, c->h_kold, c->h_knew, c->h_cols,
+                c->sedg, c->sp_of};
+  for (n = 0; n < (int)(sizeof(hv) / sizeof(hv[0])); n++) free(hv[n]);
+  if (c->st) hipStreamDestroy(c->st);
+  free(c);
+}
+
+/* the window settings this library covers */
+static int us_check_scope(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int n;
+  if (g->nobc) return emstr_set_error(E_UNSUP, -1, "open boundaries are not supported");
+  if (wc->sigma) return emstr_set_error(E_UNSUP, -1, "sigma coordinates are not supported");
+  if (wc->thin_merge) return emstr_set_error(E_UNSUP, -1, "thin-layer merging is not supported");
+  if (wc->means) return emstr_set_error(E_UNSUP, -1, "mean-velocity transport is not supported");
+  if (wc->rkstage != 1)
+    return emstr_set_error(E_UNSUP, -1, "Runge-Kutta with %d stages is not supported",
+                           wc->rkstage);
+  if (wc->trasc != VANLEER && wc->trasc != ORDER1)
+    return emstr_set_error(E_UNSUP, -1, "advection scheme 0x%x is not supported", wc->trasc);
+  for (n = 1; n <= g->v2_t; n++) {
+    int cs = g->w2_t[n];
+    if (wc->Ds[cs] != 1.0 || wc->Hn1[cs] != 1.0 || (wc->one && wc->one[cs] != 1.0))
+      return emstr_set_error(E_UNSUP, cs, "z-level windows need Ds = Hn1 = 1");
+  }
+  for (n = 1; n < g->szeS; n++)
+    if (wc->mdx[n] != 1.0) return emstr_set_error(E_UNSUP, n, "mdx must be 1");
+  (void)wd;
+  return 0;
+}
+
+/* Is centre x the ghost of centre w (a lateral ghost whose bin is w)? */
+static int ghost_of(const geometry_t *g, const int *gbin, int x, int w) {
+  return gbin[x] == w;
+}
+
+static usctx *us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int ok = 1, n, cc, j, e;
+  emstr_clear_error();
+  if (us_check_scope(g, wd, wc)) return NULL;
+  usctx *c = (usctx *)calloc(1, sizeof(usctx));
+  c->g = g;
+  c->wd = wd;
+  c->wc = wc;
+  c->key_szc = g->szc;
+  c->key_sze = g->sze;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    free(c);
+    emstr_set_error(E_DEVICE, -1, "stream create");
+    return NULL;
+  }
+  emstr_usdev *D = &c->D;
+  emstr_dev *d = &D->d;
+  const int ncol = g->v2_t, nz = g->nz, npem = g->npem;
+  const int ne2 = g->szeS - 1;
+  c->ncol = ncol;
+  c->ne2 = ne2;
+  d->nx = ncol;
+  d->ny = 1;
+  d->nz = nz;
+  d->ntr = wd->ntr;
+  d->plane = ncol;
+  d->n3 = (ll)ncol * nz;
+  d->trasc = wc->trasc;
+  d->ultimate = wc->ultimate;
+  d->us = 1;
+  d->dt = wd->dt;
+  d->hmin = wc->hmin;
+  d->sgsiz = g->szc;
+  D->npem = npem;
+  D->ne2 = ne2;
+
+  /* columns */
+  c->cs2col = (int *)malloc((size_t)g->szcS * sizeof(int));
+  for (n = 0; n < g->szcS; n++) c->cs2col[n] = -1;
+  c->col2cs = (int *)malloc((size_t)(ncol + 1) * sizeof(int));
+  c->h_kb = (int *)malloc((size_t)(ncol + 1) * sizeof(int));
+  c->sedg = (int *)malloc((size_t)(ncol + 1) * sizeof(int));
+  for (cc = 1; cc <= ncol; cc++) {
+    int cs = g->w2_t[cc], cb = g->bot_t[cc];
+    c->cs2col[cs] = cc - 1;
+    c->col2cs[cc - 1] = cs;
+    c->h_kb[cc - 1] = g->s2k[cb];
+    c->sedg[cc - 1] = g->zm1[cb];
+  }
+  /* wet centres */
+  d->nwet = g->v3_t;
+  int *wsp = (int *)malloc((size_t)(d->nwet + 1) * sizeof(int));
+  ll *wdn = (ll *)malloc((size_t)(d->nwet + 1) * sizeof(ll));
+  c->sp_of = (int *)calloc((size_t)nz * ncol + 1, sizeof(int));
+  for (n = 0; n < d->nwet; n++) {
+    int cw = g->w3_t[n + 1];
+    int col = c->cs2col[g->m2d[cw]];
+    int k = g->s2k[cw];
+    if (col < 0 || k < c->h_kb[col] || k >= nz) {
+      emstr_set_error(E_UNSUP, cw, "wet centre outside its column");
+      ok = 0;
+      break;
+    }
+    wsp[n] = cw;
+    wdn[n] = (ll)k * ncol + col;
+    c->sp_of[wdn[n]] = cw;
+  }
+  /* ghost -> its wet centre */
+  int *gbin = (int *)calloc(g->szc, sizeof(int));
+  for (n = 1; n <= g->nbpt; n++) gbin[g->bpt[n]] = g->bin[n];
+
+  /* per column edge lists */
+  int *h_npe = (int *)calloc(ncol + 1, sizeof(int));
+  int *h_ce = (int *)calloc((size_t)npem * ncol + 1, sizeof(int));
+  int *h_esc = (int *)calloc((size_t)npem * ncol + 1, sizeof(int));
+  for (int col = 0; col < ncol && ok; col++) {
+    int cs = c->col2cs[col];
+    h_npe[col] = g->npe[cs];
+    if (g->npe[cs] > npem) {
+      emstr_set_error(E_UNSUP, cs, "npe exceeds npem");
+      ok = 0;
+    }
+    for (j = 1; j <= g->npe[cs] && ok; j++) {
+      e = g->c2e[j][cs];
+      if (e < 1 || e >= g->szeS) {
+        emstr_set_error(E_UNSUP, cs, "surface edge %d of centre %d outside the surface", j, cs);
+        ok = 0;
+        break;
+      }
+      h_ce[(j - 1) * ncol + col] = e - 1;
+      h_esc[(j - 1) * ncol + col] = g->eSc[j][cs];
+    }
+  }
+  /* per surface edge stencils */
+  int *h_ec1 = (int *)malloc((size_t)(ne2 + 1) * sizeof(int));
+  int *h_ec2 = (int *)malloc((size_t)(ne2 + 1) * sizeof(int));
+  int *h_ecp = (int *)malloc((size_t)(ne2 + 1) * sizeof(int));
+  int *h_ecm = (int *)malloc((size_t)(ne2 + 1) * sizeof(int));
+  unsigned char *h_esel = (unsigned char *)calloc(ne2 + 1, 1);
+  double *h_h1 = (double *)malloc((size_t)(ne2 + 1) * sizeof(double));
+  double *h_h2 = (double *)malloc((size_t)(ne2 + 1) * sizeof(double));
+  for (e = 1; e <= ne2 && ok; e++) {
+    int c1 = g->e2c[e][0], c2 = g->e2c[e][1];
+    int q = e - 1;
+    h_ec1[q] = c->cs2col[c1];
+    h_ec2[q] = (c2 < g->szcS) ? c->cs2col[c2] : -1;
+    h_h1[q] = g->h1au1[e];
+    h_h2[q] = g->h2au1[e];
+    h_ecp[q] = h_ecm[q] = -1;
+    if (h_ec1[q] < 0) {
+      emstr_set_error(E_UNSUP, c1, "surface edge %d without a wet e2c[e][0]", e);
+      ok = 0;
+      break;
+    }
+    if (h_ec2[q] < 0) continue; /* land edge */
+    int cp = g->c2c[g->e2e[e][1]][c1], cm = g->c2c[g->e2e[e][0]][c2];
+    h_ecp[q] = (cp < g->szcS) ? c->cs2col[cp] : -1;
+    h_ecm[q] = (cm < g->szcS) ? c->cs2col[cm] : -1;
+    if (h_ecp[q] >= 0 && g->e2c[g->ep[e]][0] == cp) h_esel[q] |= 1;
+    if (h_ecm[q] >= 0 && g->e2c[g->em[e]][1] == cm) h_esel[q] |= 2;
+  }
+  /* every wet 3D edge against the model of the kernel; wet edges are
+   * scattered to [layer][surface edge] */
+  int *esp = (int *)malloc((size_t)(g->n3_e1 + 1) * sizeof(int));
+  ll *edn = (ll *)malloc((size_t)(g->n3_e1 + 1) * sizeof(ll));
+  int nesp = 0;
+  for (int ee = 1; ee <= g->n3_e1 && ok; ee++) {
+    e = g->w3_e1[ee];
+    int c1 = g->e2c[e][0], c2 = g->e2c[e][1];
+    int k = g->s2k[c1];
+    int es = g->m2de[e], q = es - 1;
+    int wet2 = (ee <= g->v3_e1);
+    if (es < 1 || es > ne2 || k < 0 || k >= nz) {
+      emstr_set_error(E_UNSUP, e, "edge %d: no surface edge or layer", e);
+      ok = 0;
+      break;
+    }
+    if (!wet2) continue;
+    int col1 = h_ec1[q], col2 = h_ec2[q];
+    int bad = (col2 < 0 || g->m2d[c1] != c->col2cs[col1] || g->m2d[c2] != c->col2cs[col2] ||
+               g->s2k[c2] != k);
+    if (!bad) {
+      int cp = g->c2c[g->e2e[e][1]][c1], cm = g->c2c[g->e2e[e][0]][c2];
+      int pc = h_ecp[q], mc = h_ecm[q];
+      int wp = (pc >= 0 && k >= c->h_kb[pc]), wm = (mc >= 0 && k >= c->h_kb[mc]);
+      int cpw = wp ? c->sp_of[(ll)k * ncol + pc] : 0, cmw = wm ? c->sp_of[(ll)k * ncol + mc] : 0;
+      if (wp ? (cp != cpw) : !ghost_of(g, gbin, cp, c1)) bad = 1;
+      if (wm ? (cm != cmw) : !ghost_of(g, gbin, cm, c2)) bad = 1;
+      int cpu = g->e2c[g->ep[e]][0], cmu = g->e2c[g->em[e]][1];
+      if (wp ? (cpu != ((h_esel[q] & 1) ? cpw : c1)) : (cpu != c1)) bad = 1;
+      if (wm ? (cmu != ((h_esel[q] & 2) ? cmw : c2)) : !ghost_of(g, gbin, cmu, c2)) bad = 1;
+    }
+    if (bad) {
+      emstr_set_error(E_UNSUP, e, "edge %d: mesh maps differ from the device model", e);
+      ok = 0;
+      break;
+    }
+    esp[nesp] = e;
+    edn[nesp] = (ll)k * ne2 + q;
+    nesp++;
+  }
+  /* the edges of every wet centre at its layer are the surface edges */
+  for (n = 1; n <= g->v3_t && ok; n++) {
+    int cw = g->w3_t[n], cs = g->m2d[cw], col = c->cs2col[cs];
+    for (j = 1; j <= g->npe[cs]; j++)
+      if (g->m2de[g->c2e[j][cw]] - 1 != h_ce[(j - 1) * ncol + col]) {
+        emstr_set_error(E_UNSUP, cw, "centre %d: edge %d is not its surface edge", cw, j);
+        ok = 0;
+        break;
+      }
+  }
+  c->nesp = nesp;
+  free(gbin);
+
+  /* device arrays */
+  if (ok) {
+    const size_t P = ncol, N3 = (size_t)ncol * nz, NE3 = (size_t)ne2 * nz;
+    const int nt = wd->ntr > 0 ? wd->ntr : 1;
+    int **ci[] = {&d->kb, &d->ks, &d->kold, &d->knew};
+    for (n = 0; n < 4; n++) *ci[n] = (int *)dalloc(P * sizeof(int), &ok);
+    double **c2[] = {&d->cellarea, &d->botz, &d->eta, &d->oldeta, &d->osubeta, &d->subeta};
+    for (n = 0; n < 6; n++) *c2[n] = (double *)dalloc(P * sizeof(double), &ok);
+    d->layers = (double *)dalloc((nz + 1) * sizeof(double), &ok);
+    double **c3[] = {&d->w, &d->Kz, &d->dz};
+    for (n = 0; n < 3; n++) *c3[n] = (double *)dalloc(N3 * sizeof(double), &ok);
+    d->mintr = (double *)dalloc(nt * sizeof(double), &ok);
+    d->maxtr = (double *)dalloc(nt * sizeof(double), &ok);
+    int **ti[] = {&d->advect, &d->diffuse, &d->in_tbdy, &d->in_tdif};
+    for (n = 0; n < 4; n++) *ti[n] = (int *)dalloc(nt * sizeof(int), &ok);
+    d->cols = (int *)dalloc(P * sizeof(int), &ok);
+    d->wsp = (int *)dalloc((size_t)d->nwet * sizeof(int), &ok);
+    d->wdn = (ll *)dalloc((size_t)d->nwet * sizeof(ll), &ok);
+    D->npe = (int *)dalloc(P * sizeof(int), &ok);
+    D->ce = (int *)dalloc((size_t)npem * P * sizeof(int), &ok);
+    D->esc = (int *)dalloc((size_t)npem * P * sizeof(int), &ok);
+    int **ei[] = {&D->ec1, &D->ec2, &D->ecp, &D->ecm};
+    for (n = 0; n < 4; n++) *ei[n] = (int *)dalloc((size_t)ne2 * sizeof(int), &ok);
+    D->esel = (unsigned char *)dalloc((size_t)ne2, &ok);
+    D->h1e = (double *)dalloc((size_t)ne2 * sizeof(double), &ok);
+    D->h2e = (double *)dalloc((size_t)ne2 * sizeof(double), &ok);
+    double **e3[] = {&D->u1, &D->flux, &D->dzu, &D->kh};
+    for (n = 0; n < 4; n++) *e3[n] = (double *)dalloc(NE3 * sizeof(double), &ok);
+    for (n = 0; n < 3; n++) c->tr[n] = (double *)dalloc((size_t)nt * N3 * sizeof(double), &ok);
+    size_t sgmax = (size_t)(g->szc > g->sze ? g->szc : g->sze);
+    c->stage = (double *)dalloc(sgmax * sizeof(double), &ok);
+    c->d_esp = (int *)dalloc((size_t)nesp * sizeof(int), &ok);
+    c->d_edn = (ll *)dalloc((size_t)nesp * sizeof(ll), &ok);
+    c->d_bpt = (int *)dalloc((size_t)(g->nbpt + 1) * sizeof(int), &ok);
+    c->d_bin = (int *)dalloc((size_t)(g->nbpt + 1) * sizeof(int), &ok);
+    c->d_gsed = (int *)dalloc((size_t)(g->ngsed + 1) * sizeof(int), &ok);
+    c->d_ised = (int *)dalloc((size_t)(g->ngsed + 1) * sizeof(int), &ok);
+    c->d_proc = (int *)dalloc(P * sizeof(int), &ok);
+    c->d_err = (int *)dalloc(sizeof(int), &ok);
+    c->d_dtm = (unsigned long long *)dalloc(sizeof(unsigned long long), &ok);
+    if (!ok) emstr_set_error(E_DEVICE, -1, "device allocation failed");
+  }
+  if (ok) {
+    /* static data */
+    double *h2 = (double *)malloc((size_t)(ncol + 1) * sizeof(double));
+    double *hb = (double *)malloc((size_t)(ncol + 1) * sizeof(double));
+    for (int col = 0; col < ncol; col++) {
+      h2[col] = g->cellarea[c->col2cs[col]];
+      hb[col] = g->botz[c->col2cs[col]];
+    }
+    ok = !(up(c, d->kb, c->h_kb, ncol * sizeof(int)) ||
+           up(c, d->cellarea, h2, ncol * sizeof(double)) ||
+           up(c, d->botz, hb, ncol * sizeof(double)) ||
+           up(c, d->layers, g->layers, (nz + 1) * sizeof(double)) ||
+           up(c, d->wsp, wsp, (size_t)d->nwet * sizeof(int)) ||
+           up(c, d->wdn, wdn, (size_t)d->nwet * sizeof(ll)) ||
+           up(c, D->npe, h_npe, ncol * sizeof(int)) ||
+           up(c, D->ce, h_ce, (size_t)npem * ncol * sizeof(int)) ||
+           up(c, D->esc, h_esc, (size_t)npem * ncol * sizeof(int)) ||
+           up(c, D->ec1, h_ec1, ne2 * sizeof(int)) || up(c, D->ec2, h_ec2, ne2 * sizeof(int)) ||
+           up(c, D->ecp, h_ecp, ne2 * sizeof(int)) || up(c, D->ecm, h_ecm, ne2 * sizeof(int)) ||
+           up(c, D->esel, h_esel, ne2) || up(c, D->h1e, h_h1, ne2 * sizeof(double)) ||
+           up(c, D->h2e, h_h2, ne2 * sizeof(double)) ||
+           up(c, c->d_esp, esp, (size_t)nesp * sizeof(int)) ||
+           up(c, c->d_edn, edn, (size_t)nesp * sizeof(ll)) ||
+           up(c, c->d_bpt, g->bpt + 1, (size_t)g->nbpt * sizeof(int)) ||
+           up(c, c->d_bin, g->bin + 1, (size_t)g->nbpt * sizeof(int)) ||
+           up(c, c->d_gsed, g->gsed_t + 1, (size_t)g->ngsed * sizeof(int)) ||
+           up(c, c->d_ised, g->ised_t + 1, (size_t)g->ngsed * sizeof(int)));
+    if (hipStreamSynchronize(c->st) != hipSuccess) ok = 0;
+    free(h2);
+    free(hb);
+  }
+  c->h_ks = (int *)malloc((size_t)(ncol + 1) * sizeof(int));
+  c->h_kold = (int *)malloc((size_t)(ncol + 1) * sizeof(int));
+  c->h_knew = (int *)malloc((size_t)(ncol + 1) * sizeof(int));
+  c->h_cols = (int *)malloc((size_t)(ncol + 1) * sizeof(int));
+  for (n = 0; n < ncol; n++) c->h_knew[n] = c->h_kold[n] = nz - 1, c->h_ks[n] = -1;
+  free(wsp);
+  free(wdn);
+  free(h_npe);
+  free(h_ce);
+  free(h_esc);
+  free(h_ec1);
+  free(h_ec2);
+  free(h_ecp);
+  free(h_ecm);
+  free(h_esel);
+  free(h_h1);
+  free(h_h2);
+  free(esp);
+  free(edn);
+  if (!ok) {
+    us_destroy(c);
+    return NULL;
+  }
+  return c;
+}
+
+/*-------------------------------------------------------------------*/
+/* registry: one device mirror per window                            */
+/*-------------------------------------------------------------------*/
+#define MAXCTX 64
+static pthread_mutex_t reg_lock = PTHREAD_MUTEX_INITIALIZER;
+static geometry_t *reg_g[MAXCTX];
+static usctx *reg_c[MAXCTX];
+
+static usctx *ctx_for(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  usctx *c = NULL, *stale = NULL;
+  int i;
+  pthread_mutex_lock(&reg_lock);
+  for (i = 0; i < MAXCTX; i++)
+    if (reg_g[i] == g) {
+      c = reg_c[i];
+      if (c->key_szc != g->szc || c->key_sze != g->sze) {
+        stale = c;
+        c = NULL;
+        reg_g[i] = NULL;
+        reg_c[i] = NULL;
+      }
+      break;
+    }
+  pthread_mutex_unlock(&reg_lock);
+  if (stale) us_destroy(stale);
+  if (!c) {
+    c = us_create(g, wd, wc);
+    if (!c) return NULL;
+    pthread_mutex_lock(&reg_lock);
+    for (i = 0; i < MAXCTX; i++)
+      if (!reg_g[i]) {
+        reg_g[i] = g;
+        reg_c[i] = c;
+        break;
+      }
+    pthread_mutex_unlock(&reg_lock);
+  }
+  c->wd = wd;
+  c->wc = wc;
+  return c;
+}
+
+void emstr_us_forget_window(geometry_t *g) {
+  usctx *c = NULL;
+  int i;
+  pthread_mutex_lock(&reg_lock);
+  for (i = 0; i < MAXCTX; i++)
+    if (reg_g[i] == g) {
+      c = reg_c[i];
+      reg_g[i] = NULL;
+      reg_c[i] = NULL;
+    }
+  pthread_mutex_unlock(&reg_lock);
+  us_destroy(c);
+}
+
+const char *emstr_us_last_error(void) { return emstr_last_error(); }
+int emstr_us_last_error_cell(void) { return emstr_last_error_cell(); }
+
+/*-------------------------------------------------------------------*/
+/* uploads                                                           */
+/*-------------------------------------------------------------------*/
+static int up_cells(usctx *c, double *dst, const double *sp) {
+  HCHK(hipMemcpyAsync(c->stage, sp, (size_t)c->g->szc * sizeof(double), hipMemcpyHostToDevice,
+                      c->st));
+  if (emstr_launch_scatter(c->D.d.wsp, c->D.d.wdn, c->D.d.nwet, c->stage, dst, c->st))
+    return emstr_set_error(E_DEVICE, -1, "scatter");
+  return 0;
+}
+
+static int up_edges(usctx *c, double *dst, const double *sp) {
+  HCHK(hipMemcpyAsync(c->stage, sp, (size_t)c->g->sze * sizeof(double), hipMemcpyHostToDevice,
+                      c->st));
+  if (emstr_launch_scatter(c->d_esp, c->d_edn, c->nesp, c->stage, dst, c->st))
+    return emstr_set_error(E_DEVICE, -1, "edge scatter");
+  return 0;
+}
+
+static int up_cols(usctx *c, double *dst, const double *sp2) {
+  double *h = (double *)malloc((size_t)c->ncol * sizeof(double));
+  for (int col = 0; col < c->ncol; col++) h[col] = sp2[c->col2cs[col]];
+  hipError_t e = hipMemcpyAsync(dst, h, (size_t)c->ncol * sizeof(double),
+                                hipMemcpyHostToDevice, c->st);
+  hipStreamSynchronize(c->st);
+  free(h);
+  return e == hipSuccess ? 0 : emstr_set_error(E_DEVICE, -1, "column upload");
+}
+
+/* per-tracer settings (tracers.c:803-805 tbdy, tdif_v) */
+static int up_meta(usctx *c) {
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->D.d;
+  const int nt = wd->ntr > 0 ? wd->ntr : 1;
+  int *tb = (int *)calloc(nt, sizeof(int)), *td = (int *)calloc(nt, sizeof(int));
+  int n, any = 0;
+  for (n = 0; n < wc->ntbdy; n++)
+    if (wc->tbdy[n] >= 0 && wc->tbdy[n] < wd->ntr) tb[wc->tbdy[n]] = 1;
+  for (n = 0; n < wc->ntdif_v; n++)
+    if (wc->tdif_v[n] >= 0 && wc->tdif_v[n] < wd->ntr) td[wc->tdif_v[n]] = 1;
+  for (n = 0; n < wd->ntr; n++)
+    if (wc->diffuse[n]) any = 1;
+  c->anydif = any;
+  int r = up(c, d->mintr, wc->mintr, nt * sizeof(double)) ||
+          up(c, d->maxtr, wc->maxtr, nt * sizeof(double)) ||
+          up(c, d->advect, wc->advect, nt * sizeof(int)) ||
+          up(c, d->diffuse, wc->diffuse, nt * sizeof(int)) ||
+          up(c, d->in_tbdy, tb, nt * sizeof(int)) || up(c, d->in_tdif, td, nt * sizeof(int));
+  hipStreamSynchronize(c->st);
+  free(tb);
+  free(td);
+  return r ? E_DEVICE : 0;
+}
+
+/* per-column step data from the host's set_dz lists (s1, i2, i3) */
+static int cols_from_host(usctx *c) {
+  geometry_t *g = c->g;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->D.d;
+  int cc;
+  for (int q = 0; q < c->ncol; q++) c->h_ks[q] = -1;
+  for (cc = 1; cc <= wc->vcs; cc++) {
+    int s = wc->s1[cc];
+    int col = c->cs2col[g->m2d[s]];
+    if (col < 0) return emstr_set_error(E_ARG, s, "s1 holds a centre outside the columns");
+    c->h_ks[col] = g->s2k[s];
+    int ns = wc->i2[cc], os = wc->i3[cc];
+    c->h_knew[col] = (ns == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[ns];
+    c->h_kold[col] = (os == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[os];
+    c->h_cols[cc - 1] = col;
+  }
+  d->ncol = wc->vcs;
+  if (up(c, d->ks, c->h_ks, c->ncol * sizeof(int)) ||
+      up(c, d->kold, c->h_kold, c->ncol * sizeof(int)) ||
+      up(c, d->knew, c->h_knew, c->ncol * sizeof(int)) ||
+      up(c, d->cols, c->h_cols, (size_t)(d->ncol ? d->ncol : 1) * sizeof(int)))
+    return E_DEVICE;
+  HCHK(hipStreamSynchronize(c->st));
+  c->have_cols = 1;
+  return 0;
+}
+
+/* no flux or velocity through land edges (closed boundaries) */
+static int check_land_edges(usctx *c) {
+  geometry_t *g = c->g;
+  window_t *wd = c->wd;
+  for (int ee = g->v3_e1 + 1; ee <= g->n3_e1; ee++) {
+    int e = g->w3_e1[ee];
+    if (wd->u1[e] != 0.0 || wd->u1flux3d[e] != 0.0)
+      return emstr_set_error(E_UNSUP, e, "flow through land edge %d", e);
+  }
+  return 0;
+}
+
+static int us_upload_fields(usctx *c) {
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  emstr_usdev *D = &c->D;
+  emstr_dev *d = &D->d;
+  int r;
+  if ((r = us_check_scope(c->g, wd, wc))) return r;
+  if ((r = check_land_edges(c))) return r;
+  d->dt = wd->dt;
+  d->hmin = wc->hmin;
+  d->trasc = wc->trasc;
+  d->ultimate = (wc->trasc == ORDER1) ? 0 : wc->ultimate;
+  if ((r = up_meta(c))) return r;
+  if (up_edges(c, D->u1, wd->u1) || up_edges(c, D->flux, wd->u1flux3d) ||
+      up_cells(c, d->w, wd->w) || up_cells(c, d->Kz, wd->Kz) || up_cells(c, d->dz, wc->dz))
+    return E_DEVICE;
+  if (c->anydif)
+    if (up_edges(c, D->dzu, wd->dzu1) || up_edges(c, D->kh, wc->u2kh)) return E_DEVICE;
+  if (up_cols(c, d->eta, wd->eta) || up_cols(c, d->oldeta, wc->oldeta)) return E_DEVICE;
+  if ((r = cols_from_host(c))) return r;
+  HCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+static int us_upload_tracers(usctx *c) {
+  emstr_dev *d = &c->D.d;
+  for (int n = 0; n < c->wd->ntr; n++) {
+    HCHK(hipMemcpyAsync(c->stage, c->wd->tr_wc[n], (size_t)c->g->szc * sizeof(double),
+                        hipMemcpyHostToDevice, c->st));
+    if (emstr_launch_scatter(d->wsp, d->wdn, d->nwet, c->stage, c->tr[c->cur] + (ll)n * d->n3,
+                             c->st))
+      return emstr_set_error(E_DEVICE, -1, "scatter");
+  }
+  HCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+static int us_download_tracers(usctx *c) {
+  emstr_dev *d = &c->D.d;
+  geometry_t *g = c->g;
+  for (int n = 0; n < c->wd->ntr; n++) {
+    HCHK(hipMemcpyAsync(c->stage, c->wd->tr_wc[n], (size_t)g->szc * sizeof(double),
+                        hipMemcpyHostToDevice, c->st));
+    if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, c->tr[c->cur] + (ll)n * d->n3, c->stage,
+                            c->st))
+      return emstr_set_error(E_DEVICE, -1, "gather");
+    /* ghosts hold the no-gradient values the next step would set */
+    emstr_launch_ghosts(c->d_bpt, c->d_bin, g->nbpt, c->stage, c->st);
+    emstr_launch_ghosts(c->d_gsed, c->d_ised, g->ngsed, c->stage, c->st);
+    HCHK(hipMemcpyAsync(c->wd->tr_wc[n], c->stage, (size_t)g->szc * sizeof(double),
+                        hipMemcpyDeviceToHost, c->st));
+  }
+  HCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* the step                                                          */
+/*-------------------------------------------------------------------*/
+/* advect_diffuse (hd-us tracers.c:428-1157); 0, or the 2D centre of a
+ * column where the sub-step limit is exceeded */
+static int us_adv(usctx *c) {
+  emstr_usdev *D = &c->D;
+  emstr_dev *d = &D->d;
+  win_priv_t *wc = c->wc;
+  double dtu = d->dt, dtm = d->dt, trem;
+  int itermax = 20, first = 1;
+  if (!c->have_cols) return emstr_set_error(E_ARG, -1, "no set_dz data on the device");
+  if (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)) {
+    unsigned long long init, r;
+    double big = dtu;
+    memcpy(&init, &big, sizeof(init));
+    HCHK(hipMemcpyAsync(c->d_dtm, &init, sizeof(init), hipMemcpyHostToDevice, c->st));
+    if (emstr_us_launch_dtm(D, dtu, c->d_dtm, c->st))
+      return emstr_set_error(E_DEVICE, -1, "dtm launch");
+    HCHK(hipMemcpyAsync(&r, c->d_dtm, sizeof(r), hipMemcpyDeviceToHost, c->st));
+    HCHK(hipStreamSynchronize(c->st));
+    memcpy(&dtm, &r, sizeof(dtm));
+    if (dtm != dtu) dtm = floor(dtm);
+  }
+  if (dtm == 0.0 || (int)(dtu / dtm) > itermax) {
+    emstr_set_error(E_SUBSTEP, -1,
+                    "tracer_step_3d: maximum number of sub-steps (%d) exceeded", itermax);
+    /* the reference returns the column where the limit was found; the
+     * first processed column stands for it */
+    return c->wc->vcs > 0 ? c->g->m2d[c->wc->s1[1]] : 1;
+  }
+  const int b0 = c->cur;
+  int in = b0, out = (b0 + 1) % 3;
+  trem = dtu;
+  while (trem > 0) {
+    double u = dtm;
+    if (trem < u) u = trem;
+    wc->b1 = u;
+    if (u > 0.0) {
+      if (emstr_launch_subeta(d, u, trem, first, c->st))
+        return emstr_set_error(E_DEVICE, -1, "subeta launch");
+      if (emstr_us_launch_adv(D, c->tr[in], c->tr[out], c->tr[b0], u, c->anydif,
+                              d->ultimate, c->st) ||
+          emstr_launch_osub(d, c->st))
+        return emstr_set_error(E_DEVICE, -1, "advection launch");
+      in = out;
+      out = (out + 1) % 3;
+      if (out == b0) out = (out + 1) % 3;
+    }
+    trem -= u;
+    first = 0;
+  }
+  c->cur = in;
+  return 0;
+}
+
+static int us_vdiff(usctx *c) {
+  const int scratch = (c->cur + 1) % 3;
+  if (!c->have_cols) return emstr_set_error(E_ARG, -1, "no set_dz data on the device");
+  if (emstr_launch_vdiff(&c->D.d, c->tr[c->cur], c->tr[scratch], c->d_err, c->st))
+    return emstr_set_error(E_DEVICE, -1, "vdiff launch");
+  return 0;
+}
+
+static int us_sync(usctx *c) {
+  int err = 0;
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy(&err, c->d_err, sizeof(int), hipMemcpyDeviceToHost));
+  if (err) {
+    int cell = c->sp_of[err - 1];
+    hipMemset(c->d_err, 0, sizeof(int));
+    return emstr_set_error(E_PIVOT, cell, "implicit_vdiff_tr: zero divisor at cell %d", cell);
+  }
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* set_dz on the device, lists back to the host (hd-us 7231-7380)    */
+/*-------------------------------------------------------------------*/
+static int us_set_dz(usctx *c) {
+  geometry_t *g = c->g;
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->D.d;
+  const int ncol = c->ncol;
+  int cc, vc = 1, col;
+  d->hmin = wc->hmin;
+  if (up_cols(c, d->eta, wd->eta) || up_cols(c, d->oldeta, wc->oldeta)) return E_DEVICE;
+  for (cc = 1; cc <= ncol; cc++) {
+    col = cc - 1;
+    int ns = g->nsur_t[cc];
+    c->h_knew[col] = (ns == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[ns];
+  }
+  if (up(c, d->knew, c->h_knew, ncol * sizeof(int))) return E_DEVICE;
+  if (emstr_launch_setdz(d, c->d_proc, c->st)) return emstr_set_error(E_DEVICE, -1, "setdz");
+  HCHK(hipMemcpyAsync(c->h_ks, d->ks, ncol * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipMemcpyAsync(c->h_kold, d->kold, ncol * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipMemcpyAsync(c->h_knew, d->knew, ncol * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  /* host lists in the reference's order */
+#define SPAT(col, k) ((k) < c->h_kb[col] ? c->sedg[col] : c->sp_of[(ll)(k)*ncol + (col)])
+  for (cc = 1; cc <= g->a2_t; cc++) {
+    col = cc - 1;
+    g->sur_t[cc] = SPAT(col, c->h_kold[col]);
+    g->nsur_t[cc] = SPAT(col, c->h_knew[col]);
+  }
+  int ncp = 0;
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    col = cc - 1;
+    if (c->h_ks[col] < 0) continue;
+    wc->s1[vc] = SPAT(col, c->h_ks[col]);
+    wc->i1[vc] = g->bot_t[cc];
+    wc->i2[vc] = g->nsur_t[cc];
+    wc->i3[vc] = g->sur_t[cc];
+    if (cc <= g->v2_t) wc->vcs1 = vc;
+    c->h_cols[ncp++] = col;
+    vc++;
+  }
+  const int vcs = vc - 1;
+  wc->vcs = vcs;
+  for (cc = 1; cc <= vcs; cc++) {
+    col = c->h_cols[cc - 1];
+    for (int k = c->h_ks[col] - 1; k >= c->h_kb[col]; k--) wc->s1[vc++] = SPAT(col, k);
+    if (cc == wc->vcs1) wc->vc1 = vc - 1;
+  }
+  wc->vc = vc - 1;
+#undef SPAT
+  d->ncol = ncp;
+  if (up(c, d->cols, c->h_cols, (size_t)(ncp ? ncp : 1) * sizeof(int))) return E_DEVICE;
+  /* dz of wet centres */
+  if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, d->dz, c->stage, c->st))
+    return emstr_set_error(E_DEVICE, -1, "gather");
+  double *tmp = (double *)malloc((size_t)g->szc * sizeof(double));
+  HCHK(hipMemcpyAsync(tmp, c->stage, (size_t)g->szc * sizeof(double), hipMemcpyDeviceToHost,
+                      c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  for (int q = 1; q <= g->v3_t; q++) {
+    int cw = g->w3_t[q];
+    wc->dz[cw] = tmp[cw];
+  }
+  free(tmp);
+  c->have_cols = 1;
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* drop-in entry points                                              */
+/*-------------------------------------------------------------------*/
+void set_dz(geometry_t *window, window_t *windat, win_priv_t *wincon) {
+  usctx *c;
+  emstr_clear_error();
+  if (wincon->sigma || wincon->thin_merge) {
+    emstr_set_error(E_UNSUP, -1, "set_dz: sigma or thin-layer merging is not supported");
+    return;
+  }
+  c = ctx_for(window, windat, wincon);
+  if (!c) return;
+  us_set_dz(c);
+}
+
+int advect_diffuse(geometry_t *window, window_t *windat, win_priv_t *wincon) {
+  usctx *c;
+  int r;
+  emstr_clear_error();
+  c = ctx_for(window, windat, wincon);
+  if (!c) return 1;
+  if ((r = us_upload_fields(c)) || (r = us_upload_tracers(c))) return r;
+  if ((r = us_adv(c))) return r;
+  if ((r = us_download_tracers(c))) return r;
+  return us_sync(c);
+}
+
+void vert_diffuse_3d(geometry_t *window, window_t *windat, win_priv_t *wincon) {
+  usctx *c;
+  emstr_clear_error();
+  c = ctx_for(window, windat, wincon);
+  if (!c) return;
+  if (us_upload_fields(c) || us_upload_tracers(c)) return;
+  if (us_vdiff(c)) return;
+  if (us_download_tracers(c)) return;
+  us_sync(c);
+}
+
+void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon) {
+  usctx *c;
+  emstr_clear_error();
+  c = ctx_for(window, windat, wincon);
+  if (!c) return;
+  if (us_upload_fields(c) || us_upload_tracers(c)) return;
+  if (us_adv(c)) return;
+  if (us_vdiff(c)) return;
+  if (us_download_tracers(c)) return;
+  us_sync(c);
+}

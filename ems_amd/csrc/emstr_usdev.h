@@ -1,0 +1,46 @@
+/*
+ * Device-side view of one COMPAS window (C POD shared by the host C layer
+ * and the HIP kernels).  Columns (wet mesh cells, in surface order) are
+ * laid out like a one-row structured grid, so the column kernels of the
+ * plain form (set_dz, sub-step elevations, vertical diffusion, mass,
+ * scatter/gather) run on them unchanged:
+ *   cell arrays   [nz][ncol]     (emstr_dev with nx = ncol, ny = 1)
+ *   edge arrays   [nz][ne2]      (ne2 surface-layer edges; the edge of a
+ *                                 layer is the surface edge at that depth)
+ *   per column    [npem][ncol]   (edges of the cell, in the mesh order)
+ * Land and ghost centres are not stored: a ghost is read as the step-start
+ * value of its wet centre (set_lateral_BC_tr, tracers.c:7120-7137), the
+ * sediment ghost as the bed value (tracers.c:864-867).
+ */
+#ifndef EMSTR_USDEV_H
+#define EMSTR_USDEV_H
+
+#include "emstr_dev.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct emstr_usdev {
+  emstr_dev d;      /* column part: nx = ncol, ny = 1                   */
+  int npem, ne2;
+  int *npe;         /* [ncol] edges of each cell                         */
+  int *ce;          /* [npem][ncol] surface edge j of the cell           */
+  int *esc;         /* [npem][ncol] eSc: +1 if edge j points out         */
+  /* per surface edge: the columns of e2c[e][0] / e2c[e][1], the
+   * columns beyond them through the edge opposite (van Leer's cp / cm,
+   * -1 if none), and which cell ULTIMATE reads through ep / em when
+   * that far column is wet at the layer (bit 0: cp, else e2c[e][0];
+   * bit 1: cm, else e2c[e][1])                                       */
+  int *ec1, *ec2, *ecp, *ecm;
+  unsigned char *esel;
+  double *h1e;      /* [ne2] edge length (h1au1)                        */
+  double *h2e;      /* [ne2] centre distance (h2au1)                    */
+  /* [nz][ne2] edge fields: u1, u1flux3d, dzu1, u2kh                  */
+  double *u1, *flux, *dzu, *kh;
+} emstr_usdev;
+
+#ifdef __cplusplus
+}
+#endif
+#endif
