@@ -8,9 +8,9 @@ from ems_amd import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+def declared_functions(headers=("emstr.h", "emstr_grid.h")):
     names = set()
-    for h in ("emstr.h", "emstr_grid.h"):
+    for h in headers:
         src = open(os.path.join(ROOT, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\([^;{]*\)\s*;", src, re.M):
@@ -37,3 +37,17 @@ def test_prototype_table_matches_header():
     names = declared_functions()
     table = {p[0] for p in _lib.PROTOS}
     assert names == table, (names ^ table)
+
+
+US_HEADERS = ("emstr_us.h", "emstr_mesh.h")
+
+
+def test_compas_library_exports_all_declared_symbols():
+    from ems_amd import mesh
+    names = declared_functions(US_HEADERS)
+    for n in ("tracer_step_3d", "advect_diffuse", "vert_diffuse_3d", "set_dz",
+              "emstr_mesh_build", "emstr_us_forget_window"):
+        assert n in names
+    L = C.CDLL(mesh.LIBPATH)
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
