@@ -14,6 +14,11 @@ Prints one JSON line (rank 0, N = 1).  --gpus N with N > 1 exits: this
 build runs on one GPU.
 
   --forms plain,precision,batched   which forms to run (profiling runs)
+  --workload c3-ffsl                the same basin with the structured FFSL
+                                    scheme (plain FP64 form; 1 tracer line)
+  --workload c4-compas              the COMPAS path on the 350x350 hexagon
+                                    mesh, 50 layers, 60 tracers, van Leer
+                                    (plain FP64 form)
   --copy-probe MB                   time a plain device copy of MB MiB and
                                     exit (calibrates the HBM byte counters)
 """
@@ -132,6 +137,7 @@ def main():
     ap.add_argument("--forms", default="plain,precision,batched")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0)
+    ap.add_argument("--workload", default="c3", choices=["c3", "c3-ffsl", "c4-compas"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the tracer step on one GPU only; "
@@ -140,6 +146,10 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.workload == "c3-ffsl":
+        return bench_ffsl(args)
+    if args.workload == "c4-compas":
+        return bench_compas(args)
     forms = set(args.forms.split(","))
 
     from ems_amd import _lib, configs
@@ -284,6 +294,111 @@ def main():
                                "cores": nth, "kind": "port",
                                "sample": "%d full c3 steps (all %d tracers), OpenMP over "
                                          "tracers" % (n, ntr)}
+    print(json.dumps(out))
+
+
+def _cpu_baseline(step_fn, cu, seconds, nth, what):
+    t = time.time()
+    n = 0
+    while True:
+        step_fn()
+        n += 1
+        if time.time() - t > seconds or n >= 50:
+            break
+    dt = time.time() - t
+    return {"value": cu * n / dt, "unit": "cell-updates/s", "cores": nth, "kind": "port",
+            "sample": "%d full %s steps, OpenMP over tracers" % (n, what)}
+
+
+def bench_ffsl(args):
+    """The structured FFSL scheme (plain FP64 form) on the c3 basin."""
+    from ems_amd import _lib, configs, structs as S
+    from ems_amd import tracer as T
+    L = _lib.lib()
+    case = configs.c3(seed=args.seed, ntr=args.ntr)
+    case.wc.trasc = S.FFSL
+    case.wc.ultimate = 0
+    case.wd.nstep = 1
+    L.set_dz(case.gp, case.wp, case.pp)
+    if _lib.last_error()[0]:
+        raise RuntimeError("set_dz: %s" % (_lib.last_error(),))
+    L.emstr_forget_window(case.gp)
+    ev = HipEvents()
+    K, W = min(args.steps, 20), min(args.warmup, 3)
+    c = T.Context(case, T.PLAIN_F64)
+    c.upload_fields()
+    c.vel_w()
+    c.upload()
+    c.step(W)
+    ms = timed_steps(c, ev, K) / K
+    c.close()
+    nwet, ntr = case.nwet, case.ntr
+    cu = float(nwet) * ntr
+    out = {"metric": "tracer cell-updates/s on 1 GPU (structured FFSL, FP64 plain)",
+           "value": cu / (ms * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
+           "warmup": W, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded c3 generator, seed %d)" % args.seed,
+           "config": {"workload": "c3 basin, FFSL: 400x600x40, %d tracers, dt=90 s" % ntr,
+                      "n3_wet": nwet, "ntr": ntr},
+           "step_hbm_fraction": (nwet * ntr * 16 + nwet * 56) / (ms * 1e-3) / 1e9 /
+           HBM_PEAK_GBS}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        nth = min(16, os.cpu_count() or 1)
+        O.set_dz(case.gp, case.wp, case.pp)
+        out["cpu_baseline"] = _cpu_baseline(lambda: O.step(case, nth), cu, args.cpu_seconds,
+                                            nth, "c3 FFSL")
+    print(json.dumps(out))
+
+
+def bench_compas(args):
+    """The COMPAS path (plain FP64 form) on the C4 hexagon mesh."""
+    from ems_amd import mesh as MS
+    t0 = time.time()
+    case = MS.c4(seed=args.seed)
+    print("bench: c4 built in %.1f s: %d wet cells x %d tracers, %d wet edges"
+          % (time.time() - t0, case.nwet, case.ntr, case.g.v3_e1), file=sys.stderr, flush=True)
+    LU = MS.lib()
+    LU.set_dz(case.gp, case.wp, case.pp)
+    if MS.last_error()[0]:
+        raise RuntimeError("set_dz: %s" % (MS.last_error(),))
+    LU.emstr_us_forget_window(case.gp)
+    case.w_from_continuity()
+    ev = HipEvents()
+    K, W = min(args.steps, 50), min(args.warmup, 5)
+    c = MS.Context(case)
+    c.upload()
+    c.step(W)
+    c.sync()
+    a, b = ev.event(), ev.event()
+    ev.record(a, c.stream)
+    c.step(K)
+    ev.record(b, c.stream)
+    ms = ev.elapsed_ms(a, b) / K
+    c.sync()
+    c.close()
+    nwet, ntr, ne = case.nwet, case.ntr, case.g.v3_e1
+    cu = float(nwet) * ntr
+    out = {"metric": "tracer cell-updates/s on 1 GPU (COMPAS, FP64 plain)",
+           "value": cu / (ms * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
+           "warmup": W, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded C4 hexagon generator, seed %d)" % args.seed,
+           "config": {"workload": "c4 mesh: 350x350 jittered hexagons, 1.5 km edges, 50 "
+                                  "layers, 3-400 m, %d tracers, dt=180 s, van Leer "
+                                  "(COMPAS FFSL is not built)" % ntr,
+                      "n3_wet": nwet, "wet_edges": ne, "ntr": ntr},
+           # per step: every tracer read and written once (plain FP64 in and
+           # out), w, Kz, dz per cell and u1, u1flux3d per edge
+           "step_hbm_fraction": (nwet * ntr * 16 + nwet * 24 + ne * 16) / (ms * 1e-3) /
+           1e9 / HBM_PEAK_GBS}
+    if not args.no_cpu:
+        from oracle import oracle_us as OU
+        nth = min(16, os.cpu_count() or 1)
+        OU.set_dz(case.gp, case.wp, case.pp)
+        out["cpu_baseline"] = _cpu_baseline(lambda: OU.step(case, nth), cu, args.cpu_seconds,
+                                            nth, "c4 COMPAS")
     print(json.dumps(out))
 
 

@@ -336,6 +336,7 @@ emstr_mesh *emstr_mesh_build(int nmc, int npem, const int *npe, const int *nbr,
   /* metrics and fields (filled by the caller) */
   g->h1au1 = (double *)zalloc(szeS, sizeof(double));
   g->h2au1 = (double *)zalloc(szeS, sizeof(double));
+  g->h1acell = (double *)zalloc(szeS, sizeof(double));
   g->costhu1 = (double *)zalloc(szeS, sizeof(double));
   g->sinthu1 = (double *)zalloc(szeS, sizeof(double));
   g->cellarea = (double *)zalloc(szcS, sizeof(double));
@@ -435,7 +436,7 @@ void emstr_mesh_free(emstr_mesh *M) {
     void *p[] = {g->layers, g->s2i, g->s2k, g->npe, g->m2d, g->m2de, g->zp1, g->zm1,
                  g->zp1e, g->zm1e, g->ep, g->em, g->w3_t, g->w3_e1, g->w2_t, g->w2_e1, g->bpt,
                  g->bin, g->gsed_t, g->ised_t, g->sur_t, g->nsur_t, g->bot_t,
-                 g->h1au1, g->h2au1, g->costhu1, g->sinthu1, g->cellarea, g->cellx,
+                 g->h1au1, g->h2au1, g->h1acell, g->costhu1, g->sinthu1, g->cellarea, g->cellx,
                  g->celly, g->botz, g->gridz, g->cellz};
     for (n = 0; n < (int)(sizeof(p) / sizeof(p[0])); n++) free(p[n]);
     free2(g->c2c);

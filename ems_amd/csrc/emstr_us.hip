@@ -6,9 +6,10 @@
  *
  * Kernels (layout in emstr_usdev.h):
  *   k_usadv<SCH>  one thread per wet centre, loop over tracers inside:
- *                 the fluxes through the cell's edges (van Leer on edges
- *                 with ULTIMATE, or first order, tracers.c:1637-1743,
- *                 2368-2573, 3785-3833; horizontal diffusion,
+ *                 the fluxes through the cell's edges (van Leer or
+ *                 QUICKEST on edges with ULTIMATE, or first order,
+ *                 tracers.c:1637-1743, 2055-2151, 2368-2573, 3785-3833;
+ *                 horizontal diffusion,
  *                 tracers.c:5383-5449), the vertical fluxes, the edge-sign
  *                 divergence (tracers.c:949-959) and the one-stage
  *                 Runge-Kutta update (tracers.c:1000-1016, 1170-1214);
@@ -26,6 +27,7 @@
 #include "emstr_us_kernels.h"
 
 #define ORDER1 0x000002
+#define QUICKEST 0x000020
 #define VANLEER 0x000080
 
 typedef long long ll;
@@ -123,7 +125,22 @@ __device__ __forceinline__ double us_edge_flux(const emstr_usdev &D, int e, int 
       const double tp1 = t[i1], tm1 = t[i2];
       const double tp2 = wp ? t[(ll)k * P + cp] : ts[i1];
       const double tm2 = wm ? t[(ll)k * P + cm] : ts[i2];
-      F = us_vl(vel, cn, tp1, tp2, tm1, tm2);
+      if (SCH == QUICKEST) {
+        /* tracers.c:2098-2123 */
+        const double h0 = D.h2e[e], hm = D.qhm[e], hp = D.qhp[e];
+        const double db = 2.0 / (hm + h0);
+        const double df = 2.0 / (hp + h0);
+        const double face = D.qd3[e] * (tm1 - tp1) + tp1;
+        const double grad = tp1 - tm1;
+        double curv = 0.0;
+        if (cn > 0.0)
+          curv = db * (hm * tp1 + h0 * tm2) - 2.0 * tm1;
+        else if (cn < 0.0)
+          curv = df * (h0 * tp2 + hp * tm1) - 2.0 * tp1;
+        F = face - 0.5 * cn * grad - (1.0 - cn * cn) * curv / 6.0;
+      } else {
+        F = us_vl(vel, cn, tp1, tp2, tm1, tm2);
+      }
       if (ult) {
         const unsigned sel = D.esel[e];
         const double trp = (wp && (sel & 1)) ? tp2 : tp1;
@@ -173,10 +190,29 @@ __device__ __forceinline__ double us_fz(const emstr_dev &d, int col, int k, int 
     return dtu * (0.5 * (w + fabs(w)) * tzm1 + 0.5 * (w - fabs(w)) * tc);
   const double tzm2 = t[(ll)max(k - 2, kbc) * P + col];
   const double tzp1 = t[(ll)min(k + 1, d.nz - 1) * P + col];
+  const double dzc = d.dz[idx];
   const double dzb = (k > kbc) ? d.dz[idx - P] : 0.0;
-  const double w9 = 0.5 * (d.dz[idx] + dzb);
+  const double w9 = 0.5 * (dzc + dzb);
   const double cn = fmod(w * dtu / (w9 * 1.0), 1.0);
-  double F = us_vl(w, cn, tc, tzp1, tzm1, tzm2);
+  double F;
+  if (SCH == QUICKEST) {
+    /* tracers.c:2125-2150; w9 of the sediment ghost is 0 */
+    const double w5 = dzb / (dzb + dzc);
+    const double w9m = (k > kbc) ? 0.5 * (dzb + ((k - 1 > kbc) ? d.dz[idx - 2 * P] : 0.0)) : 0.0;
+    const double w9p = (k + 1 < d.nz) ? 0.5 * (d.dz[idx + P] + dzc) : w9;
+    const double db = 2.0 / (w9m + w9);
+    const double df = 2.0 / (w9 + w9p);
+    const double face = w5 * (tc - tzm1) + tzm1;
+    const double grad = tc - tzm1;
+    double curv = 0.0;
+    if (cn > 0.0)
+      curv = db * (w9m * tc + w9 * tzm2) - 2.0 * tzm1;
+    else if (cn < 0.0)
+      curv = df * (w9 * tzp1 + w9p * tzm1) - 2.0 * tc;
+    F = face - 0.5 * cn * grad - (1.0 - cn * cn) * curv / 6.0;
+  } else {
+    F = us_vl(w, cn, tc, tzp1, tzm1, tzm2);
+  }
   if (ult) F = us_ult(cn, F, tc, tzm1, tzp1, tzm2);
   return F * (w * dtu);
 }
@@ -359,6 +395,9 @@ extern "C" int emstr_us_launch_adv(const emstr_usdev *D, const double *tin, doub
       break;
     case ORDER1:
       k_usadv<ORDER1><<<g, b, 0, st>>>(*D, tin, tout, tstart, dtu, anydif, 0);
+      break;
+    case QUICKEST:
+      k_usadv<QUICKEST><<<g, b, 0, st>>>(*D, tin, tout, tstart, dtu, anydif, ult);
       break;
     default:
       return -1;

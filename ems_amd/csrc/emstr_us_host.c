@@ -10,8 +10,8 @@
  * checked against the 3D maps of every layer when the context is made.
  *
  * Scope (rejected with an error otherwise): z-levels, closed boundaries,
- * one Runge-Kutta stage, schemes van Leer (with or without ULTIMATE) and
- * first order, no mean-velocity transport, no thin-layer merging, and no
+ * one Runge-Kutta stage, schemes van Leer and QUICKEST (with or without
+ * ULTIMATE) and first order, no mean-velocity transport, no thin-layer merging, and no
  * flux or velocity through land edges.
  */
 #include <hip/hip_runtime.h>
@@ -45,7 +45,7 @@ int emstr_last_error_cell(void);
       return emstr_set_error(E_DEVICE, -1, "%s: %s", #x, hipGetErrorString(_e)); \
   } while (0)
 
-typedef struct usctx {
+typedef struct emstr_us_ctx {
   geometry_t *g;
   window_t *wd;
   win_priv_t *wc;
@@ -96,7 +96,7 @@ static void us_destroy(usctx *c) {
                 d->osubeta, d->subeta, d->layers, d->w, d->Kz, d->dz, d->mintr, d->maxtr,
                 d->advect, d->diffuse, d->in_tbdy, d->in_tdif, d->cols, d->wsp, d->wdn,
                 D->npe, D->ce, D->esc, D->ec1, D->ec2, D->ecp, D->ecm, D->esel, D->h1e,
-                D->h2e, D->u1, D->flux, D->dzu, D->kh, c->tr[0], c->tr[1], c->tr[2],
+                D->h2e, D->qd3, D->qhm, D->qhp, D->u1, D->flux, D->dzu, D->kh, c->tr[0], c->tr[1], c->tr[2],
                 c->stage, c->d_esp, c->d_edn, c->d_bpt, c->d_bin, c->d_gsed, c->d_ised,
                 c->d_proc, c->d_err, c->d_dtm};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
@@ -118,7 +118,7 @@ static int us_check_scope(geometry_t *g, window_t *wd, win_priv_t *wc) {
   if (wc->rkstage != 1)
     return emstr_set_error(E_UNSUP, -1, "Runge-Kutta with %d stages is not supported",
                            wc->rkstage);
-  if (wc->trasc != VANLEER && wc->trasc != ORDER1)
+  if (wc->trasc != VANLEER && wc->trasc != ORDER1 && wc->trasc != QUICKEST)
     return emstr_set_error(E_UNSUP, -1, "advection scheme 0x%x is not supported", wc->trasc);
   for (n = 1; n <= g->v2_t; n++) {
     int cs = g->w2_t[n];
@@ -237,6 +237,9 @@ static usctx *us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
   unsigned char *h_esel = (unsigned char *)calloc(ne2 + 1, 1);
   double *h_h1 = (double *)malloc((size_t)(ne2 + 1) * sizeof(double));
   double *h_h2 = (double *)malloc((size_t)(ne2 + 1) * sizeof(double));
+  double *h_qd3 = (double *)malloc((size_t)(ne2 + 1) * sizeof(double));
+  double *h_qhm = (double *)malloc((size_t)(ne2 + 1) * sizeof(double));
+  double *h_qhp = (double *)malloc((size_t)(ne2 + 1) * sizeof(double));
   for (e = 1; e <= ne2 && ok; e++) {
     int c1 = g->e2c[e][0], c2 = g->e2c[e][1];
     int q = e - 1;
@@ -245,6 +248,17 @@ static usctx *us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
     h_h1[q] = g->h1au1[e];
     h_h2[q] = g->h2au1[e];
     h_ecp[q] = h_ecm[q] = -1;
+    /* QUICKEST metrics (tracers.c:600-603, 2082-2085): em/ep of the
+     * surface edge are surface edges */
+    if (g->em[e] < 1 || g->em[e] > ne2 || g->ep[e] < 1 || g->m2de[g->ep[e]] > ne2 ||
+        g->m2de[g->ep[e]] < 1) {
+      emstr_set_error(E_UNSUP, e, "surface edge %d: em/ep outside the surface", e);
+      ok = 0;
+      break;
+    }
+    h_qd3[q] = 0.5 * g->h1acell[e] / g->h2au1[e];
+    h_qhm[q] = g->h2au1[g->em[e]];
+    h_qhp[q] = g->h2au1[g->m2de[g->ep[e]]];
     if (h_ec1[q] < 0) {
       emstr_set_error(E_UNSUP, c1, "surface edge %d without a wet e2c[e][0]", e);
       ok = 0;
@@ -336,6 +350,9 @@ static usctx *us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
     D->esel = (unsigned char *)dalloc((size_t)ne2, &ok);
     D->h1e = (double *)dalloc((size_t)ne2 * sizeof(double), &ok);
     D->h2e = (double *)dalloc((size_t)ne2 * sizeof(double), &ok);
+    D->qd3 = (double *)dalloc((size_t)ne2 * sizeof(double), &ok);
+    D->qhm = (double *)dalloc((size_t)ne2 * sizeof(double), &ok);
+    D->qhp = (double *)dalloc((size_t)ne2 * sizeof(double), &ok);
     double **e3[] = {&D->u1, &D->flux, &D->dzu, &D->kh};
     for (n = 0; n < 4; n++) *e3[n] = (double *)dalloc(NE3 * sizeof(double), &ok);
     for (n = 0; n < 3; n++) c->tr[n] = (double *)dalloc((size_t)nt * N3 * sizeof(double), &ok);
@@ -373,6 +390,9 @@ static usctx *us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
            up(c, D->ecp, h_ecp, ne2 * sizeof(int)) || up(c, D->ecm, h_ecm, ne2 * sizeof(int)) ||
            up(c, D->esel, h_esel, ne2) || up(c, D->h1e, h_h1, ne2 * sizeof(double)) ||
            up(c, D->h2e, h_h2, ne2 * sizeof(double)) ||
+           up(c, D->qd3, h_qd3, ne2 * sizeof(double)) ||
+           up(c, D->qhm, h_qhm, ne2 * sizeof(double)) ||
+           up(c, D->qhp, h_qhp, ne2 * sizeof(double)) ||
            up(c, c->d_esp, esp, (size_t)nesp * sizeof(int)) ||
            up(c, c->d_edn, edn, (size_t)nesp * sizeof(ll)) ||
            up(c, c->d_bpt, g->bpt + 1, (size_t)g->nbpt * sizeof(int)) ||
@@ -400,6 +420,9 @@ static usctx *us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
   free(h_esel);
   free(h_h1);
   free(h_h2);
+  free(h_qd3);
+  free(h_qhm);
+  free(h_qhp);
   free(esp);
   free(edn);
   if (!ok) {
@@ -813,3 +836,41 @@ void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon) {
   if (us_download_tracers(c)) return;
   us_sync(c);
 }
+
+/*-------------------------------------------------------------------*/
+/* device-resident interface (no counterpart in the reference)       */
+/*-------------------------------------------------------------------*/
+emstr_us_ctx *emstr_us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  return us_create(g, wd, wc);
+}
+
+void emstr_us_destroy(emstr_us_ctx *c) { us_destroy(c); }
+
+int emstr_us_upload_fields(emstr_us_ctx *c) {
+  emstr_clear_error();
+  return us_upload_fields(c);
+}
+
+int emstr_us_upload_tracers(emstr_us_ctx *c) {
+  emstr_clear_error();
+  return us_upload_tracers(c);
+}
+
+int emstr_us_download_tracers(emstr_us_ctx *c) {
+  emstr_clear_error();
+  return us_download_tracers(c);
+}
+
+int emstr_us_steps(emstr_us_ctx *c, int nsteps) {
+  int r;
+  emstr_clear_error();
+  for (int k = 0; k < nsteps; k++) {
+    if ((r = us_adv(c))) return r;
+    if ((r = us_vdiff(c))) return r;
+  }
+  return 0;
+}
+
+int emstr_us_sync(emstr_us_ctx *c) { return us_sync(c); }
+
+void *emstr_us_stream(emstr_us_ctx *c) { return c ? (void *)c->st : NULL; }

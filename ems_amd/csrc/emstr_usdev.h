@@ -36,6 +36,9 @@ typedef struct emstr_usdev {
   unsigned char *esel;
   double *h1e;      /* [ne2] edge length (h1au1)                        */
   double *h2e;      /* [ne2] centre distance (h2au1)                    */
+  /* QUICKEST (tracers.c:592-603, 2080-2123): d3 = 0.5*h1acell/h2au1 and
+   * the centre distances of the surface edges em / ep                */
+  double *qd3, *qhm, *qhp;
   /* [nz][ne2] edge fields: u1, u1flux3d, dzu1, u2kh                  */
   double *u1, *flux, *dzu, *kh;
 } emstr_usdev;

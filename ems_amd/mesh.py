@@ -46,6 +46,19 @@ def lib():
             if hasattr(L, fn):
                 getattr(L, fn).argtypes = [G, W, P]
                 getattr(L, fn).restype = rt
+        if hasattr(L, "emstr_us_create"):
+            L.emstr_us_create.restype = C.c_void_p
+            L.emstr_us_create.argtypes = [G, W, P]
+            L.emstr_us_destroy.argtypes = [C.c_void_p]
+            L.emstr_us_destroy.restype = None
+            for fn in ("emstr_us_upload_fields", "emstr_us_upload_tracers",
+                       "emstr_us_download_tracers", "emstr_us_sync"):
+                getattr(L, fn).argtypes = [C.c_void_p]
+                getattr(L, fn).restype = C.c_int
+            L.emstr_us_steps.argtypes = [C.c_void_p, C.c_int]
+            L.emstr_us_steps.restype = C.c_int
+            L.emstr_us_stream.argtypes = [C.c_void_p]
+            L.emstr_us_stream.restype = C.c_void_p
         if hasattr(L, "emstr_us_last_error"):
             L.emstr_us_last_error.restype = C.c_char_p
             L.emstr_us_last_error_cell.restype = C.c_int
@@ -95,11 +108,15 @@ class Mesh2D:
         mx, my = 0.5 * (px + qx), 0.5 * (py + qy)
         # centre distance across each edge; at the mesh edge, twice the
         # distance from the centre to the edge midpoint
-        d = 2.0 * np.hypot(mx - self.cx[:, None], my - self.cy[:, None])
+        self.h2c = np.hypot(mx - self.cx[:, None], my - self.cy[:, None])
+        d = 2.0 * self.h2c
         m_ok = self.nbr >= 0
         q = np.where(m_ok, self.nbr, 0)
         dn = np.hypot(self.cx[q] - self.cx[:, None], self.cy[q] - self.cy[:, None])
         self.edist = np.where(m_ok, dn, d)
+        # width across the cell from edge j to the opposite edge
+        jo = (np.arange(self.npem) + self.npem // 2) % self.npem
+        self.ewidth = self.h2c + self.h2c[:, jo]
 
 
 def hexagons(nx, ny, edge=1500.0, jitter=0.1, rng=None):
@@ -236,6 +253,7 @@ class MeshCase:
         m, j = np.divmod(self.e2m[e2], npem)
         self.ae2(g.h1au1)[e2] = mesh.elen[m, j]
         self.ae2(g.h2au1)[e2] = mesh.edist[m, j]
+        self.ae2(g.h1acell)[e2] = mesh.ewidth[m, j]
         # positive u1 flows into e2c[e][0] = m: the inward normal of m
         self.ae2(g.costhu1)[e2] = -mesh.onx[m, j]
         self.ae2(g.sinthu1)[e2] = -mesh.ony[m, j]
@@ -457,6 +475,9 @@ def quads_from_case(sc):
     e_len[:, 2], e_dist[:, 2] = h2au1[jj, ii], h1au1[jj, ii]
     e_len[:, 3], e_dist[:, 3] = h1au2[jj, ii], h2au2[jj, ii]
     mesh.elen, mesh.edist = e_len, e_dist
+    # cell widths across each edge: x widths for E/W edges, y for N/S
+    h1c, h2c = M["h1acell"], M["h2acell"]
+    mesh.ewidth = np.stack([h1c[jj, ii], h2c[jj, ii], h1c[jj, ii], h2c[jj, ii]], axis=1)
     depth = np.where(sc.kbot.ravel() < nz, -sc.botz2d.ravel(), np.nan)
     case = MeshCase("quad_" + sc.name, mesh, depth, sc.layers, sc.ntr)
     if not np.array_equal(case.kbot, sc.kbot.ravel()):
@@ -510,3 +531,46 @@ def quads_from_case(sc):
     case.set_tracers(T.reshape(sc.ntr, nz, ny, nx)[:, kk, mk // nx, mk % nx])
     case.shoc_dense = (kk, mk // nx, mk % nx)
     return case
+
+
+class Context:
+    """A device-resident COMPAS window (emstr_us_create)."""
+
+    def __init__(self, case):
+        L = lib()
+        self.case = case
+        self.ptr = L.emstr_us_create(case.gp, case.wp, case.pp)
+        if not self.ptr:
+            raise RuntimeError("emstr_us_create: %s (cell %d)" % last_error())
+        self.stream = L.emstr_us_stream(self.ptr)
+
+    def _chk(self, r, what):
+        if r:
+            raise RuntimeError("%s: %s (cell %d)" % ((what,) + last_error()))
+
+    def upload(self):
+        L = lib()
+        self._chk(L.emstr_us_upload_fields(self.ptr), "upload_fields")
+        self._chk(L.emstr_us_upload_tracers(self.ptr), "upload_tracers")
+
+    def step(self, n=1):
+        self._chk(lib().emstr_us_steps(self.ptr, n), "steps")
+
+    def sync(self):
+        self._chk(lib().emstr_us_sync(self.ptr), "sync")
+
+    def download(self):
+        self._chk(lib().emstr_us_download_tracers(self.ptr), "download")
+
+    def close(self):
+        if self.ptr:
+            lib().emstr_us_destroy(self.ptr)
+            self.ptr = None
+
+
+def c4(seed=5, ntr=60, trasc=U.VANLEER):
+    """The C4 mesh: 350 x 350 jittered hexagons (1.5 km edges), 50 layers
+    over Gaussian ridges 3-400 m, transports from a vertex streamfunction
+    with max |u1| = 1.2 m/s, Kz decaying with depth, 60 tracers, dt = 180 s."""
+    return hex_basin(name="c4", nx=350, ny=350, nz=50, ntr=ntr, edge=1500.0, dmin=3.0,
+                     dmax=400.0, land_frac=0.2, vmax=1.2, dt=180.0, trasc=trasc, seed=seed)

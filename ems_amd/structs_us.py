@@ -37,7 +37,7 @@ class Geometry(C.Structure):
         ("nbpt", _I), ("bpt", _PI), ("bin", _PI),
         ("ngsed", _I), ("gsed_t", _PI), ("ised_t", _PI),
         ("sur_t", _PI), ("nsur_t", _PI), ("bot_t", _PI),
-        ("h1au1", _PD), ("h2au1", _PD), ("cellarea", _PD),
+        ("h1au1", _PD), ("h2au1", _PD), ("h1acell", _PD), ("cellarea", _PD),
         ("costhu1", _PD), ("sinthu1", _PD), ("cellx", _PD), ("celly", _PD),
         ("botz", _PD), ("gridz", _PD), ("cellz", _PD),
     ]

@@ -90,6 +90,8 @@ typedef struct geometry {
   /* Metrics                                                          */
   double *h1au1;    /* [szeS] edge length                             */
   double *h2au1;    /* [szeS] distance between the centres of an edge */
+  double *h1acell;  /* [szeS] distance between edge e and the opposite
+                     * edge of e2c[e][0] (pp_us.c:1495-1508)            */
   double *cellarea; /* [szcS]                                         */
   double *costhu1, *sinthu1; /* [szeS] edge normal direction          */
   double *cellx, *celly;     /* [szcS] centre coordinates             */
@@ -160,6 +162,23 @@ int advect_diffuse(geometry_t *window, window_t *windat, win_priv_t *wincon);
 void vert_diffuse_3d(geometry_t *window, window_t *windat, win_priv_t *wincon);
 /* model/hd-us/tracers/tracers.c:7231 (z-level branch) */
 void set_dz(geometry_t *window, window_t *windat, win_priv_t *wincon);
+
+/*-------------------------------------------------------------------*/
+/* Device-resident interface (no counterpart in the reference, which  */
+/* keeps its state on the host): a context mirrors one window on the  */
+/* GPU; fields and tracers stay in HBM between steps.                 */
+/*-------------------------------------------------------------------*/
+typedef struct emstr_us_ctx emstr_us_ctx;
+emstr_us_ctx *emstr_us_create(geometry_t *window, window_t *windat, win_priv_t *wincon);
+void emstr_us_destroy(emstr_us_ctx *ctx);
+/* fields, per-column lists and dz from the host structures          */
+int emstr_us_upload_fields(emstr_us_ctx *ctx);
+int emstr_us_upload_tracers(emstr_us_ctx *ctx);
+int emstr_us_download_tracers(emstr_us_ctx *ctx);
+/* nsteps of advect_diffuse + vert_diffuse_3d on the device           */
+int emstr_us_steps(emstr_us_ctx *ctx, int nsteps);
+int emstr_us_sync(emstr_us_ctx *ctx);
+void *emstr_us_stream(emstr_us_ctx *ctx); /* hipStream_t of the context */
 
 /* Errors of the last call on this thread ("" when none) */
 const char *emstr_us_last_error(void);

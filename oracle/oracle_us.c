@@ -4,8 +4,8 @@
  * A CPU restatement, in strict IEEE FP64, of the EMS COMPAS tracer step
  * (model/hd-us/tracers/tracers.c) for one z-level window with closed
  * lateral boundaries, no point sources, regions or multi-dt, Runge-Kutta
- * with one stage (the default), schemes van Leer and first order, with or
- * without ULTIMATE.  Each function follows the reference function cited
+ * with one stage (the default), schemes van Leer, QUICKEST and first
+ * order, with or without ULTIMATE.  Each function follows the reference function cited
  * above it with the same loop order and the same order of floating-point
  * operations.  Built with gcc -O3 -fno-fast-math -ffp-contract=off.
  *
@@ -175,6 +175,53 @@ static void vl_faces(geometry_t *g, window_t *wd, win_priv_t *wc, const double *
   }
 }
 
+/* tracers.c:2055-2151  non-uniform QUICKEST on edges (the spacings of
+ * the edges beyond through em/ep of the surface edge) and layers */
+static void qk_faces(geometry_t *g, window_t *wd, win_priv_t *wc, const double *tr,
+                     double *Fx, double *Fz) {
+  int ee, cc;
+  (void)wd;
+  memset(Fx, 0, g->sze * sizeof(double));
+  memset(Fz, 0, g->szc * sizeof(double));
+  for (ee = 1; ee <= g->n3_e1; ee++) {
+    int e = g->w3_e1[ee];
+    int es = g->m2de[e];
+    int em = g->em[es];
+    int ep = g->ep[es];
+    int c1 = g->e2c[e][0];
+    int c2 = g->e2c[e][1];
+    int cm = g->c2c[g->e2e[e][0]][c2];
+    int cp = g->c2c[g->e2e[e][1]][c1];
+    double trp = tr[cp], trm = tr[cm];
+    double cx = wc->w6[e];
+    double db = 2.0 / (g->h2au1[em] + g->h2au1[es]);
+    double df = 2.0 / (g->h2au1[ep] + g->h2au1[es]);
+    double face = wc->d3[es] * (tr[c2] - tr[c1]) + tr[c1];
+    double grad = tr[c1] - tr[c2];
+    double curv = 0.0;
+    if (cx > 0.0)
+      curv = db * (g->h2au1[em] * tr[c1] + g->h2au1[es] * trm) - 2.0 * tr[c2];
+    else if (cx < 0.0)
+      curv = df * (g->h2au1[es] * trp + g->h2au1[g->m2de[ep]] * tr[c2]) - 2.0 * tr[c1];
+    Fx[e] = face - 0.5 * cx * grad - (1.0 - cx * cx) * curv / 6.0;
+  }
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int zp1 = g->zp1[c], zm1 = g->zm1[c], zm2 = g->zm1[zm1];
+    double cz = wc->w8[c];
+    double db = 2.0 / (wc->w9[zm1] + wc->w9[c]);
+    double df = 2.0 / (wc->w9[c] + wc->w9[zp1]);
+    double face = wc->w5[c] * (tr[c] - tr[zm1]) + tr[zm1];
+    double grad = tr[c] - tr[zm1];
+    double curv = 0.0;
+    if (cz > 0.0)
+      curv = db * (wc->w9[zm1] * tr[c] + wc->w9[c] * tr[zm2]) - 2.0 * tr[zm1];
+    else if (cz < 0.0)
+      curv = df * (wc->w9[c] * tr[zp1] + wc->w9[zp1] * tr[zm1]) - 2.0 * tr[c];
+    Fz[c] = face - 0.5 * cz * grad - (1.0 - cz * cz) * curv / 6.0;
+  }
+}
+
 /* tracers.c:1753-1788 */
 static void o1_faces(geometry_t *g, window_t *wd, win_priv_t *wc, const double *tr,
                      double *Fx, double *Fz, double dtu) {
@@ -237,7 +284,10 @@ static void oracle_us_advect(geometry_t *g, window_t *wd, win_priv_t *wc, const 
     o1_faces(g, wd, wc, tr, Fx, Fz, dtu);
     return;
   }
-  vl_faces(g, wd, wc, tr, Fx, Fz);
+  if (wc->trasc == QUICKEST)
+    qk_faces(g, wd, wc, tr, Fx, Fz);
+  else
+    vl_faces(g, wd, wc, tr, Fx, Fz);
   if (wc->ultimate) {
     /* the outer cells through the edge maps ep/em (tracers.c:1697-1700) */
     for (ee = 1; ee <= g->n3_e1; ee++) {
@@ -504,6 +554,18 @@ int oracle_us_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int
     for (cc = 1; cc <= g->b3_t; cc++) {
       int c = g->w3_t[cc];
       wc->w9[c] = 0.5 * (wc->dz[c] + wc->dz[g->zm1[c]]);
+    }
+  } else {
+    /* tracers.c:592-603 (first order also passes here) */
+    for (cc = 1; cc <= g->b3_t; cc++) {
+      int c = g->w3_t[cc];
+      int zm1 = g->zm1[c];
+      wc->w5[c] = wc->dz[zm1] / (wc->dz[zm1] + wc->dz[c]);
+      wc->w9[c] = 0.5 * (wc->dz[c] + wc->dz[zm1]);
+    }
+    for (ee = 1; ee <= g->n2_e1; ee++) {
+      int e = g->w2_e1[ee];
+      wc->d3[e] = 0.5 * g->h1acell[e] / g->h2au1[e];
     }
   }
 

@@ -58,6 +58,8 @@ CASES = [
     ("vanleer", dict(trasc=U.VANLEER), U.NONE, 4),
     ("vanleer+ultimate", dict(trasc=U.VANLEER, ultimate=1), U.NONE, 4),
     ("order1", dict(trasc=U.ORDER1), U.NONE, 4),
+    ("quickest", dict(trasc=U.QUICKEST), U.NONE, 4),
+    ("quickest+ultimate", dict(trasc=U.QUICKEST, ultimate=1), U.NONE, 4),
     ("vanleer_substeps", dict(trasc=U.VANLEER, dt=3600.0, vmax=1.2), U.SUB_STEP, 2),
 ]
 
@@ -76,9 +78,10 @@ def test_hex_step_bit_exact(name, kw, stab, nsteps):
     assert_exact(gpu, ora)
 
 
-def test_quad_mesh_step_bit_exact():
-    say("case quads of a structured window")
-    sc = configs.small(seed=21, ntr=3, dt=120.0, trasc=S.VANLEER, ultimate=1)
+@pytest.mark.parametrize("trasc", [S.VANLEER, S.QUICKEST])
+def test_quad_mesh_step_bit_exact(trasc):
+    say("case quads of a structured window", trasc)
+    sc = configs.small(seed=21, ntr=3, dt=120.0, trasc=trasc, ultimate=1)
     sc.prepare(O.set_dz)
     qc = MS.quads_from_case(sc)
     OU.set_dz(qc.gp, qc.wp, qc.pp)

@@ -1,5 +1,8 @@
 """Pinning of the COMPAS oracle (oracle/oracle_us.c).
 
+QUICKEST with ULTIMATE is where the limiter acts (van Leer is TVD, so
+ULTIMATE barely changes it); the quadrilateral equivalence covers both.
+
 The strongest check: on the quadrilateral mesh of a structured window the
 COMPAS tracer step (edge stencils through c2c/e2e, edge signs eSc, the
 Runge-Kutta update, surf_concrk) must equal the SHOC oracle's step, which
@@ -25,7 +28,8 @@ def _rel(a, b):
     return (np.abs(a - b) / np.maximum(np.abs(b), 1e-300)).max()
 
 
-@pytest.mark.parametrize("trasc,ult", [(S.VANLEER, 0), (S.VANLEER, 1), (S.ORDER1, 0)])
+@pytest.mark.parametrize("trasc,ult", [(S.VANLEER, 0), (S.VANLEER, 1), (S.ORDER1, 0),
+                                      (S.QUICKEST, 0), (S.QUICKEST, 1)])
 def test_quad_mesh_step_equals_structured_step(trasc, ult):
     sc = configs.small(seed=21, ntr=3, dt=120.0, trasc=trasc, ultimate=ult)
     sc.prepare(O.set_dz)
@@ -98,6 +102,7 @@ def test_quad_mesh_moving_surface_and_diffusion():
 
 
 @pytest.mark.parametrize("trasc,ult,stab", [(U.VANLEER, 0, U.NONE), (U.VANLEER, 1, U.NONE),
+                                            (U.QUICKEST, 1, U.NONE),
                                             (U.ORDER1, 0, U.NONE),
                                             (U.VANLEER, 0, U.SUB_STEP)])
 def test_hex_constancy_and_mass(trasc, ult, stab):
