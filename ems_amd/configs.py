@@ -338,10 +338,11 @@ def c1():
 
 def curvilinear(name, nxi, nyi, nz, dmin, dmax, land_frac, vmax, ntr,
                 trasc, dt, seed, ultimate=0, n_span=0, alpha=0.5,
-                kz_max=1e-1, kz_min=1e-5, hmin=None):
+                kz_max=1e-1, kz_min=1e-5, hmin=None, top_layer=None):
     """Seeded curvilinear basin (spacing 500-2000 m), ridge bathymetry,
     16-mode streamfunction, mixed-layer Kz; tracers: 2 T/S-like then
-    log-normal, the last n_span of which span 1e-6..1e2."""
+    log-normal, the last n_span of which span 1e-6..1e2.  top_layer: the
+    thickness of the surface layer (uniform layers below it)."""
     rng = np.random.default_rng(seed)
     nx, ny = nxi + 2, nyi + 2
     dx = _tanh_spacing(nx, rng, 500.0, 2000.0)
@@ -349,7 +350,10 @@ def curvilinear(name, nxi, nyi, nz, dmin, dmax, land_frac, vmax, ntr,
     Xc, Yc = _cell_xy(dx, dy)
     depth = _bathymetry(nxi, nyi, Xc[1:-1, 1:-1], Yc[1:-1, 1:-1], rng, 12,
                         land_frac, dmin, dmax)
-    layers = np.linspace(-dmax, 0.0, nz + 1)
+    if top_layer is None:
+        layers = np.linspace(-dmax, 0.0, nz + 1)
+    else:
+        layers = np.concatenate([np.linspace(-dmax, -top_layer, nz), [0.0]])
     botz = np.full((ny, nx), np.nan)
     botz[1:-1, 1:-1] = -depth
     botz = _snap_bed(botz, layers)
@@ -415,6 +419,28 @@ def c2(seed=2):
                        S.QUICKEST, 120.0, seed, ultimate=0)
     case.steps = 500
     return case
+
+
+def c5(seed=5, ndom=64):
+    """The multi-domain set: ndom windows with nx, ny ~ U[20, 150],
+    5-40 layers and 5-20 tracers each, built as c2; every fourth window is
+    stiff (0.5 m surface layer, Kz up to 1 m2/s, dt = 600 s, diffusion
+    numbers up to ~1e4)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for d in range(ndom):
+        nxi, nyi = (int(v) for v in rng.integers(20, 151, 2))
+        nz = int(rng.integers(5, 41))
+        ntr = int(rng.integers(5, 21))
+        stiff = (d % 4 == 3)
+        case = curvilinear("c5_%d" % d, nxi, nyi, nz, 5.0, 200.0, 0.30, 1.0, ntr,
+                           S.QUICKEST if not stiff else S.VANLEER,
+                           600.0 if stiff else 120.0, int(rng.integers(1 << 30)),
+                           kz_max=1.0 if stiff else 1e-1,
+                           top_layer=0.5 if stiff else None)
+        case.stiff = stiff
+        out.append(case)
+    return out
 
 
 def c3(seed=3, ntr=40):

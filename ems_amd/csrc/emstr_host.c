@@ -78,7 +78,7 @@ struct emstr_ctx {
 /*-------------------------------------------------------------------*/
 /* Context registry: one device mirror per window (drop-in calls).   */
 /*-------------------------------------------------------------------*/
-#define MAXCTX 64
+#define MAXCTX 1024 /* windows with a device mirror at once */
 static pthread_mutex_t reg_lock = PTHREAD_MUTEX_INITIALIZER;
 static geometry_t *reg_g[MAXCTX];
 static emstr_ctx *reg_c[MAXCTX];
@@ -126,13 +126,20 @@ static emstr_ctx *ctx_for(geometry_t *g, window_t *wd, win_priv_t *wc) {
   pthread_mutex_lock(&reg_lock);
   if (!c) {
     c = emstr_create(g, wd, wc, EMSTR_PLAIN_F64);
-    if (c)
+    if (c) {
       for (i = 0; i < MAXCTX; i++)
         if (!reg_g[i]) {
           reg_g[i] = g;
           reg_c[i] = c;
           break;
         }
+      if (i == MAXCTX) {
+        pthread_mutex_unlock(&reg_lock);
+        emstr_destroy(c);
+        emstr_set_error(EMSTR_EARG, -1, "more than %d windows with a device mirror", MAXCTX);
+        return NULL;
+      }
+    }
   }
   pthread_mutex_unlock(&reg_lock);
   if (c) {

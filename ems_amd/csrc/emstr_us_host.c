@@ -435,7 +435,7 @@ static usctx *us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
 /*-------------------------------------------------------------------*/
 /* registry: one device mirror per window                            */
 /*-------------------------------------------------------------------*/
-#define MAXCTX 64
+#define MAXCTX 1024 /* windows with a device mirror at once */
 static pthread_mutex_t reg_lock = PTHREAD_MUTEX_INITIALIZER;
 static geometry_t *reg_g[MAXCTX];
 static usctx *reg_c[MAXCTX];
@@ -468,6 +468,11 @@ static usctx *ctx_for(geometry_t *g, window_t *wd, win_priv_t *wc) {
         break;
       }
     pthread_mutex_unlock(&reg_lock);
+    if (i == MAXCTX) {
+      us_destroy(c);
+      emstr_set_error(E_ARG, -1, "more than %d windows with a device mirror", MAXCTX);
+      return NULL;
+    }
   }
   c->wd = wd;
   c->wc = wc;
