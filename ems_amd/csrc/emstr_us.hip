@@ -5,12 +5,17 @@
  * -ffp-contract=off (no FMA contraction).
  *
  * Kernels (layout in emstr_usdev.h):
- *   k_usadv<SCH>  one thread per wet centre, loop over tracers inside:
- *                 the fluxes through the cell's edges (van Leer or
+ *   k_usflux<SCH> one thread per wet edge and layer, loop over tracers
+ *                 inside: the edge flux (van Leer or
  *                 QUICKEST on edges with ULTIMATE, or first order,
  *                 tracers.c:1637-1743, 2055-2151, 2368-2573, 3785-3833;
- *                 horizontal diffusion,
- *                 tracers.c:5383-5449), the vertical fluxes, the edge-sign
+ *                 horizontal diffusion, tracers.c:5383-5449) into a
+ *                 [tracer][layer][edge] buffer;
+ *   k_usfz<SCH>   one thread per wet centre: the vertical fluxes of the
+ *                 cells to process (tracers.c:2405-2412, 2125-2150,
+ *                 1777-1785) into a [tracer][layer][column] buffer;
+ *   k_usadv<SCH>  one thread per wet centre, loop over tracers inside:
+ *                 the edge-sign
  *                 divergence (tracers.c:949-959) and the one-stage
  *                 Runge-Kutta update (tracers.c:1000-1016, 1170-1214);
  *                 the processed surface centre also does surf_concrk
@@ -157,10 +162,10 @@ __device__ __forceinline__ double us_edge_flux(const emstr_usdev &D, int e, int 
   return F;
 }
 
-/* dtracer of the cell at layer k of column col (tracers.c:949-959) */
-template <int SCH>
-__device__ double us_dtracer(const emstr_usdev &D, int col, int k, const double *t,
-                             const double *ts, double dtu, int ult, int adv, int dif) {
+/* dtracer of the cell at layer k of column col (tracers.c:949-959); the
+ * edge fluxes come from fxn ([nz][ne2], k_usflux) */
+__device__ double us_dtracer(const emstr_usdev &D, int col, int k,
+                             const double *__restrict__ fxn, double dtu) {
   const emstr_dev &d = D.d;
   const int ncol = (int)d.plane;
   double dtr = 0.0;
@@ -168,10 +173,31 @@ __device__ double us_dtracer(const emstr_usdev &D, int col, int k, const double 
     const int e = D.ce[j * ncol + col];
     const int other = (D.ec1[e] == col) ? D.ec2[e] : D.ec1[e];
     double F = 0.0;
-    if (wetk(d.kb, other, k)) F = us_edge_flux<SCH>(D, e, k, t, ts, dtu, ult, adv, dif);
+    if (wetk(d.kb, other, k)) F = fxn[(ll)k * D.ne2 + e];
     dtr += ((double)D.esc[j * ncol + col] * F * dtu);
   }
   return dtr;
+}
+
+/* the flux of every wet edge at every layer for every tracer (advect and
+ * hor_diffuse, tracers.c:1637-1743, 5383-5449), one thread per wet edge */
+template <int SCH>
+__global__ void __launch_bounds__(256)
+k_usflux(emstr_usdev D, const double *__restrict__ tin, const double *__restrict__ tstart,
+         const ll *__restrict__ edn, int nesp, double *__restrict__ fx, double dtu,
+         int anydif, int ult) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nesp) return;
+  const ll ie = edn[q];
+  const int k = (int)(ie / D.ne2), e = (int)(ie % D.ne2);
+  const ll N3 = d.n3, NE3 = (ll)d.nz * D.ne2;
+  for (int n = 0; n < d.ntr; n++) {
+    if (!d.in_tbdy[n]) continue;
+    const int adv = d.advect[n], dif = anydif ? d.diffuse[n] : 0;
+    fx[n * NE3 + ie] =
+        us_edge_flux<SCH>(D, e, k, tin + n * N3, tstart + n * N3, dtu, ult, adv, dif);
+  }
 }
 
 /* vertical flux through the lower face of the cell at layer k
@@ -227,36 +253,58 @@ __device__ __forceinline__ double us_rk1(double ovol, double nvol, double ntr0, 
   return tr;
 }
 
-/* new value of the sub-surface cell at layer k (tracers.c:1000-1016) */
+/* the vertical flux through the lower face of every cell to process
+ * (k <= ks of a processed column) for every advected tracer, one thread
+ * per wet cell (dense indices wdn) */
 template <int SCH>
-__device__ double us_subsurface(const emstr_usdev &D, int col, int k, int kbc,
-                                const double *t, const double *ts, double dtu, int ult,
-                                int adv, int dif) {
+__global__ void __launch_bounds__(256)
+k_usfz(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ wdn, int nwet,
+       double *__restrict__ fz, double dtu, int ult) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nwet) return;
+  const ll P = d.plane;
+  const ll idx = wdn[q];
+  const int col = (int)(idx % P), k = (int)(idx / P);
+  const int ksc = d.ks[col];
+  if (ksc < 0 || k > ksc) return;
+  const int kbc = d.kb[col];
+  const ll N3 = d.n3;
+  for (int n = 0; n < d.ntr; n++)
+    if (d.in_tbdy[n] && d.advect[n])
+      fz[n * N3 + idx] = us_fz<SCH>(d, col, k, kbc, tin + n * N3, dtu, ult);
+}
+
+/* new value of the sub-surface cell at layer k (tracers.c:1000-1016) */
+__device__ double us_subsurface(const emstr_usdev &D, int col, int k, const double *t,
+                                const double *fxn, const double *fzn, double dtu, int adv) {
   const emstr_dev &d = D.d;
   const ll idx = (ll)k * d.plane + col;
-  const double dtr = us_dtracer<SCH>(D, col, k, t, ts, dtu, ult, adv, dif);
+  const double dtr = us_dtracer(D, col, k, fxn, dtu);
   double FzB = 0.0, FzT = 0.0;
   if (adv) {
-    FzB = us_fz<SCH>(d, col, k, kbc, t, dtu, ult);
-    FzT = us_fz<SCH>(d, col, k + 1, kbc, t, dtu, ult);
+    FzB = fzn[idx];
+    FzT = fzn[idx + d.plane];
   }
   const double grad = -((dtr / d.cellarea[col] + (FzT - FzB)) / d.dz[idx]);
   return us_rk1(1.0, 1.0, t[idx], grad);
 }
 
+#define US_MAXNPE 8
+
 template <int SCH>
 __global__ void __launch_bounds__(256)
 k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout,
-        const double *__restrict__ tstart, double dtu, int anydif, int ult) {
+        const double *__restrict__ fx, const double *__restrict__ fz,
+        const ll *__restrict__ wdn, int nwet, double dtu) {
   const emstr_dev &d = D.d;
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.z;
-  if (col >= d.nx) return;
-  const int kbc = d.kb[col];
-  if (k < kbc) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nwet) return;
   const ll P = d.plane;
-  const ll idx = (ll)k * P + col;
-  const ll N3 = d.n3;
+  const ll idx = wdn[q];
+  const int col = (int)(idx % P), k = (int)(idx / P);
+  const int kbc = d.kb[col];
+  const ll N3 = d.n3, NE3 = (ll)d.nz * D.ne2;
   const int ksc = d.ks[col];
   if (ksc < 0) { /* column not processed: unchanged */
     for (int n = 0; n < d.ntr; n++) tout[n * N3 + idx] = tin[n * N3 + idx];
@@ -265,14 +313,41 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
   if (k > ksc) return; /* above the processed surface: written below */
 
   if (k < ksc) {
+    /* the cell's edges at this layer: surface edge, sign, wet */
+    const int npe = D.npe[col], ncol = (int)P;
+    ll eo[US_MAXNPE];
+    double sg[US_MAXNPE];
+#pragma unroll
+    for (int j = 0; j < US_MAXNPE; j++) {
+      eo[j] = -1;
+      sg[j] = 0.0;
+      if (j < npe) {
+        const int e = D.ce[j * ncol + col];
+        const int other = (D.ec1[e] == col) ? D.ec2[e] : D.ec1[e];
+        if (wetk(d.kb, other, k)) eo[j] = (ll)k * D.ne2 + e;
+        sg[j] = (double)D.esc[j * ncol + col];
+      }
+    }
+    const double area = d.cellarea[col], dzc = d.dz[idx];
     for (int n = 0; n < d.ntr; n++) {
       const double *t = tin + n * N3;
       if (!d.in_tbdy[n]) {
         tout[n * N3 + idx] = t[idx];
         continue;
       }
-      const int adv = d.advect[n], dif = anydif ? d.diffuse[n] : 0;
-      double r = us_subsurface<SCH>(D, col, k, kbc, t, tstart + n * N3, dtu, ult, adv, dif);
+      const double *fxn = fx + n * NE3;
+      /* dtracer (tracers.c:949-959) */
+      double dtr = 0.0;
+#pragma unroll
+      for (int j = 0; j < US_MAXNPE; j++)
+        if (j < npe) dtr += (sg[j] * ((eo[j] >= 0) ? fxn[eo[j]] : 0.0) * dtu);
+      double FzB = 0.0, FzT = 0.0;
+      if (d.advect[n]) {
+        FzB = fz[n * N3 + idx];
+        FzT = fz[n * N3 + idx + P];
+      }
+      const double grad = -((dtr / area + (FzT - FzB)) / dzc);
+      double r = us_rk1(1.0, 1.0, t[idx], grad);
       /* tr_bounds clip (tracers.c:3328-3331) */
       if (d.mintr[n] <= 1e-5 && r < d.mintr[n]) r = d.mintr[n];
       tout[n * N3 + idx] = r;
@@ -290,19 +365,19 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
   const int ktop = d.nz - 1;
   for (int n = 0; n < d.ntr; n++) {
     const double *t = tin + n * N3;
-    const double *ts = tstart + n * N3;
+    const double *fxn = fx + n * NE3;
     double *o = tout + n * N3;
     if (!d.in_tbdy[n]) {
       for (int kk = k; kk <= ktop; kk++) o[(ll)kk * P + col] = t[(ll)kk * P + col];
       continue;
     }
-    const int adv = d.advect[n], dif = anydif ? d.diffuse[n] : 0;
+    const int adv = d.advect[n];
     double trs;
     if (!adv) {
       trs = t[idx];
     } else {
       /* surf_concrk (tracers.c:3085-3260) and runge_kutta */
-      const double FzB = us_fz<SCH>(d, col, k, kbc, t, dtu, ult);
+      const double FzB = fz[n * N3 + idx];
       const double top = -FzB * area;
       int c = k;
       double waterbot = (c == kbc) ? botz : lay[c];
@@ -315,7 +390,7 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
         vol = (watertop - waterbot) * area;
         ovol += vol;
         surftr += (t[(ll)c * P + col] * vol);
-        grad -= us_dtracer<SCH>(D, col, c, t, ts, dtu, ult, adv, dif);
+        grad -= us_dtracer(D, col, c, fxn, dtu);
         c = c + 1;
         waterbot = lay[c];
       }
@@ -324,11 +399,11 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
       vol = (watertop - waterbot) * area;
       ovol += vol;
       surftr += t[(ll)c * P + col] * vol;
-      grad -= us_dtracer<SCH>(D, col, c, t, ts, dtu, ult, adv, dif);
+      grad -= us_dtracer(D, col, c, fxn, dtu);
       if (c != ktop) {
         do {
           c = c + 1;
-          grad -= us_dtracer<SCH>(D, col, c, t, ts, dtu, ult, adv, dif);
+          grad -= us_dtracer(D, col, c, fxn, dtu);
         } while (c != ktop);
       }
       if (nvol == 0.0 || (surftr < 0.0 && dzs <= d.hmin)) surftr = 0.0;
@@ -341,7 +416,7 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
     const double dzco = (kold >= kbc) ? d.dz[(ll)kold * P + col] : 0.0;
     if (k > kbc && dzco < d.hmin) {
       if (trs < d.mintr[n] || trs > d.maxtr[n])
-        trs = us_subsurface<SCH>(D, col, k - 1, kbc, t, ts, dtu, ult, adv, dif);
+        trs = us_subsurface(D, col, k - 1, t, fxn, fz + n * N3, dtu, adv);
     }
     /* clip (tracers.c:3328-3331) */
     if (d.mintr[n] <= 1e-5 && trs < d.mintr[n]) trs = d.mintr[n];
@@ -382,22 +457,35 @@ __global__ void k_usdtm(emstr_usdev D, double dtu0, unsigned long long *out) {
   if (dtm < dtu0) atomicMin(out, (unsigned long long)__double_as_longlong(dtm));
 }
 
-extern "C" int emstr_us_launch_adv(const emstr_usdev *D, const double *tin, double *tout,
-                                   const double *tstart, double dtu, int anydif, int ult,
-                                   void *stream) {
+template <int SCH>
+static void us_adv_launch(const emstr_usdev *D, const double *tin, double *tout,
+                          const double *tstart, const long long *edn, int nesp, double *fx,
+                          double *fz, double dtu, int anydif, int ult, hipStream_t st) {
   const emstr_dev *d = &D->d;
-  dim3 b(256, 1, 1);
-  dim3 g((d->nx + 255) / 256, 1, d->nz);
+  if (nesp)
+    k_usflux<SCH><<<(nesp + 255) / 256, 256, 0, st>>>(*D, tin, tstart, edn, nesp, fx, dtu,
+                                                       anydif, ult);
+  if (d->nwet) {
+    k_usfz<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, tin, d->wdn, d->nwet, fz, dtu, ult);
+    k_usadv<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, tin, tout, fx, fz, d->wdn,
+                                                         d->nwet, dtu);
+  }
+}
+
+extern "C" int emstr_us_launch_adv(const emstr_usdev *D, const double *tin, double *tout,
+                                   const double *tstart, const long long *edn, int nesp,
+                                   double *fx, double *fz, double dtu, int anydif, int ult,
+                                   void *stream) {
   hipStream_t st = (hipStream_t)stream;
-  switch (d->trasc) {
+  switch (D->d.trasc) {
     case VANLEER:
-      k_usadv<VANLEER><<<g, b, 0, st>>>(*D, tin, tout, tstart, dtu, anydif, ult);
+      us_adv_launch<VANLEER>(D, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, ult, st);
       break;
     case ORDER1:
-      k_usadv<ORDER1><<<g, b, 0, st>>>(*D, tin, tout, tstart, dtu, anydif, 0);
+      us_adv_launch<ORDER1>(D, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, 0, st);
       break;
     case QUICKEST:
-      k_usadv<QUICKEST><<<g, b, 0, st>>>(*D, tin, tout, tstart, dtu, anydif, ult);
+      us_adv_launch<QUICKEST>(D, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, ult, st);
       break;
     default:
       return -1;

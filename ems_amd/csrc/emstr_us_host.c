@@ -66,6 +66,8 @@ typedef struct emstr_us_ctx {
   int *d_bpt, *d_bin, *d_gsed, *d_ised;
   int *d_proc, *d_err;
   unsigned long long *d_dtm;
+  double *d_fx;  /* edge fluxes [ntr][nz][ne2], made on first use */
+  double *d_fz;  /* vertical fluxes [ntr][nz][ncol], made on first use */
   int anydif, have_cols;
   int key_szc, key_sze;
 } usctx;
@@ -98,7 +100,7 @@ static void us_destroy(usctx *c) {
                 D->npe, D->ce, D->esc, D->ec1, D->ec2, D->ecp, D->ecm, D->esel, D->h1e,
                 D->h2e, D->qd3, D->qhm, D->qhp, D->u1, D->flux, D->dzu, D->kh, c->tr[0], c->tr[1], c->tr[2],
                 c->stage, c->d_esp, c->d_edn, c->d_bpt, c->d_bin, c->d_gsed, c->d_ised,
-                c->d_proc, c->d_err, c->d_dtm};
+                c->d_proc, c->d_err, c->d_dtm, c->d_fx, c->d_fz};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->cs2col, c->col2cs, c->h_kb, c->h_ks, c->h_kold, c->h_knew, c->h_cols,
@@ -214,7 +216,7 @@ static usctx *us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
   for (int col = 0; col < ncol && ok; col++) {
     int cs = c->col2cs[col];
     h_npe[col] = g->npe[cs];
-    if (g->npe[cs] > npem) {
+    if (g->npe[cs] > npem || g->npe[cs] > 8) {
       emstr_set_error(E_UNSUP, cs, "npe exceeds npem");
       ok = 0;
     }
@@ -678,6 +680,19 @@ static int us_adv(usctx *c) {
      * first processed column stands for it */
     return c->wc->vcs > 0 ? c->g->m2d[c->wc->s1[1]] : 1;
   }
+  if (!c->d_fx || !c->d_fz) {
+    const size_t nt = (size_t)(d->ntr > 0 ? d->ntr : 1);
+    const size_t bytes = nt * d->nz * c->ne2 * sizeof(double);
+    const size_t bz = nt * (size_t)d->n3 * sizeof(double) + (size_t)c->ncol * sizeof(double);
+    if (!c->d_fx && hipMalloc((void **)&c->d_fx, bytes) != hipSuccess) {
+      c->d_fx = NULL;
+      return emstr_set_error(E_DEVICE, -1, "edge flux buffer (%zu bytes)", bytes);
+    }
+    if (hipMalloc((void **)&c->d_fz, bz) != hipSuccess) {
+      c->d_fz = NULL;
+      return emstr_set_error(E_DEVICE, -1, "vertical flux buffer (%zu bytes)", bz);
+    }
+  }
   const int b0 = c->cur;
   int in = b0, out = (b0 + 1) % 3;
   trem = dtu;
@@ -688,8 +703,8 @@ static int us_adv(usctx *c) {
     if (u > 0.0) {
       if (emstr_launch_subeta(d, u, trem, first, c->st))
         return emstr_set_error(E_DEVICE, -1, "subeta launch");
-      if (emstr_us_launch_adv(D, c->tr[in], c->tr[out], c->tr[b0], u, c->anydif,
-                              d->ultimate, c->st) ||
+      if (emstr_us_launch_adv(D, c->tr[in], c->tr[out], c->tr[b0], c->d_edn, c->nesp,
+                              c->d_fx, c->d_fz, u, c->anydif, d->ultimate, c->st) ||
           emstr_launch_osub(d, c->st))
         return emstr_set_error(E_DEVICE, -1, "advection launch");
       in = out;
