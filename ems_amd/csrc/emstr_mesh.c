@@ -10,6 +10,7 @@
  * Single window, closed boundaries: the cells and edges to process are
  * the wet ones ([1..v3] = [1..a3]), the ghosts follow up to n3.
  */
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -370,6 +371,12 @@ emstr_mesh *emstr_mesh_build(int nmc, int npem, const int *npe, const int *nbr,
   wd->Kz = (double *)zalloc(szc, sizeof(double));
   wd->eta = (double *)zalloc(szcS, sizeof(double));
   wd->topz = (double *)zalloc(szcS, sizeof(double));
+  wd->u2 = (double *)zalloc(sze, sizeof(double));
+  wd->wtop = (double *)zalloc(szcS, sizeof(double));
+  wd->wbot = (double *)zalloc(szcS, sizeof(double));
+  /* lateral ghosts -> their wet centre */
+  g->wgst = (int *)zalloc(szc, sizeof(int));
+  for (n = 1; n <= g->nbpt; n++) g->wgst[g->bpt[n]] = g->bin[n];
   wd->dt = wd->dttr = 1.0;
 
   wc->trasc = VANLEER;
@@ -432,13 +439,18 @@ void emstr_mesh_free(emstr_mesh *M) {
   geometry_t *g = M->window;
   window_t *wd = M->windat;
   win_priv_t *wc = M->wincon;
+  const int szc = g ? g->szc : 0;
   if (g) {
     void *p[] = {g->layers, g->s2i, g->s2k, g->npe, g->m2d, g->m2de, g->zp1, g->zm1,
                  g->zp1e, g->zm1e, g->ep, g->em, g->w3_t, g->w3_e1, g->w2_t, g->w2_e1, g->bpt,
                  g->bin, g->gsed_t, g->ised_t, g->sur_t, g->nsur_t, g->bot_t,
                  g->h1au1, g->h2au1, g->h1acell, g->costhu1, g->sinthu1, g->cellarea, g->cellx,
-                 g->celly, g->botz, g->gridz, g->cellz};
+                 g->celly, g->botz, g->gridz, g->cellz, g->wgst, g->nvc, g->m2dv,
+                 g->gridx, g->gridy, g->u1x, g->u1y, g->costhu2, g->sinthu2};
     for (n = 0; n < (int)(sizeof(p) / sizeof(p[0])); n++) free(p[n]);
+    free2(g->c2v);
+    free2(g->v2c);
+    free2(g->e2v);
     free2(g->c2c);
     free2(g->c2e);
     free2(g->e2c);
@@ -449,11 +461,21 @@ void emstr_mesh_free(emstr_mesh *M) {
   if (wd) {
     for (n = 0; n < wd->ntr; n++) free(wd->tr_wc[n]);
     void *p[] = {wd->tr_wc, wd->u1, wd->u1flux3d, wd->dzu1, wd->u, wd->v, wd->w,
-                 wd->Kz, wd->eta, wd->topz};
+                 wd->Kz, wd->eta, wd->topz, wd->u2, wd->wtop, wd->wbot};
     for (n = 0; n < (int)(sizeof(p) / sizeof(p[0])); n++) free(p[n]);
     free(wd);
   }
   if (wc) {
+    if (wc->lw) {
+      for (n = 0; n < szc; n++) {
+        free(wc->lw[n].cells);
+        if (wc->lw[n].B) {
+          free(wc->lw[n].B[0]);
+          free(wc->lw[n].B);
+        }
+      }
+      free(wc->lw);
+    }
     void *p[] = {wc->advect, wc->diffuse, wc->tbdy, wc->tdif_v, wc->mintr, wc->maxtr,
                  wc->oldeta, wc->dz, wc->Ds, wc->Hn1, wc->Hs, wc->one, wc->u1kh,
                  wc->u2kh, wc->mdx, wc->s1, wc->i1, wc->i2, wc->i3, wc->i4, wc->i5,
@@ -469,5 +491,225 @@ void emstr_mesh_free(emstr_mesh *M) {
   free(M->eidx);
   free(M->e2m);
   free(M->sedg);
+  free(M->v2m);
   free(M);
+}
+
+/*-------------------------------------------------------------------*/
+/* Vertices (pp_us.c:1900-2100): vertex j of a centre is the corner  */
+/* where its edge j starts (counter-clockwise), e2v[e] runs along edge */
+/* j of e2c[e][0]; a vertex exists in layer k when a wet centre of    */
+/* that layer has it as a corner; v2c lists the wet centres of the    */
+/* vertex's layer around it (ghosts are not mapped, pp_us.c:2027).    */
+/*-------------------------------------------------------------------*/
+int emstr_mesh_vertices(emstr_mesh *M, int nv, const int *vid) {
+  geometry_t *g = M->window;
+  const int nmc = M->nmc, npem = M->npem, nz = M->nz;
+  const int *kbot = M->kbot;
+  int k, m, j, n;
+  if (!g || nv <= 0 || g->c2v) return -1;
+#define VID(m, j) vid[(size_t)(m) * npem + (j) - 1]
+#define WETK(m, k) ((m) >= 0 && (k) >= kbot[m])
+#define IDX(k, m) M->idx3[(size_t)(k) * nmc + (m)]
+  for (m = 0; m < nmc; m++) {
+    if (kbot[m] >= nz) continue;
+    for (j = 1; j <= g->npe[IDX(nz - 1, m)]; j++)
+      if (VID(m, j) < 0 || VID(m, j) >= nv) return -2;
+  }
+  int *num = (int *)zalloc((size_t)nz * nv, sizeof(int)); /* [k][vertex] */
+  int *cnt = (int *)zalloc((size_t)nz * nv, sizeof(int));
+  int v = 1, szvS = 0;
+  for (k = nz - 1; k >= 0; k--) {
+    for (m = 0; m < nmc; m++) {
+      if (!WETK(m, k)) continue;
+      const int np = g->npe[IDX(nz - 1, m)];
+      for (j = 1; j <= np; j++) {
+        int *t = &num[(size_t)k * nv + VID(m, j)];
+        if (!*t) *t = v++;
+        cnt[(size_t)k * nv + VID(m, j)]++;
+      }
+    }
+    if (k == nz - 1) szvS = v;
+  }
+  const int szv = v;
+  int nvcm = 0;
+  for (n = 0; n < nv; n++)
+    if (cnt[(size_t)(nz - 1) * nv + n] > nvcm) nvcm = cnt[(size_t)(nz - 1) * nv + n];
+  g->szv = szv;
+  g->szvS = szvS;
+  g->nvcm = nvcm;
+  g->c2v = alloc2(npem + 1, g->szc);
+  g->v2c = alloc2(szv, nvcm + 1);
+  g->nvc = (int *)zalloc(szvS, sizeof(int));
+  g->m2dv = (int *)zalloc(szv, sizeof(int));
+  g->e2v = alloc2(g->sze, 2);
+  M->v2m = (int *)zalloc(szvS, sizeof(int));
+  for (k = nz - 1; k >= 0; k--)
+    for (m = 0; m < nmc; m++) {
+      if (!WETK(m, k)) continue;
+      const int c = IDX(k, m), np = g->npe[IDX(nz - 1, m)];
+      for (j = 1; j <= np; j++) {
+        const int mv = VID(m, j);
+        const int v3 = num[(size_t)k * nv + mv], vs = num[(size_t)(nz - 1) * nv + mv];
+        g->c2v[j][c] = v3;
+        g->m2dv[v3] = vs;
+        if (k == nz - 1) {
+          g->nvc[vs]++;
+          M->v2m[vs] = mv;
+        }
+        for (n = 1; n <= nvcm; n++)
+          if (!g->v2c[v3][n]) {
+            g->v2c[v3][n] = c;
+            break;
+          }
+      }
+    }
+  for (int e = 1; e < g->sze; e++) {
+    const int c0 = g->e2c[e][0], jj = g->e2e[e][0];
+    const int m0 = g->s2i[c0], np = g->npe[g->m2d[c0]];
+    if (m0 < 0 || jj < 1) continue;
+    g->e2v[e][0] = g->c2v[jj][c0];
+    g->e2v[e][1] = g->c2v[jj % np + 1][c0];
+  }
+  g->gridx = (double *)zalloc(szvS, sizeof(double));
+  g->gridy = (double *)zalloc(szvS, sizeof(double));
+  g->u1x = (double *)zalloc(g->szeS, sizeof(double));
+  g->u1y = (double *)zalloc(g->szeS, sizeof(double));
+  g->costhu2 = (double *)zalloc(g->szeS, sizeof(double));
+  g->sinthu2 = (double *)zalloc(g->szeS, sizeof(double));
+  M->nv = nv;
+  free(num);
+  free(cnt);
+  return 0;
+#undef VID
+#undef WETK
+#undef IDX
+}
+
+/*-------------------------------------------------------------------*/
+/* Least-squares weights of every wet centre (build_linear_weights,   */
+/* tracers.c:4278-4440, with df = 2, and get_linear_metrics,          */
+/* tracers.c:4602-4673): the stencil is the centre and the wet        */
+/* centres around its vertices in its layer (stencil() ST_SIZED of    */
+/* size 3, diagnostics/monitor.c:2015-2050), each followed by the     */
+/* ghosts across its edges; rows (1, cellx[c]-x, celly[c]-y) at the   */
+/* surface positions of the stencil centres.  B is the pseudo-inverse */
+/* of those rows; the reference takes it from an SVD (svd_lsq_B), here */
+/* from the eigen-decomposition of the 3x3 normal matrix with the same */
+/* cut of near-zero singular values.                                   */
+/*-------------------------------------------------------------------*/
+static void jacobi3(double A[3][3], double V[3][3], double ev[3]) {
+  int i, j, p, q, sweep;
+  for (i = 0; i < 3; i++)
+    for (j = 0; j < 3; j++) V[i][j] = (i == j) ? 1.0 : 0.0;
+  for (sweep = 0; sweep < 50; sweep++) {
+    double off = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
+    if (off == 0.0) break;
+    for (p = 0; p < 2; p++)
+      for (q = p + 1; q < 3; q++) {
+        if (A[p][q] == 0.0) continue;
+        double th = 0.5 * (A[q][q] - A[p][p]) / A[p][q];
+        double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+        for (i = 0; i < 3; i++) {
+          double aip = A[i][p], aiq = A[i][q];
+          A[i][p] = c * aip - sn * aiq;
+          A[i][q] = sn * aip + c * aiq;
+        }
+        for (i = 0; i < 3; i++) {
+          double api = A[p][i], aqi = A[q][i];
+          A[p][i] = c * api - sn * aqi;
+          A[q][i] = sn * api + c * aqi;
+        }
+        for (i = 0; i < 3; i++) {
+          double vip = V[i][p], viq = V[i][q];
+          V[i][p] = c * vip - sn * viq;
+          V[i][q] = sn * vip + c * viq;
+        }
+      }
+  }
+  for (i = 0; i < 3; i++) ev[i] = A[i][i];
+}
+
+int emstr_mesh_lsq(emstr_mesh *M) {
+  geometry_t *g = M->window;
+  win_priv_t *wc = M->wincon;
+  int cc, j, i, n;
+  if (!g || !g->c2v || wc->lw) return -1;
+  wc->lw = (qweights *)zalloc(g->szc, sizeof(qweights));
+  int *mask = (int *)zalloc(g->szcS, sizeof(int));
+  int *st = (int *)zalloc(64, sizeof(int));
+  int *cells = (int *)zalloc(64 * (g->npem + 1), sizeof(int));
+  double *px = (double *)zalloc(64 * (g->npem + 1), sizeof(double));
+  double *py = (double *)zalloc(64 * (g->npem + 1), sizeof(double));
+  for (cc = 1; cc <= g->a3_t; cc++) {
+    const int c = g->w3_t[cc], cs = g->m2d[c];
+    int ns = 0, nc = 0;
+    /* stencil: the centre, then the wet centres around its vertices */
+    st[ns++] = c;
+    mask[cs] = 1;
+    for (j = 1; j <= g->npe[cs]; j++) {
+      const int v = g->c2v[j][c];
+      if (!v) continue;
+      const int vs = g->m2dv[v];
+      for (i = 1; i <= g->nvc[vs]; i++) {
+        const int c1 = g->v2c[v][i];
+        if (c1 && !mask[g->m2d[c1]] && ns < 64) {
+          st[ns++] = c1;
+          mask[g->m2d[c1]] = 1;
+        }
+      }
+    }
+    for (i = 0; i < ns; i++) mask[g->m2d[st[i]]] = 0;
+    /* each stencil centre, then the ghosts across its edges */
+    for (i = 0; i < ns; i++) {
+      const int cn = st[i], cns = g->m2d[cn];
+      cells[nc] = cn;
+      px[nc] = g->cellx[cns];
+      py[nc] = g->celly[cns];
+      nc++;
+      for (n = 1; n <= g->npe[cns]; n++) {
+        const int cg = g->c2c[n][cn];
+        if (g->wgst[cg]) {
+          cells[nc] = cg;
+          px[nc] = g->cellx[g->m2d[cg]];
+          py[nc] = g->celly[g->m2d[cg]];
+          nc++;
+        }
+      }
+    }
+    /* pseudo-inverse of the rows (1, x, y) */
+    double A[3][3] = {{0}}, V[3][3], ev[3];
+    for (i = 0; i < nc; i++) {
+      const double r[3] = {1.0, g->cellx[cs] - px[i], g->celly[cs] - py[i]};
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) A[a][b] += r[a] * r[b];
+    }
+    jacobi3(A, V, ev);
+    double emax = fmax(fabs(ev[0]), fmax(fabs(ev[1]), fabs(ev[2])));
+    double Ainv[3][3] = {{0}};
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        for (int l = 0; l < 3; l++)
+          if (fabs(ev[l]) > 1e-12 * emax) Ainv[a][b] += V[a][l] * V[b][l] / ev[l];
+    qweights *lw = &wc->lw[c];
+    lw->ncells = nc;
+    lw->cells = (int *)zalloc(nc, sizeof(int));
+    lw->B = (double **)zalloc(nc, sizeof(double *));
+    lw->B[0] = (double *)zalloc((size_t)nc * 3, sizeof(double));
+    for (i = 0; i < nc; i++) {
+      const double r[3] = {1.0, g->cellx[cs] - px[i], g->celly[cs] - py[i]};
+      lw->cells[i] = cells[i];
+      lw->B[i] = lw->B[0] + (size_t)i * 3;
+      for (int a = 0; a < 3; a++)
+        lw->B[i][a] = Ainv[a][0] * r[0] + Ainv[a][1] * r[1] + Ainv[a][2] * r[2];
+    }
+    lw->beta = 1.0;
+  }
+  free(mask);
+  free(st);
+  free(cells);
+  free(px);
+  free(py);
+  return 0;
 }

@@ -40,6 +40,10 @@ def lib():
                                        C.POINTER(C.c_double), C.c_int]
         L.emstr_mesh_free.argtypes = [C.POINTER(U.Mesh)]
         L.emstr_mesh_free.restype = None
+        L.emstr_mesh_vertices.argtypes = [C.POINTER(U.Mesh), C.c_int, _PI]
+        L.emstr_mesh_vertices.restype = C.c_int
+        L.emstr_mesh_lsq.argtypes = [C.POINTER(U.Mesh)]
+        L.emstr_mesh_lsq.restype = C.c_int
         G, W, P = C.POINTER(U.Geometry), C.POINTER(U.Window), C.POINTER(U.WinPriv)
         for fn, rt in (("tracer_step_3d", None), ("advect_diffuse", C.c_int),
                        ("vert_diffuse_3d", None), ("set_dz", None)):
@@ -209,7 +213,13 @@ class MeshCase:
         self.wet = U.view(g.w3_t, self.szc, np.int32)[1:g.v3_t + 1].copy()
         self.e2c = U.view2(g.e2c, self.sze, 2)
         self.dt = 1.0
+        vid = np.ascontiguousarray(mesh.vid, dtype=np.int32)
+        self._keep.append(vid)
+        if L.emstr_mesh_vertices(M, int(mesh.vx.size), _iptr(vid)) != 0:
+            raise ValueError("emstr_mesh_vertices failed")
         self._fill_metrics()
+        if L.emstr_mesh_lsq(M) != 0:
+            raise ValueError("emstr_mesh_lsq failed")
 
     # -- views --------------------------------------------------------
     def ac(self, ptr):
@@ -259,7 +269,8 @@ class MeshCase:
         self.ae2(g.sinthu1)[e2] = -mesh.ony[m, j]
         c2 = np.arange(1, self.szcS)
         mc = self.s2i[c2]
-        # a surface ghost takes the metrics of its wet centre
+        # a surface ghost takes the metrics of its wet centre (its
+        # position is set below)
         bpt = U.view(g.bpt, g.nbpt + 1, np.int32)[1:]
         bin_ = U.view(g.bin, g.nbpt + 1, np.int32)[1:]
         owner = np.full(self.szc, -1, dtype=np.int64)
@@ -270,6 +281,35 @@ class MeshCase:
         self.a2(g.cellx)[c2] = mesh.cx[mm]
         self.a2(g.celly)[c2] = mesh.cy[mm]
         self.a2(g.botz)[c2] = self.botz2d[mm]
+        # vertices, edge midpoints and tangents (edge j of m runs from
+        # vid[m, j-1] to vid[m, j]); positive u2 points along that edge
+        v2m = U.view(self.M.contents.v2m, g.szvS, np.int32)[1:]
+        U.view(g.gridx, g.szvS, np.float64)[1:] = mesh.vx[v2m]
+        U.view(g.gridy, g.szvS, np.float64)[1:] = mesh.vy[v2m]
+        pa = mesh.vid[m, j]  # j is 0-based here
+        pb = mesh.vid[m, (j + 1) % mesh.npe[m]]
+        ax, ay = mesh.vx[pa], mesh.vy[pa]
+        bx, by = mesh.vx[pb], mesh.vy[pb]
+        self.ae2(g.u1x)[e2] = 0.5 * (ax + bx)
+        self.ae2(g.u1y)[e2] = 0.5 * (ay + by)
+        self.ae2(g.costhu2)[e2] = (bx - ax) / mesh.elen[m, j]
+        self.ae2(g.sinthu2)[e2] = (by - ay) / mesh.elen[m, j]
+        # a surface ghost sits at the mirror image of its wet centre in
+        # the edge midpoint (pp_us.c:6015-6034)
+        gsurf = bpt[self.s2k[bpt] == self.nz - 1]
+        gin = bin_[self.s2k[bpt] == self.nz - 1]
+        c2c = U.view2(g.c2c, npem + 1, self.szc)
+        c2e = U.view2(g.c2e, npem + 1, self.szc)
+        for jj in range(1, npem + 1):
+            sel = c2c[jj, gin] == gsurf
+            gi, gg = gin[sel], gsurf[sel]
+            es = c2e[jj, gi]
+            x = self.ae2(g.u1x)[es] - self.a2(g.cellx)[gi]
+            y = self.ae2(g.u1y)[es] - self.a2(g.celly)[gi]
+            d = np.sqrt(x * x + y * y)
+            r = np.arctan2(y, x)
+            self.a2(g.cellx)[gg] = self.ae2(g.u1x)[es] + d * np.cos(r)
+            self.a2(g.celly)[gg] = self.ae2(g.u1y)[es] + d * np.sin(r)
 
     # -- fields -------------------------------------------------------
     def set_transport(self, psi, vmax, alpha=0.3):
@@ -312,6 +352,76 @@ class MeshCase:
         self.ae(self.wd.u1flux3d)[e] = -flux[kk, mm, jj]
         self.ae(self.wd.dzu1)[e] = dzf[kk, mm, jj]
         self._dzf = dzf
+        self.set_tangential()
+
+    def _edge_dirs(self):
+        g = self.g
+        e = np.arange(1, self.sze)
+        es = U.view(g.m2de, self.sze, np.int32)[e]
+        n = np.stack([self.ae2(g.costhu1)[es], self.ae2(g.sinthu1)[es]], axis=1)
+        t = np.stack([self.ae2(g.costhu2)[es], self.ae2(g.sinthu2)[es]], axis=1)
+        return e, es, n, t
+
+    def set_tangential(self):
+        """Cell-centred (u, v) of every wet centre fitted to its edge
+        normal velocities (least squares weighted by edge length, land
+        edges carrying zero), and the tangential edge velocity u2 from
+        the mean of the centred velocities on either side (the centre's
+        own at land)."""
+        g = self.g
+        npem = self.mesh.npem
+        e, es, n, t = self._edge_dirs()
+        u1 = self.ae(self.wd.u1)
+        c2e = U.view2(g.c2e, npem + 1, self.szc)
+        npe = U.view(g.npe, self.szcS, np.int32)
+        m2d = U.view(g.m2d, self.szc, np.int32)
+        c = self.wet
+        h1 = self.ae2(g.h1au1)
+        m2de = U.view(g.m2de, self.sze, np.int32)
+        A = np.zeros((c.size, 2, 2))
+        b = np.zeros((c.size, 2))
+        for j in range(1, npem + 1):
+            on = npe[m2d[c]] >= j
+            ee = np.where(on, c2e[j, c], 0)
+            ss = m2de[ee]
+            nn = np.stack([self.ae2(g.costhu1)[ss], self.ae2(g.sinthu1)[ss]], axis=1)
+            L = np.where(on, h1[ss], 0.0)
+            A += L[:, None, None] * nn[:, :, None] * nn[:, None, :]
+            b += (L * np.where(on, u1[ee], 0.0))[:, None] * nn
+        det = A[:, 0, 0] * A[:, 1, 1] - A[:, 0, 1] * A[:, 1, 0]
+        det = np.where(det != 0.0, det, 1.0)
+        uc = (A[:, 1, 1] * b[:, 0] - A[:, 0, 1] * b[:, 1]) / det
+        vc = (A[:, 0, 0] * b[:, 1] - A[:, 1, 0] * b[:, 0]) / det
+        u, v = self.ac(self.wd.u), self.ac(self.wd.v)
+        u[:] = 0.0
+        v[:] = 0.0
+        u[c] = uc
+        v[c] = vc
+        c1, c2 = self.e2c[e, 0], self.e2c[e, 1]
+        wet2 = np.isin(c2, c)
+        um = np.where(wet2, 0.5 * (u[c1] + u[c2]), u[c1])
+        vm = np.where(wet2, 0.5 * (v[c1] + v[c2]), v[c1])
+        self.ae(self.wd.u2)[e] = um * t[:, 0] + vm * t[:, 1]
+
+    def set_uniform_velocity(self, U0, V0):
+        """A uniform horizontal velocity (U0, V0) on every wet edge
+        (normal part u1, tangential part u2, u1flux3d = u1*dzu1*h1au1 with
+        full layer thicknesses), zero at land edges; w = 0."""
+        g = self.g
+        e, es, n, t = self._edge_dirs()
+        c2 = self.e2c[e, 1]
+        wet2 = np.isin(c2, self.wet)
+        lay = self.layers
+        k = self.s2k[self.e2c[e, 0]]
+        dzu = lay[k + 1] - lay[k]
+        u1 = np.where(wet2, U0 * n[:, 0] + V0 * n[:, 1], 0.0)
+        self.ae(self.wd.u1)[e] = u1
+        self.ae(self.wd.u2)[e] = U0 * t[:, 0] + V0 * t[:, 1]
+        self.ae(self.wd.dzu1)[e] = np.where(wet2, dzu, 0.0)
+        self.ae(self.wd.u1flux3d)[e] = u1 * np.where(wet2, dzu, 0.0) * self.ae2(g.h1au1)[es]
+        self.ac(self.wd.u)[self.wet] = U0
+        self.ac(self.wd.v)[self.wet] = V0
+        self.ac(self.wd.w)[:] = 0.0
 
     def w_from_continuity(self):
         """w on lower faces from the layer divergence, bed upwards:

@@ -17,6 +17,10 @@ QUICKEST = 0x000020
 VANLEER = 0x000080
 FFSL = 0x080000
 NONE = 0x80000000
+MONOTONIC = 0x0010
+CLIP = 0x8000000
+V6898 = 0x000800
+F_FFSL = 0x100
 SUB_STEP = 2
 
 
@@ -40,6 +44,18 @@ class Geometry(C.Structure):
         ("h1au1", _PD), ("h2au1", _PD), ("h1acell", _PD), ("cellarea", _PD),
         ("costhu1", _PD), ("sinthu1", _PD), ("cellx", _PD), ("celly", _PD),
         ("botz", _PD), ("gridz", _PD), ("cellz", _PD),
+        ("szv", _I), ("szvS", _I), ("nvcm", _I),
+        ("c2v", _PPI), ("v2c", _PPI), ("nvc", _PI), ("m2dv", _PI), ("e2v", _PPI),
+        ("gridx", _PD), ("gridy", _PD), ("u1x", _PD), ("u1y", _PD),
+        ("costhu2", _PD), ("sinthu2", _PD), ("wgst", _PI),
+    ]
+
+
+class QWeights(C.Structure):
+    _fields_ = [
+        ("B", _PPD), ("w", _D * 6), ("ncells", _I), ("cells", _PI),
+        ("tmx", _D), ("tmn", _D), ("orf", _I), ("xr", _D), ("yr", _D),
+        ("beta", _D), ("alpha", _D),
     ]
 
 
@@ -49,6 +65,7 @@ class Window(C.Structure):
         ("nstep", _I),
         ("u1", _PD), ("u1flux3d", _PD), ("dzu1", _PD), ("u", _PD), ("v", _PD),
         ("w", _PD), ("Kz", _PD), ("eta", _PD), ("topz", _PD),
+        ("u2", _PD), ("wtop", _PD), ("wbot", _PD),
     ]
 
 
@@ -65,7 +82,8 @@ class WinPriv(C.Structure):
         ("i1", _PI), ("i2", _PI), ("i3", _PI), ("i4", _PI), ("i5", _PI),
     ] + [("w%d" % i, _PD) for i in range(1, 11)] + \
         [("d%d" % i, _PD) for i in range(1, 5)] + \
-        [("v%d" % i, _PD) for i in range(1, 8)] + [("b1", _D)]
+        [("v%d" % i, _PD) for i in range(1, 8)] + [("b1", _D)] + \
+        [("flag", _I), ("fillf", _I), ("compatible", _I), ("lw", C.POINTER(QWeights))]
 
 
 class Mesh(C.Structure):
@@ -74,7 +92,7 @@ class Mesh(C.Structure):
         ("wincon", C.POINTER(WinPriv)),
         ("nmc", _I), ("npem", _I), ("nz", _I), ("ntr", _I),
         ("kbot", _PI), ("idx3", _PI), ("gidx", _PI), ("eidx", _PI), ("e2m", _PI),
-        ("sedg", _PI),
+        ("sedg", _PI), ("nv", _I), ("v2m", _PI),
     ]
 
 

@@ -26,6 +26,13 @@ extern "C" {
 #define LAGRANGE 0x000100
 #define FFSL 0x080000
 
+/* FFSL settings: model/hd-us/include/hd_params.h */
+#define TRANSPORT 0x0040  /* means: mean-velocity transport             */
+#define MONOTONIC 0x0010  /* fillf                                      */
+#define CLIP 0x8000000    /* fillf                                      */
+#define V6898 0x000800    /* compatible                                 */
+#define F_FFSL 0x100      /* flag: ffsl_init ran on the previous call    */
+
 /* Stability flags */
 #define SUB_STEP 2
 #define SUB_STEP_NOSURF 4
@@ -98,7 +105,38 @@ typedef struct geometry {
   double *botz;     /* [szcS] bed height (negative down)              */
   double *gridz;    /* [szc] height of the lower face of each cell    */
   double *cellz;    /* [szc] height of the cell centre                */
+
+  /* Vertices and edge geometry for FFSL (transport.c:1476-1826,
+   * tracers.c:4682-4871; pp_us.c:1900-2100).  A vertex of layer k is a
+   * corner of the centres of that layer; vertices of the surface layer
+   * are numbered first (1..szvS-1).                                    */
+  int szv, szvS;    /* 3D / 2D vertex array sizes                     */
+  int nvcm;         /* most centres around a vertex                   */
+  int **c2v;        /* [npem+1][szc] vertex j of a centre (0: none)   */
+  int **v2c;        /* [szv][nvcm+1] wet centres of the vertex's layer
+                     * around it, 0-padded; ghosts are not mapped     */
+  int *nvc;         /* [szvS] centres around a surface vertex         */
+  int *m2dv;        /* [szv] vertex -> surface vertex                 */
+  int **e2v;        /* [sze][2] end vertices of an edge               */
+  double *gridx, *gridy;     /* [szvS] vertex coordinates             */
+  double *u1x, *u1y;         /* [szeS] edge midpoints (u1 points)      */
+  double *costhu2, *sinthu2; /* [szeS] edge tangential direction      */
+  int *wgst;        /* [szc] wet centre of a lateral ghost (0: none)  */
 } geometry_t;
+
+/* Least-squares linear reconstruction of one centre (lib/include/
+ * lsqq.h:33-45): w = sum_j tr[cells[j]] * B[j][.], value at (x, y) =
+ * w[0] + beta * (w[1] * (cellx - x) + w[2] * (celly - y))             */
+typedef struct {
+  double **B;       /* [ncells][3]                                    */
+  double w[6];
+  int ncells;
+  int *cells;       /* [ncells] stencil centres (ghosts included)     */
+  double tmx, tmn;
+  int orf;
+  double xr, yr;
+  double beta, alpha;
+} qweights;
 
 /*-------------------------------------------------------------------*/
 /* Window data (fields of the COMPAS window_t)                       */
@@ -116,6 +154,8 @@ typedef struct window {
   double *Kz;       /* [szc] vertical diffusivity, lower face         */
   double *eta;      /* [szcS] elevation at t+1                        */
   double *topz;     /* [szcS]                                         */
+  double *u2;       /* [sze] edge-tangential velocity (FFSL)          */
+  double *wtop, *wbot; /* [szcS] vertical velocity at the surface / bed */
 } window_t;
 
 /*-------------------------------------------------------------------*/
@@ -148,6 +188,11 @@ typedef struct win_priv {
   double *d1, *d2, *d3, *d4;              /* [max(szcS,szeS)]         */
   double *v1, *v2, *v3, *v4, *v5, *v6, *v7; /* [nz+1]                 */
   double b1;        /* current sub-step                               */
+  /* FFSL (transport.c:1961-2723)                                     */
+  int flag;         /* F_FFSL toggled by ffsl_init                    */
+  int fillf;        /* FILL_METHOD (MONOTONIC / CLIP: rejected)       */
+  int compatible;   /* V6898: rejected with FFSL                      */
+  qweights *lw;     /* [szc] weights of build_linear_weights (2D fit) */
 } win_priv_t;
 
 /*-------------------------------------------------------------------*/

@@ -5,7 +5,10 @@
  * (model/hd-us/tracers/tracers.c) for one z-level window with closed
  * lateral boundaries, no point sources, regions or multi-dt, Runge-Kutta
  * with one stage (the default), schemes van Leer, QUICKEST and first
- * order, with or without ULTIMATE.  Each function follows the reference function cited
+ * order, with or without ULTIMATE, and FFSL (ffsl_don with streamlines
+ * that stay in their upstream cell, the linear least-squares
+ * interpolant, no ULTIMATE or clipping fill).  Each function follows the
+ * reference function cited
  * above it with the same loop order and the same order of floating-point
  * operations.  Built with gcc -O3 -fno-fast-math -ffp-contract=off.
  *
@@ -442,7 +445,13 @@ void oracle_us_tr_bounds(geometry_t *g, window_t *wd, win_priv_t *wc) {
 /*-------------------------------------------------------------------*/
 typedef struct {
   double *Fx, *Fz, *dtracer, *rk, *gr;
+  /* FFSL, per tracer (transport.c:1961-2598) */
+  double *trm, *moc, *lww, *lwb, *cx, *cy, *ntr, *dint, *cour;
+  int *cl, *mask;
 } uscr_t;
+
+static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr, double *Fx,
+                        double *Fz, double dtu, uscr_t *s);
 
 static void us_tracer_substep(geometry_t *g, window_t *wd, win_priv_t *wc, int n, double dtu,
                               int slf, int ksf, const double *osubeta,
@@ -460,7 +469,12 @@ static void us_tracer_substep(geometry_t *g, window_t *wd, win_priv_t *wc, int n
     int cb = g->bot_t[cc];
     tr[g->zm1[cb]] = tr[cb];
   }
-  if (wc->advect[n]) oracle_us_advect(g, wd, wc, tr, Fx, Fz, dtu);
+  if (wc->advect[n]) {
+    if (wc->trasc & FFSL) {
+      if (wd->dttr != 0.0) us_ffsl_don(g, wd, wc, tr, Fx, Fz, dtu, s);
+    } else
+      oracle_us_advect(g, wd, wc, tr, Fx, Fz, dtu);
+  }
   if (wc->diffuse[n]) oracle_us_hor_diffuse(g, wd, wc, tr, Fx);
   for (cc = 1; cc <= g->b3_t; cc++) {
     int c = g->w3_t[cc];
@@ -504,24 +518,525 @@ static uscr_t *uscr_get(int nt, int n) {
   int t;
   if (uscr && uscr_nt >= nt && uscr_n == n) return uscr;
   for (t = 0; t < uscr_nt; t++) {
-    free(uscr[t].Fx);
-    free(uscr[t].Fz);
-    free(uscr[t].dtracer);
-    free(uscr[t].rk);
-    free(uscr[t].gr);
+    void *p[] = {uscr[t].Fx, uscr[t].Fz, uscr[t].dtracer, uscr[t].rk, uscr[t].gr,
+                 uscr[t].trm, uscr[t].moc, uscr[t].lww, uscr[t].lwb, uscr[t].cx,
+                 uscr[t].cy, uscr[t].ntr, uscr[t].dint, uscr[t].cour, uscr[t].cl,
+                 uscr[t].mask};
+    for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++) free(p[i]);
   }
   free(uscr);
   uscr = (uscr_t *)calloc(nt, sizeof(uscr_t));
   for (t = 0; t < nt; t++) {
-    uscr[t].Fx = (double *)calloc(n, sizeof(double));
-    uscr[t].Fz = (double *)calloc(n, sizeof(double));
-    uscr[t].dtracer = (double *)calloc(n, sizeof(double));
-    uscr[t].rk = (double *)calloc(n, sizeof(double));
-    uscr[t].gr = (double *)calloc(n, sizeof(double));
+    double **dp[] = {&uscr[t].Fx, &uscr[t].Fz, &uscr[t].dtracer, &uscr[t].rk, &uscr[t].gr,
+                     &uscr[t].trm, &uscr[t].moc, &uscr[t].lwb, &uscr[t].cx, &uscr[t].cy,
+                     &uscr[t].ntr, &uscr[t].dint, &uscr[t].cour};
+    for (size_t i = 0; i < sizeof(dp) / sizeof(dp[0]); i++)
+      *dp[i] = (double *)calloc(n, sizeof(double));
+    uscr[t].lww = (double *)calloc((size_t)3 * n, sizeof(double));
+    uscr[t].cl = (int *)calloc(n + 1, sizeof(int));
+    uscr[t].mask = (int *)calloc(n, sizeof(int));
   }
   uscr_nt = nt;
   uscr_n = n;
   return uscr;
+}
+
+/*===================================================================*/
+/* FFSL on the unstructured mesh (trasc & FFSL), single window, closed */
+/* boundaries, no mean-velocity transport, no ULTIMATE, no clipping   */
+/* fill: the path taken by ffsl_don with its defaults (tranf =        */
+/* T_VERT|T_HORZ|T_HV, doint = 1, momo2 = 0, vscheme = V_VC, tf = 0.55) */
+/* and the linear least-squares interpolant (L_LSLIN).                */
+/*===================================================================*/
+#define SMALLV 1e-10  /* transport.c:24 */
+
+/* tracer-independent arrays of one step / sub-step */
+static double *ff_nw = NULL, *ff_crfzf = NULL, *ff_crfzc = NULL;
+static int *ff_clzf = NULL, *ff_clzc = NULL, *ff_e2ee = NULL;
+static int ff_szc = 0, ff_sze = 0;
+
+static void ff_alloc(geometry_t *g) {
+  if (ff_szc == g->szc && ff_sze == g->sze) return;
+  free(ff_nw);
+  free(ff_crfzf);
+  free(ff_crfzc);
+  free(ff_clzf);
+  free(ff_clzc);
+  free(ff_e2ee);
+  ff_nw = (double *)calloc(g->szc, sizeof(double));
+  ff_crfzf = (double *)calloc(g->szc, sizeof(double));
+  ff_crfzc = (double *)calloc(g->szc, sizeof(double));
+  ff_clzf = (int *)calloc(g->szc, sizeof(int));
+  ff_clzc = (int *)calloc(g->szc, sizeof(int));
+  /* land edges are never given an index (transport.c:2154-2162), so
+   * theirs stays 0 */
+  ff_e2ee = (int *)calloc(g->sze, sizeof(int));
+  ff_szc = g->szc;
+  ff_sze = g->sze;
+}
+
+/* transport.c:5118-5160  set_tr_nograd (no OBCs) */
+static void ff_nograd(geometry_t *g, win_priv_t *wc, double *tr) {
+  int cc;
+  for (cc = 1; cc <= wc->vcs1; cc++) {
+    int c, c2, zp1;
+    c = c2 = wc->s1[cc];
+    zp1 = g->zp1[c];
+    while (c != zp1) {
+      tr[zp1] = tr[c2];
+      c = zp1;
+      zp1 = g->zp1[c];
+    }
+  }
+  for (cc = 1; cc <= g->nbpt; cc++) tr[g->bpt[cc]] = tr[g->bin[cc]];
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    int c = g->bot_t[cc];
+    tr[g->zm1[c]] = tr[c];
+  }
+}
+
+/* transport.c:6098-6141  vel_center_w */
+static void ff_vel_center_w(geometry_t *g, window_t *wd, double *nw) {
+  int cc;
+  const double *w = wd->w;
+  memset(nw, 0, g->szc * sizeof(double));
+  for (cc = 1; cc <= g->a2_t; cc++) {
+    int c = g->sur_t[cc];
+    int cb = g->bot_t[cc];
+    int cs = g->m2d[c];
+    double wtop = wd->wtop[cs];
+    double wbot = w[c];
+    nw[c] = 0.5 * (wbot + wtop);
+    if (fabs(nw[c]) < SMALLV) nw[c] = SMALLV;
+    while (c < cb) {
+      c = g->zm1[c];
+      wtop = wbot;
+      wbot = (c == cb) ? wd->wbot[cs] : w[c];
+      nw[c] = 0.5 * (wtop + wbot);
+      if (fabs(nw[c]) < SMALLV) nw[c] = SMALLV;
+    }
+  }
+}
+
+/* transport.c:2607-2723  ffsl_init: runs on every other call (the
+ * F_FFSL toggle; particles would take the other call).  The centred
+ * velocities of the lateral ghosts are the edge velocity rotated to
+ * east / north, written into windat->u, v as the reference does; the
+ * layer spacings dzz / cellz and the Delaunay structures only serve
+ * hd_trans_interp (multi-segment streamlines, rejected here) and the
+ * sediment values of nu / nv / nw are never read on this path.       */
+static void ff_init(geometry_t *g, window_t *wd, win_priv_t *wc, double *nw) {
+  int cc, j;
+  if (wc->flag & F_FFSL) {
+    wc->flag &= ~F_FFSL;
+    return;
+  }
+  ff_vel_center_w(g, wd, nw);
+  for (cc = 1; cc <= g->nbpt; cc++) {
+    int c = g->bpt[cc], ci = g->bin[cc], e = 0;
+    for (j = 1; j <= g->npe[g->m2d[ci]]; j++)
+      if (g->c2c[j][ci] == c) e = g->c2e[j][ci];
+    int es = g->m2de[e];
+    wd->u[c] = wd->u1[e] * g->costhu1[es] + wd->u2[e] * g->costhu2[es];
+    wd->v[c] = wd->u1[e] * g->sinthu1[es] + wd->u2[e] * g->sinthu2[es];
+    nw[c] = nw[ci];
+  }
+  wc->flag |= F_FFSL;
+}
+
+/* tracers.c:2732-2770  ff_sl_do_vert */
+static void ff_do_vert(const double *vel, double dt, const double *h, int ss, int se,
+                       const int *sdo, const int *fmap, const int *bmap, int *cl,
+                       double *crf) {
+  int cc;
+  for (cc = ss; cc <= se; cc++) {
+    int c = sdo[cc], cp, cpb;
+    double ltraj, lrem, dz;
+    if (vel[c] == 0.0) continue;
+    ltraj = fabs(vel[c]) * dt;
+    lrem = ltraj;
+    cp = cpb = cl[c];
+    dz = fabs((h[cp] > 0) ? h[cp] : h[c]);
+    while (dz < lrem) {
+      lrem -= dz;
+      cp = (vel[c] < 0.0) ? fmap[cp] : bmap[cp];
+      if (cpb == cp) {
+        lrem = fmod(lrem, dz);
+        break;
+      }
+      if (h[cp] > 0) dz = fabs(h[cp]);
+      cpb = cp;
+    }
+    crf[c] = max(min(lrem / dz, 1.0), 0.0);
+    cl[c] = cp;
+  }
+}
+
+/* tracers.c:2670-2721  prep_ff_sl (vertical maps of a sub-step) */
+static void ff_prep(geometry_t *g, window_t *wd, win_priv_t *wc, double dt) {
+  int cc;
+  memset(ff_crfzf, 0, g->szc * sizeof(double));
+  memset(ff_crfzc, 0, g->szc * sizeof(double));
+  memset(ff_clzf, 0, g->szc * sizeof(int));
+  memset(ff_clzc, 0, g->szc * sizeof(int));
+  for (cc = 1; cc <= g->a3_t; cc++) {
+    int c = g->w3_t[cc];
+    ff_clzc[c] = (ff_nw[c] < 0.0) ? g->zp1[c] : g->w3_t[cc];
+  }
+  ff_do_vert(ff_nw, dt, wc->w9, 1, g->a3_t, g->w3_t, g->zp1, g->zm1, ff_clzc, ff_crfzc);
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    ff_clzf[c] = (wd->w[c] > 0.0) ? g->zm1[c] : wc->s1[cc];
+  }
+  ff_do_vert(wd->w, dt, wc->dz, 1, wc->vc, wc->s1, g->zp1, g->zm1, ff_clzf, ff_crfzf);
+}
+
+/* transport.c:1744-1826  intersect */
+static int ff_intersect(geometry_t *g, int c, int j, double xn, double yn, double xs,
+                        double ys, double *xi, double *yi) {
+  int cs = g->m2d[c];
+  int es = g->c2e[j][cs];
+  double s1, s2;
+  if (es) {
+    int v1 = g->e2v[es][0];
+    int v2 = g->e2v[es][1];
+    if (v1 && v2) {
+      double v1x = g->gridx[v1], v1y = g->gridy[v1];
+      double v2x = g->gridx[v2], v2y = g->gridy[v2];
+      double dx = fabs(v1x - v2x);
+      if (dx) {
+        s1 = (v2y - v1y) / (v2x - v1x);
+        if (xs == xn)
+          *xi = xn;
+        else {
+          s2 = (ys - yn) / (xs - xn);
+          *xi = (s2 * xn - s1 * v1x + v1y - yn) / (s2 - s1);
+        }
+        *yi = s1 * (*xi - v1x) + v1y;
+      } else {
+        s1 = (v2x - v1x) / (v2y - v1y);
+        if (ys == yn)
+          *yi = yn;
+        else {
+          s2 = (xs - xn) / (ys - yn);
+          *yi = (s1 * v1y - s2 * yn + xn - v1x) / (s1 - s2);
+        }
+        *xi = s1 * (*yi - v1y) + v1x;
+      }
+      if (*xi >= min(v1x, v2x) && *xi <= max(v1x, v2x) && *yi >= min(v1y, v2y) &&
+          *yi <= max(v1y, v2y)) {
+        if (*xi >= min(xn, xs) && *xi <= max(xn, xs) && *yi >= min(yn, ys) &&
+            *yi <= max(yn, ys))
+          return 1;
+      }
+    }
+  }
+  return 0;
+}
+
+/* transport.c:1789-1797  get_dist */
+static double ff_dist(double x1, double y1, double x2, double y2) {
+  double dx = x2 - x1;
+  double dy = y2 - y1;
+  return sqrt(dx * dx + dy * dy);
+}
+
+/* transport.c:1476-1559  get_posc (metres, not geographic); a crossing
+ * into a lateral ghost (wgst != 0, the L_GHOST cells) ends the
+ * streamline at the edge */
+static int ff_posc(geometry_t *g, int *ei, int ci, double u, double v, double *cx,
+                   double *cy, double dt, double *odist) {
+  int cn, j, e = 0, found = 0;
+  const int cis = g->m2d[ci];
+  double xin = *cx, yin = *cy, dist, dir, slon, slat, sinth, costh, xi = 0.0, yi = 0.0;
+  dist = sqrt(u * u + v * v) * dt * 1.0;
+  dir = atan2(v, u);
+  sinth = sin(dir);
+  costh = cos(dir);
+  slon = xin - dist * costh;
+  slat = yin - dist * sinth;
+  if (dist == 0.0) {
+    *odist = 0.0;
+    return -ci;
+  }
+  cn = ci;
+  for (j = 1; j <= g->npe[cis]; j++) {
+    e = g->c2e[j][ci];
+    if (e != *ei && (found = ff_intersect(g, ci, j, *cx, *cy, slon, slat, &xi, &yi))) {
+      cn = g->c2c[j][ci];
+      break;
+    }
+  }
+  *ei = e;
+  if (g->wgst[cn]) {
+    *odist = ff_dist(*cx, *cy, xi, yi) / 1.0;
+    *cx = xi;
+    *cy = yi;
+    return -ci;
+  }
+  if (!found) {
+    *odist = ff_dist(*cx, *cy, slon, slat) / 1.0;
+    *cx = slon;
+    *cy = slat;
+    return -ci;
+  }
+  *odist = ff_dist(*cx, *cy, xi, yi) / 1.0;
+  *cx = xi;
+  *cy = yi;
+  return cn;
+}
+
+/* tracers.c:4682-4708  get_linear_value, df = 2 */
+static double ff_lin_value(geometry_t *g, const double *lww, const double *lwb, int co,
+                           double xi, double yi) {
+  int cs = g->m2d[co];
+  double x = g->cellx[cs] - xi;
+  double y = g->celly[cs] - yi;
+  const double *w = lww + (size_t)3 * co;
+  return w[0] + lwb[co] * (w[1] * x + w[2] * y);
+}
+
+/* tracers.c:4718-4871  get_linear_limit, df = 2: weights w from the
+ * stencil, w[0] reset to the centre value, beta the largest factor
+ * that keeps the values at the cell's vertices within the range of the
+ * centres around each vertex (lw->beta is 1 while the vertex values are
+ * taken); ghost centres take their own value with beta = 0 */
+static void ff_linear_limit(geometry_t *g, win_priv_t *wc, const double *tr, double *lww,
+                            double *lwb) {
+  int cc, j, jj;
+  for (cc = 1; cc <= g->a3_t; cc++) {
+    int c = g->w3_t[cc];
+    int cs = g->m2d[c];
+    qweights *lw = &wc->lw[c];
+    double *w = lww + (size_t)3 * c;
+    double beta = 1.0;
+    w[0] = w[1] = w[2] = 0.0;
+    for (j = 0; j < lw->ncells; j++) {
+      int cn = lw->cells[j];
+      for (jj = 0; jj < 3; jj++) w[jj] += (tr[cn] * lw->B[j][jj]);
+    }
+    w[0] = tr[c];
+    for (j = 1; j <= g->npe[cs]; j++) {
+      int v = g->c2v[j][c];
+      double vmin = HUGE_VAL, vmax = -HUGE_VAL;
+      if (v) {
+        int vs = g->m2dv[v];
+        double x = g->cellx[cs] - g->gridx[vs];
+        double y = g->celly[cs] - g->gridy[vs];
+        double val = w[0] + 1.0 * (w[1] * x + w[2] * y);
+        double d = val - w[0];
+        for (jj = 1; jj <= g->nvc[vs]; jj++) {
+          int c1 = g->v2c[v][jj];
+          if (!c1) continue;
+          vmin = min(vmin, tr[c1]);
+          vmax = max(vmax, tr[c1]);
+        }
+        if (d != 0.0 && val > vmax) beta = max(min((vmax - w[0]) / d, beta), 0.0);
+        if (d != 0.0 && val < vmin) beta = max(min((vmin - w[0]) / d, beta), 0.0);
+      }
+    }
+    lwb[c] = min(max(beta, 0.0), 1.0);
+  }
+  for (cc = 1; cc <= g->nbpt; cc++) {
+    int c = g->bpt[cc];
+    double *w = lww + (size_t)3 * c;
+    w[0] = tr[c];
+    w[1] = w[2] = 0.0;
+    lwb[c] = 0.0;
+  }
+}
+
+/* transport.c:8216-8260  ffsl_van_leer (no monotone diagnostics) */
+static void ff_van_leer(double *F, const double *tr, const double *vel, const double *crf,
+                        const int *cl, int c, const int *fmap, const int *bmap) {
+  int cp, bm1, bm2;
+  double df;
+  if (crf[c] == 0.0) return;
+  cp = cl[c];
+  if (vel[c] > 0.0) cp = fmap[cp];
+  bm1 = bmap[cp];
+  bm2 = bmap[bm1];
+  if (vel[c] > 0.0) {
+    df = (tr[bm1] - tr[bm2]) * (tr[cp] - tr[bm1]);
+    if (df > 0.0)
+      df = 2.0 * df / (tr[cp] - tr[bm2]);
+    else
+      df = 0.0;
+    F[c] = tr[bm1] + 0.5 * (1.0 - crf[c]) * df;
+  } else if (vel[c] < 0.0) {
+    df = (tr[cp] - tr[bm1]) * (tr[fmap[cp]] - tr[cp]);
+    if (df > 0.0)
+      df = 2.0 * df / (tr[fmap[cp]] - tr[bm1]);
+    else
+      df = 0.0;
+    F[c] = tr[cp] - 0.5 * (1.0 - crf[c]) * df;
+  }
+}
+
+/* test access: the streamline of 3D edge e through its upstream cell
+ * for velocity (u, v) over dt; returns the signed cell of get_posc and
+ * the end point and length */
+int oracle_us_ffsl_posc(geometry_t *g, int e, int ci, double u, double v, double dt,
+                        double *out) {
+  int ei = e;
+  int es = g->m2de[e];
+  double cx = g->u1x[es], cy = g->u1y[es], dist = 0.0;
+  int r = ff_posc(g, &ei, ci, u, v, &cx, &cy, dt, &dist);
+  out[0] = cx;
+  out[1] = cy;
+  out[2] = dist;
+  return r;
+}
+
+/* test access: get_linear_limit of tr into w [3*szc] and beta [szc] */
+void oracle_us_ffsl_limit(geometry_t *g, win_priv_t *wc, const double *tr, double *w,
+                          double *beta) {
+  ff_linear_limit(g, wc, tr, w, beta);
+}
+
+/* transport.c:1961-2598  ffsl_don.  A streamline that leaves its first
+ * cell into a wet cell needs hd_trans_interp (Delaunay interpolation of
+ * the centred velocities); that case is reported (error 3).          */
+static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr, double *Fx,
+                        double *Fz, double dtu, uscr_t *s) {
+  const double tf = 0.55;
+  const double *u1 = wd->u1, *u2 = wd->u2, *w = wd->w, *u1v = wd->u1flux3d;
+  double *cour = s->cour, *cx = s->cx, *cy = s->cy, *tr_mod = s->trm, *tr_moc = s->moc;
+  double *ntr = s->ntr, *dint = s->dint;
+  int *cl = s->cl, *mask = s->mask;
+  int cc, ee, n;
+
+  /* the largest Courant number of each cell (transport.c:2058-2068) */
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int c2 = g->m2d[c];
+    cour[c] = fabs(wc->w8[c]);
+    for (n = 1; n <= g->npe[c2]; n++) {
+      int e = g->c2e[n][c];
+      cour[c] = max(cour[c], fabs(wc->w6[e]));
+    }
+  }
+  memset(cx, 0, g->sze * sizeof(double));
+  memset(cy, 0, g->sze * sizeof(double));
+  memset(ntr, 0, g->sze * sizeof(double));
+  memset(dint, 0, g->sze * sizeof(double));
+  memset(tr_moc, 0, g->szc * sizeof(double));
+  memset(mask, 0, g->szc * sizeof(int));
+
+  /* z-direction transverse terms (transport.c:2082-2106) */
+  memcpy(tr_mod, tr, g->szc * sizeof(double));
+  for (cc = 1; cc <= g->a3_t; cc++) {
+    int c = g->w3_t[cc];
+    int zp1 = (ff_nw[c] < 0.0) ? ff_clzc[c] : g->zm1[ff_clzc[c]];
+    int ci = (ff_nw[c] < 0.0) ? g->zm1[zp1] : ff_clzc[c];
+    tr_mod[c] = tf * tr[ci] + (1.0 - tf) * tr[c];
+    if (ff_crfzc[c] > 0.) tr_mod[c] += tf * (ff_crfzc[c] * (tr[zp1] - tr[ci]));
+  }
+  ff_nograd(g, wc, tr_mod);
+
+  /* streamline start points and upstream cells (transport.c:2132-2173) */
+  for (ee = 1; ee <= g->a3_e1; ee++) {
+    int e = g->w3_e1[ee];
+    int e2 = g->m2de[e];
+    cx[e] = g->u1x[e2];
+    cy[e] = g->u1y[e2];
+  }
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    int c = g->w3_t[cc];
+    int c2 = g->m2d[c];
+    for (n = 1; n <= g->npe[c2]; n++) {
+      int e = g->c2e[n][c];
+      ee = ff_e2ee[e];
+      if (u1[e] > 0.0)
+        cl[ee] = (g->eSc[n][c2] < 0) ? g->c2c[n][c] : c;
+      else
+        cl[ee] = (g->eSc[n][c2] < 0) ? c : g->c2c[n][c];
+    }
+  }
+  ff_linear_limit(g, wc, tr_mod, s->lww, s->lwb);
+
+  /* trace back and integrate (transport.c:2192-2354) */
+  for (ee = 1; ee <= g->a3_e1; ee++) {
+    int e = g->w3_e1[ee];
+    int es = g->m2de[e];
+    int c1 = g->e2c[e][0], c2 = g->e2c[e][1];
+    int ei = e, first = 0;
+    double time_left = dtu, dist, trs, tre, px, py;
+    double u = u1[e] * g->costhu1[es] + u2[e] * g->costhu2[es];
+    double v = u1[e] * g->sinthu1[es] + u2[e] * g->sinthu2[es];
+    while (cl[ee] > 0) {
+      int ci = cl[ee];
+      if (first) {
+        ous_err = 3;
+        break;
+      }
+      first = 1;
+      px = cx[e];
+      py = cy[e];
+      cl[ee] = ff_posc(g, &ei, ci, u, v, &cx[e], &cy[e], time_left, &dist);
+      if (cl[ee] > 0) {
+        double d1 = sqrt(u * u + v * v);
+        time_left -= dist / d1;
+      }
+      trs = ff_lin_value(g, s->lww, s->lwb, ci, px, py);
+      tre = ff_lin_value(g, s->lww, s->lwb, ci, cx[e], cy[e]);
+      dint[e] += dist;
+      ntr[e] += 0.5 * dist * (trs + tre);
+      if (cl[ee] < 0) {
+        tre = ff_lin_value(g, s->lww, s->lwb, ci, cx[e], cy[e]);
+        tr_moc[c1] += tre;
+        tr_moc[c2] += tre;
+        mask[c1]++;
+        mask[c2]++;
+      }
+    }
+    Fx[e] = (dint[e]) ? ntr[e] / dint[e] : Fx[e];
+  }
+  for (ee = 1; ee <= g->n3_e1; ee++) {
+    int e = g->w3_e1[ee];
+    Fx[e] *= u1v[e];
+  }
+
+  /* horizontal transverse terms (transport.c:2412-2420) */
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    int c = g->w3_t[cc];
+    if (mask[c]) tr_moc[c] /= (double)mask[c];
+    tr_moc[c] = tf * tr_moc[c] + (1.0 - tf) * tr[c];
+  }
+  ff_nograd(g, wc, tr_moc);
+
+  /* vertical fluxes, V_VC (transport.c:2441-2490) */
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc];
+    int zm1 = g->zm1[c];
+    int ci, it = 0;
+    double dz, dist;
+    if (cour[c] < 1.0)
+      ff_van_leer(Fz, tr, w, ff_crfzf, ff_clzf, c, g->zp1, g->zm1);
+    else
+      ff_van_leer(Fz, tr_moc, w, ff_crfzf, ff_clzf, c, g->zp1, g->zm1);
+    if (w[c] == 0.0 || zm1 == g->zm1[zm1]) {
+      Fz[c] = 0.0;
+      continue;
+    }
+    ci = ff_clzf[c];
+    Fz[c] *= (ff_crfzf[c] * wc->dz[ci]);
+    ci = (w[c] > 0.0) ? g->zm1[c] : wc->s1[cc];
+    dz = wc->dz[ci];
+    dist = fabs(w[c] * dtu);
+    while (dz < dist) {
+      /* a zero thickness would never end the walk: report it */
+      if (++it > 4 * g->nz + 4) {
+        ous_err = 4;
+        break;
+      }
+      Fz[c] += (dz * tr_moc[ci]);
+      dist -= dz;
+      ci = (w[c] < 0.0) ? g->zp1[ci] : g->zm1[ci];
+      if (wc->dz[ci] > 0) dz = wc->dz[ci];
+    }
+    if (w[c] < 0) Fz[c] *= -1.0;
+  }
 }
 
 /*-------------------------------------------------------------------*/
@@ -555,6 +1070,26 @@ int oracle_us_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int
       int c = g->w3_t[cc];
       wc->w9[c] = 0.5 * (wc->dz[c] + wc->dz[g->zm1[c]]);
     }
+  } else if (wc->trasc & FFSL) {
+    /* tracers.c:567-590: spacings, the centred w of Leonard et al.
+     * (1996), then ffsl_init */
+    itermax = 200;
+    sf = wc->trsf;
+    ff_alloc(g);
+    memset(ff_nw, 0, g->szc * sizeof(double));
+    for (cc = 1; cc <= g->a3_t; cc++) {
+      int c = g->w3_t[cc];
+      int zp1 = g->zp1[c];
+      wc->w9[c] = 0.5 * (wc->dz[c] + wc->dz[g->zm1[c]]);
+      ff_nw[c] = 0.0;
+      if ((w[c] >= 0.0) && (w[zp1] > 0.0))
+        ff_nw[c] = w[c];
+      else if ((w[c] < 0.0) && (w[zp1] <= 0.0))
+        ff_nw[c] = w[zp1];
+    }
+    ff_init(g, wd, wc, ff_nw);
+    /* edge -> position in w3_e1 (transport.c:2160), the same every call */
+    for (ee = 1; ee <= g->a3_e1; ee++) ff_e2ee[g->w3_e1[ee]] = ee;
   } else {
     /* tracers.c:592-603 (first order also passes here) */
     for (cc = 1; cc <= g->b3_t; cc++) {
@@ -581,7 +1116,14 @@ int oracle_us_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int
       for (j = 1; j <= g->npe[cs]; j++) {
         int e = g->c2e[j][c];
         int e2 = g->m2de[e];
-        d4 = u1[e];
+        if (wc->trasc & FFSL) {
+          /* the one-sided Lipschitz number (tracers.c:624-631) */
+          int c1 = g->e2c[e][0], c2 = g->e2c[e][1];
+          d4 = (wd->u[c1] * g->costhu1[e2] + wd->v[c1] * g->sinthu1[e2]) -
+               (wd->u[c2] * g->costhu1[e2] + wd->v[c2] * g->sinthu1[e2]);
+          d4 = (d4 < 0.0) ? -d4 : 0.0;
+        } else
+          d4 = u1[e];
         if (fabs(d4) < minval) d4 = minval;
         d1 = sf * fabs(g->h2au1[e2] / d4);
         if (d1 < dtm) {
@@ -589,7 +1131,13 @@ int oracle_us_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int
           ii = cs;
         }
       }
-      d4 = (cc <= vcs) ? 0.0 : 0.5 * (w[c] + w[zp1]);
+      if (wc->trasc & FFSL) {
+        /* tracers.c:652-656; the bed cell (W_BOT) has none */
+        d4 = (cc <= vcs) ? 0.0 : w[zp1] - w[c];
+        if (g->zm1[g->zm1[c]] == g->zm1[c]) d4 = 0.0;
+        d4 = (d4 < 0.0) ? -d4 : 0.0;
+      } else
+        d4 = (cc <= vcs) ? 0.0 : 0.5 * (w[c] + w[zp1]);
       if (fabs(d4) < minval) d4 = minval;
       /* the reference scales by Ds of the last edge's centre; Ds = 1
        * at z-levels, so the column's own Ds is used */
@@ -636,6 +1184,7 @@ int oracle_us_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int
         subeta[c2] = wc->oldeta[c2] + (1.0 + (dtu - trem) / wd->dt) *
                                           (wd->eta[c2] - wc->oldeta[c2]);
       }
+      if (wc->trasc & FFSL) ff_prep(g, wd, wc, dtu);
       {
         int nt = nthreads > 0 ? nthreads : 1;
         int nsz = g->szc > g->sze ? g->szc : g->sze;
@@ -663,7 +1212,8 @@ int oracle_us_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int
     trem -= dtu;
     slf = 0;
   }
-  return 0;
+  /* FFSL: a multi-segment streamline (3) or an endless vertical walk (4) */
+  return ous_err ? -ous_err : 0;
 }
 
 int oracle_us_advect_diffuse(geometry_t *g, window_t *wd, win_priv_t *wc) {

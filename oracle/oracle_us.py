@@ -7,6 +7,8 @@ import ctypes as C
 import os
 import subprocess
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liboracle_us.so")
 _lib = None
@@ -42,6 +44,12 @@ def lib():
         L.oracle_us_tmass.argtypes = [G, W, P, C.c_int]
         L.oracle_us_tmass.restype = C.c_double
         L.oracle_us_last_error.restype = C.c_int
+        _PD = C.POINTER(C.c_double)
+        L.oracle_us_ffsl_posc.argtypes = [G, C.c_int, C.c_int, C.c_double, C.c_double,
+                                          C.c_double, _PD]
+        L.oracle_us_ffsl_posc.restype = C.c_int
+        L.oracle_us_ffsl_limit.argtypes = [G, P, _PD, _PD, _PD]
+        L.oracle_us_ffsl_limit.restype = None
         _lib = L
     return _lib
 
@@ -68,3 +76,23 @@ def vert_diffuse(case, nthreads=1):
 
 def tmass(case, n):
     return lib().oracle_us_tmass(case.gp, case.wp, case.pp, n)
+
+
+def ffsl_posc(case, e, ci, u, v, dt):
+    """get_posc of edge e from its upstream cell ci: (signed cell, x, y,
+    distance)."""
+    out = np.zeros(3)
+    r = lib().oracle_us_ffsl_posc(case.gp, int(e), int(ci), float(u), float(v), float(dt),
+                                  out.ctypes.data_as(C.POINTER(C.c_double)))
+    return r, out[0], out[1], out[2]
+
+
+def ffsl_limit(case, tr):
+    """get_linear_limit of the sparse array tr: weights [szc, 3], beta [szc]."""
+    t = np.ascontiguousarray(tr, dtype=np.float64)
+    w = np.zeros(3 * case.szc)
+    b = np.zeros(case.szc)
+    P = C.POINTER(C.c_double)
+    lib().oracle_us_ffsl_limit(case.gp, case.pp, t.ctypes.data_as(P), w.ctypes.data_as(P),
+                               b.ctypes.data_as(P))
+    return w.reshape(case.szc, 3), b
