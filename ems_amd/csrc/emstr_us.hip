@@ -22,7 +22,8 @@
  *                 (tracers.c:3085-3260) and tr_bounds (tracers.c:3270-3370)
  *                 for its column.
  *   k_usdtm       minimum sub-step over the cells to process
- *                 (tracers.c:601-683).
+ *                 (tracers.c:601-683), Courant or (FFSL) Lipschitz.
+ * The FFSL fluxes are emstr_us_ffsl.hip's; k_usadv then updates.
  * The column kernels (set_dz, sub-step elevations, vertical diffusion,
  * mass, scatter/gather) are those of the plain SHOC form.
  */
@@ -428,7 +429,12 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
 /* minimum sub-step (tracers.c:601-683): edge velocities over the centre
  * distance, w over dz below the surface; positive doubles compare as
  * their bit patterns */
-__global__ void k_usdtm(emstr_usdev D, double dtu0, unsigned long long *out) {
+/* FFSL (tracers.c:624-631, 652-656): the one-sided Lipschitz number
+ * of the centred velocities across each edge (a land edge's far side is
+ * its ghost, gu/gv) and of w across the cell, scaled by trsf */
+__global__ void k_usdtm(emstr_usdev D, double dtu0, unsigned long long *out, int ffsl,
+                        double sff, const double *__restrict__ gu,
+                        const double *__restrict__ gv) {
   const emstr_dev &d = D.d;
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.z;
@@ -439,18 +445,42 @@ __global__ void k_usdtm(emstr_usdev D, double dtu0, unsigned long long *out) {
   const ll P = d.plane;
   const ll idx = (ll)k * P + col;
   const int ncol = (int)P;
-  const double sf = 0.8, minval = 1e-10;
+  const double sf = ffsl ? sff : 0.8, minval = 1e-10;
   double dtm = dtu0, d4;
   for (int j = 0; j < D.npe[col]; j++) {
     const int e = D.ce[j * ncol + col];
-    d4 = D.u1[(ll)k * D.ne2 + e];
+    const ll ie = (ll)k * D.ne2 + e;
+    if (ffsl) {
+      const int c1 = D.ec1[e], c2 = D.ec2[e];
+      double ua, va, ub, vb;
+      if (wetk(d.kb, c1, k) && wetk(d.kb, c2, k)) {
+        ua = D.uc[(ll)k * P + c1];
+        va = D.vc[(ll)k * P + c1];
+        ub = D.uc[(ll)k * P + c2];
+        vb = D.vc[(ll)k * P + c2];
+      } else {
+        /* a land edge joins the wet centre (e2c[e][0]) to its ghost */
+        ua = D.uc[idx];
+        va = D.vc[idx];
+        ub = gu[ie];
+        vb = gv[ie];
+      }
+      d4 = (ua * D.n1x[e] + va * D.n1y[e]) - (ub * D.n1x[e] + vb * D.n1y[e]);
+      d4 = (d4 < 0.0) ? -d4 : 0.0;
+    } else
+      d4 = D.u1[ie];
     if (fabs(d4) < minval) d4 = minval;
     const double d1 = sf * fabs(D.h2e[e] / d4);
     if (d1 < dtm) dtm = d1;
   }
   const bool surf = (k == ksc);
   const ll zp = (k + 1 < d.nz) ? idx + P : idx;
-  d4 = surf ? 0.0 : 0.5 * (d.w[idx] + d.w[zp]);
+  if (ffsl) {
+    d4 = surf ? 0.0 : d.w[zp] - d.w[idx];
+    if (k == kbc) d4 = 0.0;
+    d4 = (d4 < 0.0) ? -d4 : 0.0;
+  } else
+    d4 = surf ? 0.0 : 0.5 * (d.w[idx] + d.w[zp]);
   if (fabs(d4) < minval) d4 = minval;
   const double d3 = surf ? dtu0 : sf * fabs(d.dz[idx] * 1.0 / d4);
   if (d3 > 0.0 && d3 < dtm) dtm = d3;
@@ -494,10 +524,22 @@ extern "C" int emstr_us_launch_adv(const emstr_usdev *D, const double *tin, doub
 }
 
 extern "C" int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned long long *out,
+                                   double sf, const double *gu, const double *gv,
                                    void *stream) {
   const emstr_dev *d = &D->d;
   dim3 b(256, 1, 1);
   dim3 g((d->nx + 255) / 256, 1, d->nz);
-  k_usdtm<<<g, b, 0, (hipStream_t)stream>>>(*D, dtu0, out);
+  k_usdtm<<<g, b, 0, (hipStream_t)stream>>>(*D, dtu0, out, D->ffsl, sf, gu, gv);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+/* the cell update from given edge and vertical fluxes (FFSL) */
+extern "C" int emstr_us_launch_update(const emstr_usdev *D, const double *tin, double *tout,
+                                      const double *fx, const double *fz, double dtu,
+                                      void *stream) {
+  const emstr_dev *d = &D->d;
+  if (d->nwet)
+    k_usadv<VANLEER><<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
+        *D, tin, tout, fx, fz, d->wdn, d->nwet, dtu);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

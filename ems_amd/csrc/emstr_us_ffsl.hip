@@ -1,0 +1,557 @@
+/*
+ * COMPAS FFSL on the GPU, plain FP64 (model/hd-us/tracers/transport.c
+ * ffsl_don :1961-2598, ffsl_init :2607-2723, get_posc :1476-1559,
+ * intersect :1744-1826; tracers.c prep_ff_sl :2670-2721, ff_sl_do_vert
+ * :2732-2770, get_linear_value :4682-4708, get_linear_limit :4718-4871).
+ *
+ * Cells are [layer][column] as in the rest of the COMPAS path; inside a
+ * column a layer index stands for a centre, layer kb-1 for the sediment
+ * ghost (which reads the bed value), and the top layer maps to itself
+ * upwards.  Kernels, in launch order:
+ *   k_ffnw     per step: the centred vertical velocity (Leonard et al.
+ *              choice, or vel_center_w on the calls where ffsl_init runs)
+ *   k_ffsub    per sub-step: vertical departure layers and fractions of
+ *              the centres (nw, w9) and of the faces of the cells to
+ *              process (w, dz), and their largest Courant number
+ *   k_ffline   per sub-step, one thread per wet edge: the streamline back
+ *              from the edge midpoint through its upstream cell
+ *              (get_posc / intersect), stored as offsets from the cell
+ *              centre and a length; a streamline that reaches a wet
+ *              neighbour is reported (it needs hd_trans_interp)
+ *   k_ffmod    tr_mod: the vertical transverse tracer, no-gradient above
+ *              the processed surface
+ *   k_fflsq    the least-squares slopes of every centre and the vertex
+ *              limiter beta
+ *   k_ffedge   per wet edge: the streamline mean (trapezoid of the
+ *              limited reconstruction), times u1flux3d, plus horizontal
+ *              diffusion; the end value for the transverse terms
+ *   k_ffmoc    tr_moc: the mean end value over the cell's wet edges, in
+ *              the order the reference accumulates them
+ *   k_fffz     the vertical flux of the cells to process (V_VC: van Leer
+ *              on the fractional part, tr_moc over whole layers)
+ * The cell update, surf_concrk and tr_bounds are k_usadv's.  Every
+ * per-tracer kernel runs one thread per cell or edge over all tracers.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "emstr_us_kernels.h"
+
+typedef long long ll;
+
+#define FF_TF 0.55     /* transport.c:2014 */
+#define FF_SMALL 1e-10 /* transport.c:24 */
+#define FF_CH 8        /* tracers per pass in k_fflsq */
+
+__device__ __forceinline__ double fmx(double a, double b) { return (a > b) ? a : b; }
+__device__ __forceinline__ double fmn(double a, double b) { return (a < b) ? a : b; }
+
+__device__ __forceinline__ bool ffwet(const int *__restrict__ kb, int col, int k) {
+  return col >= 0 && k >= kb[col];
+}
+/* vertical maps by layer: the bed maps down to the sediment ghost (kb-1),
+ * which maps to itself; the sediment maps up to the bed */
+__device__ __forceinline__ int ffzm1(int L, int kb) { return (L >= kb) ? L - 1 : L; }
+__device__ __forceinline__ int ffzp1(int L, int kb, int nz) {
+  return (L < kb) ? kb : ((L < nz - 1) ? L + 1 : L);
+}
+/* layer thickness; 0 for the sediment ghost */
+__device__ __forceinline__ double ffdz(const emstr_dev &d, int col, int L, int kb) {
+  return (L < kb) ? 0.0 : d.dz[(ll)L * d.plane + col];
+}
+/* w9 = 0.5 (dz + dz below) of a centre (tracers.c:571-575); 0 for the
+ * sediment ghost, whose entry the reference never writes */
+__device__ __forceinline__ double ffw9(const emstr_dev &d, int col, int L, int kb) {
+  return (L < kb) ? 0.0 : 0.5 * (ffdz(d, col, L, kb) + ffdz(d, col, L - 1, kb));
+}
+
+/*-------------------------------------------------------------------*/
+/* per step: centred vertical velocity                               */
+/*-------------------------------------------------------------------*/
+__global__ void __launch_bounds__(256) k_ffnw(emstr_usdev D, int init, double *__restrict__ nw) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.nwet) return;
+  const ll P = d.plane;
+  const ll idx = d.wdn[q];
+  const int col = (int)(idx % P), k = (int)(idx / P);
+  const int kb = d.kb[col];
+  const double *w = d.w;
+  double v;
+  if (!init) {
+    /* tracers.c:579-586 */
+    const double wc = w[idx], wp = w[(k < d.nz - 1) ? idx + P : idx];
+    v = 0.0;
+    if ((wc >= 0.0) && (wp > 0.0))
+      v = wc;
+    else if ((wc < 0.0) && (wp <= 0.0))
+      v = wp;
+  } else {
+    /* vel_center_w (transport.c:6098-6141) from the old surface down */
+    const int ko = d.kold[col];
+    if (k > ko) {
+      v = 0.0;
+    } else {
+      if (k == ko)
+        v = 0.5 * (w[idx] + D.wtop[col]);
+      else
+        v = 0.5 * (w[idx + P] + ((k == kb) ? D.wbot[col] : w[idx]));
+      if (fabs(v) < FF_SMALL) v = FF_SMALL;
+    }
+  }
+  nw[idx] = v;
+}
+
+/*-------------------------------------------------------------------*/
+/* per sub-step: vertical maps (prep_ff_sl) and Courant numbers      */
+/*-------------------------------------------------------------------*/
+/* ff_sl_do_vert for one centre or face: h is w9 (centres) or dz (faces) */
+template <bool W9>
+__device__ __forceinline__ void ffdovert(const emstr_dev &d, int col, int k, int kb,
+                                         double vel, double dt, int &cl, double &crf) {
+  crf = 0.0;
+  if (vel == 0.0) return;
+  const int nz = d.nz;
+#define FFH(L) (W9 ? ffw9(d, col, (L), kb) : ffdz(d, col, (L), kb))
+  const double lt = fabs(vel) * dt;
+  double lrem = lt;
+  int cp = cl, cpb = cl;
+  double dz = fabs((FFH(cp) > 0) ? FFH(cp) : FFH(k));
+  while (dz < lrem) {
+    lrem -= dz;
+    cp = (vel < 0.0) ? ffzp1(cp, kb, nz) : ffzm1(cp, kb);
+    if (cpb == cp) {
+      lrem = fmod(lrem, dz);
+      break;
+    }
+    if (FFH(cp) > 0) dz = fabs(FFH(cp));
+    cpb = cp;
+  }
+#undef FFH
+  crf = fmx(fmn(lrem / dz, 1.0), 0.0);
+  cl = cp;
+}
+
+__global__ void __launch_bounds__(256)
+k_ffsub(emstr_usdev D, double dtu, const double *__restrict__ nw, int *__restrict__ clc,
+        double *__restrict__ crc, int *__restrict__ clf, double *__restrict__ crf,
+        double *__restrict__ cour) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.nwet) return;
+  const ll P = d.plane;
+  const ll idx = d.wdn[q];
+  const int col = (int)(idx % P), k = (int)(idx / P);
+  const int kb = d.kb[col], ks = d.ks[col], nz = d.nz;
+  /* centres (tracers.c:2697-2706) */
+  {
+    const double v = nw[idx];
+    int cl = (v < 0.0) ? ffzp1(k, kb, nz) : k;
+    double r;
+    ffdovert<true>(d, col, k, kb, v, dtu, cl, r);
+    clc[idx] = cl;
+    crc[idx] = r;
+  }
+  /* faces of the cells to process (tracers.c:2711-2720) */
+  if (ks >= 0 && k <= ks) {
+    const double v = d.w[idx];
+    int cl = (v > 0.0) ? ffzm1(k, kb) : k;
+    double r;
+    ffdovert<false>(d, col, k, kb, v, dtu, cl, r);
+    clf[idx] = cl;
+    crf[idx] = r;
+    /* the largest Courant number (tracers.c:751-759, transport.c:2058-2068) */
+    const double w8 = fmod(v * dtu / (ffw9(d, col, k, kb) * 1.0), 1.0);
+    double cr = fabs(w8);
+    const int ncol = (int)P;
+    for (int j = 0; j < D.npe[col]; j++) {
+      const int e = D.ce[j * ncol + col];
+      const double w6 = fmod(D.u1[(ll)k * D.ne2 + e] * dtu / D.h2e[e], 1.0);
+      cr = fmx(cr, fabs(w6));
+    }
+    cour[idx] = cr;
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* per sub-step: streamlines                                         */
+/*-------------------------------------------------------------------*/
+/* intersect (transport.c:1744-1826) with edge es of the surface layer */
+__device__ __forceinline__ int ffcross(const emstr_usdev &D, int es, double xn, double yn,
+                                       double xs, double ys, double &xi, double &yi) {
+  const double v1x = D.ev[4 * es + 0], v1y = D.ev[4 * es + 1];
+  const double v2x = D.ev[4 * es + 2], v2y = D.ev[4 * es + 3];
+  double s1, s2;
+  const double dx = fabs(v1x - v2x);
+  if (dx) {
+    s1 = (v2y - v1y) / (v2x - v1x);
+    if (xs == xn)
+      xi = xn;
+    else {
+      s2 = (ys - yn) / (xs - xn);
+      xi = (s2 * xn - s1 * v1x + v1y - yn) / (s2 - s1);
+    }
+    yi = s1 * (xi - v1x) + v1y;
+  } else {
+    s1 = (v2x - v1x) / (v2y - v1y);
+    if (ys == yn)
+      yi = yn;
+    else {
+      s2 = (xs - xn) / (ys - yn);
+      yi = (s1 * v1y - s2 * yn + xn - v1x) / (s1 - s2);
+    }
+    xi = s1 * (yi - v1y) + v1x;
+  }
+  if (xi >= fmn(v1x, v2x) && xi <= fmx(v1x, v2x) && yi >= fmn(v1y, v2y) &&
+      yi <= fmx(v1y, v2y)) {
+    if (xi >= fmn(xn, xs) && xi <= fmx(xn, xs) && yi >= fmn(yn, ys) && yi <= fmx(yn, ys))
+      return 1;
+  }
+  return 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_ffline(emstr_usdev D, double dtu, const ll *__restrict__ edn, int nesp,
+         int *__restrict__ lci, double *__restrict__ ld, int *__restrict__ err) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nesp) return;
+  const ll ie = edn[q];
+  const ll NE3 = (ll)d.nz * D.ne2;
+  const int k = (int)(ie / D.ne2), e = (int)(ie % D.ne2);
+  const int ncol = (int)d.plane;
+  const double u1 = D.u1[ie], u2 = D.u2[ie];
+  const double u = u1 * D.n1x[e] + u2 * D.t2x[e];
+  const double v = u1 * D.n1y[e] + u2 * D.t2y[e];
+  /* the upstream centre (transport.c:2162-2173) */
+  const int ci = (u1 > 0.0) ? D.ec2[e] : D.ec1[e];
+  const double px = D.ux[e], py = D.uy[e];
+  /* get_posc (transport.c:1476-1559), metres */
+  const double dist = sqrt(u * u + v * v) * dtu * 1.0;
+  const double dir = atan2(v, u);
+  const double sinth = sin(dir), costh = cos(dir);
+  const double slon = px - dist * costh, slat = py - dist * sinth;
+  double ex = px, ey = py, od = 0.0;
+  if (dist != 0.0) {
+    int found = 0, cn = ci;
+    double xi = 0.0, yi = 0.0;
+    for (int j = 0; j < D.npe[ci]; j++) {
+      const int ej = D.ce[j * ncol + ci];
+      if (ej != e && (found = ffcross(D, ej, px, py, slon, slat, xi, yi))) {
+        cn = (D.ec1[ej] == ci) ? D.ec2[ej] : D.ec1[ej];
+        break;
+      }
+    }
+    if (found && ffwet(d.kb, cn, k)) {
+      /* the streamline continues in a wet cell: multi-segment */
+      atomicMax(err, 1);
+    }
+    if (found) {
+      ex = xi;
+      ey = yi;
+    } else {
+      ex = slon;
+      ey = slat;
+    }
+    const double dx = ex - px, dy = ey - py;
+    od = sqrt(dx * dx + dy * dy) / 1.0;
+  }
+  lci[ie] = ci;
+  const double cxc = D.cx[ci], cyc = D.cy[ci];
+  ld[0 * NE3 + ie] = cxc - px;
+  ld[1 * NE3 + ie] = cyc - py;
+  ld[2 * NE3 + ie] = cxc - ex;
+  ld[3 * NE3 + ie] = cyc - ey;
+  ld[4 * NE3 + ie] = od;
+}
+
+/*-------------------------------------------------------------------*/
+/* per tracer                                                        */
+/*-------------------------------------------------------------------*/
+/* tr_mod (transport.c:2082-2106 with set_tr_nograd) */
+__global__ void __launch_bounds__(256)
+k_ffmod(emstr_usdev D, const double *__restrict__ tin, const double *__restrict__ nw,
+        const int *__restrict__ clc, const double *__restrict__ crc, double *__restrict__ tm) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.nwet) return;
+  const ll P = d.plane, N3 = d.n3;
+  const ll idx = d.wdn[q];
+  const int col = (int)(idx % P), k = (int)(idx / P);
+  const int kb = d.kb[col], ks = d.ks[col];
+  const int kk = (ks >= 0 && k > ks) ? ks : k; /* above the surface: its value */
+  const ll ik = (ll)kk * P + col;
+  const double vel = nw[ik], crf = crc[ik];
+  const int cl = clc[ik];
+  const int zp = (vel < 0.0) ? cl : ffzm1(cl, kb);
+  const int ci = (vel < 0.0) ? ffzm1(zp, kb) : cl;
+  const ll iz = (ll)((zp < kb) ? kb : zp) * P + col;
+  const ll ic = (ll)((ci < kb) ? kb : ci) * P + col;
+  for (int n = 0; n < d.ntr; n++) {
+    if (!d.in_tbdy[n] || !d.advect[n]) continue;
+    const double *t = tin + n * N3;
+    double r = FF_TF * t[ic] + (1.0 - FF_TF) * t[ik];
+    if (crf > 0.) r += FF_TF * (crf * (t[iz] - t[ic]));
+    tm[n * N3 + idx] = r;
+  }
+}
+
+/* get_linear_limit, df = 2: slopes w1, w2 from the stencil, beta from the
+ * vertex ranges (lw->beta is 1 while the vertex values are taken) */
+__global__ void __launch_bounds__(256)
+k_fflsq(emstr_usdev D, const double *__restrict__ tm, double *__restrict__ lq) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.nwet) return;
+  const ll P = d.plane, N3 = d.n3, NT3 = (ll)d.ntr * N3;
+  const ll idx = d.wdn[q];
+  const int col = (int)(idx % P), k = (int)(idx / P);
+  const int ncol = (int)P, npe = D.npe[col];
+  const int o0 = D.lsq_off[q], o1 = D.lsq_off[q + 1];
+  const double cxc = D.cx[col], cyc = D.cy[col];
+  for (int n0 = 0; n0 < d.ntr; n0 += FF_CH) {
+    double a1[FF_CH], a2[FF_CH];
+    bool on[FF_CH];
+#pragma unroll
+    for (int c = 0; c < FF_CH; c++) {
+      const int n = n0 + c;
+      on[c] = (n < d.ntr) && d.in_tbdy[n] && d.advect[n];
+      a1[c] = 0.0;
+      a2[c] = 0.0;
+    }
+    for (int i = o0; i < o1; i++) {
+      const ll is = D.lsq_idx[i];
+      const double b1 = D.lsq_B[3 * (ll)i + 1], b2 = D.lsq_B[3 * (ll)i + 2];
+#pragma unroll
+      for (int c = 0; c < FF_CH; c++)
+        if (on[c]) {
+          const double t = tm[(n0 + c) * N3 + is];
+          a1[c] += (t * b1);
+          a2[c] += (t * b2);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < FF_CH; c++) {
+      if (!on[c]) continue;
+      const int n = n0 + c;
+      const double *t = tm + n * N3;
+      const double w0 = t[idx], w1 = a1[c], w2 = a2[c];
+      double beta = 1.0;
+      for (int j = 0; j < npe; j++) {
+        const int vt = D.cvt[j * ncol + col];
+        const double x = cxc - D.vx[vt], y = cyc - D.vy[vt];
+        const double val = w0 + 1.0 * (w1 * x + w2 * y);
+        const double dd = val - w0;
+        double vmin = __builtin_huge_val(), vmax = -__builtin_huge_val();
+        for (int i = 0; i < D.nvcm; i++) {
+          const int cv = D.vcol[vt * D.nvcm + i];
+          if (cv < 0) break;
+          if (!ffwet(d.kb, cv, k)) continue;
+          const double tv = t[(ll)k * P + cv];
+          vmin = fmn(vmin, tv);
+          vmax = fmx(vmax, tv);
+        }
+        if (dd != 0.0 && val > vmax) beta = fmx(fmn((vmax - w0) / dd, beta), 0.0);
+        if (dd != 0.0 && val < vmin) beta = fmx(fmn((vmin - w0) / dd, beta), 0.0);
+      }
+      lq[0 * NT3 + n * N3 + idx] = w1;
+      lq[1 * NT3 + n * N3 + idx] = w2;
+      lq[2 * NT3 + n * N3 + idx] = fmn(fmx(beta, 0.0), 1.0);
+    }
+  }
+}
+
+/* edge fluxes (transport.c:2192-2386) and horizontal diffusion
+ * (tracers.c:5438-5446) */
+__global__ void __launch_bounds__(256)
+k_ffedge(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ edn, int nesp,
+         const int *__restrict__ lci, const double *__restrict__ ld,
+         const double *__restrict__ tm, const double *__restrict__ lq, double *__restrict__ fx,
+         double *__restrict__ te, int anydif) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nesp) return;
+  const ll ie = edn[q];
+  const ll P = d.plane, N3 = d.n3, NT3 = (ll)d.ntr * N3, NE3 = (ll)d.nz * D.ne2;
+  const int k = (int)(ie / D.ne2), e = (int)(ie % D.ne2);
+  const int ci = lci[ie];
+  const ll cidx = (ll)k * P + ci;
+  const double dxs = ld[ie], dys = ld[NE3 + ie], dxe = ld[2 * NE3 + ie];
+  const double dye = ld[3 * NE3 + ie], dist = ld[4 * NE3 + ie];
+  const double f = D.flux[ie];
+  const ll i1 = (ll)k * P + D.ec1[e], i2 = (ll)k * P + D.ec2[e];
+  for (int n = 0; n < d.ntr; n++) {
+    if (!d.in_tbdy[n]) continue;
+    double F = 0.0;
+    if (d.advect[n]) {
+      const double w0 = tm[n * N3 + cidx];
+      const double w1 = lq[n * N3 + cidx], w2 = lq[NT3 + n * N3 + cidx];
+      const double b = lq[2 * NT3 + n * N3 + cidx];
+      const double trs = w0 + b * (w1 * dxs + w2 * dys);
+      const double tre = w0 + b * (w1 * dxe + w2 * dye);
+      const double dint = 0.0 + dist;
+      const double ntr = 0.0 + 0.5 * dist * (trs + tre);
+      F = (dint != 0.0) ? ntr / dint : 0.0;
+      F *= f;
+      te[n * NE3 + ie] = tre;
+    }
+    if (anydif && d.diffuse[n]) {
+      const double *t = tin + n * N3;
+      const double csx = D.dzu[ie] * D.h1e[e] * 1.0;
+      F -= (csx * D.kh[ie] * (t[i1] - t[i2]) / D.h2e[e]);
+    }
+    fx[n * NE3 + ie] = F;
+  }
+}
+
+/* tr_moc (transport.c:2323-2330, 2412-2420 with set_tr_nograd) */
+__global__ void __launch_bounds__(256)
+k_ffmoc(emstr_usdev D, const double *__restrict__ tin, const double *__restrict__ te,
+        double *__restrict__ moc) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.nwet) return;
+  const ll P = d.plane, N3 = d.n3, NE3 = (ll)d.nz * D.ne2;
+  const ll idx = d.wdn[q];
+  const int col = (int)(idx % P), k = (int)(idx / P);
+  const int ks = d.ks[col], ncol = (int)P;
+  const int kk = (ks >= 0 && k > ks) ? ks : k;
+  const ll ik = (ll)kk * P + col;
+  ll el[8];
+  int m = 0;
+  for (int s = 0; s < D.npe[col] && s < 8; s++) {
+    const int j = D.eord[s * ncol + col];
+    const int e = D.ce[j * ncol + col];
+    const int other = (D.ec1[e] == col) ? D.ec2[e] : D.ec1[e];
+    if (ffwet(d.kb, other, kk)) el[m++] = (ll)kk * D.ne2 + e;
+  }
+  for (int n = 0; n < d.ntr; n++) {
+    if (!d.in_tbdy[n] || !d.advect[n]) continue;
+    const double *tn = te + n * NE3;
+    double r = 0.0;
+    for (int i = 0; i < m; i++) r += tn[el[i]];
+    if (m) r /= (double)m;
+    r = FF_TF * r + (1.0 - FF_TF) * tin[n * N3 + ik];
+    moc[n * N3 + idx] = r;
+  }
+}
+
+/* vertical fluxes (transport.c:2441-2490, ffsl_van_leer :8216-8260) */
+__global__ void __launch_bounds__(256)
+k_fffz(emstr_usdev D, double dtu, const double *__restrict__ tin, const double *__restrict__ moc,
+       const int *__restrict__ clf, const double *__restrict__ crf,
+       const double *__restrict__ cour, double *__restrict__ fz, int *__restrict__ err) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.nwet) return;
+  const ll P = d.plane, N3 = d.n3;
+  const ll idx = d.wdn[q];
+  const int col = (int)(idx % P), k = (int)(idx / P);
+  const int kb = d.kb[col], ks = d.ks[col], nz = d.nz;
+  if (ks < 0 || k > ks) return;
+  const double w = d.w[idx];
+  const int cl = clf[idx];
+  const double cr = crf[idx];
+  const bool usemoc = !(cour[idx] < 1.0);
+  /* the van Leer stencil (fixed per cell) */
+  int cp = cl;
+  if (w > 0.0) cp = ffzp1(cp, kb, nz);
+  const int bm1 = ffzm1(cp, kb), bm2 = ffzm1(bm1, kb), fp = ffzp1(cp, kb, nz);
+  const double fr = (w == 0.0 || k == kb) ? 0.0 : cr * ffdz(d, col, cl, kb);
+  for (int n = 0; n < d.ntr; n++) {
+    if (!d.in_tbdy[n] || !d.advect[n]) continue;
+    const double *t = tin + n * N3, *mc = moc + n * N3;
+#define FFT(L) t[(ll)(((L) < kb) ? kb : (L)) * P + col]
+#define FFM(L) mc[(ll)(((L) < kb) ? kb : (((L) > ks) ? ks : (L))) * P + col]
+#define FFV(L) (usemoc ? FFM(L) : FFT(L))
+    double F = 0.0, df;
+    if (cr != 0.0) {
+      if (w > 0.0) {
+        df = (FFV(bm1) - FFV(bm2)) * (FFV(cp) - FFV(bm1));
+        if (df > 0.0)
+          df = 2.0 * df / (FFV(cp) - FFV(bm2));
+        else
+          df = 0.0;
+        F = FFV(bm1) + 0.5 * (1.0 - cr) * df;
+      } else if (w < 0.0) {
+        df = (FFV(cp) - FFV(bm1)) * (FFV(fp) - FFV(cp));
+        if (df > 0.0)
+          df = 2.0 * df / (FFV(fp) - FFV(bm1));
+        else
+          df = 0.0;
+        F = FFV(cp) - 0.5 * (1.0 - cr) * df;
+      }
+    }
+    if (w == 0.0 || k == kb) {
+      F = 0.0;
+    } else {
+      F *= fr;
+      /* whole layers of the trajectory (transport.c:2462-2486) */
+      int ci = (w > 0.0) ? ffzm1(k, kb) : k;
+      double dz = ffdz(d, col, ci, kb);
+      double dist = fabs(w * dtu);
+      int it = 0;
+      while (dz < dist) {
+        if (++it > 4 * nz + 4) {
+          atomicMax(err, 2);
+          break;
+        }
+        F += (dz * FFM(ci));
+        dist -= dz;
+        ci = (w < 0.0) ? ffzp1(ci, kb, nz) : ffzm1(ci, kb);
+        const double dn = ffdz(d, col, ci, kb);
+        if (dn > 0) dz = dn;
+      }
+      if (w < 0) F *= -1.0;
+    }
+#undef FFT
+#undef FFM
+#undef FFV
+    fz[n * N3 + idx] = F;
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* launchers                                                         */
+/*-------------------------------------------------------------------*/
+extern "C" int emstr_us_launch_ffsl_step(const emstr_usdev *D, int init, double *nw,
+                                         void *stream) {
+  const emstr_dev *d = &D->d;
+  if (d->nwet)
+    k_ffnw<<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(*D, init, nw);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_us_launch_ffsl_sub(const emstr_usdev *D, double dtu, const double *nw,
+                                        int *clc, double *crc, int *clf, double *crf,
+                                        double *cour, const long long *edn, int nesp, int *lci,
+                                        double *ld, int *err, void *stream) {
+  const emstr_dev *d = &D->d;
+  hipStream_t st = (hipStream_t)stream;
+  if (d->nwet)
+    k_ffsub<<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, dtu, nw, clc, crc, clf, crf, cour);
+  if (nesp) k_ffline<<<(nesp + 255) / 256, 256, 0, st>>>(*D, dtu, edn, nesp, lci, ld, err);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_us_launch_ffsl_flux(const emstr_usdev *D, double dtu, const double *tin,
+                                         const double *nw, const int *clc, const double *crc,
+                                         const int *clf, const double *crf, const double *cour,
+                                         const long long *edn, int nesp, const int *lci,
+                                         const double *ld, double *tm, double *lq, double *te,
+                                         double *moc, double *fx, double *fz, int anydif,
+                                         int *err, void *stream) {
+  const emstr_dev *d = &D->d;
+  hipStream_t st = (hipStream_t)stream;
+  const int gw = (d->nwet + 255) / 256, ge = (nesp + 255) / 256;
+  if (d->nwet) {
+    k_ffmod<<<gw, 256, 0, st>>>(*D, tin, nw, clc, crc, tm);
+    k_fflsq<<<gw, 256, 0, st>>>(*D, tm, lq);
+  }
+  if (nesp) k_ffedge<<<ge, 256, 0, st>>>(*D, tin, edn, nesp, lci, ld, tm, lq, fx, te, anydif);
+  if (d->nwet) {
+    k_ffmoc<<<gw, 256, 0, st>>>(*D, tin, te, moc);
+    k_fffz<<<gw, 256, 0, st>>>(*D, dtu, tin, moc, clf, crf, cour, fz, err);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}

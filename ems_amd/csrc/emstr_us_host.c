@@ -11,8 +11,10 @@
  *
  * Scope (rejected with an error otherwise): z-levels, closed boundaries,
  * one Runge-Kutta stage, schemes van Leer and QUICKEST (with or without
- * ULTIMATE) and first order, no mean-velocity transport, no thin-layer merging, and no
- * flux or velocity through land edges.
+ * ULTIMATE), first order and FFSL (without ULTIMATE or a clipping fill,
+ * streamlines that stay in their upstream cell), no mean-velocity
+ * transport, no thin-layer merging, and no flux or velocity through land
+ * edges.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -68,6 +70,11 @@ typedef struct emstr_us_ctx {
   unsigned long long *d_dtm;
   double *d_fx;  /* edge fluxes [ntr][nz][ne2], made on first use */
   double *d_fz;  /* vertical fluxes [ntr][nz][ncol], made on first use */
+  /* FFSL work arrays, made on first use (see emstr_us_kernels.h)      */
+  int ffgeom;        /* surface geometry, vertices and stencils uploaded */
+  int ghost_edges;   /* ghost velocities are the rotated edge velocity   */
+  double *d_nw, *d_crc, *d_crf, *d_cour, *d_ld, *d_tm, *d_lq, *d_te, *d_moc;
+  int *d_clc, *d_clf, *d_lci, *d_fferr;
   int anydif, have_cols;
   int key_szc, key_sze;
 } usctx;
@@ -100,7 +107,12 @@ static void us_destroy(usctx *c) {
                 D->npe, D->ce, D->esc, D->ec1, D->ec2, D->ecp, D->ecm, D->esel, D->h1e,
                 D->h2e, D->qd3, D->qhm, D->qhp, D->u1, D->flux, D->dzu, D->kh, c->tr[0], c->tr[1], c->tr[2],
                 c->stage, c->d_esp, c->d_edn, c->d_bpt, c->d_bin, c->d_gsed, c->d_ised,
-                c->d_proc, c->d_err, c->d_dtm, c->d_fx, c->d_fz};
+                c->d_proc, c->d_err, c->d_dtm, c->d_fx, c->d_fz, D->cx, D->cy, D->ux,
+                D->uy, D->n1x, D->n1y, D->t2x, D->t2y, D->ev, D->cvt, D->vx, D->vy,
+                D->vcol, D->eord, D->lsq_off, D->lsq_idx, D->lsq_B, D->u2, D->wtop,
+                D->wbot, D->uc, D->vc, D->gu0, D->gv0, D->gue, D->gve, c->d_nw, c->d_crc,
+                c->d_crf, c->d_cour, c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc,
+                c->d_clc, c->d_clf, c->d_lci, c->d_fferr};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->cs2col, c->col2cs, c->h_kb, c->h_ks, c->h_kold, c->h_knew, c->h_cols,
@@ -120,8 +132,22 @@ static int us_check_scope(geometry_t *g, window_t *wd, win_priv_t *wc) {
   if (wc->rkstage != 1)
     return emstr_set_error(E_UNSUP, -1, "Runge-Kutta with %d stages is not supported",
                            wc->rkstage);
-  if (wc->trasc != VANLEER && wc->trasc != ORDER1 && wc->trasc != QUICKEST)
+  if (wc->trasc != VANLEER && wc->trasc != ORDER1 && wc->trasc != QUICKEST &&
+      wc->trasc != FFSL)
     return emstr_set_error(E_UNSUP, -1, "advection scheme 0x%x is not supported", wc->trasc);
+  if (wc->trasc == FFSL) {
+    if (wc->ultimate)
+      return emstr_set_error(E_UNSUP, -1, "FFSL with ULTIMATE is not supported");
+    if (wc->fillf & (MONOTONIC | CLIP))
+      return emstr_set_error(E_UNSUP, -1, "FFSL with a MONOTONIC or CLIP fill is not supported");
+    if (wc->compatible & V6898)
+      return emstr_set_error(E_UNSUP, -1, "FFSL with V6898 compatibility is not supported");
+    if (!wc->lw || !g->c2v || !g->v2c || !g->nvc || !g->m2dv || !g->e2v || !g->gridx ||
+        !g->gridy || !g->u1x || !g->u1y || !g->costhu2 || !g->sinthu2 || !wd->u2 ||
+        !wd->wtop || !wd->wbot || !wd->u || !wd->v)
+      return emstr_set_error(E_ARG, -1, "FFSL needs vertex maps, edge geometry, u2, wtop, "
+                                        "wbot, u, v and the least-squares weights");
+  }
   for (n = 1; n <= g->v2_t; n++) {
     int cs = g->w2_t[n];
     if (wc->Ds[cs] != 1.0 || wc->Hn1[cs] != 1.0 || (wc->one && wc->one[cs] != 1.0))
@@ -560,6 +586,12 @@ static int cols_from_host(usctx *c) {
   emstr_dev *d = &c->D.d;
   int cc;
   for (int q = 0; q < c->ncol; q++) c->h_ks[q] = -1;
+  /* old / new surface of every column (vel_center_w walks all of them) */
+  for (cc = 1; cc <= c->ncol; cc++) {
+    int col = cc - 1, ns = g->nsur_t[cc], os = g->sur_t[cc];
+    c->h_knew[col] = (ns == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[ns];
+    c->h_kold[col] = (os == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[os];
+  }
   for (cc = 1; cc <= wc->vcs; cc++) {
     int s = wc->s1[cc];
     int col = c->cs2col[g->m2d[s]];
@@ -593,6 +625,8 @@ static int check_land_edges(usctx *c) {
   return 0;
 }
 
+static int us_ffsl_fields(usctx *c);
+
 static int us_upload_fields(usctx *c) {
   window_t *wd = c->wd;
   win_priv_t *wc = c->wc;
@@ -613,6 +647,8 @@ static int us_upload_fields(usctx *c) {
     if (up_edges(c, D->dzu, wd->dzu1) || up_edges(c, D->kh, wc->u2kh)) return E_DEVICE;
   if (up_cols(c, d->eta, wd->eta) || up_cols(c, d->oldeta, wc->oldeta)) return E_DEVICE;
   if ((r = cols_from_host(c))) return r;
+  D->ffsl = (wc->trasc == FFSL);
+  if (D->ffsl && (r = us_ffsl_fields(c))) return r;
   HCHK(hipStreamSynchronize(c->st));
   return 0;
 }
@@ -650,6 +686,347 @@ static int us_download_tracers(usctx *c) {
 }
 
 /*-------------------------------------------------------------------*/
+/* FFSL: surface geometry, vertices, least-squares stencils           */
+/*-------------------------------------------------------------------*/
+static int icmp(const void *a, const void *b) {
+  int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+/* Builds and checks the device model of the FFSL maps: every wet centre's
+ * vertices are those of its surface centre, the wet centres around each
+ * of them in its layer are the columns around the surface vertex that
+ * are wet there, its wet edges in ascending order of w3_e1 follow one
+ * per-column order, and every least-squares stencil reads wet centres
+ * or ghosts (which read their wet centre). */
+static int us_ffsl_geom(usctx *c) {
+  geometry_t *g = c->g;
+  win_priv_t *wc = c->wc;
+  emstr_usdev *D = &c->D;
+  emstr_dev *d = &D->d;
+  const int ncol = c->ncol, ne2 = c->ne2, nz = g->nz, npem = g->npem;
+  const int nvl = g->szvS - 1, nvcm = g->nvcm;
+  int ok = 1, n, j, col, e;
+  if (nvl < 1 || nvcm < 1) return emstr_set_error(E_ARG, -1, "FFSL: no vertices");
+  double *hcx = (double *)malloc((size_t)ncol * sizeof(double));
+  double *hcy = (double *)malloc((size_t)ncol * sizeof(double));
+  double *he[6];
+  for (n = 0; n < 6; n++) he[n] = (double *)malloc((size_t)ne2 * sizeof(double));
+  double *hev = (double *)malloc((size_t)ne2 * 4 * sizeof(double));
+  int *hcvt = (int *)calloc((size_t)npem * ncol, sizeof(int));
+  int *heord = (int *)calloc((size_t)npem * ncol, sizeof(int));
+  double *hvx = (double *)malloc((size_t)nvl * sizeof(double));
+  double *hvy = (double *)malloc((size_t)nvl * sizeof(double));
+  int *hvcol = (int *)malloc((size_t)nvl * nvcm * sizeof(int));
+  int *epos = (int *)calloc(g->sze, sizeof(int));
+  int *gbin = (int *)calloc(g->szc, sizeof(int));
+  for (n = 1; n <= g->nbpt; n++) gbin[g->bpt[n]] = g->bin[n];
+  for (n = 1; n <= g->a3_e1; n++) epos[g->w3_e1[n]] = n;
+  for (col = 0; col < ncol; col++) {
+    hcx[col] = g->cellx[c->col2cs[col]];
+    hcy[col] = g->celly[c->col2cs[col]];
+  }
+  for (e = 1; e <= ne2 && ok; e++) {
+    const int q = e - 1, v1 = g->e2v[e][0], v2 = g->e2v[e][1];
+    he[0][q] = g->u1x[e];
+    he[1][q] = g->u1y[e];
+    he[2][q] = g->costhu1[e];
+    he[3][q] = g->sinthu1[e];
+    he[4][q] = g->costhu2[e];
+    he[5][q] = g->sinthu2[e];
+    if (v1 < 1 || v1 >= g->szvS || v2 < 1 || v2 >= g->szvS) {
+      emstr_set_error(E_UNSUP, e, "surface edge %d: end vertices outside the surface", e);
+      ok = 0;
+      break;
+    }
+    hev[4 * q + 0] = g->gridx[v1];
+    hev[4 * q + 1] = g->gridy[v1];
+    hev[4 * q + 2] = g->gridx[v2];
+    hev[4 * q + 3] = g->gridy[v2];
+  }
+  for (n = 0; n < nvl && ok; n++) {
+    const int v = n + 1;
+    hvx[n] = g->gridx[v];
+    hvy[n] = g->gridy[v];
+    if (g->nvc[v] > nvcm) {
+      emstr_set_error(E_UNSUP, v, "vertex %d: nvc above nvcm", v);
+      ok = 0;
+    }
+    for (j = 0; j < nvcm; j++) {
+      int cj = (j < g->nvc[v]) ? g->v2c[v][j + 1] : 0;
+      int cl = (cj > 0 && cj < g->szcS) ? c->cs2col[cj] : -1;
+      if (cj && cl < 0) {
+        emstr_set_error(E_UNSUP, v, "vertex %d: centre %d is not a surface centre", v, cj);
+        ok = 0;
+        break;
+      }
+      hvcol[n * nvcm + j] = cl;
+    }
+    /* -1 padding at the end only */
+    int m = 0;
+    for (j = 0; j < nvcm; j++)
+      if (hvcol[n * nvcm + j] >= 0) hvcol[n * nvcm + m++] = hvcol[n * nvcm + j];
+    for (; m < nvcm; m++) hvcol[n * nvcm + m] = -1;
+  }
+  for (col = 0; col < ncol && ok; col++) {
+    const int cs = c->col2cs[col], np = g->npe[cs];
+    int sl[8];
+    for (j = 0; j < np; j++) {
+      const int v = g->c2v[j + 1][cs];
+      if (v < 1 || v >= g->szvS) {
+        emstr_set_error(E_UNSUP, cs, "centre %d: vertex %d outside the surface", cs, j + 1);
+        ok = 0;
+        break;
+      }
+      hcvt[j * ncol + col] = v - 1;
+      sl[j] = j;
+    }
+    /* edge slots by ascending surface edge */
+    for (int a = 1; a < np; a++)
+      for (int b = a; b > 0 && g->c2e[sl[b] + 1][cs] < g->c2e[sl[b - 1] + 1][cs]; b--) {
+        int t = sl[b];
+        sl[b] = sl[b - 1];
+        sl[b - 1] = t;
+      }
+    for (j = 0; j < np; j++) heord[j * ncol + col] = sl[j];
+  }
+  /* every wet centre against the model */
+  int *lsq_off = (int *)calloc((size_t)d->nwet + 1, sizeof(int));
+  size_t nent = 0;
+  for (n = 0; n < d->nwet && ok; n++) {
+    const int cw = g->w3_t[n + 1];
+    const qweights *lw = &wc->lw[cw];
+    if (lw->ncells < 1 || !lw->cells || !lw->B) {
+      emstr_set_error(E_ARG, cw, "centre %d: no least-squares stencil", cw);
+      ok = 0;
+      break;
+    }
+    lsq_off[n] = (int)nent;
+    nent += lw->ncells;
+    if (nent > 0x7fffffff) {
+      emstr_set_error(E_UNSUP, cw, "least-squares stencils too large");
+      ok = 0;
+    }
+  }
+  if (ok) lsq_off[d->nwet] = (int)nent;
+  ll *lidx = ok ? (ll *)malloc((nent ? nent : 1) * sizeof(ll)) : NULL;
+  double *lB = ok ? (double *)malloc((nent ? nent : 1) * 3 * sizeof(double)) : NULL;
+  for (n = 0; n < d->nwet && ok; n++) {
+    const int cw = g->w3_t[n + 1], cs = g->m2d[cw], k = g->s2k[cw];
+    col = c->cs2col[cs];
+    const int np = g->npe[cs];
+    for (j = 1; j <= np && ok; j++) {
+      /* vertex j: same surface vertex, wet centres around it = columns
+       * of the surface vertex wet at layer k */
+      const int v = g->c2v[j][cw], vs = hcvt[(j - 1) * ncol + col] + 1;
+      if (v < 1 || v >= g->szv || g->m2dv[v] != vs) {
+        emstr_set_error(E_UNSUP, cw, "centre %d: vertex %d is not its surface vertex", cw, j);
+        ok = 0;
+        break;
+      }
+      int a[16], b[16], na = 0, nb = 0;
+      for (int i = 1; i <= g->nvc[vs] && na < 16; i++)
+        if (g->v2c[v][i]) a[na++] = g->v2c[v][i];
+      for (int i = 0; i < nvcm && nb < 16; i++) {
+        int cv = hvcol[(vs - 1) * nvcm + i];
+        if (cv >= 0 && k >= c->h_kb[cv]) b[nb++] = c->sp_of[(ll)k * ncol + cv];
+      }
+      qsort(a, na, sizeof(int), icmp);
+      qsort(b, nb, sizeof(int), icmp);
+      if (na != nb || memcmp(a, b, na * sizeof(int))) {
+        emstr_set_error(E_UNSUP, cw, "centre %d: centres around vertex %d differ", cw, j);
+        ok = 0;
+        break;
+      }
+    }
+    /* wet edges in the order of w3_e1 */
+    int last = 0;
+    for (int sl = 0; sl < np && ok; sl++) {
+      const int jj = heord[sl * ncol + col] + 1;
+      const int e3 = g->c2e[jj][cw];
+      if (!epos[e3]) continue; /* not a wet edge */
+      if (epos[e3] < last) {
+        emstr_set_error(E_UNSUP, cw, "centre %d: edge order differs from w3_e1", cw);
+        ok = 0;
+      }
+      last = epos[e3];
+    }
+    /* stencil */
+    const qweights *lw = &wc->lw[cw];
+    for (int i = 0; i < lw->ncells && ok; i++) {
+      int cn = lw->cells[i];
+      if (cn > 0 && cn < g->szc && gbin[cn]) cn = gbin[cn];
+      int ccol = (cn > 0 && cn < g->szc) ? c->cs2col[g->m2d[cn]] : -1;
+      int ck = (cn > 0 && cn < g->szc) ? g->s2k[cn] : -1;
+      if (ccol < 0 || ck < c->h_kb[ccol] || ck >= nz || c->sp_of[(ll)ck * ncol + ccol] != cn) {
+        emstr_set_error(E_UNSUP, cw, "centre %d: stencil entry %d is not wet or a ghost", cw,
+                        lw->cells[i]);
+        ok = 0;
+        break;
+      }
+      const size_t o = (size_t)lsq_off[n] + i;
+      lidx[o] = (ll)ck * ncol + ccol;
+      lB[3 * o + 0] = lw->B[i][0];
+      lB[3 * o + 1] = lw->B[i][1];
+      lB[3 * o + 2] = lw->B[i][2];
+    }
+  }
+  if (ok) {
+    D->cx = (double *)dalloc((size_t)ncol * sizeof(double), &ok);
+    D->cy = (double *)dalloc((size_t)ncol * sizeof(double), &ok);
+    double **ep[] = {&D->ux, &D->uy, &D->n1x, &D->n1y, &D->t2x, &D->t2y};
+    for (n = 0; n < 6; n++) *ep[n] = (double *)dalloc((size_t)ne2 * sizeof(double), &ok);
+    D->ev = (double *)dalloc((size_t)ne2 * 4 * sizeof(double), &ok);
+    D->cvt = (int *)dalloc((size_t)npem * ncol * sizeof(int), &ok);
+    D->eord = (int *)dalloc((size_t)npem * ncol * sizeof(int), &ok);
+    D->vx = (double *)dalloc((size_t)nvl * sizeof(double), &ok);
+    D->vy = (double *)dalloc((size_t)nvl * sizeof(double), &ok);
+    D->vcol = (int *)dalloc((size_t)nvl * nvcm * sizeof(int), &ok);
+    D->lsq_off = (int *)dalloc(((size_t)d->nwet + 1) * sizeof(int), &ok);
+    D->lsq_idx = (ll *)dalloc((nent ? nent : 1) * sizeof(ll), &ok);
+    D->lsq_B = (double *)dalloc((nent ? nent : 1) * 3 * sizeof(double), &ok);
+    if (!ok) emstr_set_error(E_DEVICE, -1, "FFSL geometry allocation failed");
+  }
+  if (ok) {
+    D->nvl = nvl;
+    D->nvcm = nvcm;
+    double **ep[] = {&D->ux, &D->uy, &D->n1x, &D->n1y, &D->t2x, &D->t2y};
+    int r = up(c, D->cx, hcx, ncol * sizeof(double)) || up(c, D->cy, hcy, ncol * sizeof(double));
+    for (n = 0; n < 6; n++) r = r || up(c, *ep[n], he[n], ne2 * sizeof(double));
+    r = r || up(c, D->ev, hev, (size_t)ne2 * 4 * sizeof(double)) ||
+        up(c, D->cvt, hcvt, (size_t)npem * ncol * sizeof(int)) ||
+        up(c, D->eord, heord, (size_t)npem * ncol * sizeof(int)) ||
+        up(c, D->vx, hvx, nvl * sizeof(double)) || up(c, D->vy, hvy, nvl * sizeof(double)) ||
+        up(c, D->vcol, hvcol, (size_t)nvl * nvcm * sizeof(int)) ||
+        up(c, D->lsq_off, lsq_off, ((size_t)d->nwet + 1) * sizeof(int)) ||
+        up(c, D->lsq_idx, lidx, (nent ? nent : 1) * sizeof(ll)) ||
+        up(c, D->lsq_B, lB, (nent ? nent : 1) * 3 * sizeof(double));
+    if (hipStreamSynchronize(c->st) != hipSuccess) r = 1;
+    if (r) {
+      emstr_set_error(E_DEVICE, -1, "FFSL geometry upload failed");
+      ok = 0;
+    }
+  }
+  free(hcx);
+  free(hcy);
+  for (n = 0; n < 6; n++) free(he[n]);
+  free(hev);
+  free(hcvt);
+  free(heord);
+  free(hvx);
+  free(hvy);
+  free(hvcol);
+  free(epos);
+  free(gbin);
+  free(lsq_off);
+  free(lidx);
+  free(lB);
+  if (!ok) return E_UNSUP;
+  c->ffgeom = 1;
+  return 0;
+}
+
+/* the rotated edge velocity of every land edge's ghost (ffsl_init,
+ * transport.c:2652-2666) */
+static double ghost_u(const geometry_t *g, const window_t *wd, int e, int comp) {
+  const int es = g->m2de[e];
+  return comp ? wd->u1[e] * g->sinthu1[es] + wd->u2[e] * g->sinthu2[es]
+              : wd->u1[e] * g->costhu1[es] + wd->u2[e] * g->costhu2[es];
+}
+
+/* FFSL fields: u2 on wet edges, wtop / wbot, and, when sub-stepping, the
+ * centred velocities of wet centres and of the ghosts across land edges
+ * (as the window holds them, and rotated from the edge) */
+static int us_ffsl_fields(usctx *c) {
+  geometry_t *g = c->g;
+  window_t *wd = c->wd;
+  emstr_usdev *D = &c->D;
+  emstr_dev *d = &D->d;
+  const size_t N3 = (size_t)c->ncol * d->nz, NE3 = (size_t)c->ne2 * d->nz;
+  int ok = 1, r;
+  if (!c->ffgeom && (r = us_ffsl_geom(c))) return r;
+  if (!D->u2) {
+    D->u2 = (double *)dalloc(NE3 * sizeof(double), &ok);
+    D->wtop = (double *)dalloc((size_t)c->ncol * sizeof(double), &ok);
+    D->wbot = (double *)dalloc((size_t)c->ncol * sizeof(double), &ok);
+    if (!ok) return emstr_set_error(E_DEVICE, -1, "FFSL field allocation failed");
+  }
+  if (up_edges(c, D->u2, wd->u2) || up_cols(c, D->wtop, wd->wtop) ||
+      up_cols(c, D->wbot, wd->wbot))
+    return E_DEVICE;
+  c->ghost_edges = 0;
+  if (!(c->wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER))) return 0;
+  if (!D->uc) {
+    double **a[] = {&D->uc, &D->vc};
+    for (int n = 0; n < 2; n++) *a[n] = (double *)dalloc(N3 * sizeof(double), &ok);
+    double **b[] = {&D->gu0, &D->gv0, &D->gue, &D->gve};
+    for (int n = 0; n < 4; n++) *b[n] = (double *)dalloc(NE3 * sizeof(double), &ok);
+    if (!ok) return emstr_set_error(E_DEVICE, -1, "FFSL field allocation failed");
+  }
+  if (up_cells(c, D->uc, wd->u) || up_cells(c, D->vc, wd->v)) return E_DEVICE;
+  double *h = (double *)calloc(NE3 * 4, sizeof(double));
+  for (int ee = g->v3_e1 + 1; ee <= g->n3_e1; ee++) {
+    const int e = g->w3_e1[ee], cg = g->e2c[e][1];
+    const size_t i = (size_t)g->s2k[g->e2c[e][0]] * c->ne2 + (g->m2de[e] - 1);
+    h[i] = wd->u[cg];
+    h[NE3 + i] = wd->v[cg];
+    h[2 * NE3 + i] = ghost_u(g, wd, e, 0);
+    h[3 * NE3 + i] = ghost_u(g, wd, e, 1);
+  }
+  r = up(c, D->gu0, h, NE3 * sizeof(double)) || up(c, D->gv0, h + NE3, NE3 * sizeof(double)) ||
+      up(c, D->gue, h + 2 * NE3, NE3 * sizeof(double)) ||
+      up(c, D->gve, h + 3 * NE3, NE3 * sizeof(double));
+  hipStreamSynchronize(c->st);
+  free(h);
+  return r ? emstr_set_error(E_DEVICE, -1, "FFSL ghost velocities") : 0;
+}
+
+/* ffsl_init's toggle (transport.c:2624-2627, 2722): 1 if it runs on this
+ * call; it then writes the ghost velocities into the window's u, v */
+static int us_ffsl_init(usctx *c) {
+  geometry_t *g = c->g;
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  if (wc->flag & F_FFSL) {
+    wc->flag &= ~F_FFSL;
+    return 0;
+  }
+  for (int ee = g->v3_e1 + 1; ee <= g->n3_e1; ee++) {
+    const int e = g->w3_e1[ee], cg = g->e2c[e][1];
+    wd->u[cg] = ghost_u(g, wd, e, 0);
+    wd->v[cg] = ghost_u(g, wd, e, 1);
+  }
+  wc->flag |= F_FFSL;
+  c->ghost_edges = 1;
+  return 1;
+}
+
+static int us_ffsl_alloc(usctx *c) {
+  emstr_dev *d = &c->D.d;
+  const size_t N3 = (size_t)d->n3, NE3 = (size_t)c->ne2 * d->nz;
+  const size_t nt = (size_t)(d->ntr > 0 ? d->ntr : 1);
+  int ok = 1;
+  if (c->d_nw) return 0;
+  double **a[] = {&c->d_nw, &c->d_crc, &c->d_crf, &c->d_cour};
+  for (int n = 0; n < 4; n++) *a[n] = (double *)dalloc(N3 * sizeof(double), &ok);
+  c->d_clc = (int *)dalloc(N3 * sizeof(int), &ok);
+  c->d_clf = (int *)dalloc(N3 * sizeof(int), &ok);
+  c->d_lci = (int *)dalloc(NE3 * sizeof(int), &ok);
+  c->d_ld = (double *)dalloc(5 * NE3 * sizeof(double), &ok);
+  c->d_tm = (double *)dalloc(nt * N3 * sizeof(double), &ok);
+  c->d_lq = (double *)dalloc(3 * nt * N3 * sizeof(double), &ok);
+  c->d_te = (double *)dalloc(nt * NE3 * sizeof(double), &ok);
+  c->d_moc = (double *)dalloc(nt * N3 * sizeof(double), &ok);
+  c->d_fferr = (int *)dalloc(sizeof(int), &ok);
+  if (ok && hipMemset(c->d_fferr, 0, sizeof(int)) != hipSuccess) ok = 0;
+  if (!ok) {
+    emstr_set_error(E_DEVICE, -1, "FFSL work arrays (%zu tracers)", nt);
+    return E_DEVICE;
+  }
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
 /* the step                                                          */
 /*-------------------------------------------------------------------*/
 /* advect_diffuse (hd-us tracers.c:428-1157); 0, or the 2D centre of a
@@ -659,14 +1036,25 @@ static int us_adv(usctx *c) {
   emstr_dev *d = &D->d;
   win_priv_t *wc = c->wc;
   double dtu = d->dt, dtm = d->dt, trem;
-  int itermax = 20, first = 1;
+  int itermax = 20, first = 1, r0;
+  const int ffsl = D->ffsl;
   if (!c->have_cols) return emstr_set_error(E_ARG, -1, "no set_dz data on the device");
+  if (ffsl) {
+    /* tracers.c:567-590: centred w, ffsl_init */
+    itermax = 200;
+    if ((r0 = us_ffsl_alloc(c))) return r0;
+    const int init = us_ffsl_init(c);
+    if (emstr_us_launch_ffsl_step(D, init, c->d_nw, c->st))
+      return emstr_set_error(E_DEVICE, -1, "FFSL step launch");
+  }
   if (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)) {
     unsigned long long init, r;
     double big = dtu;
     memcpy(&init, &big, sizeof(init));
     HCHK(hipMemcpyAsync(c->d_dtm, &init, sizeof(init), hipMemcpyHostToDevice, c->st));
-    if (emstr_us_launch_dtm(D, dtu, c->d_dtm, c->st))
+    const double *gu = c->ghost_edges ? D->gue : D->gu0;
+    const double *gv = c->ghost_edges ? D->gve : D->gv0;
+    if (emstr_us_launch_dtm(D, dtu, c->d_dtm, wc->trsf, gu, gv, c->st))
       return emstr_set_error(E_DEVICE, -1, "dtm launch");
     HCHK(hipMemcpyAsync(&r, c->d_dtm, sizeof(r), hipMemcpyDeviceToHost, c->st));
     HCHK(hipStreamSynchronize(c->st));
@@ -703,9 +1091,20 @@ static int us_adv(usctx *c) {
     if (u > 0.0) {
       if (emstr_launch_subeta(d, u, trem, first, c->st))
         return emstr_set_error(E_DEVICE, -1, "subeta launch");
-      if (emstr_us_launch_adv(D, c->tr[in], c->tr[out], c->tr[b0], c->d_edn, c->nesp,
-                              c->d_fx, c->d_fz, u, c->anydif, d->ultimate, c->st) ||
-          emstr_launch_osub(d, c->st))
+      if (ffsl) {
+        if (emstr_us_launch_ffsl_sub(D, u, c->d_nw, c->d_clc, c->d_crc, c->d_clf, c->d_crf,
+                                     c->d_cour, c->d_edn, c->nesp, c->d_lci, c->d_ld,
+                                     c->d_fferr, c->st) ||
+            emstr_us_launch_ffsl_flux(D, u, c->tr[in], c->d_nw, c->d_clc, c->d_crc, c->d_clf,
+                                      c->d_crf, c->d_cour, c->d_edn, c->nesp, c->d_lci,
+                                      c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc, c->d_fx,
+                                      c->d_fz, c->anydif, c->d_fferr, c->st) ||
+            emstr_us_launch_update(D, c->tr[in], c->tr[out], c->d_fx, c->d_fz, u, c->st) ||
+            emstr_launch_osub(d, c->st))
+          return emstr_set_error(E_DEVICE, -1, "FFSL launch");
+      } else if (emstr_us_launch_adv(D, c->tr[in], c->tr[out], c->tr[b0], c->d_edn, c->nesp,
+                                     c->d_fx, c->d_fz, u, c->anydif, d->ultimate, c->st) ||
+                 emstr_launch_osub(d, c->st))
         return emstr_set_error(E_DEVICE, -1, "advection launch");
       in = out;
       out = (out + 1) % 3;
@@ -729,6 +1128,18 @@ static int us_vdiff(usctx *c) {
 static int us_sync(usctx *c) {
   int err = 0;
   HCHK(hipStreamSynchronize(c->st));
+  if (c->d_fferr) {
+    HCHK(hipMemcpy(&err, c->d_fferr, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+      hipMemset(c->d_fferr, 0, sizeof(int));
+      if (err == 1)
+        return emstr_set_error(E_UNSUP, -1, "FFSL: a streamline leaves its upstream cell into "
+                                            "a wet cell (multi-segment streamlines need "
+                                            "hd_trans_interp, not supported)");
+      return emstr_set_error(E_UNSUP, -1, "FFSL: a vertical trajectory does not end "
+                                          "(zero layer thickness)");
+    }
+  }
   HCHK(hipMemcpy(&err, c->d_err, sizeof(int), hipMemcpyDeviceToHost));
   if (err) {
     int cell = c->sp_of[err - 1];

@@ -19,9 +19,33 @@ extern "C" {
 int emstr_us_launch_adv(const emstr_usdev *D, const double *tin, double *tout,
                         const double *tstart, const long long *edn, int nesp, double *fx,
                         double *fz, double dtu, int anydif, int ult, void *stream);
-/* minimum sub-step; *out must hold the bits of dtu0 before the launch */
-int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned long long *out,
-                        void *stream);
+/* minimum sub-step; *out must hold the bits of dtu0 before the launch;
+ * sf, gu, gv (ghost velocities [nz][ne2]) are read under FFSL only     */
+int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned long long *out, double sf,
+                        const double *gu, const double *gv, void *stream);
+/* the cell update, surf_concrk and tr_bounds from fluxes fx, fz */
+int emstr_us_launch_update(const emstr_usdev *D, const double *tin, double *tout,
+                           const double *fx, const double *fz, double dtu, void *stream);
+
+/* FFSL (emstr_us_ffsl.hip).  Per step: the centred w into nw [nz][ncol]
+ * (init: vel_center_w, else the Leonard et al. choice).  Per sub-step:
+ * the vertical departure layers / fractions of centres (clc, crc) and of
+ * the faces of the cells to process (clf, crf), their Courant numbers,
+ * and the streamline of every wet edge (lci [nz][ne2], ld [5][nz][ne2]).
+ * Per sub-step and tracer: tm, lq [3], te, moc, then fx and fz.  err
+ * gets 1 for a streamline that leaves its cell into a wet cell, 2 for a
+ * vertical walk that does not end.                                     */
+int emstr_us_launch_ffsl_step(const emstr_usdev *D, int init, double *nw, void *stream);
+int emstr_us_launch_ffsl_sub(const emstr_usdev *D, double dtu, const double *nw, int *clc,
+                             double *crc, int *clf, double *crf, double *cour,
+                             const long long *edn, int nesp, int *lci, double *ld, int *err,
+                             void *stream);
+int emstr_us_launch_ffsl_flux(const emstr_usdev *D, double dtu, const double *tin,
+                              const double *nw, const int *clc, const double *crc,
+                              const int *clf, const double *crf, const double *cour,
+                              const long long *edn, int nesp, const int *lci, const double *ld,
+                              double *tm, double *lq, double *te, double *moc, double *fx,
+                              double *fz, int anydif, int *err, void *stream);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,28 @@ typedef struct emstr_usdev {
   double *qd3, *qhm, *qhp;
   /* [nz][ne2] edge fields: u1, u1flux3d, dzu1, u2kh                  */
   double *u1, *flux, *dzu, *kh;
+
+  /* FFSL (transport.c:1961-2723): geometry of the surface layer       */
+  int ffsl;         /* the window's scheme is FFSL                      */
+  int nvl, nvcm;    /* surface vertices; most columns around one        */
+  double *cx, *cy;  /* [ncol] centre positions                          */
+  double *ux, *uy;  /* [ne2] edge midpoints (streamline starts)         */
+  double *n1x, *n1y, *t2x, *t2y; /* [ne2] costhu1/sinthu1, costhu2/sinthu2 */
+  double *ev;       /* [ne2][4] end vertices x1, y1, x2, y2 (e2v)       */
+  int *cvt;         /* [npem][ncol] vertex j of the column (0..nvl-1)   */
+  double *vx, *vy;  /* [nvl] vertex positions                           */
+  int *vcol;        /* [nvl][nvcm] columns around the vertex, -1 padded */
+  int *eord;        /* [npem][ncol] edge slots in ascending edge order  */
+  /* least-squares stencils of the wet centres, by wet-list position:
+   * entries lsq_off[q]..lsq_off[q+1]-1 read the dense cell lsq_idx (a
+   * ghost reads its wet centre) with weights lsq_B[3*entry + 1..2]    */
+  int *lsq_off;
+  long long *lsq_idx;
+  double *lsq_B;
+  /* fields: [nz][ne2] u2; [ncol] wtop, wbot; [nz][ncol] u, v of the wet
+   * centres; [nz][ne2] u, v of the ghost across a land edge (gu0/gv0
+   * from the window's arrays, gue/gve rotated from the edge velocity) */
+  double *u2, *wtop, *wbot, *uc, *vc, *gu0, *gv0, *gue, *gve;
 } emstr_usdev;
 
 #ifdef __cplusplus

@@ -20,8 +20,12 @@ def say(*a):
 
 
 def run_both(case, nsteps, which="step"):
+    """The GPU run, then the oracle from the same state: tracers, and for
+    FFSL the ffsl_init toggle and the centred velocities it writes."""
     L = MS.lib()
     snap = H.snapshot(case)
+    flag0 = case.wc.flag
+    uv0 = case.ac(case.wd.u).copy(), case.ac(case.wd.v).copy()
     for _ in range(nsteps):
         if which == "step":
             L.tracer_step_3d(case.gp, case.wp, case.pp)
@@ -33,7 +37,10 @@ def run_both(case, nsteps, which="step"):
         assert msg == "", (msg, cell)
     gpu = case.tracers()
     b1_gpu = case.wc.b1
+    flag_gpu, uv_gpu = case.wc.flag, (case.ac(case.wd.u).copy(), case.ac(case.wd.v).copy())
     H.restore(case, snap)
+    case.wc.flag = flag0
+    case.ac(case.wd.u)[:], case.ac(case.wd.v)[:] = uv0
     for _ in range(nsteps):
         if which == "step":
             assert OU.step(case) == 0
@@ -43,6 +50,9 @@ def run_both(case, nsteps, which="step"):
             assert OU.vert_diffuse(case) == 0
     ora = case.tracers()
     assert b1_gpu == case.wc.b1
+    assert flag_gpu == case.wc.flag
+    assert np.array_equal(uv_gpu[0], case.ac(case.wd.u))
+    assert np.array_equal(uv_gpu[1], case.ac(case.wd.v))
     L.emstr_us_forget_window(case.gp)
     return gpu, ora
 
@@ -155,6 +165,71 @@ def test_set_dz_lists_bit_exact():
                                   U.view(getattr(b.g, f), b.szcS, np.int32)[1:n2 + 1]), f
         assert np.array_equal(a.ac(a.wc.dz)[a.wet], b.ac(b.wc.dz)[b.wet])
     L.emstr_us_forget_window(a.gp)
+
+
+FFSL_CASES = [
+    # name, hex_basin kw, stab, trsf, nsteps
+    ("ffsl", dict(), U.NONE, 0.8, 4),
+    ("ffsl_substeps", dict(dt=120.0), U.SUB_STEP, 0.002, 3),
+    ("ffsl_fast", dict(dt=300.0, vmax=1.2), U.NONE, 0.8, 3),
+]
+
+
+@pytest.mark.parametrize("name,kw,stab,trsf,nsteps", FFSL_CASES)
+def test_hex_ffsl_step(name, kw, stab, trsf, nsteps):
+    """FFSL streamlines, least squares and transverse terms; ffsl_init
+    runs on every other call, so several steps cover both branches.  The
+    streamline directions come from atan2/sin/cos, whose last bit may
+    differ between the host's and the device's libraries: tracers agree
+    to 1e-12 relative."""
+    say("case hex", name)
+    case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=3, seed=17, trasc=U.FFSL, **kw)
+    case.wc.stab = stab
+    case.wc.trsf = trsf
+    case.prepare(OU.set_dz)
+    t0 = case.tracers()
+    flag0 = case.wc.flag
+    gpu, ora = run_both(case, nsteps)
+    assert np.abs(ora - t0).max() > 1e-3
+    if stab == U.SUB_STEP:
+        assert case.wc.b1 < case.dt
+    assert case.wc.flag == (flag0 ^ (U.F_FFSL if nsteps % 2 else 0))
+    assert_exact(gpu, ora)
+
+
+def test_hex_ffsl_moving_surface_diffusion():
+    say("case hex ffsl moving surface + diffusion")
+    case = MS.hex_basin(nx=20, ny=16, nz=6, ntr=3, seed=19, dmin=2.0, dmax=30.0,
+                        trasc=U.FFSL)
+    w2, old, eta, rng, dl = _moving(case, 5)
+    OU.set_dz(case.gp, case.wp, case.pp)
+    eta[w2] = old[w2] + rng.uniform(-0.9, 0.9, len(w2)) * dl
+    U.view(case.wc.diffuse, 3, np.int32)[:] = [1, 0, 1]
+    case.ae(case.wc.u2kh)[:] = rng.uniform(0.0, 30.0, case.sze)
+    case.prepare(OU.set_dz)
+    gpu, ora = run_both(case, 2)
+    assert_exact(gpu, ora)
+
+
+def test_quad_mesh_ffsl_step():
+    say("case quads ffsl")
+    sc = configs.small(seed=23, ntr=2, dt=120.0, trasc=S.VANLEER)
+    sc.prepare(O.set_dz)
+    qc = MS.quads_from_case(sc)
+    qc.set_tangential()
+    qc.scheme(U.FFSL, 0, 120.0)
+    OU.set_dz(qc.gp, qc.wp, qc.pp)
+    gpu, ora = run_both(qc, 3)
+    assert_exact(gpu, ora)
+
+
+def test_ffsl_multi_segment_is_reported():
+    say("case hex ffsl multi-segment")
+    case = MS.hex_basin(nx=12, ny=10, nz=3, ntr=1, seed=2, trasc=U.FFSL, dt=12000.0)
+    case.prepare(OU.set_dz)
+    MS.lib().tracer_step_3d(case.gp, case.wp, case.pp)
+    assert "multi-segment" in MS.last_error()[0]
+    MS.lib().emstr_us_forget_window(case.gp)
 
 
 def test_unsupported_settings_are_reported():
