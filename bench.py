@@ -19,6 +19,8 @@ build runs on one GPU.
   --workload c4-compas              the COMPAS path on the 350x350 hexagon
                                     mesh, 50 layers, 60 tracers, van Leer
                                     (plain FP64 form)
+  --workload c4-compas-ffsl         the same mesh with COMPAS FFSL (plain
+                                    FP64 form)
   --copy-probe MB                   time a plain device copy of MB MiB and
                                     exit (calibrates the HBM byte counters)
 """
@@ -137,7 +139,8 @@ def main():
     ap.add_argument("--forms", default="plain,precision,batched")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0)
-    ap.add_argument("--workload", default="c3", choices=["c3", "c3-ffsl", "c4-compas"])
+    ap.add_argument("--workload", default="c3",
+                    choices=["c3", "c3-ffsl", "c4-compas", "c4-compas-ffsl"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the tracer step on one GPU only; "
@@ -148,7 +151,7 @@ def main():
         return
     if args.workload == "c3-ffsl":
         return bench_ffsl(args)
-    if args.workload == "c4-compas":
+    if args.workload in ("c4-compas", "c4-compas-ffsl"):
         return bench_compas(args)
     forms = set(args.forms.split(","))
 
@@ -353,10 +356,12 @@ def bench_ffsl(args):
 
 
 def bench_compas(args):
-    """The COMPAS path (plain FP64 form) on the C4 hexagon mesh."""
-    from ems_amd import mesh as MS
+    """The COMPAS path (plain FP64 form) on the C4 hexagon mesh, van Leer
+    or FFSL (streamlines, least squares, transverse terms)."""
+    from ems_amd import mesh as MS, structs_us as U
     t0 = time.time()
-    case = MS.c4(seed=args.seed)
+    ffsl = args.workload == "c4-compas-ffsl"
+    case = MS.c4(seed=args.seed, trasc=U.FFSL if ffsl else U.VANLEER)
     print("bench: c4 built in %.1f s: %d wet cells x %d tracers, %d wet edges"
           % (time.time() - t0, case.nwet, case.ntr, case.g.v3_e1), file=sys.stderr, flush=True)
     LU = MS.lib()
@@ -386,8 +391,8 @@ def bench_compas(args):
            "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (seeded C4 hexagon generator, seed %d)" % args.seed,
            "config": {"workload": "c4 mesh: 350x350 jittered hexagons, 1.5 km edges, 50 "
-                                  "layers, 3-400 m, %d tracers, dt=180 s, van Leer "
-                                  "(COMPAS FFSL is not built)" % ntr,
+                                  "layers, 3-400 m, %d tracers, dt=180 s, %s"
+                                  % (ntr, "FFSL (FP64 plain)" if ffsl else "van Leer"),
                       "n3_wet": nwet, "wet_edges": ne, "ntr": ntr},
            # per step: every tracer read and written once (plain FP64 in and
            # out), w, Kz, dz per cell and u1, u1flux3d per edge

@@ -41,7 +41,10 @@ typedef long long ll;
 
 #define FF_TF 0.55     /* transport.c:2014 */
 #define FF_SMALL 1e-10 /* transport.c:24 */
-#define FF_CH 8        /* tracers per pass in k_fflsq */
+#ifndef FF_CH
+#define FF_CH 4 /* tracers per pass in k_fflsq (fewer registers, more waves) */
+#endif
+#define FF_MAXA 4      /* most centres around a vertex (nvcm) */
 
 __device__ __forceinline__ double fmx(double a, double b) { return (a > b) ? a : b; }
 __device__ __forceinline__ double fmn(double a, double b) { return (a < b) ? a : b; }
@@ -310,7 +313,7 @@ k_fflsq(emstr_usdev D, const double *__restrict__ tm, double *__restrict__ lq) {
   const int o0 = D.lsq_off[q], o1 = D.lsq_off[q + 1];
   const double cxc = D.cx[col], cyc = D.cy[col];
   for (int n0 = 0; n0 < d.ntr; n0 += FF_CH) {
-    double a1[FF_CH], a2[FF_CH];
+    double a1[FF_CH], a2[FF_CH], w0[FF_CH], beta[FF_CH];
     bool on[FF_CH];
 #pragma unroll
     for (int c = 0; c < FF_CH; c++) {
@@ -318,10 +321,12 @@ k_fflsq(emstr_usdev D, const double *__restrict__ tm, double *__restrict__ lq) {
       on[c] = (n < d.ntr) && d.in_tbdy[n] && d.advect[n];
       a1[c] = 0.0;
       a2[c] = 0.0;
+      w0[c] = on[c] ? tm[n * N3 + idx] : 0.0;
+      beta[c] = 1.0;
     }
     for (int i = o0; i < o1; i++) {
       const ll is = D.lsq_idx[i];
-      const double b1 = D.lsq_B[3 * (ll)i + 1], b2 = D.lsq_B[3 * (ll)i + 2];
+      const double b1 = D.lsq_B[2 * (ll)i], b2 = D.lsq_B[2 * (ll)i + 1];
 #pragma unroll
       for (int c = 0; c < FF_CH; c++)
         if (on[c]) {
@@ -330,33 +335,41 @@ k_fflsq(emstr_usdev D, const double *__restrict__ tm, double *__restrict__ lq) {
           a2[c] += (t * b2);
         }
     }
+    /* vertices in order; each tracer's beta sees them in that order */
+    for (int j = 0; j < npe; j++) {
+      const int vt = D.cvt[j * ncol + col];
+      const double x = cxc - D.vx[vt], y = cyc - D.vy[vt];
+      int vcl[FF_MAXA], nv = 0;
+#pragma unroll
+      for (int i = 0; i < FF_MAXA; i++) {
+        const int cv = (i < D.nvcm) ? D.vcol[vt * D.nvcm + i] : -1;
+        if (cv >= 0 && ffwet(d.kb, cv, k)) vcl[nv++] = cv;
+      }
+#pragma unroll
+      for (int c = 0; c < FF_CH; c++) {
+        if (!on[c]) continue;
+        const double *t = tm + (n0 + c) * N3 + (ll)k * P;
+        const double val = w0[c] + 1.0 * (a1[c] * x + a2[c] * y);
+        const double dd = val - w0[c];
+        double vmin = __builtin_huge_val(), vmax = -__builtin_huge_val();
+#pragma unroll
+        for (int i = 0; i < FF_MAXA; i++) {
+          if (i >= nv) break;
+          const double tv = t[vcl[i]];
+          vmin = fmn(vmin, tv);
+          vmax = fmx(vmax, tv);
+        }
+        if (dd != 0.0 && val > vmax) beta[c] = fmx(fmn((vmax - w0[c]) / dd, beta[c]), 0.0);
+        if (dd != 0.0 && val < vmin) beta[c] = fmx(fmn((vmin - w0[c]) / dd, beta[c]), 0.0);
+      }
+    }
 #pragma unroll
     for (int c = 0; c < FF_CH; c++) {
       if (!on[c]) continue;
       const int n = n0 + c;
-      const double *t = tm + n * N3;
-      const double w0 = t[idx], w1 = a1[c], w2 = a2[c];
-      double beta = 1.0;
-      for (int j = 0; j < npe; j++) {
-        const int vt = D.cvt[j * ncol + col];
-        const double x = cxc - D.vx[vt], y = cyc - D.vy[vt];
-        const double val = w0 + 1.0 * (w1 * x + w2 * y);
-        const double dd = val - w0;
-        double vmin = __builtin_huge_val(), vmax = -__builtin_huge_val();
-        for (int i = 0; i < D.nvcm; i++) {
-          const int cv = D.vcol[vt * D.nvcm + i];
-          if (cv < 0) break;
-          if (!ffwet(d.kb, cv, k)) continue;
-          const double tv = t[(ll)k * P + cv];
-          vmin = fmn(vmin, tv);
-          vmax = fmx(vmax, tv);
-        }
-        if (dd != 0.0 && val > vmax) beta = fmx(fmn((vmax - w0) / dd, beta), 0.0);
-        if (dd != 0.0 && val < vmin) beta = fmx(fmn((vmin - w0) / dd, beta), 0.0);
-      }
-      lq[0 * NT3 + n * N3 + idx] = w1;
-      lq[1 * NT3 + n * N3 + idx] = w2;
-      lq[2 * NT3 + n * N3 + idx] = fmn(fmx(beta, 0.0), 1.0);
+      lq[0 * NT3 + n * N3 + idx] = a1[c];
+      lq[1 * NT3 + n * N3 + idx] = a2[c];
+      lq[2 * NT3 + n * N3 + idx] = fmn(fmx(beta[c], 0.0), 1.0);
     }
   }
 }

@@ -708,6 +708,7 @@ static int us_ffsl_geom(usctx *c) {
   const int nvl = g->szvS - 1, nvcm = g->nvcm;
   int ok = 1, n, j, col, e;
   if (nvl < 1 || nvcm < 1) return emstr_set_error(E_ARG, -1, "FFSL: no vertices");
+  if (nvcm > 4) return emstr_set_error(E_UNSUP, -1, "FFSL: more than 4 centres at a vertex");
   double *hcx = (double *)malloc((size_t)ncol * sizeof(double));
   double *hcy = (double *)malloc((size_t)ncol * sizeof(double));
   double *he[6];
@@ -810,7 +811,7 @@ static int us_ffsl_geom(usctx *c) {
   }
   if (ok) lsq_off[d->nwet] = (int)nent;
   ll *lidx = ok ? (ll *)malloc((nent ? nent : 1) * sizeof(ll)) : NULL;
-  double *lB = ok ? (double *)malloc((nent ? nent : 1) * 3 * sizeof(double)) : NULL;
+  double *lB = ok ? (double *)malloc((nent ? nent : 1) * 2 * sizeof(double)) : NULL;
   for (n = 0; n < d->nwet && ok; n++) {
     const int cw = g->w3_t[n + 1], cs = g->m2d[cw], k = g->s2k[cw];
     col = c->cs2col[cs];
@@ -866,9 +867,8 @@ static int us_ffsl_geom(usctx *c) {
       }
       const size_t o = (size_t)lsq_off[n] + i;
       lidx[o] = (ll)ck * ncol + ccol;
-      lB[3 * o + 0] = lw->B[i][0];
-      lB[3 * o + 1] = lw->B[i][1];
-      lB[3 * o + 2] = lw->B[i][2];
+      lB[2 * o + 0] = lw->B[i][1];
+      lB[2 * o + 1] = lw->B[i][2];
     }
   }
   if (ok) {
@@ -884,7 +884,7 @@ static int us_ffsl_geom(usctx *c) {
     D->vcol = (int *)dalloc((size_t)nvl * nvcm * sizeof(int), &ok);
     D->lsq_off = (int *)dalloc(((size_t)d->nwet + 1) * sizeof(int), &ok);
     D->lsq_idx = (ll *)dalloc((nent ? nent : 1) * sizeof(ll), &ok);
-    D->lsq_B = (double *)dalloc((nent ? nent : 1) * 3 * sizeof(double), &ok);
+    D->lsq_B = (double *)dalloc((nent ? nent : 1) * 2 * sizeof(double), &ok);
     if (!ok) emstr_set_error(E_DEVICE, -1, "FFSL geometry allocation failed");
   }
   if (ok) {
@@ -900,7 +900,7 @@ static int us_ffsl_geom(usctx *c) {
         up(c, D->vcol, hvcol, (size_t)nvl * nvcm * sizeof(int)) ||
         up(c, D->lsq_off, lsq_off, ((size_t)d->nwet + 1) * sizeof(int)) ||
         up(c, D->lsq_idx, lidx, (nent ? nent : 1) * sizeof(ll)) ||
-        up(c, D->lsq_B, lB, (nent ? nent : 1) * 3 * sizeof(double));
+        up(c, D->lsq_B, lB, (nent ? nent : 1) * 2 * sizeof(double));
     if (hipStreamSynchronize(c->st) != hipSuccess) r = 1;
     if (r) {
       emstr_set_error(E_DEVICE, -1, "FFSL geometry upload failed");

@@ -55,7 +55,8 @@ typedef struct emstr_usdev {
   int *eord;        /* [npem][ncol] edge slots in ascending edge order  */
   /* least-squares stencils of the wet centres, by wet-list position:
    * entries lsq_off[q]..lsq_off[q+1]-1 read the dense cell lsq_idx (a
-   * ghost reads its wet centre) with weights lsq_B[3*entry + 1..2]    */
+   * ghost reads its wet centre) with the slope weights lsq_B[2*entry],
+   * lsq_B[2*entry + 1] (the constant term is not used)               */
   int *lsq_off;
   long long *lsq_idx;
   double *lsq_B;
