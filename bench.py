@@ -21,6 +21,8 @@ build runs on one GPU.
                                     (plain FP64 form)
   --workload c4-compas-ffsl         the same mesh with COMPAS FFSL (plain
                                     FP64 form)
+  --workload c5-ensemble            the 64-window set in the batched FP32
+                                    form, one ensemble hipGraph
   --copy-probe MB                   time a plain device copy of MB MiB and
                                     exit (calibrates the HBM byte counters)
 """
@@ -140,7 +142,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0)
     ap.add_argument("--workload", default="c3",
-                    choices=["c3", "c3-ffsl", "c4-compas", "c4-compas-ffsl"])
+                    choices=["c3", "c3-ffsl", "c4-compas", "c4-compas-ffsl", "c5-ensemble"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the tracer step on one GPU only; "
@@ -153,6 +155,8 @@ def main():
         return bench_ffsl(args)
     if args.workload in ("c4-compas", "c4-compas-ffsl"):
         return bench_compas(args)
+    if args.workload == "c5-ensemble":
+        return bench_ensemble(args)
     forms = set(args.forms.split(","))
 
     from ems_amd import _lib, configs
@@ -404,6 +408,75 @@ def bench_compas(args):
         OU.set_dz(case.gp, case.wp, case.pp)
         out["cpu_baseline"] = _cpu_baseline(lambda: OU.step(case, nth), cu, args.cpu_seconds,
                                             nth, "c4 COMPAS")
+    print(json.dumps(out))
+
+
+def bench_ensemble(args):
+    """The 64-window set c5 (van Leer, the batched form's scheme) in the
+    batched FP32 form: one ensemble hipGraph with a branch per window,
+    against the same windows stepped one context after another."""
+    from ems_amd import _lib, configs, structs as S
+    from ems_amd import tracer as T
+    L = _lib.lib()
+    t0 = time.time()
+    doms = configs.c5(ndom=64, trasc=S.VANLEER)
+    ctxs = []
+    for c in doms:
+        L.set_dz(c.gp, c.wp, c.pp)
+        if _lib.last_error()[0]:
+            raise RuntimeError("set_dz: %s" % (_lib.last_error(),))
+        L.emstr_forget_window(c.gp)
+        p = T.Context(c, T.PLAIN_F64)
+        p.upload_fields()
+        p.vel_w()
+        p.close()
+        b = T.Context(c, T.BATCH_F32)
+        b.upload()
+        ctxs.append(b)
+    cu = float(sum(c.nwet * c.ntr for c in doms))
+    print("bench: c5 built in %.1f s: %d windows, %.3g cell-tracers"
+          % (time.time() - t0, len(doms), cu), file=sys.stderr, flush=True)
+    K = max(2, min(args.steps, 100) // 2 * 2)
+    # one window after another (each replays its own two-step graph)
+    for b in ctxs:
+        b.step(2)
+        b.sync()
+    t = time.time()
+    for b in ctxs:
+        b.step(K)
+        b.sync()
+    sep_ms = (time.time() - t) * 1e3 / K
+    # the ensemble graph
+    ens = T.Ensemble(ctxs)
+    ens.step(2)
+    ens.sync()
+    ev = HipEvents()
+    a, e = ev.event(), ev.event()
+    ev.record(a, ens.stream)
+    ens.step(K)
+    ev.record(e, ens.stream)
+    ms = ev.elapsed_ms(a, e) / K
+    ens.sync()
+    ens.close()
+    for b in ctxs:
+        b.close()
+    out = {"metric": "tracer cell-updates/s on 1 GPU (64-window ensemble, FP32 batched)",
+           "value": cu / (ms * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
+           "warmup": 2, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded c5 generator, seed 5, van Leer)",
+           "config": {"workload": "c5: 64 windows, nx, ny in [20, 150], 5-40 layers, 5-20 "
+                                  "tracers, one ensemble hipGraph per two steps",
+                      "windows": len(doms), "cell_tracers": cu},
+           "one_after_another_ms_per_step": sep_ms,
+           "ensemble_speedup": sep_ms / ms}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        nth = min(16, os.cpu_count() or 1)
+        for c in doms:
+            O.set_dz(c.gp, c.wp, c.pp)
+        out["cpu_baseline"] = _cpu_baseline(lambda: [O.step(c, nth) for c in doms], cu,
+                                            args.cpu_seconds, nth, "c5 (64 windows)")
     print(json.dumps(out))
 
 

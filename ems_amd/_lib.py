@@ -43,6 +43,11 @@ PROTOS = [
     ("emstr_kernel_times", C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     ("emstr_sync", C.c_int, [_CTX]),
     ("emstr_stream", C.c_void_p, [_CTX]),
+    ("emstr_ensemble_create", C.c_void_p, [C.POINTER(_CTX), C.c_int]),
+    ("emstr_ensemble_steps", C.c_int, [C.c_void_p, C.c_int]),
+    ("emstr_ensemble_sync", C.c_int, [C.c_void_p]),
+    ("emstr_ensemble_stream", C.c_void_p, [C.c_void_p]),
+    ("emstr_ensemble_destroy", None, [C.c_void_p]),
     ("emstr_forget_window", None, [_G]),
     ("emstr_last_error", C.c_char_p, []),
     ("emstr_last_error_cell", C.c_int, []),

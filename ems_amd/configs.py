@@ -421,11 +421,12 @@ def c2(seed=2):
     return case
 
 
-def c5(seed=5, ndom=64):
+def c5(seed=5, ndom=64, trasc=None):
     """The multi-domain set: ndom windows with nx, ny ~ U[20, 150],
     5-40 layers and 5-20 tracers each, built as c2; every fourth window is
     stiff (0.5 m surface layer, Kz up to 1 m2/s, dt = 600 s, diffusion
-    numbers up to ~1e4)."""
+    numbers up to ~1e4).  trasc, if given, is every window's scheme (the
+    batched form runs van Leer)."""
     rng = np.random.default_rng(seed)
     out = []
     for d in range(ndom):
@@ -434,7 +435,8 @@ def c5(seed=5, ndom=64):
         ntr = int(rng.integers(5, 21))
         stiff = (d % 4 == 3)
         case = curvilinear("c5_%d" % d, nxi, nyi, nz, 5.0, 200.0, 0.30, 1.0, ntr,
-                           S.QUICKEST if not stiff else S.VANLEER,
+                           trasc if trasc is not None else
+                           (S.QUICKEST if not stiff else S.VANLEER),
                            600.0 if stiff else 120.0, int(rng.integers(1 << 30)),
                            kz_max=1.0 if stiff else 1e-1,
                            top_layer=0.5 if stiff else None)

@@ -1177,6 +1177,21 @@ extern "C" int emstr_batch_steps(struct emstr_batch *b, const emstr_dev *d, int 
   return 0;
 }
 
+/* two steps recorded onto a stream that is being captured (an ensemble
+ * graph branch); the buffers end where they started */
+extern "C" int emstr_batch_record2(struct emstr_batch *b, const emstr_dev *d, int stab,
+                                   void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (stab & 22) return emstr_set_error(EMSTR_EUNSUPPORTED, -1, "batched form: no sub-stepping");
+  if (b->prof && b->prof->on)
+    return emstr_set_error(EMSTR_EARG, -1, "ensemble capture with kernel timing on");
+  const int cur0 = b->cur;
+  if (batch_launch(b, d, st) || batch_launch(b, d, st))
+    return emstr_set_error(EMSTR_EDEVICE, -1, "ensemble capture");
+  b->cur = cur0;
+  return 0;
+}
+
 extern "C" int emstr_batch_put(struct emstr_batch *b, const emstr_dev *d, int n,
                                const double *sparse, void *stream) {
   if (d->nwet == 0) return 0;

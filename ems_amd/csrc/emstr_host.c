@@ -764,6 +764,121 @@ int emstr_steps(emstr_ctx *c, int nsteps) {
   return 0;
 }
 
+/*-------------------------------------------------------------------*/
+/* Ensembles: one hipGraph with a branch per batched context          */
+/*-------------------------------------------------------------------*/
+struct emstr_ensemble {
+  int n;
+  emstr_ctx **c;
+  hipStream_t s0;
+  hipStream_t *br;
+  hipEvent_t fork, *join;
+  hipGraph_t graph;
+  hipGraphExec_t gexec;
+};
+
+void emstr_ensemble_destroy(emstr_ensemble *e) {
+  if (!e) return;
+  if (e->s0) hipStreamSynchronize(e->s0);
+  if (e->gexec) hipGraphExecDestroy(e->gexec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  for (int i = 0; i < e->n; i++) {
+    if (e->br && e->br[i]) hipStreamDestroy(e->br[i]);
+    if (e->join && e->join[i]) hipEventDestroy(e->join[i]);
+  }
+  if (e->fork) hipEventDestroy(e->fork);
+  if (e->s0) hipStreamDestroy(e->s0);
+  free(e->br);
+  free(e->join);
+  free(e->c);
+  free(e);
+}
+
+emstr_ensemble *emstr_ensemble_create(emstr_ctx **ctxs, int n) {
+  int i;
+  emstr_clear_error();
+  if (!ctxs || n < 1) {
+    emstr_set_error(EMSTR_EARG, -1, "ensemble: no contexts");
+    return NULL;
+  }
+  for (i = 0; i < n; i++) {
+    emstr_ctx *c = ctxs[i];
+    if (!c || !c->batch || !c->have_cols) {
+      emstr_set_error(EMSTR_EARG, i, "ensemble: context %d is not a batched context with "
+                                     "uploaded fields", i);
+      return NULL;
+    }
+    if (hipStreamSynchronize(c->st) != hipSuccess) {
+      emstr_set_error(EMSTR_EDEVICE, i, "ensemble: context %d stream", i);
+      return NULL;
+    }
+  }
+  emstr_ensemble *e = (emstr_ensemble *)calloc(1, sizeof(emstr_ensemble));
+  e->n = n;
+  e->c = (emstr_ctx **)malloc((size_t)n * sizeof(emstr_ctx *));
+  memcpy(e->c, ctxs, (size_t)n * sizeof(emstr_ctx *));
+  e->br = (hipStream_t *)calloc(n, sizeof(hipStream_t));
+  e->join = (hipEvent_t *)calloc(n, sizeof(hipEvent_t));
+  int ok = hipStreamCreateWithFlags(&e->s0, hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
+  for (i = 0; i < n && ok; i++)
+    ok = hipStreamCreateWithFlags(&e->br[i], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    emstr_set_error(EMSTR_EDEVICE, -1, "ensemble: streams");
+    emstr_ensemble_destroy(e);
+    return NULL;
+  }
+  /* fork: every branch waits on the capture stream, records its
+   * window's two steps, and the capture stream joins them all */
+  if (hipStreamBeginCapture(e->s0, hipStreamCaptureModeThreadLocal) != hipSuccess ||
+      hipEventRecord(e->fork, e->s0) != hipSuccess) {
+    emstr_set_error(EMSTR_EDEVICE, -1, "ensemble: capture");
+    emstr_ensemble_destroy(e);
+    return NULL;
+  }
+  int rec = 0;
+  for (i = 0; i < n && !rec; i++) {
+    emstr_ctx *c = e->c[i];
+    if (hipStreamWaitEvent(e->br[i], e->fork, 0) != hipSuccess) rec = EMSTR_EDEVICE;
+    if (!rec) rec = emstr_batch_record2(c->batch, &c->d, c->wc->stab, e->br[i]);
+    if (!rec && (hipEventRecord(e->join[i], e->br[i]) != hipSuccess ||
+                 hipStreamWaitEvent(e->s0, e->join[i], 0) != hipSuccess))
+      rec = EMSTR_EDEVICE;
+  }
+  hipGraph_t g = NULL;
+  hipError_t ec = hipStreamEndCapture(e->s0, &g);
+  e->graph = g;
+  if (rec || ec != hipSuccess || !g ||
+      hipGraphInstantiate(&e->gexec, e->graph, NULL, NULL, 0) != hipSuccess) {
+    if (!rec) emstr_set_error(EMSTR_EDEVICE, -1, "ensemble: graph instantiate");
+    emstr_ensemble_destroy(e);
+    return NULL;
+  }
+  return e;
+}
+
+int emstr_ensemble_steps(emstr_ensemble *e, int nsteps) {
+  emstr_clear_error();
+  if (!e) return emstr_set_error(EMSTR_EARG, -1, "ensemble: null");
+  if (nsteps < 0 || (nsteps & 1))
+    return emstr_set_error(EMSTR_EARG, -1, "ensemble: nsteps must be even");
+  /* work the contexts queued on their own streams comes first */
+  for (int i = 0; i < e->n; i++) HCHK(hipStreamSynchronize(e->c[i]->st));
+  for (int k = 0; k < nsteps; k += 2)
+    if (hipGraphLaunch(e->gexec, e->s0) != hipSuccess)
+      return emstr_set_error(EMSTR_EDEVICE, -1, "ensemble: graph launch");
+  return 0;
+}
+
+int emstr_ensemble_sync(emstr_ensemble *e) {
+  if (!e) return EMSTR_EARG;
+  HCHK(hipStreamSynchronize(e->s0));
+  return 0;
+}
+
+void *emstr_ensemble_stream(emstr_ensemble *e) { return e ? (void *)e->s0 : NULL; }
+
 int emstr_tune(emstr_ctx *c, int tile_x, int tile_y, int grid, int use_graph) {
   if (!c) return EMSTR_EARG;
   if (use_graph >= 0) c->use_graph = use_graph;

@@ -93,3 +93,34 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class Ensemble:
+    """Batched-form contexts stepped together: one hipGraph with a branch
+    per context (emstr_ensemble_create); steps must be even."""
+
+    def __init__(self, contexts):
+        L = _lib.lib()
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.ptr for c in self.contexts])
+        self.ptr = L.emstr_ensemble_create(arr, len(self.contexts))
+        if not self.ptr:
+            raise RuntimeError("emstr_ensemble_create: %s (cell %d)" % _lib.last_error())
+        self.stream = L.emstr_ensemble_stream(self.ptr)
+
+    def step(self, n=2):
+        _chk(_lib.lib().emstr_ensemble_steps(self.ptr, n), "emstr_ensemble_steps")
+
+    def sync(self):
+        _chk(_lib.lib().emstr_ensemble_sync(self.ptr), "emstr_ensemble_sync")
+
+    def close(self):
+        if self.ptr:
+            _lib.lib().emstr_ensemble_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -240,6 +240,21 @@ int emstr_sync(emstr_ctx *ctx);
 /* Stream the context runs on (hipStream_t as void*).                  */
 void *emstr_stream(emstr_ctx *ctx);
 
+/* Ensemble of batched-form contexts (EMSTR_BATCH_F32 windows of any
+ * sizes, e.g. independent domains): every context's step is captured as
+ * one branch of a single hipGraph, so one launch advances all of them
+ * and small windows run side by side on the GPU.  No counterpart in the
+ * reference, which steps windows one after another (tracers.c:142-209).
+ * The contexts must have their fields and tracers uploaded; while the
+ * ensemble exists they are stepped only through it.  nsteps must be
+ * even (the graph holds two steps).                                    */
+typedef struct emstr_ensemble emstr_ensemble;
+emstr_ensemble *emstr_ensemble_create(emstr_ctx **ctxs, int n);
+int emstr_ensemble_steps(emstr_ensemble *ens, int nsteps);
+int emstr_ensemble_sync(emstr_ensemble *ens);
+void *emstr_ensemble_stream(emstr_ensemble *ens); /* hipStream_t */
+void emstr_ensemble_destroy(emstr_ensemble *ens);
+
 /* Release the device mirror the drop-in entry points keep for a window
  * (call before freeing or reusing the window's arrays).               */
 void emstr_forget_window(geometry_t *window);

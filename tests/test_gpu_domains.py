@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from ems_amd import _lib, configs
+from ems_amd import structs as S
 from ems_amd import tracer as T
 from oracle import oracle as O
 from tests import helpers as H
@@ -100,3 +101,59 @@ def test_c2_quickest_bit_exact():
     ref = case.tracers()
     L.emstr_forget_window(case.gp)
     assert np.array_equal(gpu, ref), np.abs(gpu - ref).max()
+
+
+def _batched_pair(c):
+    """Two batched contexts of one window from the same host state."""
+    L = _lib.lib()
+    L.set_dz(c.gp, c.wp, c.pp)
+    L.emstr_forget_window(c.gp)
+    p = T.Context(c, T.PLAIN_F64)
+    p.upload_fields()
+    p.vel_w()
+    p.close()
+    a = T.Context(c, T.BATCH_F32)
+    a.upload()
+    b = T.Context(c, T.BATCH_F32)
+    b.upload()
+    return a, b
+
+
+def test_c5_ensemble_graph_equals_separate_steps():
+    """All 64 windows (van Leer, the batched form's scheme) in one
+    ensemble graph, against each window stepped on its own: the same
+    kernels on the same data, so the results agree bit for bit."""
+    say("c5: 64 windows, one ensemble graph vs separate steps")
+    doms = configs.c5(ndom=64, trasc=S.VANLEER)
+    pairs = [_batched_pair(c) for c in doms]
+    for a, _ in pairs:
+        a.step(4)
+    ens = T.Ensemble([b for _, b in pairs])
+    ens.step(4)
+    ens.sync()
+    ens.close()
+    for c, (a, b) in zip(doms, pairs):
+        a.download()
+        ta = c.tracers()
+        b.download()
+        tb = c.tracers()
+        a.close()
+        b.close()
+        assert np.array_equal(ta, tb), (c.name, np.abs(ta - tb).max())
+
+
+def test_ensemble_rejects_odd_steps_and_plain_contexts():
+    say("ensemble argument checks")
+    doms = configs.c5(ndom=2, trasc=S.VANLEER)
+    a, b = _batched_pair(doms[0])
+    ens = T.Ensemble([a, b])
+    with pytest.raises(RuntimeError, match="even"):
+        ens.step(3)
+    ens.close()
+    a.close()
+    b.close()
+    p = T.Context(doms[1], T.PLAIN_F64)
+    p.upload_fields()
+    with pytest.raises(RuntimeError, match="batched"):
+        T.Ensemble([p])
+    p.close()
