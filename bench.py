@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--ntr", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the batched form's default launch shape")
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--forms", default="plain,precision,batched")
     ap.add_argument("--profile-steps", type=int, default=20)
@@ -180,6 +182,7 @@ def main():
     names = ["advect", "vdiff", "prep", "other", "fused"]
     res = {}
     ctx = {}
+    tuned = None
     for form, kind in (("plain", T.PLAIN_F64), ("precision", T.PREC_F32),
                        ("batched", T.BATCH_F32)):
         if form not in forms:
@@ -189,6 +192,20 @@ def main():
             c.upload_fields()
             c.vel_w()
         c.upload()
+        if form == "batched" and not args.no_tune:
+            # launch shape by measurement, on a throwaway context so that
+            # the measured one starts from the same state as the others
+            from ems_amd import autotune
+            tc = T.Context(case, kind)
+            tc.upload()
+            best, trials = autotune.tune_context(tc)
+            tc.close()
+            c.tune(best[0], best[1], best[2])
+            tuned = {"tile": [best[0], best[1]], "grid": best[2],
+                               "ms_per_step": best[3],
+                               "trials": [[t[0], t[1], t[2], round(t[3], 4)] for t in trials]}
+            print("bench: autotune -> tile %dx%d grid %d (%.4f ms/step)" % best,
+                  file=sys.stderr, flush=True)
         m0 = c.mass()
         c.step(W)
         ms = timed_steps(c, ev, K)
@@ -229,6 +246,8 @@ def main():
                            "traffic": measured_traffic(kname),
                            "alg_bytes_per_launch": alg, "avg_launch_ms": avg_ms}
         out["step_hbm_fraction"] = (nwet * ntr * 8 + nwet * 56) / (rB["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        if tuned:
+            out["autotune"] = tuned
     kms = {}
     for form, r in res.items():
         kms[form] = {names[i]: r["kt"][i] / PK for i in range(5) if r["kc"][i]}
@@ -446,6 +465,13 @@ def bench_ensemble(args):
         b.step(K)
         b.sync()
     sep_ms = (time.time() - t) * 1e3 / K
+    # per-window grids for the ensemble, by measurement
+    tuned = None
+    if not args.no_tune:
+        from ems_amd import autotune
+        sc, tms, trials = autotune.tune_ensemble(ctxs)
+        tuned = {"grid_scale": sc, "ms_per_step": tms,
+                 "trials": [[t[0], round(t[1], 4)] for t in trials]}
     # the ensemble graph
     ens = T.Ensemble(ctxs)
     ens.step(2)
@@ -470,6 +496,8 @@ def bench_ensemble(args):
                       "windows": len(doms), "cell_tracers": cu},
            "one_after_another_ms_per_step": sep_ms,
            "ensemble_speedup": sep_ms / ms}
+    if tuned:
+        out["autotune"] = tuned
     if not args.no_cpu:
         from oracle import oracle as O
         nth = min(16, os.cpu_count() or 1)

@@ -34,6 +34,7 @@ PROTOS = [
     ("emstr_step", C.c_int, [_CTX]),
     ("emstr_steps", C.c_int, [_CTX, C.c_int]),
     ("emstr_tune", C.c_int, [_CTX, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("emstr_launch_shape", C.c_int, [_CTX] + [C.POINTER(C.c_int)] * 4),
     ("emstr_advect_diffuse", C.c_int, [_CTX]),
     ("emstr_vert_diffuse", C.c_int, [_CTX]),
     ("emstr_tracer_mass", C.c_int, [_CTX, C.POINTER(C.c_double)]),

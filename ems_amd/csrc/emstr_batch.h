@@ -21,6 +21,8 @@ int emstr_batch_fields(struct emstr_batch *b, const emstr_dev *d, void *stream);
 /* nsteps tracer steps; with use_graph pairs of steps replay a hipGraph */
 int emstr_batch_steps(struct emstr_batch *b, const emstr_dev *d, int stab, int nsteps,
                       void *stream, int use_graph);
+void emstr_batch_shape(const struct emstr_batch *b, int *tile_x, int *tile_y, int *grid,
+                       int *grid_full);
 int emstr_batch_record2(struct emstr_batch *b, const emstr_dev *d, int stab, void *stream);
 int emstr_batch_put(struct emstr_batch *b, const emstr_dev *d, int n, const double *sparse,
                     void *stream);

@@ -89,6 +89,7 @@ struct emstr_batch {
   int niter;  /* list length per slot (a multiple of 64)               */
   int gridA;  /* kernel A workgroups (4 wave slots each)               */
   ll nlist;   /* gridA * 4 * niter                                     */
+  int grid_full, gridA_full; /* the grids that fill the GPU          */
 };
 
 __device__ __forceinline__ ll cidx(ll col, int nz, int k) { return col * nz + k; }
@@ -1040,6 +1041,8 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
       per_cu < 1)
     per_cu = 1;
   b->gridA = (ncu * per_cu + 7) / 8 * 8;
+  b->grid_full = b->grid;
+  b->gridA_full = b->gridA;
   build_order(b, h_kb, d->nz);
   if (build_cells(b, h_kb, d->nz)) {
     emstr_batch_destroy(b);
@@ -1065,13 +1068,26 @@ extern "C" int emstr_batch_tune(struct emstr_batch *b, const int *h_kb, int tile
                                 int grid) {
   if (tile_x > 0) b->tile_x = tile_x;
   if (tile_y > 0) b->tile_y = tile_y;
-  if (grid > 0) b->gridA = (grid + 7) / 8 * 8;
+  if (grid > 0) {
+    /* the advection grid as asked; the column grid in proportion */
+    b->gridA = (grid + 7) / 8 * 8;
+    long long g2 = (long long)b->grid_full * b->gridA / b->gridA_full;
+    b->grid = (int)(g2 < 1 ? 1 : (g2 > b->grid_full ? b->grid_full : g2));
+  }
   build_order(b, h_kb, b->nz);
   if (build_cells(b, h_kb, b->nz)) return -1;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec), b->gexec = NULL;
   if (b->graph) (void)hipGraphDestroy(b->graph), b->graph = NULL;
   b->graph_ok = 0;
   return 0;
+}
+
+extern "C" void emstr_batch_shape(const struct emstr_batch *b, int *tile_x, int *tile_y,
+                                  int *grid, int *grid_full) {
+  *tile_x = b->tile_x;
+  *tile_y = b->tile_y;
+  *grid = b->gridA;
+  *grid_full = b->gridA_full;
 }
 
 extern "C" int emstr_batch_fields(struct emstr_batch *b, const emstr_dev *d, void *stream) {

@@ -879,6 +879,12 @@ int emstr_ensemble_sync(emstr_ensemble *e) {
 
 void *emstr_ensemble_stream(emstr_ensemble *e) { return e ? (void *)e->s0 : NULL; }
 
+int emstr_launch_shape(emstr_ctx *c, int *tile_x, int *tile_y, int *grid, int *grid_full) {
+  if (!c || !c->batch || !tile_x || !tile_y || !grid || !grid_full) return EMSTR_EARG;
+  emstr_batch_shape(c->batch, tile_x, tile_y, grid, grid_full);
+  return 0;
+}
+
 int emstr_tune(emstr_ctx *c, int tile_x, int tile_y, int grid, int use_graph) {
   if (!c) return EMSTR_EARG;
   if (use_graph >= 0) c->use_graph = use_graph;

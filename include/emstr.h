@@ -215,9 +215,13 @@ int emstr_step(emstr_ctx *ctx);
 int emstr_steps(emstr_ctx *ctx, int nsteps);
 /* Batched form launch shape: tile (tile_x x tile_y columns) of the cell
  * and column orders, workgroups of the persistent advection grid (a
- * multiple of 8), graph replay on/off; 0 (or -1 for use_graph) keeps a
- * value. */
+ * multiple of 8; the column-solve grid follows in proportion), graph
+ * replay on/off; 0 (or -1 for use_graph) keeps a value.  Results do not
+ * depend on the shape.  ems_amd/autotune.py picks it by measurement. */
 int emstr_tune(emstr_ctx *ctx, int tile_x, int tile_y, int grid, int use_graph);
+/* The batched form's current shape and the advection grid that fills
+ * the GPU (EMSTR_EARG for the other forms). */
+int emstr_launch_shape(emstr_ctx *ctx, int *tile_x, int *tile_y, int *grid, int *grid_full);
 /* Only the advection / only the vertical diffusion part.              */
 int emstr_advect_diffuse(emstr_ctx *ctx);
 int emstr_vert_diffuse(emstr_ctx *ctx);
