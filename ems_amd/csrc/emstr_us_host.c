@@ -70,6 +70,8 @@ typedef struct emstr_us_ctx {
   unsigned long long *d_dtm;
   double *d_fx;  /* edge fluxes [ntr][nz][ne2], made on first use */
   double *d_fz;  /* vertical fluxes [ntr][nz][ncol], made on first use */
+  int *h_esp;    /* wet edges: sparse index (host copy of d_esp)         */
+  ll *h_edn;     /* wet edges: dense index (host copy of d_edn)          */
   /* FFSL work arrays, made on first use (see emstr_us_kernels.h)      */
   int ffgeom;        /* surface geometry, vertices and stencils uploaded */
   int ghost_edges;   /* ghost velocities are the rotated edge velocity   */
@@ -116,7 +118,7 @@ static void us_destroy(usctx *c) {
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->cs2col, c->col2cs, c->h_kb, c->h_ks, c->h_kold, c->h_knew, c->h_cols,
-                c->sedg, c->sp_of};
+                c->sedg, c->sp_of, c->h_esp, c->h_edn};
   for (n = 0; n < (int)(sizeof(hv) / sizeof(hv[0])); n++) free(hv[n]);
   if (c->st) hipStreamDestroy(c->st);
   free(c);
@@ -451,8 +453,8 @@ static usctx *us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
   free(h_qd3);
   free(h_qhm);
   free(h_qhp);
-  free(esp);
-  free(edn);
+  c->h_esp = esp;
+  c->h_edn = edn;
   if (!ok) {
     us_destroy(c);
     return NULL;
@@ -1266,6 +1268,58 @@ void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon) {
   if (us_vdiff(c)) return;
   if (us_download_tracers(c)) return;
   us_sync(c);
+}
+
+/*-------------------------------------------------------------------*/
+/* FFSL maps of the last sub-step, in the window's numbering          */
+/*-------------------------------------------------------------------*/
+static int us_ffsl_maps(usctx *c, int *cl, int *clzf, int *clzc) {
+  geometry_t *g = c->g;
+  emstr_dev *d = &c->D.d;
+  const int ncol = c->ncol, ne2 = c->ne2, nz = g->nz;
+  const size_t N3 = (size_t)ncol * nz, NE3 = (size_t)ne2 * nz;
+  int q;
+  if (!c->d_lci) return emstr_set_error(E_ARG, -1, "no FFSL step has run on this window");
+  int *hl = (int *)malloc(NE3 * sizeof(int));
+  int *hf = (int *)malloc(N3 * sizeof(int));
+  int *hc = (int *)malloc(N3 * sizeof(int));
+  int r = hipMemcpy(hl, c->d_lci, NE3 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(hf, c->d_clf, N3 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(hc, c->d_clc, N3 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess;
+  if (!r) {
+#define SPL(col, L) ((L) < c->h_kb[col] ? c->sedg[col] : c->sp_of[(ll)(L) * ncol + (col)])
+    /* upstream centre of every wet edge */
+    for (q = 0; q < c->nesp; q++) {
+      const ll ie = c->h_edn[q];
+      const int k = (int)(ie / ne2);
+      cl[c->h_esp[q]] = c->sp_of[(ll)k * ncol + hl[ie]];
+    }
+    /* departure centres of the faces to process and of every wet centre */
+    for (q = 0; q < d->nwet; q++) {
+      const int cw = g->w3_t[q + 1];
+      const int col = c->cs2col[g->m2d[cw]], k = g->s2k[cw];
+      const size_t i = (size_t)k * ncol + col;
+      clzc[cw] = SPL(col, hc[i]);
+      if (c->h_ks[col] >= 0 && k <= c->h_ks[col]) clzf[cw] = SPL(col, hf[i]);
+    }
+#undef SPL
+  }
+  free(hl);
+  free(hf);
+  free(hc);
+  return r ? emstr_set_error(E_DEVICE, -1, "FFSL map download") : 0;
+}
+
+int emstr_us_ffsl_maps(geometry_t *window, int *cl, int *clzf, int *clzc) {
+  usctx *c = NULL;
+  int i;
+  emstr_clear_error();
+  pthread_mutex_lock(&reg_lock);
+  for (i = 0; i < MAXCTX; i++)
+    if (reg_g[i] == window) c = reg_c[i];
+  pthread_mutex_unlock(&reg_lock);
+  if (!c) return emstr_set_error(E_ARG, -1, "no device mirror for this window");
+  return us_ffsl_maps(c, cl, clzf, clzc);
 }
 
 /*-------------------------------------------------------------------*/

@@ -68,6 +68,8 @@ def lib():
             L.emstr_us_last_error_cell.restype = C.c_int
             L.emstr_us_forget_window.argtypes = [G]
             L.emstr_us_forget_window.restype = None
+            L.emstr_us_ffsl_maps.argtypes = [G] + [C.POINTER(C.c_int)] * 3
+            L.emstr_us_ffsl_maps.restype = C.c_int
         _lib = L
     return _lib
 
@@ -684,3 +686,15 @@ def c4(seed=5, ntr=60, trasc=U.VANLEER):
     with max |u1| = 1.2 m/s, Kz decaying with depth, 60 tracers, dt = 180 s."""
     return hex_basin(name="c4", nx=350, ny=350, nz=50, ntr=ntr, edge=1500.0, dmin=3.0,
                      dmax=400.0, land_frac=0.2, vmax=1.2, dt=180.0, trasc=trasc, seed=seed)
+
+
+def ffsl_maps(case):
+    """The FFSL maps of the window's last sub-step on the device:
+    (cl [sze], clzf [szc], clzc [szc]) in the window's numbering."""
+    cl = np.zeros(case.sze, dtype=np.int32)
+    cf = np.zeros(case.szc, dtype=np.int32)
+    cc = np.zeros(case.szc, dtype=np.int32)
+    r = lib().emstr_us_ffsl_maps(case.gp, _iptr(cl), _iptr(cf), _iptr(cc))
+    if r:
+        raise RuntimeError("emstr_us_ffsl_maps: %s (cell %d)" % last_error())
+    return cl, cf, cc

@@ -225,6 +225,15 @@ int emstr_us_steps(emstr_us_ctx *ctx, int nsteps);
 int emstr_us_sync(emstr_us_ctx *ctx);
 void *emstr_us_stream(emstr_us_ctx *ctx); /* hipStream_t of the context */
 
+/* The FFSL maps of the window's last sub-step on the device, in the
+ * window's numbering (prep_ff_sl and ffsl_don's upstream cells): cl[e]
+ * the centre a wet edge's streamline starts in (sze entries), clzf[c]
+ * the departure centre of the lower face of every cell to process and
+ * clzc[c] of every wet centre (szc entries; the sediment ghost may
+ * appear).  No counterpart in the reference, which keeps them in
+ * scratch; for checking the maps. */
+int emstr_us_ffsl_maps(geometry_t *window, int *cl, int *clzf, int *clzc);
+
 /* Errors of the last call on this thread ("" when none) */
 const char *emstr_us_last_error(void);
 int emstr_us_last_error_cell(void);

@@ -888,6 +888,19 @@ int oracle_us_ffsl_posc(geometry_t *g, int e, int ci, double u, double v, double
   return r;
 }
 
+/* test access: the FFSL maps of the last sub-step (one thread): the
+ * upstream centre of every wet edge (cl, by edge), clzf of the cells to
+ * process and clzc of the wet centres */
+void oracle_us_ffsl_maps(geometry_t *g, win_priv_t *wc, int *cl, int *clzf, int *clzc) {
+  int ee, cc;
+  for (ee = 1; ee <= g->a3_e1; ee++) {
+    int v = uscr ? uscr[0].cl[ee] : 0;
+    cl[g->w3_e1[ee]] = v < 0 ? -v : v;
+  }
+  for (cc = 1; cc <= g->a3_t; cc++) clzc[g->w3_t[cc]] = ff_clzc[g->w3_t[cc]];
+  for (cc = 1; cc <= wc->vc; cc++) clzf[wc->s1[cc]] = ff_clzf[wc->s1[cc]];
+}
+
 /* test access: get_linear_limit of tr into w [3*szc] and beta [szc] */
 void oracle_us_ffsl_limit(geometry_t *g, win_priv_t *wc, const double *tr, double *w,
                           double *beta) {

@@ -50,6 +50,9 @@ def lib():
         L.oracle_us_ffsl_posc.restype = C.c_int
         L.oracle_us_ffsl_limit.argtypes = [G, P, _PD, _PD, _PD]
         L.oracle_us_ffsl_limit.restype = None
+        _PI = C.POINTER(C.c_int)
+        L.oracle_us_ffsl_maps.argtypes = [G, P, _PI, _PI, _PI]
+        L.oracle_us_ffsl_maps.restype = None
         _lib = L
     return _lib
 
@@ -96,3 +99,15 @@ def ffsl_limit(case, tr):
     lib().oracle_us_ffsl_limit(case.gp, case.pp, t.ctypes.data_as(P), w.ctypes.data_as(P),
                                b.ctypes.data_as(P))
     return w.reshape(case.szc, 3), b
+
+
+def ffsl_maps(case):
+    """The FFSL maps of the oracle's last sub-step (single thread):
+    (cl [sze], clzf [szc], clzc [szc]), 0 where not set."""
+    cl = np.zeros(case.sze, dtype=np.int32)
+    cf = np.zeros(case.szc, dtype=np.int32)
+    cc = np.zeros(case.szc, dtype=np.int32)
+    P = C.POINTER(C.c_int)
+    lib().oracle_us_ffsl_maps(case.gp, case.pp, cl.ctypes.data_as(P), cf.ctypes.data_as(P),
+                              cc.ctypes.data_as(P))
+    return cl, cf, cc

@@ -223,6 +223,53 @@ def test_quad_mesh_ffsl_step():
     assert_exact(gpu, ora)
 
 
+def _maps_gpu_vs_oracle(case):
+    """One step on the GPU and in the oracle from the same state; the
+    FFSL maps of the step (upstream centre of every wet edge, departure
+    centres of faces and centres) must be identical."""
+    L = MS.lib()
+    snap = H.snapshot(case)
+    flag0 = case.wc.flag
+    uv0 = case.ac(case.wd.u).copy(), case.ac(case.wd.v).copy()
+    L.tracer_step_3d(case.gp, case.wp, case.pp)
+    assert MS.last_error()[0] == "", MS.last_error()
+    gcl, gcf, gcc = MS.ffsl_maps(case)
+    L.emstr_us_forget_window(case.gp)
+    H.restore(case, snap)
+    case.wc.flag = flag0
+    case.ac(case.wd.u)[:], case.ac(case.wd.v)[:] = uv0
+    assert OU.step(case) == 0
+    ocl, ocf, occ = OU.ffsl_maps(case)
+    g = case.g
+    wet_e = U.view(g.w3_e1, case.sze, np.int32)[1:g.a3_e1 + 1]
+    s1 = U.view(case.wc.s1, case.szc, np.int32)[1:case.wc.vc + 1]
+    assert np.all(gcl[wet_e] > 0)
+    assert np.array_equal(gcl[wet_e], ocl[wet_e])
+    assert np.array_equal(gcf[s1], ocf[s1])
+    assert np.array_equal(gcc[case.wet], occ[case.wet])
+    # the maps are not trivial: some streamlines start in e2c[e][1]
+    # (positive u1) and some departure centres are not the cell itself
+    assert np.any(gcl[wet_e] == case.e2c[wet_e, 1]) and np.any(gcl[wet_e] == case.e2c[wet_e, 0])
+    assert np.any(gcf[s1] != s1)
+
+
+def test_ffsl_maps_bit_exact_hex():
+    say("case hex ffsl maps")
+    case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=1, seed=29, trasc=U.FFSL, dt=300.0)
+    case.prepare(OU.set_dz)
+    _maps_gpu_vs_oracle(case)
+
+
+def test_ffsl_maps_bit_exact_c4():
+    """The C4 mesh (350 x 350 hexagons, 50 layers), one tracer: the maps
+    do not depend on the tracer."""
+    say("case c4 ffsl maps (mesh build ~30 s)")
+    case = MS.c4(seed=3, ntr=1, trasc=U.FFSL)
+    case.prepare(OU.set_dz)
+    say("c4 built:", case.nwet, "wet cells")
+    _maps_gpu_vs_oracle(case)
+
+
 def test_ffsl_multi_segment_is_reported():
     say("case hex ffsl multi-segment")
     case = MS.hex_basin(nx=12, ny=10, nz=3, ntr=1, seed=2, trasc=U.FFSL, dt=12000.0)
