@@ -12,7 +12,7 @@
  *              (tracers.c:773-925); the processed surface cell also does
  *              surf_conc (tracers.c:3047-3207) and tr_bounds
  *              (tracers.c:3217-3324) for its column.
- *   k_vdiff    64 columns x 4 tracer lanes per block: the column's
+ *   k_vdiff    64 columns x 16 tracer lanes per block: the column's
  *              tridiagonal matrix is built and factored once
  *              (tracers.c:3836-3889, 4185-4195) into LDS and reused by
  *              every tracer's forward/back substitution
@@ -1106,7 +1106,9 @@ extern "C" int emstr_launch_ffsl_fluxes(const emstr_dev *d, const double *t,
 /* k_vdiff: implicit vertical diffusion, in place                    */
 /*-------------------------------------------------------------------*/
 #define VD_COLS 64
-#define VD_TLANES 4
+#ifndef VD_TLANES
+#define VD_TLANES 16 /* tracer lanes per block: 16 waves share one column matrix in LDS */
+#endif
 
 __global__ void __launch_bounds__(VD_COLS *VD_TLANES)
 k_vdiff(emstr_dev d, double *__restrict__ tr, double *__restrict__ scratch,
