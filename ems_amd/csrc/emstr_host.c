@@ -60,6 +60,7 @@ struct emstr_ctx {
   int *d_proc;
   int *d_err;
   int *d_fferr;   /* FFSL trajectory walk bound exceeded */
+  double *d_face; /* face fluxes [3][ntr][n3] of the face-once update */
   unsigned long long *d_dtm;
   double *d_part, *h_part;
   double *d_wbot;
@@ -448,7 +449,7 @@ void emstr_destroy(emstr_ctx *c) {
                 c->d_gsed, c->d_ised, c->d_proc, c->d_err, c->d_dtm, c->d_part,
                 c->d_wbot, c->tr[0], c->tr[1], c->tr[2], c->d_fferr, d->fnu, d->fnv,
                 d->fnw, d->fw9, d->fcl, d->fcrf, d->ftmx, d->ftmy, d->ftmz, d->ftm,
-                d->ffx, d->ffy, d->ffz};
+                d->ffx, d->ffy, d->ffz, c->d_face};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->h_kb, c->cc2col, c->col2cc, c->sedg, c->topc, c->h_ks,
@@ -658,6 +659,15 @@ static int adv_plain(emstr_ctx *c) {
     itermax = 200;
     if (ffsl_alloc(c)) return EMSTR_EDEVICE;
   }
+  /* face fluxes of the face-once update, made on first use */
+  const size_t fn3 = (size_t)(d->ntr > 0 ? d->ntr : 1) * (size_t)d->n3;
+  if (!ffsl && !c->d_face) {
+    if (hipMalloc((void **)&c->d_face, 3 * fn3 * sizeof(double)) != hipSuccess) {
+      c->d_face = NULL;
+      return emstr_set_error(EMSTR_EDEVICE, -1, "face flux buffers (%zu bytes)",
+                             3 * fn3 * sizeof(double));
+    }
+  }
   if (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)) {
     unsigned long long init;
     double big = dtu;
@@ -698,8 +708,8 @@ static int adv_plain(emstr_ctx *c) {
                                    n, n + 1, c->st);
         }
       } else {
-        bad = emstr_launch_adv(d, c->tr[in], c->tr[out], c->tr[b0], u, trem, c->anydif, 0,
-                               d->ntr, c->st);
+        bad = emstr_launch_adv_faces(d, c->tr[in], c->tr[out], c->tr[b0], u, c->anydif,
+                                     c->d_face, c->d_face + fn3, c->d_face + 2 * fn3, c->st);
       }
       prof_end(c->prof, pi, c->st);
       if (bad || emstr_launch_osub(d, c->st))

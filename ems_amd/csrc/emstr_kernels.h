@@ -10,10 +10,17 @@
 extern "C" {
 #endif
 
-/* plain FP64 form (emstr_plain.hip) */
+/* plain FP64 form (emstr_plain.hip): the FFSL update of tracers n0..n1-1
+ * from the trajectory fluxes */
 int emstr_launch_adv(const emstr_dev *d, const double *tin, double *tout,
                      const double *tstart, double dtu, double trem, int anydif,
                      int n0, int n1, void *stream);
+/* the same update face-once (schemes other than FFSL): the west, south
+ * and bottom face fluxes of every cell and tracer into fw, fs, fb
+ * ([ntr][nz][ny][nx] each), then the cell updates from them */
+int emstr_launch_adv_faces(const emstr_dev *d, const double *tin, double *tout,
+                           const double *tstart, double dtu, int anydif, double *fw,
+                           double *fs, double *fb, void *stream);
 /* FFSL: cell-centred velocities and trajectories of one sub-step, then
  * the fluxes of one tracer (t: its buffer, ts: its step-start buffer);
  * err is set when a trajectory crosses more cells than the walk bound */

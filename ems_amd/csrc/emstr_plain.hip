@@ -5,13 +5,17 @@
  * -ffp-contract=off (no FMA contraction).
  *
  * Kernels (dense layout described in emstr_dev.h):
- *   k_adv      one thread per wet cell, loop over tracers inside:
- *              face values and fluxes (van Leer / QUICKEST / QUICKEST_CO /
- *              ORDER1 / ORDER2 / ORDER4, optional ULTIMATE and horizontal
- *              diffusion), flux divergence and sub-surface update
- *              (tracers.c:773-925); the processed surface cell also does
- *              surf_conc (tracers.c:3047-3207) and tr_bounds
- *              (tracers.c:3217-3324) for its column.
+ *   k_adv      FFSL: one thread per wet cell, loop over tracers inside:
+ *              fluxes from the trajectory integrals, flux divergence and
+ *              sub-surface update (tracers.c:773-925); the processed
+ *              surface cell also does surf_conc (tracers.c:3047-3207) and
+ *              tr_bounds (tracers.c:3217-3324) for its column.
+ *   k_face / k_upd  the other schemes, face-once: k_face computes each
+ *              cell's west, south and bottom face values and fluxes per
+ *              tracer (van Leer / QUICKEST / QUICKEST_CO / ORDER1 /
+ *              ORDER2 / ORDER4 / ORDER2_UW, optional ULTIMATE and
+ *              horizontal diffusion) into buffers; k_upd does the
+ *              divergence, update, surf_conc and tr_bounds from them.
  *   k_vdiff    64 columns x 16 tracer lanes per block: the column's
  *              tridiagonal matrix is built and factored once
  *              (tracers.c:3836-3889, 4185-4195) into LDS and reused by
@@ -630,6 +634,193 @@ k_adv(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout,
     if (d.mintr[n] <= 1e-5 && trs < d.mintr[n]) trs = d.mintr[n];
     /* the surface value fills every layer above (tracers.c:3306-3322) */
     for (int kk = k; kk <= ktop; kk++) o[(ll)kk * d.plane + col] = trs;
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* Face-once split of k_adv (schemes other than FFSL): k_face computes  */
+/* each cell's own west, south and bottom face fluxes once per tracer   */
+/* (advection times the transport, then horizontal diffusion), and      */
+/* k_upd updates the cells from them.  The east face of a cell is the   */
+/* west face of its east neighbour: the stencil walk from either side   */
+/* reaches the same cells and ghosts, and the face coefficients are the */
+/* same expressions, so the result equals k_adv's bit for bit.          */
+/*-------------------------------------------------------------------*/
+template <int SCH>
+__global__ void __launch_bounds__(256)
+k_face(emstr_dev d, const double *__restrict__ tin, const double *__restrict__ tstart,
+       double dtu, int anydif, double *__restrict__ fw, double *__restrict__ fs,
+       double *__restrict__ fb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (k < kbc) return;
+  const ll idx = (ll)k * d.plane + col;
+  const ll N3 = d.n3;
+  const int ksc = d.ks[col];
+  const int courf = (d.ultimate || d.trasc >= QUICKEST) ? 1 : 0;
+  const int ult = d.ultimate;
+  /* vertical faces only of the cells to process */
+  const bool vert = (ksc >= 0 && k <= ksc);
+  const Sten s = make_sten(d, col, k, kbc);
+  CellCo cc;
+  cell_coef<SCH>(d, col, idx, k, kbc, s, dtu, false, courf, cc);
+  if (anydif) diff_coef(d, col, idx, s, cc);
+  for (int n = 0; n < d.ntr; n++) {
+    if (!d.in_tbdy[n]) continue;
+    const int adv = d.advect[n], dif = anydif ? d.diffuse[n] : 0;
+    const Vals v = load_vals(tin + n * N3, tstart + n * N3, idx, s);
+    double FxW = 0.0, FyS = 0.0, FzB = 0.0;
+    if (adv) {
+      FxW = face_flux<SCH>(v.xm2, v.xm1, v.c, v.xp1, cc.W, ult);
+      FyS = face_flux<SCH>(v.ym2, v.ym1, v.c, v.yp1, cc.S, ult);
+      if (vert) {
+        if (SCH == ORDER1)
+          FzB = o1_fz(cc.B.vel, cc.dtu, v.zm1, v.c);
+        else
+          FzB = face_flux<SCH>(v.zm2, v.zm1, v.c, v.zp1, cc.B, ult);
+      }
+    }
+    if (dif) {
+      FxW -= (cc.kW * (v.c - v.xm1) / cc.hW);
+      FyS -= (cc.kS * (v.c - v.ym1) / cc.hS);
+    }
+    fw[n * N3 + idx] = FxW;
+    fs[n * N3 + idx] = FyS;
+    if (vert) fb[n * N3 + idx] = FzB;
+  }
+}
+
+/* dtracer of the cell at layer k of column col from the face fluxes */
+__device__ __forceinline__ double face_dtr(const emstr_dev &d, const double *__restrict__ fw,
+                                           const double *__restrict__ fs, ll col, int k,
+                                           double dtu) {
+  const ll idx = (ll)k * d.plane + col, nx = d.nx;
+  const double FxW = fw[idx], FyS = fs[idx];
+  const double FxE = wetat(d.kb, col + 1, k) ? fw[idx + 1] : 0.0;
+  const double FyN = wetat(d.kb, col + nx, k) ? fs[idx + nx] : 0.0;
+  return (FxE - FxW + FyN - FyS) * dtu;
+}
+
+template <int SCH>
+__global__ void __launch_bounds__(256)
+k_upd(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout, double dtu,
+      const double *__restrict__ fw, const double *__restrict__ fs,
+      const double *__restrict__ fb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (k < kbc) return;
+  const ll P = d.plane;
+  const ll idx = (ll)k * P + col;
+  const ll N3 = d.n3;
+  const int ksc = d.ks[col];
+  if (ksc < 0) { /* column not processed: unchanged */
+    for (int n = 0; n < d.ntr; n++) tout[n * N3 + idx] = tin[n * N3 + idx];
+    return;
+  }
+  if (k > ksc) return; /* above the processed surface: written below */
+  const double area = d.cellarea[col], dzc = d.dz[idx];
+
+  if (k < ksc) {
+    for (int n = 0; n < d.ntr; n++) {
+      const double *t = tin + n * N3;
+      if (!d.in_tbdy[n]) {
+        tout[n * N3 + idx] = t[idx];
+        continue;
+      }
+      const double dtr = face_dtr(d, fw + n * N3, fs + n * N3, col, k, dtu);
+      double FzB = 0.0, FzT = 0.0;
+      if (d.advect[n]) {
+        FzB = fb[n * N3 + idx];
+        FzT = fb[n * N3 + idx + P];
+      }
+      double r = subsurface(t[idx], dtr, area, FzT, FzB, dzc);
+      /* tr_bounds clip (tracers.c:3279-3285) for s1 cells */
+      if (d.mintr[n] <= 1e-5 && r < d.mintr[n]) r = d.mintr[n];
+      tout[n * N3 + idx] = r;
+    }
+    return;
+  }
+
+  /* ---- processed surface cell of the column (k == ksc) ---- */
+  const int kold = d.kold[col];
+  const double subeta = d.subeta[col];
+  const double osubeta = d.osubeta[col];
+  const double botz = d.botz[col];
+  const double *lay = d.layers;
+  const int ktop = d.nz - 1;
+  for (int n = 0; n < d.ntr; n++) {
+    const double *t = tin + n * N3;
+    const double *fwn = fw + n * N3, *fsn = fs + n * N3, *fbn = fb + n * N3;
+    double *o = tout + n * N3;
+    if (!d.in_tbdy[n]) {
+      for (int kk = k; kk <= ktop; kk++) o[(ll)kk * P + col] = t[(ll)kk * P + col];
+      continue;
+    }
+    const int adv = d.advect[n];
+    double trs;
+    if (!adv) {
+      trs = t[idx];
+    } else {
+      const double dtr0 = face_dtr(d, fwn, fsn, col, k, dtu);
+      /* surf_conc (tracers.c:3047-3207) */
+      double fcbot = -fbn[idx] * area;
+      int c = k;
+      double waterbot = (c == kbc) ? botz : lay[c];
+      double dzs = subeta - waterbot;
+      double svol = dzs * area;
+      double surftr = -fcbot;
+      double watertop;
+      double dtrc = dtr0;
+      while (c != kold && c < ktop) {
+        watertop = lay[c + 1];
+        surftr += (t[(ll)c * P + col] * area * (watertop - waterbot) - dtrc);
+        c = c + 1;
+        waterbot = lay[c];
+        dtrc = face_dtr(d, fwn, fsn, col, c, dtu);
+      }
+      watertop = osubeta;
+      waterbot = (c == kbc) ? botz : lay[c];
+      surftr += (t[(ll)c * P + col] * area * (watertop - waterbot) - dtrc);
+      if (c != ktop) {
+        do {
+          c = c + 1;
+          surftr -= (face_dtr(d, fwn, fsn, col, c, dtu));
+        } while (c != ktop);
+      }
+      if (svol == 0.0 || (surftr < 0.0 && dzs <= d.hmin))
+        surftr = 0.0;
+      else
+        surftr /= svol;
+      /* tracers.c:953-954 */
+      trs = (botz > 0.0 && dzc < d.hmin) ? t[idx] : surftr;
+    }
+    /* tr_bounds, thin surface layer (tracers.c:3250-3272) */
+    const double dzco = (kold >= kbc) ? d.dz[(ll)kold * P + col] : 0.0;
+    if (k > kbc && dzco < d.hmin) {
+      if (trs < d.mintr[n] || trs > d.maxtr[n]) {
+        /* the full update of the layer below */
+        const ll ib = idx - P;
+        const double dtrb = face_dtr(d, fwn, fsn, col, k - 1, dtu);
+        double FzB = 0.0, FzT = 0.0;
+        if (adv) {
+          FzB = fbn[ib];
+          FzT = fbn[idx];
+        }
+        trs = subsurface(t[ib], dtrb, area, FzT, FzB, d.dz[ib]);
+      }
+    }
+    /* clip (tracers.c:3279-3285) */
+    if (d.mintr[n] <= 1e-5 && trs < d.mintr[n]) trs = d.mintr[n];
+    /* the surface value fills every layer above (tracers.c:3306-3322) */
+    for (int kk = k; kk <= ktop; kk++) o[(ll)kk * P + col] = trs;
   }
 }
 
@@ -1422,30 +1613,47 @@ extern "C" int emstr_launch_adv(const emstr_dev *d, const double *tin,
   dim3 b(64, 4, 1);
   dim3 g = grid3(d, b);
   hipStream_t st = (hipStream_t)stream;
+  /* FFSL only: the other schemes take the face-once path */
+  if (d->trasc != FFSL) return -1;
+  k_adv<FFSL><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <int SCH>
+static void face_launch(const emstr_dev *d, const double *tin, double *tout,
+                        const double *tstart, double dtu, int anydif, double *fw, double *fs,
+                        double *fb, hipStream_t st) {
+  dim3 b(64, 4, 1);
+  dim3 g = grid3(d, b);
+  k_face<SCH><<<g, b, 0, st>>>(*d, tin, tstart, dtu, anydif, fw, fs, fb);
+  k_upd<SCH><<<g, b, 0, st>>>(*d, tin, tout, dtu, fw, fs, fb);
+}
+
+extern "C" int emstr_launch_adv_faces(const emstr_dev *d, const double *tin, double *tout,
+                                      const double *tstart, double dtu, int anydif,
+                                      double *fw, double *fs, double *fb, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
   switch (d->trasc) {
-    case FFSL:
-      k_adv<FFSL><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
-      break;
     case VANLEER:
-      k_adv<VANLEER><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
+      face_launch<VANLEER>(d, tin, tout, tstart, dtu, anydif, fw, fs, fb, st);
       break;
     case QUICKEST:
-      k_adv<QUICKEST><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
+      face_launch<QUICKEST>(d, tin, tout, tstart, dtu, anydif, fw, fs, fb, st);
       break;
     case QUICKEST_CO:
-      k_adv<QUICKEST_CO><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
+      face_launch<QUICKEST_CO>(d, tin, tout, tstart, dtu, anydif, fw, fs, fb, st);
       break;
     case ORDER1:
-      k_adv<ORDER1><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
+      face_launch<ORDER1>(d, tin, tout, tstart, dtu, anydif, fw, fs, fb, st);
       break;
     case ORDER2:
-      k_adv<ORDER2><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
+      face_launch<ORDER2>(d, tin, tout, tstart, dtu, anydif, fw, fs, fb, st);
       break;
     case ORDER4:
-      k_adv<ORDER4><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
+      face_launch<ORDER4>(d, tin, tout, tstart, dtu, anydif, fw, fs, fb, st);
       break;
     case ORDER2_UW:
-      k_adv<ORDER2_UW><<<g, b, 0, st>>>(*d, tin, tout, tstart, dtu, trem, anydif, n0, n1);
+      face_launch<ORDER2_UW>(d, tin, tout, tstart, dtu, anydif, fw, fs, fb, st);
       break;
     default:
       return -1;
