@@ -28,9 +28,14 @@
  *   k_ffmoc    tr_moc: the mean end value over the cell's wet edges, in
  *              the order the reference accumulates them
  *   k_fffz     the vertical flux of the cells to process (V_VC: van Leer
- *              on the fractional part, tr_moc over whole layers)
- * The cell update, surf_concrk and tr_bounds are k_usadv's.  Every
- * per-tracer kernel runs one thread per cell or edge over all tracers.
+ *              on the fractional part, tr_moc over whole layers); run
+ *              first without tracers to mark the tr_moc centres the
+ *              fluxes read, so that k_ffedge stores end values and k_ffmoc
+ *              computes tr_moc for those only
+ * The cell update, surf_concrk and tr_bounds are k_usadv's.  k_fflsq
+ * runs a wave per cell with a lane per tracer over tracer rows written by
+ * k_ffmod; the other per-tracer kernels run one thread per cell or edge
+ * over all tracers.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -41,10 +46,6 @@ typedef long long ll;
 
 #define FF_TF 0.55     /* transport.c:2014 */
 #define FF_SMALL 1e-10 /* transport.c:24 */
-#ifndef FF_CH
-#define FF_CH 4 /* tracers per pass in k_fflsq (fewer registers, more waves) */
-#endif
-#define FF_MAXA 4      /* most centres around a vertex (nvcm) */
 
 __device__ __forceinline__ double fmx(double a, double b) { return (a > b) ? a : b; }
 __device__ __forceinline__ double fmn(double a, double b) { return (a < b) ? a : b; }
@@ -271,106 +272,184 @@ k_ffline(emstr_usdev D, double dtu, const ll *__restrict__ edn, int nesp,
 /*-------------------------------------------------------------------*/
 /* per tracer                                                        */
 /*-------------------------------------------------------------------*/
-/* tr_mod (transport.c:2082-2106 with set_tr_nograd) */
+/* tr_mod (transport.c:2082-2106 with set_tr_nograd), written as rows of
+ * the tracers of one wet centre, tq[q * tqs + n]: a block takes FM_C
+ * consecutive wet centres, reads each tracer along the centres and writes
+ * the rows through LDS.  Tracers that are not advected hold 0. */
+#define FM_C 32 /* wet centres per block */
 __global__ void __launch_bounds__(256)
 k_ffmod(emstr_usdev D, const double *__restrict__ tin, const double *__restrict__ nw,
-        const int *__restrict__ clc, const double *__restrict__ crc, double *__restrict__ tm) {
+        const int *__restrict__ clc, const double *__restrict__ crc, double *__restrict__ tq) {
+  __shared__ double S[FM_C][65];
   const emstr_dev &d = D.d;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= d.nwet) return;
-  const ll P = d.plane, N3 = d.n3;
-  const ll idx = d.wdn[q];
-  const int col = (int)(idx % P), k = (int)(idx / P);
-  const int kb = d.kb[col], ks = d.ks[col];
-  const int kk = (ks >= 0 && k > ks) ? ks : k; /* above the surface: its value */
-  const ll ik = (ll)kk * P + col;
-  const double vel = nw[ik], crf = crc[ik];
-  const int cl = clc[ik];
-  const int zp = (vel < 0.0) ? cl : ffzm1(cl, kb);
-  const int ci = (vel < 0.0) ? ffzm1(zp, kb) : cl;
-  const ll iz = (ll)((zp < kb) ? kb : zp) * P + col;
-  const ll ic = (ll)((ci < kb) ? kb : ci) * P + col;
-  for (int n = 0; n < d.ntr; n++) {
-    if (!d.in_tbdy[n] || !d.advect[n]) continue;
-    const double *t = tin + n * N3;
-    double r = FF_TF * t[ic] + (1.0 - FF_TF) * t[ik];
-    if (crf > 0.) r += FF_TF * (crf * (t[iz] - t[ic]));
-    tm[n * N3 + idx] = r;
+  const int t = threadIdx.x, c = t & (FM_C - 1), g = t / FM_C;
+  const int q0 = blockIdx.x * FM_C, q = q0 + c;
+  const int nc = min(FM_C, d.nwet - q0);
+  const ll P = d.plane, N3 = d.n3, ts = D.tqs;
+  ll ik = 0, iz = 0, ic = 0;
+  double crf = 0.0;
+  if (q < d.nwet) {
+    const ll idx = d.wdn[q];
+    const int col = (int)(idx % P), k = (int)(idx / P);
+    const int kb = d.kb[col], ks = d.ks[col];
+    const int kk = (ks >= 0 && k > ks) ? ks : k; /* above the surface: its value */
+    ik = (ll)kk * P + col;
+    const double vel = nw[ik];
+    crf = crc[ik];
+    const int cl = clc[ik];
+    const int zp = (vel < 0.0) ? cl : ffzm1(cl, kb);
+    const int ci = (vel < 0.0) ? ffzm1(zp, kb) : cl;
+    iz = (ll)((zp < kb) ? kb : zp) * P + col;
+    ic = (ll)((ci < kb) ? kb : ci) * P + col;
+  }
+  for (int n0 = 0; n0 < d.ntr; n0 += 64) {
+    const int tn = min(64, d.ntr - n0);
+    if (q < d.nwet)
+#pragma unroll 2
+      for (int j = g; j < tn; j += 256 / FM_C) {
+        const int n = n0 + j;
+        double r = 0.0;
+        if (d.in_tbdy[n] && d.advect[n]) {
+          const double *tt = tin + n * N3;
+          r = FF_TF * tt[ic] + (1.0 - FF_TF) * tt[ik];
+          if (crf > 0.) r += FF_TF * (crf * (tt[iz] - tt[ic]));
+        }
+        S[c][j] = r;
+      }
+    __syncthreads();
+    for (int f = t; f < nc * 64; f += 256) {
+      const int cc = f >> 6, j = f & 63;
+      if (j < tn) tq[(ll)(q0 + cc) * ts + n0 + j] = S[cc][j];
+    }
+    __syncthreads();
   }
 }
 
+/* block b's position when blocks are dealt to the 8 XCDs round robin and
+ * each XCD should take one contiguous range of positions (its L2 then
+ * holds the rows its neighbouring centres share) */
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int full = nb / 8, rem = nb % 8, x = b % 8;
+  return x * full + min(x, rem) + b / 8;
+}
+
+/* lane l's value of a wave-uniform broadcast */
+__device__ __forceinline__ int rdl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double rdl(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 /* get_linear_limit, df = 2: slopes w1, w2 from the stencil, beta from the
- * vertex ranges (lw->beta is 1 while the vertex values are taken) */
+ * vertex ranges (lw->beta is 1 while the vertex values are taken).  One
+ * wave per wet centre with a lane per tracer: the stencil's slots and
+ * weights and the list of distinct centres are loaded an entry a lane and
+ * broadcast, the vertex offsets and ranges come through the scalar unit,
+ * the distinct centres' tr_mod rows (at most EMSTR_FF_ROWS) are loaded at
+ * once, and the stencil and the vertex ranges take them by slot.  A block
+ * takes LS_C consecutive centres and writes w1, w2, beta and w0 = tr_mod
+ * as [4][tracer][cell] planes through LDS. */
+#define LS_C 8 /* wet centres per block */
 __global__ void __launch_bounds__(256)
-k_fflsq(emstr_usdev D, const double *__restrict__ tm, double *__restrict__ lq) {
+k_fflsq(emstr_usdev D, const double *__restrict__ tq, double *__restrict__ lq) {
+  __shared__ double S[4][64][LS_C + 1];
   const emstr_dev &d = D.d;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= d.nwet) return;
-  const ll P = d.plane, N3 = d.n3, NT3 = (ll)d.ntr * N3;
-  const ll idx = d.wdn[q];
-  const int col = (int)(idx % P), k = (int)(idx / P);
-  const int ncol = (int)P, npe = D.npe[col];
-  const int o0 = D.lsq_off[q], o1 = D.lsq_off[q + 1];
-  const double cxc = D.cx[col], cyc = D.cy[col];
-  for (int n0 = 0; n0 < d.ntr; n0 += FF_CH) {
-    double a1[FF_CH], a2[FF_CH], w0[FF_CH], beta[FF_CH];
-    bool on[FF_CH];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int q0 = xcd_block(blockIdx.x, gridDim.x) * LS_C;
+  const int nc = min(LS_C, d.nwet - q0);
+  const ll P = d.plane, N3 = d.n3, NT3 = (ll)d.ntr * N3, ts = D.tqs;
+  /* the block's centres, a centre a lane */
+  int m_col = 0, m_npe = 0, m_o0 = 0, m_o1 = 0, m_r0 = 0, m_r1 = 0;
+  if (lane < nc) {
+    const int q = q0 + lane;
+    m_col = (int)(d.wdn[q] % P);
+    m_o0 = D.lsq_off[q];
+    m_o1 = D.lsq_off[q + 1];
+    m_r0 = D.lr_off[q];
+    m_r1 = D.lr_off[q + 1];
+    m_npe = D.npe[m_col];
+  }
+  /* the write-out: thread (cell ccw, tracer jw + 32 h) of plane pl */
+  const int ccw = threadIdx.x & (LS_C - 1), jw = threadIdx.x / LS_C;
+  const ll iw = (ccw < nc) ? d.wdn[q0 + ccw] : 0;
+  for (int n0 = 0; n0 < d.ntr; n0 += 64) {
+    const int n = n0 + lane, nl = (n < d.ntr) ? n : 0;
+    for (int cc = w; cc < nc; cc += 4) {
+      const int q = q0 + cc;
+      const int col = rdl(m_col, cc), npe = rdl(m_npe, cc);
+      const int o0 = rdl(m_o0, cc), ns = rdl(m_o1, cc) - o0;
+      const int r0 = rdl(m_r0, cc), nr = rdl(m_r1, cc) - r0;
+      /* stencil and row list, an entry a lane (lanes past the end repeat
+       * entry 0: no divergent loads; only entries are broadcast) */
+      const int ls = (lane < ns) ? lane : 0, lr = (lane < nr) ? lane : 0;
+      const int e_s = D.lsq_s[o0 + ls];
+      const double e_b1 = D.lsq_B[2 * (ll)(o0 + ls)], e_b2 = D.lsq_B[2 * (ll)(o0 + ls) + 1];
+      const int e_rq = D.lr_q[r0 + lr];
+      /* vertex offsets and ranges (padded to EMSTR_FF_NPE) */
+      const double *xy = D.vxy + (ll)col * (2 * EMSTR_FF_NPE);
+      const unsigned *vw = D.vsl + (ll)q * EMSTR_FF_NPE;
+      double vx[EMSTR_FF_NPE], vy[EMSTR_FF_NPE];
+      unsigned wd[EMSTR_FF_NPE];
 #pragma unroll
-    for (int c = 0; c < FF_CH; c++) {
-      const int n = n0 + c;
-      on[c] = (n < d.ntr) && d.in_tbdy[n] && d.advect[n];
-      a1[c] = 0.0;
-      a2[c] = 0.0;
-      w0[c] = on[c] ? tm[n * N3 + idx] : 0.0;
-      beta[c] = 1.0;
-    }
-    for (int i = o0; i < o1; i++) {
-      const ll is = D.lsq_idx[i];
-      const double b1 = D.lsq_B[2 * (ll)i], b2 = D.lsq_B[2 * (ll)i + 1];
-#pragma unroll
-      for (int c = 0; c < FF_CH; c++)
-        if (on[c]) {
-          const double t = tm[(n0 + c) * N3 + is];
-          a1[c] += (t * b1);
-          a2[c] += (t * b2);
-        }
-    }
-    /* vertices in order; each tracer's beta sees them in that order */
-    for (int j = 0; j < npe; j++) {
-      const int vt = D.cvt[j * ncol + col];
-      const double x = cxc - D.vx[vt], y = cyc - D.vy[vt];
-      int vcl[FF_MAXA], nv = 0;
-#pragma unroll
-      for (int i = 0; i < FF_MAXA; i++) {
-        const int cv = (i < D.nvcm) ? D.vcol[vt * D.nvcm + i] : -1;
-        if (cv >= 0 && ffwet(d.kb, cv, k)) vcl[nv++] = cv;
+      for (int j = 0; j < EMSTR_FF_NPE; j++) {
+        vx[j] = xy[2 * j];
+        vy[j] = xy[2 * j + 1];
+        wd[j] = vw[j];
       }
+      /* rows of the distinct centres; lanes past the last tracer read
+       * tracer 0 and are not written out */
+      double t[EMSTR_FF_ROWS];
 #pragma unroll
-      for (int c = 0; c < FF_CH; c++) {
-        if (!on[c]) continue;
-        const double *t = tm + (n0 + c) * N3 + (ll)k * P;
-        const double val = w0[c] + 1.0 * (a1[c] * x + a2[c] * y);
-        const double dd = val - w0[c];
+      for (int s = 0; s < EMSTR_FF_ROWS; s++) {
+        const double *row = tq + (ll)rdl(e_rq, s) * ts;
+        t[s] = (s < nr) ? row[nl] : 0.0;
+      }
+      const double w0 = t[0];
+      double a1 = 0.0, a2 = 0.0, beta = 1.0;
+      for (int i = 0; i < ns; i++) {
+        const double tv = t[rdl(e_s, i)];
+        a1 += (tv * rdl(e_b1, i));
+        a2 += (tv * rdl(e_b2, i));
+      }
+      /* vertices in order; the wet centres around one in column order */
+#pragma unroll
+      for (int j = 0; j < EMSTR_FF_NPE; j++) {
+        if (j >= npe) break;
+        unsigned word = wd[j];
         double vmin = __builtin_huge_val(), vmax = -__builtin_huge_val();
-#pragma unroll
-        for (int i = 0; i < FF_MAXA; i++) {
-          if (i >= nv) break;
-          const double tv = t[vcl[i]];
+        for (int i = 0; i < 4; i++, word >>= 8) {
+          const int sl = (int)(word & 0xffu);
+          if (sl == 0xff) break;
+          const double tv = t[sl];
           vmin = fmn(vmin, tv);
           vmax = fmx(vmax, tv);
         }
-        if (dd != 0.0 && val > vmax) beta[c] = fmx(fmn((vmax - w0[c]) / dd, beta[c]), 0.0);
-        if (dd != 0.0 && val < vmin) beta[c] = fmx(fmn((vmin - w0[c]) / dd, beta[c]), 0.0);
+        const double val = w0 + 1.0 * (a1 * vx[j] + a2 * vy[j]);
+        const double dd = val - w0;
+        if (dd != 0.0 && val > vmax) beta = fmx(fmn((vmax - w0) / dd, beta), 0.0);
+        if (dd != 0.0 && val < vmin) beta = fmx(fmn((vmin - w0) / dd, beta), 0.0);
       }
+      S[0][lane][cc] = a1;
+      S[1][lane][cc] = a2;
+      S[2][lane][cc] = fmn(fmx(beta, 0.0), 1.0);
+      S[3][lane][cc] = w0;
     }
+    __syncthreads();
+    const int tn = min(64, d.ntr - n0);
+    if (ccw < nc)
 #pragma unroll
-    for (int c = 0; c < FF_CH; c++) {
-      if (!on[c]) continue;
-      const int n = n0 + c;
-      lq[0 * NT3 + n * N3 + idx] = a1[c];
-      lq[1 * NT3 + n * N3 + idx] = a2[c];
-      lq[2 * NT3 + n * N3 + idx] = fmn(fmx(beta[c], 0.0), 1.0);
-    }
+      for (int h = 0; h < 2; h++) {
+        const int j = jw + 32 * h, nn = n0 + j;
+        if (j < tn && d.in_tbdy[nn] && d.advect[nn]) {
+          double *o = lq + nn * N3 + iw;
+#pragma unroll
+          for (int pl = 0; pl < 4; pl++) o[pl * NT3] = S[pl][j][ccw];
+        }
+      }
+    __syncthreads();
   }
 }
 
@@ -379,8 +458,8 @@ k_fflsq(emstr_usdev D, const double *__restrict__ tm, double *__restrict__ lq) {
 __global__ void __launch_bounds__(256)
 k_ffedge(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ edn, int nesp,
          const int *__restrict__ lci, const double *__restrict__ ld,
-         const double *__restrict__ tm, const double *__restrict__ lq, double *__restrict__ fx,
-         double *__restrict__ te, int anydif) {
+         const double *__restrict__ lq, double *__restrict__ fx,
+         double *__restrict__ te, int anydif, const int *__restrict__ need, int stamp) {
   const emstr_dev &d = D.d;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nesp) return;
@@ -393,11 +472,13 @@ k_ffedge(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ e
   const double dye = ld[3 * NE3 + ie], dist = ld[4 * NE3 + ie];
   const double f = D.flux[ie];
   const ll i1 = (ll)k * P + D.ec1[e], i2 = (ll)k * P + D.ec2[e];
+  /* the end value is read only by the tr_moc of the centres either side */
+  const bool wte = (D.ec1[e] >= 0 && need[i1] == stamp) || (D.ec2[e] >= 0 && need[i2] == stamp);
   for (int n = 0; n < d.ntr; n++) {
     if (!d.in_tbdy[n]) continue;
     double F = 0.0;
     if (d.advect[n]) {
-      const double w0 = tm[n * N3 + cidx];
+      const double w0 = lq[3 * NT3 + n * N3 + cidx];
       const double w1 = lq[n * N3 + cidx], w2 = lq[NT3 + n * N3 + cidx];
       const double b = lq[2 * NT3 + n * N3 + cidx];
       const double trs = w0 + b * (w1 * dxs + w2 * dys);
@@ -406,7 +487,7 @@ k_ffedge(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ e
       const double ntr = 0.0 + 0.5 * dist * (trs + tre);
       F = (dint != 0.0) ? ntr / dint : 0.0;
       F *= f;
-      te[n * NE3 + ie] = tre;
+      if (wte) te[n * NE3 + ie] = tre;
     }
     if (anydif && d.diffuse[n]) {
       const double *t = tin + n * N3;
@@ -417,15 +498,18 @@ k_ffedge(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ e
   }
 }
 
-/* tr_moc (transport.c:2323-2330, 2412-2420 with set_tr_nograd) */
+/* tr_moc (transport.c:2323-2330, 2412-2420 with set_tr_nograd), of the
+ * centres the vertical fluxes read (need[idx] == stamp; all of them at or
+ * below the surface, where the no-gradient copy is the centre itself) */
 __global__ void __launch_bounds__(256)
 k_ffmoc(emstr_usdev D, const double *__restrict__ tin, const double *__restrict__ te,
-        double *__restrict__ moc) {
+        double *__restrict__ moc, const int *__restrict__ need, int stamp) {
   const emstr_dev &d = D.d;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.nwet) return;
   const ll P = d.plane, N3 = d.n3, NE3 = (ll)d.nz * D.ne2;
   const ll idx = d.wdn[q];
+  if (need[idx] != stamp) return;
   const int col = (int)(idx % P), k = (int)(idx / P);
   const int ks = d.ks[col], ncol = (int)P;
   const int kk = (ks >= 0 && k > ks) ? ks : k;
@@ -449,11 +533,21 @@ k_ffmoc(emstr_usdev D, const double *__restrict__ tin, const double *__restrict_
   }
 }
 
-/* vertical fluxes (transport.c:2441-2490, ffsl_van_leer :8216-8260) */
+__device__ __forceinline__ double ffmark(int *need, ll i, int stamp) {
+  need[i] = stamp;
+  return 0.0;
+}
+
+/* vertical fluxes (transport.c:2441-2490, ffsl_van_leer :8216-8260).
+ * MARK: one pass that follows the same stencil and whole-layer walk
+ * without tracer values and sets need[] = stamp at every tr_moc centre
+ * the fluxes read (tr_moc is read only there) */
+template <bool MARK>
 __global__ void __launch_bounds__(256)
 k_fffz(emstr_usdev D, double dtu, const double *__restrict__ tin, const double *__restrict__ moc,
        const int *__restrict__ clf, const double *__restrict__ crf,
-       const double *__restrict__ cour, double *__restrict__ fz, int *__restrict__ err) {
+       const double *__restrict__ cour, double *__restrict__ fz, int *__restrict__ err,
+       int *__restrict__ need, int stamp) {
   const emstr_dev &d = D.d;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.nwet) return;
@@ -471,11 +565,12 @@ k_fffz(emstr_usdev D, double dtu, const double *__restrict__ tin, const double *
   if (w > 0.0) cp = ffzp1(cp, kb, nz);
   const int bm1 = ffzm1(cp, kb), bm2 = ffzm1(bm1, kb), fp = ffzp1(cp, kb, nz);
   const double fr = (w == 0.0 || k == kb) ? 0.0 : cr * ffdz(d, col, cl, kb);
-  for (int n = 0; n < d.ntr; n++) {
-    if (!d.in_tbdy[n] || !d.advect[n]) continue;
+  for (int n = 0; n < (MARK ? 1 : d.ntr); n++) {
+    if (!MARK && (!d.in_tbdy[n] || !d.advect[n])) continue;
     const double *t = tin + n * N3, *mc = moc + n * N3;
-#define FFT(L) t[(ll)(((L) < kb) ? kb : (L)) * P + col]
-#define FFM(L) mc[(ll)(((L) < kb) ? kb : (((L) > ks) ? ks : (L))) * P + col]
+#define FFT(L) (MARK ? 0.0 : t[(ll)(((L) < kb) ? kb : (L)) * P + col])
+#define FFL(L) ((ll)(((L) < kb) ? kb : (((L) > ks) ? ks : (L))) * P + col)
+#define FFM(L) (MARK ? ffmark(need, FFL(L), stamp) : mc[FFL(L)])
 #define FFV(L) (usemoc ? FFM(L) : FFT(L))
     double F = 0.0, df;
     if (cr != 0.0) {
@@ -518,9 +613,10 @@ k_fffz(emstr_usdev D, double dtu, const double *__restrict__ tin, const double *
       if (w < 0) F *= -1.0;
     }
 #undef FFT
+#undef FFL
 #undef FFM
 #undef FFV
-    fz[n * N3 + idx] = F;
+    if (!MARK) fz[n * N3 + idx] = F;
   }
 }
 
@@ -553,18 +649,20 @@ extern "C" int emstr_us_launch_ffsl_flux(const emstr_usdev *D, double dtu, const
                                          const long long *edn, int nesp, const int *lci,
                                          const double *ld, double *tm, double *lq, double *te,
                                          double *moc, double *fx, double *fz, int anydif,
-                                         int *err, void *stream) {
+                                         int *need, int stamp, int *err, void *stream) {
   const emstr_dev *d = &D->d;
   hipStream_t st = (hipStream_t)stream;
   const int gw = (d->nwet + 255) / 256, ge = (nesp + 255) / 256;
   if (d->nwet) {
-    k_ffmod<<<gw, 256, 0, st>>>(*D, tin, nw, clc, crc, tm);
-    k_fflsq<<<gw, 256, 0, st>>>(*D, tm, lq);
+    k_fffz<true><<<gw, 256, 0, st>>>(*D, dtu, tin, moc, clf, crf, cour, fz, err, need, stamp);
+    k_ffmod<<<(d->nwet + FM_C - 1) / FM_C, 256, 0, st>>>(*D, tin, nw, clc, crc, tm);
+    k_fflsq<<<(d->nwet + LS_C - 1) / LS_C, 256, 0, st>>>(*D, tm, lq);
   }
-  if (nesp) k_ffedge<<<ge, 256, 0, st>>>(*D, tin, edn, nesp, lci, ld, tm, lq, fx, te, anydif);
+  if (nesp)
+    k_ffedge<<<ge, 256, 0, st>>>(*D, tin, edn, nesp, lci, ld, lq, fx, te, anydif, need, stamp);
   if (d->nwet) {
-    k_ffmoc<<<gw, 256, 0, st>>>(*D, tin, te, moc);
-    k_fffz<<<gw, 256, 0, st>>>(*D, dtu, tin, moc, clf, crf, cour, fz, err);
+    k_ffmoc<<<gw, 256, 0, st>>>(*D, tin, te, moc, need, stamp);
+    k_fffz<false><<<gw, 256, 0, st>>>(*D, dtu, tin, moc, clf, crf, cour, fz, err, need, stamp);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -77,6 +77,7 @@ typedef struct emstr_us_ctx {
   int ghost_edges;   /* ghost velocities are the rotated edge velocity   */
   double *d_nw, *d_crc, *d_crf, *d_cour, *d_ld, *d_tm, *d_lq, *d_te, *d_moc;
   int *d_clc, *d_clf, *d_lci, *d_fferr;
+  int *d_need, ffstamp; /* tr_moc centres read, by sub-step stamp */
   int anydif, have_cols;
   int key_szc, key_sze;
 } usctx;
@@ -110,11 +111,10 @@ static void us_destroy(usctx *c) {
                 D->h2e, D->qd3, D->qhm, D->qhp, D->u1, D->flux, D->dzu, D->kh, c->tr[0], c->tr[1], c->tr[2],
                 c->stage, c->d_esp, c->d_edn, c->d_bpt, c->d_bin, c->d_gsed, c->d_ised,
                 c->d_proc, c->d_err, c->d_dtm, c->d_fx, c->d_fz, D->cx, D->cy, D->ux,
-                D->uy, D->n1x, D->n1y, D->t2x, D->t2y, D->ev, D->cvt, D->vx, D->vy,
-                D->vcol, D->eord, D->lsq_off, D->lsq_idx, D->lsq_B, D->u2, D->wtop,
+                D->uy, D->n1x, D->n1y, D->t2x, D->t2y, D->ev, D->vxy, D->eord, D->lsq_off, D->lsq_s, D->lr_off, D->lr_q, D->vsl, D->lsq_B, D->u2, D->wtop,
                 D->wbot, D->uc, D->vc, D->gu0, D->gv0, D->gue, D->gve, c->d_nw, c->d_crc,
                 c->d_crf, c->d_cour, c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc,
-                c->d_clc, c->d_clf, c->d_lci, c->d_fferr};
+                c->d_clc, c->d_clf, c->d_lci, c->d_fferr, c->d_need};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->cs2col, c->col2cs, c->h_kb, c->h_ks, c->h_kold, c->h_knew, c->h_cols,
@@ -690,6 +690,16 @@ static int us_download_tracers(usctx *c) {
 /*-------------------------------------------------------------------*/
 /* FFSL: surface geometry, vertices, least-squares stencils           */
 /*-------------------------------------------------------------------*/
+/* slot of wet centre q in the row list rw[0..*nr-1], appended if new;
+ * -1 when the list is full */
+static int ff_slot(int *rw, int *nr, int q) {
+  for (int i = 0; i < *nr; i++)
+    if (rw[i] == q) return i;
+  if (*nr >= EMSTR_FF_ROWS) return -1;
+  rw[*nr] = q;
+  return (*nr)++;
+}
+
 static int icmp(const void *a, const void *b) {
   int x = *(const int *)a, y = *(const int *)b;
   return (x > y) - (x < y);
@@ -711,6 +721,8 @@ static int us_ffsl_geom(usctx *c) {
   int ok = 1, n, j, col, e;
   if (nvl < 1 || nvcm < 1) return emstr_set_error(E_ARG, -1, "FFSL: no vertices");
   if (nvcm > 4) return emstr_set_error(E_UNSUP, -1, "FFSL: more than 4 centres at a vertex");
+  if (npem > EMSTR_FF_NPE)
+    return emstr_set_error(E_UNSUP, -1, "FFSL: more than %d edges a centre", EMSTR_FF_NPE);
   double *hcx = (double *)malloc((size_t)ncol * sizeof(double));
   double *hcy = (double *)malloc((size_t)ncol * sizeof(double));
   double *he[6];
@@ -793,6 +805,29 @@ static int us_ffsl_geom(usctx *c) {
       }
     for (j = 0; j < np; j++) heord[j * ncol + col] = sl[j];
   }
+  /* centre minus vertex j of every column (zero past its vertices) */
+  double *hvxy = (double *)calloc((size_t)ncol * EMSTR_FF_NPE * 2, sizeof(double));
+  for (col = 0; col < ncol && ok && hvxy; col++)
+    for (j = 0; j < g->npe[c->col2cs[col]]; j++) {
+      const int vt = hcvt[j * ncol + col];
+      hvxy[((size_t)col * EMSTR_FF_NPE + j) * 2 + 0] = hcx[col] - hvx[vt];
+      hvxy[((size_t)col * EMSTR_FF_NPE + j) * 2 + 1] = hcy[col] - hvy[vt];
+    }
+  /* wet-list position of every dense cell; every cell from the bed up
+   * must be on the wet list (the vertex ranges read them by position) */
+  int *hiq = (int *)malloc((size_t)nz * ncol * sizeof(int));
+  for (size_t i = 0; i < (size_t)nz * ncol; i++) hiq[i] = -1;
+  for (n = 0; n < d->nwet && ok; n++) {
+    const int cw = g->w3_t[n + 1];
+    hiq[(size_t)g->s2k[cw] * ncol + c->cs2col[g->m2d[cw]]] = n;
+  }
+  for (col = 0; col < ncol && ok; col++)
+    for (int k = c->h_kb[col]; k < nz && ok; k++)
+      if (hiq[(size_t)k * ncol + col] < 0) {
+        emstr_set_error(E_UNSUP, c->col2cs[col], "column of centre %d: layer %d is not wet",
+                        c->col2cs[col], k);
+        ok = 0;
+      }
   /* every wet centre against the model */
   int *lsq_off = (int *)calloc((size_t)d->nwet + 1, sizeof(int));
   size_t nent = 0;
@@ -804,6 +839,11 @@ static int us_ffsl_geom(usctx *c) {
       ok = 0;
       break;
     }
+    if (lw->ncells > 64) { /* the kernel holds a stencil entry a lane */
+      emstr_set_error(E_UNSUP, cw, "centre %d: more than 64 least-squares stencil entries", cw);
+      ok = 0;
+      break;
+    }
     lsq_off[n] = (int)nent;
     nent += lw->ncells;
     if (nent > 0x7fffffff) {
@@ -812,8 +852,20 @@ static int us_ffsl_geom(usctx *c) {
     }
   }
   if (ok) lsq_off[d->nwet] = (int)nent;
-  ll *lidx = ok ? (ll *)malloc((nent ? nent : 1) * sizeof(ll)) : NULL;
+  int *lslot = ok ? (int *)malloc((nent ? nent : 1) * sizeof(int)) : NULL;
   double *lB = ok ? (double *)malloc((nent ? nent : 1) * 2 * sizeof(double)) : NULL;
+  /* the distinct centres each wet centre reads (itself first, then the
+   * stencil's, then the vertex ranges'), by wet-list position, and the
+   * vertex ranges as slots of that list, four bytes per vertex */
+  int *lr_off = ok ? (int *)malloc(((size_t)d->nwet + 1) * sizeof(int)) : NULL;
+  int *lr_q = ok ? (int *)malloc(((size_t)d->nwet * EMSTR_FF_ROWS + 1) * sizeof(int)) : NULL;
+  unsigned *vsl =
+      ok ? (unsigned *)malloc(((size_t)d->nwet * EMSTR_FF_NPE + 1) * sizeof(unsigned)) : NULL;
+  size_t nrow = 0;
+  if (ok && (!lslot || !lB || !lr_off || !lr_q || !vsl || !hvxy)) {
+    emstr_set_error(E_DEVICE, -1, "FFSL stencil tables: host allocation");
+    ok = 0;
+  }
   for (n = 0; n < d->nwet && ok; n++) {
     const int cw = g->w3_t[n + 1], cs = g->m2d[cw], k = g->s2k[cw];
     col = c->cs2col[cs];
@@ -856,6 +908,9 @@ static int us_ffsl_geom(usctx *c) {
     }
     /* stencil */
     const qweights *lw = &wc->lw[cw];
+    int *rw = lr_q + nrow, nr = 1;
+    rw[0] = n;
+    lr_off[n] = (int)nrow;
     for (int i = 0; i < lw->ncells && ok; i++) {
       int cn = lw->cells[i];
       if (cn > 0 && cn < g->szc && gbin[cn]) cn = gbin[cn];
@@ -868,24 +923,56 @@ static int us_ffsl_geom(usctx *c) {
         break;
       }
       const size_t o = (size_t)lsq_off[n] + i;
-      lidx[o] = (ll)ck * ncol + ccol;
+      const int sl = ff_slot(rw, &nr, hiq[(size_t)ck * ncol + ccol]);
+      if (sl < 0) {
+        emstr_set_error(E_UNSUP, cw, "centre %d: more than %d centres in its stencil", cw,
+                        EMSTR_FF_ROWS);
+        ok = 0;
+        break;
+      }
+      lslot[o] = sl;
       lB[2 * o + 0] = lw->B[i][1];
       lB[2 * o + 1] = lw->B[i][2];
     }
+    /* vertex ranges: the wet centres around each vertex in column order */
+    for (j = 0; j < EMSTR_FF_NPE && ok; j++) {
+      unsigned word = 0xffffffffu;
+      if (j < np) {
+        const int vs = hcvt[j * ncol + col];
+        int m = 0;
+        for (int i = 0; i < nvcm; i++) {
+          const int cv = hvcol[vs * nvcm + i];
+          if (cv < 0) break;
+          if (k < c->h_kb[cv]) continue;
+          const int sl = ff_slot(rw, &nr, hiq[(size_t)k * ncol + cv]);
+          if (sl < 0) {
+            emstr_set_error(E_UNSUP, cw, "centre %d: more than %d centres in its stencil", cw,
+                            EMSTR_FF_ROWS);
+            ok = 0;
+            break;
+          }
+          word = (word & ~(0xffu << (8 * m))) | ((unsigned)sl << (8 * m));
+          m++;
+        }
+      }
+      vsl[(size_t)n * EMSTR_FF_NPE + j] = word;
+    }
+    nrow += nr;
   }
+  if (ok) lr_off[d->nwet] = (int)nrow;
   if (ok) {
     D->cx = (double *)dalloc((size_t)ncol * sizeof(double), &ok);
     D->cy = (double *)dalloc((size_t)ncol * sizeof(double), &ok);
     double **ep[] = {&D->ux, &D->uy, &D->n1x, &D->n1y, &D->t2x, &D->t2y};
     for (n = 0; n < 6; n++) *ep[n] = (double *)dalloc((size_t)ne2 * sizeof(double), &ok);
     D->ev = (double *)dalloc((size_t)ne2 * 4 * sizeof(double), &ok);
-    D->cvt = (int *)dalloc((size_t)npem * ncol * sizeof(int), &ok);
+    D->vxy = (double *)dalloc((size_t)ncol * EMSTR_FF_NPE * 2 * sizeof(double), &ok);
     D->eord = (int *)dalloc((size_t)npem * ncol * sizeof(int), &ok);
-    D->vx = (double *)dalloc((size_t)nvl * sizeof(double), &ok);
-    D->vy = (double *)dalloc((size_t)nvl * sizeof(double), &ok);
-    D->vcol = (int *)dalloc((size_t)nvl * nvcm * sizeof(int), &ok);
     D->lsq_off = (int *)dalloc(((size_t)d->nwet + 1) * sizeof(int), &ok);
-    D->lsq_idx = (ll *)dalloc((nent ? nent : 1) * sizeof(ll), &ok);
+    D->lsq_s = (int *)dalloc((nent ? nent : 1) * sizeof(int), &ok);
+    D->lr_off = (int *)dalloc(((size_t)d->nwet + 1) * sizeof(int), &ok);
+    D->lr_q = (int *)dalloc((nrow ? nrow : 1) * sizeof(int), &ok);
+    D->vsl = (unsigned *)dalloc(((size_t)d->nwet * EMSTR_FF_NPE + 1) * sizeof(unsigned), &ok);
     D->lsq_B = (double *)dalloc((nent ? nent : 1) * 2 * sizeof(double), &ok);
     if (!ok) emstr_set_error(E_DEVICE, -1, "FFSL geometry allocation failed");
   }
@@ -896,12 +983,13 @@ static int us_ffsl_geom(usctx *c) {
     int r = up(c, D->cx, hcx, ncol * sizeof(double)) || up(c, D->cy, hcy, ncol * sizeof(double));
     for (n = 0; n < 6; n++) r = r || up(c, *ep[n], he[n], ne2 * sizeof(double));
     r = r || up(c, D->ev, hev, (size_t)ne2 * 4 * sizeof(double)) ||
-        up(c, D->cvt, hcvt, (size_t)npem * ncol * sizeof(int)) ||
+        up(c, D->vxy, hvxy, (size_t)ncol * EMSTR_FF_NPE * 2 * sizeof(double)) ||
         up(c, D->eord, heord, (size_t)npem * ncol * sizeof(int)) ||
-        up(c, D->vx, hvx, nvl * sizeof(double)) || up(c, D->vy, hvy, nvl * sizeof(double)) ||
-        up(c, D->vcol, hvcol, (size_t)nvl * nvcm * sizeof(int)) ||
         up(c, D->lsq_off, lsq_off, ((size_t)d->nwet + 1) * sizeof(int)) ||
-        up(c, D->lsq_idx, lidx, (nent ? nent : 1) * sizeof(ll)) ||
+        up(c, D->lsq_s, lslot, (nent ? nent : 1) * sizeof(int)) ||
+        up(c, D->lr_off, lr_off, ((size_t)d->nwet + 1) * sizeof(int)) ||
+        up(c, D->lr_q, lr_q, (nrow ? nrow : 1) * sizeof(int)) ||
+        up(c, D->vsl, vsl, ((size_t)d->nwet * EMSTR_FF_NPE + 1) * sizeof(unsigned)) ||
         up(c, D->lsq_B, lB, (nent ? nent : 1) * 2 * sizeof(double));
     if (hipStreamSynchronize(c->st) != hipSuccess) r = 1;
     if (r) {
@@ -921,8 +1009,13 @@ static int us_ffsl_geom(usctx *c) {
   free(epos);
   free(gbin);
   free(lsq_off);
-  free(lidx);
+  free(lslot);
   free(lB);
+  free(hiq);
+  free(lr_off);
+  free(lr_q);
+  free(vsl);
+  free(hvxy);
   if (!ok) return E_UNSUP;
   c->ffgeom = 1;
   return 0;
@@ -1003,6 +1096,14 @@ static int us_ffsl_init(usctx *c) {
   return 1;
 }
 
+/* the stamp of a sub-step's tr_moc marks (need[] starts at zero).  A
+ * stamp that comes round again after 2^31 sub-steps can only leave extra
+ * centres marked, which costs work but changes no value read. */
+static int us_ffstamp(usctx *c) {
+  if (++c->ffstamp <= 0) c->ffstamp = 1;
+  return c->ffstamp;
+}
+
 static int us_ffsl_alloc(usctx *c) {
   emstr_dev *d = &c->D.d;
   const size_t N3 = (size_t)d->n3, NE3 = (size_t)c->ne2 * d->nz;
@@ -1013,10 +1114,14 @@ static int us_ffsl_alloc(usctx *c) {
   for (int n = 0; n < 4; n++) *a[n] = (double *)dalloc(N3 * sizeof(double), &ok);
   c->d_clc = (int *)dalloc(N3 * sizeof(int), &ok);
   c->d_clf = (int *)dalloc(N3 * sizeof(int), &ok);
+  c->d_need = (int *)dalloc(N3 * sizeof(int), &ok); /* zeroed; stamps start at 1 */
+  c->ffstamp = 0;
   c->d_lci = (int *)dalloc(NE3 * sizeof(int), &ok);
   c->d_ld = (double *)dalloc(5 * NE3 * sizeof(double), &ok);
-  c->d_tm = (double *)dalloc(nt * N3 * sizeof(double), &ok);
-  c->d_lq = (double *)dalloc(3 * nt * N3 * sizeof(double), &ok);
+  /* tr_mod rows [nwet][tqs], tqs = tracers rounded up to 16 (128 bytes) */
+  c->D.tqs = (int)((nt + 15) / 16 * 16);
+  c->d_tm = (double *)dalloc(((size_t)d->nwet + 1) * c->D.tqs * sizeof(double), &ok);
+  c->d_lq = (double *)dalloc(4 * nt * N3 * sizeof(double), &ok);
   c->d_te = (double *)dalloc(nt * NE3 * sizeof(double), &ok);
   c->d_moc = (double *)dalloc(nt * N3 * sizeof(double), &ok);
   c->d_fferr = (int *)dalloc(sizeof(int), &ok);
@@ -1100,7 +1205,8 @@ static int us_adv(usctx *c) {
             emstr_us_launch_ffsl_flux(D, u, c->tr[in], c->d_nw, c->d_clc, c->d_crc, c->d_clf,
                                       c->d_crf, c->d_cour, c->d_edn, c->nesp, c->d_lci,
                                       c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc, c->d_fx,
-                                      c->d_fz, c->anydif, c->d_fferr, c->st) ||
+                                      c->d_fz, c->anydif, c->d_need, us_ffstamp(c),
+                                      c->d_fferr, c->st) ||
             emstr_us_launch_update(D, c->tr[in], c->tr[out], c->d_fx, c->d_fz, u, c->st) ||
             emstr_launch_osub(d, c->st))
           return emstr_set_error(E_DEVICE, -1, "FFSL launch");

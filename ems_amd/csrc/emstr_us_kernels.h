@@ -32,7 +32,10 @@ int emstr_us_launch_update(const emstr_usdev *D, const double *tin, double *tout
  * the vertical departure layers / fractions of centres (clc, crc) and of
  * the faces of the cells to process (clf, crf), their Courant numbers,
  * and the streamline of every wet edge (lci [nz][ne2], ld [5][nz][ne2]).
- * Per sub-step and tracer: tm, lq [3], te, moc, then fx and fz.  err
+ * Per sub-step and tracer: tm (rows [nwet][tqs]), lq [4][ntr][nz*ncol]
+ * (w1, w2, beta, tr_mod), te, moc, then fx and fz.  need [nz][ncol]
+ * gets stamp (distinct per call) at the centres whose tr_moc the vertical
+ * fluxes read; te and moc are computed for those only.  err
  * gets 1 for a streamline that leaves its cell into a wet cell, 2 for a
  * vertical walk that does not end.                                     */
 int emstr_us_launch_ffsl_step(const emstr_usdev *D, int init, double *nw, void *stream);
@@ -45,7 +48,8 @@ int emstr_us_launch_ffsl_flux(const emstr_usdev *D, double dtu, const double *ti
                               const int *clf, const double *crf, const double *cour,
                               const long long *edn, int nesp, const int *lci, const double *ld,
                               double *tm, double *lq, double *te, double *moc, double *fx,
-                              double *fz, int anydif, int *err, void *stream);
+                              double *fz, int anydif, int *need, int stamp, int *err,
+                              void *stream);
 
 #ifdef __cplusplus
 }

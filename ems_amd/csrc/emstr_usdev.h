@@ -21,6 +21,9 @@
 extern "C" {
 #endif
 
+#define EMSTR_FF_ROWS 16 /* most distinct centres one FFSL reconstruction reads */
+#define EMSTR_FF_NPE 8   /* most vertices (edges) of an FFSL centre          */
+
 typedef struct emstr_usdev {
   emstr_dev d;      /* column part: nx = ncol, ny = 1                   */
   int npem, ne2;
@@ -44,22 +47,28 @@ typedef struct emstr_usdev {
 
   /* FFSL (transport.c:1961-2723): geometry of the surface layer       */
   int ffsl;         /* the window's scheme is FFSL                      */
-  int nvl, nvcm;    /* surface vertices; most columns around one        */
+  int nvl, nvcm;    /* surface vertices; most columns around one (host) */
   double *cx, *cy;  /* [ncol] centre positions                          */
   double *ux, *uy;  /* [ne2] edge midpoints (streamline starts)         */
   double *n1x, *n1y, *t2x, *t2y; /* [ne2] costhu1/sinthu1, costhu2/sinthu2 */
   double *ev;       /* [ne2][4] end vertices x1, y1, x2, y2 (e2v)       */
-  int *cvt;         /* [npem][ncol] vertex j of the column (0..nvl-1)   */
-  double *vx, *vy;  /* [nvl] vertex positions                           */
-  int *vcol;        /* [nvl][nvcm] columns around the vertex, -1 padded */
+  double *vxy;      /* [ncol][EMSTR_FF_NPE][2] centre minus vertex j, x
+                     * and y, of the column's vertices in order          */
   int *eord;        /* [npem][ncol] edge slots in ascending edge order  */
-  /* least-squares stencils of the wet centres, by wet-list position:
-   * entries lsq_off[q]..lsq_off[q+1]-1 read the dense cell lsq_idx (a
+  /* what the limited reconstruction of wet centre q (wet-list position)
+   * reads: the distinct wet centres lr_q[lr_off[q]..lr_off[q+1]-1]
+   * (itself first, at most EMSTR_FF_ROWS); its least-squares stencil,
+   * entries lsq_off[q]..lsq_off[q+1]-1, as slots lsq_s of that list (a
    * ghost reads its wet centre) with the slope weights lsq_B[2*entry],
-   * lsq_B[2*entry + 1] (the constant term is not used)               */
+   * lsq_B[2*entry + 1] (the constant term is not used); and vertex j's
+   * wet centres as the bytes of vsl[q*EMSTR_FF_NPE + j], 0xff ending
+   * the list */
   int *lsq_off;
-  long long *lsq_idx;
+  int *lsq_s;
   double *lsq_B;
+  int *lr_off, *lr_q;
+  unsigned *vsl;
+  int tqs;          /* row stride of the [nwet][tqs] tr_mod rows        */
   /* fields: [nz][ne2] u2; [ncol] wtop, wbot; [nz][ncol] u, v of the wet
    * centres; [nz][ne2] u, v of the ghost across a land edge (gu0/gv0
    * from the window's arrays, gue/gve rotated from the edge velocity) */

@@ -197,6 +197,25 @@ def test_hex_ffsl_step(name, kw, stab, trsf, nsteps):
     assert_exact(gpu, ora)
 
 
+def test_hex_ffsl_vertical_courant_above_one():
+    """A vertical velocity 300 times that of continuity: vertical Courant
+    numbers up to 3.5, so the vertical fluxes read tr_moc (the mean
+    streamline end value) in the fractional part and over whole layers,
+    and the GPU computes tr_moc only where it is read."""
+    say("case hex ffsl, w x 300")
+    case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=3, seed=17, trasc=U.FFSL, dt=300.0)
+    case.wc.trsf = 0.8
+    case.prepare(OU.set_dz)
+    w = case.ac(case.wd.w)
+    w *= 300.0
+    wet = case.wet
+    cr = np.abs(w[wet]) * case.dt / np.maximum(case.ac(case.wc.dz)[wet], 1e-9)
+    assert (cr > 1.0).sum() >= 10
+    gpu, ora = run_both(case, 2)
+    assert np.isfinite(ora).all()
+    assert_exact(gpu, ora)
+
+
 def test_hex_ffsl_moving_surface_diffusion():
     say("case hex ffsl moving surface + diffusion")
     case = MS.hex_basin(nx=20, ny=16, nz=6, ntr=3, seed=19, dmin=2.0, dmax=30.0,
