@@ -282,21 +282,26 @@ __global__ void k_colB(emstr_dev d, emstr_batch b) {
 /* the fused step                                                    */
 /*-------------------------------------------------------------------*/
 /* van Leer face value (tracers.c:2097-2118) for transport q through the
- * face between bm1 and c; f is 1 - cn for q > 0 and 1 + cn for q < 0 (the
- * prep folds the Courant number and the limiter's factor 2 into it).  The
- * limiter's division uses the hardware reciprocal (about 1 ulp). */
-__device__ __forceinline__ float vlf(float bm2, float bm1, float c, float fp1,
-                                     float q, float f) {
+ * face between bm1 and c, returned as its anomaly F - ref against a
+ * reference value close to it; f is 1 - cn for q > 0 and 1 + cn for
+ * q < 0 (the prep folds the Courant number and the limiter's factor 2
+ * into it).  The upwind value's anomaly is an exact FP32 difference for
+ * neighbouring values within a factor of two, so the only rounding left
+ * is relative to the small limited term and to the anomaly itself, not to
+ * the tracer value.  The limiter's division uses the hardware reciprocal
+ * (about 1 ulp). */
+__device__ __forceinline__ float vla(float bm2, float bm1, float c, float fp1,
+                                     float q, float f, float ref) {
   if (q > 0.0f) {
     const float df = (bm1 - bm2) * (c - bm1);
     const float t = df * __builtin_amdgcn_rcpf(c - bm2);
-    return (df > 0.0f) ? bm1 + f * t : bm1;
+    return (df > 0.0f) ? (bm1 - ref) + f * t : bm1 - ref;
   } else if (q < 0.0f) {
     const float df = (c - bm1) * (fp1 - c);
     const float t = df * __builtin_amdgcn_rcpf(fp1 - bm1);
-    return (df > 0.0f) ? c - f * t : c;
+    return (df > 0.0f) ? (c - ref) - f * t : c - ref;
   }
-  return c; /* no transport: F - c = 0 */
+  return c - ref; /* no transport: its flux term vanishes */
 }
 
 /*-------------------------------------------------------------------*/
@@ -395,11 +400,11 @@ __device__ __forceinline__ void load_h(HVals &h, const Nbr &nb, int k, uint32_t 
 __device__ __forceinline__ float hflux(const HVals &h, bool ew, bool nw, float v0, float ref,
                                        float qW, float fW, float qE, float fE, float qS,
                                        float fS, float qN, float fN) {
-  const float FW = vlf(h.w2, h.w1, v0, h.e1, qW, fW);
-  const float FE = ew ? vlf(h.w1, v0, h.e1, h.e2, qE, fE) : ref;
-  const float FS = vlf(h.s2, h.s1, v0, h.n1, qS, fS);
-  const float FN = nw ? vlf(h.s1, v0, h.n1, h.n2, qN, fN) : ref;
-  return (FE - ref) * qE - (FW - ref) * qW + (FN - ref) * qN - (FS - ref) * qS;
+  const float AW = vla(h.w2, h.w1, v0, h.e1, qW, fW, ref);
+  const float AE = ew ? vla(h.w1, v0, h.e1, h.e2, qE, fE, ref) : 0.0f;
+  const float AS = vla(h.s2, h.s1, v0, h.n1, qS, fS, ref);
+  const float AN = nw ? vla(h.s1, v0, h.n1, h.n2, qN, fN, ref) : 0.0f;
+  return AE * qE - AW * qW + AN * qN - AS * qS;
 }
 
 /*-------------------------------------------------------------------*/
@@ -539,8 +544,7 @@ k_advB(const AArgs a, const float *__restrict__ tin, float *__restrict__ tout,
     char *oc = (char *)tout + col * colb;
     const float v0 = v.v0;
     const float qB = rdl(cw, C_QB);
-    const float FB = vlf(v.zm2, v.zm1, v0, v.zp1, qB, rdl(cw, C_CB));
-    const float fba = (FB - v0) * qB;
+    const float fba = vla(v.zm2, v.zm1, v0, v.zp1, qB, rdl(cw, C_CB), v0) * qB;
     HVals h;
     h.w1 = v.w1;
     h.w2 = v.w2;
@@ -558,8 +562,7 @@ k_advB(const AArgs a, const float *__restrict__ tin, float *__restrict__ tout,
       /* sub-surface update (tracers.c:912-925) in anomaly form; the top
        * face is the bottom face of the cell above */
       const float qT = rdl(cw, C_N + C_QB);
-      const float FT = vlf(v.zm1, v0, v.zp1, v.zp2, qT, rdl(cw, C_N + C_CB));
-      const float ht = hh + (FT - v0) * qT - fba;
+      const float ht = hh + vla(v.zm1, v0, v.zp1, v.zp2, qT, rdl(cw, C_N + C_CB), v0) * qT - fba;
       const double an = (double)v0 - (double)((ht + v0 * rdl(cw, C_D)) * rdl(cw, C_RV));
       float bk = sround(an, key);
       /* the thin-layer repair of tr_bounds needs the unclipped value */
