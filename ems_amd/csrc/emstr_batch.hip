@@ -466,6 +466,106 @@ __device__ __forceinline__ void st13_load(St13 &v, int &cw, const char *tin,
   cw = __float_as_int(co[(col * nz + k) * C_N + (lane & 31)]);
 }
 
+/* The update of one wet cell k <= ks of column col for one tracer lane,
+ * from its 13-point stencil v and coefficient words cw (lanes 0..15: cell
+ * k, 16..31: cell k + 1); m: open neighbours at layer k (M_*).  Writes the
+ * advected value b of a sub-surface cell, or the raw new surface value
+ * (surf_conc) of the processed surface cell, to tout. */
+__device__ __forceinline__ void adv_cell(const St13 &v, int cw, ll col, int k, int m, int ks,
+                                         int flags, const char *tinb, float *tout,
+                                         const ColInfo *__restrict__ cinfo, const float *co,
+                                         const double *__restrict__ layers, float *scr,
+                                         uint32_t lb, uint32_t rowb, ll colb, ll jb, int nz,
+                                         uint32_t hl, float mn, bool clip, bool act,
+                                         double hmin) {
+  const int top = nz - 1;
+  const uint32_t o = lb + k * rowb;
+  char *oc = (char *)tout + col * colb;
+  const float v0 = v.v0;
+  const float qB = rdl(cw, C_QB);
+  const float fba = vla(v.zm2, v.zm1, v0, v.zp1, qB, rdl(cw, C_CB), v0) * qB;
+  HVals h;
+  h.w1 = v.w1;
+  h.w2 = v.w2;
+  h.e1 = v.e1;
+  h.e2 = v.e2;
+  h.s1 = v.s1;
+  h.s2 = v.s2;
+  h.n1 = v.n1;
+  h.n2 = v.n2;
+  const float hh = hflux(h, (m & M_E1) != 0, (m & M_N1) != 0, v0, v0, rdl(cw, C_QW),
+                         rdl(cw, C_CW), rdl(cw, C_QE), rdl(cw, C_CE), rdl(cw, C_QS),
+                         rdl(cw, C_CS), rdl(cw, C_QN), rdl(cw, C_CN));
+  const uint32_t key = srkey((uint32_t)(col * nz) + (uint32_t)k, hl);
+  if (k < ks) {
+    /* sub-surface update (tracers.c:912-925) in anomaly form; the top
+     * face is the bottom face of the cell above */
+    const float qT = rdl(cw, C_N + C_QB);
+    const float ht = hh + vla(v.zm1, v0, v.zp1, v.zp2, qT, rdl(cw, C_N + C_CB), v0) * qT - fba;
+    const double an = (double)v0 - (double)((ht + v0 * rdl(cw, C_D)) * rdl(cw, C_RV));
+    float bk = sround(an, key);
+    /* the thin-layer repair of tr_bounds needs the unclipped value */
+    if (k == ks - 1 && (flags & F_THIN) && act) stb((char *)scr + col * colb, o, bk);
+    if (clip && bk < mn) bk = mn;
+    if (act) stb(oc, o, bk);
+  } else {
+    /* processed surface cell: surf_conc (tracers.c:3047-3207) as an
+     * FP64 volume balance, v0 * Vc + anomaly terms, Vc in FP64 */
+    const ColInfo ci = cinfo[col];
+    const char *tc = tinb + col * colb;
+    double m2 = (double)fba - (double)hh;
+    if (flags & F_MOVE) {
+      const int kb = ci.kb;
+      int cc = ks;
+      double wb = (ks == kb) ? ci.botz : layers[ks];
+      while (cc != ci.kold && cc < top) {
+        const double vol = ci.area * (layers[cc + 1] - wb);
+        if (cc > ks) m2 += ((double)ldb(tc, lb + cc * rowb) - (double)v0) * vol;
+        cc++;
+        wb = layers[cc];
+      }
+      if (cc > ks)
+        m2 += ((double)ldb(tc, lb + cc * rowb) - (double)v0) *
+              (ci.area * (ci.osub - ((cc == kb) ? ci.botz : layers[cc])));
+      Nbr nb;
+      nb.c = tc;
+      nb.w1 = tc - colb;
+      nb.w2 = tc - 2 * colb;
+      nb.e1 = tc + colb;
+      nb.e2 = tc + 2 * colb;
+      nb.s1 = tc - jb;
+      nb.s2 = tc - 2 * jb;
+      nb.n1 = tc + jb;
+      nb.n2 = tc + 2 * jb;
+      nb.kw1 = ci.kw1;
+      nb.kw2 = ci.kw2;
+      nb.ke1 = ci.ke1;
+      nb.ke2 = ci.ke2;
+      nb.ks1 = ci.ks1;
+      nb.ks2 = ci.ks2;
+      nb.kn1 = ci.kn1;
+      nb.kn2 = ci.kn2;
+      const float *cob = co + col * nz * C_N;
+      for (int ka = ks + 1; ka <= top; ka++) {
+        HVals ha;
+        load_h(ha, nb, ka, lb + ka * rowb);
+        const float *ca = cob + ka * C_N;
+        m2 -= (double)hflux(ha, ka >= ci.ke1, ka >= ci.kn1, ldb(tc, lb + ka * rowb), v0,
+                            ca[C_QW], ca[C_CW], ca[C_QE], ca[C_CE], ca[C_QS], ca[C_CS],
+                            ca[C_QN], ca[C_CN]);
+      }
+    }
+    double surftr = (double)v0 * ci.vc + m2;
+    if (ci.svol == 0.0 || (surftr < 0.0 && ci.dzs <= hmin))
+      surftr = 0.0;
+    else
+      surftr /= ci.svol;
+    /* tracers.c:953-954; tr_bounds is applied in kernel B */
+    const float trs = (flags & F_KEEPOLD) ? v0 : sround(surftr, key);
+    if (act) stb(oc, o, trs);
+  }
+}
+
 /*
  * Kernel A: every wet cell of every column (all tracers of one cell per
  * wave, lanes = tracers), cells taken in layer-synchronous tile order so
@@ -484,7 +584,7 @@ k_advB(const AArgs a, const float *__restrict__ tin, float *__restrict__ tout,
        const int *__restrict__ kfl, const ColInfo *__restrict__ cinfo,
        const float *__restrict__ mintr, const double *__restrict__ layers,
        float *__restrict__ scr, const int *__restrict__ stepctr) {
-  const int nz = a.nz, NT = a.ntr, top = nz - 1, nact = a.nact;
+  const int nz = a.nz, NT = a.ntr, nact = a.nact;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = min(lane, nact - 1); /* inactive lanes shadow the last tracer */
@@ -510,121 +610,41 @@ k_advB(const AArgs a, const float *__restrict__ tin, float *__restrict__ tout,
     dnxt = myc[64 + lane];
     knxt = myk[64 + lane];
   }
-  St13 vn;
-  int cwn = 0;
-  {
-    const unsigned long long d0 = rdl64(dcur, 0);
-    if (d0 != NO_CELL) st13_load(vn, cwn, tinb, co, d0, colb, jb, lb, rowb, nz, lane);
-  }
-  for (int j = 0; j < niter; j++) {
-    const int jl = j & 63;
-    const unsigned long long d = rdl64(dcur, jl);
-    const int kf = __builtin_amdgcn_readlane(kcur, jl);
-    const St13 v = vn;
-    const int cw = cwn;
-    /* the next cell's loads */
-    const unsigned long long dn = (jl < 63) ? rdl64(dcur, jl + 1) : rdl64(dnxt, 0);
-    if (j + 1 < niter && dn != NO_CELL)
-      st13_load(vn, cwn, tinb, co, dn, colb, jb, lb, rowb, nz, lane);
-    if (jl == 63) {
-      dcur = dnxt;
-      kcur = knxt;
-      if (j + 65 < niter) {
-        dnxt = myc[j + 65 + lane];
-        knxt = myk[j + 65 + lane];
-      }
-    }
-    if (d == NO_CELL) continue;
+  /* Two cells per iteration with fixed stencil registers: the loads of
+   * one cell are in flight while the other is computed.  Padding entries
+   * load the rows of a stand-in cell (column 0, layer 0, no open
+   * neighbours: always in bounds) so that every load is issued
+   * unconditionally and is never computed. */
+  auto live = [](unsigned long long d) -> unsigned long long { return d == NO_CELL ? 0ULL : d; };
+  auto one = [&](const St13 &v, int cw, unsigned long long d, int kf) {
+    if (d == NO_CELL) return;
     const int k = (int)((d >> 32) & 255);
     const int ks = (kf & 255) - 1, flags = kf >> 8;
-    if (k > ks) continue; /* above the processed surface, or not processed */
-    const ll col = (ll)(uint32_t)d;
-    const int m = (int)((d >> 40) & 255);
-    const uint32_t o = lb + k * rowb;
-    char *oc = (char *)tout + col * colb;
-    const float v0 = v.v0;
-    const float qB = rdl(cw, C_QB);
-    const float fba = vla(v.zm2, v.zm1, v0, v.zp1, qB, rdl(cw, C_CB), v0) * qB;
-    HVals h;
-    h.w1 = v.w1;
-    h.w2 = v.w2;
-    h.e1 = v.e1;
-    h.e2 = v.e2;
-    h.s1 = v.s1;
-    h.s2 = v.s2;
-    h.n1 = v.n1;
-    h.n2 = v.n2;
-    const float hh = hflux(h, (m & M_E1) != 0, (m & M_N1) != 0, v0, v0, rdl(cw, C_QW),
-                           rdl(cw, C_CW), rdl(cw, C_QE), rdl(cw, C_CE), rdl(cw, C_QS),
-                           rdl(cw, C_CS), rdl(cw, C_QN), rdl(cw, C_CN));
-    const uint32_t key = srkey((uint32_t)(col * nz) + (uint32_t)k, hl);
-    if (k < ks) {
-      /* sub-surface update (tracers.c:912-925) in anomaly form; the top
-       * face is the bottom face of the cell above */
-      const float qT = rdl(cw, C_N + C_QB);
-      const float ht = hh + vla(v.zm1, v0, v.zp1, v.zp2, qT, rdl(cw, C_N + C_CB), v0) * qT - fba;
-      const double an = (double)v0 - (double)((ht + v0 * rdl(cw, C_D)) * rdl(cw, C_RV));
-      float bk = sround(an, key);
-      /* the thin-layer repair of tr_bounds needs the unclipped value */
-      if (k == ks - 1 && (flags & F_THIN) && act) stb((char *)scr + col * colb, o, bk);
-      if (clip && bk < mn) bk = mn;
-      if (act) stb(oc, o, bk);
-    } else {
-      /* processed surface cell: surf_conc (tracers.c:3047-3207) as an
-       * FP64 volume balance, v0 * Vc + anomaly terms, Vc in FP64 */
-      const ColInfo ci = cinfo[col];
-      const char *tc = tinb + col * colb;
-      double m2 = (double)fba - (double)hh;
-      if (flags & F_MOVE) {
-        const int kb = ci.kb;
-        int cc = ks;
-        double wb = (ks == kb) ? ci.botz : layers[ks];
-        while (cc != ci.kold && cc < top) {
-          const double vol = ci.area * (layers[cc + 1] - wb);
-          if (cc > ks) m2 += ((double)ldb(tc, lb + cc * rowb) - (double)v0) * vol;
-          cc++;
-          wb = layers[cc];
-        }
-        if (cc > ks)
-          m2 += ((double)ldb(tc, lb + cc * rowb) - (double)v0) *
-                (ci.area * (ci.osub - ((cc == kb) ? ci.botz : layers[cc])));
-        Nbr nb;
-        nb.c = tc;
-        nb.w1 = tc - colb;
-        nb.w2 = tc - 2 * colb;
-        nb.e1 = tc + colb;
-        nb.e2 = tc + 2 * colb;
-        nb.s1 = tc - jb;
-        nb.s2 = tc - 2 * jb;
-        nb.n1 = tc + jb;
-        nb.n2 = tc + 2 * jb;
-        nb.kw1 = ci.kw1;
-        nb.kw2 = ci.kw2;
-        nb.ke1 = ci.ke1;
-        nb.ke2 = ci.ke2;
-        nb.ks1 = ci.ks1;
-        nb.ks2 = ci.ks2;
-        nb.kn1 = ci.kn1;
-        nb.kn2 = ci.kn2;
-        const float *cob = co + col * nz * C_N;
-        for (int ka = ks + 1; ka <= top; ka++) {
-          HVals ha;
-          load_h(ha, nb, ka, lb + ka * rowb);
-          const float *ca = cob + ka * C_N;
-          m2 -= (double)hflux(ha, ka >= ci.ke1, ka >= ci.kn1, ldb(tc, lb + ka * rowb), v0,
-                              ca[C_QW], ca[C_CW], ca[C_QE], ca[C_CE], ca[C_QS], ca[C_CS],
-                              ca[C_QN], ca[C_CN]);
-        }
+    if (k > ks) return; /* above the processed surface, or not processed */
+    adv_cell(v, cw, (ll)(uint32_t)d, k, (int)((d >> 40) & 255), ks, flags, tinb, tout, cinfo,
+             co, layers, scr, lb, rowb, colb, jb, nz, hl, mn, clip, act, a.hmin);
+  };
+  St13 va, vb;
+  int cwa = 0, cwb = 0;
+  st13_load(va, cwa, tinb, co, live(rdl64(dcur, 0)), colb, jb, lb, rowb, nz, lane);
+  for (int j = 0; j < niter; j += 2) { /* niter is a multiple of 64 */
+    const int jl = j & 63;
+    const unsigned long long dA = rdl64(dcur, jl), dB = rdl64(dcur, jl + 1);
+    const int kfA = __builtin_amdgcn_readlane(kcur, jl);
+    const int kfB = __builtin_amdgcn_readlane(kcur, jl + 1);
+    st13_load(vb, cwb, tinb, co, live(dB), colb, jb, lb, rowb, nz, lane);
+    one(va, cwa, dA, kfA);
+    if (jl == 62) {
+      dcur = dnxt;
+      kcur = knxt;
+      if (j + 66 < niter) {
+        dnxt = myc[j + 66 + lane];
+        knxt = myk[j + 66 + lane];
       }
-      double surftr = (double)v0 * ci.vc + m2;
-      if (ci.svol == 0.0 || (surftr < 0.0 && ci.dzs <= a.hmin))
-        surftr = 0.0;
-      else
-        surftr /= ci.svol;
-      /* tracers.c:953-954; tr_bounds is applied in kernel B */
-      const float trs = (flags & F_KEEPOLD) ? v0 : sround(surftr, key);
-      if (act) stb(oc, o, trs);
     }
+    const unsigned long long dN = (j + 2 < niter) ? rdl64(dcur, (jl + 2) & 63) : NO_CELL;
+    st13_load(va, cwa, tinb, co, live(dN), colb, jb, lb, rowb, nz, lane);
+    one(vb, cwb, dB, kfB);
   }
 }
 
