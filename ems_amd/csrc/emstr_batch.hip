@@ -220,6 +220,7 @@ __global__ void k_colB(emstr_dev d, emstr_batch b) {
     vol += area * (c.osub - ((cc == kbc) ? c.botz : d.layers[cc]));
     double out = 0.0;
     bool above = false; /* transport through a face above the processed cell */
+#pragma unroll 4
     for (int k = ksc; k <= top; k++) {
       const ll idx = (ll)k * P + col;
       const bool we = k >= d.kb[col + 1], wn = k >= d.kb[col + nx];
@@ -244,10 +245,15 @@ __global__ void k_colB(emstr_dev d, emstr_batch b) {
       flags |= F_SINGLE;
       c.cb = cs;
     } else {
-      int q = cs;
-      while (q >= kbc && d.dz[(ll)q * P + col] != 0.0) q--;
-      c.cb = q + 1;
+      /* one above the highest zero-thickness layer at or below cs (the
+       * run of open layers ending at cs); loads independent of each other */
+      int zq = kbc - 1;
+#pragma unroll 8
+      for (int k = kbc; k <= cs; k++)
+        if (d.dz[(ll)k * P + col] == 0.0) zq = k;
+      c.cb = zq + 1;
       if (c.cb >= cs) flags |= F_SINGLE;
+#pragma unroll 4
       for (int k = c.cb + 1; k <= cs; k++) {
         const double dzc = d.dz[(ll)k * P + col];
         const double dzm = d.dz[(ll)(k - 1) * P + col];
@@ -258,6 +264,7 @@ __global__ void k_colB(emstr_dev d, emstr_batch b) {
        * stored as ud and 1/div */
       if (!(flags & F_SINGLE)) {
         float divp = 1.0f, gpp = 0.0f;
+#pragma unroll 4
         for (int k = c.cb; k <= cs; k++) {
           float *cf = b.co + cidx(col, d.nz, k) * C_N;
           const float gm = (k > c.cb) ? cf[C_GM] : 0.0f, gp = cf[C_GP];
