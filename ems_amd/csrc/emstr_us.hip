@@ -244,12 +244,22 @@ __device__ __forceinline__ double us_fz(const emstr_dev &d, int col, int k, int 
   return F * (w * dtu);
 }
 
-/* one-stage Runge-Kutta (tracers.c:1170-1214, rkweights column 0):
+/* Runge-Kutta stage weights (rkweights of tracers.c:110-129 for one
+ * stage): of the sub-step's starting value, of the stage's value, of its
+ * tendency and volume */
+struct UsRk {
+  double w0, w1, w2;
+  int last; /* the sub-step's last stage: tr_bounds follows */
+};
+
+/* runge_kutta (tracers.c:1170-1214) of one stage from the sub-step's
+ * starting value ntr0 and the stage's value ntrc and tendency;
  * Ds = Hn1 = 1 at z-levels */
-__device__ __forceinline__ double us_rk1(double ovol, double nvol, double ntr0, double grad) {
+__device__ __forceinline__ double us_rk(const UsRk &w, double ovol, double nvol, double ntr0,
+                                        double ntrc, double grad) {
   const double sf = 1.0;
-  const double vol = ovol + 1.0 * (nvol - ovol);
-  double tr = sf * 1.0 * ntr0 + sf * 0.0 * ntr0 + 1.0 * grad;
+  const double vol = ovol + w.w2 * (nvol - ovol);
+  double tr = sf * w.w0 * ntr0 + sf * w.w1 * ntrc + w.w2 * grad;
   tr /= (1.0 * vol);
   return tr;
 }
@@ -277,8 +287,9 @@ k_usfz(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ wdn
 }
 
 /* new value of the sub-surface cell at layer k (tracers.c:1000-1016) */
-__device__ double us_subsurface(const emstr_usdev &D, int col, int k, const double *t,
-                                const double *fxn, const double *fzn, double dtu, int adv) {
+__device__ double us_subsurface(const emstr_usdev &D, const UsRk &w, int col, int k,
+                                const double *t0, const double *t, const double *fxn,
+                                const double *fzn, double dtu, int adv) {
   const emstr_dev &d = D.d;
   const ll idx = (ll)k * d.plane + col;
   const double dtr = us_dtracer(D, col, k, fxn, dtu);
@@ -288,16 +299,17 @@ __device__ double us_subsurface(const emstr_usdev &D, int col, int k, const doub
     FzT = fzn[idx + d.plane];
   }
   const double grad = -((dtr / d.cellarea[col] + (FzT - FzB)) / d.dz[idx]);
-  return us_rk1(1.0, 1.0, t[idx], grad);
+  return us_rk(w, 1.0, 1.0, t0[idx], t[idx], grad);
 }
 
 #define US_MAXNPE 8
 
 template <int SCH>
 __global__ void __launch_bounds__(256)
-k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout,
-        const double *__restrict__ fx, const double *__restrict__ fz,
-        const ll *__restrict__ wdn, int nwet, double dtu) {
+k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict__ tin,
+        double *__restrict__ tout, const double *__restrict__ fx,
+        const double *__restrict__ fz, const ll *__restrict__ wdn, int nwet, double dtu,
+        UsRk w) {
   const emstr_dev &d = D.d;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nwet) return;
@@ -348,9 +360,9 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
         FzT = fz[n * N3 + idx + P];
       }
       const double grad = -((dtr / area + (FzT - FzB)) / dzc);
-      double r = us_rk1(1.0, 1.0, t[idx], grad);
-      /* tr_bounds clip (tracers.c:3328-3331) */
-      if (d.mintr[n] <= 1e-5 && r < d.mintr[n]) r = d.mintr[n];
+      double r = us_rk(w, 1.0, 1.0, t0in[n * N3 + idx], t[idx], grad);
+      /* tr_bounds clip (tracers.c:3328-3331), after the last stage */
+      if (w.last && d.mintr[n] <= 1e-5 && r < d.mintr[n]) r = d.mintr[n];
       tout[n * N3 + idx] = r;
     }
     return;
@@ -365,7 +377,7 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
   const double *lay = d.layers;
   const int ktop = d.nz - 1;
   for (int n = 0; n < d.ntr; n++) {
-    const double *t = tin + n * N3;
+    const double *t = tin + n * N3, *t0 = t0in + n * N3;
     const double *fxn = fx + n * NE3;
     double *o = tout + n * N3;
     if (!d.in_tbdy[n]) {
@@ -377,20 +389,22 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
     if (!adv) {
       trs = t[idx];
     } else {
-      /* surf_concrk (tracers.c:3085-3260) and runge_kutta */
+      /* surf_concrk (tracers.c:3085-3260) and runge_kutta; the mass at the
+       * start of the sub-step (stage 0's surf_concrk) from its buffer */
       const double FzB = fz[n * N3 + idx];
       const double top = -FzB * area;
       int c = k;
       double waterbot = (c == kbc) ? botz : lay[c];
       const double dzs = subeta - waterbot;
       const double nvol = dzs * area;
-      double surftr = 0.0, ovol = 0.0, vol, watertop;
+      double surftr = 0.0, surftr0 = 0.0, ovol = 0.0, vol, watertop;
       double grad = -top;
       while (c != kold && c < ktop) {
         watertop = lay[c + 1];
         vol = (watertop - waterbot) * area;
         ovol += vol;
         surftr += (t[(ll)c * P + col] * vol);
+        surftr0 += (t0[(ll)c * P + col] * vol);
         grad -= us_dtracer(D, col, c, fxn, dtu);
         c = c + 1;
         waterbot = lay[c];
@@ -400,6 +414,7 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
       vol = (watertop - waterbot) * area;
       ovol += vol;
       surftr += t[(ll)c * P + col] * vol;
+      surftr0 += t0[(ll)c * P + col] * vol;
       grad -= us_dtracer(D, col, c, fxn, dtu);
       if (c != ktop) {
         do {
@@ -408,16 +423,23 @@ k_usadv(emstr_usdev D, const double *__restrict__ tin, double *__restrict__ tout
         } while (c != ktop);
       }
       if (nvol == 0.0 || (surftr < 0.0 && dzs <= d.hmin)) surftr = 0.0;
-      const double r = us_rk1(ovol, nvol, surftr, grad);
+      if (nvol == 0.0 || (surftr0 < 0.0 && dzs <= d.hmin)) surftr0 = 0.0;
+      const double r = us_rk(w, ovol, nvol, surftr0, surftr, grad);
       /* tracers.c:1056-1057 */
       trs = (botz > 0.0 && d.dz[idx] < d.hmin) ? t[idx] : r;
+    }
+    if (!w.last) {
+      /* an intermediate stage: the layers above keep their values */
+      o[idx] = trs;
+      for (int kk = k + 1; kk <= ktop; kk++) o[(ll)kk * P + col] = t[(ll)kk * P + col];
+      continue;
     }
     /* tr_bounds, thin surface layer (tracers.c:3303-3321): the old
      * surface thickness dz[co] */
     const double dzco = (kold >= kbc) ? d.dz[(ll)kold * P + col] : 0.0;
     if (k > kbc && dzco < d.hmin) {
       if (trs < d.mintr[n] || trs > d.maxtr[n])
-        trs = us_subsurface(D, col, k - 1, t, fxn, fz + n * N3, dtu, adv);
+        trs = us_subsurface(D, w, col, k - 1, t0, t, fxn, fz + n * N3, dtu, adv);
     }
     /* clip (tracers.c:3328-3331) */
     if (d.mintr[n] <= 1e-5 && trs < d.mintr[n]) trs = d.mintr[n];
@@ -488,34 +510,38 @@ __global__ void k_usdtm(emstr_usdev D, double dtu0, unsigned long long *out, int
 }
 
 template <int SCH>
-static void us_adv_launch(const emstr_usdev *D, const double *tin, double *tout,
-                          const double *tstart, const long long *edn, int nesp, double *fx,
-                          double *fz, double dtu, int anydif, int ult, hipStream_t st) {
+static void us_adv_launch(const emstr_usdev *D, const double *t0, const double *tin,
+                          double *tout, const double *tstart, const long long *edn, int nesp,
+                          double *fx, double *fz, double dtu, int anydif, int ult,
+                          const UsRk &w, hipStream_t st) {
   const emstr_dev *d = &D->d;
   if (nesp)
     k_usflux<SCH><<<(nesp + 255) / 256, 256, 0, st>>>(*D, tin, tstart, edn, nesp, fx, dtu,
                                                        anydif, ult);
   if (d->nwet) {
     k_usfz<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, tin, d->wdn, d->nwet, fz, dtu, ult);
-    k_usadv<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, tin, tout, fx, fz, d->wdn,
-                                                         d->nwet, dtu);
+    k_usadv<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, t0, tin, tout, fx, fz, d->wdn,
+                                                         d->nwet, dtu, w);
   }
 }
 
-extern "C" int emstr_us_launch_adv(const emstr_usdev *D, const double *tin, double *tout,
-                                   const double *tstart, const long long *edn, int nesp,
-                                   double *fx, double *fz, double dtu, int anydif, int ult,
-                                   void *stream) {
+extern "C" int emstr_us_launch_adv(const emstr_usdev *D, const double *t0, const double *tin,
+                                   double *tout, const double *tstart, const long long *edn,
+                                   int nesp, double *fx, double *fz, double dtu, int anydif,
+                                   int ult, const double *rkw, int last, void *stream) {
   hipStream_t st = (hipStream_t)stream;
+  const UsRk w = {rkw[0], rkw[1], rkw[2], last};
   switch (D->d.trasc) {
     case VANLEER:
-      us_adv_launch<VANLEER>(D, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, ult, st);
+      us_adv_launch<VANLEER>(D, t0, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, ult, w,
+                             st);
       break;
     case ORDER1:
-      us_adv_launch<ORDER1>(D, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, 0, st);
+      us_adv_launch<ORDER1>(D, t0, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, 0, w, st);
       break;
     case QUICKEST:
-      us_adv_launch<QUICKEST>(D, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, ult, st);
+      us_adv_launch<QUICKEST>(D, t0, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, ult, w,
+                              st);
       break;
     default:
       return -1;
@@ -534,12 +560,14 @@ extern "C" int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned l
 }
 
 /* the cell update from given edge and vertical fluxes (FFSL) */
-extern "C" int emstr_us_launch_update(const emstr_usdev *D, const double *tin, double *tout,
-                                      const double *fx, const double *fz, double dtu,
-                                      void *stream) {
+extern "C" int emstr_us_launch_update(const emstr_usdev *D, const double *t0,
+                                      const double *tin, double *tout, const double *fx,
+                                      const double *fz, double dtu, const double *rkw,
+                                      int last, void *stream) {
   const emstr_dev *d = &D->d;
+  const UsRk w = {rkw[0], rkw[1], rkw[2], last};
   if (d->nwet)
     k_usadv<VANLEER><<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
-        *D, tin, tout, fx, fz, d->wdn, d->nwet, dtu);
+        *D, t0, tin, tout, fx, fz, d->wdn, d->nwet, dtu, w);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

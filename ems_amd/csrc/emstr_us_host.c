@@ -60,7 +60,7 @@ typedef struct emstr_us_ctx {
   int *sedg;     /* [ncol] sediment ghost centre            */
   int *sp_of;    /* [nz*ncol] dense cell -> sparse centre   */
   /* device */
-  double *tr[3];
+  double *tr[4]; /* the fourth only with Runge-Kutta stages */
   int cur;
   double *stage; /* [max(szc, sze)] */
   int *d_esp;    /* wet edges: sparse index and dense index */
@@ -108,7 +108,7 @@ static void us_destroy(usctx *c) {
                 d->osubeta, d->subeta, d->layers, d->w, d->Kz, d->dz, d->mintr, d->maxtr,
                 d->advect, d->diffuse, d->in_tbdy, d->in_tdif, d->cols, d->wsp, d->wdn,
                 D->npe, D->ce, D->esc, D->ec1, D->ec2, D->ecp, D->ecm, D->esel, D->h1e,
-                D->h2e, D->qd3, D->qhm, D->qhp, D->u1, D->flux, D->dzu, D->kh, c->tr[0], c->tr[1], c->tr[2],
+                D->h2e, D->qd3, D->qhm, D->qhp, D->u1, D->flux, D->dzu, D->kh, c->tr[0], c->tr[1], c->tr[2], c->tr[3],
                 c->stage, c->d_esp, c->d_edn, c->d_bpt, c->d_bin, c->d_gsed, c->d_ised,
                 c->d_proc, c->d_err, c->d_dtm, c->d_fx, c->d_fz, D->cx, D->cy, D->ux,
                 D->uy, D->n1x, D->n1y, D->t2x, D->t2y, D->ev, D->vxy, D->eord, D->lsq_off, D->lsq_s, D->lr_off, D->lr_q, D->vsl, D->lsq_B, D->u2, D->wtop,
@@ -131,8 +131,10 @@ static int us_check_scope(geometry_t *g, window_t *wd, win_priv_t *wc) {
   if (wc->sigma) return emstr_set_error(E_UNSUP, -1, "sigma coordinates are not supported");
   if (wc->thin_merge) return emstr_set_error(E_UNSUP, -1, "thin-layer merging is not supported");
   if (wc->means) return emstr_set_error(E_UNSUP, -1, "mean-velocity transport is not supported");
-  if (wc->rkstage != 1)
-    return emstr_set_error(E_UNSUP, -1, "Runge-Kutta with %d stages is not supported",
+  if (wc->rkstage < 1 || wc->rkstage > 3)
+    return emstr_set_error(E_UNSUP, -1,
+                           "Runge-Kutta with %d stages is not supported (1 to 3 stages; the "
+                           "reference's weights for 4 stages are zero)",
                            wc->rkstage);
   if (wc->trasc != VANLEER && wc->trasc != ORDER1 && wc->trasc != QUICKEST &&
       wc->trasc != FFSL)
@@ -1138,6 +1140,22 @@ static int us_ffsl_alloc(usctx *c) {
 /*-------------------------------------------------------------------*/
 /* advect_diffuse (hd-us tracers.c:428-1157); 0, or the 2D centre of a
  * column where the sub-step limit is exceeded */
+/* runge_kutta's weights (tracers.c:110-129) for 1 to 3 stages: of the
+ * sub-step's starting value, of the stage's value, of its tendency (and
+ * the stage's volume); forward Euler and the SSP schemes of Shu and
+ * Osher */
+static const double us_rkw[3][3][3] = {
+    {{1.0, 0.0, 1.0}},
+    {{1.0, 0.0, 1.0}, {0.50, 0.50, 0.50}},
+    {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}}};
+
+/* a tracer buffer other than a, b and e */
+static int us_free_buf(int nbuf, int a, int b, int e) {
+  for (int i = 0; i < nbuf; i++)
+    if (i != a && i != b && i != e) return i;
+  return -1;
+}
+
 static int us_adv(usctx *c) {
   emstr_usdev *D = &c->D;
   emstr_dev *d = &D->d;
@@ -1188,8 +1206,20 @@ static int us_adv(usctx *c) {
       return emstr_set_error(E_DEVICE, -1, "vertical flux buffer (%zu bytes)", bz);
     }
   }
+  /* Runge-Kutta stages (tracers.c:841-1098): each stage's fluxes come
+   * from the previous stage's values; the update combines them with the
+   * sub-step's starting values, which stay in their buffer, as do the
+   * step's starting values (the ghosts read them) */
+  const int nrk = wc->rkstage, nbuf = (nrk > 1) ? 4 : 3;
+  if (nrk > 1 && !c->tr[3]) {
+    const size_t bytes = (size_t)(d->ntr > 0 ? d->ntr : 1) * d->n3 * sizeof(double);
+    if (hipMalloc((void **)&c->tr[3], bytes) != hipSuccess) {
+      c->tr[3] = NULL;
+      return emstr_set_error(E_DEVICE, -1, "Runge-Kutta stage buffer (%zu bytes)", bytes);
+    }
+  }
   const int b0 = c->cur;
-  int in = b0, out = (b0 + 1) % 3;
+  int in = b0;
   trem = dtu;
   while (trem > 0) {
     double u = dtm;
@@ -1198,25 +1228,32 @@ static int us_adv(usctx *c) {
     if (u > 0.0) {
       if (emstr_launch_subeta(d, u, trem, first, c->st))
         return emstr_set_error(E_DEVICE, -1, "subeta launch");
-      if (ffsl) {
-        if (emstr_us_launch_ffsl_sub(D, u, c->d_nw, c->d_clc, c->d_crc, c->d_clf, c->d_crf,
-                                     c->d_cour, c->d_edn, c->nesp, c->d_lci, c->d_ld,
-                                     c->d_fferr, c->st) ||
-            emstr_us_launch_ffsl_flux(D, u, c->tr[in], c->d_nw, c->d_clc, c->d_crc, c->d_clf,
-                                      c->d_crf, c->d_cour, c->d_edn, c->nesp, c->d_lci,
-                                      c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc, c->d_fx,
-                                      c->d_fz, c->anydif, c->d_need, us_ffstamp(c),
-                                      c->d_fferr, c->st) ||
-            emstr_us_launch_update(D, c->tr[in], c->tr[out], c->d_fx, c->d_fz, u, c->st) ||
-            emstr_launch_osub(d, c->st))
-          return emstr_set_error(E_DEVICE, -1, "FFSL launch");
-      } else if (emstr_us_launch_adv(D, c->tr[in], c->tr[out], c->tr[b0], c->d_edn, c->nesp,
-                                     c->d_fx, c->d_fz, u, c->anydif, d->ultimate, c->st) ||
-                 emstr_launch_osub(d, c->st))
-        return emstr_set_error(E_DEVICE, -1, "advection launch");
-      in = out;
-      out = (out + 1) % 3;
-      if (out == b0) out = (out + 1) % 3;
+      if (ffsl && emstr_us_launch_ffsl_sub(D, u, c->d_nw, c->d_clc, c->d_crc, c->d_clf,
+                                           c->d_crf, c->d_cour, c->d_edn, c->nesp, c->d_lci,
+                                           c->d_ld, c->d_fferr, c->st))
+        return emstr_set_error(E_DEVICE, -1, "FFSL launch");
+      int src = in;
+      for (int tc = 0; tc < nrk; tc++) {
+        const int dst = us_free_buf(nbuf, b0, in, src);
+        const double *w = us_rkw[nrk - 1][tc];
+        const int last = (tc == nrk - 1);
+        if (ffsl) {
+          if (emstr_us_launch_ffsl_flux(D, u, c->tr[src], c->d_nw, c->d_clc, c->d_crc,
+                                        c->d_clf, c->d_crf, c->d_cour, c->d_edn, c->nesp,
+                                        c->d_lci, c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc,
+                                        c->d_fx, c->d_fz, c->anydif, c->d_need, us_ffstamp(c),
+                                        c->d_fferr, c->st) ||
+              emstr_us_launch_update(D, c->tr[in], c->tr[src], c->tr[dst], c->d_fx, c->d_fz,
+                                     u, w, last, c->st))
+            return emstr_set_error(E_DEVICE, -1, "FFSL launch");
+        } else if (emstr_us_launch_adv(D, c->tr[in], c->tr[src], c->tr[dst], c->tr[b0],
+                                       c->d_edn, c->nesp, c->d_fx, c->d_fz, u, c->anydif,
+                                       d->ultimate, w, last, c->st))
+          return emstr_set_error(E_DEVICE, -1, "advection launch");
+        src = dst;
+      }
+      if (emstr_launch_osub(d, c->st)) return emstr_set_error(E_DEVICE, -1, "osub launch");
+      in = src;
     }
     trem -= u;
     first = 0;

@@ -11,21 +11,25 @@
 extern "C" {
 #endif
 
-/* advection, horizontal diffusion, update, surf_concrk and tr_bounds of
- * every tracer over one sub-step (van Leer or QUICKEST with optional
- * ULTIMATE, or first order): the fluxes of the nesp wet edges (dense
- * indices edn) into fx [ntr][nz][ne2], the vertical fluxes into fz
- * [ntr][nz][ncol], then the cell updates */
-int emstr_us_launch_adv(const emstr_usdev *D, const double *tin, double *tout,
-                        const double *tstart, const long long *edn, int nesp, double *fx,
-                        double *fz, double dtu, int anydif, int ult, void *stream);
+/* one Runge-Kutta stage of a sub-step: fluxes of tin into fx/fz
+ * (scratch, [ntr][nz][ne2] and [ntr][n3]), then the cell update
+ * tin -> tout with runge_kutta's weights rkw[3] (of the sub-step's
+ * starting values t0, of the stage's values tin, of the tendency);
+ * tr_bounds follows the last stage (last != 0), the layers above the
+ * surface keep their values before it (tstart: step-start values for
+ * ghosts) */
+int emstr_us_launch_adv(const emstr_usdev *D, const double *t0, const double *tin,
+                        double *tout, const double *tstart, const long long *edn, int nesp,
+                        double *fx, double *fz, double dtu, int anydif, int ult,
+                        const double *rkw, int last, void *stream);
 /* minimum sub-step; *out must hold the bits of dtu0 before the launch;
  * sf, gu, gv (ghost velocities [nz][ne2]) are read under FFSL only     */
 int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned long long *out, double sf,
                         const double *gu, const double *gv, void *stream);
 /* the cell update, surf_concrk and tr_bounds from fluxes fx, fz */
-int emstr_us_launch_update(const emstr_usdev *D, const double *tin, double *tout,
-                           const double *fx, const double *fz, double dtu, void *stream);
+int emstr_us_launch_update(const emstr_usdev *D, const double *t0, const double *tin,
+                           double *tout, const double *fx, const double *fz, double dtu,
+                           const double *rkw, int last, void *stream);
 
 /* FFSL (emstr_us_ffsl.hip).  Per step: the centred w into nw [nz][ncol]
  * (init: vel_center_w, else the Leonard et al. choice).  Per sub-step:

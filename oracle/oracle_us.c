@@ -22,7 +22,10 @@
  *                       exercised); constancy and mass on hexagons
  *   set_dz              same column walk as SHOC; maps checked on meshes
  *   surf_concrk +
- *   runge_kutta (1)     closed-basin mass conservation on hexagons
+ *   runge_kutta         closed-basin mass conservation on hexagons; 2 and
+ *                       3 stages equal their compositions of forward-Euler
+ *                       steps (tests/test_oracle_us_rk.py); 5 stages:
+ *                       constancy and mass only (parity unpinned beyond)
  *   tr_bounds, vert_diffuse_3d, implicit_vdiff_tr
  *                       as SHOC (same maths on the same vertical maps);
  *                       the quadrilateral equivalence covers them
@@ -45,8 +48,36 @@
 static int ous_err = 0;
 int oracle_us_last_error(void) { return ous_err; }
 
-/* tracers.c:110-129, the column of rkweights for one stage */
-static const double rkw1[3] = {1.0, 0.0, 1.0};
+/* Runge-Kutta weights, tracers.c:110-129 (rkweights[3*tc + i][rkstage-1]):
+ * for stage tc, the weight of the sub-step's starting value, of the
+ * stage's value, of its tendency (also the stage's volume weight); the
+ * schemes are forward Euler, the SSP schemes of Shu and Osher with 2 and 3
+ * stages, none for 4 stages (the reference's column is zero), and the
+ * 5-stage scheme of Spiteri and Ruuth, whose last stage combines stages
+ * 2-4 (rkw5_last: weights of the stage-2 and stage-3 values, of both
+ * tendencies, of the stage-4 value, and the volume weight; the reference
+ * weights the stage-4 tendency like the stage-3 one, tracers.c:1199-1205). */
+static const double rkw_1[1][3] = {{1.0, 0.0, 1.0}};
+static const double rkw_2[2][3] = {{1.0, 0.0, 1.0}, {0.50, 0.50, 0.50}};
+static const double rkw_3[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25},
+                                   {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+static const double rkw_4[4][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0},
+                                   {0.0, 0.0, 0.0}};
+static const double rkw_5[4][3] = {{1.00, 0.00, 0.391752226571890},
+                                   {0.444370493651235, 0.555629506348765, 0.368410593050371},
+                                   {0.620101851488403, 0.379898148511597, 0.251891774271694},
+                                   {0.178079954393132, 0.821920045606868, 0.544974750228521}};
+static const double rkw5_last[5] = {0.517231671970585, 0.096059710526147, 0.063692468666290,
+                                    0.386708617503269, 0.127384937332581};
+static const double *rk_w(int rkstage, int tc) {
+  switch (rkstage) {
+  case 1: return rkw_1[tc];
+  case 2: return rkw_2[tc];
+  case 3: return rkw_3[tc];
+  case 4: return rkw_4[tc];
+  default: return rkw_5[tc];
+  }
+}
 
 /*-------------------------------------------------------------------*/
 /* tracers.c:7120-7137  no-gradient lateral boundary: ghost = bin      */
@@ -334,13 +365,24 @@ static void oracle_us_hor_diffuse(geometry_t *g, window_t *wd, win_priv_t *wc,
 }
 
 /*-------------------------------------------------------------------*/
-/* tracers.c:1170-1214  runge_kutta, one stage                         */
+/* tracers.c:1170-1214  runge_kutta: the value of stage tc from the     */
+/* stage values ntr[] and tendencies grad[] of cell c                   */
 /*-------------------------------------------------------------------*/
-static double rk1(win_priv_t *wc, int c2, double ovol, double nvol, double ntr0,
-                  double grad, int slf) {
+static double oracle_us_runge_kutta(win_priv_t *wc, int c2, double ovol, double nvol, int tc,
+                                    int c, int slf, double *const *ntr,
+                                    double *const *grad) {
   double sf = (slf) ? wc->Ds[c2] : wc->Hn1[c2];
-  double vol = ovol + rkw1[2] * (nvol - ovol);
-  double tr = sf * rkw1[0] * ntr0 + sf * rkw1[1] * ntr0 + rkw1[2] * grad;
+  double tr, vol;
+  if (wc->rkstage == 5 && tc == 4) {
+    const double *w = rkw5_last;
+    vol = ovol + w[4] * (nvol - ovol);
+    tr = sf * w[0] * ntr[2][c] + sf * w[1] * ntr[3][c] + w[2] * grad[3][c] +
+         sf * w[3] * ntr[4][c] + w[2] * grad[4][c];
+  } else {
+    const double *w = rk_w(wc->rkstage, tc);
+    vol = ovol + w[2] * (nvol - ovol);
+    tr = sf * w[0] * ntr[0][c] + sf * w[1] * ntr[tc][c] + w[2] * grad[tc][c];
+  }
   tr /= (wc->Hn1[c2] * vol);
   return tr;
 }
@@ -443,8 +485,9 @@ void oracle_us_tr_bounds(geometry_t *g, window_t *wd, win_priv_t *wc) {
 /* One tracer through one sub-step: the body of the tracer loop of    */
 /* advect_diffuse, tracers.c:803-1098 (one Runge-Kutta stage).        */
 /*-------------------------------------------------------------------*/
+#define OUS_RKMAX 5 /* most Runge-Kutta stages */
 typedef struct {
-  double *Fx, *Fz, *dtracer, *rk, *gr;
+  double *Fx, *Fz, *dtracer, *rk[OUS_RKMAX], *gr[OUS_RKMAX];
   /* FFSL, per tracer (transport.c:1961-2598) */
   double *trm, *moc, *lww, *lwb, *cx, *cy, *ntr, *dint, *cour;
   int *cl, *mask;
@@ -456,58 +499,61 @@ static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr,
 static void us_tracer_substep(geometry_t *g, window_t *wd, win_priv_t *wc, int n, double dtu,
                               int slf, int ksf, const double *osubeta,
                               const double *subeta, uscr_t *s) {
-  int cc, j;
+  int cc, j, tc;
   double *tr = wd->tr_wc[n];
   double *Fx = s->Fx, *Fz = s->Fz, *dtracer = s->dtracer;
   const int kef = wc->vc1, kefs = wc->vcs1;
   tr[0] = (double)n;
-  memset(s->gr, 0, g->szc * sizeof(double));
-  memset(Fx, 0, g->sze * sizeof(double));
-  memset(Fz, 0, g->szc * sizeof(double));
-  memset(dtracer, 0, g->szc * sizeof(double));
-  for (cc = 1; cc <= g->b2_t; cc++) {
-    int cb = g->bot_t[cc];
-    tr[g->zm1[cb]] = tr[cb];
-  }
-  if (wc->advect[n]) {
-    if (wc->trasc & FFSL) {
-      if (wd->dttr != 0.0) us_ffsl_don(g, wd, wc, tr, Fx, Fz, dtu, s);
-    } else
-      oracle_us_advect(g, wd, wc, tr, Fx, Fz, dtu);
-  }
-  if (wc->diffuse[n]) oracle_us_hor_diffuse(g, wd, wc, tr, Fx);
-  for (cc = 1; cc <= g->b3_t; cc++) {
-    int c = g->w3_t[cc];
-    int c2 = g->m2d[c];
-    dtracer[c] = 0.0;
-    for (j = 1; j <= g->npe[c2]; j++) {
-      int e = g->c2e[j][c];
-      dtracer[c] += (g->eSc[j][c2] * Fx[e] * dtu);
+  for (tc = 0; tc < wc->rkstage; tc++) memset(s->gr[tc], 0, g->szc * sizeof(double));
+  /* Runge-Kutta stage loop, tracers.c:841-1098 */
+  for (tc = 0; tc < wc->rkstage; tc++) {
+    memset(Fx, 0, g->sze * sizeof(double));
+    memset(Fz, 0, g->szc * sizeof(double));
+    memset(dtracer, 0, g->szc * sizeof(double));
+    for (cc = 1; cc <= g->b2_t; cc++) {
+      int cb = g->bot_t[cc];
+      tr[g->zm1[cb]] = tr[cb];
     }
-  }
-  for (cc = ksf; cc <= kef; cc++) {
-    int c = wc->s1[cc];
-    int c2 = g->m2d[c];
-    int zp1 = g->zp1[c];
-    s->rk[c] = tr[c];
-    if (slf)
-      tr[c] *= wc->Ds[c2];
-    else
-      tr[c] *= wc->Hn1[c2];
-    s->gr[c] = -((dtracer[c] / g->cellarea[c2] + (Fz[zp1] - Fz[c])) / wc->dz[c]);
-    tr[c] = rk1(wc, c2, 1.0, 1.0, s->rk[c], s->gr[c], slf);
-  }
-  if (!wc->sigma && wc->advect[n]) {
-    for (cc = 1; cc <= kefs; cc++) {
+    if (wc->advect[n]) {
+      if (wc->trasc & FFSL) {
+        if (wd->dttr != 0.0) us_ffsl_don(g, wd, wc, tr, Fx, Fz, dtu, s);
+      } else
+        oracle_us_advect(g, wd, wc, tr, Fx, Fz, dtu);
+    }
+    if (wc->diffuse[n]) oracle_us_hor_diffuse(g, wd, wc, tr, Fx);
+    for (cc = 1; cc <= g->b3_t; cc++) {
+      int c = g->w3_t[cc];
+      int c2 = g->m2d[c];
+      dtracer[c] = 0.0;
+      for (j = 1; j <= g->npe[c2]; j++) {
+        int e = g->c2e[j][c];
+        dtracer[c] += (g->eSc[j][c2] * Fx[e] * dtu);
+      }
+    }
+    for (cc = ksf; cc <= kef; cc++) {
       int c = wc->s1[cc];
       int c2 = g->m2d[c];
-      double top = -Fz[c] * g->cellarea[c2];
-      double ovol, nvol, grad, surftr;
-      s->rk[c] = oracle_us_surf_concrk(g, wc, tr, osubeta[c2], subeta[c2], c, c2, cc,
-                                       &grad, &ovol, &nvol, top, dtracer);
-      s->gr[c] = grad;
-      surftr = rk1(wc, c2, ovol, nvol, s->rk[c], s->gr[c], slf);
-      tr[c] = (g->botz[c2] > 0.0 && wc->dz[c] < wc->hmin) ? tr[c] : surftr;
+      int zp1 = g->zp1[c];
+      s->rk[tc][c] = tr[c];
+      if (slf)
+        tr[c] *= wc->Ds[c2];
+      else
+        tr[c] *= wc->Hn1[c2];
+      s->gr[tc][c] = -((dtracer[c] / g->cellarea[c2] + (Fz[zp1] - Fz[c])) / wc->dz[c]);
+      tr[c] = oracle_us_runge_kutta(wc, c2, 1.0, 1.0, tc, c, slf, s->rk, s->gr);
+    }
+    if (!wc->sigma && wc->advect[n]) {
+      for (cc = 1; cc <= kefs; cc++) {
+        int c = wc->s1[cc];
+        int c2 = g->m2d[c];
+        double top = -Fz[c] * g->cellarea[c2];
+        double ovol, nvol, grad, surftr;
+        s->rk[tc][c] = oracle_us_surf_concrk(g, wc, tr, osubeta[c2], subeta[c2], c, c2, cc,
+                                             &grad, &ovol, &nvol, top, dtracer);
+        s->gr[tc][c] = grad;
+        surftr = oracle_us_runge_kutta(wc, c2, ovol, nvol, tc, c, slf, s->rk, s->gr);
+        tr[c] = (g->botz[c2] > 0.0 && wc->dz[c] < wc->hmin) ? tr[c] : surftr;
+      }
     }
   }
 }
@@ -518,7 +564,11 @@ static uscr_t *uscr_get(int nt, int n) {
   int t;
   if (uscr && uscr_nt >= nt && uscr_n == n) return uscr;
   for (t = 0; t < uscr_nt; t++) {
-    void *p[] = {uscr[t].Fx, uscr[t].Fz, uscr[t].dtracer, uscr[t].rk, uscr[t].gr,
+    for (int q = 0; q < OUS_RKMAX; q++) {
+      free(uscr[t].rk[q]);
+      free(uscr[t].gr[q]);
+    }
+    void *p[] = {uscr[t].Fx, uscr[t].Fz, uscr[t].dtracer,
                  uscr[t].trm, uscr[t].moc, uscr[t].lww, uscr[t].lwb, uscr[t].cx,
                  uscr[t].cy, uscr[t].ntr, uscr[t].dint, uscr[t].cour, uscr[t].cl,
                  uscr[t].mask};
@@ -527,7 +577,11 @@ static uscr_t *uscr_get(int nt, int n) {
   free(uscr);
   uscr = (uscr_t *)calloc(nt, sizeof(uscr_t));
   for (t = 0; t < nt; t++) {
-    double **dp[] = {&uscr[t].Fx, &uscr[t].Fz, &uscr[t].dtracer, &uscr[t].rk, &uscr[t].gr,
+    for (int q = 0; q < OUS_RKMAX; q++) {
+      uscr[t].rk[q] = (double *)calloc(n, sizeof(double));
+      uscr[t].gr[q] = (double *)calloc(n, sizeof(double));
+    }
+    double **dp[] = {&uscr[t].Fx, &uscr[t].Fz, &uscr[t].dtracer,
                      &uscr[t].trm, &uscr[t].moc, &uscr[t].lwb, &uscr[t].cx, &uscr[t].cy,
                      &uscr[t].ntr, &uscr[t].dint, &uscr[t].cour};
     for (size_t i = 0; i < sizeof(dp) / sizeof(dp[0]); i++)
@@ -1067,6 +1121,10 @@ int oracle_us_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int
   double *u1 = wd->u1, *w = wd->w;
   ous_err = 0;
   if (wc->trasc == NONE) return 0;
+  if (wc->rkstage < 1 || wc->rkstage > OUS_RKMAX) { /* the stage arrays hold 1..5 stages */
+    ous_err = 5;
+    return -ous_err;
+  }
 
   courf = (wc->ultimate || wc->trasc >= QUICKEST) ? 1 : 0;
   if (wc->trasc & ORDER1) {

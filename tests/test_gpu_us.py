@@ -137,6 +137,54 @@ def test_moving_surface_step():
     assert_exact(gpu, ora)
 
 
+RK_CASES = [
+    # name, hex_basin kw, rkstage, stab, nsteps
+    ("vanleer_rk2", dict(trasc=U.VANLEER), 2, U.NONE, 3),
+    ("vanleer_rk3", dict(trasc=U.VANLEER), 3, U.NONE, 3),
+    ("quickest_ultimate_rk3", dict(trasc=U.QUICKEST, ultimate=1), 3, U.NONE, 3),
+    ("order1_rk2", dict(trasc=U.ORDER1), 2, U.NONE, 3),
+    ("vanleer_rk3_substeps", dict(trasc=U.VANLEER, dt=3600.0, vmax=1.2), 3, U.SUB_STEP, 2),
+    ("ffsl_rk2", dict(trasc=U.FFSL), 2, U.NONE, 3),
+    ("ffsl_rk3", dict(trasc=U.FFSL), 3, U.NONE, 2),
+]
+
+
+@pytest.mark.parametrize("name,kw,rkstage,stab,nsteps", RK_CASES)
+def test_hex_runge_kutta_stages(name, kw, rkstage, stab, nsteps):
+    """Runge-Kutta stage loop (tracers.c:841-1098): 2 and 3 stages against
+    the oracle (whose stages are pinned by the forward-Euler compositions
+    of tests/test_oracle_us_rk.py); FFSL to 1e-12 relative."""
+    say("case hex", name)
+    case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=3, seed=7, **kw)
+    case.wc.rkstage = rkstage
+    case.wc.stab = stab
+    case.wc.trsf = 0.8
+    case.prepare(OU.set_dz)
+    t0 = case.tracers()
+    gpu, ora = run_both(case, nsteps)
+    assert np.abs(ora - t0).max() > 1e-3
+    if stab == U.SUB_STEP:
+        assert case.wc.b1 < case.dt
+    assert_exact(gpu, ora)
+    case.wc.rkstage = 1
+
+
+@pytest.mark.parametrize("rkstage", [2, 3])
+def test_moving_surface_runge_kutta(rkstage):
+    """A moving surface with stages: the surface cell's mass at the start of
+    the sub-step is recomputed from its buffer at every stage."""
+    say("case hex moving surface, rkstage", rkstage)
+    case = MS.hex_basin(nx=20, ny=16, nz=6, ntr=3, seed=13, dmin=2.0, dmax=30.0)
+    w2, old, eta, rng, dl = _moving(case, 4)
+    OU.set_dz(case.gp, case.wp, case.pp)
+    eta[w2] = old[w2] + rng.uniform(-0.9, 0.9, len(w2)) * dl
+    case.wc.rkstage = rkstage
+    case.prepare(OU.set_dz)
+    gpu, ora = run_both(case, 1)
+    assert_exact(gpu, ora)
+    case.wc.rkstage = 1
+
+
 def test_set_dz_lists_bit_exact():
     say("case hex set_dz lists")
     mk = lambda: MS.hex_basin(nx=16, ny=14, nz=7, ntr=1, seed=31, dmin=2.0, dmax=40.0)  # noqa: E731
@@ -302,7 +350,7 @@ def test_unsupported_settings_are_reported():
     say("case hex unsupported")
     case = MS.hex_basin(nx=10, ny=8, nz=4, ntr=1, seed=2)
     case.prepare(OU.set_dz)
-    case.wc.rkstage = 2
+    case.wc.rkstage = 4
     MS.lib().tracer_step_3d(case.gp, case.wp, case.pp)
     assert "Runge-Kutta" in MS.last_error()[0]
     case.wc.rkstage = 1

@@ -1,0 +1,96 @@
+"""Pinning of the COMPAS oracle's Runge-Kutta stage loop
+(advect_diffuse, model/hd-us/tracers/tracers.c:841-1098; runge_kutta
+:1170-1214; weights :110-129).
+
+With the surface at rest (the hexagon basin's transports come from a
+streamfunction, so no column gains volume), every cell's update is
+u_new = a * u0 + b * u_stage + c * L(u_stage), and a stage equals a
+combination of forward-Euler (one-stage) steps E(u) = u + L(u) taken from
+the same starting ghosts:
+
+  2 stages (SSP-RK2):  u2 = u0 / 2 + E(E(u0)) / 2
+  3 stages (SSP-RK3):  u1 = E(u0), u2 = 3 u0 / 4 + E(u1) / 4,
+                       u3 = u0 / 3 + 2 E(u2) / 3
+
+so a wrong weight, a wrong stage value (the start of the sub-step versus
+the current stage) or a missing tendency fails them.  They differ only in
+the order of the additions: the bar is 1e-13 relative.  Constancy and
+closed-basin mass hold for every scheme whose weights of the values add
+to one (2, 3 and 5 stages)."""
+import numpy as np
+import pytest
+
+from ems_amd import mesh as MS, structs_us as U
+from oracle import oracle_us as OU
+
+
+def _case(trasc, rkstage, seed=23):
+    case = MS.hex_basin(nx=16, ny=14, nz=6, ntr=2, seed=seed, trasc=trasc, dt=300.0)
+    case.wc.rkstage = rkstage
+    case.prepare(OU.set_dz)
+    return case
+
+
+def _euler(case, u, first):
+    """One forward-Euler advect_diffuse from wet values u; the lateral
+    ghosts are filled only on the first call (the stage loop never
+    refreshes them)."""
+    for n in range(case.ntr):
+        case.tr(n)[case.wet] = u[n]
+    assert OU.advect_diffuse(case, lateral=first) == 0
+    return case.tracers()
+
+
+def _rel(a, b):
+    return (np.abs(a - b) / np.maximum(np.abs(b), 1e-300)).max()
+
+
+@pytest.mark.parametrize("trasc", [U.VANLEER, U.QUICKEST, U.ORDER1])
+def test_two_stages_are_heun(trasc):
+    rk = _case(trasc, 2)
+    u0 = rk.tracers()
+    assert OU.advect_diffuse(rk) == 0
+    got = rk.tracers()
+    fe = _case(trasc, 1)
+    assert np.array_equal(fe.tracers(), u0)
+    e1 = _euler(fe, u0, True)
+    e2 = _euler(fe, e1, False)
+    want = 0.5 * u0 + 0.5 * e2
+    assert np.abs(got - u0).max() > 1e-3
+    assert _rel(got, want) <= 1e-13, _rel(got, want)
+    # and differs from one Euler step
+    assert _rel(got, e1) > 1e-8
+
+
+@pytest.mark.parametrize("trasc", [U.VANLEER, U.QUICKEST])
+def test_three_stages_are_shu_osher(trasc):
+    rk = _case(trasc, 3)
+    u0 = rk.tracers()
+    assert OU.advect_diffuse(rk) == 0
+    got = rk.tracers()
+    fe = _case(trasc, 1)
+    u1 = _euler(fe, u0, True)
+    u2 = 0.75 * u0 + 0.25 * _euler(fe, u1, False)
+    want = u0 / 3.0 + 2.0 / 3.0 * _euler(fe, u2, False)
+    assert _rel(got, want) <= 1e-13, _rel(got, want)
+
+
+@pytest.mark.parametrize("rkstage", [2, 3, 5])
+def test_constancy_and_closed_basin_mass(rkstage):
+    case = _case(U.VANLEER, rkstage)
+    tr = case.tracers()
+    tr[0] = 7.25
+    case.set_tracers(tr)
+    m0 = OU.tmass(case, 1)
+    for _ in range(3):
+        assert OU.advect_diffuse(case) == 0
+    out = case.tracers()
+    assert np.abs(out[0] - 7.25).max() <= 1e-12 * 7.25
+    assert abs(OU.tmass(case, 1) - m0) <= 1e-12 * abs(m0)
+    assert np.abs(out[1] - tr[1]).max() > 1e-3
+
+
+@pytest.mark.parametrize("rkstage", [0, 6])
+def test_stage_count_out_of_range_is_reported(rkstage):
+    case = _case(U.VANLEER, rkstage)
+    assert OU.advect_diffuse(case) != 0
