@@ -250,6 +250,11 @@ __device__ __forceinline__ double us_fz(const emstr_dev &d, int col, int k, int 
 struct UsRk {
   double w0, w1, w2;
   int last; /* the sub-step's last stage: tr_bounds follows */
+  /* 5 stages: mode 1 stores the tendency (stage 3), mode 2 is the last
+   * stage, w0 * ntr2 + w1 * ntr3 + w2 * (grad3 + grad) + w3 * ntr, volume
+   * weight w4 (tracers.c:1199-1205) */
+  double w3, w4;
+  int mode;
 };
 
 /* runge_kutta (tracers.c:1170-1214) of one stage from the sub-step's
@@ -263,6 +268,25 @@ __device__ __forceinline__ double us_rk(const UsRk &w, double ovol, double nvol,
   tr /= (1.0 * vol);
   return tr;
 }
+
+/* the last of 5 stages from the values of stages 2-4 and the tendencies
+ * of stages 3 and 4 */
+__device__ __forceinline__ double us_rk5(const UsRk &w, double ovol, double nvol, double ntr2,
+                                         double ntr3, double grad3, double ntr4, double grad) {
+  const double sf = 1.0;
+  const double vol = ovol + w.w4 * (nvol - ovol);
+  double tr = sf * w.w0 * ntr2 + sf * w.w1 * ntr3 + w.w2 * grad3 + sf * w.w3 * ntr4 +
+              w.w2 * grad;
+  tr /= (1.0 * vol);
+  return tr;
+}
+
+/* the stage arrays of one tracer: the sub-step's starting values, the
+ * inputs of stages 2 and 3 and the tendency of stage 3 (5 stages only) */
+struct UsStage {
+  const double *t0, *t2, *t3;
+  double *g3;
+};
 
 /* the vertical flux through the lower face of every cell to process
  * (k <= ks of a processed column) for every advected tracer, one thread
@@ -288,7 +312,7 @@ k_usfz(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ wdn
 
 /* new value of the sub-surface cell at layer k (tracers.c:1000-1016) */
 __device__ double us_subsurface(const emstr_usdev &D, const UsRk &w, int col, int k,
-                                const double *t0, const double *t, const double *fxn,
+                                const UsStage &sa, const double *t, const double *fxn,
                                 const double *fzn, double dtu, int adv) {
   const emstr_dev &d = D.d;
   const ll idx = (ll)k * d.plane + col;
@@ -299,7 +323,8 @@ __device__ double us_subsurface(const emstr_usdev &D, const UsRk &w, int col, in
     FzT = fzn[idx + d.plane];
   }
   const double grad = -((dtr / d.cellarea[col] + (FzT - FzB)) / d.dz[idx]);
-  return us_rk(w, 1.0, 1.0, t0[idx], t[idx], grad);
+  if (w.mode == 2) return us_rk5(w, 1.0, 1.0, sa.t2[idx], sa.t3[idx], sa.g3[idx], t[idx], grad);
+  return us_rk(w, 1.0, 1.0, sa.t0[idx], t[idx], grad);
 }
 
 #define US_MAXNPE 8
@@ -309,7 +334,8 @@ __global__ void __launch_bounds__(256)
 k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict__ tin,
         double *__restrict__ tout, const double *__restrict__ fx,
         const double *__restrict__ fz, const ll *__restrict__ wdn, int nwet, double dtu,
-        UsRk w) {
+        UsRk w, const double *__restrict__ t2in, const double *__restrict__ t3in,
+        double *__restrict__ g3in) {
   const emstr_dev &d = D.d;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nwet) return;
@@ -360,7 +386,13 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
         FzT = fz[n * N3 + idx + P];
       }
       const double grad = -((dtr / area + (FzT - FzB)) / dzc);
-      double r = us_rk(w, 1.0, 1.0, t0in[n * N3 + idx], t[idx], grad);
+      double r;
+      if (w.mode == 2)
+        r = us_rk5(w, 1.0, 1.0, t2in[n * N3 + idx], t3in[n * N3 + idx], g3in[n * N3 + idx],
+                   t[idx], grad);
+      else
+        r = us_rk(w, 1.0, 1.0, t0in[n * N3 + idx], t[idx], grad);
+      if (w.mode == 1) g3in[n * N3 + idx] = grad;
       /* tr_bounds clip (tracers.c:3328-3331), after the last stage */
       if (w.last && d.mintr[n] <= 1e-5 && r < d.mintr[n]) r = d.mintr[n];
       tout[n * N3 + idx] = r;
@@ -380,6 +412,12 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
     const double *t = tin + n * N3, *t0 = t0in + n * N3;
     const double *fxn = fx + n * NE3;
     double *o = tout + n * N3;
+    UsStage sa = {t0, nullptr, nullptr, nullptr};
+    if (w.mode) {
+      sa.t2 = t2in + n * N3;
+      sa.t3 = t3in + n * N3;
+      sa.g3 = g3in + n * N3;
+    }
     if (!d.in_tbdy[n]) {
       for (int kk = k; kk <= ktop; kk++) o[(ll)kk * P + col] = t[(ll)kk * P + col];
       continue;
@@ -397,14 +435,17 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
       double waterbot = (c == kbc) ? botz : lay[c];
       const double dzs = subeta - waterbot;
       const double nvol = dzs * area;
-      double surftr = 0.0, surftr0 = 0.0, ovol = 0.0, vol, watertop;
+      /* 5 stages, last: the masses of stages 2 and 3 in place of it */
+      const double *ta = (w.mode == 2) ? sa.t2 : t0, *tb = (w.mode == 2) ? sa.t3 : t0;
+      double surftr = 0.0, surftr0 = 0.0, surftr3 = 0.0, ovol = 0.0, vol, watertop;
       double grad = -top;
       while (c != kold && c < ktop) {
         watertop = lay[c + 1];
         vol = (watertop - waterbot) * area;
         ovol += vol;
         surftr += (t[(ll)c * P + col] * vol);
-        surftr0 += (t0[(ll)c * P + col] * vol);
+        surftr0 += (ta[(ll)c * P + col] * vol);
+        surftr3 += (tb[(ll)c * P + col] * vol);
         grad -= us_dtracer(D, col, c, fxn, dtu);
         c = c + 1;
         waterbot = lay[c];
@@ -414,7 +455,8 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
       vol = (watertop - waterbot) * area;
       ovol += vol;
       surftr += t[(ll)c * P + col] * vol;
-      surftr0 += t0[(ll)c * P + col] * vol;
+      surftr0 += ta[(ll)c * P + col] * vol;
+      surftr3 += tb[(ll)c * P + col] * vol;
       grad -= us_dtracer(D, col, c, fxn, dtu);
       if (c != ktop) {
         do {
@@ -424,7 +466,13 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
       }
       if (nvol == 0.0 || (surftr < 0.0 && dzs <= d.hmin)) surftr = 0.0;
       if (nvol == 0.0 || (surftr0 < 0.0 && dzs <= d.hmin)) surftr0 = 0.0;
-      const double r = us_rk(w, ovol, nvol, surftr0, surftr, grad);
+      if (nvol == 0.0 || (surftr3 < 0.0 && dzs <= d.hmin)) surftr3 = 0.0;
+      double r;
+      if (w.mode == 2)
+        r = us_rk5(w, ovol, nvol, surftr0, surftr3, sa.g3[idx], surftr, grad);
+      else
+        r = us_rk(w, ovol, nvol, surftr0, surftr, grad);
+      if (w.mode == 1) sa.g3[idx] = grad;
       /* tracers.c:1056-1057 */
       trs = (botz > 0.0 && d.dz[idx] < d.hmin) ? t[idx] : r;
     }
@@ -439,7 +487,7 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
     const double dzco = (kold >= kbc) ? d.dz[(ll)kold * P + col] : 0.0;
     if (k > kbc && dzco < d.hmin) {
       if (trs < d.mintr[n] || trs > d.maxtr[n])
-        trs = us_subsurface(D, w, col, k - 1, t0, t, fxn, fz + n * N3, dtu, adv);
+        trs = us_subsurface(D, w, col, k - 1, sa, t, fxn, fz + n * N3, dtu, adv);
     }
     /* clip (tracers.c:3328-3331) */
     if (d.mintr[n] <= 1e-5 && trs < d.mintr[n]) trs = d.mintr[n];
@@ -510,38 +558,52 @@ __global__ void k_usdtm(emstr_usdev D, double dtu0, unsigned long long *out, int
 }
 
 template <int SCH>
-static void us_adv_launch(const emstr_usdev *D, const double *t0, const double *tin,
+static void us_adv_launch(const emstr_usdev *D, const double *const *ts, const double *tin,
                           double *tout, const double *tstart, const long long *edn, int nesp,
                           double *fx, double *fz, double dtu, int anydif, int ult,
-                          const UsRk &w, hipStream_t st) {
+                          const UsRk &w, double *g3, hipStream_t st) {
   const emstr_dev *d = &D->d;
   if (nesp)
     k_usflux<SCH><<<(nesp + 255) / 256, 256, 0, st>>>(*D, tin, tstart, edn, nesp, fx, dtu,
                                                        anydif, ult);
   if (d->nwet) {
     k_usfz<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, tin, d->wdn, d->nwet, fz, dtu, ult);
-    k_usadv<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, t0, tin, tout, fx, fz, d->wdn,
-                                                         d->nwet, dtu, w);
+    k_usadv<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, ts[0], tin, tout, fx, fz, d->wdn,
+                                                         d->nwet, dtu, w, ts[1], ts[2], g3);
   }
 }
 
-extern "C" int emstr_us_launch_adv(const emstr_usdev *D, const double *t0, const double *tin,
-                                   double *tout, const double *tstart, const long long *edn,
-                                   int nesp, double *fx, double *fz, double dtu, int anydif,
-                                   int ult, const double *rkw, int last, void *stream) {
+static UsRk us_rkargs(const double *rkw, int last, int mode) {
+  UsRk w;
+  w.w0 = rkw[0];
+  w.w1 = rkw[1];
+  w.w2 = rkw[2];
+  w.w3 = (mode == 2) ? rkw[3] : 0.0;
+  w.w4 = (mode == 2) ? rkw[4] : 0.0;
+  w.last = last;
+  w.mode = mode;
+  return w;
+}
+
+extern "C" int emstr_us_launch_adv(const emstr_usdev *D, const double *const *ts,
+                                   const double *tin, double *tout, const double *tstart,
+                                   const long long *edn, int nesp, double *fx, double *fz,
+                                   double dtu, int anydif, int ult, const double *rkw, int last,
+                                   int mode, double *g3, void *stream) {
   hipStream_t st = (hipStream_t)stream;
-  const UsRk w = {rkw[0], rkw[1], rkw[2], last};
+  const UsRk w = us_rkargs(rkw, last, mode);
   switch (D->d.trasc) {
     case VANLEER:
-      us_adv_launch<VANLEER>(D, t0, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, ult, w,
-                             st);
+      us_adv_launch<VANLEER>(D, ts, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, ult, w,
+                             g3, st);
       break;
     case ORDER1:
-      us_adv_launch<ORDER1>(D, t0, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, 0, w, st);
+      us_adv_launch<ORDER1>(D, ts, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, 0, w, g3,
+                            st);
       break;
     case QUICKEST:
-      us_adv_launch<QUICKEST>(D, t0, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, ult, w,
-                              st);
+      us_adv_launch<QUICKEST>(D, ts, tin, tout, tstart, edn, nesp, fx, fz, dtu, anydif, ult, w,
+                              g3, st);
       break;
     default:
       return -1;
@@ -560,14 +622,14 @@ extern "C" int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned l
 }
 
 /* the cell update from given edge and vertical fluxes (FFSL) */
-extern "C" int emstr_us_launch_update(const emstr_usdev *D, const double *t0,
+extern "C" int emstr_us_launch_update(const emstr_usdev *D, const double *const *ts,
                                       const double *tin, double *tout, const double *fx,
                                       const double *fz, double dtu, const double *rkw,
-                                      int last, void *stream) {
+                                      int last, int mode, double *g3, void *stream) {
   const emstr_dev *d = &D->d;
-  const UsRk w = {rkw[0], rkw[1], rkw[2], last};
+  const UsRk w = us_rkargs(rkw, last, mode);
   if (d->nwet)
     k_usadv<VANLEER><<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
-        *D, t0, tin, tout, fx, fz, d->wdn, d->nwet, dtu, w);
+        *D, ts[0], tin, tout, fx, fz, d->wdn, d->nwet, dtu, w, ts[1], ts[2], g3);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

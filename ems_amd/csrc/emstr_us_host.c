@@ -60,7 +60,8 @@ typedef struct emstr_us_ctx {
   int *sedg;     /* [ncol] sediment ghost centre            */
   int *sp_of;    /* [nz*ncol] dense cell -> sparse centre   */
   /* device */
-  double *tr[4]; /* the fourth only with Runge-Kutta stages */
+  double *tr[6]; /* the fourth to sixth only with Runge-Kutta stages */
+  double *d_g3;  /* 5 stages: the stage-3 tendency [ntr][n3] */
   int cur;
   double *stage; /* [max(szc, sze)] */
   int *d_esp;    /* wet edges: sparse index and dense index */
@@ -108,7 +109,7 @@ static void us_destroy(usctx *c) {
                 d->osubeta, d->subeta, d->layers, d->w, d->Kz, d->dz, d->mintr, d->maxtr,
                 d->advect, d->diffuse, d->in_tbdy, d->in_tdif, d->cols, d->wsp, d->wdn,
                 D->npe, D->ce, D->esc, D->ec1, D->ec2, D->ecp, D->ecm, D->esel, D->h1e,
-                D->h2e, D->qd3, D->qhm, D->qhp, D->u1, D->flux, D->dzu, D->kh, c->tr[0], c->tr[1], c->tr[2], c->tr[3],
+                D->h2e, D->qd3, D->qhm, D->qhp, D->u1, D->flux, D->dzu, D->kh, c->tr[0], c->tr[1], c->tr[2], c->tr[3], c->tr[4], c->tr[5], c->d_g3,
                 c->stage, c->d_esp, c->d_edn, c->d_bpt, c->d_bin, c->d_gsed, c->d_ised,
                 c->d_proc, c->d_err, c->d_dtm, c->d_fx, c->d_fz, D->cx, D->cy, D->ux,
                 D->uy, D->n1x, D->n1y, D->t2x, D->t2y, D->ev, D->vxy, D->eord, D->lsq_off, D->lsq_s, D->lr_off, D->lr_q, D->vsl, D->lsq_B, D->u2, D->wtop,
@@ -131,10 +132,10 @@ static int us_check_scope(geometry_t *g, window_t *wd, win_priv_t *wc) {
   if (wc->sigma) return emstr_set_error(E_UNSUP, -1, "sigma coordinates are not supported");
   if (wc->thin_merge) return emstr_set_error(E_UNSUP, -1, "thin-layer merging is not supported");
   if (wc->means) return emstr_set_error(E_UNSUP, -1, "mean-velocity transport is not supported");
-  if (wc->rkstage < 1 || wc->rkstage > 3)
+  if (wc->rkstage < 1 || wc->rkstage > 5 || wc->rkstage == 4)
     return emstr_set_error(E_UNSUP, -1,
-                           "Runge-Kutta with %d stages is not supported (1 to 3 stages; the "
-                           "reference's weights for 4 stages are zero)",
+                           "Runge-Kutta with %d stages is not supported (1, 2, 3 or 5 stages; "
+                           "the reference's weights for 4 stages are zero)",
                            wc->rkstage);
   if (wc->trasc != VANLEER && wc->trasc != ORDER1 && wc->trasc != QUICKEST &&
       wc->trasc != FFSL)
@@ -1148,11 +1149,25 @@ static const double us_rkw[3][3][3] = {
     {{1.0, 0.0, 1.0}},
     {{1.0, 0.0, 1.0}, {0.50, 0.50, 0.50}},
     {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}}};
+/* 5 stages (Spiteri and Ruuth): stages 0-3 as above; the last combines
+ * the values of stages 2, 3 and 4 and the tendencies of stages 3 and 4,
+ * both weighted as the reference weights them (tracers.c:1199-1205), and
+ * the volume weight */
+static const double us_rkw5[4][3] = {{1.00, 0.00, 0.391752226571890},
+                                     {0.444370493651235, 0.555629506348765, 0.368410593050371},
+                                     {0.620101851488403, 0.379898148511597, 0.251891774271694},
+                                     {0.178079954393132, 0.821920045606868, 0.544974750228521}};
+static const double us_rkw5_last[5] = {0.517231671970585, 0.096059710526147,
+                                       0.063692468666290, 0.386708617503269,
+                                       0.127384937332581};
 
-/* a tracer buffer other than a, b and e */
-static int us_free_buf(int nbuf, int a, int b, int e) {
-  for (int i = 0; i < nbuf; i++)
-    if (i != a && i != b && i != e) return i;
+/* a tracer buffer not in x[0..nx-1] (-1 entries ignored) */
+static int us_free_buf(int nbuf, const int *x, int nx) {
+  for (int i = 0; i < nbuf; i++) {
+    int used = 0;
+    for (int j = 0; j < nx; j++) used |= (x[j] == i);
+    if (!used) return i;
+  }
   return -1;
 }
 
@@ -1210,12 +1225,17 @@ static int us_adv(usctx *c) {
    * from the previous stage's values; the update combines them with the
    * sub-step's starting values, which stay in their buffer, as do the
    * step's starting values (the ghosts read them) */
-  const int nrk = wc->rkstage, nbuf = (nrk > 1) ? 4 : 3;
-  if (nrk > 1 && !c->tr[3]) {
+  const int nrk = wc->rkstage, nbuf = (nrk == 5) ? 6 : (nrk > 1) ? 4 : 3;
+  {
     const size_t bytes = (size_t)(d->ntr > 0 ? d->ntr : 1) * d->n3 * sizeof(double);
-    if (hipMalloc((void **)&c->tr[3], bytes) != hipSuccess) {
-      c->tr[3] = NULL;
-      return emstr_set_error(E_DEVICE, -1, "Runge-Kutta stage buffer (%zu bytes)", bytes);
+    for (int i = 3; i < nbuf; i++)
+      if (!c->tr[i] && hipMalloc((void **)&c->tr[i], bytes) != hipSuccess) {
+        c->tr[i] = NULL;
+        return emstr_set_error(E_DEVICE, -1, "Runge-Kutta stage buffer (%zu bytes)", bytes);
+      }
+    if (nrk == 5 && !c->d_g3 && hipMalloc((void **)&c->d_g3, bytes) != hipSuccess) {
+      c->d_g3 = NULL;
+      return emstr_set_error(E_DEVICE, -1, "Runge-Kutta tendency buffer (%zu bytes)", bytes);
     }
   }
   const int b0 = c->cur;
@@ -1232,25 +1252,36 @@ static int us_adv(usctx *c) {
                                            c->d_crf, c->d_cour, c->d_edn, c->nesp, c->d_lci,
                                            c->d_ld, c->d_fferr, c->st))
         return emstr_set_error(E_DEVICE, -1, "FFSL launch");
-      int src = in;
+      int src = in, sin[5] = {in, -1, -1, -1, -1};
       for (int tc = 0; tc < nrk; tc++) {
-        const int dst = us_free_buf(nbuf, b0, in, src);
-        const double *w = us_rkw[nrk - 1][tc];
+        /* with 5 stages the inputs of stages 2 and 3 stay until the last */
+        const int keep[5] = {b0, in, src, (nrk == 5) ? sin[2] : -1, (nrk == 5) ? sin[3] : -1};
+        const int dst = us_free_buf(nbuf, keep, 5);
+        const double *w;
+        int mode = 0;
+        if (nrk == 5) {
+          w = (tc < 4) ? us_rkw5[tc] : us_rkw5_last;
+          mode = (tc == 3) ? 1 : (tc == 4) ? 2 : 0;
+        } else
+          w = us_rkw[nrk - 1][tc];
         const int last = (tc == nrk - 1);
+        const double *ts[3] = {c->tr[in], (sin[2] >= 0) ? c->tr[sin[2]] : NULL,
+                               (sin[3] >= 0) ? c->tr[sin[3]] : NULL};
         if (ffsl) {
           if (emstr_us_launch_ffsl_flux(D, u, c->tr[src], c->d_nw, c->d_clc, c->d_crc,
                                         c->d_clf, c->d_crf, c->d_cour, c->d_edn, c->nesp,
                                         c->d_lci, c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc,
                                         c->d_fx, c->d_fz, c->anydif, c->d_need, us_ffstamp(c),
                                         c->d_fferr, c->st) ||
-              emstr_us_launch_update(D, c->tr[in], c->tr[src], c->tr[dst], c->d_fx, c->d_fz,
-                                     u, w, last, c->st))
+              emstr_us_launch_update(D, ts, c->tr[src], c->tr[dst], c->d_fx, c->d_fz, u, w,
+                                     last, mode, c->d_g3, c->st))
             return emstr_set_error(E_DEVICE, -1, "FFSL launch");
-        } else if (emstr_us_launch_adv(D, c->tr[in], c->tr[src], c->tr[dst], c->tr[b0],
-                                       c->d_edn, c->nesp, c->d_fx, c->d_fz, u, c->anydif,
-                                       d->ultimate, w, last, c->st))
+        } else if (emstr_us_launch_adv(D, ts, c->tr[src], c->tr[dst], c->tr[b0], c->d_edn,
+                                       c->nesp, c->d_fx, c->d_fz, u, c->anydif, d->ultimate, w,
+                                       last, mode, c->d_g3, c->st))
           return emstr_set_error(E_DEVICE, -1, "advection launch");
         src = dst;
+        if (tc + 1 < 5) sin[tc + 1] = dst;
       }
       if (emstr_launch_osub(d, c->st)) return emstr_set_error(E_DEVICE, -1, "osub launch");
       in = src;

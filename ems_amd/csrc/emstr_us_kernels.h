@@ -13,23 +13,25 @@ extern "C" {
 
 /* one Runge-Kutta stage of a sub-step: fluxes of tin into fx/fz
  * (scratch, [ntr][nz][ne2] and [ntr][n3]), then the cell update
- * tin -> tout with runge_kutta's weights rkw[3] (of the sub-step's
- * starting values t0, of the stage's values tin, of the tendency);
- * tr_bounds follows the last stage (last != 0), the layers above the
- * surface keep their values before it (tstart: step-start values for
- * ghosts) */
-int emstr_us_launch_adv(const emstr_usdev *D, const double *t0, const double *tin,
+ * tin -> tout with runge_kutta's weights rkw (of the sub-step's starting
+ * values ts[0], of the stage's values tin, of the tendency); tr_bounds
+ * follows the last stage (last != 0), the layers above the surface keep
+ * their values before it (tstart: step-start values for ghosts).  With 5
+ * stages, mode 1 (stage 3) stores the tendency in g3 [ntr][n3] and mode 2
+ * (stage 4) combines the values of stages 2 and 3 (ts[1], ts[2]), g3, the
+ * stage's values and tendency with rkw[0..4] */
+int emstr_us_launch_adv(const emstr_usdev *D, const double *const *ts, const double *tin,
                         double *tout, const double *tstart, const long long *edn, int nesp,
                         double *fx, double *fz, double dtu, int anydif, int ult,
-                        const double *rkw, int last, void *stream);
+                        const double *rkw, int last, int mode, double *g3, void *stream);
 /* minimum sub-step; *out must hold the bits of dtu0 before the launch;
  * sf, gu, gv (ghost velocities [nz][ne2]) are read under FFSL only     */
 int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned long long *out, double sf,
                         const double *gu, const double *gv, void *stream);
 /* the cell update, surf_concrk and tr_bounds from fluxes fx, fz */
-int emstr_us_launch_update(const emstr_usdev *D, const double *t0, const double *tin,
+int emstr_us_launch_update(const emstr_usdev *D, const double *const *ts, const double *tin,
                            double *tout, const double *fx, const double *fz, double dtu,
-                           const double *rkw, int last, void *stream);
+                           const double *rkw, int last, int mode, double *g3, void *stream);
 
 /* FFSL (emstr_us_ffsl.hip).  Per step: the centred w into nw [nz][ncol]
  * (init: vel_center_w, else the Leonard et al. choice).  Per sub-step:

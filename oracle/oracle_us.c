@@ -24,8 +24,8 @@
  *   surf_concrk +
  *   runge_kutta         closed-basin mass conservation on hexagons; 2 and
  *                       3 stages equal their compositions of forward-Euler
- *                       steps (tests/test_oracle_us_rk.py); 5 stages:
- *                       constancy and mass only (parity unpinned beyond)
+ *                       steps (tests/test_oracle_us_rk.py), as do 5
+ *                       stages
  *   tr_bounds, vert_diffuse_3d, implicit_vdiff_tr
  *                       as SHOC (same maths on the same vertical maps);
  *                       the quadrilateral equivalence covers them

@@ -146,14 +146,18 @@ RK_CASES = [
     ("vanleer_rk3_substeps", dict(trasc=U.VANLEER, dt=3600.0, vmax=1.2), 3, U.SUB_STEP, 2),
     ("ffsl_rk2", dict(trasc=U.FFSL), 2, U.NONE, 3),
     ("ffsl_rk3", dict(trasc=U.FFSL), 3, U.NONE, 2),
+    ("vanleer_rk5", dict(trasc=U.VANLEER), 5, U.NONE, 2),
+    ("quickest_ultimate_rk5", dict(trasc=U.QUICKEST, ultimate=1), 5, U.NONE, 2),
+    ("vanleer_rk5_substeps", dict(trasc=U.VANLEER, dt=3600.0, vmax=1.2), 5, U.SUB_STEP, 1),
+    ("ffsl_rk5", dict(trasc=U.FFSL), 5, U.NONE, 2),
 ]
 
 
 @pytest.mark.parametrize("name,kw,rkstage,stab,nsteps", RK_CASES)
 def test_hex_runge_kutta_stages(name, kw, rkstage, stab, nsteps):
-    """Runge-Kutta stage loop (tracers.c:841-1098): 2 and 3 stages against
-    the oracle (whose stages are pinned by the forward-Euler compositions
-    of tests/test_oracle_us_rk.py); FFSL to 1e-12 relative."""
+    """Runge-Kutta stage loop (tracers.c:841-1098): 2, 3 and 5 stages
+    against the oracle (whose stages are pinned by the forward-Euler
+    compositions of tests/test_oracle_us_rk.py); FFSL to 1e-12 relative."""
     say("case hex", name)
     case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=3, seed=7, **kw)
     case.wc.rkstage = rkstage
@@ -169,7 +173,7 @@ def test_hex_runge_kutta_stages(name, kw, rkstage, stab, nsteps):
     case.wc.rkstage = 1
 
 
-@pytest.mark.parametrize("rkstage", [2, 3])
+@pytest.mark.parametrize("rkstage", [2, 3, 5])
 def test_moving_surface_runge_kutta(rkstage):
     """A moving surface with stages: the surface cell's mass at the start of
     the sub-step is recomputed from its buffer at every stage."""

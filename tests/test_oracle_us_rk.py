@@ -75,6 +75,34 @@ def test_three_stages_are_shu_osher(trasc):
     assert _rel(got, want) <= 1e-13, _rel(got, want)
 
 
+def test_five_stages_are_their_composition():
+    """Spiteri-Ruuth with the reference's weights: stage tc gives
+    a u0 + b u_tc + c (E(u_tc) - u_tc); the last combines stages 2-4 and
+    weights both tendencies with the stage-3 weight (tracers.c:1199-1205)."""
+    w = [(1.00, 0.00, 0.391752226571890),
+         (0.444370493651235, 0.555629506348765, 0.368410593050371),
+         (0.620101851488403, 0.379898148511597, 0.251891774271694),
+         (0.178079954393132, 0.821920045606868, 0.544974750228521)]
+    last = (0.517231671970585, 0.096059710526147, 0.063692468666290, 0.386708617503269)
+    rk = _case(U.VANLEER, 5)
+    u0 = rk.tracers()
+    assert OU.advect_diffuse(rk) == 0
+    got = rk.tracers()
+    fe = _case(U.VANLEER, 1)
+    st = [u0]
+    first = True
+    for a, b, c in w:
+        u = st[-1]
+        e = _euler(fe, u, first)
+        first = False
+        st.append(a * u0 + b * u + c * (e - u))
+    e3 = _euler(fe, st[3], False)
+    e4 = _euler(fe, st[4], False)
+    want = (last[0] * st[2] + last[1] * st[3] + last[2] * (e3 - st[3]) + last[3] * st[4] +
+            last[2] * (e4 - st[4]))
+    assert _rel(got, want) <= 1e-12, _rel(got, want)
+
+
 @pytest.mark.parametrize("rkstage", [2, 3, 5])
 def test_constancy_and_closed_basin_mass(rkstage):
     case = _case(U.VANLEER, rkstage)
