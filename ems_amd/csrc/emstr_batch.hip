@@ -311,6 +311,39 @@ __device__ __forceinline__ float vla(float bm2, float bm1, float c, float fp1,
   return c - ref; /* no transport: its flux term vanishes */
 }
 
+/* vla for the faces whose downstream cell (for q > 0) is the reference
+ * cell, c == ref (west, south, bottom faces of a cell): the same values,
+ * with c - ref = 0 written out */
+__device__ __forceinline__ float vla_c(float bm2, float bm1, float c, float fp1, float q,
+                                       float f) {
+  if (q > 0.0f) {
+    const float df = (bm1 - bm2) * (c - bm1);
+    const float t = df * __builtin_amdgcn_rcpf(c - bm2);
+    return (df > 0.0f) ? (bm1 - c) + f * t : bm1 - c;
+  } else if (q < 0.0f) {
+    const float df = (c - bm1) * (fp1 - c);
+    const float t = df * __builtin_amdgcn_rcpf(fp1 - bm1);
+    return (df > 0.0f) ? __builtin_fmaf(-f, t, 0.0f) : 0.0f;
+  }
+  return 0.0f;
+}
+
+/* vla for the faces whose upstream cell (for q > 0) is the reference
+ * cell, bm1 == ref (east, north, top faces) */
+__device__ __forceinline__ float vla_b(float bm2, float bm1, float c, float fp1, float q,
+                                       float f) {
+  if (q > 0.0f) {
+    const float df = (bm1 - bm2) * (c - bm1);
+    const float t = df * __builtin_amdgcn_rcpf(c - bm2);
+    return (df > 0.0f) ? __builtin_fmaf(f, t, 0.0f) : 0.0f;
+  } else if (q < 0.0f) {
+    const float df = (c - bm1) * (fp1 - c);
+    const float t = df * __builtin_amdgcn_rcpf(fp1 - bm1);
+    return (df > 0.0f) ? (c - bm1) - f * t : c - bm1;
+  }
+  return c - bm1;
+}
+
 /*-------------------------------------------------------------------*/
 /* deterministic stochastic rounding                                 */
 /*-------------------------------------------------------------------*/
@@ -414,6 +447,17 @@ __device__ __forceinline__ float hflux(const HVals &h, bool ew, bool nw, float v
   return AE * qE - AW * qW + AN * qN - AS * qS;
 }
 
+/* hflux of a cell against its own value (ref == v0) */
+__device__ __forceinline__ float hflux_own(const HVals &h, bool ew, bool nw, float v0, float qW,
+                                           float fW, float qE, float fE, float qS, float fS,
+                                           float qN, float fN) {
+  const float AW = vla_c(h.w2, h.w1, v0, h.e1, qW, fW);
+  const float AE = ew ? vla_b(h.w1, v0, h.e1, h.e2, qE, fE) : 0.0f;
+  const float AS = vla_c(h.s2, h.s1, v0, h.n1, qS, fS);
+  const float AN = nw ? vla_b(h.s1, v0, h.n1, h.n2, qN, fN) : 0.0f;
+  return AE * qE - AW * qW + AN * qN - AS * qS;
+}
+
 /*-------------------------------------------------------------------*/
 /* kernel A: cell-parallel advection                                  */
 /*-------------------------------------------------------------------*/
@@ -490,7 +534,7 @@ __device__ __forceinline__ void adv_cell(const St13 &v, int cw, ll col, int k, i
   char *oc = (char *)tout + col * colb;
   const float v0 = v.v0;
   const float qB = rdl(cw, C_QB);
-  const float fba = vla(v.zm2, v.zm1, v0, v.zp1, qB, rdl(cw, C_CB), v0) * qB;
+  const float fba = vla_c(v.zm2, v.zm1, v0, v.zp1, qB, rdl(cw, C_CB)) * qB;
   HVals h;
   h.w1 = v.w1;
   h.w2 = v.w2;
@@ -500,15 +544,15 @@ __device__ __forceinline__ void adv_cell(const St13 &v, int cw, ll col, int k, i
   h.s2 = v.s2;
   h.n1 = v.n1;
   h.n2 = v.n2;
-  const float hh = hflux(h, (m & M_E1) != 0, (m & M_N1) != 0, v0, v0, rdl(cw, C_QW),
-                         rdl(cw, C_CW), rdl(cw, C_QE), rdl(cw, C_CE), rdl(cw, C_QS),
-                         rdl(cw, C_CS), rdl(cw, C_QN), rdl(cw, C_CN));
+  const float hh = hflux_own(h, (m & M_E1) != 0, (m & M_N1) != 0, v0, rdl(cw, C_QW),
+                             rdl(cw, C_CW), rdl(cw, C_QE), rdl(cw, C_CE), rdl(cw, C_QS),
+                             rdl(cw, C_CS), rdl(cw, C_QN), rdl(cw, C_CN));
   const uint32_t key = srkey((uint32_t)(col * nz) + (uint32_t)k, hl);
   if (k < ks) {
     /* sub-surface update (tracers.c:912-925) in anomaly form; the top
      * face is the bottom face of the cell above */
     const float qT = rdl(cw, C_N + C_QB);
-    const float ht = hh + vla(v.zm1, v0, v.zp1, v.zp2, qT, rdl(cw, C_N + C_CB), v0) * qT - fba;
+    const float ht = hh + vla_b(v.zm1, v0, v.zp1, v.zp2, qT, rdl(cw, C_N + C_CB)) * qT - fba;
     const double an = (double)v0 - (double)((ht + v0 * rdl(cw, C_D)) * rdl(cw, C_RV));
     float bk = sround(an, key);
     /* the thin-layer repair of tr_bounds needs the unclipped value */
