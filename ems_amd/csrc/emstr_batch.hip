@@ -311,37 +311,29 @@ __device__ __forceinline__ float vla(float bm2, float bm1, float c, float fp1,
   return c - ref; /* no transport: its flux term vanishes */
 }
 
-/* vla for the faces whose downstream cell (for q > 0) is the reference
- * cell, c == ref (west, south, bottom faces of a cell): the same values,
- * with c - ref = 0 written out */
+/* vla of a cell's own faces without branches: for q < 0 the stencil is
+ * mirrored, (bm2, bm1, c) -> (fp1, c, bm1), which leaves df unchanged
+ * and negates the reciprocal, so both signs share one expression with the
+ * same values bit for bit; q == 0 keeps the upwind anomaly (c - ref).
+ * ref is the cell's own value: c on its west, south and bottom faces, bm1
+ * on its east, north and top faces. */
+__device__ __forceinline__ float vla_m(float bm2, float bm1, float c, float fp1, float q,
+                                       float f, float ref) {
+  const bool pos = q > 0.0f;
+  const float p2 = pos ? bm2 : fp1, p1 = pos ? bm1 : c, p0 = pos ? c : bm1;
+  const float df = (p1 - p2) * (p0 - p1);
+  const float t = df * __builtin_amdgcn_rcpf(p0 - p2);
+  const float r0 = p1 - ref;
+  const float r = __builtin_fmaf(f, (df > 0.0f) ? t : 0.0f, r0);
+  return (q != 0.0f) ? r : r0;
+}
 __device__ __forceinline__ float vla_c(float bm2, float bm1, float c, float fp1, float q,
                                        float f) {
-  if (q > 0.0f) {
-    const float df = (bm1 - bm2) * (c - bm1);
-    const float t = df * __builtin_amdgcn_rcpf(c - bm2);
-    return (df > 0.0f) ? (bm1 - c) + f * t : bm1 - c;
-  } else if (q < 0.0f) {
-    const float df = (c - bm1) * (fp1 - c);
-    const float t = df * __builtin_amdgcn_rcpf(fp1 - bm1);
-    return (df > 0.0f) ? __builtin_fmaf(-f, t, 0.0f) : 0.0f;
-  }
-  return 0.0f;
+  return vla_m(bm2, bm1, c, fp1, q, f, c);
 }
-
-/* vla for the faces whose upstream cell (for q > 0) is the reference
- * cell, bm1 == ref (east, north, top faces) */
 __device__ __forceinline__ float vla_b(float bm2, float bm1, float c, float fp1, float q,
                                        float f) {
-  if (q > 0.0f) {
-    const float df = (bm1 - bm2) * (c - bm1);
-    const float t = df * __builtin_amdgcn_rcpf(c - bm2);
-    return (df > 0.0f) ? __builtin_fmaf(f, t, 0.0f) : 0.0f;
-  } else if (q < 0.0f) {
-    const float df = (c - bm1) * (fp1 - c);
-    const float t = df * __builtin_amdgcn_rcpf(fp1 - bm1);
-    return (df > 0.0f) ? (c - bm1) - f * t : c - bm1;
-  }
-  return c - bm1;
+  return vla_m(bm2, bm1, c, fp1, q, f, bm1);
 }
 
 /*-------------------------------------------------------------------*/
