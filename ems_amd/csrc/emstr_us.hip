@@ -98,69 +98,108 @@ __device__ __forceinline__ double us_ult(double cr, double F, double trc, double
   return F;
 }
 
+/* van Leer face value with the stencil mirrored for a negative velocity:
+ * (tm2, tm1, tp1) for vel > 0, (tp2, tp1, tm1) for vel < 0.  The mirrored
+ * limiter product is the same product of negated differences and its
+ * quotient the negated quotient, so the face value equals us_vl's to the
+ * last bit, without a branch on the sign */
+__device__ __forceinline__ double us_vlm(bool pos, double omc, double tp1, double tp2,
+                                         double tm1, double tm2) {
+  const double a2 = pos ? tm2 : tp2, a1 = pos ? tm1 : tp1, a0 = pos ? tp1 : tm1;
+  double df = (a1 - a2) * (a0 - a1);
+  df = (df > 0.0) ? 2.0 * df / (a0 - a2) : 0.0;
+  return a1 + 0.5 * omc * df;
+}
+
 /*
- * Flux through wet surface edge e at layer k for one tracer: the face
+ * Fluxes through wet surface edge e at layer k for every tracer: the face
  * value by the scheme, ULTIMATE, times u1flux3d (advect(),
  * tracers.c:1659-1728), then horizontal diffusion (tracers.c:5438-5446).
  * A far cell that is not wet at this layer is a ghost of the near cell:
- * its step-start value.
+ * its step-start value.  What depends on the edge only (transport,
+ * Courant number, the wetness of the far cells, the diffusion
+ * coefficient) is computed once, outside the loop over tracers.
  */
 template <int SCH>
-__device__ __forceinline__ double us_edge_flux(const emstr_usdev &D, int e, int k,
-                                               const double *__restrict__ t,
-                                               const double *__restrict__ ts, double dtu,
-                                               int ult, int adv, int dif) {
+__device__ __forceinline__ void us_edge_fluxes(const emstr_usdev &D, int e, int k, ll ie,
+                                               const double *__restrict__ tin,
+                                               const double *__restrict__ tstart,
+                                               double *__restrict__ fx, ll NE3, double dtu,
+                                               int anydif, int ult) {
   const emstr_dev &d = D.d;
-  const ll P = d.plane;
+  const ll P = d.plane, N3 = d.n3;
   const int c1 = D.ec1[e], c2 = D.ec2[e];
   const ll i1 = (ll)k * P + c1, i2 = (ll)k * P + c2;
-  const ll ie = (ll)k * D.ne2 + e;
-  double F = 0.0;
-  if (adv) {
-    if (SCH == ORDER1) {
-      /* tracers.c:545-549, 1772-1775 */
-      const double f = D.flux[ie];
-      const double w6 = 0.5 * (f + fabs(f));
-      const double w7 = 0.5 * (f - fabs(f));
-      F = w6 * t[i2] + w7 * t[i1];
-    } else {
-      const double vel = D.u1[ie];
-      const double cn = fmod(vel * dtu / D.h2e[e], 1.0);
-      const int cp = D.ecp[e], cm = D.ecm[e];
-      const bool wp = wetk(d.kb, cp, k), wm = wetk(d.kb, cm, k);
-      const double tp1 = t[i1], tm1 = t[i2];
-      const double tp2 = wp ? t[(ll)k * P + cp] : ts[i1];
-      const double tm2 = wm ? t[(ll)k * P + cm] : ts[i2];
-      if (SCH == QUICKEST) {
-        /* tracers.c:2098-2123 */
-        const double h0 = D.h2e[e], hm = D.qhm[e], hp = D.qhp[e];
-        const double db = 2.0 / (hm + h0);
-        const double df = 2.0 / (hp + h0);
-        const double face = D.qd3[e] * (tm1 - tp1) + tp1;
-        const double grad = tp1 - tm1;
-        double curv = 0.0;
-        if (cn > 0.0)
-          curv = db * (hm * tp1 + h0 * tm2) - 2.0 * tm1;
-        else if (cn < 0.0)
-          curv = df * (h0 * tp2 + hp * tm1) - 2.0 * tp1;
-        F = face - 0.5 * cn * grad - (1.0 - cn * cn) * curv / 6.0;
-      } else {
-        F = us_vl(vel, cn, tp1, tp2, tm1, tm2);
-      }
-      if (ult) {
-        const unsigned sel = D.esel[e];
-        const double trp = (wp && (sel & 1)) ? tp2 : tp1;
-        const double trm = wm ? ((sel & 2) ? tm2 : tm1) : ts[i2];
-        F = us_ult(cn, F, tp1, tm1, trp, trm);
-      }
-      F *= D.flux[ie];
+  const double fl = D.flux[ie];
+  double vel = 0.0, cn = 0.0, omc = 0.0, h0 = 0.0, hm = 0.0, hp = 0.0, q3 = 0.0;
+  bool wp = false, wm = false;
+  unsigned sel = 0;
+  ll jp = i1, jm = i2;
+  if (SCH != ORDER1) {
+    vel = D.u1[ie];
+    h0 = D.h2e[e];
+    cn = fmod(vel * dtu / h0, 1.0);
+    omc = 1.0 + ((vel > 0.0) ? -cn : cn);
+    const int cp = D.ecp[e], cm = D.ecm[e];
+    wp = wetk(d.kb, cp, k);
+    wm = wetk(d.kb, cm, k);
+    if (wp) jp = (ll)k * P + cp;
+    if (wm) jm = (ll)k * P + cm;
+    if (SCH == QUICKEST) {
+      hm = D.qhm[e];
+      hp = D.qhp[e];
+      q3 = D.qd3[e];
     }
+    if (ult) sel = D.esel[e];
   }
-  if (dif) {
-    const double csx = D.dzu[ie] * D.h1e[e] * 1.0;
-    F -= (csx * D.kh[ie] * (t[i1] - t[i2]) / D.h2e[e]);
+  /* the far values: the neighbour's at this layer, or the ghost's */
+  const double *__restrict__ bp = wp ? tin : tstart;
+  const double *__restrict__ bm = wm ? tin : tstart;
+  double cx = 0.0, h2 = 1.0;
+  if (anydif) {
+    cx = D.dzu[ie] * D.h1e[e] * 1.0 * D.kh[ie];
+    h2 = D.h2e[e];
   }
-  return F;
+  for (int n = 0; n < d.ntr; n++) {
+    if (!d.in_tbdy[n]) continue;
+    const int adv = d.advect[n], dif = anydif ? d.diffuse[n] : 0;
+    const ll o = (ll)n * N3;
+    const double tp1 = tin[o + i1], tm1 = tin[o + i2];
+    double F = 0.0;
+    if (adv) {
+      if (SCH == ORDER1) {
+        /* tracers.c:545-549, 1772-1775 */
+        const double w6 = 0.5 * (fl + fabs(fl));
+        const double w7 = 0.5 * (fl - fabs(fl));
+        F = w6 * tm1 + w7 * tp1;
+      } else {
+        const double tp2 = bp[o + jp], tm2 = bm[o + jm];
+        if (SCH == QUICKEST) {
+          /* tracers.c:2098-2123 */
+          const double db = 2.0 / (hm + h0);
+          const double df = 2.0 / (hp + h0);
+          const double face = q3 * (tm1 - tp1) + tp1;
+          const double grad = tp1 - tm1;
+          double curv = 0.0;
+          if (cn > 0.0)
+            curv = db * (hm * tp1 + h0 * tm2) - 2.0 * tm1;
+          else if (cn < 0.0)
+            curv = df * (h0 * tp2 + hp * tm1) - 2.0 * tp1;
+          F = face - 0.5 * cn * grad - (1.0 - cn * cn) * curv / 6.0;
+        } else {
+          F = (vel > 0.0 || vel < 0.0) ? us_vlm(vel > 0.0, omc, tp1, tp2, tm1, tm2) : 0.0;
+        }
+        if (ult) {
+          const double trp = (wp && (sel & 1)) ? tp2 : tp1;
+          const double trm = wm ? ((sel & 2) ? tm2 : tm1) : tm2;
+          F = us_ult(cn, F, tp1, tm1, trp, trm);
+        }
+        F *= fl;
+      }
+    }
+    if (dif) F -= (cx * (tp1 - tm1) / h2);
+    fx[n * NE3 + ie] = F;
+  }
 }
 
 /* dtracer of the cell at layer k of column col (tracers.c:949-959); the
@@ -192,13 +231,7 @@ k_usflux(emstr_usdev D, const double *__restrict__ tin, const double *__restrict
   if (q >= nesp) return;
   const ll ie = edn[q];
   const int k = (int)(ie / D.ne2), e = (int)(ie % D.ne2);
-  const ll N3 = d.n3, NE3 = (ll)d.nz * D.ne2;
-  for (int n = 0; n < d.ntr; n++) {
-    if (!d.in_tbdy[n]) continue;
-    const int adv = d.advect[n], dif = anydif ? d.diffuse[n] : 0;
-    fx[n * NE3 + ie] =
-        us_edge_flux<SCH>(D, e, k, tin + n * N3, tstart + n * N3, dtu, ult, adv, dif);
-  }
+  us_edge_fluxes<SCH>(D, e, k, ie, tin, tstart, fx, (ll)d.nz * D.ne2, dtu, anydif, ult);
 }
 
 /* vertical flux through the lower face of the cell at layer k
