@@ -42,29 +42,6 @@ __device__ __forceinline__ bool wetk(const int *__restrict__ kb, int col, int k)
   return col >= 0 && k >= kb[col];
 }
 
-/* van Leer face value from tracer values (tracers.c:2544-2573); *set is
- * false when the velocity is zero (the face keeps its zero) */
-__device__ __forceinline__ double us_vl(double vel, double cn, double tp1, double tp2,
-                                        double tm1, double tm2) {
-  double df;
-  if (vel > 0.0) {
-    df = (tm1 - tm2) * (tp1 - tm1);
-    if (df > 0.0)
-      df = 2.0 * df / (tp1 - tm2);
-    else
-      df = 0.0;
-    return tm1 + 0.5 * (1.0 - cn) * df;
-  } else if (vel < 0.0) {
-    df = (tp1 - tm1) * (tp2 - tp1);
-    if (df > 0.0)
-      df = 2.0 * df / (tp2 - tm1);
-    else
-      df = 0.0;
-    return tp1 - 0.5 * (1.0 + cn) * df;
-  }
-  return 0.0;
-}
-
 /* ultimate_filter (tracers.c:3785-3833) on face value F; trc: tracer at
  * the face's cell c, trcm1 at its cell cm1 */
 __device__ __forceinline__ double us_ult(double cr, double F, double trc, double trcm1,
@@ -98,11 +75,13 @@ __device__ __forceinline__ double us_ult(double cr, double F, double trc, double
   return F;
 }
 
-/* van Leer face value with the stencil mirrored for a negative velocity:
- * (tm2, tm1, tp1) for vel > 0, (tp2, tp1, tm1) for vel < 0.  The mirrored
- * limiter product is the same product of negated differences and its
- * quotient the negated quotient, so the face value equals us_vl's to the
- * last bit, without a branch on the sign */
+/* van Leer face value (tracers.c:2544-2573) with the stencil mirrored
+ * for a negative velocity: the stencil is (tm2, tm1, tp1) for vel > 0 and
+ * (tp2, tp1, tm1) for vel < 0, omc is 1 - cn and 1 + cn.  For vel < 0 the
+ * reference forms df = (tp1 - tm1) (tp2 - tp1), 2 df / (tp2 - tm1) and
+ * tp1 - 0.5 (1 + cn) df; the mirrored product is the same product of
+ * negated differences and its quotient the negated quotient, so the face
+ * value is the same to the last bit, without a branch on the sign */
 __device__ __forceinline__ double us_vlm(bool pos, double omc, double tp1, double tp2,
                                          double tm1, double tm2) {
   const double a2 = pos ? tm2 : tp2, a1 = pos ? tm1 : tp1, a0 = pos ? tp1 : tm1;
@@ -234,49 +213,6 @@ k_usflux(emstr_usdev D, const double *__restrict__ tin, const double *__restrict
   us_edge_fluxes<SCH>(D, e, k, ie, tin, tstart, fx, (ll)d.nz * D.ne2, dtu, anydif, ult);
 }
 
-/* vertical flux through the lower face of the cell at layer k
- * (tracers.c:2405-2412 van Leer, 1777-1785 first order, 1712-1720
- * ULTIMATE, 1735-1738 times w*dtu); tracer values at the layers below
- * the bed are the bed value (sediment ghost) */
-template <int SCH>
-__device__ __forceinline__ double us_fz(const emstr_dev &d, int col, int k, int kbc,
-                                        const double *__restrict__ t, double dtu, int ult) {
-  const ll P = d.plane;
-  const ll idx = (ll)k * P + col;
-  const double w = d.w[idx];
-  const double tc = t[idx];
-  const double tzm1 = t[(ll)max(k - 1, kbc) * P + col];
-  if (SCH == ORDER1)
-    return dtu * (0.5 * (w + fabs(w)) * tzm1 + 0.5 * (w - fabs(w)) * tc);
-  const double tzm2 = t[(ll)max(k - 2, kbc) * P + col];
-  const double tzp1 = t[(ll)min(k + 1, d.nz - 1) * P + col];
-  const double dzc = d.dz[idx];
-  const double dzb = (k > kbc) ? d.dz[idx - P] : 0.0;
-  const double w9 = 0.5 * (dzc + dzb);
-  const double cn = fmod(w * dtu / (w9 * 1.0), 1.0);
-  double F;
-  if (SCH == QUICKEST) {
-    /* tracers.c:2125-2150; w9 of the sediment ghost is 0 */
-    const double w5 = dzb / (dzb + dzc);
-    const double w9m = (k > kbc) ? 0.5 * (dzb + ((k - 1 > kbc) ? d.dz[idx - 2 * P] : 0.0)) : 0.0;
-    const double w9p = (k + 1 < d.nz) ? 0.5 * (d.dz[idx + P] + dzc) : w9;
-    const double db = 2.0 / (w9m + w9);
-    const double df = 2.0 / (w9 + w9p);
-    const double face = w5 * (tc - tzm1) + tzm1;
-    const double grad = tc - tzm1;
-    double curv = 0.0;
-    if (cn > 0.0)
-      curv = db * (w9m * tc + w9 * tzm2) - 2.0 * tzm1;
-    else if (cn < 0.0)
-      curv = df * (w9 * tzp1 + w9p * tzm1) - 2.0 * tc;
-    F = face - 0.5 * cn * grad - (1.0 - cn * cn) * curv / 6.0;
-  } else {
-    F = us_vl(w, cn, tc, tzp1, tzm1, tzm2);
-  }
-  if (ult) F = us_ult(cn, F, tc, tzm1, tzp1, tzm2);
-  return F * (w * dtu);
-}
-
 /* Runge-Kutta stage weights (rkweights of tracers.c:110-129 for one
  * stage): of the sub-step's starting value, of the stage's value, of its
  * tendency and volume */
@@ -323,7 +259,11 @@ struct UsStage {
 
 /* the vertical flux through the lower face of every cell to process
  * (k <= ks of a processed column) for every advected tracer, one thread
- * per wet cell (dense indices wdn) */
+ * per wet cell (dense indices wdn): van Leer (tracers.c:2405-2412), first
+ * order (1777-1785), QUICKEST (2125-2150), ULTIMATE (1712-1720), times
+ * w*dtu (1735-1738).  Tracer values at the layers below the bed are the
+ * bed value (sediment ghost).  The Courant number and the layer
+ * thicknesses are the cell's, computed once for every tracer */
 template <int SCH>
 __global__ void __launch_bounds__(256)
 k_usfz(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ wdn, int nwet,
@@ -338,9 +278,53 @@ k_usfz(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ wdn
   if (ksc < 0 || k > ksc) return;
   const int kbc = d.kb[col];
   const ll N3 = d.n3;
-  for (int n = 0; n < d.ntr; n++)
-    if (d.in_tbdy[n] && d.advect[n])
-      fz[n * N3 + idx] = us_fz<SCH>(d, col, k, kbc, tin + n * N3, dtu, ult);
+  const double w = d.w[idx];
+  const ll jm1 = (ll)max(k - 1, kbc) * P + col;
+  const ll jm2 = (ll)max(k - 2, kbc) * P + col;
+  const ll jp1 = (ll)min(k + 1, d.nz - 1) * P + col;
+  const double wp = 0.5 * (w + fabs(w)), wm = 0.5 * (w - fabs(w));
+  double cn = 0.0, omc = 0.0, w5 = 0.0, w9 = 0.0, w9m = 0.0, w9p = 0.0, db = 0.0, df = 0.0;
+  if (SCH != ORDER1) {
+    const double dzc = d.dz[idx];
+    const double dzb = (k > kbc) ? d.dz[idx - P] : 0.0;
+    w9 = 0.5 * (dzc + dzb);
+    cn = fmod(w * dtu / (w9 * 1.0), 1.0);
+    omc = 1.0 + ((w > 0.0) ? -cn : cn);
+    if (SCH == QUICKEST) {
+      /* w9 of the sediment ghost is 0 */
+      w5 = dzb / (dzb + dzc);
+      w9m = (k > kbc) ? 0.5 * (dzb + ((k - 1 > kbc) ? d.dz[idx - 2 * P] : 0.0)) : 0.0;
+      w9p = (k + 1 < d.nz) ? 0.5 * (d.dz[idx + P] + dzc) : w9;
+      db = 2.0 / (w9m + w9);
+      df = 2.0 / (w9 + w9p);
+    }
+  }
+  for (int n = 0; n < d.ntr; n++) {
+    if (!(d.in_tbdy[n] && d.advect[n])) continue;
+    const double *__restrict__ t = tin + n * N3;
+    const double tc = t[idx], tzm1 = t[jm1];
+    double F;
+    if (SCH == ORDER1) {
+      F = dtu * (wp * tzm1 + wm * tc);
+    } else {
+      const double tzm2 = t[jm2], tzp1 = t[jp1];
+      if (SCH == QUICKEST) {
+        const double face = w5 * (tc - tzm1) + tzm1;
+        const double grad = tc - tzm1;
+        double curv = 0.0;
+        if (cn > 0.0)
+          curv = db * (w9m * tc + w9 * tzm2) - 2.0 * tzm1;
+        else if (cn < 0.0)
+          curv = df * (w9 * tzp1 + w9p * tzm1) - 2.0 * tc;
+        F = face - 0.5 * cn * grad - (1.0 - cn * cn) * curv / 6.0;
+      } else {
+        F = (w > 0.0 || w < 0.0) ? us_vlm(w > 0.0, omc, tc, tzp1, tzm1, tzm2) : 0.0;
+      }
+      if (ult) F = us_ult(cn, F, tc, tzm1, tzp1, tzm2);
+      F = F * (w * dtu);
+    }
+    fz[n * N3 + idx] = F;
+  }
 }
 
 /* new value of the sub-surface cell at layer k (tracers.c:1000-1016) */
@@ -387,19 +371,30 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
   if (k < ksc) {
     /* the cell's edges at this layer: surface edge, sign, wet */
     const int npe = D.npe[col], ncol = (int)P;
+    /* eo: the flux's index, an in-range stand-in (edge 0 at this layer)
+     * where the far cell is dry or the edge is missing, so the loads carry
+     * no condition; ok selects */
     ll eo[US_MAXNPE];
+    bool ok[US_MAXNPE];
     double sg[US_MAXNPE];
 #pragma unroll
     for (int j = 0; j < US_MAXNPE; j++) {
-      eo[j] = -1;
+      eo[j] = (ll)k * D.ne2;
+      ok[j] = false;
       sg[j] = 0.0;
       if (j < npe) {
         const int e = D.ce[j * ncol + col];
         const int other = (D.ec1[e] == col) ? D.ec2[e] : D.ec1[e];
-        if (wetk(d.kb, other, k)) eo[j] = (ll)k * D.ne2 + e;
+        ok[j] = wetk(d.kb, other, k);
+        if (ok[j]) eo[j] += e;
         sg[j] = (double)D.esc[j * ncol + col];
       }
     }
+    /* the most edges of a cell in this wave: a uniform bound on the loads */
+    int npw = 0;
+#pragma unroll
+    for (int j = 0; j < US_MAXNPE; j++)
+      if (__any(j < npe)) npw = j + 1;
     const double area = d.cellarea[col], dzc = d.dz[idx];
     for (int n = 0; n < d.ntr; n++) {
       const double *t = tin + n * N3;
@@ -409,10 +404,15 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
       }
       const double *fxn = fx + n * NE3;
       /* dtracer (tracers.c:949-959) */
+      double fv[US_MAXNPE];
+#pragma unroll
+      for (int j = 0; j < US_MAXNPE; j++) fv[j] = (j < npw) ? fxn[eo[j]] : 0.0;
       double dtr = 0.0;
 #pragma unroll
-      for (int j = 0; j < US_MAXNPE; j++)
-        if (j < npe) dtr += (sg[j] * ((eo[j] >= 0) ? fxn[eo[j]] : 0.0) * dtu);
+      for (int j = 0; j < US_MAXNPE; j++) {
+        const double a = dtr + (sg[j] * (ok[j] ? fv[j] : 0.0) * dtu);
+        dtr = (j < npe) ? a : dtr;
+      }
       double FzB = 0.0, FzT = 0.0;
       if (d.advect[n]) {
         FzB = fz[n * N3 + idx];
