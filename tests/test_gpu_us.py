@@ -137,6 +137,31 @@ def test_moving_surface_step():
     assert_exact(gpu, ora)
 
 
+@pytest.mark.parametrize("trasc,ult", [(U.VANLEER, 0), (U.VANLEER, 1), (U.QUICKEST, 1)])
+def test_zero_transport_and_flat_fields(trasc, ult):
+    """Edges and faces without transport, and tracer plateaus where the
+    limiter's product is exactly zero or neighbours are equal: the faces
+    (branch-free van Leer with the mirrored stencil) keep the reference's
+    values."""
+    say("case hex zero transport, plateaus", trasc, ult)
+    case = MS.hex_basin(nx=20, ny=16, nz=6, ntr=3, seed=29, trasc=trasc, ultimate=ult)
+    rng = np.random.default_rng(5)
+    u1, f = case.ae(case.wd.u1), case.ae(case.wd.u1flux3d)
+    still = rng.random(u1.shape) < 0.3
+    u1[still] = 0.0
+    f[still] = 0.0
+    w = case.ac(case.wd.w)
+    w[rng.random(w.shape) < 0.3] = 0.0
+    case.prepare(OU.set_dz)
+    tr = case.tracers()
+    tr[0] = np.round(tr[0])
+    tr[1] = np.where(rng.random(tr[1].shape) < 0.5, 3.0, tr[1])
+    case.set_tracers(tr)
+    gpu, ora = run_both(case, 3)
+    assert np.abs(ora - tr).max() > 1e-3
+    assert np.array_equal(gpu, ora), np.abs(gpu - ora).max()
+
+
 RK_CASES = [
     # name, hex_basin kw, rkstage, stab, nsteps
     ("vanleer_rk2", dict(trasc=U.VANLEER), 2, U.NONE, 3),
