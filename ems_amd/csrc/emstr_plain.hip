@@ -145,29 +145,24 @@ struct QkGeom { /* QUICKEST metric weights of one face */
   double hm, h0, hp; /* spacings at bm1, c, fp1 (h1au1 / w9)          */
 };
 
-/* tracers.c:2097-2118 */
+/* tracers.c:2097-2118, with the stencil mirrored for a negative velocity:
+ * (bm2, bm1, c) for vel > 0, (fp1, c, bm1) for vel < 0.  For vel < 0 the
+ * reference forms df = (c - bm1) (fp1 - c), 2 df / (fp1 - bm1) and
+ * c - 0.5 (1 + cn) df; the mirrored product is the same product of negated
+ * differences and its quotient the negated quotient, so the face value is
+ * the same to the last bit, and no branch on the sign is taken (in a wave
+ * with both signs the quotient was computed twice) */
 __device__ __forceinline__ double vl_face(double bm2, double bm1, double c,
                                           double fp1, double vel, double cn,
                                           bool &set) {
-  double df;
-  set = true;
-  if (vel > 0.0) {
-    df = (bm1 - bm2) * (c - bm1);
-    if (df > 0.0)
-      df = 2.0 * df / (c - bm2);
-    else
-      df = 0.0;
-    return bm1 + 0.5 * (1.0 - cn) * df;
-  } else if (vel < 0.0) {
-    df = (c - bm1) * (fp1 - c);
-    if (df > 0.0)
-      df = 2.0 * df / (fp1 - bm1);
-    else
-      df = 0.0;
-    return c - 0.5 * (1.0 + cn) * df;
-  }
-  set = false;
-  return 0.0;
+  set = vel > 0.0 || vel < 0.0;
+  const bool pos = vel > 0.0;
+  const double a2 = pos ? bm2 : fp1, a1 = pos ? bm1 : c, a0 = pos ? c : bm1;
+  const double omc = 1.0 + (pos ? -cn : cn);
+  double df = (a1 - a2) * (a0 - a1);
+  df = (df > 0.0) ? 2.0 * df / (a0 - a2) : 0.0;
+  const double F = a1 + 0.5 * omc * df;
+  return set ? F : 0.0;
 }
 
 /* tracers.c:1862-1884 (non-uniform QUICKEST) */
