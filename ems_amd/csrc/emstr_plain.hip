@@ -724,13 +724,20 @@ k_upd(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout, do
   const double area = d.cellarea[col], dzc = d.dz[idx];
 
   if (k < ksc) {
+    /* face_dtr with the east and north wet tests taken once: a dry
+     * neighbour's face reads the cell's own (in range) and is dropped */
+    const bool wE = wetat(d.kb, col + 1, k), wN = wetat(d.kb, col + d.nx, k);
+    const ll iE = idx + (wE ? 1 : 0), iN = idx + (wN ? (ll)d.nx : 0);
     for (int n = 0; n < d.ntr; n++) {
       const double *t = tin + n * N3;
       if (!d.in_tbdy[n]) {
         tout[n * N3 + idx] = t[idx];
         continue;
       }
-      const double dtr = face_dtr(d, fw + n * N3, fs + n * N3, col, k, dtu);
+      const double *fwn = fw + n * N3, *fsn = fs + n * N3;
+      const double FxW = fwn[idx], FyS = fsn[idx], vE = fwn[iE], vN = fsn[iN];
+      const double FxE = wE ? vE : 0.0, FyN = wN ? vN : 0.0;
+      const double dtr = (FxE - FxW + FyN - FyS) * dtu;
       double FzB = 0.0, FzT = 0.0;
       if (d.advect[n]) {
         FzB = fb[n * N3 + idx];
