@@ -147,20 +147,20 @@ struct Fc {
 };
 
 /* van Leer (tracers.c:2097-2118); the sign of q is the sign of the
- * velocity */
+ * velocity.  The stencil is mirrored for q < 0: (bm2, bm1, c) for q > 0,
+ * (fp1, c, bm1) for q < 0, with 1 - cn and 1 + cn.  The mirrored product is
+ * the same product of negated differences and its quotient the negated
+ * quotient of the reference's c - 0.5 (1 + cn) df, so the value is the same
+ * to the last bit without a branch on the sign */
 __device__ __forceinline__ float vlf(float bm2, float bm1, float c, float fp1,
                                      float q, float cn) {
-  float df;
-  if (q > 0.0f) {
-    df = (bm1 - bm2) * (c - bm1);
-    df = (df > 0.0f) ? 2.0f * df / (c - bm2) : 0.0f;
-    return bm1 + 0.5f * (1.0f - cn) * df;
-  } else if (q < 0.0f) {
-    df = (c - bm1) * (fp1 - c);
-    df = (df > 0.0f) ? 2.0f * df / (fp1 - bm1) : 0.0f;
-    return c - 0.5f * (1.0f + cn) * df;
-  }
-  return 0.0f;
+  const bool pos = q > 0.0f;
+  const float a2 = pos ? bm2 : fp1, a1 = pos ? bm1 : c, a0 = pos ? c : bm1;
+  const float omc = 1.0f + (pos ? -cn : cn);
+  float df = (a1 - a2) * (a0 - a1);
+  df = (df > 0.0f) ? 2.0f * df / (a0 - a2) : 0.0f;
+  const float F = a1 + 0.5f * omc * df;
+  return (pos || q < 0.0f) ? F : 0.0f;
 }
 
 /* QUICKEST (tracers.c:1862-1884) */
