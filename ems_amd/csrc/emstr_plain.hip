@@ -908,7 +908,8 @@ __device__ __forceinline__ double p_val(const double *a, const double *t, const 
                                         ll p) {
   const int g = pgh(p);
   const ll idx = pidx(p);
-  return (g == P_WET) ? a[idx] : ((g == P_SED) ? t[idx] : ts[idx]);
+  const double *b = (g == P_WET) ? a : ((g == P_SED) ? t : ts);
+  return b[idx];
 }
 /* a 3-D cell field (dz, w9) at p: 0 at ghosts (never set there) */
 __device__ __forceinline__ double p_f3(const double *f, ll p) {
@@ -1172,24 +1173,17 @@ __device__ __forceinline__ double ff_vl(const emstr_dev &d, const double *a, con
     bm2 = p_zm1(d, bm1);
     if (vel < 0.0) fp = p_zp1(d, cp);
   }
-  double df;
+  /* the stencil mirrored for vel <= 0: (bm2, bm1, cp) for vel > 0,
+   * (fp, cp, bm1) otherwise; the reference's acp - 0.5 (1 - crf) df of
+   * df = (acp - abm1) (afp - acp), 2 df / (afp - abm1) is the same value
+   * with the negated quotient, to the last bit, without a branch */
+  const bool pos = vel > 0.0;
   const double acp = p_val(a, t, ts, cp), abm1 = p_val(a, t, ts, bm1);
-  if (vel > 0.0) {
-    const double abm2 = p_val(a, t, ts, bm2);
-    df = (abm1 - abm2) * (acp - abm1);
-    if (df > 0.0)
-      df = 2.0 * df / (acp - abm2);
-    else
-      df = 0.0;
-    return abm1 + 0.5 * (1.0 - crf) * df;
-  }
-  const double afp = p_val(a, t, ts, fp);
-  df = (acp - abm1) * (afp - acp);
-  if (df > 0.0)
-    df = 2.0 * df / (afp - abm1);
-  else
-    df = 0.0;
-  return acp - 0.5 * (1.0 - crf) * df;
+  const double a2 = p_val(a, t, ts, pos ? bm2 : fp);
+  const double a1 = pos ? abm1 : acp, a0 = pos ? acp : abm1;
+  double df = (a1 - a2) * (a0 - a1);
+  df = (df > 0.0) ? 2.0 * df / (a0 - a2) : 0.0;
+  return a1 + 0.5 * (1.0 - crf) * df;
 }
 
 /* fluxes along DIR of one tracer from the modified tracer ftm
