@@ -572,23 +572,19 @@ k_fffz(emstr_usdev D, double dtu, const double *__restrict__ tin, const double *
 #define FFL(L) ((ll)(((L) < kb) ? kb : (((L) > ks) ? ks : (L))) * P + col)
 #define FFM(L) (MARK ? ffmark(need, FFL(L), stamp) : mc[FFL(L)])
 #define FFV(L) (usemoc ? FFM(L) : FFT(L))
-    double F = 0.0, df;
-    if (cr != 0.0) {
-      if (w > 0.0) {
-        df = (FFV(bm1) - FFV(bm2)) * (FFV(cp) - FFV(bm1));
-        if (df > 0.0)
-          df = 2.0 * df / (FFV(cp) - FFV(bm2));
-        else
-          df = 0.0;
-        F = FFV(bm1) + 0.5 * (1.0 - cr) * df;
-      } else if (w < 0.0) {
-        df = (FFV(cp) - FFV(bm1)) * (FFV(fp) - FFV(cp));
-        if (df > 0.0)
-          df = 2.0 * df / (FFV(fp) - FFV(bm1));
-        else
-          df = 0.0;
-        F = FFV(cp) - 0.5 * (1.0 - cr) * df;
-      }
+    /* the stencil mirrored for w < 0: (bm2, bm1, cp) for w > 0, (fp, cp,
+     * bm1) for w < 0; the reference's cp - 0.5 (1 - cr) df of
+     * df = (cp - bm1) (fp - cp), 2 df / (fp - bm1) is the same value with
+     * the negated quotient, to the last bit, and the same three cells are
+     * read (and marked) */
+    double F = 0.0;
+    if (cr != 0.0 && (w > 0.0 || w < 0.0)) {
+      const bool pos = w > 0.0;
+      const int l2 = pos ? bm2 : fp, l1 = pos ? bm1 : cp, l0 = pos ? cp : bm1;
+      const double a2 = FFV(l2), a1 = FFV(l1), a0 = FFV(l0);
+      double df = (a1 - a2) * (a0 - a1);
+      df = (df > 0.0) ? 2.0 * df / (a0 - a2) : 0.0;
+      F = a1 + 0.5 * (1.0 - cr) * df;
     }
     if (w == 0.0 || k == kb) {
       F = 0.0;
