@@ -311,8 +311,10 @@ k_ffmod(emstr_usdev D, const double *__restrict__ tin, const double *__restrict_
         double r = 0.0;
         if (d.in_tbdy[n] && d.advect[n]) {
           const double *tt = tin + n * N3;
-          r = FF_TF * tt[ic] + (1.0 - FF_TF) * tt[ik];
-          if (crf > 0.) r += FF_TF * (crf * (tt[iz] - tt[ic]));
+          const double tc = tt[ic], tk = tt[ik], tz = tt[iz]; /* iz in range */
+          r = FF_TF * tc + (1.0 - FF_TF) * tk;
+          const double rz = r + FF_TF * (crf * (tz - tc));
+          r = (crf > 0.) ? rz : r;
         }
         S[c][j] = r;
       }
