@@ -21,6 +21,8 @@ _P = C.POINTER(S.WinPriv)
 _CTX = C.c_void_p
 PROTOS = [
     # drop-in entry points (include/emstr.h)
+    ("tracer_step", None, [C.POINTER(S.Master), C.POINTER(_G), C.POINTER(_W), C.POINTER(_P),
+                           C.c_int]),
     ("tracer_step_3d", None, [_G, _W, _P]),
     ("advect_diffuse", C.c_int, [_G, _W, _P]),
     ("vert_diffuse_3d", None, [_G, _W, _P]),

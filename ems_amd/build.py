@@ -57,10 +57,11 @@ def build_us(verbose=False):
              ["-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     for s in US_SHARED_HIP:
-        o = os.path.join(BUILD, s.replace(".hip", ".o"))
-        if not os.path.exists(o):
-            _run([hipcc] + HIPFLAGS + HIP_EXTRA.get(s, []) + INC +
-                 ["-c", os.path.join(CSRC, s), "-o", o])
+        # compiled again here (never an object left by build()), so that
+        # the COMPAS library always links the current source
+        o = os.path.join(BUILD, "us_" + s.replace(".hip", ".o"))
+        _run([hipcc] + HIPFLAGS + HIP_EXTRA.get(s, []) + INC +
+             ["-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     for s in US_C_SRC:
         o = os.path.join(BUILD, "us_" + s.replace(".c", ".o"))

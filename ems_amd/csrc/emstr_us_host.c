@@ -1444,6 +1444,51 @@ void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon) {
   us_sync(c);
 }
 
+/* model/hd-us/tracers/tracers.c:177-239 over closed z-level windows: the
+ * lateral ghosts of every window's tracers take their interior values,
+ * then each window's step runs on the GPU in the order twin and its
+ * tracers go back to the master, then the mass diagnostics.  The other
+ * calls there do nothing at closed basins without multi-dt, flushing or
+ * regions; the edge flux ghosts are not read (no flux through land). */
+void tracer_step(master_t *master, geometry_t **window, window_t **windat,
+                 win_priv_t **wincon, int nwindows) {
+  int nn, n, cc, t;
+  emstr_clear_error();
+  if (!master || !window || !windat || !wincon || nwindows < 1) {
+    emstr_set_error(E_ARG, -1, "tracer_step: null structure or no windows");
+    if (master) master->crf = RS_RESTART;
+    return;
+  }
+  if (master->mode2d) {
+    emstr_set_error(E_UNSUP, -1, "tracer_step: 2D mode (tracer_step_2d) is not built");
+    master->crf = RS_RESTART;
+    return;
+  }
+  const int *twin = wincon[1]->twin;
+  for (nn = 1; nn <= nwindows; nn++) {
+    n = twin ? twin[nn] : nn;
+    geometry_t *g = window[n];
+    window_t *wd = windat[n];
+    for (t = 0; t < wd->ntr; t++) {
+      double *tr = wd->tr_wc[t];
+      for (cc = 1; cc <= g->nbpt; cc++) tr[g->bpt[cc]] = tr[g->bin[cc]];
+    }
+  }
+  for (nn = 1; nn <= nwindows; nn++) {
+    n = twin ? twin[nn] : nn;
+    tracer_step_3d(window[n], windat[n], wincon[n]);
+    if (emstr_us_last_error()[0]) {
+      master->crf = RS_RESTART;
+      return;
+    }
+    if (master->win_data_empty_3d) master->win_data_empty_3d(master, window[n], windat[n], TRACERS);
+  }
+  for (nn = 1; nn <= nwindows; nn++) {
+    n = twin ? twin[nn] : nn;
+    if (master->win_data_empty_3d) master->win_data_empty_3d(master, window[n], windat[n], TMASS);
+  }
+}
+
 /*-------------------------------------------------------------------*/
 /* FFSL maps of the last sub-step, in the window's numbering          */
 /*-------------------------------------------------------------------*/

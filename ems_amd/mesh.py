@@ -50,6 +50,9 @@ def lib():
             if hasattr(L, fn):
                 getattr(L, fn).argtypes = [G, W, P]
                 getattr(L, fn).restype = rt
+        L.tracer_step.argtypes = [C.POINTER(U.Master), C.POINTER(G), C.POINTER(W),
+                                  C.POINTER(P), C.c_int]
+        L.tracer_step.restype = None
         if hasattr(L, "emstr_us_create"):
             L.emstr_us_create.restype = C.c_void_p
             L.emstr_us_create.argtypes = [G, W, P]

@@ -67,7 +67,13 @@ class WinPriv(C.Structure):
         ("d1", _PD), ("d2", _PD), ("d3", _PD), ("d4", _PD),
         ("v1", _PD), ("v2", _PD), ("v3", _PD), ("v4", _PD), ("v5", _PD),
         ("v6", _PD), ("v7", _PD),
-        ("b1", _D),
+        ("b1", _D), ("twin", _PI),
+    ]
+
+
+class Master(C.Structure):
+    _fields_ = [
+        ("mode2d", _I), ("crf", _I), ("win_data_empty_3d", C.c_void_p),
     ]
 
 
@@ -89,6 +95,9 @@ VANLEER = 0x000080
 ORDER2_UW = 0x002000
 FFSL = 0x080000
 SUB_STEP = 2
+TMASS = 16
+TRACERS = 256
+RS_RESTART = 0x0004
 NONE = 0x80000000 - (1 << 32)  # as a signed C int
 
 

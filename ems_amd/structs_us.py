@@ -83,7 +83,14 @@ class WinPriv(C.Structure):
     ] + [("w%d" % i, _PD) for i in range(1, 11)] + \
         [("d%d" % i, _PD) for i in range(1, 5)] + \
         [("v%d" % i, _PD) for i in range(1, 8)] + [("b1", _D)] + \
-        [("flag", _I), ("fillf", _I), ("compatible", _I), ("lw", C.POINTER(QWeights))]
+        [("flag", _I), ("fillf", _I), ("compatible", _I), ("lw", C.POINTER(QWeights)),
+         ("twin", _PI)]
+
+
+class Master(C.Structure):
+    _fields_ = [
+        ("mode2d", _I), ("crf", _I), ("win_data_empty_3d", C.c_void_p),
+    ]
 
 
 class Mesh(C.Structure):

@@ -162,13 +162,42 @@ typedef struct win_priv {
   double *d1, *d2, *d3, *d4;
   double *v1, *v2, *v3, *v4, *v5, *v6, *v7;
   double b1; /* current sub-step (set in advect_diffuse)             */
+  int *twin; /* order to process the windows, twin[1..nwindows]
+                (tracer_step reads wincon[1]->twin; NULL: 1..nwindows) */
 } win_priv_t;
+
+/*-------------------------------------------------------------------*/
+/* Master data (the fields of the reference master_t,               */
+/* model/hd/include/sparse.h:1050-2390, that tracer_step reads)     */
+/*-------------------------------------------------------------------*/
+/* master_t copy modes: hd_params.h:295-299 ; crash flag: :735      */
+#define TMASS 16
+#define TRACERS 256
+#define RS_RESTART 0x0004
+typedef struct master master_t;
+struct master {
+  int mode2d; /* 2D mode: only the 3D step is built (rejected)        */
+  int crf;    /* crash recovery flag: set to RS_RESTART on a failure  */
+  /* master <- window copies after each window's step (TRACERS) and
+   * of the mass diagnostics (TMASS); NULL when the caller keeps no
+   * master (one window)                                              */
+  void (*win_data_empty_3d)(master_t *master, geometry_t *window, window_t *windat,
+                            int mode);
+};
 
 /*-------------------------------------------------------------------*/
 /* Drop-in entry points (reference signatures).  They run the step  */
 /* on the GPU: fields are uploaded, the tracers are updated on the  */
 /* device and copied back into windat->tr_wc in place.              */
 /*-------------------------------------------------------------------*/
+/* model/hd/tracers/tracers.c:142-209: every window of the step, 1-based
+ * arrays window[1..nwindows] in the order wincon[1]->twin; per window the
+ * lateral ghost fill of the tracers (set_lateral_BC_tr, tracers.c:5574),
+ * tracer_step_3d on the GPU and master->win_data_empty_3d(TRACERS), then
+ * the TMASS copies.  A failure sets master->crf = RS_RESTART and returns,
+ * as the reference's crash path does (tracers.c:119, 175).            */
+void tracer_step(master_t *master, geometry_t **window, window_t **windat,
+                 win_priv_t **wincon, int nwindows);
 /* model/hd/tracers/tracers.c:239 */
 void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon);
 /* model/hd/tracers/tracers.c:401 ; returns 0 on success */

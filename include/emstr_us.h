@@ -193,11 +193,38 @@ typedef struct win_priv {
   int fillf;        /* FILL_METHOD (MONOTONIC / CLIP: rejected)       */
   int compatible;   /* V6898: rejected with FFSL                      */
   qweights *lw;     /* [szc] weights of build_linear_weights (2D fit) */
+  int *twin;        /* order to process the windows, twin[1..nwindows]
+                       (tracer_step reads wincon[1]->twin; NULL: 1..n) */
 } win_priv_t;
+
+/*-------------------------------------------------------------------*/
+/* Master data (the fields of the COMPAS master_t,                    */
+/* model/hd-us/include/sparse.h:1300-2390, that tracer_step reads)    */
+/*-------------------------------------------------------------------*/
+/* copy modes: model/hd-us/include/hd_params.h ; crash flag RS_RESTART */
+#define TMASS 16
+#define TRACERS 256
+#define RS_RESTART 0x0004
+typedef struct master master_t;
+struct master {
+  int mode2d; /* 2D mode: only the 3D step is built (rejected)          */
+  int crf;    /* crash recovery flag: set to RS_RESTART on a failure    */
+  /* master <- window copies (TRACERS after each window, then TMASS);
+   * NULL when the caller keeps no master                              */
+  void (*win_data_empty_3d)(master_t *master, geometry_t *window, window_t *windat,
+                            int mode);
+};
 
 /*-------------------------------------------------------------------*/
 /* Drop-in entry points (COMPAS signatures)                          */
 /*-------------------------------------------------------------------*/
+/* model/hd-us/tracers/tracers.c:177-239: every window, 1-based arrays in
+ * the order wincon[1]->twin: lateral ghost fill of the tracers
+ * (set_lateral_BC_tr), tracer_step_3d on the GPU, win_data_empty_3d
+ * (TRACERS), then the TMASS copies; a failure sets master->crf =
+ * RS_RESTART and returns.                                             */
+void tracer_step(master_t *master, geometry_t **window, window_t **windat,
+                 win_priv_t **wincon, int nwindows);
 /* model/hd-us/tracers/tracers.c:261 */
 void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon);
 /* model/hd-us/tracers/tracers.c:428; 0 on success, else the column

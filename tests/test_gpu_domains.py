@@ -5,6 +5,7 @@ point, as the reference's window loop calls tracer_step_3d per window
 every window; the FP32 forms must stay within 1e-5 of each tracer's
 range, with relative mass drift within 1e-6, on the stiff windows whose
 diffusion numbers reach ~1e4."""
+import ctypes as C
 import sys
 
 import numpy as np
@@ -43,6 +44,50 @@ def test_c5_windows_interleaved_bit_exact():
             msg, cell = _lib.last_error()
             assert msg == "", (c.name, msg, cell)
     gpu = [c.tracers() for c in doms]
+    for c, snap, g in zip(doms, snaps, gpu):
+        H.restore(c, snap)
+        for _ in range(2):
+            assert O.step(c, 8) == 0
+        ref = c.tracers()
+        L.emstr_forget_window(c.gp)
+        assert np.array_equal(g, ref), (c.name, np.abs(g - ref).max())
+
+
+def test_tracer_step_window_loop_bit_exact():
+    """tracer_step (tracers.c:142-209) over 12 c5 windows in a shuffled
+    twin order, with a master callback that records the copies: every
+    window equals the oracle bit for bit, the copies come per window
+    (TRACERS) in the twin order, then TMASS for every window."""
+    say("tracer_step: 12 windows, twin order, master callbacks")
+    doms = configs.c5(ndom=12)
+    L = _lib.lib()
+    n = len(doms)
+    for c in doms:
+        c.prepare(O.set_dz)
+    snaps = [H.snapshot(c) for c in doms]
+    G = (C.POINTER(S.Geometry) * (n + 1))(None, *[c.gp for c in doms])
+    W = (C.POINTER(S.Window) * (n + 1))(None, *[c.wp for c in doms])
+    P = (C.POINTER(S.WinPriv) * (n + 1))(None, *[c.pp for c in doms])
+    order = [0] + list(np.random.default_rng(4).permutation(n) + 1)
+    twin = (C.c_int * (n + 1))(*order)
+    doms[0].wc.twin = C.cast(twin, C.POINTER(C.c_int))
+    seen = []
+    wins = {C.addressof(c.g): i + 1 for i, c in enumerate(doms)}
+    CB = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(S.Geometry), C.POINTER(S.Window), C.c_int)
+
+    def empty(master, window, windat, mode):
+        seen.append((wins[C.addressof(window.contents)], mode))
+
+    cb = CB(empty)
+    m = S.Master(mode2d=0, crf=0, win_data_empty_3d=C.cast(cb, C.c_void_p))
+    for _ in range(2):
+        L.tracer_step(C.byref(m), G, W, P, n)
+        msg, cell = _lib.last_error()
+        assert msg == "" and m.crf == 0, (msg, cell)
+    want = ([(w, S.TRACERS) for w in order[1:]] + [(w, S.TMASS) for w in order[1:]]) * 2
+    assert seen == want
+    gpu = [c.tracers() for c in doms]
+    doms[0].wc.twin = None
     for c, snap, g in zip(doms, snaps, gpu):
         H.restore(c, snap)
         for _ in range(2):
