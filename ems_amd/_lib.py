@@ -40,6 +40,7 @@ PROTOS = [
     ("emstr_advect_diffuse", C.c_int, [_CTX]),
     ("emstr_vert_diffuse", C.c_int, [_CTX]),
     ("emstr_tracer_mass", C.c_int, [_CTX, C.POINTER(C.c_double)]),
+    ("emstr_tracer_mass_noalloc", C.c_int, [_CTX, C.POINTER(C.c_double)]),
     ("emstr_device_set_dz", C.c_int, [_CTX]),
     ("emstr_device_vel_w", C.c_int, [_CTX]),
     ("emstr_profile", C.c_int, [_CTX, C.c_int]),

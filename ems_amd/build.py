@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 INC = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SRC = ["emstr_plain.hip", "emstr_prec.hip", "emstr_batch.hip"]
+HIP_SRC = ["emstr_plain.hip", "emstr_prec.hip", "emstr_batch.hip", "emstr_scratch.hip"]
 C_SRC = ["emstr_host.c", "emstr_grid.c", "emstr_err.c"]
 
 HIPFLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",

@@ -25,6 +25,7 @@
 #include "emstr_batch.h"
 #include "emstr_prec.h"
 #include "emstr_prof.h"
+#include "emstr_scratch.h"
 
 #define HCHK(x)                                                              \
   do {                                                                       \
@@ -940,6 +941,37 @@ int emstr_tracer_mass(emstr_ctx *c, double *mass) {
     }
     mass[n] = s + e;
   }
+  return 0;
+}
+
+/* emstr_tracer_mass without any allocation: the partials and the totals
+ * in the library's fixed device scratch (emstr_scratch.hip), the final
+ * reduction on the device in the host reduction's order. */
+int emstr_tracer_mass_noalloc(emstr_ctx *c, double *mass) {
+  emstr_dev *d = &c->d;
+  const long long nb = emstr_mass_blocks(d);
+  const long long need = 2 * nb * d->ntr + d->ntr;
+  emstr_clear_error();
+  if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
+  if (need > EMSTR_SCRATCH_DOUBLES)
+    return emstr_set_error(EMSTR_EARG, -1,
+                           "mass without allocation: %lld partials for %d tracers exceed the "
+                           "%d doubles of scratch (2 * ceil(columns / 256) * ntr + ntr)",
+                           need, d->ntr, EMSTR_SCRATCH_DOUBLES);
+  double *s = emstr_scratch_ptr();
+  if (!s) return EMSTR_EDEVICE;
+  double *tot = s + 2 * nb * d->ntr;
+  if (c->batch) {
+    if (emstr_batch_mass(c->batch, d, s, c->st)) return EMSTR_EDEVICE;
+  } else if (c->prec) {
+    if (emstr_prec_mass(c->prec, d, s, c->st)) return EMSTR_EDEVICE;
+  } else if (emstr_launch_mass(d, c->tr[c->cur], NULL, s, c->st)) {
+    return emstr_set_error(EMSTR_EDEVICE, -1, "mass launch");
+  }
+  if (emstr_launch_mass_reduce(s, (int)nb, d->ntr, tot, c->st))
+    return emstr_set_error(EMSTR_EDEVICE, -1, "mass reduction launch");
+  HCHK(hipMemcpyAsync(mass, tot, (size_t)d->ntr * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
   return 0;
 }
 

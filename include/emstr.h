@@ -263,6 +263,16 @@ int emstr_device_set_dz(emstr_ctx *ctx);
 int emstr_device_vel_w(emstr_ctx *ctx);
 /* Domain tracer mass sum(tr*cellarea*dz) over s1, FP64 compensated.   */
 int emstr_tracer_mass(emstr_ctx *ctx, double *mass /* [ntr] */);
+/* emstr_tracer_mass that allocates nothing (no hipMalloc, hipFree or
+ * allocating library call): no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid it).  The partial sums and totals
+ * live in a fixed 4 MiB device array of the library (one per GPU, kept
+ * until the process ends), which limits 2 * ceil(columns / 256) * ntr +
+ * ntr to 524288 doubles (EMSTR_EARG beyond).  Same values, bit for bit,
+ * as emstr_tracer_mass.  Calls that overlap on one GPU share that array
+ * and race.                                                            */
+int emstr_tracer_mass_noalloc(emstr_ctx *ctx, double *mass /* [ntr] */);
 /* Per-kernel timing with HIP events on the context's stream: enable
  * (and reset), then read accumulated milliseconds and launch counts per
  * kernel class [advection, vertical diffusion, prep, other, fused].   */

@@ -27,7 +27,7 @@ int main(void) {
   void *eps[] = {(void *)tracer_step, (void *)tracer_step_3d, (void *)advect_diffuse,
                  (void *)vert_diffuse_3d, (void *)set_dz, (void *)emstr_create,
                  (void *)emstr_steps, (void *)emstr_tracer_mass,
-                 (void *)emstr_forget_window};
+                 (void *)emstr_tracer_mass_noalloc, (void *)emstr_forget_window};
   for (unsigned i = 0; i < sizeof(eps) / sizeof(eps[0]); i++)
     if (!eps[i]) return 10 + (int)i;
   master_t m;
