@@ -200,12 +200,12 @@ def main():
             tc.upload()
             best, trials = autotune.tune_context(tc)
             tc.close()
-            c.tune(best[0], best[1], best[2])
-            tuned = {"tile": [best[0], best[1]], "grid": best[2],
-                               "ms_per_step": best[3],
-                               "trials": [[t[0], t[1], t[2], round(t[3], 4)] for t in trials]}
-            print("bench: autotune -> tile %dx%d grid %d (%.4f ms/step)" % best,
-                  file=sys.stderr, flush=True)
+            c.tune(best[0], best[1], 0)
+            tuned = {"block": [best[0], 64 // best[0]], "strip": best[1],
+                     "ms_per_step": best[2],
+                     "trials": [[t[0], t[1], round(t[2], 4)] for t in trials]}
+            print("bench: autotune -> block %d x %d, strip %d (%.4f ms/step)"
+                  % (best[0], 64 // best[0], best[1], best[2]), file=sys.stderr, flush=True)
         m0 = c.mass()
         c.step(W)
         ms = timed_steps(c, ev, K)
@@ -222,7 +222,7 @@ def main():
            "config": {"workload": "c3: 400x600x40 curvilinear basin, 25% land, 2-300 m, "
                                   "40 tracers, van Leer, dt=90 s",
                       "n3_wet": nwet, "ntr": ntr, "columns": int(case.g.v2_t),
-                      "form": "batched FP32 (fused persistent kernel, hipGraph)"}}
+                      "form": "batched FP32 (fused advection + column solve, hipGraph)"}}
     if "batched" in res:
         rB = res["batched"]
         out["value"] = cu / (rB["ms"] * 1e-3)
@@ -233,13 +233,14 @@ def main():
         # advection reads and writes every tracer of every wet cell once and
         # reads the cell's 16 coefficient words; the vertical solve reads
         # and writes every tracer once and reads 4 coefficient words
-        alg = {0: nwet * (ntr * 8 + 16 * 4),
-               1: nwet * (ntr * 8 + 4 * 4),
-               4: nwet * (ntr * 8 + 16 * 4),
-               2: nwet * (7 * 8 + 16 * 4)}.get(dom, nwet * ntr * 8)
+        # the fused step reads and writes every tracer of every wet cell
+        # once and reads the cell's 12 coefficient words; the prep reads
+        # the 7 FP64 fields and writes the coefficients
+        alg = {4: nwet * (ntr * 8 + 12 * 4),
+               2: nwet * (7 * 8 + 12 * 4)}.get(dom, nwet * ntr * 8)
         avg_ms = ktB[dom] / max(kcB[dom], 1)
         achieved = alg / (avg_ms * 1e-3) / 1e9
-        kname = {0: "k_advB", 1: "k_vertB"}.get(dom)
+        kname = {4: "k_step2", 2: "k_prep2"}.get(dom)
         out["roofline"] = {"bound": "hbm", "kernel": "batched " + names[dom],
                            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS,
@@ -469,8 +470,8 @@ def bench_ensemble(args):
     tuned = None
     if not args.no_tune:
         from ems_amd import autotune
-        sc, tms, trials = autotune.tune_ensemble(ctxs)
-        tuned = {"grid_scale": sc, "ms_per_step": tms,
+        bw, tms, trials = autotune.tune_ensemble(ctxs)
+        tuned = {"block_width": bw, "ms_per_step": tms,
                  "trials": [[t[0], round(t[1], 4)] for t in trials]}
     # the ensemble graph
     ens = T.Ensemble(ctxs)

@@ -1,26 +1,34 @@
 """Autotuner for the batched FP32 form's launch shapes.
 
-The batched kernels walk cells and columns in tile order (tile_x x tile_y
-columns per tile) over persistent grids sized to fill the GPU.  Which tile
-keeps a wave's neighbours in L2 best, and how many workgroups a window
-should occupy when several windows share the GPU (an ensemble graph),
-depend on the window's shape and depth distribution, so they are picked
-by timing a few steps of each candidate with HIP events on the context's
-own stream.  Results do not depend on the shape (the stochastic rounding
-is keyed by cell, tracer and step, not by the thread), so tuning never
-changes the answer, only the time.
+The batched step runs one wave per (block of 64 columns, tracer).  Two
+shape parameters change its speed but not its results:
+  * the block shape, tile_x x 64 / tile_x columns (tile_x = 8, 16, 32 or
+    64): a wave walks from the deepest bed of its block to the top, so
+    blocks over uneven bathymetry waste lanes on the shallow columns
+    (wide blocks bin columns of one row together, square blocks columns
+    of one patch), while wider blocks read longer aligned row segments;
+  * the strip width (tile_y, columns): blocks are ordered by vertical
+    strips, row by row inside a strip, and the sequence is cut into 8
+    work-balanced chunks, one per XCD; the strip width sets how far apart
+    a block and the block holding its south halo rows run on one XCD
+    (L2 reuse) against how many halo columns fall between XCDs.
+Which is fastest depends on the window's shape and depth distribution,
+so they are picked by timing a few steps of each candidate with HIP
+events on the context's own stream.  Results do not depend on the shape
+(the stochastic rounding is keyed by cell, tracer and step, not by the
+thread), so tuning never changes the answer, only the time.
 
-    tune_context(ctx)            tiles and grid of one batched context
-    tune_ensemble(ctxs)          per-window grids, in proportion to each
-                                 window's cells, for one ensemble graph
+    tune_context(ctx)            block and strip of one batched context
+    tune_ensemble(ctxs)          one block width for every window of an
+                                 ensemble graph
 """
 import ctypes as C
 
 from . import _lib
 from . import tracer as T
 
-TILES = [(32, 32), (16, 16), (64, 16), (16, 64), (32, 8), (8, 32), (64, 64)]
-GRID_SCALES = [1.0, 0.5, 0.25]
+BLOCKS = [16]
+STRIPS = [32, 64, 128, 256]
 
 
 class _Events:
@@ -53,75 +61,69 @@ class _Events:
 
 
 def shape(ctx):
-    """(tile_x, tile_y, grid, full_grid) of a batched context."""
+    """(block width, strip width, workgroups per tracer pass, same) of a
+    batched context."""
     v = [C.c_int() for _ in range(4)]
     T._chk(_lib.lib().emstr_launch_shape(ctx.ptr, *[C.byref(x) for x in v]),
            "emstr_launch_shape")
     return tuple(x.value for x in v)
 
 
-def tune_context(ctx, steps=4, tiles=TILES, scales=GRID_SCALES):
+def tune_context(ctx, steps=4, blocks=BLOCKS, strips=STRIPS):
     """Times `steps` steps (an even number: one graph replay holds two) of
-    every tile in `tiles` at the full grid, then the best tile at each
-    grid scale; applies the fastest and returns (best, trials) where best
-    is (tile_x, tile_y, grid, ms_per_step)."""
+    every block width in `blocks` at the first strip width, then the best
+    block at every strip width; applies the fastest and returns (best,
+    trials) where best is (block width, strip width, ms_per_step)."""
     if ctx.form != T.BATCH_F32:
         raise ValueError("tune_context: batched contexts only")
     steps = max(2, steps // 2 * 2)
     L = _lib.lib()
-    full_grid = shape(ctx)[3]
     ev = _Events()
     trials = []
 
-    def run(tx, ty, grid):
-        T._chk(L.emstr_tune(ctx.ptr, tx, ty, grid, 1), "emstr_tune")
+    def run(tx, sw):
+        T._chk(L.emstr_tune(ctx.ptr, tx, sw, 0, 1), "emstr_tune")
         ctx.step(2)  # captures the graph for this shape
         ctx.sync()
         ms = ev.time(ctx.stream, lambda: ctx.step(steps)) / steps
-        trials.append((tx, ty, grid, ms))
+        trials.append((tx, sw, ms))
         return ms
 
-    for tx, ty in tiles:
-        run(tx, ty, full_grid)
-    tx, ty, _, _ = min(trials, key=lambda r: r[3])
-    for s in scales:
-        if s != 1.0:
-            run(tx, ty, max(8, int(full_grid * s) // 8 * 8))
-    best = min(trials, key=lambda r: r[3])
-    T._chk(L.emstr_tune(ctx.ptr, best[0], best[1], best[2], 1), "emstr_tune")
+    for tx in blocks:
+        run(tx, strips[0])
+    tx = min(trials, key=lambda r: r[2])[0]
+    for sw in strips[1:]:
+        run(tx, sw)
+    best = min(trials, key=lambda r: r[2])
+    T._chk(L.emstr_tune(ctx.ptr, best[0], best[1], 0, 1), "emstr_tune")
     ctx.step(2)
     ctx.sync()
     ev.close()
     return best, trials
 
 
-def tune_ensemble(ctxs, steps=4, scales=(1.0, 0.5, 0.25, 0.125, 0.0625)):
-    """Per-window advection grids for one ensemble graph: window i gets
-    max(8, s * full_grid * cells_i / max_cells) workgroups; every scale s
+def tune_ensemble(ctxs, steps=4, blocks=BLOCKS):
+    """One block width for every window of an ensemble graph: each width
     is timed on a fresh ensemble graph and the fastest is kept.  Returns
-    (best_scale, ms_per_step, trials) and leaves the contexts tuned to it."""
+    (best_width, ms_per_step, trials) and leaves the contexts tuned to it."""
     steps = max(2, steps // 2 * 2)
     L = _lib.lib()
     ev = _Events()
-    cells = [c.case.nwet for c in ctxs]
-    cmax = float(max(cells))
-    full_grid = shape(ctxs[0])[3]
     trials = []
 
-    def apply(s):
-        for c, n in zip(ctxs, cells):
-            g = max(8, int(s * full_grid * n / cmax) // 8 * 8)
-            T._chk(L.emstr_tune(c.ptr, 0, 0, g, 1), "emstr_tune")
+    def apply(tx):
+        for c in ctxs:
+            T._chk(L.emstr_tune(c.ptr, tx, 0, 0, 1), "emstr_tune")
 
-    for s in scales:
-        apply(s)
+    for tx in blocks:
+        apply(tx)
         ens = T.Ensemble(ctxs)
         ens.step(2)
         ens.sync()
         ms = ev.time(ens.stream, lambda: ens.step(steps)) / steps
         ens.sync()
         ens.close()
-        trials.append((s, ms))
+        trials.append((tx, ms))
     best = min(trials, key=lambda r: r[1])
     apply(best[0])
     ev.close()

@@ -1,36 +1,48 @@
 /*
- * Batched form of the EMS tracer step on MI355X (gfx950): all tracers of
- * the step over all water columns, FP32 state with FP64 budgets.
+ * Batched form of the EMS tracer step on MI355X (gfx950): every tracer of
+ * the step over every water column in one fused pass, FP32 state with FP64
+ * budgets.
  *
- * Layout: tracers are stored tracer-innermost, [column][layer][tracer] in
- * FP32, so the tracers of one cell are one contiguous 4*ntr-byte row and
- * a wave with one lane per tracer reads a cell with one coalesced load.
- * Everything that does not depend on the tracer (face transports, van
- * Leer factors, the normalised column matrix and its LU factors) is
- * prepared per cell once per step (k_prepB, k_colB) and read by all lanes.
+ * Layout (HBM): tracers [ntr][nz][nyp][nxp] FP32, the plane padded to
+ * nxp = 64-multiple and nyp = 32-multiple columns (the padding is land), so
+ * that a wave's block of columns (TX x 64/TX, TX consecutive x columns per
+ * row) reads whole, aligned row segments.  Per cell, three float4 words of
+ * tracer-independent coefficients [nz][nyp][nxp][3]: A = (qW, fW, qS, fS)
+ * the transports and van Leer factors of the west and south faces, B =
+ * (qB, fB, 1/V, D) those of the bottom face, the inverse volume and the net
+ * FP64 volume divergence, C = (gm, gp, ud, 1/div) the normalised column
+ * matrix and its LU factors; the east, north and top faces are the west,
+ * south and bottom faces of the neighbouring cells.  Per column a Col2
+ * record (beds of the eight stencil neighbours, surfaces, diffusion
+ * extent, flags, FP64 surface volume balance).
  *
  * Per step:
- *   k_prepB / k_colB  per-cell coefficients, per-column information (the
- *                     FP64 surface volume balance, diffusion extent,
- *                     stiffness, LU factors of the column matrix);
- *   k_advB            cell-parallel advection: face values (van Leer,
- *                     tracers.c:2097-2118) on the 13-point stencil and the
- *                     flux divergence and sub-surface update
- *                     (tracers.c:888-925) in anomaly form
- *                       tr - (sum_f s_f (F_f - tr) q_f + tr*D) / V,
- *                     D the FP64 net volume divergence of the cell, so that
- *                     rounding the transports q to FP32 only perturbs the
- *                     small (F - tr) terms; the processed surface cell by
- *                     surf_conc (tracers.c:3047-3207) as an FP64 volume
- *                     balance;
- *   k_vertB           per column: tr_bounds (tracers.c:3217-3324), then the
- *                     implicit vertical diffusion (tracers.c:3782-3957,
- *                     4067-4214) in increment form, A (x - b) = dzdt*b - A b,
- *                     with one FP64 residual pass of iterative refinement
- *                     in columns whose diffusion number exceeds IR_DMAX.
+ *   k_prep2   one thread per column: coefficients from the FP64 fields and
+ *             the column record (tracers.c:496-578, 3841-3889 restated);
+ *   k_step2   one wave per (block of 64 columns, tracer): every lane walks
+ *             its column from the bed to the top with a five-layer register
+ *             window of the tracer, the horizontal neighbours of the layer
+ *             read from L1/L2, and computes
+ *               - van Leer face values (tracers.c:2097-2118) as anomalies
+ *                 against the cell value and the sub-surface update
+ *                 (tracers.c:888-925) in anomaly form
+ *                   tr - (sum_f s_f (F_f - tr) q_f + tr D) / V,
+ *                 kept in the wave's LDS slice,
+ *               - the processed surface cell by surf_conc
+ *                 (tracers.c:3047-3207) as an FP64 volume balance, the
+ *                 layers above it folded in on the way up,
+ *             then tr_bounds (tracers.c:3217-3324) and the implicit
+ *             vertical diffusion (tracers.c:3782-3957, 4067-4214) of the
+ *             column in increment form, A (x - b) = dzdt b - A b, with one
+ *             FP64 residual pass of iterative refinement where the
+ *             diffusion number exceeds IR_DMAX, and writes the column once.
+ * The blocks are dealt to the 8 XCDs in work-balanced chunks of vertical
+ * strips, and the tracers of one block run back to back on one XCD, so
+ * its coefficients and the halo rows of the block before it are L2 hits.
  * Every FP32 state write is stochastically rounded from an FP64 value with
- * a deterministic hash of (cell, tracer, step), which keeps sub-ulp
- * increments from biasing the state and the tracer mass.
+ * a deterministic hash of (cell, tracer, step), so sub-ulp increments do
+ * not bias the state or the tracer mass, and results do not depend on the
+ * launch shape.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -46,172 +58,158 @@
 #define VANLEER 0x000080
 #define IR_DMAX 1.0
 #define NZMAX 64
+#define SWAVES 4 /* waves per workgroup: four blocks of a tile, one tracer */
 
 typedef long long ll;
 
-/* coefficient slots per cell */
-enum { C_QW, C_CW, C_QE, C_CE, C_QS, C_CS, C_QN, C_CN, C_QB, C_CB, C_RV, C_D,
-       C_GM, C_GP, C_UD, C_RD, C_N };
-
-/* per-column information (two 64-byte scalar loads) */
-struct ColInfo {
-  int kb, ks, kold, cs, cb, flags, kw1, kw2, ke1, ke2, ks1, ks2, kn1, kn2, pad0, pad1;
-  double vc;   /* consistent old volume of the surface cells + inflow (FP64) */
-  double svol; /* new surface volume                                        */
-  double dzs;  /* subeta - waterbot of the processed surface cell           */
-  double area, osub, botz, pad2, pad3;
-};
 enum { F_PROC = 1, F_SINGLE = 2, F_STIFF = 4, F_THIN = 8, F_KEEPOLD = 16, F_MOVE = 32 };
 
+/* per-column record (112 bytes) */
+struct Col2 {
+  int kb;    /* bed layer, nz for land                                 */
+  int ks;    /* processed surface layer, -1: column not processed      */
+  int kold;  /* old surface layer                                      */
+  int cs;    /* new surface layer (top of the diffusion extent)        */
+  int cb;    /* bottom of the diffusion extent                         */
+  int flags;
+  int nbw;   /* bed layers of the x neighbours: w1 | w2 << 8 | e1 << 16 | e2 << 24 */
+  int nbs;   /* of the y neighbours: s1 | s2 << 8 | n1 << 16 | n2 << 24 */
+  double vc;   /* consistent old volume of the surface cells + inflow */
+  double svol; /* new surface volume                                   */
+  double dzs;  /* subeta - waterbot of the processed surface cell      */
+  double area, osub, botz;
+  double pad0, pad1, pad2, pad3;
+};
+
 struct emstr_batch {
-  int ntr, nt64, nz, nx, ny, ncol;
-  ll plane;
-  float *tr[2];      /* [col][k][ntr] FP32                                    */
+  int ntr, nz, nx, ny, nxp, nyp;
+  ll P2;           /* nxp * nyp                                          */
+  ll vol;          /* nz * P2: one tracer                                */
+  float *tr[2];    /* [ntr][nz][nyp][nxp]                                */
   int cur;
-  float *co;         /* [col][k][C_N] coefficients                            */
-  ColInfo *ci;       /* [plane]                                               */
-  int *order;        /* [nord] column processing order (-1: none)             */
-  int nord;
-  float *scr;        /* refinement scratch [col][k][ntr] (stiff columns)      */
-  float *mintr, *maxtr; /* [ntr]                                               */
-  int grid;          /* persistent grid size                                  */
-  int tile_x, tile_y;
+  float4 *co;      /* [nz][nyp][nxp][3]                                  */
+  Col2 *ci;        /* [nyp][nxp]                                         */
+  float *scr;      /* refinement scratch, the tracer layout              */
+  float *mintr, *maxtr;
+  int *stepctr;
+  /* block lists: per XCD a sequence of blocks (i0 | j0 << 16, kmin) */
+  int tx;          /* block width (columns per row)                      */
+  int strip;       /* strip width in columns                             */
+  int *blk;        /* [16 + 2 nblk]: per XCD first block and count, then
+                      (i0 | j0 << 16, kmin) per block                    */
+  int nblk;
+  int boff[8], bcnt[8], bmax;
+  int kmin;        /* deepest bed of any block                           */
   emstr_prof *prof;
   hipGraph_t graph;
   hipGraphExec_t gexec;
-  int graph_ok;
-  int graph_cur; /* buffer parity the graph was captured for */
-  int *stepctr;  /* device step counter (stochastic rounding keys) */
-  /* kernel A: per wave slot lists of cell descriptors and of their
-   * columns' (processed surface + 1) | flags << 8 */
-  unsigned long long *cells;
-  int *kfl;
-  int niter;  /* list length per slot (a multiple of 64)               */
-  int gridA;  /* kernel A workgroups (4 wave slots each)               */
-  ll nlist;   /* gridA * 4 * niter                                     */
-  int grid_full, gridA_full; /* the grids that fill the GPU          */
+  int graph_ok, graph_cur;
 };
 
-__device__ __forceinline__ ll cidx(ll col, int nz, int k) { return col * nz + k; }
+__device__ __forceinline__ float4 ld4(const float4 *p) { return *p; }
 
 /*-------------------------------------------------------------------*/
-/* prep: per-cell FP32 coefficients from the FP64 fields              */
+/* prep: coefficients and the column record, one thread per column    */
 /*-------------------------------------------------------------------*/
 /* the factor of the limited term of the van Leer face value for
- * transport q with Courant number cn: 1 - cn if q > 0, 1 + cn if q < 0 */
+ * transport q with Courant number cn: 1 - cn if q > 0, 1 + cn if q < 0
+ * (the limiter's factor 2 and the 1/2 of tracers.c:2108 cancel) */
 __device__ __forceinline__ float vlfac(double q, double cn) {
   return (q > 0.0) ? (float)(1.0 - cn) : (q < 0.0) ? (float)(1.0 + cn) : 1.0f;
 }
 
-__global__ void k_prepB(emstr_dev d, emstr_batch b) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i >= d.nx || j >= d.ny) return;
-  const ll col = (ll)j * d.nx + i;
-  const int kbc = d.kb[col];
-  if (k < kbc) return;
-  const int ksc = d.ks[col];
-  const ll P = d.plane, nx = d.nx;
-  const ll idx = (ll)k * P + col;
-  const double dt = d.dt, area = d.cellarea[col];
-  const double dz = d.dz[idx];
-  const double dzb = (k > kbc) ? d.dz[idx - P] : 0.0;
-  const bool we = k >= d.kb[col + 1], wn = k >= d.kb[col + nx];
-  const bool top = (ksc >= 0 && k < ksc);
-  double qW = d.u1flux3d[idx] * dt, qS = d.u2flux3d[idx] * dt;
-  double qE = we ? d.u1flux3d[idx + 1] * dt : 0.0;
-  double qN = wn ? d.u2flux3d[idx + nx] * dt : 0.0;
-  double qB = d.w[idx] * dt * area;
-  /* the top face's transport and Courant number are the bottom face's
-   * of the cell above (identical expressions) */
-  const double qT = top ? d.w[idx + P] * dt * area : 0.0;
-  /* the van Leer face value needs 0.5 (1 - cn) for q > 0 and 0.5 (1 + cn)
-   * for q < 0 (tracers.c:2097-2118); the factor 2 of the limiter is
-   * folded in, so the slot holds 1 - cn or 1 + cn */
-  const double cW = d.u1[idx] * dt / d.h1au1[col];
-  const double cE = we ? d.u1[idx + 1] * dt / d.h1au1[col + 1] : 0.0;
-  const double cS = d.u2[idx] * dt / d.h2au2[col];
-  const double cN = wn ? d.u2[idx + nx] * dt / d.h2au2[col + nx] : 0.0;
-  const double cB = d.w[idx] * dt / (0.5 * (dz + dzb));
-  float c[C_N];
-  c[C_QW] = (float)qW;
-  c[C_CW] = vlfac(qW, cW);
-  c[C_QE] = (float)qE;
-  c[C_CE] = vlfac(qE, cE);
-  c[C_QS] = (float)qS;
-  c[C_CS] = vlfac(qS, cS);
-  c[C_QN] = (float)qN;
-  c[C_CN] = vlfac(qN, cN);
-  c[C_QB] = (float)qB;
-  c[C_CB] = vlfac(qB, cB);
-  c[C_RV] = (dz != 0.0) ? (float)(1.0 / (area * dz)) : 0.0f;
-  c[C_D] = (float)(qE - qW + qN - qS + qT - qB);
-  /* vertical diffusion rows of the new surface column, normalised by
-   * dzdt: gm = Cm1/dzdt, gp = Cp1/dzdt (tracers.c:3841-3889) */
-  float gm = 0.0f, gp = 0.0f;
-  const int cs = d.knew[col];
-  if (ksc >= 0 && k <= cs && cs - 1 >= kbc) {
-    const double dzdt = dz / dt;
-    if (k > kbc) gm = (float)(-d.Kz[idx] / (0.5 * (dzb + dz)) / dzdt);
-    if (k < cs) gp = (float)(-d.Kz[idx + P] / (0.5 * (dz + d.dz[idx + P])) / dzdt);
-  }
-  c[C_GM] = gm;
-  c[C_GP] = gp;
-  c[C_UD] = 0.0f;
-  c[C_RD] = 1.0f;
-  float4 *c4 = (float4 *)(b.co + cidx(col, d.nz, k) * C_N);
-  for (int w = 0; w < C_N / 4; w++)
-    c4[w] = make_float4(c[4 * w], c[4 * w + 1], c[4 * w + 2], c[4 * w + 3]);
-}
+struct PArgs {
+  int nxp, nyp;
+  ll P2;
+};
 
-/* per-column information, one thread per processed-or-wet column */
-__global__ void k_colB(emstr_dev d, emstr_batch b) {
+__global__ void __launch_bounds__(256) k_prep2(emstr_dev d, PArgs a, float4 *__restrict__ co,
+                                               Col2 *__restrict__ ci) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= d.nx || j >= d.ny) return;
-  const ll col = (ll)j * d.nx + i;
-  const int kbc = d.kb[col];
-  ColInfo c;
+  if (i >= a.nxp || j >= a.nyp) return;
+  const ll c2 = (ll)j * a.nxp + i;
+  Col2 c;
   memset(&c, 0, sizeof(c));
-  c.kb = kbc;
   c.ks = -1;
+  c.kb = d.nz;
+  const bool in = i < d.nx && j < d.ny;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = in ? d.kb[col] : d.nz;
   if (kbc >= d.nz) {
-    b.ci[col] = c;
+    c.nbw = c.nbs = 0x01010101 * d.nz;
+    ci[c2] = c;
     return;
   }
   const ll P = d.plane, nx = d.nx;
+  const int top = d.nz - 1;
+  const double dt = d.dt, area = d.cellarea[col];
+  /* the fields through restrict pointers: the coefficient stores do not
+   * alias them, so the loads of several layers can be in flight */
+  const double *__restrict__ U1F = d.u1flux3d, *__restrict__ U2F = d.u2flux3d;
+  const double *__restrict__ W = d.w, *__restrict__ U1 = d.u1, *__restrict__ U2 = d.u2;
+  const double *__restrict__ DZ = d.dz, *__restrict__ KZ = d.Kz;
+  c.kb = kbc;
+  /* neighbour beds; a second neighbour counts only if the first is wet */
+  const int w1 = d.kb[col - 1], e1 = d.kb[col + 1], s1 = d.kb[col - nx], n1 = d.kb[col + nx];
+  const int w2 = (w1 < d.nz) ? d.kb[col - 2] : d.nz, e2 = (e1 < d.nz) ? d.kb[col + 2] : d.nz;
+  const int s2 = (s1 < d.nz) ? d.kb[col - 2 * nx] : d.nz;
+  const int n2 = (n1 < d.nz) ? d.kb[col + 2 * nx] : d.nz;
+  c.nbw = w1 | (w2 << 8) | (e1 << 16) | (e2 << 24);
+  c.nbs = s1 | (s2 << 8) | (n1 << 16) | (n2 << 24);
   const int ksc = d.ks[col];
+  const int cs = d.knew[col];
   c.ks = ksc;
   c.kold = d.kold[col];
-  c.cs = d.knew[col];
-  c.kw1 = d.kb[col - 1];
-  c.ke1 = d.kb[col + 1];
-  c.ks1 = d.kb[col - nx];
-  c.kn1 = d.kb[col + nx];
-  c.kw2 = (c.kw1 < d.nz) ? d.kb[col - 2] : d.nz;
-  c.ke2 = (c.ke1 < d.nz) ? d.kb[col + 2] : d.nz;
-  c.ks2 = (c.ks1 < d.nz) ? d.kb[col - 2 * nx] : d.nz;
-  c.kn2 = (c.kn1 < d.nz) ? d.kb[col + 2 * nx] : d.nz;
-  c.area = d.cellarea[col];
+  c.cs = cs;
+  c.area = area;
   c.botz = d.botz[col];
   c.osub = d.osubeta[col];
+  /* per-cell coefficients, bed to top */
+  double dzb = 0.0;
+#pragma unroll 4
+  for (int k = kbc; k <= top; k++) {
+    const ll idx = (ll)k * P + col;
+    const double dz = DZ[idx];
+    const bool we = k >= e1, wn = k >= n1;
+    const bool tp = (ksc >= 0 && k < ksc);
+    const double qW = U1F[idx] * dt, qS = U2F[idx] * dt;
+    const double qE = we ? U1F[idx + 1] * dt : 0.0;
+    const double qN = wn ? U2F[idx + nx] * dt : 0.0;
+    const double qB = W[idx] * dt * area;
+    /* the top face's transport is the bottom face's of the cell above */
+    const double qT = tp ? W[idx + P] * dt * area : 0.0;
+    const double cW = U1[idx] * dt / d.h1au1[col];
+    const double cS = U2[idx] * dt / d.h2au2[col];
+    const double cB = W[idx] * dt / (0.5 * (dz + dzb));
+    float gm = 0.0f, gp = 0.0f;
+    if (ksc >= 0 && k <= cs && cs - 1 >= kbc) {
+      /* rows of the implicit vertical diffusion (tracers.c:3841-3889),
+       * normalised by dzdt */
+      const double dzdt = dz / dt;
+      if (k > kbc) gm = (float)(-KZ[idx] / (0.5 * (dzb + dz)) / dzdt);
+      if (k < cs) gp = (float)(-KZ[idx + P] / (0.5 * (dz + DZ[idx + P])) / dzdt);
+    }
+    float4 *o = co + ((ll)k * a.P2 + c2) * 3;
+    o[0] = make_float4((float)qW, vlfac(qW, cW), (float)qS, vlfac(qS, cS));
+    o[1] = make_float4((float)qB, vlfac(qB, cB), (dz != 0.0) ? (float)(1.0 / (area * dz)) : 0.0f,
+                       (float)(qE - qW + qN - qS + qT - qB));
+    o[2] = make_float4(gm, gp, 0.0f, 1.0f);
+    dzb = dz;
+  }
   int flags = 0;
   if (ksc >= 0) {
     flags |= F_PROC;
-    const double dt = d.dt, area = c.area;
-    /* surface volume balance in FP64 (surf_conc): old volumes of the cells
-     * from the processed surface to the old surface, minus the net
+    /* surface volume balance in FP64 (surf_conc): old volumes of the
+     * cells from the processed surface to the old surface, minus the net
      * horizontal outflow of every cell from it to the top, plus the inflow
      * through its bottom face */
-    const int kold = c.kold, top = d.nz - 1;
-    double waterbot = (ksc == kbc) ? c.botz : d.layers[ksc];
-    const double subeta = d.subeta[col];
-    c.dzs = subeta - waterbot;
+    const int kold = c.kold;
+    const double waterbot = (ksc == kbc) ? c.botz : d.layers[ksc];
+    c.dzs = d.subeta[col] - waterbot;
     c.svol = c.dzs * area;
-    double vol = 0.0;
+    double vol = 0.0, wb = waterbot;
     int cc = ksc;
-    double wb = waterbot;
     while (cc != kold && cc < top) {
       vol += area * (d.layers[cc + 1] - wb);
       cc++;
@@ -219,62 +217,54 @@ __global__ void k_colB(emstr_dev d, emstr_batch b) {
     }
     vol += area * (c.osub - ((cc == kbc) ? c.botz : d.layers[cc]));
     double out = 0.0;
-    bool above = false; /* transport through a face above the processed cell */
-#pragma unroll 4
+    bool above = false;
     for (int k = ksc; k <= top; k++) {
       const ll idx = (ll)k * P + col;
-      const bool we = k >= d.kb[col + 1], wn = k >= d.kb[col + nx];
-      double qE = we ? d.u1flux3d[idx + 1] * dt : 0.0;
-      double qN = wn ? d.u2flux3d[idx + nx] * dt : 0.0;
+      const bool we = k >= e1, wn = k >= n1;
+      const double qE = we ? d.u1flux3d[idx + 1] * dt : 0.0;
+      const double qN = wn ? d.u2flux3d[idx + nx] * dt : 0.0;
       out += qE - d.u1flux3d[idx] * dt + qN - d.u2flux3d[idx] * dt;
       if (k > ksc && (qE != 0.0 || qN != 0.0 || d.u1flux3d[idx] != 0.0 || d.u2flux3d[idx] != 0.0))
         above = true;
     }
-    const double qB = d.w[(ll)ksc * P + col] * dt * area;
-    c.vc = vol - out + qB;
-    /* cells other than the processed one enter the balance when the old
-     * surface is above it or any layer lies above it */
+    c.vc = vol - out + d.w[(ll)ksc * P + col] * dt * area;
     if (kold != ksc || above) flags |= F_MOVE;
     /* thin old surface layer (tracers.c:3250) */
     const double dzco = (kold >= kbc) ? d.dz[(ll)kold * P + col] : 0.0;
     if (ksc > kbc && dzco < d.hmin) flags |= F_THIN;
     if (c.botz > 0.0 && d.dz[(ll)ksc * P + col] < d.hmin) flags |= F_KEEPOLD;
     /* vertical diffusion extent (tracers.c:3836-3866) */
-    const int cs = c.cs;
     if (cs < kbc || cs - 1 < kbc || d.dz[(ll)(cs - 1) * P + col] == 0.0) {
       flags |= F_SINGLE;
       c.cb = cs;
     } else {
-      /* one above the highest zero-thickness layer at or below cs (the
-       * run of open layers ending at cs); loads independent of each other */
+      /* one above the highest zero-thickness layer at or below cs */
       int zq = kbc - 1;
-#pragma unroll 8
       for (int k = kbc; k <= cs; k++)
         if (d.dz[(ll)k * P + col] == 0.0) zq = k;
       c.cb = zq + 1;
       if (c.cb >= cs) flags |= F_SINGLE;
-#pragma unroll 4
       for (int k = c.cb + 1; k <= cs; k++) {
         const double dzc = d.dz[(ll)k * P + col];
         const double dzm = d.dz[(ll)(k - 1) * P + col];
         if (d.Kz[(ll)k * P + col] * dt / (dzc * 0.5 * (dzm + dzc)) > IR_DMAX) flags |= F_STIFF;
       }
-      /* LU factorisation of the normalised column matrix (rows gm, 1 - gm -
-       * gp, gp): ud_k = gp_{k-1} / div_{k-1}, div_k = 1 - gm - gp - gm ud_k;
-       * stored as ud and 1/div */
+      /* LU factors of the normalised matrix (rows gm, 1 - gm - gp, gp):
+       * ud_k = gp_{k-1} / div_{k-1}, div_k = 1 - gm - gp - gm ud_k */
       if (!(flags & F_SINGLE)) {
         float divp = 1.0f, gpp = 0.0f;
-#pragma unroll 4
         for (int k = c.cb; k <= cs; k++) {
-          float *cf = b.co + cidx(col, d.nz, k) * C_N;
-          const float gm = (k > c.cb) ? cf[C_GM] : 0.0f, gp = cf[C_GP];
+          float4 *o = co + ((ll)k * a.P2 + c2) * 3 + 2;
+          float4 m = *o;
+          const float gm = (k > c.cb) ? m.x : 0.0f, gp = m.y;
           float ud = 0.0f, dv = 1.0f - gm - gp;
           if (k > c.cb) {
             ud = gpp / divp;
             dv = dv - gm * ud;
           }
-          cf[C_UD] = ud;
-          cf[C_RD] = 1.0f / dv;
+          m.z = ud;
+          m.w = 1.0f / dv;
+          *o = m;
           divp = dv;
           gpp = gp;
         }
@@ -282,64 +272,32 @@ __global__ void k_colB(emstr_dev d, emstr_batch b) {
     }
   }
   c.flags = flags;
-  b.ci[col] = c;
+  ci[c2] = c;
 }
 
 /*-------------------------------------------------------------------*/
-/* the fused step                                                    */
+/* van Leer face values as anomalies                                  */
 /*-------------------------------------------------------------------*/
-/* van Leer face value (tracers.c:2097-2118) for transport q through the
- * face between bm1 and c, returned as its anomaly F - ref against a
- * reference value close to it; f is 1 - cn for q > 0 and 1 + cn for
- * q < 0 (the prep folds the Courant number and the limiter's factor 2
- * into it).  The upwind value's anomaly is an exact FP32 difference for
- * neighbouring values within a factor of two, so the only rounding left
- * is relative to the small limited term and to the anomaly itself, not to
- * the tracer value.  The limiter's division uses the hardware reciprocal
- * (about 1 ulp). */
-__device__ __forceinline__ float vla(float bm2, float bm1, float c, float fp1,
-                                     float q, float f, float ref) {
-  if (q > 0.0f) {
-    const float df = (bm1 - bm2) * (c - bm1);
-    const float t = df * __builtin_amdgcn_rcpf(c - bm2);
-    return (df > 0.0f) ? (bm1 - ref) + f * t : bm1 - ref;
-  } else if (q < 0.0f) {
-    const float df = (c - bm1) * (fp1 - c);
-    const float t = df * __builtin_amdgcn_rcpf(fp1 - bm1);
-    return (df > 0.0f) ? (c - ref) - f * t : c - ref;
-  }
-  return c - ref; /* no transport: its flux term vanishes */
-}
-
-/* vla of a cell's own faces without branches: for q < 0 the stencil is
- * mirrored, (bm2, bm1, c) -> (fp1, c, bm1), which leaves df unchanged
- * and negates the reciprocal, so both signs share one expression with the
- * same values bit for bit; q == 0 keeps the upwind anomaly (c - ref).
- * ref is the cell's own value: c on its west, south and bottom faces, bm1
- * on its east, north and top faces. */
-__device__ __forceinline__ float vla_m(float bm2, float bm1, float c, float fp1, float q,
-                                       float f, float ref) {
+/* Van Leer face value (tracers.c:2097-2118) for transport q through the
+ * face between bm1 and c, as its anomaly F - ref.  For q < 0 the stencil
+ * is mirrored, (bm2, bm1, c) -> (fp1, c, bm1), which leaves df unchanged
+ * and negates the reciprocal, so both signs share one expression; f is
+ * 1 - cn for q > 0 and 1 + cn for q < 0.  With q == 0 the flux term
+ * vanishes whatever the value.  The upwind value's anomaly is an exact
+ * FP32 difference for neighbouring values within a factor of two, so the
+ * rounding left is relative to the limited term and the anomaly itself. */
+__device__ __forceinline__ float vla(float bm2, float bm1, float c, float fp1, float q, float f,
+                                     float ref) {
   const bool pos = q > 0.0f;
   const float p2 = pos ? bm2 : fp1, p1 = pos ? bm1 : c, p0 = pos ? c : bm1;
   const float df = (p1 - p2) * (p0 - p1);
   const float t = df * __builtin_amdgcn_rcpf(p0 - p2);
-  const float r0 = p1 - ref;
-  const float r = __builtin_fmaf(f, (df > 0.0f) ? t : 0.0f, r0);
-  return (q != 0.0f) ? r : r0;
-}
-__device__ __forceinline__ float vla_c(float bm2, float bm1, float c, float fp1, float q,
-                                       float f) {
-  return vla_m(bm2, bm1, c, fp1, q, f, c);
-}
-__device__ __forceinline__ float vla_b(float bm2, float bm1, float c, float fp1, float q,
-                                       float f) {
-  return vla_m(bm2, bm1, c, fp1, q, f, bm1);
+  return __builtin_fmaf(f, (df > 0.0f) ? t : 0.0f, p1 - ref);
 }
 
 /*-------------------------------------------------------------------*/
 /* deterministic stochastic rounding                                 */
 /*-------------------------------------------------------------------*/
-/* 32-bit integer hash (xorshift-multiply finaliser) */
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352dU;
@@ -349,7 +307,7 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
   return x;
 }
 
-/* per-cell key: one multiply round over the cell index and the lane's
+/* per-cell key: one multiply round over the cell index and the wave's
  * (tracer, step) hash */
 __device__ __forceinline__ uint32_t srkey(uint32_t cellkey, uint32_t hl) {
   uint32_t x = cellkey ^ hl;
@@ -362,9 +320,7 @@ __device__ __forceinline__ uint32_t srkey(uint32_t cellkey, uint32_t hl) {
 /* Round the FP64 value x to one of its two neighbouring FP32 values, the
  * upper one with probability equal to the remainder's fraction of an FP32
  * ulp: a uniform 29-bit number is added to the 29 mantissa bits that FP32
- * drops, which are then cleared.  Unbiased, so a state that changes by less
- * than an ulp per step does not drift systematically.  r is a uniform
- * 32-bit hash of (cell, tracer, step, stage). */
+ * drops, which are then cleared. */
 __device__ __forceinline__ float sround(double x, uint32_t r) {
   unsigned long long u = (unsigned long long)__double_as_longlong(x);
   u += (unsigned long long)(r >> 3);
@@ -373,535 +329,417 @@ __device__ __forceinline__ float sround(double x, uint32_t r) {
 }
 
 /*-------------------------------------------------------------------*/
-/* the fused step: one wave walks one water column                    */
+/* the fused step                                                    */
 /*-------------------------------------------------------------------*/
-#define WPB 4 /* waves per workgroup, each on its own columns */
-
-struct KArgs {
-  int nz, ntr, nord, g; /* g: which 64 tracers this launch takes */
-  int nact;             /* active lanes: min(64, ntr - 64 g)       */
-  int pad;
-  ll nx;
+struct SArgs {
+  int nz, ntr, nx;      /* nx: unpadded, for the rounding keys          */
+  int nxp;
+  ll P2, vol;
   double hmin, dt;
-  ll plane;
+  ll P;                 /* unpadded plane of the FP64 fields            */
+  int nzl;              /* layers of a wave's LDS slice                 */
 };
 
-/* lane access: p is a wave-uniform byte address, o the lane's byte offset */
-__device__ __forceinline__ float ldb(const char *p, uint32_t o) {
-  return *(const float *)(p + o);
+/* raw buffer access with 32-bit byte offsets from a wave-uniform base
+ * (resource word 3 of the gfx9 family); a read past the end returns 0 */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
 }
-__device__ __forceinline__ void stb(char *p, uint32_t o, float v) { *(float *)(p + o) = v; }
-__device__ __forceinline__ float rdl(int v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(v, l));
+__device__ __forceinline__ float bl1(__amdgpu_buffer_rsrc_t r, uint32_t o) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0));
+}
+__device__ __forceinline__ float2 bl2(__amdgpu_buffer_rsrc_t r, uint32_t o) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, o, 0, 0);
+  return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+__device__ __forceinline__ float4 bl4(__amdgpu_buffer_rsrc_t r, uint32_t o) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0);
+  return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                     __uint_as_float(v[3]));
 }
 
-/* Base addresses of the columns around one column; a neighbour that is
- * land or below the bed at layer k is replaced by the cell the map walk
- * came from, whose ghost holds that value after the lateral boundary fill
- * (tracers.c:5574-5593).  All wave-uniform. */
-struct Nbr {
-  const char *c, *w1, *w2, *e1, *e2, *s1, *s2, *n1, *n2;
-  int kw1, kw2, ke1, ke2, ks1, ks2, kn1, kn2;
+/* DPP moves inside rows of 16 lanes: lane l takes lane l - 1, l - 2
+ * (shr) or l + 1, l + 2 (shl); a lane without a source keeps 0 */
+__device__ __forceinline__ float shr1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float shr2(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float shl1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float shl2(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x102, 0xf, 0xf, false));
+}
+
+/* the loads of one layer: the column at layer k + 2 (the window's new
+ * entry), the four y neighbours, the x neighbours beyond the lane's row
+ * segment (edge lanes only), the cell's west/south coefficient word, the
+ * north neighbour's south transport, the bottom-face word of the cell
+ * above (this cell's top face) and, for the segment's last lane, the east
+ * neighbour's west transport */
+struct Lay {
+  float z2, s1, s2, n1, n2, xw1, xw2, xe1, xe2;
+  float4 A, U, M;
+  float2 N, E;
 };
 
-struct HVals {
-  float w1, w2, e1, e2, s1, s2, n1, n2;
-};
-
-/* the eight horizontal neighbours at layer k; o = lane offset + k * row */
-__device__ __forceinline__ void load_h(HVals &h, const Nbr &nb, int k, uint32_t o) {
-  const bool w1 = k >= nb.kw1, e1 = k >= nb.ke1, s1 = k >= nb.ks1, n1 = k >= nb.kn1;
-  const char *pw1 = w1 ? nb.w1 : nb.c, *pe1 = e1 ? nb.e1 : nb.c;
-  const char *ps1 = s1 ? nb.s1 : nb.c, *pn1 = n1 ? nb.n1 : nb.c;
-  const char *pw2 = (w1 && k >= nb.kw2) ? nb.w2 : pw1;
-  const char *pe2 = (e1 && k >= nb.ke2) ? nb.e2 : pe1;
-  const char *ps2 = (s1 && k >= nb.ks2) ? nb.s2 : ps1;
-  const char *pn2 = (n1 && k >= nb.kn2) ? nb.n2 : pn1;
-  h.w1 = ldb(pw1, o);
-  h.w2 = ldb(pw2, o);
-  h.e1 = ldb(pe1, o);
-  h.e2 = ldb(pe2, o);
-  h.s1 = ldb(ps1, o);
-  h.s2 = ldb(ps2, o);
-  h.n1 = ldb(pn1, o);
-  h.n2 = ldb(pn2, o);
-}
-
-/* horizontal anomaly outflow sum_f s_f (F_f - ref) q_f of a cell whose
- * value is v0, from the face transports q and van Leer factors f */
-__device__ __forceinline__ float hflux(const HVals &h, bool ew, bool nw, float v0, float ref,
-                                       float qW, float fW, float qE, float fE, float qS,
-                                       float fS, float qN, float fN) {
-  const float AW = vla(h.w2, h.w1, v0, h.e1, qW, fW, ref);
-  const float AE = ew ? vla(h.w1, v0, h.e1, h.e2, qE, fE, ref) : 0.0f;
-  const float AS = vla(h.s2, h.s1, v0, h.n1, qS, fS, ref);
-  const float AN = nw ? vla(h.s1, v0, h.n1, h.n2, qN, fN, ref) : 0.0f;
-  return AE * qE - AW * qW + AN * qN - AS * qS;
-}
-
-/* hflux of a cell against its own value (ref == v0) */
-__device__ __forceinline__ float hflux_own(const HVals &h, bool ew, bool nw, float v0, float qW,
-                                           float fW, float qE, float fE, float qS, float fS,
-                                           float qN, float fN) {
-  const float AW = vla_c(h.w2, h.w1, v0, h.e1, qW, fW);
-  const float AE = ew ? vla_b(h.w1, v0, h.e1, h.e2, qE, fE) : 0.0f;
-  const float AS = vla_c(h.s2, h.s1, v0, h.n1, qS, fS);
-  const float AN = nw ? vla_b(h.s1, v0, h.n1, h.n2, qN, fN) : 0.0f;
-  return AE * qE - AW * qW + AN * qN - AS * qS;
-}
-
-/*-------------------------------------------------------------------*/
-/* kernel A: cell-parallel advection                                  */
-/*-------------------------------------------------------------------*/
-/* Cell descriptor: column (bits 0-31), layer k (32-39), open neighbours
- * at layer k (40-47, M_* below; a second neighbour is open only if the
- * first one is), bed layer kb (48-55).  ~0 marks padding. */
-enum { M_W1 = 1, M_W2 = 2, M_E1 = 4, M_E2 = 8, M_S1 = 16, M_S2 = 32, M_N1 = 64, M_N2 = 128 };
-#define NO_CELL (~0ULL)
-
-struct AArgs {
-  int nz, ntr, g, nact; /* g: which 64 tracers this launch takes */
-  int niter;            /* iterations per wave slot (a multiple of 64) */
-  int pad;
-  ll nx;
-  double hmin;
-};
-
-/* the 13-point stencil of one cell for one tracer lane */
-struct St13 {
-  float v0, zm1, zm2, zp1, zp2, w1, w2, e1, e2, s1, s2, n1, n2;
-};
-
-__device__ __forceinline__ unsigned long long rdl64(unsigned long long v, int l) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-  return ((unsigned long long)hi << 32) | lo;
-}
-
-/* Issue the loads of the stencil of cell dsc (wave-uniform) and of its
- * coefficient words (lanes 0..15: cell k, 16..31: cell k + 1). */
-__device__ __forceinline__ void st13_load(St13 &v, int &cw, const char *tin,
-                                          const float *co, unsigned long long dsc, ll colb,
-                                          ll jb, uint32_t lb, uint32_t rowb, int nz,
-                                          int lane) {
-  const ll col = (ll)(uint32_t)dsc;
-  const int k = (int)((dsc >> 32) & 255), m = (int)((dsc >> 40) & 255);
-  const int kb = (int)((dsc >> 48) & 255), top = nz - 1;
-  const char *c = tin + col * colb;
-  const uint32_t o = lb + k * rowb;
-  const char *pw1 = (m & M_W1) ? c - colb : c, *pe1 = (m & M_E1) ? c + colb : c;
-  const char *ps1 = (m & M_S1) ? c - jb : c, *pn1 = (m & M_N1) ? c + jb : c;
-  const char *pw2 = (m & M_W2) ? c - 2 * colb : pw1, *pe2 = (m & M_E2) ? c + 2 * colb : pe1;
-  const char *ps2 = (m & M_S2) ? c - 2 * jb : ps1, *pn2 = (m & M_N2) ? c + 2 * jb : pn1;
-  v.v0 = ldb(c, o);
-  v.zm1 = ldb(c, lb + max(k - 1, kb) * rowb);
-  v.zm2 = ldb(c, lb + max(k - 2, kb) * rowb);
-  v.zp1 = ldb(c, lb + min(k + 1, top) * rowb);
-  v.zp2 = ldb(c, lb + min(k + 2, top) * rowb);
-  v.w1 = ldb(pw1, o);
-  v.w2 = ldb(pw2, o);
-  v.e1 = ldb(pe1, o);
-  v.e2 = ldb(pe2, o);
-  v.s1 = ldb(ps1, o);
-  v.s2 = ldb(ps2, o);
-  v.n1 = ldb(pn1, o);
-  v.n2 = ldb(pn2, o);
-  cw = __float_as_int(co[(col * nz + k) * C_N + (lane & 31)]);
-}
-
-/* The update of one wet cell k <= ks of column col for one tracer lane,
- * from its 13-point stencil v and coefficient words cw (lanes 0..15: cell
- * k, 16..31: cell k + 1); m: open neighbours at layer k (M_*).  Writes the
- * advected value b of a sub-surface cell, or the raw new surface value
- * (surf_conc) of the processed surface cell, to tout. */
-__device__ __forceinline__ void adv_cell(const St13 &v, int cw, ll col, int k, int m, int ks,
-                                         int flags, const char *tinb, float *tout,
-                                         const ColInfo *__restrict__ cinfo, const float *co,
-                                         const double *__restrict__ layers, float *scr,
-                                         uint32_t lb, uint32_t rowb, ll colb, ll jb, int nz,
-                                         uint32_t hl, float mn, bool clip, bool act,
-                                         double hmin) {
-  const int top = nz - 1;
-  const uint32_t o = lb + k * rowb;
-  char *oc = (char *)tout + col * colb;
-  const float v0 = v.v0;
-  const float qB = rdl(cw, C_QB);
-  const float fba = vla_c(v.zm2, v.zm1, v0, v.zp1, qB, rdl(cw, C_CB)) * qB;
-  HVals h;
-  h.w1 = v.w1;
-  h.w2 = v.w2;
-  h.e1 = v.e1;
-  h.e2 = v.e2;
-  h.s1 = v.s1;
-  h.s2 = v.s2;
-  h.n1 = v.n1;
-  h.n2 = v.n2;
-  const float hh = hflux_own(h, (m & M_E1) != 0, (m & M_N1) != 0, v0, rdl(cw, C_QW),
-                             rdl(cw, C_CW), rdl(cw, C_QE), rdl(cw, C_CE), rdl(cw, C_QS),
-                             rdl(cw, C_CS), rdl(cw, C_QN), rdl(cw, C_CN));
-  const uint32_t key = srkey((uint32_t)(col * nz) + (uint32_t)k, hl);
-  if (k < ks) {
-    /* sub-surface update (tracers.c:912-925) in anomaly form; the top
-     * face is the bottom face of the cell above */
-    const float qT = rdl(cw, C_N + C_QB);
-    const float ht = hh + vla_b(v.zm1, v0, v.zp1, v.zp2, qT, rdl(cw, C_N + C_CB)) * qT - fba;
-    const double an = (double)v0 - (double)((ht + v0 * rdl(cw, C_D)) * rdl(cw, C_RV));
-    float bk = sround(an, key);
-    /* the thin-layer repair of tr_bounds needs the unclipped value */
-    if (k == ks - 1 && (flags & F_THIN) && act) stb((char *)scr + col * colb, o, bk);
-    if (clip && bk < mn) bk = mn;
-    if (act) stb(oc, o, bk);
-  } else {
-    /* processed surface cell: surf_conc (tracers.c:3047-3207) as an
-     * FP64 volume balance, v0 * Vc + anomaly terms, Vc in FP64 */
-    const ColInfo ci = cinfo[col];
-    const char *tc = tinb + col * colb;
-    double m2 = (double)fba - (double)hh;
-    if (flags & F_MOVE) {
-      const int kb = ci.kb;
-      int cc = ks;
-      double wb = (ks == kb) ? ci.botz : layers[ks];
-      while (cc != ci.kold && cc < top) {
-        const double vol = ci.area * (layers[cc + 1] - wb);
-        if (cc > ks) m2 += ((double)ldb(tc, lb + cc * rowb) - (double)v0) * vol;
-        cc++;
-        wb = layers[cc];
-      }
-      if (cc > ks)
-        m2 += ((double)ldb(tc, lb + cc * rowb) - (double)v0) *
-              (ci.area * (ci.osub - ((cc == kb) ? ci.botz : layers[cc])));
-      Nbr nb;
-      nb.c = tc;
-      nb.w1 = tc - colb;
-      nb.w2 = tc - 2 * colb;
-      nb.e1 = tc + colb;
-      nb.e2 = tc + 2 * colb;
-      nb.s1 = tc - jb;
-      nb.s2 = tc - 2 * jb;
-      nb.n1 = tc + jb;
-      nb.n2 = tc + 2 * jb;
-      nb.kw1 = ci.kw1;
-      nb.kw2 = ci.kw2;
-      nb.ke1 = ci.ke1;
-      nb.ke2 = ci.ke2;
-      nb.ks1 = ci.ks1;
-      nb.ks2 = ci.ks2;
-      nb.kn1 = ci.kn1;
-      nb.kn2 = ci.kn2;
-      const float *cob = co + col * nz * C_N;
-      for (int ka = ks + 1; ka <= top; ka++) {
-        HVals ha;
-        load_h(ha, nb, ka, lb + ka * rowb);
-        const float *ca = cob + ka * C_N;
-        m2 -= (double)hflux(ha, ka >= ci.ke1, ka >= ci.kn1, ldb(tc, lb + ka * rowb), v0,
-                            ca[C_QW], ca[C_CW], ca[C_QE], ca[C_CE], ca[C_QS], ca[C_CS],
-                            ca[C_QN], ca[C_CN]);
+/* One workgroup per (tile of 16 x 4 columns, group of 4 tracers): wave w
+ * is row j0 + w of the tile; lane l is column i0 + l % 16 of that row for
+ * tracer 4 g + l / 16.  The four tracer rows of a wave read the same
+ * coefficient words (one request per cell), and a lane's x neighbours are
+ * the lanes beside it (DPP).  Every lane walks its column from the row's
+ * deepest bed to the top; the loads of layer k + 1 are issued before
+ * layer k is computed; the LDS slice holds the advected values b of the
+ * lane's column, the scratch buffer its forward-sweep values. */
+__global__ void __launch_bounds__(64 * SWAVES)
+k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
+        const float4 *__restrict__ co, const Col2 *__restrict__ cinfo,
+        const int *__restrict__ blk, const float *__restrict__ mintr,
+        const float *__restrict__ maxtr, const double *__restrict__ layers,
+        const double *__restrict__ Kz, const double *__restrict__ dzf,
+        float *__restrict__ scr, const int *__restrict__ stepctr) {
+  extern __shared__ float lds[];
+  const int xcd = blockIdx.x & 7;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int q = blockIdx.x >> 3;
+  const int NG = (a.ntr + 3) >> 2;
+  const int bq = q / NG, g = q - bq * NG;
+  /* blk: [0..7] first tile of each XCD, [8..15] tile counts, then per
+   * tile (i0 | j0 << 16, the four rows' deepest beds as bytes) */
+  if (bq >= blk[8 + xcd]) return;
+  const int bi = blk[xcd] + bq;
+  const int b0 = blk[16 + 2 * bi], kmin = (blk[17 + 2 * bi] >> (8 * wv)) & 255;
+  if (kmin >= a.nz) return;
+  const int lane = threadIdx.x & 63;
+  const int cl = lane & 15;
+  const int nr = 4 * g + (lane >> 4);
+  const bool live = nr < a.ntr;
+  const int n = live ? nr : a.ntr - 1;
+  const int i = (b0 & 0xffff) + cl, j = (b0 >> 16) + wv;
+  const int nz = a.nz, top = nz - 1;
+  const uint32_t P2 = (uint32_t)a.P2, nxp = (uint32_t)a.nxp;
+  const uint32_t col = (uint32_t)j * nxp + (uint32_t)i;
+  const Col2 ci = cinfo[col];
+  const int kb = ci.kb, ks = ci.ks, flags = ci.flags;
+  const float mn = mintr[n], mx = maxtr[n];
+  const bool clip = mn <= 1e-5f;
+  const uint32_t hl = mix32((uint32_t)n * 0x85EBCA77U ^ (uint32_t)stepctr[0] * 0xC2B2AE3DU);
+  const uint32_t cellkey0 = (uint32_t)(((ll)j * a.nx + i) * nz);
+  const __amdgpu_buffer_rsrc_t RT = rsrc(tin, (uint32_t)(a.vol * 4 * a.ntr));
+  const __amdgpu_buffer_rsrc_t RC = rsrc(co, (uint32_t)(a.vol * 48));
+  float *OB = tout + (ll)n * a.vol;
+  float *SB = scr + (ll)n * a.vol;
+  const float4 *__restrict__ CC = co + col * 3 + 2; /* the column's matrix words */
+  /* byte offsets: of the lane's column in its tracer, of a layer, of the
+   * column's coefficients in a layer, of a coefficient layer */
+  const uint32_t c4 = (uint32_t)(n * a.vol * 4) + col * 4u, L4 = P2 * 4u;
+  const uint32_t c48 = col * 48u, L48 = P2 * 48u;
+  const int nw1 = ci.nbw & 255, nw2 = (ci.nbw >> 8) & 255, ne1 = (ci.nbw >> 16) & 255,
+            ne2 = (ci.nbw >> 24) & 255;
+  const int ns1 = ci.nbs & 255, ns2 = (ci.nbs >> 8) & 255, nn1 = (ci.nbs >> 16) & 255,
+            nn2 = (ci.nbs >> 24) & 255;
+  const bool proc = (flags & F_PROC) != 0;
+  const bool move = (flags & F_MOVE) != 0;
+  /* the lane reads layers kb..ks (and the layers above for a moving
+   * surface) */
+  const int kend = !proc ? -1 : (move ? top : ks);
+  float *L = lds + wv * a.nzl * 64 + lane;
+  const uint32_t ry = nxp * 4u;
+  auto clampk = [&](int k) { return (uint32_t)min(max(k, kb), top); };
+  /* Issue the loads of layer k (k in kb..kend for this lane; another lane
+   * keeps its registers).  A neighbour that is land or below the bed at
+   * layer k is read as the cell the map walk came from
+   * (tracers.c:5574-5593). */
+  auto issue = [&](int k, Lay &y) {
+    y.z2 = bl1(RT, clampk(k + 2) * L4 + c4);
+    if (k < kb) return;
+    /* the west word of every wet cell: the lane to the west takes its
+     * transport as its east face's */
+    const uint32_t oc = (uint32_t)k * L48 + c48;
+    y.A = bl4(RC, oc);
+    if (k > kend) return;
+    const uint32_t o = (uint32_t)k * L4 + c4;
+    const bool bw1 = k >= nw1, be1 = k >= ne1, bs1 = k >= ns1, bn1 = k >= nn1;
+    const uint32_t ds1 = bs1 ? ry : 0u, dn1 = bn1 ? ry : 0u;
+    const uint32_t ds2 = (bs1 && k >= ns2) ? 2 * ry : ds1, dn2 = (bn1 && k >= nn2) ? 2 * ry : dn1;
+    y.s1 = bl1(RT, o - ds1);
+    y.s2 = bl1(RT, o - ds2);
+    y.n1 = bl1(RT, o + dn1);
+    y.n2 = bl1(RT, o + dn2);
+    /* x neighbours outside the row segment */
+    if (cl < 2) {
+      const uint32_t dw1 = bw1 ? 4u : 0u, dw2 = (bw1 && k >= nw2) ? 8u : dw1;
+      y.xw2 = bl1(RT, o - dw2);
+      if (cl == 0) y.xw1 = bl1(RT, o - dw1);
+    } else if (cl > 13) {
+      const uint32_t de1 = be1 ? 4u : 0u, de2 = (be1 && k >= ne2) ? 8u : de1;
+      y.xe2 = bl1(RT, o + de2);
+      if (cl == 15) y.xe1 = bl1(RT, o + de1);
+    }
+    y.U = bl4(RC, (k < top ? oc + L48 : oc) + 16u);
+    y.M = bl4(RC, oc + 32u);
+    const float2 nn = bl2(RC, oc + (bn1 ? ry * 12u : 0u) + 8u);
+    y.N = make_float2(bn1 ? nn.x : 0.0f, nn.y);
+    if (cl == 15) y.E = bl2(RC, oc + (be1 ? 48u : 0u));
+  };
+  /* register window of the column: layers k-2 .. k+2, clamped to the bed
+   * (the sediment ghost holds the bed value) and to the top */
+  float zm2 = bl1(RT, clampk(kmin - 2) * L4 + c4), zm1 = bl1(RT, clampk(kmin - 1) * L4 + c4);
+  float t0 = bl1(RT, clampk(kmin) * L4 + c4), zp1 = bl1(RT, clampk(kmin + 1) * L4 + c4);
+  float zp2 = 0.0f;
+  float4 Bk = bl4(RC, clampk(kmin) * L48 + c48 + 16u); /* this layer's bottom face */
+  double m2 = 0.0;    /* surface cell: anomaly terms of the volume balance */
+  float v0s = 0.0f;   /* surface cell: its old value                       */
+  float bunc = 0.0f;  /* unclipped advected value below the surface        */
+  float aT = 0.0f;    /* top-face anomaly of the layer below, against it   */
+  const int kold = ci.kold;
+  /* the column solve's forward sweep runs in the walk for the rows whose
+   * right-hand side lies below the surface: row k - 1 at layer k */
+  const bool solve = proc && !(flags & F_SINGLE);
+  const int cb = ci.cb, cs = ci.cs;
+  float bb1 = 0.0f, bb2 = 0.0f; /* b of the two layers below            */
+  float sp = 0.0f;              /* s of the last forward row            */
+  float4 Mp = make_float4(0.0f, 0.0f, 0.0f, 1.0f); /* matrix row below  */
+  float *Ls = L;                /* s_k of the lane's column in its LDS slice */
+  Lay ya, yb;
+  issue(kmin, ya);
+  auto layer = [&](int k, const Lay &y) {
+    const bool has = k >= kb;
+    zp2 = y.z2;
+    /* x neighbours at layer k from the lanes beside this one */
+    const float l1 = shr1(t0), l2 = shr2(t0), r1 = shl1(t0), r2 = shl2(t0);
+    const float qEl = shl1(y.A.x), fEl = shl1(y.A.y);
+    const bool sub = proc && has && k < ks;
+    const bool srf = proc && k == ks;
+    const bool upp = proc && move && k > ks;
+    if (!proc && has && live) OB[(uint32_t)k * P2 + col] = t0;
+    if (sub || srf || upp) {
+      const bool bw1 = k >= nw1, be1 = k >= ne1;
+      const bool bw2 = bw1 && k >= nw2, be2 = be1 && k >= ne2;
+      const float w1 = (cl == 0) ? y.xw1 : (bw1 ? l1 : t0);
+      const float w2 = (cl < 2) ? y.xw2 : (bw2 ? l2 : w1);
+      const float e1 = (cl == 15) ? y.xe1 : (be1 ? r1 : t0);
+      const float e2 = (cl > 13) ? y.xe2 : (be2 ? r2 : e1);
+      const float qE = be1 ? ((cl == 15) ? y.E.x : qEl) : 0.0f;
+      const float fE = (cl == 15) ? y.E.y : fEl;
+      const float ref = upp ? v0s : t0;
+      const float AW = vla(w2, w1, t0, e1, y.A.x, y.A.y, ref);
+      const float AE = vla(w1, t0, e1, e2, qE, fE, ref);
+      const float AS = vla(y.s2, y.s1, t0, y.n1, y.A.z, y.A.w, ref);
+      const float AN = vla(y.s1, t0, y.n1, y.n2, y.N.x, y.N.y, ref);
+      const float hh = AE * qE - AW * y.A.x + AN * y.N.x - AS * y.A.z;
+      if (upp) {
+        /* a layer above the processed surface cell (tracers.c:3117-3193) */
+        m2 -= (double)hh;
+        if (k < kold)
+          m2 += ((double)t0 - (double)v0s) * (ci.area * (layers[k + 1] - layers[k]));
+        else if (k == kold)
+          m2 += ((double)t0 - (double)v0s) * (ci.area * (ci.osub - layers[k]));
+      } else {
+        /* the bottom face is the top face of the layer below, against
+         * this cell's value; zero at the bed (the ghost holds the bed
+         * value) */
+        const float fba = (k > kb) ? (aT + (zm1 - t0)) * Bk.x : 0.0f;
+        if (sub) {
+          aT = vla(zm1, t0, zp1, zp2, y.U.x, y.U.y, t0);
+          const double an = (double)t0 - (double)((hh + aT * y.U.x - fba + t0 * Bk.w) * Bk.z);
+          float bk = sround(an, srkey(cellkey0 + (uint32_t)k, hl));
+          if (k == ks - 1) bunc = bk;
+          if (clip && bk < mn) bk = mn;
+          if (live) OB[(uint32_t)k * P2 + col] = bk;
+          const int r = k - 1;
+          if (solve && r >= cb) {
+            /* r_k = -(gm (b_{k-1} - b_k) + gp (b_{k+1} - b_k)),
+             * s_k = (r_k - gm s_{k-1}) / div_k */
+            const float gm = (r > cb) ? Mp.x : 0.0f;
+            const float bm = (r > cb) ? bb2 : bb1;
+            const float rr = -(gm * (bm - bb1) + Mp.y * (bk - bb1));
+            sp = (rr - gm * sp) * Mp.w;
+            Ls[(r - kmin) * 64] = sp;
+          }
+          bb2 = bb1;
+          bb1 = bk;
+        } else {
+          m2 = (double)fba - (double)hh;
+          v0s = t0;
+        }
       }
     }
-    double surftr = (double)v0 * ci.vc + m2;
-    if (ci.svol == 0.0 || (surftr < 0.0 && ci.dzs <= hmin))
+    /* the next layer's bottom face: loaded with this layer (below the
+     * bed, the bed cell's word loaded at the start stays); this layer's
+     * matrix row for the next forward row */
+    if (has) {
+      Bk = y.U;
+      Mp = y.M;
+    }
+    /* advance the window */
+    zm2 = zm1;
+    zm1 = t0;
+    t0 = zp1;
+    zp1 = zp2;
+  };
+  int k = kmin;
+  for (; k + 1 <= top; k += 2) {
+    issue(k + 1, yb);
+    layer(k, ya);
+    issue(k + 2, ya);
+    layer(k + 1, yb);
+  }
+  if (k <= top) layer(k, ya);
+  if (!proc || !live) return;
+
+  /* surface value (surf_conc), then tr_bounds at the surface */
+  float trs;
+  {
+    double surftr = (double)v0s * ci.vc + m2;
+    if (ci.svol == 0.0 || (surftr < 0.0 && ci.dzs <= a.hmin))
       surftr = 0.0;
     else
       surftr /= ci.svol;
-    /* tracers.c:953-954; tr_bounds is applied in kernel B */
-    const float trs = (flags & F_KEEPOLD) ? v0 : sround(surftr, key);
-    if (act) stb(oc, o, trs);
-  }
-}
-
-/*
- * Kernel A: every wet cell of every column (all tracers of one cell per
- * wave, lanes = tracers), cells taken in layer-synchronous tile order so
- * that the neighbours a cell reads were read by other waves moments
- * before.  Each wave slot has its own list of cells (descriptors and the
- * column's processed surface and flags, 64 at a time, lane-distributed),
- * and the stencil and coefficients of its next cell are loaded while the
- * current one is computed.  Writes the advected value b of sub-surface
- * cells and the raw new surface value (surf_conc) of the processed surface
- * cell to the output; tr_bounds and the vertical solve follow in kernel B.
- */
-template <int SCH>
-__global__ void __launch_bounds__(256)
-k_advB(const AArgs a, const float *__restrict__ tin, float *__restrict__ tout,
-       const float *__restrict__ co, const unsigned long long *__restrict__ cells,
-       const int *__restrict__ kfl, const ColInfo *__restrict__ cinfo,
-       const float *__restrict__ mintr, const double *__restrict__ layers,
-       float *__restrict__ scr, const int *__restrict__ stepctr) {
-  const int nz = a.nz, NT = a.ntr, nact = a.nact;
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int li = min(lane, nact - 1); /* inactive lanes shadow the last tracer */
-  const int ln = a.g * 64 + li;
-  const bool act = lane < nact;
-  const uint32_t lb = (uint32_t)ln * 4u;
-  const uint32_t rowb = (uint32_t)NT * 4u;
-  const ll colb = (ll)nz * NT * 4;
-  const ll jb = a.nx * colb;
-  const float mn = mintr[ln];
-  const bool clip = mn <= 1e-5f;
-  const uint32_t hl = mix32((uint32_t)ln * 0x85EBCA77U ^ (uint32_t)stepctr[0] * 0xC2B2AE3DU);
-  const char *tinb = (const char *)tin;
-  const int slot = blockIdx.x * 4 + wv;
-  const unsigned long long *myc = cells + (ll)slot * a.niter;
-  const int *myk = kfl + (ll)slot * a.niter;
-  const int niter = a.niter;
-  if (niter == 0) return;
-
-  unsigned long long dcur = myc[lane], dnxt = NO_CELL;
-  int kcur = myk[lane], knxt = 0;
-  if (niter > 64) {
-    dnxt = myc[64 + lane];
-    knxt = myk[64 + lane];
-  }
-  /* Two cells per iteration with fixed stencil registers: the loads of
-   * one cell are in flight while the other is computed.  Padding entries
-   * load the rows of a stand-in cell (column 0, layer 0, no open
-   * neighbours: always in bounds) so that every load is issued
-   * unconditionally and is never computed. */
-  auto live = [](unsigned long long d) -> unsigned long long { return d == NO_CELL ? 0ULL : d; };
-  auto one = [&](const St13 &v, int cw, unsigned long long d, int kf) {
-    if (d == NO_CELL) return;
-    const int k = (int)((d >> 32) & 255);
-    const int ks = (kf & 255) - 1, flags = kf >> 8;
-    if (k > ks) return; /* above the processed surface, or not processed */
-    adv_cell(v, cw, (ll)(uint32_t)d, k, (int)((d >> 40) & 255), ks, flags, tinb, tout, cinfo,
-             co, layers, scr, lb, rowb, colb, jb, nz, hl, mn, clip, act, a.hmin);
-  };
-  St13 va, vb;
-  int cwa = 0, cwb = 0;
-  st13_load(va, cwa, tinb, co, live(rdl64(dcur, 0)), colb, jb, lb, rowb, nz, lane);
-  for (int j = 0; j < niter; j += 2) { /* niter is a multiple of 64 */
-    const int jl = j & 63;
-    const unsigned long long dA = rdl64(dcur, jl), dB = rdl64(dcur, jl + 1);
-    const int kfA = __builtin_amdgcn_readlane(kcur, jl);
-    const int kfB = __builtin_amdgcn_readlane(kcur, jl + 1);
-    st13_load(vb, cwb, tinb, co, live(dB), colb, jb, lb, rowb, nz, lane);
-    one(va, cwa, dA, kfA);
-    if (jl == 62) {
-      dcur = dnxt;
-      kcur = knxt;
-      if (j + 66 < niter) {
-        dnxt = myc[j + 66 + lane];
-        knxt = myk[j + 66 + lane];
-      }
-    }
-    const unsigned long long dN = (j + 2 < niter) ? rdl64(dcur, (jl + 2) & 63) : NO_CELL;
-    st13_load(va, cwa, tinb, co, live(dN), colb, jb, lb, rowb, nz, lane);
-    one(vb, cwb, dB, kfB);
-  }
-}
-
-/* per list entry: the processed surface (+1) and flags of its column */
-__global__ void k_kflB(const unsigned long long *__restrict__ cells, ll n,
-                       const ColInfo *__restrict__ cinfo, int *__restrict__ kfl) {
-  const ll i = (ll)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const unsigned long long d = cells[i];
-  int v = 0;
-  if (d != NO_CELL) {
-    const ColInfo *c = cinfo + (uint32_t)d;
-    v = (c->ks + 1) | (c->flags << 8);
-  }
-  kfl[i] = v;
-}
-
-/*-------------------------------------------------------------------*/
-/* kernel B: tr_bounds and the vertical solve, one wave per column     */
-/*-------------------------------------------------------------------*/
-/*
- * Lanes = tracers.  Applies tr_bounds at the processed surface (thin old
- * surface layer, clip; tracers.c:3217-3324), fills the layers above with
- * the surface value, then solves (A/dzdt) d = b - (A/dzdt) b over the
- * diffusion extent (tracers.c:3782-3957, 4067-4214) with the LU factors
- * from the prep and writes x = b + d.  The advected values b are read
- * eight layers at a time; s_k and ud_k live in the wave's LDS slice.
- */
-
-template <int SCH>
-__global__ void __launch_bounds__(64 * WPB)
-k_vertB(const KArgs a, const float *__restrict__ tin, float *__restrict__ tout,
-        const float *__restrict__ co, const ColInfo *__restrict__ cinfo,
-        const int *__restrict__ order, const float *__restrict__ mintr,
-        const float *__restrict__ maxtr, const double *__restrict__ Kz,
-        const double *__restrict__ dzf, float *__restrict__ scr,
-        const int *__restrict__ stepctr) {
-  extern __shared__ float lds[];
-  const int nz = a.nz, NT = a.ntr, top = nz - 1, nact = a.nact;
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int li = min(lane, nact - 1);
-  float *Ls = lds + wv * nz * (nact + 1);
-  float *Lud = Ls + nz * nact;
-  const int ln = a.g * 64 + li;
-  const bool act = lane < nact;
-  const uint32_t lb = (uint32_t)ln * 4u;
-  const uint32_t rowb = (uint32_t)NT * 4u;
-  const ll colb = (ll)nz * NT * 4;
-  const float mn = mintr[ln];
-  const float mx = maxtr[ln];
-  const bool clip = mn <= 1e-5f;
-  const uint32_t hl = mix32((uint32_t)ln * 0x85EBCA77U ^ (uint32_t)stepctr[0] * 0xC2B2AE3DU) ^
-                      0x5bd1e995U;
-  const int nslot = gridDim.x * WPB;
-
-  for (int q = blockIdx.x * WPB + wv; q < a.nord; q += nslot) {
-    const int colq = order[q];
-    if (colq < 0) continue;
-    const ll col = colq;
-    const ColInfo ci = cinfo[col];
-    const int kb = ci.kb, ks = ci.ks;
-    char *oc = (char *)tout + col * colb;
-    const char *tc = (const char *)tin + col * colb;
-    if (!(ci.flags & F_PROC)) {
-      for (int k = kb; k <= top; k++)
-        if (act) stb(oc, lb + k * rowb, ldb(tc, lb + k * rowb));
-      continue;
-    }
-    /* tr_bounds at the surface */
-    float trs = ldb(oc, lb + ks * rowb);
-    if ((ci.flags & F_THIN) && (trs < mn || trs > mx))
-      trs = ldb((const char *)scr + col * colb, lb + (ks - 1) * rowb);
+    trs = (flags & F_KEEPOLD) ? v0s : sround(surftr, srkey(cellkey0 + (uint32_t)ks, hl));
+    if ((flags & F_THIN) && (trs < mn || trs > mx)) trs = bunc;
     if (clip && trs < mn) trs = mn;
-    for (int k = ks; k <= top; k++)
-      if (act) stb(oc, lb + k * rowb, trs);
-    if (ci.flags & F_SINGLE) continue;
-    const int cb = ci.cb, cs = ci.cs;
-    const float *cob = co + col * nz * C_N;
-    const uint32_t hcol = (uint32_t)(col * nz);
-    /* forward sweep: s_k = (r_k - gm_k s_{k-1}) / div_k,
-     * r_k = -(gm (b_{k-1} - b_k) + gp (b_{k+1} - b_k)) */
-    float sp = 0.0f, bkm = 0.0f, bk = 0.0f;
-    for (int k0 = cb; k0 <= cs; k0 += 8) {
-      /* coefficient words of layers k0..k0+3 and k0+4..k0+7, lane l holding
-       * word l, issued with the b loads of the chunk */
-      const int cA = __float_as_int(cob[k0 * C_N + lane]);
-      const int cB = __float_as_int(cob[(k0 + 4) * C_N + lane]);
-      float bb[9];
-#pragma unroll
-      for (int j = 0; j < 9; j++) {
-        const int k = min(k0 + j, cs);
-        bb[j] = (k < ks) ? ldb(oc, lb + k * rowb) : trs;
-      }
-      if (k0 == cb) {
-        bk = bb[0];
-        bkm = bk;
-      }
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int k = k0 + j;
-        if (k <= cs) {
-          const int cv = (j < 4) ? cA : cB;
-          const int w = 16 * (j & 3);
-          const float gm = (k > cb) ? rdl(cv, w + C_GM) : 0.0f, gp = rdl(cv, w + C_GP);
-          const float bkp = (k < cs) ? bb[j + 1] : bk;
-          const float r = -(gm * (bkm - bk) + gp * (bkp - bk));
-          sp = (r - gm * sp) * rdl(cv, w + C_RD);
-          Ls[k * nact + li] = sp;
-          Lud[k] = rdl(cv, w + C_UD);
-          bkm = bk;
-          bk = bkp;
-        }
-      }
+  }
+  /* the surface value in the layers above the surface outside the
+   * solved range (the advected values are in the output column) */
+  const int lo = solve ? cb : top + 1, hi = solve ? cs : top;
+  for (int kk = ks; kk <= top; kk++)
+    if (kk < lo || kk > hi) OB[(uint32_t)kk * P2 + col] = trs;
+  if (!solve) return;
+  const uint32_t hv = hl ^ 0x5bd1e995U;
+  /* the remaining forward rows (their right-hand side reaches the
+   * surface value), b below the surface from the registers */
+
+  {
+    const int rs = max(cb, ks - 1);
+    for (int r = rs; r <= cs; r++) {
+      const float4 M = CC[(ll)r * P2 * 3];
+      const float gm = (r > cb) ? M.x : 0.0f;
+      /* b at r - 1, r, r + 1: r >= ks - 1, so the values below the
+       * surface are the walk's last two */
+      const float b0 = (r >= ks) ? trs : bb1;
+      const float bm = (r > cb) ? ((r - 1 >= ks) ? trs : (r - 1 == ks - 1 ? bb1 : bb2)) : b0;
+      const float bp = (r < cs) ? trs : b0;
+      const float rr = -(gm * (bm - b0) + M.y * (bp - b0));
+      sp = (rr - gm * sp) * M.w;
+      Ls[(r - kmin) * 64] = sp;
     }
-    /* back substitution: d_k = s_k - ud_{k+1} d_{k+1} */
-    float dn = 0.0f;
-    const bool stiff = (ci.flags & F_STIFF) != 0;
-    for (int k0 = cs; k0 >= cb; k0 -= 8) {
-      float bb[8];
+  }
+  /* Back substitution d_k = s_k - ud_{k+1} d_{k+1}, x = b + d, eight rows
+   * at a time: their b and ud are loaded before their results are stored
+   * over b (d goes to the scratch column in a stiff column) */
+  const bool stiff = (flags & F_STIFF) != 0;
+  {
+    float dn = 0.0f, udn = 0.0f;
+    for (int r0 = cs; r0 >= cb; r0 -= 8) {
+      float bq[8], uq[8];
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int k = max(k0 - j, cb);
-        bb[j] = (k < ks) ? ldb(oc, lb + k * rowb) : trs;
+      for (int jj = 0; jj < 8; jj++) {
+        const int r = r0 - jj;
+        const int rc = max(r, cb);
+        bq[jj] = (rc < ks) ? OB[(uint32_t)rc * P2 + col] : trs;
+        uq[jj] = CC[(ll)rc * P2 * 3].z;
       }
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int k = k0 - j;
-        if (k >= cb) {
-          const float sk = Ls[k * nact + li];
-          dn = (k == cs) ? sk : sk - Lud[k + 1] * dn;
+      for (int jj = 0; jj < 8; jj++) {
+        const int r = r0 - jj;
+        if (r >= cb) {
+          const uint32_t o = (uint32_t)r * P2 + col;
+          const float sk = Ls[(r - kmin) * 64];
+          dn = (r == cs) ? sk : sk - udn * dn;
           if (stiff)
-            Ls[k * nact + li] = dn;
-          else if (act)
-            stb(oc, lb + k * rowb,
-                sround((double)bb[j] + (double)dn, srkey(hcol + (uint32_t)k, hl)));
+            SB[o] = dn;
+          else
+            OB[o] = sround((double)bq[jj] + (double)dn, srkey(cellkey0 + (uint32_t)r, hv));
+          udn = uq[jj];
         }
       }
     }
-    if (stiff) {
-      /* one FP64 residual pass of iterative refinement, the matrix and
-       * right-hand side in FP64 from Kz and dz (tracers.c:3841-3889) */
-      const ll P = a.plane;
-      const double dt = a.dt;
-      char *sc = (char *)scr + col * colb;
-      float e = 0.0f;
-      for (int k = cb; k <= cs; k++) {
-        const ll ix = (ll)k * P + col;
-        const double dzc = dzf[ix], dzdt = dzc / dt;
-        double cm1 = 0.0, cp1 = 0.0;
-        if (k > cb) cm1 = -Kz[ix] / (0.5 * (dzf[ix - P] + dzc));
-        if (k < cs) cp1 = -Kz[ix + P] / (0.5 * (dzc + dzf[ix + P]));
-        const double bv = (double)((k < ks) ? ldb(oc, lb + k * rowb) : trs);
-        const double dk = (double)Ls[k * nact + li];
-        const double bvm = (k > cb) ? (double)((k - 1 < ks) ? ldb(oc, lb + (k - 1) * rowb) : trs) : bv;
-        const double bvp = (k < cs) ? (double)((k + 1 < ks) ? ldb(oc, lb + (k + 1) * rowb) : trs) : bv;
-        const double dkm = (k > cb) ? (double)Ls[(k - 1) * nact + li] : 0.0;
-        const double dkp = (k < cs) ? (double)Ls[(k + 1) * nact + li] : 0.0;
-        const double rhs = -(cm1 * (bvm - bv) + cp1 * (bvp - bv));
-        const double Ad = cm1 * dkm + (dzdt - cm1 - cp1) * dk + cp1 * dkp;
-        const float rho = (float)((rhs - Ad) / dzdt);
-        const float *cm = cob + k * C_N;
-        const float gm = (k > cb) ? cm[C_GM] : 0.0f;
-        e = (rho - gm * e) * cm[C_RD];
-        stb(sc, lb + k * rowb, e);
-      }
-      float ec = 0.0f;
-      for (int k = cs; k >= cb; k--) {
-        const float ek = ldb(sc, lb + k * rowb);
-        ec = (k == cs) ? ek : ek - Lud[k + 1] * ec;
-        stb(sc, lb + k * rowb, ec);
-      }
-      /* every residual is formed before any b_k is replaced */
-      for (int k = cb; k <= cs; k++) {
-        const float bv = (k < ks) ? ldb(oc, lb + k * rowb) : trs;
-        const float ek = ldb(sc, lb + k * rowb);
-        if (act)
-          stb(oc, lb + k * rowb,
-              sround((double)bv + (double)Ls[k * nact + li] + (double)ek,
-                     srkey(hcol + (uint32_t)k, hl)));
-      }
+  }
+  if (!stiff) return;
+  /* one FP64 residual pass of iterative refinement, the matrix and the
+   * right-hand side in FP64 from Kz and dz (tracers.c:3841-3889); the
+   * correction is solved with the FP32 factors; b is in the output
+   * column, d in the scratch column, the correction in the LDS slice */
+  {
+    const ll P = a.P;
+    const ll fc = (ll)(j) * a.nx + i;
+    const double dt = a.dt;
+    auto bval = [=](int kk) { return (kk < ks) ? OB[(uint32_t)kk * P2 + col] : trs; };
+    float e = 0.0f;
+    for (int kk = cb; kk <= cs; kk++) {
+      const ll ix = (ll)kk * P + fc;
+      const uint32_t o = (uint32_t)kk * P2 + col;
+      const double dzc = dzf[ix], dzdt = dzc / dt;
+      double cm1 = 0.0, cp1 = 0.0;
+      if (kk > cb) cm1 = -Kz[ix] / (0.5 * (dzf[ix - P] + dzc));
+      if (kk < cs) cp1 = -Kz[ix + P] / (0.5 * (dzc + dzf[ix + P]));
+      const double bv = (double)bval(kk);
+      const double dk = (double)SB[o];
+      const double bvm = (kk > cb) ? (double)bval(kk - 1) : bv;
+      const double bvp = (kk < cs) ? (double)bval(kk + 1) : bv;
+      const double dkm = (kk > cb) ? (double)SB[o - P2] : 0.0;
+      const double dkp = (kk < cs) ? (double)SB[o + P2] : 0.0;
+      const double rhs = -(cm1 * (bvm - bv) + cp1 * (bvp - bv));
+      const double Ad = cm1 * dkm + (dzdt - cm1 - cp1) * dk + cp1 * dkp;
+      const float rho = (float)((rhs - Ad) / dzdt);
+      const float4 M = CC[(ll)kk * P2 * 3];
+      const float gm = (kk > cb) ? M.x : 0.0f;
+      e = (rho - gm * e) * M.w;
+      Ls[(kk - kmin) * 64] = e;
+    }
+    float ec = 0.0f, udn = 0.0f;
+    for (int kk = cs; kk >= cb; kk--) {
+      const float ek = Ls[(kk - kmin) * 64];
+      ec = (kk == cs) ? ek : ek - udn * ec;
+      udn = CC[(ll)kk * P2 * 3].z;
+      Ls[(kk - kmin) * 64] = ec;
+    }
+    /* every residual is formed before any value is replaced */
+    for (int kk = cb; kk <= cs; kk++) {
+      const uint32_t o = (uint32_t)kk * P2 + col;
+      OB[o] = sround((double)bval(kk) + (double)SB[o] + (double)Ls[(kk - kmin) * 64],
+                     srkey(cellkey0 + (uint32_t)kk, hv));
     }
   }
 }
 
 /* advance the stochastic-rounding step counter (runs after the step) */
-__global__ void k_tick(int *ctr) { *ctr += 1; }
+__global__ void k_tick2(int *ctr) { *ctr += 1; }
 
 /*-------------------------------------------------------------------*/
 /* layout conversion and mass                                        */
 /*-------------------------------------------------------------------*/
-__global__ void k_putB(const int *__restrict__ wsp, const ll *__restrict__ wdn, int nwet,
-                       ll plane, int nz, int NT, int n, const double *__restrict__ sp,
-                       float *__restrict__ a, float *__restrict__ a2) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nwet) return;
-  const ll idx = wdn[q];
-  const ll col = idx % plane;
-  const int k = (int)(idx / plane);
-  const ll e = cidx(col, nz, k) * NT + n;
-  const float v = (float)sp[wsp[q]];
-  a[e] = v;
-  a2[e] = v;
+struct LArgs {
+  ll plane, P2, vol;
+  int nx, nxp, nwet;
+};
+
+__device__ __forceinline__ ll bidx(const LArgs &a, ll idx) {
+  const int k = (int)(idx / a.plane);
+  const ll col = idx - (ll)k * a.plane;
+  const ll jj = col / a.nx, ii = col - jj * a.nx;
+  return (ll)k * a.P2 + jj * a.nxp + ii;
 }
 
-__global__ void k_getB(const int *__restrict__ wsp, const ll *__restrict__ wdn, int nwet,
-                       ll plane, int nz, int NT, int n, const float *__restrict__ a,
-                       double *__restrict__ sp) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nwet) return;
-  const ll idx = wdn[q];
-  const ll col = idx % plane;
-  const int k = (int)(idx / plane);
-  sp[wsp[q]] = (double)a[cidx(col, nz, k) * NT + n];
+__global__ void k_put2(LArgs a, const int *__restrict__ wsp, const ll *__restrict__ wdn, int n,
+                       const double *__restrict__ sp, float *__restrict__ t0,
+                       float *__restrict__ t1) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.nwet) return;
+  const ll e = (ll)n * a.vol + bidx(a, wdn[q]);
+  const float v = (float)sp[wsp[q]];
+  t0[e] = v;
+  t1[e] = v;
+}
+
+__global__ void k_get2(LArgs a, const int *__restrict__ wsp, const ll *__restrict__ wdn, int n,
+                       const float *__restrict__ t, double *__restrict__ sp) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.nwet) return;
+  sp[wsp[q]] = (double)t[(ll)n * a.vol + bidx(a, wdn[q])];
 }
 
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
@@ -911,7 +749,7 @@ __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e
 }
 
 /* per-block (sum, err) of sum(tr*area*dz) over the processed cells */
-__global__ void k_massB(emstr_dev d, emstr_batch b, const float *__restrict__ tr,
+__global__ void k_mass2(emstr_dev d, LArgs la, int nz, const float *__restrict__ tr,
                         double *__restrict__ part) {
   const int n = blockIdx.y;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -920,9 +758,11 @@ __global__ void k_massB(emstr_dev d, emstr_batch b, const float *__restrict__ tr
   if (q < d.ncol) {
     const ll col = d.cols[q];
     const int kbc = d.kb[col], ksc = d.ks[col];
-    const double a = d.cellarea[col];
+    const double ar = d.cellarea[col];
+    const ll jj = col / d.nx, ii = col - jj * d.nx;
+    const float *t = tr + (ll)n * la.vol + jj * la.nxp + ii;
     for (int k = ksc; k >= kbc; k--) {
-      double m = (double)tr[cidx(col, b.nz, k) * b.ntr + n] * a * d.dz[(ll)k * d.plane + col];
+      const double m = (double)t[(ll)k * la.P2] * ar * d.dz[(ll)k * d.plane + col];
       double s2, e2;
       two_sum(s, m, s2, e2);
       s = s2;
@@ -950,106 +790,89 @@ __global__ void k_massB(emstr_dev d, emstr_batch b, const float *__restrict__ tr
 /*-------------------------------------------------------------------*/
 /* host interface                                                    */
 /*-------------------------------------------------------------------*/
-/* dynamic LDS of kernel B: one [nz][nact + 1] slice per wave */
-static size_t vert_lds(int nz, int nact) {
-  return (size_t)WPB * nz * (nact + 1) * sizeof(float);
-}
+static int shape_ok(int tx) { return tx == 16; }
 
-/* Cell lists of kernel A.  Cells are ordered tile by tile (tile_x x
- * tile_y columns, row-major tiles), inside a tile layer by layer from the
- * deepest, inside a layer row-major; the sequence is split into 8
- * contiguous bands, one per XCD (workgroups are dealt to the XCDs round
- * robin).  The S wave slots of XCD x take band x in turns: cell c of the
- * band is iteration c / S of the slot of rank c % S, so at any time the
- * slots of one XCD work on one stretch of one tile layer. */
-static int build_cells(emstr_batch *b, const int *kb, int nz) {
-  const int nx = b->nx, ny = b->ny, tx = b->tile_x, ty = b->tile_y;
-  ll n = 0;
-  for (ll c = 0; c < b->plane; c++)
-    if (kb[c] < nz) n += nz - kb[c];
-  unsigned long long *seq = (unsigned long long *)malloc(sizeof(unsigned long long) * (size_t)(n + 1));
-  ll q = 0;
-  for (int J = 0; J < ny; J += ty)
-    for (int I = 0; I < nx; I += tx)
-      for (int k = 0; k < nz; k++)
-        for (int j = J; j < J + ty && j < ny; j++)
-          for (int i = I; i < I + tx && i < nx; i++) {
-            const ll col = (ll)j * nx + i;
-            if (kb[col] > k) continue;
-            const bool w1 = kb[col - 1] <= k, e1 = kb[col + 1] <= k;
-            const bool s1 = kb[col - nx] <= k, n1 = kb[col + nx] <= k;
-            const bool w2 = w1 && kb[col - 2] <= k, e2 = e1 && kb[col + 2] <= k;
-            const bool s2 = s1 && kb[col - 2 * nx] <= k, n2 = n1 && kb[col + 2 * nx] <= k;
-            const unsigned m = (w1 ? M_W1 : 0) | (w2 ? M_W2 : 0) | (e1 ? M_E1 : 0) |
-                               (e2 ? M_E2 : 0) | (s1 ? M_S1 : 0) | (s2 ? M_S2 : 0) |
-                               (n1 ? M_N1 : 0) | (n2 ? M_N2 : 0);
-            seq[q++] = (unsigned long long)(uint32_t)col | ((unsigned long long)k << 32) |
-                       ((unsigned long long)m << 40) | ((unsigned long long)kb[col] << 48);
-          }
-  const ll nslot = (ll)b->gridA * 4, S = nslot / 8;
-  const ll P = (n + 7) / 8;
-  ll niter = (P + S - 1) / S;
-  niter = (niter + 63) / 64 * 64;
-  if (niter == 0) niter = 64;
-  const ll total = nslot * niter;
-  unsigned long long *lst = (unsigned long long *)malloc(sizeof(unsigned long long) * (size_t)total);
-  for (ll t = 0; t < total; t++) lst[t] = NO_CELL;
-  for (int x = 0; x < 8; x++)
-    for (ll c = 0; c < P; c++) {
-      const ll src = x * P + c;
-      if (src >= n) break;
-      const ll r = c % S, it = c / S;
-      const ll slot = ((r / 4) * 8 + x) * 4 + (r % 4);
-      lst[slot * niter + it] = seq[src];
+/* Tiles of 16 x 4 columns over the padded plane (a row of 16 columns per
+ * wave of a workgroup); all-land tiles dropped.
+ * Order: vertical strips of `strip` columns, in each strip the tiles row
+ * by row (so a tile's south halo rows are those of a tile processed
+ * shortly before it); the sequence is cut into 8 contiguous chunks of
+ * equal work (wet cells), one per XCD. */
+static int build_blocks(emstr_batch *b, const int *kb) {
+  const int tx = 16, ty = 1, nxp = b->nxp, nyp = b->nyp, nz = b->nz;
+  const int tty = SWAVES * ty;
+  const int strip = b->strip < tx ? tx : b->strip / tx * tx;
+  const int nbx = nxp / tx, nby = (nyp + tty - 1) / tty;
+  int *seq = (int *)malloc(sizeof(int) * (16 + 2 * (size_t)nbx * nby));
+  ll *work = (ll *)malloc(sizeof(ll) * (size_t)nbx * nby + 1);
+  int ns = 0, kmin_all = nz;
+  ll total = 0;
+  for (int s0 = 0; s0 < nxp; s0 += strip)
+    for (int by = 0; by < nby; by++)
+      for (int i0 = s0; i0 < s0 + strip && i0 < nxp; i0 += tx) {
+        const int j0 = by * tty;
+        int kms = 0, kt = nz;
+        ll wk = 0;
+        for (int w = 0; w < SWAVES; w++) {
+          int km = nz;
+          for (int jj = j0 + w * ty; jj < j0 + (w + 1) * ty; jj++)
+            for (int ii = i0; ii < i0 + tx; ii++) {
+              if (ii >= b->nx || jj >= b->ny) continue;
+              const int v = kb[(ll)jj * b->nx + ii];
+              if (v < nz) {
+                km = v < km ? v : km;
+                wk += nz - v;
+              }
+            }
+          kms |= km << (8 * w);
+          kt = km < kt ? km : kt;
+        }
+        if (kt >= nz) continue;
+        seq[16 + 2 * ns] = i0 | (j0 << 16);
+        seq[17 + 2 * ns] = kms;
+        work[ns] = wk;
+        total += wk;
+        kmin_all = kt < kmin_all ? kt : kmin_all;
+        ns++;
+      }
+  /* 8 chunks of equal work */
+  int x = 0;
+  ll acc = 0;
+  for (int q = 0; q < 8; q++) b->bcnt[q] = 0;
+  b->boff[0] = 0;
+  for (int s = 0; s < ns; s++) {
+    while (x < 7 && acc >= (total * (x + 1) + 7) / 8) {
+      x++;
+      b->boff[x] = s;
     }
-  if (b->cells) (void)hipFree(b->cells);
-  if (b->kfl) (void)hipFree(b->kfl);
-  b->cells = NULL;
-  b->kfl = NULL;
-  bool ok = hipMalloc(&b->cells, sizeof(unsigned long long) * (size_t)total) == hipSuccess;
-  ok &= hipMalloc(&b->kfl, sizeof(int) * (size_t)total) == hipSuccess;
+    b->bcnt[x]++;
+    acc += work[s];
+  }
+  for (int q = x + 1; q < 8; q++) b->boff[q] = ns;
+  b->bmax = 0;
+  for (int q = 0; q < 8; q++) b->bmax = b->bcnt[q] > b->bmax ? b->bcnt[q] : b->bmax;
+  b->kmin = kmin_all;
+  for (int q = 0; q < 8; q++) {
+    seq[q] = b->boff[q];
+    seq[8 + q] = b->bcnt[q];
+  }
+  if (b->blk) (void)hipFree(b->blk);
+  b->blk = NULL;
+  b->nblk = ns;
+  bool ok = hipMalloc(&b->blk, sizeof(int) * (16 + 2 * (size_t)ns)) == hipSuccess;
   if (ok)
-    ok = hipMemcpy(b->cells, lst, sizeof(unsigned long long) * (size_t)total,
-                   hipMemcpyHostToDevice) == hipSuccess;
-  free(lst);
+    ok = hipMemcpy(b->blk, seq, sizeof(int) * (16 + 2 * (size_t)ns), hipMemcpyHostToDevice) ==
+         hipSuccess;
   free(seq);
-  if (!ok) return emstr_set_error(EMSTR_EDEVICE, -1, "batched form: cell lists");
-  b->niter = (int)niter;
-  b->nlist = total;
+  free(work);
+  if (!ok) return emstr_set_error(EMSTR_EDEVICE, -1, "batched form: block lists");
   return 0;
 }
 
-/* Column order: tiles of tile_x x tile_y columns in row-major tile
- * order, columns row-major inside a tile; the wet columns of that order
- * are split into 8 contiguous bands.  Position q of the persistent loop is
- * taken by wave q % WPB of workgroup q / WPB, and workgroups are dealt to
- * the 8 XCDs round robin, so position q maps to band (q / WPB) % 8 and the
- * waves resident on one XCD walk one band together. */
-static void build_order(emstr_batch *b, const int *kb, int nz) {
-  const int nx = b->nx, ny = b->ny, tx = b->tile_x, ty = b->tile_y;
-  int *lin = (int *)malloc(sizeof(int) * (size_t)b->plane);
-  int n = 0;
-  for (int J = 0; J < ny; J += ty)
-    for (int I = 0; I < nx; I += tx)
-      for (int j = J; j < J + ty && j < ny; j++)
-        for (int i = I; i < I + tx && i < nx; i++) {
-          const ll col = (ll)j * nx + i;
-          if (kb[col] < nz) lin[n++] = (int)col;
-        }
-  b->ncol = n;
-  /* positions per band, a multiple of WPB */
-  const int per = ((n + 7) / 8 + WPB - 1) / WPB * WPB;
-  b->nord = 8 * per;
-  int *ord = (int *)malloc(sizeof(int) * (size_t)(b->nord > 0 ? b->nord : 1));
-  for (int q = 0; q < b->nord; q++) {
-    const int m = q / WPB;
-    const int band = m % 8, idx = (m / 8) * WPB + q % WPB;
-    const int src = band * ((n + 7) / 8) + idx;
-    ord[q] = (idx < (n + 7) / 8 && src < n) ? lin[src] : -1;
-  }
-  hipMemcpy(b->order, ord, sizeof(int) * (size_t)b->nord, hipMemcpyHostToDevice);
-  free(ord);
-  free(lin);
+static void drop_graph(emstr_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec), b->gexec = NULL;
+  if (b->graph) (void)hipGraphDestroy(b->graph), b->graph = NULL;
+  b->graph_ok = 0;
 }
 
 extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int *h_kb,
@@ -1059,24 +882,35 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
     emstr_set_error(EMSTR_EUNSUPPORTED, -1, "batched form: at most %d layers", NZMAX);
     return NULL;
   }
+  if (d->nx >= 65536 - 64 || d->ny >= 65536 - 8) {
+    emstr_set_error(EMSTR_EUNSUPPORTED, -1, "batched form: grids up to 65472 x 65528");
+    return NULL;
+  }
   emstr_batch *b = (emstr_batch *)calloc(1, sizeof(emstr_batch));
   b->ntr = d->ntr;
-  b->nt64 = (d->ntr + 63) / 64;
   b->nz = d->nz;
   b->nx = d->nx;
   b->ny = d->ny;
-  b->plane = d->plane;
-  b->tile_x = 32;
-  b->tile_y = 32;
-  const size_t ncell = (size_t)d->plane * d->nz;
+  b->nxp = (d->nx + 63) / 64 * 64;
+  b->nyp = (d->ny + 31) / 32 * 32; /* tiles up to 4 x 8 rows */
+  b->P2 = (ll)b->nxp * b->nyp;
+  b->vol = (ll)d->nz * b->P2;
+  if (b->vol * 48 > 0x7fffffffLL || b->vol * 4 * d->ntr > 0xffffffffLL) {
+    /* buffer offsets of the coefficients and tracers are 32-bit */
+    emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                    "batched form: at most %lld cells per window and 4 GiB of tracers",
+                    0x7fffffffLL / 48);
+    free(b);
+    return NULL;
+  }
+  b->tx = 16;
+  b->strip = 64;
+  const size_t nt = (size_t)b->vol * d->ntr;
   bool ok = true;
-  for (int q = 0; q < 2; q++)
-    ok &= hipMalloc(&b->tr[q], ncell * d->ntr * sizeof(float)) == hipSuccess;
-  ok &= hipMalloc(&b->scr, ncell * d->ntr * sizeof(float)) == hipSuccess;
-  /* 8 layers of padding: the walk reads coefficients in 4-layer chunks */
-  ok &= hipMalloc(&b->co, (ncell + 8) * C_N * sizeof(float)) == hipSuccess;
-  ok &= hipMalloc(&b->ci, (size_t)d->plane * sizeof(ColInfo)) == hipSuccess;
-  ok &= hipMalloc(&b->order, ((size_t)d->plane + 8 * WPB + 8) * sizeof(int)) == hipSuccess;
+  for (int q = 0; q < 2; q++) ok &= hipMalloc(&b->tr[q], nt * sizeof(float)) == hipSuccess;
+  ok &= hipMalloc(&b->scr, nt * sizeof(float)) == hipSuccess;
+  ok &= hipMalloc(&b->co, (size_t)b->vol * 3 * sizeof(float4)) == hipSuccess;
+  ok &= hipMalloc(&b->ci, (size_t)b->P2 * sizeof(Col2)) == hipSuccess;
   ok &= hipMalloc(&b->mintr, (d->ntr + 1) * sizeof(float)) == hipSuccess;
   ok &= hipMalloc(&b->maxtr, (d->ntr + 1) * sizeof(float)) == hipSuccess;
   ok &= hipMalloc(&b->stepctr, sizeof(int)) == hipSuccess;
@@ -1085,32 +919,12 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
     emstr_batch_destroy(b);
     return NULL;
   }
-  hipMemset(b->co, 0, (ncell + 8) * C_N * sizeof(float));
+  /* land and below-bed cells are never written: zero coefficients give
+   * no transport through a face into land */
+  hipMemset(b->co, 0, (size_t)b->vol * 3 * sizeof(float4));
+  for (int q = 0; q < 2; q++) hipMemset(b->tr[q], 0, nt * sizeof(float));
   hipMemset(b->stepctr, 0, sizeof(int));
-  int dev = 0, ncu = 256;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  /* persistent grids: as many workgroups as can be resident at once */
-  int per_cu = 0;
-  const size_t lds = vert_lds(b->nz, b->ntr < 64 ? b->ntr : 64);
-  if (lds > 48 * 1024)
-    (void)hipFuncSetAttribute((const void *)k_vertB<VANLEER>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_vertB<VANLEER>,
-                                                   64 * WPB, lds) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  b->grid = ncu * per_cu;
-  per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_advB<VANLEER>, 256,
-                                                   0) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  b->gridA = (ncu * per_cu + 7) / 8 * 8;
-  b->grid_full = b->grid;
-  b->gridA_full = b->gridA;
-  build_order(b, h_kb, d->nz);
-  if (build_cells(b, h_kb, d->nz)) {
+  if (build_blocks(b, h_kb)) {
     emstr_batch_destroy(b);
     return NULL;
   }
@@ -1119,41 +933,38 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
 
 extern "C" void emstr_batch_destroy(struct emstr_batch *b) {
   if (!b) return;
-  if (b->gexec) hipGraphExecDestroy(b->gexec);
-  if (b->graph) hipGraphDestroy(b->graph);
-  void *v[] = {b->tr[0], b->tr[1], b->scr,     b->co,    b->ci,  b->order,
-               b->mintr, b->maxtr, b->stepctr, b->cells, b->kfl};
-  for (int q = 0; q < 11; q++)
+  drop_graph(b);
+  void *v[] = {b->tr[0], b->tr[1], b->scr, b->co, b->ci, b->mintr, b->maxtr, b->stepctr, b->blk};
+  for (unsigned q = 0; q < sizeof(v) / sizeof(v[0]); q++)
     if (v[q]) hipFree(v[q]);
   free(b);
 }
 
 extern "C" void emstr_batch_set_prof(struct emstr_batch *b, struct emstr_prof *p) { b->prof = p; }
 
+/* tile_x: tile width (16 columns, the DPP row); tile_y: strip width in
+ * columns of the XCD order; grid is not used by this kernel (the grid
+ * follows from the tiles) */
 extern "C" int emstr_batch_tune(struct emstr_batch *b, const int *h_kb, int tile_x, int tile_y,
                                 int grid) {
-  if (tile_x > 0) b->tile_x = tile_x;
-  if (tile_y > 0) b->tile_y = tile_y;
-  if (grid > 0) {
-    /* the advection grid as asked; the column grid in proportion */
-    b->gridA = (grid + 7) / 8 * 8;
-    long long g2 = (long long)b->grid_full * b->gridA / b->gridA_full;
-    b->grid = (int)(g2 < 1 ? 1 : (g2 > b->grid_full ? b->grid_full : g2));
+  (void)grid;
+  if (tile_x > 0) {
+    if (!shape_ok(tile_x))
+      return emstr_set_error(EMSTR_EARG, -1, "batched form: block width 16");
+    b->tx = tile_x;
   }
-  build_order(b, h_kb, b->nz);
-  if (build_cells(b, h_kb, b->nz)) return -1;
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec), b->gexec = NULL;
-  if (b->graph) (void)hipGraphDestroy(b->graph), b->graph = NULL;
-  b->graph_ok = 0;
+  if (tile_y > 0) b->strip = tile_y;
+  if (build_blocks(b, h_kb)) return -1;
+  drop_graph(b);
   return 0;
 }
 
 extern "C" void emstr_batch_shape(const struct emstr_batch *b, int *tile_x, int *tile_y,
                                   int *grid, int *grid_full) {
-  *tile_x = b->tile_x;
-  *tile_y = b->tile_y;
-  *grid = b->gridA;
-  *grid_full = b->gridA_full;
+  *tile_x = b->tx;
+  *tile_y = b->strip;
+  *grid = 8 * b->bmax;
+  *grid_full = 8 * b->bmax;
 }
 
 extern "C" int emstr_batch_fields(struct emstr_batch *b, const emstr_dev *d, void *stream) {
@@ -1172,56 +983,50 @@ extern "C" int emstr_batch_fields(struct emstr_batch *b, const emstr_dev *d, voi
   free(mn);
   free(mx);
   free(h);
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec), b->gexec = NULL;
-  if (b->graph) (void)hipGraphDestroy(b->graph), b->graph = NULL;
-  b->graph_ok = 0;
+  drop_graph(b);
   if (emstr_launch_subeta(d, d->dt, d->dt, 1, st)) return -1;
   return 0;
 }
 
-/* the per-step work: prep + fused kernel (captured in a graph) */
+static void launch_step(emstr_batch *b, const SArgs &sa, const float *in, float *out,
+                        const emstr_dev *d, hipStream_t st) {
+  const size_t lds = (size_t)SWAVES * (b->nz - b->kmin) * 64 * sizeof(float);
+  const int ng = (b->ntr + 3) / 4;
+  k_step2<<<(unsigned)(8 * b->bmax * ng), 64 * SWAVES, lds, st>>>(
+      sa, in, out, b->co, b->ci, b->blk, b->mintr, b->maxtr, d->layers, d->Kz, d->dz, b->scr,
+      b->stepctr);
+}
+
+/* the per-step work: prep + fused step (captured in a graph) */
 static int batch_launch(emstr_batch *b, const emstr_dev *d, hipStream_t st) {
   dim3 bp(64, 4, 1);
-  dim3 gp((d->nx + 63) / 64, (d->ny + 3) / 4, d->nz);
+  dim3 gp((b->nxp + 63) / 64, (b->nyp + 3) / 4, 1);
+  PArgs pa;
+  pa.nxp = b->nxp;
+  pa.nyp = b->nyp;
+  pa.P2 = b->P2;
   int pi = prof_begin(b->prof, KC_PREP, st);
-  k_prepB<<<gp, bp, 0, st>>>(*d, *b);
-  k_colB<<<dim3((d->nx + 63) / 64, (d->ny + 3) / 4, 1), bp, 0, st>>>(*d, *b);
+  k_prep2<<<gp, bp, 0, st>>>(*d, pa, b->co, b->ci);
+  prof_end(b->prof, pi, st);
   const float *in = b->tr[b->cur];
   float *out = b->tr[1 - b->cur];
-  if (b->nlist > 0)
-    k_kflB<<<(unsigned)((b->nlist + 255) / 256), 256, 0, st>>>(b->cells, b->nlist, b->ci,
-                                                                 b->kfl);
-  prof_end(b->prof, pi, st);
-  KArgs ka;
-  ka.nz = b->nz;
-  ka.ntr = b->ntr;
-  ka.nord = b->nord;
-  ka.nx = b->nx;
-  ka.hmin = d->hmin;
-  ka.dt = d->dt;
-  ka.plane = d->plane;
-  ka.pad = 0;
-  AArgs aa;
-  aa.nz = b->nz;
-  aa.ntr = b->ntr;
-  aa.niter = b->niter;
-  aa.pad = 0;
-  aa.nx = b->nx;
-  aa.hmin = d->hmin;
-  for (int g = 0; g < b->nt64; g++) {
-    ka.g = aa.g = g;
-    ka.nact = aa.nact = b->ntr - 64 * g < 64 ? b->ntr - 64 * g : 64;
-    pi = prof_begin(b->prof, KC_ADV, st);
-    k_advB<VANLEER><<<b->gridA, 256, 0, st>>>(aa, in, out, b->co, b->cells, b->kfl, b->ci,
-                                              b->mintr, d->layers, b->scr, b->stepctr);
-    prof_end(b->prof, pi, st);
-    pi = prof_begin(b->prof, KC_VDIFF, st);
-    k_vertB<VANLEER><<<b->grid, 64 * WPB, vert_lds(b->nz, ka.nact), st>>>(
-        ka, in, out, b->co, b->ci, b->order, b->mintr, b->maxtr, d->Kz, d->dz, b->scr,
-        b->stepctr);
+  SArgs sa;
+  sa.nz = b->nz;
+  sa.ntr = b->ntr;
+  sa.nx = b->nx;
+  sa.nxp = b->nxp;
+  sa.P2 = b->P2;
+  sa.vol = b->vol;
+  sa.hmin = d->hmin;
+  sa.dt = d->dt;
+  sa.P = d->plane;
+  sa.nzl = b->nz - b->kmin;
+  if (b->bmax > 0) {
+    pi = prof_begin(b->prof, KC_FUSED, st);
+    launch_step(b, sa, in, out, d, st);
     prof_end(b->prof, pi, st);
   }
-  k_tick<<<1, 1, 0, st>>>(b->stepctr);
+  k_tick2<<<1, 1, 0, st>>>(b->stepctr);
   b->cur = 1 - b->cur;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -1274,20 +1079,30 @@ extern "C" int emstr_batch_record2(struct emstr_batch *b, const emstr_dev *d, in
   return 0;
 }
 
+static LArgs largs(const emstr_batch *b, const emstr_dev *d) {
+  LArgs a;
+  a.plane = d->plane;
+  a.P2 = b->P2;
+  a.vol = b->vol;
+  a.nx = d->nx;
+  a.nxp = b->nxp;
+  a.nwet = d->nwet;
+  return a;
+}
+
 extern "C" int emstr_batch_put(struct emstr_batch *b, const emstr_dev *d, int n,
                                const double *sparse, void *stream) {
   if (d->nwet == 0) return 0;
-  k_putB<<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
-      d->wsp, d->wdn, d->nwet, d->plane, d->nz, b->ntr, n, sparse, b->tr[b->cur],
-      b->tr[1 - b->cur]);
+  k_put2<<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
+      largs(b, d), d->wsp, d->wdn, n, sparse, b->tr[b->cur], b->tr[1 - b->cur]);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 extern "C" int emstr_batch_get(struct emstr_batch *b, const emstr_dev *d, int n,
                                double *sparse, void *stream) {
   if (d->nwet == 0) return 0;
-  k_getB<<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
-      d->wsp, d->wdn, d->nwet, d->plane, d->nz, b->ntr, n, b->tr[b->cur], sparse);
+  k_get2<<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(largs(b, d), d->wsp, d->wdn,
+                                                                  n, b->tr[b->cur], sparse);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1295,6 +1110,7 @@ extern "C" int emstr_batch_mass(struct emstr_batch *b, const emstr_dev *d, doubl
                                 void *stream) {
   int nb = (d->ncol + 255) / 256;
   if (nb == 0) return 0;
-  k_massB<<<dim3(nb, d->ntr, 1), 256, 0, (hipStream_t)stream>>>(*d, *b, b->tr[b->cur], part);
+  k_mass2<<<dim3(nb, d->ntr, 1), 256, 0, (hipStream_t)stream>>>(*d, largs(b, d), b->nz,
+                                                                 b->tr[b->cur], part);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -31,25 +31,34 @@ def test_tuned_shape_keeps_results():
     p.close()
     tc = T.Context(case, T.BATCH_F32)
     tc.upload()
-    best, trials = autotune.tune_context(tc, tiles=[(32, 32), (8, 8), (16, 32)],
-                                         scales=(1.0, 0.5))
-    full = autotune.shape(tc)[3]
+    best, trials = autotune.tune_context(tc, blocks=[16], strips=[64, 16, 32, 256])
     tc.close()
     assert len(trials) == 4 and best in trials
-    assert all(t[3] > 0 for t in trials)
+    assert all(t[2] > 0 for t in trials)
     out = []
     snap = H.snapshot(case)
-    for shape in (None, best[:3], (8, 8, 8)):
+    for shape in (None, best[:2], (16, 16), (16, 256), (16, 32)):
         H.restore(case, snap)
         c = T.Context(case, T.BATCH_F32)
         c.upload()
         if shape:
-            c.tune(*shape)
-            tx, ty, g, gf = autotune.shape(c)
-            assert (tx, ty, g) == tuple(shape) and gf == full
+            c.tune(shape[0], shape[1], 0)
+            tx, sw, g, gf = autotune.shape(c)
+            assert (tx, sw) == tuple(shape) and g == gf and g % 8 == 0
         c.step(6)
         c.download()
         out.append(case.tracers())
         c.close()
-    assert np.array_equal(out[0], out[1])
-    assert np.array_equal(out[0], out[2])
+    for o in out[1:]:
+        assert np.array_equal(out[0], o)
+
+
+def test_bad_block_width_rejected():
+    case = configs.small(seed=6, nxi=20, nyi=16, nz=5, ntr=2)
+    L = _lib.lib()
+    L.set_dz(case.gp, case.wp, case.pp)
+    L.emstr_forget_window(case.gp)
+    c = T.Context(case, T.BATCH_F32)
+    with pytest.raises(T.EmstrError, match="block width"):
+        c.tune(12, 0, 0)
+    c.close()
