@@ -28,7 +28,7 @@ from . import _lib
 from . import tracer as T
 
 BLOCKS = [16]
-STRIPS = [32, 64, 128, 256]
+STRIPS = [256, 64, 128, 512, 1024]
 
 
 class _Events:

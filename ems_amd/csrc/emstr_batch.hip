@@ -1,24 +1,29 @@
 /*
  * Batched form of the EMS tracer step on MI355X (gfx950): every tracer of
- * the step over every water column in one fused pass, FP32 state with FP64
- * budgets.
+ * the step over every water column in one fused pass, FP32 arithmetic on a
+ * 32-bit-significand state, FP64 budgets.
  *
- * Layout (HBM): tracers [ntr][nz][nyp][nxp] FP32, the plane padded to
+ * Layout (HBM): tracers [ntr][nz][nyp][nxp] FP32 heads and, beside them,
+ * one 8-bit tail per value (the state's next eight significand bits), the
+ * tails of four tracers packed in one word [ntr/4][nz][nyp][nxp][4]; the
+ * plane padded to
  * nxp = 64-multiple and nyp = 32-multiple columns (the padding is land), so
  * that a wave's block of columns (TX x 64/TX, TX consecutive x columns per
  * row) reads whole, aligned row segments.  Per cell, three float4 words of
  * tracer-independent coefficients [nz][nyp][nxp][3]: A = (qW, fW, qS, fS)
  * the transports and van Leer factors of the west and south faces, B =
  * (qB, fB, 1/V, D) those of the bottom face, the inverse volume and the net
- * FP64 volume divergence, C = (gm, gp, ud, 1/div) the normalised column
+ * FP64 volume divergence, C = (gm, gp, 1/div, ud) the normalised column
  * matrix and its LU factors; the east, north and top faces are the west,
  * south and bottom faces of the neighbouring cells.  Per column a Col2
  * record (beds of the eight stencil neighbours, surfaces, diffusion
  * extent, flags, FP64 surface volume balance).
  *
  * Per step:
- *   k_prep2   one thread per column: coefficients from the FP64 fields and
- *             the column record (tracers.c:496-578, 3841-3889 restated);
+ *   k_cell2   one thread per cell: coefficients from the FP64 fields
+ *             (tracers.c:496-578, 3841-3889 restated);
+ *   k_col2    one thread per column: the column record and the LU factors
+ *             of the column matrix;
  *   k_step2   one wave per (block of 64 columns, tracer): every lane walks
  *             its column from the bed to the top with a five-layer register
  *             window of the tracer, the horizontal neighbours of the layer
@@ -39,10 +44,14 @@
  * The blocks are dealt to the 8 XCDs in work-balanced chunks of vertical
  * strips, and the tracers of one block run back to back on one XCD, so
  * its coefficients and the halo rows of the block before it are L2 hits.
- * Every FP32 state write is stochastically rounded from an FP64 value with
- * a deterministic hash of (cell, tracer, step), so sub-ulp increments do
- * not bias the state or the tracer mass, and results do not depend on the
- * launch shape.
+ * A cell's new value t0 + (advective increment) + (diffusive increment) is
+ * formed in FP64 and rounded to the 32-bit state once per step,
+ * stochastically, with a deterministic hash of (cell, tracer, step), so
+ * sub-ulp increments do not bias the state or the tracer mass, and results
+ * do not depend on the launch shape.  The vertical differences of the walk
+ * and of the column solve carry the tails (head differences plus tail
+ * differences); the horizontal stencil uses the heads, whose truncation is
+ * common to the cell and its neighbours.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -85,7 +94,8 @@ struct emstr_batch {
   int ntr, nz, nx, ny, nxp, nyp;
   ll P2;           /* nxp * nyp                                          */
   ll vol;          /* nz * P2: one tracer                                */
-  float *tr[2];    /* [ntr][nz][nyp][nxp]                                */
+  float *tr[2];    /* [ntr][nz][nyp][nxp], the 24-bit heads              */
+  uint8_t *rm[2];  /* [ntr/4][nz][nyp][nxp][4], the 8-bit tails          */
   int cur;
   float4 *co;      /* [nz][nyp][nxp][3]                                  */
   Col2 *ci;        /* [nyp][nxp]                                         */
@@ -123,8 +133,52 @@ struct PArgs {
   ll P2;
 };
 
-__global__ void __launch_bounds__(256) k_prep2(emstr_dev d, PArgs a, float4 *__restrict__ co,
-                                               Col2 *__restrict__ ci) {
+/* per-cell coefficients, one thread per cell of the padded grid (cells
+ * below the bed and land keep their zero words) */
+__global__ void __launch_bounds__(256) k_cell2(emstr_dev d, PArgs a, float4 *__restrict__ co) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (k < kbc) return;
+  const ll P = d.plane, nx = d.nx;
+  const ll idx = (ll)k * P + col;
+  const double dt = d.dt, area = d.cellarea[col];
+  const int ksc = d.ks[col], cs = d.knew[col];
+  const double dz = d.dz[idx];
+  const double dzb = (k > kbc) ? d.dz[idx - P] : 0.0;
+  const bool we = k >= d.kb[col + 1], wn = k >= d.kb[col + nx];
+  const bool tp = (ksc >= 0 && k < ksc);
+  const double qW = d.u1flux3d[idx] * dt, qS = d.u2flux3d[idx] * dt;
+  const double qE = we ? d.u1flux3d[idx + 1] * dt : 0.0;
+  const double qN = wn ? d.u2flux3d[idx + nx] * dt : 0.0;
+  const double qB = d.w[idx] * dt * area;
+  /* the top face's transport is the bottom face's of the cell above */
+  const double qT = tp ? d.w[idx + P] * dt * area : 0.0;
+  const double cW = d.u1[idx] * dt / d.h1au1[col];
+  const double cS = d.u2[idx] * dt / d.h2au2[col];
+  const double cB = d.w[idx] * dt / (0.5 * (dz + dzb));
+  float gm = 0.0f, gp = 0.0f;
+  if (ksc >= 0 && k <= cs && cs - 1 >= kbc) {
+    /* rows of the implicit vertical diffusion (tracers.c:3841-3889),
+     * normalised by dzdt */
+    const double dzdt = dz / dt;
+    if (k > kbc) gm = (float)(-d.Kz[idx] / (0.5 * (dzb + dz)) / dzdt);
+    if (k < cs) gp = (float)(-d.Kz[idx + P] / (0.5 * (dz + d.dz[idx + P])) / dzdt);
+  }
+  float4 *o = co + ((ll)k * a.P2 + (ll)j * a.nxp + i) * 3;
+  o[0] = make_float4((float)qW, vlfac(qW, cW), (float)qS, vlfac(qS, cS));
+  o[1] = make_float4((float)qB, vlfac(qB, cB), (dz != 0.0) ? (float)(1.0 / (area * dz)) : 0.0f,
+                     (float)(qE - qW + qN - qS + qT - qB));
+  o[2] = make_float4(gm, gp, 1.0f, 0.0f);
+}
+
+/* the column record and the LU factors of the column matrix, one thread
+ * per column of the padded plane (after k_cell2) */
+__global__ void __launch_bounds__(256) k_col2(emstr_dev d, PArgs a, float4 *__restrict__ co,
+                                              Col2 *__restrict__ ci) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= a.nxp || j >= a.nyp) return;
@@ -144,11 +198,6 @@ __global__ void __launch_bounds__(256) k_prep2(emstr_dev d, PArgs a, float4 *__r
   const ll P = d.plane, nx = d.nx;
   const int top = d.nz - 1;
   const double dt = d.dt, area = d.cellarea[col];
-  /* the fields through restrict pointers: the coefficient stores do not
-   * alias them, so the loads of several layers can be in flight */
-  const double *__restrict__ U1F = d.u1flux3d, *__restrict__ U2F = d.u2flux3d;
-  const double *__restrict__ W = d.w, *__restrict__ U1 = d.u1, *__restrict__ U2 = d.u2;
-  const double *__restrict__ DZ = d.dz, *__restrict__ KZ = d.Kz;
   c.kb = kbc;
   /* neighbour beds; a second neighbour counts only if the first is wet */
   const int w1 = d.kb[col - 1], e1 = d.kb[col + 1], s1 = d.kb[col - nx], n1 = d.kb[col + nx];
@@ -165,38 +214,6 @@ __global__ void __launch_bounds__(256) k_prep2(emstr_dev d, PArgs a, float4 *__r
   c.area = area;
   c.botz = d.botz[col];
   c.osub = d.osubeta[col];
-  /* per-cell coefficients, bed to top */
-  double dzb = 0.0;
-#pragma unroll 4
-  for (int k = kbc; k <= top; k++) {
-    const ll idx = (ll)k * P + col;
-    const double dz = DZ[idx];
-    const bool we = k >= e1, wn = k >= n1;
-    const bool tp = (ksc >= 0 && k < ksc);
-    const double qW = U1F[idx] * dt, qS = U2F[idx] * dt;
-    const double qE = we ? U1F[idx + 1] * dt : 0.0;
-    const double qN = wn ? U2F[idx + nx] * dt : 0.0;
-    const double qB = W[idx] * dt * area;
-    /* the top face's transport is the bottom face's of the cell above */
-    const double qT = tp ? W[idx + P] * dt * area : 0.0;
-    const double cW = U1[idx] * dt / d.h1au1[col];
-    const double cS = U2[idx] * dt / d.h2au2[col];
-    const double cB = W[idx] * dt / (0.5 * (dz + dzb));
-    float gm = 0.0f, gp = 0.0f;
-    if (ksc >= 0 && k <= cs && cs - 1 >= kbc) {
-      /* rows of the implicit vertical diffusion (tracers.c:3841-3889),
-       * normalised by dzdt */
-      const double dzdt = dz / dt;
-      if (k > kbc) gm = (float)(-KZ[idx] / (0.5 * (dzb + dz)) / dzdt);
-      if (k < cs) gp = (float)(-KZ[idx + P] / (0.5 * (dz + DZ[idx + P])) / dzdt);
-    }
-    float4 *o = co + ((ll)k * a.P2 + c2) * 3;
-    o[0] = make_float4((float)qW, vlfac(qW, cW), (float)qS, vlfac(qS, cS));
-    o[1] = make_float4((float)qB, vlfac(qB, cB), (dz != 0.0) ? (float)(1.0 / (area * dz)) : 0.0f,
-                       (float)(qE - qW + qN - qS + qT - qB));
-    o[2] = make_float4(gm, gp, 0.0f, 1.0f);
-    dzb = dz;
-  }
   int flags = 0;
   if (ksc >= 0) {
     flags |= F_PROC;
@@ -238,35 +255,48 @@ __global__ void __launch_bounds__(256) k_prep2(emstr_dev d, PArgs a, float4 *__r
       flags |= F_SINGLE;
       c.cb = cs;
     } else {
-      /* one above the highest zero-thickness layer at or below cs */
+      /* one above the highest zero-thickness layer at or below cs; the
+       * loads of the scan are independent of each other */
+      const double *__restrict__ DZ = d.dz, *__restrict__ KZ = d.Kz;
       int zq = kbc - 1;
+#pragma unroll 8
       for (int k = kbc; k <= cs; k++)
-        if (d.dz[(ll)k * P + col] == 0.0) zq = k;
+        if (DZ[(ll)k * P + col] == 0.0) zq = k;
       c.cb = zq + 1;
       if (c.cb >= cs) flags |= F_SINGLE;
+#pragma unroll 8
       for (int k = c.cb + 1; k <= cs; k++) {
-        const double dzc = d.dz[(ll)k * P + col];
-        const double dzm = d.dz[(ll)(k - 1) * P + col];
-        if (d.Kz[(ll)k * P + col] * dt / (dzc * 0.5 * (dzm + dzc)) > IR_DMAX) flags |= F_STIFF;
+        const double dzc = DZ[(ll)k * P + col];
+        const double dzm = DZ[(ll)(k - 1) * P + col];
+        if (KZ[(ll)k * P + col] * dt / (dzc * 0.5 * (dzm + dzc)) > IR_DMAX) flags |= F_STIFF;
       }
       /* LU factors of the normalised matrix (rows gm, 1 - gm - gp, gp):
-       * ud_k = gp_{k-1} / div_{k-1}, div_k = 1 - gm - gp - gm ud_k */
+       * ud_k = gp_{k-1} / div_{k-1}, div_k = 1 - gm - gp - gm ud_k; eight
+       * rows loaded before their factors are stored */
       if (!(flags & F_SINGLE)) {
         float divp = 1.0f, gpp = 0.0f;
-        for (int k = c.cb; k <= cs; k++) {
-          float4 *o = co + ((ll)k * a.P2 + c2) * 3 + 2;
-          float4 m = *o;
-          const float gm = (k > c.cb) ? m.x : 0.0f, gp = m.y;
-          float ud = 0.0f, dv = 1.0f - gm - gp;
-          if (k > c.cb) {
-            ud = gpp / divp;
-            dv = dv - gm * ud;
+        float4 *oc = co + c2 * 3 + 2;
+        for (int k0 = c.cb; k0 <= cs; k0 += 8) {
+          float4 m[8];
+#pragma unroll
+          for (int q = 0; q < 8; q++) m[q] = oc[(ll)min(k0 + q, cs) * a.P2 * 3];
+#pragma unroll
+          for (int q = 0; q < 8; q++) {
+            const int k = k0 + q;
+            if (k <= cs) {
+              const float gm = (k > c.cb) ? m[q].x : 0.0f, gp = m[q].y;
+              float ud = 0.0f, dv = 1.0f - gm - gp;
+              if (k > c.cb) {
+                ud = gpp / divp;
+                dv = dv - gm * ud;
+              }
+              m[q].z = 1.0f / dv;
+              m[q].w = ud;
+              oc[(ll)k * a.P2 * 3] = m[q];
+              divp = dv;
+              gpp = gp;
+            }
           }
-          m.z = ud;
-          m.w = 1.0f / dv;
-          *o = m;
-          divp = dv;
-          gpp = gp;
         }
       }
     }
@@ -295,6 +325,21 @@ __device__ __forceinline__ float vla(float bm2, float bm1, float c, float fp1, f
   return __builtin_fmaf(f, (df > 0.0f) ? t : 0.0f, p1 - ref);
 }
 
+/* The same with the tails of the values (e*, of the four stencil values
+ * and of ref): every difference is the heads' difference plus the
+ * tails', so that it keeps the state's 32 bits. */
+__device__ __forceinline__ float vlad(float bm2, float bm1, float c, float fp1, float ebm2,
+                                      float ebm1, float ec, float efp1, float q, float f,
+                                      float ref, float eref) {
+  const bool pos = q > 0.0f;
+  const float p2 = pos ? bm2 : fp1, p1 = pos ? bm1 : c, p0 = pos ? c : bm1;
+  const float e2 = pos ? ebm2 : efp1, e1 = pos ? ebm1 : ec, e0 = pos ? ec : ebm1;
+  const float d12 = (p1 - p2) + (e1 - e2), d01 = (p0 - p1) + (e0 - e1);
+  const float df = d12 * d01;
+  const float t = df * __builtin_amdgcn_rcpf(d01 + d12);
+  return __builtin_fmaf(f, (df > 0.0f) ? t : 0.0f, (p1 - ref) + (e1 - eref));
+}
+
 /*-------------------------------------------------------------------*/
 /* deterministic stochastic rounding                                 */
 /*-------------------------------------------------------------------*/
@@ -317,15 +362,34 @@ __device__ __forceinline__ uint32_t srkey(uint32_t cellkey, uint32_t hl) {
   return x;
 }
 
-/* Round the FP64 value x to one of its two neighbouring FP32 values, the
- * upper one with probability equal to the remainder's fraction of an FP32
- * ulp: a uniform 29-bit number is added to the 29 mantissa bits that FP32
- * drops, which are then cleared. */
-__device__ __forceinline__ float sround(double x, uint32_t r) {
+/* Compensated state: a value is held as an FP32 head h, the value
+ * truncated to 24 significant bits, and an 8-bit tail r, its next eight
+ * bits, so that the state keeps 32 significant bits:
+ *   x = h + sign(h) r 2^(e - 31),  e the exponent of h.
+ * enc rounds the FP64 value x to 32 bits stochastically (a uniform 21-bit
+ * number added to the 21 bits dropped, which are then cleared) and splits
+ * it; values below the FP32 normal range keep r = 0. */
+__device__ __forceinline__ void enc(double x, uint32_t rnd, float &h, uint32_t &r) {
   unsigned long long u = (unsigned long long)__double_as_longlong(x);
-  u += (unsigned long long)(r >> 3);
-  u &= ~0x1FFFFFFFULL;
-  return (float)__longlong_as_double((long long)u);
+  u += (unsigned long long)(rnd >> 11);
+  u &= ~0x1FFFFFULL;
+  const bool tiny = fabs(x) < 1.1754943508222875e-38;
+  h = tiny ? (float)x : (float)__longlong_as_double((long long)(u & ~0x1FFFFFFFULL));
+  r = tiny ? 0u : (uint32_t)(u >> 21) & 0xFFu;
+}
+/* the stored value in FP64 (exact) */
+__device__ __forceinline__ double dec(float h, uint32_t r) {
+  return __longlong_as_double(__double_as_longlong((double)h) | ((long long)r << 21));
+}
+/* the tail as an FP32 number (exact; zero below 2^-96) */
+__device__ __forceinline__ float tail(float h, uint32_t r) {
+  const uint32_t hb = __float_as_uint(h);
+  const float sc = __uint_as_float((hb & 0xff800000u) - (31u << 23));
+  return ((hb & 0x7f800000u) > (31u << 23)) ? (float)r * sc : 0.0f;
+}
+/* the tail byte of h + t (t = tail(h, r)) */
+__device__ __forceinline__ uint32_t tailbyte(float h, float t) {
+  return (uint32_t)(__double_as_longlong((double)h + (double)t) >> 21) & 0xFFu;
 }
 
 /*-------------------------------------------------------------------*/
@@ -345,32 +409,43 @@ struct SArgs {
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
 }
-__device__ __forceinline__ float bl1(__amdgpu_buffer_rsrc_t r, uint32_t o) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0));
+/* o: the lane's byte offset; so: a wave-uniform byte offset (a scalar
+ * register), added to it (the range check covers o only: the buffers
+ * carry slack beyond their last element for the reads past it) */
+__device__ __forceinline__ float bl1(__amdgpu_buffer_rsrc_t r, uint32_t o, uint32_t so = 0) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0));
 }
-__device__ __forceinline__ float2 bl2(__amdgpu_buffer_rsrc_t r, uint32_t o) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, o, 0, 0);
+__device__ __forceinline__ float2 bl2(__amdgpu_buffer_rsrc_t r, uint32_t o, uint32_t so = 0) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
   return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
 }
-__device__ __forceinline__ float4 bl4(__amdgpu_buffer_rsrc_t r, uint32_t o) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0);
+__device__ __forceinline__ float3 bl3(__amdgpu_buffer_rsrc_t r, uint32_t o, uint32_t so = 0) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, o, so, 0);
+  return make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+}
+__device__ __forceinline__ float4 bl4(__amdgpu_buffer_rsrc_t r, uint32_t o, uint32_t so = 0) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, o, so, 0);
   return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
                      __uint_as_float(v[3]));
 }
-
 /* DPP moves inside rows of 16 lanes: lane l takes lane l - 1, l - 2
- * (shr) or l + 1, l + 2 (shl); a lane without a source keeps 0 */
-__device__ __forceinline__ float shr1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+ * (shr) or l + 1, l + 2 (shl); a lane whose source is outside its row
+ * takes old */
+__device__ __forceinline__ float shr1o(float old, float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x111, 0xf, 0xf, false));
 }
-__device__ __forceinline__ float shr2(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xf, 0xf, false));
+__device__ __forceinline__ float shr2o(float old, float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x112, 0xf, 0xf, false));
 }
-__device__ __forceinline__ float shl1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xf, 0xf, false));
+__device__ __forceinline__ float shl1o(float old, float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x101, 0xf, 0xf, false));
 }
-__device__ __forceinline__ float shl2(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x102, 0xf, 0xf, false));
+__device__ __forceinline__ float shl2o(float old, float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x102, 0xf, 0xf, false));
 }
 
 /* the loads of one layer: the column at layer k + 2 (the window's new
@@ -380,8 +455,10 @@ __device__ __forceinline__ float shl2(float v) {
  * above (this cell's top face) and, for the segment's last lane, the east
  * neighbour's west transport */
 struct Lay {
-  float z2, s1, s2, n1, n2, xw1, xw2, xe1, xe2;
-  float4 A, U, M;
+  float z2, s1, s2, n1, n2, x1, x2;
+  uint32_t z2r; /* the tail of z2 */
+  float4 A, U;
+  float3 M; /* gm, gp, 1/div: the walk's part of the matrix word */
   float2 N, E;
 };
 
@@ -393,8 +470,9 @@ struct Lay {
  * deepest bed to the top; the loads of layer k + 1 are issued before
  * layer k is computed; the LDS slice holds the advected values b of the
  * lane's column, the scratch buffer its forward-sweep values. */
-__global__ void __launch_bounds__(64 * SWAVES)
+__global__ void __launch_bounds__(64 * SWAVES) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
+        const uint8_t *__restrict__ rin, uint8_t *__restrict__ rout,
         const float4 *__restrict__ co, const Col2 *__restrict__ cinfo,
         const int *__restrict__ blk, const float *__restrict__ mintr,
         const float *__restrict__ maxtr, const double *__restrict__ layers,
@@ -429,12 +507,21 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
   const uint32_t cellkey0 = (uint32_t)(((ll)j * a.nx + i) * nz);
   const __amdgpu_buffer_rsrc_t RT = rsrc(tin, (uint32_t)(a.vol * 4 * a.ntr));
   const __amdgpu_buffer_rsrc_t RC = rsrc(co, (uint32_t)(a.vol * 48));
+  const __amdgpu_buffer_rsrc_t RR = rsrc(rin, (uint32_t)(a.vol * 4 * NG));
   float *OB = tout + (ll)n * a.vol;
+  /* the tails of the lane's tracer: byte 4 (k P2 + col) of the group's words */
+  uint8_t *OR = rout + (ll)g * a.vol * 4 + (lane >> 4);
   float *SB = scr + (ll)n * a.vol;
   const float4 *__restrict__ CC = co + col * 3 + 2; /* the column's matrix words */
+  const float *__restrict__ CU = (const float *)CC + 3; /* their ud entries */
   /* byte offsets: of the lane's column in its tracer, of a layer, of the
    * column's coefficients in a layer, of a coefficient layer */
   const uint32_t c4 = (uint32_t)(n * a.vol * 4) + col * 4u, L4 = P2 * 4u;
+  const uint32_t cr = (uint32_t)(g * a.vol * 4) + col * 4u + (uint32_t)(lane >> 4);
+  /* the tail of the lane's cell at layer k, as a byte */
+  auto rb = [&](int k) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(RR, (uint32_t)k * L4 + cr, 0, 0);
+  };
   const uint32_t c48 = col * 48u, L48 = P2 * 48u;
   const int nw1 = ci.nbw & 255, nw2 = (ci.nbw >> 8) & 255, ne1 = (ci.nbw >> 16) & 255,
             ne2 = (ci.nbw >> 24) & 255;
@@ -442,128 +529,157 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
             nn2 = (ci.nbs >> 24) & 255;
   const bool proc = (flags & F_PROC) != 0;
   const bool move = (flags & F_MOVE) != 0;
-  /* the lane reads layers kb..ks (and the layers above for a moving
-   * surface) */
-  const int kend = !proc ? -1 : (move ? top : ks);
   float *L = lds + wv * a.nzl * 64 + lane;
   const uint32_t ry = nxp * 4u;
   auto clampk = [&](int k) { return (uint32_t)min(max(k, kb), top); };
-  /* Issue the loads of layer k (k in kb..kend for this lane; another lane
-   * keeps its registers).  A neighbour that is land or below the bed at
-   * layer k is read as the cell the map walk came from
-   * (tracers.c:5574-5593). */
+  /* the lane's offsets of its y neighbours and of the x neighbours beyond
+   * its row segment (x1: lane 0 its west, lane 1 its second west, lane 14
+   * its second east, lane 15 its east neighbour; x2: lanes 0 and 15 their
+   * second neighbours; other lanes their own cell) */
+  const uint32_t os1 = c4 - ry, os2 = c4 - 2 * ry, on1 = c4 + ry, on2 = c4 + 2 * ry;
+  const uint32_t ox1 = (cl == 0) ? c4 - 4 : (cl == 1) ? c4 - 8 : (cl == 14) ? c4 + 8
+                     : (cl == 15) ? c4 + 4 : c4;
+  const uint32_t ox2 = (cl == 0) ? c4 - 8 : (cl == 15) ? c4 + 8 : c4;
+  const uint32_t oN = c48 + ry * 12u + 8u;
+  /* Issue the loads of layer k: every lane the same words at its own
+   * offsets, the layer's offset uniform (below a lane's bed, above its
+   * surface or at a neighbour that is land the words are read and not
+   * used; a neighbour that is land or below the bed at layer k is read as
+   * the cell the map walk came from, tracers.c:5574-5593, when the layer
+   * is computed). */
   auto issue = [&](int k, Lay &y) {
-    y.z2 = bl1(RT, clampk(k + 2) * L4 + c4);
-    if (k < kb) return;
-    /* the west word of every wet cell: the lane to the west takes its
-     * transport as its east face's */
-    const uint32_t oc = (uint32_t)k * L48 + c48;
-    y.A = bl4(RC, oc);
-    if (k > kend) return;
-    const uint32_t o = (uint32_t)k * L4 + c4;
-    const bool bw1 = k >= nw1, be1 = k >= ne1, bs1 = k >= ns1, bn1 = k >= nn1;
-    const uint32_t ds1 = bs1 ? ry : 0u, dn1 = bn1 ? ry : 0u;
-    const uint32_t ds2 = (bs1 && k >= ns2) ? 2 * ry : ds1, dn2 = (bn1 && k >= nn2) ? 2 * ry : dn1;
-    y.s1 = bl1(RT, o - ds1);
-    y.s2 = bl1(RT, o - ds2);
-    y.n1 = bl1(RT, o + dn1);
-    y.n2 = bl1(RT, o + dn2);
-    /* x neighbours outside the row segment */
-    if (cl < 2) {
-      const uint32_t dw1 = bw1 ? 4u : 0u, dw2 = (bw1 && k >= nw2) ? 8u : dw1;
-      y.xw2 = bl1(RT, o - dw2);
-      if (cl == 0) y.xw1 = bl1(RT, o - dw1);
-    } else if (cl > 13) {
-      const uint32_t de1 = be1 ? 4u : 0u, de2 = (be1 && k >= ne2) ? 8u : de1;
-      y.xe2 = bl1(RT, o + de2);
-      if (cl == 15) y.xe1 = bl1(RT, o + de1);
-    }
-    y.U = bl4(RC, (k < top ? oc + L48 : oc) + 16u);
-    y.M = bl4(RC, oc + 32u);
-    const float2 nn = bl2(RC, oc + (bn1 ? ry * 12u : 0u) + 8u);
-    y.N = make_float2(bn1 ? nn.x : 0.0f, nn.y);
-    if (cl == 15) y.E = bl2(RC, oc + (be1 ? 48u : 0u));
+    const uint32_t kz = (uint32_t)min(k + 2, top) * L4;
+    y.z2 = bl1(RT, c4, kz);
+    y.z2r = (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(RR, cr, kz, 0);
+    const uint32_t kt = (uint32_t)k * L4, kc = (uint32_t)k * L48;
+    y.A = bl4(RC, c48, kc);
+    y.s1 = bl1(RT, os1, kt);
+    y.s2 = bl1(RT, os2, kt);
+    y.n1 = bl1(RT, on1, kt);
+    y.n2 = bl1(RT, on2, kt);
+    y.x1 = bl1(RT, ox1, kt);
+    y.x2 = bl1(RT, ox2, kt);
+    y.U = bl4(RC, c48 + 16u, k < top ? kc + L48 : kc);
+    y.M = bl3(RC, c48 + 32u, kc);
+    y.N = bl2(RC, oN, kc);
+    y.E = bl2(RC, c48 + 48u, kc);
   };
   /* register window of the column: layers k-2 .. k+2, clamped to the bed
    * (the sediment ghost holds the bed value) and to the top */
   float zm2 = bl1(RT, clampk(kmin - 2) * L4 + c4), zm1 = bl1(RT, clampk(kmin - 1) * L4 + c4);
   float t0 = bl1(RT, clampk(kmin) * L4 + c4), zp1 = bl1(RT, clampk(kmin + 1) * L4 + c4);
   float zp2 = 0.0f;
+  /* and their tails (the vertical differences carry them) */
+  float dm2 = tail(zm2, rb((int)clampk(kmin - 2))), dm1 = tail(zm1, rb((int)clampk(kmin - 1)));
+  float d00 = tail(t0, rb((int)clampk(kmin))), dp1 = tail(zp1, rb((int)clampk(kmin + 1)));
+  float dp2 = 0.0f;
+  /* the bed cell (the window's value below the bed) */
+  const float tbed = bl1(RT, (uint32_t)min(kb, top) * L4 + c4);
+  const uint32_t rbed = rb(min(kb, top));
   float4 Bk = bl4(RC, clampk(kmin) * L48 + c48 + 16u); /* this layer's bottom face */
   double m2 = 0.0;    /* surface cell: anomaly terms of the volume balance */
   float v0s = 0.0f;   /* surface cell: its old value                       */
-  float bunc = 0.0f;  /* unclipped advected value below the surface        */
+  float v0d = 0.0f;   /* and its tail                                      */
+  double bunc = 0.0;  /* unclipped advected value below the surface        */
   float aT = 0.0f;    /* top-face anomaly of the layer below, against it   */
   const int kold = ci.kold;
   /* the column solve's forward sweep runs in the walk for the rows whose
    * right-hand side lies below the surface: row k - 1 at layer k */
   const bool solve = proc && !(flags & F_SINGLE);
   const int cb = ci.cb, cs = ci.cs;
-  float bb1 = 0.0f, bb2 = 0.0f; /* b of the two layers below            */
+  float dd1 = 0.0f, dd2 = 0.0f; /* advective increments of the two layers below */
+  double bks1 = 0.0, bks2 = 0.0; /* b of the two layers below the surface      */
   float sp = 0.0f;              /* s of the last forward row            */
-  float4 Mp = make_float4(0.0f, 0.0f, 0.0f, 1.0f); /* matrix row below  */
+  float3 Mp = make_float3(0.0f, 0.0f, 1.0f); /* matrix row below      */
   float *Ls = L;                /* s_k of the lane's column in its LDS slice */
-  Lay ya, yb;
+  Lay ya = {}, yb = {};
   issue(kmin, ya);
   auto layer = [&](int k, const Lay &y) {
     const bool has = k >= kb;
-    zp2 = y.z2;
-    /* x neighbours at layer k from the lanes beside this one */
-    const float l1 = shr1(t0), l2 = shr2(t0), r1 = shl1(t0), r2 = shl2(t0);
-    const float qEl = shl1(y.A.x), fEl = shl1(y.A.y);
+    {
+      const bool bed = k + 2 < kb;
+      zp2 = bed ? tbed : y.z2;
+      dp2 = tail(zp2, bed ? rbed : y.z2r);
+    }
+    /* x neighbours at layer k from the lanes beside this one, at the
+     * segment's ends from the loads beyond it */
+    const float l1 = shr1o(y.x1, t0), l2 = shr2o((cl == 0) ? y.x2 : y.x1, t0);
+    const float r1 = shl1o(y.x1, t0), r2 = shl2o((cl == 15) ? y.x2 : y.x1, t0);
+    /* the east face's transport and factor: the west word of the lane to
+     * the east, or of the east neighbour's word at the segment's end */
+    const float qEl = shl1o(y.E.x, y.A.x), fEl = shl1o(y.E.y, y.A.y);
     const bool sub = proc && has && k < ks;
     const bool srf = proc && k == ks;
     const bool upp = proc && move && k > ks;
-    if (!proc && has && live) OB[(uint32_t)k * P2 + col] = t0;
+    if (!proc && has && live) {
+      OB[(uint32_t)k * P2 + col] = t0;
+      OR[((uint32_t)k * P2 + col) * 4u] = (uint8_t)tailbyte(t0, d00);
+    }
     if (sub || srf || upp) {
       const bool bw1 = k >= nw1, be1 = k >= ne1;
       const bool bw2 = bw1 && k >= nw2, be2 = be1 && k >= ne2;
-      const float w1 = (cl == 0) ? y.xw1 : (bw1 ? l1 : t0);
-      const float w2 = (cl < 2) ? y.xw2 : (bw2 ? l2 : w1);
-      const float e1 = (cl == 15) ? y.xe1 : (be1 ? r1 : t0);
-      const float e2 = (cl > 13) ? y.xe2 : (be2 ? r2 : e1);
-      const float qE = be1 ? ((cl == 15) ? y.E.x : qEl) : 0.0f;
-      const float fE = (cl == 15) ? y.E.y : fEl;
+      const bool bs1 = k >= ns1, bn1 = k >= nn1;
+      const bool bs2 = bs1 && k >= ns2, bn2 = bn1 && k >= nn2;
+      const float w1 = bw1 ? l1 : t0, w2 = bw2 ? l2 : w1;
+      const float e1 = be1 ? r1 : t0, e2 = be2 ? r2 : e1;
+      const float s1 = bs1 ? y.s1 : t0, s2 = bs2 ? y.s2 : s1;
+      const float n1 = bn1 ? y.n1 : t0, n2 = bn2 ? y.n2 : n1;
+      const float qE = be1 ? qEl : 0.0f, qN = bn1 ? y.N.x : 0.0f;
+      /* (a neighbour that is land at layer k has a zero word: its factor
+       * must stay finite, the transport is zero) */
+      const float fE = be1 ? fEl : 1.0f, fN = bn1 ? y.N.y : 1.0f;
       const float ref = upp ? v0s : t0;
       const float AW = vla(w2, w1, t0, e1, y.A.x, y.A.y, ref);
       const float AE = vla(w1, t0, e1, e2, qE, fE, ref);
-      const float AS = vla(y.s2, y.s1, t0, y.n1, y.A.z, y.A.w, ref);
-      const float AN = vla(y.s1, t0, y.n1, y.n2, y.N.x, y.N.y, ref);
-      const float hh = AE * qE - AW * y.A.x + AN * y.N.x - AS * y.A.z;
+      const float AS = vla(s2, s1, t0, n1, y.A.z, y.A.w, ref);
+      const float AN = vla(s1, t0, n1, n2, qN, fN, ref);
+      const float hh = AE * qE - AW * y.A.x + AN * qN - AS * y.A.z;
       if (upp) {
         /* a layer above the processed surface cell (tracers.c:3117-3193) */
         m2 -= (double)hh;
+        const double an = ((double)t0 + (double)d00) - ((double)v0s + (double)v0d);
         if (k < kold)
-          m2 += ((double)t0 - (double)v0s) * (ci.area * (layers[k + 1] - layers[k]));
+          m2 += an * (ci.area * (layers[k + 1] - layers[k]));
         else if (k == kold)
-          m2 += ((double)t0 - (double)v0s) * (ci.area * (ci.osub - layers[k]));
+          m2 += an * (ci.area * (ci.osub - layers[k]));
       } else {
         /* the bottom face is the top face of the layer below, against
          * this cell's value; zero at the bed (the ghost holds the bed
          * value) */
-        const float fba = (k > kb) ? (aT + (zm1 - t0)) * Bk.x : 0.0f;
+        const float fba = (k > kb) ? (aT + ((zm1 - t0) + (dm1 - d00))) * Bk.x : 0.0f;
         if (sub) {
-          aT = vla(zm1, t0, zp1, zp2, y.U.x, y.U.y, t0);
-          const double an = (double)t0 - (double)((hh + aT * y.U.x - fba + t0 * Bk.w) * Bk.z);
-          float bk = sround(an, srkey(cellkey0 + (uint32_t)k, hl));
-          if (k == ks - 1) bunc = bk;
-          if (clip && bk < mn) bk = mn;
-          if (live) OB[(uint32_t)k * P2 + col] = bk;
+          aT = vlad(zm1, t0, zp1, zp2, dm1, d00, dp1, dp2, y.U.x, y.U.y, t0, d00);
+          /* the advected value b = t0 + dlt: its increment is kept in
+           * FP32 (the output column holds it until the column's values
+           * are formed and rounded once, after the solve) */
+          float dlt = -((hh + aT * y.U.x - fba + (t0 + d00) * Bk.w) * Bk.z);
+          const double T = (double)t0 + (double)d00;
+          const double bf = T + (double)dlt;
+          if (k == ks - 1) bunc = bf;
+          if (clip && bf < (double)mn) dlt = (float)((double)mn - T);
+          if (live) OB[(uint32_t)k * P2 + col] = dlt;
           const int r = k - 1;
           if (solve && r >= cb) {
             /* r_k = -(gm (b_{k-1} - b_k) + gp (b_{k+1} - b_k)),
-             * s_k = (r_k - gm s_{k-1}) / div_k */
+             * s_k = (r_k - gm s_{k-1}) / div_k, the differences of b
+             * from those of t0 and of the increments */
             const float gm = (r > cb) ? Mp.x : 0.0f;
-            const float bm = (r > cb) ? bb2 : bb1;
-            const float rr = -(gm * (bm - bb1) + Mp.y * (bk - bb1));
-            sp = (rr - gm * sp) * Mp.w;
+            const float dm = (r > cb) ? ((zm2 - zm1) + (dm2 - dm1)) + (dd2 - dd1) : 0.0f;
+            const float dp = ((t0 - zm1) + (d00 - dm1)) + (dlt - dd1);
+            const float rr = -(gm * dm + Mp.y * dp);
+            sp = (rr - gm * sp) * Mp.z;
             Ls[(r - kmin) * 64] = sp;
           }
-          bb2 = bb1;
-          bb1 = bk;
+          if (k == ks - 1) {
+            bks1 = T + (double)dlt;
+            bks2 = ((double)zm1 + (double)dm1) + (double)dd1;
+          }
+          dd2 = dd1;
+          dd1 = dlt;
         } else {
           m2 = (double)fba - (double)hh;
           v0s = t0;
+          v0d = d00;
         }
       }
     }
@@ -579,6 +695,10 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
     zm1 = t0;
     t0 = zp1;
     zp1 = zp2;
+    dm2 = dm1;
+    dm1 = d00;
+    d00 = dp1;
+    dp1 = dp2;
   };
   int k = kmin;
   for (; k + 1 <= top; k += 2) {
@@ -590,57 +710,80 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
   if (k <= top) layer(k, ya);
   if (!proc || !live) return;
 
-  /* surface value (surf_conc), then tr_bounds at the surface */
-  float trs;
+  /* surface value (surf_conc) in FP64, then tr_bounds at the surface */
+  double surf;
   {
-    double surftr = (double)v0s * ci.vc + m2;
-    if (ci.svol == 0.0 || (surftr < 0.0 && ci.dzs <= a.hmin))
-      surftr = 0.0;
+    surf = ((double)v0s + (double)v0d) * ci.vc + m2;
+    if (ci.svol == 0.0 || (surf < 0.0 && ci.dzs <= a.hmin))
+      surf = 0.0;
     else
-      surftr /= ci.svol;
-    trs = (flags & F_KEEPOLD) ? v0s : sround(surftr, srkey(cellkey0 + (uint32_t)ks, hl));
-    if ((flags & F_THIN) && (trs < mn || trs > mx)) trs = bunc;
-    if (clip && trs < mn) trs = mn;
+      surf /= ci.svol;
+    if (flags & F_KEEPOLD) surf = (double)v0s + (double)v0d;
+    if ((flags & F_THIN) && (surf < (double)mn || surf > (double)mx)) surf = bunc;
+    if (clip && surf < (double)mn) surf = (double)mn;
   }
-  /* the surface value in the layers above the surface outside the
-   * solved range (the advected values are in the output column) */
+  const uint32_t hv = hl ^ 0x5bd1e995U;
+  /* a cell's new value, rounded once to the 32-bit state */
+  auto put = [&](uint32_t o, double v, uint32_t key) {
+    float h;
+    uint32_t r;
+    enc(v, key, h, r);
+    OB[o] = h;
+    OR[o * 4u] = (uint8_t)r;
+  };
+  /* the stored value of the lane's cell at layer kk */
+  auto told = [&](int kk) { return dec(bl1(RT, (uint32_t)kk * L4 + c4), rb(kk)); };
+  /* the layers above the surface outside the solved range take the
+   * surface value, rounded once */
+  float trs;
+  uint32_t trr;
+  enc(surf, srkey(cellkey0 + (uint32_t)ks, hv), trs, trr);
   const int lo = solve ? cb : top + 1, hi = solve ? cs : top;
   for (int kk = ks; kk <= top; kk++)
-    if (kk < lo || kk > hi) OB[(uint32_t)kk * P2 + col] = trs;
+    if (kk < lo || kk > hi) {
+      OB[(uint32_t)kk * P2 + col] = trs;
+      OR[((uint32_t)kk * P2 + col) * 4u] = (uint8_t)trr;
+    }
+  /* cells below the surface outside the solved range: t0 + dlt, rounded */
+  for (int kk = kb; kk < min(ks, lo); kk++) {
+    const uint32_t o = (uint32_t)kk * P2 + col;
+    put(o, told(kk) + (double)OB[o], srkey(cellkey0 + (uint32_t)kk, hv));
+  }
   if (!solve) return;
-  const uint32_t hv = hl ^ 0x5bd1e995U;
   /* the remaining forward rows (their right-hand side reaches the
-   * surface value), b below the surface from the registers */
-
+   * surface value), b below the surface from the walk's last two layers */
   {
     const int rs = max(cb, ks - 1);
     for (int r = rs; r <= cs; r++) {
       const float4 M = CC[(ll)r * P2 * 3];
       const float gm = (r > cb) ? M.x : 0.0f;
-      /* b at r - 1, r, r + 1: r >= ks - 1, so the values below the
-       * surface are the walk's last two */
-      const float b0 = (r >= ks) ? trs : bb1;
-      const float bm = (r > cb) ? ((r - 1 >= ks) ? trs : (r - 1 == ks - 1 ? bb1 : bb2)) : b0;
-      const float bp = (r < cs) ? trs : b0;
-      const float rr = -(gm * (bm - b0) + M.y * (bp - b0));
-      sp = (rr - gm * sp) * M.w;
+      const double b0 = (r >= ks) ? surf : bks1;
+      const double bm = (r - 1 >= ks) ? surf : (r - 1 == ks - 1 ? bks1 : bks2);
+      const float dm = (r > cb) ? (float)(bm - b0) : 0.0f;
+      const float dp = (r < cs) ? (float)(surf - b0) : 0.0f;
+      const float rr = -(gm * dm + M.y * dp);
+      sp = (rr - gm * sp) * M.z;
       Ls[(r - kmin) * 64] = sp;
     }
   }
-  /* Back substitution d_k = s_k - ud_{k+1} d_{k+1}, x = b + d, eight rows
-   * at a time: their b and ud are loaded before their results are stored
-   * over b (d goes to the scratch column in a stiff column) */
+  /* Back substitution d_k = s_k - ud_{k+1} d_{k+1}, x = t0 + dlt + d
+   * (or the surface value + d) in FP64, rounded once; eight rows at a
+   * time, their t0, dlt and ud loaded before their results are stored
+   * over dlt (d goes to the scratch column in a stiff column) */
   const bool stiff = (flags & F_STIFF) != 0;
   {
     float dn = 0.0f, udn = 0.0f;
     for (int r0 = cs; r0 >= cb; r0 -= 8) {
-      float bq[8], uq[8];
+      float tq[8], dq[8], uq[8];
+      uint32_t rq[8];
 #pragma unroll
       for (int jj = 0; jj < 8; jj++) {
-        const int r = r0 - jj;
-        const int rc = max(r, cb);
-        bq[jj] = (rc < ks) ? OB[(uint32_t)rc * P2 + col] : trs;
-        uq[jj] = CC[(ll)rc * P2 * 3].z;
+        const int rc = max(r0 - jj, cb);
+        const bool below = rc < ks;
+        tq[jj] = below ? bl1(RT, (uint32_t)rc * L4 + c4) : 0.0f;
+        rq[jj] = below ? rb(rc) : 0u;
+        dq[jj] = below ? OB[(uint32_t)rc * P2 + col] : 0.0f;
+        uq[jj] = CU[(ll)rc * P2 * 12];
       }
 #pragma unroll
       for (int jj = 0; jj < 8; jj++) {
@@ -649,10 +792,12 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
           const uint32_t o = (uint32_t)r * P2 + col;
           const float sk = Ls[(r - kmin) * 64];
           dn = (r == cs) ? sk : sk - udn * dn;
-          if (stiff)
+          if (stiff) {
             SB[o] = dn;
-          else
-            OB[o] = sround((double)bq[jj] + (double)dn, srkey(cellkey0 + (uint32_t)r, hv));
+          } else {
+            const double bfull = (r < ks) ? dec(tq[jj], rq[jj]) + (double)dq[jj] : surf;
+            put(o, bfull + (double)dn, srkey(cellkey0 + (uint32_t)r, hv));
+          }
           udn = uq[jj];
         }
       }
@@ -661,13 +806,19 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
   if (!stiff) return;
   /* one FP64 residual pass of iterative refinement, the matrix and the
    * right-hand side in FP64 from Kz and dz (tracers.c:3841-3889); the
-   * correction is solved with the FP32 factors; b is in the output
+   * correction is solved with the FP32 factors; dlt is in the output
    * column, d in the scratch column, the correction in the LDS slice */
   {
     const ll P = a.P;
     const ll fc = (ll)(j) * a.nx + i;
     const double dt = a.dt;
-    auto bval = [=](int kk) { return (kk < ks) ? OB[(uint32_t)kk * P2 + col] : trs; };
+    auto bval = [=](int kk) {
+      return (kk < ks) ? dec(bl1(RT, (uint32_t)kk * L4 + c4),
+                             (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(
+                                 RR, (uint32_t)kk * L4 + cr, 0, 0)) +
+                             (double)OB[(uint32_t)kk * P2 + col]
+                       : surf;
+    };
     float e = 0.0f;
     for (int kk = cb; kk <= cs; kk++) {
       const ll ix = (ll)kk * P + fc;
@@ -676,10 +827,10 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
       double cm1 = 0.0, cp1 = 0.0;
       if (kk > cb) cm1 = -Kz[ix] / (0.5 * (dzf[ix - P] + dzc));
       if (kk < cs) cp1 = -Kz[ix + P] / (0.5 * (dzc + dzf[ix + P]));
-      const double bv = (double)bval(kk);
+      const double bv = bval(kk);
       const double dk = (double)SB[o];
-      const double bvm = (kk > cb) ? (double)bval(kk - 1) : bv;
-      const double bvp = (kk < cs) ? (double)bval(kk + 1) : bv;
+      const double bvm = (kk > cb) ? bval(kk - 1) : bv;
+      const double bvp = (kk < cs) ? bval(kk + 1) : bv;
       const double dkm = (kk > cb) ? (double)SB[o - P2] : 0.0;
       const double dkp = (kk < cs) ? (double)SB[o + P2] : 0.0;
       const double rhs = -(cm1 * (bvm - bv) + cp1 * (bvp - bv));
@@ -687,21 +838,22 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
       const float rho = (float)((rhs - Ad) / dzdt);
       const float4 M = CC[(ll)kk * P2 * 3];
       const float gm = (kk > cb) ? M.x : 0.0f;
-      e = (rho - gm * e) * M.w;
+      e = (rho - gm * e) * M.z;
       Ls[(kk - kmin) * 64] = e;
     }
     float ec = 0.0f, udn = 0.0f;
     for (int kk = cs; kk >= cb; kk--) {
       const float ek = Ls[(kk - kmin) * 64];
       ec = (kk == cs) ? ek : ek - udn * ec;
-      udn = CC[(ll)kk * P2 * 3].z;
+      udn = CC[(ll)kk * P2 * 3].w;
       Ls[(kk - kmin) * 64] = ec;
     }
     /* every residual is formed before any value is replaced */
     for (int kk = cb; kk <= cs; kk++) {
       const uint32_t o = (uint32_t)kk * P2 + col;
-      OB[o] = sround((double)bval(kk) + (double)SB[o] + (double)Ls[(kk - kmin) * 64],
-                     srkey(cellkey0 + (uint32_t)kk, hv));
+      const double bv = bval(kk);
+      put(o, bv + (double)SB[o] + (double)Ls[(kk - kmin) * 64],
+          srkey(cellkey0 + (uint32_t)kk, hv));
     }
   }
 }
@@ -724,22 +876,35 @@ __device__ __forceinline__ ll bidx(const LArgs &a, ll idx) {
   return (ll)k * a.P2 + jj * a.nxp + ii;
 }
 
+/* byte offset of the tail of tracer n at cell index c of its volume */
+__device__ __forceinline__ ll toff(const LArgs &a, int n, ll c) {
+  return (((ll)(n >> 2) * a.vol + c) << 2) + (n & 3);
+}
+
+/* FP64 -> head and tail, rounded to nearest at 32 bits, into both buffers */
 __global__ void k_put2(LArgs a, const int *__restrict__ wsp, const ll *__restrict__ wdn, int n,
                        const double *__restrict__ sp, float *__restrict__ t0,
-                       float *__restrict__ t1) {
+                       float *__restrict__ t1, uint8_t *__restrict__ r0,
+                       uint8_t *__restrict__ r1) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.nwet) return;
-  const ll e = (ll)n * a.vol + bidx(a, wdn[q]);
-  const float v = (float)sp[wsp[q]];
-  t0[e] = v;
-  t1[e] = v;
+  const ll c = bidx(a, wdn[q]), e = (ll)n * a.vol + c, o = toff(a, n, c);
+  float h;
+  uint32_t r;
+  enc(sp[wsp[q]], 0x80000000u, h, r);
+  t0[e] = h;
+  t1[e] = h;
+  r0[o] = (uint8_t)r;
+  r1[o] = (uint8_t)r;
 }
 
 __global__ void k_get2(LArgs a, const int *__restrict__ wsp, const ll *__restrict__ wdn, int n,
-                       const float *__restrict__ t, double *__restrict__ sp) {
+                       const float *__restrict__ t, const uint8_t *__restrict__ rm,
+                       double *__restrict__ sp) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.nwet) return;
-  sp[wsp[q]] = (double)t[(ll)n * a.vol + bidx(a, wdn[q])];
+  const ll c = bidx(a, wdn[q]);
+  sp[wsp[q]] = dec(t[(ll)n * a.vol + c], rm[toff(a, n, c)]);
 }
 
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
@@ -750,7 +915,7 @@ __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e
 
 /* per-block (sum, err) of sum(tr*area*dz) over the processed cells */
 __global__ void k_mass2(emstr_dev d, LArgs la, int nz, const float *__restrict__ tr,
-                        double *__restrict__ part) {
+                        const uint8_t *__restrict__ rm, double *__restrict__ part) {
   const int n = blockIdx.y;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ double ss[256], se[256];
@@ -760,9 +925,11 @@ __global__ void k_mass2(emstr_dev d, LArgs la, int nz, const float *__restrict__
     const int kbc = d.kb[col], ksc = d.ks[col];
     const double ar = d.cellarea[col];
     const ll jj = col / d.nx, ii = col - jj * d.nx;
-    const float *t = tr + (ll)n * la.vol + jj * la.nxp + ii;
+    const ll c0 = jj * la.nxp + ii;
+    const float *t = tr + (ll)n * la.vol + c0;
     for (int k = ksc; k >= kbc; k--) {
-      const double m = (double)t[(ll)k * la.P2] * ar * d.dz[(ll)k * d.plane + col];
+      const double v = dec(t[(ll)k * la.P2], rm[toff(la, n, (ll)k * la.P2 + c0)]);
+      const double m = v * ar * d.dz[(ll)k * d.plane + col];
       double s2, e2;
       two_sum(s, m, s2, e2);
       s = s2;
@@ -895,7 +1062,7 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   b->nyp = (d->ny + 31) / 32 * 32; /* tiles up to 4 x 8 rows */
   b->P2 = (ll)b->nxp * b->nyp;
   b->vol = (ll)d->nz * b->P2;
-  if (b->vol * 48 > 0x7fffffffLL || b->vol * 4 * d->ntr > 0xffffffffLL) {
+  if (b->vol * 48 > 0x7fffffffLL || b->vol * 4 * ((d->ntr + 3) / 4 * 4) > 0xffffffffLL) {
     /* buffer offsets of the coefficients and tracers are 32-bit */
     emstr_set_error(EMSTR_EUNSUPPORTED, -1,
                     "batched form: at most %lld cells per window and 4 GiB of tracers",
@@ -907,9 +1074,15 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   b->strip = 64;
   const size_t nt = (size_t)b->vol * d->ntr;
   bool ok = true;
-  for (int q = 0; q < 2; q++) ok &= hipMalloc(&b->tr[q], nt * sizeof(float)) == hipSuccess;
+  const size_t nrm = (size_t)b->vol * 4 * ((d->ntr + 3) / 4);
+  /* slack for the neighbour reads of the last layer past the end: two
+   * rows of tracers, one row of coefficient words */
+  const size_t trb = nt * sizeof(float) + (size_t)b->nxp * 8 + 256;
+  const size_t cob = (size_t)b->vol * 3 * sizeof(float4) + (size_t)b->nxp * 48 + 256;
+  for (int q = 0; q < 2; q++) ok &= hipMalloc(&b->tr[q], trb) == hipSuccess;
+  for (int q = 0; q < 2; q++) ok &= hipMalloc(&b->rm[q], nrm) == hipSuccess;
   ok &= hipMalloc(&b->scr, nt * sizeof(float)) == hipSuccess;
-  ok &= hipMalloc(&b->co, (size_t)b->vol * 3 * sizeof(float4)) == hipSuccess;
+  ok &= hipMalloc(&b->co, cob) == hipSuccess;
   ok &= hipMalloc(&b->ci, (size_t)b->P2 * sizeof(Col2)) == hipSuccess;
   ok &= hipMalloc(&b->mintr, (d->ntr + 1) * sizeof(float)) == hipSuccess;
   ok &= hipMalloc(&b->maxtr, (d->ntr + 1) * sizeof(float)) == hipSuccess;
@@ -921,8 +1094,9 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   }
   /* land and below-bed cells are never written: zero coefficients give
    * no transport through a face into land */
-  hipMemset(b->co, 0, (size_t)b->vol * 3 * sizeof(float4));
-  for (int q = 0; q < 2; q++) hipMemset(b->tr[q], 0, nt * sizeof(float));
+  hipMemset(b->co, 0, cob);
+  for (int q = 0; q < 2; q++) hipMemset(b->tr[q], 0, trb);
+  for (int q = 0; q < 2; q++) hipMemset(b->rm[q], 0, nrm);
   hipMemset(b->stepctr, 0, sizeof(int));
   if (build_blocks(b, h_kb)) {
     emstr_batch_destroy(b);
@@ -934,7 +1108,7 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
 extern "C" void emstr_batch_destroy(struct emstr_batch *b) {
   if (!b) return;
   drop_graph(b);
-  void *v[] = {b->tr[0], b->tr[1], b->scr, b->co, b->ci, b->mintr, b->maxtr, b->stepctr, b->blk};
+  void *v[] = {b->tr[0], b->tr[1], b->rm[0], b->rm[1], b->scr, b->co, b->ci, b->mintr, b->maxtr, b->stepctr, b->blk};
   for (unsigned q = 0; q < sizeof(v) / sizeof(v[0]); q++)
     if (v[q]) hipFree(v[q]);
   free(b);
@@ -989,11 +1163,12 @@ extern "C" int emstr_batch_fields(struct emstr_batch *b, const emstr_dev *d, voi
 }
 
 static void launch_step(emstr_batch *b, const SArgs &sa, const float *in, float *out,
-                        const emstr_dev *d, hipStream_t st) {
+                        const uint8_t *rin, uint8_t *rout, const emstr_dev *d,
+                        hipStream_t st) {
   const size_t lds = (size_t)SWAVES * (b->nz - b->kmin) * 64 * sizeof(float);
   const int ng = (b->ntr + 3) / 4;
   k_step2<<<(unsigned)(8 * b->bmax * ng), 64 * SWAVES, lds, st>>>(
-      sa, in, out, b->co, b->ci, b->blk, b->mintr, b->maxtr, d->layers, d->Kz, d->dz, b->scr,
+      sa, in, out, rin, rout, b->co, b->ci, b->blk, b->mintr, b->maxtr, d->layers, d->Kz, d->dz, b->scr,
       b->stepctr);
 }
 
@@ -1006,7 +1181,8 @@ static int batch_launch(emstr_batch *b, const emstr_dev *d, hipStream_t st) {
   pa.nyp = b->nyp;
   pa.P2 = b->P2;
   int pi = prof_begin(b->prof, KC_PREP, st);
-  k_prep2<<<gp, bp, 0, st>>>(*d, pa, b->co, b->ci);
+  k_cell2<<<dim3((d->nx + 63) / 64, (d->ny + 3) / 4, d->nz), bp, 0, st>>>(*d, pa, b->co);
+  k_col2<<<gp, bp, 0, st>>>(*d, pa, b->co, b->ci);
   prof_end(b->prof, pi, st);
   const float *in = b->tr[b->cur];
   float *out = b->tr[1 - b->cur];
@@ -1023,7 +1199,7 @@ static int batch_launch(emstr_batch *b, const emstr_dev *d, hipStream_t st) {
   sa.nzl = b->nz - b->kmin;
   if (b->bmax > 0) {
     pi = prof_begin(b->prof, KC_FUSED, st);
-    launch_step(b, sa, in, out, d, st);
+    launch_step(b, sa, in, out, b->rm[b->cur], b->rm[1 - b->cur], d, st);
     prof_end(b->prof, pi, st);
   }
   k_tick2<<<1, 1, 0, st>>>(b->stepctr);
@@ -1094,7 +1270,8 @@ extern "C" int emstr_batch_put(struct emstr_batch *b, const emstr_dev *d, int n,
                                const double *sparse, void *stream) {
   if (d->nwet == 0) return 0;
   k_put2<<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
-      largs(b, d), d->wsp, d->wdn, n, sparse, b->tr[b->cur], b->tr[1 - b->cur]);
+      largs(b, d), d->wsp, d->wdn, n, sparse, b->tr[b->cur], b->tr[1 - b->cur], b->rm[b->cur],
+      b->rm[1 - b->cur]);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1102,7 +1279,8 @@ extern "C" int emstr_batch_get(struct emstr_batch *b, const emstr_dev *d, int n,
                                double *sparse, void *stream) {
   if (d->nwet == 0) return 0;
   k_get2<<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(largs(b, d), d->wsp, d->wdn,
-                                                                  n, b->tr[b->cur], sparse);
+                                                                  n, b->tr[b->cur],
+                                                                  b->rm[b->cur], sparse);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1111,6 +1289,7 @@ extern "C" int emstr_batch_mass(struct emstr_batch *b, const emstr_dev *d, doubl
   int nb = (d->ncol + 255) / 256;
   if (nb == 0) return 0;
   k_mass2<<<dim3(nb, d->ntr, 1), 256, 0, (hipStream_t)stream>>>(*d, largs(b, d), b->nz,
-                                                                 b->tr[b->cur], part);
+                                                                 b->tr[b->cur], b->rm[b->cur],
+                                                                 part);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
