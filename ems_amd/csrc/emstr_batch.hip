@@ -381,11 +381,11 @@ __device__ __forceinline__ void enc(double x, uint32_t rnd, float &h, uint32_t &
 __device__ __forceinline__ double dec(float h, uint32_t r) {
   return __longlong_as_double(__double_as_longlong((double)h) | ((long long)r << 21));
 }
-/* the tail as an FP32 number (exact; zero below 2^-96) */
+/* the tail as an FP32 number: r 2^(e - 31) with the sign of h (exact
+ * above 2^-95; h = m 2^f with m in [0.5, 1) gives f = e + 1) */
 __device__ __forceinline__ float tail(float h, uint32_t r) {
-  const uint32_t hb = __float_as_uint(h);
-  const float sc = __uint_as_float((hb & 0xff800000u) - (31u << 23));
-  return ((hb & 0x7f800000u) > (31u << 23)) ? (float)r * sc : 0.0f;
+  return __builtin_amdgcn_ldexpf(__builtin_copysignf((float)r, h),
+                                 __builtin_amdgcn_frexp_expf(h) - 32);
 }
 /* the tail byte of h + t (t = tail(h, r)) */
 __device__ __forceinline__ uint32_t tailbyte(float h, float t) {
@@ -470,7 +470,7 @@ struct Lay {
  * deepest bed to the top; the loads of layer k + 1 are issued before
  * layer k is computed; the LDS slice holds the advected values b of the
  * lane's column, the scratch buffer its forward-sweep values. */
-__global__ void __launch_bounds__(64 * SWAVES) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ void __launch_bounds__(64 * SWAVES)
 k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
         const uint8_t *__restrict__ rin, uint8_t *__restrict__ rout,
         const float4 *__restrict__ co, const Col2 *__restrict__ cinfo,
@@ -653,10 +653,10 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
            * FP32 (the output column holds it until the column's values
            * are formed and rounded once, after the solve) */
           float dlt = -((hh + aT * y.U.x - fba + (t0 + d00) * Bk.w) * Bk.z);
-          const double T = (double)t0 + (double)d00;
-          const double bf = T + (double)dlt;
-          if (k == ks - 1) bunc = bf;
-          if (clip && bf < (double)mn) dlt = (float)((double)mn - T);
+          /* tr_bounds below the surface: the clip test in FP32 (a value
+           * within an ulp of the minimum may go either way) */
+          const float du = dlt;
+          if (clip && t0 + (d00 + dlt) < mn) dlt = (mn - t0) - d00;
           if (live) OB[(uint32_t)k * P2 + col] = dlt;
           const int r = k - 1;
           if (solve && r >= cb) {
@@ -671,6 +671,8 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
             Ls[(r - kmin) * 64] = sp;
           }
           if (k == ks - 1) {
+            const double T = (double)t0 + (double)d00;
+            bunc = T + (double)du;
             bks1 = T + (double)dlt;
             bks2 = ((double)zm1 + (double)dm1) + (double)dd1;
           }
@@ -768,40 +770,52 @@ k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
   }
   /* Back substitution d_k = s_k - ud_{k+1} d_{k+1}, x = t0 + dlt + d
    * (or the surface value + d) in FP64, rounded once; eight rows at a
-   * time, their t0, dlt and ud loaded before their results are stored
-   * over dlt (d goes to the scratch column in a stiff column) */
+   * time, the t0, dlt and ud of the next eight rows loaded before this
+   * chunk's results are stored over its dlt (d goes to the scratch column
+   * in a stiff column) */
   const bool stiff = (flags & F_STIFF) != 0;
   {
     float dn = 0.0f, udn = 0.0f;
-    for (int r0 = cs; r0 >= cb; r0 -= 8) {
-      float tq[8], dq[8], uq[8];
-      uint32_t rq[8];
-#pragma unroll
-      for (int jj = 0; jj < 8; jj++) {
-        const int rc = max(r0 - jj, cb);
-        const bool below = rc < ks;
-        tq[jj] = below ? bl1(RT, (uint32_t)rc * L4 + c4) : 0.0f;
-        rq[jj] = below ? rb(rc) : 0u;
-        dq[jj] = below ? OB[(uint32_t)rc * P2 + col] : 0.0f;
-        uq[jj] = CU[(ll)rc * P2 * 12];
+    float ta[8], da[8], ua[8], tb[8], db[8], ub[8];
+    uint32_t ra[8], rb2[8];
+#define EMSTR_FETCH(R0, T, RR8, D, U)                               \
+  _Pragma("unroll") for (int jj = 0; jj < 8; jj++) {                \
+    const int rc = max((R0) - jj, cb);                              \
+    T[jj] = bl1(RT, (uint32_t)rc * L4 + c4);                        \
+    RR8[jj] = rb(rc);                                               \
+    D[jj] = OB[(uint32_t)rc * P2 + col];                            \
+    U[jj] = CU[(ll)rc * P2 * 12];                                   \
+  }
+#define EMSTR_SOLVE(R0, T, RR8, D, U)                                              \
+  _Pragma("unroll") for (int jj = 0; jj < 8; jj++) {                               \
+    const int r = (R0) - jj;                                                       \
+    if (r >= cb) {                                                                 \
+      const uint32_t o = (uint32_t)r * P2 + col;                                   \
+      const float sk = Ls[(r - kmin) * 64];                                        \
+      dn = (r == cs) ? sk : sk - udn * dn;                                         \
+      if (stiff) {                                                                 \
+        SB[o] = dn;                                                                \
+      } else {                                                                     \
+        const double bfull = (r < ks) ? dec(T[jj], RR8[jj]) + (double)D[jj] : surf; \
+        put(o, bfull + (double)dn, srkey(cellkey0 + (uint32_t)r, hv));             \
+      }                                                                            \
+      udn = U[jj];                                                                 \
+    }                                                                              \
+  }
+    EMSTR_FETCH(cs, ta, ra, da, ua)
+    for (int r0 = cs; r0 >= cb; r0 -= 16) {
+      if (r0 - 8 >= cb) {
+        EMSTR_FETCH(r0 - 8, tb, rb2, db, ub)
       }
-#pragma unroll
-      for (int jj = 0; jj < 8; jj++) {
-        const int r = r0 - jj;
-        if (r >= cb) {
-          const uint32_t o = (uint32_t)r * P2 + col;
-          const float sk = Ls[(r - kmin) * 64];
-          dn = (r == cs) ? sk : sk - udn * dn;
-          if (stiff) {
-            SB[o] = dn;
-          } else {
-            const double bfull = (r < ks) ? dec(tq[jj], rq[jj]) + (double)dq[jj] : surf;
-            put(o, bfull + (double)dn, srkey(cellkey0 + (uint32_t)r, hv));
-          }
-          udn = uq[jj];
-        }
+      EMSTR_SOLVE(r0, ta, ra, da, ua)
+      if (r0 - 8 < cb) break;
+      if (r0 - 16 >= cb) {
+        EMSTR_FETCH(r0 - 16, ta, ra, da, ua)
       }
+      EMSTR_SOLVE(r0 - 8, tb, rb2, db, ub)
     }
+#undef EMSTR_FETCH
+#undef EMSTR_SOLVE
   }
   if (!stiff) return;
   /* one FP64 residual pass of iterative refinement, the matrix and the

@@ -17,8 +17,15 @@
  * volume transports q = flux*dt (m3 per step) and Courant numbers, and the
  * column matrix is built and factored in FP64 (k_mat32) and stored
  * row-normalised in FP32, so the tracer kernels read 4-byte coefficients.
- * The update tr - (sum of face transports)/(area*dz) is the reference's
- * tr - ((dtracer/area + Fz[zp1] - Fz[c])/dz) with the same fluxes.
+ * The update is the reference's tr - ((dtracer/area + Fz[zp1] - Fz[c])/dz)
+ * in anomaly form, tr - (sum_f s_f (F_f - tr) q_f + tr D) / V with D the
+ * net volume divergence of the cell (FP64 in the prep): the face values
+ * are computed from the stencil's differences against the cell value, so
+ * their rounding is relative to those differences.  The column solve is
+ * in increment form, A (x - b) = dzdt b - A b.  Each new FP32 value is
+ * formed in FP64 and rounded stochastically (a deterministic hash of cell,
+ * tracer, step and stage), so sub-ulp increments do not bias the state or
+ * the mass.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -42,12 +49,42 @@ struct emstr_prec {
   /* per-cell FP32 coefficients [n3] (W, S and bottom face of the cell) */
   float *qW, *cW, *qS, *cS, *qB, *cB;
   float *rvol;       /* 1/(area*dz)                                     */
+  float *div;        /* net volume divergence per step (FP64 -> FP32)    */
+  float *mg;         /* upper off-diagonal of the normalised matrix      */
+  float *scr;        /* [ntr][n3] refinement scratch                     */
+  unsigned nstep;    /* steps taken (stochastic rounding keys)           */
   float *w5, *w9;    /* QUICKEST vertical weights                       */
   float *mb, *mr, *mu; /* row-normalised factored column matrix         */
   int *stiff;        /* [plane] column gets the FP64 refinement pass    */
   int trasc, ultimate;
   emstr_prof *prof;
 };
+
+/*-------------------------------------------------------------------*/
+/* deterministic stochastic rounding                                 */
+/*-------------------------------------------------------------------*/
+__device__ __forceinline__ uint32_t pmix(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+/* key of (cell, tracer, step, stage) */
+__device__ __forceinline__ uint32_t pkey(ll idx, int n, unsigned step, unsigned stage) {
+  return pmix((uint32_t)idx ^ pmix((uint32_t)n * 0x85EBCA77U ^ step * 0xC2B2AE3DU ^
+                                   stage * 0x27D4EB2FU));
+}
+/* the FP64 value x rounded to one of its two FP32 neighbours, the upper
+ * with probability equal to the remainder's fraction of an ulp (a uniform
+ * 29-bit number added to the 29 bits FP32 drops, then cleared) */
+__device__ __forceinline__ float psround(double x, uint32_t r) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  u += (unsigned long long)(r >> 3);
+  u &= ~0x1FFFFFFFULL;
+  return (float)__longlong_as_double((long long)u);
+}
 
 /*-------------------------------------------------------------------*/
 /* coefficient prep (FP64 in, FP32 out), one thread per wet cell      */
@@ -72,6 +109,16 @@ __global__ void k_prep32(emstr_dev d, emstr_prec p, double dtu) {
   p.qB[idx] = (float)(d.w[idx] * dtu * area);
   p.cB[idx] = (float)(d.w[idx] * dtu / w9);
   p.rvol[idx] = (dz != 0.0) ? (float)(1.0 / (area * dz)) : 0.0f;
+  {
+    /* E, N and top faces are the W, S and bottom faces of the neighbours */
+    const int ksc = d.ks[col];
+    const double qW = d.u1flux3d[idx], qS = d.u2flux3d[idx];
+    const double qE = (k >= d.kb[col + 1]) ? d.u1flux3d[idx + 1] : 0.0;
+    const double qN = (k >= d.kb[col + d.nx]) ? d.u2flux3d[idx + d.nx] : 0.0;
+    const double qB = d.w[idx] * area;
+    const double qT = (ksc >= 0 && k < ksc) ? d.w[idx + d.plane] * area : 0.0;
+    p.div[idx] = (float)((qE - qW + qN - qS + qT - qB) * dtu);
+  }
   if (p.w5) {
     p.w5[idx] = (float)(dzb / (dzb + dz));
     p.w9[idx] = (float)w9;
@@ -79,8 +126,9 @@ __global__ void k_prep32(emstr_dev d, emstr_prec p, double dtu) {
 }
 
 /* column matrix, FP64 build and factor, stored row-normalised in FP32:
- * the solve of A x = dzdt*b becomes s_k = mr_k*(b_k - mb_k*s_{k-1}),
- * x_k = s_k - mu_{k+1}*x_{k+1} (tracers.c:3841-3889, 4185-4205). */
+ * the solve of A x = dzdt*r becomes s_k = mr_k*(r_k - mb_k*s_{k-1}),
+ * x_k = s_k - mu_{k+1}*x_{k+1} (tracers.c:3841-3889, 4185-4205); mb and mg
+ * are the off-diagonals over dzdt. */
 __global__ void k_mat32(emstr_dev d, emstr_prec p) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.ncol) return;
@@ -107,6 +155,7 @@ __global__ void k_mat32(emstr_dev d, emstr_prec p) {
   p.mb[ic] = 0.0f;
   p.mr[ic] = (float)(dzdt / div);
   p.mu[ic] = 0.0f;
+  p.mg[ic] = (float)(Cp1 / dzdt);
   double Cp1prev = Cp1;
   for (int kk = cb + 1; kk <= cs; kk++) {
     double dzc = dzp, dzf_c = dzf_p, Kc = Kp;
@@ -129,6 +178,7 @@ __global__ void k_mat32(emstr_dev d, emstr_prec p) {
     p.mb[ic] = (float)(Cm1 / dzdt);
     p.mr[ic] = (float)(dzdt / div);
     p.mu[ic] = (float)ud;
+    p.mg[ic] = (float)(Cp1 / dzdt);
     Cp1prev = Cp1;
   }
   p.stiff[col] = stiff;
@@ -297,50 +347,60 @@ __device__ __forceinline__ void co32(const emstr_dev &d, const emstr_prec &p,
   }
 }
 
-/* fluxes of one cell: returns the horizontal sum FE-FW+FN-FS */
+/* Fluxes of one cell as anomalies: the stencil values less the cell's
+ * value c0 (the schemes are translation invariant, so each face value
+ * comes out as F - c0); returns the horizontal sum of (F - c0) q over
+ * E - W + N - S, and the bottom and top terms in FB, FT. */
 template <int SCH, int ULT>
 __device__ __forceinline__ float fl32(const float *__restrict__ t, ll idx,
                                       const St32 &s, const Co32 &c, bool top,
                                       float &FB, float &FT) {
   const float c0 = t[idx];
-  const float xm1 = t[idx + s.xm1], xm2 = t[idx + s.xm2], xp1 = t[idx + s.xp1],
-              xp2 = t[idx + s.xp2];
-  const float ym1 = t[idx + s.ym1], ym2 = t[idx + s.ym2], yp1 = t[idx + s.yp1],
-              yp2 = t[idx + s.yp2];
-  const float zm1 = t[idx + s.zm1], zm2 = t[idx + s.zm2], zp1 = t[idx + s.zp1],
-              zp2 = t[idx + s.zp2];
-  float FW = flux32<SCH, ULT>(xm2, xm1, c0, xp1, c.W);
-  float FE = s.wp1 ? flux32<SCH, ULT>(xm1, c0, xp1, xp2, c.E) : 0.0f;
-  float FS = flux32<SCH, ULT>(ym2, ym1, c0, yp1, c.S);
-  float FN = s.sp1 ? flux32<SCH, ULT>(ym1, c0, yp1, yp2, c.N) : 0.0f;
-  FB = flux32<SCH, ULT>(zm2, zm1, c0, zp1, c.B);
-  FT = top ? flux32<SCH, ULT>(zm1, c0, zp1, zp2, c.T) : 0.0f;
+  const float xm1 = t[idx + s.xm1] - c0, xm2 = t[idx + s.xm2] - c0,
+              xp1 = t[idx + s.xp1] - c0, xp2 = t[idx + s.xp2] - c0;
+  const float ym1 = t[idx + s.ym1] - c0, ym2 = t[idx + s.ym2] - c0,
+              yp1 = t[idx + s.yp1] - c0, yp2 = t[idx + s.yp2] - c0;
+  const float zm1 = t[idx + s.zm1] - c0, zm2 = t[idx + s.zm2] - c0,
+              zp1 = t[idx + s.zp1] - c0, zp2 = t[idx + s.zp2] - c0;
+  float FW = flux32<SCH, ULT>(xm2, xm1, 0.0f, xp1, c.W);
+  float FE = s.wp1 ? flux32<SCH, ULT>(xm1, 0.0f, xp1, xp2, c.E) : 0.0f;
+  float FS = flux32<SCH, ULT>(ym2, ym1, 0.0f, yp1, c.S);
+  float FN = s.sp1 ? flux32<SCH, ULT>(ym1, 0.0f, yp1, yp2, c.N) : 0.0f;
+  FB = flux32<SCH, ULT>(zm2, zm1, 0.0f, zp1, c.B);
+  FT = top ? flux32<SCH, ULT>(zm1, 0.0f, zp1, zp2, c.T) : 0.0f;
   return FE - FW + FN - FS;
 }
 
-/* horizontal divergence of the cell at layer k (tracer t) */
+/* the horizontal volume divergence of the cell, in FP64 */
+__device__ __forceinline__ double hvol(const Co32 &c) {
+  return ((double)c.E.q - (double)c.W.q) + ((double)c.N.q - (double)c.S.q);
+}
+
+/* horizontal flux divergence (flux form, FP64) of the cell at layer k */
 template <int SCH, int ULT>
-__device__ float hdiv32(const emstr_dev &d, const emstr_prec &p,
-                        const float *__restrict__ t, ll col, int k, int kbc) {
+__device__ double hdiv32(const emstr_dev &d, const emstr_prec &p,
+                         const float *__restrict__ t, ll col, int k, int kbc) {
   const ll idx = (ll)k * d.plane + col;
   St32 s = sten32(d, col, k, kbc);
   Co32 c;
   co32<SCH>(d, p, col, idx, k, s, false, c);
   float FB, FT;
-  return fl32<SCH, ULT>(t, idx, s, c, false, FB, FT);
+  const float h = fl32<SCH, ULT>(t, idx, s, c, false, FB, FT);
+  return (double)h + (double)t[idx] * hvol(c);
 }
 
-/* unclipped sub-surface update of the cell at layer k */
+/* unclipped sub-surface update of the cell at layer k, in FP64 */
 template <int SCH, int ULT>
-__device__ float upd32(const emstr_dev &d, const emstr_prec &p,
-                       const float *__restrict__ t, ll col, int k, int kbc) {
+__device__ double upd32(const emstr_dev &d, const emstr_prec &p,
+                        const float *__restrict__ t, ll col, int k, int kbc) {
   const ll idx = (ll)k * d.plane + col;
   St32 s = sten32(d, col, k, kbc);
   Co32 c;
   co32<SCH>(d, p, col, idx, k, s, true, c);
   float FB, FT;
-  float h = fl32<SCH, ULT>(t, idx, s, c, true, FB, FT);
-  return t[idx] - (h + (FT - FB)) * c.rv;
+  const float c0 = t[idx];
+  const float h = fl32<SCH, ULT>(t, idx, s, c, true, FB, FT);
+  return (double)c0 - (double)((h + (FT - FB) + c0 * p.div[idx]) * c.rv);
 }
 
 /*-------------------------------------------------------------------*/
@@ -379,10 +439,11 @@ k_adv32(emstr_dev d, emstr_prec p, const float *__restrict__ tin,
         continue;
       }
       float FB, FT;
-      float h = fl32<SCH, ULT>(t, idx, s, c, true, FB, FT);
-      float r = t[idx] - (h + (FT - FB)) * c.rv;
-      if (d.mintr[n] <= 1e-5 && r < (float)d.mintr[n]) r = (float)d.mintr[n];
-      tout[n * N3 + idx] = r;
+      const float c0 = t[idx];
+      const float h = fl32<SCH, ULT>(t, idx, s, c, true, FB, FT);
+      double r = (double)c0 - (double)((h + (FT - FB) + c0 * p.div[idx]) * c.rv);
+      if (d.mintr[n] <= 1e-5 && r < d.mintr[n]) r = d.mintr[n];
+      tout[n * N3 + idx] = psround(r, pkey(idx, n, p.nstep, 0));
     }
     return;
   }
@@ -405,23 +466,26 @@ k_adv32(emstr_dev d, emstr_prec p, const float *__restrict__ tin,
       trs = (double)t[idx];
     } else {
       float FB, FT;
-      float h0 = fl32<SCH, ULT>(t, idx, s, c, false, FB, FT);
+      const float c0 = t[idx];
+      const float h0 = fl32<SCH, ULT>(t, idx, s, c, false, FB, FT);
       int cc = k;
       double waterbot = (cc == kbc) ? botz : lay[cc];
       double dzs = subeta - waterbot;
       double svol = dzs * area;
-      double surftr = (double)FB; /* -fcbot: transport through the bottom */
-      double dtrc = (double)h0;
+      /* -fcbot, the transport through the bottom, and the horizontal
+       * divergence, in flux form */
+      double surftr = (double)FB + (double)c0 * (double)c.B.q;
+      double dtrc = (double)h0 + (double)c0 * hvol(c);
       while (cc != kold && cc < ktop) {
         surftr += (double)t[(ll)cc * P + col] * area * (lay[cc + 1] - waterbot) - dtrc;
         cc++;
         waterbot = lay[cc];
-        dtrc = (double)hdiv32<SCH, ULT>(d, p, t, col, cc, kbc);
+        dtrc = hdiv32<SCH, ULT>(d, p, t, col, cc, kbc);
       }
       surftr += (double)t[(ll)cc * P + col] * area * (osubeta - waterbot) - dtrc;
       while (cc != ktop) {
         cc++;
-        surftr -= (double)hdiv32<SCH, ULT>(d, p, t, col, cc, kbc);
+        surftr -= hdiv32<SCH, ULT>(d, p, t, col, cc, kbc);
       }
       if (svol == 0.0 || (surftr < 0.0 && dzs <= d.hmin))
         surftr = 0.0;
@@ -429,12 +493,12 @@ k_adv32(emstr_dev d, emstr_prec p, const float *__restrict__ tin,
         surftr /= svol;
       trs = (botz > 0.0 && d.dz[idx] < d.hmin) ? (double)t[idx] : surftr;
     }
-    float r = (float)trs;
     /* thin surface layer (tracers.c:3250-3272) */
     const double dzco = (kold >= kbc) ? d.dz[(ll)kold * P + col] : 0.0;
-    if (k > kbc && dzco < d.hmin && ((double)r < d.mintr[n] || (double)r > d.maxtr[n]))
-      r = d.advect[n] ? upd32<SCH, ULT>(d, p, t, col, k - 1, kbc) : t[idx - P];
-    if (d.mintr[n] <= 1e-5 && r < (float)d.mintr[n]) r = (float)d.mintr[n];
+    if (k > kbc && dzco < d.hmin && (trs < d.mintr[n] || trs > d.maxtr[n]))
+      trs = d.advect[n] ? upd32<SCH, ULT>(d, p, t, col, k - 1, kbc) : (double)t[idx - P];
+    if (d.mintr[n] <= 1e-5 && trs < d.mintr[n]) trs = d.mintr[n];
+    const float r = psround(trs, pkey(idx, n, p.nstep, 0));
     for (int kk = k; kk <= ktop; kk++) o[(ll)kk * P + col] = r;
   }
 }
@@ -444,7 +508,7 @@ k_adv32(emstr_dev d, emstr_prec p, const float *__restrict__ tin,
 /* iterative refinement in stiff columns                              */
 /*-------------------------------------------------------------------*/
 __global__ void __launch_bounds__(256)
-k_vdiff32(emstr_dev d, emstr_prec p, float *__restrict__ tin,
+k_vdiff32(emstr_dev d, emstr_prec p, const float *__restrict__ tin,
           float *__restrict__ tout) {
   const int q = blockIdx.x * 64 + threadIdx.x;
   const int tl = threadIdx.y;
@@ -465,7 +529,7 @@ k_vdiff32(emstr_dev d, emstr_prec p, float *__restrict__ tin,
   const int stiff = solve ? p.stiff[col] : 0;
   const double dt = d.dt;
   for (int n = tl; n < d.ntr; n += 4) {
-    float *b = tin + n * N3;
+    const float *b = tin + n * N3;
     float *x = tout + n * N3;
     if (!d.in_tdif[n] || !solve) {
       for (int kk = kbc; kk <= top; kk++) x[(ll)kk * P + col] = b[(ll)kk * P + col];
@@ -473,48 +537,63 @@ k_vdiff32(emstr_dev d, emstr_prec p, float *__restrict__ tin,
     }
     for (int kk = kbc; kk < cb; kk++) x[(ll)kk * P + col] = b[(ll)kk * P + col];
     for (int kk = cs + 1; kk <= top; kk++) x[(ll)kk * P + col] = b[(ll)kk * P + col];
-    float s = 0.0f;
+    /* increment form: A d = dzdt b - A b, the right-hand side row k
+     * -(gm (b_{k-1} - b_k) + gp (b_{k+1} - b_k)) over dzdt from the
+     * differences of b; forward sweep into x, back sweep d into x */
+    float s = 0.0f, bm = 0.0f;
     for (int kk = cb; kk <= cs; kk++) {
       const ll ic = (ll)kk * P + col;
-      s = p.mr[ic] * (b[ic] - p.mb[ic] * s);
+      const float bc = b[ic];
+      const float dm = (kk > cb) ? bm - bc : 0.0f;
+      const float dp = (kk < cs) ? b[ic + P] - bc : 0.0f;
+      const float r = -(p.mb[ic] * dm + p.mg[ic] * dp);
+      s = p.mr[ic] * (r - p.mb[ic] * s);
       x[ic] = s;
+      bm = bc;
     }
     for (int kk = cs - 1; kk >= cb; kk--) {
       const ll ic = (ll)kk * P + col;
       s = x[ic] - p.mu[ic + P] * s;
       x[ic] = s;
     }
-    if (!stiff) continue;
-    /* r = dzdt*b - A x in FP64 (A from Kz and dz as tracers.c:3841-3889);
-     * correction: forward sweep stored in b (no longer needed), back
-     * sweep adds it to x */
-    float e = 0.0f;
+    if (!stiff) {
+      for (int kk = cb; kk <= cs; kk++) {
+        const ll ic = (ll)kk * P + col;
+        x[ic] = psround((double)b[ic] + (double)x[ic], pkey(ic, n, p.nstep, 1));
+      }
+      continue;
+    }
+    /* one FP64 residual pass: R = dzdt b - A (b + d) (A from Kz and dz as
+     * tracers.c:3841-3889); its forward sweep in the scratch column, the
+     * back sweep adds the correction, x = b + d + e rounded once */
+    float *e = p.scr + n * N3;
+    float es = 0.0f;
     double dzm = 0.0;
     for (int kk = cb; kk <= cs; kk++) {
       const ll ic = (ll)kk * P + col;
       const double dzc = d.dz[ic];
       const double dzdt = dzc / dt;
-      const double xc = (double)x[ic];
+      const double xc = (double)b[ic] + (double)x[ic];
       double cm1 = 0.0, cp1 = 0.0, xm = 0.0, xp = 0.0;
       if (kk > cb) {
         cm1 = -d.Kz[ic] / (0.5 * (dzm + dzc));
-        xm = (double)x[ic - P];
+        xm = (double)b[ic - P] + (double)x[ic - P];
       }
       if (kk < cs) {
         cp1 = -d.Kz[ic + P] / (0.5 * (dzc + d.dz[ic + P]));
-        xp = (double)x[ic + P];
+        xp = (double)b[ic + P] + (double)x[ic + P];
       }
       const double C = dzdt - cm1 - cp1;
       const double r = dzdt * (double)b[ic] - (cm1 * xm + C * xc + cp1 * xp);
-      e = p.mr[ic] * ((float)(r / dzdt) - p.mb[ic] * e);
-      b[ic] = e;
+      es = p.mr[ic] * ((float)(r / dzdt) - p.mb[ic] * es);
+      e[ic] = es;
       dzm = dzc;
     }
     float ec = 0.0f;
     for (int kk = cs; kk >= cb; kk--) {
       const ll ic = (ll)kk * P + col;
-      ec = (kk == cs) ? b[ic] : b[ic] - p.mu[ic + P] * ec;
-      x[ic] = x[ic] + ec;
+      ec = (kk == cs) ? e[ic] : e[ic] - p.mu[ic + P] * ec;
+      x[ic] = psround((double)b[ic] + (double)x[ic] + (double)ec, pkey(ic, n, p.nstep, 1));
     }
   }
 }
@@ -591,8 +670,9 @@ extern "C" struct emstr_prec *emstr_prec_create(const emstr_dev *d, void *stream
   bool ok = true;
   for (int b = 0; b < 2; b++) ok &= hipMalloc(&p->tr[b], nt * n3 * sizeof(float)) == hipSuccess;
   float **f[] = {&p->qW, &p->cW, &p->qS, &p->cS, &p->qB, &p->cB, &p->rvol,
-                 &p->mb, &p->mr, &p->mu};
-  for (int q = 0; q < 10; q++) ok &= hipMalloc(f[q], n3 * sizeof(float)) == hipSuccess;
+                 &p->mb, &p->mr, &p->mu, &p->div, &p->mg};
+  for (int q = 0; q < 12; q++) ok &= hipMalloc(f[q], n3 * sizeof(float)) == hipSuccess;
+  ok &= hipMalloc(&p->scr, nt * n3 * sizeof(float)) == hipSuccess;
   if (d->trasc == QUICKEST) {
     ok &= hipMalloc(&p->w5, n3 * sizeof(float)) == hipSuccess;
     ok &= hipMalloc(&p->w9, n3 * sizeof(float)) == hipSuccess;
@@ -604,7 +684,8 @@ extern "C" struct emstr_prec *emstr_prec_create(const emstr_dev *d, void *stream
     return NULL;
   }
   for (int b = 0; b < 2; b++) hipMemset(p->tr[b], 0, nt * n3 * sizeof(float));
-  for (int q = 0; q < 10; q++) hipMemset(*f[q], 0, n3 * sizeof(float));
+  for (int q = 0; q < 12; q++) hipMemset(*f[q], 0, n3 * sizeof(float));
+  hipMemset(p->scr, 0, nt * n3 * sizeof(float));
   if (p->w5) hipMemset(p->w5, 0, n3 * sizeof(float)), hipMemset(p->w9, 0, n3 * sizeof(float));
   hipMemset(p->stiff, 0, (size_t)d->plane * sizeof(int));
   p->cur = 0;
@@ -614,8 +695,8 @@ extern "C" struct emstr_prec *emstr_prec_create(const emstr_dev *d, void *stream
 extern "C" void emstr_prec_destroy(struct emstr_prec *p) {
   if (!p) return;
   void *v[] = {p->tr[0], p->tr[1], p->qW, p->cW, p->qS, p->cS, p->qB, p->cB,
-               p->rvol, p->w5, p->w9, p->mb, p->mr, p->mu, p->stiff};
-  for (int q = 0; q < 15; q++)
+               p->rvol, p->w5, p->w9, p->mb, p->mr, p->mu, p->stiff, p->div, p->mg, p->scr};
+  for (int q = 0; q < 18; q++)
     if (v[q]) hipFree(v[q]);
   free(p);
 }
@@ -693,6 +774,7 @@ extern "C" int emstr_prec_vdiff(struct emstr_prec *p, const emstr_dev *d, int *e
   k_vdiff32<<<(d->ncol + 63) / 64, b, 0, st>>>(*d, *p, p->tr[p->cur], p->tr[1 - p->cur]);
   prof_end(p->prof, pi, st);
   p->cur = 1 - p->cur;
+  p->nstep++;
   return hipGetLastError() == hipSuccess ? 0 : emstr_set_error(EMSTR_EDEVICE, -1, "vdiff32");
 }
 

@@ -88,3 +88,52 @@ def test_stiff_columns_use_fp64_refinement():
         O.step(case)
     e = errors(f32, case.tracers())
     assert e.max() <= 1e-5, e
+
+
+def pointwise(a, ref, floor=1e-3):
+    """max |a - ref| / |ref| per tracer over the cells where |ref| is at
+    least floor times the tracer's largest magnitude"""
+    big = np.abs(ref) >= floor * np.abs(ref).max(axis=1, keepdims=True)
+    rel = np.where(big, np.abs(a - ref) / np.maximum(np.abs(ref), 1e-300), 0.0)
+    return rel.max(axis=1)
+
+
+def test_c2_precision_vs_plain_500_steps():
+    """The 200x300x30 curvilinear case, 10 tracers, QUICKEST, dt = 120 s:
+    the FP32 precision form against the FP64 plain form (bit-exact with
+    the oracle) over 500 steps.  Error (|difference| over each tracer's
+    largest magnitude) within 1e-5 at steps 100 and 500, the pointwise
+    relative error over cells above 1e-3 of the largest magnitude reported,
+    and the relative mass drift within 1e-6 against the FP64 form's mass at
+    step 500."""
+    say("prec c2 500 steps vs plain")
+    from ems_amd import _lib
+    case = configs.c2()
+    L = _lib.lib()
+    L.set_dz(case.gp, case.wp, case.pp)
+    L.emstr_forget_window(case.gp)
+    p = T.Context(case, T.PLAIN_F64)
+    p.upload_fields()
+    p.vel_w()
+    p.upload()
+    q = T.Context(case, T.PREC_F32)
+    q.upload()
+    done = 0
+    for upto in (100, 500):
+        p.step(upto - done)
+        q.step(upto - done)
+        done = upto
+        p.download()
+        ref = case.tracers()
+        q.download()
+        got = case.tracers()
+        e = errors(got, ref)
+        say("c2 step %d: error %.3g (tracer %d), pointwise above 1e-3: %.3g"
+            % (upto, e.max(), int(np.argmax(e)), pointwise(got, ref).max()))
+        assert e.max() <= 1e-5, (upto, e)
+    mq, mp = q.mass(), p.mass()
+    p.close()
+    q.close()
+    vs64 = np.abs(mq - mp) / np.abs(mp)
+    say("c2 mass against the FP64 form after 500 steps: %.3g" % vs64.max())
+    assert vs64.max() <= 1e-6, vs64
