@@ -22,7 +22,12 @@ build runs on one GPU.
   --workload c4-compas-ffsl         the same mesh with COMPAS FFSL (plain
                                     FP64 form)
   --workload c5-ensemble            the 64-window set in the batched FP32
-                                    form, one ensemble hipGraph
+                                    form, one ensemble hipGraph, with the
+                                    per-window error and mass drift against
+                                    the FP64 plain form
+  --workload c2-precision           the 200x300x30 QUICKEST case: the FP32
+                                    precision form against the FP64 plain
+                                    form, error and drift over 500 steps
   --copy-probe MB                   time a plain device copy of MB MiB and
                                     exit (calibrates the HBM byte counters)
 """
@@ -97,7 +102,7 @@ def measured_traffic(kernel):
     """HBM bytes per dispatch of a kernel on this workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
     calibration recorded there); None if the kernel is not listed."""
-    path = os.path.join(ROOT, "profiles", "advB_c3_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "batched_c3_hbm_traffic.md")
     if kernel is None or not os.path.exists(path):
         return None
     for line in open(path):
@@ -135,7 +140,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--ntr", type=int, default=40)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps of the accuracy comparison against the FP64 form")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the batched form's default launch shape")
@@ -144,7 +151,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0)
     ap.add_argument("--workload", default="c3",
-                    choices=["c3", "c3-ffsl", "c4-compas", "c4-compas-ffsl", "c5-ensemble"])
+                    choices=["c3", "c3-ffsl", "c4-compas", "c4-compas-ffsl", "c5-ensemble",
+                             "c2-precision"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the tracer step on one GPU only; "
@@ -159,6 +167,8 @@ def main():
         return bench_compas(args)
     if args.workload == "c5-ensemble":
         return bench_ensemble(args)
+    if args.workload == "c2-precision":
+        return bench_c2(args)
     forms = set(args.forms.split(","))
 
     from ems_amd import _lib, configs
@@ -229,15 +239,13 @@ def main():
         out["ms_per_step"] = rB["ms"]
         ktB, kcB = rB["kt"], rB["kc"]
         dom = int(np.argmax(ktB))
-        # algorithmic bytes per launch of the dominant kernel (FP32 tracers):
-        # advection reads and writes every tracer of every wet cell once and
-        # reads the cell's 16 coefficient words; the vertical solve reads
-        # and writes every tracer once and reads 4 coefficient words
-        # the fused step reads and writes every tracer of every wet cell
-        # once and reads the cell's 12 coefficient words; the prep reads
-        # the 7 FP64 fields and writes the coefficients
-        alg = {4: nwet * (ntr * 8 + 12 * 4),
-               2: nwet * (7 * 8 + 12 * 4)}.get(dom, nwet * ntr * 8)
+        # algorithmic bytes per launch of the dominant kernel: the fused
+        # step reads and writes every tracer of every wet cell once (a
+        # 4-byte head and a 1-byte tail each way) and reads the cell's 12
+        # coefficient words; the prep reads the 7 FP64 fields and writes
+        # the coefficients
+        alg = {4: nwet * (ntr * 10 + 12 * 4),
+               2: nwet * (7 * 8 + 12 * 4)}.get(dom, nwet * ntr * 10)
         avg_ms = ktB[dom] / max(kcB[dom], 1)
         achieved = alg / (avg_ms * 1e-3) / 1e9
         kname = {4: "k_step2"}.get(dom)
@@ -246,7 +254,8 @@ def main():
                            "frac": achieved / HBM_PEAK_GBS,
                            "traffic": measured_traffic(kname),
                            "alg_bytes_per_launch": alg, "avg_launch_ms": avg_ms}
-        out["step_hbm_fraction"] = (nwet * ntr * 8 + nwet * 56) / (rB["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        # per step: the tracer state in and out, the 7 FP64 fields in
+        out["step_hbm_fraction"] = (nwet * ntr * 10 + nwet * 56) / (rB["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
         if tuned:
             out["autotune"] = tuned
     kms = {}
@@ -263,25 +272,24 @@ def main():
     if "precision" in res:
         out["fp32_precision_unfused"] = {"value": cu / (res["precision"]["ms"] * 1e-3),
                                          "ms_per_step": res["precision"]["ms"]}
-    if "plain" in res and "batched" in res:
-        # accuracy of the batched FP32 form against the FP64 plain form
-        c64, cB = ctx["plain"], ctx["batched"]
-        m64_1, mB_1 = c64.mass(), cB.mass()
+    if "plain" in res:
+        # accuracy of the FP32 forms against the FP64 plain form, every
+        # context carried on to the same step count (accuracy_steps)
+        c64 = ctx["plain"]
+        extra = max(0, args.accuracy_steps - nsteps)
+        c64.step(extra)
+        m64 = c64.mass()
         c64.download()
         ref = case.tracers()
-        cB.download()
-        fB = case.tracers()
-        span = np.maximum(np.abs(ref).max(axis=1), 1e-30)
-        err = (np.abs(fB - ref).max(axis=1) / span)
-        rms = np.sqrt(((fB - ref) ** 2).mean(axis=1)) / span
-        m64_0, mB_0 = res["plain"]["m0"], res["batched"]["m0"]
-        driftB = np.abs(mB_1 - mB_0) / np.abs(mB_0)
-        drift64 = np.abs(m64_1 - m64_0) / np.abs(m64_0)
-        out["accuracy"] = {"steps": nsteps, "max_rel_err": float(err.max()),
-                           "rms_rel_err": float(rms.max()),
-                           "mass_drift_fp32": float(driftB.max()),
-                           "mass_drift_fp64": float(drift64.max()),
-                           "worst_tracer": int(np.argmax(err))}
+        for form, key in (("batched", "accuracy"), ("precision", "accuracy_precision")):
+            if form not in ctx:
+                continue
+            cf = ctx[form]
+            cf.step(extra)
+            mf = cf.mass()
+            cf.download()
+            out[key] = accuracy(case.tracers(), ref, mf, res[form]["m0"], m64,
+                                res["plain"]["m0"], nsteps + extra)
     for c in ctx.values():
         c.close()
 
@@ -306,35 +314,92 @@ def main():
         # CPU baseline: the oracle (FP64 C restatement) with OpenMP over
         # tracers on the host cores, a bounded number of steps.
         from oracle import oracle as O
-        nth = min(16, os.cpu_count() or 1)
         O.lib()
         O.set_dz(case.gp, case.wp, case.pp)
-        t = time.time()
-        n = 0
-        while True:
-            O.step(case, nth)
-            n += 1
-            if time.time() - t > args.cpu_seconds or n >= 50:
-                break
-        dt = time.time() - t
-        out["cpu_baseline"] = {"value": cu * n / dt, "unit": "cell-updates/s",
-                               "cores": nth, "kind": "port",
-                               "sample": "%d full c3 steps (all %d tracers), OpenMP over "
-                                         "tracers" % (n, ntr)}
+        out["cpu_baseline"] = cpu_baselines([case], O.step, args.cpu_seconds, "c3")
     print(json.dumps(out))
 
 
-def _cpu_baseline(step_fn, cu, seconds, nth, what):
-    t = time.time()
-    n = 0
-    while True:
-        step_fn()
-        n += 1
-        if time.time() - t > seconds or n >= 50:
-            break
-    dt = time.time() - t
-    return {"value": cu * n / dt, "unit": "cell-updates/s", "cores": nth, "kind": "port",
-            "sample": "%d full %s steps, OpenMP over tracers" % (n, what)}
+def cpu_info():
+    """(model name, logical CPUs, CPUs this process may run on)"""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
+
+
+def cpu_baselines(cases, step, seconds, what):
+    """The oracle (the FP64 C restatement of the reference step) on the
+    host: every usable core (OpenMP over tracers, at most 16 threads) on
+    the whole step, then one core on the first eighth of the tracers
+    (the tracer-independent part of a step is then a larger share, so that
+    rate is a lower bound); about seconds / 2 each.  step(case, threads)
+    runs one oracle step of one case."""
+    model, ncpu, avail = cpu_info()
+    ntr = max(c.ntr for c in cases)
+    nth = max(1, min(16, avail))
+
+    def sample(threads, m, budget):
+        saved = [(c.wc.ntbdy, c.wc.ntdif_v) for c in cases]
+        try:
+            for c in cases:
+                c.wc.ntbdy = min(c.wc.ntbdy, m)
+                c.wc.ntdif_v = min(c.wc.ntdif_v, m)
+            cu = float(sum(c.nwet * c.wc.ntbdy for c in cases))
+            t = time.time()
+            n = 0
+            while True:
+                for c in cases:
+                    step(c, threads)
+                n += 1
+                if time.time() - t > budget or n >= 50:
+                    break
+            dt = time.time() - t
+        finally:
+            for c, (a, b) in zip(cases, saved):
+                c.wc.ntbdy, c.wc.ntdif_v = a, b
+        return cu * n / dt, n
+
+    used = min(nth, ntr)
+    v_all, n_all = sample(nth, ntr, seconds / 2)
+    m1 = max(1, ntr // 8)
+    v_one, n_one = sample(1, m1, seconds / 2)
+    return {"value": v_all, "unit": "cell-updates/s", "cores": used, "kind": "port",
+            "sample": "%d %s steps, all tracers, OpenMP over tracers on %d threads"
+                      % (n_all, what, used),
+            "single_core": {"value": v_one, "unit": "cell-updates/s", "cores": 1,
+                            "sample": "%d %s steps of the first %d tracers, one thread"
+                                      % (n_one, what, m1)},
+            "nproc": ncpu, "usable_cpus": avail, "cpu_model": model}
+
+
+def accuracy(got, ref, m_got, m_got0, m_ref, m_ref0, steps, floor=1e-3):
+    """Error of a form against the FP64 plain form after `steps` steps.
+    max_rel_err (the bar): max |got - ref| over the tracer's largest |ref|;
+    max_pointwise_rel_err: max |got - ref| / |ref| over the cells holding at
+    least floor times the tracer's largest |ref|; mass drift of the form
+    against its own start and against the FP64 form's mass."""
+    span = np.maximum(np.abs(ref).max(axis=1), 1e-30)
+    err = np.abs(got - ref).max(axis=1) / span
+    rms = np.sqrt(((got - ref) ** 2).mean(axis=1)) / span
+    big = np.abs(ref) >= floor * span[:, None]
+    pw = np.where(big, np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300), 0.0).max(axis=1)
+    return {"steps": steps, "max_rel_err": float(err.max()),
+            "max_pointwise_rel_err": float(pw.max()), "pointwise_floor": floor,
+            "rms_rel_err": float(rms.max()),
+            "mass_drift": float((np.abs(m_got - m_got0) / np.abs(m_got0)).max()),
+            "mass_vs_fp64": float((np.abs(m_got - m_ref) / np.abs(m_ref)).max()),
+            "mass_drift_fp64": float((np.abs(m_ref - m_ref0) / np.abs(m_ref0)).max()),
+            "worst_tracer": int(np.argmax(err))}
 
 
 def bench_ffsl(args):
@@ -372,10 +437,8 @@ def bench_ffsl(args):
            HBM_PEAK_GBS}
     if not args.no_cpu:
         from oracle import oracle as O
-        nth = min(16, os.cpu_count() or 1)
         O.set_dz(case.gp, case.wp, case.pp)
-        out["cpu_baseline"] = _cpu_baseline(lambda: O.step(case, nth), cu, args.cpu_seconds,
-                                            nth, "c3 FFSL")
+        out["cpu_baseline"] = cpu_baselines([case], O.step, args.cpu_seconds, "c3 FFSL")
     print(json.dumps(out))
 
 
@@ -424,10 +487,67 @@ def bench_compas(args):
            1e9 / HBM_PEAK_GBS}
     if not args.no_cpu:
         from oracle import oracle_us as OU
-        nth = min(16, os.cpu_count() or 1)
         OU.set_dz(case.gp, case.wp, case.pp)
-        out["cpu_baseline"] = _cpu_baseline(lambda: OU.step(case, nth), cu, args.cpu_seconds,
-                                            nth, "c4 COMPAS")
+        out["cpu_baseline"] = cpu_baselines([case], OU.step, args.cpu_seconds, "c4 COMPAS")
+    print(json.dumps(out))
+
+
+def bench_c2(args):
+    """The 200x300x30 curvilinear case (10 tracers, QUICKEST, dt = 120 s):
+    the FP32 precision form against the FP64 plain form, rate of each and
+    the precision form's error and mass drift after accuracy_steps (at
+    most 500) steps."""
+    from ems_amd import _lib, configs
+    from ems_amd import tracer as T
+    L = _lib.lib()
+    case = configs.c2()
+    L.set_dz(case.gp, case.wp, case.pp)
+    if _lib.last_error()[0]:
+        raise RuntimeError("set_dz: %s" % (_lib.last_error(),))
+    L.emstr_forget_window(case.gp)
+    ev = HipEvents()
+    K, W = args.steps, args.warmup
+    nwet, ntr = case.nwet, case.ntr
+    cu = float(nwet) * ntr
+    p = T.Context(case, T.PLAIN_F64)
+    p.upload_fields()
+    p.vel_w()
+    p.upload()
+    q = T.Context(case, T.PREC_F32)
+    q.upload()
+    mp0, mq0 = p.mass(), q.mass()
+    res = {}
+    for name, c in (("plain", p), ("precision", q)):
+        c.step(W)
+        res[name] = timed_steps(c, ev, K) / K
+        print("bench: c2 %s %.4f ms/step" % (name, res[name]), file=sys.stderr, flush=True)
+    S_ACC = max(W + K, min(args.accuracy_steps, 500))
+    for c in (p, q):
+        c.step(S_ACC - (W + K))
+    mp, mq = p.mass(), q.mass()
+    p.download()
+    ref = case.tracers()
+    q.download()
+    acc = accuracy(case.tracers(), ref, mq, mq0, mp, mp0, S_ACC)
+    p.close()
+    q.close()
+    ms = res["precision"]
+    out = {"metric": "tracer cell-updates/s on 1 GPU (C2, FP32 precision form vs FP64 plain)",
+           "value": cu / (ms * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
+           "warmup": W, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded c2 generator, seed 2)",
+           "config": {"workload": "c2: 200x300x30 curvilinear basin, 30% land, 5-200 m, 10 "
+                                  "tracers, QUICKEST, dt=120 s",
+                      "n3_wet": nwet, "ntr": ntr, "form": "precision FP32 (FP64 budgets, "
+                      "refinement pass, layer thicknesses)"},
+           "fp64_plain": {"value": cu / (res["plain"] * 1e-3), "ms_per_step": res["plain"]},
+           "ratio_fp32_over_fp64": res["plain"] / ms,
+           "accuracy": acc}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.set_dz(case.gp, case.wp, case.pp)
+        out["cpu_baseline"] = cpu_baselines([case], O.step, args.cpu_seconds, "c2")
     print(json.dumps(out))
 
 
@@ -499,13 +619,48 @@ def bench_ensemble(args):
            "ensemble_speedup": sep_ms / ms}
     if tuned:
         out["autotune"] = tuned
+    # per-window accuracy: fresh FP64 plain contexts and a fresh ensemble of
+    # batched contexts from the same initial state, S steps each
+    S_ACC = max(2, min(args.accuracy_steps, 200) // 2 * 2)
+    errs, pws, drifts, vs64 = [], [], [], []
+    plains, fresh = [], []
+    for c in doms:
+        p = T.Context(c, T.PLAIN_F64)
+        p.upload()
+        plains.append(p)
+        b = T.Context(c, T.BATCH_F32)
+        b.upload()
+        fresh.append(b)
+    m0 = [b.mass() for b in fresh]
+    ens = T.Ensemble(fresh)
+    ens.step(S_ACC)
+    ens.sync()
+    ens.close()
+    for c, p, b, mb0 in zip(doms, plains, fresh, m0):
+        p.step(S_ACC)
+        mp = p.mass()
+        p.download()
+        ref = c.tracers()
+        mb = b.mass()
+        b.download()
+        a = accuracy(c.tracers(), ref, mb, mb0, mp, mp, S_ACC)
+        errs.append(a["max_rel_err"])
+        pws.append(a["max_pointwise_rel_err"])
+        drifts.append(a["mass_drift"])
+        vs64.append(a["mass_vs_fp64"])
+        p.close()
+        b.close()
+    out["accuracy"] = {"steps": S_ACC, "max_rel_err": max(errs),
+                       "max_pointwise_rel_err": max(pws), "pointwise_floor": 1e-3,
+                       "mass_drift": max(drifts), "mass_vs_fp64": max(vs64),
+                       "per_window": {"max_rel_err": [float("%.3g" % e) for e in errs],
+                                      "mass_drift": [float("%.3g" % e) for e in drifts],
+                                      "stiff": [bool(getattr(c, "stiff", False)) for c in doms]}}
     if not args.no_cpu:
         from oracle import oracle as O
-        nth = min(16, os.cpu_count() or 1)
         for c in doms:
             O.set_dz(c.gp, c.wp, c.pp)
-        out["cpu_baseline"] = _cpu_baseline(lambda: [O.step(c, nth) for c in doms], cu,
-                                            args.cpu_seconds, nth, "c5 (64 windows)")
+        out["cpu_baseline"] = cpu_baselines(doms, O.step, args.cpu_seconds, "c5 (64 windows)")
     print(json.dumps(out))
 
 
