@@ -19,8 +19,9 @@ build runs on one GPU.
   --workload c4-compas              the COMPAS path on the 350x350 hexagon
                                     mesh, 50 layers, 60 tracers, van Leer
                                     (plain FP64 form)
-  --workload c4-compas-ffsl         the same mesh with COMPAS FFSL (plain
-                                    FP64 form)
+  --workload c4-compas-ffsl         the same mesh with COMPAS FFSL: the FP32
+                                    form beside the FP64 plain form, error
+                                    and drift over 500 steps
   --workload c5-ensemble            the 64-window set in the batched FP32
                                     form, one ensemble hipGraph, with the
                                     per-window error and mass drift against
@@ -443,9 +444,12 @@ def bench_ffsl(args):
 
 
 def bench_compas(args):
-    """The COMPAS path (plain FP64 form) on the C4 hexagon mesh, van Leer
-    or FFSL (streamlines, least squares, transverse terms)."""
+    """The COMPAS path on the C4 hexagon mesh: van Leer (FP64 plain form),
+    or FFSL (streamlines, least squares, transverse terms) in the FP32 form
+    beside the FP64 plain form, with the FP32 form's error and mass drift
+    against the FP64 form after accuracy_steps (at most 500) steps."""
     from ems_amd import mesh as MS, structs_us as U
+    from oracle import oracle_us as OU
     t0 = time.time()
     ffsl = args.workload == "c4-compas-ffsl"
     case = MS.c4(seed=args.seed, trasc=U.FFSL if ffsl else U.VANLEER)
@@ -459,34 +463,63 @@ def bench_compas(args):
     case.w_from_continuity()
     ev = HipEvents()
     K, W = min(args.steps, 50), min(args.warmup, 5)
-    c = MS.Context(case)
-    c.upload()
-    c.step(W)
-    c.sync()
-    a, b = ev.event(), ev.event()
-    ev.record(a, c.stream)
-    c.step(K)
-    ev.record(b, c.stream)
-    ms = ev.elapsed_ms(a, b) / K
-    c.sync()
-    c.close()
     nwet, ntr, ne = case.nwet, case.ntr, case.g.v3_e1
     cu = float(nwet) * ntr
-    out = {"metric": "tracer cell-updates/s on 1 GPU (COMPAS, FP64 plain)",
-           "value": cu / (ms * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
-           "warmup": W, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64",
+    forms = [("plain", MS.PLAIN_F64)] + ([("fp32", MS.BATCH_F32)] if ffsl else [])
+    ctxs, ms, m0 = {}, {}, {}
+    flag0 = case.wc.flag
+    for name, form in forms:
+        # each context starts from the window's FFSL toggle state
+        case.wc.flag = flag0
+        c = MS.Context(case, form)
+        c.upload()
+        m0[name] = np.array([OU.tmass(case, n) for n in range(ntr)])
+        c.step(W)
+        c.sync()
+        a, b = ev.event(), ev.event()
+        ev.record(a, c.stream)
+        c.step(K)
+        ev.record(b, c.stream)
+        ms[name] = ev.elapsed_ms(a, b) / K
+        c.sync()
+        ctxs[name] = (c, case.wc.flag)
+        print("bench: c4 %s %.4f ms/step" % (name, ms[name]), file=sys.stderr, flush=True)
+    head = "fp32" if ffsl else "plain"
+    out = {"metric": "tracer cell-updates/s on 1 GPU (COMPAS, %s)"
+                     % ("FFSL, FP32 form vs FP64 plain" if ffsl else "van Leer, FP64 plain"),
+           "value": cu / (ms[head] * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
+           "warmup": W, "ms_per_step": ms[head], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32" if ffsl else "f64",
            "data": "synthetic (seeded C4 hexagon generator, seed %d)" % args.seed,
            "config": {"workload": "c4 mesh: 350x350 jittered hexagons, 1.5 km edges, 50 "
                                   "layers, 3-400 m, %d tracers, dt=180 s, %s"
-                                  % (ntr, "FFSL (FP64 plain)" if ffsl else "van Leer"),
+                                  % (ntr, "FFSL (FP32 form: FP32 per-tracer FFSL work over "
+                                     "FP64 maps, streamlines and state)"
+                                     if ffsl else "van Leer"),
                       "n3_wet": nwet, "wet_edges": ne, "ntr": ntr},
-           # per step: every tracer read and written once (plain FP64 in and
-           # out), w, Kz, dz per cell and u1, u1flux3d per edge
-           "step_hbm_fraction": (nwet * ntr * 16 + nwet * 24 + ne * 16) / (ms * 1e-3) /
-           1e9 / HBM_PEAK_GBS}
+           # per step: every tracer read and written once (FP64 state in both
+           # forms), w, Kz, dz per cell and u1, u1flux3d per edge
+           "step_hbm_fraction": (nwet * ntr * 16 + nwet * 24 + ne * 16) /
+           (ms[head] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if ffsl:
+        out["fp64_plain"] = {"value": cu / (ms["plain"] * 1e-3), "ms_per_step": ms["plain"]}
+        out["ratio_fp32_over_fp64"] = ms["plain"] / ms["fp32"]
+        # both carried on to the same step count from the same start
+        S_ACC = max(W + K, min(args.accuracy_steps, 500))
+        res = {}
+        for name in ("plain", "fp32"):
+            c, flag = ctxs[name]
+            case.wc.flag = flag
+            c.step(S_ACC - (W + K))
+            c.sync()
+            c.download()
+            res[name] = (case.tracers(), np.array([OU.tmass(case, n) for n in range(ntr)]))
+        out["accuracy"] = accuracy(res["fp32"][0], res["plain"][0], res["fp32"][1], m0["fp32"],
+                                   res["plain"][1], m0["plain"], S_ACC)
+    for c, _ in ctxs.values():
+        c.close()
+    LU.emstr_us_forget_window(case.gp)
     if not args.no_cpu:
-        from oracle import oracle_us as OU
         OU.set_dz(case.gp, case.wp, case.pp)
         out["cpu_baseline"] = cpu_baselines([case], OU.step, args.cpu_seconds, "c4 COMPAS")
     print(json.dumps(out))

@@ -183,8 +183,9 @@ __device__ __forceinline__ void us_edge_fluxes(const emstr_usdev &D, int e, int 
 
 /* dtracer of the cell at layer k of column col (tracers.c:949-959); the
  * edge fluxes come from fxn ([nz][ne2], k_usflux) */
+template <typename T>
 __device__ double us_dtracer(const emstr_usdev &D, int col, int k,
-                             const double *__restrict__ fxn, double dtu) {
+                             const T *__restrict__ fxn, double dtu) {
   const emstr_dev &d = D.d;
   const int ncol = (int)d.plane;
   double dtr = 0.0;
@@ -192,7 +193,7 @@ __device__ double us_dtracer(const emstr_usdev &D, int col, int k,
     const int e = D.ce[j * ncol + col];
     const int other = (D.ec1[e] == col) ? D.ec2[e] : D.ec1[e];
     double F = 0.0;
-    if (wetk(d.kb, other, k)) F = fxn[(ll)k * D.ne2 + e];
+    if (wetk(d.kb, other, k)) F = (double)fxn[(ll)k * D.ne2 + e];
     dtr += ((double)D.esc[j * ncol + col] * F * dtu);
   }
   return dtr;
@@ -328,16 +329,17 @@ k_usfz(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ wdn
 }
 
 /* new value of the sub-surface cell at layer k (tracers.c:1000-1016) */
+template <typename T>
 __device__ double us_subsurface(const emstr_usdev &D, const UsRk &w, int col, int k,
-                                const UsStage &sa, const double *t, const double *fxn,
-                                const double *fzn, double dtu, int adv) {
+                                const UsStage &sa, const double *t, const T *fxn,
+                                const T *fzn, double dtu, int adv) {
   const emstr_dev &d = D.d;
   const ll idx = (ll)k * d.plane + col;
   const double dtr = us_dtracer(D, col, k, fxn, dtu);
   double FzB = 0.0, FzT = 0.0;
   if (adv) {
-    FzB = fzn[idx];
-    FzT = fzn[idx + d.plane];
+    FzB = (double)fzn[idx];
+    FzT = (double)fzn[idx + d.plane];
   }
   const double grad = -((dtr / d.cellarea[col] + (FzT - FzB)) / d.dz[idx]);
   if (w.mode == 2) return us_rk5(w, 1.0, 1.0, sa.t2[idx], sa.t3[idx], sa.g3[idx], t[idx], grad);
@@ -346,11 +348,11 @@ __device__ double us_subsurface(const emstr_usdev &D, const UsRk &w, int col, in
 
 #define US_MAXNPE 8
 
-template <int SCH>
+template <int SCH, typename T>
 __global__ void __launch_bounds__(256)
 k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict__ tin,
-        double *__restrict__ tout, const double *__restrict__ fx,
-        const double *__restrict__ fz, const ll *__restrict__ wdn, int nwet, double dtu,
+        double *__restrict__ tout, const T *__restrict__ fx,
+        const T *__restrict__ fz, const ll *__restrict__ wdn, int nwet, double dtu,
         UsRk w, const double *__restrict__ t2in, const double *__restrict__ t3in,
         double *__restrict__ g3in) {
   const emstr_dev &d = D.d;
@@ -402,11 +404,11 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
         tout[n * N3 + idx] = t[idx];
         continue;
       }
-      const double *fxn = fx + n * NE3;
+      const T *fxn = fx + n * NE3;
       /* dtracer (tracers.c:949-959) */
       double fv[US_MAXNPE];
 #pragma unroll
-      for (int j = 0; j < US_MAXNPE; j++) fv[j] = (j < npw) ? fxn[eo[j]] : 0.0;
+      for (int j = 0; j < US_MAXNPE; j++) fv[j] = (j < npw) ? (double)fxn[eo[j]] : 0.0;
       double dtr = 0.0;
 #pragma unroll
       for (int j = 0; j < US_MAXNPE; j++) {
@@ -415,8 +417,8 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
       }
       double FzB = 0.0, FzT = 0.0;
       if (d.advect[n]) {
-        FzB = fz[n * N3 + idx];
-        FzT = fz[n * N3 + idx + P];
+        FzB = (double)fz[n * N3 + idx];
+        FzT = (double)fz[n * N3 + idx + P];
       }
       const double grad = -((dtr / area + (FzT - FzB)) / dzc);
       double r;
@@ -443,7 +445,7 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
   const int ktop = d.nz - 1;
   for (int n = 0; n < d.ntr; n++) {
     const double *t = tin + n * N3, *t0 = t0in + n * N3;
-    const double *fxn = fx + n * NE3;
+    const T *fxn = fx + n * NE3;
     double *o = tout + n * N3;
     UsStage sa = {t0, nullptr, nullptr, nullptr};
     if (w.mode) {
@@ -462,7 +464,7 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
     } else {
       /* surf_concrk (tracers.c:3085-3260) and runge_kutta; the mass at the
        * start of the sub-step (stage 0's surf_concrk) from its buffer */
-      const double FzB = fz[n * N3 + idx];
+      const double FzB = (double)fz[n * N3 + idx];
       const double top = -FzB * area;
       int c = k;
       double waterbot = (c == kbc) ? botz : lay[c];
@@ -601,8 +603,8 @@ static void us_adv_launch(const emstr_usdev *D, const double *const *ts, const d
                                                        anydif, ult);
   if (d->nwet) {
     k_usfz<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, tin, d->wdn, d->nwet, fz, dtu, ult);
-    k_usadv<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, ts[0], tin, tout, fx, fz, d->wdn,
-                                                         d->nwet, dtu, w, ts[1], ts[2], g3);
+    k_usadv<SCH, double><<<(d->nwet + 255) / 256, 256, 0, st>>>(
+        *D, ts[0], tin, tout, fx, fz, d->wdn, d->nwet, dtu, w, ts[1], ts[2], g3);
   }
 }
 
@@ -662,7 +664,19 @@ extern "C" int emstr_us_launch_update(const emstr_usdev *D, const double *const 
   const emstr_dev *d = &D->d;
   const UsRk w = us_rkargs(rkw, last, mode);
   if (d->nwet)
-    k_usadv<VANLEER><<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
+    k_usadv<VANLEER, double><<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
         *D, ts[0], tin, tout, fx, fz, d->wdn, d->nwet, dtu, w, ts[1], ts[2], g3);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+/* the same from the FP32 form's fluxes (FP64 state), one stage */
+extern "C" int emstr_us_launch_update32(const emstr_usdev *D, const double *t0, const double *tin,
+                                        double *tout, const float *fx, const float *fz,
+                                        double dtu, const double *rkw, void *stream) {
+  const emstr_dev *d = &D->d;
+  const UsRk w = us_rkargs(rkw, 1, 0);
+  if (d->nwet)
+    k_usadv<VANLEER, float><<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
+        *D, t0, tin, tout, fx, fz, d->wdn, d->nwet, dtu, w, nullptr, nullptr, nullptr);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -35,7 +35,12 @@
  * The cell update, surf_concrk and tr_bounds are k_usadv's.  k_fflsq
  * runs a wave per cell with a lane per tracer over tracer rows written by
  * k_ffmod; the other per-tracer kernels run one thread per cell or edge
- * over all tracers.
+ * over all tracers.  The per-tracer kernels are templates on the type T
+ * of their intermediates (tr_mod, slopes and beta, end values, tr_moc,
+ * fluxes) and of the arithmetic on tracer values: double for the plain
+ * form (the reference's operations and order), float for the FP32 form,
+ * whose tracer state stays FP64 and whose maps, streamlines and geometry
+ * are the plain form's (FP64, the same kernels).
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -49,6 +54,8 @@ typedef long long ll;
 
 __device__ __forceinline__ double fmx(double a, double b) { return (a > b) ? a : b; }
 __device__ __forceinline__ double fmn(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ float fmx(float a, float b) { return (a > b) ? a : b; }
+__device__ __forceinline__ float fmn(float a, float b) { return (a < b) ? a : b; }
 
 __device__ __forceinline__ bool ffwet(const int *__restrict__ kb, int col, int k) {
   return col >= 0 && k >= kb[col];
@@ -277,17 +284,18 @@ k_ffline(emstr_usdev D, double dtu, const ll *__restrict__ edn, int nesp,
  * consecutive wet centres, reads each tracer along the centres and writes
  * the rows through LDS.  Tracers that are not advected hold 0. */
 #define FM_C 32 /* wet centres per block */
+template <typename T>
 __global__ void __launch_bounds__(256)
 k_ffmod(emstr_usdev D, const double *__restrict__ tin, const double *__restrict__ nw,
-        const int *__restrict__ clc, const double *__restrict__ crc, double *__restrict__ tq) {
-  __shared__ double S[FM_C][65];
+        const int *__restrict__ clc, const double *__restrict__ crc, T *__restrict__ tq) {
+  __shared__ T S[FM_C][65];
   const emstr_dev &d = D.d;
   const int t = threadIdx.x, c = t & (FM_C - 1), g = t / FM_C;
   const int q0 = blockIdx.x * FM_C, q = q0 + c;
   const int nc = min(FM_C, d.nwet - q0);
   const ll P = d.plane, N3 = d.n3, ts = D.tqs;
   ll ik = 0, iz = 0, ic = 0;
-  double crf = 0.0;
+  T crf = 0;
   if (q < d.nwet) {
     const ll idx = d.wdn[q];
     const int col = (int)(idx % P), k = (int)(idx / P);
@@ -295,7 +303,7 @@ k_ffmod(emstr_usdev D, const double *__restrict__ tin, const double *__restrict_
     const int kk = (ks >= 0 && k > ks) ? ks : k; /* above the surface: its value */
     ik = (ll)kk * P + col;
     const double vel = nw[ik];
-    crf = crc[ik];
+    crf = (T)crc[ik];
     const int cl = clc[ik];
     const int zp = (vel < 0.0) ? cl : ffzm1(cl, kb);
     const int ci = (vel < 0.0) ? ffzm1(zp, kb) : cl;
@@ -308,13 +316,13 @@ k_ffmod(emstr_usdev D, const double *__restrict__ tin, const double *__restrict_
 #pragma unroll 2
       for (int j = g; j < tn; j += 256 / FM_C) {
         const int n = n0 + j;
-        double r = 0.0;
+        T r = 0;
         if (d.in_tbdy[n] && d.advect[n]) {
           const double *tt = tin + n * N3;
-          const double tc = tt[ic], tk = tt[ik], tz = tt[iz]; /* iz in range */
-          r = FF_TF * tc + (1.0 - FF_TF) * tk;
-          const double rz = r + FF_TF * (crf * (tz - tc));
-          r = (crf > 0.) ? rz : r;
+          const T tc = (T)tt[ic], tk = (T)tt[ik], tz = (T)tt[iz]; /* iz in range */
+          r = (T)FF_TF * tc + (T)(1.0 - FF_TF) * tk;
+          const T rz = r + (T)FF_TF * (crf * (tz - tc));
+          r = (crf > 0) ? rz : r;
         }
         S[c][j] = r;
       }
@@ -337,6 +345,9 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 /* lane l's value of a wave-uniform broadcast */
 __device__ __forceinline__ int rdl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rdl(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 __device__ __forceinline__ double rdl(double v, int l) {
   const long long b = __double_as_longlong(v);
   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
@@ -354,9 +365,10 @@ __device__ __forceinline__ double rdl(double v, int l) {
  * takes LS_C consecutive centres and writes w1, w2, beta and w0 = tr_mod
  * as [4][tracer][cell] planes through LDS. */
 #define LS_C 8 /* wet centres per block */
+template <typename T>
 __global__ void __launch_bounds__(256)
-k_fflsq(emstr_usdev D, const double *__restrict__ tq, double *__restrict__ lq) {
-  __shared__ double S[4][64][LS_C + 1];
+k_fflsq(emstr_usdev D, const T *__restrict__ tq, T *__restrict__ lq) {
+  __shared__ T S[4][64][LS_C + 1];
   const emstr_dev &d = D.d;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -388,31 +400,31 @@ k_fflsq(emstr_usdev D, const double *__restrict__ tq, double *__restrict__ lq) {
        * entry 0: no divergent loads; only entries are broadcast) */
       const int ls = (lane < ns) ? lane : 0, lr = (lane < nr) ? lane : 0;
       const int e_s = D.lsq_s[o0 + ls];
-      const double e_b1 = D.lsq_B[2 * (ll)(o0 + ls)], e_b2 = D.lsq_B[2 * (ll)(o0 + ls) + 1];
+      const T e_b1 = (T)D.lsq_B[2 * (ll)(o0 + ls)], e_b2 = (T)D.lsq_B[2 * (ll)(o0 + ls) + 1];
       const int e_rq = D.lr_q[r0 + lr];
       /* vertex offsets and ranges (padded to EMSTR_FF_NPE) */
       const double *xy = D.vxy + (ll)col * (2 * EMSTR_FF_NPE);
       const unsigned *vw = D.vsl + (ll)q * EMSTR_FF_NPE;
-      double vx[EMSTR_FF_NPE], vy[EMSTR_FF_NPE];
+      T vx[EMSTR_FF_NPE], vy[EMSTR_FF_NPE];
       unsigned wd[EMSTR_FF_NPE];
 #pragma unroll
       for (int j = 0; j < EMSTR_FF_NPE; j++) {
-        vx[j] = xy[2 * j];
-        vy[j] = xy[2 * j + 1];
+        vx[j] = (T)xy[2 * j];
+        vy[j] = (T)xy[2 * j + 1];
         wd[j] = vw[j];
       }
       /* rows of the distinct centres; lanes past the last tracer read
        * tracer 0 and are not written out */
-      double t[EMSTR_FF_ROWS];
+      T t[EMSTR_FF_ROWS];
 #pragma unroll
       for (int s = 0; s < EMSTR_FF_ROWS; s++) {
-        const double *row = tq + (ll)rdl(e_rq, s) * ts;
-        t[s] = (s < nr) ? row[nl] : 0.0;
+        const T *row = tq + (ll)rdl(e_rq, s) * ts;
+        t[s] = (s < nr) ? row[nl] : (T)0;
       }
-      const double w0 = t[0];
-      double a1 = 0.0, a2 = 0.0, beta = 1.0;
+      const T w0 = t[0];
+      T a1 = 0, a2 = 0, beta = 1;
       for (int i = 0; i < ns; i++) {
-        const double tv = t[rdl(e_s, i)];
+        const T tv = t[rdl(e_s, i)];
         a1 += (tv * rdl(e_b1, i));
         a2 += (tv * rdl(e_b2, i));
       }
@@ -421,22 +433,22 @@ k_fflsq(emstr_usdev D, const double *__restrict__ tq, double *__restrict__ lq) {
       for (int j = 0; j < EMSTR_FF_NPE; j++) {
         if (j >= npe) break;
         unsigned word = wd[j];
-        double vmin = __builtin_huge_val(), vmax = -__builtin_huge_val();
+        T vmin = (T)__builtin_huge_val(), vmax = -(T)__builtin_huge_val();
         for (int i = 0; i < 4; i++, word >>= 8) {
           const int sl = (int)(word & 0xffu);
           if (sl == 0xff) break;
-          const double tv = t[sl];
+          const T tv = t[sl];
           vmin = fmn(vmin, tv);
           vmax = fmx(vmax, tv);
         }
-        const double val = w0 + 1.0 * (a1 * vx[j] + a2 * vy[j]);
-        const double dd = val - w0;
-        if (dd != 0.0 && val > vmax) beta = fmx(fmn((vmax - w0) / dd, beta), 0.0);
-        if (dd != 0.0 && val < vmin) beta = fmx(fmn((vmin - w0) / dd, beta), 0.0);
+        const T val = w0 + (T)1 * (a1 * vx[j] + a2 * vy[j]);
+        const T dd = val - w0;
+        if (dd != (T)0 && val > vmax) beta = fmx(fmn((vmax - w0) / dd, beta), (T)0);
+        if (dd != (T)0 && val < vmin) beta = fmx(fmn((vmin - w0) / dd, beta), (T)0);
       }
       S[0][lane][cc] = a1;
       S[1][lane][cc] = a2;
-      S[2][lane][cc] = fmn(fmx(beta, 0.0), 1.0);
+      S[2][lane][cc] = fmn(fmx(beta, (T)0), (T)1);
       S[3][lane][cc] = w0;
     }
     __syncthreads();
@@ -446,7 +458,7 @@ k_fflsq(emstr_usdev D, const double *__restrict__ tq, double *__restrict__ lq) {
       for (int h = 0; h < 2; h++) {
         const int j = jw + 32 * h, nn = n0 + j;
         if (j < tn && d.in_tbdy[nn] && d.advect[nn]) {
-          double *o = lq + nn * N3 + iw;
+          T *o = lq + nn * N3 + iw;
 #pragma unroll
           for (int pl = 0; pl < 4; pl++) o[pl * NT3] = S[pl][j][ccw];
         }
@@ -457,11 +469,12 @@ k_fflsq(emstr_usdev D, const double *__restrict__ tq, double *__restrict__ lq) {
 
 /* edge fluxes (transport.c:2192-2386) and horizontal diffusion
  * (tracers.c:5438-5446) */
+template <typename T>
 __global__ void __launch_bounds__(256)
 k_ffedge(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ edn, int nesp,
          const int *__restrict__ lci, const double *__restrict__ ld,
-         const double *__restrict__ lq, double *__restrict__ fx,
-         double *__restrict__ te, int anydif, const int *__restrict__ need, int stamp) {
+         const T *__restrict__ lq, T *__restrict__ fx,
+         T *__restrict__ te, int anydif, const int *__restrict__ need, int stamp) {
   const emstr_dev &d = D.d;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nesp) return;
@@ -470,31 +483,31 @@ k_ffedge(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ e
   const int k = (int)(ie / D.ne2), e = (int)(ie % D.ne2);
   const int ci = lci[ie];
   const ll cidx = (ll)k * P + ci;
-  const double dxs = ld[ie], dys = ld[NE3 + ie], dxe = ld[2 * NE3 + ie];
-  const double dye = ld[3 * NE3 + ie], dist = ld[4 * NE3 + ie];
-  const double f = D.flux[ie];
+  const T dxs = (T)ld[ie], dys = (T)ld[NE3 + ie], dxe = (T)ld[2 * NE3 + ie];
+  const T dye = (T)ld[3 * NE3 + ie], dist = (T)ld[4 * NE3 + ie];
+  const T f = (T)D.flux[ie];
   const ll i1 = (ll)k * P + D.ec1[e], i2 = (ll)k * P + D.ec2[e];
   /* the end value is read only by the tr_moc of the centres either side */
   const bool wte = (D.ec1[e] >= 0 && need[i1] == stamp) || (D.ec2[e] >= 0 && need[i2] == stamp);
   for (int n = 0; n < d.ntr; n++) {
     if (!d.in_tbdy[n]) continue;
-    double F = 0.0;
+    T F = 0;
     if (d.advect[n]) {
-      const double w0 = lq[3 * NT3 + n * N3 + cidx];
-      const double w1 = lq[n * N3 + cidx], w2 = lq[NT3 + n * N3 + cidx];
-      const double b = lq[2 * NT3 + n * N3 + cidx];
-      const double trs = w0 + b * (w1 * dxs + w2 * dys);
-      const double tre = w0 + b * (w1 * dxe + w2 * dye);
-      const double dint = 0.0 + dist;
-      const double ntr = 0.0 + 0.5 * dist * (trs + tre);
-      F = (dint != 0.0) ? ntr / dint : 0.0;
+      const T w0 = lq[3 * NT3 + n * N3 + cidx];
+      const T w1 = lq[n * N3 + cidx], w2 = lq[NT3 + n * N3 + cidx];
+      const T b = lq[2 * NT3 + n * N3 + cidx];
+      const T trs = w0 + b * (w1 * dxs + w2 * dys);
+      const T tre = w0 + b * (w1 * dxe + w2 * dye);
+      const T dint = (T)0 + dist;
+      const T ntr = (T)0 + (T)0.5 * dist * (trs + tre);
+      F = (dint != (T)0) ? ntr / dint : (T)0;
       F *= f;
       if (wte) te[n * NE3 + ie] = tre;
     }
     if (anydif && d.diffuse[n]) {
       const double *t = tin + n * N3;
       const double csx = D.dzu[ie] * D.h1e[e] * 1.0;
-      F -= (csx * D.kh[ie] * (t[i1] - t[i2]) / D.h2e[e]);
+      F -= (T)(csx * D.kh[ie] * (t[i1] - t[i2]) / D.h2e[e]);
     }
     fx[n * NE3 + ie] = F;
   }
@@ -503,9 +516,10 @@ k_ffedge(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ e
 /* tr_moc (transport.c:2323-2330, 2412-2420 with set_tr_nograd), of the
  * centres the vertical fluxes read (need[idx] == stamp; all of them at or
  * below the surface, where the no-gradient copy is the centre itself) */
+template <typename T>
 __global__ void __launch_bounds__(256)
-k_ffmoc(emstr_usdev D, const double *__restrict__ tin, const double *__restrict__ te,
-        double *__restrict__ moc, const int *__restrict__ need, int stamp) {
+k_ffmoc(emstr_usdev D, const double *__restrict__ tin, const T *__restrict__ te,
+        T *__restrict__ moc, const int *__restrict__ need, int stamp) {
   const emstr_dev &d = D.d;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.nwet) return;
@@ -526,29 +540,30 @@ k_ffmoc(emstr_usdev D, const double *__restrict__ tin, const double *__restrict_
   }
   for (int n = 0; n < d.ntr; n++) {
     if (!d.in_tbdy[n] || !d.advect[n]) continue;
-    const double *tn = te + n * NE3;
-    double r = 0.0;
+    const T *tn = te + n * NE3;
+    T r = 0;
     for (int i = 0; i < m; i++) r += tn[el[i]];
-    if (m) r /= (double)m;
-    r = FF_TF * r + (1.0 - FF_TF) * tin[n * N3 + ik];
+    if (m) r /= (T)m;
+    r = (T)FF_TF * r + (T)(1.0 - FF_TF) * (T)tin[n * N3 + ik];
     moc[n * N3 + idx] = r;
   }
 }
 
-__device__ __forceinline__ double ffmark(int *need, ll i, int stamp) {
+template <typename T>
+__device__ __forceinline__ T ffmark(int *need, ll i, int stamp) {
   need[i] = stamp;
-  return 0.0;
+  return (T)0;
 }
 
 /* vertical fluxes (transport.c:2441-2490, ffsl_van_leer :8216-8260).
  * MARK: one pass that follows the same stencil and whole-layer walk
  * without tracer values and sets need[] = stamp at every tr_moc centre
  * the fluxes read (tr_moc is read only there) */
-template <bool MARK>
+template <bool MARK, typename T>
 __global__ void __launch_bounds__(256)
-k_fffz(emstr_usdev D, double dtu, const double *__restrict__ tin, const double *__restrict__ moc,
+k_fffz(emstr_usdev D, double dtu, const double *__restrict__ tin, const T *__restrict__ moc,
        const int *__restrict__ clf, const double *__restrict__ crf,
-       const double *__restrict__ cour, double *__restrict__ fz, int *__restrict__ err,
+       const double *__restrict__ cour, T *__restrict__ fz, int *__restrict__ err,
        int *__restrict__ need, int stamp) {
   const emstr_dev &d = D.d;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -567,31 +582,33 @@ k_fffz(emstr_usdev D, double dtu, const double *__restrict__ tin, const double *
   if (w > 0.0) cp = ffzp1(cp, kb, nz);
   const int bm1 = ffzm1(cp, kb), bm2 = ffzm1(bm1, kb), fp = ffzp1(cp, kb, nz);
   const double fr = (w == 0.0 || k == kb) ? 0.0 : cr * ffdz(d, col, cl, kb);
+  const T crT = (T)cr, frT = (T)fr;
   for (int n = 0; n < (MARK ? 1 : d.ntr); n++) {
     if (!MARK && (!d.in_tbdy[n] || !d.advect[n])) continue;
-    const double *t = tin + n * N3, *mc = moc + n * N3;
-#define FFT(L) (MARK ? 0.0 : t[(ll)(((L) < kb) ? kb : (L)) * P + col])
+    const double *t = tin + n * N3;
+    const T *mc = moc + n * N3;
+#define FFT(L) (MARK ? (T)0 : (T)t[(ll)(((L) < kb) ? kb : (L)) * P + col])
 #define FFL(L) ((ll)(((L) < kb) ? kb : (((L) > ks) ? ks : (L))) * P + col)
-#define FFM(L) (MARK ? ffmark(need, FFL(L), stamp) : mc[FFL(L)])
+#define FFM(L) (MARK ? ffmark<T>(need, FFL(L), stamp) : mc[FFL(L)])
 #define FFV(L) (usemoc ? FFM(L) : FFT(L))
     /* the stencil mirrored for w < 0: (bm2, bm1, cp) for w > 0, (fp, cp,
      * bm1) for w < 0; the reference's cp - 0.5 (1 - cr) df of
      * df = (cp - bm1) (fp - cp), 2 df / (fp - bm1) is the same value with
      * the negated quotient, to the last bit, and the same three cells are
      * read (and marked) */
-    double F = 0.0;
+    T F = 0;
     if (cr != 0.0 && (w > 0.0 || w < 0.0)) {
       const bool pos = w > 0.0;
       const int l2 = pos ? bm2 : fp, l1 = pos ? bm1 : cp, l0 = pos ? cp : bm1;
-      const double a2 = FFV(l2), a1 = FFV(l1), a0 = FFV(l0);
-      double df = (a1 - a2) * (a0 - a1);
-      df = (df > 0.0) ? 2.0 * df / (a0 - a2) : 0.0;
-      F = a1 + 0.5 * (1.0 - cr) * df;
+      const T a2 = FFV(l2), a1 = FFV(l1), a0 = FFV(l0);
+      T df = (a1 - a2) * (a0 - a1);
+      df = (df > (T)0) ? (T)2 * df / (a0 - a2) : (T)0;
+      F = a1 + (T)0.5 * ((T)1 - crT) * df;
     }
     if (w == 0.0 || k == kb) {
-      F = 0.0;
+      F = 0;
     } else {
-      F *= fr;
+      F *= frT;
       /* whole layers of the trajectory (transport.c:2462-2486) */
       int ci = (w > 0.0) ? ffzm1(k, kb) : k;
       double dz = ffdz(d, col, ci, kb);
@@ -602,13 +619,13 @@ k_fffz(emstr_usdev D, double dtu, const double *__restrict__ tin, const double *
           atomicMax(err, 2);
           break;
         }
-        F += (dz * FFM(ci));
+        F += ((T)dz * FFM(ci));
         dist -= dz;
         ci = (w < 0.0) ? ffzp1(ci, kb, nz) : ffzm1(ci, kb);
         const double dn = ffdz(d, col, ci, kb);
         if (dn > 0) dz = dn;
       }
-      if (w < 0) F *= -1.0;
+      if (w < 0) F *= (T)-1;
     }
 #undef FFT
 #undef FFL
@@ -641,6 +658,30 @@ extern "C" int emstr_us_launch_ffsl_sub(const emstr_usdev *D, double dtu, const 
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+template <typename T>
+static int ffsl_flux(const emstr_usdev *D, double dtu, const double *tin, const double *nw,
+                     const int *clc, const double *crc, const int *clf, const double *crf,
+                     const double *cour, const long long *edn, int nesp, const int *lci,
+                     const double *ld, T *tm, T *lq, T *te, T *moc, T *fx, T *fz, int anydif,
+                     int *need, int stamp, int *err, hipStream_t st) {
+  const emstr_dev *d = &D->d;
+  const int gw = (d->nwet + 255) / 256, ge = (nesp + 255) / 256;
+  if (d->nwet) {
+    k_fffz<true, T><<<gw, 256, 0, st>>>(*D, dtu, tin, moc, clf, crf, cour, fz, err, need, stamp);
+    k_ffmod<T><<<(d->nwet + FM_C - 1) / FM_C, 256, 0, st>>>(*D, tin, nw, clc, crc, tm);
+    k_fflsq<T><<<(d->nwet + LS_C - 1) / LS_C, 256, 0, st>>>(*D, tm, lq);
+  }
+  if (nesp)
+    k_ffedge<T><<<ge, 256, 0, st>>>(*D, tin, edn, nesp, lci, ld, lq, fx, te, anydif, need,
+                                    stamp);
+  if (d->nwet) {
+    k_ffmoc<T><<<gw, 256, 0, st>>>(*D, tin, te, moc, need, stamp);
+    k_fffz<false, T><<<gw, 256, 0, st>>>(*D, dtu, tin, moc, clf, crf, cour, fz, err, need,
+                                         stamp);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 extern "C" int emstr_us_launch_ffsl_flux(const emstr_usdev *D, double dtu, const double *tin,
                                          const double *nw, const int *clc, const double *crc,
                                          const int *clf, const double *crf, const double *cour,
@@ -648,19 +689,19 @@ extern "C" int emstr_us_launch_ffsl_flux(const emstr_usdev *D, double dtu, const
                                          const double *ld, double *tm, double *lq, double *te,
                                          double *moc, double *fx, double *fz, int anydif,
                                          int *need, int stamp, int *err, void *stream) {
-  const emstr_dev *d = &D->d;
-  hipStream_t st = (hipStream_t)stream;
-  const int gw = (d->nwet + 255) / 256, ge = (nesp + 255) / 256;
-  if (d->nwet) {
-    k_fffz<true><<<gw, 256, 0, st>>>(*D, dtu, tin, moc, clf, crf, cour, fz, err, need, stamp);
-    k_ffmod<<<(d->nwet + FM_C - 1) / FM_C, 256, 0, st>>>(*D, tin, nw, clc, crc, tm);
-    k_fflsq<<<(d->nwet + LS_C - 1) / LS_C, 256, 0, st>>>(*D, tm, lq);
-  }
-  if (nesp)
-    k_ffedge<<<ge, 256, 0, st>>>(*D, tin, edn, nesp, lci, ld, lq, fx, te, anydif, need, stamp);
-  if (d->nwet) {
-    k_ffmoc<<<gw, 256, 0, st>>>(*D, tin, te, moc, need, stamp);
-    k_fffz<false><<<gw, 256, 0, st>>>(*D, dtu, tin, moc, clf, crf, cour, fz, err, need, stamp);
-  }
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return ffsl_flux<double>(D, dtu, tin, nw, clc, crc, clf, crf, cour, edn, nesp, lci, ld, tm,
+                           lq, te, moc, fx, fz, anydif, need, stamp, err, (hipStream_t)stream);
+}
+
+/* the same with FP32 intermediates and arithmetic (the FP32 form) */
+extern "C" int emstr_us_launch_ffsl_flux32(const emstr_usdev *D, double dtu, const double *tin,
+                                           const double *nw, const int *clc, const double *crc,
+                                           const int *clf, const double *crf,
+                                           const double *cour, const long long *edn, int nesp,
+                                           const int *lci, const double *ld, float *tm,
+                                           float *lq, float *te, float *moc, float *fx,
+                                           float *fz, int anydif, int *need, int stamp,
+                                           int *err, void *stream) {
+  return ffsl_flux<float>(D, dtu, tin, nw, clc, crc, clf, crf, cour, edn, nesp, lci, ld, tm, lq,
+                          te, moc, fx, fz, anydif, need, stamp, err, (hipStream_t)stream);
 }

@@ -81,6 +81,12 @@ typedef struct emstr_us_ctx {
   int *d_need, ffstamp; /* tr_moc centres read, by sub-step stamp */
   int anydif, have_cols;
   int key_szc, key_sze;
+  /* the FP32 form (EMSTR_US_BATCH_F32): FP32 per-tracer intermediates
+   * and arithmetic of the FFSL fluxes over the FP64 tracer state; the
+   * maps, streamlines, geometry, update and column solve are the plain
+   * form's */
+  int f32;
+  float *f_fx, *f_fz, *f_tm, *f_lq, *f_te, *f_moc;
 } usctx;
 
 static void *dalloc(size_t bytes, int *ok) {
@@ -115,7 +121,8 @@ static void us_destroy(usctx *c) {
                 D->uy, D->n1x, D->n1y, D->t2x, D->t2y, D->ev, D->vxy, D->eord, D->lsq_off, D->lsq_s, D->lr_off, D->lr_q, D->vsl, D->lsq_B, D->u2, D->wtop,
                 D->wbot, D->uc, D->vc, D->gu0, D->gv0, D->gue, D->gve, c->d_nw, c->d_crc,
                 c->d_crf, c->d_cour, c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc,
-                c->d_clc, c->d_clf, c->d_lci, c->d_fferr, c->d_need};
+                c->d_clc, c->d_clf, c->d_lci, c->d_fferr, c->d_need, c->f_fx, c->f_fz,
+                c->f_tm, c->f_lq, c->f_te, c->f_moc};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->cs2col, c->col2cs, c->h_kb, c->h_ks, c->h_kold, c->h_knew, c->h_cols,
@@ -1136,6 +1143,26 @@ static int us_ffsl_alloc(usctx *c) {
   return 0;
 }
 
+/* the FP32 form's fluxes and FFSL intermediates */
+static int us_f32_alloc(usctx *c) {
+  emstr_dev *d = &c->D.d;
+  const size_t N3 = (size_t)d->n3, NE3 = (size_t)c->ne2 * d->nz;
+  const size_t nt = (size_t)(d->ntr > 0 ? d->ntr : 1);
+  int ok = 1;
+  if (c->f_fx) return 0;
+  c->f_fx = (float *)dalloc(nt * NE3 * sizeof(float), &ok);
+  c->f_fz = (float *)dalloc((nt * N3 + (size_t)c->ncol) * sizeof(float), &ok);
+  c->f_tm = (float *)dalloc(((size_t)d->nwet + 1) * ((nt + 15) / 16 * 16) * sizeof(float), &ok);
+  c->f_lq = (float *)dalloc(4 * nt * N3 * sizeof(float), &ok);
+  c->f_te = (float *)dalloc(nt * NE3 * sizeof(float), &ok);
+  c->f_moc = (float *)dalloc(nt * N3 * sizeof(float), &ok);
+  if (!ok) {
+    emstr_set_error(E_DEVICE, -1, "FP32 form buffers (%zu tracers)", nt);
+    return E_DEVICE;
+  }
+  return 0;
+}
+
 /*-------------------------------------------------------------------*/
 /* the step                                                          */
 /*-------------------------------------------------------------------*/
@@ -1179,6 +1206,11 @@ static int us_adv(usctx *c) {
   int itermax = 20, first = 1, r0;
   const int ffsl = D->ffsl;
   if (!c->have_cols) return emstr_set_error(E_ARG, -1, "no set_dz data on the device");
+  if (c->f32) {
+    if (!ffsl || wc->rkstage != 1)
+      return emstr_set_error(E_UNSUP, -1, "FP32 form: FFSL with one Runge-Kutta stage only");
+    if ((r0 = us_f32_alloc(c))) return r0;
+  }
   if (ffsl) {
     /* tracers.c:567-590: centred w, ffsl_init */
     itermax = 200;
@@ -1267,7 +1299,17 @@ static int us_adv(usctx *c) {
         const int last = (tc == nrk - 1);
         const double *ts[3] = {c->tr[in], (sin[2] >= 0) ? c->tr[sin[2]] : NULL,
                                (sin[3] >= 0) ? c->tr[sin[3]] : NULL};
-        if (ffsl) {
+        if (c->f32) {
+          /* the FP32 form (one stage): the same maps, FP32 fluxes */
+          if (emstr_us_launch_ffsl_flux32(D, u, c->tr[src], c->d_nw, c->d_clc, c->d_crc,
+                                          c->d_clf, c->d_crf, c->d_cour, c->d_edn, c->nesp,
+                                          c->d_lci, c->d_ld, c->f_tm, c->f_lq, c->f_te,
+                                          c->f_moc, c->f_fx, c->f_fz, c->anydif, c->d_need,
+                                          us_ffstamp(c), c->d_fferr, c->st) ||
+              emstr_us_launch_update32(D, c->tr[in], c->tr[src], c->tr[dst], c->f_fx,
+                                       c->f_fz, u, w, c->st))
+            return emstr_set_error(E_DEVICE, -1, "FFSL launch");
+        } else if (ffsl) {
           if (emstr_us_launch_ffsl_flux(D, u, c->tr[src], c->d_nw, c->d_clc, c->d_crc,
                                         c->d_clf, c->d_crf, c->d_cour, c->d_edn, c->nesp,
                                         c->d_lci, c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc,
@@ -1546,6 +1588,20 @@ int emstr_us_ffsl_maps(geometry_t *window, int *cl, int *clzf, int *clzc) {
 /*-------------------------------------------------------------------*/
 emstr_us_ctx *emstr_us_create(geometry_t *g, window_t *wd, win_priv_t *wc) {
   return us_create(g, wd, wc);
+}
+
+emstr_us_ctx *emstr_us_create_form(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
+  if (form != EMSTR_US_PLAIN_F64 && form != EMSTR_US_BATCH_F32) {
+    emstr_set_error(E_ARG, -1, "unknown form %d", form);
+    return NULL;
+  }
+  if (form == EMSTR_US_BATCH_F32 && (wc->trasc != FFSL || wc->rkstage != 1)) {
+    emstr_set_error(E_UNSUP, -1, "FP32 form: FFSL with one Runge-Kutta stage only");
+    return NULL;
+  }
+  usctx *c = us_create(g, wd, wc);
+  if (c) c->f32 = (form == EMSTR_US_BATCH_F32);
+  return c;
 }
 
 void emstr_us_destroy(emstr_us_ctx *c) { us_destroy(c); }

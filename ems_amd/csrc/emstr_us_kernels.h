@@ -32,6 +32,11 @@ int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned long long *o
 int emstr_us_launch_update(const emstr_usdev *D, const double *const *ts, const double *tin,
                            double *tout, const double *fx, const double *fz, double dtu,
                            const double *rkw, int last, int mode, double *g3, void *stream);
+/* The FP32 form: one forward-Euler update of the FP64 tracers from FP32
+ * edge and vertical fluxes. */
+int emstr_us_launch_update32(const emstr_usdev *D, const double *t0, const double *tin,
+                             double *tout, const float *fx, const float *fz, double dtu,
+                             const double *rkw, void *stream);
 
 /* FFSL (emstr_us_ffsl.hip).  Per step: the centred w into nw [nz][ncol]
  * (init: vel_center_w, else the Leonard et al. choice).  Per sub-step:
@@ -56,6 +61,13 @@ int emstr_us_launch_ffsl_flux(const emstr_usdev *D, double dtu, const double *ti
                               double *tm, double *lq, double *te, double *moc, double *fx,
                               double *fz, int anydif, int *need, int stamp, int *err,
                               void *stream);
+int emstr_us_launch_ffsl_flux32(const emstr_usdev *D, double dtu, const double *tin,
+                                const double *nw, const int *clc, const double *crc,
+                                const int *clf, const double *crf, const double *cour,
+                                const long long *edn, int nesp, const int *lci, const double *ld,
+                                float *tm, float *lq, float *te, float *moc, float *fx,
+                                float *fz, int anydif, int *need, int stamp, int *err,
+                                void *stream);
 
 #ifdef __cplusplus
 }

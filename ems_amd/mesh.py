@@ -56,6 +56,8 @@ def lib():
         if hasattr(L, "emstr_us_create"):
             L.emstr_us_create.restype = C.c_void_p
             L.emstr_us_create.argtypes = [G, W, P]
+            L.emstr_us_create_form.restype = C.c_void_p
+            L.emstr_us_create_form.argtypes = [G, W, P, C.c_int]
             L.emstr_us_destroy.argtypes = [C.c_void_p]
             L.emstr_us_destroy.restype = None
             for fn in ("emstr_us_upload_fields", "emstr_us_upload_tracers",
@@ -649,14 +651,15 @@ def quads_from_case(sc):
 
 
 class Context:
-    """A device-resident COMPAS window (emstr_us_create)."""
+    """A device-resident COMPAS window (emstr_us_create_form): form PLAIN_F64
+    (the reference's FP64 operations) or BATCH_F32 (FFSL, one stage)."""
 
-    def __init__(self, case):
+    def __init__(self, case, form=0):
         L = lib()
         self.case = case
-        self.ptr = L.emstr_us_create(case.gp, case.wp, case.pp)
+        self.ptr = L.emstr_us_create_form(case.gp, case.wp, case.pp, form)
         if not self.ptr:
-            raise RuntimeError("emstr_us_create: %s (cell %d)" % last_error())
+            raise RuntimeError("emstr_us_create_form: %s (cell %d)" % last_error())
         self.stream = L.emstr_us_stream(self.ptr)
 
     def _chk(self, r, what):
@@ -681,6 +684,9 @@ class Context:
         if self.ptr:
             lib().emstr_us_destroy(self.ptr)
             self.ptr = None
+
+
+PLAIN_F64, BATCH_F32 = 0, 2  # emstr_us_create_form's forms
 
 
 def c4(seed=5, ntr=60, trasc=U.VANLEER):

@@ -242,6 +242,18 @@ void set_dz(geometry_t *window, window_t *windat, win_priv_t *wincon);
 /*-------------------------------------------------------------------*/
 typedef struct emstr_us_ctx emstr_us_ctx;
 emstr_us_ctx *emstr_us_create(geometry_t *window, window_t *windat, win_priv_t *wincon);
+/* Forms: EMSTR_US_PLAIN_F64 (emstr_us_create's: the reference's FP64
+ * operations, bit-exact), EMSTR_US_BATCH_F32 (FFSL with one Runge-Kutta
+ * stage: the per-tracer FFSL work, tr_mod, least-squares slopes and
+ * limiter, streamline means, tr_moc, edge and vertical fluxes, in FP32
+ * for every tracer of the step per launch, over the plain form's FP64
+ * maps, streamlines and geometry; the tracer state, the update and the
+ * column solve stay FP64; other settings are refused: NULL and
+ * emstr_us_last_error). */
+#define EMSTR_US_PLAIN_F64 0
+#define EMSTR_US_BATCH_F32 2
+emstr_us_ctx *emstr_us_create_form(geometry_t *window, window_t *windat, win_priv_t *wincon,
+                                   int form);
 void emstr_us_destroy(emstr_us_ctx *ctx);
 /* fields, per-column lists and dz from the host structures          */
 int emstr_us_upload_fields(emstr_us_ctx *ctx);
