@@ -348,13 +348,71 @@ __device__ double us_subsurface(const emstr_usdev &D, const UsRk &w, int col, in
 
 #define US_MAXNPE 8
 
-template <int SCH, typename T>
+/* FFSL CLIP (clip_ffsl, transport.c:4107-4138): the range a centre's new
+ * value is clamped to.  Its vertical streamline's (transport.c:2461,
+ * 2480-2486): the tracer of the face's upstream centre, then of every
+ * whole layer the trajectory crosses (the sediment ghost reads the bed
+ * value; thicknesses and walk as in the vertical flux).  Then each edge's
+ * (transport.c:4122-4127): a wet edge's is the range over the
+ * least-squares stencil of its upstream centre (transport.c:2207,
+ * 2309-2311), a land edge's is zero (its bounds are never written and
+ * were allocated zero).  min and max are the reference's macros. */
+__device__ __forceinline__ double mn_(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double mx_(double a, double b) { return (a > b) ? a : b; }
+__device__ void us_clip_range(const emstr_usdev &D, int col, int k, int n, double dtu,
+                              const double *__restrict__ tin, const double *__restrict__ bmn,
+                              const double *__restrict__ bmx, double &lo, double &hi) {
+  const emstr_dev &d = D.d;
+  const ll P = d.plane, N3 = d.n3;
+  const int kb = d.kb[col], nz = d.nz, ncol = (int)P;
+  const double *t = tin + n * N3;
+  const double w = d.w[(ll)k * P + col];
+  auto zm1 = [&](int L) { return (L >= kb) ? L - 1 : L; };
+  auto zp1 = [&](int L) { return (L < kb) ? kb : ((L < nz - 1) ? L + 1 : L); };
+  auto tv = [&](int L) { return t[(ll)((L < kb) ? kb : L) * P + col]; };
+  auto dzl = [&](int L) { return (L < kb) ? 0.0 : d.dz[(ll)L * P + col]; };
+  lo = hi = tv((w > 0.0) ? zm1(k) : k);
+  if (!(w == 0.0 || k == kb)) {
+    int ci = (w > 0.0) ? zm1(k) : k, it = 0;
+    double dz = dzl(ci), dist = fabs(w * dtu);
+    while (dz < dist) {
+      if (++it > 4 * nz + 4) break; /* reported by the vertical flux */
+      const double v = tv(ci);
+      lo = mn_(lo, v);
+      hi = mx_(hi, v);
+      dist -= dz;
+      ci = (w < 0.0) ? zp1(ci) : zm1(ci);
+      const double dn = dzl(ci);
+      if (dn > 0) dz = dn;
+    }
+  }
+  for (int j = 0; j < D.npe[col]; j++) {
+    const int e = D.ce[j * ncol + col];
+    const int other = (D.ec1[e] == col) ? D.ec2[e] : D.ec1[e];
+    if (!wetk(d.kb, other, k)) {
+      lo = mn_(lo, 0.0);
+      hi = mx_(hi, 0.0);
+    } else {
+      const int up = (D.u1[(ll)k * D.ne2 + e] > 0.0) ? D.ec2[e] : D.ec1[e];
+      const ll iu = n * N3 + (ll)k * P + up;
+      lo = mn_(lo, bmn[iu]);
+      hi = mx_(hi, bmx[iu]);
+    }
+  }
+}
+/* tr = max(min(mx, tr), mn) (transport.c:4137) */
+__device__ __forceinline__ double us_clamp(double v, double lo, double hi) {
+  return mx_(mn_(hi, v), lo);
+}
+
+template <int SCH, typename T, bool CLIP>
 __global__ void __launch_bounds__(256)
 k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict__ tin,
         double *__restrict__ tout, const T *__restrict__ fx,
         const T *__restrict__ fz, const ll *__restrict__ wdn, int nwet, double dtu,
         UsRk w, const double *__restrict__ t2in, const double *__restrict__ t3in,
-        double *__restrict__ g3in) {
+        double *__restrict__ g3in, const double *__restrict__ bmn,
+        const double *__restrict__ bmx) {
   const emstr_dev &d = D.d;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nwet) return;
@@ -428,6 +486,11 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
       else
         r = us_rk(w, 1.0, 1.0, t0in[n * N3 + idx], t[idx], grad);
       if (w.mode == 1) g3in[n * N3 + idx] = grad;
+      if (CLIP) {
+        double lo, hi;
+        us_clip_range(D, col, k, n, dtu, tin, bmn, bmx, lo, hi);
+        r = us_clamp(r, lo, hi);
+      }
       /* tr_bounds clip (tracers.c:3328-3331), after the last stage */
       if (w.last && d.mintr[n] <= 1e-5 && r < d.mintr[n]) r = d.mintr[n];
       tout[n * N3 + idx] = r;
@@ -511,6 +574,11 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
       /* tracers.c:1056-1057 */
       trs = (botz > 0.0 && d.dz[idx] < d.hmin) ? t[idx] : r;
     }
+    if (CLIP) {
+      double lo, hi;
+      us_clip_range(D, col, k, n, dtu, tin, bmn, bmx, lo, hi);
+      trs = us_clamp(trs, lo, hi);
+    }
     if (!w.last) {
       /* an intermediate stage: the layers above keep their values */
       o[idx] = trs;
@@ -521,8 +589,15 @@ k_usadv(emstr_usdev D, const double *__restrict__ t0in, const double *__restrict
      * surface thickness dz[co] */
     const double dzco = (kold >= kbc) ? d.dz[(ll)kold * P + col] : 0.0;
     if (k > kbc && dzco < d.hmin) {
-      if (trs < d.mintr[n] || trs > d.maxtr[n])
+      if (trs < d.mintr[n] || trs > d.maxtr[n]) {
         trs = us_subsurface(D, w, col, k - 1, sa, t, fxn, fz + n * N3, dtu, adv);
+        if (CLIP) {
+          /* the layer below, as its own thread clamped it */
+          double lo, hi;
+          us_clip_range(D, col, k - 1, n, dtu, tin, bmn, bmx, lo, hi);
+          trs = us_clamp(trs, lo, hi);
+        }
+      }
     }
     /* clip (tracers.c:3328-3331) */
     if (d.mintr[n] <= 1e-5 && trs < d.mintr[n]) trs = d.mintr[n];
@@ -603,8 +678,9 @@ static void us_adv_launch(const emstr_usdev *D, const double *const *ts, const d
                                                        anydif, ult);
   if (d->nwet) {
     k_usfz<SCH><<<(d->nwet + 255) / 256, 256, 0, st>>>(*D, tin, d->wdn, d->nwet, fz, dtu, ult);
-    k_usadv<SCH, double><<<(d->nwet + 255) / 256, 256, 0, st>>>(
-        *D, ts[0], tin, tout, fx, fz, d->wdn, d->nwet, dtu, w, ts[1], ts[2], g3);
+    k_usadv<SCH, double, false><<<(d->nwet + 255) / 256, 256, 0, st>>>(
+        *D, ts[0], tin, tout, fx, fz, d->wdn, d->nwet, dtu, w, ts[1], ts[2], g3, nullptr,
+        nullptr);
   }
 }
 
@@ -660,23 +736,85 @@ extern "C" int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned l
 extern "C" int emstr_us_launch_update(const emstr_usdev *D, const double *const *ts,
                                       const double *tin, double *tout, const double *fx,
                                       const double *fz, double dtu, const double *rkw,
-                                      int last, int mode, double *g3, void *stream) {
+                                      int last, int mode, double *g3, const double *bmn,
+                                      const double *bmx, void *stream) {
   const emstr_dev *d = &D->d;
   const UsRk w = us_rkargs(rkw, last, mode);
-  if (d->nwet)
-    k_usadv<VANLEER, double><<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
-        *D, ts[0], tin, tout, fx, fz, d->wdn, d->nwet, dtu, w, ts[1], ts[2], g3);
+  const int gw = (d->nwet + 255) / 256;
+  hipStream_t st = (hipStream_t)stream;
+  if (!d->nwet) return 0;
+  if (bmn && bmx)
+    k_usadv<VANLEER, double, true><<<gw, 256, 0, st>>>(*D, ts[0], tin, tout, fx, fz, d->wdn,
+                                                       d->nwet, dtu, w, ts[1], ts[2], g3, bmn,
+                                                       bmx);
+  else
+    k_usadv<VANLEER, double, false><<<gw, 256, 0, st>>>(*D, ts[0], tin, tout, fx, fz, d->wdn,
+                                                        d->nwet, dtu, w, ts[1], ts[2], g3,
+                                                        nullptr, nullptr);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 /* the same from the FP32 form's fluxes (FP64 state), one stage */
 extern "C" int emstr_us_launch_update32(const emstr_usdev *D, const double *t0, const double *tin,
                                         double *tout, const float *fx, const float *fz,
-                                        double dtu, const double *rkw, void *stream) {
+                                        double dtu, const double *rkw, const double *bmn,
+                                        const double *bmx, void *stream) {
   const emstr_dev *d = &D->d;
   const UsRk w = us_rkargs(rkw, 1, 0);
+  const int gw = (d->nwet + 255) / 256;
+  hipStream_t st = (hipStream_t)stream;
+  if (!d->nwet) return 0;
+  if (bmn && bmx)
+    k_usadv<VANLEER, float, true><<<gw, 256, 0, st>>>(*D, t0, tin, tout, fx, fz, d->wdn,
+                                                      d->nwet, dtu, w, nullptr, nullptr, nullptr,
+                                                      bmn, bmx);
+  else
+    k_usadv<VANLEER, float, false><<<gw, 256, 0, st>>>(*D, t0, tin, tout, fx, fz, d->wdn,
+                                                       d->nwet, dtu, w, nullptr, nullptr,
+                                                       nullptr, nullptr, nullptr);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+/* FFSL CLIP: each wet centre's range over its least-squares stencil
+ * (transport.c:2207, 2309-2311: the upstream value, then get_local_bounds
+ * mode 2, :3824-3852, over lw->cells in their order), per tracer; a
+ * lateral ghost in the stencil holds the step-start value of its wet
+ * centre (tracers.c:185-200).  A column with water that is not processed
+ * is reported (err 3): its cells would be clamped to bounds left from an
+ * earlier call. */
+__global__ void __launch_bounds__(256)
+k_ffbnd(emstr_usdev D, const double *__restrict__ tin, const double *__restrict__ tst,
+        double *__restrict__ bmn, double *__restrict__ bmx, int *__restrict__ err) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.nwet) return;
+  const ll P = d.plane, N3 = d.n3;
+  const ll idx = d.wdn[q];
+  const int col = (int)(idx % P), k = (int)(idx / P);
+  if (d.ks[col] < 0) atomicMax(err, 3);
+  const int o0 = D.lsq_off[q], o1 = D.lsq_off[q + 1];
+  const ll lay = (ll)k * P;
+  for (int n = 0; n < d.ntr; n++) {
+    if (!d.in_tbdy[n] || !d.advect[n]) continue;
+    const double *t = tin + n * N3, *ts = tst + n * N3;
+    double lo = t[idx], hi = lo;
+    for (int o = o0; o < o1; o++) {
+      const int gc = D.lsq_gc[o];
+      const double v = (gc >= 0) ? t[lay + gc] : ts[lay + ~gc];
+      lo = mn_(lo, v);
+      hi = mx_(hi, v);
+    }
+    bmn[n * N3 + idx] = lo;
+    bmx[n * N3 + idx] = hi;
+  }
+}
+
+extern "C" int emstr_us_launch_ffsl_bounds(const emstr_usdev *D, const double *tin,
+                                           const double *tstart, double *bmn, double *bmx,
+                                           int *err, void *stream) {
+  const emstr_dev *d = &D->d;
   if (d->nwet)
-    k_usadv<VANLEER, float><<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(
-        *D, t0, tin, tout, fx, fz, d->wdn, d->nwet, dtu, w, nullptr, nullptr, nullptr);
+    k_ffbnd<<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(*D, tin, tstart, bmn, bmx,
+                                                                    err);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

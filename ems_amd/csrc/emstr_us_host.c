@@ -87,6 +87,9 @@ typedef struct emstr_us_ctx {
    * form's */
   int f32;
   float *f_fx, *f_fz, *f_tm, *f_lq, *f_te, *f_moc;
+  /* CLIP (clip_ffsl): each wet centre's tracer range over its
+   * least-squares stencil [ntr][nz][ncol], made on first use */
+  double *d_bmn, *d_bmx;
 } usctx;
 
 static void *dalloc(size_t bytes, int *ok) {
@@ -122,7 +125,7 @@ static void us_destroy(usctx *c) {
                 D->wbot, D->uc, D->vc, D->gu0, D->gv0, D->gue, D->gve, c->d_nw, c->d_crc,
                 c->d_crf, c->d_cour, c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc,
                 c->d_clc, c->d_clf, c->d_lci, c->d_fferr, c->d_need, c->f_fx, c->f_fz,
-                c->f_tm, c->f_lq, c->f_te, c->f_moc};
+                c->f_tm, c->f_lq, c->f_te, c->f_moc, D->lsq_gc, c->d_bmn, c->d_bmx};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->cs2col, c->col2cs, c->h_kb, c->h_ks, c->h_kold, c->h_knew, c->h_cols,
@@ -150,8 +153,28 @@ static int us_check_scope(geometry_t *g, window_t *wd, win_priv_t *wc) {
   if (wc->trasc == FFSL) {
     if (wc->ultimate)
       return emstr_set_error(E_UNSUP, -1, "FFSL with ULTIMATE is not supported");
-    if (wc->fillf & (MONOTONIC | CLIP))
-      return emstr_set_error(E_UNSUP, -1, "FFSL with a MONOTONIC or CLIP fill is not supported");
+    /* FILL_METHOD (tracers.c:1092-1093, clip_ffsl transport.c:4107-4247).
+     * MONOTONIC alone clamps to bounds that ffsl_don writes only under
+     * CLIP or ULTIMATE (transport.c:2207-2210, 2461-2462): the reference
+     * clips every tracer to zero and its mass rescaling then divides by
+     * zero.  CLIP (with or without MONOTONIC, whose fill its branch
+     * returns before) is built where its bounds are the step's: one
+     * Runge-Kutta stage (a second stage would read cells above the
+     * surface clamped to bounds kept from an earlier call) and every
+     * tracer of the step advected (ffsl_don writes the bounds of the
+     * advected tracers only). */
+    if ((wc->fillf & MONOTONIC) && !(wc->fillf & CLIP))
+      return emstr_set_error(E_UNSUP, -1,
+                             "FFSL with FILL_METHOD MONOTONIC without CLIP is not supported (the "
+                             "reference clamps to bounds it never sets)");
+    if (wc->fillf & CLIP) {
+      if (wc->rkstage != 1)
+        return emstr_set_error(E_UNSUP, -1, "FFSL with CLIP: one Runge-Kutta stage only");
+      for (n = 0; n < wc->ntbdy; n++)
+        if (!wc->advect[wc->tbdy[n]])
+          return emstr_set_error(E_UNSUP, wc->tbdy[n],
+                                 "FFSL with CLIP: every tracer of the step must be advected");
+    }
     if (wc->compatible & V6898)
       return emstr_set_error(E_UNSUP, -1, "FFSL with V6898 compatibility is not supported");
     if (!wc->lw || !g->c2v || !g->v2c || !g->nvc || !g->m2dv || !g->e2v || !g->gridx ||
@@ -863,6 +886,7 @@ static int us_ffsl_geom(usctx *c) {
   }
   if (ok) lsq_off[d->nwet] = (int)nent;
   int *lslot = ok ? (int *)malloc((nent ? nent : 1) * sizeof(int)) : NULL;
+  int *lgc = ok ? (int *)malloc((nent ? nent : 1) * sizeof(int)) : NULL;
   double *lB = ok ? (double *)malloc((nent ? nent : 1) * 2 * sizeof(double)) : NULL;
   /* the distinct centres each wet centre reads (itself first, then the
    * stencil's, then the vertex ranges'), by wet-list position, and the
@@ -872,7 +896,7 @@ static int us_ffsl_geom(usctx *c) {
   unsigned *vsl =
       ok ? (unsigned *)malloc(((size_t)d->nwet * EMSTR_FF_NPE + 1) * sizeof(unsigned)) : NULL;
   size_t nrow = 0;
-  if (ok && (!lslot || !lB || !lr_off || !lr_q || !vsl || !hvxy)) {
+  if (ok && (!lslot || !lgc || !lB || !lr_off || !lr_q || !vsl || !hvxy)) {
     emstr_set_error(E_DEVICE, -1, "FFSL stencil tables: host allocation");
     ok = 0;
   }
@@ -923,7 +947,8 @@ static int us_ffsl_geom(usctx *c) {
     lr_off[n] = (int)nrow;
     for (int i = 0; i < lw->ncells && ok; i++) {
       int cn = lw->cells[i];
-      if (cn > 0 && cn < g->szc && gbin[cn]) cn = gbin[cn];
+      const int isg = (cn > 0 && cn < g->szc && gbin[cn]) ? 1 : 0;
+      if (isg) cn = gbin[cn];
       int ccol = (cn > 0 && cn < g->szc) ? c->cs2col[g->m2d[cn]] : -1;
       int ck = (cn > 0 && cn < g->szc) ? g->s2k[cn] : -1;
       if (ccol < 0 || ck < c->h_kb[ccol] || ck >= nz || c->sp_of[(ll)ck * ncol + ccol] != cn) {
@@ -941,6 +966,9 @@ static int us_ffsl_geom(usctx *c) {
         break;
       }
       lslot[o] = sl;
+      /* the entry's column; a ghost (~column) reads the step-start
+       * value of its wet centre */
+      lgc[o] = isg ? ~ccol : ccol;
       lB[2 * o + 0] = lw->B[i][1];
       lB[2 * o + 1] = lw->B[i][2];
     }
@@ -980,6 +1008,7 @@ static int us_ffsl_geom(usctx *c) {
     D->eord = (int *)dalloc((size_t)npem * ncol * sizeof(int), &ok);
     D->lsq_off = (int *)dalloc(((size_t)d->nwet + 1) * sizeof(int), &ok);
     D->lsq_s = (int *)dalloc((nent ? nent : 1) * sizeof(int), &ok);
+    D->lsq_gc = (int *)dalloc((nent ? nent : 1) * sizeof(int), &ok);
     D->lr_off = (int *)dalloc(((size_t)d->nwet + 1) * sizeof(int), &ok);
     D->lr_q = (int *)dalloc((nrow ? nrow : 1) * sizeof(int), &ok);
     D->vsl = (unsigned *)dalloc(((size_t)d->nwet * EMSTR_FF_NPE + 1) * sizeof(unsigned), &ok);
@@ -997,6 +1026,7 @@ static int us_ffsl_geom(usctx *c) {
         up(c, D->eord, heord, (size_t)npem * ncol * sizeof(int)) ||
         up(c, D->lsq_off, lsq_off, ((size_t)d->nwet + 1) * sizeof(int)) ||
         up(c, D->lsq_s, lslot, (nent ? nent : 1) * sizeof(int)) ||
+        up(c, D->lsq_gc, lgc, (nent ? nent : 1) * sizeof(int)) ||
         up(c, D->lr_off, lr_off, ((size_t)d->nwet + 1) * sizeof(int)) ||
         up(c, D->lr_q, lr_q, (nrow ? nrow : 1) * sizeof(int)) ||
         up(c, D->vsl, vsl, ((size_t)d->nwet * EMSTR_FF_NPE + 1) * sizeof(unsigned)) ||
@@ -1020,6 +1050,7 @@ static int us_ffsl_geom(usctx *c) {
   free(gbin);
   free(lsq_off);
   free(lslot);
+  free(lgc);
   free(lB);
   free(hiq);
   free(lr_off);
@@ -1299,6 +1330,22 @@ static int us_adv(usctx *c) {
         const int last = (tc == nrk - 1);
         const double *ts[3] = {c->tr[in], (sin[2] >= 0) ? c->tr[sin[2]] : NULL,
                                (sin[3] >= 0) ? c->tr[sin[3]] : NULL};
+        /* CLIP (one stage): each centre's range over its stencil, of the
+         * stage's values (ghosts: the step's starting values) */
+        const int clip = ffsl && (wc->fillf & CLIP);
+        if (clip) {
+          if (!c->d_bmn) {
+            const size_t bytes = (size_t)(d->ntr > 0 ? d->ntr : 1) * d->n3 * sizeof(double);
+            int ok = 1;
+            c->d_bmn = (double *)dalloc(bytes, &ok);
+            c->d_bmx = (double *)dalloc(bytes, &ok);
+            if (!ok) return emstr_set_error(E_DEVICE, -1, "CLIP bounds (%zu bytes)", bytes);
+          }
+          if (emstr_us_launch_ffsl_bounds(D, c->tr[src], c->tr[b0], c->d_bmn, c->d_bmx,
+                                          c->d_fferr, c->st))
+            return emstr_set_error(E_DEVICE, -1, "CLIP bounds launch");
+        }
+        const double *bmn = clip ? c->d_bmn : NULL, *bmx = clip ? c->d_bmx : NULL;
         if (c->f32) {
           /* the FP32 form (one stage): the same maps, FP32 fluxes */
           if (emstr_us_launch_ffsl_flux32(D, u, c->tr[src], c->d_nw, c->d_clc, c->d_crc,
@@ -1307,7 +1354,7 @@ static int us_adv(usctx *c) {
                                           c->f_moc, c->f_fx, c->f_fz, c->anydif, c->d_need,
                                           us_ffstamp(c), c->d_fferr, c->st) ||
               emstr_us_launch_update32(D, c->tr[in], c->tr[src], c->tr[dst], c->f_fx,
-                                       c->f_fz, u, w, c->st))
+                                       c->f_fz, u, w, bmn, bmx, c->st))
             return emstr_set_error(E_DEVICE, -1, "FFSL launch");
         } else if (ffsl) {
           if (emstr_us_launch_ffsl_flux(D, u, c->tr[src], c->d_nw, c->d_clc, c->d_crc,
@@ -1316,7 +1363,7 @@ static int us_adv(usctx *c) {
                                         c->d_fx, c->d_fz, c->anydif, c->d_need, us_ffstamp(c),
                                         c->d_fferr, c->st) ||
               emstr_us_launch_update(D, ts, c->tr[src], c->tr[dst], c->d_fx, c->d_fz, u, w,
-                                     last, mode, c->d_g3, c->st))
+                                     last, mode, c->d_g3, bmn, bmx, c->st))
             return emstr_set_error(E_DEVICE, -1, "FFSL launch");
         } else if (emstr_us_launch_adv(D, ts, c->tr[src], c->tr[dst], c->tr[b0], c->d_edn,
                                        c->nesp, c->d_fx, c->d_fz, u, c->anydif, d->ultimate, w,
@@ -1354,6 +1401,10 @@ static int us_sync(usctx *c) {
         return emstr_set_error(E_UNSUP, -1, "FFSL: a streamline leaves its upstream cell into "
                                             "a wet cell (multi-segment streamlines need "
                                             "hd_trans_interp, not supported)");
+      if (err == 3)
+        return emstr_set_error(E_UNSUP, -1, "FFSL with CLIP: a column with water is not "
+                                            "processed (its cells would be clamped to bounds "
+                                            "kept from an earlier call)");
       return emstr_set_error(E_UNSUP, -1, "FFSL: a vertical trajectory does not end "
                                           "(zero layer thickness)");
     }

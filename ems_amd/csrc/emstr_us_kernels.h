@@ -28,15 +28,24 @@ int emstr_us_launch_adv(const emstr_usdev *D, const double *const *ts, const dou
  * sf, gu, gv (ghost velocities [nz][ne2]) are read under FFSL only     */
 int emstr_us_launch_dtm(const emstr_usdev *D, double dtu0, unsigned long long *out, double sf,
                         const double *gu, const double *gv, void *stream);
-/* the cell update, surf_concrk and tr_bounds from fluxes fx, fz */
+/* the cell update, surf_concrk and tr_bounds from fluxes fx, fz; with
+ * bmn, bmx (not NULL) the FFSL CLIP of clip_ffsl before tr_bounds, the
+ * centre ranges from emstr_us_launch_ffsl_bounds (one stage only) */
 int emstr_us_launch_update(const emstr_usdev *D, const double *const *ts, const double *tin,
                            double *tout, const double *fx, const double *fz, double dtu,
-                           const double *rkw, int last, int mode, double *g3, void *stream);
+                           const double *rkw, int last, int mode, double *g3,
+                           const double *bmn, const double *bmx, void *stream);
 /* The FP32 form: one forward-Euler update of the FP64 tracers from FP32
- * edge and vertical fluxes. */
+ * edge and vertical fluxes (bmn, bmx as above). */
 int emstr_us_launch_update32(const emstr_usdev *D, const double *t0, const double *tin,
                              double *tout, const float *fx, const float *fz, double dtu,
-                             const double *rkw, void *stream);
+                             const double *rkw, const double *bmn, const double *bmx,
+                             void *stream);
+/* FFSL CLIP: the range of every wet centre's tracer values over its
+ * least-squares stencil (itself first; ghosts from tstart) into bmn, bmx
+ * [ntr][n3]; err gets 3 where a column with water is not processed */
+int emstr_us_launch_ffsl_bounds(const emstr_usdev *D, const double *tin, const double *tstart,
+                                double *bmn, double *bmx, int *err, void *stream);
 
 /* FFSL (emstr_us_ffsl.hip).  Per step: the centred w into nw [nz][ncol]
  * (init: vel_center_w, else the Leonard et al. choice).  Per sub-step:

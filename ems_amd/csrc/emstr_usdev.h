@@ -65,6 +65,9 @@ typedef struct emstr_usdev {
    * the list */
   int *lsq_off;
   int *lsq_s;
+  int *lsq_gc;      /* the column of each stencil entry's cell at the
+                     * centre's layer, ~column for a lateral ghost (read
+                     * as the step-start value of that wet centre)      */
   double *lsq_B;
   int *lr_off, *lr_q;
   unsigned *vsl;

@@ -491,10 +491,15 @@ typedef struct {
   /* FFSL, per tracer (transport.c:1961-2598) */
   double *trm, *moc, *lww, *lwb, *cx, *cy, *ntr, *dint, *cour;
   int *cl, *mask;
+  /* clip_ffsl bounds (CLIP): smin / smax by edge, sminz / smaxz by
+   * centre, the cell range mn / mx (transport.c:1988-1991, 4118-4127);
+   * zero at allocation, land edges are never written */
+  double *smin, *smax, *sminz, *smaxz, *cmn, *cmx;
 } uscr_t;
 
 static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr, double *Fx,
                         double *Fz, double dtu, uscr_t *s);
+static void ff_clip(geometry_t *g, win_priv_t *wc, double *tr, uscr_t *s);
 
 static void us_tracer_substep(geometry_t *g, window_t *wd, win_priv_t *wc, int n, double dtu,
                               int slf, int ksf, const double *osubeta,
@@ -555,6 +560,8 @@ static void us_tracer_substep(geometry_t *g, window_t *wd, win_priv_t *wc, int n
         tr[c] = (g->botz[c2] > 0.0 && wc->dz[c] < wc->hmin) ? tr[c] : surftr;
       }
     }
+    /* tracers.c:1092-1093 */
+    if ((wc->trasc & FFSL) && (wc->fillf & CLIP)) ff_clip(g, wc, tr, s);
   }
 }
 
@@ -571,7 +578,8 @@ static uscr_t *uscr_get(int nt, int n) {
     void *p[] = {uscr[t].Fx, uscr[t].Fz, uscr[t].dtracer,
                  uscr[t].trm, uscr[t].moc, uscr[t].lww, uscr[t].lwb, uscr[t].cx,
                  uscr[t].cy, uscr[t].ntr, uscr[t].dint, uscr[t].cour, uscr[t].cl,
-                 uscr[t].mask};
+                 uscr[t].mask, uscr[t].smin, uscr[t].smax, uscr[t].sminz,
+                 uscr[t].smaxz, uscr[t].cmn, uscr[t].cmx};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++) free(p[i]);
   }
   free(uscr);
@@ -583,7 +591,9 @@ static uscr_t *uscr_get(int nt, int n) {
     }
     double **dp[] = {&uscr[t].Fx, &uscr[t].Fz, &uscr[t].dtracer,
                      &uscr[t].trm, &uscr[t].moc, &uscr[t].lwb, &uscr[t].cx, &uscr[t].cy,
-                     &uscr[t].ntr, &uscr[t].dint, &uscr[t].cour};
+                     &uscr[t].ntr, &uscr[t].dint, &uscr[t].cour, &uscr[t].smin,
+                     &uscr[t].smax, &uscr[t].sminz, &uscr[t].smaxz, &uscr[t].cmn,
+                     &uscr[t].cmx};
     for (size_t i = 0; i < sizeof(dp) / sizeof(dp[0]); i++)
       *dp[i] = (double *)calloc(n, sizeof(double));
     uscr[t].lww = (double *)calloc((size_t)3 * n, sizeof(double));
@@ -597,8 +607,8 @@ static uscr_t *uscr_get(int nt, int n) {
 
 /*===================================================================*/
 /* FFSL on the unstructured mesh (trasc & FFSL), single window, closed */
-/* boundaries, no mean-velocity transport, no ULTIMATE, no clipping   */
-/* fill: the path taken by ffsl_don with its defaults (tranf =        */
+/* boundaries, no mean-velocity transport, no ULTIMATE, the CLIP fill */
+/* (clip_ffsl): the path taken by ffsl_don with its defaults (tranf = */
 /* T_VERT|T_HORZ|T_HV, doint = 1, momo2 = 0, vscheme = V_VC, tf = 0.55) */
 /* and the linear least-squares interpolant (L_LSLIN).                */
 /*===================================================================*/
@@ -970,7 +980,9 @@ static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr,
   const double *u1 = wd->u1, *u2 = wd->u2, *w = wd->w, *u1v = wd->u1flux3d;
   double *cour = s->cour, *cx = s->cx, *cy = s->cy, *tr_mod = s->trm, *tr_moc = s->moc;
   double *ntr = s->ntr, *dint = s->dint;
+  double *smin = s->smin, *smax = s->smax, *sminz = s->sminz, *smaxz = s->smaxz;
   int *cl = s->cl, *mask = s->mask;
+  const int clip = (wc->fillf & CLIP) != 0;
   int cc, ee, n;
 
   /* the largest Courant number of each cell (transport.c:2058-2068) */
@@ -1031,6 +1043,8 @@ static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr,
     double time_left = dtu, dist, trs, tre, px, py;
     double u = u1[e] * g->costhu1[es] + u2[e] * g->costhu2[es];
     double v = u1[e] * g->sinthu1[es] + u2[e] * g->sinthu2[es];
+    /* the streamline's bounds start at the upstream value (transport.c:2207) */
+    if (clip) smin[e] = smax[e] = tr[cl[ee]];
     while (cl[ee] > 0) {
       int ci = cl[ee];
       if (first) {
@@ -1047,6 +1061,18 @@ static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr,
       }
       trs = ff_lin_value(g, s->lww, s->lwb, ci, px, py);
       tre = ff_lin_value(g, s->lww, s->lwb, ci, cx[e], cy[e]);
+      /* widened by the tracer over the least-squares stencil of the
+       * segment's centre (transport.c:2309-2311, get_local_bounds mode 2
+       * :3824-3852; called twice there, the second call changes nothing) */
+      if (clip) {
+        const qweights *lw = &wc->lw[abs(cl[ee])];
+        int j;
+        for (j = 0; j < lw->ncells; j++) {
+          int cn = lw->cells[j];
+          smin[e] = min(smin[e], tr[cn]);
+          smax[e] = max(smax[e], tr[cn]);
+        }
+      }
       dint[e] += dist;
       ntr[e] += 0.5 * dist * (trs + tre);
       if (cl[ee] < 0) {
@@ -1082,6 +1108,8 @@ static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr,
       ff_van_leer(Fz, tr, w, ff_crfzf, ff_clzf, c, g->zp1, g->zm1);
     else
       ff_van_leer(Fz, tr_moc, w, ff_crfzf, ff_clzf, c, g->zp1, g->zm1);
+    /* transport.c:2461 */
+    if (clip) sminz[c] = smaxz[c] = tr[(w[c] > 0.0) ? zm1 : c];
     if (w[c] == 0.0 || zm1 == g->zm1[zm1]) {
       Fz[c] = 0.0;
       continue;
@@ -1097,12 +1125,40 @@ static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr,
         ous_err = 4;
         break;
       }
+      /* transport.c:2483-2486 */
+      if (clip) {
+        sminz[c] = min(sminz[c], tr[ci]);
+        smaxz[c] = max(smaxz[c], tr[ci]);
+      }
       Fz[c] += (dz * tr_moc[ci]);
       dist -= dz;
       ci = (w[c] < 0.0) ? g->zp1[ci] : g->zm1[ci];
       if (wc->dz[ci] > 0) dz = wc->dz[ci];
     }
     if (w[c] < 0) Fz[c] *= -1.0;
+  }
+}
+
+/* transport.c:4107-4247  clip_ffsl, CLIP branch: each centre's range
+ * is its vertical streamline's and those of its edges (a land edge's
+ * stays zero, as allocated), and the tracer is clamped to it */
+static void ff_clip(geometry_t *g, win_priv_t *wc, double *tr, uscr_t *s) {
+  int cc, ee;
+  double *mn = s->cmn, *mx = s->cmx;
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    int c = g->w3_t[cc];
+    int c2 = g->m2d[c];
+    mn[c] = s->sminz[c];
+    mx[c] = s->smaxz[c];
+    for (ee = 1; ee <= g->npe[c2]; ee++) {
+      int e = g->c2e[ee][c];
+      mn[c] = min(mn[c], s->smin[e]);
+      mx[c] = max(mx[c], s->smax[e]);
+    }
+  }
+  for (cc = 1; cc <= g->v3_t; cc++) {
+    int c = g->w3_t[cc];
+    tr[c] = max(min(mx[c], tr[c]), mn[c]);
   }
 }
 
@@ -1157,6 +1213,27 @@ int oracle_us_advect_diffuse_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int
         ff_nw[c] = w[c];
       else if ((w[c] < 0.0) && (w[zp1] <= 0.0))
         ff_nw[c] = w[zp1];
+    }
+    /* FILL_METHOD (tracers.c:1092-1093): CLIP is built where its bounds
+     * come from the step's values (one Runge-Kutta stage, every column
+     * with water processed: a cell of another column is clamped to the
+     * bounds its scratch entry kept from an earlier call); MONOTONIC
+     * alone clamps to bounds the tracer step never writes (error 6) */
+    if ((wc->fillf & MONOTONIC) && !(wc->fillf & CLIP)) {
+      ous_err = 6;
+      return -ous_err;
+    }
+    if (wc->fillf & CLIP) {
+      int bad = (wc->rkstage != 1);
+      char *pc = (char *)calloc(g->szcS, 1);
+      for (cc = 1; cc <= wc->vcs1; cc++) pc[g->m2d[wc->s1[cc]]] = 1;
+      for (cc = 1; cc <= g->v3_t && !bad; cc++)
+        if (!pc[g->m2d[g->w3_t[cc]]]) bad = 1;
+      free(pc);
+      if (bad) {
+        ous_err = 6;
+        return -ous_err;
+      }
     }
     ff_init(g, wd, wc, ff_nw);
     /* edge -> position in w3_e1 (transport.c:2160), the same every call */

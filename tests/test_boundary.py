@@ -155,8 +155,8 @@ def test_compas_out_of_scope_windows_rejected(what):
         hc.wc.sigma = 1
         want = "sigma"
     elif what == "fill":
-        hc.wc.fillf = U.MONOTONIC if hasattr(U, "MONOTONIC") else 0x0010
-        want = "MONOTONIC or CLIP"
+        hc.wc.fillf = U.MONOTONIC
+        want = "MONOTONIC without CLIP"
     else:
         hc.wc.rkstage = 4
         want = "4 stages"

@@ -274,6 +274,83 @@ def test_hex_ffsl_step(name, kw, stab, trsf, nsteps):
     assert_exact(gpu, ora)
 
 
+CLIP_CASES = [
+    # name, hex_basin kw, stab, trsf, nsteps, fill
+    ("clip", dict(dt=600.0), U.NONE, 0.8, 3, U.CLIP),
+    ("clip_monotonic", dict(dt=600.0), U.NONE, 0.8, 2, U.CLIP | U.MONOTONIC),
+    ("clip_substeps", dict(dt=120.0), U.SUB_STEP, 0.002, 3, U.CLIP),
+]
+
+
+@pytest.mark.parametrize("name,kw,stab,trsf,nsteps,fill", CLIP_CASES)
+def test_hex_ffsl_clip(name, kw, stab, trsf, nsteps, fill):
+    """FILL_METHOD CLIP (clip_ffsl): each centre clamped to the range of
+    its streamlines' stencils; tracers within 1e-12 of the oracle, and
+    the clip acts (the result differs from the unclipped step)."""
+    say("case hex", name)
+    res = {}
+    for f in (0, fill):
+        case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=3, seed=17, trasc=U.FFSL, **kw)
+        case.wc.stab = stab
+        case.wc.trsf = trsf
+        case.wc.fillf = f
+        case.prepare(OU.set_dz)
+        gpu, ora = run_both(case, nsteps)
+        assert_exact(gpu, ora)
+        res[f] = gpu
+    assert (res[0] != res[fill]).sum() > 50
+
+
+def test_hex_ffsl_clip_moving_surface():
+    """The surface moves across layers in some columns, and every column
+    stays processed (the clip is refused otherwise)."""
+    say("case hex ffsl clip, moving surface + diffusion")
+    case = MS.hex_basin(nx=20, ny=16, nz=6, ntr=3, seed=19, dmin=6.0, dmax=30.0,
+                        trasc=U.FFSL, dt=300.0)
+    rng = np.random.default_rng(5)
+    g = case.g
+    w2 = U.view(g.w2_t, case.szcS, np.int32)[1:g.v2_t + 1]
+    dl = case.layers[1] - case.layers[0]
+    old, eta = case.a2(case.wc.oldeta), case.a2(case.wd.eta)
+    old[w2] = rng.uniform(-0.7, 0.0, len(w2)) * dl
+    eta[w2] = old[w2] + rng.uniform(-0.5, 0.5, len(w2)) * dl
+    U.view(case.wc.diffuse, 3, np.int32)[:] = [1, 0, 1]
+    case.ae(case.wc.u2kh)[:] = rng.uniform(0.0, 30.0, case.sze)
+    case.wc.fillf = U.CLIP
+    case.prepare(OU.set_dz)
+    gpu, ora = run_both(case, 2)
+    assert_exact(gpu, ora)
+
+
+def test_hex_ffsl_clip_settings_outside_scope():
+    """MONOTONIC without CLIP (clamps to bounds the reference never sets),
+    CLIP with two Runge-Kutta stages, CLIP with a tracer not advected."""
+    L = MS.lib()
+    for fill, rk, adv, what in ((U.MONOTONIC, 1, 1, "MONOTONIC"), (U.CLIP, 2, 1, "one Runge"),
+                                (U.CLIP, 1, 0, "advected")):
+        say("case hex ffsl clip refused:", what)
+        case = MS.hex_basin(nx=12, ny=10, nz=4, ntr=2, seed=2, trasc=U.FFSL, dt=300.0)
+        case.wc.fillf = fill
+        case.wc.rkstage = rk
+        U.view(case.wc.advect, 2, np.int32)[1] = adv
+        case.prepare(OU.set_dz)
+        L.tracer_step_3d(case.gp, case.wp, case.pp)
+        assert what in MS.last_error()[0], MS.last_error()
+        L.emstr_us_forget_window(case.gp)
+    say("case hex ffsl clip refused: a column not processed")
+    case = MS.hex_basin(nx=20, ny=16, nz=6, ntr=3, seed=19, dmin=2.0, dmax=30.0,
+                        trasc=U.FFSL, dt=300.0)
+    w2, old, eta, rng, dl = _moving(case, 5)
+    OU.set_dz(case.gp, case.wp, case.pp)
+    eta[w2] = old[w2] + rng.uniform(-0.9, 0.9, len(w2)) * dl
+    case.wc.fillf = U.CLIP
+    case.prepare(OU.set_dz)
+    assert case.wc.vcs1 < case.g.v2_t
+    L.tracer_step_3d(case.gp, case.wp, case.pp)
+    assert "not processed" in MS.last_error()[0], MS.last_error()
+    L.emstr_us_forget_window(case.gp)
+
+
 def test_hex_ffsl_vertical_courant_above_one():
     """A vertical velocity 300 times that of continuity: vertical Courant
     numbers up to 3.5, so the vertical fluxes read tr_moc (the mean
@@ -364,6 +441,26 @@ def test_ffsl_maps_bit_exact_c4():
     case.prepare(OU.set_dz)
     say("c4 built:", case.nwet, "wet cells")
     _maps_gpu_vs_oracle(case)
+
+
+@pytest.mark.parametrize("trasc", [U.FFSL, U.VANLEER])
+def test_c4_tracer_values_vs_oracle(trasc):
+    """The full C4 mesh (350 x 350 jittered hexagons, 50 layers), three
+    tracers, two steps through the drop-in tracer_step_3d against the
+    oracle: every tracer value within 1e-12 relative (FFSL: the streamline
+    directions come from atan2/sin/cos of two libraries), van Leer bit for
+    bit."""
+    say("case c4 tracer values, trasc", trasc, "(mesh build ~30 s)")
+    case = MS.c4(seed=3, ntr=3, trasc=trasc)
+    case.prepare(OU.set_dz)
+    say("c4 built:", case.nwet, "wet cells")
+    start = case.tracers().copy()
+    gpu, ora = run_both(case, 2)
+    if trasc == U.VANLEER:
+        assert np.array_equal(gpu, ora)
+    else:
+        assert_exact(gpu, ora)
+    assert not np.array_equal(gpu, start)
 
 
 def test_ffsl_multi_segment_is_reported():

@@ -86,6 +86,17 @@ def test_hex_ffsl_fp32_vs_plain():
     assert np.abs(case.tracers() - t0).max() > 1e-3
 
 
+def test_hex_ffsl_fp32_clip_vs_plain():
+    """The FP32 form with FILL_METHOD CLIP against the plain form with it:
+    40 steps."""
+    say("case hex ffsl fp32 clip")
+    case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=3, seed=17, trasc=U.FFSL, dt=600.0)
+    case.wc.trsf = 0.8
+    case.wc.fillf = U.CLIP
+    case.prepare(OU.set_dz)
+    compare(case, (10, 40))
+
+
 def test_fp32_form_refuses_other_schemes():
     case = MS.hex_basin(nx=10, ny=8, nz=4, ntr=1, seed=3, trasc=U.VANLEER)
     L = MS.lib()

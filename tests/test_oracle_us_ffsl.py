@@ -21,7 +21,11 @@ What fixes each part, independently of the formulas restated there:
       polygons, not from the maps), and beta is the largest such factor
   the whole step
       constancy and closed-basin mass conservation on hexagons with
-      vertical motion; ffsl_init runs on every other call"""
+      vertical motion; ffsl_init runs on every other call
+  clip_ffsl (FILL_METHOD CLIP)
+      a translated linear field is left alone in the interior; where the
+      clip acts the new value is a nearby cell's step-start value or
+      zero; the global range holds"""
 import numpy as np
 import pytest
 
@@ -258,3 +262,100 @@ def test_multi_segment_streamline_is_reported():
     case.set_tracers(np.ones((1, case.nwet)))
     OU.set_dz(case.gp, case.wp, case.pp)
     assert OU.advect_diffuse(case) == -3
+
+
+# ---- FILL_METHOD CLIP (clip_ffsl, transport.c:4107-4138) ----------------
+
+def test_clip_leaves_a_translated_linear_field():
+    """A linear field in uniform flow moves exactly, and each interior
+    cell's new value lies inside the range of the upstream stencils: the
+    clip changes nothing there (bounds that were too tight, e.g. only the
+    cell's own value, would clip every cell downstream of a gradient)."""
+    dt, U0, V0 = 180.0, 0.6, -0.35
+    res = {}
+    for fill in (0, U.CLIP):
+        case = _flat()
+        case.set_uniform_velocity(U0, V0)
+        case.scheme(U.FFSL, 0, dt)
+        case.wc.fillf = fill
+        cx, cy = _centres(case)
+        t0 = 30.0 + 2e-4 * cx - 3e-4 * cy
+        case.set_tracers(t0[None, :])
+        OU.set_dz(case.gp, case.wp, case.pp)
+        assert OU.advect_diffuse(case) == 0
+        res[fill] = case.tracers()[0]
+    inner = _interior(cx, cy, 12000.0)
+    exp = 30.0 + 2e-4 * (cx - U0 * dt) - 3e-4 * (cy - V0 * dt)
+    assert np.abs(res[U.CLIP] - exp)[inner].max() < 1e-11
+    assert np.array_equal(res[U.CLIP][inner], res[0][inner])
+
+
+def _clip_pair(seed, steps=1):
+    out = {}
+    for fill in (0, U.CLIP):
+        case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=3, seed=seed, trasc=U.FFSL, dt=600.0)
+        case.wc.fillf = fill
+        case.prepare(OU.set_dz)
+        t0 = np.stack([case.tr(n).copy() for n in range(case.ntr)])
+        for _ in range(steps):
+            assert OU.advect_diffuse(case) == 0
+        out[fill] = np.stack([case.tr(n).copy() for n in range(case.ntr)])
+    return case, t0, out
+
+
+@pytest.mark.parametrize("seed", [1, 4])
+def test_clip_clamps_to_neighbouring_values(seed):
+    """Where the clip acts, the new value is exactly the step-start value
+    of a cell near the cell (its stencils' cells and its column), or zero
+    (a land edge's bounds): the clamp is to data, not to a reconstruction;
+    a cell is moved up to a minimum or down to a maximum, never past the
+    unclipped value."""
+    case, t0, out = _clip_pair(seed)
+    g = case.g
+    m2d = U.view(g.m2d, case.szc, np.int32)
+    cellx = U.view(g.cellx, case.szcS, np.float64)
+    celly = U.view(g.celly, case.szcS, np.float64)
+    mint = U.view(case.wc.mintr, case.ntr, np.float64)
+    wet = case.wet
+    x, y = cellx[m2d[wet]], celly[m2d[wet]]
+    nclip = 0
+    for n in range(case.ntr):
+        a, b = out[0][n][wet], out[U.CLIP][n][wet]
+        diff = np.nonzero(a != b)[0]
+        nclip += diff.size
+        for i in diff:
+            near = np.hypot(x - x[i], y - y[i]) < 4.0 * 1500.0
+            cand = set(t0[n][wet][near].tolist()) | {0.0, float(mint[n])}
+            assert float(b[i]) in cand, (n, i, b[i], a[i])
+            # clamped toward the range: raised to a minimum or lowered to a
+            # maximum, by the amount the unclipped value was out of it
+            assert b[i] != a[i]
+    assert nclip > 50
+
+
+def test_clip_keeps_the_global_range():
+    case, t0, out = _clip_pair(2, steps=3)
+    wet = case.wet
+    for n in range(case.ntr):
+        lo = min(0.0, t0[n][wet].min())
+        hi = max(0.0, t0[n][wet].max())
+        got = out[U.CLIP][n][wet]
+        assert got.min() >= lo and got.max() <= hi
+
+
+def test_clip_settings_outside_scope_are_reported():
+    for fill, rk in ((U.MONOTONIC, 1), (U.CLIP, 2), (U.CLIP | U.MONOTONIC, 3)):
+        case = MS.hex_basin(nx=12, ny=10, nz=4, ntr=1, seed=2, trasc=U.FFSL, dt=300.0)
+        case.wc.fillf = fill
+        case.wc.rkstage = rk
+        case.prepare(OU.set_dz)
+        assert OU.advect_diffuse(case) == -6
+    # CLIP with MONOTONIC is CLIP (its branch returns before the fill)
+    res = []
+    for fill in (U.CLIP, U.CLIP | U.MONOTONIC):
+        case = MS.hex_basin(nx=12, ny=10, nz=4, ntr=1, seed=2, trasc=U.FFSL, dt=300.0)
+        case.wc.fillf = fill
+        case.prepare(OU.set_dz)
+        assert OU.advect_diffuse(case) == 0
+        res.append(case.tracers())
+    assert np.array_equal(res[0], res[1])
