@@ -43,6 +43,7 @@ PROTOS = [
     ("emstr_tracer_mass_noalloc", C.c_int, [_CTX, C.POINTER(C.c_double)]),
     ("emstr_device_set_dz", C.c_int, [_CTX]),
     ("emstr_device_vel_w", C.c_int, [_CTX]),
+    ("emstr_device_set_flux_3d", C.c_int, [_CTX]),
     ("emstr_profile", C.c_int, [_CTX, C.c_int]),
     ("emstr_kernel_times", C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     ("emstr_sync", C.c_int, [_CTX]),

@@ -372,6 +372,10 @@ emstr_grid *emstr_grid_build(int nx, int ny, int nz, const double *layers,
   wd->eta = (double *)zalloc(S, sizeof(double));
   wd->topz = (double *)zalloc(S, sizeof(double));
   wd->wbot = (double *)zalloc(S, sizeof(double));
+  wd->u1flux = (double *)zalloc(S, sizeof(double));
+  wd->u2flux = (double *)zalloc(S, sizeof(double));
+  wd->u1vm = (double *)zalloc(sgsiz, sizeof(double));
+  wd->u2vm = (double *)zalloc(sgsiz, sizeof(double));
 
   /* win_priv_t: defaults of model/hd/inputs/readparam.c:133-145 */
   wc->trasc = VANLEER;
@@ -424,6 +428,15 @@ emstr_grid *emstr_grid_build(int nx, int ny, int nz, const double *layers,
   for (n = 0; n < 4; n++) *dv[n] = (double *)zalloc(S, sizeof(double));
   double **vv[] = {&wc->v1, &wc->v2, &wc->v3, &wc->v4, &wc->v5, &wc->v6, &wc->v7};
   for (n = 0; n < 7; n++) *vv[n] = (double *)zalloc(nz + 2, sizeof(double));
+  /* the momentum step's face lists: surface faces at the top layer, no
+   * column one cell deep (callers set them from the elevation) */
+  gw->sur_e1 = (int *)zalloc(S, sizeof(int));
+  gw->sur_e2 = (int *)zalloc(S, sizeof(int));
+  for (n = 1; n <= gw->a2_e1; n++) gw->sur_e1[n] = gw->w2_e1[n];
+  for (n = 1; n <= gw->a2_e2; n++) gw->sur_e2[n] = gw->w2_e2[n];
+  wc->cbot_e1 = (int *)zalloc(S, sizeof(int));
+  wc->cbot_e2 = (int *)zalloc(S, sizeof(int));
+  wc->ncbot_e1 = wc->ncbot_e2 = 0;
   return G;
 #undef WET
 }
@@ -442,14 +455,16 @@ void emstr_grid_free(emstr_grid *G) {
                  g->w2_t, g->w2_e1, g->w2_e2, g->bpt, g->bin, g->gsed_t, g->ised_t,
                  g->sur_t, g->nsur_t, g->bot_t, g->m2d, g->xp1, g->xm1, g->yp1,
                  g->ym1, g->zp1, g->zm1, g->h1acell, g->h2acell, g->h1au1, g->h2au2,
-                 g->h2au1, g->h1au2, g->cellarea, g->botz, g->gridz, g->cellz};
+                 g->h2au1, g->h1au2, g->cellarea, g->botz, g->gridz, g->cellz,
+                 g->sur_e1, g->sur_e2};
     for (n = 0; n < (int)(sizeof(a) / sizeof(a[0])); n++) free(a[n]);
     free(g);
   }
   if (w) {
     for (n = 0; n < w->ntr; n++) free(w->tr_wc[n]);
     void *a[] = {w->tr_wc, w->u1, w->u2, w->w, w->u1flux3d, w->u2flux3d, w->dzu1,
-                 w->dzu2, w->Kz, w->eta, w->topz, w->wbot};
+                 w->dzu2, w->Kz, w->eta, w->topz, w->wbot, w->u1flux, w->u2flux,
+                 w->u1vm, w->u2vm};
     for (n = 0; n < (int)(sizeof(a) / sizeof(a[0])); n++) free(a[n]);
     free(w);
   }
@@ -459,7 +474,7 @@ void emstr_grid_free(emstr_grid *G) {
                  p->u1kh, p->u2kh, p->s1, p->i1, p->i2, p->i3, p->i4, p->i5,
                  p->w1, p->w2, p->w3, p->w4, p->w5, p->w6, p->w7, p->w8, p->w9,
                  p->w10, p->d1, p->d2, p->d3, p->d4, p->v1, p->v2, p->v3, p->v4,
-                 p->v5, p->v6, p->v7};
+                 p->v5, p->v6, p->v7, p->cbot_e1, p->cbot_e2};
     for (n = 0; n < (int)(sizeof(a) / sizeof(a[0])); n++) free(a[n]);
     free(p);
   }

@@ -75,6 +75,14 @@ struct emstr_ctx {
   struct emstr_batch *batch; /* batched form state (EMSTR_BATCH_F32) */
   int use_graph;
   emstr_prof *prof;        /* per-kernel event timing                 */
+  /* set_flux_3d on the device, made on first use: the dense positions
+   * of the window's wet e1 / e2 faces, the surface face layer of each
+   * column, the faces one cell deep, staging for dzu and the 2D terms */
+  unsigned char *d_fw1, *d_fw2;
+  int *h_dn;               /* [sgsiz] dense index of a wet cell (-1)  */
+  int *h_ksur, *d_ksur;    /* [plane]                                 */
+  ll *h_cb, *d_cb;         /* [sgsizS] dense faces one cell deep      */
+  double *d_fa, *d_h, *d_m, *d_uf;
 };
 
 /*-------------------------------------------------------------------*/
@@ -450,11 +458,13 @@ void emstr_destroy(emstr_ctx *c) {
                 c->d_gsed, c->d_ised, c->d_proc, c->d_err, c->d_dtm, c->d_part,
                 c->d_wbot, c->tr[0], c->tr[1], c->tr[2], c->d_fferr, d->fnu, d->fnv,
                 d->fnw, d->fw9, d->fcl, d->fcrf, d->ftmx, d->ftmy, d->ftmz, d->ftm,
-                d->ffx, d->ffy, d->ffz, c->d_face};
+                d->ffx, d->ffy, d->ffz, c->d_face, c->d_fw1, c->d_fw2, c->d_ksur,
+                c->d_cb, c->d_fa, c->d_h, c->d_m, c->d_uf};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->h_kb, c->cc2col, c->col2cc, c->sedg, c->topc, c->h_ks,
-                c->h_kold, c->h_knew, c->h_proc, c->h_cols, c->sp_of, c->h_part};
+                c->h_kold, c->h_knew, c->h_proc, c->h_cols, c->sp_of, c->h_part,
+                c->h_dn, c->h_ksur, c->h_cb};
   for (n = 0; n < (int)(sizeof(hv) / sizeof(hv[0])); n++) free(hv[n]);
   if (c->st) hipStreamDestroy(c->st);
   pthread_mutex_lock(&reg_lock);
@@ -1082,6 +1092,135 @@ int emstr_device_vel_w(emstr_ctx *c) {
   HCHK(hipMemcpyAsync(c->wd->w, c->stage, (size_t)g->sgsiz * sizeof(double),
                       hipMemcpyDeviceToHost, c->st));
   HCHK(hipStreamSynchronize(c->st));
+  return form_fields(c);
+}
+
+/* set_flux_3d (model/hd/momentum/vel3d.c:4041-4191, VEL3D mode) on
+ * the device: the face volume fluxes from the window's face velocities
+ * and thicknesses (or, under FFSL, its mean volume fluxes), into the
+ * context's fields and windat->u1flux3d / u2flux3d. */
+static int flux3d_setup(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  emstr_dev *d = &c->d;
+  const ll n3 = d->n3, plane = d->plane;
+  int ok = 1, cc, dir;
+  c->h_dn = (int *)malloc((size_t)g->sgsiz * sizeof(int));
+  c->h_ksur = (int *)malloc((size_t)plane * sizeof(int));
+  c->h_cb = (ll *)malloc((size_t)(g->sgsiz + 1) * sizeof(ll));
+  unsigned char *fw = (unsigned char *)malloc((size_t)n3);
+  if (!c->h_dn || !c->h_ksur || !c->h_cb || !fw) {
+    free(fw);
+    return emstr_set_error(EMSTR_EDEVICE, -1, "set_flux_3d: host allocation");
+  }
+  for (ll i = 0; i < g->sgsiz; i++) c->h_dn[i] = -1;
+  for (ll i = 0; i < n3; i++)
+    if (c->sp_of[i] > 0) c->h_dn[c->sp_of[i]] = (int)i;
+  c->d_fw1 = (unsigned char *)dalloc((size_t)n3, &ok);
+  c->d_fw2 = (unsigned char *)dalloc((size_t)n3, &ok);
+  c->d_ksur = (int *)dalloc((size_t)plane * sizeof(int), &ok);
+  c->d_cb = (ll *)dalloc((size_t)(g->sgsiz + 1) * sizeof(ll), &ok);
+  c->d_fa = (double *)dalloc((size_t)n3 * sizeof(double), &ok);
+  c->d_h = (double *)dalloc((size_t)plane * sizeof(double), &ok);
+  c->d_m = (double *)dalloc((size_t)plane * sizeof(double), &ok);
+  c->d_uf = (double *)dalloc((size_t)plane * sizeof(double), &ok);
+  for (dir = 1; dir <= 2 && ok; dir++) {
+    const int *w3 = (dir == 1) ? g->w3_e1 : g->w3_e2;
+    const int nb = (dir == 1) ? g->b3_e1 : g->b3_e2;
+    memset(fw, 0, (size_t)n3);
+    for (cc = 1; cc <= nb; cc++) {
+      const int f = w3[cc];
+      if (f < 1 || f >= g->sgsiz || c->h_dn[f] < 0) {
+        free(fw);
+        return emstr_set_error(EMSTR_EUNSUPPORTED, f,
+                               "set_flux_3d: e%d face %d is not the face of a wet cell", dir, f);
+      }
+      fw[c->h_dn[f]] = 1;
+    }
+    if (hipMemcpy(dir == 1 ? c->d_fw1 : c->d_fw2, fw, (size_t)n3, hipMemcpyHostToDevice) !=
+        hipSuccess)
+      ok = 0;
+  }
+  free(fw);
+  if (!ok) return emstr_set_error(EMSTR_EDEVICE, -1, "set_flux_3d: device allocation");
+  return 0;
+}
+
+int emstr_device_set_flux_3d(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->d;
+  const int ffsl = (wc->trasc == FFSL);
+  int cc, dir, r;
+  emstr_clear_error();
+  if (ffsl ? (!wd->u1vm || !wd->u2vm)
+           : (!g->sur_e1 || !g->sur_e2 || !wd->dzu1 || !wd->dzu2 ||
+              ((wc->ncbot_e1 > 0 || wc->ncbot_e2 > 0) &&
+               (!wd->u1flux || !wd->u2flux || !wc->cbot_e1 || !wc->cbot_e2))))
+    return emstr_set_error(EMSTR_EARG, -1,
+                           "set_flux_3d needs sur_e1/e2, dzu1/2 and, for faces one cell deep, "
+                           "u1flux/u2flux and cbot_e1/e2 (FFSL: u1vm/u2vm)");
+  if (!c->d_fw1 && (r = flux3d_setup(c))) return r;
+  for (dir = 1; dir <= 2; dir++) {
+    const int *w3 = (dir == 1) ? g->w3_e1 : g->w3_e2;
+    const int nb = (dir == 1) ? g->b3_e1 : g->b3_e2, nn = (dir == 1) ? g->n3_e1 : g->n3_e2;
+    double *u = (dir == 1) ? wd->u1 : wd->u2;
+    double *uvm = (dir == 1) ? wd->u1vm : wd->u2vm;
+    double *out = (dir == 1) ? d->u1flux3d : d->u2flux3d;
+    double *hout = (dir == 1) ? wd->u1flux3d : wd->u2flux3d;
+    int ncb = 0;
+    if (ffsl) {
+      /* the velocities past b3 are zeroed on the window (vel3d.c:4069-4078) */
+      for (cc = nb + 1; cc <= nn; cc++) {
+        u[w3[cc]] = 0.0;
+        uvm[w3[cc]] = 0.0;
+      }
+      if (upload3d(c, c->d_fa, uvm)) return EMSTR_EDEVICE;
+    } else {
+      const int *w2 = (dir == 1) ? g->w2_e1 : g->w2_e2;
+      const int *sur = (dir == 1) ? g->sur_e1 : g->sur_e2;
+      const int nb2 = (dir == 1) ? g->b2_e1 : g->b2_e2;
+      const int *cbl = (dir == 1) ? wc->cbot_e1 : wc->cbot_e2;
+      const int ncbl = (dir == 1) ? wc->ncbot_e1 : wc->ncbot_e2;
+      for (ll q = 0; q < d->plane; q++) c->h_ksur[q] = d->nz;
+      for (cc = 1; cc <= nb2; cc++) {
+        const int cs = w2[cc], f = sur[cc];
+        if (f < 1 || f >= g->sgsiz)
+          return emstr_set_error(EMSTR_EARG, cc, "set_flux_3d: surface face of column %d", cc);
+        c->h_ksur[(ll)g->s2j[cs] * d->nx + g->s2i[cs]] = g->s2k[f];
+      }
+      for (cc = 1; cc <= ncbl; cc++) {
+        const int f = cbl[cc];
+        if (f < 1 || f >= g->sgsiz || c->h_dn[f] < 0)
+          return emstr_set_error(EMSTR_EUNSUPPORTED, f,
+                                 "set_flux_3d: face %d one cell deep is not a wet cell's", f);
+        c->h_cb[ncb++] = c->h_dn[f];
+      }
+      HCHK(hipMemcpyAsync(c->d_ksur, c->h_ksur, (size_t)d->plane * sizeof(int),
+                          hipMemcpyHostToDevice, c->st));
+      if (ncb)
+        HCHK(hipMemcpyAsync(c->d_cb, c->h_cb, (size_t)ncb * sizeof(ll), hipMemcpyHostToDevice,
+                            c->st));
+      if (upload3d(c, (dir == 1) ? d->u1 : d->u2, u) ||
+          upload3d(c, c->d_fa, (dir == 1) ? wd->dzu1 : wd->dzu2) ||
+          upload2d(c, c->d_h, (dir == 1) ? g->h2au1 : g->h1au2) ||
+          upload2d(c, c->d_m, (dir == 1) ? wc->mdx : wc->mdy))
+        return EMSTR_EDEVICE;
+      if (ncb && upload2d(c, c->d_uf, (dir == 1) ? wd->u1flux : wd->u2flux))
+        return EMSTR_EDEVICE;
+    }
+    if (emstr_launch_flux3d(d, (dir == 1) ? c->d_fw1 : c->d_fw2, c->d_ksur,
+                            (dir == 1) ? d->u1 : d->u2, c->d_fa, c->d_h, c->d_m,
+                            ffsl ? c->d_fa : NULL, ncb, c->d_cb, c->d_uf, wd->dt, out, c->st))
+      return emstr_set_error(EMSTR_EDEVICE, -1, "set_flux_3d launch");
+    /* the window's array: zero, then the wet cells' faces */
+    HCHK(hipMemsetAsync(c->stage, 0, (size_t)g->sgsiz * sizeof(double), c->st));
+    if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, out, c->stage, c->st))
+      return emstr_set_error(EMSTR_EDEVICE, -1, "gather");
+    HCHK(hipMemcpyAsync(hout, c->stage, (size_t)g->sgsiz * sizeof(double),
+                        hipMemcpyDeviceToHost, c->st));
+    HCHK(hipStreamSynchronize(c->st));
+  }
   return form_fields(c);
 }
 

@@ -46,6 +46,15 @@ int emstr_launch_mass(const emstr_dev *d, const double *tr, double *colmass,
                       double *part, void *stream);
 /* w from continuity on the device (per processed column) */
 int emstr_launch_velw(const emstr_dev *d, const double *wbot, void *stream);
+/* set_flux_3d of one face direction into out [n3]: wet faces fw, the
+ * surface face layer of each column ksur [plane], face velocity u and
+ * thickness dzu [n3], face length hf and metric md [plane]; the ncb
+ * faces one cell deep (dense indices cb) take uf [plane] / dt; uvm not
+ * NULL: the FFSL copy of the mean volume fluxes                        */
+int emstr_launch_flux3d(const emstr_dev *d, const unsigned char *fw, const int *ksur,
+                        const double *u, const double *dzu, const double *hf, const double *md,
+                        const double *uvm, int ncb, const long long *cb, const double *uf,
+                        double dt, double *out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1755,6 +1755,59 @@ __global__ void k_velw(emstr_dev d, const double *__restrict__ wbot) {
   }
 }
 
+/*-------------------------------------------------------------------*/
+/* set_flux_3d (model/hd/momentum/vel3d.c:4041-4191, VEL3D mode), one  */
+/* face direction, one thread per dense face: the window's wet faces   */
+/* (fw) carry velocity x face thickness x face length x metric factor  */
+/* in the reference's order, faces above the column's surface face     */
+/* (ksur) and every other face zero; then k_fluxcb gives the faces one */
+/* cell deep the 2D flux over the step / dt (the reference writes them */
+/* before zeroing above the surface: a listed face above it stays      */
+/* zero).  Under FFSL (uvm not NULL) the wet faces carry the mean      */
+/* volume flux and nothing else is changed.                            */
+/*-------------------------------------------------------------------*/
+__global__ void __launch_bounds__(256)
+k_flux3d(ll n3, ll plane, const unsigned char *__restrict__ fw, const int *__restrict__ ksur,
+         const double *__restrict__ u, const double *__restrict__ dzu,
+         const double *__restrict__ hf, const double *__restrict__ md,
+         const double *__restrict__ uvm, double *__restrict__ out) {
+  const ll idx = (ll)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n3) return;
+  const ll col = idx % plane;
+  const int k = (int)(idx / plane);
+  double f = 0.0;
+  if (uvm) {
+    if (fw[idx]) f = uvm[idx];
+  } else {
+    if (fw[idx]) f = u[idx] * dzu[idx] * hf[col] * md[col];
+    if (k > ksur[col]) f = 0.0;
+  }
+  out[idx] = f;
+}
+
+__global__ void __launch_bounds__(256)
+k_fluxcb(int n, const ll *__restrict__ list, ll plane, const int *__restrict__ ksur,
+         const double *__restrict__ uf, double dt, double *__restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const ll idx = list[q], col = idx % plane;
+  if ((int)(idx / plane) <= ksur[col]) out[idx] = uf[col] / dt;
+}
+
+extern "C" int emstr_launch_flux3d(const emstr_dev *d, const unsigned char *fw, const int *ksur,
+                                   const double *u, const double *dzu, const double *hf,
+                                   const double *md, const double *uvm, int ncb,
+                                   const long long *cb, const double *uf, double dt,
+                                   double *out, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (d->n3)
+    k_flux3d<<<(unsigned)((d->n3 + 255) / 256), 256, 0, st>>>(d->n3, d->plane, fw, ksur, u, dzu,
+                                                              hf, md, uvm, out);
+  if (!uvm && ncb > 0)
+    k_fluxcb<<<(ncb + 255) / 256, 256, 0, st>>>(ncb, cb, d->plane, ksur, uf, dt, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 extern "C" int emstr_launch_velw(const emstr_dev *d, const double *wbot,
                                  void *stream) {
   if (d->ncol == 0) return 0;

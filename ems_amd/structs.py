@@ -38,6 +38,7 @@ class Geometry(C.Structure):
         ("h1acell", _PD), ("h2acell", _PD), ("h1au1", _PD), ("h2au2", _PD),
         ("h2au1", _PD), ("h1au2", _PD), ("cellarea", _PD),
         ("botz", _PD), ("gridz", _PD), ("cellz", _PD),
+        ("sur_e1", _PI), ("sur_e2", _PI),
     ]
 
 
@@ -49,6 +50,7 @@ class Window(C.Structure):
         ("u1flux3d", _PD), ("u2flux3d", _PD),
         ("dzu1", _PD), ("dzu2", _PD), ("Kz", _PD),
         ("eta", _PD), ("topz", _PD), ("wbot", _PD),
+        ("u1flux", _PD), ("u2flux", _PD), ("u1vm", _PD), ("u2vm", _PD),
     ]
 
 
@@ -68,6 +70,7 @@ class WinPriv(C.Structure):
         ("v1", _PD), ("v2", _PD), ("v3", _PD), ("v4", _PD), ("v5", _PD),
         ("v6", _PD), ("v7", _PD),
         ("b1", _D), ("twin", _PI),
+        ("cbot_e1", _PI), ("cbot_e2", _PI), ("ncbot_e1", _I), ("ncbot_e2", _I),
     ]
 
 

@@ -44,6 +44,9 @@ class Context:
     def vel_w(self):
         _chk(_lib.lib().emstr_device_vel_w(self.ptr), "emstr_device_vel_w")
 
+    def set_flux_3d(self):
+        _chk(_lib.lib().emstr_device_set_flux_3d(self.ptr), "emstr_device_set_flux_3d")
+
     def upload(self):
         L = _lib.lib()
         _chk(L.emstr_upload_fields(self.ptr), "emstr_upload_fields")

@@ -106,6 +106,10 @@ typedef struct geometry {
   double *botz;  /* 2D: bed height (negative down)                    */
   double *gridz; /* 3D: height of the lower face of each cell         */
   double *cellz; /* 3D: height of the cell centre                     */
+  /* Surface e1 / e2 face of each column of w2_e1 / w2_e2 (size
+   * a2_e1 + 1 / a2_e2 + 1; set by the momentum step from the higher
+   * elevation either side of the face), read by set_flux_3d          */
+  int *sur_e1, *sur_e2;
 } geometry_t;
 
 /*-------------------------------------------------------------------*/
@@ -125,6 +129,13 @@ typedef struct window {
   double *eta;                   /* 2D elevation at t+1              */
   double *topz;                  /* 2D surface height                */
   double *wbot;                  /* 2D w at the bed                  */
+  double *u1flux, *u2flux;       /* 2D e1 / e2 volume fluxes summed
+                                    over the 2D steps of the 3D step
+                                    (m3; set_flux_3d's faces one cell
+                                    deep)                             */
+  double *u1vm, *u2vm;           /* 3D mean volume fluxes (m3/s): the
+                                    fluxes set_flux_3d copies under
+                                    FFSL                              */
 } window_t;
 
 /*-------------------------------------------------------------------*/
@@ -164,6 +175,10 @@ typedef struct win_priv {
   double b1; /* current sub-step (set in advect_diffuse)             */
   int *twin; /* order to process the windows, twin[1..nwindows]
                 (tracer_step reads wincon[1]->twin; NULL: 1..nwindows) */
+  /* e1 / e2 faces whose column is one cell deep, cbot_e1[1..ncbot_e1]
+   * (set by the momentum step; set_flux_3d gives them the 2D flux)  */
+  int *cbot_e1, *cbot_e2;
+  int ncbot_e1, ncbot_e2;
 } win_priv_t;
 
 /*-------------------------------------------------------------------*/
@@ -261,6 +276,15 @@ int emstr_device_set_dz(emstr_ctx *ctx);
 /* w from continuity on the device (model/hd/momentum/vel3d.c:4680-4711)
  * from windat->u1flux3d/u2flux3d/wbot; copies w into windat->w.         */
 int emstr_device_vel_w(emstr_ctx *ctx);
+/* set_flux_3d on the device (model/hd/momentum/vel3d.c:4041-4191, VEL3D
+ * mode, no mean fields): u1flux3d = u1 dzu1 h2au1 mdx over the wet e1
+ * faces (likewise e2), the 2D flux / dt at the faces one cell deep
+ * (cbot_e1/e2), zero above each column's surface face (sur_e1/e2) and
+ * at every other face; under FFSL the mean volume fluxes u1vm / u2vm,
+ * with the velocities of the faces past b3 zeroed on the window.  The
+ * fluxes stay on the device for the step and are copied into
+ * windat->u1flux3d / u2flux3d.                                        */
+int emstr_device_set_flux_3d(emstr_ctx *ctx);
 /* Domain tracer mass sum(tr*cellarea*dz) over s1, FP64 compensated.   */
 int emstr_tracer_mass(emstr_ctx *ctx, double *mass /* [ntr] */);
 /* emstr_tracer_mass that allocates nothing (no hipMalloc, hipFree or

@@ -1359,3 +1359,104 @@ int oracle_max_threads(void) {
   return 1;
 #endif
 }
+
+/*-------------------------------------------------------------------*/
+/* model/hd/momentum/vel3d.c:4041-4191  set_flux_3d, VEL3D mode (no  */
+/* mean fields): face volume fluxes from the face velocities.  Under */
+/* FFSL the fluxes are the mean volume fluxes u1vm / u2vm and the     */
+/* velocities of the faces past b3 are zeroed.                        */
+/*-------------------------------------------------------------------*/
+#define FFSL_SCHEME 0x080000
+void oracle_set_flux_3d(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int c, cc, cs;
+  const double *u = wd->u1, *v = wd->u2;
+  memset(wd->u1flux3d, 0, g->sgsiz * sizeof(double));
+  memset(wd->u2flux3d, 0, g->sgsiz * sizeof(double));
+  if (wc->trasc == FFSL_SCHEME) {
+    for (cc = g->b3_e1 + 1; cc <= g->n3_e1; cc++) {
+      c = g->w3_e1[cc];
+      wd->u1[c] = 0.0;
+      wd->u1vm[c] = 0.0;
+    }
+    for (cc = g->b3_e2 + 1; cc <= g->n3_e2; cc++) {
+      c = g->w3_e2[cc];
+      wd->u2[c] = 0.0;
+      wd->u2vm[c] = 0.0;
+    }
+    for (cc = 1; cc <= g->b3_e1; cc++) {
+      c = g->w3_e1[cc];
+      wd->u1flux3d[c] = wd->u1vm[c];
+    }
+    for (cc = 1; cc <= g->b3_e2; cc++) {
+      c = g->w3_e2[cc];
+      wd->u2flux3d[c] = wd->u2vm[c];
+    }
+    return;
+  }
+  for (cc = 1; cc <= g->b3_e1; cc++) {
+    c = g->w3_e1[cc];
+    cs = g->m2d[c];
+    wd->u1flux3d[c] = u[c] * wd->dzu1[c] * g->h2au1[cs] * wc->mdx[cs];
+  }
+  for (cc = 1; cc <= wc->ncbot_e1; cc++) {
+    c = wc->cbot_e1[cc];
+    cs = g->m2d[c];
+    wd->u1flux3d[c] = wd->u1flux[cs] / wd->dt;
+  }
+  for (cc = 1; cc <= g->b2_e1; cc++) {
+    c = g->w2_e1[cc];
+    cs = g->sur_e1[cc];
+    while (c < cs) {
+      wd->u1flux3d[c] = 0.0;
+      c = g->zm1[c];
+    }
+  }
+  for (cc = 1; cc <= g->b3_e2; cc++) {
+    c = g->w3_e2[cc];
+    cs = g->m2d[c];
+    wd->u2flux3d[c] = v[c] * wd->dzu2[c] * g->h1au2[cs] * wc->mdy[cs];
+  }
+  for (cc = 1; cc <= wc->ncbot_e2; cc++) {
+    c = wc->cbot_e2[cc];
+    cs = g->m2d[c];
+    wd->u2flux3d[c] = wd->u2flux[cs] / wd->dt;
+  }
+  for (cc = 1; cc <= g->b2_e2; cc++) {
+    c = g->w2_e2[cc];
+    cs = g->sur_e2[cc];
+    while (c < cs) {
+      wd->u2flux3d[c] = 0.0;
+      c = g->zm1[c];
+    }
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* model/hd/momentum/vel3d.c:4609-4711  vel_w_update, z-levels: w     */
+/* from continuity, bottom up over the processed columns of set_dz,   */
+/* from the bed velocity wbot (vel_w_bounds' result, given), without  */
+/* sources, relaxation or precipitation (waterss = 0).                */
+/*-------------------------------------------------------------------*/
+void oracle_vel_w_update(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int c, cc, cs;
+  const int *bottom = wc->i1, *sur = wc->i3;
+  memset(wd->w, 0, g->sgsiz * sizeof(double));
+  for (cc = 1; cc <= wc->vcs; cc++) {
+    double fcbot, fctop;
+    c = bottom[cc];
+    cs = g->m2d[c];
+    wd->w[c] = wd->wbot[cs];
+    fcbot = wd->w[c] * g->cellarea[cs];
+    while (c > sur[cc]) {
+      int xp1 = g->xp1[c];
+      int yp1 = g->yp1[c];
+      int zp1 = g->zp1[c];
+      fctop = fcbot + wd->u1flux3d[c] - wd->u1flux3d[xp1] + wd->u2flux3d[c] -
+              wd->u2flux3d[yp1] + 0.0;
+      wd->w[zp1] = fctop / g->cellarea[cs];
+      fcbot = fctop;
+      c = zp1;
+    }
+  }
+}
+

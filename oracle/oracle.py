@@ -41,6 +41,9 @@ def lib():
                    "oracle_tracer_step_3d_mt"):
             getattr(L, fn).argtypes = [G, W, P, C.c_int]
             getattr(L, fn).restype = C.c_int
+        for fn in ("oracle_set_flux_3d", "oracle_vel_w_update"):
+            getattr(L, fn).argtypes = [G, W, P]
+            getattr(L, fn).restype = None
         L.oracle_check_tmass.argtypes = [G, W, P, C.c_int]
         L.oracle_check_tmass.restype = C.c_double
         L.oracle_max_threads.restype = C.c_int
@@ -85,3 +88,14 @@ def ffsl_maps(case):
     n = case.sgsiz
     return ([np.ctypeslib.as_array(cl[i], shape=(n,)).copy() for i in range(6)],
             [np.ctypeslib.as_array(crf[i], shape=(n,)).copy() for i in range(6)])
+
+
+def set_flux_3d(case):
+    """set_flux_3d (model/hd/momentum/vel3d.c:4041-4191, VEL3D mode)."""
+    lib().oracle_set_flux_3d(case.gp, case.wp, case.pp)
+
+
+def vel_w_update(case):
+    """vel_w_update's continuity loop (vel3d.c:4680-4711); needs set_dz."""
+    lib().oracle_vel_w_update(case.gp, case.wp, case.pp)
+

@@ -235,16 +235,80 @@ def test_set_dz_maps_bit_exact():
     L.emstr_forget_window(a.gp)
 
 
-def test_device_vel_w_matches_host():
+def test_device_vel_w_matches_oracle():
     say("case vel_w")
     case = configs.small(seed=2, ntr=1)
     case.prepare(O.set_dz)
-    w_host = case.a3(case.wd.w).copy()
+    O.vel_w_update(case)
+    w_ora = case.a3(case.wd.w).copy()
+    case.a3(case.wd.w)[:] = 0.0
     ctx = T.Context(case)
     ctx.upload_fields()
     ctx.vel_w()
-    assert np.array_equal(case.a3(case.wd.w)[case.wet], w_host[case.wet])
+    assert np.array_equal(case.a3(case.wd.w)[case.wet], w_ora[case.wet])
     ctx.close()
+
+
+def _flux_inputs(case, rng):
+    """Some columns' surface faces one layer down and some faces one cell
+    deep with a 2D flux, in both directions."""
+    g, wd, wc = case.g, case.wd, case.wc
+    zm1 = S.view(g.zm1, case.sgsiz, np.int32)
+    m2d = S.view(g.m2d, case.sgsiz, np.int32)
+    for d in (1, 2):
+        w2 = S.view(g.w2_e1 if d == 1 else g.w2_e2, case.sgsizS, np.int32)
+        nb2 = g.b2_e1 if d == 1 else g.b2_e2
+        sur = S.view(g.sur_e1 if d == 1 else g.sur_e2, case.sgsizS, np.int32)
+        pick = rng.choice(nb2, 30, replace=False)
+        for p in pick:
+            top = w2[p + 1]
+            if zm1[top] != top and zm1[zm1[top]] != zm1[top]:
+                sur[p + 1] = zm1[top]
+        rest = [w2[p + 1] for p in range(nb2) if p not in set(pick.tolist())][:15]
+        cb = S.view(wc.cbot_e1 if d == 1 else wc.cbot_e2, case.sgsizS, np.int32)
+        cb[1:len(rest) + 1] = rest
+        if d == 1:
+            wc.ncbot_e1 = len(rest)
+        else:
+            wc.ncbot_e2 = len(rest)
+        uf = case.a2(wd.u1flux if d == 1 else wd.u2flux)
+        uf[m2d[rest]] = rng.uniform(-40.0, 40.0, len(rest)) * case.dt
+
+
+@pytest.mark.parametrize("ffsl", [False, True])
+def test_device_set_flux_3d_matches_oracle(ffsl):
+    """set_flux_3d on the device (vel3d.c:4041-4191) against the oracle,
+    bit for bit: wet faces, faces one cell deep, faces above lowered
+    surface faces; under FFSL the mean-flux copy and the zeroed
+    velocities past b3."""
+    say("case set_flux_3d, ffsl", ffsl)
+    case = configs.small(seed=8, ntr=1)
+    case.prepare(O.set_dz)
+    rng = np.random.default_rng(3)
+    _flux_inputs(case, rng)
+    if ffsl:
+        case.a3(case.wd.u1vm)[:] = rng.normal(0.0, 10.0, case.sgsiz)
+        case.a3(case.wd.u2vm)[:] = rng.normal(0.0, 10.0, case.sgsiz)
+    keep = {f: case.a3(getattr(case.wd, f)).copy() for f in ("u1", "u2", "u1vm", "u2vm")}
+    ctx = T.Context(case)
+    ctx.upload_fields()
+    if ffsl:
+        case.wc.trasc = S.FFSL
+    ctx.set_flux_3d()
+    gpu = [case.a3(case.wd.u1flux3d).copy(), case.a3(case.wd.u2flux3d).copy()]
+    gvel = [case.a3(getattr(case.wd, f)).copy() for f in ("u1", "u2", "u1vm", "u2vm")]
+    ctx.close()
+    for f, a in keep.items():
+        case.a3(getattr(case.wd, f))[:] = a
+    case.a3(case.wd.u1flux3d)[:] = 5.0
+    case.a3(case.wd.u2flux3d)[:] = 5.0
+    O.set_flux_3d(case)
+    ora = [case.a3(case.wd.u1flux3d), case.a3(case.wd.u2flux3d)]
+    for a, b in zip(gpu, ora):
+        assert np.array_equal(a, b)
+        assert np.abs(b).max() > 0
+    for a, f in zip(gvel, ("u1", "u2", "u1vm", "u2vm")):
+        assert np.array_equal(a, case.a3(getattr(case.wd, f))), f
 
 
 def test_device_mass_and_context_loop():
