@@ -44,6 +44,8 @@ PROTOS = [
     ("emstr_device_set_dz", C.c_int, [_CTX]),
     ("emstr_device_vel_w", C.c_int, [_CTX]),
     ("emstr_device_set_flux_3d", C.c_int, [_CTX]),
+    ("emstr_transport_step", C.c_int, [_CTX]),
+    ("emstr_transport_download", C.c_int, [_CTX]),
     ("emstr_profile", C.c_int, [_CTX, C.c_int]),
     ("emstr_kernel_times", C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     ("emstr_sync", C.c_int, [_CTX]),

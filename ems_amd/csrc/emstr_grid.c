@@ -353,8 +353,6 @@ emstr_grid *emstr_grid_build(int nx, int ny, int nz, const double *layers,
       gw->cellz[n] = layers[0];
     }
   }
-  free(gdir);
-  free(idx3);
 
   /* window_t */
   wd->ntr = ntr;
@@ -437,6 +435,44 @@ emstr_grid *emstr_grid_build(int nx, int ny, int nz, const double *layers,
   wc->cbot_e1 = (int *)zalloc(S, sizeof(int));
   wc->cbot_e2 = (int *)zalloc(S, sizeof(int));
   wc->ncbot_e1 = wc->ncbot_e2 = 0;
+  /* face beds: the shallower bed either side of the face (the cell's own
+   * where the neighbour is land), and the bottom face of each face
+   * column in the layer of that bed */
+  gw->botzu1 = (double *)zalloc(S, sizeof(double));
+  gw->botzu2 = (double *)zalloc(S, sizeof(double));
+  gw->bot_e1 = (int *)zalloc(S, sizeof(int));
+  gw->bot_e2 = (int *)zalloc(S, sizeof(int));
+  for (n = 1; n <= nv2; n++) {
+    const int cs = w2_t[n];
+    const int ii = s2i[cs], jj = s2j[cs];
+    for (int dir = 1; dir <= 2; dir++) {
+      const int in = (dir == 1) ? ii - 1 : ii, jn = (dir == 1) ? jj : jj - 1;
+      int kb = kbot[(size_t)jj * nx + ii];
+      double bz = gw->botz[cs];
+      if (in >= 0 && jn >= 0) {
+        const size_t qn = (size_t)jn * nx + in;
+        if (kbot[qn] < nz) {
+          const int cn = idx3[((size_t)(nz - 1) * ny + jn) * nx + in];
+          if (gw->botz[cn] > bz) bz = gw->botz[cn];
+          if (kbot[qn] > kb) kb = kbot[qn];
+        }
+      }
+      const int cb = idx3[((size_t)kb * ny + jj) * nx + ii];
+      if (dir == 1) {
+        gw->botzu1[cs] = bz;
+        gw->bot_e1[n] = cb;
+      } else {
+        gw->botzu2[cs] = bz;
+        gw->bot_e2[n] = cb;
+      }
+    }
+  }
+  wd->etab = (double *)zalloc(S, sizeof(double));
+  wd->wtop = (double *)zalloc(S, sizeof(double));
+  wd->detadt = (double *)zalloc(S, sizeof(double));
+  wc->nonlinear = 0;
+  free(gdir);
+  free(idx3);
   return G;
 #undef WET
 }
@@ -456,7 +492,7 @@ void emstr_grid_free(emstr_grid *G) {
                  g->sur_t, g->nsur_t, g->bot_t, g->m2d, g->xp1, g->xm1, g->yp1,
                  g->ym1, g->zp1, g->zm1, g->h1acell, g->h2acell, g->h1au1, g->h2au2,
                  g->h2au1, g->h1au2, g->cellarea, g->botz, g->gridz, g->cellz,
-                 g->sur_e1, g->sur_e2};
+                 g->sur_e1, g->sur_e2, g->botzu1, g->botzu2, g->bot_e1, g->bot_e2};
     for (n = 0; n < (int)(sizeof(a) / sizeof(a[0])); n++) free(a[n]);
     free(g);
   }
@@ -464,7 +500,7 @@ void emstr_grid_free(emstr_grid *G) {
     for (n = 0; n < w->ntr; n++) free(w->tr_wc[n]);
     void *a[] = {w->tr_wc, w->u1, w->u2, w->w, w->u1flux3d, w->u2flux3d, w->dzu1,
                  w->dzu2, w->Kz, w->eta, w->topz, w->wbot, w->u1flux, w->u2flux,
-                 w->u1vm, w->u2vm};
+                 w->u1vm, w->u2vm, w->etab, w->wtop, w->detadt};
     for (n = 0; n < (int)(sizeof(a) / sizeof(a[0])); n++) free(a[n]);
     free(w);
   }

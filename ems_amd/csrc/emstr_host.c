@@ -83,6 +83,13 @@ struct emstr_ctx {
   int *h_ksur, *d_ksur;    /* [plane]                                 */
   ll *h_cb, *d_cb;         /* [sgsizS] dense faces one cell deep      */
   double *d_fa, *d_h, *d_m, *d_uf;
+  int *d_ksur2;            /* the e2 faces' surface layers [plane]    */
+  /* the offline transport driver: bottom face layer of each face
+   * column, face beds, face thicknesses; knew_dev: the new surface of
+   * the last device set_dz is the device's (not the window's nsur_t) */
+  int *d_kbf1, *d_kbf2;
+  double *d_bzu1, *d_bzu2, *d_dzu1, *d_dzu2;
+  int knew_dev;
 };
 
 /*-------------------------------------------------------------------*/
@@ -459,7 +466,8 @@ void emstr_destroy(emstr_ctx *c) {
                 c->d_wbot, c->tr[0], c->tr[1], c->tr[2], c->d_fferr, d->fnu, d->fnv,
                 d->fnw, d->fw9, d->fcl, d->fcrf, d->ftmx, d->ftmy, d->ftmz, d->ftm,
                 d->ffx, d->ffy, d->ffz, c->d_face, c->d_fw1, c->d_fw2, c->d_ksur,
-                c->d_cb, c->d_fa, c->d_h, c->d_m, c->d_uf};
+                c->d_cb, c->d_fa, c->d_h, c->d_m, c->d_uf, c->d_ksur2, c->d_kbf1,
+                c->d_kbf2, c->d_bzu1, c->d_bzu2, c->d_dzu1, c->d_dzu2};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->h_kb, c->cc2col, c->col2cc, c->sedg, c->topc, c->h_ks,
@@ -988,24 +996,26 @@ int emstr_tracer_mass_noalloc(emstr_ctx *c, double *mass) {
 /*-------------------------------------------------------------------*/
 /* device set_dz and w, and the reference's index lists              */
 /*-------------------------------------------------------------------*/
-int emstr_device_set_dz(emstr_ctx *c) {
+/* set_dz on the device; knew_dev: the previous new surface is the one
+ * the device's last set_dz left, else the window's nsur_t */
+static int setdz_run(emstr_ctx *c, int knew_dev) {
   geometry_t *g = c->g;
   window_t *wd = c->wd;
   win_priv_t *wc = c->wc;
   emstr_dev *d = &c->d;
   const ll plane = d->plane;
   int cc;
-  emstr_clear_error();
   d->hmin = wc->hmin;
   if (upload2d(c, d->eta, wd->eta) || upload2d(c, d->oldeta, wc->oldeta))
     return EMSTR_EDEVICE;
-  /* the previous new surface is the host's nsur_t */
-  for (cc = 1; cc <= g->a2_t; cc++) {
-    ll col = c->cc2col[cc];
-    int ns = g->nsur_t[cc];
-    c->h_knew[col] = (ns == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[ns];
+  if (!knew_dev) {
+    for (cc = 1; cc <= g->a2_t; cc++) {
+      ll col = c->cc2col[cc];
+      int ns = g->nsur_t[cc];
+      c->h_knew[col] = (ns == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[ns];
+    }
+    HCHK(hipMemcpyAsync(d->knew, c->h_knew, plane * sizeof(int), hipMemcpyHostToDevice, c->st));
   }
-  HCHK(hipMemcpyAsync(d->knew, c->h_knew, plane * sizeof(int), hipMemcpyHostToDevice, c->st));
   if (emstr_launch_setdz(d, c->d_proc, c->st)) return emstr_set_error(EMSTR_EDEVICE, -1, "setdz");
   HCHK(hipMemcpyAsync(c->h_ks, d->ks, plane * sizeof(int), hipMemcpyDeviceToHost, c->st));
   HCHK(hipMemcpyAsync(c->h_kold, d->kold, plane * sizeof(int), hipMemcpyDeviceToHost, c->st));
@@ -1021,6 +1031,14 @@ int emstr_device_set_dz(emstr_ctx *c) {
   HCHK(hipMemcpyAsync(d->cols, c->h_cols, (size_t)ncol * sizeof(int), hipMemcpyHostToDevice, c->st));
   HCHK(hipStreamSynchronize(c->st));
   c->have_cols = 1;
+  return 0;
+}
+
+int emstr_device_set_dz(emstr_ctx *c) {
+  int r;
+  emstr_clear_error();
+  if ((r = setdz_run(c, 0))) return r;
+  c->knew_dev = 0;
   return form_fields(c);
 }
 
@@ -1076,24 +1094,6 @@ static int lists_to_host(emstr_ctx *c) {
 
 /* w from continuity on the device for the processed columns; the
  * host's windat->w receives it. */
-int emstr_device_vel_w(emstr_ctx *c) {
-  emstr_dev *d = &c->d;
-  geometry_t *g = c->g;
-  emstr_clear_error();
-  if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
-  if (upload3d(c, d->u1flux3d, c->wd->u1flux3d) || upload3d(c, d->u2flux3d, c->wd->u2flux3d) ||
-      upload2d(c, c->d_wbot, c->wd->wbot))
-    return EMSTR_EDEVICE;
-  HCHK(hipMemsetAsync(d->w, 0, (size_t)d->n3 * sizeof(double), c->st));
-  if (emstr_launch_velw(d, c->d_wbot, c->st)) return emstr_set_error(EMSTR_EDEVICE, -1, "velw");
-  HCHK(hipMemsetAsync(c->stage, 0, (size_t)g->sgsiz * sizeof(double), c->st));
-  if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, d->w, c->stage, c->st))
-    return emstr_set_error(EMSTR_EDEVICE, -1, "gather");
-  HCHK(hipMemcpyAsync(c->wd->w, c->stage, (size_t)g->sgsiz * sizeof(double),
-                      hipMemcpyDeviceToHost, c->st));
-  HCHK(hipStreamSynchronize(c->st));
-  return form_fields(c);
-}
 
 /* set_flux_3d (model/hd/momentum/vel3d.c:4041-4191, VEL3D mode) on
  * the device: the face volume fluxes from the window's face velocities
@@ -1118,6 +1118,7 @@ static int flux3d_setup(emstr_ctx *c) {
   c->d_fw1 = (unsigned char *)dalloc((size_t)n3, &ok);
   c->d_fw2 = (unsigned char *)dalloc((size_t)n3, &ok);
   c->d_ksur = (int *)dalloc((size_t)plane * sizeof(int), &ok);
+  c->d_ksur2 = (int *)dalloc((size_t)plane * sizeof(int), &ok);
   c->d_cb = (ll *)dalloc((size_t)(g->sgsiz + 1) * sizeof(ll), &ok);
   c->d_fa = (double *)dalloc((size_t)n3 * sizeof(double), &ok);
   c->d_h = (double *)dalloc((size_t)plane * sizeof(double), &ok);
@@ -1145,43 +1146,39 @@ static int flux3d_setup(emstr_ctx *c) {
   return 0;
 }
 
-int emstr_device_set_flux_3d(emstr_ctx *c) {
+/* One face direction of set_flux_3d.  ksur_dev: the surface face layer
+ * of each column is already on the device (the transport driver's
+ * set_dz_at_u), else it comes from the window's sur_e1 / sur_e2; dzu:
+ * the device face thicknesses, else the window's are uploaded; to_host:
+ * copy the fluxes into windat->u1flux3d / u2flux3d. */
+static int flux3d_dir(emstr_ctx *c, int dir, int ksur_dev, const double *dzu, int to_host) {
   geometry_t *g = c->g;
   window_t *wd = c->wd;
   win_priv_t *wc = c->wc;
   emstr_dev *d = &c->d;
   const int ffsl = (wc->trasc == FFSL);
-  int cc, dir, r;
-  emstr_clear_error();
-  if (ffsl ? (!wd->u1vm || !wd->u2vm)
-           : (!g->sur_e1 || !g->sur_e2 || !wd->dzu1 || !wd->dzu2 ||
-              ((wc->ncbot_e1 > 0 || wc->ncbot_e2 > 0) &&
-               (!wd->u1flux || !wd->u2flux || !wc->cbot_e1 || !wc->cbot_e2))))
-    return emstr_set_error(EMSTR_EARG, -1,
-                           "set_flux_3d needs sur_e1/e2, dzu1/2 and, for faces one cell deep, "
-                           "u1flux/u2flux and cbot_e1/e2 (FFSL: u1vm/u2vm)");
-  if (!c->d_fw1 && (r = flux3d_setup(c))) return r;
-  for (dir = 1; dir <= 2; dir++) {
-    const int *w3 = (dir == 1) ? g->w3_e1 : g->w3_e2;
-    const int nb = (dir == 1) ? g->b3_e1 : g->b3_e2, nn = (dir == 1) ? g->n3_e1 : g->n3_e2;
-    double *u = (dir == 1) ? wd->u1 : wd->u2;
-    double *uvm = (dir == 1) ? wd->u1vm : wd->u2vm;
-    double *out = (dir == 1) ? d->u1flux3d : d->u2flux3d;
-    double *hout = (dir == 1) ? wd->u1flux3d : wd->u2flux3d;
-    int ncb = 0;
-    if (ffsl) {
-      /* the velocities past b3 are zeroed on the window (vel3d.c:4069-4078) */
-      for (cc = nb + 1; cc <= nn; cc++) {
-        u[w3[cc]] = 0.0;
-        uvm[w3[cc]] = 0.0;
-      }
-      if (upload3d(c, c->d_fa, uvm)) return EMSTR_EDEVICE;
-    } else {
+  const int *w3 = (dir == 1) ? g->w3_e1 : g->w3_e2;
+  const int nb = (dir == 1) ? g->b3_e1 : g->b3_e2, nn = (dir == 1) ? g->n3_e1 : g->n3_e2;
+  double *u = (dir == 1) ? wd->u1 : wd->u2;
+  double *uvm = (dir == 1) ? wd->u1vm : wd->u2vm;
+  double *out = (dir == 1) ? d->u1flux3d : d->u2flux3d;
+  double *hout = (dir == 1) ? wd->u1flux3d : wd->u2flux3d;
+  int *ks = (dir == 1) ? c->d_ksur : c->d_ksur2;
+  int cc, ncb = 0;
+  if (ffsl) {
+    /* the velocities past b3 are zeroed on the window (vel3d.c:4069-4078) */
+    for (cc = nb + 1; cc <= nn; cc++) {
+      u[w3[cc]] = 0.0;
+      uvm[w3[cc]] = 0.0;
+    }
+    if (upload3d(c, c->d_fa, uvm)) return EMSTR_EDEVICE;
+  } else {
+    const int *cbl = (dir == 1) ? wc->cbot_e1 : wc->cbot_e2;
+    const int ncbl = (dir == 1) ? wc->ncbot_e1 : wc->ncbot_e2;
+    if (!ksur_dev) {
       const int *w2 = (dir == 1) ? g->w2_e1 : g->w2_e2;
       const int *sur = (dir == 1) ? g->sur_e1 : g->sur_e2;
       const int nb2 = (dir == 1) ? g->b2_e1 : g->b2_e2;
-      const int *cbl = (dir == 1) ? wc->cbot_e1 : wc->cbot_e2;
-      const int ncbl = (dir == 1) ? wc->ncbot_e1 : wc->ncbot_e2;
       for (ll q = 0; q < d->plane; q++) c->h_ksur[q] = d->nz;
       for (cc = 1; cc <= nb2; cc++) {
         const int cs = w2[cc], f = sur[cc];
@@ -1189,39 +1186,225 @@ int emstr_device_set_flux_3d(emstr_ctx *c) {
           return emstr_set_error(EMSTR_EARG, cc, "set_flux_3d: surface face of column %d", cc);
         c->h_ksur[(ll)g->s2j[cs] * d->nx + g->s2i[cs]] = g->s2k[f];
       }
-      for (cc = 1; cc <= ncbl; cc++) {
-        const int f = cbl[cc];
-        if (f < 1 || f >= g->sgsiz || c->h_dn[f] < 0)
-          return emstr_set_error(EMSTR_EUNSUPPORTED, f,
-                                 "set_flux_3d: face %d one cell deep is not a wet cell's", f);
-        c->h_cb[ncb++] = c->h_dn[f];
-      }
-      HCHK(hipMemcpyAsync(c->d_ksur, c->h_ksur, (size_t)d->plane * sizeof(int),
-                          hipMemcpyHostToDevice, c->st));
-      if (ncb)
-        HCHK(hipMemcpyAsync(c->d_cb, c->h_cb, (size_t)ncb * sizeof(ll), hipMemcpyHostToDevice,
-                            c->st));
-      if (upload3d(c, (dir == 1) ? d->u1 : d->u2, u) ||
-          upload3d(c, c->d_fa, (dir == 1) ? wd->dzu1 : wd->dzu2) ||
-          upload2d(c, c->d_h, (dir == 1) ? g->h2au1 : g->h1au2) ||
-          upload2d(c, c->d_m, (dir == 1) ? wc->mdx : wc->mdy))
-        return EMSTR_EDEVICE;
-      if (ncb && upload2d(c, c->d_uf, (dir == 1) ? wd->u1flux : wd->u2flux))
-        return EMSTR_EDEVICE;
+      HCHK(hipMemcpyAsync(ks, c->h_ksur, (size_t)d->plane * sizeof(int), hipMemcpyHostToDevice,
+                          c->st));
     }
-    if (emstr_launch_flux3d(d, (dir == 1) ? c->d_fw1 : c->d_fw2, c->d_ksur,
-                            (dir == 1) ? d->u1 : d->u2, c->d_fa, c->d_h, c->d_m,
-                            ffsl ? c->d_fa : NULL, ncb, c->d_cb, c->d_uf, wd->dt, out, c->st))
-      return emstr_set_error(EMSTR_EDEVICE, -1, "set_flux_3d launch");
+    for (cc = 1; cc <= ncbl; cc++) {
+      const int f = cbl[cc];
+      if (f < 1 || f >= g->sgsiz || c->h_dn[f] < 0)
+        return emstr_set_error(EMSTR_EUNSUPPORTED, f,
+                               "set_flux_3d: face %d one cell deep is not a wet cell's", f);
+      c->h_cb[ncb++] = c->h_dn[f];
+    }
+    if (ncb)
+      HCHK(hipMemcpyAsync(c->d_cb, c->h_cb, (size_t)ncb * sizeof(ll), hipMemcpyHostToDevice,
+                          c->st));
+    if (upload3d(c, (dir == 1) ? d->u1 : d->u2, u) ||
+        (!dzu && upload3d(c, c->d_fa, (dir == 1) ? wd->dzu1 : wd->dzu2)) ||
+        upload2d(c, c->d_h, (dir == 1) ? g->h2au1 : g->h1au2) ||
+        upload2d(c, c->d_m, (dir == 1) ? wc->mdx : wc->mdy))
+      return EMSTR_EDEVICE;
+    if (ncb && upload2d(c, c->d_uf, (dir == 1) ? wd->u1flux : wd->u2flux))
+      return EMSTR_EDEVICE;
+  }
+  if (emstr_launch_flux3d(d, (dir == 1) ? c->d_fw1 : c->d_fw2, ks, (dir == 1) ? d->u1 : d->u2,
+                          dzu ? dzu : c->d_fa, c->d_h, c->d_m, ffsl ? c->d_fa : NULL, ncb,
+                          c->d_cb, c->d_uf, wd->dt, out, c->st))
+    return emstr_set_error(EMSTR_EDEVICE, -1, "set_flux_3d launch");
+  if (to_host) {
     /* the window's array: zero, then the wet cells' faces */
     HCHK(hipMemsetAsync(c->stage, 0, (size_t)g->sgsiz * sizeof(double), c->st));
     if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, out, c->stage, c->st))
       return emstr_set_error(EMSTR_EDEVICE, -1, "gather");
     HCHK(hipMemcpyAsync(hout, c->stage, (size_t)g->sgsiz * sizeof(double),
                         hipMemcpyDeviceToHost, c->st));
-    HCHK(hipStreamSynchronize(c->st));
   }
+  HCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+static int flux3d_check(emstr_ctx *c, int need_sur) {
+  geometry_t *g = c->g;
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  const int ffsl = (wc->trasc == FFSL);
+  if (ffsl ? (!wd->u1vm || !wd->u2vm)
+           : ((need_sur && (!g->sur_e1 || !g->sur_e2 || !wd->dzu1 || !wd->dzu2)) ||
+              ((wc->ncbot_e1 > 0 || wc->ncbot_e2 > 0) &&
+               (!wd->u1flux || !wd->u2flux || !wc->cbot_e1 || !wc->cbot_e2))))
+    return emstr_set_error(EMSTR_EARG, -1,
+                           "set_flux_3d needs sur_e1/e2, dzu1/2 and, for faces one cell deep, "
+                           "u1flux/u2flux and cbot_e1/e2 (FFSL: u1vm/u2vm)");
+  if (!c->d_fw1) return flux3d_setup(c);
+  return 0;
+}
+
+int emstr_device_set_flux_3d(emstr_ctx *c) {
+  int r;
+  emstr_clear_error();
+  if ((r = flux3d_check(c, 1))) return r;
+  if ((r = flux3d_dir(c, 1, 0, NULL, 1)) || (r = flux3d_dir(c, 2, 0, NULL, 1))) return r;
   return form_fields(c);
+}
+
+/* w from continuity of the device's fluxes (no upload of the fluxes). */
+static int velw_run(emstr_ctx *c, int to_host) {
+  emstr_dev *d = &c->d;
+  geometry_t *g = c->g;
+  if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
+  if (upload2d(c, c->d_wbot, c->wd->wbot)) return EMSTR_EDEVICE;
+  HCHK(hipMemsetAsync(d->w, 0, (size_t)d->n3 * sizeof(double), c->st));
+  if (emstr_launch_velw(d, c->d_wbot, c->st)) return emstr_set_error(EMSTR_EDEVICE, -1, "velw");
+  if (to_host) {
+    HCHK(hipMemsetAsync(c->stage, 0, (size_t)g->sgsiz * sizeof(double), c->st));
+    if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, d->w, c->stage, c->st))
+      return emstr_set_error(EMSTR_EDEVICE, -1, "gather");
+    HCHK(hipMemcpyAsync(c->wd->w, c->stage, (size_t)g->sgsiz * sizeof(double),
+                        hipMemcpyDeviceToHost, c->st));
+  }
+  HCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int emstr_device_vel_w(emstr_ctx *c) {
+  int r;
+  emstr_clear_error();
+  if (!c->have_cols) return emstr_set_error(EMSTR_EARG, -1, "no set_dz data on the device");
+  if (upload3d(c, c->d.u1flux3d, c->wd->u1flux3d) ||
+      upload3d(c, c->d.u2flux3d, c->wd->u2flux3d))
+    return EMSTR_EDEVICE;
+  if ((r = velw_run(c, 1))) return r;
+  return form_fields(c);
+}
+
+/* The offline transport driver's per-window tables, made on first use:
+ * the bottom face layer of each e1 / e2 face column, the face beds. */
+static int trans_setup(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  emstr_dev *d = &c->d;
+  const ll plane = d->plane;
+  int ok = 1, cc, dir;
+  int *kbf = (int *)malloc((size_t)plane * sizeof(int));
+  if (!kbf) return emstr_set_error(EMSTR_EDEVICE, -1, "transport: host allocation");
+  for (dir = 1; dir <= 2; dir++) {
+    const int *w2 = (dir == 1) ? g->w2_e1 : g->w2_e2;
+    const int *bot = (dir == 1) ? g->bot_e1 : g->bot_e2;
+    const double *bz = (dir == 1) ? g->botzu1 : g->botzu2;
+    const int nb2 = (dir == 1) ? g->b2_e1 : g->b2_e2;
+    for (ll q = 0; q < plane; q++) kbf[q] = -1;
+    for (cc = 1; cc <= nb2; cc++) {
+      const int cs = w2[cc], cb = bot[cc];
+      const ll col = (ll)g->s2j[cs] * d->nx + g->s2i[cs];
+      const int k = (cb >= 1 && cb < g->sgsiz) ? g->s2k[cb] : -1;
+      if (k < c->h_kb[col] || k >= d->nz || (ll)g->s2j[cb] * d->nx + g->s2i[cb] != col ||
+          bz[cs] < g->layers[k] || bz[cs] > g->layers[k + 1]) {
+        free(kbf);
+        return emstr_set_error(EMSTR_EUNSUPPORTED, cs,
+                               "transport: e%d face column %d: bottom face %d is not a wet cell "
+                               "of the column holding the face bed",
+                               dir, cs, cb);
+      }
+      kbf[col] = k;
+    }
+    int **dk = (dir == 1) ? &c->d_kbf1 : &c->d_kbf2;
+    double **db = (dir == 1) ? &c->d_bzu1 : &c->d_bzu2;
+    double **dz = (dir == 1) ? &c->d_dzu1 : &c->d_dzu2;
+    *dk = (int *)dalloc((size_t)plane * sizeof(int), &ok);
+    *db = (double *)dalloc((size_t)plane * sizeof(double), &ok);
+    *dz = (double *)dalloc((size_t)d->n3 * sizeof(double), &ok);
+    if (!ok) break;
+    if (hipMemcpy(*dk, kbf, (size_t)plane * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        upload2d(c, *db, bz))
+      ok = 0;
+  }
+  free(kbf);
+  if (!ok) return emstr_set_error(EMSTR_EDEVICE, -1, "transport: device tables");
+  return 0;
+}
+
+/* One offline transport step of the window (model/hd/tracers/transport.c
+ * :94-530 for an exact grid, closed boundaries, no fills, a linear
+ * surface and CONS_W; then tracer_step and transport_post): the stored
+ * flow of the step is in windat (u1, u2, Kz, eta). */
+int emstr_transport_step(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->d;
+  int cc, r, dir;
+  emstr_clear_error();
+  if (wc->nonlinear)
+    return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                           "transport: vel_w_bounds with surface slope terms (nonlinear)");
+  if (!wd->etab || !wd->detadt || !wd->wtop || !wd->wbot || !g->botzu1 || !g->botzu2 ||
+      !g->bot_e1 || !g->bot_e2 || wd->dttr == 0.0)
+    return emstr_set_error(EMSTR_EARG, -1,
+                           "transport: needs etab, detadt, wtop, wbot, botzu1/2, bot_e1/2 and "
+                           "a tracer step dttr");
+  if ((r = flux3d_check(c, 0))) return r;
+  if (!c->d_kbf1 && (r = trans_setup(c))) return r;
+  /* vel_w_bounds, linear (vel3d.c:5144-5216; transport.c:346-354) on
+   * the window's 2D arrays: detadt, the elevation's lateral ghosts,
+   * wtop = detadt, wbot = 0, etab = eta */
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    const int cw = g->w2_t[cc];
+    wd->detadt[cw] = (wd->eta[cw] - wd->etab[cw]) / wd->dttr;
+  }
+  memset(wd->wbot, 0, (size_t)g->sgsizS * sizeof(double));
+  for (cc = 1; cc <= g->nbpt; cc++)
+    if (g->bpt[cc] <= g->sgnumS) wd->eta[g->bpt[cc]] = wd->eta[g->bin[cc]];
+  memcpy(wd->wtop, wd->detadt, (size_t)g->sgsizS * sizeof(double));
+  memcpy(wd->etab, wd->eta, (size_t)g->sgsizS * sizeof(double));
+  /* the stored fields */
+  d->dt = wd->dt;
+  if (upload3d(c, d->Kz, wd->Kz) || upload2d(c, d->eta, wd->eta)) return EMSTR_EDEVICE;
+  /* set_dz_at_u1 / u2, set_flux_3d (transport.c:438-440) */
+  for (dir = 1; dir <= 2; dir++) {
+    double *dzu = (dir == 1) ? c->d_dzu1 : c->d_dzu2;
+    if (emstr_launch_dzu(d, dir, (dir == 1) ? c->d_kbf1 : c->d_kbf2,
+                         (dir == 1) ? c->d_bzu1 : c->d_bzu2, dzu,
+                         (dir == 1) ? c->d_ksur : c->d_ksur2, c->st))
+      return emstr_set_error(EMSTR_EDEVICE, -1, "set_dz_at_u launch");
+    if ((r = flux3d_dir(c, dir, 1, dzu, 0))) return r;
+    /* horizontal diffusion and FFSL read the face thicknesses */
+    if (c->have_kh)
+      HCHK(hipMemcpyAsync((dir == 1) ? d->dzu1 : d->dzu2, dzu, (size_t)d->n3 * sizeof(double),
+                          hipMemcpyDeviceToDevice, c->st));
+  }
+  /* set_dz, vel_w_update (transport.c:457-462) */
+  if ((r = setdz_run(c, c->knew_dev))) return r;
+  c->knew_dev = 1;
+  if ((r = velw_run(c, 0))) return r;
+  if ((r = form_fields(c))) return r;
+  /* tracer_step */
+  if ((r = emstr_step(c))) return r;
+  /* transport_post: the new oldeta (transport.c:543) */
+  memcpy(wc->oldeta, wd->eta, (size_t)g->sgsizS * sizeof(double));
+  HCHK(hipMemcpyAsync(d->oldeta, d->eta, (size_t)d->plane * sizeof(double),
+                      hipMemcpyDeviceToDevice, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+/* The flow of the last transport step into the window's host arrays
+ * (dzu1, dzu2, u1flux3d, u2flux3d, w and dz at the wet cells). */
+int emstr_transport_download(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  emstr_dev *d = &c->d;
+  emstr_clear_error();
+  if (!c->d_kbf1) return emstr_set_error(EMSTR_EARG, -1, "no transport step on the device");
+  const double *src[6] = {c->d_dzu1, c->d_dzu2, d->u1flux3d, d->u2flux3d, d->w, d->dz};
+  double *dst[6] = {c->wd->dzu1, c->wd->dzu2, c->wd->u1flux3d, c->wd->u2flux3d, c->wd->w,
+                    c->wc->dz};
+  for (int i = 0; i < 6; i++) {
+    HCHK(hipMemsetAsync(c->stage, 0, (size_t)g->sgsiz * sizeof(double), c->st));
+    if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, src[i], c->stage, c->st))
+      return emstr_set_error(EMSTR_EDEVICE, -1, "gather");
+    HCHK(hipMemcpyAsync(dst[i], c->stage, (size_t)g->sgsiz * sizeof(double),
+                        hipMemcpyDeviceToHost, c->st));
+  }
+  HCHK(hipStreamSynchronize(c->st));
+  return 0;
 }
 
 /*-------------------------------------------------------------------*/

@@ -46,6 +46,12 @@ int emstr_launch_mass(const emstr_dev *d, const double *tr, double *colmass,
                       double *part, void *stream);
 /* w from continuity on the device (per processed column) */
 int emstr_launch_velw(const emstr_dev *d, const double *wbot, void *stream);
+/* set_dz_at_u1 (dir 1) / set_dz_at_u2 (dir 2) into dzu [n3] and the
+ * surface face layer of each column into ksur [plane]; kbf [plane] the
+ * bottom face layer of each face column (-1: none), botzu [plane] the
+ * face beds                                                            */
+int emstr_launch_dzu(const emstr_dev *d, int dir, const int *kbf, const double *botzu,
+                     double *dzu, int *ksur, void *stream);
 /* set_flux_3d of one face direction into out [n3]: wet faces fw, the
  * surface face layer of each column ksur [plane], face velocity u and
  * thickness dzu [n3], face length hf and metric md [plane]; the ncb

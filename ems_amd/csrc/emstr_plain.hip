@@ -1808,6 +1808,63 @@ extern "C" int emstr_launch_flux3d(const emstr_dev *d, const unsigned char *fw, 
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+/*-------------------------------------------------------------------*/
+/* set_dz_at_u1 / set_dz_at_u2, z-levels (model/hd/momentum/vel3d.c:   */
+/* 760-852), one thread per column: for a face column (kbf >= 0, the  */
+/* layer of its bottom face) the surface face is the highest layer    */
+/* whose lower face is not above the higher elevation either side (a  */
+/* land neighbour reads the column's own elevation, its lateral       */
+/* ghost's value) and not below the face bed; thicknesses from it     */
+/* down to the bottom face (the bottom one to the face bed), the       */
+/* surface face's above it, zero below.  ksur gets the surface face's  */
+/* layer (nz for a column that is not a face column).  dir 1: e1       */
+/* faces (neighbour at i-1), 2: e2 (j-1).                              */
+/*-------------------------------------------------------------------*/
+__global__ void __launch_bounds__(256)
+k_dzu(emstr_dev d, int dir, const int *__restrict__ kbf, const double *__restrict__ botzu,
+      double *__restrict__ dzu, int *__restrict__ ksur) {
+  const ll col = (ll)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= d.plane) return;
+  const int nz = d.nz, kf = kbf[col];
+  const ll P = d.plane;
+  if (kf < 0) {
+    for (int k = 0; k < nz; k++) dzu[(ll)k * P + col] = 0.0;
+    ksur[col] = nz;
+    return;
+  }
+  const ll i = col % d.nx, j = col / d.nx;
+  const ll nb = (dir == 1) ? col - 1 : col - d.nx;
+  const bool haven = (dir == 1) ? (i > 0) : (j > 0);
+  const double en = (haven && d.kb[nb] < nz) ? d.eta[nb] : d.eta[col];
+  const double ec = d.eta[col];
+  double top = (ec > en) ? ec : en;
+  top = (top > botzu[col]) ? top : botzu[col];
+  const double ms = top;
+  /* the walk down from the top layer stops at the sediment ghost */
+  int ks = nz - 1;
+  const int kb = d.kb[col];
+  while (ks >= kb && d.layers[ks] > ms) ks--;
+  ksur[col] = ks;
+  for (int k = 0; k < kf; k++) dzu[(ll)k * P + col] = 0.0;
+  double t = ms;
+  for (int k = ks; k > kf; k--) {
+    const double bt = d.layers[k];
+    dzu[(ll)k * P + col] = t - bt;
+    t = bt;
+  }
+  dzu[(ll)kf * P + col] = t - botzu[col];
+  const double st = dzu[(ll)ks * P + col];
+  for (int k = ks + 1; k < nz; k++) dzu[(ll)k * P + col] = st;
+}
+
+extern "C" int emstr_launch_dzu(const emstr_dev *d, int dir, const int *kbf,
+                                const double *botzu, double *dzu, int *ksur, void *stream) {
+  if (d->plane == 0) return 0;
+  k_dzu<<<(unsigned)((d->plane + 255) / 256), 256, 0, (hipStream_t)stream>>>(*d, dir, kbf,
+                                                                             botzu, dzu, ksur);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 extern "C" int emstr_launch_velw(const emstr_dev *d, const double *wbot,
                                  void *stream) {
   if (d->ncol == 0) return 0;

@@ -39,6 +39,7 @@ class Geometry(C.Structure):
         ("h2au1", _PD), ("h1au2", _PD), ("cellarea", _PD),
         ("botz", _PD), ("gridz", _PD), ("cellz", _PD),
         ("sur_e1", _PI), ("sur_e2", _PI),
+        ("botzu1", _PD), ("botzu2", _PD), ("bot_e1", _PI), ("bot_e2", _PI),
     ]
 
 
@@ -51,6 +52,7 @@ class Window(C.Structure):
         ("dzu1", _PD), ("dzu2", _PD), ("Kz", _PD),
         ("eta", _PD), ("topz", _PD), ("wbot", _PD),
         ("u1flux", _PD), ("u2flux", _PD), ("u1vm", _PD), ("u2vm", _PD),
+        ("etab", _PD), ("wtop", _PD), ("detadt", _PD),
     ]
 
 
@@ -71,6 +73,7 @@ class WinPriv(C.Structure):
         ("v6", _PD), ("v7", _PD),
         ("b1", _D), ("twin", _PI),
         ("cbot_e1", _PI), ("cbot_e2", _PI), ("ncbot_e1", _I), ("ncbot_e2", _I),
+        ("nonlinear", _I),
     ]
 
 

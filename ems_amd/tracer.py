@@ -47,6 +47,12 @@ class Context:
     def set_flux_3d(self):
         _chk(_lib.lib().emstr_device_set_flux_3d(self.ptr), "emstr_device_set_flux_3d")
 
+    def transport_step(self):
+        _chk(_lib.lib().emstr_transport_step(self.ptr), "emstr_transport_step")
+
+    def transport_download(self):
+        _chk(_lib.lib().emstr_transport_download(self.ptr), "emstr_transport_download")
+
     def upload(self):
         L = _lib.lib()
         _chk(L.emstr_upload_fields(self.ptr), "emstr_upload_fields")

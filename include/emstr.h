@@ -110,6 +110,11 @@ typedef struct geometry {
    * a2_e1 + 1 / a2_e2 + 1; set by the momentum step from the higher
    * elevation either side of the face), read by set_flux_3d          */
   int *sur_e1, *sur_e2;
+  /* Bed height of each e1 / e2 face (2D) and the bottom face of each
+   * column of w2_e1 / w2_e2 (size a2_e1 + 1 / a2_e2 + 1), read by
+   * set_dz_at_u1 / u2                                                 */
+  double *botzu1, *botzu2;
+  int *bot_e1, *bot_e2;
 } geometry_t;
 
 /*-------------------------------------------------------------------*/
@@ -136,6 +141,9 @@ typedef struct window {
   double *u1vm, *u2vm;           /* 3D mean volume fluxes (m3/s): the
                                     fluxes set_flux_3d copies under
                                     FFSL                              */
+  double *etab;                  /* 2D elevation at the last transport
+                                    step (vel_w_bounds' detadt)       */
+  double *wtop, *detadt;         /* 2D surface w and d(eta)/dt        */
 } window_t;
 
 /*-------------------------------------------------------------------*/
@@ -179,6 +187,7 @@ typedef struct win_priv {
    * (set by the momentum step; set_flux_3d gives them the 2D flux)  */
   int *cbot_e1, *cbot_e2;
   int ncbot_e1, ncbot_e2;
+  int nonlinear;  /* surface slope terms in vel_w_bounds (rejected)    */
 } win_priv_t;
 
 /*-------------------------------------------------------------------*/
@@ -285,6 +294,22 @@ int emstr_device_vel_w(emstr_ctx *ctx);
  * fluxes stay on the device for the step and are copied into
  * windat->u1flux3d / u2flux3d.                                        */
 int emstr_device_set_flux_3d(emstr_ctx *ctx);
+/* One offline transport step of the window on the device: what
+ * hd_step_trans (model/hd/master/hd_step.c:319-507) runs per window for
+ * an exact transport grid, closed boundaries, no fills and CONS_W:
+ * transport_step (model/hd/tracers/transport.c:94-530: the linear
+ * vel_w_bounds, set_dz_at_u1/u2, set_flux_3d, set_dz, vel_w_update),
+ * tracer_step and transport_post (oldeta = eta).  The caller puts the
+ * step's stored flow in the window (u1, u2, Kz, eta: what TRANS_DATA
+ * holds for the step); windat->etab/detadt/wtop/wbot and wincon->oldeta
+ * are updated on the host as the reference updates them; the face
+ * thicknesses, fluxes, w, dz and tracers stay on the device
+ * (emstr_transport_download / emstr_download_tracers copy them out).
+ * Surface slope terms (wincon->nonlinear) are refused.                */
+int emstr_transport_step(emstr_ctx *ctx);
+/* The last transport step's dzu1, dzu2, u1flux3d, u2flux3d, w and dz at
+ * the wet cells into the window's host arrays.                        */
+int emstr_transport_download(emstr_ctx *ctx);
 /* Domain tracer mass sum(tr*cellarea*dz) over s1, FP64 compensated.   */
 int emstr_tracer_mass(emstr_ctx *ctx, double *mass /* [ntr] */);
 /* emstr_tracer_mass that allocates nothing (no hipMalloc, hipFree or

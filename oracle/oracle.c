@@ -1460,3 +1460,90 @@ void oracle_vel_w_update(geometry_t *g, window_t *wd, win_priv_t *wc) {
   }
 }
 
+/*-------------------------------------------------------------------*/
+/* model/hd/momentum/vel3d.c:760-852  set_dz_at_u1 / set_dz_at_u2,    */
+/* z-levels: the surface face of each face column below the higher    */
+/* elevation either side (not below the face bed), the face            */
+/* thicknesses from it down to the bottom face, the surface face's     */
+/* thickness copied to the faces above it.  dir 1: e1, 2: e2.          */
+/*-------------------------------------------------------------------*/
+static double omax(double a, double b) { return (a > b) ? a : b; }
+
+void oracle_set_dz_at_u(geometry_t *g, window_t *wd, int dir) {
+  const int nb2 = (dir == 1) ? g->b2_e1 : g->b2_e2;
+  const int *w2 = (dir == 1) ? g->w2_e1 : g->w2_e2;
+  int *sur = (dir == 1) ? g->sur_e1 : g->sur_e2;
+  const int *bot = (dir == 1) ? g->bot_e1 : g->bot_e2;
+  const int *nbm = (dir == 1) ? g->xm1 : g->ym1;
+  const double *botzu = (dir == 1) ? g->botzu1 : g->botzu2;
+  double *dzu = (dir == 1) ? wd->dzu1 : wd->dzu2;
+  double *maxeta = (double *)calloc(g->sgsizS, sizeof(double));
+  int cc, c, c3, cs, cb, zm1;
+  double top, bt;
+  for (cc = 1; cc <= nb2; cc++) {
+    cs = c = w2[cc];
+    top = maxeta[cs] = omax(wd->eta[cs], wd->eta[nbm[cs]]);
+    top = maxeta[cs] = omax(top, botzu[cs]);
+    zm1 = g->zm1[c];
+    while (c != zm1 && g->gridz[c] > top) {
+      c = zm1;
+      zm1 = g->zm1[c];
+    }
+    sur[cc] = c;
+  }
+  memset(dzu, 0, g->sgsiz * sizeof(double));
+  for (cc = 1; cc <= nb2; cc++) {
+    c = c3 = sur[cc];
+    cs = g->m2d[c];
+    cb = bot[cc];
+    top = maxeta[cs];
+    while (c != cb) {
+      bt = g->gridz[c];
+      dzu[c] = top - bt;
+      top = bt;
+      c = g->zm1[c];
+    }
+    dzu[cb] = top - botzu[cs];
+    c = c3;
+    top = dzu[c];
+    while (c != cs) {
+      c = g->zp1[c];
+      dzu[c] = top;
+    }
+  }
+  free(maxeta);
+}
+
+/*-------------------------------------------------------------------*/
+/* One offline transport step of one window: transport_step           */
+/* (model/hd/tracers/transport.c:94-530) for an exact grid, closed     */
+/* boundaries, no fills, a linear surface and CONS_W, then tracer_step */
+/* (tracers.c:142-209) and transport_post (the new oldeta).  The       */
+/* stored flow of the step (u1, u2, Kz, eta) is in the window.         */
+/* Returns tracer_step_3d's code.                                      */
+/*-------------------------------------------------------------------*/
+int oracle_transport_step_mt(geometry_t *g, window_t *wd, win_priv_t *wc, int nthreads) {
+  int cc, r;
+  /* vel_w_bounds, linear (vel3d.c:5144-5216, transport.c:346-354):
+   * the elevation's lateral ghosts, wtop = detadt, wbot = 0 */
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    int c = g->w2_t[cc];
+    wd->detadt[c] = (wd->eta[c] - wd->etab[c]) / wd->dttr;
+  }
+  memset(wd->wbot, 0, g->sgsizS * sizeof(double));
+  for (cc = 1; cc <= g->nbpt; cc++)
+    if (g->bpt[cc] <= g->sgnumS) wd->eta[g->bpt[cc]] = wd->eta[g->bin[cc]];
+  memcpy(wd->wtop, wd->detadt, g->sgsizS * sizeof(double));
+  memcpy(wd->etab, wd->eta, g->sgsizS * sizeof(double));
+  /* transport.c:438-462 */
+  oracle_set_dz_at_u(g, wd, 1);
+  oracle_set_dz_at_u(g, wd, 2);
+  oracle_set_flux_3d(g, wd, wc);
+  oracle_set_dz(g, wd, wc);
+  oracle_vel_w_update(g, wd, wc);
+  r = oracle_tracer_step_3d_mt(g, wd, wc, nthreads);
+  /* transport_post (transport.c:543) */
+  memcpy(wc->oldeta, wd->eta, g->sgsizS * sizeof(double));
+  return r;
+}
+

@@ -44,6 +44,10 @@ def lib():
         for fn in ("oracle_set_flux_3d", "oracle_vel_w_update"):
             getattr(L, fn).argtypes = [G, W, P]
             getattr(L, fn).restype = None
+        L.oracle_set_dz_at_u.argtypes = [G, W, C.c_int]
+        L.oracle_set_dz_at_u.restype = None
+        L.oracle_transport_step_mt.argtypes = [G, W, P, C.c_int]
+        L.oracle_transport_step_mt.restype = C.c_int
         L.oracle_check_tmass.argtypes = [G, W, P, C.c_int]
         L.oracle_check_tmass.restype = C.c_double
         L.oracle_max_threads.restype = C.c_int
@@ -98,4 +102,15 @@ def set_flux_3d(case):
 def vel_w_update(case):
     """vel_w_update's continuity loop (vel3d.c:4680-4711); needs set_dz."""
     lib().oracle_vel_w_update(case.gp, case.wp, case.pp)
+
+
+def set_dz_at_u(case, direction):
+    """set_dz_at_u1 (direction 1) / set_dz_at_u2 (2), vel3d.c:760-852."""
+    lib().oracle_set_dz_at_u(case.gp, case.wp, direction)
+
+
+def transport_step(case, nthreads=1):
+    """One offline transport step (transport_step + tracer_step +
+    transport_post) from the stored flow in the window."""
+    return lib().oracle_transport_step_mt(case.gp, case.wp, case.pp, nthreads)
 

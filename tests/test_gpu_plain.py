@@ -350,3 +350,53 @@ def test_zero_pivot_reported_not_fatal():
     top = case.wet[(case.s2k[case.wet] == 1)]
     assert cell in set(top.tolist())
     L.emstr_forget_window(case.gp)
+
+
+def test_transport_steps_match_oracle():
+    """The offline transport driver (emstr_transport_step: the linear
+    vel_w_bounds, set_dz_at_u1/u2, set_flux_3d, set_dz, vel_w_update, the
+    tracer step, oldeta) over five steps of stored flow that changes in
+    time, with an elevation that moves across a layer face in some
+    columns: tracers, face thicknesses, fluxes, w and dz bit for bit
+    against the oracle's offline step."""
+    from tests.test_oracle_fluxes import _stored_flow
+    say("case transport steps")
+    case = configs.small(seed=11, ntr=3)
+    case.prepare(O.set_dz)
+    case.wd.dttr = case.wd.dt
+    g = case.g
+    u0 = case.a3(case.wd.u1).copy()
+    v0 = case.a3(case.wd.u2).copy()
+    dl = case.layers[-1] - case.layers[-2]
+    keep2 = {n: case.a2(getattr(case.wd, n)).copy() for n in ("eta", "etab")}
+    oldeta = case.a2(case.wc.oldeta).copy()
+    isur = {n: S.view(getattr(g, n), case.sgsizS, np.int32).copy() for n in ("sur_t", "nsur_t")}
+    t0 = case.tracers().copy()
+    ctx = T.Context(case)
+    ctx.upload()
+    for n in range(5):
+        _stored_flow(case, n, u0, v0, 0.45 * dl)
+        ctx.transport_step()
+    ctx.download()
+    ctx.transport_download()
+    gpu = case.tracers().copy()
+    flow = {n: case.a3(getattr(case.wd, n)).copy()
+            for n in ("dzu1", "dzu2", "u1flux3d", "u2flux3d", "w")}
+    flow["dz"] = case.a3(case.wc.dz).copy()
+    ctx.close()
+    case.set_tracers(t0)
+    for n, a in keep2.items():
+        case.a2(getattr(case.wd, n))[:] = a
+    case.a2(case.wc.oldeta)[:] = oldeta
+    for n, a in isur.items():
+        S.view(getattr(g, n), case.sgsizS, np.int32)[:] = a
+    for n in range(5):
+        _stored_flow(case, n, u0, v0, 0.45 * dl)
+        assert O.transport_step(case) == 0
+    ora = case.tracers()
+    assert np.abs(ora - t0).max() > 1e-3
+    assert np.array_equal(gpu, ora)
+    wet = case.wet
+    for n, a in flow.items():
+        b = case.a3(case.wc.dz) if n == "dz" else case.a3(getattr(case.wd, n))
+        assert np.array_equal(a[wet], b[wet]), n

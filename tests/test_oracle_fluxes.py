@@ -146,3 +146,108 @@ def test_vel_w_update_closes_each_column():
         assert abs(w[top] * a + inflow[-1]) <= 1e-10 * wmax * a
         ncol += 1
     assert ncol > 50
+
+
+# ---- set_dz_at_u1 / u2 and the offline transport step -------------------
+
+def _eta_pattern(case, amp, phase=0.0):
+    """A smooth elevation of amplitude amp (m) on the wet columns, with the
+    lateral ghosts holding their interior value."""
+    g = case.g
+    w2 = S.view(g.w2_t, case.sgsizS, np.int32)[1:g.v2_t + 1]
+    i = case.s2i[w2].astype(float)
+    j = case.s2j[w2].astype(float)
+    eta = case.a2(case.wd.eta)
+    eta[w2] = amp * np.sin(0.35 * i + phase) * np.cos(0.27 * j - phase)
+    return w2
+
+
+def test_set_dz_at_u_column_geometry():
+    """Per face column: the surface face is the layer holding the higher
+    elevation either side (not below the face bed), the thicknesses from
+    it to the bottom face telescope to (that elevation - face bed), the
+    faces above carry the surface face's thickness, those below zero."""
+    case = _case(seed=7)
+    g = case.g
+    dl = case.layers[-1] - case.layers[-2]
+    _eta_pattern(case, 1.4 * dl)
+    eta = case.a2(case.wd.eta)
+    bpt = S.view(g.bpt, g.nbpt + 1, np.int32)[1:]
+    bin_ = S.view(g.bin, g.nbpt + 1, np.int32)[1:]
+    sm = bpt <= g.sgnumS
+    eta[bpt[sm]] = eta[bin_[sm]]
+    zm1 = S.view(g.zm1, case.sgsiz, np.int32)
+    zp1 = S.view(g.zp1, case.sgsiz, np.int32)
+    crossed = 0
+    for d in (1, 2):
+        O.set_dz_at_u(case, d)
+        dzu = case.a3(case.wd.dzu1 if d == 1 else case.wd.dzu2)
+        nb2 = g.b2_e1 if d == 1 else g.b2_e2
+        w2 = S.view(g.w2_e1 if d == 1 else g.w2_e2, case.sgsizS, np.int32)[1:nb2 + 1]
+        sur = S.view(g.sur_e1 if d == 1 else g.sur_e2, case.sgsizS, np.int32)[1:nb2 + 1]
+        bot = S.view(g.bot_e1 if d == 1 else g.bot_e2, case.sgsizS, np.int32)[1:nb2 + 1]
+        bz = case.a2(g.botzu1 if d == 1 else g.botzu2)
+        nbm = S.view(g.xm1 if d == 1 else g.ym1, case.sgsiz, np.int32)
+        for cs, su, cb in zip(w2, sur, bot):
+            top = max(eta[cs], eta[nbm[cs]], bz[cs])
+            k = case.s2k[su]
+            assert case.layers[k] <= top and (k == case.nz - 1 or top < case.layers[k + 1])
+            crossed += int(k < case.nz - 1)
+            col = []
+            c = su
+            while True:
+                col.append(c)
+                if c == cb:
+                    break
+                c = zm1[c]
+            assert abs(dzu[col].sum() - (top - bz[cs])) <= 1e-12 * max(1.0, top - bz[cs])
+            c = su
+            while c != cs:
+                c = zp1[c]
+                assert dzu[c] == dzu[su]
+            c = zm1[cb]
+            while zm1[c] != c:
+                assert dzu[c] == 0.0
+                c = zm1[c]
+    assert crossed > 10
+
+
+def _stored_flow(case, n, u0, v0, amp):
+    """The stored flow of transport step n: velocities modulated in time,
+    an elevation pattern moving in phase."""
+    s = 1.0 + 0.3 * np.sin(0.9 * n)
+    case.a3(case.wd.u1)[:] = u0 * s
+    case.a3(case.wd.u2)[:] = v0 * (2.0 - s)
+    _eta_pattern(case, amp, phase=0.4 * n)
+
+
+def test_transport_step_steady_flow_is_the_online_step():
+    """With the stored flow equal to the window's prepared fields (flat
+    surface), an offline step recomputes the face thicknesses, fluxes and
+    w that the prepared fields hold, so it equals the online tracer step
+    to rounding."""
+    case = _case(seed=9)
+    case.wd.dttr = case.wd.dt
+    t0 = case.tracers().copy()
+    O.step(case)
+    online = case.tracers().copy()
+    case.set_tracers(t0)
+    assert O.transport_step(case) == 0
+    off = case.tracers()
+    assert np.abs(online - t0).max() > 1e-3
+    assert np.abs(off - online).max() <= 1e-12 * np.abs(online).max()
+
+
+def test_transport_steps_conserve_mass_with_time_varying_flow():
+    case = _case(seed=10)
+    case.wd.dttr = case.wd.dt
+    u0 = case.a3(case.wd.u1).copy()
+    v0 = case.a3(case.wd.u2).copy()
+    m0 = np.array([O.tmass(case, n) for n in range(case.ntr)])
+    t0 = case.tracers().copy()
+    for n in range(5):
+        _stored_flow(case, n, u0, v0, 0.0)
+        assert O.transport_step(case) == 0
+    m1 = np.array([O.tmass(case, n) for n in range(case.ntr)])
+    assert np.abs(case.tracers() - t0).max() > 1e-3
+    assert np.all(np.abs(m1 - m0) <= 1e-12 * np.abs(m0))
