@@ -211,12 +211,12 @@ def main():
             tc.upload()
             best, trials = autotune.tune_context(tc)
             tc.close()
-            c.tune(best[0], best[1], 0)
-            tuned = {"block": [best[0], 64 // best[0]], "strip": best[1],
-                     "ms_per_step": best[2],
-                     "trials": [[t[0], t[1], round(t[2], 4)] for t in trials]}
-            print("bench: autotune -> block %d x %d, strip %d (%.4f ms/step)"
-                  % (best[0], 64 // best[0], best[1], best[2]), file=sys.stderr, flush=True)
+            c.tune(best[0], best[1], best[2])
+            tuned = {"block": [best[0], 4], "strip": best[1], "depth_bins": best[2],
+                     "ms_per_step": best[3],
+                     "trials": [[t[0], t[1], t[2], round(t[3], 4)] for t in trials]}
+            print("bench: autotune -> block %d x 4, strip %d, %d depth bins (%.4f ms/step)"
+                  % (best[0], best[1], best[2], best[3]), file=sys.stderr, flush=True)
         m0 = c.mass()
         c.step(W)
         ms = timed_steps(c, ev, K)

@@ -11,7 +11,11 @@ shape parameters change its speed but not its results:
     strips, row by row inside a strip, and the sequence is cut into 8
     work-balanced chunks, one per XCD; the strip width sets how far apart
     a block and the block holding its south halo rows run on one XCD
-    (L2 reuse) against how many halo columns fall between XCDs.
+    (L2 reuse) against how many halo columns fall between XCDs;
+  * the depth bins (grid): the blocks of each XCD's chunk are ordered by
+    the layers their waves walk, in that many bins, deepest first (1: the
+    strip order), so long walks start first and short ones fill the end
+    of the chunk, at the cost of the strip order's halo reuse.
 Which is fastest depends on the window's shape and depth distribution,
 so they are picked by timing a few steps of each candidate with HIP
 events on the context's own stream.  Results do not depend on the shape
@@ -29,6 +33,7 @@ from . import tracer as T
 
 BLOCKS = [16]
 STRIPS = [256, 64, 128, 512, 1024]
+BINS = [1, 2, 4, 8]
 
 
 class _Events:
@@ -61,19 +66,20 @@ class _Events:
 
 
 def shape(ctx):
-    """(block width, strip width, workgroups per tracer pass, same) of a
-    batched context."""
+    """(block width, strip width, depth bins, workgroups per tracer group)
+    of a batched context."""
     v = [C.c_int() for _ in range(4)]
     T._chk(_lib.lib().emstr_launch_shape(ctx.ptr, *[C.byref(x) for x in v]),
            "emstr_launch_shape")
     return tuple(x.value for x in v)
 
 
-def tune_context(ctx, steps=4, blocks=BLOCKS, strips=STRIPS):
+def tune_context(ctx, steps=4, blocks=BLOCKS, strips=STRIPS, bins=BINS):
     """Times `steps` steps (an even number: one graph replay holds two) of
-    every block width in `blocks` at the first strip width, then the best
-    block at every strip width; applies the fastest and returns (best,
-    trials) where best is (block width, strip width, ms_per_step)."""
+    every block width in `blocks` at the first strip width and bin count,
+    then the best block at every strip width, then the best of those at
+    every bin count; applies the fastest and returns (best, trials) where
+    best is (block width, strip width, depth bins, ms_per_step)."""
     if ctx.form != T.BATCH_F32:
         raise ValueError("tune_context: batched contexts only")
     steps = max(2, steps // 2 * 2)
@@ -81,21 +87,24 @@ def tune_context(ctx, steps=4, blocks=BLOCKS, strips=STRIPS):
     ev = _Events()
     trials = []
 
-    def run(tx, sw):
-        T._chk(L.emstr_tune(ctx.ptr, tx, sw, 0, 1), "emstr_tune")
+    def run(tx, sw, nb):
+        T._chk(L.emstr_tune(ctx.ptr, tx, sw, nb, 1), "emstr_tune")
         ctx.step(2)  # captures the graph for this shape
         ctx.sync()
         ms = ev.time(ctx.stream, lambda: ctx.step(steps)) / steps
-        trials.append((tx, sw, ms))
+        trials.append((tx, sw, nb, ms))
         return ms
 
     for tx in blocks:
-        run(tx, strips[0])
-    tx = min(trials, key=lambda r: r[2])[0]
+        run(tx, strips[0], bins[0])
+    tx = min(trials, key=lambda r: r[3])[0]
     for sw in strips[1:]:
-        run(tx, sw)
-    best = min(trials, key=lambda r: r[2])
-    T._chk(L.emstr_tune(ctx.ptr, best[0], best[1], 0, 1), "emstr_tune")
+        run(tx, sw, bins[0])
+    sw = min(trials, key=lambda r: r[3])[1]
+    for nb in bins[1:]:
+        run(tx, sw, nb)
+    best = min(trials, key=lambda r: r[3])
+    T._chk(L.emstr_tune(ctx.ptr, best[0], best[1], best[2], 1), "emstr_tune")
     ctx.step(2)
     ctx.sync()
     ev.close()

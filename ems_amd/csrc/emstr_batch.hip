@@ -105,6 +105,7 @@ struct emstr_batch {
   /* block lists: per XCD a sequence of blocks (i0 | j0 << 16, kmin) */
   int tx;          /* block width (columns per row)                      */
   int strip;       /* strip width in columns                             */
+  int nbins;       /* depth bins of the tile order in each XCD's chunk   */
   int *blk;        /* [16 + 2 nblk]: per XCD first block and count, then
                       (i0 | j0 << 16, kmin) per block                    */
   int nblk;
@@ -978,7 +979,12 @@ static int shape_ok(int tx) { return tx == 16; }
  * Order: vertical strips of `strip` columns, in each strip the tiles row
  * by row (so a tile's south halo rows are those of a tile processed
  * shortly before it); the sequence is cut into 8 contiguous chunks of
- * equal work (wet cells), one per XCD. */
+ * equal work (wet cells), one per XCD.  With nbins > 1 the tiles of each
+ * chunk are then binned by depth: the layers the tile's four waves walk
+ * (each from its row's deepest bed to the top) split into nbins equal
+ * ranges over the window, deepest bin first, the strip order kept
+ * inside a bin, so the longest walks start first and short ones fill
+ * the end of the chunk. */
 static int build_blocks(emstr_batch *b, const int *kb) {
   const int tx = 16, ty = 1, nxp = b->nxp, nyp = b->nyp, nz = b->nz;
   const int tty = SWAVES * ty;
@@ -1037,6 +1043,39 @@ static int build_blocks(emstr_batch *b, const int *kb) {
     seq[q] = b->boff[q];
     seq[8 + q] = b->bcnt[q];
   }
+  if (b->nbins > 1 && ns > 1) {
+    /* walk length of each tile; bins over the window's range */
+    int *len = (int *)malloc(sizeof(int) * (size_t)ns);
+    int *tmp = (int *)malloc(sizeof(int) * 2 * (size_t)ns);
+    int lmin = 4 * nz, lmax = 0;
+    for (int t = 0; t < ns; t++) {
+      const int kms = seq[17 + 2 * t];
+      int l = 0;
+      for (int w = 0; w < SWAVES; w++) {
+        const int km = (kms >> (8 * w)) & 255;
+        if (km < nz) l += nz - km;
+      }
+      len[t] = l;
+      lmin = l < lmin ? l : lmin;
+      lmax = l > lmax ? l : lmax;
+    }
+    const int span = lmax - lmin + 1, nb = b->nbins;
+    for (int q = 0; q < 8; q++) {
+      const int o = b->boff[q], n = b->bcnt[q];
+      int m = 0;
+      for (int bin = 0; bin < nb; bin++)
+        for (int t = o; t < o + n; t++) {
+          const int bt = (int)((ll)(lmax - len[t]) * nb / span);
+          if (bt != bin) continue;
+          tmp[2 * m] = seq[16 + 2 * t];
+          tmp[2 * m + 1] = seq[17 + 2 * t];
+          m++;
+        }
+      memcpy(seq + 16 + 2 * o, tmp, sizeof(int) * 2 * (size_t)n);
+    }
+    free(len);
+    free(tmp);
+  }
   if (b->blk) (void)hipFree(b->blk);
   b->blk = NULL;
   b->nblk = ns;
@@ -1086,6 +1125,7 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   }
   b->tx = 16;
   b->strip = 64;
+  b->nbins = 1;
   const size_t nt = (size_t)b->vol * d->ntr;
   bool ok = true;
   const size_t nrm = (size_t)b->vol * 4 * ((d->ntr + 3) / 4);
@@ -1131,17 +1171,20 @@ extern "C" void emstr_batch_destroy(struct emstr_batch *b) {
 extern "C" void emstr_batch_set_prof(struct emstr_batch *b, struct emstr_prof *p) { b->prof = p; }
 
 /* tile_x: tile width (16 columns, the DPP row); tile_y: strip width in
- * columns of the XCD order; grid is not used by this kernel (the grid
- * follows from the tiles) */
+ * columns of the XCD order; grid: depth bins of the tile order (the
+ * launch grid follows from the tiles) */
 extern "C" int emstr_batch_tune(struct emstr_batch *b, const int *h_kb, int tile_x, int tile_y,
                                 int grid) {
-  (void)grid;
   if (tile_x > 0) {
     if (!shape_ok(tile_x))
       return emstr_set_error(EMSTR_EARG, -1, "batched form: block width 16");
     b->tx = tile_x;
   }
   if (tile_y > 0) b->strip = tile_y;
+  if (grid > 0) {
+    if (grid > 64) return emstr_set_error(EMSTR_EARG, -1, "batched form: 1 to 64 depth bins");
+    b->nbins = grid;
+  }
   if (build_blocks(b, h_kb)) return -1;
   drop_graph(b);
   return 0;
@@ -1151,7 +1194,7 @@ extern "C" void emstr_batch_shape(const struct emstr_batch *b, int *tile_x, int 
                                   int *grid, int *grid_full) {
   *tile_x = b->tx;
   *tile_y = b->strip;
-  *grid = 8 * b->bmax;
+  *grid = b->nbins;
   *grid_full = 8 * b->bmax;
 }
 

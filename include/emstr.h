@@ -266,14 +266,16 @@ int emstr_download_tracers(emstr_ctx *ctx);
 int emstr_step(emstr_ctx *ctx);
 /* nsteps tracer steps (the batched form replays a captured hipGraph).  */
 int emstr_steps(emstr_ctx *ctx, int nsteps);
-/* Batched form launch shape: tile (tile_x x tile_y columns) of the cell
- * and column orders, workgroups of the persistent advection grid (a
- * multiple of 8; the column-solve grid follows in proportion), graph
- * replay on/off; 0 (or -1 for use_graph) keeps a value.  Results do not
- * depend on the shape.  ems_amd/autotune.py picks it by measurement. */
+/* Batched form launch shape: tile_x the tile width (16 columns, one
+ * wave's row), tile_y the strip width (columns) of the tile order dealt
+ * to the 8 XCDs, grid the number of depth bins the tiles of each XCD's
+ * chunk are ordered by (deepest first; 1: strip order), graph replay
+ * on/off; 0 (or -1 for use_graph) keeps a value.  Results do not depend
+ * on the shape.  ems_amd/autotune.py picks it by measurement. */
 int emstr_tune(emstr_ctx *ctx, int tile_x, int tile_y, int grid, int use_graph);
-/* The batched form's current shape and the advection grid that fills
- * the GPU (EMSTR_EARG for the other forms). */
+/* The batched form's current shape (tile width, strip width, depth
+ * bins) and its workgroups per tracer group (EMSTR_EARG for the other
+ * forms). */
 int emstr_launch_shape(emstr_ctx *ctx, int *tile_x, int *tile_y, int *grid, int *grid_full);
 /* Only the advection / only the vertical diffusion part.              */
 int emstr_advect_diffuse(emstr_ctx *ctx);

@@ -31,20 +31,21 @@ def test_tuned_shape_keeps_results():
     p.close()
     tc = T.Context(case, T.BATCH_F32)
     tc.upload()
-    best, trials = autotune.tune_context(tc, blocks=[16], strips=[64, 16, 32, 256])
+    best, trials = autotune.tune_context(tc, blocks=[16], strips=[64, 16, 32, 256],
+                                         bins=[1, 3, 8])
     tc.close()
-    assert len(trials) == 4 and best in trials
-    assert all(t[2] > 0 for t in trials)
+    assert len(trials) == 6 and best in trials
+    assert all(t[3] > 0 for t in trials)
     out = []
     snap = H.snapshot(case)
-    for shape in (None, best[:2], (16, 16), (16, 256), (16, 32)):
+    for shape in (None, best[:3], (16, 16, 1), (16, 256, 2), (16, 32, 5), (16, 64, 64)):
         H.restore(case, snap)
         c = T.Context(case, T.BATCH_F32)
         c.upload()
         if shape:
-            c.tune(shape[0], shape[1], 0)
-            tx, sw, g, gf = autotune.shape(c)
-            assert (tx, sw) == tuple(shape) and g == gf and g % 8 == 0
+            c.tune(shape[0], shape[1], shape[2])
+            tx, sw, nb, gf = autotune.shape(c)
+            assert (tx, sw, nb) == tuple(shape) and gf % 8 == 0
         c.step(6)
         c.download()
         out.append(case.tracers())
