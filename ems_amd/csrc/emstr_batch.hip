@@ -10,29 +10,33 @@
  * nxp = 64-multiple and nyp = 32-multiple columns (the padding is land), so
  * that a wave's block of columns (TX x 64/TX, TX consecutive x columns per
  * row) reads whole, aligned row segments.  Per cell, three float4 words of
- * tracer-independent coefficients [nz][nyp][nxp][3]: A = (qW, fW, qS, fS)
- * the transports and van Leer factors of the west and south faces, B =
- * (qB, fB, 1/V, D) those of the bottom face, the inverse volume and the net
- * FP64 volume divergence, C = (gm, gp, 1/div, ud) the normalised column
- * matrix and its LU factors; the east, north and top faces are the west,
- * south and bottom faces of the neighbouring cells.  Per column a Col2
- * record (beds of the eight stencil neighbours, surfaces, diffusion
- * extent, flags, FP64 surface volume balance).
+ * tracer-independent coefficients [nz][nyp][nxp][3]: A = (qW, GW, qS, GS)
+ * the transports of the west and south faces and the weights of their
+ * limited van Leer terms, B = (qB, GB, 1/V, D) those of the bottom face,
+ * the inverse volume and the net FP64 volume divergence, C = (gm, gp,
+ * 1/div, ud) the normalised column matrix and its LU factors; the east,
+ * north and top faces are the west, south and bottom faces of the
+ * neighbouring cells.  Wetness is folded into the words: a face whose
+ * near upwind cell is dry has a zero transport there, a face whose far
+ * upwind cell is dry a zero limiter weight.  Per column a Col2 record
+ * (surfaces, diffusion extent, flags, FP64 surface volume balance).
  *
  * Per step:
  *   k_cell2   one thread per cell: coefficients from the FP64 fields
  *             (tracers.c:496-578, 3841-3889 restated);
  *   k_col2    one thread per column: the column record and the LU factors
  *             of the column matrix;
- *   k_step2   one wave per (block of 64 columns, tracer): every lane walks
- *             its column from the bed to the top with a five-layer register
- *             window of the tracer, the horizontal neighbours of the layer
- *             read from L1/L2, and computes
- *               - van Leer face values (tracers.c:2097-2118) as anomalies
- *                 against the cell value and the sub-surface update
- *                 (tracers.c:888-925) in anomaly form
- *                   tr - (sum_f s_f (F_f - tr) q_f + tr D) / V,
- *                 kept in the wave's LDS slice,
+ *   k_step2   one workgroup per (tile of 16 x 4 columns, 4 tracers): every
+ *             lane walks its column from the bed to the top with a window
+ *             of the tracer's values and vertical differences, the x
+ *             neighbours from the lanes beside it, the y neighbours read
+ *             from L1/L2, and computes
+ *               - the sub-surface update (tracers.c:888-925) in anomaly
+ *                 form, tr - (sum_f s_f q_f (F_f - tr) + tr D) / V, each
+ *                 face's term the upwind value's difference against the
+ *                 cell times max(q, 0) or min(q, 0), plus its limited van
+ *                 Leer term G H (tracers.c:2097-2118), all from the
+ *                 stencil's differences, each formed once per layer,
  *               - the processed surface cell by surf_conc
  *                 (tracers.c:3047-3207) as an FP64 volume balance, the
  *                 layers above it folded in on the way up,
@@ -45,16 +49,15 @@
  * strips, and the tracers of one block run back to back on one XCD, so
  * its coefficients and the halo rows of the block before it are L2 hits.
  * A cell's new value t0 + (advective increment) + (diffusive increment) is
- * formed in FP64 and rounded to the 32-bit state once per step,
- * stochastically, with a deterministic hash of (cell, tracer, step), so
- * sub-ulp increments do not bias the state or the tracer mass, and results
- * do not depend on the launch shape.  The vertical differences of the walk
- * and of the column solve carry the tails (head differences plus tail
- * differences); the horizontal stencil uses the heads, whose truncation is
- * common to the cell and its neighbours.
+ * formed in FP64 and rounded to nearest to the 32-bit state once per
+ * step.  The vertical differences of the walk and of the column solve
+ * carry the tails (head differences plus tail differences); the
+ * horizontal stencil uses the heads, whose truncation is common to the
+ * cell and its neighbours.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include <stdlib.h>
 #include <string.h>
 
@@ -67,7 +70,13 @@
 #define VANLEER 0x000080
 #define IR_DMAX 1.0
 #define NZMAX 64
-#define SWAVES 4 /* waves per workgroup: four blocks of a tile, one tracer */
+#define SWAVES 4 /* rows of a tile: the four 16-lane rows of a wave */
+/* tracers per lane: a tracer group's values of a cell are TPL consecutive
+ * floats (heads) and bytes (tails), one wide load each */
+#ifndef EMSTR_TPL
+#define EMSTR_TPL 4
+#endif
+constexpr int TPL = EMSTR_TPL;
 
 typedef long long ll;
 
@@ -99,9 +108,10 @@ struct emstr_batch {
   int cur;
   float4 *co;      /* [nz][nyp][nxp][3]                                  */
   Col2 *ci;        /* [nyp][nxp]                                         */
-  float *scr;      /* refinement scratch, the tracer layout              */
+  float *scr;      /* forward-sweep values, then the solve's increments, the
+                      tracer layout                                      */
+  float *scr2;     /* the refinement's corrections, the tracer layout    */
   float *mintr, *maxtr;
-  int *stepctr;
   /* block lists: per XCD a sequence of blocks (i0 | j0 << 16, kmin) */
   int tx;          /* block width (columns per row)                      */
   int strip;       /* strip width in columns                             */
@@ -122,11 +132,19 @@ __device__ __forceinline__ float4 ld4(const float4 *p) { return *p; }
 /*-------------------------------------------------------------------*/
 /* prep: coefficients and the column record, one thread per column    */
 /*-------------------------------------------------------------------*/
-/* the factor of the limited term of the van Leer face value for
- * transport q with Courant number cn: 1 - cn if q > 0, 1 + cn if q < 0
- * (the limiter's factor 2 and the 1/2 of tracers.c:2108 cancel) */
-__device__ __forceinline__ float vlfac(double q, double cn) {
-  return (q > 0.0) ? (float)(1.0 - cn) : (q < 0.0) ? (float)(1.0 + cn) : 1.0f;
+/* The weight of the limited term of a van Leer face (tracers.c:2097-2118)
+ * in the face's flux, G = q (F - t_up) / H: with the upwind value t_up,
+ * the limiter H(a, b) = a b / (a + b) for a b > 0 (else 0) of the
+ * upwind difference a and the difference across the face b, and the
+ * factor (1 - cn) of the face value for q > 0, (1 + cn) for q < 0 (the
+ * limiter's factor 2 and the 1/2 of tracers.c:2108 cancel), G = |q| (1 -
+ * |cn|).  G is zero where the stencil's far upwind cell is dry at the
+ * face's layer: the reference reads that cell's ghost, which holds the
+ * near upwind value (tracers.c:5574-5593 laterally, the sediment ghost
+ * tracers.c:827 below the bed, the self-mapping top), so the upwind
+ * difference and the limited term vanish. */
+__device__ __forceinline__ float vlg(double q, double cn, bool farwet) {
+  return farwet ? (float)(fabs(q) * ((q > 0.0) ? 1.0 - cn : 1.0 + cn)) : 0.0f;
 }
 
 struct PArgs {
@@ -151,6 +169,10 @@ __global__ void __launch_bounds__(256) k_cell2(emstr_dev d, PArgs a, float4 *__r
   const double dz = d.dz[idx];
   const double dzb = (k > kbc) ? d.dz[idx - P] : 0.0;
   const bool we = k >= d.kb[col + 1], wn = k >= d.kb[col + nx];
+  /* the west and south neighbours (the faces' near upwind cells for q >
+   * 0) wet at layer k; their own west and south neighbours are read only
+   * then (a wet cell is not on the grid's edge) */
+  const bool ww = k >= d.kb[col - 1], ws = k >= d.kb[col - nx];
   const bool tp = (ksc >= 0 && k < ksc);
   const double qW = d.u1flux3d[idx] * dt, qS = d.u2flux3d[idx] * dt;
   const double qE = we ? d.u1flux3d[idx + 1] * dt : 0.0;
@@ -161,6 +183,11 @@ __global__ void __launch_bounds__(256) k_cell2(emstr_dev d, PArgs a, float4 *__r
   const double cW = d.u1[idx] * dt / d.h1au1[col];
   const double cS = d.u2[idx] * dt / d.h2au2[col];
   const double cB = d.w[idx] * dt / (0.5 * (dz + dzb));
+  /* far upwind cells: for q > 0 the second neighbour through the near
+   * one, for q < 0 the neighbour beyond this cell */
+  const bool fW = (qW > 0.0) ? (ww && k >= d.kb[col - 2]) : we;
+  const bool fS = (qS > 0.0) ? (ws && k >= d.kb[col - 2 * nx]) : wn;
+  const bool fB = (qB > 0.0) ? (k - 2 >= kbc) : (k + 1 < d.nz);
   float gm = 0.0f, gp = 0.0f;
   if (ksc >= 0 && k <= cs && cs - 1 >= kbc) {
     /* rows of the implicit vertical diffusion (tracers.c:3841-3889),
@@ -169,9 +196,16 @@ __global__ void __launch_bounds__(256) k_cell2(emstr_dev d, PArgs a, float4 *__r
     if (k > kbc) gm = (float)(-d.Kz[idx] / (0.5 * (dzb + dz)) / dzdt);
     if (k < cs) gp = (float)(-d.Kz[idx + P] / (0.5 * (dz + d.dz[idx + P])) / dzdt);
   }
+  /* A face whose near upwind cell is dry carries the cell's own value
+   * (its ghost's), so its anomaly term vanishes: its transport is stored
+   * as zero in the word (the net divergence keeps it).  The bed cell's
+   * bottom face reads the sediment ghost, the bed value. */
+  const bool oW = ww, oS = ws, oB = k > kbc;
   float4 *o = co + ((ll)k * a.P2 + (ll)j * a.nxp + i) * 3;
-  o[0] = make_float4((float)qW, vlfac(qW, cW), (float)qS, vlfac(qS, cS));
-  o[1] = make_float4((float)qB, vlfac(qB, cB), (dz != 0.0) ? (float)(1.0 / (area * dz)) : 0.0f,
+  o[0] = make_float4(oW ? (float)qW : 0.0f, oW ? vlg(qW, cW, fW) : 0.0f, oS ? (float)qS : 0.0f,
+                     oS ? vlg(qS, cS, fS) : 0.0f);
+  o[1] = make_float4(oB ? (float)qB : 0.0f, oB ? vlg(qB, cB, fB) : 0.0f,
+                     (dz != 0.0) ? (float)(1.0 / (area * dz)) : 0.0f,
                      (float)(qE - qW + qN - qS + qT - qB));
   o[2] = make_float4(gm, gp, 1.0f, 0.0f);
 }
@@ -309,70 +343,33 @@ __global__ void __launch_bounds__(256) k_col2(emstr_dev d, PArgs a, float4 *__re
 /*-------------------------------------------------------------------*/
 /* van Leer face values as anomalies                                  */
 /*-------------------------------------------------------------------*/
-/* Van Leer face value (tracers.c:2097-2118) for transport q through the
- * face between bm1 and c, as its anomaly F - ref.  For q < 0 the stencil
- * is mirrored, (bm2, bm1, c) -> (fp1, c, bm1), which leaves df unchanged
- * and negates the reciprocal, so both signs share one expression; f is
- * 1 - cn for q > 0 and 1 + cn for q < 0.  With q == 0 the flux term
- * vanishes whatever the value.  The upwind value's anomaly is an exact
- * FP32 difference for neighbouring values within a factor of two, so the
- * rounding left is relative to the limited term and the anomaly itself. */
-__device__ __forceinline__ float vla(float bm2, float bm1, float c, float fp1, float q, float f,
-                                     float ref) {
-  const bool pos = q > 0.0f;
-  const float p2 = pos ? bm2 : fp1, p1 = pos ? bm1 : c, p0 = pos ? c : bm1;
-  const float df = (p1 - p2) * (p0 - p1);
-  const float t = df * __builtin_amdgcn_rcpf(p0 - p2);
-  return __builtin_fmaf(f, (df > 0.0f) ? t : 0.0f, p1 - ref);
-}
-
-/* The same with the tails of the values (e*, of the four stencil values
- * and of ref): every difference is the heads' difference plus the
- * tails', so that it keeps the state's 32 bits. */
-__device__ __forceinline__ float vlad(float bm2, float bm1, float c, float fp1, float ebm2,
-                                      float ebm1, float ec, float efp1, float q, float f,
-                                      float ref, float eref) {
-  const bool pos = q > 0.0f;
-  const float p2 = pos ? bm2 : fp1, p1 = pos ? bm1 : c, p0 = pos ? c : bm1;
-  const float e2 = pos ? ebm2 : efp1, e1 = pos ? ebm1 : ec, e0 = pos ? ec : ebm1;
-  const float d12 = (p1 - p2) + (e1 - e2), d01 = (p0 - p1) + (e0 - e1);
-  const float df = d12 * d01;
-  const float t = df * __builtin_amdgcn_rcpf(d01 + d12);
-  return __builtin_fmaf(f, (df > 0.0f) ? t : 0.0f, (p1 - ref) + (e1 - eref));
-}
-
-/*-------------------------------------------------------------------*/
-/* deterministic stochastic rounding                                 */
-/*-------------------------------------------------------------------*/
-__device__ __forceinline__ uint32_t mix32(uint32_t x) {
-  x ^= x >> 16;
-  x *= 0x7feb352dU;
-  x ^= x >> 15;
-  x *= 0x846ca68bU;
-  x ^= x >> 16;
-  return x;
-}
-
-/* per-cell key: one multiply round over the cell index and the wave's
- * (tracer, step) hash */
-__device__ __forceinline__ uint32_t srkey(uint32_t cellkey, uint32_t hl) {
-  uint32_t x = cellkey ^ hl;
-  x ^= x >> 16;
-  x *= 0x7feb352dU;
-  x ^= x >> 15;
-  return x;
+/* The van Leer limiter of a face (tracers.c:2097-2118) in difference
+ * form.  For q > 0 the reference's df = (t[bm1] - t[bm2]) (t[c] -
+ * t[bm1]) and t[c] - t[bm2] are a b and a + b, a = dl the upwind
+ * difference and b = dm the difference across the face; for q < 0 the
+ * mirrored stencil (fp1, c, bm1) gives -a b / (a + b) with a = dr, the
+ * sign going into the face's weight.  H = a b / (a + b) for a b > 0,
+ * else 0 (first order).  Every difference is formed once per layer and
+ * shared by the faces that read it. */
+__device__ __forceinline__ float vlh(bool pos, float dl, float dr, float dm) {
+  const float up = pos ? dl : dr;
+  const float df = up * dm;
+  const float t = df * __builtin_amdgcn_rcpf(up + dm);
+  return (df > 0.0f) ? t : 0.0f;
 }
 
 /* Compensated state: a value is held as an FP32 head h, the value
  * truncated to 24 significant bits, and an 8-bit tail r, its next eight
  * bits, so that the state keeps 32 significant bits:
  *   x = h + sign(h) r 2^(e - 31),  e the exponent of h.
- * enc rounds the FP64 value x to 32 bits stochastically (a uniform 21-bit
- * number added to the 21 bits dropped, which are then cleared) and splits
- * it; values below the FP32 normal range keep r = 0. */
-__device__ __forceinline__ void enc(double x, uint32_t rnd, float &h, uint32_t &r) {
+ * enc rounds the FP64 value x to 32 bits, to nearest (half of the 21
+ * dropped bits added, which are then cleared), and splits it; values
+ * below the FP32 normal range keep r = 0.  One rounding per step of at
+ * most 2^-32 of the value keeps 1000 steps within 2.4e-7 even if every
+ * rounding went the same way. */
+__device__ __forceinline__ void enc(double x, float &h, uint32_t &r) {
   unsigned long long u = (unsigned long long)__double_as_longlong(x);
-  u += (unsigned long long)(rnd >> 11);
+  u += 0x100000ULL;
   u &= ~0x1FFFFFULL;
   const bool tiny = fabs(x) < 1.1754943508222875e-38;
   h = tiny ? (float)x : (float)__longlong_as_double((long long)(u & ~0x1FFFFFFFULL));
@@ -397,12 +394,11 @@ __device__ __forceinline__ uint32_t tailbyte(float h, float t) {
 /* the fused step                                                    */
 /*-------------------------------------------------------------------*/
 struct SArgs {
-  int nz, ntr, nx;      /* nx: unpadded, for the rounding keys          */
-  int nxp;
+  int nz, ntr, nxp;
   ll P2, vol;
   double hmin, dt;
   ll P;                 /* unpadded plane of the FP64 fields            */
-  int nzl;              /* layers of a wave's LDS slice                 */
+  int nx;               /* unpadded row length of the FP64 fields       */
 };
 
 /* raw buffer access with 32-bit byte offsets from a wave-uniform base
@@ -413,9 +409,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t b
 /* o: the lane's byte offset; so: a wave-uniform byte offset (a scalar
  * register), added to it (the range check covers o only: the buffers
  * carry slack beyond their last element for the reads past it) */
-__device__ __forceinline__ float bl1(__amdgpu_buffer_rsrc_t r, uint32_t o, uint32_t so = 0) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0));
-}
 __device__ __forceinline__ float2 bl2(__amdgpu_buffer_rsrc_t r, uint32_t o, uint32_t so = 0) {
   const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
   return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
@@ -429,16 +422,47 @@ __device__ __forceinline__ float4 bl4(__amdgpu_buffer_rsrc_t r, uint32_t o, uint
   return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
                      __uint_as_float(v[3]));
 }
-/* DPP moves inside rows of 16 lanes: lane l takes lane l - 1, l - 2
- * (shr) or l + 1, l + 2 (shl); a lane whose source is outside its row
- * takes old */
+/* the T values of one cell of a tracer group (one wide load) */
+template <int T>
+__device__ __forceinline__ void blt(float (&v)[T], __amdgpu_buffer_rsrc_t r, uint32_t o,
+                                    uint32_t so) {
+  if constexpr (T == 4) {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, o, so, 0);
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = __uint_as_float(x[u]);
+  } else {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
+#pragma unroll
+    for (int u = 0; u < 2; u++) v[u] = __uint_as_float(x[u]);
+  }
+}
+/* the T tail bytes of one cell of a tracer group, as one word */
+template <int T>
+__device__ __forceinline__ uint32_t blr(__amdgpu_buffer_rsrc_t r, uint32_t o, uint32_t so) {
+  if constexpr (T == 4)
+    return __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
+  else
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, o, so, 0);
+}
+template <int T> __device__ __forceinline__ void stt(float *p, const float (&v)[T]) {
+  if constexpr (T == 4)
+    *(float4 *)p = make_float4(v[0], v[1], v[2], v[3]);
+  else
+    *(float2 *)p = make_float2(v[0], v[1]);
+}
+template <int T> __device__ __forceinline__ void str(uint8_t *p, uint32_t w) {
+  if constexpr (T == 4)
+    *(uint32_t *)p = w;
+  else
+    *(uint16_t *)p = (uint16_t)w;
+}
+__device__ __forceinline__ uint32_t byteof(uint32_t w, int u) { return (w >> (8 * u)) & 0xFFu; }
+
+/* DPP moves inside rows of 16 lanes: lane l takes lane l - 1 (shr) or
+ * l + 1, l + 2 (shl); a lane whose source is outside its row takes old */
 __device__ __forceinline__ float shr1o(float old, float v) {
   return __int_as_float(
       __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x111, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float shr2o(float old, float v) {
-  return __int_as_float(
-      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x112, 0xf, 0xf, false));
 }
 __device__ __forceinline__ float shl1o(float old, float v) {
   return __int_as_float(
@@ -449,432 +473,545 @@ __device__ __forceinline__ float shl2o(float old, float v) {
       __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x102, 0xf, 0xf, false));
 }
 
-/* the loads of one layer: the column at layer k + 2 (the window's new
- * entry), the four y neighbours, the x neighbours beyond the lane's row
- * segment (edge lanes only), the cell's west/south coefficient word, the
- * north neighbour's south transport, the bottom-face word of the cell
- * above (this cell's top face) and, for the segment's last lane, the east
- * neighbour's west transport */
-struct Lay {
-  float z2, s1, s2, n1, n2, x1, x2;
-  uint32_t z2r; /* the tail of z2 */
-  float4 A, U;
-  float3 M; /* gm, gp, 1/div: the walk's part of the matrix word */
+/* the loads of one layer k: the lane's column at layer k + 2 (the
+ * window's new entry), its four y neighbours and the x neighbours beyond
+ * its row segment (edge lanes), for the T tracers of the group; the
+ * cell's west/south and bottom coefficient words, its top face (the
+ * bottom word of the cell above), the matrix row of the layer below (the
+ * forward row), the north neighbour's south face and, for the segment's
+ * last lane, the east neighbour's west face */
+template <int T> struct Lay {
+  float z2[T], s1[T], s2[T], n1[T], n2[T], x1[T], x2[T];
+  uint32_t z2r; /* the tails of z2 */
+  float4 A, B;
+  float2 U;
+  float3 M;
   float2 N, E;
 };
 
-/* One workgroup per (tile of 16 x 4 columns, group of 4 tracers): wave w
- * is row j0 + w of the tile; lane l is column i0 + l % 16 of that row for
- * tracer 4 g + l / 16.  The four tracer rows of a wave read the same
- * coefficient words (one request per cell), and a lane's x neighbours are
- * the lanes beside it (DPP).  Every lane walks its column from the row's
- * deepest bed to the top; the loads of layer k + 1 are issued before
- * layer k is computed; the LDS slice holds the advected values b of the
- * lane's column, the scratch buffer its forward-sweep values. */
-__global__ void __launch_bounds__(64 * SWAVES)
-k_step2(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
+/* One wave per (tile of 16 x 4 columns, group of T tracers): lane l is
+ * column i0 + l % 16 of row j0 + l / 16 and carries the T tracers of the
+ * group, held tracer-innermost in memory so that one wide load brings
+ * them all; the tracer-independent words (coefficients, wetness,
+ * upwind directions) are loaded and evaluated once for the T tracers.  A
+ * lane's x neighbours are the lanes beside it (DPP); every lane walks its
+ * column from the tile's deepest bed to the top, the loads of layer k +
+ * 1 issued before layer k is computed.  The advective increments go to
+ * the output column and the forward-sweep values to the scratch column
+ * until the back substitution forms each cell's value once. */
+template <int T>
+__global__ void __launch_bounds__(64)
+k_step4(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
         const uint8_t *__restrict__ rin, uint8_t *__restrict__ rout,
         const float4 *__restrict__ co, const Col2 *__restrict__ cinfo,
         const int *__restrict__ blk, const float *__restrict__ mintr,
         const float *__restrict__ maxtr, const double *__restrict__ layers,
         const double *__restrict__ Kz, const double *__restrict__ dzf,
-        float *__restrict__ scr, const int *__restrict__ stepctr) {
-  extern __shared__ float lds[];
+        float *__restrict__ scr, float *__restrict__ scr2) {
   const int xcd = blockIdx.x & 7;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int q = blockIdx.x >> 3;
-  const int NG = (a.ntr + 3) >> 2;
+  const int NG = (a.ntr + T - 1) / T;
   const int bq = q / NG, g = q - bq * NG;
   /* blk: [0..7] first tile of each XCD, [8..15] tile counts, then per
    * tile (i0 | j0 << 16, the four rows' deepest beds as bytes) */
   if (bq >= blk[8 + xcd]) return;
   const int bi = blk[xcd] + bq;
-  const int b0 = blk[16 + 2 * bi], kmin = (blk[17 + 2 * bi] >> (8 * wv)) & 255;
+  const int b0 = blk[16 + 2 * bi], kms = blk[17 + 2 * bi];
+  const int kmin = min(min(kms & 255, (kms >> 8) & 255), min((kms >> 16) & 255, kms >> 24 & 255));
   if (kmin >= a.nz) return;
   const int lane = threadIdx.x & 63;
   const int cl = lane & 15;
-  const int nr = 4 * g + (lane >> 4);
-  const bool live = nr < a.ntr;
-  const int n = live ? nr : a.ntr - 1;
-  const int i = (b0 & 0xffff) + cl, j = (b0 >> 16) + wv;
+  const int i = (b0 & 0xffff) + cl, j = (b0 >> 16) + (lane >> 4);
   const int nz = a.nz, top = nz - 1;
   const uint32_t P2 = (uint32_t)a.P2, nxp = (uint32_t)a.nxp;
   const uint32_t col = (uint32_t)j * nxp + (uint32_t)i;
   const Col2 ci = cinfo[col];
   const int kb = ci.kb, ks = ci.ks, flags = ci.flags;
-  const float mn = mintr[n], mx = maxtr[n];
-  const bool clip = mn <= 1e-5f;
-  const uint32_t hl = mix32((uint32_t)n * 0x85EBCA77U ^ (uint32_t)stepctr[0] * 0xC2B2AE3DU);
-  const uint32_t cellkey0 = (uint32_t)(((ll)j * a.nx + i) * nz);
-  const __amdgpu_buffer_rsrc_t RT = rsrc(tin, (uint32_t)(a.vol * 4 * a.ntr));
-  const __amdgpu_buffer_rsrc_t RC = rsrc(co, (uint32_t)(a.vol * 48));
-  const __amdgpu_buffer_rsrc_t RR = rsrc(rin, (uint32_t)(a.vol * 4 * NG));
-  float *OB = tout + (ll)n * a.vol;
-  /* the tails of the lane's tracer: byte 4 (k P2 + col) of the group's words */
-  uint8_t *OR = rout + (ll)g * a.vol * 4 + (lane >> 4);
-  float *SB = scr + (ll)n * a.vol;
+  /* the group's tracer bounds (wave-uniform; padding tracers take the
+   * last tracer's) */
+  float mn[T], mx[T];
+  bool clip[T];
+#pragma unroll
+  for (int u = 0; u < T; u++) {
+    const int n = min(g * T + u, a.ntr - 1);
+    mn[u] = mintr[n];
+    mx[u] = maxtr[n];
+    clip[u] = mn[u] <= 1e-5f;
+  }
+  constexpr uint32_t VB = 4 * T; /* bytes of one cell's heads */
+  const uint32_t vol = (uint32_t)a.vol;
+  const __amdgpu_buffer_rsrc_t RT = rsrc(tin, vol * 4u * (uint32_t)(NG * T));
+  const __amdgpu_buffer_rsrc_t RC = rsrc(co, vol * 48u);
+  const __amdgpu_buffer_rsrc_t RR = rsrc(rin, vol * (uint32_t)(NG * T));
+  /* element offsets of the lane's column in its group (heads: T floats,
+   * tails: T bytes per cell) */
+  const uint32_t ce = (uint32_t)g * vol + col;
+  float *OB = tout + (size_t)ce * T;
+  uint8_t *OR = rout + (size_t)ce * T;
+  float *SB = scr + (size_t)ce * T;
+  float *SE = scr2 + (size_t)ce * T;
+  /* byte offsets: of the lane's column (heads, tails), of a layer */
+  const uint32_t c4 = ce * VB, L4 = P2 * VB;
+  const uint32_t cr = ce * (uint32_t)T, LR = P2 * (uint32_t)T;
+  const uint32_t c48 = col * 48u, L48 = P2 * 48u;
   const float4 *__restrict__ CC = co + col * 3 + 2; /* the column's matrix words */
   const float *__restrict__ CU = (const float *)CC + 3; /* their ud entries */
-  /* byte offsets: of the lane's column in its tracer, of a layer, of the
-   * column's coefficients in a layer, of a coefficient layer */
-  const uint32_t c4 = (uint32_t)(n * a.vol * 4) + col * 4u, L4 = P2 * 4u;
-  const uint32_t cr = (uint32_t)(g * a.vol * 4) + col * 4u + (uint32_t)(lane >> 4);
-  /* the tail of the lane's cell at layer k, as a byte */
-  auto rb = [&](int k) {
-    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(RR, (uint32_t)k * L4 + cr, 0, 0);
-  };
-  const uint32_t c48 = col * 48u, L48 = P2 * 48u;
-  const int nw1 = ci.nbw & 255, nw2 = (ci.nbw >> 8) & 255, ne1 = (ci.nbw >> 16) & 255,
-            ne2 = (ci.nbw >> 24) & 255;
-  const int ns1 = ci.nbs & 255, ns2 = (ci.nbs >> 8) & 255, nn1 = (ci.nbs >> 16) & 255,
-            nn2 = (ci.nbs >> 24) & 255;
   const bool proc = (flags & F_PROC) != 0;
   const bool move = (flags & F_MOVE) != 0;
-  float *L = lds + wv * a.nzl * 64 + lane;
-  const uint32_t ry = nxp * 4u;
-  auto clampk = [&](int k) { return (uint32_t)min(max(k, kb), top); };
+  const bool solve = proc && !(flags & F_SINGLE);
+  const int cb = ci.cb, cs = ci.cs, kold = ci.kold;
+  const uint32_t ry = nxp * VB;
   /* the lane's offsets of its y neighbours and of the x neighbours beyond
-   * its row segment (x1: lane 0 its west, lane 1 its second west, lane 14
-   * its second east, lane 15 its east neighbour; x2: lanes 0 and 15 their
-   * second neighbours; other lanes their own cell) */
+   * its row segment (x1: lane 0 its west, lane 15 its east neighbour; x2:
+   * lane 0 its second west, lanes 14 and 15 their second east neighbours;
+   * other lanes their own cell) */
   const uint32_t os1 = c4 - ry, os2 = c4 - 2 * ry, on1 = c4 + ry, on2 = c4 + 2 * ry;
-  const uint32_t ox1 = (cl == 0) ? c4 - 4 : (cl == 1) ? c4 - 8 : (cl == 14) ? c4 + 8
-                     : (cl == 15) ? c4 + 4 : c4;
-  const uint32_t ox2 = (cl == 0) ? c4 - 8 : (cl == 15) ? c4 + 8 : c4;
-  const uint32_t oN = c48 + ry * 12u + 8u;
+  const uint32_t ox1 = (cl == 0) ? c4 - VB : (cl == 15) ? c4 + VB : c4;
+  const uint32_t ox2 = (cl == 0) ? c4 - 2 * VB : (cl >= 14) ? c4 + 2 * VB : c4;
+  const uint32_t oN = c48 + nxp * 48u + 8u;
   /* Issue the loads of layer k: every lane the same words at its own
    * offsets, the layer's offset uniform (below a lane's bed, above its
    * surface or at a neighbour that is land the words are read and not
-   * used; a neighbour that is land or below the bed at layer k is read as
-   * the cell the map walk came from, tracers.c:5574-5593, when the layer
-   * is computed). */
-  auto issue = [&](int k, Lay &y) {
-    const uint32_t kz = (uint32_t)min(k + 2, top) * L4;
-    y.z2 = bl1(RT, c4, kz);
-    y.z2r = (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(RR, cr, kz, 0);
+   * used, or used with a zero weight: the unused cells hold zeros) */
+  auto issue = [&](int k, Lay<T> &y) {
+    const uint32_t kz = (uint32_t)min(k + 2, top);
+    blt<T>(y.z2, RT, c4, kz * L4);
+    y.z2r = blr<T>(RR, cr, kz * LR);
     const uint32_t kt = (uint32_t)k * L4, kc = (uint32_t)k * L48;
     y.A = bl4(RC, c48, kc);
-    y.s1 = bl1(RT, os1, kt);
-    y.s2 = bl1(RT, os2, kt);
-    y.n1 = bl1(RT, on1, kt);
-    y.n2 = bl1(RT, on2, kt);
-    y.x1 = bl1(RT, ox1, kt);
-    y.x2 = bl1(RT, ox2, kt);
-    y.U = bl4(RC, c48 + 16u, k < top ? kc + L48 : kc);
-    y.M = bl3(RC, c48 + 32u, kc);
+    blt<T>(y.s1, RT, os1, kt);
+    blt<T>(y.n1, RT, on1, kt);
+    blt<T>(y.s2, RT, os2, kt);
+    blt<T>(y.n2, RT, on2, kt);
+    blt<T>(y.x1, RT, ox1, kt);
+    blt<T>(y.x2, RT, ox2, kt);
+    y.B = bl4(RC, c48 + 16u, kc);
+    y.U = bl2(RC, c48 + 16u, k < top ? kc + L48 : kc);
+    y.M = bl3(RC, c48 + 32u, (uint32_t)__builtin_amdgcn_readfirstlane(max(k - 1, 0)) * L48);
     y.N = bl2(RC, oN, kc);
     y.E = bl2(RC, c48 + 48u, kc);
   };
-  /* register window of the column: layers k-2 .. k+2, clamped to the bed
-   * (the sediment ghost holds the bed value) and to the top */
-  float zm2 = bl1(RT, clampk(kmin - 2) * L4 + c4), zm1 = bl1(RT, clampk(kmin - 1) * L4 + c4);
-  float t0 = bl1(RT, clampk(kmin) * L4 + c4), zp1 = bl1(RT, clampk(kmin + 1) * L4 + c4);
-  float zp2 = 0.0f;
-  /* and their tails (the vertical differences carry them) */
-  float dm2 = tail(zm2, rb((int)clampk(kmin - 2))), dm1 = tail(zm1, rb((int)clampk(kmin - 1)));
-  float d00 = tail(t0, rb((int)clampk(kmin))), dp1 = tail(zp1, rb((int)clampk(kmin + 1)));
-  float dp2 = 0.0f;
-  /* the bed cell (the window's value below the bed) */
-  const float tbed = bl1(RT, (uint32_t)min(kb, top) * L4 + c4);
-  const uint32_t rbed = rb(min(kb, top));
-  float4 Bk = bl4(RC, clampk(kmin) * L48 + c48 + 16u); /* this layer's bottom face */
-  double m2 = 0.0;    /* surface cell: anomaly terms of the volume balance */
-  float v0s = 0.0f;   /* surface cell: its old value                       */
-  float v0d = 0.0f;   /* and its tail                                      */
-  double bunc = 0.0;  /* unclipped advected value below the surface        */
-  float aT = 0.0f;    /* top-face anomaly of the layer below, against it   */
-  const int kold = ci.kold;
-  /* the column solve's forward sweep runs in the walk for the rows whose
-   * right-hand side lies below the surface: row k - 1 at layer k */
-  const bool solve = proc && !(flags & F_SINGLE);
-  const int cb = ci.cb, cs = ci.cs;
-  float dd1 = 0.0f, dd2 = 0.0f; /* advective increments of the two layers below */
-  double bks1 = 0.0, bks2 = 0.0; /* b of the two layers below the surface      */
-  float sp = 0.0f;              /* s of the last forward row            */
-  float3 Mp = make_float3(0.0f, 0.0f, 1.0f); /* matrix row below      */
-  float *Ls = L;                /* s_k of the lane's column in its LDS slice */
-  Lay ya = {}, yb = {};
-  issue(kmin, ya);
-  auto layer = [&](int k, const Lay &y) {
-    const bool has = k >= kb;
-    {
-      const bool bed = k + 2 < kb;
-      zp2 = bed ? tbed : y.z2;
-      dp2 = tail(zp2, bed ? rbed : y.z2r);
+  /* Register window of the column, per tracer: the values of layers k and
+   * k + 1 (heads and tails) and the vertical differences D(m) = x(m) -
+   * x(m - 1) of layers k - 1 .. k + 1, clamped to the top (the top layer
+   * maps to itself).  Below a lane's bed the window holds the zeros of the
+   * unused cells: every term that would read them has a zero weight (the
+   * bed cell's bottom transport, the limiter weights of faces whose far
+   * upwind cell is the sediment ghost) and they are finite. */
+  float t0[T], d00[T], zp1[T], dp1[T], Dzm1[T], Dz0[T], Dz1[T];
+  {
+    const int k0 = kmin;
+    float h0[T], h1[T], hm1[T], hm2[T];
+    blt<T>(h0, RT, c4, (uint32_t)min(k0, top) * L4);
+    blt<T>(h1, RT, c4, (uint32_t)min(k0 + 1, top) * L4);
+    blt<T>(hm1, RT, c4, (uint32_t)max(k0 - 1, 0) * L4);
+    blt<T>(hm2, RT, c4, (uint32_t)max(k0 - 2, 0) * L4);
+    const uint32_t r0 = blr<T>(RR, cr, (uint32_t)min(k0, top) * LR);
+    const uint32_t r1 = blr<T>(RR, cr, (uint32_t)min(k0 + 1, top) * LR);
+    const uint32_t rm1 = blr<T>(RR, cr, (uint32_t)max(k0 - 1, 0) * LR);
+    const uint32_t rm2 = blr<T>(RR, cr, (uint32_t)max(k0 - 2, 0) * LR);
+#pragma unroll
+    for (int u = 0; u < T; u++) {
+      t0[u] = h0[u];
+      zp1[u] = h1[u];
+      d00[u] = tail(h0[u], byteof(r0, u));
+      dp1[u] = tail(h1[u], byteof(r1, u));
+      const float zm1 = (k0 >= 1) ? hm1[u] : 0.0f, zm2 = (k0 >= 2) ? hm2[u] : 0.0f;
+      const float em1 = (k0 >= 1) ? tail(hm1[u], byteof(rm1, u)) : 0.0f;
+      const float em2 = (k0 >= 2) ? tail(hm2[u], byteof(rm2, u)) : 0.0f;
+      Dzm1[u] = (zm1 - zm2) + (em1 - em2);
+      Dz0[u] = (t0[u] - zm1) + (d00[u] - em1);
+      Dz1[u] = (zp1[u] - t0[u]) + (dp1[u] - d00[u]);
     }
-    /* x neighbours at layer k from the lanes beside this one, at the
-     * segment's ends from the loads beyond it */
-    const float l1 = shr1o(y.x1, t0), l2 = shr2o((cl == 0) ? y.x2 : y.x1, t0);
-    const float r1 = shl1o(y.x1, t0), r2 = shl2o((cl == 15) ? y.x2 : y.x1, t0);
-    /* the east face's transport and factor: the west word of the lane to
-     * the east, or of the east neighbour's word at the segment's end */
-    const float qEl = shl1o(y.E.x, y.A.x), fEl = shl1o(y.E.y, y.A.y);
-    const bool sub = proc && has && k < ks;
-    const bool srf = proc && k == ks;
-    const bool upp = proc && move && k > ks;
-    if (!proc && has && live) {
-      OB[(uint32_t)k * P2 + col] = t0;
-      OR[((uint32_t)k * P2 + col) * 4u] = (uint8_t)tailbyte(t0, d00);
+  }
+  /* per tracer: the advective increments of the two layers below, the
+   * last forward-sweep value, the bottom face's limited flux term (the top
+   * face of the layer below) */
+  float dd1[T], dd2[T], sp[T], GHp[T];
+#pragma unroll
+  for (int u = 0; u < T; u++) dd1[u] = dd2[u] = sp[u] = GHp[u] = 0.0f;
+  /* surface cell (from the layer below it on): the anomaly terms of its
+   * volume balance, its old value (head, tail), the unclipped increment of
+   * the layer below it */
+  double m2[T];
+  float v0s[T], v0d[T], duk[T];
+#pragma unroll
+  for (int u = 0; u < T; u++) {
+    m2[u] = 0.0;
+    v0s[u] = v0d[u] = duk[u] = 0.0f;
+  }
+  /* the layers where no lane of the wave is within a layer of its
+   * processed surface, walked without the surface terms */
+  int kA = proc ? ks - 1 : nz;
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) kA = min(kA, __shfl_xor(kA, m));
+  kA = __builtin_amdgcn_readfirstlane(kA);
+
+  /* one layer; SURF: the surface terms may occur */
+  auto layer = [&](auto surf_c, int k, const Lay<T> &y) {
+    constexpr bool SURF = decltype(surf_c)::value;
+    const bool has = k >= kb;
+    /* the window's new entry and the differences of every tracer */
+    float zp2[T], dp2[T], Dz2[T], Dx0[T], Dxm[T], Dxp[T], Dxp2[T];
+#pragma unroll
+    for (int u = 0; u < T; u++) {
+      zp2[u] = y.z2[u];
+      dp2[u] = tail(zp2[u], byteof(y.z2r, u));
+      Dz2[u] = (zp2[u] - zp1[u]) + (dp2[u] - dp1[u]);
+      /* x differences D(i - 1) .. D(i + 2) (D(i) = t(i) - t(i - 1)): the
+       * lanes beside this one, at the row segment's ends the loads
+       * beyond it */
+      const float l1 = shr1o(y.x1[u], t0[u]), r1 = shl1o(y.x1[u], t0[u]);
+      Dx0[u] = t0[u] - l1;
+      Dxm[u] = shr1o(y.x1[u] - y.x2[u], Dx0[u]);
+      Dxp[u] = shl1o(r1 - t0[u], Dx0[u]);
+      Dxp2[u] = shl2o(y.x2[u] - r1, Dx0[u]);
+    }
+    /* the east face's transport and limiter weight: the west word of the
+     * lane to the east, or of the east neighbour at the segment's end */
+    const float qE = shl1o(y.E.x, y.A.x), gE = shl1o(y.E.y, y.A.y);
+    const bool sub = proc && has && (!SURF || k < ks);
+    const bool srf = SURF && proc && k == ks;
+    const bool upp = SURF && proc && move && k > ks;
+    if (!proc && has) {
+      stt<T>(OB + (size_t)k * P2 * T, t0);
+      uint32_t w = 0;
+#pragma unroll
+      for (int u = 0; u < T; u++) w |= tailbyte(t0[u], d00[u]) << (8 * u);
+      str<T>(OR + (size_t)k * P2 * T, w);
     }
     if (sub || srf || upp) {
-      const bool bw1 = k >= nw1, be1 = k >= ne1;
-      const bool bw2 = bw1 && k >= nw2, be2 = be1 && k >= ne2;
-      const bool bs1 = k >= ns1, bn1 = k >= nn1;
-      const bool bs2 = bs1 && k >= ns2, bn2 = bn1 && k >= nn2;
-      const float w1 = bw1 ? l1 : t0, w2 = bw2 ? l2 : w1;
-      const float e1 = be1 ? r1 : t0, e2 = be2 ? r2 : e1;
-      const float s1 = bs1 ? y.s1 : t0, s2 = bs2 ? y.s2 : s1;
-      const float n1 = bn1 ? y.n1 : t0, n2 = bn2 ? y.n2 : n1;
-      const float qE = be1 ? qEl : 0.0f, qN = bn1 ? y.N.x : 0.0f;
-      /* (a neighbour that is land at layer k has a zero word: its factor
-       * must stay finite, the transport is zero) */
-      const float fE = be1 ? fEl : 1.0f, fN = bn1 ? y.N.y : 1.0f;
-      const float ref = upp ? v0s : t0;
-      const float AW = vla(w2, w1, t0, e1, y.A.x, y.A.y, ref);
-      const float AE = vla(w1, t0, e1, e2, qE, fE, ref);
-      const float AS = vla(s2, s1, t0, n1, y.A.z, y.A.w, ref);
-      const float AN = vla(s1, t0, n1, n2, qN, fN, ref);
-      const float hh = AE * qE - AW * y.A.x + AN * qN - AS * y.A.z;
+      const float qW = y.A.x, gW = y.A.y, qS = y.A.z, gS = y.A.w, qN = y.N.x, gN = y.N.y;
+      const bool pW = qW > 0.0f, pE = qE > 0.0f, pS = qS > 0.0f, pN = qN > 0.0f;
+      const float qWp = fmaxf(qW, 0.0f), qEm = fminf(qE, 0.0f), qSp = fmaxf(qS, 0.0f),
+                  qNm = fminf(qN, 0.0f), qBp = fmaxf(y.B.x, 0.0f);
+      /* per tracer: the sum over the lateral faces of s q (F - t0) (the
+       * upwind value's difference against the cell, then the limited
+       * terms) and the bottom face's term -qB (F - t0) */
+      float hh[T], bot[T];
+#pragma unroll
+      for (int u = 0; u < T; u++) {
+        const float Dy0 = t0[u] - y.s1[u], Dym = y.s1[u] - y.s2[u];
+        const float Dyp = y.n1[u] - t0[u], Dyp2 = y.n2[u] - y.n1[u];
+        float h = qWp * Dx0[u] + qEm * Dxp[u] + qSp * Dy0 + qNm * Dyp;
+        h += gE * vlh(pE, Dx0[u], Dxp2[u], Dxp[u]) - gW * vlh(pW, Dxm[u], Dxp[u], Dx0[u]) +
+             gN * vlh(pN, Dy0, Dyp2, Dyp) - gS * vlh(pS, Dym, Dyp, Dy0);
+        hh[u] = h;
+        bot[u] = qBp * Dz0[u] - GHp[u];
+      }
       if (upp) {
-        /* a layer above the processed surface cell (tracers.c:3117-3193) */
-        m2 -= (double)hh;
-        const double an = ((double)t0 + (double)d00) - ((double)v0s + (double)v0d);
-        if (k < kold)
-          m2 += an * (ci.area * (layers[k + 1] - layers[k]));
-        else if (k == kold)
-          m2 += an * (ci.area * (ci.osub - layers[k]));
-      } else {
-        /* the bottom face is the top face of the layer below, against
-         * this cell's value; zero at the bed (the ghost holds the bed
-         * value) */
-        const float fba = (k > kb) ? (aT + ((zm1 - t0) + (dm1 - d00))) * Bk.x : 0.0f;
-        if (sub) {
-          aT = vlad(zm1, t0, zp1, zp2, dm1, d00, dp1, dp2, y.U.x, y.U.y, t0, d00);
+        /* a layer above the processed surface cell (tracers.c:3117-3193):
+         * the lateral anomalies against the surface cell's old value */
+        const float dh = (qE - qW) + (qN - qS);
+        const double vk = (k < kold) ? ci.area * (layers[k + 1] - layers[k])
+                        : (k == kold) ? ci.area * (ci.osub - layers[k]) : 0.0;
+#pragma unroll
+        for (int u = 0; u < T; u++) {
+          m2[u] -= (double)(hh[u] + (t0[u] - v0s[u]) * dh);
+          const double an = ((double)t0[u] + (double)d00[u]) - ((double)v0s[u] + (double)v0d[u]);
+          m2[u] += an * vk;
+        }
+      } else if (sub) {
+        const float qT = y.U.x, gT = y.U.y, qTm = fminf(qT, 0.0f);
+        const bool pT = qT > 0.0f;
+        const int r = k - 1;
+        const bool fwd = solve && r >= cb;
+        const float gm = (r > cb) ? y.M.x : 0.0f;
+        float dl[T], sv[T];
+#pragma unroll
+        for (int u = 0; u < T; u++) {
+          /* the top face (the bottom face of the layer above) */
+          const float GHT = gT * vlh(pT, Dz0[u], Dz2[u], Dz1[u]);
+          const float tp = qTm * Dz1[u] + GHT;
           /* the advected value b = t0 + dlt: its increment is kept in
            * FP32 (the output column holds it until the column's values
            * are formed and rounded once, after the solve) */
-          float dlt = -((hh + aT * y.U.x - fba + (t0 + d00) * Bk.w) * Bk.z);
+          float dlt = -(((hh[u] + tp) + bot[u]) + (t0[u] + d00[u]) * y.B.w) * y.B.z;
           /* tr_bounds below the surface: the clip test in FP32 (a value
            * within an ulp of the minimum may go either way) */
-          const float du = dlt;
-          if (clip && t0 + (d00 + dlt) < mn) dlt = (mn - t0) - d00;
-          if (live) OB[(uint32_t)k * P2 + col] = dlt;
-          const int r = k - 1;
-          if (solve && r >= cb) {
-            /* r_k = -(gm (b_{k-1} - b_k) + gp (b_{k+1} - b_k)),
-             * s_k = (r_k - gm s_{k-1}) / div_k, the differences of b
-             * from those of t0 and of the increments */
-            const float gm = (r > cb) ? Mp.x : 0.0f;
-            const float dm = (r > cb) ? ((zm2 - zm1) + (dm2 - dm1)) + (dd2 - dd1) : 0.0f;
-            const float dp = ((t0 - zm1) + (d00 - dm1)) + (dlt - dd1);
-            const float rr = -(gm * dm + Mp.y * dp);
-            sp = (rr - gm * sp) * Mp.z;
-            Ls[(r - kmin) * 64] = sp;
-          }
-          if (k == ks - 1) {
-            const double T = (double)t0 + (double)d00;
-            bunc = T + (double)du;
-            bks1 = T + (double)dlt;
-            bks2 = ((double)zm1 + (double)dm1) + (double)dd1;
-          }
-          dd2 = dd1;
-          dd1 = dlt;
-        } else {
-          m2 = (double)fba - (double)hh;
-          v0s = t0;
-          v0d = d00;
+          if (SURF) duk[u] = dlt;
+          if (clip[u] && t0[u] + (d00[u] + dlt) < mn[u]) dlt = (mn[u] - t0[u]) - d00[u];
+          dl[u] = dlt;
+          /* the forward row r = k - 1: r_k = -(gm (b_{k-1} - b_k) + gp
+           * (b_{k+1} - b_k)), s_k = (r_k - gm s_{k-1}) / div_k, the
+           * differences of b from the window's and the increments' */
+          const float dm = (r > cb) ? (dd2[u] - dd1[u]) - Dzm1[u] : 0.0f;
+          const float dp = Dz0[u] + (dlt - dd1[u]);
+          const float rr = -(gm * dm + y.M.y * dp);
+          sv[u] = (rr - gm * sp[u]) * y.M.z;
+          dd2[u] = dd1[u];
+          dd1[u] = dlt;
+          GHp[u] = GHT;
+        }
+        stt<T>(OB + (size_t)k * P2 * T, dl);
+        if (fwd) {
+#pragma unroll
+          for (int u = 0; u < T; u++) sp[u] = sv[u];
+          stt<T>(SB + (size_t)r * P2 * T, sv);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < T; u++) {
+          m2[u] = -((double)hh[u] + (double)bot[u]);
+          v0s[u] = t0[u];
+          v0d[u] = d00[u];
         }
       }
     }
-    /* the next layer's bottom face: loaded with this layer (below the
-     * bed, the bed cell's word loaded at the start stays); this layer's
-     * matrix row for the next forward row */
-    if (has) {
-      Bk = y.U;
-      Mp = y.M;
-    }
     /* advance the window */
-    zm2 = zm1;
-    zm1 = t0;
-    t0 = zp1;
-    zp1 = zp2;
-    dm2 = dm1;
-    dm1 = d00;
-    d00 = dp1;
-    dp1 = dp2;
+#pragma unroll
+    for (int u = 0; u < T; u++) {
+      t0[u] = zp1[u];
+      d00[u] = dp1[u];
+      zp1[u] = zp2[u];
+      dp1[u] = dp2[u];
+      Dzm1[u] = Dz0[u];
+      Dz0[u] = Dz1[u];
+      Dz1[u] = Dz2[u];
+    }
   };
+  using NoSurf = std::integral_constant<bool, false>;
+  using Surf = std::integral_constant<bool, true>;
+  Lay<T> ya, yb;
+  issue(kmin, ya);
   int k = kmin;
+  for (; k + 1 < kA; k += 2) {
+    issue(k + 1, yb);
+    layer(NoSurf{}, k, ya);
+    issue(k + 2, ya);
+    layer(NoSurf{}, k + 1, yb);
+  }
   for (; k + 1 <= top; k += 2) {
     issue(k + 1, yb);
-    layer(k, ya);
+    layer(Surf{}, k, ya);
     issue(k + 2, ya);
-    layer(k + 1, yb);
+    layer(Surf{}, k + 1, yb);
   }
-  if (k <= top) layer(k, ya);
-  if (!proc || !live) return;
+  if (k <= top) layer(Surf{}, k, ya);
+  if (!proc) return;
 
-  /* surface value (surf_conc) in FP64, then tr_bounds at the surface */
-  double surf;
-  {
-    surf = ((double)v0s + (double)v0d) * ci.vc + m2;
-    if (ci.svol == 0.0 || (surf < 0.0 && ci.dzs <= a.hmin))
-      surf = 0.0;
-    else
-      surf /= ci.svol;
-    if (flags & F_KEEPOLD) surf = (double)v0s + (double)v0d;
-    if ((flags & F_THIN) && (surf < (double)mn || surf > (double)mx)) surf = bunc;
-    if (clip && surf < (double)mn) surf = (double)mn;
-  }
-  const uint32_t hv = hl ^ 0x5bd1e995U;
-  /* a cell's new value, rounded once to the 32-bit state */
-  auto put = [&](uint32_t o, double v, uint32_t key) {
-    float h;
-    uint32_t r;
-    enc(v, key, h, r);
-    OB[o] = h;
-    OR[o * 4u] = (uint8_t)r;
+  /* the stored value of the lane's cells at layer kk, and the increments
+   * the walk left in the output column */
+  auto told = [&](int kk, double (&v)[T]) {
+    float h[T];
+    blt<T>(h, RT, c4, (uint32_t)kk * L4);
+    const uint32_t w = blr<T>(RR, cr, (uint32_t)kk * LR);
+#pragma unroll
+    for (int u = 0; u < T; u++) v[u] = dec(h[u], byteof(w, u));
   };
-  /* the stored value of the lane's cell at layer kk */
-  auto told = [&](int kk) { return dec(bl1(RT, (uint32_t)kk * L4 + c4), rb(kk)); };
+  /* a cell's new values, rounded once to the 32-bit state */
+  auto put = [&](int kk, const double (&v)[T]) {
+    float h[T];
+    uint32_t w = 0;
+#pragma unroll
+    for (int u = 0; u < T; u++) {
+      uint32_t r;
+      enc(v[u], h[u], r);
+      w |= r << (8 * u);
+    }
+    stt<T>(OB + (size_t)kk * P2 * T, h);
+    str<T>(OR + (size_t)kk * P2 * T, w);
+  };
+  /* b of the layers below the surface that the forward rows read, and
+   * the unclipped advected value below the surface */
+  double bk1[T], bk2[T], bun[T];
+#pragma unroll
+  for (int u = 0; u < T; u++) bk1[u] = bk2[u] = bun[u] = 0.0;
+  if (ks - 1 >= kb) {
+    told(ks - 1, bk1);
+#pragma unroll
+    for (int u = 0; u < T; u++) {
+      bun[u] = bk1[u] + (double)duk[u];
+      bk1[u] += (double)OB[(size_t)(ks - 1) * P2 * T + u];
+    }
+  }
+  /* surface value (surf_conc) in FP64, then tr_bounds at the surface */
+  double surf[T];
+#pragma unroll
+  for (int u = 0; u < T; u++) {
+    double s = ((double)v0s[u] + (double)v0d[u]) * ci.vc + m2[u];
+    if (ci.svol == 0.0 || (s < 0.0 && ci.dzs <= a.hmin))
+      s = 0.0;
+    else
+      s /= ci.svol;
+    if (flags & F_KEEPOLD) s = (double)v0s[u] + (double)v0d[u];
+    if ((flags & F_THIN) && (s < (double)mn[u] || s > (double)mx[u])) s = bun[u];
+    if (clip[u] && s < (double)mn[u]) s = (double)mn[u];
+    surf[u] = s;
+  }
   /* the layers above the surface outside the solved range take the
    * surface value, rounded once */
-  float trs;
-  uint32_t trr;
-  enc(surf, srkey(cellkey0 + (uint32_t)ks, hv), trs, trr);
   const int lo = solve ? cb : top + 1, hi = solve ? cs : top;
   for (int kk = ks; kk <= top; kk++)
-    if (kk < lo || kk > hi) {
-      OB[(uint32_t)kk * P2 + col] = trs;
-      OR[((uint32_t)kk * P2 + col) * 4u] = (uint8_t)trr;
-    }
+    if (kk < lo || kk > hi) put(kk, surf);
   /* cells below the surface outside the solved range: t0 + dlt, rounded */
   for (int kk = kb; kk < min(ks, lo); kk++) {
-    const uint32_t o = (uint32_t)kk * P2 + col;
-    put(o, told(kk) + (double)OB[o], srkey(cellkey0 + (uint32_t)kk, hv));
+    double v[T];
+    told(kk, v);
+#pragma unroll
+    for (int u = 0; u < T; u++) v[u] += (double)OB[(size_t)kk * P2 * T + u];
+    put(kk, v);
   }
   if (!solve) return;
   /* the remaining forward rows (their right-hand side reaches the
-   * surface value), b below the surface from the walk's last two layers */
+   * surface value), b below the surface from the layers below it */
   {
     const int rs = max(cb, ks - 1);
+    if (rs == ks - 1 && rs > cb) {
+      told(ks - 2, bk2);
+#pragma unroll
+      for (int u = 0; u < T; u++) bk2[u] += (double)OB[(size_t)(ks - 2) * P2 * T + u];
+    }
     for (int r = rs; r <= cs; r++) {
-      const float4 M = CC[(ll)r * P2 * 3];
+      const float4 M = CC[(size_t)r * P2 * 3];
       const float gm = (r > cb) ? M.x : 0.0f;
-      const double b0 = (r >= ks) ? surf : bks1;
-      const double bm = (r - 1 >= ks) ? surf : (r - 1 == ks - 1 ? bks1 : bks2);
-      const float dm = (r > cb) ? (float)(bm - b0) : 0.0f;
-      const float dp = (r < cs) ? (float)(surf - b0) : 0.0f;
-      const float rr = -(gm * dm + M.y * dp);
-      sp = (rr - gm * sp) * M.z;
-      Ls[(r - kmin) * 64] = sp;
+      float sv[T];
+#pragma unroll
+      for (int u = 0; u < T; u++) {
+        const double b0 = (r >= ks) ? surf[u] : bk1[u];
+        const float dm = (r <= cb) ? 0.0f : (r == ks - 1) ? (float)(bk2[u] - bk1[u])
+                       : (r == ks) ? (float)(bk1[u] - surf[u]) : 0.0f;
+        const float dp = (r < cs) ? (float)(surf[u] - b0) : 0.0f;
+        const float rr = -(gm * dm + M.y * dp);
+        sp[u] = (rr - gm * sp[u]) * M.z;
+        sv[u] = sp[u];
+      }
+      stt<T>(SB + (size_t)r * P2 * T, sv);
     }
   }
-  /* Back substitution d_k = s_k - ud_{k+1} d_{k+1}, x = t0 + dlt + d
-   * (or the surface value + d) in FP64, rounded once; eight rows at a
-   * time, the t0, dlt and ud of the next eight rows loaded before this
-   * chunk's results are stored over its dlt (d goes to the scratch column
-   * in a stiff column) */
+  /* Back substitution d_k = s_k - ud_{k+1} d_{k+1}, x = t0 + dlt + d (or
+   * the surface value + d) in FP64, rounded once; two rows at a time, the
+   * next two rows' t0, dlt, s and ud loaded before this pair is stored
+   * (d goes to the scratch column in a stiff column) */
   const bool stiff = (flags & F_STIFF) != 0;
   {
-    float dn = 0.0f, udn = 0.0f;
-    float ta[8], da[8], ua[8], tb[8], db[8], ub[8];
-    uint32_t ra[8], rb2[8];
-#define EMSTR_FETCH(R0, T, RR8, D, U)                               \
-  _Pragma("unroll") for (int jj = 0; jj < 8; jj++) {                \
-    const int rc = max((R0) - jj, cb);                              \
-    T[jj] = bl1(RT, (uint32_t)rc * L4 + c4);                        \
-    RR8[jj] = rb(rc);                                               \
-    D[jj] = OB[(uint32_t)rc * P2 + col];                            \
-    U[jj] = CU[(ll)rc * P2 * 12];                                   \
-  }
-#define EMSTR_SOLVE(R0, T, RR8, D, U)                                              \
-  _Pragma("unroll") for (int jj = 0; jj < 8; jj++) {                               \
-    const int r = (R0) - jj;                                                       \
-    if (r >= cb) {                                                                 \
-      const uint32_t o = (uint32_t)r * P2 + col;                                   \
-      const float sk = Ls[(r - kmin) * 64];                                        \
-      dn = (r == cs) ? sk : sk - udn * dn;                                         \
-      if (stiff) {                                                                 \
-        SB[o] = dn;                                                                \
-      } else {                                                                     \
-        const double bfull = (r < ks) ? dec(T[jj], RR8[jj]) + (double)D[jj] : surf; \
-        put(o, bfull + (double)dn, srkey(cellkey0 + (uint32_t)r, hv));             \
-      }                                                                            \
-      udn = U[jj];                                                                 \
-    }                                                                              \
-  }
-    EMSTR_FETCH(cs, ta, ra, da, ua)
-    for (int r0 = cs; r0 >= cb; r0 -= 16) {
-      if (r0 - 8 >= cb) {
-        EMSTR_FETCH(r0 - 8, tb, rb2, db, ub)
+    float dn[T], udn = 0.0f;
+#pragma unroll
+    for (int u = 0; u < T; u++) dn[u] = 0.0f;
+    struct Row {
+      float t[T], d[T], s[T];
+      uint32_t w;
+      float ud;
+    };
+    auto fetch = [&](int r, Row &x) {
+      const int rc = max(r, cb);
+      blt<T>(x.t, RT, c4, (uint32_t)rc * L4);
+      x.w = blr<T>(RR, cr, (uint32_t)rc * LR);
+#pragma unroll
+      for (int u = 0; u < T; u++) {
+        x.d[u] = OB[(size_t)rc * P2 * T + u];
+        x.s[u] = SB[(size_t)rc * P2 * T + u];
       }
-      EMSTR_SOLVE(r0, ta, ra, da, ua)
-      if (r0 - 8 < cb) break;
-      if (r0 - 16 >= cb) {
-        EMSTR_FETCH(r0 - 16, ta, ra, da, ua)
+      x.ud = CU[(size_t)rc * P2 * 12];
+    };
+    auto solve_row = [&](int r, const Row &x) {
+      if (r < cb) return;
+#pragma unroll
+      for (int u = 0; u < T; u++) dn[u] = (r == cs) ? x.s[u] : x.s[u] - udn * dn[u];
+      if (stiff) {
+        stt<T>(SB + (size_t)r * P2 * T, dn);
+      } else {
+        double v[T];
+#pragma unroll
+        for (int u = 0; u < T; u++)
+          v[u] = ((r < ks) ? dec(x.t[u], byteof(x.w, u)) + (double)x.d[u] : surf[u]) +
+                 (double)dn[u];
+        put(r, v);
       }
-      EMSTR_SOLVE(r0 - 8, tb, rb2, db, ub)
+      udn = x.ud;
+    };
+    Row ra0, ra1, rb0, rb1;
+    fetch(cs, ra0);
+    fetch(cs - 1, ra1);
+    for (int r0 = cs; r0 >= cb; r0 -= 4) {
+      if (r0 - 2 >= cb) {
+        fetch(r0 - 2, rb0);
+        fetch(r0 - 3, rb1);
+      }
+      solve_row(r0, ra0);
+      solve_row(r0 - 1, ra1);
+      if (r0 - 2 < cb) break;
+      if (r0 - 4 >= cb) {
+        fetch(r0 - 4, ra0);
+        fetch(r0 - 5, ra1);
+      }
+      solve_row(r0 - 2, rb0);
+      solve_row(r0 - 3, rb1);
     }
-#undef EMSTR_FETCH
-#undef EMSTR_SOLVE
   }
   if (!stiff) return;
   /* one FP64 residual pass of iterative refinement, the matrix and the
    * right-hand side in FP64 from Kz and dz (tracers.c:3841-3889); the
    * correction is solved with the FP32 factors; dlt is in the output
-   * column, d in the scratch column, the correction in the LDS slice */
+   * column, d in the scratch column, the correction in the second
+   * scratch column */
   {
     const ll P = a.P;
-    const ll fc = (ll)(j) * a.nx + i;
+    const ll fc = (ll)j * a.nx + i;
     const double dt = a.dt;
-    auto bval = [=](int kk) {
-      return (kk < ks) ? dec(bl1(RT, (uint32_t)kk * L4 + c4),
-                             (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(
-                                 RR, (uint32_t)kk * L4 + cr, 0, 0)) +
-                             (double)OB[(uint32_t)kk * P2 + col]
-                       : surf;
+    auto bval = [&](int kk, double (&v)[T]) {
+      if (kk < ks) {
+        told(kk, v);
+#pragma unroll
+        for (int u = 0; u < T; u++) v[u] += (double)OB[(size_t)kk * P2 * T + u];
+      } else {
+#pragma unroll
+        for (int u = 0; u < T; u++) v[u] = surf[u];
+      }
     };
-    float e = 0.0f;
+    float e[T];
+#pragma unroll
+    for (int u = 0; u < T; u++) e[u] = 0.0f;
     for (int kk = cb; kk <= cs; kk++) {
       const ll ix = (ll)kk * P + fc;
-      const uint32_t o = (uint32_t)kk * P2 + col;
+      const size_t o = (size_t)kk * P2 * T;
       const double dzc = dzf[ix], dzdt = dzc / dt;
       double cm1 = 0.0, cp1 = 0.0;
       if (kk > cb) cm1 = -Kz[ix] / (0.5 * (dzf[ix - P] + dzc));
       if (kk < cs) cp1 = -Kz[ix + P] / (0.5 * (dzc + dzf[ix + P]));
-      const double bv = bval(kk);
-      const double dk = (double)SB[o];
-      const double bvm = (kk > cb) ? bval(kk - 1) : bv;
-      const double bvp = (kk < cs) ? bval(kk + 1) : bv;
-      const double dkm = (kk > cb) ? (double)SB[o - P2] : 0.0;
-      const double dkp = (kk < cs) ? (double)SB[o + P2] : 0.0;
-      const double rhs = -(cm1 * (bvm - bv) + cp1 * (bvp - bv));
-      const double Ad = cm1 * dkm + (dzdt - cm1 - cp1) * dk + cp1 * dkp;
-      const float rho = (float)((rhs - Ad) / dzdt);
-      const float4 M = CC[(ll)kk * P2 * 3];
+      double bv[T], bvm[T], bvp[T];
+      bval(kk, bv);
+      if (kk > cb) bval(kk - 1, bvm);
+      if (kk < cs) bval(kk + 1, bvp);
+      const float4 M = CC[(size_t)kk * P2 * 3];
       const float gm = (kk > cb) ? M.x : 0.0f;
-      e = (rho - gm * e) * M.z;
-      Ls[(kk - kmin) * 64] = e;
+#pragma unroll
+      for (int u = 0; u < T; u++) {
+        const double bm = (kk > cb) ? bvm[u] : bv[u], bp = (kk < cs) ? bvp[u] : bv[u];
+        const double dk = (double)SB[o + u];
+        const double dkm = (kk > cb) ? (double)SB[o - (size_t)P2 * T + u] : 0.0;
+        const double dkp = (kk < cs) ? (double)SB[o + (size_t)P2 * T + u] : 0.0;
+        const double rhs = -(cm1 * (bm - bv[u]) + cp1 * (bp - bv[u]));
+        const double Ad = cm1 * dkm + (dzdt - cm1 - cp1) * dk + cp1 * dkp;
+        const float rho = (float)((rhs - Ad) / dzdt);
+        e[u] = (rho - gm * e[u]) * M.z;
+        SE[o + u] = e[u];
+      }
     }
-    float ec = 0.0f, udn = 0.0f;
+    float ec[T], udn = 0.0f;
+#pragma unroll
+    for (int u = 0; u < T; u++) ec[u] = 0.0f;
     for (int kk = cs; kk >= cb; kk--) {
-      const float ek = Ls[(kk - kmin) * 64];
-      ec = (kk == cs) ? ek : ek - udn * ec;
-      udn = CC[(ll)kk * P2 * 3].w;
-      Ls[(kk - kmin) * 64] = ec;
+      const size_t o = (size_t)kk * P2 * T;
+#pragma unroll
+      for (int u = 0; u < T; u++) {
+        const float ek = SE[o + u];
+        ec[u] = (kk == cs) ? ek : ek - udn * ec[u];
+        SE[o + u] = ec[u];
+      }
+      udn = CC[(size_t)kk * P2 * 3].w;
     }
     /* every residual is formed before any value is replaced */
     for (int kk = cb; kk <= cs; kk++) {
-      const uint32_t o = (uint32_t)kk * P2 + col;
-      const double bv = bval(kk);
-      put(o, bv + (double)SB[o] + (double)Ls[(kk - kmin) * 64],
-          srkey(cellkey0 + (uint32_t)kk, hv));
+      const size_t o = (size_t)kk * P2 * T;
+      double v[T];
+      bval(kk, v);
+#pragma unroll
+      for (int u = 0; u < T; u++) v[u] += (double)SB[o + u] + (double)SE[o + u];
+      put(kk, v);
     }
   }
 }
-
-/* advance the stochastic-rounding step counter (runs after the step) */
-__global__ void k_tick2(int *ctr) { *ctr += 1; }
 
 /*-------------------------------------------------------------------*/
 /* layout conversion and mass                                        */
@@ -891,9 +1028,10 @@ __device__ __forceinline__ ll bidx(const LArgs &a, ll idx) {
   return (ll)k * a.P2 + jj * a.nxp + ii;
 }
 
-/* byte offset of the tail of tracer n at cell index c of its volume */
+/* element offset of tracer n at cell index c of its volume, heads (in
+ * floats) and tails (in bytes) alike: [n / TPL][vol][TPL] */
 __device__ __forceinline__ ll toff(const LArgs &a, int n, ll c) {
-  return (((ll)(n >> 2) * a.vol + c) << 2) + (n & 3);
+  return ((ll)(n / TPL) * a.vol + c) * TPL + (n % TPL);
 }
 
 /* FP64 -> head and tail, rounded to nearest at 32 bits, into both buffers */
@@ -903,10 +1041,10 @@ __global__ void k_put2(LArgs a, const int *__restrict__ wsp, const ll *__restric
                        uint8_t *__restrict__ r1) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.nwet) return;
-  const ll c = bidx(a, wdn[q]), e = (ll)n * a.vol + c, o = toff(a, n, c);
+  const ll c = bidx(a, wdn[q]), e = toff(a, n, c), o = e;
   float h;
   uint32_t r;
-  enc(sp[wsp[q]], 0x80000000u, h, r);
+  enc(sp[wsp[q]], h, r);
   t0[e] = h;
   t1[e] = h;
   r0[o] = (uint8_t)r;
@@ -919,7 +1057,7 @@ __global__ void k_get2(LArgs a, const int *__restrict__ wsp, const ll *__restric
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.nwet) return;
   const ll c = bidx(a, wdn[q]);
-  sp[wsp[q]] = dec(t[(ll)n * a.vol + c], rm[toff(a, n, c)]);
+  sp[wsp[q]] = dec(t[toff(a, n, c)], rm[toff(a, n, c)]);
 }
 
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
@@ -941,9 +1079,9 @@ __global__ void k_mass2(emstr_dev d, LArgs la, int nz, const float *__restrict__
     const double ar = d.cellarea[col];
     const ll jj = col / d.nx, ii = col - jj * d.nx;
     const ll c0 = jj * la.nxp + ii;
-    const float *t = tr + (ll)n * la.vol + c0;
     for (int k = ksc; k >= kbc; k--) {
-      const double v = dec(t[(ll)k * la.P2], rm[toff(la, n, (ll)k * la.P2 + c0)]);
+      const ll o = toff(la, n, (ll)k * la.P2 + c0);
+      const double v = dec(tr[o], rm[o]);
       const double m = v * ar * d.dz[(ll)k * d.plane + col];
       double s2, e2;
       two_sum(s, m, s2, e2);
@@ -1115,7 +1253,8 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   b->nyp = (d->ny + 31) / 32 * 32; /* tiles up to 4 x 8 rows */
   b->P2 = (ll)b->nxp * b->nyp;
   b->vol = (ll)d->nz * b->P2;
-  if (b->vol * 48 > 0x7fffffffLL || b->vol * 4 * ((d->ntr + 3) / 4 * 4) > 0xffffffffLL) {
+  const ll ntp = (ll)(d->ntr + TPL - 1) / TPL * TPL; /* tracers with the group padding */
+  if (b->vol * 48 > 0x7fffffffLL || b->vol * 4 * ntp > 0xffffffffLL) {
     /* buffer offsets of the coefficients and tracers are 32-bit */
     emstr_set_error(EMSTR_EUNSUPPORTED, -1,
                     "batched form: at most %lld cells per window and 4 GiB of tracers",
@@ -1126,21 +1265,21 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   b->tx = 16;
   b->strip = 64;
   b->nbins = 1;
-  const size_t nt = (size_t)b->vol * d->ntr;
+  const size_t nt = (size_t)b->vol * ntp;
   bool ok = true;
-  const size_t nrm = (size_t)b->vol * 4 * ((d->ntr + 3) / 4);
+  const size_t nrm = (size_t)b->vol * ntp;
   /* slack for the neighbour reads of the last layer past the end: two
    * rows of tracers, one row of coefficient words */
-  const size_t trb = nt * sizeof(float) + (size_t)b->nxp * 8 + 256;
+  const size_t trb = nt * sizeof(float) + (size_t)b->nxp * 8 * TPL + 256;
   const size_t cob = (size_t)b->vol * 3 * sizeof(float4) + (size_t)b->nxp * 48 + 256;
   for (int q = 0; q < 2; q++) ok &= hipMalloc(&b->tr[q], trb) == hipSuccess;
   for (int q = 0; q < 2; q++) ok &= hipMalloc(&b->rm[q], nrm) == hipSuccess;
   ok &= hipMalloc(&b->scr, nt * sizeof(float)) == hipSuccess;
+  ok &= hipMalloc(&b->scr2, nt * sizeof(float)) == hipSuccess;
   ok &= hipMalloc(&b->co, cob) == hipSuccess;
   ok &= hipMalloc(&b->ci, (size_t)b->P2 * sizeof(Col2)) == hipSuccess;
   ok &= hipMalloc(&b->mintr, (d->ntr + 1) * sizeof(float)) == hipSuccess;
   ok &= hipMalloc(&b->maxtr, (d->ntr + 1) * sizeof(float)) == hipSuccess;
-  ok &= hipMalloc(&b->stepctr, sizeof(int)) == hipSuccess;
   if (!ok) {
     emstr_set_error(EMSTR_EDEVICE, -1, "batched form: device allocation failed");
     emstr_batch_destroy(b);
@@ -1151,7 +1290,6 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   hipMemset(b->co, 0, cob);
   for (int q = 0; q < 2; q++) hipMemset(b->tr[q], 0, trb);
   for (int q = 0; q < 2; q++) hipMemset(b->rm[q], 0, nrm);
-  hipMemset(b->stepctr, 0, sizeof(int));
   if (build_blocks(b, h_kb)) {
     emstr_batch_destroy(b);
     return NULL;
@@ -1162,7 +1300,7 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
 extern "C" void emstr_batch_destroy(struct emstr_batch *b) {
   if (!b) return;
   drop_graph(b);
-  void *v[] = {b->tr[0], b->tr[1], b->rm[0], b->rm[1], b->scr, b->co, b->ci, b->mintr, b->maxtr, b->stepctr, b->blk};
+  void *v[] = {b->tr[0], b->tr[1], b->rm[0], b->rm[1], b->scr, b->scr2, b->co, b->ci, b->mintr, b->maxtr, b->blk};
   for (unsigned q = 0; q < sizeof(v) / sizeof(v[0]); q++)
     if (v[q]) hipFree(v[q]);
   free(b);
@@ -1222,11 +1360,10 @@ extern "C" int emstr_batch_fields(struct emstr_batch *b, const emstr_dev *d, voi
 static void launch_step(emstr_batch *b, const SArgs &sa, const float *in, float *out,
                         const uint8_t *rin, uint8_t *rout, const emstr_dev *d,
                         hipStream_t st) {
-  const size_t lds = (size_t)SWAVES * (b->nz - b->kmin) * 64 * sizeof(float);
-  const int ng = (b->ntr + 3) / 4;
-  k_step2<<<(unsigned)(8 * b->bmax * ng), 64 * SWAVES, lds, st>>>(
-      sa, in, out, rin, rout, b->co, b->ci, b->blk, b->mintr, b->maxtr, d->layers, d->Kz, d->dz, b->scr,
-      b->stepctr);
+  const int ng = (b->ntr + TPL - 1) / TPL;
+  k_step4<TPL><<<(unsigned)(8 * b->bmax * ng), 64, 0, st>>>(
+      sa, in, out, rin, rout, b->co, b->ci, b->blk, b->mintr, b->maxtr, d->layers, d->Kz, d->dz,
+      b->scr, b->scr2);
 }
 
 /* the per-step work: prep + fused step (captured in a graph) */
@@ -1253,13 +1390,11 @@ static int batch_launch(emstr_batch *b, const emstr_dev *d, hipStream_t st) {
   sa.hmin = d->hmin;
   sa.dt = d->dt;
   sa.P = d->plane;
-  sa.nzl = b->nz - b->kmin;
   if (b->bmax > 0) {
     pi = prof_begin(b->prof, KC_FUSED, st);
     launch_step(b, sa, in, out, b->rm[b->cur], b->rm[1 - b->cur], d, st);
     prof_end(b->prof, pi, st);
   }
-  k_tick2<<<1, 1, 0, st>>>(b->stepctr);
   b->cur = 1 - b->cur;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
