@@ -399,7 +399,13 @@ struct SArgs {
   double hmin, dt;
   ll P;                 /* unpadded plane of the FP64 fields            */
   int nx;               /* unpadded row length of the FP64 fields       */
+  int kl;               /* the lowest layer whose forward-sweep values
+                           the wave keeps in LDS                        */
 };
+
+/* LDS per wave for the forward-sweep values of the top layers: 20 KiB, so
+ * that eight waves fit a compute unit's 160 KiB */
+constexpr int LDS_WAVE = 20480;
 
 /* raw buffer access with 32-bit byte offsets from a wave-uniform base
  * (resource word 3 of the gfx9 family); a read past the end returns 0 */
@@ -550,6 +556,29 @@ k_step4(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
   uint8_t *OR = rout + (size_t)ce * T;
   float *SB = scr + (size_t)ce * T;
   float *SE = scr2 + (size_t)ce * T;
+  /* the forward-sweep values s_r of the lane's column: in the wave's LDS
+   * for r >= kl (one row of 64 lanes x T per layer), in the scratch
+   * column below */
+  extern __shared__ float lsh[];
+  auto sput = [&](int r, const float (&v)[T]) {
+    if (r >= a.kl) {
+      float *q = lsh + ((r - a.kl) * 64 + (threadIdx.x & 63)) * T;
+#pragma unroll
+      for (int u = 0; u < T; u++) q[u] = v[u];
+    } else {
+      stt<T>(SB + (size_t)r * P2 * T, v);
+    }
+  };
+  auto sget = [&](int r, float (&v)[T]) {
+    if (r >= a.kl) {
+      const float *q = lsh + ((r - a.kl) * 64 + (threadIdx.x & 63)) * T;
+#pragma unroll
+      for (int u = 0; u < T; u++) v[u] = q[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < T; u++) v[u] = SB[(size_t)r * P2 * T + u];
+    }
+  };
   /* byte offsets: of the lane's column (heads, tails), of a layer */
   const uint32_t c4 = ce * VB, L4 = P2 * VB;
   const uint32_t cr = ce * (uint32_t)T, LR = P2 * (uint32_t)T;
@@ -747,7 +776,7 @@ k_step4(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
         if (fwd) {
 #pragma unroll
           for (int u = 0; u < T; u++) sp[u] = sv[u];
-          stt<T>(SB + (size_t)r * P2 * T, sv);
+          sput(r, sv);
         }
       } else {
 #pragma unroll
@@ -876,7 +905,7 @@ k_step4(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
         sp[u] = (rr - gm * sp[u]) * M.z;
         sv[u] = sp[u];
       }
-      stt<T>(SB + (size_t)r * P2 * T, sv);
+      sput(r, sv);
     }
   }
   /* Back substitution d_k = s_k - ud_{k+1} d_{k+1}, x = t0 + dlt + d (or
@@ -898,10 +927,8 @@ k_step4(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
       blt<T>(x.t, RT, c4, (uint32_t)rc * L4);
       x.w = blr<T>(RR, cr, (uint32_t)rc * LR);
 #pragma unroll
-      for (int u = 0; u < T; u++) {
-        x.d[u] = OB[(size_t)rc * P2 * T + u];
-        x.s[u] = SB[(size_t)rc * P2 * T + u];
-      }
+      for (int u = 0; u < T; u++) x.d[u] = OB[(size_t)rc * P2 * T + u];
+      sget(rc, x.s);
       x.ud = CU[(size_t)rc * P2 * 12];
     };
     auto solve_row = [&](int r, const Row &x) {
@@ -1361,7 +1388,8 @@ static void launch_step(emstr_batch *b, const SArgs &sa, const float *in, float 
                         const uint8_t *rin, uint8_t *rout, const emstr_dev *d,
                         hipStream_t st) {
   const int ng = (b->ntr + TPL - 1) / TPL;
-  k_step4<TPL><<<(unsigned)(8 * b->bmax * ng), 64, 0, st>>>(
+  const size_t lds = (size_t)(b->nz - sa.kl) * 64 * TPL * sizeof(float);
+  k_step4<TPL><<<(unsigned)(8 * b->bmax * ng), 64, lds, st>>>(
       sa, in, out, rin, rout, b->co, b->ci, b->blk, b->mintr, b->maxtr, d->layers, d->Kz, d->dz,
       b->scr, b->scr2);
 }
@@ -1390,6 +1418,7 @@ static int batch_launch(emstr_batch *b, const emstr_dev *d, hipStream_t st) {
   sa.hmin = d->hmin;
   sa.dt = d->dt;
   sa.P = d->plane;
+  sa.kl = max(0, b->nz - LDS_WAVE / (64 * TPL * (int)sizeof(float)));
   if (b->bmax > 0) {
     pi = prof_begin(b->prof, KC_FUSED, st);
     launch_step(b, sa, in, out, b->rm[b->cur], b->rm[1 - b->cur], d, st);
