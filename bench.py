@@ -249,7 +249,7 @@ def main():
                2: nwet * (7 * 8 + 12 * 4)}.get(dom, nwet * ntr * 10)
         avg_ms = ktB[dom] / max(kcB[dom], 1)
         achieved = alg / (avg_ms * 1e-3) / 1e9
-        kname = {4: "k_step2"}.get(dom)
+        kname = {4: "k_step4"}.get(dom)
         out["roofline"] = {"bound": "hbm", "kernel": "batched " + names[dom],
                            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS,
