@@ -27,6 +27,7 @@ PROTOS = [
     ("advect_diffuse", C.c_int, [_G, _W, _P]),
     ("vert_diffuse_3d", None, [_G, _W, _P]),
     ("set_dz", None, [_G, _W, _P]),
+    ("auxiliary_routines", None, [_G, _W, _P]),
     # device-resident interface
     ("emstr_create", _CTX, [_G, _W, _P, C.c_int]),
     ("emstr_destroy", None, [_CTX]),
@@ -41,6 +42,7 @@ PROTOS = [
     ("emstr_vert_diffuse", C.c_int, [_CTX]),
     ("emstr_tracer_mass", C.c_int, [_CTX, C.POINTER(C.c_double)]),
     ("emstr_tracer_mass_noalloc", C.c_int, [_CTX, C.POINTER(C.c_double)]),
+    ("emstr_auxiliary", C.c_int, [_CTX]),
     ("emstr_device_set_dz", C.c_int, [_CTX]),
     ("emstr_device_vel_w", C.c_int, [_CTX]),
     ("emstr_device_set_flux_3d", C.c_int, [_CTX]),

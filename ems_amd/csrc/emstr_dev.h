@@ -39,6 +39,8 @@ typedef struct emstr_dev {
 
   /* 3D fields [nz][ny][nx] */
   double *u1, *u2, *w, *u1flux3d, *u2flux3d, *Kz, *dz;
+  double *dzg; /* [ny][nx] dz of the column's sediment ghost (layer kb-1,
+                  written by reset_dz, copied up by set_dz); NULL: 0      */
   double *dzu1, *dzu2, *u1kh, *u2kh; /* horizontal diffusion        */
 
   /* per tracer [ntr] */

@@ -471,6 +471,15 @@ emstr_grid *emstr_grid_build(int nx, int ny, int nz, const double *layers,
   wd->wtop = (double *)zalloc(S, sizeof(double));
   wd->detadt = (double *)zalloc(S, sizeof(double));
   wc->nonlinear = 0;
+  /* auxiliary_routines: no temperature / salinity / age tracer, nothing
+   * to decay, no relaxation (the caller sets them); reset_dz's lists */
+  wd->tno = wd->sno = -1;
+  wc->ntdec = 0;
+  wc->tdec = (int *)zalloc(nt1, sizeof(int));
+  wc->dectr = (double *)zalloc(nt1, sizeof(double));
+  wc->trinfo_3d = (tracer_info_t *)zalloc(nt1, sizeof(tracer_info_t));
+  wc->s2 = (int *)zalloc(sgsiz, sizeof(int));
+  wc->c1 = (char *)zalloc(sgsiz, sizeof(char));
   free(gdir);
   free(idx3);
   return G;
@@ -510,7 +519,8 @@ void emstr_grid_free(emstr_grid *G) {
                  p->u1kh, p->u2kh, p->s1, p->i1, p->i2, p->i3, p->i4, p->i5,
                  p->w1, p->w2, p->w3, p->w4, p->w5, p->w6, p->w7, p->w8, p->w9,
                  p->w10, p->d1, p->d2, p->d3, p->d4, p->v1, p->v2, p->v3, p->v4,
-                 p->v5, p->v6, p->v7, p->cbot_e1, p->cbot_e2};
+                 p->v5, p->v6, p->v7, p->cbot_e1, p->cbot_e2, p->tdec, p->dectr,
+                 p->trinfo_3d, p->s2, p->c1};
     for (n = 0; n < (int)(sizeof(a) / sizeof(a[0])); n++) free(a[n]);
     free(p);
   }

@@ -90,6 +90,16 @@ struct emstr_ctx {
   int *d_kbf1, *d_kbf2;
   double *d_bzu1, *d_bzu2, *d_dzu1, *d_dzu2;
   int knew_dev;
+  /* auxiliary_routines, made on first use: relaxation fields and time
+   * constants, the age mask (0/1), cell centres, the decay tables, the
+   * per-column flag of reset_dz's cells to process */
+  double *d_rtemp, *d_rsalt, *d_ttc, *d_stc, *d_agem, *d_cellz;
+  int *d_tdec, *d_dtr3;     /* d_dtr3, d_dsc: second halves of d_tdec, d_dec */
+  double *d_dec, *d_dsc;
+  int ndec_cap;
+  int *d_p2, *h_p2;
+  /* host staging of the sediment ghosts' dz, [plane] */
+  double *h_dzg;
 };
 
 /*-------------------------------------------------------------------*/
@@ -328,6 +338,8 @@ emstr_ctx *emstr_create(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
   for (n = 0; n < (int)(sizeof(d2) / sizeof(d2[0])); n++)
     *d2[n] = (double *)dalloc(plane * sizeof(double), &ok);
   c->d_wbot = (double *)dalloc(plane * sizeof(double), &ok);
+  d->dzg = (double *)dalloc(plane * sizeof(double), &ok);
+  c->h_dzg = (double *)calloc(plane, sizeof(double));
   d->layers = (double *)dalloc((d->nz + 1) * sizeof(double), &ok);
   double **d3[] = {&d->u1, &d->u2, &d->w, &d->u1flux3d, &d->u2flux3d, &d->Kz, &d->dz};
   for (n = 0; n < (int)(sizeof(d3) / sizeof(d3[0])); n++)
@@ -467,12 +479,14 @@ void emstr_destroy(emstr_ctx *c) {
                 d->fnw, d->fw9, d->fcl, d->fcrf, d->ftmx, d->ftmy, d->ftmz, d->ftm,
                 d->ffx, d->ffy, d->ffz, c->d_face, c->d_fw1, c->d_fw2, c->d_ksur,
                 c->d_cb, c->d_fa, c->d_h, c->d_m, c->d_uf, c->d_ksur2, c->d_kbf1,
-                c->d_kbf2, c->d_bzu1, c->d_bzu2, c->d_dzu1, c->d_dzu2};
+                c->d_kbf2, c->d_bzu1, c->d_bzu2, c->d_dzu1, c->d_dzu2, c->d_rtemp,
+                c->d_rsalt, c->d_ttc, c->d_stc, c->d_agem, c->d_cellz, c->d_tdec,
+                c->d_dec, c->d_p2, d->dzg};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->h_kb, c->cc2col, c->col2cc, c->sedg, c->topc, c->h_ks,
                 c->h_kold, c->h_knew, c->h_proc, c->h_cols, c->sp_of, c->h_part,
-                c->h_dn, c->h_ksur, c->h_cb};
+                c->h_dn, c->h_ksur, c->h_cb, c->h_p2, c->h_dzg};
   for (n = 0; n < (int)(sizeof(hv) / sizeof(hv[0])); n++) free(hv[n]);
   if (c->st) hipStreamDestroy(c->st);
   pthread_mutex_lock(&reg_lock);
@@ -526,6 +540,14 @@ static int columns_from_host(emstr_ctx *c) {
   const ll plane = d->plane;
   int cc;
   for (ll q = 0; q < plane; q++) c->h_ks[q] = -1;
+  /* the old and new surfaces of every wet column (reset_dz reads the
+   * columns that are not processed too) */
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    const ll col = c->cc2col[cc];
+    const int ns = g->nsur_t[cc], os = g->sur_t[cc];
+    c->h_knew[col] = (ns == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[ns];
+    c->h_kold[col] = (os == c->sedg[col]) ? c->h_kb[col] - 1 : g->s2k[os];
+  }
   for (cc = 1; cc <= wc->vcs; cc++) {
     int s = wc->s1[cc];
     ll col = (ll)g->s2j[s] * d->nx + g->s2i[s];
@@ -583,6 +605,14 @@ int emstr_upload_fields(emstr_ctx *c) {
       upload3d(c, d->u2flux3d, wd->u2flux3d) || upload3d(c, d->Kz, wd->Kz) ||
       upload3d(c, d->dz, wc->dz))
     return EMSTR_EDEVICE;
+  /* the sediment ghosts' dz (reset_dz writes them, set_dz copies them up
+   * a column whose old surface is the ghost) */
+  for (int cc = 1; cc <= c->g->a2_t; cc++) {
+    const ll col = c->cc2col[cc];
+    c->h_dzg[col] = wc->dz[c->sedg[col]];
+  }
+  HCHK(hipMemcpyAsync(d->dzg, c->h_dzg, (size_t)d->plane * sizeof(double),
+                      hipMemcpyHostToDevice, c->st));
   d->nstep0 = (wd->nstep == 0);
   if (d->trasc == FFSL && !c->have_kh)
     return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
@@ -1376,8 +1406,9 @@ int emstr_transport_step(emstr_ctx *c) {
   c->knew_dev = 1;
   if ((r = velw_run(c, 0))) return r;
   if ((r = form_fields(c))) return r;
-  /* tracer_step */
+  /* tracer_step (tracer_step_3d: the transport, then auxiliary_routines) */
   if ((r = emstr_step(c))) return r;
+  if ((r = emstr_auxiliary(c))) return r;
   /* transport_post: the new oldeta (transport.c:543) */
   memcpy(wc->oldeta, wd->eta, (size_t)g->sgsizS * sizeof(double));
   HCHK(hipMemcpyAsync(d->oldeta, d->eta, (size_t)d->plane * sizeof(double),
@@ -1404,6 +1435,204 @@ int emstr_transport_download(emstr_ctx *c) {
                         hipMemcpyDeviceToHost, c->st));
   }
   HCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* auxiliary_routines (tracers.c:4296-4490): reset_dz, calc_age,      */
+/* tracer_decay, do_ts_relax                                          */
+/*-------------------------------------------------------------------*/
+/* tracer_decay.c:45-52: seconds per unit of a decay timescale */
+static double units_scale(const char *u) {
+  if (strcmp(u, "days") == 0 || strcmp(u, "day") == 0) return 86400.0;
+  if (strcmp(u, "hours") == 0 || strcmp(u, "hour") == 0) return 3600.0;
+  if (strcmp(u, "minutes") == 0 || strcmp(u, "minute") == 0) return 60.0;
+  return 1.0;
+}
+
+/* a 3D field into a device buffer made on first use */
+static int aux_field(emstr_ctx *c, double **dst, const double *sp) {
+  int ok = 1;
+  if (!*dst) *dst = (double *)dalloc((size_t)c->d.n3 * sizeof(double), &ok);
+  if (!ok) return emstr_set_error(EMSTR_EDEVICE, -1, "auxiliary_routines: device allocation");
+  return upload3d(c, *dst, sp);
+}
+
+int emstr_auxiliary(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->d;
+  int n, nn, ok = 1, r;
+  AuxArgs a;
+  memset(&a, 0, sizeof(a));
+  a.age = a.tno = a.sno = -1;
+  a.dttr = wd->dttr;
+  /* the age tracer: agetr points at one of the 3D tracers */
+  if (wd->agetr && wc->agemsk) {
+    for (n = 0; n < wd->ntr; n++)
+      if (wd->tr_wc[n] == wd->agetr) a.age = n;
+    if (a.age < 0)
+      return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                             "auxiliary_routines: the age tracer is not a 3D tracer of the window");
+    double *m = (double *)malloc((size_t)g->sgsiz * sizeof(double));
+    for (n = 0; n < g->sgsiz; n++) m[n] = wc->agemsk[n] ? 1.0 : 0.0;
+    r = aux_field(c, &c->d_agem, m);
+    HCHK(hipStreamSynchronize(c->st));
+    free(m);
+    if (r) return r;
+    a.agemsk = c->d_agem;
+  }
+  /* the decay tables (tracer_decay.c:40-52) */
+  if (wc->ntdec > 0) {
+    if (!wc->tdec || !wc->dectr)
+      return emstr_set_error(EMSTR_EARG, -1, "auxiliary_routines: ntdec without tdec/dectr");
+    int *td = (int *)malloc(sizeof(int) * 2 * (size_t)wc->ntdec);
+    double *dv = (double *)malloc(sizeof(double) * 2 * (size_t)wc->ntdec);
+    int *t3 = td + wc->ntdec;
+    double *sc = dv + wc->ntdec;
+    for (nn = 0; nn < wc->ntdec; nn++) {
+      n = wc->tdec[nn];
+      const int flag = wc->trinfo_3d ? wc->trinfo_3d[n].flag : 0;
+      if (n < 0 || n >= wd->ntr || ((flag & DE_TR3) && ((int)wc->dectr[n] < 0 ||
+                                                        (int)wc->dectr[n] >= wd->ntr))) {
+        free(td), free(dv);
+        return emstr_set_error(EMSTR_EARG, -1, "auxiliary_routines: decay tracer out of range");
+      }
+      if (flag & DE_TR2) {
+        free(td), free(dv);
+        return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                               "auxiliary_routines: decay timescale from a 2D tracer (DE_TR2)");
+      }
+      td[nn] = n;
+      dv[nn] = wc->dectr[n];
+      t3[nn] = (flag & DE_TR3) ? (int)wc->dectr[n] : -1;
+      sc[nn] = (flag & DE_TR3) ? units_scale(wc->trinfo_3d[t3[nn]].units) : 1.0;
+    }
+    if (wc->ntdec > c->ndec_cap) {
+      if (c->d_tdec) hipFree(c->d_tdec), hipFree(c->d_dec), c->d_tdec = NULL;
+      c->d_tdec = (int *)dalloc(sizeof(int) * 2 * (size_t)wc->ntdec, &ok);
+      c->d_dec = (double *)dalloc(sizeof(double) * 2 * (size_t)wc->ntdec, &ok);
+      if (!ok) {
+        free(td), free(dv);
+        return emstr_set_error(EMSTR_EDEVICE, -1, "auxiliary_routines: device allocation");
+      }
+      c->ndec_cap = wc->ntdec;
+    }
+    c->d_dtr3 = c->d_tdec + wc->ntdec;
+    c->d_dsc = c->d_dec + wc->ntdec;
+    HCHK(hipMemcpyAsync(c->d_tdec, td, sizeof(int) * 2 * (size_t)wc->ntdec,
+                        hipMemcpyHostToDevice, c->st));
+    HCHK(hipMemcpyAsync(c->d_dec, dv, sizeof(double) * 2 * (size_t)wc->ntdec,
+                        hipMemcpyHostToDevice, c->st));
+    HCHK(hipStreamSynchronize(c->st));
+    free(td);
+    free(dv);
+    a.ntdec = wc->ntdec;
+    a.tdec = c->d_tdec;
+    a.dtr3 = c->d_tdec + wc->ntdec;
+    a.dec = c->d_dec;
+    a.dsc = c->d_dec + wc->ntdec;
+  }
+  /* temperature and salinity relaxation (tracers.c:6129-6190) */
+  for (int q = 0; q < 2; q++) {
+    const int tn = q ? wd->sno : wd->tno;
+    if (tn < 0 || tn >= wd->ntr) continue;
+    if (q) a.sno = tn; else a.tno = tn;
+    double *rf = q ? wd->rsalt : wd->rtemp, *tc = q ? wd->salt_tc : wd->temp_tc;
+    if (!rf || !wc->trinfo_3d) continue;
+    const tracer_info_t *ti = &wc->trinfo_3d[tn];
+    if ((r = aux_field(c, q ? &c->d_rsalt : &c->d_rtemp, rf))) return r;
+    const int tcf = (tc && (ti->tctype & (RLX_ADPT | RLX_REG | RLX_OBC))) ? 1 : 0;
+    if (tcf && (r = aux_field(c, q ? &c->d_stc : &c->d_ttc, tc))) return r;
+    if (q) {
+      a.rsalt = c->d_rsalt, a.salt_tc = tcf ? c->d_stc : NULL, a.rrS = ti->relax_rate;
+      a.tcS = tcf, a.grdS = (ti->flag & RLX_GRD) ? 1 : 0, a.depthS = ti->relax_dum;
+    } else {
+      a.rtemp = c->d_rtemp, a.temp_tc = tcf ? c->d_ttc : NULL, a.rrT = ti->relax_rate;
+      a.tcT = tcf, a.grdT = (ti->flag & RLX_GRD) ? 1 : 0, a.depthT = ti->relax_dum;
+    }
+  }
+  if ((a.grdT || a.grdS) && (r = aux_field(c, &c->d_cellz, g->cellz))) return r;
+  a.cellz = c->d_cellz;
+  a.tracers = a.age >= 0 || a.ntdec > 0 ||
+              ((a.tno >= 0 || a.sno >= 0) && (!wc->sigma || a.rtemp || a.rsalt));
+  if (a.tracers && c->form != EMSTR_PLAIN_F64)
+    return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                           "auxiliary_routines: age, decay and T/S relaxation in the plain form only");
+  if (!c->d_p2) {
+    c->d_p2 = (int *)dalloc((size_t)d->plane * sizeof(int), &ok);
+    c->h_p2 = (int *)calloc((size_t)d->plane, sizeof(int));
+    if (!ok || !c->h_p2) return emstr_set_error(EMSTR_EDEVICE, -1, "auxiliary_routines: allocation");
+  }
+  if (emstr_launch_aux(d, &a, a.tracers ? c->tr[c->cur] : NULL, c->d_p2, c->st))
+    return emstr_set_error(EMSTR_EDEVICE, -1, "auxiliary_routines launch");
+  /* the reference fills the relaxation fields in place */
+  const double *fsrc[2] = {a.grdT ? c->d_rtemp : NULL, a.grdS ? c->d_rsalt : NULL};
+  double *fdst[2] = {wd->rtemp, wd->rsalt};
+  for (int q = 0; q < 2; q++) {
+    if (!fsrc[q]) continue;
+    if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, fsrc[q], c->stage, c->st))
+      return emstr_set_error(EMSTR_EDEVICE, -1, "gather");
+    double *tmp = (double *)malloc((size_t)g->sgsiz * sizeof(double));
+    HCHK(hipMemcpyAsync(tmp, c->stage, (size_t)g->sgsiz * sizeof(double), hipMemcpyDeviceToHost,
+                        c->st));
+    HCHK(hipStreamSynchronize(c->st));
+    for (int q3 = 1; q3 <= g->v3_t; q3++) fdst[q][g->w3_t[q3]] = tmp[g->w3_t[q3]];
+    free(tmp);
+  }
+  return 0;
+}
+
+/* reset_dz's outputs on the host structures: the surface thicknesses
+ * (dz of every wet cell from the device), the cells to process s2 with
+ * vc2, vcs2, vca2, vci2 and i1 (same order as tracers.c:5944-5972) and
+ * the mask c1 */
+static int aux_to_host(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  win_priv_t *wc = c->wc;
+  emstr_dev *d = &c->d;
+  int cc, vc = 1;
+  HCHK(hipMemcpyAsync(c->h_p2, c->d_p2, (size_t)d->plane * sizeof(int), hipMemcpyDeviceToHost,
+                      c->st));
+  if (emstr_launch_gather(d->wsp, d->wdn, d->nwet, d->dz, c->stage, c->st))
+    return emstr_set_error(EMSTR_EDEVICE, -1, "gather");
+  double *tmp = (double *)malloc((size_t)g->sgsiz * sizeof(double));
+  HCHK(hipMemcpyAsync(tmp, c->stage, (size_t)g->sgsiz * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  for (int q = 1; q <= g->v3_t; q++) wc->dz[g->w3_t[q]] = tmp[g->w3_t[q]];
+  free(tmp);
+  HCHK(hipMemcpyAsync(c->h_dzg, d->dzg, (size_t)d->plane * sizeof(double),
+                      hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  for (cc = 1; cc <= g->a2_t; cc++) {
+    const ll col = c->cc2col[cc];
+    wc->dz[c->sedg[col]] = c->h_dzg[col];
+  }
+  if (wc->sigma || !wc->s2) return 0;
+  ll *cols = (ll *)malloc(sizeof(ll) * ((size_t)g->b2_t + 1));
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    const ll col = c->cc2col[cc];
+    if (!c->h_p2[col]) continue;
+    wc->s2[vc] = sp_at(c, col, c->h_knew[col]);
+    wc->i1[vc] = g->bot_t[cc];
+    if (cc == g->v2_t) wc->vca2 = vc;
+    cols[vc] = col;
+    vc++;
+  }
+  const int vcs = vc - 1;
+  wc->vcs2 = vcs;
+  for (cc = 1; cc <= vcs; cc++) {
+    const ll col = cols[cc];
+    for (int k = c->h_knew[col] - 1; k >= c->h_kb[col]; k--) wc->s2[vc++] = sp_at(c, col, k);
+    if (cc == wc->vca2) wc->vci2 = vc - 1;
+  }
+  wc->vc2 = vc - 1;
+  free(cols);
+  if (wc->c1) {
+    memset(wc->c1, 0, (size_t)g->sgsiz);
+    for (cc = 1; cc <= wc->vc2; cc++) wc->c1[wc->s2[cc]] = 1;
+  }
   return 0;
 }
 
@@ -1441,7 +1670,20 @@ void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon) {
   if (!c) return;
   if (emstr_upload_fields(c) || emstr_upload_tracers(c)) return;
   if (emstr_step(c)) return;
+  /* tracers.c:289 */
+  if (emstr_auxiliary(c)) return;
   if (emstr_download_tracers(c)) return;
+  if (aux_to_host(c)) return;
+  emstr_sync(c);
+}
+
+void auxiliary_routines(geometry_t *window, window_t *windat, win_priv_t *wincon) {
+  emstr_ctx *c = ctx_for(window, windat, wincon);
+  if (!c) return;
+  if (emstr_upload_fields(c) || emstr_upload_tracers(c)) return;
+  if (emstr_auxiliary(c)) return;
+  if (emstr_download_tracers(c)) return;
+  if (aux_to_host(c)) return;
   emstr_sync(c);
 }
 

@@ -30,6 +30,31 @@ int emstr_launch_ffsl_fluxes(const emstr_dev *d, const double *t, const double *
 int emstr_launch_vdiff(const emstr_dev *d, double *tr, double *scratch,
                        int *err, void *stream);
 int emstr_launch_setdz(const emstr_dev *d, int *proc, void *stream);
+/* auxiliary_routines' state changes (emstr_plain.hip): reset_dz of dz
+ * (per-column flag p2: a cell to process for the updated elevation),
+ * then, if tracers: the no-gradient relaxation fields, per cell the age,
+ * decay and T/S relaxation, the T/S copy above the new surface.  The
+ * decay tables, relaxation fields and age mask are device arrays. */
+typedef struct AuxArgs {
+  int tracers;             /* any tracer-changing part                */
+  int age;                 /* age tracer number, -1: none            */
+  const double *agemsk;    /* [n3] 0/1                                */
+  double dttr;
+  int ntdec;
+  const int *tdec;         /* [ntdec] tracers to decay, in order      */
+  const double *dec;       /* [ntdec] decay constants                 */
+  const int *dtr3;         /* [ntdec] 3D tracer of the timescale, -1  */
+  const double *dsc;       /* [ntdec] seconds per unit                */
+  int tno, sno;            /* -1: absent                              */
+  double *rtemp, *rsalt;   /* [n3] relaxation fields (NULL: none)     */
+  const double *temp_tc, *salt_tc; /* [n3] time constants (NULL)     */
+  double rrT, rrS;         /* relaxation rates                        */
+  int tcT, tcS;            /* time-constant relaxation on             */
+  int grdT, grdS;          /* no-gradient relaxation field            */
+  double depthT, depthS;
+  const double *cellz;     /* [n3] cell centres (no-gradient fields)  */
+} AuxArgs;
+int emstr_launch_aux(const emstr_dev *d, const AuxArgs *a, double *tr, int *p2, void *stream);
 int emstr_launch_subeta(const emstr_dev *d, double dtu, double trem, int init,
                         void *stream);
 int emstr_launch_osub(const emstr_dev *d, void *stream);

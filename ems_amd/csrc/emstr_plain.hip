@@ -1460,8 +1460,8 @@ __global__ void k_setdz(emstr_dev d, int *__restrict__ proc) {
   }
   d.dz[(ll)kbc * P + col] = tp - d.botz[col];
   /* cells above the old surface take its thickness; a ghost surface
-   * (kold < kbc) has thickness 0 */
-  double dzs = (kold >= kbc) ? d.dz[(ll)kold * P + col] : 0.0;
+   * (kold < kbc) the ghost's */
+  double dzs = (kold >= kbc) ? d.dz[(ll)kold * P + col] : (d.dzg ? d.dzg[col] : 0.0);
   for (int kk = max(kold, kbc - 1) + 1; kk < d.nz; kk++) d.dz[(ll)kk * P + col] = dzs;
 }
 
@@ -1870,4 +1870,141 @@ extern "C" int emstr_launch_velw(const emstr_dev *d, const double *wbot,
   if (d->ncol == 0) return 0;
   k_velw<<<(d->ncol + 255) / 256, 256, 0, (hipStream_t)stream>>>(*d, wbot);
   return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+/*-------------------------------------------------------------------*/
+/* auxiliary_routines (tracers.c:4296-4490): the parts that change   */
+/* the window's state after the transport                            */
+/*-------------------------------------------------------------------*/
+/* reset_dz (tracers.c:5900-5932), per wet column: the cells from the
+ * deeper of the old and new surfaces up to the new surface take the
+ * full layer thickness from the deeper cell's lower face (or the bed),
+ * the new surface cell the height of the updated elevation above its
+ * lower face (kept if the elevation is not above it); layer kb - 1
+ * stands for the sediment ghost, whose lower face is layers[kb - 1]
+ * (its thickness goes to dzg, which set_dz copies up a column whose
+ * old surface is the ghost).  p2: the column is a cell to process
+ * for the updated elevation (tracers.c:5946-5952), its surface layer
+ * the new surface. */
+__global__ void k_resetdz(emstr_dev d, int *__restrict__ p2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (kbc >= d.nz) return;
+  const ll P = d.plane;
+  const double *lay = d.layers;
+  const int knew = d.knew[col];
+  int k = min(d.kold[col], knew);
+  double bot = fmx(lay[max(k, 0)], d.botz[col]);
+  while (k < knew) {
+    const double top = lay[k + 1];
+    if (k >= kbc) d.dz[(ll)k * P + col] = top - bot;
+    else d.dzg[col] = top - bot;
+    bot = top;
+    k++;
+  }
+  const double top = d.eta[col];
+  if (top > bot) {
+    if (k >= kbc) d.dz[(ll)k * P + col] = top - bot;
+    else d.dzg[col] = top - bot;
+  }
+  p2[col] = (d.eta[col] > 0.05 * d.hmin + d.botz[col] && knew >= kbc) ? 1 : 0;
+}
+
+/* do_ts_relax's no-gradient relaxation field (tracers.c:6137-6148), per
+ * wet column: below the first cell from the top whose centre is not
+ * above depth (or the bed cell), every cell above it takes its value */
+__global__ void k_relaxgrd(emstr_dev d, const double *__restrict__ cellz, double *__restrict__ rf,
+                           double depth) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (kbc >= d.nz) return;
+  const ll P = d.plane;
+  int k = d.nz - 1;
+  while (cellz[(ll)k * P + col] > depth && k > kbc) k--;
+  const double v = rf[(ll)k * P + col];
+  for (int kk = k + 1; kk < d.nz; kk++) rf[(ll)kk * P + col] = v;
+}
+
+/* Per wet cell, in the reference's order: calc_age (tracers.c:6966-6971),
+ * tracer_decay over the cells to process (tracer_decay.c:55-61; the
+ * tracers of tdec in order), do_ts_relax's relaxation of temperature then
+ * salinity (tracers.c:6150-6190) */
+__global__ void k_auxcell(emstr_dev d, AuxArgs a, double *__restrict__ tr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  const int kbc = d.kb[col];
+  if (k < kbc) return;
+  const ll idx = (ll)k * d.plane + col;
+  const ll N3 = d.n3;
+  if (a.age >= 0 && a.agemsk[idx] != 0.0) tr[a.age * N3 + idx] += a.dttr * (1.0 / 86400.0);
+  const int ksc = d.ks[col];
+  if (ksc >= 0 && k <= ksc) {
+    for (int nn = 0; nn < a.ntdec; nn++) {
+      const double decay = a.dec[nn];
+      if (decay == 0.0) continue;
+      double *t = tr + a.tdec[nn] * N3;
+      const double dd = (a.dtr3[nn] >= 0) ? tr[a.dtr3[nn] * N3 + idx] : decay;
+      t[idx] -= d.dt * t[idx] / (dd * a.dsc[nn]);
+    }
+  }
+  for (int q = 0; q < 2; q++) {
+    const int n = q ? a.sno : a.tno;
+    const double *rf = q ? a.rsalt : a.rtemp;
+    const double *tc = q ? a.salt_tc : a.temp_tc;
+    const double rr = q ? a.rrS : a.rrT;
+    const int tcf = q ? a.tcS : a.tcT;
+    if (n < 0 || !rf) continue;
+    double *t = tr + n * N3;
+    if (rr != 0.0) t[idx] -= d.dt * rr * (t[idx] - rf[idx]);
+    if (tc && tcf) {
+      const double r2 = (tc[idx] != 0.0) ? (1.0 / 86400.0) / tc[idx] : 0.0;
+      t[idx] -= d.dt * (t[idx] - rf[idx]) * r2;
+    }
+  }
+}
+
+/* do_ts_relax's no-gradient above the surface (tracers.c:6193-6204):
+ * the cells above the new surface of every column to process for the
+ * updated elevation take its temperature and salinity */
+__global__ void k_tsabove(emstr_dev d, const int *__restrict__ p2, double *__restrict__ tr,
+                          int tno, int sno) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= d.nx || j >= d.ny) return;
+  const ll col = (ll)j * d.nx + i;
+  if (d.kb[col] >= d.nz || !p2[col]) return;
+  const ll P = d.plane, N3 = d.n3;
+  const int ks2 = d.knew[col];
+  for (int q = 0; q < 2; q++) {
+    const int n = q ? sno : tno;
+    if (n < 0) continue;
+    double *t = tr + n * N3;
+    const double v = t[(ll)ks2 * P + col];
+    for (int kk = ks2 + 1; kk < d.nz; kk++) t[(ll)kk * P + col] = v;
+  }
+}
+
+extern "C" int emstr_launch_aux(const emstr_dev *d, const AuxArgs *a, double *tr, int *p2,
+                                void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  dim3 b(64, 4, 1), g((d->nx + 63) / 64, (d->ny + 3) / 4, 1);
+  k_resetdz<<<g, b, 0, st>>>(*d, p2);
+  if (a->tracers) {
+    if (a->tno >= 0 && a->rtemp && a->grdT)
+      k_relaxgrd<<<g, b, 0, st>>>(*d, a->cellz, a->rtemp, a->depthT);
+    if (a->sno >= 0 && a->rsalt && a->grdS)
+      k_relaxgrd<<<g, b, 0, st>>>(*d, a->cellz, a->rsalt, a->depthS);
+    k_auxcell<<<dim3(g.x, g.y, d->nz), b, 0, st>>>(*d, *a, tr);
+    if (!d->sigma) k_tsabove<<<g, b, 0, st>>>(*d, p2, tr, a->tno, a->sno);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -43,6 +43,15 @@ class Geometry(C.Structure):
     ]
 
 
+class TracerInfo(C.Structure):
+    """tracer_info_t of include/emstr.h (the fields auxiliary_routines
+    reads)"""
+    _fields_ = [
+        ("units", C.c_char * 64), ("flag", _I), ("relax_rate", _D),
+        ("relax_dum", _D), ("tctype", _I),
+    ]
+
+
 class Window(C.Structure):
     _fields_ = [
         ("ntr", _I), ("tr_wc", C.POINTER(_PD)), ("dt", _D), ("dttr", _D),
@@ -53,6 +62,8 @@ class Window(C.Structure):
         ("eta", _PD), ("topz", _PD), ("wbot", _PD),
         ("u1flux", _PD), ("u2flux", _PD), ("u1vm", _PD), ("u2vm", _PD),
         ("etab", _PD), ("wtop", _PD), ("detadt", _PD),
+        ("tno", _I), ("sno", _I), ("rtemp", _PD), ("rsalt", _PD),
+        ("temp_tc", _PD), ("salt_tc", _PD), ("agetr", _PD),
     ]
 
 
@@ -74,6 +85,10 @@ class WinPriv(C.Structure):
         ("b1", _D), ("twin", _PI),
         ("cbot_e1", _PI), ("cbot_e2", _PI), ("ncbot_e1", _I), ("ncbot_e2", _I),
         ("nonlinear", _I),
+        ("ntdec", _I), ("tdec", _PI), ("dectr", _PD),
+        ("trinfo_3d", C.POINTER(TracerInfo)), ("agemsk", C.POINTER(C.c_short)),
+        ("s2", _PI), ("vc2", _I), ("vcs2", _I), ("vca2", _I), ("vci2", _I),
+        ("c1", C.POINTER(C.c_char)),
     ]
 
 
@@ -104,6 +119,12 @@ SUB_STEP = 2
 TMASS = 16
 TRACERS = 256
 RS_RESTART = 0x0004
+DE_TR2 = 0x0400
+DE_TR3 = 0x0800
+RLX_ADPT = 0x0008
+RLX_GRD = 0x0400
+RLX_REG = 0x0800
+RLX_OBC = 0x1000
 NONE = 0x80000000 - (1 << 32)  # as a signed C int
 
 

@@ -55,6 +55,29 @@ extern "C" {
  * hd_params.h:42 */
 #define DRY_FRAC 0.05
 
+/* Tracer attribute flags read by auxiliary_routines: decay timescale
+ * from a 2D / 3D tracer (hd_params.h:674-675), relaxation time-constant
+ * types and the no-gradient relaxation field (hd_params.h:705-714) */
+#define DE_TR2 0x0400
+#define DE_TR3 0x0800
+#define RLX_ADPT 0x0008
+#define RLX_GRD 0x0400
+#define RLX_REG 0x0800
+#define RLX_OBC 0x1000
+
+/* Tracer attributes: the fields of the reference tracer_info_t
+ * (lib/include/tracer.h:50-125) that auxiliary_routines reads.  The
+ * units string is shorter than the reference's MAXSTRLEN; only "day",
+ * "days", "hour", "hours", "minute", "minutes" are told apart. */
+#define EMSTR_UNITS_LEN 64
+typedef struct tracer_info {
+  char units[EMSTR_UNITS_LEN];
+  int flag;          /* DE_TR2 / DE_TR3, RLX_GRD                         */
+  double relax_rate; /* relaxation rate (1/s)                            */
+  double relax_dum;  /* depth below which the relaxation field is kept  */
+  int tctype;        /* relaxation time-constant type (RLX_*)            */
+} tracer_info_t;
+
 /*-------------------------------------------------------------------*/
 /* Grid structure (fields of the reference geometry_t)              */
 /*-------------------------------------------------------------------*/
@@ -144,6 +167,13 @@ typedef struct window {
   double *etab;                  /* 2D elevation at the last transport
                                     step (vel_w_bounds' detadt)       */
   double *wtop, *detadt;         /* 2D surface w and d(eta)/dt        */
+  /* auxiliary_routines (tracers.c:4296-4490): temperature and salinity
+   * (tracer numbers; -1: absent), their relaxation fields and time
+   * constants (3D, NULL: none), the age tracer (NULL: none)          */
+  int tno, sno;
+  double *rtemp, *rsalt;
+  double *temp_tc, *salt_tc;
+  double *agetr;
 } window_t;
 
 /*-------------------------------------------------------------------*/
@@ -188,6 +218,20 @@ typedef struct win_priv {
   int *cbot_e1, *cbot_e2;
   int ncbot_e1, ncbot_e2;
   int nonlinear;  /* surface slope terms in vel_w_bounds (rejected)    */
+  /* auxiliary_routines: tracers to decay tdec[0..ntdec-1] with their
+   * decay constants dectr[n] (s; or the 3D tracer number holding the
+   * timescale, DE_TR3), tracer attributes trinfo_3d[ntr], the age mask
+   * (3D, NULL: none); reset_dz's cells to process for the updated
+   * elevation s2[1..vc2] (surface cells first: 1..vca2 wet, ..vcs2 OBC;
+   * then sub-surface: ..vci2 wet, ..vc2 OBC) and the wet-cell mask c1 */
+  int ntdec;
+  int *tdec;
+  double *dectr;
+  tracer_info_t *trinfo_3d;
+  short *agemsk;
+  int *s2;
+  int vc2, vcs2, vca2, vci2;
+  char *c1;
 } win_priv_t;
 
 /*-------------------------------------------------------------------*/
@@ -230,6 +274,18 @@ int advect_diffuse(geometry_t *window, window_t *windat, win_priv_t *wincon);
 void vert_diffuse_3d(geometry_t *window, window_t *windat, win_priv_t *wincon);
 /* model/hd/tracers/tracers.c:5682 (z-level branch) */
 void set_dz(geometry_t *window, window_t *windat, win_priv_t *wincon);
+/* model/hd/tracers/tracers.c:4296-4490, the parts that change the
+ * window's state after the transport: reset_dz (tracers.c:5900-5980:
+ * surface thicknesses for the updated elevation, the cells-to-process
+ * vector s2 and the mask c1), calc_age (:6960-6973), tracer_decay
+ * (tracer_decay.c:26-63) and do_ts_relax (:6122-6207), in the
+ * reference's order, on the GPU.  The diagnostics and process
+ * libraries it also calls (mixed layer, flushing, percentiles, steric
+ * height, vorticity, numbers, profiles, DHW, alerts, region and total
+ * mass, sediments, ecology, waves, T/S increments) are not built.  A
+ * decay timescale from a 2D tracer (DE_TR2) is refused.  tracer_step_3d
+ * calls it after vert_diffuse_3d, as tracers.c:289 does.             */
+void auxiliary_routines(geometry_t *window, window_t *windat, win_priv_t *wincon);
 
 /*-------------------------------------------------------------------*/
 /* Device-resident interface (no counterpart in the reference: the  */
@@ -280,6 +336,13 @@ int emstr_launch_shape(emstr_ctx *ctx, int *tile_x, int *tile_y, int *grid, int 
 /* Only the advection / only the vertical diffusion part.              */
 int emstr_advect_diffuse(emstr_ctx *ctx);
 int emstr_vert_diffuse(emstr_ctx *ctx);
+/* auxiliary_routines (tracers.c:4296-4490, the parts listed at the
+ * drop-in) on the device state of a plain-form context: dz, age, decay
+ * and T/S relaxation; the relaxation fields, time constants and age mask
+ * are uploaded from the window each call.  EMSTR_EUNSUPPORTED for the
+ * FP32 forms when a decay, relaxation or age tracer is set (their
+ * reset_dz runs).                                                      */
+int emstr_auxiliary(emstr_ctx *ctx);
 /* set_dz on the device (z-level branch of tracers.c:5682-5849) from
  * windat->eta, wincon->oldeta and window->nsur_t; keeps the per-column
  * results on the device (the drop-in set_dz() also writes the host lists). */

@@ -1323,15 +1323,210 @@ void oracle_vert_diffuse_3d(geometry_t *g, window_t *wd, win_priv_t *wc) {
 }
 
 /*-------------------------------------------------------------------*/
+/* tracers.c:5900-5980  reset_dz: surface thicknesses for the updated */
+/* elevation, cells to process s2 (z-levels) and the wet-cell mask c1 */
+/*-------------------------------------------------------------------*/
+void oracle_reset_dz(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int cc, c, cs, cn, zp1, zm1, vc, vcs;
+  double top, bot;
+
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    c = max(g->sur_t[cc], g->nsur_t[cc]);
+    cs = g->m2d[c];
+    cn = g->nsur_t[cc];
+    zp1 = g->zp1[c];
+    bot = max(g->gridz[c], g->botz[cs]);
+    while (c > cn) {
+      top = g->gridz[zp1];
+      wc->dz[c] = top - bot;
+      bot = top;
+      c = zp1;
+      zp1 = g->zp1[c];
+    }
+    top = wd->eta[cs];
+    wc->dz[c] = (top > bot) ? top - bot : wc->dz[c];
+  }
+
+  if (!wc->sigma) {
+    vc = 1;
+    for (cc = 1; cc <= g->b2_t; cc++) {
+      c = g->nsur_t[cc];
+      cs = g->m2d[c];
+      top = wd->eta[cs];
+      bot = DRY_FRAC * wc->hmin + g->botz[cs];
+      if (top > bot && c != g->zm1[c]) {
+        wc->s2[vc] = c;
+        wc->i1[vc] = g->bot_t[cc];
+        if (cc == g->v2_t) wc->vca2 = vc;
+        vc++;
+      }
+    }
+    vcs = vc - 1;
+    wc->vcs2 = vcs;
+    for (cc = 1; cc <= vcs; cc++) {
+      c = g->zm1[wc->s2[cc]];
+      zm1 = g->zm1[c];
+      while (c != zm1) {
+        wc->s2[vc] = c;
+        vc++;
+        c = zm1;
+        zm1 = g->zm1[c];
+      }
+      if (cc == wc->vca2) wc->vci2 = vc - 1;
+    }
+    vc--;
+    wc->vc2 = vc;
+  }
+
+  memset(wc->c1, 0, g->sgsiz * sizeof(char));
+  for (cc = 1; cc <= wc->vc2; cc++) wc->c1[wc->s2[cc]] = 1;
+}
+
+/* the age tracer's number (agetr points at one of the 3D tracers) */
+static int age_tracer(window_t *wd) {
+  int n;
+  if (!wd->agetr) return -1;
+  for (n = 0; n < wd->ntr; n++)
+    if (wd->tr_wc[n] == wd->agetr) return n;
+  return -1;
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:6960-6973  calc_age                                      */
+/*-------------------------------------------------------------------*/
+void oracle_calc_age(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int c, cc;
+  double s = 1.0 / 86400.0;
+  for (cc = 1; cc <= g->b3_t; cc++) {
+    c = g->w3_t[cc];
+    if (wc->agemsk[c]) wd->agetr[c] += wd->dttr * s;
+  }
+}
+
+/* tracer_decay.c:45-52: seconds per unit of a decay timescale */
+static double units_scale(const char *u) {
+  if (strcmp(u, "days") == 0 || strcmp(u, "day") == 0) return 86400.0;
+  if (strcmp(u, "hours") == 0 || strcmp(u, "hour") == 0) return 3600.0;
+  if (strcmp(u, "minutes") == 0 || strcmp(u, "minute") == 0) return 60.0;
+  return 1.0;
+}
+
+/*-------------------------------------------------------------------*/
+/* tracer_decay.c:26-63  tracer_decay over the cells to process s1;   */
+/* the timescale is the constant, or (DE_TR3) the value of the 3D     */
+/* tracer whose number the constant holds, in that tracer's units     */
+/* (the reference compares an unset string when neither flag is set:  */
+/* taken as seconds)                                                  */
+/*-------------------------------------------------------------------*/
+void oracle_tracer_decay(geometry_t *g, window_t *wd, win_priv_t *wc, double decay,
+                         double *tracer, tracer_info_t *tr) {
+  int c, cc;
+  double d, sc = 1.0;
+  int tn;
+  (void)g;
+  if (decay == 0.0) return;
+  tn = (int)decay;
+  if (tr->flag & DE_TR2) {
+    oracle_err = 5; /* 2D tracers are not carried */
+    return;
+  }
+  if (tr->flag & DE_TR3) sc = units_scale(wc->trinfo_3d[tn].units);
+  for (cc = 1; cc <= wc->vc; cc++) {
+    c = wc->s1[cc];
+    d = decay;
+    if (tr->flag & DE_TR3) d = wd->tr_wc[tn][c];
+    tracer[c] -= wd->dt * tracer[c] / (d * sc);
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:6122-6207  do_ts_relax (temperature tracer tno, salinity */
+/* sno; a negative number: the tracer is absent)                      */
+/*-------------------------------------------------------------------*/
+static void relax_one(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr, double *rf,
+                      double *tc, tracer_info_t *ti) {
+  int cc, c, c3;
+  double rr = ti->relax_rate;
+  double sf = 1.0 / 86400.0;
+  int tcf = ti->tctype;
+  int ngf = (ti->flag & RLX_GRD) ? 1 : 0;
+  (void)wc;
+  if (rf && ngf) {
+    double depth = ti->relax_dum;
+    for (cc = 1; cc <= g->b2_t; cc++) {
+      c = g->w2_t[cc];
+      while (g->cellz[c] > depth && c < g->bot_t[cc]) c = g->zm1[c];
+      c3 = c;
+      while (c != g->zp1[c]) {
+        c = g->zp1[c];
+        rf[c] = rf[c3];
+      }
+    }
+  }
+  if (rf && rr) {
+    for (cc = 1; cc <= g->b3_t; cc++) {
+      c = g->w3_t[cc];
+      tr[c] -= wd->dt * rr * (tr[c] - rf[c]);
+    }
+  }
+  if (rf && tc && tcf & (RLX_ADPT | RLX_REG | RLX_OBC)) {
+    for (cc = 1; cc <= g->b3_t; cc++) {
+      c = g->w3_t[cc];
+      rr = (tc[c]) ? sf / tc[c] : 0.0;
+      tr[c] -= wd->dt * (tr[c] - rf[c]) * rr;
+    }
+  }
+}
+
+void oracle_do_ts_relax(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int cc, c;
+  double *temp = (wd->tno >= 0) ? wd->tr_wc[wd->tno] : NULL;
+  double *sal = (wd->sno >= 0) ? wd->tr_wc[wd->sno] : NULL;
+  if (temp) relax_one(g, wd, wc, temp, wd->rtemp, wd->temp_tc, &wc->trinfo_3d[wd->tno]);
+  if (sal) relax_one(g, wd, wc, sal, wd->rsalt, wd->salt_tc, &wc->trinfo_3d[wd->sno]);
+  /* a no-gradient above the surface */
+  if (!wc->sigma) {
+    for (cc = 1; cc <= wc->vcs2; cc++) {
+      int cs;
+      c = cs = wc->s2[cc];
+      while (c != g->zp1[c]) {
+        c = g->zp1[c];
+        if (temp) temp[c] = temp[cs];
+        if (sal) sal[c] = sal[cs];
+      }
+    }
+  }
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:4296-4490  auxiliary_routines: the parts that change the */
+/* window's state (reset_dz, calc_age, tracer_decay, do_ts_relax), in */
+/* the reference's order                                              */
+/*-------------------------------------------------------------------*/
+void oracle_auxiliary_routines(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int nn;
+  oracle_reset_dz(g, wd, wc);
+  if (wd->agetr && wc->agemsk) oracle_calc_age(g, wd, wc);
+  for (nn = 0; nn < wc->ntdec; nn++) {
+    int n = wc->tdec[nn];
+    oracle_tracer_decay(g, wd, wc, wc->dectr[n], wd->tr_wc[n], &wc->trinfo_3d[n]);
+  }
+  oracle_do_ts_relax(g, wd, wc);
+}
+
+/*-------------------------------------------------------------------*/
 /* tracers.c:239-321  (the parts on the path for a closed window):    */
-/* lateral ghosts (tracers.c:153-156), advect_diffuse, vert_diffuse   */
+/* lateral ghosts (tracers.c:153-156), advect_diffuse, vert_diffuse,  */
+/* auxiliary_routines                                                 */
 /*-------------------------------------------------------------------*/
 int oracle_tracer_step_3d_mt(geometry_t *g, window_t *wd, win_priv_t *wc,
                              int nthreads) {
   oracle_set_lateral_BC_tr(g, wd);
   if (oracle_advect_diffuse_mt(g, wd, wc, nthreads)) return 1;
   if (oracle_vert_diffuse_3d_mt(g, wd, wc, nthreads)) return 4;
-  return 0;
+  oracle_err = 0;
+  oracle_auxiliary_routines(g, wd, wc);
+  return oracle_err;
 }
 
 void oracle_tracer_step_3d(geometry_t *g, window_t *wd, win_priv_t *wc) {

@@ -41,6 +41,10 @@ def lib():
                    "oracle_tracer_step_3d_mt"):
             getattr(L, fn).argtypes = [G, W, P, C.c_int]
             getattr(L, fn).restype = C.c_int
+        for fn in ("oracle_reset_dz", "oracle_calc_age", "oracle_do_ts_relax",
+                   "oracle_auxiliary_routines"):
+            getattr(L, fn).argtypes = [G, W, P]
+            getattr(L, fn).restype = None
         for fn in ("oracle_set_flux_3d", "oracle_vel_w_update"):
             getattr(L, fn).argtypes = [G, W, P]
             getattr(L, fn).restype = None
@@ -114,3 +118,10 @@ def transport_step(case, nthreads=1):
     transport_post) from the stored flow in the window."""
     return lib().oracle_transport_step_mt(case.gp, case.wp, case.pp, nthreads)
 
+
+def auxiliary_routines(case):
+    """auxiliary_routines' state changes (reset_dz, calc_age, tracer_decay,
+    do_ts_relax); returns the oracle's error code (5: a DE_TR2 decay)."""
+    L = lib()
+    L.oracle_auxiliary_routines(case.gp, case.wp, case.pp)
+    return L.oracle_last_error()

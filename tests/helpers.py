@@ -80,10 +80,28 @@ def scheme(case, trasc, ultimate=0, dt=60.0, mintr=0.0, maxtr=1e35):
     S.view(wc.maxtr, case.ntr, np.float64)[:] = maxtr
 
 
+def _lists(case):
+    """set_dz's cell list s1 and its bed cells i1 (reset_dz in
+    auxiliary_routines writes its own list into i1, tracers.c:5948),
+    as numpy views over their sizes (3D and 2D sparse)."""
+    wc = case.wc
+    return [S.view(wc.s1, case.sgsiz, np.int32), S.view(wc.i1, case.sgsizS, np.int32)]
+
+
 def snapshot(case):
-    return [case.tr(n).copy() for n in range(case.ntr)]
+    """The tracers and, on a z-level (SHOC) window, the layer thicknesses
+    and the cell lists s1/i1 (tracer_step_3d's auxiliary_routines resets
+    the surface thicknesses and i1, tracers.c:289)."""
+    snap = [case.tr(n).copy() for n in range(case.ntr)]
+    if hasattr(case, "a3"):
+        snap += [case.a3(case.wc.dz).copy()] + [a.copy() for a in _lists(case)]
+    return snap
 
 
 def restore(case, snap):
     for n in range(case.ntr):
         case.tr(n)[:] = snap[n]
+    if hasattr(case, "a3"):
+        case.a3(case.wc.dz)[:] = snap[case.ntr]
+        for a, b in zip(_lists(case), snap[case.ntr + 1:]):
+            a[:] = b
