@@ -238,9 +238,13 @@ k_ffline(emstr_usdev D, double dtu, const ll *__restrict__ edn, int nesp,
   const int ci = (u1 > 0.0) ? D.ec2[e] : D.ec1[e];
   const double px = D.ux[e], py = D.uy[e];
   /* get_posc (transport.c:1476-1559), metres */
-  const double dist = sqrt(u * u + v * v) * dtu * 1.0;
-  const double dir = atan2(v, u);
-  const double sinth = sin(dir), costh = cos(dir);
+  /* the direction's cosine and sine as u / s and v / s, the exact values
+   * of cos(atan2(v, u)) and sin(atan2(v, u)) (correctly rounded division
+   * and square root, so the end point does not depend on a math
+   * library); at s = 0 cosine 1 and sine 0 as atan2(0, 0) = 0 gives */
+  const double s = sqrt(u * u + v * v);
+  const double dist = s * dtu * 1.0;
+  const double costh = (s > 0.0) ? u / s : 1.0, sinth = (s > 0.0) ? v / s : 0.0;
   const double slon = px - dist * costh, slat = py - dist * sinth;
   double ex = px, ey = py, od = 0.0;
   if (dist != 0.0) {

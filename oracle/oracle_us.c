@@ -812,11 +812,17 @@ static int ff_posc(geometry_t *g, int *ei, int ci, double u, double v, double *c
                    double *cy, double dt, double *odist) {
   int cn, j, e = 0, found = 0;
   const int cis = g->m2d[ci];
-  double xin = *cx, yin = *cy, dist, dir, slon, slat, sinth, costh, xi = 0.0, yi = 0.0;
-  dist = sqrt(u * u + v * v) * dt * 1.0;
-  dir = atan2(v, u);
-  sinth = sin(dir);
-  costh = cos(dir);
+  double xin = *cx, yin = *cy, dist, slon, slat, sinth, costh, xi = 0.0, yi = 0.0;
+  /* The reference takes the direction's cosine and sine as
+   * cos(atan2(v, u)) and sin(atan2(v, u)) from the C library, whose last
+   * bit differs between libraries; their exact values u / s and v / s
+   * (s = sqrt(u^2 + v^2), correctly rounded here and on the device) make
+   * the end point the same on every platform.  At s = 0, atan2(0, 0) = 0:
+   * cosine 1, sine 0. */
+  const double s = sqrt(u * u + v * v);
+  dist = s * dt * 1.0;
+  costh = (s > 0.0) ? u / s : 1.0;
+  sinth = (s > 0.0) ? v / s : 0.0;
   slon = xin - dist * costh;
   slat = yin - dist * sinth;
   if (dist == 0.0) {

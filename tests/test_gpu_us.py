@@ -61,7 +61,7 @@ def assert_exact(gpu, ora):
     d = np.abs(gpu - ora)
     scale = np.maximum(np.abs(ora), 1e-300)
     rel = (d / scale).max()
-    assert np.array_equal(gpu, ora) or rel <= 1e-12, ("max rel diff", rel, int((d > 0).sum()))
+    assert np.array_equal(gpu, ora), ("max rel diff", rel, int((d > 0).sum()))
 
 
 CASES = [
@@ -182,7 +182,7 @@ RK_CASES = [
 def test_hex_runge_kutta_stages(name, kw, rkstage, stab, nsteps):
     """Runge-Kutta stage loop (tracers.c:841-1098): 2, 3 and 5 stages
     against the oracle (whose stages are pinned by the forward-Euler
-    compositions of tests/test_oracle_us_rk.py); FFSL to 1e-12 relative."""
+    compositions of tests/test_oracle_us_rk.py), every scheme bit for bit."""
     say("case hex", name)
     case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=3, seed=7, **kw)
     case.wc.rkstage = rkstage
@@ -256,9 +256,8 @@ FFSL_CASES = [
 def test_hex_ffsl_step(name, kw, stab, trsf, nsteps):
     """FFSL streamlines, least squares and transverse terms; ffsl_init
     runs on every other call, so several steps cover both branches.  The
-    streamline directions come from atan2/sin/cos, whose last bit may
-    differ between the host's and the device's libraries: tracers agree
-    to 1e-12 relative."""
+    streamline directions are u / |u| and v / |u| on both sides (no math
+    library), so tracers agree bit for bit."""
     say("case hex", name)
     case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=3, seed=17, trasc=U.FFSL, **kw)
     case.wc.stab = stab
@@ -285,7 +284,7 @@ CLIP_CASES = [
 @pytest.mark.parametrize("name,kw,stab,trsf,nsteps,fill", CLIP_CASES)
 def test_hex_ffsl_clip(name, kw, stab, trsf, nsteps, fill):
     """FILL_METHOD CLIP (clip_ffsl): each centre clamped to the range of
-    its streamlines' stencils; tracers within 1e-12 of the oracle, and
+    its streamlines' stencils; tracers bit for bit with the oracle, and
     the clip acts (the result differs from the unclipped step)."""
     say("case hex", name)
     res = {}
@@ -447,19 +446,14 @@ def test_ffsl_maps_bit_exact_c4():
 def test_c4_tracer_values_vs_oracle(trasc):
     """The full C4 mesh (350 x 350 jittered hexagons, 50 layers), three
     tracers, two steps through the drop-in tracer_step_3d against the
-    oracle: every tracer value within 1e-12 relative (FFSL: the streamline
-    directions come from atan2/sin/cos of two libraries), van Leer bit for
-    bit."""
+    oracle: every tracer value bit for bit, FFSL and van Leer."""
     say("case c4 tracer values, trasc", trasc, "(mesh build ~30 s)")
     case = MS.c4(seed=3, ntr=3, trasc=trasc)
     case.prepare(OU.set_dz)
     say("c4 built:", case.nwet, "wet cells")
     start = case.tracers().copy()
     gpu, ora = run_both(case, 2)
-    if trasc == U.VANLEER:
-        assert np.array_equal(gpu, ora)
-    else:
-        assert_exact(gpu, ora)
+    assert_exact(gpu, ora)
     assert not np.array_equal(gpu, start)
 
 
