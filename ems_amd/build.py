@@ -42,7 +42,7 @@ def _run(cmd):
 
 # libemstr_us.so: the COMPAS path (unstructured meshes)
 US_HIP_SRC = ["emstr_us.hip", "emstr_us_ffsl.hip"]
-US_C_SRC = ["emstr_mesh.c", "emstr_us_host.c", "emstr_err.c"]
+US_C_SRC = ["emstr_mesh.c", "emstr_lsq.c", "emstr_us_host.c", "emstr_err.c"]
 # the column kernels of the plain form (set_dz, vertical diffusion, ...)
 US_SHARED_HIP = ["emstr_plain.hip"]
 

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include "../../include/emstr_mesh.h"
+#include "emstr_lsq.h"
 
 static void *zalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
 
@@ -598,43 +599,10 @@ int emstr_mesh_vertices(emstr_mesh *M, int nv, const int *vid) {
 /* from the eigen-decomposition of the 3x3 normal matrix with the same */
 /* cut of near-zero singular values.                                   */
 /*-------------------------------------------------------------------*/
-static void jacobi3(double A[3][3], double V[3][3], double ev[3]) {
-  int i, j, p, q, sweep;
-  for (i = 0; i < 3; i++)
-    for (j = 0; j < 3; j++) V[i][j] = (i == j) ? 1.0 : 0.0;
-  for (sweep = 0; sweep < 50; sweep++) {
-    double off = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
-    if (off == 0.0) break;
-    for (p = 0; p < 2; p++)
-      for (q = p + 1; q < 3; q++) {
-        if (A[p][q] == 0.0) continue;
-        double th = 0.5 * (A[q][q] - A[p][p]) / A[p][q];
-        double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-        double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
-        for (i = 0; i < 3; i++) {
-          double aip = A[i][p], aiq = A[i][q];
-          A[i][p] = c * aip - sn * aiq;
-          A[i][q] = sn * aip + c * aiq;
-        }
-        for (i = 0; i < 3; i++) {
-          double api = A[p][i], aqi = A[q][i];
-          A[p][i] = c * api - sn * aqi;
-          A[q][i] = sn * api + c * aqi;
-        }
-        for (i = 0; i < 3; i++) {
-          double vip = V[i][p], viq = V[i][q];
-          V[i][p] = c * vip - sn * viq;
-          V[i][q] = sn * vip + c * viq;
-        }
-      }
-  }
-  for (i = 0; i < 3; i++) ev[i] = A[i][i];
-}
-
 int emstr_mesh_lsq(emstr_mesh *M) {
   geometry_t *g = M->window;
   win_priv_t *wc = M->wincon;
-  int cc, j, i, n;
+  int cc, i, n;
   if (!g || !g->c2v || wc->lw) return -1;
   wc->lw = (qweights *)zalloc(g->szc, sizeof(qweights));
   int *mask = (int *)zalloc(g->szcS, sizeof(int));
@@ -642,25 +610,12 @@ int emstr_mesh_lsq(emstr_mesh *M) {
   int *cells = (int *)zalloc(64 * (g->npem + 1), sizeof(int));
   double *px = (double *)zalloc(64 * (g->npem + 1), sizeof(double));
   double *py = (double *)zalloc(64 * (g->npem + 1), sizeof(double));
+  double *Bw = (double *)zalloc(3 * 64 * (g->npem + 1), sizeof(double));
   for (cc = 1; cc <= g->a3_t; cc++) {
     const int c = g->w3_t[cc], cs = g->m2d[c];
-    int ns = 0, nc = 0;
+    int nc = 0;
     /* stencil: the centre, then the wet centres around its vertices */
-    st[ns++] = c;
-    mask[cs] = 1;
-    for (j = 1; j <= g->npe[cs]; j++) {
-      const int v = g->c2v[j][c];
-      if (!v) continue;
-      const int vs = g->m2dv[v];
-      for (i = 1; i <= g->nvc[vs]; i++) {
-        const int c1 = g->v2c[v][i];
-        if (c1 && !mask[g->m2d[c1]] && ns < 64) {
-          st[ns++] = c1;
-          mask[g->m2d[c1]] = 1;
-        }
-      }
-    }
-    for (i = 0; i < ns; i++) mask[g->m2d[st[i]]] = 0;
+    const int ns = emstr_lsq_stencil(g, c, st, 64, mask);
     /* each stencil centre, then the ghosts across its edges */
     for (i = 0; i < ns; i++) {
       const int cn = st[i], cns = g->m2d[cn];
@@ -679,30 +634,20 @@ int emstr_mesh_lsq(emstr_mesh *M) {
       }
     }
     /* pseudo-inverse of the rows (1, x, y) */
-    double A[3][3] = {{0}}, V[3][3], ev[3];
     for (i = 0; i < nc; i++) {
-      const double r[3] = {1.0, g->cellx[cs] - px[i], g->celly[cs] - py[i]};
-      for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) A[a][b] += r[a] * r[b];
+      px[i] = g->cellx[cs] - px[i];
+      py[i] = g->celly[cs] - py[i];
     }
-    jacobi3(A, V, ev);
-    double emax = fmax(fabs(ev[0]), fmax(fabs(ev[1]), fabs(ev[2])));
-    double Ainv[3][3] = {{0}};
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++)
-        for (int l = 0; l < 3; l++)
-          if (fabs(ev[l]) > 1e-12 * emax) Ainv[a][b] += V[a][l] * V[b][l] / ev[l];
+    emstr_lsq_pinv3(nc, px, py, Bw);
     qweights *lw = &wc->lw[c];
     lw->ncells = nc;
     lw->cells = (int *)zalloc(nc, sizeof(int));
     lw->B = (double **)zalloc(nc, sizeof(double *));
     lw->B[0] = (double *)zalloc((size_t)nc * 3, sizeof(double));
     for (i = 0; i < nc; i++) {
-      const double r[3] = {1.0, g->cellx[cs] - px[i], g->celly[cs] - py[i]};
       lw->cells[i] = cells[i];
       lw->B[i] = lw->B[0] + (size_t)i * 3;
-      for (int a = 0; a < 3; a++)
-        lw->B[i][a] = Ainv[a][0] * r[0] + Ainv[a][1] * r[1] + Ainv[a][2] * r[2];
+      for (int a = 0; a < 3; a++) lw->B[i][a] = Bw[3 * i + a];
     }
     lw->beta = 1.0;
   }
@@ -711,5 +656,6 @@ int emstr_mesh_lsq(emstr_mesh *M) {
   free(cells);
   free(px);
   free(py);
+  free(Bw);
   return 0;
 }

@@ -355,7 +355,9 @@ __device__ double us_subsurface(const emstr_usdev &D, const UsRk &w, int col, in
  * value; thicknesses and walk as in the vertical flux).  Then each edge's
  * (transport.c:4122-4127): a wet edge's is the range over the
  * least-squares stencil of its upstream centre (transport.c:2207,
- * 2309-2311), a land edge's is zero (its bounds are never written and
+ * 2309-2311; for a streamline of several segments the upstream value
+ * and the stencil ranges of the cells it crosses into), a land edge's
+ * is zero (its bounds are never written and
  * were allocated zero).  min and max are the reference's macros. */
 __device__ __forceinline__ double mn_(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double mx_(double a, double b) { return (a > b) ? a : b; }
@@ -395,8 +397,23 @@ __device__ void us_clip_range(const emstr_usdev &D, int col, int k, int n, doubl
     } else {
       const int up = (D.u1[(ll)k * D.ne2 + e] > 0.0) ? D.ec2[e] : D.ec1[e];
       const ll iu = n * N3 + (ll)k * P + up;
-      lo = mn_(lo, bmn[iu]);
-      hi = mx_(hi, bmx[iu]);
+      const int r0 = D.lnx ? D.lnx[(ll)k * D.ne2 + e] : -1;
+      if (r0 < 0) {
+        lo = mn_(lo, bmn[iu]);
+        hi = mx_(hi, bmx[iu]);
+      } else {
+        /* a streamline of several segments: the upstream value, then
+         * the stencil range of each cell it crosses into, the last one
+         * (where it ends) included */
+        const double tu = t[(ll)k * P + up];
+        lo = mn_(lo, tu);
+        hi = mx_(hi, tu);
+        for (int r = r0; r >= 0; r = D.sg_nx[r]) {
+          const ll ir = n * N3 + (ll)k * P + D.sg_ci[r];
+          lo = mn_(lo, bmn[ir]);
+          hi = mx_(hi, bmx[ir]);
+        }
+      }
     }
   }
 }

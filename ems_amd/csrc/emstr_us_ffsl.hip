@@ -13,11 +13,15 @@
  *   k_ffsub    per sub-step: vertical departure layers and fractions of
  *              the centres (nw, w9) and of the faces of the cells to
  *              process (w, dz), and their largest Courant number
+ *   k_ffvfit   on the calls where ffsl_init runs: the velocity fit of
+ *              every wet centre (delaunay_reinit / lsql_rebuild)
  *   k_ffline   per sub-step, one thread per wet edge: the streamline back
  *              from the edge midpoint through its upstream cell
  *              (get_posc / intersect), stored as offsets from the cell
- *              centre and a length; a streamline that reaches a wet
- *              neighbour is reported (it needs hd_trans_interp)
+ *              centre and a length; where it crosses into a wet cell it
+ *              goes on from the crossing with the velocity fitted there
+ *              (hd_trans_interp), each later segment a record chained
+ *              from the edge
  *   k_ffmod    tr_mod: the vertical transverse tracer, no-gradient above
  *              the processed surface
  *   k_fflsq    the least-squares slopes of every centre and the vertex
@@ -149,6 +153,9 @@ k_ffsub(emstr_usdev D, double dtu, const double *__restrict__ nw, int *__restric
         double *__restrict__ cour) {
   const emstr_dev &d = D.d;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  /* the streamline records of this sub-step start from zero (k_ffline
+   * runs after this kernel on the stream) */
+  if (q == 0 && D.sg_cnt) *D.sg_cnt = 0;
   if (q >= d.nwet) return;
   const ll P = d.plane;
   const ll idx = d.wdn[q];
@@ -221,6 +228,55 @@ __device__ __forceinline__ int ffcross(const emstr_usdev &D, int es, double xn, 
   return 0;
 }
 
+/* the velocity fit of every wet centre when ffsl_init runs
+ * (delaunay_reinit of u and v, transport.c:4728-4929, lsql_rebuild,
+ * lib/interp/lsq_lin/lsql.c:262-287): the slopes sum v_j bx_j, v_j by_j
+ * in the table's order from 0, the value the centre's own; a ghost's v_j
+ * is the edge velocity rotated to east / north, which ffsl_init stores
+ * there (transport.c:2652-2666) */
+__global__ void __launch_bounds__(256) k_ffvfit(emstr_usdev D) {
+  const emstr_dev &d = D.d;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.nwet) return;
+  const ll N3 = d.n3, idx = d.wdn[q];
+  double w1u = 0.0, w2u = 0.0, w1v = 0.0, w2v = 0.0;
+  for (int o = D.vls_off[q]; o < D.vls_off[q + 1]; o++) {
+    const ll ent = D.vls_ent[o];
+    double zu, zv;
+    if (ent >= 0) {
+      zu = D.uc[ent];
+      zv = D.vc[ent];
+    } else {
+      const ll ie = ~ent;
+      const int e = (int)(ie % D.ne2);
+      zu = D.u1[ie] * D.n1x[e] + D.u2[ie] * D.t2x[e];
+      zv = D.u1[ie] * D.n1y[e] + D.u2[ie] * D.t2y[e];
+    }
+    w1u += zu * D.vls_bx[o];
+    w2u += zu * D.vls_by[o];
+    w1v += zv * D.vls_bx[o];
+    w2v += zv * D.vls_by[o];
+  }
+  D.vfit[idx] = D.uc[idx];
+  D.vfit[N3 + idx] = w1u;
+  D.vfit[2 * N3 + idx] = w2u;
+  D.vfit[3 * N3 + idx] = D.vc[idx];
+  D.vfit[4 * N3 + idx] = w1v;
+  D.vfit[5 * N3 + idx] = w2v;
+}
+
+/* ffsl_don's streamline loop (transport.c:2192-2333) for one wet edge:
+ * get_posc (transport.c:1476-1559, metres) from the edge midpoint through
+ * the upstream centre; where the segment crosses into a wet cell, the
+ * next one starts at the crossing, skips the edge crossed, runs for the
+ * time left with the velocity fitted there, and so on (at most 100
+ * segments, as the reference).  The velocity of a later segment is
+ * hd_trans_interp's (transport.c:5171-5296): its depth is the centre's
+ * own (pz = cellz[c]), so the vertical blend of the fits of the cell and
+ * of the cell above gives the cell's fit exactly; that fit at (x, y) is
+ * value + (x slope (x - xc) + y slope (y - yc)) (lsql_interpolate_point
+ * with beta 1).  err: 1 a later segment before ffsl_init has run on this
+ * context, 5 more than 100 segments, 6 the records full. */
 __global__ void __launch_bounds__(256)
 k_ffline(emstr_usdev D, double dtu, const ll *__restrict__ edn, int nesp,
          int *__restrict__ lci, double *__restrict__ ld, int *__restrict__ err) {
@@ -228,56 +284,100 @@ k_ffline(emstr_usdev D, double dtu, const ll *__restrict__ edn, int nesp,
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nesp) return;
   const ll ie = edn[q];
-  const ll NE3 = (ll)d.nz * D.ne2;
+  const ll NE3 = (ll)d.nz * D.ne2, N3 = d.n3;
   const int k = (int)(ie / D.ne2), e = (int)(ie % D.ne2);
   const int ncol = (int)d.plane;
   const double u1 = D.u1[ie], u2 = D.u2[ie];
-  const double u = u1 * D.n1x[e] + u2 * D.t2x[e];
-  const double v = u1 * D.n1y[e] + u2 * D.t2y[e];
+  double u = u1 * D.n1x[e] + u2 * D.t2x[e];
+  double v = u1 * D.n1y[e] + u2 * D.t2y[e];
   /* the upstream centre (transport.c:2162-2173) */
-  const int ci = (u1 > 0.0) ? D.ec2[e] : D.ec1[e];
-  const double px = D.ux[e], py = D.uy[e];
-  /* get_posc (transport.c:1476-1559), metres */
-  /* the direction's cosine and sine as u / s and v / s, the exact values
-   * of cos(atan2(v, u)) and sin(atan2(v, u)) (correctly rounded division
-   * and square root, so the end point does not depend on a math
-   * library); at s = 0 cosine 1 and sine 0 as atan2(0, 0) = 0 gives */
-  const double s = sqrt(u * u + v * v);
-  const double dist = s * dtu * 1.0;
-  const double costh = (s > 0.0) ? u / s : 1.0, sinth = (s > 0.0) ? v / s : 0.0;
-  const double slon = px - dist * costh, slat = py - dist * sinth;
-  double ex = px, ey = py, od = 0.0;
-  if (dist != 0.0) {
-    int found = 0, cn = ci;
-    double xi = 0.0, yi = 0.0;
-    for (int j = 0; j < D.npe[ci]; j++) {
-      const int ej = D.ce[j * ncol + ci];
-      if (ej != e && (found = ffcross(D, ej, px, py, slon, slat, xi, yi))) {
-        cn = (D.ec1[ej] == ci) ? D.ec2[ej] : D.ec1[ej];
+  int ci = (u1 > 0.0) ? D.ec2[e] : D.ec1[e];
+  double px = D.ux[e], py = D.uy[e], tleft = dtu;
+  int ei = e, prev = -1;
+  lci[ie] = ci;
+  if (D.lnx) D.lnx[ie] = -1;
+  for (int nseg = 1;; nseg++) {
+    if (nseg > 1) {
+      if (!D.vfit_ok) {
+        atomicMax(err, 1);
         break;
       }
+      if (nseg > 100) {
+        atomicMax(err, 5);
+        break;
+      }
+      const ll ic = (ll)k * ncol + ci;
+      const double xp = px - D.cx[ci], yp = py - D.cy[ci];
+      u = D.vfit[ic] + (D.vfit[N3 + ic] * xp + D.vfit[2 * N3 + ic] * yp);
+      v = D.vfit[3 * N3 + ic] + (D.vfit[4 * N3 + ic] * xp + D.vfit[5 * N3 + ic] * yp);
     }
-    if (found && ffwet(d.kb, cn, k)) {
-      /* the streamline continues in a wet cell: multi-segment */
-      atomicMax(err, 1);
+    /* the direction's cosine and sine as u / s and v / s, the exact
+     * values of cos(atan2(v, u)) and sin(atan2(v, u)) (correctly rounded
+     * division and square root, so the end point does not depend on a
+     * math library); at s = 0 cosine 1 and sine 0 as atan2(0, 0) = 0
+     * gives */
+    const double s = sqrt(u * u + v * v);
+    const double dist = s * tleft * 1.0;
+    const double costh = (s > 0.0) ? u / s : 1.0, sinth = (s > 0.0) ? v / s : 0.0;
+    const double slon = px - dist * costh, slat = py - dist * sinth;
+    double ex = px, ey = py, od = 0.0;
+    int found = 0, cn = ci, ecross = ei;
+    if (dist != 0.0) {
+      double xi = 0.0, yi = 0.0;
+      for (int j = 0; j < D.npe[ci]; j++) {
+        const int ej = D.ce[j * ncol + ci];
+        if (ej != ei && (found = ffcross(D, ej, px, py, slon, slat, xi, yi))) {
+          cn = (D.ec1[ej] == ci) ? D.ec2[ej] : D.ec1[ej];
+          ecross = ej;
+          break;
+        }
+      }
+      if (found) {
+        ex = xi;
+        ey = yi;
+      } else {
+        ex = slon;
+        ey = slat;
+      }
+      const double dx = ex - px, dy = ey - py;
+      od = sqrt(dx * dx + dy * dy) / 1.0;
     }
-    if (found) {
-      ex = xi;
-      ey = yi;
+    const double cxc = D.cx[ci], cyc = D.cy[ci];
+    if (nseg == 1) {
+      ld[0 * NE3 + ie] = cxc - px;
+      ld[1 * NE3 + ie] = cyc - py;
+      ld[2 * NE3 + ie] = cxc - ex;
+      ld[3 * NE3 + ie] = cyc - ey;
+      ld[4 * NE3 + ie] = od;
     } else {
-      ex = slon;
-      ey = slat;
+      const int r = atomicAdd(D.sg_cnt, 1);
+      if (r >= D.sg_cap) {
+        atomicMax(err, 6);
+        break;
+      }
+      D.sg_ci[r] = ci;
+      D.sg_nx[r] = -1;
+      double *sd = D.sg_d + (ll)r * 5;
+      sd[0] = cxc - px;
+      sd[1] = cyc - py;
+      sd[2] = cxc - ex;
+      sd[3] = cyc - ey;
+      sd[4] = od;
+      if (prev < 0)
+        D.lnx[ie] = r;
+      else
+        D.sg_nx[prev] = r;
+      prev = r;
     }
-    const double dx = ex - px, dy = ey - py;
-    od = sqrt(dx * dx + dy * dy) / 1.0;
+    /* on into a wet cell: the time the segment took (transport.c:
+     * 2286-2290) */
+    if (!(found && ffwet(d.kb, cn, k))) break;
+    tleft -= od / sqrt(u * u + v * v);
+    ci = cn;
+    ei = ecross;
+    px = ex;
+    py = ey;
   }
-  lci[ie] = ci;
-  const double cxc = D.cx[ci], cyc = D.cy[ci];
-  ld[0 * NE3 + ie] = cxc - px;
-  ld[1 * NE3 + ie] = cyc - py;
-  ld[2 * NE3 + ie] = cxc - ex;
-  ld[3 * NE3 + ie] = cyc - ey;
-  ld[4 * NE3 + ie] = od;
 }
 
 /*-------------------------------------------------------------------*/
@@ -490,6 +590,7 @@ k_ffedge(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ e
   const T dxs = (T)ld[ie], dys = (T)ld[NE3 + ie], dxe = (T)ld[2 * NE3 + ie];
   const T dye = (T)ld[3 * NE3 + ie], dist = (T)ld[4 * NE3 + ie];
   const T f = (T)D.flux[ie];
+  const int nx0 = D.lnx ? D.lnx[ie] : -1;
   const ll i1 = (ll)k * P + D.ec1[e], i2 = (ll)k * P + D.ec2[e];
   /* the end value is read only by the tr_moc of the centres either side */
   const bool wte = (D.ec1[e] >= 0 && need[i1] == stamp) || (D.ec2[e] >= 0 && need[i2] == stamp);
@@ -500,10 +601,23 @@ k_ffedge(emstr_usdev D, const double *__restrict__ tin, const ll *__restrict__ e
       const T w0 = lq[3 * NT3 + n * N3 + cidx];
       const T w1 = lq[n * N3 + cidx], w2 = lq[NT3 + n * N3 + cidx];
       const T b = lq[2 * NT3 + n * N3 + cidx];
-      const T trs = w0 + b * (w1 * dxs + w2 * dys);
-      const T tre = w0 + b * (w1 * dxe + w2 * dye);
-      const T dint = (T)0 + dist;
-      const T ntr = (T)0 + (T)0.5 * dist * (trs + tre);
+      T trs = w0 + b * (w1 * dxs + w2 * dys);
+      T tre = w0 + b * (w1 * dxe + w2 * dye);
+      T dint = (T)0 + dist;
+      T ntr = (T)0 + (T)0.5 * dist * (trs + tre);
+      /* later segments, each from its own centre's reconstruction */
+      for (int r = nx0; r >= 0; r = D.sg_nx[r]) {
+        const ll ri = (ll)k * P + D.sg_ci[r];
+        const double *sd = D.sg_d + (ll)r * 5;
+        const T r0 = lq[3 * NT3 + n * N3 + ri];
+        const T r1 = lq[n * N3 + ri], r2 = lq[NT3 + n * N3 + ri];
+        const T rb = lq[2 * NT3 + n * N3 + ri];
+        const T rd = (T)sd[4];
+        trs = r0 + rb * (r1 * (T)sd[0] + r2 * (T)sd[1]);
+        tre = r0 + rb * (r1 * (T)sd[2] + r2 * (T)sd[3]);
+        dint += rd;
+        ntr += (T)0.5 * rd * (trs + tre);
+      }
       F = (dint != (T)0) ? ntr / dint : (T)0;
       F *= f;
       if (wte) te[n * NE3 + ie] = tre;
@@ -647,6 +761,13 @@ extern "C" int emstr_us_launch_ffsl_step(const emstr_usdev *D, int init, double 
   const emstr_dev *d = &D->d;
   if (d->nwet)
     k_ffnw<<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(*D, init, nw);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int emstr_us_launch_ffsl_vfit(const emstr_usdev *D, void *stream) {
+  const emstr_dev *d = &D->d;
+  if (d->nwet && D->vls_off)
+    k_ffvfit<<<(d->nwet + 255) / 256, 256, 0, (hipStream_t)stream>>>(*D);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -66,6 +66,10 @@ typedef struct emstr_us_ctx {
   double *stage; /* [max(szc, sze)] */
   int *d_esp;    /* wet edges: sparse index and dense index */
   ll *d_edn;
+  int *d_lesp;   /* FFSL: land edges, sparse and dense index (their u1, u2
+                    give the ghost velocities of the streamline fit)   */
+  ll *d_ledn;
+  int nlesp;
   int *d_bpt, *d_bin, *d_gsed, *d_ised;
   int *d_proc, *d_err;
   unsigned long long *d_dtm;
@@ -125,7 +129,9 @@ static void us_destroy(usctx *c) {
                 D->wbot, D->uc, D->vc, D->gu0, D->gv0, D->gue, D->gve, c->d_nw, c->d_crc,
                 c->d_crf, c->d_cour, c->d_ld, c->d_tm, c->d_lq, c->d_te, c->d_moc,
                 c->d_clc, c->d_clf, c->d_lci, c->d_fferr, c->d_need, c->f_fx, c->f_fz,
-                c->f_tm, c->f_lq, c->f_te, c->f_moc, D->lsq_gc, c->d_bmn, c->d_bmx};
+                c->f_tm, c->f_lq, c->f_te, c->f_moc, D->lsq_gc, c->d_bmn, c->d_bmx,
+                D->vls_off, D->vls_ent, D->vls_bx, D->vls_by, D->vfit, D->lnx, D->sg_ci,
+                D->sg_nx, D->sg_d, D->sg_cnt, c->d_lesp, c->d_ledn};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
   void *hv[] = {c->cs2col, c->col2cs, c->h_kb, c->h_ks, c->h_kold, c->h_knew, c->h_cols,
@@ -578,6 +584,16 @@ static int up_edges(usctx *c, double *dst, const double *sp) {
   return 0;
 }
 
+/* the land edges' values of an edge array (FFSL: the ghost velocities) */
+static int up_land_edges(usctx *c, double *dst, const double *sp) {
+  if (!c->nlesp) return 0;
+  HCHK(hipMemcpyAsync(c->stage, sp, (size_t)c->g->sze * sizeof(double), hipMemcpyHostToDevice,
+                      c->st));
+  if (emstr_launch_scatter(c->d_lesp, c->d_ledn, c->nlesp, c->stage, dst, c->st))
+    return emstr_set_error(E_DEVICE, -1, "land edge scatter");
+  return 0;
+}
+
 static int up_cols(usctx *c, double *dst, const double *sp2) {
   double *h = (double *)malloc((size_t)c->ncol * sizeof(double));
   for (int col = 0; col < c->ncol; col++) h[col] = sp2[c->col2cs[col]];
@@ -736,6 +752,87 @@ static int ff_slot(int *rw, int *nr, int q) {
 static int icmp(const void *a, const void *b) {
   int x = *(const int *)a, y = *(const int *)b;
   return (x > y) - (x < y);
+}
+
+/* The streamline velocity fit (emstr_us_vlsq_build) as device tables by
+ * wet centre, with the fit and the segment records of multi-segment
+ * streamlines; a ghost entry becomes its edge at the centre's layer. */
+static int us_ffsl_vlsq(usctx *c) {
+  geometry_t *g = c->g;
+  emstr_usdev *D = &c->D;
+  emstr_dev *d = &D->d;
+  const int ncol = c->ncol, ne2 = c->ne2;
+  const size_t N3 = (size_t)d->n3, NE3 = (size_t)ne2 * d->nz;
+  emstr_us_vlsq t;
+  int ok = 1, q;
+  if (emstr_us_vlsq_build(g, &t))
+    return emstr_set_error(E_UNSUP, -1, "FFSL: streamline velocity fit (no vertex maps)");
+  int *off = (int *)malloc(((size_t)d->nwet + 1) * sizeof(int));
+  ll *ent = (ll *)malloc(((size_t)t.nent + 1) * sizeof(ll));
+  double *bx = (double *)malloc(((size_t)t.nent + 1) * sizeof(double));
+  double *by = (double *)malloc(((size_t)t.nent + 1) * sizeof(double));
+  int m = 0;
+  for (q = 0; q < d->nwet && off && ent && bx && by; q++) {
+    const int cw = g->w3_t[q + 1], k = g->s2k[cw];
+    off[q] = m;
+    for (int o = t.vo[cw]; o < t.vo[cw] + t.vn[cw]; o++, m++) {
+      const int x = t.cell[o];
+      if (g->wgst[x])
+        ent[m] = ~((ll)k * ne2 + (g->m2de[t.edge[o]] - 1));
+      else
+        ent[m] = (ll)g->s2k[x] * ncol + c->cs2col[g->m2d[x]];
+      bx[m] = t.bx[o];
+      by[m] = t.by[o];
+    }
+  }
+  if (off) off[d->nwet] = m;
+  emstr_us_vlsq_free(&t);
+  if (!off || !ent || !bx || !by) ok = 0;
+  D->vls_off = (int *)dalloc(((size_t)d->nwet + 1) * sizeof(int), &ok);
+  D->vls_ent = (ll *)dalloc(((size_t)m + 1) * sizeof(ll), &ok);
+  D->vls_bx = (double *)dalloc(((size_t)m + 1) * sizeof(double), &ok);
+  D->vls_by = (double *)dalloc(((size_t)m + 1) * sizeof(double), &ok);
+  D->vfit = (double *)dalloc(6 * N3 * sizeof(double), &ok);
+  D->lnx = (int *)dalloc(NE3 * sizeof(int), &ok);
+  D->sg_cap = (int)(NE3 / 8 > 65536 ? NE3 / 8 : 65536);
+  D->sg_ci = (int *)dalloc((size_t)D->sg_cap * sizeof(int), &ok);
+  D->sg_nx = (int *)dalloc((size_t)D->sg_cap * sizeof(int), &ok);
+  D->sg_d = (double *)dalloc((size_t)D->sg_cap * 5 * sizeof(double), &ok);
+  D->sg_cnt = (int *)dalloc(sizeof(int), &ok);
+  D->vfit_ok = 0;
+  /* land edges: the ghost velocity of the fit is the land edge's u1, u2
+   * rotated, so those go to the dense edge arrays too */
+  int nl = 0;
+  int *lesp = (int *)malloc(((size_t)(g->n3_e1 - g->v3_e1) + 1) * sizeof(int));
+  ll *ledn = (ll *)malloc(((size_t)(g->n3_e1 - g->v3_e1) + 1) * sizeof(ll));
+  for (int ee = g->v3_e1 + 1; ee <= g->n3_e1 && lesp && ledn; ee++) {
+    const int e = g->w3_e1[ee];
+    lesp[nl] = e;
+    ledn[nl] = (ll)g->s2k[g->e2c[e][0]] * ne2 + (g->m2de[e] - 1);
+    nl++;
+  }
+  if (!lesp || !ledn) ok = 0;
+  c->nlesp = nl;
+  c->d_lesp = (int *)dalloc(((size_t)nl + 1) * sizeof(int), &ok);
+  c->d_ledn = (ll *)dalloc(((size_t)nl + 1) * sizeof(ll), &ok);
+  if (ok && (up(c, c->d_lesp, lesp, ((size_t)nl + 1) * sizeof(int)) ||
+             up(c, c->d_ledn, ledn, ((size_t)nl + 1) * sizeof(ll))))
+    ok = 0;
+  /* no records until a streamline has several segments */
+  if (ok && hipMemset(D->lnx, 0xff, NE3 * sizeof(int)) != hipSuccess) ok = 0;
+  if (ok && (up(c, D->vls_off, off, ((size_t)d->nwet + 1) * sizeof(int)) ||
+             up(c, D->vls_ent, ent, ((size_t)m + 1) * sizeof(ll)) ||
+             up(c, D->vls_bx, bx, ((size_t)m + 1) * sizeof(double)) ||
+             up(c, D->vls_by, by, ((size_t)m + 1) * sizeof(double)) ||
+             hipStreamSynchronize(c->st) != hipSuccess))
+    ok = 0;
+  free(off);
+  free(ent);
+  free(bx);
+  free(by);
+  free(lesp);
+  free(ledn);
+  return ok ? 0 : emstr_set_error(E_DEVICE, -1, "FFSL streamline velocity fit tables");
 }
 
 /* Builds and checks the device model of the FFSL maps: every wet centre's
@@ -1058,6 +1155,7 @@ static int us_ffsl_geom(usctx *c) {
   free(vsl);
   free(hvxy);
   if (!ok) return E_UNSUP;
+  if ((n = us_ffsl_vlsq(c))) return n;
   c->ffgeom = 1;
   return 0;
 }
@@ -1087,19 +1185,25 @@ static int us_ffsl_fields(usctx *c) {
     D->wbot = (double *)dalloc((size_t)c->ncol * sizeof(double), &ok);
     if (!ok) return emstr_set_error(E_DEVICE, -1, "FFSL field allocation failed");
   }
-  if (up_edges(c, D->u2, wd->u2) || up_cols(c, D->wtop, wd->wtop) ||
+  if (up_edges(c, D->u2, wd->u2) || up_land_edges(c, D->u2, wd->u2) ||
+      up_land_edges(c, D->u1, wd->u1) || up_cols(c, D->wtop, wd->wtop) ||
       up_cols(c, D->wbot, wd->wbot))
     return E_DEVICE;
   c->ghost_edges = 0;
-  if (!(c->wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER))) return 0;
+  /* the centred velocities of the wet centres: the streamline velocity
+   * fit, and the sub-step's Courant numbers */
   if (!D->uc) {
     double **a[] = {&D->uc, &D->vc};
     for (int n = 0; n < 2; n++) *a[n] = (double *)dalloc(N3 * sizeof(double), &ok);
+    if (!ok) return emstr_set_error(E_DEVICE, -1, "FFSL field allocation failed");
+  }
+  if (up_cells(c, D->uc, wd->u) || up_cells(c, D->vc, wd->v)) return E_DEVICE;
+  if (!(c->wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER))) return 0;
+  if (!D->gu0) {
     double **b[] = {&D->gu0, &D->gv0, &D->gue, &D->gve};
     for (int n = 0; n < 4; n++) *b[n] = (double *)dalloc(NE3 * sizeof(double), &ok);
     if (!ok) return emstr_set_error(E_DEVICE, -1, "FFSL field allocation failed");
   }
-  if (up_cells(c, D->uc, wd->u) || up_cells(c, D->vc, wd->v)) return E_DEVICE;
   double *h = (double *)calloc(NE3 * 4, sizeof(double));
   for (int ee = g->v3_e1 + 1; ee <= g->n3_e1; ee++) {
     const int e = g->w3_e1[ee], cg = g->e2c[e][1];
@@ -1249,6 +1353,13 @@ static int us_adv(usctx *c) {
     const int init = us_ffsl_init(c);
     if (emstr_us_launch_ffsl_step(D, init, c->d_nw, c->st))
       return emstr_set_error(E_DEVICE, -1, "FFSL step launch");
+    /* delaunay_reinit of u and v (transport.c:2709-2711): the velocity
+     * fit of the streamlines, kept until ffsl_init runs again */
+    if (init) {
+      if (emstr_us_launch_ffsl_vfit(D, c->st))
+        return emstr_set_error(E_DEVICE, -1, "FFSL velocity fit launch");
+      D->vfit_ok = 1;
+    }
   }
   if (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)) {
     unsigned long long init, r;
@@ -1398,9 +1509,15 @@ static int us_sync(usctx *c) {
     if (err) {
       hipMemset(c->d_fferr, 0, sizeof(int));
       if (err == 1)
-        return emstr_set_error(E_UNSUP, -1, "FFSL: a streamline leaves its upstream cell into "
-                                            "a wet cell (multi-segment streamlines need "
-                                            "hd_trans_interp, not supported)");
+        return emstr_set_error(E_UNSUP, -1, "FFSL: a streamline goes on into a wet cell "
+                                            "before ffsl_init has run on this window (its "
+                                            "velocity fit would be the model's initial one)");
+      if (err == 5)
+        return emstr_set_error(E_UNSUP, -1, "FFSL: a streamline of more than 100 segments "
+                                            "(the reference stops the model)");
+      if (err == 6)
+        return emstr_set_error(E_DEVICE, -1, "FFSL: more streamline segments than the %d "
+                                             "records of a sub-step", c->D.sg_cap);
       if (err == 3)
         return emstr_set_error(E_UNSUP, -1, "FFSL with CLIP: a column with water is not "
                                             "processed (its cells would be clamped to bounds "
@@ -1595,16 +1712,28 @@ static int us_ffsl_maps(usctx *c, int *cl, int *clzf, int *clzc) {
   int *hl = (int *)malloc(NE3 * sizeof(int));
   int *hf = (int *)malloc(N3 * sizeof(int));
   int *hc = (int *)malloc(N3 * sizeof(int));
+  const int cap = c->D.sg_cap;
+  int *hn = (int *)malloc(NE3 * sizeof(int));
+  int *hsc = (int *)malloc((size_t)(cap > 0 ? cap : 1) * sizeof(int));
+  int *hsn = (int *)malloc((size_t)(cap > 0 ? cap : 1) * sizeof(int));
   int r = hipMemcpy(hl, c->d_lci, NE3 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
           hipMemcpy(hf, c->d_clf, N3 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
-          hipMemcpy(hc, c->d_clc, N3 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess;
+          hipMemcpy(hc, c->d_clc, N3 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(hn, c->D.lnx, NE3 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(hsc, c->D.sg_ci, (size_t)cap * sizeof(int), hipMemcpyDeviceToHost) !=
+              hipSuccess ||
+          hipMemcpy(hsn, c->D.sg_nx, (size_t)cap * sizeof(int), hipMemcpyDeviceToHost) !=
+              hipSuccess;
   if (!r) {
 #define SPL(col, L) ((L) < c->h_kb[col] ? c->sedg[col] : c->sp_of[(ll)(L) * ncol + (col)])
-    /* upstream centre of every wet edge */
+    /* the centre where every wet edge's streamline ends (its upstream
+     * centre unless it goes on into other cells) */
     for (q = 0; q < c->nesp; q++) {
       const ll ie = c->h_edn[q];
       const int k = (int)(ie / ne2);
-      cl[c->h_esp[q]] = c->sp_of[(ll)k * ncol + hl[ie]];
+      int col = hl[ie];
+      for (int x = hn[ie]; x >= 0 && x < cap; x = hsn[x]) col = hsc[x];
+      cl[c->h_esp[q]] = c->sp_of[(ll)k * ncol + col];
     }
     /* departure centres of the faces to process and of every wet centre */
     for (q = 0; q < d->nwet; q++) {
@@ -1619,6 +1748,9 @@ static int us_ffsl_maps(usctx *c, int *cl, int *clzf, int *clzc) {
   free(hl);
   free(hf);
   free(hc);
+  free(hn);
+  free(hsc);
+  free(hsn);
   return r ? emstr_set_error(E_DEVICE, -1, "FFSL map download") : 0;
 }
 

@@ -59,6 +59,7 @@ int emstr_us_launch_ffsl_bounds(const emstr_usdev *D, const double *tin, const d
  * gets 1 for a streamline that leaves its cell into a wet cell, 2 for a
  * vertical walk that does not end.                                     */
 int emstr_us_launch_ffsl_step(const emstr_usdev *D, int init, double *nw, void *stream);
+int emstr_us_launch_ffsl_vfit(const emstr_usdev *D, void *stream);
 int emstr_us_launch_ffsl_sub(const emstr_usdev *D, double dtu, const double *nw, int *clc,
                              double *crc, int *clf, double *crf, double *cour,
                              const long long *edn, int nesp, int *lci, double *ld, int *err,

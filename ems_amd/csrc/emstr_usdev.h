@@ -76,6 +76,23 @@ typedef struct emstr_usdev {
    * centres; [nz][ne2] u, v of the ghost across a land edge (gu0/gv0
    * from the window's arrays, gue/gve rotated from the edge velocity) */
   double *u2, *wtop, *wbot, *uc, *vc, *gu0, *gv0, *gue, *gve;
+  /* multi-segment streamlines (hd_trans_interp): the velocity fit of
+   * wet centre q reads points vls_off[q]..vls_off[q+1]-1, vls_ent >= 0
+   * the dense index of a wet centre, else ~(k * ne2 + e) for the ghost
+   * across surface edge e at layer k (the edge velocity rotated), with
+   * slope rows vls_bx, vls_by; vfit [6][nz][ncol]: value, x slope and y
+   * slope of u, then of v, rebuilt when ffsl_init runs (vfit_ok once it
+   * has).  Segments after a streamline's first: lnx[edge] the first
+   * record (-1: none), record r in column sg_ci[r] with the offsets and
+   * length sg_d[r * 5 + 0..4] (as ld) and the next record sg_nx[r];
+   * sg_cnt counts the records of the sub-step, at most sg_cap.        */
+  int *vls_off;
+  long long *vls_ent;
+  double *vls_bx, *vls_by, *vfit;
+  int vfit_ok;
+  int *lnx, *sg_ci, *sg_nx, *sg_cnt;
+  double *sg_d;
+  int sg_cap;
 } emstr_usdev;
 
 #ifdef __cplusplus

@@ -44,6 +44,10 @@ def lib():
         L.emstr_mesh_vertices.restype = C.c_int
         L.emstr_mesh_lsq.argtypes = [C.POINTER(U.Mesh)]
         L.emstr_mesh_lsq.restype = C.c_int
+        L.emstr_us_vlsq_build.argtypes = [C.POINTER(U.Geometry), C.POINTER(U.VLsq)]
+        L.emstr_us_vlsq_build.restype = C.c_int
+        L.emstr_us_vlsq_free.argtypes = [C.POINTER(U.VLsq)]
+        L.emstr_us_vlsq_free.restype = None
         G, W, P = C.POINTER(U.Geometry), C.POINTER(U.Window), C.POINTER(U.WinPriv)
         for fn, rt in (("tracer_step_3d", None), ("advect_diffuse", C.c_int),
                        ("vert_diffuse_3d", None), ("set_dz", None)):
@@ -476,7 +480,20 @@ class MeshCase:
         set_dz_fn(self.gp, self.wp, self.pp)
         self.w_from_continuity()
 
+    def vlsq(self):
+        """The streamline velocity fit's weights of this window, built by
+        the library from the mesh (emstr_us_vlsq_build), kept on the case."""
+        if getattr(self, "_vlsq", None) is None:
+            t = U.VLsq()
+            if lib().emstr_us_vlsq_build(self.gp, C.byref(t)) != 0:
+                raise ValueError("emstr_us_vlsq_build failed")
+            self._vlsq = t
+        return self._vlsq
+
     def free(self):
+        if getattr(self, "_vlsq", None) is not None:
+            lib().emstr_us_vlsq_free(C.byref(self._vlsq))
+            self._vlsq = None
         if getattr(self, "M", None):
             lib().emstr_mesh_free(self.M)
             self.M = None

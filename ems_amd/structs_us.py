@@ -112,3 +112,11 @@ def view(ptr, n, dtype):
 def view2(pptr, rows, cols):
     """numpy view of an int** allocated as one block of rows x cols."""
     return np.ctypeslib.as_array(C.cast(pptr[0], C.POINTER(C.c_int)), shape=(rows, cols))
+
+
+class VLsq(C.Structure):
+    """emstr_us_vlsq: the streamline velocity fit's weights of a window."""
+    _fields_ = [("szc", C.c_int), ("nent", C.c_int),
+                ("vo", C.POINTER(C.c_int)), ("vn", C.POINTER(C.c_int)),
+                ("cell", C.POINTER(C.c_int)), ("edge", C.POINTER(C.c_int)),
+                ("bx", C.POINTER(C.c_double)), ("by", C.POINTER(C.c_double))]

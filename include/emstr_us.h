@@ -216,6 +216,30 @@ struct master {
 };
 
 /*-------------------------------------------------------------------*/
+/* Streamline velocity fit of an FFSL window (no counterpart struct in */
+/* the reference, which keeps these weights in its interpolation       */
+/* library's structures: tran_grid_init, model/hd-us/tracers/          */
+/* transport.c:4317-4658, set_interp_points_n :5537-5640, lsql_build   */
+/* lib/interp/lsq_lin/lsql.c:47-210).  Wet centre c's fit reads the    */
+/* points cell[vo[c] .. vo[c] + vn[c] - 1]: wet centres, then lateral  */
+/* ghosts (edge[]: the ghost's edge, whose midpoint is the point; 0    */
+/* for a wet centre), with the slope rows bx, by of the pseudo-inverse */
+/* of the rows (1, x - xr, y - yr).  A fitted value at (x, y) is        */
+/* v[c] + (sum v_j bx_j (x - xr) + sum v_j by_j (y - yr)), (xr, yr) the */
+/* centre's position.  Built by the library for FFSL windows; exported */
+/* so that a checker can use the same weights.                         */
+/*-------------------------------------------------------------------*/
+typedef struct emstr_us_vlsq {
+  int szc, nent;
+  int *vo, *vn;     /* [szc] first entry and count of a wet centre      */
+  int *cell, *edge; /* [nent]                                           */
+  double *bx, *by;  /* [nent]                                           */
+} emstr_us_vlsq;
+/* 0 on success, -1 without vertex maps or memory */
+int emstr_us_vlsq_build(const geometry_t *window, emstr_us_vlsq *t);
+void emstr_us_vlsq_free(emstr_us_vlsq *t);
+
+/*-------------------------------------------------------------------*/
 /* Drop-in entry points (COMPAS signatures)                          */
 /*-------------------------------------------------------------------*/
 /* model/hd-us/tracers/tracers.c:177-239: every window, 1-based arrays in

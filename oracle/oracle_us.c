@@ -639,6 +639,112 @@ static void ff_alloc(geometry_t *g) {
   ff_sze = g->sze;
 }
 
+/* The streamline velocity fit (tran_grid_init / delaunay_reinit,
+ * transport.c:4317-4658, 4728-4929, with lsql_rebuild, lib/interp/
+ * lsq_lin/lsql.c:262-287): the weights are the window's (the table the
+ * library builds from the mesh, handed over by oracle_us_set_vlsq); the
+ * fit of every wet centre is rebuilt from the centred velocities when
+ * ffsl_init runs and kept until it runs again.  ff_fit[6][szc]: value,
+ * x slope, y slope of u, then of v.  set_dzz (transport.c:6149-6200)
+ * runs there too, into ff_dzz / ff_cellz. */
+static const emstr_us_vlsq *ff_vlsq = NULL;
+static double *ff_fit = NULL, *ff_dzz = NULL, *ff_cellz = NULL;
+static const geometry_t *ff_fit_g = NULL; /* window the fit belongs to */
+static int ff_fit_szc = 0;
+
+void oracle_us_set_vlsq(const emstr_us_vlsq *t) { ff_vlsq = t; }
+/* a window seen for the first time has no fit (a new context's state) */
+void oracle_us_forget_fit(void) { ff_fit_g = NULL; }
+
+/* transport.c:6149-6187  set_dzz (no OBCs).  The reference indexes the
+ * 2D oldeta with the 3D surface cell (transport.c:6176), which is its
+ * 2D index only in the top layer; the column's 2D index is used here.
+ * Only the cellz of a centre minus itself and dzz of wet centres reach
+ * the streamlines (hd_trans_interp), so the value does not matter. */
+static void ff_set_dzz(geometry_t *g, win_priv_t *wc) {
+  int cc;
+  memcpy(ff_dzz, wc->dz, g->szc * sizeof(double));
+  memcpy(ff_cellz, g->cellz, g->szc * sizeof(double));
+  for (cc = 1; cc <= wc->vc; cc++) {
+    int c = wc->s1[cc], c2 = g->m2d[c], zm1 = g->zm1[c];
+    ff_dzz[zm1] = 0.5 * (wc->dz[c] + wc->dz[zm1]);
+    ff_cellz[zm1] = 0.5 * (g->gridz[c] + max(g->botz[c2], g->gridz[zm1]));
+  }
+  for (cc = 1; cc <= wc->vcs; cc++) {
+    int c = wc->s1[cc], c2 = c, zm1 = g->zm1[c], cb = wc->i1[cc];
+    ff_dzz[c] = wc->dz[c] + 0.5 * wc->dz[zm1];
+    ff_cellz[c] = 0.5 * (wc->oldeta[g->m2d[c]] + g->gridz[c]);
+    while (c2 != g->zp1[c2]) {
+      c2 = g->zp1[c2];
+      ff_dzz[c2] = ff_dzz[c];
+      ff_cellz[c2] = ff_cellz[c];
+    }
+    ff_dzz[g->zm1[cb]] = wc->dz[cb];
+    ff_cellz[g->zm1[cb]] = g->botz[c2];
+  }
+}
+
+/* delaunay_reinit of u (vid 0) and v (vid 1), lsql_rebuild: the slopes
+ * sum v_j B[j][1..2] in the stencil's order from 0, the value the
+ * centre's own */
+static void ff_fit_rebuild(geometry_t *g, win_priv_t *wc, const double *u, const double *v) {
+  int cc, o, q;
+  if (ff_fit_szc != g->szc) {
+    free(ff_fit);
+    free(ff_dzz);
+    free(ff_cellz);
+    ff_fit = (double *)calloc(6 * (size_t)g->szc, sizeof(double));
+    ff_dzz = (double *)calloc(g->szc, sizeof(double));
+    ff_cellz = (double *)calloc(g->szc, sizeof(double));
+    ff_fit_szc = g->szc;
+  }
+  ff_set_dzz(g, wc);
+  ff_fit_g = NULL;
+  if (!ff_vlsq || ff_vlsq->szc != g->szc) return;
+  for (q = 0; q < 2; q++) {
+    const double *val = q ? v : u;
+    double *f = ff_fit + 3 * (size_t)q * g->szc;
+    for (cc = 1; cc <= g->a3_t; cc++) {
+      int c = g->w3_t[cc];
+      double w1 = 0.0, w2 = 0.0;
+      for (o = ff_vlsq->vo[c]; o < ff_vlsq->vo[c] + ff_vlsq->vn[c]; o++) {
+        double z0 = val[ff_vlsq->cell[o]];
+        w1 += z0 * ff_vlsq->bx[o];
+        w2 += z0 * ff_vlsq->by[o];
+      }
+      f[c] = val[c];
+      f[g->szc + c] = w1;
+      f[2 * (size_t)g->szc + c] = w2;
+    }
+  }
+  ff_fit_g = g;
+}
+
+/* lsql_interpolate_point (lsql.c:306-330, beta 1) of centre c's fit */
+static double ff_fit_at(geometry_t *g, int q, int c, double x, double y) {
+  const double *f = ff_fit + 3 * (size_t)q * g->szc;
+  int cs = g->m2d[c];
+  double xp = x - g->cellx[cs], yp = y - g->celly[cs];
+  return f[c] + 1.0 * (f[g->szc + c] * xp + f[2 * (size_t)g->szc + c] * yp);
+}
+
+/* transport.c:5171-5296  hd_trans_interp, L_LSLIN, of velocity q (0: u,
+ * 1: v) at (x, y, z) in wet centre c: the fits of the layer above
+ * (v2) and of c's layer (v1) at (x, y), then linear in z */
+static double ff_trans_interp(geometry_t *g, int q, double x, double y, double z, int c) {
+  int zp1 = g->zp1[c];
+  double v1, v2, v, d;
+  v2 = ff_fit_at(g, q, zp1, x, y);
+  /* a NaN v2 takes the point's stored value (transport.c:5237-5239),
+   * finite, which the weight d = 0 below then drops */
+  v1 = ff_fit_at(g, q, c, x, y);
+  if (isnan(v2)) v2 = v1;
+  d = z - ff_cellz[c];
+  v = d * (v2 - v1) / ff_dzz[c] + v1;
+  if (isnan(v)) ous_err = 7;
+  return v;
+}
+
 /* transport.c:5118-5160  set_tr_nograd (no OBCs) */
 static void ff_nograd(geometry_t *g, win_priv_t *wc, double *tr) {
   int cc;
@@ -685,10 +791,10 @@ static void ff_vel_center_w(geometry_t *g, window_t *wd, double *nw) {
 /* transport.c:2607-2723  ffsl_init: runs on every other call (the
  * F_FFSL toggle; particles would take the other call).  The centred
  * velocities of the lateral ghosts are the edge velocity rotated to
- * east / north, written into windat->u, v as the reference does; the
- * layer spacings dzz / cellz and the Delaunay structures only serve
- * hd_trans_interp (multi-segment streamlines, rejected here) and the
- * sediment values of nu / nv / nw are never read on this path.       */
+ * east / north, written into windat->u, v as the reference does; then
+ * set_dzz and the velocity fits of the streamlines (delaunay_reinit of
+ * u and v).  The sediment values of nu / nv / nw and the fits of w and
+ * of the sediment layer are never read on this path.                 */
 static void ff_init(geometry_t *g, window_t *wd, win_priv_t *wc, double *nw) {
   int cc, j;
   if (wc->flag & F_FFSL) {
@@ -705,6 +811,7 @@ static void ff_init(geometry_t *g, window_t *wd, win_priv_t *wc, double *nw) {
     wd->v[c] = wd->u1[e] * g->sinthu1[es] + wd->u2[e] * g->sinthu2[es];
     nw[c] = nw[ci];
   }
+  ff_fit_rebuild(g, wc, wd->u, wd->v);
   wc->flag |= F_FFSL;
 }
 
@@ -977,9 +1084,11 @@ void oracle_us_ffsl_limit(geometry_t *g, win_priv_t *wc, const double *tr, doubl
   ff_linear_limit(g, wc, tr, w, beta);
 }
 
-/* transport.c:1961-2598  ffsl_don.  A streamline that leaves its first
- * cell into a wet cell needs hd_trans_interp (Delaunay interpolation of
- * the centred velocities); that case is reported (error 3).          */
+/* transport.c:1961-2598  ffsl_don.  A streamline that leaves its cell
+ * into a wet cell goes on from the crossing point with the velocity
+ * fitted there (hd_trans_interp); without a fit (no ffsl_init on this
+ * window yet, or no velocity weights handed over) that is reported
+ * (error 3), past 100 segments error 8, a NaN fitted velocity error 7. */
 static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr, double *Fx,
                         double *Fz, double dtu, uscr_t *s) {
   const double tf = 0.55;
@@ -1045,7 +1154,7 @@ static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr,
     int e = g->w3_e1[ee];
     int es = g->m2de[e];
     int c1 = g->e2c[e][0], c2 = g->e2c[e][1];
-    int ei = e, first = 0;
+    int ei = e, first = 0, nseg = 0;
     double time_left = dtu, dist, trs, tre, px, py;
     double u = u1[e] * g->costhu1[es] + u2[e] * g->costhu2[es];
     double v = u1[e] * g->sinthu1[es] + u2[e] * g->sinthu2[es];
@@ -1053,8 +1162,21 @@ static void us_ffsl_don(geometry_t *g, window_t *wd, win_priv_t *wc, double *tr,
     if (clip) smin[e] = smax[e] = tr[cl[ee]];
     while (cl[ee] > 0) {
       int ci = cl[ee];
+      double pz = ff_cellz ? ff_cellz[ci] : 0.0;
+      /* later segments: the centred velocity fitted at the segment's
+       * start (transport.c:2221-2225); none before ffsl_init has run on
+       * this window (the reference's would come from tran_grid_init) */
       if (first) {
-        ous_err = 3;
+        if (ff_fit_g != g) {
+          ous_err = 3;
+          break;
+        }
+        u = ff_trans_interp(g, 0, cx[e], cy[e], pz, ci);
+        v = ff_trans_interp(g, 1, cx[e], cy[e], pz, ci);
+      }
+      /* transport.c:2330-2333: the reference stops the model */
+      if (++nseg > 100) {
+        ous_err = 8;
         break;
       }
       first = 1;

@@ -53,8 +53,23 @@ def lib():
         _PI = C.POINTER(C.c_int)
         L.oracle_us_ffsl_maps.argtypes = [G, P, _PI, _PI, _PI]
         L.oracle_us_ffsl_maps.restype = None
+        L.oracle_us_set_vlsq.argtypes = [C.c_void_p]
+        L.oracle_us_set_vlsq.restype = None
+        L.oracle_us_forget_fit.restype = None
         _lib = L
     return _lib
+
+
+def _vlsq(case):
+    """Hand an FFSL window's streamline velocity weights to the oracle:
+    the table the library builds from the mesh (window geometry, as the
+    least-squares weights of the tracers are)."""
+    from ems_amd import structs_us as U
+    if not getattr(case, "_oracle_seen", False):
+        lib().oracle_us_forget_fit()
+        case._oracle_seen = True
+    if case.wc.trasc == U.FFSL and hasattr(case, "vlsq"):
+        lib().oracle_us_set_vlsq(C.addressof(case.vlsq()))
 
 
 def set_dz(gp, wp, pp):
@@ -62,6 +77,7 @@ def set_dz(gp, wp, pp):
 
 
 def step(case, nthreads=1):
+    _vlsq(case)
     return lib().oracle_us_tracer_step_3d_mt(case.gp, case.wp, case.pp, nthreads)
 
 
@@ -70,6 +86,7 @@ def advect_diffuse(case, nthreads=1, lateral=True):
     filled the lateral ghosts (tracers.c:185-200)."""
     if lateral:
         lib().oracle_us_set_lateral_BC_tr(case.gp, case.wp)
+    _vlsq(case)
     return lib().oracle_us_advect_diffuse_mt(case.gp, case.wp, case.pp, nthreads)
 
 

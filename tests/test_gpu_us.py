@@ -457,12 +457,63 @@ def test_c4_tracer_values_vs_oracle(trasc):
     assert not np.array_equal(gpu, start)
 
 
-def test_ffsl_multi_segment_is_reported():
-    say("case hex ffsl multi-segment")
+MULTI_CASES = [
+    # name, hex_basin kw, FILL_METHOD, steps
+    ("multi", dict(dt=6000.0, vmax=2.0), 0, 3),
+    ("multi_long", dict(dt=12000.0, vmax=1.2), 0, 2),
+    ("multi_clip", dict(dt=6000.0, vmax=2.0), U.CLIP, 3),
+]
+
+
+@pytest.mark.parametrize("name,kw,fill,nsteps", MULTI_CASES)
+def test_hex_ffsl_multi_segment(name, kw, fill, nsteps):
+    """Streamlines that cross into wet cells go on from the crossing with
+    the velocity fitted there (hd_trans_interp), over steps where
+    ffsl_init rebuilds the fit and steps where it keeps it: tracers bit
+    for bit with the oracle; then one more step of each: the cell where
+    every streamline ends is the same, and many end beyond the cells
+    either side of their edge."""
+    say("case hex ffsl", name)
+    case = MS.hex_basin(nx=24, ny=20, nz=6, ntr=3, seed=17, trasc=U.FFSL, **kw)
+    case.wc.fillf = fill
+    case.prepare(OU.set_dz)
+    t0 = case.tracers()
+    gpu, ora = run_both(case, nsteps)
+    assert_exact(gpu, ora)
+    assert np.abs(ora - t0).max() > 1e-3
+    L = MS.lib()
+    snap = H.snapshot(case)
+    # a new context: ffsl_init runs on its first call (it builds the fit)
+    case.wc.flag &= ~U.F_FFSL
+    flag0 = case.wc.flag
+    uv0 = case.ac(case.wd.u).copy(), case.ac(case.wd.v).copy()
+    L.tracer_step_3d(case.gp, case.wp, case.pp)
+    assert MS.last_error()[0] == "", MS.last_error()
+    gcl = MS.ffsl_maps(case)[0]
+    L.emstr_us_forget_window(case.gp)
+    H.restore(case, snap)
+    case.wc.flag = flag0
+    case.ac(case.wd.u)[:], case.ac(case.wd.v)[:] = uv0
+    assert OU.step(case) == 0
+    ocl = OU.ffsl_maps(case)[0]
+    g = case.g
+    wet_e = U.view(g.w3_e1, case.sze, np.int32)[1:g.a3_e1 + 1]
+    assert np.array_equal(gcl[wet_e], ocl[wet_e])
+    beyond = (gcl[wet_e] != case.e2c[wet_e, 0]) & (gcl[wet_e] != case.e2c[wet_e, 1])
+    say("streamlines ending beyond their edge's cells:", int(beyond.sum()), "of", wet_e.size)
+    assert beyond.sum() > 50
+
+
+def test_ffsl_multi_segment_needs_a_velocity_fit():
+    """A streamline that goes on into a wet cell before ffsl_init has run
+    on the window (its toggle already set on the first call) has no
+    velocity fit: reported."""
+    say("case hex ffsl multi-segment, no fit")
     case = MS.hex_basin(nx=12, ny=10, nz=3, ntr=1, seed=2, trasc=U.FFSL, dt=12000.0)
     case.prepare(OU.set_dz)
+    case.wc.flag |= U.F_FFSL
     MS.lib().tracer_step_3d(case.gp, case.wp, case.pp)
-    assert "multi-segment" in MS.last_error()[0]
+    assert "before ffsl_init" in MS.last_error()[0]
     MS.lib().emstr_us_forget_window(case.gp)
 
 

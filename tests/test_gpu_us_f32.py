@@ -97,6 +97,16 @@ def test_hex_ffsl_fp32_clip_vs_plain():
     compare(case, (10, 40))
 
 
+def test_hex_ffsl_fp32_multi_segment_vs_plain():
+    """Streamlines of several segments (dt = 6000 s, up to 2 m/s: many
+    cross two to four cells) in the FP32 form against the plain form: 20
+    steps."""
+    say("case hex ffsl fp32 multi-segment")
+    case = MS.hex_basin(nx=24, ny=20, nz=8, ntr=3, seed=17, trasc=U.FFSL, dt=6000.0, vmax=2.0)
+    case.prepare(OU.set_dz)
+    compare(case, (5, 20))
+
+
 def test_fp32_form_refuses_other_schemes():
     case = MS.hex_basin(nx=10, ny=8, nz=4, ntr=1, seed=3, trasc=U.VANLEER)
     L = MS.lib()

@@ -11,6 +11,10 @@ What fixes each part, independently of the formulas restated there:
   vertical maps and integer-part integration
       a linear profile under uniform w moves exactly (z - w dt) at
       Courant numbers below and above one
+  multi-segment streamlines (hd_trans_interp)
+      the velocity fit's points are the polygon neighbourhood of the
+      centre and its weights reproduce linear fields; a linear field
+      carried two to three cells per step moves exactly
   get_posc / intersect
       end points against polygon geometry computed here: the full
       back-trajectory when it stays in the cell, the crossing with the
@@ -255,11 +259,93 @@ def test_ffsl_init_every_other_call():
     assert flags == [U.F_FFSL, 0, U.F_FFSL]
 
 
-def test_multi_segment_streamline_is_reported():
+# ---- multi-segment streamlines (hd_trans_interp, transport.c:5171-5296) ----
+
+def _vlsq_entries(case, c):
+    t = case.vlsq()
+    o, n = t.vo[c], t.vn[c]
+    return ([t.cell[i] for i in range(o, o + n)], [t.edge[i] for i in range(o, o + n)],
+            np.array([t.bx[i] for i in range(o, o + n)]),
+            np.array([t.by[i] for i in range(o, o + n)]))
+
+
+def test_velocity_fit_weights():
+    """The streamline velocity fit of every wet centre (the library's
+    table, built from the mesh): its wet points are the centre and the
+    wet centres of its layer whose polygons share a corner with it (from
+    the mesh polygons), its ghost points the ghosts across their land
+    edges, placed at the edge midpoints; the slope rows give back the
+    slopes of any linear field sampled at those points."""
+    case = MS.hex_basin(nx=16, ny=14, nz=4, ntr=1, seed=3, trasc=U.FFSL)
+    g, mesh = case.g, case.mesh
+    m2d = U.view(g.m2d, case.szc, np.int32)
+    m2de = U.view(g.m2de, case.sze, np.int32)
+    wgst = U.view(g.wgst, case.szc, np.int32)
+    cxs, cys = case.a2(g.cellx), case.a2(g.celly)
+    ux, uy = case.ae2(g.u1x), case.ae2(g.u1y)
+    corners = [set(mesh.vid[m, :mesh.npe[m]].tolist()) for m in range(mesh.nmc)]
+    rng = np.random.default_rng(4)
+    nghost = 0
+    for c in case.wet:
+        m, k = case.s2i[c], case.s2k[c]
+        cells, edges, bx, by = _vlsq_entries(case, c)
+        wet = [x for x in cells if not wgst[x]]
+        ghosts = [(x, e) for x, e in zip(cells, edges) if wgst[x]]
+        assert wet[0] == c and len(wet) == len(set(wet))
+        want = {case.idx3[k, m2] for m2 in range(mesh.nmc)
+                if k >= case.kbot[m2] and corners[m2] & corners[m]}
+        assert set(wet) == want, c
+        for x, e in ghosts:
+            a, b = case.e2c[e]
+            assert {a, b} == {x, wgst[x]} and wgst[x] in want
+        nghost += len(ghosts)
+        px = np.array([cxs[m2d[x]] if not wgst[x] else ux[m2de[e]]
+                       for x, e in zip(cells, edges)]) - cxs[m2d[c]]
+        py = np.array([cys[m2d[x]] if not wgst[x] else uy[m2de[e]]
+                       for x, e in zip(cells, edges)]) - cys[m2d[c]]
+        sx, sy = rng.normal(size=2) * 1e-3
+        f = 7.0 + sx * px + sy * py
+        assert abs(f @ bx - sx) < 1e-12 * 7.0 / 1000.0 + 1e-15
+        assert abs(f @ by - sy) < 1e-12 * 7.0 / 1000.0 + 1e-15
+    assert nghost > 100
+
+
+@pytest.mark.parametrize("U0,V0", [(0.9, 0.45), (-0.7, 0.6), (0.15, -1.0)])
+def test_multi_segment_linear_field_translates_exactly(U0, V0):
+    """A uniform flow crossing two to three cells per step: every
+    streamline leaves its first cell and goes on with the fitted
+    velocity (uniform, so the path stays straight); the mean of a linear
+    field over the segments, each from its own cell's reconstruction, is
+    the midpoint value, so the field moves exactly (x - U dt) wherever
+    the stencils see no coast."""
+    dt = 6000.0
+    case = _flat(nx=34, ny=30)
+    case.set_uniform_velocity(U0, V0)
+    case.scheme(U.FFSL, 0, dt)
+    cx, cy = _centres(case)
+    a, bx, by = 30.0, 2e-4, -3e-4
+    t0 = a + bx * cx + by * cy
+    case.set_tracers(t0[None, :])
+    OU.set_dz(case.gp, case.wp, case.pp)
+    assert OU.advect_diffuse(case) == 0
+    got = case.tracers()[0]
+    exp = a + bx * (cx - U0 * dt) + by * (cy - V0 * dt)
+    inner = _interior(cx, cy, 12000.0 + np.hypot(U0, V0) * dt)
+    assert inner.sum() > 40
+    assert np.abs(got - exp)[inner].max() < 1e-10
+    # the streamlines there really are multi-segment: two cells or more
+    assert np.hypot(U0, V0) * dt > 1.6 * 2 * 1500.0 * np.sqrt(3) / 2
+
+
+def test_multi_segment_needs_a_velocity_fit():
+    """Without a fit (ffsl_init has not run on this window: its flag is
+    already set on the first call) a multi-segment streamline is
+    reported."""
     case = _flat(nx=10, ny=9)
     case.set_uniform_velocity(1.0, 0.5)
     case.scheme(U.FFSL, 0, 3000.0)
     case.set_tracers(np.ones((1, case.nwet)))
+    case.wc.flag |= U.F_FFSL
     OU.set_dz(case.gp, case.wp, case.pp)
     assert OU.advect_diffuse(case) == -3
 
