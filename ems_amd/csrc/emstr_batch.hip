@@ -71,12 +71,9 @@
 #define IR_DMAX 1.0
 #define NZMAX 64
 #define SWAVES 4 /* rows of a tile: the four 16-lane rows of a wave */
-/* tracers per lane: a tracer group's values of a cell are TPL consecutive
- * floats (heads) and bytes (tails), one wide load each */
-#ifndef EMSTR_TPL
-#define EMSTR_TPL 4
-#endif
-constexpr int TPL = EMSTR_TPL;
+/* tracers per lane (1, 2 or 4, per context): a tracer group's values of
+ * a cell are that many consecutive floats (heads) and bytes (tails), one
+ * wide load each */
 
 typedef long long ll;
 
@@ -101,10 +98,11 @@ struct Col2 {
 
 struct emstr_batch {
   int ntr, nz, nx, ny, nxp, nyp;
+  int tpl;         /* tracers per lane (1, 2, 4)                         */
   ll P2;           /* nxp * nyp                                          */
   ll vol;          /* nz * P2: one tracer                                */
   float *tr[2];    /* [ntr][nz][nyp][nxp], the 24-bit heads              */
-  uint8_t *rm[2];  /* [ntr/4][nz][nyp][nxp][4], the 8-bit tails          */
+  uint8_t *rm[2];  /* [ntr/tpl][nz][nyp][nxp][tpl], the 8-bit tails      */
   int cur;
   float4 *co;      /* [nz][nyp][nxp][3]                                  */
   Col2 *ci;        /* [nyp][nxp]                                         */
@@ -436,10 +434,12 @@ __device__ __forceinline__ void blt(float (&v)[T], __amdgpu_buffer_rsrc_t r, uin
     const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, o, so, 0);
 #pragma unroll
     for (int u = 0; u < 4; u++) v[u] = __uint_as_float(x[u]);
-  } else {
+  } else if constexpr (T == 2) {
     const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
 #pragma unroll
     for (int u = 0; u < 2; u++) v[u] = __uint_as_float(x[u]);
+  } else {
+    v[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0));
   }
 }
 /* the T tail bytes of one cell of a tracer group, as one word */
@@ -447,20 +447,26 @@ template <int T>
 __device__ __forceinline__ uint32_t blr(__amdgpu_buffer_rsrc_t r, uint32_t o, uint32_t so) {
   if constexpr (T == 4)
     return __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
-  else
+  else if constexpr (T == 2)
     return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, o, so, 0);
+  else
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, o, so, 0);
 }
 template <int T> __device__ __forceinline__ void stt(float *p, const float (&v)[T]) {
   if constexpr (T == 4)
     *(float4 *)p = make_float4(v[0], v[1], v[2], v[3]);
-  else
+  else if constexpr (T == 2)
     *(float2 *)p = make_float2(v[0], v[1]);
+  else
+    *p = v[0];
 }
 template <int T> __device__ __forceinline__ void str(uint8_t *p, uint32_t w) {
   if constexpr (T == 4)
     *(uint32_t *)p = w;
-  else
+  else if constexpr (T == 2)
     *(uint16_t *)p = (uint16_t)w;
+  else
+    *p = (uint8_t)w;
 }
 __device__ __forceinline__ uint32_t byteof(uint32_t w, int u) { return (w >> (8 * u)) & 0xFFu; }
 
@@ -1045,7 +1051,7 @@ k_step4(const SArgs a, const float *__restrict__ tin, float *__restrict__ tout,
 /*-------------------------------------------------------------------*/
 struct LArgs {
   ll plane, P2, vol;
-  int nx, nxp, nwet;
+  int nx, nxp, nwet, tpl;
 };
 
 __device__ __forceinline__ ll bidx(const LArgs &a, ll idx) {
@@ -1056,9 +1062,9 @@ __device__ __forceinline__ ll bidx(const LArgs &a, ll idx) {
 }
 
 /* element offset of tracer n at cell index c of its volume, heads (in
- * floats) and tails (in bytes) alike: [n / TPL][vol][TPL] */
+ * floats) and tails (in bytes) alike: [n / tpl][vol][tpl] */
 __device__ __forceinline__ ll toff(const LArgs &a, int n, ll c) {
-  return ((ll)(n / TPL) * a.vol + c) * TPL + (n % TPL);
+  return ((ll)(n / a.tpl) * a.vol + c) * a.tpl + (n % a.tpl);
 }
 
 /* FP64 -> head and tail, rounded to nearest at 32 bits, into both buffers */
@@ -1260,6 +1266,17 @@ static void drop_graph(emstr_batch *b) {
   b->graph_ok = 0;
 }
 
+/* tracers per lane of a new context: four for 32 tracers or more, two
+ * below (measured on one MI355X: the 400 x 600 x 40 window with 40
+ * tracers takes 1.19 / 1.68 / 2.59 ms a step with four / two / one, the
+ * 64 windows of 5-20 tracers 3.48 / 3.25 / 3.42 ms).  EMSTR_BATCH_TPL
+ * (1, 2 or 4) overrides it, for such measurements. */
+static int batch_tpl(const emstr_dev *d) {
+  const char *e = getenv("EMSTR_BATCH_TPL");
+  if (e && (atoi(e) == 1 || atoi(e) == 2 || atoi(e) == 4)) return atoi(e);
+  return (d->ntr >= 32) ? 4 : 2;
+}
+
 extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int *h_kb,
                                                   void *stream) {
   (void)stream;
@@ -1280,7 +1297,8 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   b->nyp = (d->ny + 31) / 32 * 32; /* tiles up to 4 x 8 rows */
   b->P2 = (ll)b->nxp * b->nyp;
   b->vol = (ll)d->nz * b->P2;
-  const ll ntp = (ll)(d->ntr + TPL - 1) / TPL * TPL; /* tracers with the group padding */
+  b->tpl = batch_tpl(d);
+  const ll ntp = (ll)(d->ntr + b->tpl - 1) / b->tpl * b->tpl; /* tracers with the group padding */
   if (b->vol * 48 > 0x7fffffffLL || b->vol * 4 * ntp > 0xffffffffLL) {
     /* buffer offsets of the coefficients and tracers are 32-bit */
     emstr_set_error(EMSTR_EUNSUPPORTED, -1,
@@ -1297,7 +1315,7 @@ extern "C" struct emstr_batch *emstr_batch_create(const emstr_dev *d, const int 
   const size_t nrm = (size_t)b->vol * ntp;
   /* slack for the neighbour reads of the last layer past the end: two
    * rows of tracers, one row of coefficient words */
-  const size_t trb = nt * sizeof(float) + (size_t)b->nxp * 8 * TPL + 256;
+  const size_t trb = nt * sizeof(float) + (size_t)b->nxp * 8 * b->tpl + 256;
   const size_t cob = (size_t)b->vol * 3 * sizeof(float4) + (size_t)b->nxp * 48 + 256;
   for (int q = 0; q < 2; q++) ok &= hipMalloc(&b->tr[q], trb) == hipSuccess;
   for (int q = 0; q < 2; q++) ok &= hipMalloc(&b->rm[q], nrm) == hipSuccess;
@@ -1387,11 +1405,19 @@ extern "C" int emstr_batch_fields(struct emstr_batch *b, const emstr_dev *d, voi
 static void launch_step(emstr_batch *b, const SArgs &sa, const float *in, float *out,
                         const uint8_t *rin, uint8_t *rout, const emstr_dev *d,
                         hipStream_t st) {
-  const int ng = (b->ntr + TPL - 1) / TPL;
-  const size_t lds = (size_t)(b->nz - sa.kl) * 64 * TPL * sizeof(float);
-  k_step4<TPL><<<(unsigned)(8 * b->bmax * ng), 64, lds, st>>>(
-      sa, in, out, rin, rout, b->co, b->ci, b->blk, b->mintr, b->maxtr, d->layers, d->Kz, d->dz,
-      b->scr, b->scr2);
+  const int T = b->tpl, ng = (b->ntr + T - 1) / T;
+  const size_t lds = (size_t)(b->nz - sa.kl) * 64 * T * sizeof(float);
+  const unsigned nb = (unsigned)(8 * b->bmax * ng);
+#define EMSTR_STEP(TT)                                                                       \
+  k_step4<TT><<<nb, 64, lds, st>>>(sa, in, out, rin, rout, b->co, b->ci, b->blk, b->mintr,  \
+                                   b->maxtr, d->layers, d->Kz, d->dz, b->scr, b->scr2)
+  if (T == 4)
+    EMSTR_STEP(4);
+  else if (T == 2)
+    EMSTR_STEP(2);
+  else
+    EMSTR_STEP(1);
+#undef EMSTR_STEP
 }
 
 /* the per-step work: prep + fused step (captured in a graph) */
@@ -1418,7 +1444,7 @@ static int batch_launch(emstr_batch *b, const emstr_dev *d, hipStream_t st) {
   sa.hmin = d->hmin;
   sa.dt = d->dt;
   sa.P = d->plane;
-  sa.kl = max(0, b->nz - LDS_WAVE / (64 * TPL * (int)sizeof(float)));
+  sa.kl = max(0, b->nz - LDS_WAVE / (64 * b->tpl * (int)sizeof(float)));
   if (b->bmax > 0) {
     pi = prof_begin(b->prof, KC_FUSED, st);
     launch_step(b, sa, in, out, b->rm[b->cur], b->rm[1 - b->cur], d, st);
@@ -1484,6 +1510,7 @@ static LArgs largs(const emstr_batch *b, const emstr_dev *d) {
   a.nx = d->nx;
   a.nxp = b->nxp;
   a.nwet = d->nwet;
+  a.tpl = b->tpl;
   return a;
 }
 

@@ -79,6 +79,8 @@ typedef struct emstr_us_ctx {
   ll *h_edn;     /* wet edges: dense index (host copy of d_edn)          */
   /* FFSL work arrays, made on first use (see emstr_us_kernels.h)      */
   int ffgeom;        /* surface geometry, vertices and stencils uploaded */
+  unsigned fgen, vfit_gen; /* field uploads so far; the upload the
+                              streamline velocity fit was built from     */
   int ghost_edges;   /* ghost velocities are the rotated edge velocity   */
   double *d_nw, *d_crc, *d_crf, *d_cour, *d_ld, *d_tm, *d_lq, *d_te, *d_moc;
   int *d_clc, *d_clf, *d_lci, *d_fferr;
@@ -700,6 +702,7 @@ static int us_upload_fields(usctx *c) {
   if ((r = cols_from_host(c))) return r;
   D->ffsl = (wc->trasc == FFSL);
   if (D->ffsl && (r = us_ffsl_fields(c))) return r;
+  c->fgen++;
   HCHK(hipStreamSynchronize(c->st));
   return 0;
 }
@@ -1354,11 +1357,14 @@ static int us_adv(usctx *c) {
     if (emstr_us_launch_ffsl_step(D, init, c->d_nw, c->st))
       return emstr_set_error(E_DEVICE, -1, "FFSL step launch");
     /* delaunay_reinit of u and v (transport.c:2709-2711): the velocity
-     * fit of the streamlines, kept until ffsl_init runs again */
-    if (init) {
+     * fit of the streamlines, kept until ffsl_init runs again; the fit
+     * reads only fields of the upload, so with no upload since the last
+     * one it would come out the same */
+    if (init && !(D->vfit_ok && c->vfit_gen == c->fgen)) {
       if (emstr_us_launch_ffsl_vfit(D, c->st))
         return emstr_set_error(E_DEVICE, -1, "FFSL velocity fit launch");
       D->vfit_ok = 1;
+      c->vfit_gen = c->fgen;
     }
   }
   if (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)) {
