@@ -64,6 +64,7 @@ PROTOS = [
     ("emstr_grid_build", C.POINTER(S.Grid),
      [C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_double)] * 9 + [C.c_int]),
     ("emstr_grid_free", None, [C.POINTER(S.Grid)]),
+    ("emstr_grid_open", C.c_int, [C.POINTER(S.Grid), C.c_int]),
     ("emstr_vel_w_update", None, [_G, _W, _P]),
 ]
 

@@ -75,10 +75,21 @@ class Case:
         self.s2i = S.view(g.s2i, n, np.int32)
         self.s2j = S.view(g.s2j, n, np.int32)
         self.s2k = S.view(g.s2k, n, np.int32)
-        self.nwet = g.v3_t
-        self.wet = S.view(g.w3_t, n, np.int32)[1:g.v3_t + 1].copy()
+        self.nwet = g.b3_t
+        self.wet = S.view(g.w3_t, n, np.int32)[1:g.b3_t + 1].copy()
         self.dt = 1.0
         self.steps = 1
+
+    def open_boundary(self, side):
+        """Make the wet columns on one side of the grid an open boundary
+        (emstr_grid_open); returns its open_bdrys_t.  The wet-cell order
+        changes (boundary cells last): call before prepare()."""
+        if _lib.lib().emstr_grid_open(self.G, side) != 0:
+            raise ValueError("emstr_grid_open failed")
+        g = self.g
+        self.nwet = g.b3_t
+        self.wet = S.view(g.w3_t, self.sgsiz, np.int32)[1:g.b3_t + 1].copy()
+        return g.open[g.nobc - 1].contents
 
     # -- array views -------------------------------------------------
     def a3(self, ptr):

@@ -35,6 +35,8 @@ typedef struct emstr_dev {
   double *h1au1, *h2au2, *h1acell, *h2acell, *h2au1, *h1au2;
   double *cellarea, *botz, *eta, *oldeta;
   double *osubeta, *subeta; /* per sub-step surface elevations     */
+  int *obcm;    /* [ny][nx] 1 for an open-boundary column (not advected,
+                   tracers.c:811-822); NULL: none                     */
   double *layers;           /* [nz+1] layer faces                  */
 
   /* 3D fields [nz][ny][nx] */

@@ -488,11 +488,156 @@ emstr_grid *emstr_grid_build(int nx, int ny, int nz, const double *layers,
 
 void emstr_forget_window(geometry_t *g);
 
+static void free_open(geometry_t *g) {
+  for (int n = 0; g->open && n < g->nobc; n++) {
+    open_bdrys_t *o = g->open[n];
+    if (!o) continue;
+    void *a[] = {o->nmap, o->obc_t, o->oi1_t, o->oi2_t, o->bot_t, o->obc_e1, o->oi1_e1,
+                 o->obc_e2, o->oi1_e2, o->trm, o->bcond_tra, o->clampv, o->trpc,
+                 o->relax_zone_tra};
+    for (size_t q = 0; q < sizeof(a) / sizeof(a[0]); q++) free(a[q]);
+    free(o);
+  }
+  free(g->open);
+  g->open = NULL;
+  g->nobc = 0;
+}
+
+int emstr_grid_open(emstr_grid *G, int side) {
+  geometry_t *g = G->window;
+  const int nx = G->nx, ny = G->ny, ntr = G->ntr;
+  int cc, n, c;
+  const int xs = (side == L_EDGE || side == R_EDGE);
+  const int line = (side == L_EDGE || side == B_EDGE) ? 1 : (xs ? nx - 2 : ny - 2);
+  if (side != L_EDGE && side != R_EDGE && side != B_EDGE && side != F_EDGE) return -1;
+  /* the wet columns on that side; none may be open already */
+  char *on = (char *)zalloc(g->sgsizS, 1);
+  int nb = 0;
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    const int cs = g->w2_t[cc];
+    if ((xs ? g->s2i[cs] : g->s2j[cs]) != line) continue;
+    if (cc > g->v2_t) continue; /* a corner column of an earlier boundary */
+    on[cs] = 1;
+    nb++;
+  }
+  if (!nb) {
+    free(on);
+    return -1;
+  }
+  /* w2_t: interior columns, the earlier boundaries' columns, this one's;
+   * the per-column vectors follow */
+  int *w2 = (int *)zalloc(g->sgsizS, sizeof(int));
+  int *bt = (int *)zalloc(g->sgsizS, sizeof(int));
+  int *st = (int *)zalloc(g->sgsizS, sizeof(int));
+  int *ns = (int *)zalloc(g->sgsizS, sizeof(int));
+  int m = 0;
+  for (int pass = 0; pass < 3; pass++)
+    for (cc = 1; cc <= g->b2_t; cc++) {
+      const int cs = g->w2_t[cc];
+      const int grp = on[cs] ? 2 : (cc > g->v2_t ? 1 : 0);
+      if (grp != pass) continue;
+      m++;
+      w2[m] = cs;
+      bt[m] = g->bot_t[cc];
+      st[m] = g->sur_t[cc];
+      ns[m] = g->nsur_t[cc];
+    }
+  for (cc = 1; cc <= g->b2_t; cc++) {
+    g->w2_t[cc] = w2[cc];
+    g->bot_t[cc] = bt[cc];
+    g->sur_t[cc] = st[cc];
+    g->nsur_t[cc] = ns[cc];
+  }
+  g->v2_t -= nb;
+  /* w3_t in the same grouping */
+  int *w3 = (int *)zalloc(g->sgsiz, sizeof(int));
+  int nb3 = 0;
+  m = 0;
+  for (int pass = 0; pass < 3; pass++)
+    for (cc = 1; cc <= g->b3_t; cc++) {
+      c = g->w3_t[cc];
+      const int grp = on[g->m2d[c]] ? 2 : (cc > g->v3_t ? 1 : 0);
+      if (grp != pass) continue;
+      w3[++m] = c;
+      nb3 += (grp == 2);
+    }
+  for (cc = 1; cc <= g->b3_t; cc++) g->w3_t[cc] = w3[cc];
+  g->v3_t -= nb3;
+  /* the boundary: surface cells, then each column's cells below */
+  open_bdrys_t *o = (open_bdrys_t *)zalloc(1, sizeof(open_bdrys_t));
+  o->ocodex = xs ? side : 0;
+  o->ocodey = xs ? 0 : side;
+  int *in = (side == L_EDGE) ? g->xp1 : (side == R_EDGE) ? g->xm1 : (side == B_EDGE) ? g->yp1 : g->ym1;
+  o->nmap = (int *)zalloc(g->sgsiz, sizeof(int));
+  memcpy(o->nmap, in, (size_t)g->sgsiz * sizeof(int));
+  o->no2_t = nb;
+  o->no3_t = nb3;
+  const size_t n3 = (size_t)nb3 + 1;
+  o->obc_t = (int *)zalloc(n3, sizeof(int));
+  o->oi1_t = (int *)zalloc(n3, sizeof(int));
+  o->oi2_t = (int *)zalloc(n3, sizeof(int));
+  o->bot_t = (int *)zalloc((size_t)nb + 1, sizeof(int));
+  int *fb = (int *)zalloc(n3, sizeof(int)), *fi = (int *)zalloc(n3, sizeof(int));
+  m = 0;
+  for (cc = g->b2_t - nb + 1; cc <= g->b2_t; cc++) {
+    o->obc_t[++m] = g->w2_t[cc];
+    o->bot_t[m] = g->bot_t[cc];
+  }
+  for (n = 1; n <= nb; n++) {
+    c = o->obc_t[n];
+    while (c != o->bot_t[n]) {
+      c = g->zm1[c];
+      o->obc_t[++m] = c;
+    }
+  }
+  for (cc = 1; cc <= nb3; cc++) {
+    c = o->obc_t[cc];
+    o->oi1_t[cc] = o->nmap[c];
+    o->oi2_t[cc] = o->nmap[o->oi1_t[cc]];
+    /* the face on the boundary and the one a cell in (a face is its
+     * cell's west / south face) */
+    if (side == L_EDGE || side == B_EDGE) {
+      fb[cc] = c;
+      fi[cc] = o->nmap[c];
+    } else {
+      fb[cc] = (side == R_EDGE) ? g->xp1[c] : g->yp1[c];
+      fi[cc] = c;
+    }
+  }
+  if (xs) {
+    o->obc_e1 = fb;
+    o->oi1_e1 = fi;
+  } else {
+    o->obc_e2 = fb;
+    o->oi1_e2 = fi;
+  }
+  const int nt = ntr > 0 ? ntr : 1;
+  o->trm = (int *)zalloc(nt, sizeof(int));
+  o->bcond_tra = (int *)zalloc(nt, sizeof(int));
+  o->clampv = (double *)zalloc(nt, sizeof(double));
+  o->trpc = (double *)zalloc(nt, sizeof(double));
+  o->relax_zone_tra = (int *)zalloc(nt, sizeof(int));
+  for (n = 0; n < ntr; n++) {
+    o->bcond_tra[n] = NOGRAD;
+    o->trm[n] = -1;
+  }
+  g->open = (open_bdrys_t **)realloc(g->open, (size_t)(g->nobc + 1) * sizeof(open_bdrys_t *));
+  g->open[g->nobc++] = o;
+  free(on);
+  free(w2);
+  free(bt);
+  free(st);
+  free(ns);
+  free(w3);
+  return 0;
+}
+
 void emstr_grid_free(emstr_grid *G) {
   int n;
   if (!G) return;
   geometry_t *g = G->window;
   if (g) emstr_forget_window(g);
+  if (g) free_open(g);
   window_t *w = G->windat;
   win_priv_t *p = G->wincon;
   if (g) {

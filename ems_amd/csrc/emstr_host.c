@@ -98,6 +98,13 @@ struct emstr_ctx {
   double *d_dec, *d_dsc;
   int ndec_cap;
   int *d_p2, *h_p2;
+  /* open boundaries: per boundary its cells (dense), one cell in, the
+   * LINEXT chain, cellz and width (device, made once), and the per-call
+   * velocities, tracer settings and FILEIN values */
+  int nob, b0; /* b0: the buffer of the step's starting values */
+  ObcArgs *ob;
+  void **obm; /* device allocations of the boundaries */
+  int nobm;
   /* host staging of the sediment ghosts' dz, [plane] */
   double *h_dzg;
 };
@@ -114,7 +121,7 @@ static emstr_ctx *reg_c[MAXCTX];
  * same grid (a freed window's address can be reused by a new one). */
 static int ctx_matches(const emstr_ctx *c, const geometry_t *g) {
   return c->g == g && c->key_sgsiz == g->sgsiz && c->key_nx == g->nce1 &&
-         c->key_ny == g->nce2 && c->key_nz == g->nz && c->key_v3 == g->v3_t &&
+         c->key_ny == g->nce2 && c->key_nz == g->nz && c->key_v3 == g->b3_t &&
          c->key_w3 == g->w3_t && c->key_xp1 == g->xp1;
 }
 
@@ -226,9 +233,16 @@ emstr_ctx *emstr_create(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
     emstr_set_error(EMSTR_EARG, -1, "emstr_create: null structure");
     return NULL;
   }
-  if (g->nobc != 0 || wc->sigma) {
+  if (wc->sigma) {
+    emstr_set_error(EMSTR_EUNSUPPORTED, -1, "sigma coordinates are not supported");
+    return NULL;
+  }
+  if (g->nobc != 0 &&
+      (!g->open || form != EMSTR_PLAIN_F64 || wc->trasc == FFSL ||
+       (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)))) {
     emstr_set_error(EMSTR_EUNSUPPORTED, -1,
-                    "open boundaries and sigma coordinates are not supported");
+                    "open boundaries: the plain form with an Eulerian scheme and no "
+                    "sub-stepping only");
     return NULL;
   }
   if (form != EMSTR_PLAIN_F64 && form != EMSTR_PREC_F32 && form != EMSTR_BATCH_F32) {
@@ -256,7 +270,7 @@ emstr_ctx *emstr_create(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
   c->key_nx = g->nce1;
   c->key_ny = g->nce2;
   c->key_nz = g->nz;
-  c->key_v3 = g->v3_t;
+  c->key_v3 = g->b3_t;
   c->key_w3 = g->w3_t;
   c->key_xp1 = g->xp1;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
@@ -303,7 +317,7 @@ emstr_ctx *emstr_create(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
     }
   }
   /* wet cells: sparse <-> dense */
-  d->nwet = g->v3_t;
+  d->nwet = g->b3_t; /* wet cells, open-boundary cells included */
   int *wsp = (int *)malloc((size_t)d->nwet * sizeof(int));
   ll *wdn = (ll *)malloc((size_t)d->nwet * sizeof(ll));
   c->sp_of = (int *)calloc(n3, sizeof(int));
@@ -484,6 +498,10 @@ void emstr_destroy(emstr_ctx *c) {
                 c->d_dec, c->d_p2, d->dzg};
   for (n = 0; n < (int)(sizeof(dv) / sizeof(dv[0])); n++)
     if (dv[n]) hipFree(dv[n]);
+  for (n = 0; n < c->nobm; n++)
+    if (c->obm[n]) hipFree(c->obm[n]);
+  free(c->obm);
+  free(c->ob);
   void *hv[] = {c->h_kb, c->cc2col, c->col2cc, c->sedg, c->topc, c->h_ks,
                 c->h_kold, c->h_knew, c->h_proc, c->h_cols, c->sp_of, c->h_part,
                 c->h_dn, c->h_ksur, c->h_cb, c->h_p2, c->h_dzg};
@@ -591,6 +609,8 @@ static int upload_tracer_meta(emstr_ctx *c) {
   return 0;
 }
 
+static int obc_setup(emstr_ctx *c);
+
 int emstr_upload_fields(emstr_ctx *c) {
   window_t *wd = c->wd;
   win_priv_t *wc = c->wc;
@@ -605,6 +625,11 @@ int emstr_upload_fields(emstr_ctx *c) {
       upload3d(c, d->u2flux3d, wd->u2flux3d) || upload3d(c, d->Kz, wd->Kz) ||
       upload3d(c, d->dz, wc->dz))
     return EMSTR_EDEVICE;
+  /* open-boundary columns are left out of the advective update */
+  {
+    const int r0 = obc_setup(c);
+    if (r0) return r0;
+  }
   /* the sediment ghosts' dz (reset_dz writes them, set_dz copies them up
    * a column whose old surface is the ghost) */
   for (int cc = 1; cc <= c->g->a2_t; cc++) {
@@ -735,6 +760,7 @@ static int adv_plain(emstr_ctx *c) {
                            "tracer_step_3d: maximum number of sub-steps (%d) exceeded", itermax);
   /* buffers: start = cur (kept for lateral ghosts), then rotate */
   const int b0 = c->cur;
+  c->b0 = b0;
   int in = b0, out = (b0 + 1) % 3;
   trem = dtu;
   while (trem > 0) {
@@ -1114,7 +1140,7 @@ static int lists_to_host(emstr_ctx *c) {
   double *tmp = (double *)malloc((size_t)g->sgsiz * sizeof(double));
   HCHK(hipMemcpyAsync(tmp, c->stage, (size_t)g->sgsiz * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HCHK(hipStreamSynchronize(c->st));
-  for (int q = 1; q <= g->v3_t; q++) {
+  for (int q = 1; q <= g->b3_t; q++) {
     int cw = g->w3_t[q];
     wc->dz[cw] = tmp[cw];
   }
@@ -1356,6 +1382,8 @@ static int trans_setup(emstr_ctx *c) {
  * :94-530 for an exact grid, closed boundaries, no fills, a linear
  * surface and CONS_W; then tracer_step and transport_post): the stored
  * flow of the step is in windat (u1, u2, Kz, eta). */
+static int emstr_bdry(emstr_ctx *c);
+
 int emstr_transport_step(emstr_ctx *c) {
   geometry_t *g = c->g;
   window_t *wd = c->wd;
@@ -1409,6 +1437,7 @@ int emstr_transport_step(emstr_ctx *c) {
   /* tracer_step (tracer_step_3d: the transport, then auxiliary_routines) */
   if ((r = emstr_step(c))) return r;
   if ((r = emstr_auxiliary(c))) return r;
+  if ((r = emstr_bdry(c))) return r;
   /* transport_post: the new oldeta (transport.c:543) */
   memcpy(wc->oldeta, wd->eta, (size_t)g->sgsizS * sizeof(double));
   HCHK(hipMemcpyAsync(d->oldeta, d->eta, (size_t)d->plane * sizeof(double),
@@ -1578,8 +1607,176 @@ int emstr_auxiliary(emstr_ctx *c) {
     HCHK(hipMemcpyAsync(tmp, c->stage, (size_t)g->sgsiz * sizeof(double), hipMemcpyDeviceToHost,
                         c->st));
     HCHK(hipStreamSynchronize(c->st));
-    for (int q3 = 1; q3 <= g->v3_t; q3++) fdst[q][g->w3_t[q3]] = tmp[g->w3_t[q3]];
+    for (int q3 = 1; q3 <= g->b3_t; q3++) fdst[q][g->w3_t[q3]] = tmp[g->w3_t[q3]];
     free(tmp);
+  }
+  return 0;
+}
+
+/*-------------------------------------------------------------------*/
+/* open boundaries (tracers.c:4542-5165)                              */
+/*-------------------------------------------------------------------*/
+static ll dense_of(const emstr_ctx *c, int sp) {
+  const geometry_t *g = c->g;
+  return (ll)g->s2k[sp] * c->d.plane + (ll)g->s2j[sp] * c->d.nx + g->s2i[sp];
+}
+
+static void *obc_up(emstr_ctx *c, const void *h, size_t bytes, int *ok) {
+  void *p = dalloc(bytes ? bytes : 8, ok);
+  if (*ok && h && bytes && hipMemcpy(p, h, bytes, hipMemcpyHostToDevice) != hipSuccess) *ok = 0;
+  c->obm = (void **)realloc(c->obm, (size_t)(c->nobm + 1) * sizeof(void *));
+  c->obm[c->nobm++] = p;
+  return p;
+}
+
+/* the geometry of every boundary on the device, and the column flags */
+static int obc_setup(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  emstr_dev *d = &c->d;
+  int ok = 1, n, cc;
+  if (c->ob || !g->nobc) return 0;
+  c->nob = g->nobc;
+  c->ob = (ObcArgs *)calloc((size_t)g->nobc, sizeof(ObcArgs));
+  int *hm = (int *)calloc((size_t)d->plane, sizeof(int));
+  for (cc = g->v2_t + 1; cc <= g->b2_t; cc++) hm[c->cc2col[cc]] = 1;
+  /* a lateral ghost among the cells inwards holds the step's starting
+   * value of its wet cell (set_lateral_BC_tr, tracers.c:5574-5593):
+   * such a reference is ~(that cell's dense index), read from the
+   * starting buffer */
+  int *gbin = (int *)calloc((size_t)g->sgsiz, sizeof(int));
+  for (cc = 1; cc <= g->nbpt; cc++) gbin[g->bpt[cc]] = g->bin[cc];
+#define OBC_REF(sp) (gbin[sp] ? ~dense_of(c, gbin[sp]) : dense_of(c, sp))
+  d->obcm = (int *)obc_up(c, hm, (size_t)d->plane * sizeof(int), &ok);
+  free(hm);
+  for (n = 0; n < g->nobc && ok; n++) {
+    const open_bdrys_t *o = g->open[n];
+    const int m = o->no3_t, xd = (o->ocodex & (L_EDGE | R_EDGE)) != 0;
+    ll *cell = (ll *)malloc(((size_t)m + 1) * sizeof(ll));
+    ll *in1 = (ll *)malloc(((size_t)m + 1) * sizeof(ll));
+    ll *ch = (ll *)malloc(((size_t)m + 1) * 4 * sizeof(ll));
+    double *cz = (double *)malloc(((size_t)m + 1) * sizeof(double));
+    double *hat = (double *)malloc(((size_t)m + 1) * sizeof(double));
+    for (cc = 1; cc <= m; cc++) {
+      const int sp = o->obc_t[cc], q = cc - 1;
+      int e = sp;
+      cell[q] = dense_of(c, sp);
+      in1[q] = OBC_REF(o->oi1_t[cc]);
+      for (int j = 0; j < 4; j++) {
+        e = o->nmap[e];
+        ch[4 * q + j] = OBC_REF(e);
+      }
+      cz[q] = g->cellz[sp];
+      hat[q] = xd ? g->h1au1[g->m2d[sp]] : g->h2au2[g->m2d[sp]];
+    }
+    ObcArgs *a = &c->ob[n];
+    a->n = m;
+    a->cell = (const ll *)obc_up(c, cell, (size_t)m * sizeof(ll), &ok);
+    a->in1 = (const ll *)obc_up(c, in1, (size_t)m * sizeof(ll), &ok);
+    a->chain = (const ll *)obc_up(c, ch, (size_t)m * 4 * sizeof(ll), &ok);
+    a->cz = (const double *)obc_up(c, cz, (size_t)m * sizeof(double), &ok);
+    a->hat = (const double *)obc_up(c, hat, (size_t)m * sizeof(double), &ok);
+    a->vel = (const double *)obc_up(c, NULL, (size_t)m * sizeof(double), &ok);
+    const size_t nt = (size_t)(c->wd->ntr > 0 ? c->wd->ntr : 1);
+    a->bcond = (const int *)obc_up(c, NULL, nt * sizeof(int), &ok);
+    a->frow = (const int *)obc_up(c, NULL, nt * sizeof(int), &ok);
+    a->clampv = (const double *)obc_up(c, NULL, nt * sizeof(double), &ok);
+    a->trpc = (const double *)obc_up(c, NULL, nt * sizeof(double), &ok);
+    a->fv = (const double *)obc_up(c, NULL, nt * (size_t)m * sizeof(double), &ok);
+    free(cell);
+    free(in1);
+    free(ch);
+    free(cz);
+    free(hat);
+  }
+#undef OBC_REF
+  free(gbin);
+  return ok ? 0 : emstr_set_error(EMSTR_EDEVICE, -1, "open boundary tables");
+}
+
+/* bdry_tracer (tracers.c:4542-4561) after auxiliary_routines: each
+ * boundary in order, its settings checked and uploaded, then k_obc */
+static int emstr_bdry(emstr_ctx *c) {
+  geometry_t *g = c->g;
+  window_t *wd = c->wd;
+  win_priv_t *wc = c->wc;
+  int n, cc, r;
+  if (!g->nobc) return 0;
+  if ((r = obc_setup(c))) return r;
+  const int ntr = wd->ntr, nt = ntr > 0 ? ntr : 1;
+  const int bad = CUSTOM | POLEXT | CYCLIC | STATIS | PROFIL | DEPROF | DESCAL | TRFLUX |
+                  TRCONC | TRCONF;
+  for (n = 0; n < g->nobc; n++) {
+    const open_bdrys_t *o = g->open[n];
+    ObcArgs *a = &c->ob[n];
+    const int m = o->no3_t, xd = (o->ocodex & (L_EDGE | R_EDGE)) != 0;
+    int *bc = (int *)calloc((size_t)nt, sizeof(int)), *fr = (int *)calloc((size_t)nt, sizeof(int));
+    double *cv = (double *)calloc((size_t)nt, sizeof(double));
+    double *pc = (double *)calloc((size_t)nt, sizeof(double));
+    double *fv = (double *)calloc((size_t)nt * (m ? m : 1), sizeof(double));
+    double *vel = (double *)calloc((size_t)(m ? m : 1), sizeof(double));
+    int nf = 0, err = 0;
+    if (o->bgz > 0 || (o->options & OP_OBCCM)) err = 1;
+    for (int k = 0; k < wc->ntbdy && !err; k++) {
+      const int tn = wc->tbdy[k], b = o->bcond_tra[tn];
+      if ((b & bad) || (o->relax_zone_tra && o->relax_zone_tra[tn] > 0) ||
+          !(b & (NOGRAD | NOTHIN | CLAMPD | LINEXT | FILEIN)) ||
+          (o->trpc[tn] != 0.0 && !((b & NOGRAD) && (b & FILEIN))) ||
+          ((b & FILEIN) && (!o->t_transfer || !o->trm || o->trm[tn] < 0)))
+        err = 1;
+      bc[tn] = b;
+      cv[tn] = o->clampv[tn];
+      pc[tn] = o->trpc[tn];
+      if (!err && (b & FILEIN)) {
+        fr[tn] = nf;
+        for (cc = 1; cc <= m; cc++) fv[(size_t)nf * m + cc - 1] = o->t_transfer[o->trm[tn]][cc];
+        nf++;
+      }
+    }
+    /* the UPSTRM velocity of every cell, as upmeth picks it */
+    const double *u = xd ? wd->u1 : wd->u2;
+    const int *fb = xd ? o->obc_e1 : o->obc_e2, *fi = xd ? o->oi1_e1 : o->oi1_e2;
+    const double sc = ((o->ocodex & L_EDGE) || (o->ocodey & B_EDGE)) ? -1.0 : 1.0;
+    for (cc = 1; cc <= m && !err; cc++) {
+      if (o->upmeth & CENTER)
+        vel[cc - 1] = 0.5 * (u[fb[cc]] + u[fi[cc]]);
+      else if (o->upmeth & FACE)
+        vel[cc - 1] = u[fb[cc]];
+      else if (o->upmeth & ADAPTIVE)
+        vel[cc - 1] = -sc * 0.5 * (u[fb[cc]] + u[fi[cc]]) > 0.0 ? u[fb[cc]] : u[fi[cc]];
+      else
+        vel[cc - 1] = u[o->oi1_t[cc]];
+    }
+    if (!err) {
+      r = hipMemcpyAsync((void *)a->bcond, bc, nt * sizeof(int), hipMemcpyHostToDevice, c->st) ||
+          hipMemcpyAsync((void *)a->frow, fr, nt * sizeof(int), hipMemcpyHostToDevice, c->st) ||
+          hipMemcpyAsync((void *)a->clampv, cv, nt * sizeof(double), hipMemcpyHostToDevice,
+                         c->st) ||
+          hipMemcpyAsync((void *)a->trpc, pc, nt * sizeof(double), hipMemcpyHostToDevice,
+                         c->st) ||
+          (m && hipMemcpyAsync((void *)a->vel, vel, (size_t)m * sizeof(double),
+                               hipMemcpyHostToDevice, c->st)) ||
+          (nf && m && hipMemcpyAsync((void *)a->fv, fv, (size_t)nf * m * sizeof(double),
+                                     hipMemcpyHostToDevice, c->st));
+      if (!r) r = hipStreamSynchronize(c->st) != hipSuccess;
+    }
+    free(bc);
+    free(fr);
+    free(cv);
+    free(pc);
+    free(fv);
+    free(vel);
+    if (err)
+      return emstr_set_error(EMSTR_EUNSUPPORTED, -1,
+                             "open boundary %d: conditions other than NOGRAD, NOTHIN, CLAMPD, "
+                             "LINEXT, FILEIN, UPSTRM (or trpc without NOGRAD and FILEIN, flow "
+                             "relaxation, ghost zones, corner averaging)",
+                             n);
+    if (r) return emstr_set_error(EMSTR_EDEVICE, -1, "open boundary upload");
+    a->dt = wd->dt;
+    a->sc = sc;
+    a->owrite = (o->options & OP_OWRITE) ? 1 : 0;
+    if (emstr_launch_obc(&c->d, a, c->tr[c->cur], c->tr[c->b0], c->st))
+      return emstr_set_error(EMSTR_EDEVICE, -1, "open boundary launch");
   }
   return 0;
 }
@@ -1600,7 +1797,7 @@ static int aux_to_host(emstr_ctx *c) {
   double *tmp = (double *)malloc((size_t)g->sgsiz * sizeof(double));
   HCHK(hipMemcpyAsync(tmp, c->stage, (size_t)g->sgsiz * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HCHK(hipStreamSynchronize(c->st));
-  for (int q = 1; q <= g->v3_t; q++) wc->dz[g->w3_t[q]] = tmp[g->w3_t[q]];
+  for (int q = 1; q <= g->b3_t; q++) wc->dz[g->w3_t[q]] = tmp[g->w3_t[q]];
   free(tmp);
   HCHK(hipMemcpyAsync(c->h_dzg, d->dzg, (size_t)d->plane * sizeof(double),
                       hipMemcpyDeviceToHost, c->st));
@@ -1670,8 +1867,9 @@ void tracer_step_3d(geometry_t *window, window_t *windat, win_priv_t *wincon) {
   if (!c) return;
   if (emstr_upload_fields(c) || emstr_upload_tracers(c)) return;
   if (emstr_step(c)) return;
-  /* tracers.c:289 */
+  /* tracers.c:289, 294 */
   if (emstr_auxiliary(c)) return;
+  if (emstr_bdry(c)) return;
   if (emstr_download_tracers(c)) return;
   if (aux_to_host(c)) return;
   emstr_sync(c);

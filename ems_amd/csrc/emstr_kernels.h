@@ -87,6 +87,29 @@ int emstr_launch_flux3d(const emstr_dev *d, const unsigned char *fw, const int *
                         const double *uvm, int ncb, const long long *cb, const double *uf,
                         double dt, double *out, void *stream);
 
+/* one open boundary for k_obc: its n cells (dense), one cell inwards,
+ * the four inward cells of LINEXT, cellz, the UPSTRM velocity and cell
+ * width; per tracer the condition bits, CLAMPD value, trpc and FILEIN
+ * row of fv ([row][n]) */
+#define OBC_FILEIN 0x000001
+#define OBC_NOGRAD 0x000004
+#define OBC_LINEXT 0x000008
+#define OBC_NOTHIN 0x000800
+#define OBC_CLAMPD 0x004000
+#define OBC_UPSTRM 0x010000
+typedef struct ObcArgs {
+  int n, owrite;
+  const long long *cell, *in1, *chain;
+  const double *cz, *vel, *hat;
+  double dt, sc;
+  const int *bcond, *frow;
+  const double *clampv, *trpc, *fv;
+} ObcArgs;
+/* tr: the tracers, ts: the step's starting values (read for a lateral
+ * ghost among the cells inwards, encoded ~index) */
+int emstr_launch_obc(const emstr_dev *d, const ObcArgs *a, double *tr, const double *ts,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -721,6 +721,24 @@ k_upd(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout, do
     return;
   }
   if (k > ksc) return; /* above the processed surface: written below */
+  if (d.obcm && d.obcm[col]) {
+    /* an open-boundary column: the update and surf_conc run over the
+     * other columns (tracers.c:811-822, 912, 945: vc1 / vcs1), and so do
+     * tr_bounds' thin-layer reset and clip (tracers.c:3250, 3277);
+     * its copy above the surface covers every processed column
+     * (tracers.c:3306-3322) */
+    for (int n = 0; n < d.ntr; n++) {
+      const double *t = tin + n * N3;
+      double *o = tout + n * N3;
+      if (k < ksc) {
+        o[idx] = t[idx];
+      } else {
+        const double v = t[idx];
+        for (int kk = k; kk < d.nz; kk++) o[(ll)kk * P + col] = d.in_tbdy[n] ? v : t[(ll)kk * P + col];
+      }
+    }
+    return;
+  }
   const double area = d.cellarea[col], dzc = d.dz[idx];
 
   if (k < ksc) {
@@ -1911,6 +1929,75 @@ __global__ void k_resetdz(emstr_dev d, int *__restrict__ p2) {
     else d.dzg[col] = top - bot;
   }
   p2[col] = (d.eta[col] > 0.05 * d.hmin + d.botz[col] && knew >= kbc) ? 1 : 0;
+}
+
+/* bc_leastsq (boundary/bdrycond.c:180-214): the least-squares line
+ * through (1, y1) .. (4, y4), evaluated at 5, in the reference's order */
+__device__ double obc_leastsq(double y1, double y2, double y3, double y4) {
+  const double x[5] = {0.0, 1.0, 2.0, 3.0, 4.0};
+  const double y[5] = {0.0, y1, y2, y3, y4};
+  double sxy = 0.0, sx = 0.0, sy = 0.0, sx2 = 0.0;
+  const int nn = 4;
+  for (int i = 1; i <= nn; i++) {
+    sx += x[i];
+    sy += y[i];
+    sxy += (x[i] * y[i]);
+    sx2 += (x[i] * x[i]);
+  }
+  const double a1 = (sxy - (sx * sy / nn)) / (sx2 - (sx * sx) / nn);
+  const double a2 = sy / nn - a1 * sx / nn;
+  return a2 + a1 * 5.0;
+}
+
+/* set_OBC_tr (tracers.c:4569-5165), the conditions built, one thread
+ * per cell of one open boundary, every tracer of tbdy: the new value
+ * from the condition bits (NOGRAD above trpc, NOTHIN, CLAMPD, LINEXT
+ * through the inward chain, FILEIN below trpc, then UPSTRM with the
+ * velocity the host picked by upmeth), written unless NOTHIN without
+ * OP_OWRITE.  A boundary's cells read only cells inwards of it (and
+ * themselves), so its threads do not read what another writes. */
+__global__ void k_obc(emstr_dev d, ObcArgs a, double *__restrict__ tr,
+                      const double *__restrict__ tsr) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.n) return;
+  const ll N3 = d.n3;
+  const ll c = a.cell[q], c1 = a.in1[q];
+  const ll e0 = a.chain[4 * q], e1 = a.chain[4 * q + 1], e2 = a.chain[4 * q + 2],
+           e3 = a.chain[4 * q + 3];
+  for (int n = 0; n < d.ntr; n++) {
+    if (!d.in_tbdy[n]) continue;
+    double *t = tr + n * N3;
+    const double *ts = tsr + n * N3;
+    /* a cell inwards, or a lateral ghost's starting value */
+    auto rd = [&](ll e) { return (e >= 0) ? t[e] : ts[~e]; };
+    const int bc = a.bcond[n];
+    const double pc = a.trpc[n];
+    double nv = 0.0;
+    if (bc & OBC_NOGRAD) {
+      if (pc == 0.0 || a.cz[q] > pc) nv = rd(c1);
+    } else if (bc & OBC_NOTHIN) {
+      nv = t[c];
+    } else if (bc & OBC_CLAMPD) {
+      nv = a.clampv[n];
+    } else if (bc & OBC_LINEXT) {
+      nv = obc_leastsq(rd(e3), rd(e2), rd(e1), rd(e0));
+    }
+    if (bc & OBC_FILEIN)
+      if (pc == 0.0 || a.cz[q] <= pc) nv = a.fv[(ll)a.frow[n] * a.n + q];
+    if (bc & OBC_UPSTRM) {
+      const double v = a.vel[q], sc = a.sc, tc = t[c];
+      nv = tc - a.dt / a.hat[q] *
+                    (0.5 * (sc * v + fabs(v)) * (tc - rd(c1)) + 0.5 * (sc * v - fabs(v)) * (nv - tc));
+    }
+    if (!(bc & OBC_NOTHIN) || a.owrite) t[c] = nv;
+  }
+}
+
+extern "C" int emstr_launch_obc(const emstr_dev *d, const ObcArgs *a, double *tr,
+                                const double *ts, void *stream) {
+  if (a->n > 0)
+    k_obc<<<(a->n + 255) / 256, 256, 0, (hipStream_t)stream>>>(*d, *a, tr, ts);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 /* do_ts_relax's no-gradient relaxation field (tracers.c:6137-6148), per

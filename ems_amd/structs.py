@@ -14,10 +14,23 @@ _PI = C.POINTER(C.c_int)
 _PD = C.POINTER(C.c_double)
 
 
+class OpenBdry(C.Structure):
+    """open_bdrys_t (include/emstr.h)"""
+    _fields_ = [
+        ("ocodex", _I), ("ocodey", _I), ("nmap", _PI), ("no3_t", _I), ("no2_t", _I),
+        ("obc_t", _PI), ("oi1_t", _PI), ("oi2_t", _PI), ("cyc_t", _PI), ("bot_t", _PI),
+        ("obc_e1", _PI), ("oi1_e1", _PI), ("obc_e2", _PI), ("oi1_e2", _PI),
+        ("t_transfer", C.POINTER(_PD)), ("bgz", _I), ("trm", _PI), ("upmeth", _I),
+        ("bcond_tra", _PI), ("clampv", _PD), ("trpc", _PD), ("relax_zone_tra", _PI),
+        ("options", _I),
+    ]
+
+
 class Geometry(C.Structure):
     _fields_ = [
         ("sgnum", _I), ("sgnumS", _I), ("sgsiz", _I), ("sgsizS", _I),
-        ("nobc", _I), ("nce1", _I), ("nce2", _I), ("nz", _I), ("wn", _I),
+        ("nobc", _I), ("open", C.POINTER(C.POINTER(OpenBdry))),
+        ("nce1", _I), ("nce2", _I), ("nz", _I), ("wn", _I),
         ("layers", _PD),
         ("s2i", _PI), ("s2j", _PI), ("s2k", _PI),
         ("w3_t", _PI), ("w3_e1", _PI), ("w3_e2", _PI),
@@ -115,6 +128,13 @@ QUICKEST_CO = 0x000040
 VANLEER = 0x000080
 ORDER2_UW = 0x002000
 FFSL = 0x080000
+# open boundaries (include/emstr.h)
+L_EDGE, R_EDGE, B_EDGE, F_EDGE = 0x80, 0x100, 0x400, 0x800
+FILEIN, CUSTOM, NOGRAD, LINEXT, POLEXT = 0x1, 0x2, 0x4, 0x8, 0x10
+CYCLIC, NOTHIN, CLAMPD, STATIS, UPSTRM = 0x400, 0x800, 0x4000, 0x8000, 0x10000
+TRFLUX, TRCONC, TRCONF = 0x1000000, 0x2000000, 0x4000000
+OP_OWRITE, OP_OBCCM = 0x20000, 0x40000
+CENTER, FACE, ADAPTIVE = 1, 2, 8
 SUB_STEP = 2
 TMASS = 16
 TRACERS = 256

@@ -79,6 +79,61 @@ typedef struct tracer_info {
 } tracer_info_t;
 
 /*-------------------------------------------------------------------*/
+/* Open boundary (the fields of the reference open_bdrys_t,          */
+/* model/hd/include/boundary.h:203-318, that the tracer step reads). */
+/* Built: tracer conditions NOGRAD (with the depth trpc), NOTHIN,    */
+/* CLAMPD, LINEXT, FILEIN and UPSTRM, OP_OWRITE; other conditions,   */
+/* ghost zones (bgz), flow relaxation zones and scaling are refused. */
+/*-------------------------------------------------------------------*/
+#define L_EDGE 0x00000080 /* model/hd/include/hd_params.h:53-60     */
+#define R_EDGE 0x00000100
+#define B_EDGE 0x00000400
+#define F_EDGE 0x00000800
+#define FILEIN 0x000001 /* hd_params.h:136-168 */
+#define CUSTOM 0x000002
+#define NOGRAD 0x000004
+#define LINEXT 0x000008
+#define POLEXT 0x000010
+#define CYCLIC 0x000400
+#define NOTHIN 0x000800
+#define CLAMPD 0x004000
+#define STATIS 0x008000
+#define UPSTRM 0x010000
+#define PROFIL 0x100000
+#define DEPROF 0x200000
+#define DESCAL 0x400000
+#define TRFLUX 0x1000000
+#define TRCONC 0x2000000
+#define TRCONF 0x4000000
+#define OP_OWRITE 0x20000 /* hd_params.h:212 */
+#define OP_OBCCM 0x40000
+#define CENTER 1 /* hd_params.h:411-414 (upmeth) */
+#define FACE 2
+#define ADAPTIVE 8
+typedef struct open_bdrys {
+  int ocodex, ocodey; /* L_EDGE / R_EDGE, B_EDGE / F_EDGE            */
+  int *nmap;          /* [sgsiz] map one cell inwards, normal to it  */
+  int no3_t, no2_t;   /* boundary cells (3D), of them in the surface */
+  int *obc_t;         /* [1..no3_t] boundary cells, surface first    */
+  int *oi1_t, *oi2_t; /* [1..no3_t] one and two cells inwards        */
+  int *cyc_t;         /* [1..no3_t] cyclic partners (CYCLIC: refused) */
+  int *bot_t;         /* [1..no2_t] bed cell of each boundary column */
+  int *obc_e1, *oi1_e1; /* [1..no3_t] u1 face on the boundary, one in */
+  int *obc_e2, *oi1_e2; /* [1..no3_t] u2 face on the boundary, one in */
+  double **t_transfer; /* FILEIN values: t_transfer[trm[n]][1..no3_t] */
+  int bgz;            /* ghost zone (refused above 0)                */
+  int *trm;           /* [ntr] row of t_transfer of a FILEIN tracer  */
+  int upmeth;         /* UPSTRM velocities: CENTER, FACE, ADAPTIVE or
+                         0 (one cell inwards)                        */
+  int *bcond_tra;     /* [ntr] tracer condition bits                 */
+  double *clampv;     /* [ntr] CLAMPD value                          */
+  double *trpc;       /* [ntr] depth splitting NOGRAD (above) from
+                         FILEIN (below); 0: none                     */
+  int *relax_zone_tra; /* [ntr] flow relaxation zone (refused above 0) */
+  int options;        /* OP_OWRITE; OP_OBCCM refused                 */
+} open_bdrys_t;
+
+/*-------------------------------------------------------------------*/
 /* Grid structure (fields of the reference geometry_t)              */
 /*-------------------------------------------------------------------*/
 typedef struct geometry {
@@ -87,6 +142,7 @@ typedef struct geometry {
   int sgsiz;      /* sgnum + 1                                      */
   int sgsizS;     /* sgnumS + 1                                     */
   int nobc;       /* open boundaries (0: closed basin)              */
+  open_bdrys_t **open; /* [nobc]                                    */
   int nce1, nce2; /* grid dimensions                                */
   int nz;         /* number of layers                               */
   int wn;         /* window number                                  */

@@ -54,6 +54,16 @@ emstr_grid *emstr_grid_build(int nx, int ny, int nz, const double *layers,
                              int ntr);
 void emstr_grid_free(emstr_grid *g);
 
+/* Makes the wet columns on one side of the grid an open boundary:
+ * side L_EDGE (columns i = 1), R_EDGE (i = nx - 2), B_EDGE (j = 1) or
+ * F_EDGE (j = ny - 2).  The boundary columns move behind the interior
+ * ones in w2_t (v2_t < b2_t) and their cells in w3_t (v3_t < b3_t); the
+ * boundary's cells are the surface cells then each column's cells
+ * below, nmap the inward map; every tracer starts as NOGRAD.  A column
+ * of an earlier boundary (a corner) stays with it.  Returns 0, or -1
+ * when no wet column of that side is left. */
+int emstr_grid_open(emstr_grid *g, int side);
+
 /* w from continuity, bottom up: model/hd/momentum/vel3d.c:4609-4720,
  * without sources, relaxation or precipitation.  Needs set_dz() first.  */
 void emstr_vel_w_update(geometry_t *window, window_t *windat,

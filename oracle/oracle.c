@@ -1515,6 +1515,108 @@ void oracle_auxiliary_routines(geometry_t *g, window_t *wd, win_priv_t *wc) {
 }
 
 /*-------------------------------------------------------------------*/
+/* boundary/bdrycond.c:180-214  bc_leastsq: the least-squares line    */
+/* through (1, y1) .. (4, y4), evaluated at 5                         */
+/*-------------------------------------------------------------------*/
+static double obc_leastsq(double y1, double y2, double y3, double y4) {
+  const double x[5] = {0.0, 1.0, 2.0, 3.0, 4.0};
+  const double y[5] = {0.0, y1, y2, y3, y4};
+  double sxy = 0.0, sx = 0.0, sy = 0.0, sx2 = 0.0, a1, a2;
+  const int n = 4;
+  int i;
+  for (i = 1; i <= n; i++) {
+    sx += x[i];
+    sy += y[i];
+    sxy += (x[i] * y[i]);
+    sx2 += (x[i] * x[i]);
+  }
+  a1 = (sxy - (sx * sy / n)) / (sx2 - (sx * sx) / n);
+  a2 = sy / n - a1 * sx / n;
+  return a2 + a1 * 5.0;
+}
+
+/*-------------------------------------------------------------------*/
+/* tracers.c:4569-5165  set_OBC_tr, the conditions built: NOGRAD      */
+/* (above trpc when set), NOTHIN, CLAMPD, LINEXT, FILEIN (below trpc  */
+/* when set), UPSTRM (Eulerian schemes); newval is wincon->w1 as in   */
+/* the reference.  Other settings set error 9.                        */
+/*-------------------------------------------------------------------*/
+static void oracle_set_obc_tr(geometry_t *g, window_t *wd, win_priv_t *wc, open_bdrys_t *o,
+                              int tn) {
+  const int bcond = o->bcond_tra[tn];
+  const int no3 = o->no3_t;
+  double *tr = wd->tr_wc[tn], *newval = wc->w1;
+  int cc;
+  const int bad = CUSTOM | POLEXT | CYCLIC | STATIS | PROFIL | DEPROF | DESCAL | TRFLUX |
+                  TRCONC | TRCONF;
+  if ((bcond & bad) || o->bgz > 0 || (o->options & OP_OBCCM) ||
+      (o->relax_zone_tra && o->relax_zone_tra[tn] > 0) ||
+      !(bcond & (NOGRAD | NOTHIN | CLAMPD | LINEXT | FILEIN)) ||
+      (o->trpc[tn] != 0.0 && !((bcond & NOGRAD) && (bcond & FILEIN))) ||
+      ((bcond & FILEIN) && (!o->t_transfer || o->trm[tn] < 0)) ||
+      ((bcond & UPSTRM) && wc->trasc == FFSL)) {
+    oracle_err = 9;
+    return;
+  }
+  if (bcond & NOGRAD) {
+    for (cc = 1; cc <= no3; cc++) {
+      int c = o->obc_t[cc];
+      if (!o->trpc[tn] || g->cellz[c] > o->trpc[tn]) newval[c] = tr[o->oi1_t[cc]];
+    }
+  } else if (bcond & NOTHIN) {
+    for (cc = 1; cc <= no3; cc++) newval[o->obc_t[cc]] = tr[o->obc_t[cc]];
+  } else if (bcond & CLAMPD) {
+    for (cc = 1; cc <= no3; cc++) newval[o->obc_t[cc]] = o->clampv[tn];
+  } else if (bcond & LINEXT) {
+    const int *imap = o->nmap;
+    for (cc = 1; cc <= no3; cc++) {
+      int ei = o->obc_t[cc];
+      int ei0 = imap[ei], ei1 = imap[ei0], ei2 = imap[ei1], ei3 = imap[ei2];
+      newval[ei] = obc_leastsq(tr[ei3], tr[ei2], tr[ei1], tr[ei0]);
+    }
+  }
+  if (bcond & FILEIN) {
+    const double *v = o->t_transfer[o->trm[tn]];
+    for (cc = 1; cc <= no3; cc++) {
+      int c = o->obc_t[cc];
+      if (!o->trpc[tn] || g->cellz[c] <= o->trpc[tn]) newval[c] = v[cc];
+    }
+  }
+  if (bcond & UPSTRM) {
+    const int xd = (o->ocodex & (L_EDGE | R_EDGE)) != 0;
+    const double *vel = xd ? wd->u1 : wd->u2;
+    const double *hat = xd ? g->h1au1 : g->h2au2;
+    const int *cv = xd ? o->obc_e1 : o->obc_e2, *civ = xd ? o->oi1_e1 : o->oi1_e2;
+    const double sc = ((o->ocodex & L_EDGE) || (o->ocodey & B_EDGE)) ? -1.0 : 1.0;
+    for (cc = 1; cc <= no3; cc++) {
+      int c = o->obc_t[cc], c1 = o->oi1_t[cc], c2 = g->m2d[c];
+      double v;
+      if (o->upmeth & CENTER)
+        v = 0.5 * (vel[cv[cc]] + vel[civ[cc]]);
+      else if (o->upmeth & FACE)
+        v = vel[cv[cc]];
+      else if (o->upmeth & ADAPTIVE)
+        v = -sc * 0.5 * (vel[cv[cc]] + vel[civ[cc]]) > 0.0 ? vel[cv[cc]] : vel[civ[cc]];
+      else
+        v = vel[c1];
+      newval[c] = tr[c] - wd->dt / hat[c2] *
+                              (0.5 * (sc * v + fabs(v)) * (tr[c] - tr[c1]) +
+                               0.5 * (sc * v - fabs(v)) * (newval[c] - tr[c]));
+    }
+  }
+  if (!(bcond & (NOTHIN | TRCONC | TRFLUX | TRCONF)) || (o->options & OP_OWRITE))
+    for (cc = 1; cc <= no3; cc++) tr[o->obc_t[cc]] = newval[o->obc_t[cc]];
+}
+
+/* tracers.c:4542-4561  bdry_tracer: every boundary, every tracer of
+ * tbdy in order */
+void oracle_bdry_tracer(geometry_t *g, window_t *wd, win_priv_t *wc) {
+  int n, nn;
+  for (n = 0; n < g->nobc; n++)
+    for (nn = 0; nn < wc->ntbdy; nn++) oracle_set_obc_tr(g, wd, wc, g->open[n], wc->tbdy[nn]);
+}
+
+/*-------------------------------------------------------------------*/
 /* tracers.c:239-321  (the parts on the path for a closed window):    */
 /* lateral ghosts (tracers.c:153-156), advect_diffuse, vert_diffuse,  */
 /* auxiliary_routines                                                 */
@@ -1526,6 +1628,7 @@ int oracle_tracer_step_3d_mt(geometry_t *g, window_t *wd, win_priv_t *wc,
   if (oracle_vert_diffuse_3d_mt(g, wd, wc, nthreads)) return 4;
   oracle_err = 0;
   oracle_auxiliary_routines(g, wd, wc);
+  oracle_bdry_tracer(g, wd, wc);
   return oracle_err;
 }
 

@@ -111,18 +111,27 @@ def test_shoc_out_of_scope_windows_rejected(what):
     case = configs.small(seed=1, nxi=8, nyi=6, nz=3, ntr=1)
     L = _lib.lib()
     if what == "nobc":
-        case.g.nobc = 1
+        # open boundaries: the plain form only (and a boundary structure)
+        case.open_boundary(S.L_EDGE)
+        want = "open boundaries"
+        assert not L.emstr_create(case.gp, case.wp, case.pp, 1)
+        assert want in _lib.last_error()[0]
+        case.g.nobc, keep = 1, case.g.open
+        case.g.open = None
     else:
         case.wc.sigma = 1
+        want = "sigma"
     assert not L.emstr_create(case.gp, case.wp, case.pp, 0)
     msg, _ = _lib.last_error()
-    assert "open boundaries and sigma" in msg
+    assert want in msg
     # the drop-in entry points record the same error and return
     snap = case.tracers()
     L.tracer_step_3d(case.gp, case.wp, case.pp)
-    assert "open boundaries and sigma" in _lib.last_error()[0]
+    assert want in _lib.last_error()[0]
     assert L.advect_diffuse(case.gp, case.wp, case.pp) != 0
     assert np.array_equal(case.tracers(), snap)
+    if what == "nobc":
+        case.g.open = keep
 
 
 def test_shoc_tracer_step_arguments():

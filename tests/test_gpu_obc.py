@@ -1,0 +1,103 @@
+"""GPU parity of the open-boundary tracer path: the drop-in
+tracer_step_3d (plain FP64) against the oracle over three steps, with
+the boundary columns left out of the advective update and the conditions
+NOGRAD (with trpc), NOTHIN, CLAMPD, LINEXT, FILEIN and UPSTRM (every
+upmeth) on each side of the grid, two boundaries meeting at a corner,
+and flow through the boundary faces; tracers bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+from ems_amd import _lib, configs, structs as S
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def build(seed, sides, conds, upmeth=0, filein=False, trpc=None):
+    case = configs.small(seed=seed, ntr=3, nz=6, dt=60.0, land_frac=0.1)
+    case.keep = []
+    rng = np.random.default_rng(seed)
+    for side in sides:
+        o = case.open_boundary(side)
+        o.upmeth = upmeth
+        for n, b in enumerate(conds):
+            o.bcond_tra[n] = b
+            o.clampv[n] = 2.0 + n
+            if trpc is not None:
+                o.trpc[n] = trpc[n]
+        if filein:
+            m = o.no3_t
+            rows = (S._PD * 3)()
+            for n in range(3):
+                data = np.concatenate([[0.0], rng.uniform(1.0, 9.0, m)])
+                case.keep.append(data)
+                rows[n] = data.ctypes.data_as(S._PD)
+                o.trm[n] = n
+            case.keep.append(rows)
+            o.t_transfer = rows
+        # flow through the boundary faces (UPSTRM reads it)
+        xd = side in (S.L_EDGE, S.R_EDGE)
+        fb = S.view(o.obc_e1 if xd else o.obc_e2, o.no3_t + 1, np.int32)[1:]
+        vel = case.a3(case.wd.u1 if xd else case.wd.u2)
+        vel[fb] = rng.uniform(-0.4, 0.4, fb.size)
+    case.prepare(O.set_dz)
+    return case
+
+
+def run_both(case, nsteps):
+    L = _lib.lib()
+    snap = H.snapshot(case)
+    for _ in range(nsteps):
+        L.tracer_step_3d(case.gp, case.wp, case.pp)
+        msg, cell = _lib.last_error()
+        assert msg == "", (msg, cell)
+    gpu = case.tracers()
+    H.restore(case, snap)
+    for _ in range(nsteps):
+        assert O.step(case) == 0
+    ora = case.tracers()
+    L.emstr_forget_window(case.gp)
+    return gpu, ora
+
+
+CASES = [
+    ("west nograd clampd linext", 3, [S.L_EDGE], [S.NOGRAD, S.CLAMPD, S.LINEXT], 0, False, None),
+    ("east upstrm-center nothin filein", 4, [S.R_EDGE],
+     [S.UPSTRM | S.CLAMPD, S.NOTHIN, S.NOGRAD | S.FILEIN], S.CENTER, True, [0.0, 0.0, -8.0]),
+    ("south upstrm-adaptive filein", 5, [S.B_EDGE],
+     [S.UPSTRM | S.FILEIN, S.LINEXT, S.UPSTRM | S.NOGRAD], S.ADAPTIVE, True, None),
+    ("north upstrm-face", 6, [S.F_EDGE], [S.UPSTRM | S.CLAMPD, S.UPSTRM | S.NOGRAD, S.CLAMPD],
+     S.FACE, False, None),
+    ("west + north corner, upstrm one cell in", 7, [S.L_EDGE, S.F_EDGE],
+     [S.UPSTRM | S.NOGRAD, S.NOGRAD, S.CLAMPD], 0, False, None),
+]
+
+
+@pytest.mark.parametrize("name,seed,sides,conds,upmeth,filein,trpc", CASES)
+def test_obc_bit_exact(name, seed, sides, conds, upmeth, filein, trpc):
+    say("case obc", name)
+    case = build(seed, sides, conds, upmeth, filein, trpc)
+    t0 = case.tracers()
+    gpu, ora = run_both(case, 3)
+    assert np.array_equal(gpu, ora), np.abs(gpu - ora).max()
+    assert np.abs(ora - t0).max() > 1e-3
+
+
+def test_obc_settings_outside_scope_refused():
+    say("case obc refusals")
+    L = _lib.lib()
+    case = build(8, [S.L_EDGE], [S.NOGRAD, S.CYCLIC, S.NOGRAD])
+    L.tracer_step_3d(case.gp, case.wp, case.pp)
+    assert "open boundary 0" in _lib.last_error()[0]
+    L.emstr_forget_window(case.gp)
+    case.g.open[0].contents.bcond_tra[1] = S.NOGRAD
+    assert not L.emstr_create(case.gp, case.wp, case.pp, 1)
+    assert "open boundaries" in _lib.last_error()[0]
