@@ -238,8 +238,7 @@ emstr_ctx *emstr_create(geometry_t *g, window_t *wd, win_priv_t *wc, int form) {
     return NULL;
   }
   if (g->nobc != 0 &&
-      (!g->open || form != EMSTR_PLAIN_F64 || wc->trasc == FFSL ||
-       (wc->stab & (SUB_STEP | SUB_STEP_NOSURF | SUB_STEP_TRACER)))) {
+      (!g->open || form != EMSTR_PLAIN_F64 || wc->trasc == FFSL)) {
     emstr_set_error(EMSTR_EUNSUPPORTED, -1,
                     "open boundaries: the plain form with an Eulerian scheme and no "
                     "sub-stepping only");

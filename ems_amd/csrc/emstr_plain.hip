@@ -723,18 +723,23 @@ k_upd(emstr_dev d, const double *__restrict__ tin, double *__restrict__ tout, do
   if (k > ksc) return; /* above the processed surface: written below */
   if (d.obcm && d.obcm[col]) {
     /* an open-boundary column: the update and surf_conc run over the
-     * other columns (tracers.c:811-822, 912, 945: vc1 / vcs1), and so do
-     * tr_bounds' thin-layer reset and clip (tracers.c:3250, 3277);
-     * its copy above the surface covers every processed column
+     * other columns (tracers.c:811-822, 912, 945: vc1 / vcs1), and so
+     * does tr_bounds' thin-layer reset (tracers.c:3250, vcs1); its clip
+     * runs over s1[1..vc1], which holds the boundary surface cells
+     * (vcs1 < cc <= vcs) but not the cells below them (tracers.c:3277),
+     * and its copy above the surface covers every processed column
      * (tracers.c:3306-3322) */
     for (int n = 0; n < d.ntr; n++) {
       const double *t = tin + n * N3;
       double *o = tout + n * N3;
       if (k < ksc) {
         o[idx] = t[idx];
+      } else if (!d.in_tbdy[n]) {
+        for (int kk = k; kk < d.nz; kk++) o[(ll)kk * P + col] = t[(ll)kk * P + col];
       } else {
-        const double v = t[idx];
-        for (int kk = k; kk < d.nz; kk++) o[(ll)kk * P + col] = d.in_tbdy[n] ? v : t[(ll)kk * P + col];
+        double v = t[idx];
+        if (d.mintr[n] <= 1e-5 && v < d.mintr[n]) v = d.mintr[n];
+        for (int kk = k; kk < d.nz; kk++) o[(ll)kk * P + col] = v;
       }
     }
     return;

@@ -91,6 +91,35 @@ def test_obc_bit_exact(name, seed, sides, conds, upmeth, filein, trpc):
     assert np.abs(ora - t0).max() > 1e-3
 
 
+def test_obc_with_sub_steps():
+    """Sub-stepping (SUB_STEP, a time step four times the stable one) in
+    the time loop (set_dz before every step after the first, as hd_step
+    runs it): the boundary columns are left out of every sub-step's
+    update and the conditions act once, after the step; the drop-in
+    against the oracle on twin windows, bit for bit."""
+    say("case obc sub-steps")
+    L = _lib.lib()
+    conds = [S.UPSTRM | S.CLAMPD, S.NOGRAD, S.LINEXT]
+    a, b = (build(9, [S.L_EDGE, S.B_EDGE], conds, S.CENTER) for _ in range(2))
+    for c in (a, b):
+        c.wc.stab = S.SUB_STEP
+        c.wd.dt = c.wd.dttr = 2400.0
+    t0 = a.tracers()
+    for s in range(3):
+        if s:
+            L.set_dz(a.gp, a.wp, a.pp)
+            O.set_dz(b.gp, b.wp, b.pp)
+        L.tracer_step_3d(a.gp, a.wp, a.pp)
+        msg, cell = _lib.last_error()
+        assert msg == "", (msg, cell)
+        assert O.step(b) == 0
+    L.emstr_forget_window(a.gp)
+    assert a.wc.b1 < 2400.0 and b.wc.b1 < 2400.0
+    gpu, ora = a.tracers(), b.tracers()
+    assert np.array_equal(gpu, ora), np.abs(gpu - ora).max()
+    assert np.abs(ora - t0).max() > 1e-3
+
+
 def test_obc_settings_outside_scope_refused():
     say("case obc refusals")
     L = _lib.lib()
